@@ -1,0 +1,262 @@
+/*
+ * rmh.h - C ABI of librmh, the MI355X (gfx950) accelerator for the data-parallel
+ * stages of VVC (H.266) reconstruction.
+ *
+ * The library replaces three CPU stages of rustmpeg-video's VVC decoder:
+ *
+ *   residual  dequantisation (flat, dependent, transform skip) + inverse LFNST +
+ *             inverse DCT-II / DST-VII / DCT-VIII over every transform unit (TU)
+ *   inter MC  8-tap luma / 4-tap chroma fractional-sample interpolation
+ *   deblock   luma and chroma deblocking of a whole picture
+ *
+ * Every entry point takes plain pointers and sizes. Host-memory entry points
+ * copy in, run and copy out; on any error the caller's outputs are left
+ * untouched, so a caller can fall back to its own CPU path (the contract of
+ * crates/rustmpeg-video/src/av2/deblock.rs:30-35). Plan entry points keep the
+ * work description resident on the device and run on a caller-supplied HIP
+ * stream against device buffers; they are what a frame-step hipGraph captures.
+ *
+ * Status codes follow AcceleratorError (crates/rustmpeg-video/src/accelerator.rs:18-32).
+ */
+#ifndef RMH_H
+#define RMH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RMH_ABI_VERSION 1u
+
+/* ------------------------------------------------------------------------ */
+/* Status and context                                                        */
+/* ------------------------------------------------------------------------ */
+
+/* Mirrors AcceleratorError {Unavailable, InvalidInput, LimitExceeded, Execution}
+ * (crates/rustmpeg-video/src/accelerator.rs:18-32). RMH_TIMEOUT is reported by
+ * bounded waits and maps to Execution, as the Vulkan read-back timeout does
+ * (crates/rustmpeg-vulkan/src/compute/mod.rs:416-418, 833-847). */
+typedef enum rmh_status {
+    RMH_OK = 0,
+    RMH_UNAVAILABLE = 1,
+    RMH_INVALID_INPUT = 2,
+    RMH_LIMIT_EXCEEDED = 3,
+    RMH_EXECUTION = 4,
+    RMH_TIMEOUT = 5
+} rmh_status;
+
+typedef struct rmh_ctx rmh_ctx;
+typedef struct rmh_plan rmh_plan;
+typedef struct rmh_graph rmh_graph;
+
+typedef struct rmh_config {
+    int32_t device;       /* HIP device ordinal */
+    uint32_t timeout_ms;  /* bound on every blocking wait; 0 selects 30000 ms
+                             (DEFAULT_TIMEOUT, crates/rustmpeg-vulkan/src/compute/mod.rs:39) */
+    uint32_t flags;       /* reserved, must be 0 */
+} rmh_config;
+
+typedef struct rmh_device_info {
+    char name[64];
+    char arch[32];             /* e.g. "gfx950" */
+    int32_t compute_units;
+    int32_t max_clock_khz;
+    uint64_t global_mem_bytes;
+    uint32_t abi_version;
+    uint32_t reserved;
+} rmh_device_info;
+
+/* Replaces Compute::new (crates/rustmpeg-vulkan/src/compute/mod.rs:112-160):
+ * opens the device and compiles nothing at run time (kernels are prebuilt). */
+rmh_status rmh_create(const rmh_config* cfg, rmh_ctx** out);
+void rmh_destroy(rmh_ctx* ctx);
+/* Text of the last error on this context (thread-local per context). */
+const char* rmh_last_error(const rmh_ctx* ctx);
+rmh_status rmh_info(rmh_ctx* ctx, rmh_device_info* out);
+/* Sticky failure latch, as Accelerator::disable_gpu / gpu_failed
+ * (crates/rustmpeg-vulkan/src/accelerator.rs:38,106-109,163-171): once set,
+ * every call returns RMH_UNAVAILABLE. */
+void rmh_disable(rmh_ctx* ctx);
+int rmh_is_disabled(const rmh_ctx* ctx);
+/* Test hook: the next `count` device calls fail with RMH_EXECUTION before any
+ * device work is enqueued, so callers can test their fallback without a fault. */
+void rmh_inject_failures(rmh_ctx* ctx, uint32_t count);
+
+/* ------------------------------------------------------------------------ */
+/* Residual stage                                                            */
+/* ------------------------------------------------------------------------ */
+
+enum { RMH_TR_DCT2 = 0, RMH_TR_DST7 = 1, RMH_TR_DCT8 = 2 };  /* TrType, vvc/dct.rs:27-34 */
+
+enum {
+    RMH_TU_TRANSFORM_SKIP = 1u, /* dequantize_ts, no transform (vvc/transform.rs:229-246) */
+    RMH_TU_DEP_QUANT = 2u       /* dependent quantisation (vvc/dct.rs:1191-1248) */
+};
+
+/* One transform unit. 16 bytes; the descriptor fields of TuData / CuPlan
+ * (crates/rustmpeg-video/src/vvc/decode_general.rs:773-811) that the residual
+ * driver inverse_residual (decode_general.rs:4125-4199) consumes.
+ *
+ * The TU's coefficients are a raster w*h block of int16 levels starting at
+ * coeff[offset]; its residual is written as a raster w*h int16 block at
+ * residual[offset]. Columns >= w - skip or rows >= h - skip (after the
+ * default zero-out of vvc/dct.rs:1063-1077 and, with LFNST, the LFNST
+ * zero-out of decode_general.rs:4233-4236) are ignored. */
+typedef struct rmh_tu_desc {
+    uint32_t offset;     /* element offset of the TU block in coeff[] / residual[] */
+    uint8_t log2_w;      /* 0..6 (widths 1..64; w*h >= 2) */
+    uint8_t log2_h;      /* 0..6 */
+    uint8_t qp;          /* Qp' = QP + 6*(bit_depth-8), already clamped as in decode_general.rs:4135-4138 */
+    uint8_t bit_depth;   /* 8..12 */
+    uint8_t tr_hor;      /* RMH_TR_* */
+    uint8_t tr_ver;      /* RMH_TR_* */
+    uint8_t skip_w;      /* extra zero-out width (the extra_skip_width of vvc/dct.rs:1002) */
+    uint8_t skip_h;      /* extra zero-out height */
+    uint8_t lfnst_idx;   /* 0 (off), 1, 2 */
+    uint8_t lfnst_mode;  /* LFNST intra-mode index 0..94 (vvc/lfnst.rs:85-95) */
+    uint8_t flags;       /* RMH_TU_* */
+    uint8_t reserved;    /* must be 0 */
+} rmh_tu_desc;
+
+/* Host memory, synchronous. `coeff` and `residual` hold n_coeff int16 each.
+ * The new VVC trait VvcResidualBatch::residuals maps onto this call. */
+rmh_status rmh_residual_batch(rmh_ctx* ctx, const rmh_tu_desc* tus, size_t n_tu,
+                              const int16_t* coeff, size_t n_coeff, int16_t* residual);
+
+/* Plan: validates and uploads the descriptors once. */
+rmh_status rmh_residual_plan_create(rmh_ctx* ctx, const rmh_tu_desc* tus, size_t n_tu,
+                                    size_t n_coeff, rmh_plan** out);
+/* Device memory, asynchronous on `stream` (a hipStream_t, NULL = default stream). */
+rmh_status rmh_residual_plan_run(rmh_plan* plan, const int16_t* d_coeff, int16_t* d_residual,
+                                 void* stream);
+/* Unbatched baseline: one launch per TU range [first_tu, first_tu + count) in
+ * descriptor order (the same kernel, a smaller grid). */
+rmh_status rmh_residual_plan_run_range(rmh_plan* plan, size_t first_tu, size_t count,
+                                       const int16_t* d_coeff, int16_t* d_residual, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Planes                                                                    */
+/* ------------------------------------------------------------------------ */
+
+/* A sample plane: `bytes_per_sample` 1 (8-bit content) or 2 (up to 12-bit),
+ * row-major with a row pitch of `stride` samples, like Plane / Plane16
+ * (crates/rustmpeg-video/src/vvc/predict.rs:987-1000, 2866-2945). */
+typedef struct rmh_plane {
+    void* data;
+    uint32_t width;
+    uint32_t height;
+    uint32_t stride;            /* samples */
+    uint8_t bytes_per_sample;   /* 1 or 2 */
+    uint8_t bit_depth;          /* 8..12 */
+    uint16_t reserved;
+} rmh_plane;
+
+/* ------------------------------------------------------------------------ */
+/* Inter prediction (uni-prediction, 4:2:0)                                  */
+/* ------------------------------------------------------------------------ */
+
+enum { RMH_PU_ALT_HPEL = 0x10u };  /* use LUMA_ALT_HPEL_FILTER_8TAP at frac 8 (vvc/inter.rs:80) */
+
+/* One prediction unit, 16 bytes. Luma block (x, y, 1<<log2_w, 1<<log2_h) is
+ * predicted from frame `frame`'s reference with the luma MV in 1/16 sample;
+ * chroma uses the same MV at 1/32 (vvc/inter.rs:1746-1749, 1826-1827). */
+typedef struct rmh_pu_desc {
+    uint16_t x;
+    uint16_t y;
+    uint8_t log2_w;      /* low 4 bits: log2 width (3..7); high bits: RMH_PU_* flags */
+    uint8_t log2_h;      /* log2 height (3..7) */
+    uint16_t frame;      /* index into the frame table */
+    int32_t mv_x;        /* 1/16 luma sample */
+    int32_t mv_y;
+} rmh_pu_desc;
+
+typedef struct rmh_mc_frame {
+    rmh_plane ref[3];    /* Y, Cb, Cr reference (4:2:0) */
+    rmh_plane dst[3];    /* prediction output */
+} rmh_mc_frame;
+
+rmh_status rmh_mc_batch(rmh_ctx* ctx, const rmh_mc_frame* frames, size_t n_frames,
+                        const rmh_pu_desc* pus, size_t n_pu);
+/* Planes in `frames` hold device addresses. */
+rmh_status rmh_mc_plan_create(rmh_ctx* ctx, const rmh_mc_frame* frames, size_t n_frames,
+                              const rmh_pu_desc* pus, size_t n_pu, rmh_plan** out);
+rmh_status rmh_mc_plan_run(rmh_plan* plan, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Deblocking                                                                */
+/* ------------------------------------------------------------------------ */
+
+/* Edge byte of one 4-sample edge segment:
+ *   bits 0-1  bS (0 = not filtered, 1, 2)
+ *   bits 2-3  maxFilterLengthP code c, length = 2c+1 (1, 3, 5, 7)
+ *   bits 4-5  maxFilterLengthQ code
+ *   bit  6    luma horizontal edge on a CTB row boundary (vvc/filter.rs:338)
+ * The map holds 4 bytes per 4x4 luma cell (cell (i, j) covers luma x 4i..4i+3,
+ * y 4j..4j+3):
+ *   [0] luma vertical edge at x = 4i, rows 4j..4j+3
+ *   [1] luma horizontal edge at y = 4j, columns 4i..4i+3
+ *   [2] chroma vertical edge at chroma x = 2i, chroma rows 2j..2j+1
+ *   [3] chroma horizontal edge at chroma y = 2j, chroma columns 2i..2i+1
+ * (one decision group per segment, as deblock_luma_edge_bd, vvc/filter.rs:638,
+ * and deblock_chroma_edge with two lines, vvc/filter.rs:935). */
+enum {
+    RMH_EDGE_BS_MASK = 0x03u,
+    RMH_EDGE_LENP_SHIFT = 2,
+    RMH_EDGE_LENQ_SHIFT = 4,
+    RMH_EDGE_CTB_ROW = 0x40u
+};
+
+typedef struct rmh_dbk_frame {
+    rmh_plane planes[3];         /* Y, Cb, Cr, filtered in place */
+    const uint8_t* edges;        /* grid_w * grid_h * 4 bytes */
+    uint32_t grid_w;             /* ceil(luma width / 4) */
+    uint32_t grid_h;             /* ceil(luma height / 4) */
+    uint8_t qp[3];               /* QpY, QpCb, QpCr (tC/beta index base, vvc/filter.rs:653) */
+    int8_t beta_offset_div2[3];
+    int8_t tc_offset_div2[3];
+    uint8_t reserved[3];
+} rmh_dbk_frame;
+
+rmh_status rmh_deblock(rmh_ctx* ctx, const rmh_dbk_frame* frames, size_t n_frames);
+/* Planes and `edges` in `frames` hold device addresses. */
+rmh_status rmh_deblock_plan_create(rmh_ctx* ctx, const rmh_dbk_frame* frames, size_t n_frames,
+                                   rmh_plan** out);
+rmh_status rmh_deblock_plan_run(rmh_plan* plan, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Plans and frame-step graphs                                               */
+/* ------------------------------------------------------------------------ */
+
+void rmh_plan_destroy(rmh_plan* plan);
+/* Algorithmic HBM bytes one run of the plan moves (the roofline numerator). */
+uint64_t rmh_plan_bytes(const rmh_plan* plan);
+
+/* Captures the given plans, in order, into one hipGraph (one frame step). For
+ * residual plans the device buffers are bound at capture time. */
+typedef struct rmh_graph_node {
+    rmh_plan* plan;
+    const int16_t* d_coeff;      /* residual plans only */
+    int16_t* d_residual;         /* residual plans only */
+} rmh_graph_node;
+
+rmh_status rmh_graph_create(rmh_ctx* ctx, const rmh_graph_node* nodes, size_t n_nodes,
+                            rmh_graph** out);
+rmh_status rmh_graph_launch(rmh_graph* graph, void* stream);
+void rmh_graph_destroy(rmh_graph* graph);
+
+/* ------------------------------------------------------------------------ */
+/* Introspection (tests)                                                     */
+/* ------------------------------------------------------------------------ */
+
+/* Writes the n*n row-major integer matrix the kernels use for (type, n);
+ * returns RMH_INVALID_INPUT for an unsupported pair (matrix_for_any,
+ * vvc/dct.rs:251-256). */
+rmh_status rmh_transform_matrix(int32_t type, int32_t n, int32_t* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RMH_H */

@@ -1,0 +1,307 @@
+/*
+ * orc_mc.c - inter prediction stage of the CPU oracle (test infrastructure only).
+ *
+ * Restates interpolate_luma_block / interpolate_chroma_block_420
+ * (crates/rustmpeg-video/src/vvc/inter.rs:886-1182), generalised from 8 bits to
+ * bit depth bd with head_room = IF_INTERNAL_PREC - bd (for bd = 8 the function is
+ * the reference's 8-bit one term for term), the uni-prediction path of
+ * predict_inter_pu (inter.rs:1734-1848) and write_prediction
+ * (decode_general.rs:2641-2667).
+ *
+ * The reference's 16-bit functions (inter.rs:2010-2134, 2543-2671) use
+ * shift2 = 12 - bd for the uni-prediction two-dimensional case, which
+ * saturates every non-zero sample; orc_interp_luma_16_as_written restates that
+ * function as written so a test can show the defect and show that every other
+ * branch agrees with the generalised function.
+ */
+#include <stdlib.h>
+#include <string.h>
+
+#include "vvc_oracle.h"
+
+#define IF_INTERNAL_PREC 14
+#define IF_FILTER_PREC 6
+#define IF_INTERNAL_OFFS 8192
+
+/* LUMA_FILTER_8TAP (inter.rs:57-74, H.266 Table 33). */
+static const int LUMA8[16][8] = {
+    {0, 0, 0, 64, 0, 0, 0, 0},        {0, 1, -3, 63, 4, -2, 1, 0},     {-1, 2, -5, 62, 8, -3, 1, 0},
+    {-1, 3, -8, 60, 13, -4, 1, 0},    {-1, 4, -10, 58, 17, -5, 1, 0},  {-1, 4, -11, 52, 26, -8, 3, -1},
+    {-1, 3, -9, 47, 31, -10, 4, -1},  {-1, 4, -11, 45, 34, -10, 4, -1}, {-1, 4, -11, 40, 40, -11, 4, -1},
+    {-1, 4, -10, 34, 45, -11, 4, -1}, {-1, 4, -10, 31, 47, -9, 3, -1}, {-1, 3, -8, 26, 52, -11, 4, -1},
+    {0, 1, -5, 17, 58, -10, 4, -1},   {0, 1, -4, 13, 60, -8, 3, -1},   {0, 1, -3, 8, 62, -5, 2, -1},
+    {0, 1, -2, 4, 63, -3, 1, 0}};
+/* LUMA_ALT_HPEL_FILTER_8TAP (inter.rs:80). */
+static const int LUMA_ALT_HPEL[8] = {0, 3, 9, 20, 20, 9, 3, 0};
+/* CHROMA_FILTER_4TAP (vvc/predict.rs:86-119, H.266 Table 34). */
+static const int CHROMA4[32][4] = {
+    {0, 64, 0, 0},    {-1, 63, 2, 0},   {-2, 62, 4, 0},   {-2, 60, 7, -1},  {-2, 58, 10, -2},
+    {-3, 57, 12, -2}, {-4, 56, 14, -2}, {-4, 55, 15, -2}, {-4, 54, 16, -2}, {-5, 53, 18, -2},
+    {-6, 52, 20, -2}, {-6, 49, 24, -3}, {-6, 46, 28, -4}, {-5, 44, 29, -4}, {-4, 42, 30, -4},
+    {-4, 39, 33, -4}, {-4, 36, 36, -4}, {-4, 33, 39, -4}, {-4, 30, 42, -4}, {-4, 29, 44, -5},
+    {-4, 28, 46, -6}, {-3, 24, 49, -6}, {-2, 20, 52, -6}, {-2, 18, 53, -5}, {-2, 16, 54, -4},
+    {-2, 15, 55, -4}, {-2, 14, 56, -4}, {-2, 12, 57, -3}, {-2, 10, 58, -2}, {-1, 7, 60, -2},
+    {0, 4, 62, -2},   {0, 2, 63, -1}};
+
+/* sample_clamped (inter.rs:886-890). */
+static inline int32_t smp(const uint16_t* p, int stride, int pw, int ph, int x, int y) {
+    if (x < 0) x = 0;
+    if (x > pw - 1) x = pw - 1;
+    if (y < 0) y = 0;
+    if (y > ph - 1) y = ph - 1;
+    return p[(size_t)y * (size_t)stride + (size_t)x];
+}
+
+static inline int32_t clampi(int32_t v, int32_t lo, int32_t hi) { return v < lo ? lo : v > hi ? hi : v; }
+
+/* Arithmetic right shift of a signed MV (Rust `>>` on i32). */
+static inline int asr(int v, int s) { return v >= 0 ? v >> s : -((-v + (1 << s) - 1) >> s); }
+
+int orc_interp_luma(const uint16_t* plane, int stride, int pw, int ph, int bd, int bx, int by, int w,
+                    int h, int mv_x, int mv_y, int bi, int alt_hpel, int affine, int32_t* out) {
+    if (affine) return -1; /* the affine filter path is not part of this oracle */
+    int int_x = bx + asr(mv_x, 4), int_y = by + asr(mv_y, 4);
+    int fx = mv_x & 15, fy = mv_y & 15;
+    int head_room = IF_INTERNAL_PREC - bd;
+    int32_t maxv = (1 << bd) - 1;
+    const int* cx = (alt_hpel && fx == 8) ? LUMA_ALT_HPEL : LUMA8[fx];
+    const int* cy = (alt_hpel && fy == 8) ? LUMA_ALT_HPEL : LUMA8[fy];
+
+    if (fx == 0 && fy == 0) {
+        for (int y = 0; y < h; y++)
+            for (int x = 0; x < w; x++) {
+                int32_t s = smp(plane, stride, pw, ph, int_x + x, int_y + y);
+                out[y * w + x] = bi ? s * (1 << head_room) - IF_INTERNAL_OFFS : s;
+            }
+        return 0;
+    }
+    if (fy == 0 || fx == 0) {
+        int horizontal = fy == 0;
+        const int* c = horizontal ? cx : cy;
+        int shift = bi ? IF_FILTER_PREC - head_room : IF_FILTER_PREC;
+        int32_t offset = bi ? -IF_INTERNAL_OFFS * (1 << shift) : 1 << (shift - 1);
+        for (int y = 0; y < h; y++)
+            for (int x = 0; x < w; x++) {
+                int32_t sum = 0;
+                for (int k = 0; k < 8; k++) {
+                    int tx = int_x + x + (horizontal ? k - 3 : 0);
+                    int ty = int_y + y + (horizontal ? 0 : k - 3);
+                    sum += smp(plane, stride, pw, ph, tx, ty) * c[k];
+                }
+                int32_t val = (sum + offset) >> shift;
+                out[y * w + x] = bi ? val : clampi(val, 0, maxv);
+            }
+        return 0;
+    }
+    int ext_h = h + 7;
+    int32_t* tmp = malloc(sizeof(int32_t) * (size_t)(w * ext_h));
+    int shift1 = IF_FILTER_PREC - head_room;
+    int32_t offset1 = -IF_INTERNAL_OFFS * (1 << shift1);
+    for (int y = 0; y < ext_h; y++)
+        for (int x = 0; x < w; x++) {
+            int32_t sum = 0;
+            for (int k = 0; k < 8; k++)
+                sum += smp(plane, stride, pw, ph, int_x + x + k - 3, int_y - 3 + y) * cx[k];
+            tmp[y * w + x] = (sum + offset1) >> shift1;
+        }
+    int shift2 = bi ? IF_FILTER_PREC : IF_FILTER_PREC + head_room;
+    int32_t offset2 = bi ? 0 : (1 << (shift2 - 1)) + (IF_INTERNAL_OFFS << IF_FILTER_PREC);
+    for (int y = 0; y < h; y++)
+        for (int x = 0; x < w; x++) {
+            int32_t sum = 0;
+            for (int k = 0; k < 8; k++) sum += tmp[(y + k) * w + x] * cy[k];
+            int32_t val = (sum + offset2) >> shift2;
+            out[y * w + x] = bi ? val : clampi(val, 0, maxv);
+        }
+    free(tmp);
+    return 0;
+}
+
+int orc_interp_chroma(const uint16_t* plane, int stride, int pw, int ph, int bd, int cxp, int cyp,
+                      int cw, int ch, int mv_x, int mv_y, int bi, int32_t* out) {
+    int int_x = cxp + asr(mv_x, 5), int_y = cyp + asr(mv_y, 5);
+    int fx = mv_x & 31, fy = mv_y & 31;
+    int head_room = IF_INTERNAL_PREC - bd;
+    int32_t maxv = (1 << bd) - 1;
+    if (fx == 0 && fy == 0) {
+        for (int y = 0; y < ch; y++)
+            for (int x = 0; x < cw; x++) {
+                int32_t s = smp(plane, stride, pw, ph, int_x + x, int_y + y);
+                out[y * cw + x] = bi ? s * (1 << head_room) - IF_INTERNAL_OFFS : s;
+            }
+        return 0;
+    }
+    if (fy == 0 || fx == 0) {
+        int horizontal = fy == 0;
+        const int* c = CHROMA4[horizontal ? fx : fy];
+        int shift = bi ? IF_FILTER_PREC - head_room : IF_FILTER_PREC;
+        int32_t offset = bi ? -IF_INTERNAL_OFFS * (1 << shift) : 1 << (shift - 1);
+        for (int y = 0; y < ch; y++)
+            for (int x = 0; x < cw; x++) {
+                int32_t sum = 0;
+                for (int k = 0; k < 4; k++) {
+                    int tx = int_x + x + (horizontal ? k - 1 : 0);
+                    int ty = int_y + y + (horizontal ? 0 : k - 1);
+                    sum += smp(plane, stride, pw, ph, tx, ty) * c[k];
+                }
+                int32_t val = (sum + offset) >> shift;
+                out[y * cw + x] = bi ? val : clampi(val, 0, maxv);
+            }
+        return 0;
+    }
+    const int* cxf = CHROMA4[fx];
+    const int* cyf = CHROMA4[fy];
+    int ext_h = ch + 3;
+    int32_t* tmp = malloc(sizeof(int32_t) * (size_t)(cw * ext_h));
+    int shift1 = IF_FILTER_PREC - head_room;
+    int32_t offset1 = -IF_INTERNAL_OFFS * (1 << shift1);
+    for (int y = 0; y < ext_h; y++)
+        for (int x = 0; x < cw; x++) {
+            int32_t sum = 0;
+            for (int k = 0; k < 4; k++)
+                sum += smp(plane, stride, pw, ph, int_x + x - 1 + k, int_y - 1 + y) * cxf[k];
+            tmp[y * cw + x] = (sum + offset1) >> shift1;
+        }
+    int shift2 = bi ? IF_FILTER_PREC : IF_FILTER_PREC + head_room;
+    int32_t offset2 = bi ? 0 : (1 << (shift2 - 1)) + (IF_INTERNAL_OFFS << IF_FILTER_PREC);
+    for (int y = 0; y < ch; y++)
+        for (int x = 0; x < cw; x++) {
+            int32_t sum = 0;
+            for (int k = 0; k < 4; k++) sum += tmp[(y + k) * cw + x] * cyf[k];
+            int32_t val = (sum + offset2) >> shift2;
+            out[y * cw + x] = bi ? val : clampi(val, 0, maxv);
+        }
+    free(tmp);
+    return 0;
+}
+
+/* interpolate_luma_block_16 as written (inter.rs:2010-2134), regular filter only. */
+int orc_interp_luma_16_as_written(const uint16_t* plane, int stride, int pw, int ph, int bd, int bx,
+                                  int by, int w, int h, int mv_x, int mv_y, int bi, int32_t* out) {
+    int dx = asr(mv_x, 4), dy = asr(mv_y, 4);
+    int fx = mv_x & 15, fy = mv_y & 15;
+    if (bd < 8) bd = 8;
+    if (bd > 12) bd = 12;
+    int32_t maxv = (1 << bd) - 1;
+    int head_room = 14 - bd < 2 ? 2 : 14 - bd;
+    int shift1 = 6 - head_room;
+    int shift2 = bi ? 6 : 12 - bd;
+    int32_t add2 = bi ? 0 : 1 << (shift2 - 1);
+    int32_t boff = 1 << (bd + head_room - 1);
+    if (fx == 0 && fy == 0) {
+        for (int y = 0; y < h; y++)
+            for (int x = 0; x < w; x++) {
+                int32_t s = smp(plane, stride, pw, ph, bx + x + dx, by + y + dy);
+                out[y * w + x] = bi ? s * (1 << head_room) - boff : s;
+            }
+        return 0;
+    }
+    if (fy == 0 || fx == 0) {
+        int horizontal = fy == 0;
+        const int* f = LUMA8[horizontal ? fx : fy];
+        for (int y = 0; y < h; y++)
+            for (int x = 0; x < w; x++) {
+                int32_t sum = 0;
+                for (int k = 0; k < 8; k++) {
+                    int tx = bx + x + dx + (horizontal ? k - 3 : 0);
+                    int ty = by + y + dy + (horizontal ? 0 : k - 3);
+                    sum += f[k] * smp(plane, stride, pw, ph, tx, ty);
+                }
+                out[y * w + x] = bi ? (sum >> shift1) - boff : clampi((sum + 32) >> 6, 0, maxv);
+            }
+        return 0;
+    }
+    int th = h + 7;
+    int32_t* tmp = malloc(sizeof(int32_t) * (size_t)(w * th));
+    for (int ty = 0; ty < th; ty++)
+        for (int x = 0; x < w; x++) {
+            int32_t sum = 0;
+            for (int k = 0; k < 8; k++)
+                sum += LUMA8[fx][k] * smp(plane, stride, pw, ph, bx + x + dx + k - 3, by + ty - 3 + dy);
+            tmp[ty * w + x] = sum >> shift1;
+        }
+    for (int y = 0; y < h; y++)
+        for (int x = 0; x < w; x++) {
+            int32_t sum = 0;
+            for (int k = 0; k < 8; k++) sum += LUMA8[fy][k] * tmp[(y + k) * w + x];
+            out[y * w + x] = bi ? (sum >> shift2) - boff : clampi((sum + add2) >> shift2, 0, maxv);
+        }
+    free(tmp);
+    return 0;
+}
+
+/* BCW_WEIGHTS (inter.rs:128). */
+static const int BCW_WEIGHTS[5] = {-2, 3, 4, 5, 10};
+
+void orc_weighted_average_bi(const int32_t* p0, const int32_t* p1, size_t n, int bcw_idx, int bd,
+                             int32_t* out) {
+    int w1 = BCW_WEIGHTS[bcw_idx > 4 ? 4 : bcw_idx];
+    int w0 = 8 - w1;
+    int head_room = IF_INTERNAL_PREC - bd;
+    int shift_num = head_room + 3;
+    int32_t offset = (1 << (shift_num - 1)) + (IF_INTERNAL_OFFS << 3);
+    int32_t maxv = (1 << bd) - 1;
+    for (size_t i = 0; i < n; i++) out[i] = clampi((p0[i] * w0 + p1[i] * w1 + offset) >> shift_num, 0, maxv);
+}
+
+static int plane_get(const rmh_plane* p, int x, int y) {
+    if (p->bytes_per_sample == 1) return ((const uint8_t*)p->data)[(size_t)y * p->stride + x];
+    return ((const uint16_t*)p->data)[(size_t)y * p->stride + x];
+}
+
+static void plane_set(const rmh_plane* p, int x, int y, int v) {
+    if (p->bytes_per_sample == 1)
+        ((uint8_t*)p->data)[(size_t)y * p->stride + x] = (uint8_t)v;
+    else
+        ((uint16_t*)p->data)[(size_t)y * p->stride + x] = (uint16_t)v;
+}
+
+/* A uint16 copy of a plane so the interpolators above read one sample type. */
+static uint16_t* widen(const rmh_plane* p) {
+    uint16_t* o = malloc(sizeof(uint16_t) * (size_t)p->width * p->height);
+    for (uint32_t y = 0; y < p->height; y++)
+        for (uint32_t x = 0; x < p->width; x++) o[(size_t)y * p->width + x] = (uint16_t)plane_get(p, (int)x, (int)y);
+    return o;
+}
+
+size_t orc_mc_batch(const rmh_mc_frame* frames, size_t n_frames, const rmh_pu_desc* pus, size_t n_pu) {
+    uint16_t** wide = calloc(n_frames * 3, sizeof(uint16_t*));
+    size_t bad = 0;
+    for (size_t i = 0; i < n_pu && !bad; i++) {
+        const rmh_pu_desc* pu = &pus[i];
+        if (pu->frame >= n_frames) {
+            bad = i + 1;
+            break;
+        }
+        const rmh_mc_frame* f = &frames[pu->frame];
+        int w = 1 << (pu->log2_w & 15), h = 1 << (pu->log2_h & 15);
+        int alt = (pu->log2_w & RMH_PU_ALT_HPEL) != 0;
+        int bd = f->ref[0].bit_depth;
+        for (int c = 0; c < 3; c++)
+            if (!wide[pu->frame * 3 + c]) wide[pu->frame * 3 + c] = widen(&f->ref[c]);
+        int32_t* buf = malloc(sizeof(int32_t) * (size_t)(w * h));
+        const rmh_plane* r = &f->ref[0];
+        orc_interp_luma(wide[pu->frame * 3], (int)r->width, (int)r->width, (int)r->height, bd, pu->x, pu->y, w, h,
+                        pu->mv_x, pu->mv_y, 0, alt, 0, buf);
+        int maxv = (1 << bd) - 1;
+        for (int y = 0; y < h; y++)
+            for (int x = 0; x < w; x++)
+                if (pu->x + x < (int)f->dst[0].width && pu->y + y < (int)f->dst[0].height)
+                    plane_set(&f->dst[0], pu->x + x, pu->y + y, clampi(buf[y * w + x], 0, maxv));
+        int cw = w / 2, ch = h / 2;
+        for (int c = 1; c < 3; c++) {
+            const rmh_plane* rc = &f->ref[c];
+            orc_interp_chroma(wide[pu->frame * 3 + c], (int)rc->width, (int)rc->width, (int)rc->height,
+                              rc->bit_depth, pu->x / 2, pu->y / 2, cw, ch, pu->mv_x, pu->mv_y, 0, buf);
+            int cmax = (1 << rc->bit_depth) - 1;
+            for (int y = 0; y < ch; y++)
+                for (int x = 0; x < cw; x++)
+                    if (pu->x / 2 + x < (int)f->dst[c].width && pu->y / 2 + y < (int)f->dst[c].height)
+                        plane_set(&f->dst[c], pu->x / 2 + x, pu->y / 2 + y, clampi(buf[y * cw + x], 0, cmax));
+        }
+        free(buf);
+    }
+    for (size_t i = 0; i < n_frames * 3; i++) free(wide[i]);
+    free(wide);
+    return bad;
+}

@@ -1,0 +1,249 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() and
+by bench.py's cpu_baseline leg, as the checker. The product never imports it.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+_lib = None
+
+I32P = ctypes.POINTER(ctypes.c_int32)
+I16P = ctypes.POINTER(ctypes.c_int16)
+U16P = ctypes.POINTER(ctypes.c_uint16)
+
+
+def build():
+    import subprocess
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        _lib = ctypes.CDLL(LIB_PATH)
+        _setup(_lib)
+    return _lib
+
+
+def _setup(L):
+    c_int, c_size = ctypes.c_int, ctypes.c_size_t
+    L.orc_transform_matrix.argtypes = [c_int, c_int, I32P]
+    L.orc_lfnst_set.argtypes = [c_int]
+    L.orc_lfnst_transposes.argtypes = [c_int]
+    L.orc_lfnst_coeff.argtypes = [c_int] * 5
+    L.orc_coeff_scan.argtypes = [c_int, c_int, I32P]
+    L.orc_dequant_rect.argtypes = [c_int, c_int, I32P, c_int, c_int, I32P]
+    L.orc_dequant_dep_quant.argtypes = [c_int, c_int, I32P, c_int, c_int, I32P]
+    L.orc_dequant_ts.argtypes = [ctypes.c_int32, c_int]
+    L.orc_dequant_ts.restype = ctypes.c_int32
+    L.orc_quant_ts.argtypes = [ctypes.c_int32, c_int]
+    L.orc_quant_ts.restype = ctypes.c_int32
+    L.orc_inverse_lfnst.argtypes = [c_int, c_int, I32P, c_int, c_int]
+    L.orc_forward_lfnst.argtypes = [c_int, c_int, I32P, c_int, c_int]
+    L.orc_inverse_transform.argtypes = [c_int, c_int, I32P, c_int, c_int, c_int, c_int, c_int, I32P]
+    L.orc_forward_transform.argtypes = [c_int, c_int, I32P, c_int, c_int, c_int, c_int, c_int, I32P]
+    L.orc_forward_transform_square.argtypes = [c_int, I32P, c_int, c_int, c_int, c_int, I32P]
+    L.orc_quant_rect.argtypes = [c_int, c_int, I32P, c_int, c_int, c_int, I32P]
+    L.orc_quant_dep_quant.argtypes = [c_int, c_int, I32P, c_int, c_int, I32P]
+    L.orc_quant_dct.argtypes = [c_int, I32P, c_int, I32P]
+    L.orc_dequant_dct.argtypes = [c_int, I32P, c_int, I32P]
+    L.orc_residual_batch.argtypes = [ctypes.c_void_p, c_size, I16P, c_size, I16P]
+    L.orc_residual_batch.restype = c_size
+    L.orc_interp_luma.argtypes = [U16P] + [c_int] * 13 + [I32P]
+    L.orc_interp_chroma.argtypes = [U16P] + [c_int] * 11 + [I32P]
+    L.orc_interp_luma_16_as_written.argtypes = [U16P] + [c_int] * 11 + [I32P]
+    L.orc_weighted_average_bi.argtypes = [I32P, I32P, c_size, c_int, c_int, I32P]
+    L.orc_mc_batch.argtypes = [ctypes.c_void_p, c_size, ctypes.c_void_p, c_size]
+    L.orc_mc_batch.restype = c_size
+    L.orc_deblock_luma_edge.argtypes = [I32P] + [c_int] * 9
+    L.orc_deblock_chroma_edge.argtypes = [I32P] + [c_int] * 8
+    L.orc_deblock_frames.argtypes = [ctypes.c_void_p, c_size]
+    L.orc_deblock_frames.restype = c_size
+    L.orc_deblock_luma_plane.argtypes = [U16P] + [c_int] * 9
+    L.orc_deblock_chroma_plane.argtypes = [U16P] + [c_int] * 9
+    L.orc_tc_beta.argtypes = [c_int] * 5 + [I32P, I32P]
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t)
+
+
+def transform_matrix(tr_type, n):
+    out = np.zeros(n * n, np.int32)
+    if lib().orc_transform_matrix(tr_type, n, _p(out, I32P)):
+        return None
+    return out.reshape(n, n)
+
+
+def coeff_scan(w, h):
+    out = np.zeros(4096, np.int32)
+    k = lib().orc_coeff_scan(w, h, _p(out, I32P))
+    return out[:k].copy()
+
+
+def _blk(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.int32).reshape(-1))
+
+
+def dequant_rect(w, h, levels, qp, bd):
+    lv = _blk(levels)
+    out = np.zeros(w * h, np.int32)
+    assert lib().orc_dequant_rect(w, h, _p(lv, I32P), qp, bd, _p(out, I32P)) == 0
+    return out.reshape(h, w)
+
+
+def dequant_dep_quant(w, h, levels, qp, bd):
+    lv = _blk(levels)
+    out = np.zeros(w * h, np.int32)
+    assert lib().orc_dequant_dep_quant(w, h, _p(lv, I32P), qp, bd, _p(out, I32P)) == 0
+    return out.reshape(h, w)
+
+
+def dequant_ts(level, qp):
+    return lib().orc_dequant_ts(int(level), qp)
+
+
+def quant_ts(res, qp):
+    return lib().orc_quant_ts(int(res), qp)
+
+
+def inverse_lfnst(w, h, coeffs, idx, mode):
+    c = _blk(coeffs).copy()
+    rc = lib().orc_inverse_lfnst(w, h, _p(c, I32P), idx, mode)
+    return None if rc else c.reshape(h, w)
+
+
+def forward_lfnst(w, h, coeffs, idx, mode):
+    c = _blk(coeffs).copy()
+    rc = lib().orc_forward_lfnst(w, h, _p(c, I32P), idx, mode)
+    return None if rc else c.reshape(h, w)
+
+
+def inverse_transform(w, h, coeffs, th, tv, skip_w=0, skip_h=0, bd=8):
+    c = _blk(coeffs)
+    out = np.zeros(w * h, np.int32)
+    rc = lib().orc_inverse_transform(w, h, _p(c, I32P), th, tv, skip_w, skip_h, bd, _p(out, I32P))
+    return None if rc else out.reshape(h, w)
+
+
+def forward_transform(w, h, res, th, tv, skip_w=0, skip_h=0, bd=8):
+    r = _blk(res)
+    out = np.zeros(w * h, np.int32)
+    rc = lib().orc_forward_transform(w, h, _p(r, I32P), th, tv, skip_w, skip_h, bd, _p(out, I32P))
+    return None if rc else out.reshape(h, w)
+
+
+def forward_transform_square(n, res, th, tv, skip_w=0, skip_h=0):
+    r = _blk(res)
+    out = np.zeros(n * n, np.int32)
+    rc = lib().orc_forward_transform_square(n, _p(r, I32P), th, tv, skip_w, skip_h, _p(out, I32P))
+    return None if rc else out.reshape(n, n)
+
+
+def quant_rect(w, h, coeffs, qp, bd, is_intra=True):
+    c = _blk(coeffs)
+    out = np.zeros(w * h, np.int32)
+    assert lib().orc_quant_rect(w, h, _p(c, I32P), qp, bd, int(is_intra), _p(out, I32P)) == 0
+    return out.reshape(h, w)
+
+
+def quant_dep_quant(w, h, coeffs, qp, bd):
+    c = _blk(coeffs)
+    out = np.zeros(w * h, np.int32)
+    assert lib().orc_quant_dep_quant(w, h, _p(c, I32P), qp, bd, _p(out, I32P)) == 0
+    return out.reshape(h, w)
+
+
+def quant_dct(n, coeffs, qp):
+    c = _blk(coeffs)
+    out = np.zeros(n * n, np.int32)
+    rc = lib().orc_quant_dct(n, _p(c, I32P), qp, _p(out, I32P))
+    return None if rc else out.reshape(n, n)
+
+
+def dequant_dct(n, levels, qp):
+    c = _blk(levels)
+    out = np.zeros(n * n, np.int32)
+    rc = lib().orc_dequant_dct(n, _p(c, I32P), qp, _p(out, I32P))
+    return None if rc else out.reshape(n, n)
+
+
+def residual_batch(tus, coeff):
+    """tus: numpy structured array with the rmh_tu_desc dtype; coeff: int16."""
+    coeff = np.ascontiguousarray(coeff, dtype=np.int16)
+    out = np.zeros_like(coeff)
+    bad = lib().orc_residual_batch(tus.ctypes.data, len(tus), _p(coeff, I16P), coeff.size, _p(out, I16P))
+    if bad:
+        raise ValueError("oracle rejected TU %d" % (bad - 1))
+    return out
+
+
+def interp_luma(plane, bd, bx, by, w, h, mv_x, mv_y, bi=False, alt_hpel=False):
+    p = np.ascontiguousarray(plane, dtype=np.uint16)
+    out = np.zeros(w * h, np.int32)
+    assert lib().orc_interp_luma(_p(p, U16P), p.shape[1], p.shape[1], p.shape[0], bd, bx, by, w, h,
+                                 mv_x, mv_y, int(bi), int(alt_hpel), 0, _p(out, I32P)) == 0
+    return out.reshape(h, w)
+
+
+def interp_chroma(plane, bd, cx, cy, cw, ch, mv_x, mv_y, bi=False):
+    p = np.ascontiguousarray(plane, dtype=np.uint16)
+    out = np.zeros(cw * ch, np.int32)
+    assert lib().orc_interp_chroma(_p(p, U16P), p.shape[1], p.shape[1], p.shape[0], bd, cx, cy, cw, ch,
+                                   mv_x, mv_y, int(bi), _p(out, I32P)) == 0
+    return out.reshape(ch, cw)
+
+
+def interp_luma_16_as_written(plane, bd, bx, by, w, h, mv_x, mv_y, bi=False):
+    p = np.ascontiguousarray(plane, dtype=np.uint16)
+    out = np.zeros(w * h, np.int32)
+    assert lib().orc_interp_luma_16_as_written(_p(p, U16P), p.shape[1], p.shape[1], p.shape[0], bd, bx,
+                                               by, w, h, mv_x, mv_y, int(bi), _p(out, I32P)) == 0
+    return out.reshape(h, w)
+
+
+def weighted_average_bi(p0, p1, bcw_idx, bd):
+    a = _blk(p0)
+    b = _blk(p1)
+    out = np.zeros_like(a)
+    lib().orc_weighted_average_bi(_p(a, I32P), _p(b, I32P), a.size, bcw_idx, bd, _p(out, I32P))
+    return out
+
+
+def deblock_luma_edge(lines, qp_p, qp_q, bs, beta_off, tc_off, bd, len_p, len_q, at_ctb=False):
+    l = np.ascontiguousarray(np.asarray(lines, dtype=np.int32).reshape(4, 16)).copy()
+    lib().orc_deblock_luma_edge(_p(l, I32P), qp_p, qp_q, bs, beta_off, tc_off, bd, len_p, len_q, int(at_ctb))
+    return l
+
+
+def deblock_chroma_edge(lines, qp, bs, beta_off, tc_off, len_p, len_q, bd):
+    l = np.ascontiguousarray(np.asarray(lines, dtype=np.int32).reshape(-1, 8)).copy()
+    lib().orc_deblock_chroma_edge(_p(l, I32P), l.shape[0], qp, bs, beta_off, tc_off, len_p, len_q, bd)
+    return l
+
+
+def deblock_luma_plane(plane, bd, cu_step, ctb, qp, bs, beta_off=0, tc_off=0):
+    p = np.ascontiguousarray(plane, dtype=np.uint16).copy()
+    lib().orc_deblock_luma_plane(_p(p, U16P), p.shape[1], p.shape[0], bd, cu_step, ctb, qp, bs, beta_off, tc_off)
+    return p
+
+
+def deblock_chroma_plane(plane, bd, cu_step, ctb_c, qp, bs, beta_off=0, tc_off=0):
+    p = np.ascontiguousarray(plane, dtype=np.uint16).copy()
+    lib().orc_deblock_chroma_plane(_p(p, U16P), p.shape[1], p.shape[0], bd, cu_step, ctb_c, qp, bs, beta_off, tc_off)
+    return p
+
+
+def tc_beta(qp, bs, beta_off, tc_off, bd):
+    tc = ctypes.c_int32()
+    beta = ctypes.c_int32()
+    lib().orc_tc_beta(qp, bs, beta_off, tc_off, bd, ctypes.byref(tc), ctypes.byref(beta))
+    return tc.value, beta.value

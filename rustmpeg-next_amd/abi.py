@@ -1,0 +1,81 @@
+"""ctypes mirrors of the structs in include/rmh.h (no library is loaded here)."""
+import ctypes
+
+import numpy as np
+
+RMH_OK, RMH_UNAVAILABLE, RMH_INVALID_INPUT, RMH_LIMIT_EXCEEDED, RMH_EXECUTION, RMH_TIMEOUT = range(6)
+STATUS_NAMES = {0: "OK", 1: "Unavailable", 2: "InvalidInput", 3: "LimitExceeded", 4: "Execution",
+                5: "Timeout"}
+
+TR_DCT2, TR_DST7, TR_DCT8 = 0, 1, 2
+TU_TRANSFORM_SKIP, TU_DEP_QUANT = 1, 2
+PU_ALT_HPEL = 0x10
+EDGE_BS_MASK, EDGE_LENP_SHIFT, EDGE_LENQ_SHIFT, EDGE_CTB_ROW = 0x03, 2, 4, 0x40
+
+TU_DTYPE = np.dtype([
+    ("offset", "<u4"), ("log2_w", "u1"), ("log2_h", "u1"), ("qp", "u1"), ("bit_depth", "u1"),
+    ("tr_hor", "u1"), ("tr_ver", "u1"), ("skip_w", "u1"), ("skip_h", "u1"),
+    ("lfnst_idx", "u1"), ("lfnst_mode", "u1"), ("flags", "u1"), ("reserved", "u1")])
+PU_DTYPE = np.dtype([("x", "<u2"), ("y", "<u2"), ("log2_w", "u1"), ("log2_h", "u1"),
+                     ("frame", "<u2"), ("mv_x", "<i4"), ("mv_y", "<i4")])
+
+
+class Config(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int32), ("timeout_ms", ctypes.c_uint32), ("flags", ctypes.c_uint32)]
+
+
+class DeviceInfo(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char * 64), ("arch", ctypes.c_char * 32), ("compute_units", ctypes.c_int32),
+                ("max_clock_khz", ctypes.c_int32), ("global_mem_bytes", ctypes.c_uint64),
+                ("abi_version", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+
+
+class Plane(ctypes.Structure):
+    _fields_ = [("data", ctypes.c_void_p), ("width", ctypes.c_uint32), ("height", ctypes.c_uint32),
+                ("stride", ctypes.c_uint32), ("bytes_per_sample", ctypes.c_uint8), ("bit_depth", ctypes.c_uint8),
+                ("reserved", ctypes.c_uint16)]
+
+
+class McFrame(ctypes.Structure):
+    _fields_ = [("ref", Plane * 3), ("dst", Plane * 3)]
+
+
+class DbkFrame(ctypes.Structure):
+    _fields_ = [("planes", Plane * 3), ("edges", ctypes.c_void_p), ("grid_w", ctypes.c_uint32),
+                ("grid_h", ctypes.c_uint32), ("qp", ctypes.c_uint8 * 3), ("beta_offset_div2", ctypes.c_int8 * 3),
+                ("tc_offset_div2", ctypes.c_int8 * 3), ("reserved", ctypes.c_uint8 * 3)]
+
+
+class GraphNode(ctypes.Structure):
+    _fields_ = [("plan", ctypes.c_void_p), ("d_coeff", ctypes.c_void_p), ("d_residual", ctypes.c_void_p)]
+
+
+def plane_from_address(addr, width, height, stride, bytes_per_sample, bit_depth):
+    return Plane(ctypes.c_void_p(addr), width, height, stride, bytes_per_sample, bit_depth, 0)
+
+
+def plane_from_array(a, bit_depth):
+    """A host plane over a C-contiguous 2-D uint8/uint16 numpy array."""
+    assert a.ndim == 2 and a.flags["C_CONTIGUOUS"] and a.dtype in (np.uint8, np.uint16)
+    return plane_from_address(a.ctypes.data, a.shape[1], a.shape[0], a.shape[1], a.dtype.itemsize, bit_depth)
+
+
+def dbk_frame(planes, edges_addr, grid_w, grid_h, qps, beta_offsets=(0, 0, 0), tc_offsets=(0, 0, 0)):
+    f = DbkFrame()
+    for i in range(3):
+        f.planes[i] = planes[i]
+    f.edges = edges_addr
+    f.grid_w, f.grid_h = grid_w, grid_h
+    for i in range(3):
+        f.qp[i] = qps[i]
+        f.beta_offset_div2[i] = beta_offsets[i]
+        f.tc_offset_div2[i] = tc_offsets[i]
+    return f
+
+
+def mc_frame(refs, dsts):
+    f = McFrame()
+    for i in range(3):
+        f.ref[i] = refs[i]
+        f.dst[i] = dsts[i]
+    return f

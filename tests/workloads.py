@@ -1,0 +1,264 @@
+"""Seeded synthetic workloads for the parity tests and the benchmark.
+
+Frame-level generators for the three stages:
+
+  residual_frame   one 4:2:0 frame's worth of TUs (recursive tiling, sizes 4..64,
+                   MTS types, Laplacian coefficients with most zeros at high
+                   frequencies, QP 22..37)
+  inter_frame      a textured 4:2:0 reference, a seeded CU quadtree over 128x128
+                   CTUs, one uni-prediction MV per CU (integer part +-64, any
+                   fraction), and a deblocking edge map on the 8x8 grid with a
+                   random bS per 4-sample segment and filter lengths from the CU
+                   sizes (luma_lens / chroma_lens, vvc/decode_general.rs:4916-4935)
+
+The numpy dtypes mirror the C structs of include/rmh.h byte for byte.
+"""
+import numpy as np
+
+TU_DTYPE = np.dtype([
+    ("offset", "<u4"), ("log2_w", "u1"), ("log2_h", "u1"), ("qp", "u1"), ("bit_depth", "u1"),
+    ("tr_hor", "u1"), ("tr_ver", "u1"), ("skip_w", "u1"), ("skip_h", "u1"),
+    ("lfnst_idx", "u1"), ("lfnst_mode", "u1"), ("flags", "u1"), ("reserved", "u1")])
+assert TU_DTYPE.itemsize == 16
+
+PU_DTYPE = np.dtype([("x", "<u2"), ("y", "<u2"), ("log2_w", "u1"), ("log2_h", "u1"),
+                     ("frame", "<u2"), ("mv_x", "<i4"), ("mv_y", "<i4")])
+assert PU_DTYPE.itemsize == 16
+
+DCT2, DST7, DCT8 = 0, 1, 2
+TS_FLAG, DQ_FLAG = 1, 2
+
+
+# ----------------------------------------------------------------- tiling
+
+def _tile(rng, W, H, ctu, max_tu, min_tu, p_split):
+    """Recursive seeded tiling of a W x H plane into power-of-two blocks."""
+    out = []
+
+    def rec(x, y, w, h):
+        if x >= W or y >= H:
+            return
+        fits = x + w <= W and y + h <= H
+        big = max(w, h) > max_tu
+        can_h = h > min_tu
+        can_w = w > min_tu
+        if fits and not big:
+            if not (can_h or can_w) or rng.random() >= p_split(w, h):
+                out.append((x, y, w, h))
+                return
+        # choose a split: quad, horizontal binary or vertical binary
+        if not fits:
+            if x + w > W and y + h > H and can_w and can_h:
+                kind = 0
+            elif x + w > W and can_w:
+                kind = 2
+            elif can_h:
+                kind = 1
+            else:
+                kind = 2
+        else:
+            opts = []
+            if can_w and can_h and w == h:
+                opts += [0, 0]
+            if can_h and h >= w // 2 * 1 and h * 4 > w:
+                opts.append(1)
+            if can_w and w * 4 > h:
+                opts.append(2)
+            if can_h and h > w:
+                opts.append(1)
+            if can_w and w > h:
+                opts.append(2)
+            if not opts:
+                opts = [1] if can_h else [2]
+            kind = opts[int(rng.integers(0, len(opts)))]
+        if kind == 0:
+            hw, hh = w // 2, h // 2
+            rec(x, y, hw, hh)
+            rec(x + hw, y, hw, hh)
+            rec(x, y + hh, hw, hh)
+            rec(x + hw, y + hh, hw, hh)
+        elif kind == 1:
+            rec(x, y, w, h // 2)
+            rec(x, y + h // 2, w, h // 2)
+        else:
+            rec(x, y, w // 2, h)
+            rec(x + w // 2, y, w // 2, h)
+
+    for cy in range(0, H, ctu):
+        for cx in range(0, W, ctu):
+            rec(cx, cy, ctu, ctu)
+    return out
+
+
+def _p_split_residual(w, h):
+    m = max(w, h)
+    return {64: 0.93, 32: 0.72, 16: 0.42, 8: 0.16}.get(m, 0.0)
+
+
+# ---------------------------------------------------------------- residual
+
+def residual_frame(seed, width=1920, height=1080, bit_depth=10, qp_range=(22, 37), with_lfnst=False,
+                   with_ts=False, with_dq=False, zero_frac=0.85):
+    """One frame of TUs: returns (tus, coeff int16, blocks) where blocks lists
+    (plane, x, y, w, h) of every TU in descriptor order."""
+    rng = np.random.default_rng([2, seed])
+    blocks = []
+    for plane, (W, H, ctu, maxtu) in enumerate([(width, height, 64, 64),
+                                                 ((width + 1) // 2, (height + 1) // 2, 32, 32),
+                                                 ((width + 1) // 2, (height + 1) // 2, 32, 32)]):
+        for (x, y, w, h) in _tile(rng, W, H, ctu, maxtu, 4, _p_split_residual):
+            blocks.append((plane, x, y, w, h))
+    n = len(blocks)
+    tus = np.zeros(n, TU_DTYPE)
+    ws = np.array([b[3] for b in blocks])
+    hs = np.array([b[4] for b in blocks])
+    sizes = ws * hs
+    offsets = np.concatenate([[0], np.cumsum(sizes)[:-1]])
+    tus["offset"] = offsets
+    tus["log2_w"] = np.log2(ws).astype(np.uint8)
+    tus["log2_h"] = np.log2(hs).astype(np.uint8)
+    qp = rng.integers(qp_range[0], qp_range[1] + 1, n)
+    tus["qp"] = qp + 6 * (bit_depth - 8)
+    tus["bit_depth"] = bit_depth
+    th = rng.integers(0, 3, n)
+    tv = rng.integers(0, 3, n)
+    th[ws > 32] = DCT2
+    tv[hs > 32] = DCT2
+    tus["tr_hor"] = th
+    tus["tr_ver"] = tv
+    if with_lfnst:
+        sel = (rng.random(n) < 0.3) & (ws >= 4) & (hs >= 4)
+        tus["lfnst_idx"][sel] = rng.integers(1, 3, int(sel.sum()))
+        tus["lfnst_mode"][sel] = rng.integers(0, 95, int(sel.sum()))
+        tus["tr_hor"][sel] = DCT2
+        tus["tr_ver"][sel] = DCT2
+    if with_ts:
+        sel = (rng.random(n) < 0.1) & (sizes <= 1024)
+        tus["flags"][sel] |= TS_FLAG
+    if with_dq:
+        sel = (rng.random(n) < 0.2) & ((tus["flags"] & TS_FLAG) == 0)
+        tus["flags"][sel] |= DQ_FLAG
+    total = int(sizes.sum())
+    # per-sample local coordinates
+    tu_of = np.repeat(np.arange(n), sizes)
+    local = np.arange(total) - np.repeat(offsets, sizes)
+    lx = local % np.repeat(ws, sizes)
+    ly = local // np.repeat(ws, sizes)
+    scale = np.repeat((ws + hs) / 2.0, sizes)
+    # P(nonzero) decays with x + y; density tuned so ~zero_frac of samples are 0
+    p = np.exp(-(lx + ly) / (0.22 * scale + 1.0))
+    keep = rng.random(total) < p * (1.0 - zero_frac) / max(p.mean(), 1e-9)
+    mag = np.rint(rng.laplace(0.0, 12.0, total)).astype(np.int64)
+    coeff = np.where(keep, mag, 0)
+    # rare large values exercise the int16 clip of the dequantiser
+    big = rng.random(total) < 2e-4
+    coeff = np.where(big & keep, rng.integers(-32768, 32768, total), coeff)
+    coeff = np.clip(coeff, -32768, 32767).astype(np.int16)
+    del tu_of
+    return tus, coeff, blocks
+
+
+def concat_frames(frames):
+    """Concatenate [(tus, coeff)] into one batch, shifting offsets."""
+    all_tus, all_c, base = [], [], 0
+    for tus, coeff in frames:
+        t = tus.copy()
+        t["offset"] += base
+        base += coeff.size
+        all_tus.append(t)
+        all_c.append(coeff)
+    return np.concatenate(all_tus), np.concatenate(all_c)
+
+
+# ------------------------------------------------------------------- inter
+
+def luma_lens(size_p, size_q):
+    """luma_lens (vvc/decode_general.rs:4916-4922)."""
+    if size_p <= 4 or size_q <= 4:
+        return 1, 1
+    return (7 if size_p >= 32 else 3), (7 if size_q >= 32 else 3)
+
+
+def chroma_lens(size_p, size_q, ctb_boundary):
+    """chroma_lens (vvc/decode_general.rs:4928-4935)."""
+    if size_p >= 8 and size_q >= 8:
+        return (1, 3) if ctb_boundary else (3, 3)
+    return 1, 1
+
+
+def _len_code(n):
+    return (n - 1) // 2
+
+
+def edge_byte(bs, lp, lq, ctb=False):
+    return (bs & 3) | (_len_code(lp) << 2) | (_len_code(lq) << 4) | (0x40 if ctb else 0)
+
+
+def textured_plane(rng, w, h, bit_depth):
+    yy, xx = np.mgrid[0:h, 0:w].astype(np.float64)
+    maxv = (1 << bit_depth) - 1
+    v = 0.5 * maxv
+    for _ in range(4):
+        fx, fy = rng.uniform(0.005, 0.12, 2)
+        ph = rng.uniform(0, 2 * np.pi)
+        v = v + rng.uniform(0.05, 0.18) * maxv * np.sin(2 * np.pi * (fx * xx + fy * yy) + ph)
+    v = v + rng.normal(0, 0.02 * maxv, (h, w))
+    return np.clip(np.rint(v), 0, maxv).astype(np.uint16)
+
+
+def _p_split_inter(w, h):
+    return {128: 0.9, 64: 0.6, 32: 0.45, 16: 0.3}.get(max(w, h), 0.0)
+
+
+def inter_frame(seed, width=1920, height=1080, bit_depth=10, mv_range=64, qp=32, ctu=128):
+    """Returns dict(ref=[Y,Cb,Cr] uint16, pus=PU array, cus=[(x,y,w,h)], edges=uint8[gh,gw,4],
+    qp=[3], ctu=ctu)."""
+    rng = np.random.default_rng([3, seed])
+    cw, ch = (width + 1) // 2, (height + 1) // 2
+    ref = [textured_plane(rng, width, height, bit_depth), textured_plane(rng, cw, ch, bit_depth),
+           textured_plane(rng, cw, ch, bit_depth)]
+    cus = _tile(rng, width, height, ctu, ctu, 8, _p_split_inter)
+    n = len(cus)
+    pus = np.zeros(n, PU_DTYPE)
+    pus["x"] = [c[0] for c in cus]
+    pus["y"] = [c[1] for c in cus]
+    pus["log2_w"] = [int(np.log2(c[2])) for c in cus]
+    pus["log2_h"] = [int(np.log2(c[3])) for c in cus]
+    pus["mv_x"] = 16 * rng.integers(-mv_range, mv_range + 1, n) + rng.integers(0, 16, n)
+    pus["mv_y"] = 16 * rng.integers(-mv_range, mv_range + 1, n) + rng.integers(0, 16, n)
+    # CU-size maps on the 4x4 grid
+    gw, gh = (width + 3) // 4, (height + 3) // 4
+    cu_id = np.full((gh, gw), -1, np.int64)
+    cu_w = np.zeros((gh, gw), np.int64)
+    cu_h = np.zeros((gh, gw), np.int64)
+    for i, (x, y, w, h) in enumerate(cus):
+        cu_id[y // 4:(y + h + 3) // 4, x // 4:(x + w + 3) // 4] = i
+        cu_w[y // 4:(y + h + 3) // 4, x // 4:(x + w + 3) // 4] = w
+        cu_h[y // 4:(y + h + 3) // 4, x // 4:(x + w + 3) // 4] = h
+    edges = np.zeros((gh, gw, 4), np.uint8)
+    bs_v = rng.integers(0, 3, (gh, gw))
+    bs_h = rng.integers(0, 3, (gh, gw))
+    cgw, cgh = (cw + 1) // 2, (ch + 1) // 2
+    for j in range(gh):
+        for i in range(gw):
+            x, y = 4 * i, 4 * j
+            if x % 8 == 0 and i > 0 and cu_id[j, i] != cu_id[j, i - 1]:
+                lp, lq = luma_lens(cu_w[j, i - 1], cu_w[j, i])
+                edges[j, i, 0] = edge_byte(int(bs_v[j, i]), lp, lq)
+            if y % 8 == 0 and j > 0 and cu_id[j, i] != cu_id[j - 1, i]:
+                lp, lq = luma_lens(cu_h[j - 1, i], cu_h[j, i])
+                edges[j, i, 1] = edge_byte(int(bs_h[j, i]), lp, lq, ctb=(y % ctu == 0))
+            # chroma cell (i, j) covers chroma x 2i..2i+1, y 2j..2j+1 (luma 4i.., 4j..)
+            cx, cy = 2 * i, 2 * j
+            if i < cgw and j < cgh:
+                lx, ly = 2 * cx, 2 * cy
+                li, lj = lx // 4, ly // 4
+                if cx % 8 == 0 and cx > 0 and li < gw and lj < gh and cu_id[lj, li] != cu_id[lj, li - 1]:
+                    lp, lq = chroma_lens(cu_w[lj, li - 1] // 2, cu_w[lj, li] // 2, False)
+                    edges[j, i, 2] = edge_byte(int(bs_v[lj, li]), lp, lq)
+                if cy % 8 == 0 and cy > 0 and li < gw and lj < gh and cu_id[lj, li] != cu_id[lj - 1, li]:
+                    lp, lq = chroma_lens(cu_h[lj - 1, li] // 2, cu_h[lj, li] // 2, cy % (ctu // 2) == 0)
+                    edges[j, i, 3] = edge_byte(int(bs_h[lj, li]), lp, lq)
+    chroma_qp = qp if qp <= 26 else qp - 1  # default chroma QP mapping, vvc/hls.rs:778-781
+    return dict(ref=ref, pus=pus, cus=cus, edges=edges, qp=[qp, chroma_qp, chroma_qp], ctu=ctu,
+                bit_depth=bit_depth, width=width, height=height)
