@@ -1,0 +1,224 @@
+#!/usr/bin/env python3
+"""Benchmark: bit-exact batched VVC residual reconstruction on one MI355X.
+
+Workload (one step = one pass of the residual stage over one batch): every TU
+of F synthetic 1920x1080 10-bit 4:2:0 frames (recursive tiling 4x4..64x64
+favouring 8x8/16x16, DCT-II/DST-VII/DCT-VIII drawn per TU, int16 Laplacian
+coefficients with ~85% zeros toward high frequencies, QP 22..37), dequantised
+and inverse transformed to int16 residuals by one launch of the batched kernel.
+Inputs (descriptors + coefficients) are resident in HBM before timing starts.
+
+Reported beside the batched rate: the same kernel run unbatched (one launch
+per frame), the roofline of the kernel against MI355X HBM (8 TB/s), and the
+CPU oracle timed on a bounded sample on this host.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--frames F]
+    python bench.py --copy-calibration      (copy kernel of known size, for rocprofv3 --pmc scaling)
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+METRIC = "bit-exact 1080p10 VVC recon Mpix/s vs unbatched; encode fps learned vs full RDO"
+HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(REPO, "profiles", "residual_kernel_traffic.txt")
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def copy_calibration():
+    """A plain device copy of known size (torch's copy kernel), profiled with
+    rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE to scale those counters."""
+    import torch
+    n = 256 << 20
+    a = torch.ones(n, dtype=torch.uint8, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_each_way": n, "copies": 5}), flush=True)
+
+
+def read_traffic():
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary, if any."""
+    if not os.path.exists(TRAFFIC_FILE):
+        return None, None
+    vals = {}
+    for line in open(TRAFFIC_FILE):
+        if "=" in line and not line.lstrip().startswith("#"):
+            k, v = line.split("=", 1)
+            vals[k.strip()] = v.strip()
+    try:
+        return float(vals["traffic_bytes_per_launch"]), vals.get("frames")
+    except (KeyError, ValueError):
+        return None, None
+
+
+def load_package():
+    import importlib.util
+    name = "rustmpeg_next_amd"
+    if name in sys.modules:
+        return sys.modules[name]
+    d = os.path.join(REPO, "rustmpeg-next_amd")
+    spec = importlib.util.spec_from_file_location(name, os.path.join(d, "__init__.py"), submodule_search_locations=[d])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules[name] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def cpu_baseline(frames, seconds):
+    """The CPU oracle (oracle/liboracle.so, scalar C, one thread) on a bounded
+    sample: the first frames of the batch, repeated until `seconds` elapse."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import pyoracle as po
+    po.lib()
+    done_px, t0, reps = 0, time.perf_counter(), 0
+    while True:
+        for tus, coeff in frames:
+            po.residual_batch(tus, coeff)
+            done_px += 1920 * 1080
+        reps += 1
+        el = time.perf_counter() - t0
+        if el >= seconds:
+            break
+    return {"value": round(done_px / el / 1e6, 3), "unit": "Mpix/s", "cores": 1, "kind": "port",
+            "sample": "%d frame(s) of the same 1080p10 TU workload, %d pass(es), %.1f s, scalar C oracle "
+                      "(oracle/orc_residual.c) on one host core" % (len(frames), reps, el)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--frames", type=int, default=32)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-unbatched", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true")
+    args = ap.parse_args()
+
+    if args.copy_calibration:
+        copy_calibration()
+        return 0
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (the residual stage shards by "
+              "stream with no exchange; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        return 2
+
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    import workloads as W
+
+    rmh = load_package()
+    hip = rmh.hip
+    t0 = time.perf_counter()
+    frames = []
+    for f in range(args.frames):
+        tus, coeff, _ = W.residual_frame(1000 + f)
+        frames.append((tus, coeff))
+    tus, coeff = W.concat_frames(frames)
+    ranges, first = [], 0
+    for t, _ in frames:
+        ranges.append((first, len(t)))
+        first += len(t)
+    log("generated %d frames (%d TUs, %d samples) in %.1f s" % (args.frames, len(tus), coeff.size,
+                                                                 time.perf_counter() - t0))
+
+    acc = rmh.Accelerator(0)
+    info = acc.info()
+    plan = acc.residual_plan(tus, coeff.size)
+    d_coeff = hip.DeviceBuffer.from_array(coeff)
+    d_res = hip.DeviceBuffer(coeff.nbytes)
+    stream = hip.Stream()
+    s = stream.handle
+
+    # correctness gate: the first and last frame of the batch against the oracle
+    plan.run(d_coeff.ptr, d_res.ptr, s)
+    stream.synchronize()
+    got = d_res.download(np.int16)
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import pyoracle as po
+    for fi in (0, args.frames - 1):
+        t, c = frames[fi]
+        base = int(tus[ranges[fi][0]]["offset"])
+        want = po.residual_batch(t, c)
+        if not (got[base:base + c.size] == want).all():
+            log("bench: GPU residuals differ from the oracle on frame %d" % fi)
+            return 1
+    log("bit-exact check passed (frames 0 and %d)" % (args.frames - 1))
+
+    ev0, ev1 = hip.Event(), hip.Event()
+    for _ in range(args.warmup):
+        plan.run(d_coeff.ptr, d_res.ptr, s)
+    hip.device_synchronize()
+    ev0.record(s)
+    for _ in range(args.steps):
+        plan.run(d_coeff.ptr, d_res.ptr, s)
+    ev1.record(s)
+    ev1.synchronize()
+    hip.device_synchronize()
+    ms = ev0.elapsed_ms(ev1) / args.steps
+    luma_px = args.frames * 1920 * 1080
+    value = luma_px / (ms / 1e3) / 1e6
+    log("batched: %.4f ms/step, %.1f Mpix/s" % (ms, value))
+
+    unb = None
+    if not args.no_unbatched:
+        for _ in range(max(1, args.warmup)):
+            for a, n in ranges:
+                plan.run_range(a, n, d_coeff.ptr, d_res.ptr, s)
+        hip.device_synchronize()
+        ev0.record(s)
+        for _ in range(args.steps):
+            for a, n in ranges:
+                plan.run_range(a, n, d_coeff.ptr, d_res.ptr, s)
+        ev1.record(s)
+        ev1.synchronize()
+        hip.device_synchronize()
+        ums = ev0.elapsed_ms(ev1) / args.steps
+        unb = {"value": round(luma_px / (ums / 1e3) / 1e6, 1), "ms_per_step": round(ums, 4),
+               "launches_per_step": len(ranges)}
+        log("unbatched: %.4f ms/step, %.1f Mpix/s" % (ums, unb["value"]))
+
+    achieved = plan.bytes / (ms / 1e3) / 1e9
+    traffic, _ = read_traffic()
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "kernel": "residual_kernel", "algorithmic_bytes_per_launch": plan.bytes}
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(frames[:2], args.cpu_seconds)
+        cpu["cores_on_host"] = os.cpu_count()
+        log("cpu oracle: %s Mpix/s" % cpu["value"])
+
+    out = {
+        "metric": METRIC, "value": round(value, 1), "unit": "Mpix/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "int16", "data": "synthetic",
+        "config": {"workload": "batched dequant + inverse DCT-II/DST-VII/DCT-VIII over every TU of %d "
+                               "synthetic 1920x1080 10-bit 4:2:0 frames per step" % args.frames,
+                   "frames_per_step": args.frames, "tus_per_step": int(len(tus)), "samples_per_step": int(coeff.size),
+                   "parallelism": "single GPU"},
+        "unbatched": unb,
+        "vs_unbatched": round(value / unb["value"], 3) if unb else None,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "device": info,
+    }
+    print(json.dumps(out), flush=True)
+    plan.close()
+    acc.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

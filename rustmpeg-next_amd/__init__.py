@@ -1,5 +1,309 @@
-"""rustmpeg-next_amd: MI355X (gfx950) accelerator for VVC reconstruction stages.
+"""rustmpeg-next_amd: MI355X (gfx950) accelerator for the data-parallel stages of
+VVC reconstruction, over librmh (include/rmh.h).
 
-Python mirror of the accelerator surface over librmh (include/rmh.h).
+This module is the Python mirror of the accelerator surface a rustmpeg caller
+sees (crates/rustmpeg-video/src/accelerator.rs:18-32 for the error type,
+crates/rustmpeg-vulkan/src/accelerator.rs for the façade): an ``Accelerator``
+that owns one device context, and three provider objects with the shape of the
+VVC traits the drop-in adds next to ``DeblockFilter<P>``
+(crates/rustmpeg-video/src/av2/deblock.rs:36-48):
+
+    VvcResidualBatch.residuals(tus, coeffs)      dequant + LFNST + inverse transform
+    VvcInterPredBatch.predict(frames, pus)       uni-prediction 8/4-tap interpolation
+    VvcDeblockFilter.deblock(frames)             whole-picture deblocking, in place
+
+There is no CPU fallback in this package: when librmh.so is missing or the
+device is absent every call raises ``Unavailable`` and the caller runs its own
+CPU path, as the reference's codecs do when no accelerator is installed.
 """
-from . import abi  # noqa: F401
+import ctypes
+import os
+
+import numpy as np
+
+from . import abi, hip  # noqa: F401
+from .abi import PU_DTYPE, TU_DTYPE  # noqa: F401
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "librmh.so")
+
+
+class AcceleratorError(Exception):
+    """AcceleratorError (crates/rustmpeg-video/src/accelerator.rs:18-32)."""
+    status = abi.RMH_EXECUTION
+
+    def __init__(self, message, status=None):
+        super().__init__(message)
+        if status is not None:
+            self.status = status
+
+
+class Unavailable(AcceleratorError):
+    status = abi.RMH_UNAVAILABLE
+
+
+class InvalidInput(AcceleratorError):
+    status = abi.RMH_INVALID_INPUT
+
+
+class LimitExceeded(AcceleratorError):
+    status = abi.RMH_LIMIT_EXCEEDED
+
+
+class Execution(AcceleratorError):
+    status = abi.RMH_EXECUTION
+
+
+class Timeout(Execution):
+    status = abi.RMH_TIMEOUT
+
+
+_ERRORS = {abi.RMH_UNAVAILABLE: Unavailable, abi.RMH_INVALID_INPUT: InvalidInput,
+           abi.RMH_LIMIT_EXCEEDED: LimitExceeded, abi.RMH_EXECUTION: Execution, abi.RMH_TIMEOUT: Timeout}
+
+_lib = None
+
+EXPORTED = [
+    "rmh_create", "rmh_destroy", "rmh_last_error", "rmh_info", "rmh_disable", "rmh_is_disabled",
+    "rmh_inject_failures", "rmh_residual_batch", "rmh_residual_plan_create", "rmh_residual_plan_run",
+    "rmh_residual_plan_run_range", "rmh_mc_batch", "rmh_mc_plan_create", "rmh_mc_plan_run", "rmh_deblock",
+    "rmh_deblock_plan_create", "rmh_deblock_plan_run", "rmh_plan_destroy", "rmh_plan_bytes", "rmh_graph_create",
+    "rmh_graph_launch", "rmh_graph_destroy", "rmh_transform_matrix",
+]
+
+
+def load_library(path=LIB_PATH):
+    """Loads librmh.so; raises Unavailable when it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise Unavailable("librmh.so is not built (run rustmpeg-next_amd/build.py)")
+    L = ctypes.CDLL(path)
+    vp, sz, st = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int
+    sig = {
+        "rmh_create": ([ctypes.POINTER(abi.Config), ctypes.POINTER(vp)], st),
+        "rmh_destroy": ([vp], None),
+        "rmh_last_error": ([vp], ctypes.c_char_p),
+        "rmh_info": ([vp, ctypes.POINTER(abi.DeviceInfo)], st),
+        "rmh_disable": ([vp], None),
+        "rmh_is_disabled": ([vp], ctypes.c_int),
+        "rmh_inject_failures": ([vp, ctypes.c_uint32], None),
+        "rmh_residual_batch": ([vp, vp, sz, vp, sz, vp], st),
+        "rmh_residual_plan_create": ([vp, vp, sz, sz, ctypes.POINTER(vp)], st),
+        "rmh_residual_plan_run": ([vp, vp, vp, vp], st),
+        "rmh_residual_plan_run_range": ([vp, sz, sz, vp, vp, vp], st),
+        "rmh_mc_batch": ([vp, vp, sz, vp, sz], st),
+        "rmh_mc_plan_create": ([vp, vp, sz, vp, sz, ctypes.POINTER(vp)], st),
+        "rmh_mc_plan_run": ([vp, vp], st),
+        "rmh_deblock": ([vp, vp, sz], st),
+        "rmh_deblock_plan_create": ([vp, vp, sz, ctypes.POINTER(vp)], st),
+        "rmh_deblock_plan_run": ([vp, vp], st),
+        "rmh_plan_destroy": ([vp], None),
+        "rmh_plan_bytes": ([vp], ctypes.c_uint64),
+        "rmh_graph_create": ([vp, vp, sz, ctypes.POINTER(vp)], st),
+        "rmh_graph_launch": ([vp, vp], st),
+        "rmh_graph_destroy": ([vp], None),
+        "rmh_transform_matrix": ([ctypes.c_int32, ctypes.c_int32, vp], st),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = L
+    return L
+
+
+def transform_matrix(tr_type, n):
+    """The integer matrix the kernels use (rmh_transform_matrix)."""
+    out = np.zeros(n * n, np.int32)
+    if load_library().rmh_transform_matrix(tr_type, n, out.ctypes.data) != abi.RMH_OK:
+        raise InvalidInput("unsupported transform (%d, %d)" % (tr_type, n))
+    return out.reshape(n, n)
+
+
+class Plan:
+    """A validated, device-resident work description (rmh_plan)."""
+
+    def __init__(self, acc, handle, kind):
+        self.acc, self.handle, self.kind = acc, handle, kind
+
+    @property
+    def bytes(self):
+        return int(load_library().rmh_plan_bytes(self.handle))
+
+    def run(self, d_coeff=0, d_residual=0, stream=0):
+        L = load_library()
+        if self.kind == "residual":
+            self.acc._check(L.rmh_residual_plan_run(self.handle, d_coeff, d_residual, stream))
+        elif self.kind == "mc":
+            self.acc._check(L.rmh_mc_plan_run(self.handle, stream))
+        else:
+            self.acc._check(L.rmh_deblock_plan_run(self.handle, stream))
+
+    def run_range(self, first, count, d_coeff, d_residual, stream=0):
+        self.acc._check(load_library().rmh_residual_plan_run_range(self.handle, first, count, d_coeff, d_residual,
+                                                                   stream))
+
+    def close(self):
+        if self.handle:
+            load_library().rmh_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Graph:
+    """A frame step captured as one hipGraph (rmh_graph)."""
+
+    def __init__(self, acc, handle, keep):
+        self.acc, self.handle, self._keep = acc, handle, keep
+
+    def launch(self, stream=0):
+        self.acc._check(load_library().rmh_graph_launch(self.handle, stream))
+
+    def close(self):
+        if self.handle:
+            load_library().rmh_graph_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Accelerator:
+    """One device context (rmh_ctx); the counterpart of the Vulkan Accelerator
+    façade (crates/rustmpeg-vulkan/src/accelerator.rs)."""
+
+    def __init__(self, device=0, timeout_ms=0):
+        L = load_library()
+        h = ctypes.c_void_p()
+        cfg = abi.Config(device, timeout_ms, 0)
+        s = L.rmh_create(ctypes.byref(cfg), ctypes.byref(h))
+        if s != abi.RMH_OK:
+            raise _ERRORS.get(s, Execution)("rmh_create failed (%s)" % abi.STATUS_NAMES.get(s, s), s)
+        self.handle = h.value
+
+    # -- context -----------------------------------------------------------
+    def _check(self, status):
+        if status != abi.RMH_OK:
+            msg = load_library().rmh_last_error(self.handle)
+            raise _ERRORS.get(status, Execution)((msg or b"").decode(errors="replace"), status)
+
+    def info(self):
+        i = abi.DeviceInfo()
+        self._check(load_library().rmh_info(self.handle, ctypes.byref(i)))
+        return dict(name=i.name.decode(), arch=i.arch.decode(), compute_units=i.compute_units,
+                    max_clock_khz=i.max_clock_khz, global_mem_bytes=i.global_mem_bytes, abi_version=i.abi_version)
+
+    def disable(self):
+        load_library().rmh_disable(self.handle)
+
+    @property
+    def disabled(self):
+        return bool(load_library().rmh_is_disabled(self.handle))
+
+    def inject_failures(self, count):
+        load_library().rmh_inject_failures(self.handle, count)
+
+    def close(self):
+        if getattr(self, "handle", None):
+            load_library().rmh_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- host-memory stage calls -------------------------------------------
+    def residuals(self, tus, coeffs, out=None):
+        tus = np.ascontiguousarray(tus, dtype=TU_DTYPE)
+        coeffs = np.ascontiguousarray(coeffs, dtype=np.int16)
+        if out is None:
+            out = np.zeros_like(coeffs)
+        assert out.dtype == np.int16 and out.flags["C_CONTIGUOUS"] and out.size == coeffs.size
+        self._check(load_library().rmh_residual_batch(self.handle, tus.ctypes.data, len(tus), coeffs.ctypes.data,
+                                                      coeffs.size, out.ctypes.data))
+        return out
+
+    def predict(self, frames, pus):
+        """frames: list of abi.McFrame (host planes); pus: PU_DTYPE array."""
+        pus = np.ascontiguousarray(pus, dtype=PU_DTYPE)
+        arr = (abi.McFrame * len(frames))(*frames)
+        self._check(load_library().rmh_mc_batch(self.handle, ctypes.addressof(arr), len(frames), pus.ctypes.data,
+                                                len(pus)))
+
+    def deblock(self, frames):
+        """frames: list of abi.DbkFrame (host planes, filtered in place)."""
+        arr = (abi.DbkFrame * len(frames))(*frames)
+        self._check(load_library().rmh_deblock(self.handle, ctypes.addressof(arr), len(frames)))
+
+    # -- device-resident plans ---------------------------------------------
+    def residual_plan(self, tus, n_coeff):
+        tus = np.ascontiguousarray(tus, dtype=TU_DTYPE)
+        h = ctypes.c_void_p()
+        self._check(load_library().rmh_residual_plan_create(self.handle, tus.ctypes.data, len(tus), n_coeff,
+                                                            ctypes.byref(h)))
+        return Plan(self, h.value, "residual")
+
+    def mc_plan(self, frames, pus):
+        pus = np.ascontiguousarray(pus, dtype=PU_DTYPE)
+        arr = (abi.McFrame * len(frames))(*frames)
+        h = ctypes.c_void_p()
+        self._check(load_library().rmh_mc_plan_create(self.handle, ctypes.addressof(arr), len(frames),
+                                                      pus.ctypes.data, len(pus), ctypes.byref(h)))
+        return Plan(self, h.value, "mc")
+
+    def deblock_plan(self, frames):
+        arr = (abi.DbkFrame * len(frames))(*frames)
+        h = ctypes.c_void_p()
+        self._check(load_library().rmh_deblock_plan_create(self.handle, ctypes.addressof(arr), len(frames),
+                                                           ctypes.byref(h)))
+        return Plan(self, h.value, "deblock")
+
+    def graph(self, nodes):
+        """nodes: list of (plan, d_coeff, d_residual) (device addresses; 0 for non-residual plans)."""
+        arr = (abi.GraphNode * len(nodes))(*[abi.GraphNode(p.handle, c, r) for p, c, r in nodes])
+        h = ctypes.c_void_p()
+        self._check(load_library().rmh_graph_create(self.handle, ctypes.addressof(arr), len(nodes), ctypes.byref(h)))
+        return Graph(self, h.value, [p for p, _, _ in nodes])
+
+
+class VvcResidualBatch:
+    """Provider with the shape of the VvcResidualBatch trait."""
+
+    def __init__(self, acc):
+        self.acc = acc
+
+    def residuals(self, tus, coeffs, out=None):
+        return self.acc.residuals(tus, coeffs, out)
+
+
+class VvcInterPredBatch:
+    """Provider with the shape of the VvcInterPredBatch trait."""
+
+    def __init__(self, acc):
+        self.acc = acc
+
+    def predict(self, frames, pus):
+        return self.acc.predict(frames, pus)
+
+
+class VvcDeblockFilter:
+    """Provider with the shape of the VvcDeblockFilter trait (planes in place;
+    untouched on error, crates/rustmpeg-video/src/av2/deblock.rs:30-35)."""
+
+    def __init__(self, acc):
+        self.acc = acc
+
+    def deblock(self, frames):
+        return self.acc.deblock(frames)

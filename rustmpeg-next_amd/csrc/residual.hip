@@ -1,0 +1,320 @@
+// residual.hip - batched dequantisation + inverse LFNST + inverse DCT-II /
+// DST-VII / DCT-VIII over every TU of a batch, for gfx950.
+//
+// Semantics follow the reference TU driver inverse_residual / residual_of
+// (crates/rustmpeg-video/src/vvc/decode_general.rs:4100-4236) over
+// dequantize_rect_bd / dequantize_dep_quant_bd / dequantize_ts
+// (vvc/dct.rs:1151-1248, vvc/transform.rs:229-246), inverse_lfnst_rect
+// (vvc/lfnst.rs:219-297) and inverse_transform_rect_bd (vvc/dct.rs:996-1109).
+//
+// Mapping: one 64-lane wavefront (one workgroup) per TU.
+//   1. the coded region (after the default, LFNST and caller zero-outs) is read
+//      once from HBM, dequantised and stored column-major in LDS as int16;
+//   2. LFNST (if any) rewrites the top-left 4x4 / 8x8 region in LDS;
+//   3. the bounding box of non-zero coefficients shrinks both passes (an exact
+//      optimisation: zero coefficients contribute nothing);
+//   4. the vertical pass writes int16 intermediates row-major to LDS, clipped
+//      exactly like vvc/dct.rs:1094; the horizontal pass writes int16 residuals
+//      to HBM. Both passes are int16 x int16 dot products with int32
+//      accumulation (v_dot2_i32_i16): |coefficient| <= 2^15, |matrix| <= 91 and
+//      at most 32 taps keep every sum below 2^27, so int32 is exact.
+#include <hip/hip_runtime.h>
+
+#include "kernels.h"
+
+namespace rmh {
+namespace {
+
+typedef short short2v __attribute__((ext_vector_type(2)));
+
+constexpr int kColStride = 17;  // dwords per LDS column / row (16 pairs + 1 pad against bank conflicts)
+
+__constant__ int16_t kInvQuant[2][6] = {{40, 45, 51, 57, 64, 72}, {57, 64, 72, 80, 90, 102}};
+
+__device__ __forceinline__ int clip16(int64_t v) { return (int)(v < -32768 ? -32768 : v > 32767 ? 32767 : v); }
+__device__ __forceinline__ int clip16i(int v) { return v < -32768 ? -32768 : v > 32767 ? 32767 : v; }
+
+__device__ __forceinline__ int dot2(uint32_t a, uint32_t b, int acc) {
+    return __builtin_amdgcn_sdot2(__builtin_bit_cast(short2v, a), __builtin_bit_cast(short2v, b), acc, false);
+}
+
+__device__ __forceinline__ int dequant_flat(int level, int scale, int rshift) {
+    int64_t v = (int64_t)level * scale;
+    if (rshift > 0) return clip16((v + ((int64_t)1 << (rshift - 1))) >> rshift);
+    return clip16((int64_t)((uint64_t)v << (-rshift)));
+}
+
+__device__ __forceinline__ int dequant_ts(int level, int qp) {
+    int per = qp / 6, rem = qp % 6;
+    int rs = 6 - per;
+    int64_t v = (int64_t)level * kInvQuant[0][rem];
+    int64_t s = rs > 0 ? (v + ((int64_t)1 << (rs - 1))) >> rs : (int64_t)((uint64_t)v << (-rs));
+    int n = (int)(uint32_t)(uint64_t)s;
+    return clip16i(n);
+}
+
+__global__ __launch_bounds__(64) void residual_kernel(ResidualArgs a) {
+    __shared__ uint32_t ct[32 * kColStride];  // column-major coefficients: column i, int16 pairs along k
+    __shared__ uint32_t tt[64 * kColStride];  // row-major intermediates: row y, int16 pairs along i
+    __shared__ int16_t lv[32 * 32];           // raw levels for dependent quantisation
+    __shared__ int bbox[2];
+
+    const int lane = threadIdx.x;
+    const rmh_tu_desc d = a.tus[a.first_tu + blockIdx.x];
+    const int lw = d.log2_w, lh = d.log2_h;
+    const int w = 1 << lw, h = 1 << lh, n = w * h;
+    const int bd = d.bit_depth;
+    const int qp = d.qp < 4 ? 4 : d.qp;
+    const int16_t* __restrict__ cin = a.coeff + d.offset;
+    int16_t* __restrict__ rout = a.residual + d.offset;
+
+    if (d.flags & RMH_TU_TRANSFORM_SKIP) {
+        for (int e = lane; e < n; e += 64) rout[e] = (int16_t)dequant_ts(cin[e], qp);
+        return;
+    }
+
+    // Effective zero-out (vvc/dct.rs:1063-1077, decode_general.rs:4191-4236).
+    const int th = d.tr_hor, tv = d.tr_ver;
+    int sw = (th != RMH_TR_DCT2 && w == 32) ? 16 : (w > 32 ? w - 32 : 0);
+    int sh = (tv != RMH_TR_DCT2 && h == 32) ? 16 : (h > 32 ? h - 32 : 0);
+    sw = d.skip_w > sw ? d.skip_w : sw;
+    sh = d.skip_h > sh ? d.skip_h : sh;
+    const bool lfnst = d.lfnst_idx != 0;
+    if (lfnst) {
+        int nz = (w == 4 || h == 4) ? 4 : 8;
+        int lsw = w - (nz < w ? nz : w), lsh = h - (nz < h ? nz : h);
+        sw = lsw > sw ? lsw : sw;
+        sh = lsh > sh ? lsh : sh;
+    }
+    const int cw = w - sw, ch = h - sh;  // coded columns / rows that feed the transform
+    // Region held in LDS: the coded region, widened to 4x4 when LFNST reads it.
+    const int rw = lfnst && cw < 4 ? 4 : cw;
+    const int rh = lfnst && ch < 4 ? 4 : ch;
+
+    // Dequantisation parameters (vvc/dct.rs:1164-1175, 1218-1234).
+    const int lsum = lw + lh;
+    const int sqrt2 = lsum & 1;
+    const int tr_shift = 15 - bd - (lsum >> 1) - sqrt2;
+
+    if (lane < 2) bbox[lane] = -1;
+    // zero the LDS region (pairs), so odd extents read zeros
+    for (int e = lane; e < 32 * kColStride; e += 64) ct[e] = 0;
+    __syncthreads();
+
+    int16_t* ct16 = reinterpret_cast<int16_t*>(ct);
+    if (d.flags & RMH_TU_DEP_QUANT) {
+        const int lw_c = w < 32 ? w : 32, lh_c = h < 32 ? h : 32;
+        for (int e = lane; e < lw_c * lh_c; e += 64) {
+            int y = e / lw_c, x = e - y * lw_c;
+            lv[e] = cin[y * w + x];
+        }
+        __syncthreads();
+        if (lane == 0) {
+            const uint16_t* scan = a.scans + a.scan_off[lw][lh];
+            const int len = a.scan_len[lw][lh];
+            int last = -1;
+            for (int s = len - 1; s >= 0; s--) {
+                int r = scan[s];
+                if (lv[(r >> lw) * lw_c + (r & (w - 1))] != 0) {
+                    last = s;
+                    break;
+                }
+            }
+            const int qdq = qp + 1, per = qdq / 6, rem = qdq % 6;
+            const int shift = 7 - per - tr_shift;
+            int64_t inv = kInvQuant[sqrt2][rem];
+            if (shift < 0) inv = (int64_t)((uint64_t)inv << (-shift));
+            const int64_t add = shift > 0 ? ((int64_t)1 << (shift - 1)) : 0;
+            const int rs = shift > 0 ? shift : 0;
+            int state = 0;
+            for (int s = last; s >= 0; s--) {
+                int r = scan[s];
+                int x = r & (w - 1), y = r >> lw;
+                int level = lv[y * lw_c + x];
+                if (level != 0) {
+                    int64_t q = 2 * (int64_t)level + (level > 0 ? -(state >> 1) : (state >> 1));
+                    if (x < rw && y < rh) ct16[x * 2 * kColStride + y] = (int16_t)clip16((q * inv + add) >> rs);
+                }
+                state = (32040 >> ((state << 2) + ((level & 1) << 1))) & 3;
+            }
+        }
+    } else {
+        const int per = qp / 6, rem = qp % 6;
+        const int rshift = 6 - (tr_shift + per);
+        const int scale = kInvQuant[sqrt2][rem];
+        // round the region width up to a power of two so rows/columns come from shifts
+        int lrw = 0;
+        while ((1 << lrw) < rw) lrw++;
+        const int total = rh << lrw;
+        for (int e = lane; e < total; e += 64) {
+            int y = e >> lrw, x = e & ((1 << lrw) - 1);
+            if (x < rw) ct16[x * 2 * kColStride + y] = (int16_t)dequant_flat(cin[y * w + x], scale, rshift);
+        }
+    }
+    __syncthreads();
+
+    if (lfnst) {
+        // inverse_lfnst_rect (vvc/lfnst.rs:219-297): inputs are the first 16
+        // positions of the 4x4 diagonal scan.
+        const int sb = (w >= 8 && h >= 8) ? 8 : 4;
+        const int tr_size = sb == 4 ? 16 : 48;
+        const int zero_out = ((w == 4 && h == 4) || (w == 8 && h == 8)) ? 8 : 16;
+        const int set = d.lfnst_mode <= 1 ? 0 : d.lfnst_mode <= 12 ? 1 : d.lfnst_mode <= 23 ? 2
+                      : d.lfnst_mode <= 44 ? 3 : d.lfnst_mode <= 55 ? 2 : 1;
+        const bool tp = (d.lfnst_mode >= 35 && d.lfnst_mode < 67) || d.lfnst_mode >= 81;
+        const int8_t* m = sb == 8 ? a.lfnst8 + ((set * 2 + (d.lfnst_idx - 1)) * 16) * 48
+                                  : a.lfnst4 + ((set * 2 + (d.lfnst_idx - 1)) * 16) * 16;
+        const int mstride = sb == 8 ? 48 : 16;
+        // 4x4 diagonal scan positions (x, y) packed as y*4+x
+        const uint8_t scan4[16] = {0, 4, 1, 8, 5, 2, 12, 9, 6, 3, 13, 10, 7, 14, 11, 15};
+        int src[16];
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            int x = scan4[i] & 3, y = scan4[i] >> 2;
+            src[i] = ct16[x * 2 * kColStride + y];
+        }
+        int outv = 0, ox = -1, oy = -1;
+        if (lane < tr_size) {
+            int sum = 0;
+            for (int i = 0; i < zero_out; i++) sum += src[i] * (int)m[i * mstride + lane];
+            outv = clip16i((sum + 64) >> 7);
+            int j = lane;
+            if (tp) {
+                if (sb == 4) {
+                    oy = j & 3;
+                    ox = j >> 2;
+                } else if (j < 32) {
+                    oy = j & 7;
+                    ox = j >> 3;
+                } else {
+                    oy = (j - 32) & 3;
+                    ox = 4 + ((j - 32) >> 2);
+                }
+            } else {
+                if (sb == 4) {
+                    oy = j >> 2;
+                    ox = j & 3;
+                } else if (j < 32) {
+                    oy = j >> 3;
+                    ox = j & 7;
+                } else {
+                    oy = 4 + ((j - 32) >> 2);
+                    ox = (j - 32) & 3;
+                }
+            }
+        }
+        __syncthreads();
+        if (lane < tr_size && ox < rw && oy < rh) ct16[ox * 2 * kColStride + oy] = (int16_t)outv;
+        __syncthreads();
+    }
+
+    // A caller zero-out narrower than the 4x4 LFNST input leaves values outside
+    // the coded region in LDS; they must not reach the transform.
+    if (rw > cw || rh > ch) {
+        for (int e = lane; e < 32 * 32; e += 64) {
+            int x = e >> 5, y = e & 31;
+            if ((x >= cw || y >= ch) && x < rw && y < rh) ct16[x * 2 * kColStride + y] = 0;
+        }
+        __syncthreads();
+    }
+
+    // Bounding box of non-zero coefficients inside the coded region.
+    {
+        int my_r = -1, my_c = -1;
+        for (int e = lane; e < 32 * 32; e += 64) {
+            int x = e >> 5, y = e & 31;
+            if (x < cw && y < ch && ct16[x * 2 * kColStride + y] != 0) {
+                my_r = y > my_r ? y : my_r;
+                my_c = x > my_c ? x : my_c;
+            }
+        }
+        if (my_r >= 0) {
+            atomicMax(&bbox[0], my_r);
+            atomicMax(&bbox[1], my_c);
+        }
+    }
+    __syncthreads();
+    const int er = bbox[0] + 1, ec = bbox[1] + 1;  // effective rows / columns
+    if (er <= 0) {
+        for (int e = lane; e < n; e += 64) rout[e] = 0;
+        return;
+    }
+    // zero the rows beyond the coded region inside the pair that contains row er
+    // (already zero: the LDS region was cleared and only coded positions written)
+
+    const int bdc = bd < 8 ? 8 : bd > 12 ? 12 : bd;
+    if (w == 1 || h == 1) {
+        // 1-D transform (vvc/dct.rs:1010-1057): one pass, shift 21 - bd, over
+        // the column (w == 1) or row (h == 1) held in LDS column 0 / row-as-column.
+        const int len = w == 1 ? h : w;
+        const int ltype = w == 1 ? tv : th;
+        const int lg = w == 1 ? lh : lw;
+        const uint32_t* mp = a.mat + a.mat_off[ltype][lg];
+        const int shift = 6 + 15 - bdc;
+        // for h == 1 the coefficients sit in row 0 of columns 0..w-1: gather to pairs
+        const int taps = w == 1 ? er : ec;
+        const int npairs = (taps + 1) >> 1;
+        for (int j = lane; j < len; j += 64) {
+            int acc = 0;
+            for (int kk = 0; kk < npairs; kk++) {
+                int c0, c1;
+                if (w == 1) {
+                    c0 = ct16[2 * kk];
+                    c1 = ct16[2 * kk + 1];
+                } else {
+                    c0 = ct16[(2 * kk) * 2 * kColStride];
+                    c1 = 2 * kk + 1 < w ? ct16[(2 * kk + 1) * 2 * kColStride] : 0;
+                }
+                uint32_t pc = (uint32_t)(uint16_t)c0 | ((uint32_t)(uint16_t)c1 << 16);
+                acc = dot2(pc, mp[kk * len + j], acc);
+            }
+            rout[j] = (int16_t)clip16i((acc + (1 << (shift - 1))) >> shift);
+        }
+        return;
+    }
+
+    // Stage 1 (vertical): tt[y][i] = clip16((sum_k Mv[k][y] * C[k][i] + 64) >> 7), i < ec.
+    {
+        const uint32_t* mv = a.mat + a.mat_off[tv][lh];
+        const int npairs = (er + 1) >> 1;
+        const int ecp = (ec + 1) & ~1;  // even column count, the odd one padded with zero
+        const int total = ecp << lh;
+        int16_t* tt16 = reinterpret_cast<int16_t*>(tt);
+        for (int e = lane; e < total; e += 64) {
+            const int y = e & (h - 1), i = e >> lh;
+            int acc = 0;
+            if (i < ec) {
+                const uint32_t* col = ct + i * kColStride;
+                for (int kk = 0; kk < npairs; kk++) acc = dot2(col[kk], mv[kk * h + y], acc);
+                acc = clip16i((acc + 64) >> 7);
+            }
+            tt16[y * 2 * kColStride + i] = (int16_t)acc;
+        }
+    }
+    __syncthreads();
+
+    // Stage 2 (horizontal): out[y][x] = clip16((sum_k Mh[k][x] * tt[y][k] + rnd) >> (20 - bd)).
+    {
+        const uint32_t* mh = a.mat + a.mat_off[th][lw];
+        const int npairs = (ec + 1) >> 1;
+        const int shift = 6 + 15 - 1 - bdc;
+        const int rnd = 1 << (shift - 1);
+        for (int e = lane; e < n; e += 64) {
+            const int x = e & (w - 1), y = e >> lw;
+            const uint32_t* row = tt + y * kColStride;
+            int acc = 0;
+            for (int kk = 0; kk < npairs; kk++) acc = dot2(row[kk], mh[kk * w + x], acc);
+            rout[e] = (int16_t)clip16i((acc + rnd) >> shift);
+        }
+    }
+}
+
+}  // namespace
+
+hipError_t launch_residual(const ResidualArgs& a, hipStream_t stream) {
+    if (a.n_tu == 0) return hipSuccess;
+    hipLaunchKernelGGL(residual_kernel, dim3(a.n_tu), dim3(64), 0, stream, a);
+    return hipGetLastError();
+}
+
+}  // namespace rmh

@@ -1,0 +1,930 @@
+// rmh.cpp - host side of librmh: context, validation, plans, frame-step
+// graphs and the host-memory entry points of include/rmh.h.
+//
+// Mirrors the reference GPU backend's contracts: typed errors and no panics on
+// hostile sizes (crates/rustmpeg-vulkan/src/compute/mod.rs:13-16, 528-567),
+// zero work is a no-op (compute/mod.rs:697), bounded waits
+// (compute/mod.rs:416-418), results reach caller memory only after success
+// (compute/deblock.rs:300-303), and a sticky failure latch
+// (crates/rustmpeg-vulkan/src/accelerator.rs:38, 163-171).
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "kernels.h"
+#include "lfnst_data.h"
+#include "tables.h"
+
+#define RMH_EXPORT extern "C" __attribute__((visibility("default")))
+
+namespace {
+
+struct Arena {
+    void* base = nullptr;
+    size_t cap = 0;
+    bool pinned = false;
+    size_t used = 0;
+    hipError_t reserve(size_t bytes) {
+        used = 0;
+        if (bytes <= cap) return hipSuccess;
+        if (base) {
+            if (pinned)
+                hipHostFree(base);
+            else
+                hipFree(base);
+            base = nullptr;
+            cap = 0;
+        }
+        size_t want = bytes + bytes / 4 + 4096;
+        hipError_t e = pinned ? hipHostMalloc(&base, want, hipHostMallocDefault) : hipMalloc(&base, want);
+        if (e == hipSuccess) cap = want;
+        return e;
+    }
+    void* take(size_t bytes) {
+        void* p = (char*)base + used;
+        used += (bytes + 255) & ~(size_t)255;
+        return p;
+    }
+    void release() {
+        if (base) {
+            if (pinned)
+                hipHostFree(base);
+            else
+                hipFree(base);
+        }
+        base = nullptr;
+        cap = 0;
+    }
+};
+
+size_t aligned(size_t b) { return (b + 255) & ~(size_t)255; }
+
+}  // namespace
+
+struct rmh_ctx {
+    int device = 0;
+    uint32_t timeout_ms = 30000;
+    std::atomic<bool> disabled{false};
+    std::atomic<uint32_t> inject{0};
+    std::mutex mu;
+    std::string last_error;
+    hipStream_t stream = nullptr;
+    void* d_tables = nullptr;
+    rmh::DeviceTables tabs{};
+    Arena dev;
+    Arena host;
+    hipDeviceProp_t props{};
+};
+
+enum PlanKind { kResidual = 0, kMc = 1, kDeblock = 2 };
+
+struct rmh_plan {
+    rmh_ctx* ctx = nullptr;
+    int kind = 0;
+    uint64_t bytes = 0;
+    void* d_mem = nullptr;
+    // residual
+    const rmh_tu_desc* d_tus = nullptr;
+    size_t n_tu = 0, n_coeff = 0;
+    // inter
+    const rmh_mc_frame* d_frames = nullptr;
+    const rmh_pu_desc* d_pus = nullptr;
+    const rmh::McItem* d_items = nullptr;
+    size_t n_items = 0;
+    // deblock
+    const rmh_dbk_frame* d_dbk = nullptr;
+    size_t n_dbk = 0;
+    uint32_t max_cells = 0;
+};
+
+struct rmh_graph {
+    rmh_ctx* ctx = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipStream_t capture = nullptr;
+};
+
+namespace {
+
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int d) {
+        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+        if (prev != d) hipSetDevice(d);
+    }
+    ~DeviceGuard() {
+        int cur = -1;
+        if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) hipSetDevice(prev);
+    }
+};
+
+rmh_status fail(rmh_ctx* ctx, rmh_status s, const std::string& msg) {
+    if (ctx) {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        ctx->last_error = msg;
+    }
+    return s;
+}
+
+// Caller already holds ctx->mu.
+rmh_status fail_locked(rmh_ctx* ctx, rmh_status s, const std::string& msg) {
+    ctx->last_error = msg;
+    return s;
+}
+
+rmh_status hip_fail_locked(rmh_ctx* ctx, hipError_t e, const char* what) {
+    return fail_locked(ctx, RMH_EXECUTION, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// Gate shared by every device call: the failure latch and the injection hook.
+rmh_status gate(rmh_ctx* ctx) {
+    if (ctx->disabled.load()) return RMH_UNAVAILABLE;
+    uint32_t n = ctx->inject.load();
+    while (n > 0) {
+        if (ctx->inject.compare_exchange_weak(n, n - 1)) return RMH_EXECUTION;
+    }
+    return RMH_OK;
+}
+
+rmh_status wait_stream(rmh_ctx* ctx, hipStream_t s) {
+    auto deadline = std::chrono::steady_clock::now() + std::chrono::milliseconds(ctx->timeout_ms);
+    for (;;) {
+        hipError_t e = hipStreamQuery(s);
+        if (e == hipSuccess) return RMH_OK;
+        if (e != hipErrorNotReady) return hip_fail_locked(ctx, e, "device execution failed");
+        if (std::chrono::steady_clock::now() > deadline)
+            return fail_locked(ctx, RMH_TIMEOUT, "device work did not finish within the timeout");
+        std::this_thread::sleep_for(std::chrono::microseconds(20));
+    }
+}
+
+// ---------------------------------------------------------------- tables
+
+int ilog2(int v) {
+    int l = 0;
+    while ((1 << l) < v) l++;
+    return l;
+}
+
+bool matrix_ok(int type, int n) {
+    bool ok;
+    rmh::transform_entry(type, n, 0, 0, &ok);
+    return ok;
+}
+
+void cg_log2(int lx, int ly, int* a, int* b) {  // coefficient_group_log2, vvc/transform.rs:100-178
+    if (lx == 0) {
+        *a = 0;
+        *b = ly < 4 ? ly : 4;
+    } else if (ly == 0) {
+        *a = lx < 4 ? lx : 4;
+        *b = 0;
+    } else if (lx == 1) {
+        *a = 1;
+        *b = ly <= 2 ? 1 : 3;
+    } else if (ly == 1) {
+        *a = lx == 2 ? 1 : 3;
+        *b = 1;
+    } else {
+        *a = 2;
+        *b = 2;
+    }
+}
+
+void diag(int w, int h, std::vector<int>& xs, std::vector<int>& ys) {  // vvc/transform.rs:189-212
+    int line = 0, col = 0;
+    for (int i = 0; i < w * h; i++) {
+        xs.push_back(col);
+        ys.push_back(line);
+        if (col == w - 1 || line == 0) {
+            line += col + 1;
+            col = 0;
+            if (line >= h) {
+                col += line - (h - 1);
+                line = h - 1;
+            }
+        } else {
+            col++;
+            line--;
+        }
+    }
+}
+
+std::vector<uint16_t> coeff_scan(int lw, int lh) {  // CoeffScan::new, vvc/transform.rs:37-79
+    int a, b;
+    cg_log2(lw, lh, &a, &b);
+    int w = 1 << lw, h = 1 << lh;
+    int gw = (w < 32 ? w : 32) >> a, gh = (h < 32 ? h : 32) >> b;
+    std::vector<int> gx, gy, ix, iy;
+    diag(gw, gh, gx, gy);
+    diag(1 << a, 1 << b, ix, iy);
+    std::vector<uint16_t> r;
+    for (size_t g = 0; g < gx.size(); g++)
+        for (size_t i = 0; i < ix.size(); i++) r.push_back((uint16_t)((gy[g] * (1 << b) + iy[i]) * w + gx[g] * (1 << a) + ix[i]));
+    return r;
+}
+
+hipError_t build_tables(rmh_ctx* ctx) {
+    std::vector<uint32_t> mat;
+    for (int t = 0; t < 3; t++)
+        for (int l = 0; l < 7; l++) {
+            int n = 1 << l;
+            ctx->tabs.mat_off[t][l] = -1;
+            if (n < 2 || !matrix_ok(t, n)) continue;
+            ctx->tabs.mat_off[t][l] = (int32_t)mat.size();
+            bool ok;
+            for (int kk = 0; kk < n / 2; kk++)
+                for (int j = 0; j < n; j++) {
+                    uint32_t lo = (uint16_t)(int16_t)rmh::transform_entry(t, n, 2 * kk, j, &ok);
+                    uint32_t hi = (uint16_t)(int16_t)rmh::transform_entry(t, n, 2 * kk + 1, j, &ok);
+                    mat.push_back(lo | (hi << 16));
+                }
+        }
+    std::vector<uint16_t> scans;
+    for (int a = 0; a < 7; a++)
+        for (int b = 0; b < 7; b++) {
+            ctx->tabs.scan_off[a][b] = (int32_t)scans.size();
+            ctx->tabs.scan_len[a][b] = 0;
+            if (a + b == 0) continue;
+            std::vector<uint16_t> s = coeff_scan(a, b);
+            ctx->tabs.scan_len[a][b] = (int32_t)s.size();
+            scans.insert(scans.end(), s.begin(), s.end());
+        }
+    size_t o_mat = 0, o_l8 = aligned(mat.size() * 4), o_l4 = o_l8 + aligned(sizeof(RMH_LFNST8));
+    size_t o_sc = o_l4 + aligned(sizeof(RMH_LFNST4)), total = o_sc + aligned(scans.size() * 2);
+    std::vector<uint8_t> blob(total, 0);
+    memcpy(blob.data() + o_mat, mat.data(), mat.size() * 4);
+    memcpy(blob.data() + o_l8, RMH_LFNST8, sizeof(RMH_LFNST8));
+    memcpy(blob.data() + o_l4, RMH_LFNST4, sizeof(RMH_LFNST4));
+    memcpy(blob.data() + o_sc, scans.data(), scans.size() * 2);
+    hipError_t e = hipMalloc(&ctx->d_tables, total);
+    if (e != hipSuccess) return e;
+    e = hipMemcpy(ctx->d_tables, blob.data(), total, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return e;
+    char* base = (char*)ctx->d_tables;
+    ctx->tabs.mat = (const uint32_t*)(base + o_mat);
+    ctx->tabs.lfnst8 = (const int8_t*)(base + o_l8);
+    ctx->tabs.lfnst4 = (const int8_t*)(base + o_l4);
+    ctx->tabs.scans = (const uint16_t*)(base + o_sc);
+    return hipSuccess;
+}
+
+// ------------------------------------------------------------ validation
+
+// Checks one TU against the contract of inverse_residual / residual_of.
+const char* check_tu(const rmh_tu_desc& t, size_t n_coeff) {
+    if (t.log2_w > 6 || t.log2_h > 6) return "transform size above 64";
+    const int w = 1 << t.log2_w, h = 1 << t.log2_h;
+    if (w * h < 2) return "transform block smaller than 2 samples";
+    if (t.bit_depth < 8 || t.bit_depth > 12) return "bit depth outside 8..12";
+    if (t.qp > 63 + 6 * 4) return "QP' above 87";
+    if (t.flags & ~(RMH_TU_TRANSFORM_SKIP | RMH_TU_DEP_QUANT)) return "unknown TU flags";
+    if ((uint64_t)t.offset + (uint64_t)(w * h) > n_coeff) return "TU block outside the coefficient buffer";
+    if (t.flags & RMH_TU_TRANSFORM_SKIP) return nullptr;
+    if (t.tr_hor > 2 || t.tr_ver > 2) return "unknown transform type";
+    if (w == 1) {
+        if (!matrix_ok(t.tr_ver, h)) return "unsupported vertical transform";
+    } else if (h == 1) {
+        if (!matrix_ok(t.tr_hor, w)) return "unsupported horizontal transform";
+    } else if (!matrix_ok(t.tr_hor, w) || !matrix_ok(t.tr_ver, h)) {
+        return "unsupported transform type for the block size";
+    }
+    if (t.lfnst_idx > 2) return "LFNST index outside 0..2";
+    if (t.lfnst_idx) {
+        if (w < 4 || h < 4) return "LFNST requires width and height >= 4";
+        if (t.lfnst_mode > 94) return "LFNST intra mode outside 0..94";
+    }
+    int sw = (t.tr_hor != RMH_TR_DCT2 && w == 32) ? 16 : (w > 32 ? w - 32 : 0);
+    int sh = (t.tr_ver != RMH_TR_DCT2 && h == 32) ? 16 : (h > 32 ? h - 32 : 0);
+    sw = t.skip_w > sw ? t.skip_w : sw;
+    sh = t.skip_h > sh ? t.skip_h : sh;
+    if (t.lfnst_idx) {
+        int nz = (w == 4 || h == 4) ? 4 : 8;
+        sw = w - nz > sw ? w - nz : sw;
+        sh = h - nz > sh ? h - nz : sh;
+    }
+    if ((w > 1 && sw >= w) || (h > 1 && sh >= h)) return "transform zero-out leaves no coefficients";
+    return nullptr;
+}
+
+rmh_status validate_tus(rmh_ctx* ctx, const rmh_tu_desc* tus, size_t n_tu, size_t n_coeff, bool locked) {
+    if (n_tu > 0 && !tus) return (locked ? fail_locked : fail)(ctx, RMH_INVALID_INPUT, "null TU array");
+    if (n_tu > 0x7fffffffu) return (locked ? fail_locked : fail)(ctx, RMH_LIMIT_EXCEEDED, "more than 2^31-1 TUs");
+    if (n_coeff > 0xffffffffull) return (locked ? fail_locked : fail)(ctx, RMH_LIMIT_EXCEEDED, "more than 2^32 coefficients");
+    for (size_t i = 0; i < n_tu; i++) {
+        const char* why = check_tu(tus[i], n_coeff);
+        if (why) {
+            char buf[160];
+            snprintf(buf, sizeof buf, "TU %zu: %s", i, why);
+            return (locked ? fail_locked : fail)(ctx, RMH_INVALID_INPUT, buf);
+        }
+    }
+    return RMH_OK;
+}
+
+uint64_t residual_bytes(const rmh_tu_desc* tus, size_t n_tu) {
+    uint64_t s = 0;
+    for (size_t i = 0; i < n_tu; i++) s += (uint64_t)1 << (tus[i].log2_w + tus[i].log2_h);
+    return 4 * s + 16 * (uint64_t)n_tu;
+}
+
+bool plane_ok(const rmh_plane& p) {
+    return p.data && p.width > 0 && p.height > 0 && p.stride >= p.width && (p.bytes_per_sample == 1 || p.bytes_per_sample == 2) &&
+           p.bit_depth >= 8 && p.bit_depth <= 12 && (p.bytes_per_sample == 2 || p.bit_depth == 8) && p.width <= 65535 &&
+           p.height <= 65535;
+}
+
+rmh_status validate_mc(rmh_ctx* ctx, const rmh_mc_frame* frames, size_t n_frames, const rmh_pu_desc* pus, size_t n_pu) {
+    if ((n_frames && !frames) || (n_pu && !pus)) return fail_locked(ctx, RMH_INVALID_INPUT, "null frame or PU array");
+    if (n_frames > 65536) return fail_locked(ctx, RMH_LIMIT_EXCEEDED, "more than 65536 frames");
+    for (size_t f = 0; f < n_frames; f++) {
+        const rmh_mc_frame& fr = frames[f];
+        for (int c = 0; c < 3; c++)
+            if (!plane_ok(fr.ref[c]) || !plane_ok(fr.dst[c]))
+                return fail_locked(ctx, RMH_INVALID_INPUT, "frame " + std::to_string(f) + ": invalid plane");
+        for (int c = 0; c < 3; c++) {
+            if (fr.ref[c].bit_depth != fr.ref[0].bit_depth || fr.dst[c].bit_depth != fr.ref[0].bit_depth ||
+                fr.dst[c].bytes_per_sample != fr.ref[c].bytes_per_sample)
+                return fail_locked(ctx, RMH_INVALID_INPUT, "frame " + std::to_string(f) + ": mixed sample formats");
+            if (fr.dst[c].width != fr.ref[c].width || fr.dst[c].height != fr.ref[c].height)
+                return fail_locked(ctx, RMH_INVALID_INPUT, "frame " + std::to_string(f) + ": ref/dst size mismatch");
+        }
+        if (fr.ref[1].width != (fr.ref[0].width + 1) / 2 || fr.ref[1].height != (fr.ref[0].height + 1) / 2 ||
+            fr.ref[2].width != fr.ref[1].width || fr.ref[2].height != fr.ref[1].height)
+            return fail_locked(ctx, RMH_INVALID_INPUT, "frame " + std::to_string(f) + ": chroma planes are not 4:2:0");
+    }
+    for (size_t i = 0; i < n_pu; i++) {
+        const rmh_pu_desc& p = pus[i];
+        int lw = p.log2_w & 15, lh = p.log2_h & 15;
+        if ((p.log2_w & ~(15 | RMH_PU_ALT_HPEL)) || lw < 3 || lw > 7 || lh < 3 || lh > 7)
+            return fail_locked(ctx, RMH_INVALID_INPUT, "PU " + std::to_string(i) + ": size outside 8..128");
+        if (p.frame >= n_frames) return fail_locked(ctx, RMH_INVALID_INPUT, "PU " + std::to_string(i) + ": frame index");
+        const rmh_plane& y = frames[p.frame].ref[0];
+        if (p.x >= y.width || p.y >= y.height)
+            return fail_locked(ctx, RMH_INVALID_INPUT, "PU " + std::to_string(i) + ": outside the picture");
+        if (p.mv_x < -(1 << 17) || p.mv_x >= (1 << 17) || p.mv_y < -(1 << 17) || p.mv_y >= (1 << 17))
+            return fail_locked(ctx, RMH_INVALID_INPUT, "PU " + std::to_string(i) + ": MV outside 18-bit storage");
+    }
+    return RMH_OK;
+}
+
+rmh_status validate_dbk(rmh_ctx* ctx, const rmh_dbk_frame* frames, size_t n_frames) {
+    if (n_frames && !frames) return fail_locked(ctx, RMH_INVALID_INPUT, "null frame array");
+    if (n_frames > 65535) return fail_locked(ctx, RMH_LIMIT_EXCEEDED, "more than 65535 frames");
+    for (size_t f = 0; f < n_frames; f++) {
+        const rmh_dbk_frame& d = frames[f];
+        for (int c = 0; c < 3; c++)
+            if (!plane_ok(d.planes[c]) || d.planes[c].bit_depth != d.planes[0].bit_depth)
+                return fail_locked(ctx, RMH_INVALID_INPUT, "frame " + std::to_string(f) + ": invalid plane");
+        if (!d.edges) return fail_locked(ctx, RMH_INVALID_INPUT, "frame " + std::to_string(f) + ": null edge map");
+        if (d.grid_w != (d.planes[0].width + 3) / 4 || d.grid_h != (d.planes[0].height + 3) / 4)
+            return fail_locked(ctx, RMH_INVALID_INPUT, "frame " + std::to_string(f) + ": edge grid size");
+        if (d.planes[1].width != (d.planes[0].width + 1) / 2 || d.planes[1].height != (d.planes[0].height + 1) / 2 ||
+            d.planes[2].width != d.planes[1].width || d.planes[2].height != d.planes[1].height)
+            return fail_locked(ctx, RMH_INVALID_INPUT, "frame " + std::to_string(f) + ": chroma planes are not 4:2:0");
+    }
+    return RMH_OK;
+}
+
+std::vector<rmh::McItem> mc_items(const rmh_pu_desc* pus, size_t n_pu) {
+    std::vector<rmh::McItem> items;
+    for (size_t i = 0; i < n_pu; i++) {
+        int w = 1 << (pus[i].log2_w & 15), h = 1 << (pus[i].log2_h & 15);
+        for (int ty = 0; ty < (h + 15) / 16; ty++)
+            for (int tx = 0; tx < (w + 15) / 16; tx++) items.push_back({(uint32_t)i, (uint16_t)tx, (uint16_t)ty});
+    }
+    return items;
+}
+
+uint64_t mc_bytes(const rmh_mc_frame* frames, const rmh_pu_desc* pus, size_t n_pu) {
+    uint64_t b = 0;
+    for (size_t i = 0; i < n_pu; i++) {
+        uint64_t w = 1u << (pus[i].log2_w & 15), h = 1u << (pus[i].log2_h & 15);
+        uint64_t bps = frames[pus[i].frame].ref[0].bytes_per_sample;
+        b += 2 * bps * (w * h + 2 * (w / 2) * (h / 2)) + 16;
+    }
+    return b;
+}
+
+uint64_t plane_bytes(const rmh_plane& p) { return (uint64_t)p.width * p.height * p.bytes_per_sample; }
+
+rmh_status run_plan_locked_free(rmh_plan* plan, const int16_t* d_coeff, int16_t* d_residual, size_t first, size_t count,
+                                hipStream_t s, std::string* err) {
+    hipError_t e = hipSuccess;
+    if (plan->kind == kResidual) {
+        rmh::ResidualArgs a{};
+        a.tus = plan->d_tus;
+        a.coeff = d_coeff;
+        a.residual = d_residual;
+        a.mat = plan->ctx->tabs.mat;
+        a.lfnst8 = plan->ctx->tabs.lfnst8;
+        a.lfnst4 = plan->ctx->tabs.lfnst4;
+        a.scans = plan->ctx->tabs.scans;
+        a.first_tu = (uint32_t)first;
+        a.n_tu = (uint32_t)count;
+        memcpy(a.mat_off, plan->ctx->tabs.mat_off, sizeof a.mat_off);
+        memcpy(a.scan_off, plan->ctx->tabs.scan_off, sizeof a.scan_off);
+        memcpy(a.scan_len, plan->ctx->tabs.scan_len, sizeof a.scan_len);
+        e = rmh::launch_residual(a, s);
+    } else if (plan->kind == kMc) {
+        e = rmh::launch_mc(plan->d_frames, plan->d_pus, plan->d_items, (uint32_t)plan->n_items, s);
+    } else {
+        e = rmh::launch_deblock(plan->d_dbk, (uint32_t)plan->n_dbk, plan->max_cells, s);
+    }
+    if (e != hipSuccess) {
+        *err = std::string("kernel launch failed: ") + hipGetErrorString(e);
+        return RMH_EXECUTION;
+    }
+    return RMH_OK;
+}
+
+rmh_status run_plan(rmh_plan* plan, const int16_t* d_coeff, int16_t* d_residual, size_t first, size_t count, void* stream) {
+    rmh_ctx* ctx = plan->ctx;
+    rmh_status g = gate(ctx);
+    if (g != RMH_OK) return fail(ctx, g, g == RMH_UNAVAILABLE ? "accelerator disabled" : "injected device failure");
+    DeviceGuard dg(ctx->device);
+    std::string err;
+    rmh_status s = run_plan_locked_free(plan, d_coeff, d_residual, first, count, (hipStream_t)stream, &err);
+    if (s != RMH_OK) return fail(ctx, s, err);
+    return RMH_OK;
+}
+
+}  // namespace
+
+// ======================================================================
+// C ABI
+// ======================================================================
+
+RMH_EXPORT rmh_status rmh_create(const rmh_config* cfg, rmh_ctx** out) {
+    if (!out) return RMH_INVALID_INPUT;
+    *out = nullptr;
+    rmh_config c{0, 0, 0};
+    if (cfg) c = *cfg;
+    if (c.flags != 0) return RMH_INVALID_INPUT;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return RMH_UNAVAILABLE;
+    if (c.device < 0 || c.device >= n) return RMH_INVALID_INPUT;
+    rmh_ctx* ctx = new rmh_ctx();
+    ctx->device = c.device;
+    ctx->timeout_ms = c.timeout_ms ? c.timeout_ms : 30000;
+    ctx->host.pinned = true;
+    DeviceGuard dg(ctx->device);
+    if (hipGetDeviceProperties(&ctx->props, ctx->device) != hipSuccess ||
+        strncmp(ctx->props.gcnArchName, "gfx950", 6) != 0) {
+        delete ctx;
+        return RMH_UNAVAILABLE;
+    }
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess || build_tables(ctx) != hipSuccess) {
+        if (ctx->stream) hipStreamDestroy(ctx->stream);
+        if (ctx->d_tables) hipFree(ctx->d_tables);
+        delete ctx;
+        return RMH_UNAVAILABLE;
+    }
+    *out = ctx;
+    return RMH_OK;
+}
+
+RMH_EXPORT void rmh_destroy(rmh_ctx* ctx) {
+    if (!ctx) return;
+    DeviceGuard dg(ctx->device);
+    hipStreamSynchronize(ctx->stream);
+    ctx->dev.release();
+    ctx->host.release();
+    if (ctx->d_tables) hipFree(ctx->d_tables);
+    hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+RMH_EXPORT const char* rmh_last_error(const rmh_ctx* ctx) {
+    static thread_local std::string copy;
+    if (!ctx) return "null context";
+    rmh_ctx* c = const_cast<rmh_ctx*>(ctx);
+    std::lock_guard<std::mutex> lk(c->mu);
+    copy = c->last_error;
+    return copy.c_str();
+}
+
+RMH_EXPORT rmh_status rmh_info(rmh_ctx* ctx, rmh_device_info* out) {
+    if (!ctx || !out) return RMH_INVALID_INPUT;
+    memset(out, 0, sizeof *out);
+    snprintf(out->name, sizeof out->name, "%s", ctx->props.name);
+    snprintf(out->arch, sizeof out->arch, "%s", ctx->props.gcnArchName);
+    out->compute_units = ctx->props.multiProcessorCount;
+    out->max_clock_khz = ctx->props.clockRate;
+    out->global_mem_bytes = ctx->props.totalGlobalMem;
+    out->abi_version = RMH_ABI_VERSION;
+    return RMH_OK;
+}
+
+RMH_EXPORT void rmh_disable(rmh_ctx* ctx) {
+    if (ctx) ctx->disabled.store(true);
+}
+
+RMH_EXPORT int rmh_is_disabled(const rmh_ctx* ctx) { return ctx ? (int)ctx->disabled.load() : 1; }
+
+RMH_EXPORT void rmh_inject_failures(rmh_ctx* ctx, uint32_t count) {
+    if (ctx) ctx->inject.store(count);
+}
+
+RMH_EXPORT rmh_status rmh_transform_matrix(int32_t type, int32_t n, int32_t* out) {
+    if (!out || n < 2 || n > 64 || (n & (n - 1)) || !matrix_ok(type, n)) return RMH_INVALID_INPUT;
+    bool ok;
+    for (int k = 0; k < n; k++)
+        for (int j = 0; j < n; j++) out[k * n + j] = rmh::transform_entry(type, n, k, j, &ok);
+    return RMH_OK;
+}
+
+// ---------------------------------------------------------------- residual
+
+RMH_EXPORT rmh_status rmh_residual_plan_create(rmh_ctx* ctx, const rmh_tu_desc* tus, size_t n_tu, size_t n_coeff,
+                                               rmh_plan** out) {
+    if (!ctx || !out) return RMH_INVALID_INPUT;
+    *out = nullptr;
+    rmh_status v = validate_tus(ctx, tus, n_tu, n_coeff, false);
+    if (v != RMH_OK) return v;
+    if (ctx->disabled.load()) return fail(ctx, RMH_UNAVAILABLE, "accelerator disabled");
+    DeviceGuard dg(ctx->device);
+    rmh_plan* p = new rmh_plan();
+    p->ctx = ctx;
+    p->kind = kResidual;
+    p->n_tu = n_tu;
+    p->n_coeff = n_coeff;
+    p->bytes = residual_bytes(tus, n_tu);
+    if (n_tu) {
+        hipError_t e = hipMalloc(&p->d_mem, n_tu * sizeof(rmh_tu_desc));
+        if (e == hipSuccess) e = hipMemcpy(p->d_mem, tus, n_tu * sizeof(rmh_tu_desc), hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            if (p->d_mem) hipFree(p->d_mem);
+            delete p;
+            return fail(ctx, RMH_EXECUTION, std::string("descriptor upload failed: ") + hipGetErrorString(e));
+        }
+        p->d_tus = (const rmh_tu_desc*)p->d_mem;
+    }
+    *out = p;
+    return RMH_OK;
+}
+
+RMH_EXPORT rmh_status rmh_residual_plan_run(rmh_plan* plan, const int16_t* d_coeff, int16_t* d_residual, void* stream) {
+    if (!plan || plan->kind != kResidual) return RMH_INVALID_INPUT;
+    if (plan->n_tu == 0) return RMH_OK;
+    if (!d_coeff || !d_residual) return fail(plan->ctx, RMH_INVALID_INPUT, "null device buffer");
+    return run_plan(plan, d_coeff, d_residual, 0, plan->n_tu, stream);
+}
+
+RMH_EXPORT rmh_status rmh_residual_plan_run_range(rmh_plan* plan, size_t first, size_t count, const int16_t* d_coeff,
+                                                  int16_t* d_residual, void* stream) {
+    if (!plan || plan->kind != kResidual) return RMH_INVALID_INPUT;
+    if (first > plan->n_tu || count > plan->n_tu - first) return fail(plan->ctx, RMH_INVALID_INPUT, "TU range outside the plan");
+    if (count == 0) return RMH_OK;
+    if (!d_coeff || !d_residual) return fail(plan->ctx, RMH_INVALID_INPUT, "null device buffer");
+    return run_plan(plan, d_coeff, d_residual, first, count, stream);
+}
+
+RMH_EXPORT rmh_status rmh_residual_batch(rmh_ctx* ctx, const rmh_tu_desc* tus, size_t n_tu, const int16_t* coeff,
+                                         size_t n_coeff, int16_t* residual) {
+    if (!ctx) return RMH_INVALID_INPUT;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    rmh_status v = validate_tus(ctx, tus, n_tu, n_coeff, true);
+    if (v != RMH_OK) return v;
+    if (n_tu == 0) return RMH_OK;
+    if (!coeff || !residual) return fail_locked(ctx, RMH_INVALID_INPUT, "null coefficient or residual buffer");
+    rmh_status g = gate(ctx);
+    if (g != RMH_OK) return fail_locked(ctx, g, g == RMH_UNAVAILABLE ? "accelerator disabled" : "injected device failure");
+    DeviceGuard dg(ctx->device);
+    size_t tb = n_tu * sizeof(rmh_tu_desc), cb = n_coeff * sizeof(int16_t);
+    hipError_t e = ctx->dev.reserve(aligned(tb) + 2 * aligned(cb));
+    if (e != hipSuccess) return fail_locked(ctx, RMH_LIMIT_EXCEEDED, std::string("device allocation: ") + hipGetErrorString(e));
+    e = ctx->host.reserve(aligned(cb));
+    if (e != hipSuccess) return fail_locked(ctx, RMH_LIMIT_EXCEEDED, std::string("host staging: ") + hipGetErrorString(e));
+    rmh_tu_desc* d_tus = (rmh_tu_desc*)ctx->dev.take(tb);
+    int16_t* d_coeff = (int16_t*)ctx->dev.take(cb);
+    int16_t* d_res = (int16_t*)ctx->dev.take(cb);
+    int16_t* h_res = (int16_t*)ctx->host.take(cb);
+    hipStream_t s = ctx->stream;
+    if ((e = hipMemcpyAsync(d_tus, tus, tb, hipMemcpyHostToDevice, s)) != hipSuccess ||
+        (e = hipMemcpyAsync(d_coeff, coeff, cb, hipMemcpyHostToDevice, s)) != hipSuccess)
+        return hip_fail_locked(ctx, e, "upload");
+    rmh_plan tmp;
+    tmp.ctx = ctx;
+    tmp.kind = kResidual;
+    tmp.d_tus = d_tus;
+    tmp.n_tu = n_tu;
+    std::string err;
+    rmh_status r = run_plan_locked_free(&tmp, d_coeff, d_res, 0, n_tu, s, &err);
+    if (r != RMH_OK) return fail_locked(ctx, r, err);
+    if ((e = hipMemcpyAsync(h_res, d_res, cb, hipMemcpyDeviceToHost, s)) != hipSuccess) return hip_fail_locked(ctx, e, "download");
+    r = wait_stream(ctx, s);
+    if (r != RMH_OK) return r;
+    // samples not covered by any TU keep the caller's values
+    for (size_t i = 0; i < n_tu; i++) {
+        size_t n = (size_t)1 << (tus[i].log2_w + tus[i].log2_h);
+        memcpy(residual + tus[i].offset, h_res + tus[i].offset, n * sizeof(int16_t));
+    }
+    return RMH_OK;
+}
+
+// -------------------------------------------------------------------- inter
+
+RMH_EXPORT rmh_status rmh_mc_plan_create(rmh_ctx* ctx, const rmh_mc_frame* frames, size_t n_frames, const rmh_pu_desc* pus,
+                                         size_t n_pu, rmh_plan** out) {
+    if (!ctx || !out) return RMH_INVALID_INPUT;
+    *out = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        rmh_status v = validate_mc(ctx, frames, n_frames, pus, n_pu);
+        if (v != RMH_OK) return v;
+    }
+    if (ctx->disabled.load()) return fail(ctx, RMH_UNAVAILABLE, "accelerator disabled");
+    DeviceGuard dg(ctx->device);
+    std::vector<rmh::McItem> items = mc_items(pus, n_pu);
+    if (items.size() > 0x7fffffffu) return fail(ctx, RMH_LIMIT_EXCEEDED, "too many inter work items");
+    size_t fb = n_frames * sizeof(rmh_mc_frame), pb = n_pu * sizeof(rmh_pu_desc), ib = items.size() * sizeof(rmh::McItem);
+    rmh_plan* p = new rmh_plan();
+    p->ctx = ctx;
+    p->kind = kMc;
+    p->n_items = items.size();
+    p->bytes = mc_bytes(frames, pus, n_pu);
+    hipError_t e = hipMalloc(&p->d_mem, aligned(fb) + aligned(pb) + aligned(ib) + 256);
+    char* base = (char*)p->d_mem;
+    if (e == hipSuccess && fb) e = hipMemcpy(base, frames, fb, hipMemcpyHostToDevice);
+    if (e == hipSuccess && pb) e = hipMemcpy(base + aligned(fb), pus, pb, hipMemcpyHostToDevice);
+    if (e == hipSuccess && ib) e = hipMemcpy(base + aligned(fb) + aligned(pb), items.data(), ib, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        if (p->d_mem) hipFree(p->d_mem);
+        delete p;
+        return fail(ctx, RMH_EXECUTION, std::string("inter plan upload failed: ") + hipGetErrorString(e));
+    }
+    p->d_frames = (const rmh_mc_frame*)base;
+    p->d_pus = (const rmh_pu_desc*)(base + aligned(fb));
+    p->d_items = (const rmh::McItem*)(base + aligned(fb) + aligned(pb));
+    *out = p;
+    return RMH_OK;
+}
+
+RMH_EXPORT rmh_status rmh_mc_plan_run(rmh_plan* plan, void* stream) {
+    if (!plan || plan->kind != kMc) return RMH_INVALID_INPUT;
+    if (plan->n_items == 0) return RMH_OK;
+    return run_plan(plan, nullptr, nullptr, 0, 0, stream);
+}
+
+RMH_EXPORT rmh_status rmh_mc_batch(rmh_ctx* ctx, const rmh_mc_frame* frames, size_t n_frames, const rmh_pu_desc* pus,
+                                   size_t n_pu) {
+    if (!ctx) return RMH_INVALID_INPUT;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    rmh_status v = validate_mc(ctx, frames, n_frames, pus, n_pu);
+    if (v != RMH_OK) return v;
+    if (n_pu == 0) return RMH_OK;
+    rmh_status g = gate(ctx);
+    if (g != RMH_OK) return fail_locked(ctx, g, g == RMH_UNAVAILABLE ? "accelerator disabled" : "injected device failure");
+    DeviceGuard dg(ctx->device);
+    std::vector<rmh::McItem> items = mc_items(pus, n_pu);
+    size_t fb = n_frames * sizeof(rmh_mc_frame), pb = n_pu * sizeof(rmh_pu_desc), ib = items.size() * sizeof(rmh::McItem);
+    size_t need = aligned(fb) + aligned(pb) + aligned(ib), hneed = 0;
+    for (size_t f = 0; f < n_frames; f++)
+        for (int c = 0; c < 3; c++) {
+            need += 2 * aligned(plane_bytes(frames[f].ref[c]));
+            hneed += aligned(plane_bytes(frames[f].dst[c]));
+        }
+    hipError_t e = ctx->dev.reserve(need);
+    if (e != hipSuccess) return fail_locked(ctx, RMH_LIMIT_EXCEEDED, std::string("device allocation: ") + hipGetErrorString(e));
+    e = ctx->host.reserve(hneed);
+    if (e != hipSuccess) return fail_locked(ctx, RMH_LIMIT_EXCEEDED, std::string("host staging: ") + hipGetErrorString(e));
+    hipStream_t s = ctx->stream;
+    rmh_mc_frame* d_frames = (rmh_mc_frame*)ctx->dev.take(fb);
+    rmh_pu_desc* d_pus = (rmh_pu_desc*)ctx->dev.take(pb);
+    rmh::McItem* d_items = (rmh::McItem*)ctx->dev.take(ib);
+    std::vector<rmh_mc_frame> dev_frames(frames, frames + n_frames);
+    std::vector<void*> staged;
+    for (size_t f = 0; f < n_frames; f++)
+        for (int c = 0; c < 3; c++) {
+            const rmh_plane& r = frames[f].ref[c];
+            void* dr = ctx->dev.take(plane_bytes(r));
+            void* dd = ctx->dev.take(plane_bytes(r));
+            e = hipMemcpy2DAsync(dr, (size_t)r.width * r.bytes_per_sample, r.data, (size_t)r.stride * r.bytes_per_sample,
+                                 (size_t)r.width * r.bytes_per_sample, r.height, hipMemcpyHostToDevice, s);
+            if (e != hipSuccess) return hip_fail_locked(ctx, e, "reference upload");
+            dev_frames[f].ref[c].data = dr;
+            dev_frames[f].ref[c].stride = r.width;
+            dev_frames[f].dst[c].data = dd;
+            dev_frames[f].dst[c].stride = r.width;
+        }
+    if ((e = hipMemcpyAsync(d_frames, dev_frames.data(), fb, hipMemcpyHostToDevice, s)) != hipSuccess ||
+        (e = hipMemcpyAsync(d_pus, pus, pb, hipMemcpyHostToDevice, s)) != hipSuccess ||
+        (e = hipMemcpyAsync(d_items, items.data(), ib, hipMemcpyHostToDevice, s)) != hipSuccess)
+        return hip_fail_locked(ctx, e, "upload");
+    e = rmh::launch_mc(d_frames, d_pus, d_items, (uint32_t)items.size(), s);
+    if (e != hipSuccess) return hip_fail_locked(ctx, e, "kernel launch");
+    std::vector<void*> hdst;
+    for (size_t f = 0; f < n_frames; f++)
+        for (int c = 0; c < 3; c++) {
+            const rmh_plane& d = dev_frames[f].dst[c];
+            void* h = ctx->host.take(plane_bytes(d));
+            hdst.push_back(h);
+            if ((e = hipMemcpyAsync(h, d.data, plane_bytes(d), hipMemcpyDeviceToHost, s)) != hipSuccess)
+                return hip_fail_locked(ctx, e, "download");
+        }
+    rmh_status r = wait_stream(ctx, s);
+    if (r != RMH_OK) return r;
+    // Copy back only the samples some PU predicted; the rest keep the caller's values.
+    for (size_t f = 0, k = 0; f < n_frames; f++)
+        for (int c = 0; c < 3; c++, k++) {
+            const rmh_plane& d = frames[f].dst[c];
+            const uint8_t* src = (const uint8_t*)hdst[k];
+            for (size_t i = 0; i < n_pu; i++) {
+                if (pus[i].frame != f) continue;
+                int sx = c ? 1 : 0;
+                uint32_t x0 = pus[i].x >> sx, y0 = pus[i].y >> sx;
+                uint32_t w = (1u << (pus[i].log2_w & 15)) >> sx, h = (1u << (pus[i].log2_h & 15)) >> sx;
+                if (x0 >= d.width || y0 >= d.height) continue;
+                if (x0 + w > d.width) w = d.width - x0;
+                if (y0 + h > d.height) h = d.height - y0;
+                for (uint32_t y = 0; y < h; y++)
+                    memcpy((uint8_t*)d.data + ((size_t)(y0 + y) * d.stride + x0) * d.bytes_per_sample,
+                           src + ((size_t)(y0 + y) * d.width + x0) * d.bytes_per_sample, (size_t)w * d.bytes_per_sample);
+            }
+        }
+    return RMH_OK;
+}
+
+// ------------------------------------------------------------------ deblock
+
+RMH_EXPORT rmh_status rmh_deblock_plan_create(rmh_ctx* ctx, const rmh_dbk_frame* frames, size_t n_frames, rmh_plan** out) {
+    if (!ctx || !out) return RMH_INVALID_INPUT;
+    *out = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        rmh_status v = validate_dbk(ctx, frames, n_frames);
+        if (v != RMH_OK) return v;
+    }
+    if (ctx->disabled.load()) return fail(ctx, RMH_UNAVAILABLE, "accelerator disabled");
+    DeviceGuard dg(ctx->device);
+    rmh_plan* p = new rmh_plan();
+    p->ctx = ctx;
+    p->kind = kDeblock;
+    p->n_dbk = n_frames;
+    for (size_t f = 0; f < n_frames; f++) {
+        uint32_t cells = frames[f].grid_w * frames[f].grid_h;
+        if (cells > p->max_cells) p->max_cells = cells;
+        for (int c = 0; c < 3; c++) p->bytes += 2 * plane_bytes(frames[f].planes[c]);
+        p->bytes += 4ull * cells;
+    }
+    size_t fb = n_frames * sizeof(rmh_dbk_frame);
+    hipError_t e = hipMalloc(&p->d_mem, fb + 256);
+    if (e == hipSuccess && fb) e = hipMemcpy(p->d_mem, frames, fb, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        if (p->d_mem) hipFree(p->d_mem);
+        delete p;
+        return fail(ctx, RMH_EXECUTION, std::string("deblock plan upload failed: ") + hipGetErrorString(e));
+    }
+    p->d_dbk = (const rmh_dbk_frame*)p->d_mem;
+    *out = p;
+    return RMH_OK;
+}
+
+RMH_EXPORT rmh_status rmh_deblock_plan_run(rmh_plan* plan, void* stream) {
+    if (!plan || plan->kind != kDeblock) return RMH_INVALID_INPUT;
+    if (plan->n_dbk == 0) return RMH_OK;
+    return run_plan(plan, nullptr, nullptr, 0, 0, stream);
+}
+
+RMH_EXPORT rmh_status rmh_deblock(rmh_ctx* ctx, const rmh_dbk_frame* frames, size_t n_frames) {
+    if (!ctx) return RMH_INVALID_INPUT;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    rmh_status v = validate_dbk(ctx, frames, n_frames);
+    if (v != RMH_OK) return v;
+    if (n_frames == 0) return RMH_OK;
+    rmh_status g = gate(ctx);
+    if (g != RMH_OK) return fail_locked(ctx, g, g == RMH_UNAVAILABLE ? "accelerator disabled" : "injected device failure");
+    DeviceGuard dg(ctx->device);
+    size_t fb = n_frames * sizeof(rmh_dbk_frame), need = aligned(fb), hneed = 0;
+    uint32_t max_cells = 0;
+    for (size_t f = 0; f < n_frames; f++) {
+        uint32_t cells = frames[f].grid_w * frames[f].grid_h;
+        max_cells = cells > max_cells ? cells : max_cells;
+        need += aligned(4ull * cells);
+        for (int c = 0; c < 3; c++) {
+            need += aligned(plane_bytes(frames[f].planes[c]));
+            hneed += aligned(plane_bytes(frames[f].planes[c]));
+        }
+    }
+    hipError_t e = ctx->dev.reserve(need);
+    if (e != hipSuccess) return fail_locked(ctx, RMH_LIMIT_EXCEEDED, std::string("device allocation: ") + hipGetErrorString(e));
+    e = ctx->host.reserve(hneed);
+    if (e != hipSuccess) return fail_locked(ctx, RMH_LIMIT_EXCEEDED, std::string("host staging: ") + hipGetErrorString(e));
+    hipStream_t s = ctx->stream;
+    rmh_dbk_frame* d_frames = (rmh_dbk_frame*)ctx->dev.take(fb);
+    std::vector<rmh_dbk_frame> df(frames, frames + n_frames);
+    for (size_t f = 0; f < n_frames; f++) {
+        size_t eb = 4ull * frames[f].grid_w * frames[f].grid_h;
+        void* de = ctx->dev.take(eb);
+        if ((e = hipMemcpyAsync(de, frames[f].edges, eb, hipMemcpyHostToDevice, s)) != hipSuccess)
+            return hip_fail_locked(ctx, e, "edge upload");
+        df[f].edges = (const uint8_t*)de;
+        for (int c = 0; c < 3; c++) {
+            const rmh_plane& p = frames[f].planes[c];
+            void* dp = ctx->dev.take(plane_bytes(p));
+            e = hipMemcpy2DAsync(dp, (size_t)p.width * p.bytes_per_sample, p.data, (size_t)p.stride * p.bytes_per_sample,
+                                 (size_t)p.width * p.bytes_per_sample, p.height, hipMemcpyHostToDevice, s);
+            if (e != hipSuccess) return hip_fail_locked(ctx, e, "plane upload");
+            df[f].planes[c].data = dp;
+            df[f].planes[c].stride = p.width;
+        }
+    }
+    if ((e = hipMemcpyAsync(d_frames, df.data(), fb, hipMemcpyHostToDevice, s)) != hipSuccess) return hip_fail_locked(ctx, e, "upload");
+    e = rmh::launch_deblock(d_frames, (uint32_t)n_frames, max_cells, s);
+    if (e != hipSuccess) return hip_fail_locked(ctx, e, "kernel launch");
+    std::vector<void*> hp;
+    for (size_t f = 0; f < n_frames; f++)
+        for (int c = 0; c < 3; c++) {
+            const rmh_plane& p = df[f].planes[c];
+            void* h = ctx->host.take(plane_bytes(p));
+            hp.push_back(h);
+            if ((e = hipMemcpyAsync(h, p.data, plane_bytes(p), hipMemcpyDeviceToHost, s)) != hipSuccess)
+                return hip_fail_locked(ctx, e, "download");
+        }
+    rmh_status r = wait_stream(ctx, s);
+    if (r != RMH_OK) return r;
+    for (size_t f = 0, k = 0; f < n_frames; f++)
+        for (int c = 0; c < 3; c++, k++) {
+            const rmh_plane& p = frames[f].planes[c];
+            for (uint32_t y = 0; y < p.height; y++)
+                memcpy((uint8_t*)p.data + (size_t)y * p.stride * p.bytes_per_sample,
+                       (const uint8_t*)hp[k] + (size_t)y * p.width * p.bytes_per_sample, (size_t)p.width * p.bytes_per_sample);
+        }
+    return RMH_OK;
+}
+
+// ------------------------------------------------------------ plans, graphs
+
+RMH_EXPORT void rmh_plan_destroy(rmh_plan* plan) {
+    if (!plan) return;
+    DeviceGuard dg(plan->ctx->device);
+    if (plan->d_mem) hipFree(plan->d_mem);
+    delete plan;
+}
+
+RMH_EXPORT uint64_t rmh_plan_bytes(const rmh_plan* plan) { return plan ? plan->bytes : 0; }
+
+RMH_EXPORT rmh_status rmh_graph_create(rmh_ctx* ctx, const rmh_graph_node* nodes, size_t n_nodes, rmh_graph** out) {
+    if (!ctx || !out || (n_nodes && !nodes)) return RMH_INVALID_INPUT;
+    *out = nullptr;
+    for (size_t i = 0; i < n_nodes; i++) {
+        if (!nodes[i].plan || nodes[i].plan->ctx != ctx) return fail(ctx, RMH_INVALID_INPUT, "graph node without a plan of this context");
+        if (nodes[i].plan->kind == kResidual && nodes[i].plan->n_tu && (!nodes[i].d_coeff || !nodes[i].d_residual))
+            return fail(ctx, RMH_INVALID_INPUT, "residual graph node without device buffers");
+    }
+    if (ctx->disabled.load()) return fail(ctx, RMH_UNAVAILABLE, "accelerator disabled");
+    DeviceGuard dg(ctx->device);
+    rmh_graph* g = new rmh_graph();
+    g->ctx = ctx;
+    hipError_t e = hipStreamCreateWithFlags(&g->capture, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamBeginCapture(g->capture, hipStreamCaptureModeThreadLocal);
+    std::string err;
+    rmh_status st = RMH_OK;
+    if (e == hipSuccess) {
+        for (size_t i = 0; i < n_nodes && st == RMH_OK; i++) {
+            rmh_plan* p = nodes[i].plan;
+            if ((p->kind == kResidual && p->n_tu == 0) || (p->kind == kMc && p->n_items == 0) || (p->kind == kDeblock && p->n_dbk == 0))
+                continue;
+            st = run_plan_locked_free(p, nodes[i].d_coeff, nodes[i].d_residual, 0, p->n_tu, g->capture, &err);
+        }
+        hipError_t e2 = hipStreamEndCapture(g->capture, &g->graph);
+        if (e2 != hipSuccess && st == RMH_OK) e = e2;
+    }
+    if (e == hipSuccess && st == RMH_OK) e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+    if (e != hipSuccess || st != RMH_OK) {
+        if (g->exec) hipGraphExecDestroy(g->exec);
+        if (g->graph) hipGraphDestroy(g->graph);
+        if (g->capture) hipStreamDestroy(g->capture);
+        delete g;
+        return fail(ctx, RMH_EXECUTION, st != RMH_OK ? err : std::string("graph capture failed: ") + hipGetErrorString(e));
+    }
+    *out = g;
+    return RMH_OK;
+}
+
+RMH_EXPORT rmh_status rmh_graph_launch(rmh_graph* g, void* stream) {
+    if (!g) return RMH_INVALID_INPUT;
+    rmh_status gt = gate(g->ctx);
+    if (gt != RMH_OK) return fail(g->ctx, gt, gt == RMH_UNAVAILABLE ? "accelerator disabled" : "injected device failure");
+    DeviceGuard dg(g->ctx->device);
+    hipError_t e = hipGraphLaunch(g->exec, (hipStream_t)stream);
+    if (e != hipSuccess) return fail(g->ctx, RMH_EXECUTION, std::string("graph launch failed: ") + hipGetErrorString(e));
+    return RMH_OK;
+}
+
+RMH_EXPORT void rmh_graph_destroy(rmh_graph* g) {
+    if (!g) return;
+    DeviceGuard dg(g->ctx->device);
+    if (g->exec) hipGraphExecDestroy(g->exec);
+    if (g->graph) hipGraphDestroy(g->graph);
+    if (g->capture) hipStreamDestroy(g->capture);
+    delete g;
+}

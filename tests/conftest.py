@@ -49,3 +49,11 @@ def reference_text(rel):
         pytest.skip("reference source not present")
     with open(path) as f:
         return f.read()
+
+
+@pytest.fixture(scope="session")
+def acc(rmh):
+    """A device context; GPU tests fail (not skip) when it cannot be created."""
+    a = rmh.Accelerator(0)
+    yield a
+    a.close()

@@ -1,0 +1,127 @@
+"""Device-resident plans, ranged (unbatched) runs and frame-step hipGraphs
+against the host-memory entry points and the oracle."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import workloads as W
+from conftest import load_package
+
+pytestmark = pytest.mark.gpu
+
+
+def _dev_plane(hip, arr, bd, abi):
+    b = hip.DeviceBuffer.from_array(arr)
+    return b, abi.plane_from_address(b.ptr, arr.shape[1], arr.shape[0], arr.shape[1], arr.dtype.itemsize, bd)
+
+
+def test_residual_plan_and_ranges_match_oracle(acc, po):
+    hip = load_package().hip
+    tus, coeff, _ = W.residual_frame(7, width=640, height=368)
+    ref = po.residual_batch(tus, coeff)
+    plan = acc.residual_plan(tus, coeff.size)
+    assert plan.bytes == 4 * coeff.size + 16 * len(tus)
+    dc = hip.DeviceBuffer.from_array(coeff)
+    dr = hip.DeviceBuffer(coeff.nbytes)
+    s = hip.Stream()
+    plan.run(dc.ptr, dr.ptr, s.handle)
+    s.synchronize()
+    assert (dr.download(np.int16) == ref).all()
+    dr.upload(np.zeros_like(coeff))
+    n = len(tus)
+    for a in range(0, n, 997):
+        plan.run_range(a, min(997, n - a), dc.ptr, dr.ptr, s.handle)
+    s.synchronize()
+    assert (dr.download(np.int16) == ref).all()
+    plan.close()
+
+
+def test_frame_step_graph_replays(acc, po):
+    pkg = load_package()
+    hip, abi = pkg.hip, pkg.abi
+    bd = 10
+    tus, coeff, _ = W.residual_frame(8, width=320, height=192)
+    d = W.inter_frame(8, width=320, height=192)
+    ref_res = po.residual_batch(tus, coeff)
+    # oracle: MC then deblock on the prediction
+    from frames import dbk_frame_struct, mc_frame_struct
+    want = [np.zeros_like(p) for p in d["ref"]]
+    f, k = mc_frame_struct(d["ref"], want, bd)
+    arr = (type(f) * 1)(f)
+    po.lib().orc_mc_batch(ctypes.addressof(arr), 1, d["pus"].ctypes.data, len(d["pus"]))
+    g, k2 = dbk_frame_struct(want, d["edges"], bd, d["qp"])
+    arr2 = (type(g) * 1)(g)
+    po.lib().orc_deblock_frames(ctypes.addressof(arr2), 1)
+
+    bufs, refs, dsts = [], [], []
+    for p in d["ref"]:
+        b, pl = _dev_plane(hip, p, bd, abi)
+        b2, pl2 = _dev_plane(hip, np.zeros_like(p), bd, abi)
+        bufs.append((b, b2))
+        refs.append(pl)
+        dsts.append(pl2)
+    edges = hip.DeviceBuffer.from_array(np.ascontiguousarray(d["edges"]))
+    mc = acc.mc_plan([abi.mc_frame(refs, dsts)], d["pus"])
+    db = acc.deblock_plan([abi.dbk_frame(dsts, edges.ptr, d["edges"].shape[1], d["edges"].shape[0], d["qp"])])
+    rp = acc.residual_plan(tus, coeff.size)
+    dc = hip.DeviceBuffer.from_array(coeff)
+    dr = hip.DeviceBuffer(coeff.nbytes)
+    graph = acc.graph([(rp, dc.ptr, dr.ptr), (mc, 0, 0), (db, 0, 0)])
+    s = hip.Stream()
+    for rep in range(4):
+        dr.upload(np.zeros_like(coeff))
+        for _, b2 in bufs:
+            b2.upload(np.zeros(b2.nbytes // 2, np.uint16))
+        graph.launch(s.handle)
+        s.synchronize()
+        assert (dr.download(np.int16) == ref_res).all(), rep
+        for c in range(3):
+            got = bufs[c][1].download(np.uint16).reshape(want[c].shape)
+            assert (got == want[c]).all(), (rep, c)
+        # eager work on the same stream between replays
+        plan2 = acc.residual_plan(tus[:10], coeff.size)
+        plan2.run(dc.ptr, dr.ptr, s.handle)
+        s.synchronize()
+        plan2.close()
+    graph.close()
+
+
+def test_plans_run_on_torch_tensors_when_torch_loads_first():
+    """A caller that imports torch before librmh runs librmh on torch's HIP
+    runtime; plans then take torch device pointers and streams directly."""
+    import os
+    import subprocess
+    import sys
+    code = r"""
+import sys, numpy as np
+sys.path.insert(0, %r); sys.path.insert(0, %r)
+import torch
+assert torch.cuda.is_available()
+from conftest import load_package
+import pyoracle as po, workloads as W
+rmh = load_package()
+acc = rmh.Accelerator(0)
+tus, coeff, _ = W.residual_frame(11, width=256, height=128)
+plan = acc.residual_plan(tus, coeff.size)
+dc = torch.from_numpy(coeff).cuda(); dr = torch.zeros_like(dc)
+plan.run(dc.data_ptr(), dr.data_ptr(), torch.cuda.current_stream().cuda_stream)
+torch.cuda.synchronize()
+assert (dr.cpu().numpy() == po.residual_batch(tus, coeff)).all()
+print("torch interop ok")
+""" % (os.path.join(os.path.dirname(__file__)), os.path.join(os.path.dirname(os.path.dirname(__file__)), "oracle"))
+    out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert "torch interop ok" in out.stdout
+
+
+def test_disable_latch(po):
+    rmh = load_package()
+    a = rmh.Accelerator(0)
+    tus, coeff, _ = W.residual_frame(9, width=64, height=64)
+    a.residuals(tus, coeff)
+    a.disable()
+    assert a.disabled
+    with pytest.raises(rmh.Unavailable):
+        a.residuals(tus, coeff)
+    a.close()

@@ -1,0 +1,100 @@
+"""GPU residual stage (librmh, through the C ABI) against the oracle, bit-exact."""
+import numpy as np
+import pytest
+
+import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def _check(acc, po, tus, coeff):
+    ref = po.residual_batch(tus, coeff)
+    got = acc.residuals(tus, coeff)
+    bad = np.nonzero(got != ref)[0]
+    if bad.size:
+        off = tus["offset"]
+        i = int(np.searchsorted(off, bad[0], side="right") - 1)
+        raise AssertionError("first mismatch at sample %d in TU %d %s: got %d want %d (%d mismatches)"
+                             % (bad[0], i, tus[i], got[bad[0]], ref[bad[0]], bad.size))
+
+
+def test_full_1080p10_frame_bit_exact(acc, po):
+    print("residual 1080p10 frame seed 0", flush=True)
+    tus, coeff, _ = W.residual_frame(0)
+    _check(acc, po, tus, coeff)
+
+
+@pytest.mark.parametrize("bd", [8, 10, 12])
+def test_tool_mix_bit_exact(acc, po, bd):
+    print("residual tool mix bd=%d" % bd, flush=True)
+    tus, coeff, _ = W.residual_frame(10 + bd, width=480, height=272, bit_depth=bd, with_lfnst=True, with_ts=True,
+                                     with_dq=True, qp_range=(4, 51))
+    _check(acc, po, tus, coeff)
+
+
+def test_every_size_type_and_skip(acc, po):
+    """Every (w, h) from 1x2 to 64x64 with every supported type pair, extra
+    zero-outs, LFNST modes on both layouts, dense and extreme coefficients."""
+    rng = np.random.default_rng(5)
+    descs, blocks, off = [], [], 0
+    for lw in range(0, 7):
+        for lh in range(0, 7):
+            w, h = 1 << lw, 1 << lh
+            if w * h < 2:
+                continue
+            for _ in range(6):
+                t = np.zeros(1, W.TU_DTYPE)[0]
+                t["offset"], t["log2_w"], t["log2_h"] = off, lw, lh
+                t["bit_depth"] = int(rng.choice([8, 10, 12]))
+                t["qp"] = int(rng.integers(0, 64 + 6 * (t["bit_depth"] - 8)))
+                ok_h = [0] + ([1, 2] if 4 <= w <= 32 else [])
+                ok_v = [0] + ([1, 2] if 4 <= h <= 32 else [])
+                t["tr_hor"], t["tr_ver"] = int(rng.choice(ok_h)), int(rng.choice(ok_v))
+                if w >= 4 and h >= 4 and rng.random() < 0.4:
+                    t["lfnst_idx"], t["lfnst_mode"] = int(rng.integers(1, 3)), int(rng.integers(0, 95))
+                if rng.random() < 0.3 and w > 2:
+                    t["skip_w"] = int(rng.integers(0, w // 2))
+                if rng.random() < 0.3 and h > 2:
+                    t["skip_h"] = int(rng.integers(0, h // 2))
+                r = rng.random()
+                if r < 0.1:
+                    t["flags"] = 1
+                elif r < 0.25:
+                    t["flags"] = 2
+                kind = rng.integers(0, 3)
+                if kind == 0:
+                    c = rng.integers(-32768, 32768, w * h)
+                elif kind == 1:
+                    c = np.rint(rng.laplace(0, 40, w * h))
+                else:
+                    c = np.where(rng.random(w * h) < 0.1, rng.integers(-300, 300, w * h), 0)
+                descs.append(t)
+                blocks.append(np.clip(c, -32768, 32767).astype(np.int16))
+                off += w * h
+    tus = np.array(descs, dtype=W.TU_DTYPE)
+    _check(acc, po, tus, np.concatenate(blocks))
+
+
+def test_empty_and_all_zero(acc, po):
+    tus, coeff, _ = W.residual_frame(3, width=64, height=64)
+    z = np.zeros_like(coeff)
+    assert (acc.residuals(tus, z) == 0).all()
+    assert acc.residuals(tus[:0], coeff).size == coeff.size
+
+
+def test_untouched_output_outside_tus_and_on_error(acc, rmh):
+    tus, coeff, _ = W.residual_frame(4, width=64, height=64)
+    out = np.full(coeff.size + 16, 1234, np.int16)
+    c2 = np.concatenate([coeff, np.zeros(16, np.int16)])
+    acc.residuals(tus, c2, out)
+    assert (out[-16:] == 1234).all()
+    sentinel = out.copy()
+    bad = tus.copy()
+    bad[3]["log2_w"] = 7
+    with pytest.raises(rmh.InvalidInput):
+        acc.residuals(bad, c2, out)
+    assert (out == sentinel).all()
+    acc.inject_failures(1)
+    with pytest.raises(rmh.Execution):
+        acc.residuals(tus, c2, out)
+    assert (out == sentinel).all()
