@@ -27,8 +27,6 @@ namespace {
 
 typedef short short2v __attribute__((ext_vector_type(2)));
 
-constexpr int kColStride = 17;  // dwords per LDS column / row (16 pairs + 1 pad against bank conflicts)
-
 __constant__ int16_t kInvQuant[2][6] = {{40, 45, 51, 57, 64, 72}, {57, 64, 72, 80, 90, 102}};
 
 __device__ __forceinline__ int clip16(int64_t v) { return (int)(v < -32768 ? -32768 : v > 32767 ? 32767 : v); }
@@ -53,11 +51,42 @@ __device__ __forceinline__ int dequant_ts(int level, int qp) {
     return clip16i(n);
 }
 
+// LDS row pitch in dwords for both the coefficient columns and the
+// intermediate rows: 16 int16 pairs plus 4 dwords of padding, keeping every
+// row 16-byte aligned for 128-bit reads.
+constexpr int kPitch = 20;
+
+__device__ __forceinline__ int wave_max(int v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        int o = __shfl_xor(v, off);
+        v = o > v ? o : v;
+    }
+    return v;
+}
+
+// acc + sum over pairs kk < 16 of dot2(src[kk], m[kk]); m[kk] is zero past the
+// live pairs, so the 128-bit reads may run past them.
+__device__ __forceinline__ int dot_pairs(const uint32_t* __restrict__ src, const uint32_t (&m)[16], int npairs) {
+    const uint4* v4 = reinterpret_cast<const uint4*>(src);
+    int acc = 0;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        if (4 * q < npairs) {
+            uint4 v = v4[q];
+            acc = dot2(v.x, m[4 * q + 0], acc);
+            acc = dot2(v.y, m[4 * q + 1], acc);
+            acc = dot2(v.z, m[4 * q + 2], acc);
+            acc = dot2(v.w, m[4 * q + 3], acc);
+        }
+    }
+    return acc;
+}
+
 __global__ __launch_bounds__(64) void residual_kernel(ResidualArgs a) {
-    __shared__ uint32_t ct[32 * kColStride];  // column-major coefficients: column i, int16 pairs along k
-    __shared__ uint32_t tt[64 * kColStride];  // row-major intermediates: row y, int16 pairs along i
-    __shared__ int16_t lv[32 * 32];           // raw levels for dependent quantisation
-    __shared__ int bbox[2];
+    __shared__ __align__(16) uint32_t ct[32 * kPitch];  // coefficients: column i, int16 pairs along row k
+    __shared__ __align__(16) uint32_t tt[64 * kPitch];  // intermediates: row y, int16 pairs along column i
+    __shared__ int16_t lv[32 * 32];                     // raw levels for dependent quantisation
 
     const int lane = threadIdx.x;
     const rmh_tu_desc d = a.tus[a.first_tu + blockIdx.x];
@@ -87,22 +116,44 @@ __global__ __launch_bounds__(64) void residual_kernel(ResidualArgs a) {
         sh = lsh > sh ? lsh : sh;
     }
     const int cw = w - sw, ch = h - sh;  // coded columns / rows that feed the transform
-    // Region held in LDS: the coded region, widened to 4x4 when LFNST reads it.
+    // Region held in LDS: the coded region, widened to 4x4 when LFNST reads it,
+    // with rows padded to a multiple of 8 (zeros) for the 128-bit reads.
     const int rw = lfnst && cw < 4 ? 4 : cw;
     const int rh = lfnst && ch < 4 ? 4 : ch;
+    const int rh_pad = (rh + 7) & ~7;
+    const bool dq = (d.flags & RMH_TU_DEP_QUANT) != 0;
 
     // Dequantisation parameters (vvc/dct.rs:1164-1175, 1218-1234).
     const int lsum = lw + lh;
     const int sqrt2 = lsum & 1;
     const int tr_shift = 15 - bd - (lsum >> 1) - sqrt2;
 
-    if (lane < 2) bbox[lane] = -1;
-    // zero the LDS region (pairs), so odd extents read zeros
-    for (int e = lane; e < 32 * kColStride; e += 64) ct[e] = 0;
-    __syncthreads();
-
     int16_t* ct16 = reinterpret_cast<int16_t*>(ct);
-    if (d.flags & RMH_TU_DEP_QUANT) {
+    int my_r = -1, my_c = -1;
+    int lrw = 0;
+    while ((1 << lrw) < rw) lrw++;
+    if (!dq) {
+        const int per = qp / 6, rem = qp % 6;
+        const int rshift = 6 - (tr_shift + per);
+        const int scale = kInvQuant[sqrt2][rem];
+        const int total = rh_pad << lrw;
+        for (int e = lane; e < total; e += 64) {
+            const int y = e >> lrw, x = e & ((1 << lrw) - 1);
+            if (x < rw) {
+                int v = y < rh ? dequant_flat(cin[y * w + x], scale, rshift) : 0;
+                ct16[x * 2 * kPitch + y] = (int16_t)v;
+                if (v != 0 && x < cw && y < ch) {
+                    my_r = y > my_r ? y : my_r;
+                    my_c = x > my_c ? x : my_c;
+                }
+            }
+        }
+    } else {
+        const int total = rh_pad << lrw;
+        for (int e = lane; e < total; e += 64) {
+            const int y = e >> lrw, x = e & ((1 << lrw) - 1);
+            if (x < rw) ct16[x * 2 * kPitch + y] = 0;
+        }
         const int lw_c = w < 32 ? w : 32, lh_c = h < 32 ? h : 32;
         for (int e = lane; e < lw_c * lh_c; e += 64) {
             int y = e / lw_c, x = e - y * lw_c;
@@ -110,6 +161,7 @@ __global__ __launch_bounds__(64) void residual_kernel(ResidualArgs a) {
         }
         __syncthreads();
         if (lane == 0) {
+            // dequantize_dep_quant_bd (vvc/dct.rs:1191-1248): serial 4-state automaton
             const uint16_t* scan = a.scans + a.scan_off[lw][lh];
             const int len = a.scan_len[lw][lh];
             int last = -1;
@@ -133,22 +185,10 @@ __global__ __launch_bounds__(64) void residual_kernel(ResidualArgs a) {
                 int level = lv[y * lw_c + x];
                 if (level != 0) {
                     int64_t q = 2 * (int64_t)level + (level > 0 ? -(state >> 1) : (state >> 1));
-                    if (x < rw && y < rh) ct16[x * 2 * kColStride + y] = (int16_t)clip16((q * inv + add) >> rs);
+                    if (x < rw && y < rh) ct16[x * 2 * kPitch + y] = (int16_t)clip16((q * inv + add) >> rs);
                 }
                 state = (32040 >> ((state << 2) + ((level & 1) << 1))) & 3;
             }
-        }
-    } else {
-        const int per = qp / 6, rem = qp % 6;
-        const int rshift = 6 - (tr_shift + per);
-        const int scale = kInvQuant[sqrt2][rem];
-        // round the region width up to a power of two so rows/columns come from shifts
-        int lrw = 0;
-        while ((1 << lrw) < rw) lrw++;
-        const int total = rh << lrw;
-        for (int e = lane; e < total; e += 64) {
-            int y = e >> lrw, x = e & ((1 << lrw) - 1);
-            if (x < rw) ct16[x * 2 * kColStride + y] = (int16_t)dequant_flat(cin[y * w + x], scale, rshift);
         }
     }
     __syncthreads();
@@ -171,14 +211,14 @@ __global__ __launch_bounds__(64) void residual_kernel(ResidualArgs a) {
 #pragma unroll
         for (int i = 0; i < 16; i++) {
             int x = scan4[i] & 3, y = scan4[i] >> 2;
-            src[i] = ct16[x * 2 * kColStride + y];
+            src[i] = ct16[x * 2 * kPitch + y];
         }
         int outv = 0, ox = -1, oy = -1;
         if (lane < tr_size) {
             int sum = 0;
             for (int i = 0; i < zero_out; i++) sum += src[i] * (int)m[i * mstride + lane];
             outv = clip16i((sum + 64) >> 7);
-            int j = lane;
+            const int j = lane;
             if (tp) {
                 if (sb == 4) {
                     oy = j & 3;
@@ -204,54 +244,42 @@ __global__ __launch_bounds__(64) void residual_kernel(ResidualArgs a) {
             }
         }
         __syncthreads();
-        if (lane < tr_size && ox < rw && oy < rh) ct16[ox * 2 * kColStride + oy] = (int16_t)outv;
+        if (lane < tr_size && ox < rw && oy < rh) ct16[ox * 2 * kPitch + oy] = (int16_t)outv;
         __syncthreads();
     }
-
-    // A caller zero-out narrower than the 4x4 LFNST input leaves values outside
-    // the coded region in LDS; they must not reach the transform.
-    if (rw > cw || rh > ch) {
-        for (int e = lane; e < 32 * 32; e += 64) {
-            int x = e >> 5, y = e & 31;
-            if ((x >= cw || y >= ch) && x < rw && y < rh) ct16[x * 2 * kColStride + y] = 0;
-        }
-        __syncthreads();
-    }
-
-    // Bounding box of non-zero coefficients inside the coded region.
-    {
-        int my_r = -1, my_c = -1;
-        for (int e = lane; e < 32 * 32; e += 64) {
-            int x = e >> 5, y = e & 31;
-            if (x < cw && y < ch && ct16[x * 2 * kColStride + y] != 0) {
-                my_r = y > my_r ? y : my_r;
-                my_c = x > my_c ? x : my_c;
+    if (lfnst || dq) {
+        // A caller zero-out narrower than the 4x4 LFNST input leaves values
+        // outside the coded region in LDS; they must not reach the transform.
+        // The bounding box is taken after LFNST / dependent dequantisation.
+        const int total = rh_pad << lrw;
+        for (int e = lane; e < total; e += 64) {
+            const int y = e >> lrw, x = e & ((1 << lrw) - 1);
+            if (x < rw) {
+                if (x >= cw || y >= ch) {
+                    ct16[x * 2 * kPitch + y] = 0;
+                } else if (ct16[x * 2 * kPitch + y] != 0) {
+                    my_r = y > my_r ? y : my_r;
+                    my_c = x > my_c ? x : my_c;
+                }
             }
         }
-        if (my_r >= 0) {
-            atomicMax(&bbox[0], my_r);
-            atomicMax(&bbox[1], my_c);
-        }
+        __syncthreads();
     }
-    __syncthreads();
-    const int er = bbox[0] + 1, ec = bbox[1] + 1;  // effective rows / columns
+    const int er = __builtin_amdgcn_readfirstlane(wave_max(my_r)) + 1;  // effective rows
+    const int ec = __builtin_amdgcn_readfirstlane(wave_max(my_c)) + 1;  // effective columns
     if (er <= 0) {
         for (int e = lane; e < n; e += 64) rout[e] = 0;
         return;
     }
-    // zero the rows beyond the coded region inside the pair that contains row er
-    // (already zero: the LDS region was cleared and only coded positions written)
 
     const int bdc = bd < 8 ? 8 : bd > 12 ? 12 : bd;
     if (w == 1 || h == 1) {
-        // 1-D transform (vvc/dct.rs:1010-1057): one pass, shift 21 - bd, over
-        // the column (w == 1) or row (h == 1) held in LDS column 0 / row-as-column.
+        // 1-D transform (vvc/dct.rs:1010-1057): one pass, shift 21 - bd.
         const int len = w == 1 ? h : w;
         const int ltype = w == 1 ? tv : th;
         const int lg = w == 1 ? lh : lw;
         const uint32_t* mp = a.mat + a.mat_off[ltype][lg];
         const int shift = 6 + 15 - bdc;
-        // for h == 1 the coefficients sit in row 0 of columns 0..w-1: gather to pairs
         const int taps = w == 1 ? er : ec;
         const int npairs = (taps + 1) >> 1;
         for (int j = lane; j < len; j += 64) {
@@ -262,8 +290,8 @@ __global__ __launch_bounds__(64) void residual_kernel(ResidualArgs a) {
                     c0 = ct16[2 * kk];
                     c1 = ct16[2 * kk + 1];
                 } else {
-                    c0 = ct16[(2 * kk) * 2 * kColStride];
-                    c1 = 2 * kk + 1 < w ? ct16[(2 * kk + 1) * 2 * kColStride] : 0;
+                    c0 = ct16[(2 * kk) * 2 * kPitch];
+                    c1 = 2 * kk + 1 < ec ? ct16[(2 * kk + 1) * 2 * kPitch] : 0;
                 }
                 uint32_t pc = (uint32_t)(uint16_t)c0 | ((uint32_t)(uint16_t)c1 << 16);
                 acc = dot2(pc, mp[kk * len + j], acc);
@@ -273,39 +301,36 @@ __global__ __launch_bounds__(64) void residual_kernel(ResidualArgs a) {
         return;
     }
 
-    // Stage 1 (vertical): tt[y][i] = clip16((sum_k Mv[k][y] * C[k][i] + 64) >> 7), i < ec.
+    // Stage 1 (vertical): tt[y][i] = clip16((sum_k Mv[k][y] * C[k][i] + 64) >> 7)
+    // for i < ec, zero for the padding column. Lane y holds column y of Mv.
     {
+        const int k1 = (er + 1) >> 1;
         const uint32_t* mv = a.mat + a.mat_off[tv][lh];
-        const int npairs = (er + 1) >> 1;
-        const int ecp = (ec + 1) & ~1;  // even column count, the odd one padded with zero
-        const int total = ecp << lh;
+        const int y = lane & (h - 1), g = lane >> lh, G = 64 >> lh;
+        uint32_t m[16];
+#pragma unroll
+        for (int kk = 0; kk < 16; kk++) m[kk] = kk < k1 ? mv[kk * h + y] : 0u;
+        const int ecp = (ec + 1) & ~1;
         int16_t* tt16 = reinterpret_cast<int16_t*>(tt);
-        for (int e = lane; e < total; e += 64) {
-            const int y = e & (h - 1), i = e >> lh;
-            int acc = 0;
-            if (i < ec) {
-                const uint32_t* col = ct + i * kColStride;
-                for (int kk = 0; kk < npairs; kk++) acc = dot2(col[kk], mv[kk * h + y], acc);
-                acc = clip16i((acc + 64) >> 7);
-            }
-            tt16[y * 2 * kColStride + i] = (int16_t)acc;
+        for (int i = g; i < ecp; i += G) {
+            int v = 0;
+            if (i < ec) v = clip16i((dot_pairs(ct + i * kPitch, m, k1) + 64) >> 7);
+            tt16[y * 2 * kPitch + i] = (int16_t)v;
         }
     }
     __syncthreads();
 
     // Stage 2 (horizontal): out[y][x] = clip16((sum_k Mh[k][x] * tt[y][k] + rnd) >> (20 - bd)).
     {
+        const int k2 = (ec + 1) >> 1;
         const uint32_t* mh = a.mat + a.mat_off[th][lw];
-        const int npairs = (ec + 1) >> 1;
+        const int x = lane & (w - 1), g = lane >> lw, G = 64 >> lw;
+        uint32_t m[16];
+#pragma unroll
+        for (int kk = 0; kk < 16; kk++) m[kk] = kk < k2 ? mh[kk * w + x] : 0u;
         const int shift = 6 + 15 - 1 - bdc;
         const int rnd = 1 << (shift - 1);
-        for (int e = lane; e < n; e += 64) {
-            const int x = e & (w - 1), y = e >> lw;
-            const uint32_t* row = tt + y * kColStride;
-            int acc = 0;
-            for (int kk = 0; kk < npairs; kk++) acc = dot2(row[kk], mh[kk * w + x], acc);
-            rout[e] = (int16_t)clip16i((acc + rnd) >> shift);
-        }
+        for (int y = g; y < h; y += G) rout[y * w + x] = (int16_t)clip16i((dot_pairs(tt + y * kPitch, m, k2) + rnd) >> shift);
     }
 }
 
