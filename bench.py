@@ -172,14 +172,19 @@ def main():
 
     unb = None
     if not args.no_unbatched:
+        # the same kernels, one plan (one launch) per frame over that frame's buffers
+        fplans, base = [], 0
+        for t, c in frames:
+            fplans.append((acc.residual_plan(t, c.size), base))
+            base += c.nbytes
         for _ in range(max(1, args.warmup)):
-            for a, n in ranges:
-                plan.run_range(a, n, d_coeff.ptr, d_res.ptr, s)
+            for fp, b in fplans:
+                fp.run(d_coeff.ptr + b, d_res.ptr + b, s)
         hip.device_synchronize()
         ev0.record(s)
         for _ in range(args.steps):
-            for a, n in ranges:
-                plan.run_range(a, n, d_coeff.ptr, d_res.ptr, s)
+            for fp, b in fplans:
+                fp.run(d_coeff.ptr + b, d_res.ptr + b, s)
         ev1.record(s)
         ev1.synchronize()
         hip.device_synchronize()

@@ -27,6 +27,7 @@ struct ResidualArgs {
     const int8_t* lfnst8;
     const int8_t* lfnst4;
     const uint16_t* scans;
+    const uint32_t* order;  // optional TU index list (nullptr: descriptor order)
     uint32_t first_tu;
     uint32_t n_tu;
     int32_t mat_off[3][7];
@@ -35,6 +36,28 @@ struct ResidualArgs {
 };
 
 hipError_t launch_residual(const ResidualArgs& a, hipStream_t stream);
+
+// A tile of TUs of one size class (log2 w, log2 h in 2..6) processed by one
+// 256-lane workgroup of the tiled residual kernel: TUs order[first .. first+count).
+struct TileItem {
+    uint32_t first;
+    uint16_t count;
+    uint8_t lw;
+    uint8_t lh;
+};
+
+// Tiled residual kernel for the common case (2-D, both sides >= 4, no LFNST,
+// no dependent quantisation, no transform skip). `a.order` is required.
+hipError_t launch_residual_tiles(const ResidualArgs& a, const TileItem* items, uint32_t n_items, uint32_t lds_bytes,
+                                 hipStream_t stream);
+
+// LDS bytes the tiled kernel needs for a tile of `count` TUs of class (lw, lh).
+inline uint32_t residual_tile_lds(int lw, int lh, int count) {
+    int wc = (1 << lw) < 32 ? (1 << lw) : 32, hc = (1 << lh) < 32 ? (1 << lh) : 32;
+    uint32_t c = (uint32_t)count * wc * (hc / 2) * 4;         // coefficient columns
+    uint32_t t = (uint32_t)count * (1u << lh) * (wc / 2) * 4;  // intermediate rows
+    return c + t + (uint32_t)count * 32 + (uint32_t)count * 8;
+}
 
 // One inter work item: a 16x16 luma tile (and its 8x8 chroma tiles) of one PU.
 struct McItem {

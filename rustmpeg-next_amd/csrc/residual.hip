@@ -89,7 +89,8 @@ __global__ __launch_bounds__(64) void residual_kernel(ResidualArgs a) {
     __shared__ int16_t lv[32 * 32];                     // raw levels for dependent quantisation
 
     const int lane = threadIdx.x;
-    const rmh_tu_desc d = a.tus[a.first_tu + blockIdx.x];
+    const uint32_t tu_index = a.order ? a.order[a.first_tu + blockIdx.x] : a.first_tu + blockIdx.x;
+    const rmh_tu_desc d = a.tus[tu_index];
     const int lw = d.log2_w, lh = d.log2_h;
     const int w = 1 << lw, h = 1 << lh, n = w * h;
     const int bd = d.bit_depth;
@@ -334,11 +335,191 @@ __global__ __launch_bounds__(64) void residual_kernel(ResidualArgs a) {
     }
 }
 
+
+// ---------------------------------------------------------------------------
+// Tiled kernel: one 256-lane workgroup per tile of same-size TUs.
+//
+//   phase 0  per-TU parameters (offset, coded region, dequant scale/shift,
+//            matrix offsets) into LDS; bounding boxes cleared
+//   phase A  every coefficient of the tile's coded regions read with 8-byte
+//            loads, dequantised, stored column-major in LDS; per-TU bounding
+//            box of non-zero coefficients by LDS atomics
+//   phase B  vertical pass: lane (TU, row y) holds column y of Mv in registers
+//            and produces its row of intermediates for the live columns
+//   phase C  horizontal pass: lane (TU, column x) holds column x of Mh and
+//            produces its column of residuals, stored to HBM
+// Many TUs per workgroup keep many loads in flight, which a one-TU workgroup
+// cannot. Arithmetic and clipping are identical to residual_kernel.
+// ---------------------------------------------------------------------------
+
+struct TuParams {
+    int32_t off;
+    int32_t cwch;    // cw | ch << 8
+    int32_t scale;
+    int32_t rshift;
+    int32_t mh;      // dword offset of Mh pairs
+    int32_t mv;      // dword offset of Mv pairs
+    int32_t shift2;  // 20 - bd
+    int32_t pad;
+};
+
+// dot2 over pairs kk < npairs read 2 at a time (8-byte LDS reads); m[kk] = 0
+// past the live pairs.
+__device__ __forceinline__ int dot_pairs2(const uint32_t* __restrict__ src, const uint32_t (&m)[16], int npairs) {
+    const uint2* v2 = reinterpret_cast<const uint2*>(src);
+    int acc = 0;
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        if (2 * q < npairs) {
+            uint2 v = v2[q];
+            acc = dot2(v.x, m[2 * q + 0], acc);
+            acc = dot2(v.y, m[2 * q + 1], acc);
+        }
+    }
+    return acc;
+}
+
+__global__ __launch_bounds__(256) void residual_tile_kernel(ResidualArgs a, const TileItem* __restrict__ items) {
+    extern __shared__ __align__(16) uint32_t smem[];
+    const TileItem it = items[blockIdx.x];
+    const int lw = it.lw, lh = it.lh, W = 1 << lw, H = 1 << lh, G = it.count;
+    const int lwc = lw < 5 ? lw : 5, lhc = lh < 5 ? lh : 5;
+    const int Wc = 1 << lwc, Hc = 1 << lhc;
+    const int CP = Hc >> 1;  // dwords per coefficient column
+    const int TP = Wc >> 1;  // dwords per intermediate row
+    uint32_t* C = smem;
+    uint32_t* T = C + G * Wc * CP;
+    TuParams* P = reinterpret_cast<TuParams*>(T + G * H * TP);
+    int* BB = reinterpret_cast<int*>(P + G);
+    const int tid = threadIdx.x;
+
+    for (int g = tid; g < G; g += 256) {
+        const rmh_tu_desc d = a.tus[a.order[it.first + g]];
+        const int th = d.tr_hor, tv = d.tr_ver;
+        int sw = (th != RMH_TR_DCT2 && W == 32) ? 16 : (W > 32 ? W - 32 : 0);
+        int sh = (tv != RMH_TR_DCT2 && H == 32) ? 16 : (H > 32 ? H - 32 : 0);
+        sw = d.skip_w > sw ? d.skip_w : sw;
+        sh = d.skip_h > sh ? d.skip_h : sh;
+        const int qp = d.qp < 4 ? 4 : d.qp;
+        const int lsum = lw + lh, sqrt2 = lsum & 1;
+        const int tr_shift = 15 - d.bit_depth - (lsum >> 1) - sqrt2;
+        TuParams p;
+        p.off = (int32_t)d.offset;
+        p.cwch = (W - sw) | ((H - sh) << 8);
+        p.scale = kInvQuant[sqrt2][qp % 6];
+        p.rshift = 6 - (tr_shift + qp / 6);
+        p.mh = a.mat_off[th][lw];
+        p.mv = a.mat_off[tv][lh];
+        p.shift2 = 20 - d.bit_depth;
+        p.pad = 0;
+        P[g] = p;
+        BB[2 * g] = -1;
+        BB[2 * g + 1] = -1;
+    }
+    __syncthreads();
+
+    // phase A: 4 consecutive coefficients of one row per lane
+    {
+        int16_t* C16 = reinterpret_cast<int16_t*>(C);
+        const int q4 = lwc - 2;  // log2 of 4-sample groups per row
+        const int total = G << (lhc + q4);
+        for (int e = tid; e < total; e += 256) {
+            const int x0 = (e & ((1 << q4) - 1)) << 2;
+            const int r = (e >> q4) & (Hc - 1);
+            const int g = e >> (lhc + q4);
+            const TuParams p = P[g];
+            const int cw = p.cwch & 255, ch = p.cwch >> 8;
+            int v[4] = {0, 0, 0, 0};
+            if (r < ch && x0 < cw) {
+                const uint2 raw = *reinterpret_cast<const uint2*>(a.coeff + p.off + r * W + x0);
+                const int16_t c4[4] = {(int16_t)(raw.x & 0xffff), (int16_t)(raw.x >> 16), (int16_t)(raw.y & 0xffff),
+                                       (int16_t)(raw.y >> 16)};
+                int mx = -1;
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    if (x0 + k < cw) {
+                        v[k] = dequant_flat(c4[k], p.scale, p.rshift);
+                        if (v[k] != 0) mx = x0 + k;
+                    }
+                }
+                if (mx >= 0) {
+                    atomicMax(&BB[2 * g], r);
+                    atomicMax(&BB[2 * g + 1], mx);
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < 4; k++) C16[((g * Wc + x0 + k) * CP) * 2 + r] = (int16_t)v[k];
+        }
+    }
+    __syncthreads();
+
+    // phase B: vertical pass
+    {
+        int16_t* T16 = reinterpret_cast<int16_t*>(T);
+        const int lnch = (G << lh) >= 256 ? 0 : (8 - lh - 31 + __clz(G));  // log2 column chunks per (TU, row)
+        const int nch = 1 << lnch;
+        const int total = (G << lh) << lnch;
+        for (int q = tid; q < total; q += 256) {
+            const int y = q & (H - 1);
+            const int c0 = (q >> lh) & (nch - 1);
+            const int g = q >> (lh + lnch);
+            const int er = BB[2 * g] + 1, ec = BB[2 * g + 1] + 1;
+            if (er <= 0) continue;
+            const int k1 = (er + 1) >> 1;
+            const uint32_t* mv = a.mat + P[g].mv;
+            uint32_t m[16];
+#pragma unroll
+            for (int kk = 0; kk < 16; kk++) m[kk] = kk < k1 ? mv[kk * H + y] : 0u;
+            const int ecp = (ec + 1) & ~1;
+            for (int i = c0; i < ecp; i += nch) {
+                int val = 0;
+                if (i < ec) val = clip16i((dot_pairs2(C + (g * Wc + i) * CP, m, k1) + 64) >> 7);
+                T16[((g * H + y) * TP) * 2 + i] = (int16_t)val;
+            }
+        }
+    }
+    __syncthreads();
+
+    // phase C: horizontal pass
+    {
+        const int lnr = (G << lw) >= 256 ? 0 : (8 - lw - 31 + __clz(G));  // log2 row chunks per (TU, column)
+        const int nr = 1 << lnr;
+        const int total = (G << lw) << lnr;
+        for (int q = tid; q < total; q += 256) {
+            const int x = q & (W - 1);
+            const int r0 = (q >> lw) & (nr - 1);
+            const int g = q >> (lw + lnr);
+            const TuParams p = P[g];
+            const int er = BB[2 * g] + 1, ec = BB[2 * g + 1] + 1;
+            int16_t* out = a.residual + p.off;
+            if (er <= 0) {
+                for (int y = r0; y < H; y += nr) out[y * W + x] = 0;
+                continue;
+            }
+            const int k2 = (ec + 1) >> 1;
+            const uint32_t* mh = a.mat + p.mh;
+            uint32_t m[16];
+#pragma unroll
+            for (int kk = 0; kk < 16; kk++) m[kk] = kk < k2 ? mh[kk * W + x] : 0u;
+            const int rnd = 1 << (p.shift2 - 1);
+            for (int y = r0; y < H; y += nr)
+                out[y * W + x] = (int16_t)clip16i((dot_pairs2(T + (g * H + y) * TP, m, k2) + rnd) >> p.shift2);
+        }
+    }
+}
+
 }  // namespace
 
 hipError_t launch_residual(const ResidualArgs& a, hipStream_t stream) {
     if (a.n_tu == 0) return hipSuccess;
     hipLaunchKernelGGL(residual_kernel, dim3(a.n_tu), dim3(64), 0, stream, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_residual_tiles(const ResidualArgs& a, const TileItem* items, uint32_t n_items, uint32_t lds_bytes,
+                                 hipStream_t stream) {
+    if (n_items == 0) return hipSuccess;
+    hipLaunchKernelGGL(residual_tile_kernel, dim3(n_items), dim3(256), lds_bytes, stream, a, items);
     return hipGetLastError();
 }
 

@@ -13,6 +13,7 @@
 #include <chrono>
 #include <cstdio>
 #include <cstring>
+#include <algorithm>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -93,6 +94,10 @@ struct rmh_plan {
     // residual
     const rmh_tu_desc* d_tus = nullptr;
     size_t n_tu = 0, n_coeff = 0;
+    const uint32_t* d_order = nullptr;       // fast TUs grouped by size class, then the others
+    const rmh::TileItem* d_tiles = nullptr;  // tiles of the fast TUs
+    size_t n_tiles = 0, n_fast = 0;
+    uint32_t tile_lds = 0;
     // inter
     const rmh_mc_frame* d_frames = nullptr;
     const rmh_pu_desc* d_pus = nullptr;
@@ -415,6 +420,51 @@ uint64_t mc_bytes(const rmh_mc_frame* frames, const rmh_pu_desc* pus, size_t n_p
 
 uint64_t plane_bytes(const rmh_plane& p) { return (uint64_t)p.width * p.height * p.bytes_per_sample; }
 
+
+// TUs the tiled kernel handles: 2-D with both sides >= 4, no LFNST, no
+// dependent quantisation, no transform skip, 8-byte aligned block.
+bool tile_eligible(const rmh_tu_desc& t) {
+    return t.log2_w >= 2 && t.log2_h >= 2 && t.lfnst_idx == 0 && !(t.flags & (RMH_TU_TRANSFORM_SKIP | RMH_TU_DEP_QUANT)) &&
+           (t.offset & 3) == 0;
+}
+
+// Samples per tile of the tiled residual kernel (one 256-lane workgroup).
+constexpr int kTileSamples = 4096;
+
+// Groups tile-eligible TUs by size class into tiles (largest class first) and
+// appends the remaining TUs; returns the tile list and the TU order.
+void build_residual_tiles(const rmh_tu_desc* tus, size_t n_tu, std::vector<uint32_t>& order,
+                          std::vector<rmh::TileItem>& tiles, size_t* n_fast, uint32_t* lds) {
+    std::vector<uint32_t> cls[7][7];
+    std::vector<uint32_t> rest;
+    for (size_t i = 0; i < n_tu; i++) {
+        if (tile_eligible(tus[i]))
+            cls[tus[i].log2_w][tus[i].log2_h].push_back((uint32_t)i);
+        else
+            rest.push_back((uint32_t)i);
+    }
+    order.clear();
+    tiles.clear();
+    *lds = 0;
+    for (int area = 12; area >= 4; area--)
+        for (int lw = 2; lw <= 6; lw++) {
+            int lh = area - lw;
+            if (lh < 2 || lh > 6) continue;
+            std::vector<uint32_t>& v = cls[lw][lh];
+            int per = kTileSamples >> area;
+            if (per < 1) per = 1;
+            for (size_t f = 0; f < v.size(); f += per) {
+                int cnt = (int)std::min<size_t>(per, v.size() - f);
+                tiles.push_back({(uint32_t)(order.size() + f), (uint16_t)cnt, (uint8_t)lw, (uint8_t)lh});
+                uint32_t b = rmh::residual_tile_lds(lw, lh, cnt);
+                if (b > *lds) *lds = b;
+            }
+            order.insert(order.end(), v.begin(), v.end());
+        }
+    *n_fast = order.size();
+    order.insert(order.end(), rest.begin(), rest.end());
+}
+
 rmh_status run_plan_locked_free(rmh_plan* plan, const int16_t* d_coeff, int16_t* d_residual, size_t first, size_t count,
                                 hipStream_t s, std::string* err) {
     hipError_t e = hipSuccess;
@@ -427,12 +477,26 @@ rmh_status run_plan_locked_free(rmh_plan* plan, const int16_t* d_coeff, int16_t*
         a.lfnst8 = plan->ctx->tabs.lfnst8;
         a.lfnst4 = plan->ctx->tabs.lfnst4;
         a.scans = plan->ctx->tabs.scans;
-        a.first_tu = (uint32_t)first;
-        a.n_tu = (uint32_t)count;
         memcpy(a.mat_off, plan->ctx->tabs.mat_off, sizeof a.mat_off);
         memcpy(a.scan_off, plan->ctx->tabs.scan_off, sizeof a.scan_off);
         memcpy(a.scan_len, plan->ctx->tabs.scan_len, sizeof a.scan_len);
-        e = rmh::launch_residual(a, s);
+        const bool whole = first == 0 && count == plan->n_tu && plan->d_order;
+        const bool aligned8 = ((uintptr_t)d_coeff & 7) == 0;
+        if (whole && aligned8) {
+            // the tiled kernel for the common TUs, the general kernel for the rest
+            a.order = plan->d_order;
+            e = rmh::launch_residual_tiles(a, plan->d_tiles, (uint32_t)plan->n_tiles, plan->tile_lds, s);
+            if (e == hipSuccess && plan->n_tu > plan->n_fast) {
+                a.first_tu = (uint32_t)plan->n_fast;
+                a.n_tu = (uint32_t)(plan->n_tu - plan->n_fast);
+                e = rmh::launch_residual(a, s);
+            }
+        } else {
+            a.order = nullptr;
+            a.first_tu = (uint32_t)first;
+            a.n_tu = (uint32_t)count;
+            e = rmh::launch_residual(a, s);
+        }
     } else if (plan->kind == kMc) {
         e = rmh::launch_mc(plan->d_frames, plan->d_pus, plan->d_items, (uint32_t)plan->n_items, s);
     } else {
@@ -558,14 +622,25 @@ RMH_EXPORT rmh_status rmh_residual_plan_create(rmh_ctx* ctx, const rmh_tu_desc* 
     p->n_coeff = n_coeff;
     p->bytes = residual_bytes(tus, n_tu);
     if (n_tu) {
-        hipError_t e = hipMalloc(&p->d_mem, n_tu * sizeof(rmh_tu_desc));
-        if (e == hipSuccess) e = hipMemcpy(p->d_mem, tus, n_tu * sizeof(rmh_tu_desc), hipMemcpyHostToDevice);
+        std::vector<uint32_t> order;
+        std::vector<rmh::TileItem> tiles;
+        build_residual_tiles(tus, n_tu, order, tiles, &p->n_fast, &p->tile_lds);
+        size_t tb = aligned(n_tu * sizeof(rmh_tu_desc)), ob = aligned(order.size() * 4), ib = aligned(tiles.size() * sizeof(rmh::TileItem));
+        hipError_t e = hipMalloc(&p->d_mem, tb + ob + ib + 256);
+        char* base = (char*)p->d_mem;
+        if (e == hipSuccess) e = hipMemcpy(base, tus, n_tu * sizeof(rmh_tu_desc), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(base + tb, order.data(), order.size() * 4, hipMemcpyHostToDevice);
+        if (e == hipSuccess && !tiles.empty())
+            e = hipMemcpy(base + tb + ob, tiles.data(), tiles.size() * sizeof(rmh::TileItem), hipMemcpyHostToDevice);
         if (e != hipSuccess) {
             if (p->d_mem) hipFree(p->d_mem);
             delete p;
             return fail(ctx, RMH_EXECUTION, std::string("descriptor upload failed: ") + hipGetErrorString(e));
         }
-        p->d_tus = (const rmh_tu_desc*)p->d_mem;
+        p->d_tus = (const rmh_tu_desc*)base;
+        p->d_order = (const uint32_t*)(base + tb);
+        p->d_tiles = (const rmh::TileItem*)(base + tb + ob);
+        p->n_tiles = tiles.size();
     }
     *out = p;
     return RMH_OK;
@@ -599,7 +674,7 @@ RMH_EXPORT rmh_status rmh_residual_batch(rmh_ctx* ctx, const rmh_tu_desc* tus, s
     if (g != RMH_OK) return fail_locked(ctx, g, g == RMH_UNAVAILABLE ? "accelerator disabled" : "injected device failure");
     DeviceGuard dg(ctx->device);
     size_t tb = n_tu * sizeof(rmh_tu_desc), cb = n_coeff * sizeof(int16_t);
-    hipError_t e = ctx->dev.reserve(aligned(tb) + 2 * aligned(cb));
+    hipError_t e = ctx->dev.reserve(aligned(tb) + 2 * aligned(cb) + aligned(n_tu * 4) + aligned(n_tu * sizeof(rmh::TileItem)));
     if (e != hipSuccess) return fail_locked(ctx, RMH_LIMIT_EXCEEDED, std::string("device allocation: ") + hipGetErrorString(e));
     e = ctx->host.reserve(aligned(cb));
     if (e != hipSuccess) return fail_locked(ctx, RMH_LIMIT_EXCEEDED, std::string("host staging: ") + hipGetErrorString(e));
@@ -607,15 +682,26 @@ RMH_EXPORT rmh_status rmh_residual_batch(rmh_ctx* ctx, const rmh_tu_desc* tus, s
     int16_t* d_coeff = (int16_t*)ctx->dev.take(cb);
     int16_t* d_res = (int16_t*)ctx->dev.take(cb);
     int16_t* h_res = (int16_t*)ctx->host.take(cb);
-    hipStream_t s = ctx->stream;
-    if ((e = hipMemcpyAsync(d_tus, tus, tb, hipMemcpyHostToDevice, s)) != hipSuccess ||
-        (e = hipMemcpyAsync(d_coeff, coeff, cb, hipMemcpyHostToDevice, s)) != hipSuccess)
-        return hip_fail_locked(ctx, e, "upload");
+    std::vector<uint32_t> order;
+    std::vector<rmh::TileItem> tiles;
     rmh_plan tmp;
+    build_residual_tiles(tus, n_tu, order, tiles, &tmp.n_fast, &tmp.tile_lds);
+    hipStream_t s = ctx->stream;
+    uint32_t* d_order = (uint32_t*)ctx->dev.take(order.size() * 4);
+    rmh::TileItem* d_tiles = (rmh::TileItem*)ctx->dev.take(tiles.size() * sizeof(rmh::TileItem));
+    if ((e = hipMemcpyAsync(d_tus, tus, tb, hipMemcpyHostToDevice, s)) != hipSuccess ||
+        (e = hipMemcpyAsync(d_coeff, coeff, cb, hipMemcpyHostToDevice, s)) != hipSuccess ||
+        (e = hipMemcpyAsync(d_order, order.data(), order.size() * 4, hipMemcpyHostToDevice, s)) != hipSuccess ||
+        (!tiles.empty() &&
+         (e = hipMemcpyAsync(d_tiles, tiles.data(), tiles.size() * sizeof(rmh::TileItem), hipMemcpyHostToDevice, s)) != hipSuccess))
+        return hip_fail_locked(ctx, e, "upload");
     tmp.ctx = ctx;
     tmp.kind = kResidual;
     tmp.d_tus = d_tus;
     tmp.n_tu = n_tu;
+    tmp.d_order = d_order;
+    tmp.d_tiles = d_tiles;
+    tmp.n_tiles = tiles.size();
     std::string err;
     rmh_status r = run_plan_locked_free(&tmp, d_coeff, d_res, 0, n_tu, s, &err);
     if (r != RMH_OK) return fail_locked(ctx, r, err);
