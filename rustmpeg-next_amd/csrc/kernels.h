@@ -11,6 +11,8 @@ namespace rmh {
 // Device-resident constant tables, built once per context.
 struct DeviceTables {
     const uint32_t* mat = nullptr;   // transform matrices as int16 pairs: [kk][j] = (M[2kk][j], M[2kk+1][j])
+    const uint32_t* col16 = nullptr; // per column j: 16 pairs (M[2kk][j], M[2kk+1][j]), zero past the last row
+    int32_t col16_off[3][7];         // dword offset of (type, log2 n); -1 if unsupported
     const int8_t* lfnst8 = nullptr;  // [4][2][16][48]
     const int8_t* lfnst4 = nullptr;  // [4][2][16][16]
     const uint16_t* scans = nullptr; // grouped diagonal scans, raster positions
@@ -28,6 +30,8 @@ struct ResidualArgs {
     const int8_t* lfnst4;
     const uint16_t* scans;
     const uint32_t* order;  // optional TU index list (nullptr: descriptor order)
+    const uint32_t* col16;  // column-major padded matrix pairs (tiled kernel)
+    int32_t col16_off[3][7];
     uint32_t first_tu;
     uint32_t n_tu;
     int32_t mat_off[3][7];
@@ -51,12 +55,18 @@ struct TileItem {
 hipError_t launch_residual_tiles(const ResidualArgs& a, const TileItem* items, uint32_t n_items, uint32_t lds_bytes,
                                  hipStream_t stream);
 
-// LDS bytes the tiled kernel needs for a tile of `count` TUs of class (lw, lh).
-inline uint32_t residual_tile_lds(int lw, int lh, int count) {
+// TUs per full tile of class (lw, lh) and the LDS bytes such a tile needs
+// (the kernel lays LDS out for a full tile whatever the tile's count).
+inline int residual_tile_tus(int lw, int lh) {
+    int g = 4096 >> (lw + lh);
+    return g > 0 ? g : 1;
+}
+inline uint32_t residual_tile_lds(int lw, int lh) {
+    uint32_t count = (uint32_t)residual_tile_tus(lw, lh);
     int wc = (1 << lw) < 32 ? (1 << lw) : 32, hc = (1 << lh) < 32 ? (1 << lh) : 32;
-    uint32_t c = (uint32_t)count * wc * (hc / 2) * 4;         // coefficient columns
-    uint32_t t = (uint32_t)count * (1u << lh) * (wc / 2) * 4;  // intermediate rows
-    return c + t + (uint32_t)count * 32 + (uint32_t)count * 8;
+    uint32_t c = count * wc * (hc / 2) * 4;         // coefficient columns
+    uint32_t t = count * (1u << lh) * (wc / 2) * 4;  // intermediate rows
+    return c + t + count * 32 + 16;
 }
 
 // One inter work item: a 16x16 luma tile (and its 8x8 chroma tiles) of one PU.

@@ -379,20 +379,48 @@ __device__ __forceinline__ int dot_pairs2(const uint32_t* __restrict__ src, cons
     return acc;
 }
 
-__global__ __launch_bounds__(256) void residual_tile_kernel(ResidualArgs a, const TileItem* __restrict__ items) {
-    extern __shared__ __align__(16) uint32_t smem[];
-    const TileItem it = items[blockIdx.x];
-    const int lw = it.lw, lh = it.lh, W = 1 << lw, H = 1 << lh, G = it.count;
-    const int lwc = lw < 5 ? lw : 5, lhc = lh < 5 ? lh : 5;
-    const int Wc = 1 << lwc, Hc = 1 << lhc;
-    const int CP = Hc >> 1;  // dwords per coefficient column
-    const int TP = Wc >> 1;  // dwords per intermediate row
+// dequantize_rect_bd in int32: |level * scale| < 2^22, and a left shift whose
+// result leaves int16 saturates exactly like the clip of the int64 form.
+__device__ __forceinline__ int dequant32(int level, int scale, int rshift) {
+    const int v = level * scale;
+    if (rshift > 0) return clip16i((v + (1 << (rshift - 1))) >> rshift);
+    const int ls = -rshift;
+    if (ls >= 16) return v > 0 ? 32767 : v < 0 ? -32768 : 0;
+    if (v > (32767 >> ls)) return 32767;
+    if (v < (-32768 >> ls)) return -32768;
+    return v << ls;
+}
+
+__device__ __forceinline__ int wave_max_u(int v) { return __builtin_amdgcn_readfirstlane(wave_max(v)); }
+
+// 16 matrix pairs of one column (zero past the matrix) with four 128-bit loads.
+__device__ __forceinline__ void load_col16(const uint32_t* __restrict__ src, uint32_t (&m)[16]) {
+    const uint4* s4 = reinterpret_cast<const uint4*>(src);
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const uint4 v = s4[q];
+        m[4 * q + 0] = v.x;
+        m[4 * q + 1] = v.y;
+        m[4 * q + 2] = v.z;
+        m[4 * q + 3] = v.w;
+    }
+}
+
+template <int LW, int LH>
+__device__ __forceinline__ void tile_body(const ResidualArgs& a, const TileItem& it, uint32_t* smem) {
+    constexpr int W = 1 << LW, H = 1 << LH;
+    constexpr int LWC = LW < 5 ? LW : 5, LHC = LH < 5 ? LH : 5;
+    constexpr int Wc = 1 << LWC, Hc = 1 << LHC;
+    constexpr int CP = Hc / 2, TP = Wc / 2;  // dwords per coefficient column / intermediate row
+    constexpr int GMAX = (4096 >> (LW + LH)) > 0 ? (4096 >> (LW + LH)) : 1;
+    const int G = it.count;
     uint32_t* C = smem;
-    uint32_t* T = C + G * Wc * CP;
-    TuParams* P = reinterpret_cast<TuParams*>(T + G * H * TP);
-    int* BB = reinterpret_cast<int*>(P + G);
+    uint32_t* T = C + GMAX * Wc * CP;
+    TuParams* P = reinterpret_cast<TuParams*>(T + GMAX * H * TP);
+    int* TB = reinterpret_cast<int*>(P + GMAX);  // tile bounding box: max row, max column of non-zeros
     const int tid = threadIdx.x;
 
+    if (tid < 2) TB[tid] = -1;
     for (int g = tid; g < G; g += 256) {
         const rmh_tu_desc d = a.tus[a.order[it.first + g]];
         const int th = d.tr_hor, tv = d.tr_ver;
@@ -401,110 +429,155 @@ __global__ __launch_bounds__(256) void residual_tile_kernel(ResidualArgs a, cons
         sw = d.skip_w > sw ? d.skip_w : sw;
         sh = d.skip_h > sh ? d.skip_h : sh;
         const int qp = d.qp < 4 ? 4 : d.qp;
-        const int lsum = lw + lh, sqrt2 = lsum & 1;
+        constexpr int lsum = LW + LH, sqrt2 = lsum & 1;
         const int tr_shift = 15 - d.bit_depth - (lsum >> 1) - sqrt2;
         TuParams p;
         p.off = (int32_t)d.offset;
         p.cwch = (W - sw) | ((H - sh) << 8);
         p.scale = kInvQuant[sqrt2][qp % 6];
         p.rshift = 6 - (tr_shift + qp / 6);
-        p.mh = a.mat_off[th][lw];
-        p.mv = a.mat_off[tv][lh];
+        p.mh = a.col16_off[th][LW];
+        p.mv = a.col16_off[tv][LH];
         p.shift2 = 20 - d.bit_depth;
         p.pad = 0;
         P[g] = p;
-        BB[2 * g] = -1;
-        BB[2 * g + 1] = -1;
     }
     __syncthreads();
 
-    // phase A: 4 consecutive coefficients of one row per lane
+    // phase A: a lane takes 2 rows x 4 columns and stores 4 row pairs; the
+    // whole Hc x Wc region is written (zeros outside the coded region), so
+    // rows and columns past the tile bounding box read as zero later.
     {
-        int16_t* C16 = reinterpret_cast<int16_t*>(C);
-        const int q4 = lwc - 2;  // log2 of 4-sample groups per row
-        const int total = G << (lhc + q4);
+        constexpr int Q = Wc / 4, RP = Hc / 2;
+        const int total = G * RP * Q;
+        int mr = -1, mc = -1;
         for (int e = tid; e < total; e += 256) {
-            const int x0 = (e & ((1 << q4) - 1)) << 2;
-            const int r = (e >> q4) & (Hc - 1);
-            const int g = e >> (lhc + q4);
+            const int x0 = (e % Q) * 4;
+            const int rp = (e / Q) % RP;
+            const int g = e / (Q * RP);
             const TuParams p = P[g];
             const int cw = p.cwch & 255, ch = p.cwch >> 8;
-            int v[4] = {0, 0, 0, 0};
-            if (r < ch && x0 < cw) {
-                const uint2 raw = *reinterpret_cast<const uint2*>(a.coeff + p.off + r * W + x0);
-                const int16_t c4[4] = {(int16_t)(raw.x & 0xffff), (int16_t)(raw.x >> 16), (int16_t)(raw.y & 0xffff),
-                                       (int16_t)(raw.y >> 16)};
-                int mx = -1;
-#pragma unroll
-                for (int k = 0; k < 4; k++) {
-                    if (x0 + k < cw) {
-                        v[k] = dequant_flat(c4[k], p.scale, p.rshift);
-                        if (v[k] != 0) mx = x0 + k;
-                    }
-                }
-                if (mx >= 0) {
-                    atomicMax(&BB[2 * g], r);
-                    atomicMax(&BB[2 * g + 1], mx);
-                }
+            const int r0 = 2 * rp;
+            uint2 raw0 = make_uint2(0, 0), raw1 = make_uint2(0, 0);
+            if (x0 < cw) {
+                if (r0 < ch) raw0 = *reinterpret_cast<const uint2*>(a.coeff + p.off + r0 * W + x0);
+                if (r0 + 1 < ch) raw1 = *reinterpret_cast<const uint2*>(a.coeff + p.off + (r0 + 1) * W + x0);
             }
+            const int lo[4] = {(int)(int16_t)(raw0.x & 0xffff), (int)(int16_t)(raw0.x >> 16),
+                               (int)(int16_t)(raw0.y & 0xffff), (int)(int16_t)(raw0.y >> 16)};
+            const int hi[4] = {(int)(int16_t)(raw1.x & 0xffff), (int)(int16_t)(raw1.x >> 16),
+                               (int)(int16_t)(raw1.y & 0xffff), (int)(int16_t)(raw1.y >> 16)};
 #pragma unroll
-            for (int k = 0; k < 4; k++) C16[((g * Wc + x0 + k) * CP) * 2 + r] = (int16_t)v[k];
+            for (int k = 0; k < 4; k++) {
+                const bool inc = x0 + k < cw;
+                const int a0 = inc ? dequant32(lo[k], p.scale, p.rshift) : 0;
+                const int a1 = inc ? dequant32(hi[k], p.scale, p.rshift) : 0;
+                if (a0 | a1) {
+                    mc = x0 + k > mc ? x0 + k : mc;
+                    const int rr = a1 ? r0 + 1 : r0;
+                    mr = rr > mr ? rr : mr;
+                }
+                C[(g * Wc + x0 + k) * CP + rp] = (uint32_t)(uint16_t)a0 | ((uint32_t)(uint16_t)a1 << 16);
+            }
+        }
+        mr = wave_max(mr);
+        mc = wave_max(mc);
+        if ((tid & 63) == 0) {
+            atomicMax(&TB[0], mr);
+            atomicMax(&TB[1], mc);
         }
     }
     __syncthreads();
+    const int er = __builtin_amdgcn_readfirstlane(TB[0]) + 1;
+    const int ec = __builtin_amdgcn_readfirstlane(TB[1]) + 1;
+    // the horizontal pass reads intermediates two pairs at a time: produce the
+    // columns up to a multiple of 4 (zeros past the box, from zero coefficients)
+    const int k1 = (er + 1) >> 1, ec4 = (ec + 3) & ~3, k2 = (ec + 1) >> 1;
 
-    // phase B: vertical pass
-    {
+    // phase B: vertical pass, lane = (TU, row y, column chunk)
+    if (ec > 0) {
+        constexpr int NCH0 = 256 / (GMAX * H);
+        constexpr int NCH = NCH0 > 0 ? NCH0 : 1;
+        constexpr int ITEMS = GMAX * H * NCH;
         int16_t* T16 = reinterpret_cast<int16_t*>(T);
-        const int lnch = (G << lh) >= 256 ? 0 : (8 - lh - 31 + __clz(G));  // log2 column chunks per (TU, row)
-        const int nch = 1 << lnch;
-        const int total = (G << lh) << lnch;
-        for (int q = tid; q < total; q += 256) {
-            const int y = q & (H - 1);
-            const int c0 = (q >> lh) & (nch - 1);
-            const int g = q >> (lh + lnch);
-            const int er = BB[2 * g] + 1, ec = BB[2 * g + 1] + 1;
-            if (er <= 0) continue;
-            const int k1 = (er + 1) >> 1;
-            const uint32_t* mv = a.mat + P[g].mv;
-            uint32_t m[16];
 #pragma unroll
-            for (int kk = 0; kk < 16; kk++) m[kk] = kk < k1 ? mv[kk * H + y] : 0u;
-            const int ecp = (ec + 1) & ~1;
-            for (int i = c0; i < ecp; i += nch) {
-                int val = 0;
-                if (i < ec) val = clip16i((dot_pairs2(C + (g * Wc + i) * CP, m, k1) + 64) >> 7);
-                T16[((g * H + y) * TP) * 2 + i] = (int16_t)val;
+        for (int base = 0; base < ITEMS; base += 256) {
+            const int q = base + tid;
+            const int y = q & (H - 1);
+            const int c0 = (q / H) % NCH;
+            const int g = q / (H * NCH);
+            if (g < G) {
+                uint32_t m[16];
+                load_col16(a.col16 + P[g].mv + y * 16, m);
+                for (int i = c0; i < ec4; i += NCH) {
+                    const uint2* src = reinterpret_cast<const uint2*>(C + (g * Wc + i) * CP);
+                    int acc = 0;
+#pragma unroll
+                    for (int q2 = 0; q2 < CP / 2; q2++) {
+                        if (2 * q2 < k1) {
+                            const uint2 v = src[q2];
+                            acc = dot2(v.x, m[2 * q2], acc);
+                            acc = dot2(v.y, m[2 * q2 + 1], acc);
+                        }
+                    }
+                    T16[(g * H + y) * TP * 2 + i] = (int16_t)clip16i((acc + 64) >> 7);
+                }
             }
         }
     }
     __syncthreads();
 
-    // phase C: horizontal pass
+    // phase C: horizontal pass, lane = (TU, column x, row chunk)
     {
-        const int lnr = (G << lw) >= 256 ? 0 : (8 - lw - 31 + __clz(G));  // log2 row chunks per (TU, column)
-        const int nr = 1 << lnr;
-        const int total = (G << lw) << lnr;
-        for (int q = tid; q < total; q += 256) {
-            const int x = q & (W - 1);
-            const int r0 = (q >> lw) & (nr - 1);
-            const int g = q >> (lw + lnr);
-            const TuParams p = P[g];
-            const int er = BB[2 * g] + 1, ec = BB[2 * g + 1] + 1;
-            int16_t* out = a.residual + p.off;
-            if (er <= 0) {
-                for (int y = r0; y < H; y += nr) out[y * W + x] = 0;
-                continue;
-            }
-            const int k2 = (ec + 1) >> 1;
-            const uint32_t* mh = a.mat + p.mh;
-            uint32_t m[16];
+        constexpr int NR0 = 256 / (GMAX * W);
+        constexpr int NR = NR0 > 0 ? NR0 : 1;
+        constexpr int ITEMS = GMAX * W * NR;
 #pragma unroll
-            for (int kk = 0; kk < 16; kk++) m[kk] = kk < k2 ? mh[kk * W + x] : 0u;
-            const int rnd = 1 << (p.shift2 - 1);
-            for (int y = r0; y < H; y += nr)
-                out[y * W + x] = (int16_t)clip16i((dot_pairs2(T + (g * H + y) * TP, m, k2) + rnd) >> p.shift2);
+        for (int base = 0; base < ITEMS; base += 256) {
+            const int q = base + tid;
+            const int x = q & (W - 1);
+            const int r0 = (q / W) % NR;
+            const int g = q / (W * NR);
+            if (g < G) {
+                const TuParams p = P[g];
+                uint32_t m[16];
+                load_col16(a.col16 + p.mh + x * 16, m);
+                const int rnd = 1 << (p.shift2 - 1);
+                int16_t* out = a.residual + p.off;
+                for (int y = r0; y < H; y += NR) {
+                    const uint2* src = reinterpret_cast<const uint2*>(T + (g * H + y) * TP);
+                    int acc = 0;
+#pragma unroll
+                    for (int q2 = 0; q2 < TP / 2; q2++) {
+                        if (2 * q2 < k2) {
+                            const uint2 v = src[q2];
+                            acc = dot2(v.x, m[2 * q2], acc);
+                            acc = dot2(v.y, m[2 * q2 + 1], acc);
+                        }
+                    }
+                    out[y * W + x] = (int16_t)clip16i((acc + rnd) >> p.shift2);
+                }
+            }
         }
+    }
+}
+
+__global__ __launch_bounds__(256) void residual_tile_kernel(ResidualArgs a, const TileItem* __restrict__ items) {
+    extern __shared__ __align__(16) uint32_t smem[];
+    const TileItem it = items[blockIdx.x];
+    switch (it.lw * 8 + it.lh) {
+#define RMH_TILE(LW, LH) \
+    case LW * 8 + LH:    \
+        tile_body<LW, LH>(a, it, smem); \
+        break;
+        RMH_TILE(2, 2) RMH_TILE(2, 3) RMH_TILE(2, 4) RMH_TILE(2, 5) RMH_TILE(2, 6)
+        RMH_TILE(3, 2) RMH_TILE(3, 3) RMH_TILE(3, 4) RMH_TILE(3, 5) RMH_TILE(3, 6)
+        RMH_TILE(4, 2) RMH_TILE(4, 3) RMH_TILE(4, 4) RMH_TILE(4, 5) RMH_TILE(4, 6)
+        RMH_TILE(5, 2) RMH_TILE(5, 3) RMH_TILE(5, 4) RMH_TILE(5, 5) RMH_TILE(5, 6)
+        RMH_TILE(6, 2) RMH_TILE(6, 3) RMH_TILE(6, 4) RMH_TILE(6, 5) RMH_TILE(6, 6)
+#undef RMH_TILE
+        default:
+            break;
     }
 }
 

@@ -252,6 +252,22 @@ hipError_t build_tables(rmh_ctx* ctx) {
                     mat.push_back(lo | (hi << 16));
                 }
         }
+    std::vector<uint32_t> col16;
+    for (int t = 0; t < 3; t++)
+        for (int l = 0; l < 7; l++) {
+            int n = 1 << l;
+            ctx->tabs.col16_off[t][l] = -1;
+            if (n < 2 || !matrix_ok(t, n)) continue;
+            ctx->tabs.col16_off[t][l] = (int32_t)col16.size();
+            bool ok;
+            for (int j = 0; j < n; j++)
+                for (int kk = 0; kk < 16; kk++) {
+                    uint32_t lo = 0, hi = 0;
+                    if (2 * kk < n) lo = (uint16_t)(int16_t)rmh::transform_entry(t, n, 2 * kk, j, &ok);
+                    if (2 * kk + 1 < n) hi = (uint16_t)(int16_t)rmh::transform_entry(t, n, 2 * kk + 1, j, &ok);
+                    col16.push_back(lo | (hi << 16));
+                }
+        }
     std::vector<uint16_t> scans;
     for (int a = 0; a < 7; a++)
         for (int b = 0; b < 7; b++) {
@@ -263,12 +279,14 @@ hipError_t build_tables(rmh_ctx* ctx) {
             scans.insert(scans.end(), s.begin(), s.end());
         }
     size_t o_mat = 0, o_l8 = aligned(mat.size() * 4), o_l4 = o_l8 + aligned(sizeof(RMH_LFNST8));
-    size_t o_sc = o_l4 + aligned(sizeof(RMH_LFNST4)), total = o_sc + aligned(scans.size() * 2);
+    size_t o_sc = o_l4 + aligned(sizeof(RMH_LFNST4)), o_c16 = o_sc + aligned(scans.size() * 2);
+    size_t total = o_c16 + aligned(col16.size() * 4);
     std::vector<uint8_t> blob(total, 0);
     memcpy(blob.data() + o_mat, mat.data(), mat.size() * 4);
     memcpy(blob.data() + o_l8, RMH_LFNST8, sizeof(RMH_LFNST8));
     memcpy(blob.data() + o_l4, RMH_LFNST4, sizeof(RMH_LFNST4));
     memcpy(blob.data() + o_sc, scans.data(), scans.size() * 2);
+    memcpy(blob.data() + o_c16, col16.data(), col16.size() * 4);
     hipError_t e = hipMalloc(&ctx->d_tables, total);
     if (e != hipSuccess) return e;
     e = hipMemcpy(ctx->d_tables, blob.data(), total, hipMemcpyHostToDevice);
@@ -278,6 +296,7 @@ hipError_t build_tables(rmh_ctx* ctx) {
     ctx->tabs.lfnst8 = (const int8_t*)(base + o_l8);
     ctx->tabs.lfnst4 = (const int8_t*)(base + o_l4);
     ctx->tabs.scans = (const uint16_t*)(base + o_sc);
+    ctx->tabs.col16 = (const uint32_t*)(base + o_c16);
     return hipSuccess;
 }
 
@@ -428,9 +447,6 @@ bool tile_eligible(const rmh_tu_desc& t) {
            (t.offset & 3) == 0;
 }
 
-// Samples per tile of the tiled residual kernel (one 256-lane workgroup).
-constexpr int kTileSamples = 4096;
-
 // Groups tile-eligible TUs by size class into tiles (largest class first) and
 // appends the remaining TUs; returns the tile list and the TU order.
 void build_residual_tiles(const rmh_tu_desc* tus, size_t n_tu, std::vector<uint32_t>& order,
@@ -451,12 +467,11 @@ void build_residual_tiles(const rmh_tu_desc* tus, size_t n_tu, std::vector<uint3
             int lh = area - lw;
             if (lh < 2 || lh > 6) continue;
             std::vector<uint32_t>& v = cls[lw][lh];
-            int per = kTileSamples >> area;
-            if (per < 1) per = 1;
+            int per = rmh::residual_tile_tus(lw, lh);
             for (size_t f = 0; f < v.size(); f += per) {
                 int cnt = (int)std::min<size_t>(per, v.size() - f);
                 tiles.push_back({(uint32_t)(order.size() + f), (uint16_t)cnt, (uint8_t)lw, (uint8_t)lh});
-                uint32_t b = rmh::residual_tile_lds(lw, lh, cnt);
+                uint32_t b = rmh::residual_tile_lds(lw, lh);
                 if (b > *lds) *lds = b;
             }
             order.insert(order.end(), v.begin(), v.end());
@@ -480,6 +495,8 @@ rmh_status run_plan_locked_free(rmh_plan* plan, const int16_t* d_coeff, int16_t*
         memcpy(a.mat_off, plan->ctx->tabs.mat_off, sizeof a.mat_off);
         memcpy(a.scan_off, plan->ctx->tabs.scan_off, sizeof a.scan_off);
         memcpy(a.scan_len, plan->ctx->tabs.scan_len, sizeof a.scan_len);
+        a.col16 = plan->ctx->tabs.col16;
+        memcpy(a.col16_off, plan->ctx->tabs.col16_off, sizeof a.col16_off);
         const bool whole = first == 0 && count == plan->n_tu && plan->d_order;
         const bool aligned8 = ((uintptr_t)d_coeff & 7) == 0;
         if (whole && aligned8) {
