@@ -247,6 +247,27 @@ rmh_status rmh_graph_launch(rmh_graph* graph, void* stream);
 void rmh_graph_destroy(rmh_graph* graph);
 
 /* ------------------------------------------------------------------------ */
+/* Caller-side driver: intra-profile subset decode                           */
+/* ------------------------------------------------------------------------ */
+
+/* Two-pass reconstruction of one 8-bit 4:2:0 picture coded with the intra
+ * subset (CTU = CU = TU = 32 luma / 16 chroma, PLANAR or DC per CTU, chroma
+ * DM), the decode path of slice::decode_picture -> reconstruct_block
+ * (crates/rustmpeg-video/src/vvc/slice.rs:458-632, 1050-1097) split as
+ *   (i)  the caller parses the picture into a TU batch (three TUs per CTU in
+ *        raster order: Y 32x32, Cb 16x16, Cr 16x16) and one mode per CTU
+ *        (0 PLANAR, 1 DC);
+ *   (ii) every residual of the picture is computed in one device batch;
+ *   (iii) PLANAR / DC prediction (reference filter, PDPC) and the add run on
+ *        the host in coding order, since each CTU predicts from its
+ *        reconstructed neighbours.
+ * width and height are multiples of 32; y/u/v receive the reconstruction
+ * (row pitch = width, width / 2) and are untouched on error. */
+rmh_status rmh_intra_subset_decode(rmh_ctx* ctx, uint32_t width, uint32_t height, const uint8_t* modes,
+                                   const rmh_tu_desc* tus, size_t n_tu, const int16_t* coeff, size_t n_coeff,
+                                   uint8_t* y, uint8_t* u, uint8_t* v);
+
+/* ------------------------------------------------------------------------ */
 /* Introspection (tests)                                                     */
 /* ------------------------------------------------------------------------ */
 

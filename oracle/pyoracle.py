@@ -70,6 +70,10 @@ def _setup(L):
     L.orc_deblock_luma_plane.argtypes = [U16P] + [c_int] * 9
     L.orc_deblock_chroma_plane.argtypes = [U16P] + [c_int] * 9
     L.orc_tc_beta.argtypes = [c_int] * 5 + [I32P, I32P]
+    U8P = ctypes.POINTER(ctypes.c_uint8)
+    L.orc_intra_predict.argtypes = [U8P] + [c_int] * 8 + [I32P]
+    L.orc_intra_encode.argtypes = [U8P, U8P, U8P, c_int, c_int, c_int, U8P, U8P, U8P, U8P, ctypes.c_void_p, I16P]
+    L.orc_intra_decode.argtypes = [c_int, c_int, U8P, ctypes.c_void_p, I16P, U8P, U8P, U8P]
 
 
 def _p(a, t):
@@ -247,3 +251,33 @@ def tc_beta(qp, bs, beta_off, tc_off, bd):
     beta = ctypes.c_int32()
     lib().orc_tc_beta(qp, bs, beta_off, tc_off, bd, ctypes.byref(tc), ctypes.byref(beta))
     return tc.value, beta.value
+
+
+U8P = ctypes.POINTER(ctypes.c_uint8)
+
+
+def intra_encode(y, u, v, qp, tu_dtype):
+    """Subset intra encoder decisions for one 8-bit 4:2:0 frame; returns
+    (recon (y, u, v), modes, tus, coeff)."""
+    h, w = y.shape
+    n_ctu = (w // 32) * (h // 32)
+    ry, ru, rv = np.zeros_like(y), np.zeros_like(u), np.zeros_like(v)
+    modes = np.zeros(n_ctu, np.uint8)
+    tus = np.zeros(3 * n_ctu, tu_dtype)
+    coeff = np.zeros(n_ctu * (1024 + 512), np.int16)
+    args = [np.ascontiguousarray(a, dtype=np.uint8) for a in (y, u, v)]
+    n = lib().orc_intra_encode(_p(args[0], U8P), _p(args[1], U8P), _p(args[2], U8P), w, h, qp, _p(ry, U8P),
+                               _p(ru, U8P), _p(rv, U8P), _p(modes, U8P), tus.ctypes.data, _p(coeff, I16P))
+    assert n == n_ctu
+    return (ry, ru, rv), modes, tus, coeff
+
+
+def intra_decode(w, h, modes, tus, coeff):
+    ry = np.zeros((h, w), np.uint8)
+    ru = np.zeros((h // 2, w // 2), np.uint8)
+    rv = np.zeros((h // 2, w // 2), np.uint8)
+    n = lib().orc_intra_decode(w, h, _p(np.ascontiguousarray(modes, dtype=np.uint8), U8P), tus.ctypes.data,
+                               _p(np.ascontiguousarray(coeff, dtype=np.int16), I16P), _p(ry, U8P), _p(ru, U8P),
+                               _p(rv, U8P))
+    assert n == len(modes)
+    return ry, ru, rv

@@ -93,6 +93,14 @@ void orc_deblock_luma_plane(uint16_t* plane, int w, int h, int bd, int cu_step, 
 void orc_deblock_chroma_plane(uint16_t* plane, int w, int h, int bd, int cu_step, int ctb_c,
                               int qp, int bs, int beta_off, int tc_off);
 
+/* ---- intra-profile subset codec (harness for the residual stage) ---- */
+void orc_intra_predict(const uint8_t* plane, int pw, int ph, int bx, int by, int w, int h, int is_luma, int mode,
+                       int32_t* pred);
+int orc_intra_encode(const uint8_t* sy, const uint8_t* su, const uint8_t* sv, int w, int h, int qp, uint8_t* ry,
+                     uint8_t* ru, uint8_t* rv, uint8_t* modes, rmh_tu_desc* tus, int16_t* coeff);
+int orc_intra_decode(int w, int h, const uint8_t* modes, const rmh_tu_desc* tus, const int16_t* coeff, uint8_t* ry,
+                     uint8_t* ru, uint8_t* rv);
+
 /* tC and beta after bit-depth scaling (deblock_luma_edge_bd, vvc/filter.rs:653-666). */
 void orc_tc_beta(int qp, int bs, int beta_off, int tc_off, int bd, int* tc, int* beta);
 

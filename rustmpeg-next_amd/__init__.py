@@ -68,7 +68,7 @@ EXPORTED = [
     "rmh_inject_failures", "rmh_residual_batch", "rmh_residual_plan_create", "rmh_residual_plan_run",
     "rmh_residual_plan_run_range", "rmh_mc_batch", "rmh_mc_plan_create", "rmh_mc_plan_run", "rmh_deblock",
     "rmh_deblock_plan_create", "rmh_deblock_plan_run", "rmh_plan_destroy", "rmh_plan_bytes", "rmh_graph_create",
-    "rmh_graph_launch", "rmh_graph_destroy", "rmh_transform_matrix",
+    "rmh_graph_launch", "rmh_graph_destroy", "rmh_transform_matrix", "rmh_intra_subset_decode",
 ]
 
 
@@ -105,6 +105,7 @@ def load_library(path=LIB_PATH):
         "rmh_graph_launch": ([vp, vp], st),
         "rmh_graph_destroy": ([vp], None),
         "rmh_transform_matrix": ([ctypes.c_int32, ctypes.c_int32, vp], st),
+        "rmh_intra_subset_decode": ([vp, ctypes.c_uint32, ctypes.c_uint32, vp, vp, sz, vp, sz, vp, vp, vp], st),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -246,6 +247,20 @@ class Accelerator:
         """frames: list of abi.DbkFrame (host planes, filtered in place)."""
         arr = (abi.DbkFrame * len(frames))(*frames)
         self._check(load_library().rmh_deblock(self.handle, ctypes.addressof(arr), len(frames)))
+
+    def intra_subset_decode(self, width, height, modes, tus, coeffs):
+        """Two-pass decode of an intra-subset picture (rmh_intra_subset_decode):
+        the residual batch on the device, prediction and add on the host."""
+        modes = np.ascontiguousarray(modes, dtype=np.uint8)
+        tus = np.ascontiguousarray(tus, dtype=TU_DTYPE)
+        coeffs = np.ascontiguousarray(coeffs, dtype=np.int16)
+        y = np.zeros((height, width), np.uint8)
+        u = np.zeros((height // 2, width // 2), np.uint8)
+        v = np.zeros((height // 2, width // 2), np.uint8)
+        self._check(load_library().rmh_intra_subset_decode(self.handle, width, height, modes.ctypes.data,
+                                                           tus.ctypes.data, len(tus), coeffs.ctypes.data,
+                                                           coeffs.size, y.ctypes.data, u.ctypes.data, v.ctypes.data))
+        return y, u, v
 
     # -- device-resident plans ---------------------------------------------
     def residual_plan(self, tus, n_coeff):
