@@ -94,6 +94,128 @@ def cpu_baseline(frames, seconds):
                       "(oracle/orc_residual.c) on one host core" % (len(frames), reps, el)}
 
 
+def timed(hip, stream, fn, steps, warmup):
+    """Average ms of fn(stream) over `steps` calls after `warmup` calls, by HIP
+    events on `stream`."""
+    for _ in range(warmup):
+        fn(stream.handle)
+    hip.device_synchronize()
+    ev0, ev1 = hip.Event(), hip.Event()
+    ev0.record(stream.handle)
+    for _ in range(steps):
+        fn(stream.handle)
+    ev1.record(stream.handle)
+    ev1.synchronize()
+    hip.device_synchronize()
+    return ev0.elapsed_ms(ev1) / steps
+
+
+def stage_roofline(plan, ms):
+    gbs = plan.bytes / (ms / 1e3) / 1e9
+    return {"ms": round(ms, 4), "algorithmic_bytes": plan.bytes, "achieved_gbs": round(gbs, 1),
+            "frac": round(gbs / HBM_PEAK_GBS, 4)}
+
+
+def check_batch(PL, po, rmh, batch, frames, which):
+    """Bit-exact gate: the frames at indices `which` against the oracle."""
+    got = batch.download()
+    for i in which:
+        want = PL.oracle(po, rmh.abi, [frames[i]])[0]
+        for c in range(3):
+            if not (got[i][c] == want[c]).all():
+                log("bench: frame %d plane %d differs from the oracle" % (i, c))
+                return False
+    return True
+
+
+def bench_pipeline(rmh, acc, W, PL, po, n_frames, steps, warmup):
+    """P-frame reconstruction, MC -> residual add -> deblock, over n_frames
+    1080p10 frames: one frame-step graph over the batch against the same
+    plans built per frame (unbatched)."""
+    hip = rmh.hip
+    t0 = time.perf_counter()
+    frames = [W.pframe(2000 + i) for i in range(n_frames)]
+    log("pipeline: generated %d P-frames in %.1f s" % (n_frames, time.perf_counter() - t0))
+    batch = PL.DeviceBatch(acc, rmh, frames)
+    stream = hip.Stream()
+    graph = batch.graph()
+    graph.launch(stream.handle)
+    stream.synchronize()
+    if not check_batch(PL, po, rmh, batch, frames, (0, n_frames - 1)):
+        return None
+    ms = timed(hip, stream, graph.launch, steps, warmup)
+    px = batch.luma_pixels()
+    stages = {"mc": stage_roofline(batch.mc, timed(hip, stream, lambda s: batch.mc.run(stream=s), steps, 1)),
+              "recon": stage_roofline(batch.recon, timed(hip, stream, lambda s: batch.recon.run(batch.d_coeff.ptr, 0, s),
+                                                          steps, 1)),
+              "deblock": stage_roofline(batch.deblock, timed(hip, stream, lambda s: batch.deblock.run(stream=s),
+                                                             steps, 1))}
+    singles = [PL.DeviceBatch(acc, rmh, [f]) for f in frames]
+
+    def unbatched(s):
+        for b in singles:
+            b.run(s)
+    ums = timed(hip, stream, unbatched, steps, warmup)
+    for b in singles:
+        b.close()
+    graph.close()
+    batch.close()
+    out = {"workload": "%d P-frames 1920x1080 10-bit 4:2:0: MC (8/4-tap, MVs +-64 px) -> residual add "
+                       "(transform trees of the coded CUs) -> deblock, one frame-step graph" % n_frames,
+           "value": round(px / (ms / 1e3) / 1e6, 1), "unit": "Mpix/s", "ms_per_step": round(ms, 4),
+           "unbatched": {"value": round(px / (ums / 1e3) / 1e6, 1), "ms_per_step": round(ums, 4),
+                         "launches_per_step": 4 * n_frames},
+           "stages": stages}
+    out["vs_unbatched"] = round(out["value"] / out["unbatched"]["value"], 3)
+    log("pipeline: %.3f ms/step %.1f Mpix/s, unbatched %.3f ms (x%.2f)" % (ms, out["value"], ums, out["vs_unbatched"]))
+    return out
+
+
+STREAMS = [(856, 480, 8)] * 4 + [(1920, 1080, 10)] * 8 + [(3840, 2160, 10)] * 4
+
+
+def bench_streams(rmh, acc, W, PL, po, frame_steps, reps):
+    """16 concurrent streams (4x 854x480 8-bit coded as 856x480, 8x 1080p10,
+    4x 2160p10), one frame of each per frame step: one graph per frame step
+    over all streams against per-stream plans launched one after another."""
+    hip = rmh.hip
+    t0 = time.perf_counter()
+    frames = [W.pframe(3000 + i, width=w, height=h, bit_depth=bd) for i, (w, h, bd) in enumerate(STREAMS)]
+    log("streams: generated %d frames in %.1f s" % (len(frames), time.perf_counter() - t0))
+    batch = PL.DeviceBatch(acc, rmh, frames)
+    stream = hip.Stream()
+    graph = batch.graph()
+    graph.launch(stream.handle)
+    stream.synchronize()
+    if not check_batch(PL, po, rmh, batch, frames, (0, 4, 12)):
+        return None
+
+    def run_steps(s):
+        for _ in range(frame_steps):
+            graph.launch(s)
+    ms = timed(hip, stream, run_steps, reps, 1)
+    px = batch.luma_pixels() * frame_steps
+    singles = [PL.DeviceBatch(acc, rmh, [f]) for f in frames]
+
+    def unbatched(s):
+        for _ in range(frame_steps):
+            for b in singles:
+                b.run(s)
+    ums = timed(hip, stream, unbatched, reps, 1)
+    for b in singles:
+        b.close()
+    graph.close()
+    batch.close()
+    out = {"workload": "16 streams (4x 856x480 8-bit, 8x 1920x1080 10-bit, 4x 3840x2160 10-bit) x %d frame steps, "
+                       "MC -> residual add -> deblock, one graph per frame step" % frame_steps,
+           "value": round(px / (ms / 1e3) / 1e6, 1), "unit": "Mpix/s", "ms_per_frame_step": round(ms / frame_steps, 4),
+           "unbatched": {"value": round(px / (ums / 1e3) / 1e6, 1), "ms_per_frame_step": round(ums / frame_steps, 4),
+                         "launches_per_frame_step": 4 * len(frames)}}
+    out["vs_unbatched"] = round(out["value"] / out["unbatched"]["value"], 3)
+    log("streams: %.3f ms/frame step %.1f Mpix/s, unbatched x%.2f" % (ms / frame_steps, out["value"], out["vs_unbatched"]))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -104,6 +226,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-unbatched", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true")
+    ap.add_argument("--pipeline-frames", type=int, default=8)
+    ap.add_argument("--no-pipeline", action="store_true")
+    ap.add_argument("--no-streams", action="store_true")
     args = ap.parse_args()
 
     if args.copy_calibration:
@@ -199,6 +324,12 @@ def main():
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "kernel": "residual_kernel", "algorithmic_bytes_per_launch": plan.bytes}
 
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    import pipeline as PL
+    pipe = None if args.no_pipeline else bench_pipeline(rmh, acc, W, PL, po, args.pipeline_frames, args.steps,
+                                                         args.warmup)
+    streams = None if args.no_streams else bench_streams(rmh, acc, W, PL, po, 8, max(2, args.steps // 5))
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(frames[:2], args.cpu_seconds)
@@ -217,6 +348,8 @@ def main():
         "vs_unbatched": round(value / unb["value"], 3) if unb else None,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "pframe_pipeline": pipe,
+        "multi_stream": streams,
         "device": info,
     }
     print(json.dumps(out), flush=True)

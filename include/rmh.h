@@ -153,6 +153,28 @@ typedef struct rmh_plane {
     uint16_t reserved;
 } rmh_plane;
 
+/* Inter reconstruction (the add of reconstruct_tus / reconstruct_component_full,
+ * crates/rustmpeg-video/src/vvc/decode_general.rs:3885-4096, slice.rs:1920-1977):
+ * instead of writing int16 residuals, a recon plan adds each TU's residual to
+ * the prediction already in a picture plane and clips to [0, 2^bd - 1], in
+ * place. TU i lands at (x, y) of plane `plane` of frame `frame`. */
+typedef struct rmh_tu_place {
+    uint16_t frame;
+    uint8_t plane;     /* 0 Y, 1 Cb, 2 Cr */
+    uint8_t reserved;  /* must be 0 */
+    uint16_t x;
+    uint16_t y;
+} rmh_tu_place;
+
+typedef struct rmh_recon_frame {
+    rmh_plane planes[3];  /* device addresses; bit_depth must match the TUs */
+} rmh_recon_frame;
+
+/* Recon plan; run it with rmh_residual_plan_run(plan, d_coeff, NULL, stream). */
+rmh_status rmh_residual_recon_plan_create(rmh_ctx* ctx, const rmh_tu_desc* tus, size_t n_tu, size_t n_coeff,
+                                          const rmh_tu_place* place, const rmh_recon_frame* frames,
+                                          size_t n_frames, rmh_plan** out);
+
 /* ------------------------------------------------------------------------ */
 /* Inter prediction (uni-prediction, 4:2:0)                                  */
 /* ------------------------------------------------------------------------ */

@@ -527,3 +527,39 @@ size_t orc_residual_batch(const rmh_tu_desc* tus, size_t n_tu, const int16_t* co
     }
     return 0;
 }
+
+/* Residual added to the prediction held in the picture and clipped to the
+ * sample range: the add of reconstruct_tu (vvc/decode_general.rs:4086-4092,
+ * `(pred + residual).clamp(0, max_val)`), which reconstruct_tus applies to the
+ * inter prediction written by write_prediction (decode_general.rs:2641-2667,
+ * 2866-2870). Planes are host memory; TUs in descriptor order. */
+size_t orc_recon_batch(const rmh_tu_desc* tus, size_t n_tu, const int16_t* coeff, size_t n_coeff,
+                       const rmh_tu_place* place, const rmh_recon_frame* frames, size_t n_frames) {
+    int16_t res[64 * 64];
+    for (size_t i = 0; i < n_tu; i++) {
+        const rmh_tu_desc* t = &tus[i];
+        if (t->log2_w > 6 || t->log2_h > 6) return i + 1;
+        const int w = 1 << t->log2_w, h = 1 << t->log2_h;
+        if ((size_t)t->offset + (size_t)w * h > n_coeff) return i + 1;
+        if (place[i].frame >= n_frames || place[i].plane > 2) return i + 1;
+        const rmh_plane* p = &frames[place[i].frame].planes[place[i].plane];
+        if ((uint32_t)place[i].x + w > p->width || (uint32_t)place[i].y + h > p->height) return i + 1;
+        if (orc_residual_tu(t, coeff + t->offset, res)) return i + 1;
+        const int maxv = (1 << p->bit_depth) - 1;
+        for (int y = 0; y < h; y++)
+            for (int x = 0; x < w; x++) {
+                const size_t at = (size_t)(place[i].y + y) * p->stride + place[i].x + x;
+                int v = res[y * w + x];
+                if (p->bytes_per_sample == 1) {
+                    uint8_t* s = (uint8_t*)p->data + at;
+                    v += *s;
+                    *s = (uint8_t)(v < 0 ? 0 : v > maxv ? maxv : v);
+                } else {
+                    uint16_t* s = (uint16_t*)p->data + at;
+                    v += *s;
+                    *s = (uint16_t)(v < 0 ? 0 : v > maxv ? maxv : v);
+                }
+            }
+    }
+    return 0;
+}

@@ -61,6 +61,8 @@ def _setup(L):
     L.orc_interp_chroma.argtypes = [U16P] + [c_int] * 11 + [I32P]
     L.orc_interp_luma_16_as_written.argtypes = [U16P] + [c_int] * 11 + [I32P]
     L.orc_weighted_average_bi.argtypes = [I32P, I32P, c_size, c_int, c_int, I32P]
+    L.orc_recon_batch.argtypes = [ctypes.c_void_p, c_size, I16P, c_size, ctypes.c_void_p, ctypes.c_void_p, c_size]
+    L.orc_recon_batch.restype = c_size
     L.orc_mc_batch.argtypes = [ctypes.c_void_p, c_size, ctypes.c_void_p, c_size]
     L.orc_mc_batch.restype = c_size
     L.orc_deblock_luma_edge.argtypes = [I32P] + [c_int] * 9

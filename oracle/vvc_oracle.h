@@ -54,6 +54,9 @@ int orc_residual_tu(const rmh_tu_desc* tu, const int16_t* coeff, int16_t* residu
 /* Every TU of a batch; returns the index+1 of the first rejected TU, 0 on success. */
 size_t orc_residual_batch(const rmh_tu_desc* tus, size_t n_tu, const int16_t* coeff,
                           size_t n_coeff, int16_t* residual);
+/* Residual + prediction, clipped, in place in host planes (reconstruct_tu's add). */
+size_t orc_recon_batch(const rmh_tu_desc* tus, size_t n_tu, const int16_t* coeff, size_t n_coeff,
+                       const rmh_tu_place* place, const rmh_recon_frame* frames, size_t n_frames);
 
 /* ---- inter MC stage ---- */
 /* Luma interpolation (interpolate_luma_block, vvc/inter.rs:896-1051, generalised

@@ -69,6 +69,7 @@ EXPORTED = [
     "rmh_residual_plan_run_range", "rmh_mc_batch", "rmh_mc_plan_create", "rmh_mc_plan_run", "rmh_deblock",
     "rmh_deblock_plan_create", "rmh_deblock_plan_run", "rmh_plan_destroy", "rmh_plan_bytes", "rmh_graph_create",
     "rmh_graph_launch", "rmh_graph_destroy", "rmh_transform_matrix", "rmh_intra_subset_decode",
+    "rmh_residual_recon_plan_create",
 ]
 
 
@@ -92,6 +93,7 @@ def load_library(path=LIB_PATH):
         "rmh_residual_batch": ([vp, vp, sz, vp, sz, vp], st),
         "rmh_residual_plan_create": ([vp, vp, sz, sz, ctypes.POINTER(vp)], st),
         "rmh_residual_plan_run": ([vp, vp, vp, vp], st),
+        "rmh_residual_recon_plan_create": ([vp, vp, sz, sz, vp, vp, sz, ctypes.POINTER(vp)], st),
         "rmh_residual_plan_run_range": ([vp, sz, sz, vp, vp, vp], st),
         "rmh_mc_batch": ([vp, vp, sz, vp, sz], st),
         "rmh_mc_plan_create": ([vp, vp, sz, vp, sz, ctypes.POINTER(vp)], st),
@@ -268,6 +270,21 @@ class Accelerator:
         h = ctypes.c_void_p()
         self._check(load_library().rmh_residual_plan_create(self.handle, tus.ctypes.data, len(tus), n_coeff,
                                                             ctypes.byref(h)))
+        return Plan(self, h.value, "residual")
+
+    def recon_plan(self, tus, n_coeff, place, frames):
+        """Residual plan that adds each TU's residual into a picture plane
+        (place: PLACE_DTYPE array, one row per TU; frames: list of abi.ReconFrame
+        with device planes). Run it with plan.run(d_coeff, 0)."""
+        tus = np.ascontiguousarray(tus, dtype=TU_DTYPE)
+        place = np.ascontiguousarray(place, dtype=abi.PLACE_DTYPE)
+        if len(place) != len(tus):
+            raise InvalidInput("one placement per TU")
+        arr = (abi.ReconFrame * len(frames))(*frames)
+        h = ctypes.c_void_p()
+        self._check(load_library().rmh_residual_recon_plan_create(self.handle, tus.ctypes.data, len(tus), n_coeff,
+                                                                  place.ctypes.data, ctypes.addressof(arr),
+                                                                  len(frames), ctypes.byref(h)))
         return Plan(self, h.value, "residual")
 
     def mc_plan(self, frames, pus):

@@ -40,6 +40,14 @@ class McFrame(ctypes.Structure):
     _fields_ = [("ref", Plane * 3), ("dst", Plane * 3)]
 
 
+class ReconFrame(ctypes.Structure):
+    _fields_ = [("planes", Plane * 3)]
+
+
+# rmh_tu_place
+PLACE_DTYPE = np.dtype([("frame", "<u2"), ("plane", "u1"), ("reserved", "u1"), ("x", "<u2"), ("y", "<u2")])
+
+
 class DbkFrame(ctypes.Structure):
     _fields_ = [("planes", Plane * 3), ("edges", ctypes.c_void_p), ("grid_w", ctypes.c_uint32),
                 ("grid_h", ctypes.c_uint32), ("qp", ctypes.c_uint8 * 3), ("beta_offset_div2", ctypes.c_int8 * 3),

@@ -32,6 +32,8 @@ struct ResidualArgs {
     const uint32_t* order;  // optional TU index list (nullptr: descriptor order)
     const uint32_t* col16;  // column-major padded matrix pairs (tiled kernel)
     int32_t col16_off[3][7];
+    const rmh_tu_place* place;         // recon mode when non-null: add into frames' planes
+    const rmh_recon_frame* frames;
     uint32_t first_tu;
     uint32_t n_tu;
     int32_t mat_off[3][7];
@@ -66,7 +68,7 @@ inline uint32_t residual_tile_lds(int lw, int lh) {
     int wc = (1 << lw) < 32 ? (1 << lw) : 32, hc = (1 << lh) < 32 ? (1 << lh) : 32;
     uint32_t c = count * wc * (hc / 2) * 4;         // coefficient columns
     uint32_t t = count * (1u << lh) * (wc / 2) * 4;  // intermediate rows
-    return c + t + count * 32 + 16;
+    return c + t + count * 48 + 16;
 }
 
 // One inter work item: a 16x16 luma tile (and its 8x8 chroma tiles) of one PU.

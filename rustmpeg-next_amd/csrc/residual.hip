@@ -83,6 +83,41 @@ __device__ __forceinline__ int dot_pairs(const uint32_t* __restrict__ src, const
     return acc;
 }
 
+
+// Recon target of one TU: the address of sample (0, 0) of its block in its
+// plane, the row pitch (samples), the sample size and the maximum value.
+struct ReconDst {
+    char* base;
+    int stride;
+    int bps;
+    int maxv;
+};
+
+__device__ __forceinline__ ReconDst recon_dst(const rmh_tu_place* __restrict__ place,
+                                              const rmh_recon_frame* __restrict__ frames, uint32_t tu_index) {
+    const rmh_tu_place pl = place[tu_index];
+    const rmh_plane p = frames[pl.frame].planes[pl.plane];
+    ReconDst d;
+    d.bps = p.bytes_per_sample;
+    d.stride = (int)p.stride;
+    d.maxv = (1 << p.bit_depth) - 1;
+    d.base = (char*)p.data + ((size_t)pl.y * p.stride + pl.x) * d.bps;
+    return d;
+}
+
+// pred + res clipped to [0, maxv], in place (reconstruct_tu add and clip).
+__device__ __forceinline__ void recon_add(const ReconDst& d, int x, int y, int res) {
+    if (d.bps == 1) {
+        uint8_t* q = (uint8_t*)d.base + (size_t)y * d.stride + x;
+        const int v = (int)*q + res;
+        *q = (uint8_t)(v < 0 ? 0 : v > d.maxv ? d.maxv : v);
+    } else {
+        uint16_t* q = (uint16_t*)d.base + (size_t)y * d.stride + x;
+        const int v = (int)*q + res;
+        *q = (uint16_t)(v < 0 ? 0 : v > d.maxv ? d.maxv : v);
+    }
+}
+
 __global__ __launch_bounds__(64) void residual_kernel(ResidualArgs a) {
     __shared__ __align__(16) uint32_t ct[32 * kPitch];  // coefficients: column i, int16 pairs along row k
     __shared__ __align__(16) uint32_t tt[64 * kPitch];  // intermediates: row y, int16 pairs along column i
@@ -96,10 +131,19 @@ __global__ __launch_bounds__(64) void residual_kernel(ResidualArgs a) {
     const int bd = d.bit_depth;
     const int qp = d.qp < 4 ? 4 : d.qp;
     const int16_t* __restrict__ cin = a.coeff + d.offset;
-    int16_t* __restrict__ rout = a.residual + d.offset;
+    int16_t* __restrict__ rout = a.place ? nullptr : a.residual + d.offset;
+    ReconDst dst{};
+    if (a.place) dst = recon_dst(a.place, a.frames, tu_index);
+    // writes residual e (raster index in the TU) to the residual buffer or adds it to the picture
+    auto put = [&](int e, int v) {
+        if (rout)
+            rout[e] = (int16_t)v;
+        else
+            recon_add(dst, e & (w - 1), e >> lw, v);
+    };
 
     if (d.flags & RMH_TU_TRANSFORM_SKIP) {
-        for (int e = lane; e < n; e += 64) rout[e] = (int16_t)dequant_ts(cin[e], qp);
+        for (int e = lane; e < n; e += 64) put(e, dequant_ts(cin[e], qp));
         return;
     }
 
@@ -269,7 +313,7 @@ __global__ __launch_bounds__(64) void residual_kernel(ResidualArgs a) {
     const int er = __builtin_amdgcn_readfirstlane(wave_max(my_r)) + 1;  // effective rows
     const int ec = __builtin_amdgcn_readfirstlane(wave_max(my_c)) + 1;  // effective columns
     if (er <= 0) {
-        for (int e = lane; e < n; e += 64) rout[e] = 0;
+        for (int e = lane; e < n; e += 64) put(e, 0);
         return;
     }
 
@@ -297,7 +341,7 @@ __global__ __launch_bounds__(64) void residual_kernel(ResidualArgs a) {
                 uint32_t pc = (uint32_t)(uint16_t)c0 | ((uint32_t)(uint16_t)c1 << 16);
                 acc = dot2(pc, mp[kk * len + j], acc);
             }
-            rout[j] = (int16_t)clip16i((acc + (1 << (shift - 1))) >> shift);
+            put(j, clip16i((acc + (1 << (shift - 1))) >> shift));
         }
         return;
     }
@@ -331,7 +375,7 @@ __global__ __launch_bounds__(64) void residual_kernel(ResidualArgs a) {
         for (int kk = 0; kk < 16; kk++) m[kk] = kk < k2 ? mh[kk * w + x] : 0u;
         const int shift = 6 + 15 - 1 - bdc;
         const int rnd = 1 << (shift - 1);
-        for (int y = g; y < h; y += G) rout[y * w + x] = (int16_t)clip16i((dot_pairs(tt + y * kPitch, m, k2) + rnd) >> shift);
+        for (int y = g; y < h; y += G) put(y * w + x, clip16i((dot_pairs(tt + y * kPitch, m, k2) + rnd) >> shift));
     }
 }
 
@@ -360,7 +404,10 @@ struct TuParams {
     int32_t mh;      // dword offset of Mh pairs
     int32_t mv;      // dword offset of Mv pairs
     int32_t shift2;  // 20 - bd
-    int32_t pad;
+    int32_t stride;  // recon: picture row pitch (samples)
+    char* base;      // recon: sample (0, 0) of the block; null for residual output
+    int32_t bps;     // recon: bytes per sample
+    int32_t maxv;    // recon: 2^bd - 1
 };
 
 // dot2 over pairs kk < npairs read 2 at a time (8-byte LDS reads); m[kk] = 0
@@ -406,8 +453,13 @@ __device__ __forceinline__ void load_col16(const uint32_t* __restrict__ src, uin
     }
 }
 
+// `place` / `frames` come in as values read once at kernel entry: reading them
+// through `a` inside the per-TU loop makes the compiler copy the whole
+// argument block to scratch memory.
 template <int LW, int LH>
-__device__ __forceinline__ void tile_body(const ResidualArgs& a, const TileItem& it, uint32_t* smem) {
+__device__ __forceinline__ void tile_body(const ResidualArgs& a, const TileItem& it, uint32_t* smem,
+                                          const rmh_tu_place* __restrict__ place,
+                                          const rmh_recon_frame* __restrict__ frames) {
     constexpr int W = 1 << LW, H = 1 << LH;
     constexpr int LWC = LW < 5 ? LW : 5, LHC = LH < 5 ? LH : 5;
     constexpr int Wc = 1 << LWC, Hc = 1 << LHC;
@@ -439,7 +491,17 @@ __device__ __forceinline__ void tile_body(const ResidualArgs& a, const TileItem&
         p.mh = a.col16_off[th][LW];
         p.mv = a.col16_off[tv][LH];
         p.shift2 = 20 - d.bit_depth;
-        p.pad = 0;
+        p.stride = 0;
+        p.base = nullptr;
+        p.bps = 0;
+        p.maxv = 0;
+        if (place) {
+            const ReconDst rd = recon_dst(place, frames, a.order[it.first + g]);
+            p.stride = rd.stride;
+            p.base = rd.base;
+            p.bps = rd.bps;
+            p.maxv = rd.maxv;
+        }
         P[g] = p;
     }
     __syncthreads();
@@ -544,6 +606,7 @@ __device__ __forceinline__ void tile_body(const ResidualArgs& a, const TileItem&
                 load_col16(a.col16 + p.mh + x * 16, m);
                 const int rnd = 1 << (p.shift2 - 1);
                 int16_t* out = a.residual + p.off;
+                ReconDst rd{p.base, p.stride, p.bps, p.maxv};
                 for (int y = r0; y < H; y += NR) {
                     const uint2* src = reinterpret_cast<const uint2*>(T + (g * H + y) * TP);
                     int acc = 0;
@@ -555,7 +618,11 @@ __device__ __forceinline__ void tile_body(const ResidualArgs& a, const TileItem&
                             acc = dot2(v.y, m[2 * q2 + 1], acc);
                         }
                     }
-                    out[y * W + x] = (int16_t)clip16i((acc + rnd) >> p.shift2);
+                    const int r = clip16i((acc + rnd) >> p.shift2);
+                    if (rd.base)
+                        recon_add(rd, x, y, r);
+                    else
+                        out[y * W + x] = (int16_t)r;
                 }
             }
         }
@@ -565,10 +632,12 @@ __device__ __forceinline__ void tile_body(const ResidualArgs& a, const TileItem&
 __global__ __launch_bounds__(256) void residual_tile_kernel(ResidualArgs a, const TileItem* __restrict__ items) {
     extern __shared__ __align__(16) uint32_t smem[];
     const TileItem it = items[blockIdx.x];
+    const rmh_tu_place* place = a.place;
+    const rmh_recon_frame* frames = a.frames;
     switch (it.lw * 8 + it.lh) {
 #define RMH_TILE(LW, LH) \
     case LW * 8 + LH:    \
-        tile_body<LW, LH>(a, it, smem); \
+        tile_body<LW, LH>(a, it, smem, place, frames); \
         break;
         RMH_TILE(2, 2) RMH_TILE(2, 3) RMH_TILE(2, 4) RMH_TILE(2, 5) RMH_TILE(2, 6)
         RMH_TILE(3, 2) RMH_TILE(3, 3) RMH_TILE(3, 4) RMH_TILE(3, 5) RMH_TILE(3, 6)

@@ -98,6 +98,9 @@ struct rmh_plan {
     const rmh::TileItem* d_tiles = nullptr;  // tiles of the fast TUs
     size_t n_tiles = 0, n_fast = 0;
     uint32_t tile_lds = 0;
+    // recon (residual added into picture planes); null for residual output
+    const rmh_tu_place* d_place = nullptr;
+    const rmh_recon_frame* d_rframes = nullptr;
     // inter
     const rmh_mc_frame* d_frames = nullptr;
     const rmh_pu_desc* d_pus = nullptr;
@@ -353,6 +356,50 @@ rmh_status validate_tus(rmh_ctx* ctx, const rmh_tu_desc* tus, size_t n_tu, size_
     return RMH_OK;
 }
 
+bool plane_ok(const rmh_plane& p);
+
+// Recon placement: every TU block lies inside its plane, has the plane's bit
+// depth, and no two blocks of one plane overlap (the in-place add would race).
+rmh_status validate_place(rmh_ctx* ctx, const rmh_tu_desc* tus, size_t n_tu, const rmh_tu_place* place,
+                          const rmh_recon_frame* frames, size_t n_frames) {
+    if (n_tu && !place) return fail(ctx, RMH_INVALID_INPUT, "null placement array");
+    if (n_frames && !frames) return fail(ctx, RMH_INVALID_INPUT, "null frame array");
+    if (n_frames > 65535) return fail(ctx, RMH_LIMIT_EXCEEDED, "more than 65535 frames");
+    for (size_t f = 0; f < n_frames; f++)
+        for (int c = 0; c < 3; c++)
+            if (!plane_ok(frames[f].planes[c])) return fail(ctx, RMH_INVALID_INPUT, "frame " + std::to_string(f) + ": invalid plane");
+    std::vector<std::vector<uint32_t>> by_plane(n_frames * 3);
+    for (size_t i = 0; i < n_tu; i++) {
+        const rmh_tu_place& pl = place[i];
+        if (pl.frame >= n_frames || pl.plane > 2 || pl.reserved)
+            return fail(ctx, RMH_INVALID_INPUT, "TU " + std::to_string(i) + ": invalid placement");
+        const rmh_plane& p = frames[pl.frame].planes[pl.plane];
+        const uint32_t w = 1u << tus[i].log2_w, h = 1u << tus[i].log2_h;
+        if ((uint32_t)pl.x + w > p.width || (uint32_t)pl.y + h > p.height)
+            return fail(ctx, RMH_INVALID_INPUT, "TU " + std::to_string(i) + ": block outside its plane");
+        if (tus[i].bit_depth != p.bit_depth)
+            return fail(ctx, RMH_INVALID_INPUT, "TU " + std::to_string(i) + ": bit depth differs from its plane");
+        by_plane[pl.frame * 3 + pl.plane].push_back((uint32_t)i);
+    }
+    std::vector<uint8_t> cover;
+    for (size_t k = 0; k < by_plane.size(); k++) {
+        if (by_plane[k].empty()) continue;
+        const rmh_plane& p = frames[k / 3].planes[k % 3];
+        cover.assign((size_t)p.width * p.height, 0);
+        for (uint32_t i : by_plane[k]) {
+            const uint32_t w = 1u << tus[i].log2_w, h = 1u << tus[i].log2_h;
+            for (uint32_t y = 0; y < h; y++) {
+                uint8_t* row = cover.data() + (size_t)(place[i].y + y) * p.width + place[i].x;
+                for (uint32_t x = 0; x < w; x++) {
+                    if (row[x]) return fail(ctx, RMH_INVALID_INPUT, "TU " + std::to_string(i) + ": block overlaps another TU");
+                    row[x] = 1;
+                }
+            }
+        }
+    }
+    return RMH_OK;
+}
+
 uint64_t residual_bytes(const rmh_tu_desc* tus, size_t n_tu) {
     uint64_t s = 0;
     for (size_t i = 0; i < n_tu; i++) s += (uint64_t)1 << (tus[i].log2_w + tus[i].log2_h);
@@ -497,6 +544,8 @@ rmh_status run_plan_locked_free(rmh_plan* plan, const int16_t* d_coeff, int16_t*
         memcpy(a.scan_len, plan->ctx->tabs.scan_len, sizeof a.scan_len);
         a.col16 = plan->ctx->tabs.col16;
         memcpy(a.col16_off, plan->ctx->tabs.col16_off, sizeof a.col16_off);
+        a.place = plan->d_place;
+        a.frames = plan->d_rframes;
         const bool whole = first == 0 && count == plan->n_tu && plan->d_order;
         const bool aligned8 = ((uintptr_t)d_coeff & 7) == 0;
         if (whole && aligned8) {
@@ -624,12 +673,16 @@ RMH_EXPORT rmh_status rmh_transform_matrix(int32_t type, int32_t n, int32_t* out
 
 // ---------------------------------------------------------------- residual
 
-RMH_EXPORT rmh_status rmh_residual_plan_create(rmh_ctx* ctx, const rmh_tu_desc* tus, size_t n_tu, size_t n_coeff,
-                                               rmh_plan** out) {
+namespace {
+
+rmh_status create_residual_plan(rmh_ctx* ctx, const rmh_tu_desc* tus, size_t n_tu, size_t n_coeff, const rmh_tu_place* place,
+                                const rmh_recon_frame* frames, size_t n_frames, rmh_plan** out) {
     if (!ctx || !out) return RMH_INVALID_INPUT;
     *out = nullptr;
     rmh_status v = validate_tus(ctx, tus, n_tu, n_coeff, false);
     if (v != RMH_OK) return v;
+    const bool recon = place != nullptr || frames != nullptr;
+    if (recon && (v = validate_place(ctx, tus, n_tu, place, frames, n_frames)) != RMH_OK) return v;
     if (ctx->disabled.load()) return fail(ctx, RMH_UNAVAILABLE, "accelerator disabled");
     DeviceGuard dg(ctx->device);
     rmh_plan* p = new rmh_plan();
@@ -638,17 +691,28 @@ RMH_EXPORT rmh_status rmh_residual_plan_create(rmh_ctx* ctx, const rmh_tu_desc* 
     p->n_tu = n_tu;
     p->n_coeff = n_coeff;
     p->bytes = residual_bytes(tus, n_tu);
+    if (recon) {
+        // the recon add reads and writes each sample instead of writing an int16 residual
+        uint64_t extra = 0;
+        for (size_t i = 0; i < n_tu; i++)
+            extra += ((uint64_t)frames[place[i].frame].planes[place[i].plane].bytes_per_sample * 2 - 2) << (tus[i].log2_w + tus[i].log2_h);
+        p->bytes += extra + 8 * (uint64_t)n_tu;
+    }
     if (n_tu) {
         std::vector<uint32_t> order;
         std::vector<rmh::TileItem> tiles;
         build_residual_tiles(tus, n_tu, order, tiles, &p->n_fast, &p->tile_lds);
         size_t tb = aligned(n_tu * sizeof(rmh_tu_desc)), ob = aligned(order.size() * 4), ib = aligned(tiles.size() * sizeof(rmh::TileItem));
-        hipError_t e = hipMalloc(&p->d_mem, tb + ob + ib + 256);
+        size_t pb = recon ? aligned(n_tu * sizeof(rmh_tu_place)) : 0, fb = recon ? aligned(n_frames * sizeof(rmh_recon_frame)) : 0;
+        hipError_t e = hipMalloc(&p->d_mem, tb + ob + ib + pb + fb + 256);
         char* base = (char*)p->d_mem;
         if (e == hipSuccess) e = hipMemcpy(base, tus, n_tu * sizeof(rmh_tu_desc), hipMemcpyHostToDevice);
         if (e == hipSuccess) e = hipMemcpy(base + tb, order.data(), order.size() * 4, hipMemcpyHostToDevice);
         if (e == hipSuccess && !tiles.empty())
             e = hipMemcpy(base + tb + ob, tiles.data(), tiles.size() * sizeof(rmh::TileItem), hipMemcpyHostToDevice);
+        if (e == hipSuccess && recon) e = hipMemcpy(base + tb + ob + ib, place, n_tu * sizeof(rmh_tu_place), hipMemcpyHostToDevice);
+        if (e == hipSuccess && recon && n_frames)
+            e = hipMemcpy(base + tb + ob + ib + pb, frames, n_frames * sizeof(rmh_recon_frame), hipMemcpyHostToDevice);
         if (e != hipSuccess) {
             if (p->d_mem) hipFree(p->d_mem);
             delete p;
@@ -658,15 +722,36 @@ RMH_EXPORT rmh_status rmh_residual_plan_create(rmh_ctx* ctx, const rmh_tu_desc* 
         p->d_order = (const uint32_t*)(base + tb);
         p->d_tiles = (const rmh::TileItem*)(base + tb + ob);
         p->n_tiles = tiles.size();
+        if (recon) {
+            p->d_place = (const rmh_tu_place*)(base + tb + ob + ib);
+            p->d_rframes = (const rmh_recon_frame*)(base + tb + ob + ib + pb);
+        }
     }
     *out = p;
     return RMH_OK;
 }
 
+}  // namespace
+
+RMH_EXPORT rmh_status rmh_residual_plan_create(rmh_ctx* ctx, const rmh_tu_desc* tus, size_t n_tu, size_t n_coeff,
+                                               rmh_plan** out) {
+    return create_residual_plan(ctx, tus, n_tu, n_coeff, nullptr, nullptr, 0, out);
+}
+
+RMH_EXPORT rmh_status rmh_residual_recon_plan_create(rmh_ctx* ctx, const rmh_tu_desc* tus, size_t n_tu, size_t n_coeff,
+                                                     const rmh_tu_place* place, const rmh_recon_frame* frames,
+                                                     size_t n_frames, rmh_plan** out) {
+    if (!place || !frames) {
+        if (out) *out = nullptr;
+        return ctx ? fail(ctx, RMH_INVALID_INPUT, "recon plan without placement or frames") : RMH_INVALID_INPUT;
+    }
+    return create_residual_plan(ctx, tus, n_tu, n_coeff, place, frames, n_frames, out);
+}
+
 RMH_EXPORT rmh_status rmh_residual_plan_run(rmh_plan* plan, const int16_t* d_coeff, int16_t* d_residual, void* stream) {
     if (!plan || plan->kind != kResidual) return RMH_INVALID_INPUT;
     if (plan->n_tu == 0) return RMH_OK;
-    if (!d_coeff || !d_residual) return fail(plan->ctx, RMH_INVALID_INPUT, "null device buffer");
+    if (!d_coeff || (!d_residual && !plan->d_place)) return fail(plan->ctx, RMH_INVALID_INPUT, "null device buffer");
     return run_plan(plan, d_coeff, d_residual, 0, plan->n_tu, stream);
 }
 
@@ -675,7 +760,7 @@ RMH_EXPORT rmh_status rmh_residual_plan_run_range(rmh_plan* plan, size_t first, 
     if (!plan || plan->kind != kResidual) return RMH_INVALID_INPUT;
     if (first > plan->n_tu || count > plan->n_tu - first) return fail(plan->ctx, RMH_INVALID_INPUT, "TU range outside the plan");
     if (count == 0) return RMH_OK;
-    if (!d_coeff || !d_residual) return fail(plan->ctx, RMH_INVALID_INPUT, "null device buffer");
+    if (!d_coeff || (!d_residual && !plan->d_place)) return fail(plan->ctx, RMH_INVALID_INPUT, "null device buffer");
     return run_plan(plan, d_coeff, d_residual, first, count, stream);
 }
 
@@ -980,7 +1065,8 @@ RMH_EXPORT rmh_status rmh_graph_create(rmh_ctx* ctx, const rmh_graph_node* nodes
     *out = nullptr;
     for (size_t i = 0; i < n_nodes; i++) {
         if (!nodes[i].plan || nodes[i].plan->ctx != ctx) return fail(ctx, RMH_INVALID_INPUT, "graph node without a plan of this context");
-        if (nodes[i].plan->kind == kResidual && nodes[i].plan->n_tu && (!nodes[i].d_coeff || !nodes[i].d_residual))
+        if (nodes[i].plan->kind == kResidual && nodes[i].plan->n_tu &&
+            (!nodes[i].d_coeff || (!nodes[i].d_residual && !nodes[i].plan->d_place)))
             return fail(ctx, RMH_INVALID_INPUT, "residual graph node without device buffers");
     }
     if (ctx->disabled.load()) return fail(ctx, RMH_UNAVAILABLE, "accelerator disabled");

@@ -18,3 +18,9 @@ def dbk_frame_struct(planes, edges, bd, qps, beta=(0, 0, 0), tc=(0, 0, 0)):
 def mc_frame_struct(refs, dsts, bd):
     f = abi.mc_frame([abi.plane_from_array(p, bd) for p in refs], [abi.plane_from_array(p, bd) for p in dsts])
     return f, (refs, dsts)
+
+
+def recon_frame_struct(planes, bd):
+    """planes: [Y, Cb, Cr] arrays that a recon run adds residuals into."""
+    f = abi.ReconFrame((abi.Plane * 3)(*[abi.plane_from_array(p, bd) for p in planes]))
+    return f, planes

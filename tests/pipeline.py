@@ -1,0 +1,114 @@
+"""P-frame reconstruction pipeline (MC -> residual add -> deblock) over many
+frames, for the parity tests and the benchmark.
+
+A batch is a list of P-frames (dicts from workloads.pframe), possibly of
+different sizes and bit depths, as the frames of several concurrent streams.
+`DeviceBatch` holds their planes in device memory and the three batched plans
+(one MC plan, one recon plan, one deblock plan over every frame); `oracle`
+runs the CPU oracle over host copies of the same frames. Both produce the
+reconstructed, deblocked [Y, Cb, Cr] of every frame.
+"""
+import ctypes
+
+import numpy as np
+
+import workloads as W
+
+
+def sample_dtype(bit_depth):
+    return np.uint8 if bit_depth == 8 else np.uint16
+
+
+def ref_planes(f):
+    dt = sample_dtype(f["bit_depth"])
+    return [np.ascontiguousarray(p.astype(dt)) for p in f["ref"]]
+
+
+def concat_tus(frames):
+    """Every frame's TUs in one descriptor list; placements point at frame i."""
+    tus, coeff = W.concat_frames([(f["tus"], f["coeff"]) for f in frames])
+    place = np.concatenate([f["place"] for f in frames]).copy()
+    place["frame"] = np.concatenate([np.full(len(f["place"]), i, np.uint16) for i, f in enumerate(frames)])
+    pus = np.concatenate([f["pus"] for f in frames]).copy()
+    pus["frame"] = np.concatenate([np.full(len(f["pus"]), i, np.uint16) for i, f in enumerate(frames)])
+    return tus, coeff, place, pus
+
+
+def oracle(po, abi, frames):
+    """The CPU oracle: orc_mc_batch into fresh planes, orc_recon_batch, then
+    orc_deblock_frames. Returns [[Y, Cb, Cr]] per frame."""
+    tus, coeff, place, pus = concat_tus(frames)
+    refs = [ref_planes(f) for f in frames]
+    outs = [[np.zeros_like(p) for p in r] for r in refs]
+    mcf = (abi.McFrame * len(frames))(*[abi.mc_frame([abi.plane_from_array(p, f["bit_depth"]) for p in r],
+                                                     [abi.plane_from_array(p, f["bit_depth"]) for p in o])
+                                        for f, r, o in zip(frames, refs, outs)])
+    assert po.lib().orc_mc_batch(ctypes.addressof(mcf), len(frames), pus.ctypes.data, len(pus)) == 0
+    rf = (abi.ReconFrame * len(frames))(*[abi.ReconFrame((abi.Plane * 3)(*[abi.plane_from_array(p, f["bit_depth"])
+                                                                           for p in o]))
+                                          for f, o in zip(frames, outs)])
+    bad = po.lib().orc_recon_batch(tus.ctypes.data, len(tus), coeff.ctypes.data_as(po.I16P), coeff.size,
+                                   place.ctypes.data, ctypes.addressof(rf), len(frames))
+    assert bad == 0, bad
+    edges = [np.ascontiguousarray(f["edges"]) for f in frames]
+    dbf = (abi.DbkFrame * len(frames))(*[abi.dbk_frame([abi.plane_from_array(p, f["bit_depth"]) for p in o],
+                                                       e.ctypes.data, e.shape[1], e.shape[0], f["qp"])
+                                         for f, o, e in zip(frames, outs, edges)])
+    assert po.lib().orc_deblock_frames(ctypes.addressof(dbf), len(frames)) == 0
+    return outs
+
+
+class DeviceBatch:
+    """Device planes and batched plans for a list of P-frames."""
+
+    def __init__(self, acc, rmh, frames):
+        self.hip, abi = rmh.hip, rmh.abi
+        hip = self.hip
+        self.frames = frames
+        self.tus, self.coeff, self.place, self.pus = concat_tus(frames)
+        self.keep = []
+        self.dst = []  # per frame: [(DeviceBuffer, shape, dtype)] x 3
+        mc_frames, rc_frames, db_frames = [], [], []
+        for f in frames:
+            bd = f["bit_depth"]
+            refs, dsts, dbufs = [], [], []
+            for p in ref_planes(f):
+                rb = hip.DeviceBuffer.from_array(p)
+                db = hip.DeviceBuffer(p.nbytes)
+                db.zero()
+                self.keep += [rb, db]
+                refs.append(abi.plane_from_address(rb.ptr, p.shape[1], p.shape[0], p.shape[1], p.itemsize, bd))
+                dsts.append(abi.plane_from_address(db.ptr, p.shape[1], p.shape[0], p.shape[1], p.itemsize, bd))
+                dbufs.append((db, p.shape, p.dtype))
+            self.dst.append(dbufs)
+            e = np.ascontiguousarray(f["edges"])
+            eb = hip.DeviceBuffer.from_array(e)
+            self.keep.append(eb)
+            mc_frames.append(abi.mc_frame(refs, dsts))
+            rc_frames.append(abi.ReconFrame((abi.Plane * 3)(*dsts)))
+            db_frames.append(abi.dbk_frame(dsts, eb.ptr, e.shape[1], e.shape[0], f["qp"]))
+        self.d_coeff = hip.DeviceBuffer.from_array(self.coeff)
+        hip.device_synchronize()
+        self.mc = acc.mc_plan(mc_frames, self.pus)
+        self.recon = acc.recon_plan(self.tus, self.coeff.size, self.place, rc_frames)
+        self.deblock = acc.deblock_plan(db_frames)
+        self.acc = acc
+
+    def run(self, stream):
+        """One frame step: the three batched plans on one stream."""
+        self.mc.run(stream=stream)
+        self.recon.run(self.d_coeff.ptr, 0, stream)
+        self.deblock.run(stream=stream)
+
+    def graph(self):
+        return self.acc.graph([(self.mc, 0, 0), (self.recon, self.d_coeff.ptr, 0), (self.deblock, 0, 0)])
+
+    def download(self):
+        return [[b.download(dt).reshape(shape) for b, shape, dt in fr] for fr in self.dst]
+
+    def luma_pixels(self):
+        return sum(int(f["width"]) * int(f["height"]) for f in self.frames)
+
+    def close(self):
+        for p in (self.mc, self.recon, self.deblock):
+            p.close()

@@ -10,6 +10,8 @@ Frame-level generators for the three stages:
                    fraction), and a deblocking edge map on the 8x8 grid with a
                    random bS per 4-sample segment and filter lengths from the CU
                    sizes (luma_lens / chroma_lens, vvc/decode_general.rs:4916-4935)
+  pframe           inter_frame plus the residual TUs of the non-skipped CUs
+                   (a transform tree per CU) and where each TU lands
 
 The numpy dtypes mirror the C structs of include/rmh.h byte for byte.
 """
@@ -108,6 +110,12 @@ def residual_frame(seed, width=1920, height=1080, bit_depth=10, qp_range=(22, 37
                                                  ((width + 1) // 2, (height + 1) // 2, 32, 32)]):
         for (x, y, w, h) in _tile(rng, W, H, ctu, maxtu, 4, _p_split_residual):
             blocks.append((plane, x, y, w, h))
+    tus, coeff = _tu_batch(rng, blocks, bit_depth, qp_range, with_lfnst, with_ts, with_dq, zero_frac)
+    return tus, coeff, blocks
+
+
+def _tu_batch(rng, blocks, bit_depth, qp_range, with_lfnst=False, with_ts=False, with_dq=False, zero_frac=0.85):
+    """Descriptors and coefficients for blocks [(plane, x, y, w, h)], in order."""
     n = len(blocks)
     tus = np.zeros(n, TU_DTYPE)
     ws = np.array([b[3] for b in blocks])
@@ -155,7 +163,7 @@ def residual_frame(seed, width=1920, height=1080, bit_depth=10, qp_range=(22, 37
     coeff = np.where(big & keep, rng.integers(-32768, 32768, total), coeff)
     coeff = np.clip(coeff, -32768, 32767).astype(np.int16)
     del tu_of
-    return tus, coeff, blocks
+    return tus, coeff
 
 
 def concat_frames(frames):
@@ -239,26 +247,105 @@ def inter_frame(seed, width=1920, height=1080, bit_depth=10, mv_range=64, qp=32,
     bs_v = rng.integers(0, 3, (gh, gw))
     bs_h = rng.integers(0, 3, (gh, gw))
     cgw, cgh = (cw + 1) // 2, (ch + 1) // 2
-    for j in range(gh):
-        for i in range(gw):
-            x, y = 4 * i, 4 * j
-            if x % 8 == 0 and i > 0 and cu_id[j, i] != cu_id[j, i - 1]:
-                lp, lq = luma_lens(cu_w[j, i - 1], cu_w[j, i])
-                edges[j, i, 0] = edge_byte(int(bs_v[j, i]), lp, lq)
-            if y % 8 == 0 and j > 0 and cu_id[j, i] != cu_id[j - 1, i]:
-                lp, lq = luma_lens(cu_h[j - 1, i], cu_h[j, i])
-                edges[j, i, 1] = edge_byte(int(bs_h[j, i]), lp, lq, ctb=(y % ctu == 0))
-            # chroma cell (i, j) covers chroma x 2i..2i+1, y 2j..2j+1 (luma 4i.., 4j..)
-            cx, cy = 2 * i, 2 * j
-            if i < cgw and j < cgh:
-                lx, ly = 2 * cx, 2 * cy
-                li, lj = lx // 4, ly // 4
-                if cx % 8 == 0 and cx > 0 and li < gw and lj < gh and cu_id[lj, li] != cu_id[lj, li - 1]:
-                    lp, lq = chroma_lens(cu_w[lj, li - 1] // 2, cu_w[lj, li] // 2, False)
-                    edges[j, i, 2] = edge_byte(int(bs_v[lj, li]), lp, lq)
-                if cy % 8 == 0 and cy > 0 and li < gw and lj < gh and cu_id[lj, li] != cu_id[lj - 1, li]:
-                    lp, lq = chroma_lens(cu_h[lj - 1, li] // 2, cu_h[lj, li] // 2, cy % (ctu // 2) == 0)
-                    edges[j, i, 3] = edge_byte(int(bs_h[lj, li]), lp, lq)
+    ii = np.arange(gw)[None, :]
+    jj = np.arange(gh)[:, None]
+    # CU boundaries crossing the left / top side of each 4x4 cell
+    diff_v = np.zeros((gh, gw), bool)
+    diff_v[:, 1:] = cu_id[:, 1:] != cu_id[:, :-1]
+    diff_h = np.zeros((gh, gw), bool)
+    diff_h[1:, :] = cu_id[1:, :] != cu_id[:-1, :]
+    w_p = np.zeros_like(cu_w)
+    w_p[:, 1:] = cu_w[:, :-1]
+    h_p = np.zeros_like(cu_h)
+    h_p[1:, :] = cu_h[:-1, :]
+
+    def lens_luma(sp, sq):  # luma_lens, elementwise
+        small = (sp <= 4) | (sq <= 4)
+        return np.where(small, 1, np.where(sp >= 32, 7, 3)), np.where(small, 1, np.where(sq >= 32, 7, 3))
+
+    def byte(bs, lp, lq, ctb):  # edge_byte, elementwise
+        return ((bs & 3) | (((lp - 1) // 2) << 2) | (((lq - 1) // 2) << 4) | np.where(ctb, 0x40, 0)).astype(np.uint8)
+
+    lp, lq = lens_luma(w_p, cu_w)
+    edges[..., 0] = np.where(((4 * ii) % 8 == 0) & diff_v, byte(bs_v, lp, lq, False), 0)
+    lp, lq = lens_luma(h_p, cu_h)
+    edges[..., 1] = np.where(((4 * jj) % 8 == 0) & diff_h, byte(bs_h, lp, lq, (4 * jj) % ctu == 0), 0)
+    # chroma cell (i, j) covers chroma x 2i..2i+1, y 2j..2j+1 (luma 4i.., 4j..): chroma_lens
+    in_c = (ii < cgw) & (jj < cgh)
+    big = (w_p // 2 >= 8) & (cu_w // 2 >= 8)
+    lc = np.where(big, 3, 1)
+    edges[..., 2] = np.where(in_c & ((2 * ii) % 8 == 0) & diff_v, byte(bs_v, lc, lc, False), 0)
+    big = (h_p // 2 >= 8) & (cu_h // 2 >= 8)
+    at_ctb = (2 * jj) % (ctu // 2) == 0
+    lp = np.where(big, np.where(at_ctb, 1, 3), 1)
+    lq = np.where(big, 3, 1)
+    edges[..., 3] = np.where(in_c & ((2 * jj) % 8 == 0) & diff_h, byte(bs_h, lp, lq, False), 0)
     chroma_qp = qp if qp <= 26 else qp - 1  # default chroma QP mapping, vvc/hls.rs:778-781
     return dict(ref=ref, pus=pus, cus=cus, edges=edges, qp=[qp, chroma_qp, chroma_qp], ctu=ctu,
                 bit_depth=bit_depth, width=width, height=height)
+
+
+# --------------------------------------------------------------- P-frames
+
+PLACE_DTYPE = np.dtype([("frame", "<u2"), ("plane", "u1"), ("reserved", "u1"), ("x", "<u2"), ("y", "<u2")])
+assert PLACE_DTYPE.itemsize == 8
+
+
+def _split_block(rng, x, y, w, h, max_tu, min_tu, p_split, out):
+    """Transform tree of one block: forced splits down to max_tu, then seeded
+    quad / binary splits while both halves stay >= min_tu."""
+    if w > max_tu or h > max_tu:
+        if w > max_tu and h > max_tu:
+            for (dx, dy) in ((0, 0), (w // 2, 0), (0, h // 2), (w // 2, h // 2)):
+                _split_block(rng, x + dx, y + dy, w // 2, h // 2, max_tu, min_tu, p_split, out)
+        elif w > max_tu:
+            _split_block(rng, x, y, w // 2, h, max_tu, min_tu, p_split, out)
+            _split_block(rng, x + w // 2, y, w // 2, h, max_tu, min_tu, p_split, out)
+        else:
+            _split_block(rng, x, y, w, h // 2, max_tu, min_tu, p_split, out)
+            _split_block(rng, x, y + h // 2, w, h // 2, max_tu, min_tu, p_split, out)
+        return
+    can_w, can_h = w // 2 >= min_tu, h // 2 >= min_tu
+    if (can_w or can_h) and rng.random() < p_split:
+        kind = int(rng.integers(0, 3))
+        if kind == 0 and can_w and can_h:
+            for (dx, dy) in ((0, 0), (w // 2, 0), (0, h // 2), (w // 2, h // 2)):
+                _split_block(rng, x + dx, y + dy, w // 2, h // 2, max_tu, min_tu, p_split * 0.5, out)
+            return
+        if (kind == 1 and can_h) or not can_w:
+            _split_block(rng, x, y, w, h // 2, max_tu, min_tu, p_split * 0.5, out)
+            _split_block(rng, x, y + h // 2, w, h // 2, max_tu, min_tu, p_split * 0.5, out)
+            return
+        _split_block(rng, x, y, w // 2, h, max_tu, min_tu, p_split * 0.5, out)
+        _split_block(rng, x + w // 2, y, w // 2, h, max_tu, min_tu, p_split * 0.5, out)
+        return
+    out.append((x, y, w, h))
+
+
+def pframe(seed, width=1920, height=1080, bit_depth=10, mv_range=64, qp=32, ctu=128, p_skip=0.2, frame=0):
+    """A P-frame for the MC -> residual add -> deblock pipeline: inter_frame's
+    reference, PUs and edge map, plus the residual TUs of every CU that is not
+    skipped (luma transform tree inside the CU, max 64; chroma likewise at half
+    size, max 32), with their placement. Returns inter_frame's dict plus
+    tus, coeff, place (PLACE_DTYPE, frame index `frame`)."""
+    d = inter_frame(seed, width, height, bit_depth, mv_range, qp, ctu)
+    rng = np.random.default_rng([4, seed])
+    blocks = []
+    for (x, y, w, h) in d["cus"]:
+        if rng.random() < p_skip:
+            continue
+        lum = []
+        _split_block(rng, x, y, w, h, 64, 4, 0.35, lum)
+        blocks += [(0, bx, by, bw, bh) for (bx, by, bw, bh) in lum]
+        for plane in (1, 2):
+            ch = []
+            _split_block(rng, x // 2, y // 2, w // 2, h // 2, 32, 4, 0.2, ch)
+            blocks += [(plane, bx, by, bw, bh) for (bx, by, bw, bh) in ch]
+    tus, coeff = _tu_batch(rng, blocks, bit_depth, (qp - 5, qp + 5))
+    place = np.zeros(len(blocks), PLACE_DTYPE)
+    place["frame"] = frame
+    place["plane"] = [b[0] for b in blocks]
+    place["x"] = [b[1] for b in blocks]
+    place["y"] = [b[2] for b in blocks]
+    d.update(tus=tus, coeff=coeff, place=place, blocks=blocks)
+    return d
