@@ -456,7 +456,7 @@ __device__ __forceinline__ void load_col16(const uint32_t* __restrict__ src, uin
 // `place` / `frames` come in as values read once at kernel entry: reading them
 // through `a` inside the per-TU loop makes the compiler copy the whole
 // argument block to scratch memory.
-template <int LW, int LH>
+template <int LW, int LH, bool RECON>
 __device__ __forceinline__ void tile_body(const ResidualArgs& a, const TileItem& it, uint32_t* smem,
                                           const rmh_tu_place* __restrict__ place,
                                           const rmh_recon_frame* __restrict__ frames) {
@@ -495,7 +495,7 @@ __device__ __forceinline__ void tile_body(const ResidualArgs& a, const TileItem&
         p.base = nullptr;
         p.bps = 0;
         p.maxv = 0;
-        if (place) {
+        if (RECON) {
             const ReconDst rd = recon_dst(place, frames, a.order[it.first + g]);
             p.stride = rd.stride;
             p.base = rd.base;
@@ -605,8 +605,8 @@ __device__ __forceinline__ void tile_body(const ResidualArgs& a, const TileItem&
                 uint32_t m[16];
                 load_col16(a.col16 + p.mh + x * 16, m);
                 const int rnd = 1 << (p.shift2 - 1);
-                int16_t* out = a.residual + p.off;
-                ReconDst rd{p.base, p.stride, p.bps, p.maxv};
+                int16_t* out = RECON ? nullptr : a.residual + p.off;
+                const ReconDst rd{p.base, p.stride, p.bps, p.maxv};
                 for (int y = r0; y < H; y += NR) {
                     const uint2* src = reinterpret_cast<const uint2*>(T + (g * H + y) * TP);
                     int acc = 0;
@@ -619,7 +619,7 @@ __device__ __forceinline__ void tile_body(const ResidualArgs& a, const TileItem&
                         }
                     }
                     const int r = clip16i((acc + rnd) >> p.shift2);
-                    if (rd.base)
+                    if (RECON)
                         recon_add(rd, x, y, r);
                     else
                         out[y * W + x] = (int16_t)r;
@@ -629,6 +629,9 @@ __device__ __forceinline__ void tile_body(const ResidualArgs& a, const TileItem&
     }
 }
 
+// RECON: add into the picture planes (a.place / a.frames) instead of writing
+// int16 residuals.
+template <bool RECON>
 __global__ __launch_bounds__(256) void residual_tile_kernel(ResidualArgs a, const TileItem* __restrict__ items) {
     extern __shared__ __align__(16) uint32_t smem[];
     const TileItem it = items[blockIdx.x];
@@ -637,7 +640,7 @@ __global__ __launch_bounds__(256) void residual_tile_kernel(ResidualArgs a, cons
     switch (it.lw * 8 + it.lh) {
 #define RMH_TILE(LW, LH) \
     case LW * 8 + LH:    \
-        tile_body<LW, LH>(a, it, smem, place, frames); \
+        tile_body<LW, LH, RECON>(a, it, smem, place, frames); \
         break;
         RMH_TILE(2, 2) RMH_TILE(2, 3) RMH_TILE(2, 4) RMH_TILE(2, 5) RMH_TILE(2, 6)
         RMH_TILE(3, 2) RMH_TILE(3, 3) RMH_TILE(3, 4) RMH_TILE(3, 5) RMH_TILE(3, 6)
@@ -661,7 +664,10 @@ hipError_t launch_residual(const ResidualArgs& a, hipStream_t stream) {
 hipError_t launch_residual_tiles(const ResidualArgs& a, const TileItem* items, uint32_t n_items, uint32_t lds_bytes,
                                  hipStream_t stream) {
     if (n_items == 0) return hipSuccess;
-    hipLaunchKernelGGL(residual_tile_kernel, dim3(n_items), dim3(256), lds_bytes, stream, a, items);
+    if (a.place)
+        hipLaunchKernelGGL(residual_tile_kernel<true>, dim3(n_items), dim3(256), lds_bytes, stream, a, items);
+    else
+        hipLaunchKernelGGL(residual_tile_kernel<false>, dim3(n_items), dim3(256), lds_bytes, stream, a, items);
     return hipGetLastError();
 }
 
