@@ -298,6 +298,50 @@ rmh_status rmh_intra_subset_decode(rmh_ctx* ctx, uint32_t width, uint32_t height
  * vvc/dct.rs:251-256). */
 rmh_status rmh_transform_matrix(int32_t type, int32_t n, int32_t* out);
 
+/* ------------------------------------------------------------------------ */
+/* Coding-tree split search (encoder side, host code)                        */
+/* ------------------------------------------------------------------------ */
+
+/* Split modes in the order of CodingTreeSplit (vvc/transform.rs:346-359). */
+enum {
+    RMH_SPLIT_NONE = 0,
+    RMH_SPLIT_QT = 1,
+    RMH_SPLIT_BT_HOR = 2,
+    RMH_SPLIT_BT_VER = 3,
+    RMH_SPLIT_TT_HOR = 4,
+    RMH_SPLIT_TT_VER = 5
+};
+
+enum {
+    RMH_RD_FULL = 0,      /* every split of every 32x32 node */
+    RMH_RD_PRUNED = 1,    /* splits with probs >= p_prune; all six when max prob < p_fallback */
+    RMH_RD_HEURISTIC = 2  /* the one split of choose_qt_mtt_partition_rdo (vvc/slice.rs:4127-4167) */
+};
+
+typedef struct rmh_rd_stats {
+    double bits;          /* modelled bits of the chosen splits */
+    uint64_t sse;         /* luma SSE of the reconstruction */
+    uint32_t blocks;      /* 32x32 nodes */
+    uint32_t candidates;  /* splits coded */
+    uint32_t fallbacks;   /* nodes searched fully because max prob < p_fallback */
+    uint32_t reserved;
+} rmh_rd_stats;
+
+/* RD search over the six splits of each 32x32 node of an 8-bit luma picture
+ * (width, height multiples of 32), nodes in raster order; each child is a
+ * leaf CU (PLANAR/DC by SAD, DCT-II, intra quantisation at `qp`).
+ * J = SSD + lambda * bits with lambda = 0.57 * 2^((qp - 12) / 3). The
+ * reference has no full RD search; its only chooser is the heuristic above,
+ * so this entry point has no reference counterpart. It replaces the split
+ * choice of encode_picture's coding-tree loop (vvc/slice.rs:4127-4167).
+ * `probs` (6 per node, RMH_RD_PRUNED only) come from rmh_cnn_forward.
+ * Outputs: recon (width*height), splits (one per node), optional costs
+ * (6 per node: J, or -1 for splits not coded). Runs on the calling thread. */
+rmh_status rmh_rd_encode_luma(const uint8_t* src, uint32_t width, uint32_t height, uint32_t stride,
+                              uint32_t qp, uint32_t mode, const float* probs, float p_prune,
+                              float p_fallback, uint8_t* recon, uint8_t* splits, float* costs,
+                              rmh_rd_stats* stats);
+
 #ifdef __cplusplus
 }
 #endif

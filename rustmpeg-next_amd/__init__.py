@@ -69,7 +69,7 @@ EXPORTED = [
     "rmh_residual_plan_run_range", "rmh_mc_batch", "rmh_mc_plan_create", "rmh_mc_plan_run", "rmh_deblock",
     "rmh_deblock_plan_create", "rmh_deblock_plan_run", "rmh_plan_destroy", "rmh_plan_bytes", "rmh_graph_create",
     "rmh_graph_launch", "rmh_graph_destroy", "rmh_transform_matrix", "rmh_intra_subset_decode",
-    "rmh_residual_recon_plan_create",
+    "rmh_residual_recon_plan_create", "rmh_rd_encode_luma",
 ]
 
 
@@ -107,6 +107,9 @@ def load_library(path=LIB_PATH):
         "rmh_graph_launch": ([vp, vp], st),
         "rmh_graph_destroy": ([vp], None),
         "rmh_transform_matrix": ([ctypes.c_int32, ctypes.c_int32, vp], st),
+        "rmh_rd_encode_luma": ([vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                ctypes.c_uint32, vp, ctypes.c_float, ctypes.c_float, vp, vp, vp,
+                                ctypes.POINTER(abi.RdStats)], st),
         "rmh_intra_subset_decode": ([vp, ctypes.c_uint32, ctypes.c_uint32, vp, vp, sz, vp, sz, vp, vp, vp], st),
     }
     for name, (args, res) in sig.items():
@@ -115,6 +118,32 @@ def load_library(path=LIB_PATH):
         fn.restype = res
     _lib = L
     return L
+
+
+def rd_encode_luma(src, qp, mode=abi.RD_FULL, probs=None, p_prune=0.05, p_fallback=0.6, want_costs=False):
+    """RD search over the six splits of every 32x32 node of an 8-bit luma
+    picture (rmh_rd_encode_luma; host code, no device needed). Returns
+    (recon, splits[nodes_y, nodes_x], costs or None, stats dict)."""
+    src = np.ascontiguousarray(src, dtype=np.uint8)
+    h, w = src.shape
+    nb = (h // 32) * (w // 32)
+    recon = np.zeros_like(src)
+    splits = np.zeros(nb, np.uint8)
+    costs = np.zeros(nb * 6, np.float32) if want_costs else None
+    pp = None
+    if probs is not None:
+        pp = np.ascontiguousarray(probs, dtype=np.float32).reshape(-1)
+        if pp.size != nb * 6:
+            raise InvalidInput("probs must hold 6 values per 32x32 node")
+    st = abi.RdStats()
+    s = load_library().rmh_rd_encode_luma(src.ctypes.data, w, h, w, qp, mode, pp.ctypes.data if pp is not None else None,
+                                          p_prune, p_fallback, recon.ctypes.data, splits.ctypes.data,
+                                          costs.ctypes.data if costs is not None else None, ctypes.byref(st))
+    if s != abi.RMH_OK:
+        raise _ERRORS.get(s, Execution)("rmh_rd_encode_luma rejected the input")
+    stats = {"bits": st.bits, "sse": int(st.sse), "blocks": st.blocks, "candidates": st.candidates,
+             "fallbacks": st.fallbacks}
+    return recon, splits.reshape(h // 32, w // 32), (costs.reshape(nb, 6) if costs is not None else None), stats
 
 
 def transform_matrix(tr_type, n):

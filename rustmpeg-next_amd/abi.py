@@ -48,6 +48,15 @@ class ReconFrame(ctypes.Structure):
 PLACE_DTYPE = np.dtype([("frame", "<u2"), ("plane", "u1"), ("reserved", "u1"), ("x", "<u2"), ("y", "<u2")])
 
 
+class RdStats(ctypes.Structure):
+    _fields_ = [("bits", ctypes.c_double), ("sse", ctypes.c_uint64), ("blocks", ctypes.c_uint32),
+                ("candidates", ctypes.c_uint32), ("fallbacks", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+
+
+RD_FULL, RD_PRUNED, RD_HEURISTIC = 0, 1, 2
+SPLIT_NAMES = ("none", "quad", "binary_hor", "binary_ver", "ternary_hor", "ternary_ver")
+
+
 class DbkFrame(ctypes.Structure):
     _fields_ = [("planes", Plane * 3), ("edges", ctypes.c_void_p), ("grid_w", ctypes.c_uint32),
                 ("grid_h", ctypes.c_uint32), ("qp", ctypes.c_uint8 * 3), ("beta_offset_div2", ctypes.c_int8 * 3),

@@ -1,0 +1,392 @@
+// rdsearch.cpp - rate-distortion search over the coding-tree split of each
+// 32x32 luma node (encoder side, host code).
+//
+// The reference has no full RD search over splits: its only chooser is the
+// quadrant-mean heuristic choose_qt_mtt_partition_rdo
+// (crates/rustmpeg-video/src/vvc/slice.rs:4127-4167). This file writes the RD
+// search from the reference's building blocks:
+//   - the six splits and their children: CodingTreeSplit and
+//     split_coding_tree_node (vvc/transform.rs:346-409), in that order
+//     (0 none, 1 quad, 2 binary hor, 3 binary ver, 4 ternary hor, 5 ternary ver);
+//   - each child is a leaf CU coded with PLANAR or DC chosen by SAD, PLANAR on
+//     ties (choose_intra_mode, vvc/predict.rs:2706), DCT-II forward transform
+//     (forward_transform_rect_bd, vvc/dct.rs:865-991), intra quantisation
+//     (quantize_rect_bd, vvc/dct.rs:1113-1147), dequantisation and inverse
+//     transform (vvc/dct.rs:1151-1187, 996-1109), clip to 8 bits;
+//   - J = SSD + lambda * R, lambda = 0.57 * 2^((QP - 12) / 3) (the VTM intra
+//     form) and R from a static bit model of the levels (below). Both are this
+//     search's own choices: the reference has neither.
+//
+// Prediction is PLANAR / DC over rectangular blocks from the reconstructed
+// neighbours (top row w + 1, left column h + 1 samples) with nearest-available
+// substitution; there is no reference smoothing or PDPC in this search.
+//
+// Modes: full (all six splits), pruned (the splits whose predicted probability
+// is >= p_prune, or all six when the largest is < p_fallback) and heuristic
+// (the reference heuristic's one split).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "../../include/rmh.h"
+#include "tables.h"
+
+namespace {
+
+constexpr int kNode = 32;
+
+int lg2(int v) {
+    int l = 0;
+    while ((1 << l) < v) l++;
+    return l;
+}
+
+struct Dct2 {
+    int m[6][32 * 32];  // log2 n = 2..5: m[k * n + j]
+    Dct2() {
+        for (int l = 2; l <= 5; l++) {
+            const int n = 1 << l;
+            bool ok = true;
+            for (int k = 0; k < n; k++)
+                for (int j = 0; j < n; j++) m[l][k * n + j] = rmh::transform_entry(RMH_TR_DCT2, n, k, j, &ok);
+        }
+    }
+};
+
+const Dct2& dct2() {
+    static const Dct2 t;
+    return t;
+}
+
+const int kQuantScale[2][6] = {{26214, 23302, 20560, 18396, 16384, 14564}, {18396, 16384, 14564, 13107, 11651, 10280}};
+const int kInvScale[2][6] = {{40, 45, 51, 57, 64, 72}, {57, 64, 72, 80, 90, 102}};
+
+int clip16(int64_t v) { return (int)(v < -32768 ? -32768 : v > 32767 ? 32767 : v); }
+
+// forward_transform_rect_bd at bd = 8, DCT-II both ways, no zero-out (w, h <= 32).
+void forward(int w, int h, const int* r, int* out) {
+    const int lw = lg2(w), lh = lg2(h);
+    const int* mh = dct2().m[lw];
+    const int* mv = dct2().m[lh];
+    const int s1 = lw + 8 + 6 - 15, s2 = lh + 6;
+    const int64_t a1 = s1 > 0 ? (int64_t)1 << (s1 - 1) : 0, a2 = (int64_t)1 << (s2 - 1);
+    int tmp[32 * 32];
+    for (int y = 0; y < h; y++)
+        for (int k = 0; k < w; k++) {
+            int64_t s = 0;
+            for (int n = 0; n < w; n++) s += (int64_t)mh[k * w + n] * r[y * w + n];
+            tmp[k * h + y] = (int)(s1 >= 0 ? (s + a1) >> s1 : s << -s1);
+        }
+    for (int v = 0; v < h; v++)
+        for (int k = 0; k < w; k++) {
+            int64_t s = 0;
+            for (int y = 0; y < h; y++) s += (int64_t)mv[v * h + y] * tmp[k * h + y];
+            out[v * w + k] = clip16((s + a2) >> s2);
+        }
+}
+
+// quantize_rect_bd, intra rounding offset, bd = 8.
+void quantise(int w, int h, const int* c, int qp, int* lv) {
+    const int lsum = lg2(w) + lg2(h), odd = lsum & 1;
+    const int tr_shift = 15 - 8 - (lsum >> 1) - odd;
+    const int qbits = 14 + qp / 6 + tr_shift;
+    const int64_t add = (int64_t)171 << (qbits - 9 > 0 ? qbits - 9 : 0);
+    const int64_t sc = kQuantScale[odd][qp % 6];
+    for (int i = 0; i < w * h; i++) {
+        const int64_t a = c[i] < 0 ? -(int64_t)c[i] : c[i];
+        const int64_t l = (a * sc + add) >> qbits;
+        lv[i] = clip16(c[i] < 0 ? -l : l);
+    }
+}
+
+// dequantize_rect_bd then inverse_transform_rect_bd at bd = 8 (DCT-II).
+void reconstruct_residual(int w, int h, const int* lv, int qp, int* res) {
+    const int lw = lg2(w), lh = lg2(h), lsum = lw + lh, odd = lsum & 1;
+    const int tr_shift = 15 - 8 - (lsum >> 1) - odd;
+    const int rs = 6 - (tr_shift + qp / 6);
+    const int64_t sc = kInvScale[odd][qp % 6];
+    int c[32 * 32];
+    for (int i = 0; i < w * h; i++) {
+        const int64_t v = (int64_t)lv[i] * sc;
+        c[i] = clip16(rs > 0 ? (v + ((int64_t)1 << (rs - 1))) >> rs : v << -rs);
+    }
+    const int* mh = dct2().m[lw];
+    const int* mv = dct2().m[lh];
+    int tmp[32 * 32];
+    for (int i = 0; i < w; i++)
+        for (int j = 0; j < h; j++) {
+            int64_t s = 0;
+            for (int k = 0; k < h; k++) s += (int64_t)mv[k * h + j] * c[k * w + i];
+            tmp[i * h + j] = clip16((s + 64) >> 7);
+        }
+    for (int y = 0; y < h; y++)
+        for (int x = 0; x < w; x++) {
+            int64_t s = 0;
+            for (int k = 0; k < w; k++) s += (int64_t)mh[k * w + x] * tmp[k * h + y];
+            res[y * w + x] = clip16((s + 2048) >> 12);
+        }
+}
+
+// Static bit model of a leaf: cbf, the last position, and per coded position
+// 0.5 bit for a zero or 2 bits (significance, sign) plus an Exp-Golomb-like
+// 2*floor(log2 |l|) + 1 for |l| > 1, in diagonal order (x + y, then y).
+double leaf_bits(int w, int h, const int* lv) {
+    double bits = 1.0;
+    int last = -1;
+    int order[32 * 32];
+    int n = 0;
+    for (int d = 0; d < w + h - 1; d++)
+        for (int y = std::min(d, h - 1); y >= 0 && d - y < w; y--) order[n++] = y * w + (d - y);
+    for (int i = n - 1; i >= 0; i--)
+        if (lv[order[i]]) {
+            last = i;
+            break;
+        }
+    if (last < 0) return bits;
+    bits += lg2(w) + lg2(h);
+    for (int i = 0; i <= last; i++) {
+        const int a = std::abs(lv[order[i]]);
+        if (!a)
+            bits += 0.5;
+        else
+            bits += 2.0 + (a > 1 ? 2.0 * std::floor(std::log2((double)a)) + 1.0 : 0.0);
+    }
+    return bits;
+}
+
+const double kSplitBits[6] = {1.0, 2.0, 4.0, 4.0, 4.0, 4.0};
+
+// Children of a split (split_coding_tree_node, vvc/transform.rs:363-409), relative to the node.
+int children(int split, int (*c)[4]) {
+    const int n = kNode, hh = n / 2, qq = n / 4;
+    switch (split) {
+        case 0: c[0][0] = 0; c[0][1] = 0; c[0][2] = n; c[0][3] = n; return 1;
+        case 1:
+            for (int i = 0; i < 4; i++) {
+                c[i][0] = (i & 1) * hh;
+                c[i][1] = (i >> 1) * hh;
+                c[i][2] = hh;
+                c[i][3] = hh;
+            }
+            return 4;
+        case 2: c[0][0] = 0; c[0][1] = 0; c[0][2] = n; c[0][3] = hh; c[1][0] = 0; c[1][1] = hh; c[1][2] = n; c[1][3] = hh; return 2;
+        case 3: c[0][0] = 0; c[0][1] = 0; c[0][2] = hh; c[0][3] = n; c[1][0] = hh; c[1][1] = 0; c[1][2] = hh; c[1][3] = n; return 2;
+        case 4:
+            c[0][0] = 0; c[0][1] = 0; c[0][2] = n; c[0][3] = qq;
+            c[1][0] = 0; c[1][1] = qq; c[1][2] = n; c[1][3] = 2 * qq;
+            c[2][0] = 0; c[2][1] = 3 * qq; c[2][2] = n; c[2][3] = qq;
+            return 3;
+        default:
+            c[0][0] = 0; c[0][1] = 0; c[0][2] = qq; c[0][3] = n;
+            c[1][0] = qq; c[1][1] = 0; c[1][2] = 2 * qq; c[1][3] = n;
+            c[2][0] = 3 * qq; c[2][1] = 0; c[2][2] = qq; c[2][3] = n;
+            return 3;
+    }
+}
+
+struct Frame {
+    const uint8_t* src;
+    int W, H, stride;
+    uint8_t* rec;  // W x H, row pitch W
+};
+
+// One node's candidate: leaves coded in order into a local 32x32 recon.
+struct Node {
+    uint8_t rec[kNode * kNode];
+    uint8_t done[kNode * kNode];
+};
+
+// Reference sample at picture position (x, y) for a leaf of the node at
+// (bx, by): inside the node from the candidate's own recon when that sample
+// is coded; outside from the picture recon when it precedes the node in
+// raster order of 32x32 nodes. Returns -1 when unavailable.
+int ref_sample(const Frame& f, const Node& nd, int bx, int by, int x, int y) {
+    if (x < 0 || y < 0 || x >= f.W || y >= f.H) return -1;
+    if (x >= bx && x < bx + kNode && y >= by && y < by + kNode) {
+        const int i = (y - by) * kNode + (x - bx);
+        return nd.done[i] ? nd.rec[i] : -1;
+    }
+    if (y < by) return f.rec[(size_t)y * f.W + x];
+    if (y < by + kNode && x < bx) return f.rec[(size_t)y * f.W + x];
+    return -1;
+}
+
+// PLANAR (0) / DC (1) prediction of the w x h leaf at picture (x0, y0).
+void predict(const Frame& f, const Node& nd, int bx, int by, int x0, int y0, int w, int h, int mode, int* pred) {
+    // reference line in substitution order: left bottom-up (h + 1 .. 1), corner, top left-to-right
+    int line[2 * 33 + 1];
+    const int nl = h + 1, nt = w + 1, tot = nl + 1 + nt;
+    for (int i = 0; i < nl; i++) line[i] = ref_sample(f, nd, bx, by, x0 - 1, y0 + nl - 1 - i);
+    line[nl] = ref_sample(f, nd, bx, by, x0 - 1, y0 - 1);
+    for (int i = 0; i < nt; i++) line[nl + 1 + i] = ref_sample(f, nd, bx, by, x0 + i, y0 - 1);
+    int first = -1;
+    for (int i = 0; i < tot; i++)
+        if (line[i] >= 0) {
+            first = line[i];
+            break;
+        }
+    if (first < 0) {
+        for (int i = 0; i < tot; i++) line[i] = 128;
+    } else {
+        int prev = first;
+        for (int i = 0; i < tot; i++) {
+            if (line[i] < 0)
+                line[i] = prev;
+            else
+                prev = line[i];
+        }
+    }
+    int left[33], top[33];
+    for (int j = 0; j <= h; j++) left[j] = line[nl - 1 - j];
+    for (int i = 0; i <= w; i++) top[i] = line[nl + 1 + i];
+    const int lw = lg2(w), lh = lg2(h);
+    if (mode == 0) {
+        for (int y = 0; y < h; y++)
+            for (int x = 0; x < w; x++) {
+                const int pv = ((h - 1 - y) * top[x] + (y + 1) * left[h]) << lw;
+                const int ph = ((w - 1 - x) * left[y] + (x + 1) * top[w]) << lh;
+                pred[y * w + x] = (pv + ph + w * h) >> (lw + lh + 1);
+            }
+    } else {
+        int dc;
+        if (w == h) {
+            int s = 0;
+            for (int i = 0; i < w; i++) s += top[i] + left[i];
+            dc = (s + w) >> (lw + 1);
+        } else if (w > h) {
+            int s = 0;
+            for (int i = 0; i < w; i++) s += top[i];
+            dc = (s + (w >> 1)) >> lw;
+        } else {
+            int s = 0;
+            for (int i = 0; i < h; i++) s += left[i];
+            dc = (s + (h >> 1)) >> lh;
+        }
+        for (int i = 0; i < w * h; i++) pred[i] = dc;
+    }
+}
+
+// Codes one candidate split of the node at (bx, by); returns J and fills nd.
+double code_candidate(const Frame& f, int bx, int by, int split, int qp, double lambda, Node& nd, double* bits_out,
+                      uint64_t* sse_out) {
+    std::memset(nd.done, 0, sizeof nd.done);
+    int ch[4][4];
+    const int n = children(split, ch);
+    double bits = kSplitBits[split];
+    uint64_t sse = 0;
+    int pred[2][32 * 32], res[32 * 32], coef[32 * 32], lv[32 * 32], src[32 * 32];
+    for (int c = 0; c < n; c++) {
+        const int lx = ch[c][0], ly = ch[c][1], w = ch[c][2], h = ch[c][3];
+        const int x0 = bx + lx, y0 = by + ly;
+        for (int y = 0; y < h; y++)
+            for (int x = 0; x < w; x++) src[y * w + x] = f.src[(size_t)(y0 + y) * f.stride + x0 + x];
+        int sad[2];
+        for (int m = 0; m < 2; m++) {
+            predict(f, nd, bx, by, x0, y0, w, h, m, pred[m]);
+            int s = 0;
+            for (int i = 0; i < w * h; i++) s += std::abs(src[i] - pred[m][i]);
+            sad[m] = s;
+        }
+        const int m = sad[1] < sad[0] ? 1 : 0;
+        for (int i = 0; i < w * h; i++) res[i] = src[i] - pred[m][i];
+        forward(w, h, res, coef);
+        quantise(w, h, coef, qp, lv);
+        bits += 1.0 + leaf_bits(w, h, lv);
+        reconstruct_residual(w, h, lv, qp, res);
+        for (int y = 0; y < h; y++)
+            for (int x = 0; x < w; x++) {
+                int v = pred[m][y * w + x] + res[y * w + x];
+                v = v < 0 ? 0 : v > 255 ? 255 : v;
+                const int d = v - src[y * w + x];
+                sse += (uint64_t)(d * d);
+                nd.rec[(ly + y) * kNode + lx + x] = (uint8_t)v;
+                nd.done[(ly + y) * kNode + lx + x] = 1;
+            }
+    }
+    *bits_out = bits;
+    *sse_out = sse;
+    return (double)sse + lambda * bits;
+}
+
+// choose_qt_mtt_partition_rdo (vvc/slice.rs:4127-4167) for a 32x32 node.
+int heuristic_split(const Frame& f, int bx, int by, int qp) {
+    const int hw = kNode / 2;
+    auto mean = [&](int qx, int qy) {
+        int64_t s = 0;
+        for (int y = qy; y < qy + hw; y++)
+            for (int x = qx; x < qx + hw; x++) s += f.src[(size_t)y * f.stride + x];
+        return s / (hw * hw);
+    };
+    const int64_t m00 = mean(bx, by), m10 = mean(bx + hw, by), m01 = mean(bx, by + hw), m11 = mean(bx + hw, by + hw);
+    const int64_t hor = std::llabs(m00 - m01) + std::llabs(m10 - m11);
+    const int64_t ver = std::llabs(m00 - m10) + std::llabs(m01 - m11);
+    const int64_t thr = (qp > 4 ? qp : 4) / 2 + 8;
+    if (hor > thr && ver > thr) return 1;
+    if (hor > ver && hor > thr) return 2;
+    if (ver > thr) return 3;
+    return 0;
+}
+
+}  // namespace
+
+extern "C" __attribute__((visibility("default"))) rmh_status rmh_rd_encode_luma(
+    const uint8_t* src, uint32_t width, uint32_t height, uint32_t stride, uint32_t qp, uint32_t mode,
+    const float* probs, float p_prune, float p_fallback, uint8_t* recon, uint8_t* splits, float* costs,
+    rmh_rd_stats* stats) {
+    if (!src || !recon || !splits || !stats) return RMH_INVALID_INPUT;
+    if (width == 0 || height == 0 || width % kNode || height % kNode || stride < width || width > 16384 ||
+        height > 16384)
+        return RMH_INVALID_INPUT;
+    if (qp > 63 || mode > RMH_RD_HEURISTIC || (mode == RMH_RD_PRUNED && !probs)) return RMH_INVALID_INPUT;
+    const int nbx = (int)width / kNode, nby = (int)height / kNode;
+    Frame f{src, (int)width, (int)height, (int)stride, recon};
+    const double lambda = 0.57 * std::pow(2.0, ((double)qp - 12.0) / 3.0);
+    rmh_rd_stats st{};
+    Node best, cand;
+    for (int by = 0; by < nby; by++)
+        for (int bx = 0; bx < nbx; bx++) {
+            const int b = by * nbx + bx;
+            bool try_split[6] = {true, true, true, true, true, true};
+            if (mode == RMH_RD_PRUNED) {
+                const float* p = probs + 6 * (size_t)b;
+                float mx = 0.0f;
+                for (int s = 0; s < 6; s++) mx = std::max(mx, p[s]);
+                if (mx >= p_fallback) {
+                    for (int s = 0; s < 6; s++) try_split[s] = p[s] >= p_prune;
+                } else {
+                    st.fallbacks++;
+                }
+            } else if (mode == RMH_RD_HEURISTIC) {
+                const int h = heuristic_split(f, bx * kNode, by * kNode, (int)qp);
+                for (int s = 0; s < 6; s++) try_split[s] = s == h;
+            }
+            double best_j = 0.0, best_bits = 0.0;
+            uint64_t best_sse = 0;
+            int best_s = -1;
+            for (int s = 0; s < 6; s++) {
+                if (costs) costs[6 * (size_t)b + s] = -1.0f;
+                if (!try_split[s]) continue;
+                double bits;
+                uint64_t sse;
+                const double j = code_candidate(f, bx * kNode, by * kNode, s, (int)qp, lambda, cand, &bits, &sse);
+                st.candidates++;
+                if (costs) costs[6 * (size_t)b + s] = (float)j;
+                if (best_s < 0 || j < best_j) {
+                    best_j = j;
+                    best_s = s;
+                    best_bits = bits;
+                    best_sse = sse;
+                    best = cand;
+                }
+            }
+            splits[b] = (uint8_t)best_s;
+            st.bits += best_bits;
+            st.sse += best_sse;
+            for (int y = 0; y < kNode; y++)
+                std::memcpy(recon + (size_t)(by * kNode + y) * width + bx * kNode, best.rec + y * kNode, kNode);
+        }
+    st.blocks = (uint32_t)(nbx * nby);
+    *stats = st;
+    return RMH_OK;
+}
