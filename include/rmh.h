@@ -342,6 +342,39 @@ rmh_status rmh_rd_encode_luma(const uint8_t* src, uint32_t width, uint32_t heigh
                               float p_fallback, uint8_t* recon, uint8_t* splits, float* costs,
                               rmh_rd_stats* stats);
 
+/* ------------------------------------------------------------------------ */
+/* CU-split predictor: BF16 CNN on MFMA (no reference counterpart: the      */
+/* reference has no learned component; it serves RMH_RD_PRUNED above)        */
+/* ------------------------------------------------------------------------ */
+
+/* Input: 32x32 8-bit luma blocks (raster, 1024 bytes each) and QP / 51 per
+ * block. conv3x3(1->32), conv3x3(32->32) x2, ReLU, global average pool,
+ * MLP 33->64->6, softmax over the six split modes (RMH_SPLIT_*). Weights and
+ * activations are stored as bf16, sums accumulate in fp32 on MFMA. */
+typedef struct rmh_cnn rmh_cnn;
+
+size_t rmh_cnn_param_count(void);
+/* Seeded He-normal initialisation; buffers sized for batches of up to max_batch. */
+rmh_status rmh_cnn_create(rmh_ctx* ctx, uint32_t max_batch, uint64_t seed, rmh_cnn** out);
+void rmh_cnn_destroy(rmh_cnn* cnn);
+/* fp32 master parameters, rmh_cnn_param_count() floats, host memory. Setting
+ * them resets the optimiser state. */
+rmh_status rmh_cnn_get_params(rmh_cnn* cnn, float* out);
+rmh_status rmh_cnn_set_params(rmh_cnn* cnn, const float* in);
+/* Device buffers: blocks n*1024 bytes, qp n floats, probs n*6 floats. */
+rmh_status rmh_cnn_forward(rmh_cnn* cnn, const uint8_t* d_blocks, const float* d_qp, uint32_t n,
+                           float* d_probs, void* stream);
+/* Host buffers, synchronous (stages through device memory). */
+rmh_status rmh_cnn_predict(rmh_cnn* cnn, const uint8_t* blocks, const float* qp, uint32_t n, float* probs);
+/* Gradient of the mean softmax cross-entropy against labels (0..5, device
+ * bytes) into d_grads (rmh_cnn_param_count() floats) and the per-block
+ * losses into d_loss (n floats), without changing the parameters. */
+rmh_status rmh_cnn_gradients(rmh_cnn* cnn, const uint8_t* d_blocks, const float* d_qp, const uint8_t* d_labels,
+                             uint32_t n, float* d_grads, float* d_loss, void* stream);
+/* One Adam step (beta 0.9 / 0.999, eps 1e-8) at learning rate lr on the batch. */
+rmh_status rmh_cnn_train_step(rmh_cnn* cnn, const uint8_t* d_blocks, const float* d_qp,
+                              const uint8_t* d_labels, uint32_t n, float lr, float* d_loss, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -69,7 +69,9 @@ EXPORTED = [
     "rmh_residual_plan_run_range", "rmh_mc_batch", "rmh_mc_plan_create", "rmh_mc_plan_run", "rmh_deblock",
     "rmh_deblock_plan_create", "rmh_deblock_plan_run", "rmh_plan_destroy", "rmh_plan_bytes", "rmh_graph_create",
     "rmh_graph_launch", "rmh_graph_destroy", "rmh_transform_matrix", "rmh_intra_subset_decode",
-    "rmh_residual_recon_plan_create", "rmh_rd_encode_luma",
+    "rmh_residual_recon_plan_create", "rmh_rd_encode_luma", "rmh_cnn_param_count", "rmh_cnn_create",
+    "rmh_cnn_destroy", "rmh_cnn_get_params", "rmh_cnn_set_params", "rmh_cnn_forward", "rmh_cnn_predict",
+    "rmh_cnn_gradients", "rmh_cnn_train_step",
 ]
 
 
@@ -110,6 +112,15 @@ def load_library(path=LIB_PATH):
         "rmh_rd_encode_luma": ([vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
                                 ctypes.c_uint32, vp, ctypes.c_float, ctypes.c_float, vp, vp, vp,
                                 ctypes.POINTER(abi.RdStats)], st),
+        "rmh_cnn_param_count": ([], sz),
+        "rmh_cnn_create": ([vp, ctypes.c_uint32, ctypes.c_uint64, ctypes.POINTER(vp)], st),
+        "rmh_cnn_destroy": ([vp], None),
+        "rmh_cnn_get_params": ([vp, vp], st),
+        "rmh_cnn_set_params": ([vp, vp], st),
+        "rmh_cnn_forward": ([vp, vp, vp, ctypes.c_uint32, vp, vp], st),
+        "rmh_cnn_predict": ([vp, vp, vp, ctypes.c_uint32, vp], st),
+        "rmh_cnn_gradients": ([vp, vp, vp, vp, ctypes.c_uint32, vp, vp, vp], st),
+        "rmh_cnn_train_step": ([vp, vp, vp, vp, ctypes.c_uint32, ctypes.c_float, vp, vp], st),
         "rmh_intra_subset_decode": ([vp, ctypes.c_uint32, ctypes.c_uint32, vp, vp, sz, vp, sz, vp, vp, vp], st),
     }
     for name, (args, res) in sig.items():
@@ -337,6 +348,56 @@ class Accelerator:
         h = ctypes.c_void_p()
         self._check(load_library().rmh_graph_create(self.handle, ctypes.addressof(arr), len(nodes), ctypes.byref(h)))
         return Graph(self, h.value, [p for p, _, _ in nodes])
+
+
+class Cnn:
+    """The CU-split predictor (rmh_cnn): BF16 CNN on MFMA, forward and training."""
+
+    def __init__(self, acc, max_batch=256, seed=1):
+        self.acc = acc
+        h = ctypes.c_void_p()
+        acc._check(load_library().rmh_cnn_create(acc.handle, max_batch, seed, ctypes.byref(h)))
+        self.handle, self.max_batch = h.value, max_batch
+        self.n_params = int(load_library().rmh_cnn_param_count())
+
+    def params(self):
+        out = np.zeros(self.n_params, np.float32)
+        self.acc._check(load_library().rmh_cnn_get_params(self.handle, out.ctypes.data))
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        if p.size != self.n_params:
+            raise InvalidInput("expected %d parameters" % self.n_params)
+        self.acc._check(load_library().rmh_cnn_set_params(self.handle, p.ctypes.data))
+
+    def predict(self, blocks, qp):
+        """Host arrays: blocks uint8 (n, 32, 32), qp (n,) normalised -> probs (n, 6)."""
+        blocks = np.ascontiguousarray(blocks, dtype=np.uint8)
+        qp = np.ascontiguousarray(qp, dtype=np.float32)
+        n = len(blocks)
+        out = np.zeros((n, 6), np.float32)
+        for a in range(0, n, self.max_batch):
+            b = min(n, a + self.max_batch)
+            self.acc._check(load_library().rmh_cnn_predict(self.handle, blocks[a:b].ctypes.data, qp[a:b].ctypes.data,
+                                                           b - a, out[a:b].ctypes.data))
+        return out
+
+    def forward(self, d_blocks, d_qp, n, d_probs, stream=0):
+        self.acc._check(load_library().rmh_cnn_forward(self.handle, d_blocks, d_qp, n, d_probs, stream))
+
+    def gradients(self, d_blocks, d_qp, d_labels, n, d_grads, d_loss, stream=0):
+        self.acc._check(load_library().rmh_cnn_gradients(self.handle, d_blocks, d_qp, d_labels, n, d_grads, d_loss,
+                                                         stream))
+
+    def train_step(self, d_blocks, d_qp, d_labels, n, lr, d_loss, stream=0):
+        self.acc._check(load_library().rmh_cnn_train_step(self.handle, d_blocks, d_qp, d_labels, n, lr, d_loss,
+                                                          stream))
+
+    def close(self):
+        if self.handle:
+            load_library().rmh_cnn_destroy(self.handle)
+            self.handle = None
 
 
 class VvcResidualBatch:

@@ -11,6 +11,7 @@
 
 #include <atomic>
 #include <chrono>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <algorithm>
@@ -19,6 +20,7 @@
 #include <thread>
 #include <vector>
 
+#include "cnn.h"
 #include "kernels.h"
 #include "lfnst_data.h"
 #include "tables.h"
@@ -1116,4 +1118,210 @@ RMH_EXPORT void rmh_graph_destroy(rmh_graph* g) {
     if (g->graph) hipGraphDestroy(g->graph);
     if (g->capture) hipStreamDestroy(g->capture);
     delete g;
+}
+
+// ------------------------------------------------------------ CU-split CNN
+
+struct rmh_cnn {
+    rmh_ctx* ctx = nullptr;
+    uint32_t max_batch = 0;
+    void* d_mem = nullptr;
+    rmh::cnn::Buffers b{};
+    uint8_t* d_blocks = nullptr;  // staging of rmh_cnn_predict
+    float* d_qp = nullptr;
+    float* d_probs = nullptr;
+    int step = 0;
+};
+
+namespace {
+
+// splitmix64 stream -> standard normal (Box-Muller), for seeded He initialisation.
+struct InitRng {
+    uint64_t s;
+    uint64_t next() {
+        uint64_t z = (s += 0x9e3779b97f4a7c15ull);
+        z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+        z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+        return z ^ (z >> 31);
+    }
+    double uniform() { return ((next() >> 11) + 0.5) * (1.0 / 9007199254740992.0); }
+    double normal() { return std::sqrt(-2.0 * std::log(uniform())) * std::cos(6.283185307179586 * uniform()); }
+};
+
+void init_params(uint64_t seed, std::vector<float>& p) {
+    using namespace rmh::cnn;
+    p.assign(kParams, 0.0f);
+    InitRng r{seed};
+    auto fill = [&](int off, int count, double fan_in) {
+        const double sd = std::sqrt(2.0 / fan_in);
+        for (int i = 0; i < count; i++) p[off + i] = (float)(sd * r.normal());
+    };
+    fill(kOffW1, 32 * 9, 9.0);
+    fill(kOffW2, 32 * 9 * 32, 288.0);
+    fill(kOffW3, 32 * 9 * 32, 288.0);
+    fill(kOffW4, 64 * 33, 33.0);
+    fill(kOffW5, 6 * 64, 128.0);
+}
+
+rmh_status cnn_launch_status(rmh_cnn* c, hipError_t e, const char* what) {
+    if (e != hipSuccess) return fail(c->ctx, RMH_EXECUTION, std::string(what) + ": " + hipGetErrorString(e));
+    return RMH_OK;
+}
+
+}  // namespace
+
+RMH_EXPORT size_t rmh_cnn_param_count(void) { return (size_t)rmh::cnn::kParams; }
+
+RMH_EXPORT rmh_status rmh_cnn_create(rmh_ctx* ctx, uint32_t max_batch, uint64_t seed, rmh_cnn** out) {
+    if (!ctx || !out) return RMH_INVALID_INPUT;
+    *out = nullptr;
+    if (max_batch == 0 || max_batch > 65536) return fail(ctx, RMH_LIMIT_EXCEEDED, "max_batch must be 1..65536");
+    if (ctx->disabled.load()) return fail(ctx, RMH_UNAVAILABLE, "accelerator disabled");
+    using namespace rmh::cnn;
+    DeviceGuard dg(ctx->device);
+    const size_t n = max_batch, act = n * 1024 * 32 * 2;
+    const size_t sizes[] = {
+        kParams * 4u, kParams * 4u, kParams * 4u, kParams * 4u,  // params, grads, m, v
+        32 * 16 * 2, 9216 * 2, 9216 * 2, 9216 * 2, 9216 * 2,    // w1 w2 w3 w2t w3t
+        n * 1024 * 2, act, act, act, act, act,                  // x a1 a2 a3 dz dz2
+        n * 32 * 4, n * 32 * 4, n * 6 * 4, n * 9216 * 4,        // pooled dpool probs part
+        n * 1024, n * 4, n * 6 * 4};                            // staging: blocks qp probs
+    size_t total = 0;
+    for (size_t s : sizes) total += aligned(s);
+    rmh_cnn* c = new rmh_cnn();
+    c->ctx = ctx;
+    c->max_batch = max_batch;
+    hipError_t e = hipMalloc(&c->d_mem, total);
+    if (e != hipSuccess) {
+        delete c;
+        return fail(ctx, RMH_LIMIT_EXCEEDED, std::string("device allocation: ") + hipGetErrorString(e));
+    }
+    char* p = (char*)c->d_mem;
+    void* ptrs[sizeof sizes / sizeof sizes[0]];
+    for (size_t i = 0; i < sizeof sizes / sizeof sizes[0]; i++) {
+        ptrs[i] = p;
+        p += aligned(sizes[i]);
+    }
+    Buffers& b = c->b;
+    b.params = (float*)ptrs[0];
+    b.grads = (float*)ptrs[1];
+    b.adam_m = (float*)ptrs[2];
+    b.adam_v = (float*)ptrs[3];
+    b.w1 = (uint16_t*)ptrs[4];
+    b.w2 = (uint16_t*)ptrs[5];
+    b.w3 = (uint16_t*)ptrs[6];
+    b.w2t = (uint16_t*)ptrs[7];
+    b.w3t = (uint16_t*)ptrs[8];
+    b.x = (uint16_t*)ptrs[9];
+    b.a1 = (uint16_t*)ptrs[10];
+    b.a2 = (uint16_t*)ptrs[11];
+    b.a3 = (uint16_t*)ptrs[12];
+    b.dz = (uint16_t*)ptrs[13];
+    b.dz2 = (uint16_t*)ptrs[14];
+    b.pooled = (float*)ptrs[15];
+    b.dpool = (float*)ptrs[16];
+    b.probs = (float*)ptrs[17];
+    b.part = (float*)ptrs[18];
+    c->d_blocks = (uint8_t*)ptrs[19];
+    c->d_qp = (float*)ptrs[20];
+    c->d_probs = (float*)ptrs[21];
+    std::vector<float> init;
+    init_params(seed, init);
+    e = hipMemcpy(b.params, init.data(), kParams * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(b.adam_m, 0, kParams * 4);
+    if (e == hipSuccess) e = hipMemset(b.adam_v, 0, kParams * 4);
+    if (e == hipSuccess) e = pack_weights(b, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+        hipFree(c->d_mem);
+        delete c;
+        return fail(ctx, RMH_EXECUTION, std::string("network initialisation: ") + hipGetErrorString(e));
+    }
+    *out = c;
+    return RMH_OK;
+}
+
+RMH_EXPORT void rmh_cnn_destroy(rmh_cnn* c) {
+    if (!c) return;
+    DeviceGuard dg(c->ctx->device);
+    if (c->d_mem) hipFree(c->d_mem);
+    delete c;
+}
+
+RMH_EXPORT rmh_status rmh_cnn_get_params(rmh_cnn* c, float* out) {
+    if (!c || !out) return RMH_INVALID_INPUT;
+    DeviceGuard dg(c->ctx->device);
+    hipError_t e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipMemcpy(out, c->b.params, rmh::cnn::kParams * 4, hipMemcpyDeviceToHost);
+    return cnn_launch_status(c, e, "parameter download");
+}
+
+RMH_EXPORT rmh_status rmh_cnn_set_params(rmh_cnn* c, const float* in) {
+    if (!c || !in) return RMH_INVALID_INPUT;
+    DeviceGuard dg(c->ctx->device);
+    hipError_t e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipMemcpy(c->b.params, in, rmh::cnn::kParams * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(c->b.adam_m, 0, rmh::cnn::kParams * 4);
+    if (e == hipSuccess) e = hipMemset(c->b.adam_v, 0, rmh::cnn::kParams * 4);
+    if (e == hipSuccess) e = rmh::cnn::pack_weights(c->b, c->ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->ctx->stream);
+    c->step = 0;
+    return cnn_launch_status(c, e, "parameter upload");
+}
+
+RMH_EXPORT rmh_status rmh_cnn_forward(rmh_cnn* c, const uint8_t* d_blocks, const float* d_qp, uint32_t n,
+                                      float* d_probs, void* stream) {
+    if (!c) return RMH_INVALID_INPUT;
+    if (n == 0) return RMH_OK;
+    if (!d_blocks || !d_qp || !d_probs) return fail(c->ctx, RMH_INVALID_INPUT, "null device buffer");
+    if (n > c->max_batch) return fail(c->ctx, RMH_LIMIT_EXCEEDED, "batch larger than max_batch");
+    rmh_status g = gate(c->ctx);
+    if (g != RMH_OK) return fail(c->ctx, g, g == RMH_UNAVAILABLE ? "accelerator disabled" : "injected device failure");
+    DeviceGuard dg(c->ctx->device);
+    return cnn_launch_status(c, rmh::cnn::forward(c->b, d_blocks, d_qp, (int)n, d_probs, (hipStream_t)stream),
+                             "forward launch");
+}
+
+RMH_EXPORT rmh_status rmh_cnn_predict(rmh_cnn* c, const uint8_t* blocks, const float* qp, uint32_t n, float* probs) {
+    if (!c) return RMH_INVALID_INPUT;
+    if (n == 0) return RMH_OK;
+    if (!blocks || !qp || !probs) return fail(c->ctx, RMH_INVALID_INPUT, "null host buffer");
+    if (n > c->max_batch) return fail(c->ctx, RMH_LIMIT_EXCEEDED, "batch larger than max_batch");
+    rmh_ctx* ctx = c->ctx;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    rmh_status g = gate(ctx);
+    if (g != RMH_OK) return fail_locked(ctx, g, g == RMH_UNAVAILABLE ? "accelerator disabled" : "injected device failure");
+    DeviceGuard dg(ctx->device);
+    hipStream_t s = ctx->stream;
+    hipError_t e = hipMemcpyAsync(c->d_blocks, blocks, (size_t)n * 1024, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(c->d_qp, qp, (size_t)n * 4, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = rmh::cnn::forward(c->b, c->d_blocks, c->d_qp, (int)n, c->d_probs, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(probs, c->d_probs, (size_t)n * 24, hipMemcpyDeviceToHost, s);
+    if (e != hipSuccess) return hip_fail_locked(ctx, e, "predict");
+    return wait_stream(ctx, s);
+}
+
+RMH_EXPORT rmh_status rmh_cnn_gradients(rmh_cnn* c, const uint8_t* d_blocks, const float* d_qp, const uint8_t* d_labels,
+                                        uint32_t n, float* d_grads, float* d_loss, void* stream) {
+    if (!c) return RMH_INVALID_INPUT;
+    if (n == 0) return fail(c->ctx, RMH_INVALID_INPUT, "empty batch");
+    if (!d_blocks || !d_qp || !d_labels || !d_grads || !d_loss) return fail(c->ctx, RMH_INVALID_INPUT, "null device buffer");
+    if (n > c->max_batch) return fail(c->ctx, RMH_LIMIT_EXCEEDED, "batch larger than max_batch");
+    rmh_status g = gate(c->ctx);
+    if (g != RMH_OK) return fail(c->ctx, g, g == RMH_UNAVAILABLE ? "accelerator disabled" : "injected device failure");
+    DeviceGuard dg(c->ctx->device);
+    return cnn_launch_status(
+        c, rmh::cnn::gradients(c->b, d_blocks, d_qp, d_labels, (int)n, d_grads, d_loss, (hipStream_t)stream),
+        "gradient launch");
+}
+
+RMH_EXPORT rmh_status rmh_cnn_train_step(rmh_cnn* c, const uint8_t* d_blocks, const float* d_qp,
+                                         const uint8_t* d_labels, uint32_t n, float lr, float* d_loss, void* stream) {
+    if (!c) return RMH_INVALID_INPUT;
+    if (!(lr > 0.0f) || lr > 1.0f) return fail(c->ctx, RMH_INVALID_INPUT, "learning rate outside (0, 1]");
+    rmh_status st = rmh_cnn_gradients(c, d_blocks, d_qp, d_labels, n, c->b.grads, d_loss, stream);
+    if (st != RMH_OK) return st;
+    DeviceGuard dg(c->ctx->device);
+    c->step++;
+    return cnn_launch_status(c, rmh::cnn::adam(c->b, c->step, lr, (hipStream_t)stream), "optimiser launch");
 }
