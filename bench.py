@@ -216,6 +216,63 @@ def bench_streams(rmh, acc, W, PL, po, frame_steps, reps):
     return out
 
 
+CNN_MACS_PER_BLOCK = 32 * 32 * 32 * 9 + 2 * 32 * 32 * 32 * 288 + 33 * 64 + 64 * 6
+
+
+def bench_learned(rmh, acc, steps):
+    """The learned CU-split predictor: training on full-RD pairs, calibration,
+    full / pruned / heuristic encodes of held-out content, and the network's
+    forward and training-step rates on MFMA."""
+    hip = rmh.hip
+    L = rmh.learned
+    net, rep = L.run_experiment(acc, log=log)
+    t = rep["test"]
+    log("learned: pruned %.2f fps vs full %.2f fps (x%.2f), BD-rate %.3f %%, heuristic BD-rate %.3f %%" % (
+        t["pruned"]["fps"], t["full"]["fps"], t["speedup_pruned_vs_full"], t["pruned"]["bd_rate_vs_full"],
+        t["heuristic"]["bd_rate_vs_full"]))
+    # forward and train-step throughput on one batch of 256 blocks
+    n = 256
+    frames = L.synthetic_sequence(2, 1)
+    blocks = L.node_blocks(L.pad_to_nodes(frames[0]))[:n]
+    db = hip.DeviceBuffer.from_array(np.ascontiguousarray(blocks))
+    dq = hip.DeviceBuffer.from_array(np.full(n, L.qp_feature(32), np.float32))
+    dl = hip.DeviceBuffer.from_array(np.zeros(n, np.uint8))
+    dp = hip.DeviceBuffer(n * 24)
+    dloss = hip.DeviceBuffer(n * 4)
+    stream = hip.Stream()
+    fwd_ms = timed(hip, stream, lambda s: net.forward(db.ptr, dq.ptr, n, dp.ptr, s), steps, 3)
+    saved = net.params()
+    step_ms = timed(hip, stream, lambda s: net.train_step(db.ptr, dq.ptr, dl.ptr, n, 1e-6, dloss.ptr, s), steps, 3)
+    net.set_params(saved)
+    flops = 2.0 * CNN_MACS_PER_BLOCK * n
+    tflops = flops / (fwd_ms / 1e3) / 1e12
+    for b in (db, dq, dl, dp, dloss):
+        b.close()
+    net.close()
+    def strip(d):
+        return {k: (round(v, 4) if isinstance(v, float) else v) for k, v in d.items() if k != "splits"}
+    out = {"workload": "CU-split CNN (3 conv3x3 x 32ch + pool + MLP 64 -> 6, BF16 on MFMA, FP32 accumulate) trained "
+                       "on full-RD pairs of synthetic 1280x720 8-bit content at QP 22/27/32/37; encodes of held-out "
+                       "sequences",
+           "encode_fps_full": round(t["full"]["fps"], 3), "encode_fps_pruned": round(t["pruned"]["fps"], 3),
+           "encode_fps_heuristic": round(t["heuristic"]["fps"], 3),
+           "speedup_pruned_vs_full": round(t["speedup_pruned_vs_full"], 3),
+           "bd_rate_pruned_vs_full_pct": round(t["pruned"]["bd_rate_vs_full"], 3),
+           "bd_rate_heuristic_vs_full_pct": round(t["heuristic"]["bd_rate_vs_full"], 3),
+           "pruned": strip(t["pruned"]), "heuristic": strip(t["heuristic"]),
+           "untrained_same_scale": {k: round(v, 3) for k, v in rep["untrained"].items()},
+           "training": {k: rep[k] for k in ("train_pairs", "collect_seconds", "train_seconds", "epochs", "loss_first",
+                                            "loss_last", "logit_scale", "validation")},
+           "rd_points": t["points"],
+           "forward_ms_256": round(fwd_ms, 4), "train_step_ms_256": round(step_ms, 4),
+           "roofline": {"bound": "mfma", "achieved": round(tflops, 2), "peak": 2500.0, "unit": "TFLOP/s",
+                        "frac": round(tflops / 2500.0, 5), "traffic": None,
+                        "kernel": "forward pass (conv_fwd_kernel x3 + pool + MLP), 256 blocks",
+                        "algorithmic_flops_per_launch": flops}}
+    log("learned: forward %.4f ms / 256 blocks = %.1f TFLOP/s; train step %.4f ms" % (fwd_ms, tflops, step_ms))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -229,6 +286,7 @@ def main():
     ap.add_argument("--pipeline-frames", type=int, default=8)
     ap.add_argument("--no-pipeline", action="store_true")
     ap.add_argument("--no-streams", action="store_true")
+    ap.add_argument("--no-learned", action="store_true")
     args = ap.parse_args()
 
     if args.copy_calibration:
@@ -332,6 +390,8 @@ def main():
                                                          args.warmup)
     streams = None if args.no_streams else bench_streams(rmh, acc, W, PL, po, 8, max(2, args.steps // 5))
 
+    learned = None if args.no_learned else bench_learned(rmh, acc, args.steps)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(frames[:2], args.cpu_seconds)
@@ -352,6 +412,7 @@ def main():
         "cpu_baseline": cpu,
         "pframe_pipeline": pipe,
         "multi_stream": streams,
+        "learned": learned,
         "device": info,
     }
     print(json.dumps(out), flush=True)

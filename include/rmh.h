@@ -374,6 +374,10 @@ rmh_status rmh_cnn_gradients(rmh_cnn* cnn, const uint8_t* d_blocks, const float*
 /* One Adam step (beta 0.9 / 0.999, eps 1e-8) at learning rate lr on the batch. */
 rmh_status rmh_cnn_train_step(rmh_cnn* cnn, const uint8_t* d_blocks, const float* d_qp,
                               const uint8_t* d_labels, uint32_t n, float lr, float* d_loss, void* stream);
+/* The same against target distributions over the six splits (d_targets: n*6
+ * floats, each row summing to 1), e.g. a softmax of negated RD costs. */
+rmh_status rmh_cnn_train_step_soft(rmh_cnn* cnn, const uint8_t* d_blocks, const float* d_qp,
+                                   const float* d_targets, uint32_t n, float lr, float* d_loss, void* stream);
 
 #ifdef __cplusplus
 }

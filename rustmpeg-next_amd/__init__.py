@@ -71,7 +71,7 @@ EXPORTED = [
     "rmh_graph_launch", "rmh_graph_destroy", "rmh_transform_matrix", "rmh_intra_subset_decode",
     "rmh_residual_recon_plan_create", "rmh_rd_encode_luma", "rmh_cnn_param_count", "rmh_cnn_create",
     "rmh_cnn_destroy", "rmh_cnn_get_params", "rmh_cnn_set_params", "rmh_cnn_forward", "rmh_cnn_predict",
-    "rmh_cnn_gradients", "rmh_cnn_train_step",
+    "rmh_cnn_gradients", "rmh_cnn_train_step", "rmh_cnn_train_step_soft",
 ]
 
 
@@ -121,6 +121,7 @@ def load_library(path=LIB_PATH):
         "rmh_cnn_predict": ([vp, vp, vp, ctypes.c_uint32, vp], st),
         "rmh_cnn_gradients": ([vp, vp, vp, vp, ctypes.c_uint32, vp, vp, vp], st),
         "rmh_cnn_train_step": ([vp, vp, vp, vp, ctypes.c_uint32, ctypes.c_float, vp, vp], st),
+        "rmh_cnn_train_step_soft": ([vp, vp, vp, vp, ctypes.c_uint32, ctypes.c_float, vp, vp], st),
         "rmh_intra_subset_decode": ([vp, ctypes.c_uint32, ctypes.c_uint32, vp, vp, sz, vp, sz, vp, vp, vp], st),
     }
     for name, (args, res) in sig.items():
@@ -394,6 +395,10 @@ class Cnn:
         self.acc._check(load_library().rmh_cnn_train_step(self.handle, d_blocks, d_qp, d_labels, n, lr, d_loss,
                                                           stream))
 
+    def train_step_soft(self, d_blocks, d_qp, d_targets, n, lr, d_loss, stream=0):
+        self.acc._check(load_library().rmh_cnn_train_step_soft(self.handle, d_blocks, d_qp, d_targets, n, lr, d_loss,
+                                                               stream))
+
     def close(self):
         if self.handle:
             load_library().rmh_cnn_destroy(self.handle)
@@ -429,3 +434,4 @@ class VvcDeblockFilter:
 
     def deblock(self, frames):
         return self.acc.deblock(frames)
+from . import learned  # noqa: E402

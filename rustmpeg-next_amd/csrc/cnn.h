@@ -49,9 +49,10 @@ struct Buffers {
 // Forward pass: probs[n][6] (softmax).
 hipError_t forward(const Buffers& b, const uint8_t* blocks, const float* qp, int n, float* probs, hipStream_t s);
 // Gradients of the mean cross-entropy into grads[kParams] (b.grads or any
-// device buffer of kParams floats) and per-sample losses into loss[n].
-hipError_t gradients(const Buffers& b, const uint8_t* blocks, const float* qp, const uint8_t* labels, int n,
-                     float* grads, float* loss, hipStream_t s);
+// device buffer of kParams floats) and per-sample losses into loss[n]. The
+// target is labels[n] (one-hot) or, when labels is null, targets[n][6].
+hipError_t gradients(const Buffers& b, const uint8_t* blocks, const float* qp, const uint8_t* labels,
+                     const float* targets, int n, float* grads, float* loss, hipStream_t s);
 // bf16 weight copies from the fp32 master weights.
 hipError_t pack_weights(const Buffers& b, hipStream_t s);
 // One Adam step (t >= 1) from b.grads, then pack_weights.

@@ -217,8 +217,9 @@ __global__ __launch_bounds__(256) void pool_kernel(const uint16_t* __restrict__ 
 // and the per-sample MLP weight gradients part[n][kMlpParams].
 __global__ void mlp_kernel(const float* __restrict__ pooled, const float* __restrict__ qp, const float* __restrict__ w4,
                            const float* __restrict__ b4, const float* __restrict__ w5, const float* __restrict__ b5,
-                           int n, float* __restrict__ probs, const uint8_t* __restrict__ labels, float inv_batch,
-                           float* __restrict__ loss, float* __restrict__ dpool, float* __restrict__ part) {
+                           int n, float* __restrict__ probs, const uint8_t* __restrict__ labels,
+                           const float* __restrict__ targets, float inv_batch, float* __restrict__ loss,
+                           float* __restrict__ dpool, float* __restrict__ part) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     float in[33], h[64], z[6];
@@ -242,11 +243,24 @@ __global__ void mlp_kernel(const float* __restrict__ pooled, const float* __rest
         den += e[o];
     }
     for (int o = 0; o < 6; o++) probs[s * 6 + o] = e[o] / den;
-    if (!labels) return;
-    const int lab = labels[s] < 6 ? labels[s] : 5;  // labels are split modes 0..5
-    loss[s] = -(z[lab] - mx - logf(den));
+    if (!labels && !targets) return;
+    // target distribution: one-hot label (split modes 0..5) or a given distribution
+    float t[6];
+    if (targets) {
+        for (int o = 0; o < 6; o++) t[o] = targets[s * 6 + o];
+    } else {
+        const int lab = labels[s] < 6 ? labels[s] : 5;
+        for (int o = 0; o < 6; o++) t[o] = o == lab ? 1.0f : 0.0f;
+    }
+    const float lden = logf(den);
+    float l = 0.0f, tsum = 0.0f;
+    for (int o = 0; o < 6; o++) {
+        l -= t[o] * (z[o] - mx - lden);
+        tsum += t[o];
+    }
+    loss[s] = l;
     float dz[6];
-    for (int o = 0; o < 6; o++) dz[o] = (e[o] / den - (o == lab ? 1.0f : 0.0f)) * inv_batch;
+    for (int o = 0; o < 6; o++) dz[o] = (tsum * e[o] / den - t[o]) * inv_batch;
     float* pw4 = part + (size_t)s * kMlpParams;
     float* pb4 = pw4 + 64 * 33;
     float* pw5 = pb4 + 64;
@@ -326,19 +340,19 @@ hipError_t forward(const Buffers& b, const uint8_t* blocks, const float* qp, int
     hipLaunchKernelGGL(conv_fwd_kernel<32>, dim3(n, 8), dim3(256), 0, s, b.a2, b.w3, b.params + kOffB3, nullptr, b.a3, 1);
     hipLaunchKernelGGL(pool_kernel, dim3(n), dim3(256), 0, s, b.a3, b.pooled);
     hipLaunchKernelGGL(mlp_kernel, dim3(blocks_for(n, 64)), dim3(64), 0, s, b.pooled, qp, b.params + kOffW4,
-                       b.params + kOffB4, b.params + kOffW5, b.params + kOffB5, n, probs, nullptr, 0.0f, nullptr,
-                       nullptr, nullptr);
+                       b.params + kOffB4, b.params + kOffW5, b.params + kOffB5, n, probs, nullptr, nullptr, 0.0f,
+                       nullptr, nullptr, nullptr);
     return hipGetLastError();
 }
 
-hipError_t gradients(const Buffers& b, const uint8_t* blocks, const float* qp, const uint8_t* labels, int n,
-                     float* grads, float* loss, hipStream_t s) {
+hipError_t gradients(const Buffers& b, const uint8_t* blocks, const float* qp, const uint8_t* labels,
+                     const float* targets, int n, float* grads, float* loss, hipStream_t s) {
     hipError_t e = forward(b, blocks, qp, n, b.probs, s);
     if (e != hipSuccess) return e;
     const size_t act = (size_t)n * 1024 * 32;
     hipLaunchKernelGGL(mlp_kernel, dim3(blocks_for(n, 64)), dim3(64), 0, s, b.pooled, qp, b.params + kOffW4,
-                       b.params + kOffB4, b.params + kOffW5, b.params + kOffB5, n, b.probs, labels, 1.0f / (float)n,
-                       loss, b.dpool, b.part);
+                       b.params + kOffB4, b.params + kOffW5, b.params + kOffB5, n, b.probs, labels, targets,
+                       1.0f / (float)n, loss, b.dpool, b.part);
     hipLaunchKernelGGL(reduce_kernel, dim3(blocks_for(kMlpParams, 256)), dim3(256), 0, s, b.part, n, kMlpParams,
                        grads + kOffW4);
     // conv3

@@ -1301,27 +1301,53 @@ RMH_EXPORT rmh_status rmh_cnn_predict(rmh_cnn* c, const uint8_t* blocks, const f
     return wait_stream(ctx, s);
 }
 
-RMH_EXPORT rmh_status rmh_cnn_gradients(rmh_cnn* c, const uint8_t* d_blocks, const float* d_qp, const uint8_t* d_labels,
-                                        uint32_t n, float* d_grads, float* d_loss, void* stream) {
+namespace {
+
+rmh_status cnn_gradients(rmh_cnn* c, const uint8_t* d_blocks, const float* d_qp, const uint8_t* d_labels,
+                         const float* d_targets, uint32_t n, float* d_grads, float* d_loss, void* stream) {
     if (!c) return RMH_INVALID_INPUT;
     if (n == 0) return fail(c->ctx, RMH_INVALID_INPUT, "empty batch");
-    if (!d_blocks || !d_qp || !d_labels || !d_grads || !d_loss) return fail(c->ctx, RMH_INVALID_INPUT, "null device buffer");
+    if (!d_blocks || !d_qp || !(d_labels || d_targets) || !d_grads || !d_loss)
+        return fail(c->ctx, RMH_INVALID_INPUT, "null device buffer");
     if (n > c->max_batch) return fail(c->ctx, RMH_LIMIT_EXCEEDED, "batch larger than max_batch");
     rmh_status g = gate(c->ctx);
     if (g != RMH_OK) return fail(c->ctx, g, g == RMH_UNAVAILABLE ? "accelerator disabled" : "injected device failure");
     DeviceGuard dg(c->ctx->device);
-    return cnn_launch_status(
-        c, rmh::cnn::gradients(c->b, d_blocks, d_qp, d_labels, (int)n, d_grads, d_loss, (hipStream_t)stream),
-        "gradient launch");
+    return cnn_launch_status(c,
+                             rmh::cnn::gradients(c->b, d_blocks, d_qp, d_labels, d_targets, (int)n, d_grads, d_loss,
+                                                 (hipStream_t)stream),
+                             "gradient launch");
+}
+
+rmh_status cnn_step(rmh_cnn* c, float lr, void* stream) {
+    DeviceGuard dg(c->ctx->device);
+    c->step++;
+    return cnn_launch_status(c, rmh::cnn::adam(c->b, c->step, lr, (hipStream_t)stream), "optimiser launch");
+}
+
+}  // namespace
+
+RMH_EXPORT rmh_status rmh_cnn_gradients(rmh_cnn* c, const uint8_t* d_blocks, const float* d_qp, const uint8_t* d_labels,
+                                        uint32_t n, float* d_grads, float* d_loss, void* stream) {
+    if (!d_labels) return c ? fail(c->ctx, RMH_INVALID_INPUT, "null labels") : RMH_INVALID_INPUT;
+    return cnn_gradients(c, d_blocks, d_qp, d_labels, nullptr, n, d_grads, d_loss, stream);
 }
 
 RMH_EXPORT rmh_status rmh_cnn_train_step(rmh_cnn* c, const uint8_t* d_blocks, const float* d_qp,
                                          const uint8_t* d_labels, uint32_t n, float lr, float* d_loss, void* stream) {
     if (!c) return RMH_INVALID_INPUT;
     if (!(lr > 0.0f) || lr > 1.0f) return fail(c->ctx, RMH_INVALID_INPUT, "learning rate outside (0, 1]");
-    rmh_status st = rmh_cnn_gradients(c, d_blocks, d_qp, d_labels, n, c->b.grads, d_loss, stream);
-    if (st != RMH_OK) return st;
-    DeviceGuard dg(c->ctx->device);
-    c->step++;
-    return cnn_launch_status(c, rmh::cnn::adam(c->b, c->step, lr, (hipStream_t)stream), "optimiser launch");
+    if (!d_labels) return fail(c->ctx, RMH_INVALID_INPUT, "null labels");
+    rmh_status st = cnn_gradients(c, d_blocks, d_qp, d_labels, nullptr, n, c->b.grads, d_loss, stream);
+    return st != RMH_OK ? st : cnn_step(c, lr, stream);
+}
+
+RMH_EXPORT rmh_status rmh_cnn_train_step_soft(rmh_cnn* c, const uint8_t* d_blocks, const float* d_qp,
+                                              const float* d_targets, uint32_t n, float lr, float* d_loss,
+                                              void* stream) {
+    if (!c) return RMH_INVALID_INPUT;
+    if (!(lr > 0.0f) || lr > 1.0f) return fail(c->ctx, RMH_INVALID_INPUT, "learning rate outside (0, 1]");
+    if (!d_targets) return fail(c->ctx, RMH_INVALID_INPUT, "null targets");
+    rmh_status st = cnn_gradients(c, d_blocks, d_qp, nullptr, d_targets, n, c->b.grads, d_loss, stream);
+    return st != RMH_OK ? st : cnn_step(c, lr, stream);
 }
