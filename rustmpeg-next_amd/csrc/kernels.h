@@ -71,15 +71,19 @@ inline uint32_t residual_tile_lds(int lw, int lh) {
     return c + t + count * 48 + 16;
 }
 
-// One inter work item: a 16x16 luma tile (and its 8x8 chroma tiles) of one PU.
+// One inter work item: a 16x16 luma tile (and its 8x8 chroma tiles) of one
+// PU, with its sizes already clipped to the picture (built at plan creation).
 struct McItem {
-    uint32_t pu;
-    uint16_t tx;  // tile column inside the PU (units of 16 luma samples)
-    uint16_t ty;  // tile row
+    uint16_t x0, y0;  // luma tile origin in the picture
+    uint8_t tw, th;   // luma tile size, 1..16
+    uint8_t cw, ch;   // chroma tile size, 0..8
+    uint16_t frame;
+    uint16_t flags;   // bit 0: alternative half-sample luma filter
+    int32_t mv_x, mv_y;
+    uint32_t reserved;
 };
 
-hipError_t launch_mc(const rmh_mc_frame* frames, const rmh_pu_desc* pus, const McItem* items, uint32_t n_items,
-                     hipStream_t stream);
+hipError_t launch_mc(const rmh_mc_frame* frames, const McItem* items, uint32_t n_items, hipStream_t stream);
 
 // Deblocking runs two launches (all vertical edges, then all horizontal edges).
 hipError_t launch_deblock(const rmh_dbk_frame* frames, uint32_t n_frames, uint32_t max_cells, hipStream_t stream);

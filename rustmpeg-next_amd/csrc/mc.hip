@@ -8,10 +8,11 @@
 // of predict_inter_pu (inter.rs:1734-1848) and write_prediction
 // (decode_general.rs:2641-2667).
 //
-// Mapping: one 256-lane workgroup per 16x16 luma tile of a PU and the matching
-// 8x8 tiles of both chroma planes. The reference window (tile + 7 / + 3
-// samples of halo, edge-replicated) is staged once in LDS; the horizontal pass
-// writes 14-bit intermediates to LDS; the vertical pass writes the prediction.
+// Mapping: one wave per 16x16 luma tile of a PU and the matching 8x8 tiles of
+// both chroma planes, four waves per workgroup. The three reference windows
+// (tile + 7 / + 3 samples of halo, edge-replicated) are staged in LDS as
+// int16 before any filtering; the horizontal pass writes 14-bit intermediates
+// to LDS; the vertical pass writes the prediction.
 #include <hip/hip_runtime.h>
 
 #include "kernels.h"
@@ -40,127 +41,257 @@ __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? l
 __device__ __forceinline__ int load_sample(const rmh_plane& p, int x, int y) {
     x = clampi(x, 0, (int)p.width - 1);
     y = clampi(y, 0, (int)p.height - 1);
-    size_t o = (size_t)y * p.stride + x;
+    const uint32_t o = (uint32_t)y * p.stride + (uint32_t)x;
     return p.bytes_per_sample == 1 ? (int)((const uint8_t*)p.data)[o] : (int)((const uint16_t*)p.data)[o];
 }
 
 __device__ __forceinline__ void store_sample(const rmh_plane& p, int x, int y, int v) {
-    size_t o = (size_t)y * p.stride + x;
+    const uint32_t o = (uint32_t)y * p.stride + (uint32_t)x;
     if (p.bytes_per_sample == 1)
         ((uint8_t*)p.data)[o] = (uint8_t)v;
     else
         ((uint16_t*)p.data)[o] = (uint16_t)v;
 }
 
-// Interpolates a tw x th tile (tw, th <= T) whose top-left output sample is
-// (x0, y0) of `dst`; `ix`, `iy` are the integer reference positions of that
-// sample and `fx`, `fy` the fractions. TAPS = 8 (luma) or 4 (chroma).
-template <int TAPS, int T>
-__device__ void interp_tile(const rmh_plane& ref, const rmh_plane& dst, int x0, int y0, int tw, int th, int ix,
-                            int iy, const int8_t* cx, const int8_t* cy, bool hfrac, bool vfrac, int* win, int* tmp) {
-    constexpr int HALO = TAPS - 1;
-    constexpr int BEFORE = TAPS / 2 - 1;  // 3 for luma, 1 for chroma
-    constexpr int WW = T + HALO;          // window width / height
-    const int tid = threadIdx.x;
-    const int bd = ref.bit_depth;
-    const int head_room = 14 - bd;
-    const int maxv = (1 << bd) - 1;
+__device__ __forceinline__ int dot2(uint32_t a, uint32_t b, int acc) {
+    typedef short short2v __attribute__((ext_vector_type(2)));
+    return __builtin_amdgcn_sdot2(__builtin_bit_cast(short2v, a), __builtin_bit_cast(short2v, b), acc, false);
+}
 
-    for (int e = tid; e < WW * WW; e += blockDim.x) {
-        int r = e / WW, c = e - (e / WW) * WW;
-        win[e] = load_sample(ref, ix - BEFORE + c, iy - BEFORE + r);
-    }
-    __syncthreads();
-    if (!hfrac && !vfrac) {
-        for (int e = tid; e < T * T; e += blockDim.x) {
-            int y = e / T, x = e % T;
-            if (x < tw && y < th) store_sample(dst, x0 + x, y0 + y, win[(y + BEFORE) * WW + x + BEFORE]);
+// (hi:lo) >> 16: the int16 pair starting one sample after `lo`'s first.
+__device__ __forceinline__ uint32_t mid(uint32_t hi, uint32_t lo) { return __builtin_amdgcn_alignbyte(hi, lo, 2); }
+
+__device__ __forceinline__ uint32_t pack2(int a, int b) { return (uint32_t)(uint16_t)a | ((uint32_t)(uint16_t)b << 16); }
+
+// Geometry of one wave's LDS (int16): reference windows row-major with an even
+// pitch, so sample pairs (x, x + 1) at even x are aligned dwords, and the
+// horizontal-pass intermediates column-major, so vertical pairs are too.
+constexpr int kLR = 23, kLP = 24;  // luma window rows, pitch (16 + 7 samples, + 1 pad)
+constexpr int kCR = 11, kCP = 12;  // chroma window rows, pitch (8 + 3, + 1 pad)
+struct WaveLds {
+    int16_t luma[kLR * kLP];
+    int16_t ltmp[16 * kLP];        // [x][r]
+    int16_t chroma[2][kCR * kCP];
+    int16_t ctmp[2][8 * kCP];      // [x][r]
+};
+
+// One plane's tile of an item; every field is the same for all lanes of the
+// wave. Taps are kept as int16 pairs (c0, c1), (c2, c3), ...
+template <int TAPS>
+struct Tile {
+    int x0, y0;    // first output sample in dst
+    int tw, th;    // output size (clipped to the picture)
+    int ix, iy;    // integer reference position of the first output sample
+    uint32_t cx[TAPS / 2];
+    uint32_t cy[TAPS / 2];
+    int sy[TAPS];  // vertical taps as ints (vertical-only tiles)
+    bool hf, vf;   // fractional horizontally / vertically
+};
+
+// Phase 1: stage window rows iy - BEFORE .. (ROWS) and columns ix - BEFORE ..
+// (PITCH, the last one only as padding), clamped to the picture, into LDS.
+// Lane (c, r0) loads column c of rows r0, r0 + 64 / LANES_X, ...
+template <int TAPS, int ROWS, int PITCH, int LANES_X>
+__device__ __forceinline__ void stage(const rmh_plane& ref, const Tile<TAPS>& t, int16_t* win, int lane) {
+    constexpr int BEFORE = TAPS / 2 - 1, RSTEP = 64 / LANES_X;
+    const int ox = t.ix - BEFORE, oy = t.iy - BEFORE;
+    const int c = lane & (LANES_X - 1), r0 = lane / LANES_X;
+    if (c >= PITCH) return;
+    const bool inside = ox >= 0 && oy >= 0 && ox + PITCH <= (int)ref.width && oy + ROWS <= (int)ref.height;
+    if (inside) {
+        const uint32_t base = (uint32_t)(oy + r0) * ref.stride + (uint32_t)(ox + c);
+        const uint32_t step = RSTEP * ref.stride;
+        if (ref.bytes_per_sample == 1) {
+            const uint8_t* src = (const uint8_t*)ref.data + base;
+#pragma unroll 4
+            for (int r = r0, o = 0; r < ROWS; r += RSTEP, o += step) win[r * PITCH + c] = (int16_t)src[o];
+        } else {
+            const uint16_t* src = (const uint16_t*)ref.data + base;
+#pragma unroll 4
+            for (int r = r0, o = 0; r < ROWS; r += RSTEP, o += step) win[r * PITCH + c] = (int16_t)src[o];
         }
-        return;
+    } else {
+#pragma unroll 4
+        for (int r = r0; r < ROWS; r += RSTEP) win[r * PITCH + c] = (int16_t)load_sample(ref, ox + c, oy + r);
     }
-    if (!vfrac || !hfrac) {
-        // one-dimensional: (sum + 32) >> 6 clipped (inter.rs:939-1002)
-        for (int e = tid; e < T * T; e += blockDim.x) {
-            int y = e / T, x = e % T;
-            if (x < tw && y < th) {
-                int sum = 0;
-                if (hfrac) {
+}
+
+// Horizontal FIR of output pair (x, x + 1), x even, on a row of int16 samples:
+// TAPS / 2 + 1 aligned dword reads, pairs at odd offsets by alignbyte.
+template <int TAPS>
+__device__ __forceinline__ void hpair(const int16_t* row, const uint32_t (&c)[TAPS / 2], int& s0, int& s1) {
+    const uint32_t* d = reinterpret_cast<const uint32_t*>(row);
+    uint32_t v[TAPS / 2 + 1];
 #pragma unroll
-                    for (int k = 0; k < TAPS; k++) sum += cx[k] * win[(y + BEFORE) * WW + x + k];
-                } else {
+    for (int k = 0; k <= TAPS / 2; k++) v[k] = d[k];
+    s0 = 0;
+    s1 = 0;
 #pragma unroll
-                    for (int k = 0; k < TAPS; k++) sum += cy[k] * win[(y + k) * WW + x + BEFORE];
-                }
-                store_sample(dst, x0 + x, y0 + y, clampi((sum + 32) >> 6, 0, maxv));
-            }
+    for (int k = 0; k < TAPS / 2; k++) {
+        s0 = dot2(v[k], c[k], s0);
+        s1 = dot2(mid(v[k + 1], v[k]), c[k], s1);
+    }
+}
+
+// Phase 2 (two-dimensional tiles only): horizontal pass over th + TAPS - 1
+// rows into column-major intermediates, (sum - (8192 << shift1)) >> shift1
+// (interpolate_luma_block 2-D branch, vvc/inter.rs:1004-1030). Lane item e:
+// pair xp = e % NPAIR, row e / NPAIR.
+template <int TAPS, int PITCH, int NPAIR>
+__device__ __forceinline__ void hpass(const Tile<TAPS>& t, const int16_t* win, int16_t* tmp, int shift1, int lane) {
+    if (!(t.hf && t.vf)) return;
+    const int rows = t.th + TAPS - 1;
+    const int off = 8192 << shift1;
+    for (int e = lane; e < rows * NPAIR; e += 64) {
+        const int xp = e % NPAIR, r = e / NPAIR, x = 2 * xp;
+        if (x >= t.tw) continue;
+        int s0, s1;
+        hpair<TAPS>(win + r * PITCH + x, t.cx, s0, s1);
+        tmp[x * PITCH + r] = (int16_t)((s0 - off) >> shift1);
+        tmp[(x + 1) * PITCH + r] = (int16_t)((s1 - off) >> shift1);
+    }
+}
+
+// Phase 3: the output samples of one plane's tile (copy, one-dimensional or
+// vertical pass), T = full tile width.
+template <int TAPS, int PITCH, int T>
+__device__ __forceinline__ void output(const rmh_plane& dst, const Tile<TAPS>& t, const int16_t* win, const int16_t* tmp,
+                                       int head_room, int maxv, int lane) {
+    constexpr int BEFORE = TAPS / 2 - 1;
+    if (t.hf && t.vf) {
+        // vertical pass over the intermediates, output pair (y, y + 1):
+        // (sum + 2^(shift2-1) + (8192 << 6)) >> shift2, shift2 = 6 + head_room
+        const int shift2 = 6 + head_room;
+        const int off2 = (1 << (shift2 - 1)) + (8192 << 6);
+        for (int e = lane; e < T * (T / 2); e += 64) {
+            const int x = e % T, y = 2 * (e / T);
+            if (x >= t.tw || y >= t.th) continue;
+            int s0, s1;
+            hpair<TAPS>(tmp + x * PITCH + y, t.cy, s0, s1);
+            store_sample(dst, t.x0 + x, t.y0 + y, clampi((s0 + off2) >> shift2, 0, maxv));
+            if (y + 1 < t.th) store_sample(dst, t.x0 + x, t.y0 + y + 1, clampi((s1 + off2) >> shift2, 0, maxv));
         }
-        return;
-    }
-    // two-dimensional (inter.rs:1004-1050): horizontal pass over th + HALO rows
-    const int shift1 = 6 - head_room;
-    const int off1 = -8192 * (1 << shift1);
-    for (int e = tid; e < WW * T; e += blockDim.x) {
-        int r = e / T, x = e % T;
-        int sum = 0;
-#pragma unroll
-        for (int k = 0; k < TAPS; k++) sum += cx[k] * win[r * WW + x + k];
-        tmp[e] = (sum + off1) >> shift1;
-    }
-    __syncthreads();
-    const int shift2 = 6 + head_room;
-    const int off2 = (1 << (shift2 - 1)) + (8192 << 6);
-    for (int e = tid; e < T * T; e += blockDim.x) {
-        int y = e / T, x = e % T;
-        if (x < tw && y < th) {
+    } else if (t.hf) {
+        // horizontal only: (sum + 32) >> 6 clipped (vvc/inter.rs:939-970), output pair (x, x + 1)
+        for (int e = lane; e < T * (T / 2); e += 64) {
+            const int xp = e % (T / 2), y = e / (T / 2), x = 2 * xp;
+            if (x >= t.tw || y >= t.th) continue;
+            int s0, s1;
+            hpair<TAPS>(win + (y + BEFORE) * PITCH + x, t.cx, s0, s1);
+            store_sample(dst, t.x0 + x, t.y0 + y, clampi((s0 + 32) >> 6, 0, maxv));
+            if (x + 1 < t.tw) store_sample(dst, t.x0 + x + 1, t.y0 + y, clampi((s1 + 32) >> 6, 0, maxv));
+        }
+    } else if (t.vf) {
+        // vertical only (vvc/inter.rs:971-1002)
+        for (int e = lane; e < T * T; e += 64) {
+            const int x = e % T, y = e / T;
+            if (x >= t.tw || y >= t.th) continue;
             int sum = 0;
 #pragma unroll
-            for (int k = 0; k < TAPS; k++) sum += cy[k] * tmp[(y + k) * T + x];
-            store_sample(dst, x0 + x, y0 + y, clampi((sum + off2) >> shift2, 0, maxv));
+            for (int k = 0; k < TAPS; k++) sum += t.sy[k] * win[(y + k) * PITCH + x + BEFORE];
+            store_sample(dst, t.x0 + x, t.y0 + y, clampi((sum + 32) >> 6, 0, maxv));
+        }
+    } else {
+        for (int e = lane; e < T * T; e += 64) {
+            const int x = e % T, y = e / T;
+            if (x >= t.tw || y >= t.th) continue;
+            store_sample(dst, t.x0 + x, t.y0 + y, win[(y + BEFORE) * PITCH + x + BEFORE]);
         }
     }
 }
 
-__global__ __launch_bounds__(256) void mc_kernel(const rmh_mc_frame* __restrict__ frames,
-                                                 const rmh_pu_desc* __restrict__ pus,
-                                                 const McItem* __restrict__ items) {
-    __shared__ int win[23 * 23];
-    __shared__ int tmp[23 * 16];
-    const McItem it = items[blockIdx.x];
-    const rmh_pu_desc pu = pus[it.pu];
-    const rmh_mc_frame& f = frames[pu.frame];
-    const int w = 1 << (pu.log2_w & 15), h = 1 << (pu.log2_h & 15);
-    const bool alt = (pu.log2_w & RMH_PU_ALT_HPEL) != 0;
-    const int tx0 = it.tx * 16, ty0 = it.ty * 16;
-    const int tw = min(16, w - tx0), th = min(16, h - ty0);
-    const int x0 = pu.x + tx0, y0 = pu.y + ty0;
-
-    // luma: 1/16 sample
-    {
-        const int fx = pu.mv_x & 15, fy = pu.mv_y & 15;
-        const int ix = x0 + (pu.mv_x >> 4), iy = y0 + (pu.mv_y >> 4);
-        const int8_t* cx = (alt && fx == 8) ? kLumaAltHpel : kLuma8[fx];
-        const int8_t* cy = (alt && fy == 8) ? kLumaAltHpel : kLuma8[fy];
-        const int vw = min(tw, (int)f.dst[0].width - x0), vh = min(th, (int)f.dst[0].height - y0);
-        interp_tile<8, 16>(f.ref[0], f.dst[0], x0, y0, vw, vh, ix, iy, cx, cy, fx != 0, fy != 0, win, tmp);
+template <int TAPS>
+__device__ __forceinline__ void set_taps(Tile<TAPS>& t, const int8_t* cx, const int8_t* cy) {
+#pragma unroll
+    for (int k = 0; k < TAPS / 2; k++) {
+        t.cx[k] = pack2(cx[2 * k], cx[2 * k + 1]);
+        t.cy[k] = pack2(cy[2 * k], cy[2 * k + 1]);
     }
-    // chroma 4:2:0: the same MV at 1/32 sample
-    const int fx = pu.mv_x & 31, fy = pu.mv_y & 31;
-    const int cx0 = x0 / 2, cy0 = y0 / 2;
-    const int ix = cx0 + (pu.mv_x >> 5), iy = cy0 + (pu.mv_y >> 5);
-    for (int c = 1; c < 3; c++) {
-        __syncthreads();
-        const int vw = min(tw / 2, (int)f.dst[c].width - cx0), vh = min(th / 2, (int)f.dst[c].height - cy0);
-        interp_tile<4, 8>(f.ref[c], f.dst[c], cx0, cy0, vw, vh, ix, iy, kChroma4[fx], kChroma4[fy], fx != 0, fy != 0,
-                          win, tmp);
+#pragma unroll
+    for (int k = 0; k < TAPS; k++) t.sy[k] = cy[k];
+}
+
+// Workgroup index -> work index such that the workgroups one XCD runs
+// (hardware ids k, k + 8, k + 16, ... go to XCD k) take a contiguous run of
+// items: neighbouring tiles of a PU then share that XCD's L2.
+__device__ __forceinline__ uint32_t xcd_local_block(uint32_t bid, uint32_t nb) {
+    const uint32_t x = bid & 7, q = nb >> 3, r = nb & 7;
+    return x * q + (x < r ? x : r) + (bid >> 3);
+}
+
+// Four work items per 256-lane workgroup, one per wave: the 16x16 luma tile
+// of a PU (smaller for 8-wide or 8-tall PUs) and its two chroma tiles. All
+// three reference windows are requested before any filtering, so a wave
+// waits for memory once per item. Item fields, frame planes and filter taps
+// are wave-uniform and read with scalar loads.
+__global__ __launch_bounds__(256) void mc_kernel(const rmh_mc_frame* __restrict__ frames,
+                                                 const McItem* __restrict__ items, uint32_t n_items) {
+    __shared__ __align__(16) WaveLds lds[4];
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const uint32_t item = xcd_local_block(blockIdx.x, gridDim.x) * 4 + wave;
+    const bool active = item < n_items;
+    WaveLds& L = lds[wave];
+    Tile<8> tl{};
+    Tile<4> tc{};
+    const rmh_mc_frame* f = frames;
+    int bd = 8;
+    if (active) {
+        const McItem it = items[item];
+        f = &frames[it.frame];
+        const bool alt = it.flags & 1;
+        // luma: 1/16 sample
+        const int fx = it.mv_x & 15, fy = it.mv_y & 15;
+        tl.x0 = it.x0;
+        tl.y0 = it.y0;
+        tl.tw = it.tw;
+        tl.th = it.th;
+        tl.ix = tl.x0 + (it.mv_x >> 4);
+        tl.iy = tl.y0 + (it.mv_y >> 4);
+        set_taps<8>(tl, (alt && fx == 8) ? kLumaAltHpel : kLuma8[fx], (alt && fy == 8) ? kLumaAltHpel : kLuma8[fy]);
+        tl.hf = fx != 0;
+        tl.vf = fy != 0;
+        // chroma 4:2:0: the same MV at 1/32 sample
+        const int cfx = it.mv_x & 31, cfy = it.mv_y & 31;
+        tc.x0 = it.x0 / 2;
+        tc.y0 = it.y0 / 2;
+        tc.tw = it.cw;
+        tc.th = it.ch;
+        tc.ix = tc.x0 + (it.mv_x >> 5);
+        tc.iy = tc.y0 + (it.mv_y >> 5);
+        set_taps<4>(tc, kChroma4[cfx], kChroma4[cfy]);
+        tc.hf = cfx != 0;
+        tc.vf = cfy != 0;
+        bd = f->ref[0].bit_depth;
+        if (tl.tw > 0 && tl.th > 0) stage<8, kLR, kLP, 32>(f->ref[0], tl, L.luma, lane);
+        if (tc.tw > 0 && tc.th > 0) {
+            stage<4, kCR, kCP, 16>(f->ref[1], tc, L.chroma[0], lane);
+            stage<4, kCR, kCP, 16>(f->ref[2], tc, L.chroma[1], lane);
+        }
+    }
+    __syncthreads();
+    const int head_room = 14 - bd;
+    const int shift1 = 6 - head_room;
+    if (active && tl.tw > 0 && tl.th > 0) hpass<8, kLP, 8>(tl, L.luma, L.ltmp, shift1, lane);
+    if (active && tc.tw > 0 && tc.th > 0) {
+        hpass<4, kCP, 4>(tc, L.chroma[0], L.ctmp[0], shift1, lane);
+        hpass<4, kCP, 4>(tc, L.chroma[1], L.ctmp[1], shift1, lane);
+    }
+    __syncthreads();
+    if (!active) return;
+    const int maxv = (1 << bd) - 1;
+    if (tl.tw > 0 && tl.th > 0) output<8, kLP, 16>(f->dst[0], tl, L.luma, L.ltmp, head_room, maxv, lane);
+    if (tc.tw > 0 && tc.th > 0) {
+        output<4, kCP, 8>(f->dst[1], tc, L.chroma[0], L.ctmp[0], head_room, maxv, lane);
+        output<4, kCP, 8>(f->dst[2], tc, L.chroma[1], L.ctmp[1], head_room, maxv, lane);
     }
 }
 
 }  // namespace
 
-hipError_t launch_mc(const rmh_mc_frame* frames, const rmh_pu_desc* pus, const McItem* items, uint32_t n_items,
-                     hipStream_t stream) {
+hipError_t launch_mc(const rmh_mc_frame* frames, const McItem* items, uint32_t n_items, hipStream_t stream) {
     if (n_items == 0) return hipSuccess;
-    hipLaunchKernelGGL(mc_kernel, dim3(n_items), dim3(256), 0, stream, frames, pus, items);
+    hipLaunchKernelGGL(mc_kernel, dim3((n_items + 3) / 4), dim3(256), 0, stream, frames, items, n_items);
     return hipGetLastError();
 }
 

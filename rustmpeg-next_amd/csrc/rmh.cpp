@@ -408,10 +408,12 @@ uint64_t residual_bytes(const rmh_tu_desc* tus, size_t n_tu) {
     return 4 * s + 16 * (uint64_t)n_tu;
 }
 
+// Planes up to 65535 x 65535 samples whose rows span less than 4 GiB
+// (kernels address samples with 32-bit offsets).
 bool plane_ok(const rmh_plane& p) {
     return p.data && p.width > 0 && p.height > 0 && p.stride >= p.width && (p.bytes_per_sample == 1 || p.bytes_per_sample == 2) &&
            p.bit_depth >= 8 && p.bit_depth <= 12 && (p.bytes_per_sample == 2 || p.bit_depth == 8) && p.width <= 65535 &&
-           p.height <= 65535;
+           p.height <= 65535 && (uint64_t)p.stride * p.height * p.bytes_per_sample <= 0xffffffffull;
 }
 
 rmh_status validate_mc(rmh_ctx* ctx, const rmh_mc_frame* frames, size_t n_frames, const rmh_pu_desc* pus, size_t n_pu) {
@@ -466,12 +468,35 @@ rmh_status validate_dbk(rmh_ctx* ctx, const rmh_dbk_frame* frames, size_t n_fram
     return RMH_OK;
 }
 
-std::vector<rmh::McItem> mc_items(const rmh_pu_desc* pus, size_t n_pu) {
+// 16x16 luma tiles of every PU (PU order, raster inside a PU), clipped to the
+// picture; tiles wholly outside it are dropped.
+std::vector<rmh::McItem> mc_items(const rmh_mc_frame* frames, const rmh_pu_desc* pus, size_t n_pu) {
     std::vector<rmh::McItem> items;
     for (size_t i = 0; i < n_pu; i++) {
-        int w = 1 << (pus[i].log2_w & 15), h = 1 << (pus[i].log2_h & 15);
+        const rmh_pu_desc& pu = pus[i];
+        const rmh_mc_frame& f = frames[pu.frame];
+        const int w = 1 << (pu.log2_w & 15), h = 1 << (pu.log2_h & 15);
         for (int ty = 0; ty < (h + 15) / 16; ty++)
-            for (int tx = 0; tx < (w + 15) / 16; tx++) items.push_back({(uint32_t)i, (uint16_t)tx, (uint16_t)ty});
+            for (int tx = 0; tx < (w + 15) / 16; tx++) {
+                const int x0 = pu.x + 16 * tx, y0 = pu.y + 16 * ty;
+                const int tw = std::max(0, std::min({16, w - 16 * tx, (int)f.dst[0].width - x0}));
+                const int th = std::max(0, std::min({16, h - 16 * ty, (int)f.dst[0].height - y0}));
+                const int cw = std::max(0, std::min(std::min(16, w - 16 * tx) / 2, (int)f.dst[1].width - x0 / 2));
+                const int ch = std::max(0, std::min(std::min(16, h - 16 * ty) / 2, (int)f.dst[1].height - y0 / 2));
+                if ((tw == 0 || th == 0) && (cw == 0 || ch == 0)) continue;
+                rmh::McItem it{};  // a kept tile has x0 <= width and y0 <= height, so they fit 16 bits
+                it.x0 = (uint16_t)x0;
+                it.y0 = (uint16_t)y0;
+                it.tw = (uint8_t)tw;
+                it.th = (uint8_t)th;
+                it.cw = (uint8_t)cw;
+                it.ch = (uint8_t)ch;
+                it.frame = pu.frame;
+                it.flags = (pu.log2_w & RMH_PU_ALT_HPEL) ? 1 : 0;
+                it.mv_x = pu.mv_x;
+                it.mv_y = pu.mv_y;
+                items.push_back(it);
+            }
     }
     return items;
 }
@@ -566,7 +591,7 @@ rmh_status run_plan_locked_free(rmh_plan* plan, const int16_t* d_coeff, int16_t*
             e = rmh::launch_residual(a, s);
         }
     } else if (plan->kind == kMc) {
-        e = rmh::launch_mc(plan->d_frames, plan->d_pus, plan->d_items, (uint32_t)plan->n_items, s);
+        e = rmh::launch_mc(plan->d_frames, plan->d_items, (uint32_t)plan->n_items, s);
     } else {
         e = rmh::launch_deblock(plan->d_dbk, (uint32_t)plan->n_dbk, plan->max_cells, s);
     }
@@ -833,7 +858,7 @@ RMH_EXPORT rmh_status rmh_mc_plan_create(rmh_ctx* ctx, const rmh_mc_frame* frame
     }
     if (ctx->disabled.load()) return fail(ctx, RMH_UNAVAILABLE, "accelerator disabled");
     DeviceGuard dg(ctx->device);
-    std::vector<rmh::McItem> items = mc_items(pus, n_pu);
+    std::vector<rmh::McItem> items = mc_items(frames, pus, n_pu);
     if (items.size() > 0x7fffffffu) return fail(ctx, RMH_LIMIT_EXCEEDED, "too many inter work items");
     size_t fb = n_frames * sizeof(rmh_mc_frame), pb = n_pu * sizeof(rmh_pu_desc), ib = items.size() * sizeof(rmh::McItem);
     rmh_plan* p = new rmh_plan();
@@ -874,7 +899,7 @@ RMH_EXPORT rmh_status rmh_mc_batch(rmh_ctx* ctx, const rmh_mc_frame* frames, siz
     rmh_status g = gate(ctx);
     if (g != RMH_OK) return fail_locked(ctx, g, g == RMH_UNAVAILABLE ? "accelerator disabled" : "injected device failure");
     DeviceGuard dg(ctx->device);
-    std::vector<rmh::McItem> items = mc_items(pus, n_pu);
+    std::vector<rmh::McItem> items = mc_items(frames, pus, n_pu);
     size_t fb = n_frames * sizeof(rmh_mc_frame), pb = n_pu * sizeof(rmh_pu_desc), ib = items.size() * sizeof(rmh::McItem);
     size_t need = aligned(fb) + aligned(pb) + aligned(ib), hneed = 0;
     for (size_t f = 0; f < n_frames; f++)
@@ -909,7 +934,7 @@ RMH_EXPORT rmh_status rmh_mc_batch(rmh_ctx* ctx, const rmh_mc_frame* frames, siz
         (e = hipMemcpyAsync(d_pus, pus, pb, hipMemcpyHostToDevice, s)) != hipSuccess ||
         (e = hipMemcpyAsync(d_items, items.data(), ib, hipMemcpyHostToDevice, s)) != hipSuccess)
         return hip_fail_locked(ctx, e, "upload");
-    e = rmh::launch_mc(d_frames, d_pus, d_items, (uint32_t)items.size(), s);
+    e = rmh::launch_mc(d_frames, d_items, (uint32_t)items.size(), s);
     if (e != hipSuccess) return hip_fail_locked(ctx, e, "kernel launch");
     std::vector<void*> hdst;
     for (size_t f = 0; f < n_frames; f++)
