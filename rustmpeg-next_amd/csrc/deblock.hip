@@ -5,14 +5,10 @@
 // apply_deblock (vvc/decode_general.rs:4977-5187): every vertical edge of a
 // picture, then every horizontal edge.
 //
-// Mapping: one lane per 4-sample edge segment (luma) or two-line decision
-// group (chroma); one launch for all vertical edges of all pictures, a second
-// for all horizontal edges. Inside one direction the reference's sequential
-// order is equivalent to edge-parallel execution: with the VVC filter lengths
-// a segment writes at most maxFilterLength samples per side and reads at most
-// one more, so no segment reads what another segment of the same direction
-// writes. Each lane therefore writes back only the samples its filter may
-// modify.
+// Mapping: a copy of every plane into a shadow, then one workgroup per 64x32
+// luma tile (with its 32x16 chroma tiles) that filters both edge directions
+// in LDS from the shadow and writes the tile back once (deblock_tile_kernel
+// below states why this equals the reference order).
 #include <hip/hip_runtime.h>
 
 #include "kernels.h"
@@ -31,18 +27,6 @@ __constant__ int16_t kBeta[64] = {0,  0,  0,  0,  0,  0,  0,  0,  0,  0,  0,  0,
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : v > hi ? hi : v; }
 __device__ __forceinline__ int iabs(int v) { return v < 0 ? -v : v; }
-
-__device__ __forceinline__ int ld(const rmh_plane& p, int x, int y) {
-    size_t o = (size_t)y * p.stride + x;
-    return p.bytes_per_sample == 1 ? (int)((const uint8_t*)p.data)[o] : (int)((const uint16_t*)p.data)[o];
-}
-__device__ __forceinline__ void st(const rmh_plane& p, int x, int y, int v) {
-    size_t o = (size_t)y * p.stride + x;
-    if (p.bytes_per_sample == 1)
-        ((uint8_t*)p.data)[o] = (uint8_t)v;
-    else
-        ((uint16_t*)p.data)[o] = (uint16_t)v;
-}
 
 __device__ __forceinline__ void tc_beta(int qp, int bs, int beta_off, int tc_off, int bd, int& tc, int& beta) {
     int it = clampi(qp + 2 * (bs - 1) + 2 * tc_off, 0, 65);
@@ -64,17 +48,32 @@ __device__ __forceinline__ bool dsam(int p0, int p3, int q0, int q3, int dpq, in
 #define PP(l, i) (l)[7 - (i)]
 #define QQ(l, j) (l)[8 + (j)]
 
-// Filters one luma segment held as L[4][16] = p7..p0 q0..q7 in registers.
-// Returns the number of P and Q samples that may have changed (0 = none).
-__device__ __forceinline__ void luma_segment(int (*L)[16], int qp, int bs, int beta_off, int tc_off, int bd, int len_p, int len_q,
-                             bool at_ctb, int& np_out, int& nq_out) {
-    np_out = nq_out = 0;
-    const int maxv = (1 << bd) - 1;
+// One of three values by a filter length n in {3, 5, 7}, with masks rather
+// than a select of array elements (which the compiler turns into an indexed
+// load and moves the line to scratch memory).
+__device__ __forceinline__ int by_len(int n, int v3, int v5, int v7) {
+    return (-(int)(n == 3) & v3) | (-(int)(n == 5) & v5) | (-(int)(n == 7) & v7);
+}
+// p_n / q_n and p_(n-1) / q_(n-1) of a line for n in {3, 5, 7}
+__device__ __forceinline__ int p_at(const int* l, int n) { return by_len(n, PP(l, 3), PP(l, 5), PP(l, 7)); }
+__device__ __forceinline__ int q_at(const int* l, int n) { return by_len(n, QQ(l, 3), QQ(l, 5), QQ(l, 7)); }
+__device__ __forceinline__ int p_before(const int* l, int n) { return by_len(n, PP(l, 2), PP(l, 4), PP(l, 6)); }
+__device__ __forceinline__ int q_before(const int* l, int n) { return by_len(n, QQ(l, 2), QQ(l, 4), QQ(l, 6)); }
+
+// Decision of one 4-line luma segment (deblock_luma_edge_bd, vvc/filter.rs:638-679,
+// decide_luma_edge, vvc/filter.rs:311-468) from its lines 0 and 3.
+struct LumaDecision {
+    int de;        // 1 weak, 2 strong short, 3 long
+    int lp, lq;    // filter lengths in use
+    bool dep, deq; // weak filter also changes p1 / q1
+    int tc;
+};
+
+__device__ __forceinline__ bool luma_decide(const int* l0, const int* l3, int qp, int bs, int beta_off, int tc_off, int bd,
+                                            int len_p, int len_q, bool at_ctb, LumaDecision& dd) {
     int tc, beta;
     tc_beta(qp, bs, beta_off, tc_off, bd, tc, beta);
-    if (beta == 0 || tc == 0) return;
-    const int* l0 = L[0];
-    const int* l3 = L[3];
+    if (beta == 0 || tc == 0) return false;
     int dp0 = iabs(PP(l0, 2) - 2 * PP(l0, 1) + PP(l0, 0)), dp3 = iabs(PP(l3, 2) - 2 * PP(l3, 1) + PP(l3, 0));
     int dq0 = iabs(QQ(l0, 2) - 2 * QQ(l0, 1) + QQ(l0, 0)), dq3 = iabs(QQ(l3, 2) - 2 * QQ(l3, 1) + QQ(l3, 0));
     int dpq0 = dp0 + dq0, dpq3 = dp3 + dq3;
@@ -98,15 +97,15 @@ __device__ __forceinline__ void luma_segment(int (*L)[16], int qp, int bs, int b
         int sq3l = lq == 7 ? sq3 + iabs(QQ(l3, 4) - QQ(l3, 5) - QQ(l3, 6) + QQ(l3, 7)) : sq3;
         int d0l = dp0l + dq0l, d3l = dp3l + dq3l;
         if (d0l + d3l < beta) {
-            bool s0 = dsam(pl ? PP(l0, lp) : 0, pl ? PP(l0, 3) : 0, ql ? QQ(l0, lq) : 0, ql ? QQ(l0, 3) : 0, 2 * d0l,
+            bool s0 = dsam(pl ? p_at(l0, lp) : 0, pl ? PP(l0, 3) : 0, ql ? q_at(l0, lq) : 0, ql ? QQ(l0, 3) : 0, 2 * d0l,
                            sp0l, sq0l, spq0, pl, ql, beta, tc);
-            bool s3 = dsam(pl ? PP(l3, lp) : 0, pl ? PP(l3, 3) : 0, ql ? QQ(l3, lq) : 0, ql ? QQ(l3, 3) : 0, 2 * d3l,
+            bool s3 = dsam(pl ? p_at(l3, lp) : 0, pl ? PP(l3, 3) : 0, ql ? q_at(l3, lq) : 0, ql ? QQ(l3, 3) : 0, 2 * d3l,
                            sp3l, sq3l, spq3, pl, ql, beta, tc);
             if (s0 && s3) de = 3;
         }
     }
     if (de != 3) {
-        if (dpq0 + dpq3 >= beta) return;
+        if (dpq0 + dpq3 >= beta) return false;
         bool s0 = false, s3 = false;
         if (lp > 2 && lq > 2) {
             s0 = dsam(0, 0, 0, 0, 2 * dpq0, sp0, sq0, spq0, false, false, beta, tc);
@@ -122,82 +121,95 @@ __device__ __forceinline__ void luma_segment(int (*L)[16], int qp, int bs, int b
         dep = lp > 1 && lq > 1 && (dp0 + dp3) < thr;
         deq = lp > 1 && lq > 1 && (dq0 + dq3) < thr;
     }
+    dd.de = de;
+    dd.lp = lp;
+    dd.lq = lq;
+    dd.dep = dep;
+    dd.deq = deq;
+    dd.tc = tc;
+    return true;
+}
+
+// Filters one line (p7..p0 q0..q7) of a segment with the segment's decision
+// (long_tap_luma_line / short_filter_luma_line, vvc/filter.rs:477-631).
+__device__ __forceinline__ void luma_filter_line(int* line, const LumaDecision& dd, int maxv) {
+    const int de = dd.de, tc = dd.tc;
+    const int lp = dd.lp, lq = dd.lq;
+    const bool dep = dd.dep, deq = dd.deq;
+    int m[16];
 #pragma unroll
-    for (int li = 0; li < 4; li++) {
-        int* line = L[li];
-        int m[16];
+    for (int k = 0; k < 16; k++) m[k] = line[k];
+    if (de == 3) {
+        // long_tap_luma_line (vvc/filter.rs:477-582)
+        const int np = lp, nq = lq;
+        int rm;
+        if (np == 5 && nq == 5)
+            rm = (2 * (PP(m, 0) + QQ(m, 0) + PP(m, 1) + QQ(m, 1) + PP(m, 2) + QQ(m, 2)) + PP(m, 3) + QQ(m, 3) +
+                  PP(m, 4) + QQ(m, 4) + 8) >> 4;
+        else if ((np == 7 && nq == 5) || (np == 5 && nq == 7))
+            rm = (2 * (PP(m, 0) + QQ(m, 0) + PP(m, 1) + QQ(m, 1)) + PP(m, 2) + QQ(m, 2) + PP(m, 3) + QQ(m, 3) +
+                  PP(m, 4) + QQ(m, 4) + PP(m, 5) + QQ(m, 5) + 8) >> 4;
+        else if ((np == 5 && nq == 3) || (np == 3 && nq == 5))
+            rm = (PP(m, 0) + QQ(m, 0) + PP(m, 1) + QQ(m, 1) + PP(m, 2) + QQ(m, 2) + PP(m, 3) + QQ(m, 3) + 4) >> 3;
+        else if (np == 7 && nq == 3)
+            rm = (2 * (PP(m, 0) + QQ(m, 0)) + QQ(m, 0) + 2 * (QQ(m, 1) + QQ(m, 2)) + PP(m, 1) + QQ(m, 1) +
+                  PP(m, 2) + PP(m, 3) + PP(m, 4) + PP(m, 5) + PP(m, 6) + 8) >> 4;
+        else if (np == 3 && nq == 7)
+            rm = (2 * (PP(m, 0) + QQ(m, 0)) + PP(m, 0) + 2 * (PP(m, 1) + PP(m, 2)) + PP(m, 1) + QQ(m, 1) +
+                  QQ(m, 2) + QQ(m, 3) + QQ(m, 4) + QQ(m, 5) + QQ(m, 6) + 8) >> 4;
+        else
+            rm = (2 * (PP(m, 0) + QQ(m, 0)) + PP(m, 1) + QQ(m, 1) + PP(m, 2) + QQ(m, 2) + PP(m, 3) + QQ(m, 3) +
+                  PP(m, 4) + QQ(m, 4) + PP(m, 5) + QQ(m, 5) + PP(m, 6) + QQ(m, 6) + 8) >> 4;
+        int refp = (p_at(m, np) + p_before(m, np) + 1) >> 1;
+        int refq = (q_at(m, nq) + q_before(m, nq) + 1) >> 1;
 #pragma unroll
-        for (int k = 0; k < 16; k++) m[k] = line[k];
-        if (de == 3) {
-            // long_tap_luma_line (vvc/filter.rs:477-582)
-            const int np = lp, nq = lq;
-            int rm;
-            if (np == 5 && nq == 5)
-                rm = (2 * (PP(m, 0) + QQ(m, 0) + PP(m, 1) + QQ(m, 1) + PP(m, 2) + QQ(m, 2)) + PP(m, 3) + QQ(m, 3) +
-                      PP(m, 4) + QQ(m, 4) + 8) >> 4;
-            else if ((np == 7 && nq == 5) || (np == 5 && nq == 7))
-                rm = (2 * (PP(m, 0) + QQ(m, 0) + PP(m, 1) + QQ(m, 1)) + PP(m, 2) + QQ(m, 2) + PP(m, 3) + QQ(m, 3) +
-                      PP(m, 4) + QQ(m, 4) + PP(m, 5) + QQ(m, 5) + 8) >> 4;
-            else if ((np == 5 && nq == 3) || (np == 3 && nq == 5))
-                rm = (PP(m, 0) + QQ(m, 0) + PP(m, 1) + QQ(m, 1) + PP(m, 2) + QQ(m, 2) + PP(m, 3) + QQ(m, 3) + 4) >> 3;
-            else if (np == 7 && nq == 3)
-                rm = (2 * (PP(m, 0) + QQ(m, 0)) + QQ(m, 0) + 2 * (QQ(m, 1) + QQ(m, 2)) + PP(m, 1) + QQ(m, 1) +
-                      PP(m, 2) + PP(m, 3) + PP(m, 4) + PP(m, 5) + PP(m, 6) + 8) >> 4;
-            else if (np == 3 && nq == 7)
-                rm = (2 * (PP(m, 0) + QQ(m, 0)) + PP(m, 0) + 2 * (PP(m, 1) + PP(m, 2)) + PP(m, 1) + QQ(m, 1) +
-                      QQ(m, 2) + QQ(m, 3) + QQ(m, 4) + QQ(m, 5) + QQ(m, 6) + 8) >> 4;
-            else
-                rm = (2 * (PP(m, 0) + QQ(m, 0)) + PP(m, 1) + QQ(m, 1) + PP(m, 2) + QQ(m, 2) + PP(m, 3) + QQ(m, 3) +
-                      PP(m, 4) + QQ(m, 4) + PP(m, 5) + QQ(m, 5) + PP(m, 6) + QQ(m, 6) + 8) >> 4;
-            int refp = (PP(m, np) + PP(m, np - 1) + 1) >> 1;
-            int refq = (QQ(m, nq) + QQ(m, nq - 1) + 1) >> 1;
-#pragma unroll
-            for (int pos = 0; pos < 7; pos++) {
-                if (pos < np) {
-                    int f = np == 7 ? 59 - 9 * pos : np == 5 ? (pos == 0 ? 58 : pos == 1 ? 45 : pos == 2 ? 32 : pos == 3 ? 19 : 6)
-                                                             : (pos == 0 ? 53 : pos == 1 ? 32 : 11);
-                    int t = np == 7 ? (pos < 5 ? 6 - pos : 1) : np == 5 ? 6 - pos : 6 - 2 * pos;
-                    int o = PP(m, pos), cv = (tc * t) >> 1;
-                    int v = (rm * f + refp * (64 - f) + 32) >> 6;
-                    line[7 - pos] = clampi(clampi(v, o - cv, o + cv), 0, maxv);
-                }
-                if (pos < nq) {
-                    int f = nq == 7 ? 59 - 9 * pos : nq == 5 ? (pos == 0 ? 58 : pos == 1 ? 45 : pos == 2 ? 32 : pos == 3 ? 19 : 6)
-                                                             : (pos == 0 ? 53 : pos == 1 ? 32 : 11);
-                    int t = nq == 7 ? (pos < 5 ? 6 - pos : 1) : nq == 5 ? 6 - pos : 6 - 2 * pos;
-                    int o = QQ(m, pos), cv = (tc * t) >> 1;
-                    int v = (rm * f + refq * (64 - f) + 32) >> 6;
-                    line[8 + pos] = clampi(clampi(v, o - cv, o + cv), 0, maxv);
-                }
+        for (int pos = 0; pos < 7; pos++) {
+            if (pos < np) {
+                int f = np == 7 ? 59 - 9 * pos : np == 5 ? (pos == 0 ? 58 : pos == 1 ? 45 : pos == 2 ? 32 : pos == 3 ? 19 : 6)
+                                                         : (pos == 0 ? 53 : pos == 1 ? 32 : 11);
+                int t = np == 7 ? (pos < 5 ? 6 - pos : 1) : np == 5 ? 6 - pos : 6 - 2 * pos;
+                int o = PP(m, pos), cv = (tc * t) >> 1;
+                int v = (rm * f + refp * (64 - f) + 32) >> 6;
+                line[7 - pos] = clampi(clampi(v, o - cv, o + cv), 0, maxv);
             }
-        } else if (de == 2) {
-            // strong short filter (vvc/filter.rs:590-611)
-            line[7] = clampi((PP(m, 2) + 2 * PP(m, 1) + 2 * PP(m, 0) + 2 * QQ(m, 0) + QQ(m, 1) + 4) >> 3,
-                             PP(m, 0) - 3 * tc, PP(m, 0) + 3 * tc);
-            line[6] = clampi((PP(m, 2) + PP(m, 1) + PP(m, 0) + QQ(m, 0) + 2) >> 2, PP(m, 1) - 2 * tc, PP(m, 1) + 2 * tc);
-            line[5] = clampi((2 * PP(m, 3) + 3 * PP(m, 2) + PP(m, 1) + PP(m, 0) + QQ(m, 0) + 4) >> 3, PP(m, 2) - tc,
-                             PP(m, 2) + tc);
-            line[8] = clampi((PP(m, 1) + 2 * PP(m, 0) + 2 * QQ(m, 0) + 2 * QQ(m, 1) + QQ(m, 2) + 4) >> 3,
-                             QQ(m, 0) - 3 * tc, QQ(m, 0) + 3 * tc);
-            line[9] = clampi((PP(m, 0) + QQ(m, 0) + QQ(m, 1) + QQ(m, 2) + 2) >> 2, QQ(m, 1) - 2 * tc, QQ(m, 1) + 2 * tc);
-            line[10] = clampi((PP(m, 0) + QQ(m, 0) + QQ(m, 1) + 3 * QQ(m, 2) + 2 * QQ(m, 3) + 4) >> 3, QQ(m, 2) - tc,
-                              QQ(m, 2) + tc);
-        } else {
-            // weak filter (vvc/filter.rs:613-630)
-            int delta = (9 * (QQ(m, 0) - PP(m, 0)) - 3 * (QQ(m, 1) - PP(m, 1)) + 8) >> 4;
-            if (iabs(delta) < tc * 10) {
-                delta = clampi(delta, -tc, tc);
-                line[7] = clampi(PP(m, 0) + delta, 0, maxv);
-                line[8] = clampi(QQ(m, 0) - delta, 0, maxv);
-                int tc2 = tc >> 1;
-                if (dep) line[6] = clampi(PP(m, 1) + clampi((((PP(m, 2) + PP(m, 0) + 1) >> 1) - PP(m, 1) + delta) >> 1, -tc2, tc2), 0, maxv);
-                if (deq) line[9] = clampi(QQ(m, 1) + clampi((((QQ(m, 2) + QQ(m, 0) + 1) >> 1) - QQ(m, 1) - delta) >> 1, -tc2, tc2), 0, maxv);
+            if (pos < nq) {
+                int f = nq == 7 ? 59 - 9 * pos : nq == 5 ? (pos == 0 ? 58 : pos == 1 ? 45 : pos == 2 ? 32 : pos == 3 ? 19 : 6)
+                                                         : (pos == 0 ? 53 : pos == 1 ? 32 : 11);
+                int t = nq == 7 ? (pos < 5 ? 6 - pos : 1) : nq == 5 ? 6 - pos : 6 - 2 * pos;
+                int o = QQ(m, pos), cv = (tc * t) >> 1;
+                int v = (rm * f + refq * (64 - f) + 32) >> 6;
+                line[8 + pos] = clampi(clampi(v, o - cv, o + cv), 0, maxv);
             }
         }
+    } else if (de == 2) {
+        // strong short filter (vvc/filter.rs:590-611)
+        line[7] = clampi((PP(m, 2) + 2 * PP(m, 1) + 2 * PP(m, 0) + 2 * QQ(m, 0) + QQ(m, 1) + 4) >> 3,
+                         PP(m, 0) - 3 * tc, PP(m, 0) + 3 * tc);
+        line[6] = clampi((PP(m, 2) + PP(m, 1) + PP(m, 0) + QQ(m, 0) + 2) >> 2, PP(m, 1) - 2 * tc, PP(m, 1) + 2 * tc);
+        line[5] = clampi((2 * PP(m, 3) + 3 * PP(m, 2) + PP(m, 1) + PP(m, 0) + QQ(m, 0) + 4) >> 3, PP(m, 2) - tc,
+                         PP(m, 2) + tc);
+        line[8] = clampi((PP(m, 1) + 2 * PP(m, 0) + 2 * QQ(m, 0) + 2 * QQ(m, 1) + QQ(m, 2) + 4) >> 3,
+                         QQ(m, 0) - 3 * tc, QQ(m, 0) + 3 * tc);
+        line[9] = clampi((PP(m, 0) + QQ(m, 0) + QQ(m, 1) + QQ(m, 2) + 2) >> 2, QQ(m, 1) - 2 * tc, QQ(m, 1) + 2 * tc);
+        line[10] = clampi((PP(m, 0) + QQ(m, 0) + QQ(m, 1) + 3 * QQ(m, 2) + 2 * QQ(m, 3) + 4) >> 3, QQ(m, 2) - tc,
+                          QQ(m, 2) + tc);
+    } else {
+        // weak filter (vvc/filter.rs:613-630)
+        int delta = (9 * (QQ(m, 0) - PP(m, 0)) - 3 * (QQ(m, 1) - PP(m, 1)) + 8) >> 4;
+        if (iabs(delta) < tc * 10) {
+            delta = clampi(delta, -tc, tc);
+            line[7] = clampi(PP(m, 0) + delta, 0, maxv);
+            line[8] = clampi(QQ(m, 0) - delta, 0, maxv);
+            int tc2 = tc >> 1;
+            if (dep) line[6] = clampi(PP(m, 1) + clampi((((PP(m, 2) + PP(m, 0) + 1) >> 1) - PP(m, 1) + delta) >> 1, -tc2, tc2), 0, maxv);
+            if (deq) line[9] = clampi(QQ(m, 1) + clampi((((QQ(m, 2) + QQ(m, 0) + 1) >> 1) - QQ(m, 1) - delta) >> 1, -tc2, tc2), 0, maxv);
+        }
     }
-    np_out = de == 3 ? lp : de == 2 ? 3 : 2;
-    nq_out = de == 3 ? lq : de == 2 ? 3 : 2;
 }
+
+// Samples per side that a decision may change.
+__device__ __forceinline__ int luma_np(const LumaDecision& d) { return d.de == 3 ? d.lp : d.de == 2 ? 3 : 2; }
+__device__ __forceinline__ int luma_nq(const LumaDecision& d) { return d.de == 3 ? d.lq : d.de == 2 ? 3 : 2; }
 
 // deblock_chroma_edge with two lines (vvc/filter.rs:935-1057). Returns the
 // number of samples per side that may have changed.
@@ -255,76 +267,382 @@ __device__ __forceinline__ int chroma_segment(int (*L)[8], int qp, int bs, int b
     return 3;
 }
 
-// One lane per edge segment of every plane of every frame; `dir` 0 = vertical
-// edges, 1 = horizontal edges.
-__global__ __launch_bounds__(256) void deblock_kernel(const rmh_dbk_frame* __restrict__ frames, int dir) {
-    const rmh_dbk_frame& f = frames[blockIdx.y];
-    const int plane = blockIdx.z;
-    const uint32_t cell = blockIdx.x * blockDim.x + threadIdx.x;
-    if (cell >= f.grid_w * f.grid_h) return;
-    const int i = cell % f.grid_w, j = cell / f.grid_w;
-    const rmh_plane& P = f.planes[plane];
-    const int w = P.width, h = P.height, bd = P.bit_depth;
-    const uint8_t e = f.edges[(size_t)cell * 4 + (plane == 0 ? 0 : 2) + dir];
-    const int bs = e & RMH_EDGE_BS_MASK;
-    if (!bs) return;
-    const int len_p = 2 * ((e >> RMH_EDGE_LENP_SHIFT) & 3) + 1, len_q = 2 * ((e >> RMH_EDGE_LENQ_SHIFT) & 3) + 1;
-    const int qp = f.qp[plane], boff = f.beta_offset_div2[plane], toff = f.tc_offset_div2[plane];
-    if (plane == 0) {
-        const int x = 4 * i, y = 4 * j;
-        if ((dir == 0 && (i == 0 || x >= w || y >= h)) || (dir == 1 && (j == 0 || y >= h || x >= w))) return;
-        int L[4][16];
+// ---------------------------------------------------------------------------
+// Copy of every plane into its shadow (the tiled kernel reads the shadow and
+// writes the picture, so no workgroup reads samples another one has written).
+// Rows are copied in the widest unit that the two layouts allow.
+__global__ __launch_bounds__(256) void shadow_copy_kernel(const rmh_dbk_frame* __restrict__ frames,
+                                                          const rmh_plane* __restrict__ shadows) {
+    const rmh_plane& src = frames[blockIdx.y].planes[blockIdx.z];
+    const rmh_plane& dst = shadows[blockIdx.y * 3 + blockIdx.z];
+    const uint32_t row_bytes = src.width * src.bytes_per_sample;
+    const uint32_t sp = src.stride * src.bytes_per_sample, dp = dst.stride * dst.bytes_per_sample;
+    const uintptr_t al = (uintptr_t)src.data | (uintptr_t)dst.data | sp | dp | row_bytes;
+    for (uint32_t y = blockIdx.x * 4 + (threadIdx.x >> 6); y < src.height; y += gridDim.x * 4) {
+        const uint8_t* s = (const uint8_t*)src.data + (size_t)y * sp;
+        uint8_t* d = (uint8_t*)dst.data + (size_t)y * dp;
+        const uint32_t lane = threadIdx.x & 63;
+        if ((al & 15) == 0) {
+            for (uint32_t o = lane * 16; o < row_bytes; o += 1024)
+                *reinterpret_cast<uint4*>(d + o) = *reinterpret_cast<const uint4*>(s + o);
+        } else if ((al & 7) == 0) {
+            for (uint32_t o = lane * 8; o < row_bytes; o += 512)
+                *reinterpret_cast<uint2*>(d + o) = *reinterpret_cast<const uint2*>(s + o);
+        } else if ((al & 3) == 0) {
+            for (uint32_t o = lane * 4; o < row_bytes; o += 256)
+                *reinterpret_cast<uint32_t*>(d + o) = *reinterpret_cast<const uint32_t*>(s + o);
+        } else {
+            for (uint32_t o = lane; o < row_bytes; o += 64) d[o] = s[o];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Tiled deblocking: one workgroup per 64x32 luma tile of a frame and the
+// matching 32x16 tiles of both chroma planes.
+//
+//   1. the tile plus an 8-sample luma (4-sample chroma) halo, clamped at the
+//      picture edges, is read from the shadow into LDS;
+//   2. every vertical edge segment that can change a sample the tile or its
+//      horizontal pass reads is filtered in LDS (rows of the halo included);
+//   3. rows outside the picture are re-filled from the nearest picture row,
+//      as the reference's clamped gather reads them;
+//   4. every horizontal edge segment that can change a tile sample is
+//      filtered in LDS;
+//   5. the tile is written to the picture, each sample once.
+// Segments of one direction touch disjoint samples for VVC filter lengths
+// (length 1 on edges off the 8-sample grid, at most 3 (chroma) / 7 (luma) on
+// it), so filtering them in parallel equals the reference's sequential order,
+// and a tile's halo segments compute exactly what the neighbouring tile
+// computes for the same samples.
+// ---------------------------------------------------------------------------
+
+constexpr int kTW = 64, kTH = 32;                      // luma tile
+constexpr int kLWW = kTW + 16, kLWH = kTH + 16;        // luma window (8 halo each side)
+constexpr int kCTW = kTW / 2, kCTH = kTH / 2;          // chroma tile
+constexpr int kCWW = kCTW + 8, kCWH = kCTH + 8;        // chroma window (4 halo each side)
+
+__device__ __forceinline__ int ld32(const rmh_plane& p, int x, int y) {
+    const uint32_t o = (uint32_t)y * p.stride + (uint32_t)x;
+    return p.bytes_per_sample == 1 ? (int)((const uint8_t*)p.data)[o] : (int)((const uint16_t*)p.data)[o];
+}
+__device__ __forceinline__ void st32(const rmh_plane& p, int x, int y, int v) {
+    const uint32_t o = (uint32_t)y * p.stride + (uint32_t)x;
+    if (p.bytes_per_sample == 1)
+        ((uint8_t*)p.data)[o] = (uint8_t)v;
+    else
+        ((uint16_t*)p.data)[o] = (uint16_t)v;
+}
+
+// Window (wx0, wy0) .. + (WW, WH) of plane P read (clamped) from the shadow.
+// All of a lane's loads are issued before the first LDS write.
+template <int WW, int WH>
+__device__ __forceinline__ void load_window(const rmh_plane& sh, int wx0, int wy0, int16_t* win) {
+    constexpr int N = (WW * WH + 255) / 256;
+    const int w = sh.width, h = sh.height;
+    int v[N];
 #pragma unroll
-        for (int li = 0; li < 4; li++)
+    for (int i = 0; i < N; i++) {
+        const int e = threadIdx.x + 256 * i;
+        const int r = e / WW, c = e - r * WW;
+        v[i] = e < WW * WH ? ld32(sh, clampi(wx0 + c, 0, w - 1), clampi(wy0 + r, 0, h - 1)) : 0;
+    }
 #pragma unroll
-            for (int k = 0; k < 16; k++)
-                L[li][k] = dir == 0 ? ld(P, clampi(x - 8 + k, 0, w - 1), min(y + li, h - 1))
-                                    : ld(P, min(x + li, w - 1), clampi(y - 8 + k, 0, h - 1));
-        int np, nq;
-        luma_segment(L, qp, bs, boff, toff, bd, len_p, len_q, dir == 1 && (e & RMH_EDGE_CTB_ROW), np, nq);
-        if (np == 0) return;
-#pragma unroll
-        for (int li = 0; li < 4; li++)
-#pragma unroll
-            for (int k = 1; k < 15; k++) {
-                if (k < 8 - np || k >= 8 + nq) continue;
-                if (dir == 0)
-                    st(P, clampi(x - 8 + k, 0, w - 1), min(y + li, h - 1), L[li][k]);
-                else
-                    st(P, min(x + li, w - 1), clampi(y - 8 + k, 0, h - 1), L[li][k]);
-            }
+    for (int i = 0; i < N; i++) {
+        const int e = threadIdx.x + 256 * i;
+        if (e < WW * WH) win[e] = (int16_t)v[i];
+    }
+}
+
+// Rows of the window outside the picture take the nearest picture row.
+template <int WW, int WH>
+__device__ __forceinline__ void refill_rows(int wy0, int h, int16_t* win) {
+    for (int e = threadIdx.x; e < WW * WH; e += 256) {
+        const int r = e / WW, c = e - r * WW, y = wy0 + r;
+        if (y < 0 || y >= h) win[e] = win[(clampi(y, 0, h - 1) - wy0) * WW + c];
+    }
+}
+
+// Geometry of one tile's segment slots. Vertical pass: luma x = X0 .. X0 + 64
+// step 4 over window rows in 4-row segments; chroma x = CX0 .. CX0 + 32 step
+// 2 over window rows in 2-row segments (the edge map's chroma cells are 2x2
+// chroma samples). Horizontal pass: luma y = Y0 .. Y0 + 32 step 4 over the
+// tile's columns in 4-column segments; chroma y = CY0 .. CY0 + 16 step 2 over
+// 2-column segments.
+constexpr int kLVX = kTW / 4 + 1, kLVY = kLWH / 4;   // 17 x 12
+constexpr int kCVX = kCTW / 2 + 1, kCVY = kCWH / 2;  // 17 x 12
+constexpr int kLHX = kTW / 4, kLHY = kTH / 4 + 1;    // 16 x 9
+constexpr int kCHX = kCTW / 2, kCHY = kCTH / 2 + 1;  // 16 x 9
+constexpr int kNV = kLVX * kLVY + 2 * kCVX * kCVY;
+constexpr int kNH = kLHX * kLHY + 2 * kCHX * kCHY;
+constexpr int kMaxQ = kNV > kNH ? kNV : kNH;
+
+// The edge-map cells a tile needs: luma cells x / 4 = X0 / 4 .. + 16 and
+// y / 4 = Y0 / 4 - 2 .. + 11 (chroma cells are the same cells: chroma
+// sample 2i is luma sample 4i).
+constexpr int kEX = kLVX, kEY = kLVY;  // 17 x 12
+
+struct TileCtx {
+    const rmh_dbk_frame* f;
+    const uint32_t* cells;  // LDS, kEY x kEX, the 4 edge bytes of a cell
+    int W, H, CW, CH, X0, Y0, CX0, CY0, bd;
+};
+
+__device__ __forceinline__ uint8_t cell_byte(const TileCtx& t, int cx, int cy, int b) {
+    return (uint8_t)(t.cells[(cy - (t.Y0 / 4 - 2)) * kEX + (cx - t.X0 / 4)] >> (8 * b));
+}
+
+// Slot -> (plane, edge position). Returns the edge byte, 0 when the slot has
+// no edge to filter (outside the picture or bS 0).
+__device__ __forceinline__ uint8_t slot_edge(const TileCtx& t, int dir, int s, int& plane, int& x, int& y) {
+    const int nl = dir == 0 ? kLVX * kLVY : kLHX * kLHY;
+    const int cx = dir == 0 ? kCVX : kCHX, cy = dir == 0 ? kCVY : kCHY;
+    if (s < nl) {
+        plane = 0;
+        if (dir == 0) {
+            x = t.X0 + 4 * (s % kLVX);
+            y = t.Y0 - 8 + 4 * (s / kLVX);
+            if (x == 0 || x >= t.W || y < 0 || y >= t.H) return 0;
+        } else {
+            x = t.X0 + 4 * (s % kLHX);
+            y = t.Y0 + 4 * (s / kLHX);
+            if (y == 0 || y >= t.H || x >= t.W) return 0;
+        }
+        const uint8_t e = cell_byte(t, x >> 2, y >> 2, dir);
+        return (e & RMH_EDGE_BS_MASK) ? e : 0;
+    }
+    const int u = s - nl, c = u / (cx * cy), v = u % (cx * cy);
+    plane = 1 + c;
+    if (dir == 0) {
+        x = t.CX0 + 2 * (v % kCVX);
+        y = t.CY0 - 4 + 2 * (v / kCVX);
+        if (x == 0 || x >= t.CW || y < 0 || y + 2 > t.CH) return 0;
     } else {
-        const int x = 2 * i, y = 2 * j;
-        if ((dir == 0 && (i == 0 || x >= w || y + 2 > h)) || (dir == 1 && (j == 0 || y >= h || x + 2 > w))) return;
-        int L[2][8];
+        x = t.CX0 + 2 * (v % kCHX);
+        y = t.CY0 + 2 * (v / kCHX);
+        if (y == 0 || y >= t.CH || x + 2 > t.CW) return 0;
+    }
+    const uint8_t e = cell_byte(t, x >> 1, y >> 1, 2 + dir);
+    return (e & RMH_EDGE_BS_MASK) ? e : 0;
+}
+
+// Filters one luma segment of the LDS window (dir 0: vertical edge at x,
+// rows y .. y + 3; dir 1: horizontal edge at y, columns x .. x + 3). Lines 0
+// and 3 are read for the decision, the others when filtered; a line whose
+// column repeats the previous one (clamped at the right picture edge) gives
+// the same result and is skipped.
+__device__ __forceinline__ void luma_slot(const TileCtx& t, int dir, int x, int y, uint8_t e, int16_t* lw) {
+    const rmh_dbk_frame& f = *t.f;
+    const int bs = e & RMH_EDGE_BS_MASK;
+    const int len_p = 2 * ((e >> RMH_EDGE_LENP_SHIFT) & 3) + 1, len_q = 2 * ((e >> RMH_EDGE_LENQ_SHIFT) & 3) + 1;
+    // element (line li, position k) at base[cols[li] + k * ks]
+    int16_t* base;
+    int ks;
+    int cols[4];
+    if (dir == 0) {
+        base = lw + (y - (t.Y0 - 8)) * kLWW + (x - t.X0);  // column x - 8 of the window row
+        ks = 1;
 #pragma unroll
-        for (int li = 0; li < 2; li++)
+        for (int li = 0; li < 4; li++) cols[li] = li * kLWW;
+    } else {
+        // line li is column x + li (clamped into the picture as the gather does); k runs down rows y - 8 ..
+        base = lw + (y - t.Y0) * kLWW + 8;
+        ks = kLWW;
 #pragma unroll
-            for (int k = 0; k < 8; k++)
-                L[li][k] = dir == 0 ? ld(P, clampi(x - 4 + k, 0, w - 1), min(y + li, h - 1))
-                                    : ld(P, min(x + li, w - 1), clampi(y - 4 + k, 0, h - 1));
-        if (!chroma_segment(L, qp, bs, boff, toff, len_p, len_q, bd)) return;
+        for (int li = 0; li < 4; li++) cols[li] = min(x + li, t.W - 1) - t.X0;
+    }
+    int l0[16], l3[16];
 #pragma unroll
-        for (int li = 0; li < 2; li++)
+    for (int k = 0; k < 16; k++) {
+        l0[k] = base[cols[0] + k * ks];
+        l3[k] = base[cols[3] + k * ks];
+    }
+    LumaDecision d;
+    if (!luma_decide(l0, l3, f.qp[0], bs, f.beta_offset_div2[0], f.tc_offset_div2[0], t.bd, len_p, len_q,
+                     dir == 1 && (e & RMH_EDGE_CTB_ROW) != 0, d))
+        return;
+    const int np = luma_np(d), nq = luma_nq(d), maxv = (1 << t.bd) - 1;
 #pragma unroll
-            for (int k = 1; k < 7; k++) {
-                if (dir == 0)
-                    st(P, clampi(x - 4 + k, 0, w - 1), min(y + li, h - 1), L[li][k]);
-                else
-                    st(P, min(x + li, w - 1), clampi(y - 4 + k, 0, h - 1), L[li][k]);
-            }
+    for (int li = 0; li < 4; li++) {
+        if (li > 0 && cols[li] == cols[li - 1]) continue;
+        int line[16];
+#pragma unroll
+        for (int k = 0; k < 16; k++) line[k] = li == 0 ? l0[k] : li == 3 ? l3[k] : base[cols[li] + k * ks];
+        luma_filter_line(line, d, maxv);
+#pragma unroll
+        for (int k = 1; k < 15; k++)
+            if (k >= 8 - np && k < 8 + nq) base[cols[li] + k * ks] = (int16_t)line[k];
+    }
+}
+
+__device__ __forceinline__ void chroma_slot(const TileCtx& t, int dir, int plane, int x, int y, uint8_t e, int16_t* cw) {
+    const rmh_dbk_frame& f = *t.f;
+    const int bs = e & RMH_EDGE_BS_MASK;
+    const int len_p = 2 * ((e >> RMH_EDGE_LENP_SHIFT) & 3) + 1, len_q = 2 * ((e >> RMH_EDGE_LENQ_SHIFT) & 3) + 1;
+    int16_t* base;
+    int ls, ks;
+    if (dir == 0) {
+        base = cw + (y - (t.CY0 - 4)) * kCWW + (x - t.CX0);  // column x - 4
+        ls = kCWW;
+        ks = 1;
+    } else {
+        base = cw + (y - t.CY0) * kCWW + (x - t.CX0 + 4);    // row y - 4
+        ls = 1;
+        ks = kCWW;
+    }
+    int L[2][8];
+#pragma unroll
+    for (int li = 0; li < 2; li++)
+#pragma unroll
+        for (int k = 0; k < 8; k++) L[li][k] = base[li * ls + k * ks];
+    if (!chroma_segment(L, f.qp[plane], bs, f.beta_offset_div2[plane], f.tc_offset_div2[plane], len_p, len_q, t.bd))
+        return;
+#pragma unroll
+    for (int li = 0; li < 2; li++)
+#pragma unroll
+        for (int k = 1; k < 7; k++) base[li * ls + k * ks] = (int16_t)L[li][k];
+}
+
+// One pass (dir 0 vertical, 1 horizontal) over the tile: the slots with an
+// edge go to two LDS queues (luma, chroma), then each queue is filtered with
+// lanes packed densely, so a wave runs a filter only for segments that exist.
+// Slot -> (plane, x, y) without the edge-map read.
+__device__ __forceinline__ void slot_pos(const TileCtx& t, int dir, int s, int& plane, int& x, int& y) {
+    const int nl = dir == 0 ? kLVX * kLVY : kLHX * kLHY;
+    const int cx = dir == 0 ? kCVX : kCHX, cy = dir == 0 ? kCVY : kCHY;
+    if (s < nl) {
+        plane = 0;
+        x = dir == 0 ? t.X0 + 4 * (s % kLVX) : t.X0 + 4 * (s % kLHX);
+        y = dir == 0 ? t.Y0 - 8 + 4 * (s / kLVX) : t.Y0 + 4 * (s / kLHX);
+        return;
+    }
+    const int u = s - nl, c = u / (cx * cy), v = u % (cx * cy);
+    plane = 1 + c;
+    x = dir == 0 ? t.CX0 + 2 * (v % kCVX) : t.CX0 + 2 * (v % kCHX);
+    y = dir == 0 ? t.CY0 - 4 + 2 * (v / kCVX) : t.CY0 + 2 * (v / kCHX);
+}
+
+// Queue entries: slot | edge byte << 16.
+__device__ __forceinline__ void tile_pass(const TileCtx& t, int dir, int16_t* lw, int16_t (*cw)[kCWH * kCWW],
+                                          uint32_t* ql, uint32_t* qc, int* counts) {
+    constexpr int NS = (kMaxQ + 255) / 256;
+    const int nslots = dir == 0 ? kNV : kNH;
+    const int nl = dir == 0 ? kLVX * kLVY : kLHX * kLHY;
+    if (threadIdx.x < 2) counts[threadIdx.x] = 0;
+    __syncthreads();
+    uint8_t eb[NS];
+#pragma unroll
+    for (int i = 0; i < NS; i++) {
+        const int s = threadIdx.x + 256 * i;
+        int plane, x, y;
+        eb[i] = s < nslots ? slot_edge(t, dir, s, plane, x, y) : 0;
+    }
+#pragma unroll
+    for (int i = 0; i < NS; i++) {
+        const int s = threadIdx.x + 256 * i;
+        if (eb[i]) {
+            const uint32_t q = (uint32_t)s | ((uint32_t)eb[i] << 16);
+            if (s < nl)
+                ql[atomicAdd(&counts[0], 1)] = q;
+            else
+                qc[atomicAdd(&counts[1], 1)] = q;
+        }
+    }
+    __syncthreads();
+    const int n_l = counts[0], n_c = counts[1];
+    for (int i = threadIdx.x; i < n_l; i += 256) {
+        int plane, x, y;
+        const uint32_t q = ql[i];
+        slot_pos(t, dir, (int)(q & 0xffff), plane, x, y);
+        luma_slot(t, dir, x, y, (uint8_t)(q >> 16), lw);
+    }
+    for (int i = threadIdx.x; i < n_c; i += 256) {
+        int plane, x, y;
+        const uint32_t q = qc[i];
+        slot_pos(t, dir, (int)(q & 0xffff), plane, x, y);
+        chroma_slot(t, dir, plane, x, y, (uint8_t)(q >> 16), cw[plane - 1]);
+    }
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(256) void deblock_tile_kernel(const rmh_dbk_frame* __restrict__ frames,
+                                                           const rmh_plane* __restrict__ shadows) {
+    __shared__ int16_t lw[kLWH * kLWW];
+    __shared__ int16_t cwin[2][kCWH * kCWW];
+    __shared__ uint32_t ql[kMaxQ], qc[kMaxQ];
+    __shared__ uint32_t cells[kEY * kEX];
+    __shared__ int counts[2];
+    TileCtx t;
+    t.f = &frames[blockIdx.y];
+    t.cells = cells;
+    const rmh_plane& PY = t.f->planes[0];
+    t.W = PY.width;
+    t.H = PY.height;
+    const int tiles_x = (t.W + kTW - 1) / kTW, tiles_y = (t.H + kTH - 1) / kTH;
+    if ((int)blockIdx.x >= tiles_x * tiles_y) return;
+    t.X0 = (blockIdx.x % tiles_x) * kTW;
+    t.Y0 = (blockIdx.x / tiles_x) * kTH;
+    t.CX0 = t.X0 / 2;
+    t.CY0 = t.Y0 / 2;
+    t.CW = t.f->planes[1].width;
+    t.CH = t.f->planes[1].height;
+    t.bd = PY.bit_depth;
+    const rmh_plane* sh = shadows + blockIdx.y * 3;
+
+    // 1. edge cells and windows
+    if (threadIdx.x < kEY * kEX) {
+        const rmh_dbk_frame& f = *t.f;
+        const int cx = t.X0 / 4 + threadIdx.x % kEX, cy = t.Y0 / 4 - 2 + threadIdx.x / kEX;
+        uint32_t v = 0;
+        if (cx < (int)f.grid_w && cy >= 0 && cy < (int)f.grid_h) {
+            const uint8_t* e = f.edges + ((size_t)cy * f.grid_w + cx) * 4;
+            if (((uintptr_t)f.edges & 3) == 0)
+                v = *reinterpret_cast<const uint32_t*>(e);
+            else
+                v = (uint32_t)e[0] | ((uint32_t)e[1] << 8) | ((uint32_t)e[2] << 16) | ((uint32_t)e[3] << 24);
+        }
+        cells[threadIdx.x] = v;
+    }
+    load_window<kLWW, kLWH>(sh[0], t.X0 - 8, t.Y0 - 8, lw);
+    load_window<kCWW, kCWH>(sh[1], t.CX0 - 4, t.CY0 - 4, cwin[0]);
+    load_window<kCWW, kCWH>(sh[2], t.CX0 - 4, t.CY0 - 4, cwin[1]);
+    __syncthreads();
+    // 2. vertical edges
+    tile_pass(t, 0, lw, cwin, ql, qc, counts);
+    // 3. rows outside the picture read as the nearest (filtered) picture row
+    if (t.Y0 - 8 < 0 || t.Y0 + kTH + 8 > t.H) refill_rows<kLWW, kLWH>(t.Y0 - 8, t.H, lw);
+    if (t.CY0 - 4 < 0 || t.CY0 + kCTH + 4 > t.CH) {
+        refill_rows<kCWW, kCWH>(t.CY0 - 4, t.CH, cwin[0]);
+        refill_rows<kCWW, kCWH>(t.CY0 - 4, t.CH, cwin[1]);
+    }
+    __syncthreads();
+    // 4. horizontal edges
+    tile_pass(t, 1, lw, cwin, ql, qc, counts);
+    // 5. write the tile back (picture samples only)
+    for (int e = threadIdx.x; e < kTW * kTH; e += 256) {
+        const int r = e / kTW, c = e % kTW, x = t.X0 + c, y = t.Y0 + r;
+        if (x < t.W && y < t.H) st32(PY, x, y, lw[(r + 8) * kLWW + c + 8]);
+    }
+    for (int e = threadIdx.x; e < 2 * kCTW * kCTH; e += 256) {
+        const int pl = e / (kCTW * kCTH), q = e % (kCTW * kCTH), r = q / kCTW, c = q % kCTW;
+        const int x = t.CX0 + c, y = t.CY0 + r;
+        if (x < t.CW && y < t.CH) st32(t.f->planes[1 + pl], x, y, cwin[pl][(r + 4) * kCWW + c + 4]);
     }
 }
 
 }  // namespace
 
-hipError_t launch_deblock(const rmh_dbk_frame* frames, uint32_t n_frames, uint32_t max_cells, hipStream_t stream) {
-    if (n_frames == 0 || max_cells == 0) return hipSuccess;
-    dim3 grid((max_cells + 255) / 256, n_frames, 3);
-    hipLaunchKernelGGL(deblock_kernel, grid, dim3(256), 0, stream, frames, 0);
+uint32_t deblock_tiles(uint32_t width, uint32_t height) {
+    return ((width + kTW - 1) / kTW) * ((height + kTH - 1) / kTH);
+}
+
+hipError_t launch_deblock(const rmh_dbk_frame* frames, const rmh_plane* shadows, uint32_t n_frames, uint32_t max_tiles,
+                          uint32_t max_rows, hipStream_t stream) {
+    if (n_frames == 0 || max_tiles == 0) return hipSuccess;
+    hipLaunchKernelGGL(shadow_copy_kernel, dim3((max_rows + 3) / 4, n_frames, 3), dim3(256), 0, stream, frames, shadows);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(deblock_kernel, grid, dim3(256), 0, stream, frames, 1);
+    hipLaunchKernelGGL(deblock_tile_kernel, dim3(max_tiles, n_frames), dim3(256), 0, stream, frames, shadows);
     return hipGetLastError();
 }
 

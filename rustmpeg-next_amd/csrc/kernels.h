@@ -85,7 +85,12 @@ struct McItem {
 
 hipError_t launch_mc(const rmh_mc_frame* frames, const McItem* items, uint32_t n_items, hipStream_t stream);
 
-// Deblocking runs two launches (all vertical edges, then all horizontal edges).
-hipError_t launch_deblock(const rmh_dbk_frame* frames, uint32_t n_frames, uint32_t max_cells, hipStream_t stream);
+// Deblocking: a copy of every plane into `shadows` (3 per frame, same sizes,
+// device memory), then one tiled launch that reads the shadows and writes the
+// pictures. max_tiles = largest deblock_tiles() of the frames, max_rows =
+// largest luma height.
+uint32_t deblock_tiles(uint32_t width, uint32_t height);
+hipError_t launch_deblock(const rmh_dbk_frame* frames, const rmh_plane* shadows, uint32_t n_frames, uint32_t max_tiles,
+                          uint32_t max_rows, hipStream_t stream);
 
 }  // namespace rmh

@@ -110,8 +110,9 @@ struct rmh_plan {
     size_t n_items = 0;
     // deblock
     const rmh_dbk_frame* d_dbk = nullptr;
+    const rmh_plane* d_shadows = nullptr;  // 3 per frame: copies the tiled kernel reads
     size_t n_dbk = 0;
-    uint32_t max_cells = 0;
+    uint32_t max_tiles = 0, max_rows = 0;
 };
 
 struct rmh_graph {
@@ -513,6 +514,28 @@ uint64_t mc_bytes(const rmh_mc_frame* frames, const rmh_pu_desc* pus, size_t n_p
 
 uint64_t plane_bytes(const rmh_plane& p) { return (uint64_t)p.width * p.height * p.bytes_per_sample; }
 
+// Shadow planes for deblocking: same size and sample type, rows padded to 16
+// bytes. shadow_bytes sums what shadow_layout places from `base`.
+uint32_t shadow_stride(const rmh_plane& p) { return (uint32_t)(((uint64_t)p.width * p.bytes_per_sample + 15) / 16 * 16 / p.bytes_per_sample); }
+size_t shadow_bytes(const rmh_dbk_frame* frames, size_t n_frames) {
+    size_t b = 0;
+    for (size_t f = 0; f < n_frames; f++)
+        for (int c = 0; c < 3; c++) b += aligned((size_t)shadow_stride(frames[f].planes[c]) * frames[f].planes[c].height * frames[f].planes[c].bytes_per_sample);
+    return b;
+}
+std::vector<rmh_plane> shadow_layout(const rmh_dbk_frame* frames, size_t n_frames, char* base) {
+    std::vector<rmh_plane> out;
+    for (size_t f = 0; f < n_frames; f++)
+        for (int c = 0; c < 3; c++) {
+            rmh_plane q = frames[f].planes[c];
+            q.stride = shadow_stride(q);
+            q.data = base;
+            base += aligned((size_t)q.stride * q.height * q.bytes_per_sample);
+            out.push_back(q);
+        }
+    return out;
+}
+
 
 // TUs the tiled kernel handles: 2-D with both sides >= 4, no LFNST, no
 // dependent quantisation, no transform skip, 8-byte aligned block.
@@ -593,7 +616,7 @@ rmh_status run_plan_locked_free(rmh_plan* plan, const int16_t* d_coeff, int16_t*
     } else if (plan->kind == kMc) {
         e = rmh::launch_mc(plan->d_frames, plan->d_items, (uint32_t)plan->n_items, s);
     } else {
-        e = rmh::launch_deblock(plan->d_dbk, (uint32_t)plan->n_dbk, plan->max_cells, s);
+        e = rmh::launch_deblock(plan->d_dbk, plan->d_shadows, (uint32_t)plan->n_dbk, plan->max_tiles, plan->max_rows, s);
     }
     if (e != hipSuccess) {
         *err = std::string("kernel launch failed: ") + hipGetErrorString(e);
@@ -986,19 +1009,27 @@ RMH_EXPORT rmh_status rmh_deblock_plan_create(rmh_ctx* ctx, const rmh_dbk_frame*
     p->n_dbk = n_frames;
     for (size_t f = 0; f < n_frames; f++) {
         uint32_t cells = frames[f].grid_w * frames[f].grid_h;
-        if (cells > p->max_cells) p->max_cells = cells;
+        p->max_tiles = std::max(p->max_tiles, rmh::deblock_tiles(frames[f].planes[0].width, frames[f].planes[0].height));
+        p->max_rows = std::max(p->max_rows, frames[f].planes[0].height);
         for (int c = 0; c < 3; c++) p->bytes += 2 * plane_bytes(frames[f].planes[c]);
         p->bytes += 4ull * cells;
     }
-    size_t fb = n_frames * sizeof(rmh_dbk_frame);
-    hipError_t e = hipMalloc(&p->d_mem, fb + 256);
-    if (e == hipSuccess && fb) e = hipMemcpy(p->d_mem, frames, fb, hipMemcpyHostToDevice);
+    size_t fb = n_frames * sizeof(rmh_dbk_frame), sb = n_frames * 3 * sizeof(rmh_plane);
+    size_t shb = shadow_bytes(frames, n_frames);
+    hipError_t e = hipMalloc(&p->d_mem, aligned(fb) + aligned(sb) + shb + 256);
+    char* base = (char*)p->d_mem;
+    if (e == hipSuccess) {
+        std::vector<rmh_plane> sh = shadow_layout(frames, n_frames, base + aligned(fb) + aligned(sb));
+        if (fb) e = hipMemcpy(base, frames, fb, hipMemcpyHostToDevice);
+        if (e == hipSuccess && sb) e = hipMemcpy(base + aligned(fb), sh.data(), sb, hipMemcpyHostToDevice);
+    }
     if (e != hipSuccess) {
         if (p->d_mem) hipFree(p->d_mem);
         delete p;
         return fail(ctx, RMH_EXECUTION, std::string("deblock plan upload failed: ") + hipGetErrorString(e));
     }
-    p->d_dbk = (const rmh_dbk_frame*)p->d_mem;
+    p->d_dbk = (const rmh_dbk_frame*)base;
+    p->d_shadows = (const rmh_plane*)(base + aligned(fb));
     *out = p;
     return RMH_OK;
 }
@@ -1018,11 +1049,13 @@ RMH_EXPORT rmh_status rmh_deblock(rmh_ctx* ctx, const rmh_dbk_frame* frames, siz
     rmh_status g = gate(ctx);
     if (g != RMH_OK) return fail_locked(ctx, g, g == RMH_UNAVAILABLE ? "accelerator disabled" : "injected device failure");
     DeviceGuard dg(ctx->device);
-    size_t fb = n_frames * sizeof(rmh_dbk_frame), need = aligned(fb), hneed = 0;
-    uint32_t max_cells = 0;
+    size_t fb = n_frames * sizeof(rmh_dbk_frame), sb = n_frames * 3 * sizeof(rmh_plane);
+    size_t need = aligned(fb) + aligned(sb) + shadow_bytes(frames, n_frames), hneed = 0;
+    uint32_t max_tiles = 0, max_rows = 0;
     for (size_t f = 0; f < n_frames; f++) {
         uint32_t cells = frames[f].grid_w * frames[f].grid_h;
-        max_cells = cells > max_cells ? cells : max_cells;
+        max_tiles = std::max(max_tiles, rmh::deblock_tiles(frames[f].planes[0].width, frames[f].planes[0].height));
+        max_rows = std::max(max_rows, frames[f].planes[0].height);
         need += aligned(4ull * cells);
         for (int c = 0; c < 3; c++) {
             need += aligned(plane_bytes(frames[f].planes[c]));
@@ -1035,6 +1068,10 @@ RMH_EXPORT rmh_status rmh_deblock(rmh_ctx* ctx, const rmh_dbk_frame* frames, siz
     if (e != hipSuccess) return fail_locked(ctx, RMH_LIMIT_EXCEEDED, std::string("host staging: ") + hipGetErrorString(e));
     hipStream_t s = ctx->stream;
     rmh_dbk_frame* d_frames = (rmh_dbk_frame*)ctx->dev.take(fb);
+    rmh_plane* d_shadows = (rmh_plane*)ctx->dev.take(sb);
+    const std::vector<rmh_plane> sh = shadow_layout(frames, n_frames, (char*)ctx->dev.take(shadow_bytes(frames, n_frames)));
+    if ((e = hipMemcpyAsync(d_shadows, sh.data(), sb, hipMemcpyHostToDevice, s)) != hipSuccess)
+        return hip_fail_locked(ctx, e, "shadow upload");
     std::vector<rmh_dbk_frame> df(frames, frames + n_frames);
     for (size_t f = 0; f < n_frames; f++) {
         size_t eb = 4ull * frames[f].grid_w * frames[f].grid_h;
@@ -1053,7 +1090,7 @@ RMH_EXPORT rmh_status rmh_deblock(rmh_ctx* ctx, const rmh_dbk_frame* frames, siz
         }
     }
     if ((e = hipMemcpyAsync(d_frames, df.data(), fb, hipMemcpyHostToDevice, s)) != hipSuccess) return hip_fail_locked(ctx, e, "upload");
-    e = rmh::launch_deblock(d_frames, (uint32_t)n_frames, max_cells, s);
+    e = rmh::launch_deblock(d_frames, d_shadows, (uint32_t)n_frames, max_tiles, max_rows, s);
     if (e != hipSuccess) return hip_fail_locked(ctx, e, "kernel launch");
     std::vector<void*> hp;
     for (size_t f = 0; f < n_frames; f++)
