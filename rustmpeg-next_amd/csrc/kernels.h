@@ -21,6 +21,25 @@ struct DeviceTables {
     int32_t scan_len[7][7];
 };
 
+// Per tile-eligible TU, in tile order, built at plan creation: what the tiled
+// kernel needs about the TU in one 16-byte load.
+struct TileTu {
+    uint32_t off;     // element offset of the TU block
+    uint8_t cw, ch;   // coded region after zero-out (<= 32)
+    uint8_t scale;    // dequantisation scale
+    uint8_t shifts;   // dequant left shift | right shift << 4
+    uint16_t mh, mv;  // dword offsets of the column-major matrices (col16)
+    uint8_t shift2;   // second-stage shift, 20 - bit depth
+    uint8_t reserved[3];
+};
+// Recon target of a tile-eligible TU (recon plans only).
+struct TileDst {
+    char* base;       // picture sample (0, 0) of the block
+    int32_t stride;   // row pitch, samples
+    uint16_t bps;     // bytes per sample
+    uint16_t maxv;    // 2^bd - 1
+};
+
 struct ResidualArgs {
     const rmh_tu_desc* tus;
     const int16_t* coeff;
@@ -34,6 +53,8 @@ struct ResidualArgs {
     int32_t col16_off[3][7];
     const rmh_tu_place* place;         // recon mode when non-null: add into frames' planes
     const rmh_recon_frame* frames;
+    const TileTu* tile_tus;            // tiled kernel: per TU of order[0 .. n_fast)
+    const TileDst* tile_dst;           // tiled kernel, recon mode
     uint32_t first_tu;
     uint32_t n_tu;
     int32_t mat_off[3][7];
@@ -65,10 +86,10 @@ inline int residual_tile_tus(int lw, int lh) {
 }
 inline uint32_t residual_tile_lds(int lw, int lh) {
     uint32_t count = (uint32_t)residual_tile_tus(lw, lh);
-    int wc = (1 << lw) < 32 ? (1 << lw) : 32, hc = (1 << lh) < 32 ? (1 << lh) : 32;
-    uint32_t c = count * wc * (hc / 2) * 4;         // coefficient columns
+    int wc = (1 << lw) < 32 ? (1 << lw) : 32;
+    uint32_t c = count * (1u << (lw + lh)) * 2;      // coefficient columns, then the output tile
     uint32_t t = count * (1u << lh) * (wc / 2) * 4;  // intermediate rows
-    return c + t + count * 48 + 16;
+    return c + t + count * 40 + 16;                 // + per-TU parameters + tile bounding box
 }
 
 // One inter work item: a 16x16 luma tile (and its 8x8 chroma tiles) of one

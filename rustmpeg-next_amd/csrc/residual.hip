@@ -400,15 +400,23 @@ struct TuParams {
     int32_t off;
     int32_t cwch;    // cw | ch << 8
     int32_t scale;
-    int32_t rshift;
+    int32_t shifts;  // dequant left shift | right shift << 8 | second-stage shift (20 - bd) << 16
     int32_t mh;      // dword offset of Mh pairs
     int32_t mv;      // dword offset of Mv pairs
-    int32_t shift2;  // 20 - bd
     int32_t stride;  // recon: picture row pitch (samples)
-    char* base;      // recon: sample (0, 0) of the block; null for residual output
-    int32_t bps;     // recon: bytes per sample
-    int32_t maxv;    // recon: 2^bd - 1
+    int32_t bpsmax;  // recon: bytes per sample | (2^bd - 1) << 8
+    char* base;      // recon: sample (0, 0) of the block
 };
+
+// dequantize_rect_bd without branches: v = level * scale (|v| < 2^22), then
+// either (v + 2^(rs-1)) >> rs or v << ls, saturated to int16. For a left
+// shift v is first limited to +-2^16 (which saturates anyway), so the shift
+// stays inside int32 for ls <= 14.
+__device__ __forceinline__ int dequant_bf(int level, int scale, int ls, int rs) {
+    const int cl = ls ? 65536 : 0x7fffffff;
+    const int v = min(max(level * scale, -cl), cl);
+    return min(max(((v << ls) + ((1 << rs) >> 1)) >> rs, -32768), 32767);
+}
 
 // dot2 over pairs kk < npairs read 2 at a time (8-byte LDS reads); m[kk] = 0
 // past the live pairs.
@@ -426,18 +434,6 @@ __device__ __forceinline__ int dot_pairs2(const uint32_t* __restrict__ src, cons
     return acc;
 }
 
-// dequantize_rect_bd in int32: |level * scale| < 2^22, and a left shift whose
-// result leaves int16 saturates exactly like the clip of the int64 form.
-__device__ __forceinline__ int dequant32(int level, int scale, int rshift) {
-    const int v = level * scale;
-    if (rshift > 0) return clip16i((v + (1 << (rshift - 1))) >> rshift);
-    const int ls = -rshift;
-    if (ls >= 16) return v > 0 ? 32767 : v < 0 ? -32768 : 0;
-    if (v > (32767 >> ls)) return 32767;
-    if (v < (-32768 >> ls)) return -32768;
-    return v << ls;
-}
-
 __device__ __forceinline__ int wave_max_u(int v) { return __builtin_amdgcn_readfirstlane(wave_max(v)); }
 
 // 16 matrix pairs of one column (zero past the matrix) with four 128-bit loads.
@@ -453,54 +449,41 @@ __device__ __forceinline__ void load_col16(const uint32_t* __restrict__ src, uin
     }
 }
 
-// `place` / `frames` come in as values read once at kernel entry: reading them
-// through `a` inside the per-TU loop makes the compiler copy the whole
-// argument block to scratch memory.
 template <int LW, int LH, bool RECON>
-__device__ __forceinline__ void tile_body(const ResidualArgs& a, const TileItem& it, uint32_t* smem,
-                                          const rmh_tu_place* __restrict__ place,
-                                          const rmh_recon_frame* __restrict__ frames) {
+__device__ __forceinline__ void tile_body(const ResidualArgs& a, const TileItem& it, uint32_t* smem) {
     constexpr int W = 1 << LW, H = 1 << LH;
     constexpr int LWC = LW < 5 ? LW : 5, LHC = LH < 5 ? LH : 5;
     constexpr int Wc = 1 << LWC, Hc = 1 << LHC;
     constexpr int CP = Hc / 2, TP = Wc / 2;  // dwords per coefficient column / intermediate row
     constexpr int GMAX = (4096 >> (LW + LH)) > 0 ? (4096 >> (LW + LH)) : 1;
     const int G = it.count;
+    // LDS: region 0 holds the coefficients (phases A, B), then the output
+    // tile (phase C); then the intermediates; then the per-TU parameters.
     uint32_t* C = smem;
-    uint32_t* T = C + GMAX * Wc * CP;
+    int16_t* O = reinterpret_cast<int16_t*>(smem);
+    uint32_t* T = C + GMAX * W * H / 2;
     TuParams* P = reinterpret_cast<TuParams*>(T + GMAX * H * TP);
     int* TB = reinterpret_cast<int*>(P + GMAX);  // tile bounding box: max row, max column of non-zeros
     const int tid = threadIdx.x;
 
     if (tid < 2) TB[tid] = -1;
     for (int g = tid; g < G; g += 256) {
-        const rmh_tu_desc d = a.tus[a.order[it.first + g]];
-        const int th = d.tr_hor, tv = d.tr_ver;
-        int sw = (th != RMH_TR_DCT2 && W == 32) ? 16 : (W > 32 ? W - 32 : 0);
-        int sh = (tv != RMH_TR_DCT2 && H == 32) ? 16 : (H > 32 ? H - 32 : 0);
-        sw = d.skip_w > sw ? d.skip_w : sw;
-        sh = d.skip_h > sh ? d.skip_h : sh;
-        const int qp = d.qp < 4 ? 4 : d.qp;
-        constexpr int lsum = LW + LH, sqrt2 = lsum & 1;
-        const int tr_shift = 15 - d.bit_depth - (lsum >> 1) - sqrt2;
+        const TileTu d = a.tile_tus[it.first + g];
         TuParams p;
-        p.off = (int32_t)d.offset;
-        p.cwch = (W - sw) | ((H - sh) << 8);
-        p.scale = kInvQuant[sqrt2][qp % 6];
-        p.rshift = 6 - (tr_shift + qp / 6);
-        p.mh = a.col16_off[th][LW];
-        p.mv = a.col16_off[tv][LH];
-        p.shift2 = 20 - d.bit_depth;
+        p.off = (int32_t)d.off;
+        p.cwch = d.cw | (d.ch << 8);
+        p.scale = d.scale;
+        p.shifts = (d.shifts & 15) | ((d.shifts >> 4) << 8) | (d.shift2 << 16);
+        p.mh = d.mh;
+        p.mv = d.mv;
         p.stride = 0;
+        p.bpsmax = 0;
         p.base = nullptr;
-        p.bps = 0;
-        p.maxv = 0;
         if (RECON) {
-            const ReconDst rd = recon_dst(place, frames, a.order[it.first + g]);
-            p.stride = rd.stride;
-            p.base = rd.base;
-            p.bps = rd.bps;
-            p.maxv = rd.maxv;
+            const TileDst t = a.tile_dst[it.first + g];
+            p.stride = t.stride;
+            p.base = t.base;
+            p.bpsmax = t.bps | (t.maxv << 8);
         }
         P[g] = p;
     }
@@ -519,6 +502,7 @@ __device__ __forceinline__ void tile_body(const ResidualArgs& a, const TileItem&
             const int g = e / (Q * RP);
             const TuParams p = P[g];
             const int cw = p.cwch & 255, ch = p.cwch >> 8;
+            const int ls = p.shifts & 15, rs = (p.shifts >> 8) & 31;
             const int r0 = 2 * rp;
             uint2 raw0 = make_uint2(0, 0), raw1 = make_uint2(0, 0);
             if (x0 < cw) {
@@ -531,9 +515,10 @@ __device__ __forceinline__ void tile_body(const ResidualArgs& a, const TileItem&
                                (int)(int16_t)(raw1.y & 0xffff), (int)(int16_t)(raw1.y >> 16)};
 #pragma unroll
             for (int k = 0; k < 4; k++) {
+                // columns past the coded region read as zero levels (dequantised to zero)
                 const bool inc = x0 + k < cw;
-                const int a0 = inc ? dequant32(lo[k], p.scale, p.rshift) : 0;
-                const int a1 = inc ? dequant32(hi[k], p.scale, p.rshift) : 0;
+                const int a0 = dequant_bf(inc ? lo[k] : 0, p.scale, ls, rs);
+                const int a1 = dequant_bf(inc ? hi[k] : 0, p.scale, ls, rs);
                 if (a0 | a1) {
                     mc = x0 + k > mc ? x0 + k : mc;
                     const int rr = a1 ? r0 + 1 : r0;
@@ -589,7 +574,8 @@ __device__ __forceinline__ void tile_body(const ResidualArgs& a, const TileItem&
     }
     __syncthreads();
 
-    // phase C: horizontal pass, lane = (TU, column x, row chunk)
+    // phase C: horizontal pass, lane = (TU, column x, row chunk); residuals
+    // go to the output tile in LDS, picture samples are updated in place
     {
         constexpr int NR0 = 256 / (GMAX * W);
         constexpr int NR = NR0 > 0 ? NR0 : 1;
@@ -604,9 +590,9 @@ __device__ __forceinline__ void tile_body(const ResidualArgs& a, const TileItem&
                 const TuParams p = P[g];
                 uint32_t m[16];
                 load_col16(a.col16 + p.mh + x * 16, m);
-                const int rnd = 1 << (p.shift2 - 1);
-                int16_t* out = RECON ? nullptr : a.residual + p.off;
-                const ReconDst rd{p.base, p.stride, p.bps, p.maxv};
+                const int shift2 = p.shifts >> 16;
+                const int rnd = 1 << (shift2 - 1);
+                const ReconDst rd{p.base, p.stride, p.bpsmax & 255, p.bpsmax >> 8};
                 for (int y = r0; y < H; y += NR) {
                     const uint2* src = reinterpret_cast<const uint2*>(T + (g * H + y) * TP);
                     int acc = 0;
@@ -618,13 +604,24 @@ __device__ __forceinline__ void tile_body(const ResidualArgs& a, const TileItem&
                             acc = dot2(v.y, m[2 * q2 + 1], acc);
                         }
                     }
-                    const int r = clip16i((acc + rnd) >> p.shift2);
+                    const int r = min(max((acc + rnd) >> shift2, -32768), 32767);
                     if (RECON)
                         recon_add(rd, x, y, r);
                     else
-                        out[y * W + x] = (int16_t)r;
+                        O[(g * H + y) * W + x] = (int16_t)r;
                 }
             }
+        }
+    }
+    if (!RECON) {
+        // the tile's TU blocks to HBM, 4 residuals (8 bytes) per store
+        __syncthreads();
+        constexpr int CH4 = W * H / 4;
+        const int total = G * CH4;
+        for (int e = tid; e < total; e += 256) {
+            const int g = e / CH4, c = e - g * CH4;
+            const uint2 v = *reinterpret_cast<const uint2*>(O + g * W * H + 4 * c);
+            *reinterpret_cast<uint2*>(a.residual + P[g].off + 4 * c) = v;
         }
     }
 }
@@ -635,12 +632,10 @@ template <bool RECON>
 __global__ __launch_bounds__(256) void residual_tile_kernel(ResidualArgs a, const TileItem* __restrict__ items) {
     extern __shared__ __align__(16) uint32_t smem[];
     const TileItem it = items[blockIdx.x];
-    const rmh_tu_place* place = a.place;
-    const rmh_recon_frame* frames = a.frames;
     switch (it.lw * 8 + it.lh) {
 #define RMH_TILE(LW, LH) \
     case LW * 8 + LH:    \
-        tile_body<LW, LH, RECON>(a, it, smem, place, frames); \
+        tile_body<LW, LH, RECON>(a, it, smem); \
         break;
         RMH_TILE(2, 2) RMH_TILE(2, 3) RMH_TILE(2, 4) RMH_TILE(2, 5) RMH_TILE(2, 6)
         RMH_TILE(3, 2) RMH_TILE(3, 3) RMH_TILE(3, 4) RMH_TILE(3, 5) RMH_TILE(3, 6)

@@ -103,6 +103,8 @@ struct rmh_plan {
     // recon (residual added into picture planes); null for residual output
     const rmh_tu_place* d_place = nullptr;
     const rmh_recon_frame* d_rframes = nullptr;
+    const rmh::TileTu* d_tile_tus = nullptr;   // per fast TU, tile order
+    const rmh::TileDst* d_tile_dst = nullptr;  // recon plans
     // inter
     const rmh_mc_frame* d_frames = nullptr;
     const rmh_pu_desc* d_pus = nullptr;
@@ -577,6 +579,56 @@ void build_residual_tiles(const rmh_tu_desc* tus, size_t n_tu, std::vector<uint3
     order.insert(order.end(), rest.begin(), rest.end());
 }
 
+// Dequantisation scales of dequantize_rect_bd (vvc/dct.rs:37-40), plain and
+// for sqrt(2)-scaled (odd log2 w + log2 h) blocks.
+const uint8_t kInvQuantScale[2][6] = {{40, 45, 51, 57, 64, 72}, {57, 64, 72, 80, 90, 102}};
+
+// Tiled-kernel parameters of order[0 .. n_fast) (the fast TUs).
+std::vector<rmh::TileTu> build_tile_tus(const rmh::DeviceTables& tabs, const rmh_tu_desc* tus,
+                                        const std::vector<uint32_t>& order, size_t n_fast) {
+    std::vector<rmh::TileTu> out(n_fast);
+    for (size_t i = 0; i < n_fast; i++) {
+        const rmh_tu_desc& d = tus[order[i]];
+        const int lw = d.log2_w, lh = d.log2_h, W = 1 << lw, H = 1 << lh;
+        int sw = (d.tr_hor != RMH_TR_DCT2 && W == 32) ? 16 : (W > 32 ? W - 32 : 0);
+        int sh = (d.tr_ver != RMH_TR_DCT2 && H == 32) ? 16 : (H > 32 ? H - 32 : 0);
+        sw = std::max<int>(sw, d.skip_w);
+        sh = std::max<int>(sh, d.skip_h);
+        const int qp = d.qp < 4 ? 4 : d.qp;
+        const int lsum = lw + lh, sqrt2 = lsum & 1;
+        const int tr_shift = 15 - d.bit_depth - (lsum >> 1) - sqrt2;
+        const int rshift = 6 - (tr_shift + qp / 6);
+        rmh::TileTu t{};
+        t.off = d.offset;
+        t.cw = (uint8_t)(W - sw);
+        t.ch = (uint8_t)(H - sh);
+        t.scale = kInvQuantScale[sqrt2][qp % 6];
+        t.shifts = (uint8_t)((rshift < 0 ? -rshift : 0) | ((rshift > 0 ? rshift : 0) << 4));
+        t.mh = (uint16_t)tabs.col16_off[d.tr_hor][lw];
+        t.mv = (uint16_t)tabs.col16_off[d.tr_ver][lh];
+        t.shift2 = (uint8_t)(20 - d.bit_depth);
+        out[i] = t;
+    }
+    return out;
+}
+
+// Recon targets of order[0 .. n_fast) (device picture addresses).
+std::vector<rmh::TileDst> build_tile_dst(const std::vector<uint32_t>& order, size_t n_fast, const rmh_tu_place* place,
+                                         const rmh_recon_frame* frames) {
+    std::vector<rmh::TileDst> out(n_fast);
+    for (size_t i = 0; i < n_fast; i++) {
+        const rmh_tu_place& pl = place[order[i]];
+        const rmh_plane& p = frames[pl.frame].planes[pl.plane];
+        rmh::TileDst t{};
+        t.base = (char*)p.data + ((size_t)pl.y * p.stride + pl.x) * p.bytes_per_sample;
+        t.stride = (int32_t)p.stride;
+        t.bps = p.bytes_per_sample;
+        t.maxv = (uint16_t)((1 << p.bit_depth) - 1);
+        out[i] = t;
+    }
+    return out;
+}
+
 rmh_status run_plan_locked_free(rmh_plan* plan, const int16_t* d_coeff, int16_t* d_residual, size_t first, size_t count,
                                 hipStream_t s, std::string* err) {
     hipError_t e = hipSuccess;
@@ -596,8 +648,10 @@ rmh_status run_plan_locked_free(rmh_plan* plan, const int16_t* d_coeff, int16_t*
         memcpy(a.col16_off, plan->ctx->tabs.col16_off, sizeof a.col16_off);
         a.place = plan->d_place;
         a.frames = plan->d_rframes;
+        a.tile_tus = plan->d_tile_tus;
+        a.tile_dst = plan->d_tile_dst;
         const bool whole = first == 0 && count == plan->n_tu && plan->d_order;
-        const bool aligned8 = ((uintptr_t)d_coeff & 7) == 0;
+        const bool aligned8 = ((uintptr_t)d_coeff & 7) == 0 && (plan->d_place || ((uintptr_t)d_residual & 7) == 0);
         if (whole && aligned8) {
             // the tiled kernel for the common TUs, the general kernel for the rest
             a.order = plan->d_order;
@@ -754,7 +808,11 @@ rmh_status create_residual_plan(rmh_ctx* ctx, const rmh_tu_desc* tus, size_t n_t
         build_residual_tiles(tus, n_tu, order, tiles, &p->n_fast, &p->tile_lds);
         size_t tb = aligned(n_tu * sizeof(rmh_tu_desc)), ob = aligned(order.size() * 4), ib = aligned(tiles.size() * sizeof(rmh::TileItem));
         size_t pb = recon ? aligned(n_tu * sizeof(rmh_tu_place)) : 0, fb = recon ? aligned(n_frames * sizeof(rmh_recon_frame)) : 0;
-        hipError_t e = hipMalloc(&p->d_mem, tb + ob + ib + pb + fb + 256);
+        const std::vector<rmh::TileTu> ttu = build_tile_tus(ctx->tabs, tus, order, p->n_fast);
+        std::vector<rmh::TileDst> tdst;
+        if (recon) tdst = build_tile_dst(order, p->n_fast, place, frames);
+        size_t ub = aligned(ttu.size() * sizeof(rmh::TileTu)), db = aligned(tdst.size() * sizeof(rmh::TileDst));
+        hipError_t e = hipMalloc(&p->d_mem, tb + ob + ib + pb + fb + ub + db + 256);
         char* base = (char*)p->d_mem;
         if (e == hipSuccess) e = hipMemcpy(base, tus, n_tu * sizeof(rmh_tu_desc), hipMemcpyHostToDevice);
         if (e == hipSuccess) e = hipMemcpy(base + tb, order.data(), order.size() * 4, hipMemcpyHostToDevice);
@@ -763,6 +821,10 @@ rmh_status create_residual_plan(rmh_ctx* ctx, const rmh_tu_desc* tus, size_t n_t
         if (e == hipSuccess && recon) e = hipMemcpy(base + tb + ob + ib, place, n_tu * sizeof(rmh_tu_place), hipMemcpyHostToDevice);
         if (e == hipSuccess && recon && n_frames)
             e = hipMemcpy(base + tb + ob + ib + pb, frames, n_frames * sizeof(rmh_recon_frame), hipMemcpyHostToDevice);
+        char* ubase = base + tb + ob + ib + pb + fb;
+        if (e == hipSuccess && !ttu.empty()) e = hipMemcpy(ubase, ttu.data(), ttu.size() * sizeof(rmh::TileTu), hipMemcpyHostToDevice);
+        if (e == hipSuccess && !tdst.empty())
+            e = hipMemcpy(ubase + ub, tdst.data(), tdst.size() * sizeof(rmh::TileDst), hipMemcpyHostToDevice);
         if (e != hipSuccess) {
             if (p->d_mem) hipFree(p->d_mem);
             delete p;
@@ -775,7 +837,9 @@ rmh_status create_residual_plan(rmh_ctx* ctx, const rmh_tu_desc* tus, size_t n_t
         if (recon) {
             p->d_place = (const rmh_tu_place*)(base + tb + ob + ib);
             p->d_rframes = (const rmh_recon_frame*)(base + tb + ob + ib + pb);
+            p->d_tile_dst = (const rmh::TileDst*)(ubase + ub);
         }
+        p->d_tile_tus = (const rmh::TileTu*)ubase;
     }
     *out = p;
     return RMH_OK;
@@ -826,7 +890,8 @@ RMH_EXPORT rmh_status rmh_residual_batch(rmh_ctx* ctx, const rmh_tu_desc* tus, s
     if (g != RMH_OK) return fail_locked(ctx, g, g == RMH_UNAVAILABLE ? "accelerator disabled" : "injected device failure");
     DeviceGuard dg(ctx->device);
     size_t tb = n_tu * sizeof(rmh_tu_desc), cb = n_coeff * sizeof(int16_t);
-    hipError_t e = ctx->dev.reserve(aligned(tb) + 2 * aligned(cb) + aligned(n_tu * 4) + aligned(n_tu * sizeof(rmh::TileItem)));
+    hipError_t e = ctx->dev.reserve(aligned(tb) + 2 * aligned(cb) + aligned(n_tu * 4) + aligned(n_tu * sizeof(rmh::TileItem)) +
+                                    aligned(n_tu * sizeof(rmh::TileTu)));
     if (e != hipSuccess) return fail_locked(ctx, RMH_LIMIT_EXCEEDED, std::string("device allocation: ") + hipGetErrorString(e));
     e = ctx->host.reserve(aligned(cb));
     if (e != hipSuccess) return fail_locked(ctx, RMH_LIMIT_EXCEEDED, std::string("host staging: ") + hipGetErrorString(e));
@@ -841,11 +906,15 @@ RMH_EXPORT rmh_status rmh_residual_batch(rmh_ctx* ctx, const rmh_tu_desc* tus, s
     hipStream_t s = ctx->stream;
     uint32_t* d_order = (uint32_t*)ctx->dev.take(order.size() * 4);
     rmh::TileItem* d_tiles = (rmh::TileItem*)ctx->dev.take(tiles.size() * sizeof(rmh::TileItem));
+    const std::vector<rmh::TileTu> ttu = build_tile_tus(ctx->tabs, tus, order, tmp.n_fast);
+    rmh::TileTu* d_ttu = (rmh::TileTu*)ctx->dev.take(ttu.size() * sizeof(rmh::TileTu));
     if ((e = hipMemcpyAsync(d_tus, tus, tb, hipMemcpyHostToDevice, s)) != hipSuccess ||
         (e = hipMemcpyAsync(d_coeff, coeff, cb, hipMemcpyHostToDevice, s)) != hipSuccess ||
         (e = hipMemcpyAsync(d_order, order.data(), order.size() * 4, hipMemcpyHostToDevice, s)) != hipSuccess ||
         (!tiles.empty() &&
-         (e = hipMemcpyAsync(d_tiles, tiles.data(), tiles.size() * sizeof(rmh::TileItem), hipMemcpyHostToDevice, s)) != hipSuccess))
+         (e = hipMemcpyAsync(d_tiles, tiles.data(), tiles.size() * sizeof(rmh::TileItem), hipMemcpyHostToDevice, s)) != hipSuccess) ||
+        (!ttu.empty() &&
+         (e = hipMemcpyAsync(d_ttu, ttu.data(), ttu.size() * sizeof(rmh::TileTu), hipMemcpyHostToDevice, s)) != hipSuccess))
         return hip_fail_locked(ctx, e, "upload");
     tmp.ctx = ctx;
     tmp.kind = kResidual;
@@ -854,6 +923,7 @@ RMH_EXPORT rmh_status rmh_residual_batch(rmh_ctx* ctx, const rmh_tu_desc* tus, s
     tmp.d_order = d_order;
     tmp.d_tiles = d_tiles;
     tmp.n_tiles = tiles.size();
+    tmp.d_tile_tus = d_ttu;
     std::string err;
     rmh_status r = run_plan_locked_free(&tmp, d_coeff, d_res, 0, n_tu, s, &err);
     if (r != RMH_OK) return fail_locked(ctx, r, err);
