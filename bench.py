@@ -287,6 +287,7 @@ def main():
     ap.add_argument("--no-pipeline", action="store_true")
     ap.add_argument("--no-streams", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
     args = ap.parse_args()
 
     if args.copy_calibration:
@@ -337,6 +338,23 @@ def main():
             log("bench: GPU residuals differ from the oracle on frame %d" % fi)
             return 1
     log("bit-exact check passed (frames 0 and %d)" % (args.frames - 1))
+
+    tuned = None
+    if not args.no_autotune:
+        t0 = time.perf_counter()
+        tuned = plan.autotune(d_coeff.ptr, d_res.ptr, s)
+        stream.synchronize()
+        plan.run(d_coeff.ptr, d_res.ptr, s)
+        stream.synchronize()
+        got = d_res.download(np.int16)
+        for fi in (0, args.frames - 1):
+            t, c = frames[fi]
+            base = int(tus[ranges[fi][0]]["offset"])
+            if not (got[base:base + c.size] == po.residual_batch(t, c)).all():
+                log("bench: autotuned residuals differ from the oracle on frame %d" % fi)
+                return 1
+        log("autotuned tile splits in %.2f s: %s" % (time.perf_counter() - t0,
+                                                      {"%dx%d" % k: v for k, v in tuned.items() if v > 1}))
 
     ev0, ev1 = hip.Event(), hip.Event()
     for _ in range(args.warmup):
@@ -408,6 +426,8 @@ def main():
                    "parallelism": "single GPU"},
         "unbatched": unb,
         "vs_unbatched": round(value / unb["value"], 3) if unb else None,
+        "autotune": None if tuned is None else {"tus_per_tile_divisor": {"%dx%d" % k: v for k, v in tuned.items()
+                                                                          if v > 1}},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "pframe_pipeline": pipe,

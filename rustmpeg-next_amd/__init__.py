@@ -71,7 +71,7 @@ EXPORTED = [
     "rmh_graph_launch", "rmh_graph_destroy", "rmh_transform_matrix", "rmh_intra_subset_decode",
     "rmh_residual_recon_plan_create", "rmh_rd_encode_luma", "rmh_cnn_param_count", "rmh_cnn_create",
     "rmh_cnn_destroy", "rmh_cnn_get_params", "rmh_cnn_set_params", "rmh_cnn_forward", "rmh_cnn_predict",
-    "rmh_cnn_gradients", "rmh_cnn_train_step", "rmh_cnn_train_step_soft",
+    "rmh_cnn_gradients", "rmh_cnn_train_step", "rmh_cnn_train_step_soft", "rmh_residual_plan_autotune",
 ]
 
 
@@ -97,6 +97,7 @@ def load_library(path=LIB_PATH):
         "rmh_residual_plan_run": ([vp, vp, vp, vp], st),
         "rmh_residual_recon_plan_create": ([vp, vp, sz, sz, vp, vp, sz, ctypes.POINTER(vp)], st),
         "rmh_residual_plan_run_range": ([vp, sz, sz, vp, vp, vp], st),
+        "rmh_residual_plan_autotune": ([vp, vp, vp, vp, ctypes.c_uint32, vp], st),
         "rmh_mc_batch": ([vp, vp, sz, vp, sz], st),
         "rmh_mc_plan_create": ([vp, vp, sz, vp, sz, ctypes.POINTER(vp)], st),
         "rmh_mc_plan_run": ([vp, vp], st),
@@ -184,6 +185,14 @@ class Plan:
             self.acc._check(L.rmh_mc_plan_run(self.handle, stream))
         else:
             self.acc._check(L.rmh_deblock_plan_run(self.handle, stream))
+
+    def autotune(self, d_coeff, d_residual, stream=0, reps=5):
+        """Chooses the tile split per TU size class by timing (rmh_residual_plan_autotune);
+        returns {(w, h): TUs-per-tile divisor}."""
+        out = np.zeros(49, np.uint8)
+        self.acc._check(load_library().rmh_residual_plan_autotune(self.handle, d_coeff, d_residual, stream, reps,
+                                                                  out.ctypes.data))
+        return {(1 << (i // 7), 1 << (i % 7)): 1 << int(v) for i, v in enumerate(out) if i // 7 >= 2 and i % 7 >= 2}
 
     def run_range(self, first, count, d_coeff, d_residual, stream=0):
         self.acc._check(load_library().rmh_residual_plan_run_range(self.handle, first, count, d_coeff, d_residual,

@@ -97,9 +97,13 @@ struct rmh_plan {
     const rmh_tu_desc* d_tus = nullptr;
     size_t n_tu = 0, n_coeff = 0;
     const uint32_t* d_order = nullptr;       // fast TUs grouped by size class, then the others
-    const rmh::TileItem* d_tiles = nullptr;  // tiles of the fast TUs
-    size_t n_tiles = 0, n_fast = 0;
+    rmh::TileItem* d_tiles = nullptr;  // tiles of the fast TUs
+    size_t n_tiles = 0, n_fast = 0, tile_capacity = 0;
     uint32_t tile_lds = 0;
+    // fast TUs of size class (log2 w, log2 h) are order[cls_first .. + cls_count);
+    // tiles of the class hold residual_tile_tus >> split TUs each (autotuned)
+    uint32_t cls_first[7][7] = {}, cls_count[7][7] = {};
+    uint8_t split[7][7] = {};
     // recon (residual added into picture planes); null for residual output
     const rmh_tu_place* d_place = nullptr;
     const rmh_recon_frame* d_rframes = nullptr;
@@ -546,10 +550,33 @@ bool tile_eligible(const rmh_tu_desc& t) {
            (t.offset & 3) == 0;
 }
 
-// Groups tile-eligible TUs by size class into tiles (largest class first) and
-// appends the remaining TUs; returns the tile list and the TU order.
+// Largest autotuned split: a class's tiles hold residual_tile_tus >> split TUs.
+constexpr int kMaxTileSplit = 3;
+
+// Tiles of the fast TUs (largest class first) from the class ranges of
+// `order`, each class split as `split` says.
+void tiles_from_classes(const uint32_t (*first)[7], const uint32_t (*count)[7], const uint8_t (*split)[7],
+                        std::vector<rmh::TileItem>& tiles, uint32_t* lds) {
+    tiles.clear();
+    *lds = 0;
+    for (int area = 12; area >= 4; area--)
+        for (int lw = 2; lw <= 6; lw++) {
+            const int lh = area - lw;
+            if (lh < 2 || lh > 6 || count[lw][lh] == 0) continue;
+            const int per = std::max(1, rmh::residual_tile_tus(lw, lh) >> split[lw][lh]);
+            for (uint32_t f = 0; f < count[lw][lh]; f += per) {
+                const uint32_t cnt = std::min<uint32_t>(per, count[lw][lh] - f);
+                tiles.push_back({first[lw][lh] + f, (uint16_t)cnt, (uint8_t)lw, (uint8_t)lh});
+            }
+            *lds = std::max(*lds, rmh::residual_tile_lds(lw, lh));
+        }
+}
+
+// Groups tile-eligible TUs by size class (largest class first) and appends the
+// remaining TUs; fills the class ranges and the unsplit tile list.
 void build_residual_tiles(const rmh_tu_desc* tus, size_t n_tu, std::vector<uint32_t>& order,
-                          std::vector<rmh::TileItem>& tiles, size_t* n_fast, uint32_t* lds) {
+                          std::vector<rmh::TileItem>& tiles, size_t* n_fast, uint32_t* lds,
+                          uint32_t (*first)[7], uint32_t (*count)[7]) {
     std::vector<uint32_t> cls[7][7];
     std::vector<uint32_t> rest;
     for (size_t i = 0; i < n_tu; i++) {
@@ -559,24 +586,31 @@ void build_residual_tiles(const rmh_tu_desc* tus, size_t n_tu, std::vector<uint3
             rest.push_back((uint32_t)i);
     }
     order.clear();
-    tiles.clear();
-    *lds = 0;
+    for (int lw = 0; lw < 7; lw++)
+        for (int lh = 0; lh < 7; lh++) first[lw][lh] = count[lw][lh] = 0;
     for (int area = 12; area >= 4; area--)
         for (int lw = 2; lw <= 6; lw++) {
             int lh = area - lw;
             if (lh < 2 || lh > 6) continue;
-            std::vector<uint32_t>& v = cls[lw][lh];
-            int per = rmh::residual_tile_tus(lw, lh);
-            for (size_t f = 0; f < v.size(); f += per) {
-                int cnt = (int)std::min<size_t>(per, v.size() - f);
-                tiles.push_back({(uint32_t)(order.size() + f), (uint16_t)cnt, (uint8_t)lw, (uint8_t)lh});
-                uint32_t b = rmh::residual_tile_lds(lw, lh);
-                if (b > *lds) *lds = b;
-            }
-            order.insert(order.end(), v.begin(), v.end());
+            first[lw][lh] = (uint32_t)order.size();
+            count[lw][lh] = (uint32_t)cls[lw][lh].size();
+            order.insert(order.end(), cls[lw][lh].begin(), cls[lw][lh].end());
         }
     *n_fast = order.size();
     order.insert(order.end(), rest.begin(), rest.end());
+    const uint8_t none[7][7] = {};
+    tiles_from_classes(first, count, none, tiles, lds);
+}
+
+// Tiles a plan can need at the largest split.
+size_t tile_capacity(const uint32_t (*count)[7]) {
+    size_t n = 0;
+    for (int lw = 2; lw <= 6; lw++)
+        for (int lh = 2; lh <= 6; lh++) {
+            const size_t per = std::max(1, rmh::residual_tile_tus(lw, lh) >> kMaxTileSplit);
+            n += (count[lw][lh] + per - 1) / per;
+        }
+    return n;
 }
 
 // Dequantisation scales of dequantize_rect_bd (vvc/dct.rs:37-40), plain and
@@ -805,8 +839,10 @@ rmh_status create_residual_plan(rmh_ctx* ctx, const rmh_tu_desc* tus, size_t n_t
     if (n_tu) {
         std::vector<uint32_t> order;
         std::vector<rmh::TileItem> tiles;
-        build_residual_tiles(tus, n_tu, order, tiles, &p->n_fast, &p->tile_lds);
-        size_t tb = aligned(n_tu * sizeof(rmh_tu_desc)), ob = aligned(order.size() * 4), ib = aligned(tiles.size() * sizeof(rmh::TileItem));
+        build_residual_tiles(tus, n_tu, order, tiles, &p->n_fast, &p->tile_lds, p->cls_first, p->cls_count);
+        p->tile_capacity = std::max(tiles.size(), tile_capacity(p->cls_count));
+        size_t tb = aligned(n_tu * sizeof(rmh_tu_desc)), ob = aligned(order.size() * 4),
+               ib = aligned(p->tile_capacity * sizeof(rmh::TileItem));
         size_t pb = recon ? aligned(n_tu * sizeof(rmh_tu_place)) : 0, fb = recon ? aligned(n_frames * sizeof(rmh_recon_frame)) : 0;
         const std::vector<rmh::TileTu> ttu = build_tile_tus(ctx->tabs, tus, order, p->n_fast);
         std::vector<rmh::TileDst> tdst;
@@ -832,7 +868,7 @@ rmh_status create_residual_plan(rmh_ctx* ctx, const rmh_tu_desc* tus, size_t n_t
         }
         p->d_tus = (const rmh_tu_desc*)base;
         p->d_order = (const uint32_t*)(base + tb);
-        p->d_tiles = (const rmh::TileItem*)(base + tb + ob);
+        p->d_tiles = (rmh::TileItem*)(base + tb + ob);
         p->n_tiles = tiles.size();
         if (recon) {
             p->d_place = (const rmh_tu_place*)(base + tb + ob + ib);
@@ -878,6 +914,96 @@ RMH_EXPORT rmh_status rmh_residual_plan_run_range(rmh_plan* plan, size_t first, 
     return run_plan(plan, d_coeff, d_residual, first, count, stream);
 }
 
+RMH_EXPORT rmh_status rmh_residual_plan_autotune(rmh_plan* plan, const int16_t* d_coeff, int16_t* d_residual,
+                                                 void* stream, uint32_t reps, uint8_t* split_out) {
+    if (!plan || plan->kind != kResidual) return RMH_INVALID_INPUT;
+    rmh_ctx* ctx = plan->ctx;
+    if (!d_coeff || (!d_residual && !plan->d_place)) return fail(ctx, RMH_INVALID_INPUT, "null device buffer");
+    if (((uintptr_t)d_coeff & 7) || (!plan->d_place && ((uintptr_t)d_residual & 7)))
+        return fail(ctx, RMH_INVALID_INPUT, "tiled kernel needs 8-byte aligned buffers");
+    rmh_status g = gate(ctx);
+    if (g != RMH_OK) return fail(ctx, g, g == RMH_UNAVAILABLE ? "accelerator disabled" : "injected device failure");
+    DeviceGuard dg(ctx->device);
+    hipStream_t s = (hipStream_t)stream;
+    reps = reps ? std::min<uint32_t>(reps, 100) : 5;
+    rmh::ResidualArgs a{};
+    a.tus = plan->d_tus;
+    a.coeff = d_coeff;
+    a.residual = d_residual;
+    a.mat = ctx->tabs.mat;
+    a.lfnst8 = ctx->tabs.lfnst8;
+    a.lfnst4 = ctx->tabs.lfnst4;
+    a.scans = ctx->tabs.scans;
+    memcpy(a.mat_off, ctx->tabs.mat_off, sizeof a.mat_off);
+    memcpy(a.scan_off, ctx->tabs.scan_off, sizeof a.scan_off);
+    memcpy(a.scan_len, ctx->tabs.scan_len, sizeof a.scan_len);
+    a.col16 = ctx->tabs.col16;
+    memcpy(a.col16_off, ctx->tabs.col16_off, sizeof a.col16_off);
+    a.place = plan->d_place;
+    a.frames = plan->d_rframes;
+    a.tile_tus = plan->d_tile_tus;
+    a.tile_dst = plan->d_tile_dst;
+    a.order = plan->d_order;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    // coordinate descent over the classes (largest first): each candidate split
+    // of one class is timed on the whole plan's launch; a change is kept when
+    // it saves at least 1 %
+    auto time_plan = [&](const uint8_t (*split)[7], float* ms) -> hipError_t {
+        std::vector<rmh::TileItem> tiles;
+        uint32_t lds = 0;
+        tiles_from_classes(plan->cls_first, plan->cls_count, split, tiles, &lds);
+        hipError_t r = hipMemcpy(plan->d_tiles, tiles.data(), tiles.size() * sizeof(rmh::TileItem), hipMemcpyHostToDevice);
+        for (uint32_t k = 0; k < 2 && r == hipSuccess; k++)
+            r = rmh::launch_residual_tiles(a, plan->d_tiles, (uint32_t)tiles.size(), lds, s);
+        if (r == hipSuccess) r = hipEventRecord(e0, s);
+        for (uint32_t k = 0; k < reps && r == hipSuccess; k++)
+            r = rmh::launch_residual_tiles(a, plan->d_tiles, (uint32_t)tiles.size(), lds, s);
+        if (r == hipSuccess) r = hipEventRecord(e1, s);
+        if (r == hipSuccess) r = hipEventSynchronize(e1);
+        if (r == hipSuccess) r = hipEventElapsedTime(ms, e0, e1);
+        return r;
+    };
+    uint8_t cur[7][7];
+    memcpy(cur, plan->split, sizeof cur);
+    float best = 0.0f;
+    if (e == hipSuccess) e = time_plan(cur, &best);
+    for (int area = 12; area >= 4 && e == hipSuccess; area--)
+        for (int lw = 2; lw <= 6 && e == hipSuccess; lw++) {
+            const int lh = area - lw;
+            if (lh < 2 || lh > 6 || plan->cls_count[lw][lh] == 0) continue;
+            const uint8_t keep = cur[lw][lh];
+            uint8_t choice = keep;
+            for (int sp = 0; sp <= kMaxTileSplit && e == hipSuccess; sp++) {
+                if (sp == keep || (rmh::residual_tile_tus(lw, lh) >> sp) < 1) continue;
+                cur[lw][lh] = (uint8_t)sp;
+                float ms = 0.0f;
+                e = time_plan(cur, &ms);
+                if (e == hipSuccess && ms < 0.99f * best) {
+                    best = ms;
+                    choice = (uint8_t)sp;
+                }
+            }
+            cur[lw][lh] = choice;
+        }
+    if (e == hipSuccess) memcpy(plan->split, cur, sizeof cur);
+    // the whole plan's tiles with the chosen splits
+    if (e == hipSuccess) {
+        std::vector<rmh::TileItem> tiles;
+        tiles_from_classes(plan->cls_first, plan->cls_count, plan->split, tiles, &plan->tile_lds);
+        e = hipMemcpy(plan->d_tiles, tiles.data(), tiles.size() * sizeof(rmh::TileItem), hipMemcpyHostToDevice);
+        plan->n_tiles = tiles.size();
+    }
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    if (e != hipSuccess) return fail(ctx, RMH_EXECUTION, std::string("autotune: ") + hipGetErrorString(e));
+    if (split_out)
+        for (int lw = 0; lw < 7; lw++)
+            for (int lh = 0; lh < 7; lh++) split_out[lw * 7 + lh] = plan->split[lw][lh];
+    return RMH_OK;
+}
+
 RMH_EXPORT rmh_status rmh_residual_batch(rmh_ctx* ctx, const rmh_tu_desc* tus, size_t n_tu, const int16_t* coeff,
                                          size_t n_coeff, int16_t* residual) {
     if (!ctx) return RMH_INVALID_INPUT;
@@ -902,7 +1028,7 @@ RMH_EXPORT rmh_status rmh_residual_batch(rmh_ctx* ctx, const rmh_tu_desc* tus, s
     std::vector<uint32_t> order;
     std::vector<rmh::TileItem> tiles;
     rmh_plan tmp;
-    build_residual_tiles(tus, n_tu, order, tiles, &tmp.n_fast, &tmp.tile_lds);
+    build_residual_tiles(tus, n_tu, order, tiles, &tmp.n_fast, &tmp.tile_lds, tmp.cls_first, tmp.cls_count);
     hipStream_t s = ctx->stream;
     uint32_t* d_order = (uint32_t*)ctx->dev.take(order.size() * 4);
     rmh::TileItem* d_tiles = (rmh::TileItem*)ctx->dev.take(tiles.size() * sizeof(rmh::TileItem));

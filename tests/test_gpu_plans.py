@@ -125,3 +125,20 @@ def test_disable_latch(po):
     with pytest.raises(rmh.Unavailable):
         a.residuals(tus, coeff)
     a.close()
+
+
+def test_autotuned_residual_plan_is_bit_exact(acc, po):
+    hip = load_package().hip
+    tus, coeff, _ = W.residual_frame(9, width=640, height=368)
+    ref = po.residual_batch(tus, coeff)
+    plan = acc.residual_plan(tus, coeff.size)
+    dc = hip.DeviceBuffer.from_array(coeff)
+    dr = hip.DeviceBuffer(coeff.nbytes)
+    s = hip.Stream()
+    splits = plan.autotune(dc.ptr, dr.ptr, s.handle, reps=2)
+    assert set(splits.values()) <= {1, 2, 4, 8}
+    dr.upload(np.zeros_like(coeff))
+    plan.run(dc.ptr, dr.ptr, s.handle)
+    s.synchronize()
+    assert (dr.download(np.int16) == ref).all()
+    plan.close()
