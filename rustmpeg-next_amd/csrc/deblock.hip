@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include "kernels.h"
+#include "xcd.h"
 
 namespace rmh {
 namespace {
@@ -211,12 +212,15 @@ __device__ __forceinline__ void luma_filter_line(int* line, const LumaDecision& 
 __device__ __forceinline__ int luma_np(const LumaDecision& d) { return d.de == 3 ? d.lp : d.de == 2 ? 3 : 2; }
 __device__ __forceinline__ int luma_nq(const LumaDecision& d) { return d.de == 3 ? d.lq : d.de == 2 ? 3 : 2; }
 
-// deblock_chroma_edge with two lines (vvc/filter.rs:935-1057). Returns the
-// number of samples per side that may have changed.
-__device__ __forceinline__ int chroma_segment(int (*L)[8], int qp, int bs, int beta_off, int tc_off, int len_p, int len_q, int bd) {
+// deblock_chroma_edge (vvc/filter.rs:935-1057) for one line m (p3..q3) of a
+// two-line segment whose other line is o; the decision depends on both lines
+// symmetrically, so both lanes of a segment reach the same one. Returns
+// false when the segment is not filtered.
+__device__ __forceinline__ bool chroma_line(int* m, const int* o, int qp, int bs, int beta_off, int tc_off, int len_p,
+                                            int len_q, int bd) {
     len_p = len_p < 3 ? len_p : 3;
     len_q = len_q < 3 ? len_q : 3;
-    if (len_p == 1 && len_q == 1 && bs != 2) return 0;
+    if (len_p == 1 && len_q == 1 && bs != 2) return false;
     const int maxv = (1 << bd) - 1;
     int tc, beta;
     tc_beta(qp, bs, beta_off, tc_off, bd, tc, beta);
@@ -224,8 +228,8 @@ __device__ __forceinline__ int chroma_segment(int (*L)[8], int qp, int bs, int b
         int a[8], b[8];
 #pragma unroll
         for (int k = 0; k < 8; k++) {
-            a[k] = L[0][k];
-            b[k] = L[1][k];
+            a[k] = m[k];
+            b[k] = o[k];
         }
         if (len_p == 1) {
             a[0] = a[1] = a[2];
@@ -240,31 +244,27 @@ __device__ __forceinline__ int chroma_segment(int (*L)[8], int qp, int bs, int b
         bool strong = (dpq0 + dpq1) < beta && cdec(a[3], a[0], a[4], a[7], 2 * dpq0) && cdec(b[3], b[0], b[4], b[7], 2 * dpq1);
         if (!strong) len_p = len_q = 1;
     }
+    int v[8];
 #pragma unroll
-    for (int li = 0; li < 2; li++) {
-        int* l = L[li];
-        int m[8];
-#pragma unroll
-        for (int k = 0; k < 8; k++) m[k] = l[k];
-        if (len_p == 3 && len_q == 3) {
-            l[1] = clampi(clampi((3 * m[0] + 2 * m[1] + m[2] + m[3] + m[4] + 4) >> 3, m[1] - tc, m[1] + tc), 0, maxv);
-            l[2] = clampi(clampi((2 * m[0] + m[1] + 2 * m[2] + m[3] + m[4] + m[5] + 4) >> 3, m[2] - tc, m[2] + tc), 0, maxv);
-            l[3] = clampi(clampi((m[0] + m[1] + m[2] + 2 * m[3] + m[4] + m[5] + m[6] + 4) >> 3, m[3] - tc, m[3] + tc), 0, maxv);
-            l[4] = clampi(clampi((m[1] + m[2] + m[3] + 2 * m[4] + m[5] + m[6] + m[7] + 4) >> 3, m[4] - tc, m[4] + tc), 0, maxv);
-            l[5] = clampi(clampi((m[2] + m[3] + m[4] + 2 * m[5] + m[6] + 2 * m[7] + 4) >> 3, m[5] - tc, m[5] + tc), 0, maxv);
-            l[6] = clampi(clampi((m[3] + m[4] + m[5] + 2 * m[6] + 3 * m[7] + 4) >> 3, m[6] - tc, m[6] + tc), 0, maxv);
-        } else if (len_p == 1 && len_q == 3) {
-            l[3] = clampi(clampi((3 * m[2] + 2 * m[3] + m[4] + m[5] + m[6] + 4) >> 3, m[3] - tc, m[3] + tc), 0, maxv);
-            l[4] = clampi(clampi((2 * m[2] + m[3] + 2 * m[4] + m[5] + m[6] + m[7] + 4) >> 3, m[4] - tc, m[4] + tc), 0, maxv);
-            l[5] = clampi(clampi((m[2] + m[3] + m[4] + 2 * m[5] + m[6] + 2 * m[7] + 4) >> 3, m[5] - tc, m[5] + tc), 0, maxv);
-            l[6] = clampi(clampi((m[3] + m[4] + m[5] + 2 * m[6] + 3 * m[7] + 4) >> 3, m[6] - tc, m[6] + tc), 0, maxv);
-        } else {
-            int delta = clampi((4 * (m[4] - m[3]) + m[2] - m[5] + 4) >> 3, -tc, tc);
-            l[3] = clampi(m[3] + delta, 0, maxv);
-            l[4] = clampi(m[4] - delta, 0, maxv);
-        }
+    for (int k = 0; k < 8; k++) v[k] = m[k];
+    if (len_p == 3 && len_q == 3) {
+        m[1] = clampi(clampi((3 * v[0] + 2 * v[1] + v[2] + v[3] + v[4] + 4) >> 3, v[1] - tc, v[1] + tc), 0, maxv);
+        m[2] = clampi(clampi((2 * v[0] + v[1] + 2 * v[2] + v[3] + v[4] + v[5] + 4) >> 3, v[2] - tc, v[2] + tc), 0, maxv);
+        m[3] = clampi(clampi((v[0] + v[1] + v[2] + 2 * v[3] + v[4] + v[5] + v[6] + 4) >> 3, v[3] - tc, v[3] + tc), 0, maxv);
+        m[4] = clampi(clampi((v[1] + v[2] + v[3] + 2 * v[4] + v[5] + v[6] + v[7] + 4) >> 3, v[4] - tc, v[4] + tc), 0, maxv);
+        m[5] = clampi(clampi((v[2] + v[3] + v[4] + 2 * v[5] + v[6] + 2 * v[7] + 4) >> 3, v[5] - tc, v[5] + tc), 0, maxv);
+        m[6] = clampi(clampi((v[3] + v[4] + v[5] + 2 * v[6] + 3 * v[7] + 4) >> 3, v[6] - tc, v[6] + tc), 0, maxv);
+    } else if (len_p == 1 && len_q == 3) {
+        m[3] = clampi(clampi((3 * v[2] + 2 * v[3] + v[4] + v[5] + v[6] + 4) >> 3, v[3] - tc, v[3] + tc), 0, maxv);
+        m[4] = clampi(clampi((2 * v[2] + v[3] + 2 * v[4] + v[5] + v[6] + v[7] + 4) >> 3, v[4] - tc, v[4] + tc), 0, maxv);
+        m[5] = clampi(clampi((v[2] + v[3] + v[4] + 2 * v[5] + v[6] + 2 * v[7] + 4) >> 3, v[5] - tc, v[5] + tc), 0, maxv);
+        m[6] = clampi(clampi((v[3] + v[4] + v[5] + 2 * v[6] + 3 * v[7] + 4) >> 3, v[6] - tc, v[6] + tc), 0, maxv);
+    } else {
+        int delta = clampi((4 * (v[4] - v[3]) + v[2] - v[5] + 4) >> 3, -tc, tc);
+        m[3] = clampi(v[3] + delta, 0, maxv);
+        m[4] = clampi(v[4] - delta, 0, maxv);
     }
-    return 3;
+    return true;
 }
 
 // ---------------------------------------------------------------------------
@@ -426,81 +426,85 @@ __device__ __forceinline__ uint8_t slot_edge(const TileCtx& t, int dir, int s, i
     return (e & RMH_EDGE_BS_MASK) ? e : 0;
 }
 
-// Filters one luma segment of the LDS window (dir 0: vertical edge at x,
-// rows y .. y + 3; dir 1: horizontal edge at y, columns x .. x + 3). Lines 0
-// and 3 are read for the decision, the others when filtered; a line whose
-// column repeats the previous one (clamped at the right picture edge) gives
-// the same result and is skipped.
-__device__ __forceinline__ void luma_slot(const TileCtx& t, int dir, int x, int y, uint8_t e, int16_t* lw) {
+// Lane exchanges inside a quad (lanes 4m .. 4m + 3) and a pair (2m, 2m + 1)
+// of a wave; every lane of the quad or pair must be active.
+__device__ __forceinline__ int quad_lane0(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x00, 0xf, 0xf, false); }
+__device__ __forceinline__ int quad_lane3(int v) { return __builtin_amdgcn_update_dpp(0, v, 0xff, 0xf, 0xf, false); }
+__device__ __forceinline__ int pair_other(int v) { return __builtin_amdgcn_update_dpp(0, v, 0xb1, 0xf, 0xf, false); }
+
+// Line li (0..3) of one luma segment of the LDS window (dir 0: vertical edge
+// at x, row y + li; dir 1: horizontal edge at y, column x + li, clamped into
+// the picture as the reference's gather clamps it). The four lanes of a quad
+// hold the four lines of a segment: lines 0 and 3, which the decision reads,
+// come from the quad's lanes 0 and 3. A line whose column repeats the
+// previous one gives the same result and is not written twice.
+__device__ __forceinline__ void luma_line(const TileCtx& t, int dir, int x, int y, uint8_t e, int li, int16_t* lw) {
     const rmh_dbk_frame& f = *t.f;
     const int bs = e & RMH_EDGE_BS_MASK;
     const int len_p = 2 * ((e >> RMH_EDGE_LENP_SHIFT) & 3) + 1, len_q = 2 * ((e >> RMH_EDGE_LENQ_SHIFT) & 3) + 1;
-    // element (line li, position k) at base[cols[li] + k * ks]
-    int16_t* base;
+    int16_t* base;  // element k of the line at base[k * ks]
     int ks;
-    int cols[4];
+    int line[16];
     if (dir == 0) {
-        base = lw + (y - (t.Y0 - 8)) * kLWW + (x - t.X0);  // column x - 8 of the window row
+        base = lw + (y + li - (t.Y0 - 8)) * kLWW + (x - t.X0);  // column x - 8, 8-byte aligned
         ks = 1;
+        const uint2* b8 = reinterpret_cast<const uint2*>(base);
 #pragma unroll
-        for (int li = 0; li < 4; li++) cols[li] = li * kLWW;
+        for (int k = 0; k < 4; k++) {
+            const uint2 v = b8[k];
+            line[4 * k] = (int)(v.x & 0xffff);
+            line[4 * k + 1] = (int)(v.x >> 16);
+            line[4 * k + 2] = (int)(v.y & 0xffff);
+            line[4 * k + 3] = (int)(v.y >> 16);
+        }
     } else {
-        // line li is column x + li (clamped into the picture as the gather does); k runs down rows y - 8 ..
-        base = lw + (y - t.Y0) * kLWW + 8;
+        base = lw + (y - t.Y0) * kLWW + 8 + (min(x + li, t.W - 1) - t.X0);  // row y - 8
         ks = kLWW;
 #pragma unroll
-        for (int li = 0; li < 4; li++) cols[li] = min(x + li, t.W - 1) - t.X0;
+        for (int k = 0; k < 16; k++) line[k] = base[k * kLWW];
     }
     int l0[16], l3[16];
 #pragma unroll
     for (int k = 0; k < 16; k++) {
-        l0[k] = base[cols[0] + k * ks];
-        l3[k] = base[cols[3] + k * ks];
+        l0[k] = quad_lane0(line[k]);
+        l3[k] = quad_lane3(line[k]);
     }
     LumaDecision d;
     if (!luma_decide(l0, l3, f.qp[0], bs, f.beta_offset_div2[0], f.tc_offset_div2[0], t.bd, len_p, len_q,
                      dir == 1 && (e & RMH_EDGE_CTB_ROW) != 0, d))
         return;
+    if (dir == 1 && li > 0 && x + li > t.W - 1) return;
     const int np = luma_np(d), nq = luma_nq(d), maxv = (1 << t.bd) - 1;
+    luma_filter_line(line, d, maxv);
 #pragma unroll
-    for (int li = 0; li < 4; li++) {
-        if (li > 0 && cols[li] == cols[li - 1]) continue;
-        int line[16];
-#pragma unroll
-        for (int k = 0; k < 16; k++) line[k] = li == 0 ? l0[k] : li == 3 ? l3[k] : base[cols[li] + k * ks];
-        luma_filter_line(line, d, maxv);
-#pragma unroll
-        for (int k = 1; k < 15; k++)
-            if (k >= 8 - np && k < 8 + nq) base[cols[li] + k * ks] = (int16_t)line[k];
-    }
+    for (int k = 1; k < 15; k++)
+        if (k >= 8 - np && k < 8 + nq) base[k * ks] = (int16_t)line[k];
 }
 
-__device__ __forceinline__ void chroma_slot(const TileCtx& t, int dir, int plane, int x, int y, uint8_t e, int16_t* cw) {
+// Line li (0, 1) of one chroma segment; the two lanes of a pair hold its two lines.
+__device__ __forceinline__ void chroma_line_lds(const TileCtx& t, int dir, int plane, int x, int y, uint8_t e, int li,
+                                                int16_t* cw) {
     const rmh_dbk_frame& f = *t.f;
     const int bs = e & RMH_EDGE_BS_MASK;
     const int len_p = 2 * ((e >> RMH_EDGE_LENP_SHIFT) & 3) + 1, len_q = 2 * ((e >> RMH_EDGE_LENQ_SHIFT) & 3) + 1;
     int16_t* base;
-    int ls, ks;
+    int ks;
     if (dir == 0) {
-        base = cw + (y - (t.CY0 - 4)) * kCWW + (x - t.CX0);  // column x - 4
-        ls = kCWW;
+        base = cw + (y + li - (t.CY0 - 4)) * kCWW + (x - t.CX0);  // column x - 4
         ks = 1;
     } else {
-        base = cw + (y - t.CY0) * kCWW + (x - t.CX0 + 4);    // row y - 4
-        ls = 1;
+        base = cw + (y - t.CY0) * kCWW + (x + li - t.CX0 + 4);    // row y - 4
         ks = kCWW;
     }
-    int L[2][8];
+    int m[8], o[8];
 #pragma unroll
-    for (int li = 0; li < 2; li++)
+    for (int k = 0; k < 8; k++) m[k] = base[k * ks];
 #pragma unroll
-        for (int k = 0; k < 8; k++) L[li][k] = base[li * ls + k * ks];
-    if (!chroma_segment(L, f.qp[plane], bs, f.beta_offset_div2[plane], f.tc_offset_div2[plane], len_p, len_q, t.bd))
+    for (int k = 0; k < 8; k++) o[k] = pair_other(m[k]);
+    if (!chroma_line(m, o, f.qp[plane], bs, f.beta_offset_div2[plane], f.tc_offset_div2[plane], len_p, len_q, t.bd))
         return;
 #pragma unroll
-    for (int li = 0; li < 2; li++)
-#pragma unroll
-        for (int k = 1; k < 7; k++) base[li * ls + k * ks] = (int16_t)L[li][k];
+    for (int k = 1; k < 7; k++) base[k * ks] = (int16_t)m[k];
 }
 
 // One pass (dir 0 vertical, 1 horizontal) over the tile: the slots with an
@@ -549,45 +553,56 @@ __device__ __forceinline__ void tile_pass(const TileCtx& t, int dir, int16_t* lw
         }
     }
     __syncthreads();
+    // one lane per line: quads of lanes per luma segment, pairs per chroma
+    // segment (256 is a multiple of 4, so a quad or pair is active as a whole)
     const int n_l = counts[0], n_c = counts[1];
-    for (int i = threadIdx.x; i < n_l; i += 256) {
+    for (int i = threadIdx.x; i < 4 * n_l; i += 256) {
         int plane, x, y;
-        const uint32_t q = ql[i];
+        const uint32_t q = ql[i >> 2];
         slot_pos(t, dir, (int)(q & 0xffff), plane, x, y);
-        luma_slot(t, dir, x, y, (uint8_t)(q >> 16), lw);
+        luma_line(t, dir, x, y, (uint8_t)(q >> 16), i & 3, lw);
     }
-    for (int i = threadIdx.x; i < n_c; i += 256) {
+    for (int i = threadIdx.x; i < 2 * n_c; i += 256) {
         int plane, x, y;
-        const uint32_t q = qc[i];
+        const uint32_t q = qc[i >> 1];
         slot_pos(t, dir, (int)(q & 0xffff), plane, x, y);
-        chroma_slot(t, dir, plane, x, y, (uint8_t)(q >> 16), cw[plane - 1]);
+        chroma_line_lds(t, dir, plane, x, y, (uint8_t)(q >> 16), i & 1, cw[plane - 1]);
     }
     __syncthreads();
 }
 
+// One workgroup per tile; the grid covers exactly tile_start[n_frames] tiles,
+// numbered frame by frame in raster order and handed to the XCDs in contiguous
+// bands so that tiles sharing halo rows and columns share an L2.
 __global__ __launch_bounds__(256) void deblock_tile_kernel(const rmh_dbk_frame* __restrict__ frames,
-                                                           const rmh_plane* __restrict__ shadows) {
-    __shared__ int16_t lw[kLWH * kLWW];
+                                                           const rmh_plane* __restrict__ shadows,
+                                                           const uint32_t* __restrict__ tile_start, uint32_t n_frames) {
+    __shared__ __align__(16) int16_t lw[kLWH * kLWW];
     __shared__ int16_t cwin[2][kCWH * kCWW];
     __shared__ uint32_t ql[kMaxQ], qc[kMaxQ];
     __shared__ uint32_t cells[kEY * kEX];
     __shared__ int counts[2];
+    const uint32_t tile = xcd_local_block(blockIdx.x, gridDim.x);
+    uint32_t lo = 0, hi = n_frames;  // tile_start[lo] <= tile < tile_start[hi]
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (tile_start[mid] <= tile) lo = mid; else hi = mid;
+    }
     TileCtx t;
-    t.f = &frames[blockIdx.y];
+    t.f = &frames[lo];
     t.cells = cells;
     const rmh_plane& PY = t.f->planes[0];
     t.W = PY.width;
     t.H = PY.height;
-    const int tiles_x = (t.W + kTW - 1) / kTW, tiles_y = (t.H + kTH - 1) / kTH;
-    if ((int)blockIdx.x >= tiles_x * tiles_y) return;
-    t.X0 = (blockIdx.x % tiles_x) * kTW;
-    t.Y0 = (blockIdx.x / tiles_x) * kTH;
+    const int tiles_x = (t.W + kTW - 1) / kTW, k = (int)(tile - tile_start[lo]);
+    t.X0 = (k % tiles_x) * kTW;
+    t.Y0 = (k / tiles_x) * kTH;
     t.CX0 = t.X0 / 2;
     t.CY0 = t.Y0 / 2;
     t.CW = t.f->planes[1].width;
     t.CH = t.f->planes[1].height;
     t.bd = PY.bit_depth;
-    const rmh_plane* sh = shadows + blockIdx.y * 3;
+    const rmh_plane* sh = shadows + lo * 3;
 
     // 1. edge cells and windows
     if (threadIdx.x < kEY * kEX) {
@@ -636,13 +651,13 @@ uint32_t deblock_tiles(uint32_t width, uint32_t height) {
     return ((width + kTW - 1) / kTW) * ((height + kTH - 1) / kTH);
 }
 
-hipError_t launch_deblock(const rmh_dbk_frame* frames, const rmh_plane* shadows, uint32_t n_frames, uint32_t max_tiles,
-                          uint32_t max_rows, hipStream_t stream) {
-    if (n_frames == 0 || max_tiles == 0) return hipSuccess;
+hipError_t launch_deblock(const rmh_dbk_frame* frames, const rmh_plane* shadows, const uint32_t* tile_start,
+                          uint32_t n_frames, uint32_t total_tiles, uint32_t max_rows, hipStream_t stream) {
+    if (n_frames == 0 || total_tiles == 0) return hipSuccess;
     hipLaunchKernelGGL(shadow_copy_kernel, dim3((max_rows + 3) / 4, n_frames, 3), dim3(256), 0, stream, frames, shadows);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(deblock_tile_kernel, dim3(max_tiles, n_frames), dim3(256), 0, stream, frames, shadows);
+    hipLaunchKernelGGL(deblock_tile_kernel, dim3(total_tiles), dim3(256), 0, stream, frames, shadows, tile_start, n_frames);
     return hipGetLastError();
 }
 

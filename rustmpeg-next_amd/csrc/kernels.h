@@ -108,10 +108,11 @@ hipError_t launch_mc(const rmh_mc_frame* frames, const McItem* items, uint32_t n
 
 // Deblocking: a copy of every plane into `shadows` (3 per frame, same sizes,
 // device memory), then one tiled launch that reads the shadows and writes the
-// pictures. max_tiles = largest deblock_tiles() of the frames, max_rows =
-// largest luma height.
+// pictures. tile_start[n_frames + 1] (device memory) holds the running sum
+// of deblock_tiles() over the frames (tile_start[0] = 0, tile_start[n_frames]
+// = total_tiles); max_rows = largest luma height.
 uint32_t deblock_tiles(uint32_t width, uint32_t height);
-hipError_t launch_deblock(const rmh_dbk_frame* frames, const rmh_plane* shadows, uint32_t n_frames, uint32_t max_tiles,
-                          uint32_t max_rows, hipStream_t stream);
+hipError_t launch_deblock(const rmh_dbk_frame* frames, const rmh_plane* shadows, const uint32_t* tile_start,
+                          uint32_t n_frames, uint32_t total_tiles, uint32_t max_rows, hipStream_t stream);
 
 }  // namespace rmh

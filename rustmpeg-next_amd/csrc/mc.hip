@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include "kernels.h"
+#include "xcd.h"
 
 namespace rmh {
 namespace {
@@ -209,14 +210,6 @@ __device__ __forceinline__ void set_taps(Tile<TAPS>& t, const int8_t* cx, const 
     }
 #pragma unroll
     for (int k = 0; k < TAPS; k++) t.sy[k] = cy[k];
-}
-
-// Workgroup index -> work index such that the workgroups one XCD runs
-// (hardware ids k, k + 8, k + 16, ... go to XCD k) take a contiguous run of
-// items: neighbouring tiles of a PU then share that XCD's L2.
-__device__ __forceinline__ uint32_t xcd_local_block(uint32_t bid, uint32_t nb) {
-    const uint32_t x = bid & 7, q = nb >> 3, r = nb & 7;
-    return x * q + (x < r ? x : r) + (bid >> 3);
 }
 
 // Four work items per 256-lane workgroup, one per wave: the 16x16 luma tile

@@ -117,8 +117,9 @@ struct rmh_plan {
     // deblock
     const rmh_dbk_frame* d_dbk = nullptr;
     const rmh_plane* d_shadows = nullptr;  // 3 per frame: copies the tiled kernel reads
+    const uint32_t* d_tile_start = nullptr;  // [n_dbk + 1] running sum of tiles per frame
     size_t n_dbk = 0;
-    uint32_t max_tiles = 0, max_rows = 0;
+    uint32_t total_tiles = 0, max_rows = 0;
 };
 
 struct rmh_graph {
@@ -542,6 +543,14 @@ std::vector<rmh_plane> shadow_layout(const rmh_dbk_frame* frames, size_t n_frame
     return out;
 }
 
+// Running sum of the tiles of the tiled deblock launch, frame by frame.
+std::vector<uint32_t> tile_starts(const rmh_dbk_frame* frames, size_t n_frames) {
+    std::vector<uint32_t> t(n_frames + 1, 0);
+    for (size_t f = 0; f < n_frames; f++)
+        t[f + 1] = t[f] + rmh::deblock_tiles(frames[f].planes[0].width, frames[f].planes[0].height);
+    return t;
+}
+
 
 // TUs the tiled kernel handles: 2-D with both sides >= 4, no LFNST, no
 // dependent quantisation, no transform skip, 8-byte aligned block.
@@ -704,7 +713,8 @@ rmh_status run_plan_locked_free(rmh_plan* plan, const int16_t* d_coeff, int16_t*
     } else if (plan->kind == kMc) {
         e = rmh::launch_mc(plan->d_frames, plan->d_items, (uint32_t)plan->n_items, s);
     } else {
-        e = rmh::launch_deblock(plan->d_dbk, plan->d_shadows, (uint32_t)plan->n_dbk, plan->max_tiles, plan->max_rows, s);
+        e = rmh::launch_deblock(plan->d_dbk, plan->d_shadows, plan->d_tile_start, (uint32_t)plan->n_dbk, plan->total_tiles,
+                                plan->max_rows, s);
     }
     if (e != hipSuccess) {
         *err = std::string("kernel launch failed: ") + hipGetErrorString(e);
@@ -1203,21 +1213,23 @@ RMH_EXPORT rmh_status rmh_deblock_plan_create(rmh_ctx* ctx, const rmh_dbk_frame*
     p->ctx = ctx;
     p->kind = kDeblock;
     p->n_dbk = n_frames;
+    const std::vector<uint32_t> starts = tile_starts(frames, n_frames);
+    p->total_tiles = starts.back();
     for (size_t f = 0; f < n_frames; f++) {
         uint32_t cells = frames[f].grid_w * frames[f].grid_h;
-        p->max_tiles = std::max(p->max_tiles, rmh::deblock_tiles(frames[f].planes[0].width, frames[f].planes[0].height));
         p->max_rows = std::max(p->max_rows, frames[f].planes[0].height);
         for (int c = 0; c < 3; c++) p->bytes += 2 * plane_bytes(frames[f].planes[c]);
         p->bytes += 4ull * cells;
     }
     size_t fb = n_frames * sizeof(rmh_dbk_frame), sb = n_frames * 3 * sizeof(rmh_plane);
-    size_t shb = shadow_bytes(frames, n_frames);
-    hipError_t e = hipMalloc(&p->d_mem, aligned(fb) + aligned(sb) + shb + 256);
+    size_t tb = starts.size() * sizeof(uint32_t), shb = shadow_bytes(frames, n_frames);
+    hipError_t e = hipMalloc(&p->d_mem, aligned(fb) + aligned(sb) + aligned(tb) + shb + 256);
     char* base = (char*)p->d_mem;
     if (e == hipSuccess) {
-        std::vector<rmh_plane> sh = shadow_layout(frames, n_frames, base + aligned(fb) + aligned(sb));
+        std::vector<rmh_plane> sh = shadow_layout(frames, n_frames, base + aligned(fb) + aligned(sb) + aligned(tb));
         if (fb) e = hipMemcpy(base, frames, fb, hipMemcpyHostToDevice);
         if (e == hipSuccess && sb) e = hipMemcpy(base + aligned(fb), sh.data(), sb, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(base + aligned(fb) + aligned(sb), starts.data(), tb, hipMemcpyHostToDevice);
     }
     if (e != hipSuccess) {
         if (p->d_mem) hipFree(p->d_mem);
@@ -1226,6 +1238,7 @@ RMH_EXPORT rmh_status rmh_deblock_plan_create(rmh_ctx* ctx, const rmh_dbk_frame*
     }
     p->d_dbk = (const rmh_dbk_frame*)base;
     p->d_shadows = (const rmh_plane*)(base + aligned(fb));
+    p->d_tile_start = (const uint32_t*)(base + aligned(fb) + aligned(sb));
     *out = p;
     return RMH_OK;
 }
@@ -1246,11 +1259,12 @@ RMH_EXPORT rmh_status rmh_deblock(rmh_ctx* ctx, const rmh_dbk_frame* frames, siz
     if (g != RMH_OK) return fail_locked(ctx, g, g == RMH_UNAVAILABLE ? "accelerator disabled" : "injected device failure");
     DeviceGuard dg(ctx->device);
     size_t fb = n_frames * sizeof(rmh_dbk_frame), sb = n_frames * 3 * sizeof(rmh_plane);
-    size_t need = aligned(fb) + aligned(sb) + shadow_bytes(frames, n_frames), hneed = 0;
-    uint32_t max_tiles = 0, max_rows = 0;
+    const std::vector<uint32_t> starts = tile_starts(frames, n_frames);
+    const size_t tb = starts.size() * sizeof(uint32_t);
+    size_t need = aligned(fb) + aligned(sb) + aligned(tb) + shadow_bytes(frames, n_frames), hneed = 0;
+    uint32_t max_rows = 0;
     for (size_t f = 0; f < n_frames; f++) {
         uint32_t cells = frames[f].grid_w * frames[f].grid_h;
-        max_tiles = std::max(max_tiles, rmh::deblock_tiles(frames[f].planes[0].width, frames[f].planes[0].height));
         max_rows = std::max(max_rows, frames[f].planes[0].height);
         need += aligned(4ull * cells);
         for (int c = 0; c < 3; c++) {
@@ -1265,6 +1279,9 @@ RMH_EXPORT rmh_status rmh_deblock(rmh_ctx* ctx, const rmh_dbk_frame* frames, siz
     hipStream_t s = ctx->stream;
     rmh_dbk_frame* d_frames = (rmh_dbk_frame*)ctx->dev.take(fb);
     rmh_plane* d_shadows = (rmh_plane*)ctx->dev.take(sb);
+    uint32_t* d_starts = (uint32_t*)ctx->dev.take(tb);
+    if ((e = hipMemcpyAsync(d_starts, starts.data(), tb, hipMemcpyHostToDevice, s)) != hipSuccess)
+        return hip_fail_locked(ctx, e, "tile table upload");
     const std::vector<rmh_plane> sh = shadow_layout(frames, n_frames, (char*)ctx->dev.take(shadow_bytes(frames, n_frames)));
     if ((e = hipMemcpyAsync(d_shadows, sh.data(), sb, hipMemcpyHostToDevice, s)) != hipSuccess)
         return hip_fail_locked(ctx, e, "shadow upload");
@@ -1286,7 +1303,7 @@ RMH_EXPORT rmh_status rmh_deblock(rmh_ctx* ctx, const rmh_dbk_frame* frames, siz
         }
     }
     if ((e = hipMemcpyAsync(d_frames, df.data(), fb, hipMemcpyHostToDevice, s)) != hipSuccess) return hip_fail_locked(ctx, e, "upload");
-    e = rmh::launch_deblock(d_frames, d_shadows, (uint32_t)n_frames, max_tiles, max_rows, s);
+    e = rmh::launch_deblock(d_frames, d_shadows, d_starts, (uint32_t)n_frames, starts.back(), max_rows, s);
     if (e != hipSuccess) return hip_fail_locked(ctx, e, "kernel launch");
     std::vector<void*> hp;
     for (size_t f = 0; f < n_frames; f++)
