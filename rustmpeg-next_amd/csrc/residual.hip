@@ -20,6 +20,8 @@
 //      at most 32 taps keep every sum below 2^27, so int32 is exact.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "kernels.h"
 
 namespace rmh {
@@ -434,6 +436,37 @@ __device__ __forceinline__ int dot_pairs2(const uint32_t* __restrict__ src, cons
     return acc;
 }
 
+// acc + the dot product of NQ 8-byte groups (2 pairs each) of src with
+// m[0 .. 2 NQ); every read is issued before the first product.
+template <int NQ>
+__device__ __forceinline__ int dot_groups(const uint2* __restrict__ src, const uint32_t (&m)[16], int acc) {
+    if constexpr (NQ == 0) return acc;
+    uint2 v[NQ > 0 ? NQ : 1];
+#pragma unroll
+    for (int q = 0; q < NQ; q++) v[q] = src[q];
+#pragma unroll
+    for (int q = 0; q < NQ; q++) {
+        acc = dot2(v[q].x, m[2 * q], acc);
+        acc = dot2(v[q].y, m[2 * q + 1], acc);
+    }
+    return acc;
+}
+
+// f(std::integral_constant<int, Q>) for the smallest Q in [Q0, QMAX] with
+// nq <= Q (QMAX when none; nq never exceeds QMAX here): one wave-uniform branch selects a fully unrolled
+// body instead of a branch per group inside it.
+template <int Q0, int QMAX, typename F>
+__device__ __forceinline__ void with_groups(int nq, F& f) {
+    if constexpr (Q0 >= QMAX) {
+        f(std::integral_constant<int, QMAX>{});
+    } else {
+        if (nq <= Q0)
+            f(std::integral_constant<int, Q0>{});
+        else
+            with_groups<Q0 + 1, QMAX>(nq, f);
+    }
+}
+
 __device__ __forceinline__ int wave_max_u(int v) { return __builtin_amdgcn_readfirstlane(wave_max(v)); }
 
 // 16 matrix pairs of one column (zero past the matrix) with four 128-bit loads.
@@ -541,36 +574,34 @@ __device__ __forceinline__ void tile_body(const ResidualArgs& a, const TileItem&
     // columns up to a multiple of 4 (zeros past the box, from zero coefficients)
     const int k1 = (er + 1) >> 1, ec4 = (ec + 3) & ~3, k2 = (ec + 1) >> 1;
 
-    // phase B: vertical pass, lane = (TU, row y, column chunk)
+    // phase B: vertical pass, lane = (TU, row y, column chunk); rows past
+    // the bounding box are read in groups of 4 (zeros, from zero coefficients)
     if (ec > 0) {
         constexpr int NCH0 = 256 / (GMAX * H);
         constexpr int NCH = NCH0 > 0 ? NCH0 : 1;
         constexpr int ITEMS = GMAX * H * NCH;
         int16_t* T16 = reinterpret_cast<int16_t*>(T);
+        auto vpass = [&](auto nq) {
+            constexpr int NQ = decltype(nq)::value;
 #pragma unroll
-        for (int base = 0; base < ITEMS; base += 256) {
-            const int q = base + tid;
-            const int y = q & (H - 1);
-            const int c0 = (q / H) % NCH;
-            const int g = q / (H * NCH);
-            if (g < G) {
-                uint32_t m[16];
-                load_col16(a.col16 + P[g].mv + y * 16, m);
-                for (int i = c0; i < ec4; i += NCH) {
-                    const uint2* src = reinterpret_cast<const uint2*>(C + (g * Wc + i) * CP);
-                    int acc = 0;
-#pragma unroll
-                    for (int q2 = 0; q2 < CP / 2; q2++) {
-                        if (2 * q2 < k1) {
-                            const uint2 v = src[q2];
-                            acc = dot2(v.x, m[2 * q2], acc);
-                            acc = dot2(v.y, m[2 * q2 + 1], acc);
-                        }
+            for (int base = 0; base < ITEMS; base += 256) {
+                const int q = base + tid;
+                const int y = q & (H - 1);
+                const int c0 = (q / H) % NCH;
+                const int g = q / (H * NCH);
+                if (g < G) {
+                    uint32_t m[16];
+                    load_col16(a.col16 + P[g].mv + y * 16, m);
+                    const uint2* src = reinterpret_cast<const uint2*>(C + g * Wc * CP);
+                    int16_t* dst = T16 + (g * H + y) * TP * 2;
+                    for (int i = c0; i < ec4; i += NCH) {
+                        const int acc = dot_groups<NQ>(src + i * (CP / 2), m, 64);
+                        dst[i] = (int16_t)clip16i(acc >> 7);
                     }
-                    T16[(g * H + y) * TP * 2 + i] = (int16_t)clip16i((acc + 64) >> 7);
                 }
             }
-        }
+        };
+        with_groups<1, (CP / 2 > 0 ? CP / 2 : 1)>((k1 + 1) >> 1, vpass);
     }
     __syncthreads();
 
@@ -580,38 +611,36 @@ __device__ __forceinline__ void tile_body(const ResidualArgs& a, const TileItem&
         constexpr int NR0 = 256 / (GMAX * W);
         constexpr int NR = NR0 > 0 ? NR0 : 1;
         constexpr int ITEMS = GMAX * W * NR;
+        auto hpass = [&](auto nq) {
+            constexpr int NQ = decltype(nq)::value;
 #pragma unroll
-        for (int base = 0; base < ITEMS; base += 256) {
-            const int q = base + tid;
-            const int x = q & (W - 1);
-            const int r0 = (q / W) % NR;
-            const int g = q / (W * NR);
-            if (g < G) {
-                const TuParams p = P[g];
-                uint32_t m[16];
-                load_col16(a.col16 + p.mh + x * 16, m);
-                const int shift2 = p.shifts >> 16;
-                const int rnd = 1 << (shift2 - 1);
-                const ReconDst rd{p.base, p.stride, p.bpsmax & 255, p.bpsmax >> 8};
-                for (int y = r0; y < H; y += NR) {
-                    const uint2* src = reinterpret_cast<const uint2*>(T + (g * H + y) * TP);
-                    int acc = 0;
-#pragma unroll
-                    for (int q2 = 0; q2 < TP / 2; q2++) {
-                        if (2 * q2 < k2) {
-                            const uint2 v = src[q2];
-                            acc = dot2(v.x, m[2 * q2], acc);
-                            acc = dot2(v.y, m[2 * q2 + 1], acc);
-                        }
+            for (int base = 0; base < ITEMS; base += 256) {
+                const int q = base + tid;
+                const int x = q & (W - 1);
+                const int r0 = (q / W) % NR;
+                const int g = q / (W * NR);
+                if (g < G) {
+                    const TuParams p = P[g];
+                    uint32_t m[16];
+                    load_col16(a.col16 + p.mh + x * 16, m);
+                    const int shift2 = p.shifts >> 16;
+                    const int rnd = 1 << (shift2 - 1);
+                    const ReconDst rd{p.base, p.stride, p.bpsmax & 255, p.bpsmax >> 8};
+                    const uint2* src = reinterpret_cast<const uint2*>(T + g * H * TP);
+                    for (int y = r0; y < H; y += NR) {
+                        const int acc = dot_groups<NQ>(src + y * (TP / 2), m, rnd);
+                        const int r = min(max(acc >> shift2, -32768), 32767);
+                        if (RECON)
+                            recon_add(rd, x, y, r);
+                        else
+                            O[(g * H + y) * W + x] = (int16_t)r;
                     }
-                    const int r = min(max((acc + rnd) >> shift2, -32768), 32767);
-                    if (RECON)
-                        recon_add(rd, x, y, r);
-                    else
-                        O[(g * H + y) * W + x] = (int16_t)r;
                 }
             }
-        }
+        };
+        // no group at all for a tile without non-zero coefficients (phase B
+        // wrote no intermediates; every residual is zero)
+        with_groups<0, (TP / 2 > 0 ? TP / 2 : 1)>((k2 + 1) >> 1, hpass);
     }
     if (!RECON) {
         // the tile's TU blocks to HBM, 4 residuals (8 bytes) per store
