@@ -84,12 +84,18 @@ inline int residual_tile_tus(int lw, int lh) {
     int g = 4096 >> (lw + lh);
     return g > 0 ? g : 1;
 }
+// Dword pitch of a coefficient column (lc = log2 of the coded height) or an
+// intermediate row (lc = log2 of the coded width) in the tiled kernel's LDS:
+// 2^lc / 2 int16 pairs, padded to 2 mod 4 dwords.
+constexpr int residual_tile_pitch(int lc) { return lc <= 2 ? 2 : (1 << lc) / 2 + 2; }
+
 inline uint32_t residual_tile_lds(int lw, int lh) {
     uint32_t count = (uint32_t)residual_tile_tus(lw, lh);
-    int wc = (1 << lw) < 32 ? (1 << lw) : 32;
-    uint32_t c = count * (1u << (lw + lh)) * 2;      // coefficient columns, then the output tile
-    uint32_t t = count * (1u << lh) * (wc / 2) * 4;  // intermediate rows
-    return c + t + count * 40 + 16;                 // + per-TU parameters + tile bounding box
+    int lwc = lw < 5 ? lw : 5, lhc = lh < 5 ? lh : 5;
+    uint32_t coef = (1u << lwc) * residual_tile_pitch(lhc) * 4, out = (1u << (lw + lh)) * 2;
+    uint32_t c = count * (coef > out ? coef : out);                  // coefficient columns, then the output tile
+    uint32_t t = count * (1u << lh) * residual_tile_pitch(lwc) * 4;  // intermediate rows
+    return c + t + count * 40 + 16;                                  // + per-TU parameters + tile bounding box
 }
 
 // One inter work item: a 16x16 luma tile (and its 8x8 chroma tiles) of one

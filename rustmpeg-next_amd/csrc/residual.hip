@@ -410,30 +410,14 @@ struct TuParams {
     char* base;      // recon: sample (0, 0) of the block
 };
 
-// dequantize_rect_bd without branches: v = level * scale (|v| < 2^22), then
-// either (v + 2^(rs-1)) >> rs or v << ls, saturated to int16. For a left
-// shift v is first limited to +-2^16 (which saturates anyway), so the shift
-// stays inside int32 for ls <= 14.
-__device__ __forceinline__ int dequant_bf(int level, int scale, int ls, int rs) {
-    const int cl = ls ? 65536 : 0x7fffffff;
-    const int v = min(max(level * scale, -cl), cl);
-    return min(max(((v << ls) + ((1 << rs) >> 1)) >> rs, -32768), 32767);
-}
-
-// dot2 over pairs kk < npairs read 2 at a time (8-byte LDS reads); m[kk] = 0
-// past the live pairs.
-__device__ __forceinline__ int dot_pairs2(const uint32_t* __restrict__ src, const uint32_t (&m)[16], int npairs) {
-    const uint2* v2 = reinterpret_cast<const uint2*>(src);
-    int acc = 0;
-#pragma unroll
-    for (int q = 0; q < 8; q++) {
-        if (2 * q < npairs) {
-            uint2 v = v2[q];
-            acc = dot2(v.x, m[2 * q + 0], acc);
-            acc = dot2(v.y, m[2 * q + 1], acc);
-        }
-    }
-    return acc;
+// dequantize_rect_bd for one TU of the tiled kernel: with ls = 0, S = scale;
+// with 0 < ls <= 9, S = scale << ls (|level * S| <= 2^15 * 102 * 2^9 < 2^31,
+// exact); with ls >= 10, S = 2^16, since every non-zero level saturates
+// (|level * scale| << ls >= 40 * 2^10) and level * 2^16 saturates with the
+// same sign. rs = 0 whenever ls > 0. S < 2^17 and |level| <= 2^15 fit the
+// 24-bit multiplier.
+__device__ __forceinline__ int dequant_tile(int level, int S, int rnd, int rs) {
+    return min(max((__mul24(level, S & 0x1ffff) + rnd) >> rs, -32768), 32767);
 }
 
 // acc + the dot product of NQ 8-byte groups (2 pairs each) of src with
@@ -487,14 +471,19 @@ __device__ __forceinline__ void tile_body(const ResidualArgs& a, const TileItem&
     constexpr int W = 1 << LW, H = 1 << LH;
     constexpr int LWC = LW < 5 ? LW : 5, LHC = LH < 5 ? LH : 5;
     constexpr int Wc = 1 << LWC, Hc = 1 << LHC;
-    constexpr int CP = Hc / 2, TP = Wc / 2;  // dwords per coefficient column / intermediate row
+    // dword pitch of a coefficient column / an intermediate row: the live
+    // pairs, padded to 2 mod 4 dwords so that the 8 column quads of a phase A
+    // store, or 32 consecutive rows of a phase B store, fall in distinct
+    // banks; even, for 8-byte reads
+    constexpr int CP = residual_tile_pitch(LHC), TP = residual_tile_pitch(LWC);
     constexpr int GMAX = (4096 >> (LW + LH)) > 0 ? (4096 >> (LW + LH)) : 1;
+    constexpr int R0 = (W * H / 2 > Wc * CP) ? W * H / 2 : Wc * CP;  // region 0 dwords per TU
     const int G = it.count;
     // LDS: region 0 holds the coefficients (phases A, B), then the output
     // tile (phase C); then the intermediates; then the per-TU parameters.
     uint32_t* C = smem;
     int16_t* O = reinterpret_cast<int16_t*>(smem);
-    uint32_t* T = C + GMAX * W * H / 2;
+    uint32_t* T = C + GMAX * R0;
     TuParams* P = reinterpret_cast<TuParams*>(T + GMAX * H * TP);
     int* TB = reinterpret_cast<int*>(P + GMAX);  // tile bounding box: max row, max column of non-zeros
     const int tid = threadIdx.x;
@@ -505,8 +494,10 @@ __device__ __forceinline__ void tile_body(const ResidualArgs& a, const TileItem&
         TuParams p;
         p.off = (int32_t)d.off;
         p.cwch = d.cw | (d.ch << 8);
-        p.scale = d.scale;
-        p.shifts = (d.shifts & 15) | ((d.shifts >> 4) << 8) | (d.shift2 << 16);
+        // dequantisation as level * S + rnd >> rs (see dequant_tile)
+        const int ls = d.shifts & 15;
+        p.scale = ls >= 10 ? 65536 : (int)d.scale << ls;
+        p.shifts = (ls ? 0 : (d.shifts >> 4)) | (d.shift2 << 16);
         p.mh = d.mh;
         p.mv = d.mv;
         p.stride = 0;
@@ -533,31 +524,34 @@ __device__ __forceinline__ void tile_body(const ResidualArgs& a, const TileItem&
             const int x0 = (e % Q) * 4;
             const int rp = (e / Q) % RP;
             const int g = e / (Q * RP);
-            const TuParams p = P[g];
-            const int cw = p.cwch & 255, ch = p.cwch >> 8;
-            const int ls = p.shifts & 15, rs = (p.shifts >> 8) & 31;
+            const int off = P[g].off, cwch = P[g].cwch, S = P[g].scale, rs = P[g].shifts & 31;
+            const int cw = cwch & 255, ch = cwch >> 8, rnd = (1 << rs) >> 1;
             const int r0 = 2 * rp;
-            uint2 raw0 = make_uint2(0, 0), raw1 = make_uint2(0, 0);
+            uint64_t raw0 = 0, raw1 = 0;
             if (x0 < cw) {
-                if (r0 < ch) raw0 = *reinterpret_cast<const uint2*>(a.coeff + p.off + r0 * W + x0);
-                if (r0 + 1 < ch) raw1 = *reinterpret_cast<const uint2*>(a.coeff + p.off + (r0 + 1) * W + x0);
+                if (r0 < ch) raw0 = *reinterpret_cast<const uint64_t*>(a.coeff + off + r0 * W + x0);
+                if (r0 + 1 < ch) raw1 = *reinterpret_cast<const uint64_t*>(a.coeff + off + (r0 + 1) * W + x0);
+                // columns past the coded region read as zero levels
+                const int nv = cw - x0;
+                if (nv < 4) {
+                    const uint64_t keep = (1ull << (16 * nv)) - 1;
+                    raw0 &= keep;
+                    raw1 &= keep;
+                }
             }
-            const int lo[4] = {(int)(int16_t)(raw0.x & 0xffff), (int)(int16_t)(raw0.x >> 16),
-                               (int)(int16_t)(raw0.y & 0xffff), (int)(int16_t)(raw0.y >> 16)};
-            const int hi[4] = {(int)(int16_t)(raw1.x & 0xffff), (int)(int16_t)(raw1.x >> 16),
-                               (int)(int16_t)(raw1.y & 0xffff), (int)(int16_t)(raw1.y >> 16)};
+            // bounding box of the non-zero levels (a zero level dequantises to zero)
+            const uint64_t z = raw0 | raw1;
+            if (z) mc = max(mc, x0 + ((63 - __clzll(z)) >> 4));
+            mr = max(mr, raw1 ? r0 + 1 : raw0 ? r0 : -1);
+            const uint32_t w0[2] = {(uint32_t)raw0, (uint32_t)(raw0 >> 32)};
+            const uint32_t w1[2] = {(uint32_t)raw1, (uint32_t)(raw1 >> 32)};
 #pragma unroll
             for (int k = 0; k < 4; k++) {
-                // columns past the coded region read as zero levels (dequantised to zero)
-                const bool inc = x0 + k < cw;
-                const int a0 = dequant_bf(inc ? lo[k] : 0, p.scale, ls, rs);
-                const int a1 = dequant_bf(inc ? hi[k] : 0, p.scale, ls, rs);
-                if (a0 | a1) {
-                    mc = x0 + k > mc ? x0 + k : mc;
-                    const int rr = a1 ? r0 + 1 : r0;
-                    mr = rr > mr ? rr : mr;
-                }
-                C[(g * Wc + x0 + k) * CP + rp] = (uint32_t)(uint16_t)a0 | ((uint32_t)(uint16_t)a1 << 16);
+                const uint32_t u0 = w0[k >> 1], u1 = w1[k >> 1];
+                const int l0 = (k & 1) ? (int)u0 >> 16 : (int)(int16_t)(u0 & 0xffff);
+                const int l1 = (k & 1) ? (int)u1 >> 16 : (int)(int16_t)(u1 & 0xffff);
+                const int a0 = dequant_tile(l0, S, rnd, rs), a1 = dequant_tile(l1, S, rnd, rs);
+                C[(g * Wc + x0 + k) * CP + rp] = __builtin_amdgcn_perm((uint32_t)a1, (uint32_t)a0, 0x05040100u);
             }
         }
         mr = wave_max(mr);
@@ -594,14 +588,17 @@ __device__ __forceinline__ void tile_body(const ResidualArgs& a, const TileItem&
                     load_col16(a.col16 + P[g].mv + y * 16, m);
                     const uint2* src = reinterpret_cast<const uint2*>(C + g * Wc * CP);
                     int16_t* dst = T16 + (g * H + y) * TP * 2;
-                    for (int i = c0; i < ec4; i += NCH) {
+                    // NCH (1, 2 or 4) divides ec4: the same trip count in every lane
+                    const int nc = ec4 / NCH;
+                    for (int j = 0; j < nc; j++) {
+                        const int i = c0 + j * NCH;
                         const int acc = dot_groups<NQ>(src + i * (CP / 2), m, 64);
                         dst[i] = (int16_t)clip16i(acc >> 7);
                     }
                 }
             }
         };
-        with_groups<1, (CP / 2 > 0 ? CP / 2 : 1)>((k1 + 1) >> 1, vpass);
+        with_groups<1, (Hc / 4 > 0 ? Hc / 4 : 1)>((k1 + 1) >> 1, vpass);
     }
     __syncthreads();
 
@@ -627,7 +624,9 @@ __device__ __forceinline__ void tile_body(const ResidualArgs& a, const TileItem&
                     const int rnd = 1 << (shift2 - 1);
                     const ReconDst rd{p.base, p.stride, p.bpsmax & 255, p.bpsmax >> 8};
                     const uint2* src = reinterpret_cast<const uint2*>(T + g * H * TP);
-                    for (int y = r0; y < H; y += NR) {
+#pragma unroll 2
+                    for (int j = 0; j < H / NR; j++) {
+                        const int y = r0 + j * NR;
                         const int acc = dot_groups<NQ>(src + y * (TP / 2), m, rnd);
                         const int r = min(max(acc >> shift2, -32768), 32767);
                         if (RECON)
@@ -640,7 +639,7 @@ __device__ __forceinline__ void tile_body(const ResidualArgs& a, const TileItem&
         };
         // no group at all for a tile without non-zero coefficients (phase B
         // wrote no intermediates; every residual is zero)
-        with_groups<0, (TP / 2 > 0 ? TP / 2 : 1)>((k2 + 1) >> 1, hpass);
+        with_groups<0, (Wc / 4 > 0 ? Wc / 4 : 1)>((k2 + 1) >> 1, hpass);
     }
     if (!RECON) {
         // the tile's TU blocks to HBM, 4 residuals (8 bytes) per store
