@@ -400,7 +400,7 @@ def main():
         traffic = None  # the committed counter run covers another batch size
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": "residual_kernel", "algorithmic_bytes_per_launch": plan.bytes}
+                "kernel": "residual_tile_kernel<false>", "algorithmic_bytes_per_launch": plan.bytes}
 
     sys.path.insert(0, os.path.join(REPO, "tests"))
     import pipeline as PL

@@ -399,7 +399,7 @@ __global__ __launch_bounds__(64) void residual_kernel(ResidualArgs a) {
 // ---------------------------------------------------------------------------
 
 struct TuParams {
-    int32_t off;
+    uint32_t off;    // first coefficient (offset + TU size <= n_coeff < 2^32)
     int32_t cwch;    // cw | ch << 8
     int32_t scale;
     int32_t shifts;  // dequant left shift | right shift << 8 | second-stage shift (20 - bd) << 16
@@ -492,7 +492,7 @@ __device__ __forceinline__ void tile_body(const ResidualArgs& a, const TileItem&
     for (int g = tid; g < G; g += 256) {
         const TileTu d = a.tile_tus[it.first + g];
         TuParams p;
-        p.off = (int32_t)d.off;
+        p.off = d.off;
         p.cwch = d.cw | (d.ch << 8);
         // dequantisation as level * S + rnd >> rs (see dequant_tile)
         const int ls = d.shifts & 15;
@@ -524,13 +524,14 @@ __device__ __forceinline__ void tile_body(const ResidualArgs& a, const TileItem&
             const int x0 = (e % Q) * 4;
             const int rp = (e / Q) % RP;
             const int g = e / (Q * RP);
-            const int off = P[g].off, cwch = P[g].cwch, S = P[g].scale, rs = P[g].shifts & 31;
+            const uint32_t off = P[g].off;
+            const int cwch = P[g].cwch, S = P[g].scale, rs = P[g].shifts & 31;
             const int cw = cwch & 255, ch = cwch >> 8, rnd = (1 << rs) >> 1;
             const int r0 = 2 * rp;
             uint64_t raw0 = 0, raw1 = 0;
             if (x0 < cw) {
-                if (r0 < ch) raw0 = *reinterpret_cast<const uint64_t*>(a.coeff + off + r0 * W + x0);
-                if (r0 + 1 < ch) raw1 = *reinterpret_cast<const uint64_t*>(a.coeff + off + (r0 + 1) * W + x0);
+                if (r0 < ch) raw0 = *reinterpret_cast<const uint64_t*>(a.coeff + (off + (uint32_t)(r0 * W + x0)));
+                if (r0 + 1 < ch) raw1 = *reinterpret_cast<const uint64_t*>(a.coeff + (off + (uint32_t)((r0 + 1) * W + x0)));
                 // columns past the coded region read as zero levels
                 const int nv = cw - x0;
                 if (nv < 4) {
@@ -649,7 +650,7 @@ __device__ __forceinline__ void tile_body(const ResidualArgs& a, const TileItem&
         for (int e = tid; e < total; e += 256) {
             const int g = e / CH4, c = e - g * CH4;
             const uint2 v = *reinterpret_cast<const uint2*>(O + g * W * H + 4 * c);
-            *reinterpret_cast<uint2*>(a.residual + P[g].off + 4 * c) = v;
+            *reinterpret_cast<uint2*>(a.residual + (P[g].off + (uint32_t)(4 * c))) = v;
         }
     }
 }
