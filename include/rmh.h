@@ -217,6 +217,42 @@ rmh_status rmh_mc_plan_create(rmh_ctx* ctx, const rmh_mc_frame* frames, size_t n
                               const rmh_pu_desc* pus, size_t n_pu, rmh_plan** out);
 rmh_status rmh_mc_plan_run(rmh_plan* plan, void* stream);
 
+/* Bi-prediction (inter_dir 3) with CU-level weights, replacing the bi branch
+ * of predict_inter_pu (vvc/inter.rs:1753-1801) without DMVR and BDOF: both
+ * lists are interpolated to 14-bit intermediates (interpolate_luma_block /
+ * interpolate_chroma_block_420 with for_bi_pred, vvc/inter.rs:886-1182; the
+ * 8..12-bit forms vvc/inter.rs:2010-2134, 2543-2670) and combined by
+ * weighted_average_bi_16 (vvc/inter.rs:2676-2696; 8-bit weighted_average_bi,
+ * vvc/inter.rs:1189-1201). Every plane of a frame shares one bit depth. */
+typedef struct rmh_mc_bi_frame {
+    rmh_plane ref0[3];   /* L0 reference (Y, Cb, Cr, 4:2:0) */
+    rmh_plane ref1[3];   /* L1 reference, same sizes */
+    rmh_plane dst[3];    /* prediction output */
+} rmh_mc_bi_frame;
+
+/* One bi-predicted PU, 28 bytes. */
+typedef struct rmh_bi_pu_desc {
+    uint16_t x;
+    uint16_t y;
+    uint8_t log2_w;      /* low 4 bits: log2 width (3..7); high bits: RMH_PU_* flags */
+    uint8_t log2_h;      /* log2 height (3..7) */
+    uint16_t frame;      /* index into the bi frame table */
+    int32_t mv0_x;       /* L0 MV, 1/16 luma sample */
+    int32_t mv0_y;
+    int32_t mv1_x;       /* L1 MV */
+    int32_t mv1_y;
+    uint8_t bcw_idx;     /* L1 weight BCW_WEIGHTS[min(bcw_idx, 4)] / 8 of {-2, 3, 4, 5, 10}
+                          * (vvc/inter.rs:128); L0 weight 8 minus that; 2 = equal weights */
+    uint8_t reserved[3]; /* zero */
+} rmh_bi_pu_desc;
+
+/* Host memory, synchronous (as rmh_mc_batch, which it mirrors). */
+rmh_status rmh_mc_bi_batch(rmh_ctx* ctx, const rmh_mc_bi_frame* frames, size_t n_frames,
+                           const rmh_bi_pu_desc* pus, size_t n_pu);
+/* Planes in `frames` hold device addresses; run with rmh_mc_plan_run. */
+rmh_status rmh_mc_bi_plan_create(rmh_ctx* ctx, const rmh_mc_bi_frame* frames, size_t n_frames,
+                                 const rmh_bi_pu_desc* pus, size_t n_pu, rmh_plan** out);
+
 /* ------------------------------------------------------------------------ */
 /* Deblocking                                                                */
 /* ------------------------------------------------------------------------ */

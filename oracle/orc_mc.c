@@ -4,8 +4,10 @@
  * Restates interpolate_luma_block / interpolate_chroma_block_420
  * (crates/rustmpeg-video/src/vvc/inter.rs:886-1182), generalised from 8 bits to
  * bit depth bd with head_room = IF_INTERNAL_PREC - bd (for bd = 8 the function is
- * the reference's 8-bit one term for term), the uni-prediction path of
- * predict_inter_pu (inter.rs:1734-1848) and write_prediction
+ * the reference's 8-bit one term for term), the uni-prediction path and the
+ * bi-prediction path without DMVR and BDOF of predict_inter_pu
+ * (inter.rs:1734-1848), weighted_average_bi / weighted_average_bi_16
+ * (inter.rs:1189-1201, 2676-2696) and write_prediction
  * (decode_general.rs:2641-2667).
  *
  * The reference's 16-bit functions (inter.rs:2010-2134, 2543-2671) use
@@ -302,6 +304,61 @@ size_t orc_mc_batch(const rmh_mc_frame* frames, size_t n_frames, const rmh_pu_de
         free(buf);
     }
     for (size_t i = 0; i < n_frames * 3; i++) free(wide[i]);
+    free(wide);
+    return bad;
+}
+
+/* Writes the w x h block `buf` (already clipped sample values) at (x0, y0) of
+ * plane p, skipping samples outside it (write_prediction). */
+static void put_block(const rmh_plane* p, int x0, int y0, int w, int h, const int32_t* buf) {
+    for (int y = 0; y < h; y++)
+        for (int x = 0; x < w; x++)
+            if (x0 + x < (int)p->width && y0 + y < (int)p->height) plane_set(p, x0 + x, y0 + y, buf[y * w + x]);
+}
+
+size_t orc_mc_bi_batch(const rmh_mc_bi_frame* frames, size_t n_frames, const rmh_bi_pu_desc* pus, size_t n_pu) {
+    uint16_t** wide = calloc(n_frames * 6, sizeof(uint16_t*));
+    size_t bad = 0;
+    for (size_t i = 0; i < n_pu && !bad; i++) {
+        const rmh_bi_pu_desc* pu = &pus[i];
+        if (pu->frame >= n_frames) {
+            bad = i + 1;
+            break;
+        }
+        const rmh_mc_bi_frame* f = &frames[pu->frame];
+        uint16_t** wf = wide + (size_t)pu->frame * 6;
+        for (int c = 0; c < 3; c++) {
+            if (!wf[c]) wf[c] = widen(&f->ref0[c]);
+            if (!wf[3 + c]) wf[3 + c] = widen(&f->ref1[c]);
+        }
+        int w = 1 << (pu->log2_w & 15), h = 1 << (pu->log2_h & 15);
+        int alt = (pu->log2_w & RMH_PU_ALT_HPEL) != 0;
+        int bd = f->ref0[0].bit_depth;
+        int32_t* p0 = malloc(sizeof(int32_t) * (size_t)(w * h));
+        int32_t* p1 = malloc(sizeof(int32_t) * (size_t)(w * h));
+        int32_t* out = malloc(sizeof(int32_t) * (size_t)(w * h));
+        const rmh_plane* r = &f->ref0[0];
+        orc_interp_luma(wf[0], (int)r->width, (int)r->width, (int)r->height, bd, pu->x, pu->y, w, h, pu->mv0_x,
+                        pu->mv0_y, 1, alt, 0, p0);
+        orc_interp_luma(wf[3], (int)r->width, (int)r->width, (int)r->height, bd, pu->x, pu->y, w, h, pu->mv1_x,
+                        pu->mv1_y, 1, alt, 0, p1);
+        orc_weighted_average_bi(p0, p1, (size_t)(w * h), pu->bcw_idx, bd, out);
+        put_block(&f->dst[0], pu->x, pu->y, w, h, out);
+        int cw = w / 2, ch = h / 2;
+        for (int c = 1; c < 3; c++) {
+            const rmh_plane* rc = &f->ref0[c];
+            orc_interp_chroma(wf[c], (int)rc->width, (int)rc->width, (int)rc->height, bd, pu->x / 2, pu->y / 2, cw,
+                              ch, pu->mv0_x, pu->mv0_y, 1, p0);
+            orc_interp_chroma(wf[3 + c], (int)rc->width, (int)rc->width, (int)rc->height, bd, pu->x / 2, pu->y / 2,
+                              cw, ch, pu->mv1_x, pu->mv1_y, 1, p1);
+            orc_weighted_average_bi(p0, p1, (size_t)(cw * ch), pu->bcw_idx, bd, out);
+            put_block(&f->dst[c], pu->x / 2, pu->y / 2, cw, ch, out);
+        }
+        free(p0);
+        free(p1);
+        free(out);
+    }
+    for (size_t i = 0; i < n_frames * 6; i++) free(wide[i]);
     free(wide);
     return bad;
 }

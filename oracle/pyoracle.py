@@ -65,6 +65,8 @@ def _setup(L):
     L.orc_recon_batch.restype = c_size
     L.orc_mc_batch.argtypes = [ctypes.c_void_p, c_size, ctypes.c_void_p, c_size]
     L.orc_mc_batch.restype = c_size
+    L.orc_mc_bi_batch.argtypes = [ctypes.c_void_p, c_size, ctypes.c_void_p, c_size]
+    L.orc_mc_bi_batch.restype = c_size
     L.orc_deblock_luma_edge.argtypes = [I32P] + [c_int] * 9
     L.orc_deblock_chroma_edge.argtypes = [I32P] + [c_int] * 8
     L.orc_deblock_frames.argtypes = [ctypes.c_void_p, c_size]
@@ -283,3 +285,17 @@ def intra_decode(w, h, modes, tus, coeff):
                                _p(rv, U8P))
     assert n == len(modes)
     return ry, ru, rv
+
+
+def mc_bi_batch(frame, pus):
+    """orc_mc_bi_batch over one abi.McBiFrame (host planes) and a BI_PU_DTYPE array."""
+    pus = np.ascontiguousarray(pus)
+    arr = (type(frame) * 1)(frame)
+    return lib().orc_mc_bi_batch(ctypes.addressof(arr), 1, pus.ctypes.data, len(pus))
+
+
+def mc_batch(frame, pus):
+    """orc_mc_batch over one abi.McFrame (host planes) and a PU_DTYPE array."""
+    pus = np.ascontiguousarray(pus)
+    arr = (type(frame) * 1)(frame)
+    return lib().orc_mc_batch(ctypes.addressof(arr), 1, pus.ctypes.data, len(pus))

@@ -79,6 +79,11 @@ void orc_weighted_average_bi(const int32_t* p0, const int32_t* p1, size_t n, int
  * decode_general.rs:2641-2667) over a PU list. */
 size_t orc_mc_batch(const rmh_mc_frame* frames, size_t n_frames, const rmh_pu_desc* pus,
                     size_t n_pu);
+/* predict_inter_pu bi-prediction without DMVR and BDOF (vvc/inter.rs:1753-1801):
+ * both lists with for_bi_pred, weighted_average_bi_16 (vvc/inter.rs:2676-2696),
+ * then write_prediction, over a bi PU list. */
+size_t orc_mc_bi_batch(const rmh_mc_bi_frame* frames, size_t n_frames, const rmh_bi_pu_desc* pus,
+                       size_t n_pu);
 
 /* ---- deblock stage ---- */
 /* deblock_luma_edge_bd (vvc/filter.rs:638-679) on lines[4][16] = p7..p0 q0..q7. */

@@ -72,6 +72,7 @@ EXPORTED = [
     "rmh_residual_recon_plan_create", "rmh_rd_encode_luma", "rmh_cnn_param_count", "rmh_cnn_create",
     "rmh_cnn_destroy", "rmh_cnn_get_params", "rmh_cnn_set_params", "rmh_cnn_forward", "rmh_cnn_predict",
     "rmh_cnn_gradients", "rmh_cnn_train_step", "rmh_cnn_train_step_soft", "rmh_residual_plan_autotune",
+    "rmh_mc_bi_batch", "rmh_mc_bi_plan_create",
 ]
 
 
@@ -101,6 +102,8 @@ def load_library(path=LIB_PATH):
         "rmh_mc_batch": ([vp, vp, sz, vp, sz], st),
         "rmh_mc_plan_create": ([vp, vp, sz, vp, sz, ctypes.POINTER(vp)], st),
         "rmh_mc_plan_run": ([vp, vp], st),
+        "rmh_mc_bi_batch": ([vp, vp, sz, vp, sz], st),
+        "rmh_mc_bi_plan_create": ([vp, vp, sz, vp, sz, ctypes.POINTER(vp)], st),
         "rmh_deblock": ([vp, vp, sz], st),
         "rmh_deblock_plan_create": ([vp, vp, sz, ctypes.POINTER(vp)], st),
         "rmh_deblock_plan_run": ([vp, vp], st),
@@ -295,6 +298,13 @@ class Accelerator:
         self._check(load_library().rmh_mc_batch(self.handle, ctypes.addressof(arr), len(frames), pus.ctypes.data,
                                                 len(pus)))
 
+    def predict_bi(self, frames, pus):
+        """Bi-prediction: frames: list of abi.McBiFrame (host planes); pus: BI_PU_DTYPE array."""
+        pus = np.ascontiguousarray(pus, dtype=abi.BI_PU_DTYPE)
+        arr = (abi.McBiFrame * len(frames))(*frames)
+        self._check(load_library().rmh_mc_bi_batch(self.handle, ctypes.addressof(arr), len(frames),
+                                                   pus.ctypes.data, len(pus)))
+
     def deblock(self, frames):
         """frames: list of abi.DbkFrame (host planes, filtered in place)."""
         arr = (abi.DbkFrame * len(frames))(*frames)
@@ -343,6 +353,15 @@ class Accelerator:
         h = ctypes.c_void_p()
         self._check(load_library().rmh_mc_plan_create(self.handle, ctypes.addressof(arr), len(frames),
                                                       pus.ctypes.data, len(pus), ctypes.byref(h)))
+        return Plan(self, h.value, "mc")
+
+    def mc_bi_plan(self, frames, pus):
+        """Bi-prediction plan over abi.McBiFrame entries with device planes; run with plan.run()."""
+        pus = np.ascontiguousarray(pus, dtype=abi.BI_PU_DTYPE)
+        arr = (abi.McBiFrame * len(frames))(*frames)
+        h = ctypes.c_void_p()
+        self._check(load_library().rmh_mc_bi_plan_create(self.handle, ctypes.addressof(arr), len(frames),
+                                                         pus.ctypes.data, len(pus), ctypes.byref(h)))
         return Plan(self, h.value, "mc")
 
     def deblock_plan(self, frames):

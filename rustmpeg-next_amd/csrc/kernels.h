@@ -112,6 +112,21 @@ struct McItem {
 
 hipError_t launch_mc(const rmh_mc_frame* frames, const McItem* items, uint32_t n_items, hipStream_t stream);
 
+// One bi-prediction work item: the tile geometry of McItem, both MVs and the
+// BCW weight of L1 (the L0 weight is 8 minus it).
+struct McBiItem {
+    uint16_t x0, y0;  // luma tile origin in the picture
+    uint8_t tw, th;   // luma tile size, 1..16
+    uint8_t cw, ch;   // chroma tile size, 0..8
+    uint16_t frame;
+    uint8_t flags;    // bit 0: alternative half-sample luma filter
+    int8_t w1;        // L1 weight, one of -2, 3, 4, 5, 10
+    int32_t mv0_x, mv0_y, mv1_x, mv1_y;
+    uint32_t reserved;
+};
+
+hipError_t launch_mc_bi(const rmh_mc_bi_frame* frames, const McBiItem* items, uint32_t n_items, hipStream_t stream);
+
 // Deblocking: a copy of every plane into `shadows` (3 per frame, same sizes,
 // device memory), then one tiled launch that reads the shadows and writes the
 // pictures. tile_start[n_frames + 1] (device memory) holds the running sum

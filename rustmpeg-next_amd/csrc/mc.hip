@@ -1,5 +1,6 @@
-// mc.hip - uni-prediction motion compensation for gfx950: 8-tap luma and
-// 4-tap 4:2:0 chroma fractional-sample interpolation.
+// mc.hip - motion compensation for gfx950: 8-tap luma and 4-tap 4:2:0
+// chroma fractional-sample interpolation, uni-prediction (mc_kernel) and
+// bi-prediction with CU-level weights (mc_bi_kernel).
 //
 // Semantics follow interpolate_luma_block / interpolate_chroma_block_420
 // (crates/rustmpeg-video/src/vvc/inter.rs:886-1182) at bit depth bd with
@@ -85,6 +86,7 @@ struct Tile {
     int ix, iy;    // integer reference position of the first output sample
     uint32_t cx[TAPS / 2];
     uint32_t cy[TAPS / 2];
+    int sx[TAPS];  // horizontal taps as ints (bi-prediction samples)
     int sy[TAPS];  // vertical taps as ints (vertical-only tiles)
     bool hf, vf;   // fractional horizontally / vertically
 };
@@ -209,7 +211,39 @@ __device__ __forceinline__ void set_taps(Tile<TAPS>& t, const int8_t* cx, const 
         t.cy[k] = pack2(cy[2 * k], cy[2 * k + 1]);
     }
 #pragma unroll
-    for (int k = 0; k < TAPS; k++) t.sy[k] = cy[k];
+    for (int k = 0; k < TAPS; k++) {
+        t.sx[k] = cx[k];
+        t.sy[k] = cy[k];
+    }
+}
+
+// The 14-bit bi-prediction intermediate of output sample (x, y) of a tile
+// (for_bi_pred branches of interpolate_luma_block_16 / interpolate_chroma_
+// block_format_16, vvc/inter.rs:2046-2133, 2582-2669): a copy gives
+// (s << head_room) - 8192, a one-dimensional filter (sum >> shift1) - 8192,
+// a two-dimensional one the vertical sum over the horizontal pass's
+// intermediates (which already hold (sum >> shift1) - 8192), >> 6.
+template <int TAPS, int PITCH>
+__device__ __forceinline__ int bi_sample(const Tile<TAPS>& t, const int16_t* win, const int16_t* tmp, int x, int y,
+                                         int head_room, int shift1) {
+    constexpr int BEFORE = TAPS / 2 - 1;
+    int s = 0;
+    if (t.hf && t.vf) {
+#pragma unroll
+        for (int k = 0; k < TAPS; k++) s += t.sy[k] * tmp[x * PITCH + y + k];
+        return s >> 6;
+    }
+    if (t.hf) {
+#pragma unroll
+        for (int k = 0; k < TAPS; k++) s += t.sx[k] * win[(y + BEFORE) * PITCH + x + k];
+        return (s >> shift1) - 8192;
+    }
+    if (t.vf) {
+#pragma unroll
+        for (int k = 0; k < TAPS; k++) s += t.sy[k] * win[(y + k) * PITCH + x + BEFORE];
+        return (s >> shift1) - 8192;
+    }
+    return ((int)win[(y + BEFORE) * PITCH + x + BEFORE] << head_room) - 8192;
 }
 
 // Four work items per 256-lane workgroup, one per wave: the 16x16 luma tile
@@ -280,7 +314,114 @@ __global__ __launch_bounds__(256) void mc_kernel(const rmh_mc_frame* __restrict_
     }
 }
 
+// Luma and chroma tiles of one list of a bi item.
+__device__ __forceinline__ void bi_tiles(const McBiItem& it, int mv_x, int mv_y, Tile<8>& tl, Tile<4>& tc) {
+    const bool alt = it.flags & 1;
+    const int fx = mv_x & 15, fy = mv_y & 15;
+    tl.x0 = it.x0;
+    tl.y0 = it.y0;
+    tl.tw = it.tw;
+    tl.th = it.th;
+    tl.ix = tl.x0 + (mv_x >> 4);
+    tl.iy = tl.y0 + (mv_y >> 4);
+    set_taps<8>(tl, (alt && fx == 8) ? kLumaAltHpel : kLuma8[fx], (alt && fy == 8) ? kLumaAltHpel : kLuma8[fy]);
+    tl.hf = fx != 0;
+    tl.vf = fy != 0;
+    const int cfx = mv_x & 31, cfy = mv_y & 31;
+    tc.x0 = it.x0 / 2;
+    tc.y0 = it.y0 / 2;
+    tc.tw = it.cw;
+    tc.th = it.ch;
+    tc.ix = tc.x0 + (mv_x >> 5);
+    tc.iy = tc.y0 + (mv_y >> 5);
+    set_taps<4>(tc, kChroma4[cfx], kChroma4[cfy]);
+    tc.hf = cfx != 0;
+    tc.vf = cfy != 0;
+}
+
+// Weighted average of the two lists' intermediates over one plane's tile
+// (weighted_average_bi_16, vvc/inter.rs:2676-2696), T = full tile width.
+template <int TAPS, int PITCH, int T>
+__device__ __forceinline__ void bi_output(const rmh_plane& dst, const Tile<TAPS>& t0, const Tile<TAPS>& t1,
+                                          const int16_t* win0, const int16_t* tmp0, const int16_t* win1,
+                                          const int16_t* tmp1, int head_room, int w0, int w1, int maxv, int lane) {
+    const int shift1 = 6 - head_room, shift = head_room + 3;
+    const int off = (1 << (shift - 1)) + (8192 << 3);
+    for (int e = lane; e < T * T; e += 64) {
+        const int x = e % T, y = e / T;
+        if (x >= t0.tw || y >= t0.th) continue;
+        const int p0 = bi_sample<TAPS, PITCH>(t0, win0, tmp0, x, y, head_room, shift1);
+        const int p1 = bi_sample<TAPS, PITCH>(t1, win1, tmp1, x, y, head_room, shift1);
+        store_sample(dst, t0.x0 + x, t0.y0 + y, clampi((p0 * w0 + p1 * w1 + off) >> shift, 0, maxv));
+    }
+}
+
+// Bi-prediction: as mc_kernel, with the windows of both lists staged before
+// any filtering (two LDS slices per wave) and the weighted average fused into
+// the output pass.
+__global__ __launch_bounds__(256) void mc_bi_kernel(const rmh_mc_bi_frame* __restrict__ frames,
+                                                    const McBiItem* __restrict__ items, uint32_t n_items) {
+    __shared__ __align__(16) WaveLds lds[4][2];
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const uint32_t item = xcd_local_block(blockIdx.x, gridDim.x) * 4 + wave;
+    const bool active = item < n_items;
+    WaveLds& L0 = lds[wave][0];
+    WaveLds& L1 = lds[wave][1];
+    Tile<8> tl0{}, tl1{};
+    Tile<4> tc0{}, tc1{};
+    const rmh_mc_bi_frame* f = frames;
+    int bd = 8, w1 = 4;
+    if (active) {
+        const McBiItem it = items[item];
+        f = &frames[it.frame];
+        w1 = it.w1;
+        bi_tiles(it, it.mv0_x, it.mv0_y, tl0, tc0);
+        bi_tiles(it, it.mv1_x, it.mv1_y, tl1, tc1);
+        bd = f->ref0[0].bit_depth;
+        if (tl0.tw > 0 && tl0.th > 0) {
+            stage<8, kLR, kLP, 32>(f->ref0[0], tl0, L0.luma, lane);
+            stage<8, kLR, kLP, 32>(f->ref1[0], tl1, L1.luma, lane);
+        }
+        if (tc0.tw > 0 && tc0.th > 0) {
+            stage<4, kCR, kCP, 16>(f->ref0[1], tc0, L0.chroma[0], lane);
+            stage<4, kCR, kCP, 16>(f->ref0[2], tc0, L0.chroma[1], lane);
+            stage<4, kCR, kCP, 16>(f->ref1[1], tc1, L1.chroma[0], lane);
+            stage<4, kCR, kCP, 16>(f->ref1[2], tc1, L1.chroma[1], lane);
+        }
+    }
+    __syncthreads();
+    const int head_room = 14 - bd;
+    const int shift1 = 6 - head_room;
+    if (active && tl0.tw > 0 && tl0.th > 0) {
+        hpass<8, kLP, 8>(tl0, L0.luma, L0.ltmp, shift1, lane);
+        hpass<8, kLP, 8>(tl1, L1.luma, L1.ltmp, shift1, lane);
+    }
+    if (active && tc0.tw > 0 && tc0.th > 0) {
+        hpass<4, kCP, 4>(tc0, L0.chroma[0], L0.ctmp[0], shift1, lane);
+        hpass<4, kCP, 4>(tc0, L0.chroma[1], L0.ctmp[1], shift1, lane);
+        hpass<4, kCP, 4>(tc1, L1.chroma[0], L1.ctmp[0], shift1, lane);
+        hpass<4, kCP, 4>(tc1, L1.chroma[1], L1.ctmp[1], shift1, lane);
+    }
+    __syncthreads();
+    if (!active) return;
+    const int maxv = (1 << bd) - 1, w0 = 8 - w1;
+    if (tl0.tw > 0 && tl0.th > 0)
+        bi_output<8, kLP, 16>(f->dst[0], tl0, tl1, L0.luma, L0.ltmp, L1.luma, L1.ltmp, head_room, w0, w1, maxv, lane);
+    if (tc0.tw > 0 && tc0.th > 0) {
+        for (int c = 0; c < 2; c++)
+            bi_output<4, kCP, 8>(f->dst[1 + c], tc0, tc1, L0.chroma[c], L0.ctmp[c], L1.chroma[c], L1.ctmp[c], head_room,
+                                 w0, w1, maxv, lane);
+    }
+}
+
 }  // namespace
+
+hipError_t launch_mc_bi(const rmh_mc_bi_frame* frames, const McBiItem* items, uint32_t n_items, hipStream_t stream) {
+    if (n_items == 0) return hipSuccess;
+    hipLaunchKernelGGL(mc_bi_kernel, dim3((n_items + 3) / 4), dim3(256), 0, stream, frames, items, n_items);
+    return hipGetLastError();
+}
 
 hipError_t launch_mc(const rmh_mc_frame* frames, const McItem* items, uint32_t n_items, hipStream_t stream) {
     if (n_items == 0) return hipSuccess;

@@ -113,6 +113,8 @@ struct rmh_plan {
     const rmh_mc_frame* d_frames = nullptr;
     const rmh_pu_desc* d_pus = nullptr;
     const rmh::McItem* d_items = nullptr;
+    const rmh_mc_bi_frame* d_bi_frames = nullptr;  // bi-prediction plans
+    const rmh::McBiItem* d_bi_items = nullptr;
     size_t n_items = 0;
     // deblock
     const rmh_dbk_frame* d_dbk = nullptr;
@@ -458,6 +460,45 @@ rmh_status validate_mc(rmh_ctx* ctx, const rmh_mc_frame* frames, size_t n_frames
     return RMH_OK;
 }
 
+rmh_status validate_mc_bi(rmh_ctx* ctx, const rmh_mc_bi_frame* frames, size_t n_frames, const rmh_bi_pu_desc* pus,
+                          size_t n_pu) {
+    if ((n_frames && !frames) || (n_pu && !pus)) return fail_locked(ctx, RMH_INVALID_INPUT, "null frame or PU array");
+    if (n_frames > 65536) return fail_locked(ctx, RMH_LIMIT_EXCEEDED, "more than 65536 frames");
+    for (size_t f = 0; f < n_frames; f++) {
+        const rmh_mc_bi_frame& fr = frames[f];
+        const std::string tag = "frame " + std::to_string(f) + ": ";
+        for (int c = 0; c < 3; c++) {
+            if (!plane_ok(fr.ref0[c]) || !plane_ok(fr.ref1[c]) || !plane_ok(fr.dst[c]))
+                return fail_locked(ctx, RMH_INVALID_INPUT, tag + "invalid plane");
+            for (const rmh_plane* q : {&fr.ref1[c], &fr.dst[c]}) {
+                if (q->bit_depth != fr.ref0[0].bit_depth || q->bytes_per_sample != fr.ref0[c].bytes_per_sample ||
+                    fr.ref0[c].bit_depth != fr.ref0[0].bit_depth)
+                    return fail_locked(ctx, RMH_INVALID_INPUT, tag + "mixed sample formats");
+                if (q->width != fr.ref0[c].width || q->height != fr.ref0[c].height)
+                    return fail_locked(ctx, RMH_INVALID_INPUT, tag + "reference/output size mismatch");
+            }
+        }
+        if (fr.ref0[1].width != (fr.ref0[0].width + 1) / 2 || fr.ref0[1].height != (fr.ref0[0].height + 1) / 2 ||
+            fr.ref0[2].width != fr.ref0[1].width || fr.ref0[2].height != fr.ref0[1].height)
+            return fail_locked(ctx, RMH_INVALID_INPUT, tag + "chroma planes are not 4:2:0");
+    }
+    for (size_t i = 0; i < n_pu; i++) {
+        const rmh_bi_pu_desc& p = pus[i];
+        const std::string tag = "PU " + std::to_string(i) + ": ";
+        int lw = p.log2_w & 15, lh = p.log2_h & 15;
+        if ((p.log2_w & ~(15 | RMH_PU_ALT_HPEL)) || lw < 3 || lw > 7 || lh < 3 || lh > 7)
+            return fail_locked(ctx, RMH_INVALID_INPUT, tag + "size outside 8..128");
+        if (p.frame >= n_frames) return fail_locked(ctx, RMH_INVALID_INPUT, tag + "frame index");
+        const rmh_plane& y = frames[p.frame].ref0[0];
+        if (p.x >= y.width || p.y >= y.height) return fail_locked(ctx, RMH_INVALID_INPUT, tag + "outside the picture");
+        for (int32_t m : {p.mv0_x, p.mv0_y, p.mv1_x, p.mv1_y})
+            if (m < -(1 << 17) || m >= (1 << 17)) return fail_locked(ctx, RMH_INVALID_INPUT, tag + "MV outside 18-bit storage");
+        if (p.reserved[0] | p.reserved[1] | p.reserved[2])
+            return fail_locked(ctx, RMH_INVALID_INPUT, tag + "reserved bytes must be zero");
+    }
+    return RMH_OK;
+}
+
 rmh_status validate_dbk(rmh_ctx* ctx, const rmh_dbk_frame* frames, size_t n_frames) {
     if (n_frames && !frames) return fail_locked(ctx, RMH_INVALID_INPUT, "null frame array");
     if (n_frames > 65535) return fail_locked(ctx, RMH_LIMIT_EXCEEDED, "more than 65535 frames");
@@ -478,33 +519,72 @@ rmh_status validate_dbk(rmh_ctx* ctx, const rmh_dbk_frame* frames, size_t n_fram
 
 // 16x16 luma tiles of every PU (PU order, raster inside a PU), clipped to the
 // picture; tiles wholly outside it are dropped.
+// The work items of one PU: its 16x16 luma tiles clipped to the picture
+// (dst: the frame's three output planes), each with its chroma tiles;
+// emit(x0, y0, tw, th, cw, ch) per kept tile.
+template <typename F>
+void pu_tiles(const rmh_plane* dst, int px, int py, int w, int h, F&& emit) {
+    for (int ty = 0; ty < (h + 15) / 16; ty++)
+        for (int tx = 0; tx < (w + 15) / 16; tx++) {
+            const int x0 = px + 16 * tx, y0 = py + 16 * ty;
+            const int tw = std::max(0, std::min({16, w - 16 * tx, (int)dst[0].width - x0}));
+            const int th = std::max(0, std::min({16, h - 16 * ty, (int)dst[0].height - y0}));
+            const int cw = std::max(0, std::min(std::min(16, w - 16 * tx) / 2, (int)dst[1].width - x0 / 2));
+            const int ch = std::max(0, std::min(std::min(16, h - 16 * ty) / 2, (int)dst[1].height - y0 / 2));
+            if ((tw == 0 || th == 0) && (cw == 0 || ch == 0)) continue;
+            emit(x0, y0, tw, th, cw, ch);  // a kept tile has x0 <= width and y0 <= height, so they fit 16 bits
+        }
+}
+
 std::vector<rmh::McItem> mc_items(const rmh_mc_frame* frames, const rmh_pu_desc* pus, size_t n_pu) {
     std::vector<rmh::McItem> items;
     for (size_t i = 0; i < n_pu; i++) {
         const rmh_pu_desc& pu = pus[i];
-        const rmh_mc_frame& f = frames[pu.frame];
-        const int w = 1 << (pu.log2_w & 15), h = 1 << (pu.log2_h & 15);
-        for (int ty = 0; ty < (h + 15) / 16; ty++)
-            for (int tx = 0; tx < (w + 15) / 16; tx++) {
-                const int x0 = pu.x + 16 * tx, y0 = pu.y + 16 * ty;
-                const int tw = std::max(0, std::min({16, w - 16 * tx, (int)f.dst[0].width - x0}));
-                const int th = std::max(0, std::min({16, h - 16 * ty, (int)f.dst[0].height - y0}));
-                const int cw = std::max(0, std::min(std::min(16, w - 16 * tx) / 2, (int)f.dst[1].width - x0 / 2));
-                const int ch = std::max(0, std::min(std::min(16, h - 16 * ty) / 2, (int)f.dst[1].height - y0 / 2));
-                if ((tw == 0 || th == 0) && (cw == 0 || ch == 0)) continue;
-                rmh::McItem it{};  // a kept tile has x0 <= width and y0 <= height, so they fit 16 bits
-                it.x0 = (uint16_t)x0;
-                it.y0 = (uint16_t)y0;
-                it.tw = (uint8_t)tw;
-                it.th = (uint8_t)th;
-                it.cw = (uint8_t)cw;
-                it.ch = (uint8_t)ch;
-                it.frame = pu.frame;
-                it.flags = (pu.log2_w & RMH_PU_ALT_HPEL) ? 1 : 0;
-                it.mv_x = pu.mv_x;
-                it.mv_y = pu.mv_y;
-                items.push_back(it);
-            }
+        pu_tiles(frames[pu.frame].dst, pu.x, pu.y, 1 << (pu.log2_w & 15), 1 << (pu.log2_h & 15),
+                 [&](int x0, int y0, int tw, int th, int cw, int ch) {
+                     rmh::McItem it{};
+                     it.x0 = (uint16_t)x0;
+                     it.y0 = (uint16_t)y0;
+                     it.tw = (uint8_t)tw;
+                     it.th = (uint8_t)th;
+                     it.cw = (uint8_t)cw;
+                     it.ch = (uint8_t)ch;
+                     it.frame = pu.frame;
+                     it.flags = (pu.log2_w & RMH_PU_ALT_HPEL) ? 1 : 0;
+                     it.mv_x = pu.mv_x;
+                     it.mv_y = pu.mv_y;
+                     items.push_back(it);
+                 });
+    }
+    return items;
+}
+
+// BCW_WEIGHTS (vvc/inter.rs:128): the L1 weight of bcw_idx, clamped to 4 as
+// weighted_average_bi does.
+constexpr int8_t kBcwWeights[5] = {-2, 3, 4, 5, 10};
+
+std::vector<rmh::McBiItem> mc_bi_items(const rmh_mc_bi_frame* frames, const rmh_bi_pu_desc* pus, size_t n_pu) {
+    std::vector<rmh::McBiItem> items;
+    for (size_t i = 0; i < n_pu; i++) {
+        const rmh_bi_pu_desc& pu = pus[i];
+        pu_tiles(frames[pu.frame].dst, pu.x, pu.y, 1 << (pu.log2_w & 15), 1 << (pu.log2_h & 15),
+                 [&](int x0, int y0, int tw, int th, int cw, int ch) {
+                     rmh::McBiItem it{};
+                     it.x0 = (uint16_t)x0;
+                     it.y0 = (uint16_t)y0;
+                     it.tw = (uint8_t)tw;
+                     it.th = (uint8_t)th;
+                     it.cw = (uint8_t)cw;
+                     it.ch = (uint8_t)ch;
+                     it.frame = pu.frame;
+                     it.flags = (pu.log2_w & RMH_PU_ALT_HPEL) ? 1 : 0;
+                     it.w1 = kBcwWeights[pu.bcw_idx < 4 ? pu.bcw_idx : 4];
+                     it.mv0_x = pu.mv0_x;
+                     it.mv0_y = pu.mv0_y;
+                     it.mv1_x = pu.mv1_x;
+                     it.mv1_y = pu.mv1_y;
+                     items.push_back(it);
+                 });
     }
     return items;
 }
@@ -515,6 +595,18 @@ uint64_t mc_bytes(const rmh_mc_frame* frames, const rmh_pu_desc* pus, size_t n_p
         uint64_t w = 1u << (pus[i].log2_w & 15), h = 1u << (pus[i].log2_h & 15);
         uint64_t bps = frames[pus[i].frame].ref[0].bytes_per_sample;
         b += 2 * bps * (w * h + 2 * (w / 2) * (h / 2)) + 16;
+    }
+    return b;
+}
+
+// Algorithmic bytes of a bi plan: two reference reads and one write per
+// sample, plus the 28-byte descriptor.
+uint64_t mc_bi_bytes(const rmh_mc_bi_frame* frames, const rmh_bi_pu_desc* pus, size_t n_pu) {
+    uint64_t b = 0;
+    for (size_t i = 0; i < n_pu; i++) {
+        uint64_t w = 1u << (pus[i].log2_w & 15), h = 1u << (pus[i].log2_h & 15);
+        uint64_t bps = frames[pus[i].frame].ref0[0].bytes_per_sample;
+        b += 3 * bps * (w * h + 2 * (w / 2) * (h / 2)) + sizeof(rmh_bi_pu_desc);
     }
     return b;
 }
@@ -711,7 +803,8 @@ rmh_status run_plan_locked_free(rmh_plan* plan, const int16_t* d_coeff, int16_t*
             e = rmh::launch_residual(a, s);
         }
     } else if (plan->kind == kMc) {
-        e = rmh::launch_mc(plan->d_frames, plan->d_items, (uint32_t)plan->n_items, s);
+        e = plan->d_bi_items ? rmh::launch_mc_bi(plan->d_bi_frames, plan->d_bi_items, (uint32_t)plan->n_items, s)
+                             : rmh::launch_mc(plan->d_frames, plan->d_items, (uint32_t)plan->n_items, s);
     } else {
         e = rmh::launch_deblock(plan->d_dbk, plan->d_shadows, plan->d_tile_start, (uint32_t)plan->n_dbk, plan->total_tiles,
                                 plan->max_rows, s);
@@ -1164,6 +1257,121 @@ RMH_EXPORT rmh_status rmh_mc_batch(rmh_ctx* ctx, const rmh_mc_frame* frames, siz
         (e = hipMemcpyAsync(d_items, items.data(), ib, hipMemcpyHostToDevice, s)) != hipSuccess)
         return hip_fail_locked(ctx, e, "upload");
     e = rmh::launch_mc(d_frames, d_items, (uint32_t)items.size(), s);
+    if (e != hipSuccess) return hip_fail_locked(ctx, e, "kernel launch");
+    std::vector<void*> hdst;
+    for (size_t f = 0; f < n_frames; f++)
+        for (int c = 0; c < 3; c++) {
+            const rmh_plane& d = dev_frames[f].dst[c];
+            void* h = ctx->host.take(plane_bytes(d));
+            hdst.push_back(h);
+            if ((e = hipMemcpyAsync(h, d.data, plane_bytes(d), hipMemcpyDeviceToHost, s)) != hipSuccess)
+                return hip_fail_locked(ctx, e, "download");
+        }
+    rmh_status r = wait_stream(ctx, s);
+    if (r != RMH_OK) return r;
+    // Copy back only the samples some PU predicted; the rest keep the caller's values.
+    for (size_t f = 0, k = 0; f < n_frames; f++)
+        for (int c = 0; c < 3; c++, k++) {
+            const rmh_plane& d = frames[f].dst[c];
+            const uint8_t* src = (const uint8_t*)hdst[k];
+            for (size_t i = 0; i < n_pu; i++) {
+                if (pus[i].frame != f) continue;
+                int sx = c ? 1 : 0;
+                uint32_t x0 = pus[i].x >> sx, y0 = pus[i].y >> sx;
+                uint32_t w = (1u << (pus[i].log2_w & 15)) >> sx, h = (1u << (pus[i].log2_h & 15)) >> sx;
+                if (x0 >= d.width || y0 >= d.height) continue;
+                if (x0 + w > d.width) w = d.width - x0;
+                if (y0 + h > d.height) h = d.height - y0;
+                for (uint32_t y = 0; y < h; y++)
+                    memcpy((uint8_t*)d.data + ((size_t)(y0 + y) * d.stride + x0) * d.bytes_per_sample,
+                           src + ((size_t)(y0 + y) * d.width + x0) * d.bytes_per_sample, (size_t)w * d.bytes_per_sample);
+            }
+        }
+    return RMH_OK;
+}
+
+// ------------------------------------------------------------------ bi-prediction
+
+RMH_EXPORT rmh_status rmh_mc_bi_plan_create(rmh_ctx* ctx, const rmh_mc_bi_frame* frames, size_t n_frames,
+                                            const rmh_bi_pu_desc* pus, size_t n_pu, rmh_plan** out) {
+    if (!ctx || !out) return RMH_INVALID_INPUT;
+    *out = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        rmh_status v = validate_mc_bi(ctx, frames, n_frames, pus, n_pu);
+        if (v != RMH_OK) return v;
+    }
+    if (ctx->disabled.load()) return fail(ctx, RMH_UNAVAILABLE, "accelerator disabled");
+    DeviceGuard dg(ctx->device);
+    std::vector<rmh::McBiItem> items = mc_bi_items(frames, pus, n_pu);
+    if (items.size() > 0x7fffffffu) return fail(ctx, RMH_LIMIT_EXCEEDED, "too many inter work items");
+    size_t fb = n_frames * sizeof(rmh_mc_bi_frame), ib = items.size() * sizeof(rmh::McBiItem);
+    rmh_plan* p = new rmh_plan();
+    p->ctx = ctx;
+    p->kind = kMc;
+    p->n_items = items.size();
+    p->bytes = mc_bi_bytes(frames, pus, n_pu);
+    hipError_t e = hipMalloc(&p->d_mem, aligned(fb) + aligned(ib) + 256);
+    char* base = (char*)p->d_mem;
+    if (e == hipSuccess && fb) e = hipMemcpy(base, frames, fb, hipMemcpyHostToDevice);
+    if (e == hipSuccess && ib) e = hipMemcpy(base + aligned(fb), items.data(), ib, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        if (p->d_mem) hipFree(p->d_mem);
+        delete p;
+        return fail(ctx, RMH_EXECUTION, std::string("bi-prediction plan upload failed: ") + hipGetErrorString(e));
+    }
+    p->d_bi_frames = (const rmh_mc_bi_frame*)base;
+    p->d_bi_items = (const rmh::McBiItem*)(base + aligned(fb));
+    *out = p;
+    return RMH_OK;
+}
+
+RMH_EXPORT rmh_status rmh_mc_bi_batch(rmh_ctx* ctx, const rmh_mc_bi_frame* frames, size_t n_frames,
+                                      const rmh_bi_pu_desc* pus, size_t n_pu) {
+    if (!ctx) return RMH_INVALID_INPUT;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    rmh_status v = validate_mc_bi(ctx, frames, n_frames, pus, n_pu);
+    if (v != RMH_OK) return v;
+    if (n_pu == 0) return RMH_OK;
+    rmh_status g = gate(ctx);
+    if (g != RMH_OK) return fail_locked(ctx, g, g == RMH_UNAVAILABLE ? "accelerator disabled" : "injected device failure");
+    DeviceGuard dg(ctx->device);
+    std::vector<rmh::McBiItem> items = mc_bi_items(frames, pus, n_pu);
+    size_t fb = n_frames * sizeof(rmh_mc_bi_frame), ib = items.size() * sizeof(rmh::McBiItem);
+    size_t need = aligned(fb) + aligned(ib), hneed = 0;
+    for (size_t f = 0; f < n_frames; f++)
+        for (int c = 0; c < 3; c++) {
+            need += 3 * aligned(plane_bytes(frames[f].ref0[c]));
+            hneed += aligned(plane_bytes(frames[f].dst[c]));
+        }
+    hipError_t e = ctx->dev.reserve(need);
+    if (e != hipSuccess) return fail_locked(ctx, RMH_LIMIT_EXCEEDED, std::string("device allocation: ") + hipGetErrorString(e));
+    e = ctx->host.reserve(hneed);
+    if (e != hipSuccess) return fail_locked(ctx, RMH_LIMIT_EXCEEDED, std::string("host staging: ") + hipGetErrorString(e));
+    hipStream_t s = ctx->stream;
+    rmh_mc_bi_frame* d_frames = (rmh_mc_bi_frame*)ctx->dev.take(fb);
+    rmh::McBiItem* d_items = (rmh::McBiItem*)ctx->dev.take(ib);
+    std::vector<rmh_mc_bi_frame> dev_frames(frames, frames + n_frames);
+    auto upload = [&](const rmh_plane& r, rmh_plane& d) -> hipError_t {
+        void* dr = ctx->dev.take(plane_bytes(r));
+        hipError_t u = hipMemcpy2DAsync(dr, (size_t)r.width * r.bytes_per_sample, r.data, (size_t)r.stride * r.bytes_per_sample,
+                                        (size_t)r.width * r.bytes_per_sample, r.height, hipMemcpyHostToDevice, s);
+        d.data = dr;
+        d.stride = r.width;
+        return u;
+    };
+    for (size_t f = 0; f < n_frames; f++)
+        for (int c = 0; c < 3; c++) {
+            if ((e = upload(frames[f].ref0[c], dev_frames[f].ref0[c])) != hipSuccess ||
+                (e = upload(frames[f].ref1[c], dev_frames[f].ref1[c])) != hipSuccess)
+                return hip_fail_locked(ctx, e, "reference upload");
+            dev_frames[f].dst[c].data = ctx->dev.take(plane_bytes(frames[f].dst[c]));
+            dev_frames[f].dst[c].stride = frames[f].dst[c].width;
+        }
+    if ((e = hipMemcpyAsync(d_frames, dev_frames.data(), fb, hipMemcpyHostToDevice, s)) != hipSuccess ||
+        (e = hipMemcpyAsync(d_items, items.data(), ib, hipMemcpyHostToDevice, s)) != hipSuccess)
+        return hip_fail_locked(ctx, e, "upload");
+    e = rmh::launch_mc_bi(d_frames, d_items, (uint32_t)items.size(), s);
     if (e != hipSuccess) return hip_fail_locked(ctx, e, "kernel launch");
     std::vector<void*> hdst;
     for (size_t f = 0; f < n_frames; f++)

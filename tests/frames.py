@@ -20,6 +20,13 @@ def mc_frame_struct(refs, dsts, bd):
     return f, (refs, dsts)
 
 
+def mc_bi_frame_struct(refs0, refs1, dsts, bd):
+    f = abi.McBiFrame((abi.Plane * 3)(*[abi.plane_from_array(p, bd) for p in refs0]),
+                      (abi.Plane * 3)(*[abi.plane_from_array(p, bd) for p in refs1]),
+                      (abi.Plane * 3)(*[abi.plane_from_array(p, bd) for p in dsts]))
+    return f, (refs0, refs1, dsts)
+
+
 def recon_frame_struct(planes, bd):
     """planes: [Y, Cb, Cr] arrays that a recon run adds residuals into."""
     f = abi.ReconFrame((abi.Plane * 3)(*[abi.plane_from_array(p, bd) for p in planes]))

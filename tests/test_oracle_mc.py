@@ -139,3 +139,115 @@ def test_bcw_equal_weights_average_flat_planes(po):
     y0 = po.interp_luma(p0, 8, 4, 4, 8, 8, 0, 0, bi=True)
     y1 = po.interp_luma(p1, 8, 4, 4, 8, 8, 0, 0, bi=True)
     assert (po.weighted_average_bi(y0, y1, 2, 8) == 120).all()
+
+
+# ---- bi-prediction (orc_mc_bi_batch): the bi branch of predict_inter_pu
+# without DMVR / BDOF, vvc/inter.rs:1753-1801 -------------------------------
+
+BCW = [-2, 3, 4, 5, 10]  # BCW_WEIGHTS, vvc/inter.rs:128
+
+
+def _bi_setup(refs0, refs1, bd, fill=7):
+    from frames import mc_bi_frame_struct
+    dsts = [np.full_like(p, fill) for p in refs0]
+    f, keep = mc_bi_frame_struct(refs0, refs1, dsts, bd)
+    return f, keep, dsts
+
+
+def _uni(po, refs, bd, pus_bi, fill=7):
+    """The uni-prediction oracle over the L0 half of bi PUs."""
+    from frames import mc_frame_struct
+    import workloads as W
+    dsts = [np.full_like(p, fill) for p in refs]
+    f, keep = mc_frame_struct(refs, dsts, bd)
+    pus = np.zeros(len(pus_bi), W.PU_DTYPE)
+    for k in ("x", "y", "log2_w", "log2_h", "frame"):
+        pus[k] = pus_bi[k]
+    pus["mv_x"], pus["mv_y"] = pus_bi["mv0_x"], pus_bi["mv0_y"]
+    assert po.mc_batch(f, pus) == 0
+    return dsts
+
+
+@pytest.mark.parametrize("bd", [8, 10])
+def test_bi_with_identical_lists_equals_uni_at_every_fraction_and_weight(po, bd):
+    # p0 = p1 = p: (w0 p + w1 p + off) >> (hr + 3) with w0 + w1 = 8 equals the
+    # uni-prediction rounding (nested floors), for every BCW index
+    import workloads as W
+    d = W.bi_frame(5, width=96, height=64, bit_depth=bd, mv_range=6)
+    refs = d["ref0"] if bd > 8 else [p.astype(np.uint8) for p in d["ref0"]]
+    pus = d["pus"].copy()
+    n = len(pus)
+    pus["mv0_x"] = (np.arange(n) * 7) % 16 + 16 * ((np.arange(n) % 5) - 2)
+    pus["mv0_y"] = (np.arange(n) * 3) % 16 - 16
+    pus["mv1_x"], pus["mv1_y"] = pus["mv0_x"], pus["mv0_y"]
+    want = _uni(po, refs, bd, pus)
+    for idx in range(8):
+        pus["bcw_idx"] = idx
+        f, keep, got = _bi_setup(refs, refs, bd)
+        assert po.mc_bi_batch(f, pus) == 0
+        for c in range(3):
+            assert (got[c] == want[c]).all(), (idx, c)
+
+
+@pytest.mark.parametrize("bd", [8, 10])
+def test_bi_flat_planes_closed_form_for_every_bcw_index(po, bd):
+    # every interpolation of a flat plane a gives a * 2^hr - 8192, so the
+    # output is (w0 a + w1 b + 4) >> 3 clipped (BCW index clamped to 4)
+    import workloads as W
+    rng = np.random.default_rng(11)
+    maxv = (1 << bd) - 1
+    d = W.bi_frame(6, width=64, height=48, bit_depth=bd)
+    dt = np.uint16 if bd > 8 else np.uint8
+    for trial in range(4):
+        a = rng.integers(0, maxv + 1, 3)
+        b = rng.integers(0, maxv + 1, 3)
+        refs0 = [np.full(p.shape, a[c], dt) for c, p in enumerate(d["ref0"])]
+        refs1 = [np.full(p.shape, b[c], dt) for c, p in enumerate(d["ref0"])]
+        pus = d["pus"].copy()
+        pus["bcw_idx"] = (np.arange(len(pus)) + trial) % 8
+        f, keep, got = _bi_setup(refs0, refs1, bd)
+        assert po.mc_bi_batch(f, pus) == 0
+        for i, pu in enumerate(pus):
+            w1 = BCW[min(int(pu["bcw_idx"]), 4)]
+            w0 = 8 - w1
+            x, y = int(pu["x"]), int(pu["y"])
+            w, h = 1 << (int(pu["log2_w"]) & 15), 1 << int(pu["log2_h"])
+            for c in range(3):
+                s = 0 if c == 0 else 1
+                want = min(max((w0 * int(a[c]) + w1 * int(b[c]) + 4) >> 3, 0), maxv)
+                blk = got[c][y >> s:(y >> s) + (h >> s), x >> s:(x >> s) + (w >> s)]
+                assert (blk == want).all(), (trial, i, c)
+
+
+def test_bi_swapping_lists_with_mirrored_weights_is_the_same(po):
+    # BCW_WEIGHTS is symmetric about 4: index k on (L0, L1) equals index 4 - k
+    # on (L1, L0)
+    import workloads as W
+    d = W.bi_frame(7, width=128, height=96, bit_depth=10, mv_range=20)
+    pus = d["pus"].copy()
+    pus["bcw_idx"] = np.arange(len(pus)) % 5
+    f, keep, got = _bi_setup(d["ref0"], d["ref1"], 10)
+    assert po.mc_bi_batch(f, pus) == 0
+    sw = pus.copy()
+    sw["mv0_x"], sw["mv1_x"] = pus["mv1_x"], pus["mv0_x"]
+    sw["mv0_y"], sw["mv1_y"] = pus["mv1_y"], pus["mv0_y"]
+    sw["bcw_idx"] = 4 - pus["bcw_idx"]
+    f2, keep2, got2 = _bi_setup(d["ref1"], d["ref0"], 10)
+    assert po.mc_bi_batch(f2, sw) == 0
+    for c in range(3):
+        assert (got[c] == got2[c]).all()
+    # and distinct lists do differ from either uni prediction somewhere
+    assert any((got[c] != _uni(po, d["ref0"], 10, pus)[c]).any() for c in range(3))
+
+
+def test_bi_batch_known_answer_flat_80_160(po):
+    # vvc/inter.rs:2871-2879 through the PU driver: equal weights -> 120
+    import workloads as W
+    pus = np.zeros(1, W.BI_PU_DTYPE)
+    pus["x"], pus["y"], pus["log2_w"], pus["log2_h"], pus["bcw_idx"] = 4, 4, 3, 3, 2
+    refs0 = [np.full((16, 16), 80, np.uint8), np.full((8, 8), 80, np.uint8), np.full((8, 8), 80, np.uint8)]
+    refs1 = [np.full((16, 16), 160, np.uint8), np.full((8, 8), 160, np.uint8), np.full((8, 8), 160, np.uint8)]
+    f, keep, got = _bi_setup(refs0, refs1, 8, fill=0)
+    assert po.mc_bi_batch(f, pus) == 0
+    assert (got[0][4:12, 4:12] == 120).all() and (got[1][2:6, 2:6] == 120).all()
+    assert got[0][0, 0] == 0 and got[0][12, 12] == 0

@@ -10,6 +10,9 @@ Frame-level generators for the three stages:
                    fraction), and a deblocking edge map on the 8x8 grid with a
                    random bS per 4-sample segment and filter lengths from the CU
                    sizes (luma_lens / chroma_lens, vvc/decode_general.rs:4916-4935)
+  bi_frame         two textured 4:2:0 references and one bi-predicted PU per CU
+                   (independent L0/L1 MVs, BCW index, alternative half-sample
+                   filter on some PUs)
   pframe           inter_frame plus the residual TUs of the non-skipped CUs
                    (a transform tree per CU) and where each TU lands
 
@@ -26,6 +29,11 @@ assert TU_DTYPE.itemsize == 16
 PU_DTYPE = np.dtype([("x", "<u2"), ("y", "<u2"), ("log2_w", "u1"), ("log2_h", "u1"),
                      ("frame", "<u2"), ("mv_x", "<i4"), ("mv_y", "<i4")])
 assert PU_DTYPE.itemsize == 16
+
+BI_PU_DTYPE = np.dtype([("x", "<u2"), ("y", "<u2"), ("log2_w", "u1"), ("log2_h", "u1"), ("frame", "<u2"),
+                        ("mv0_x", "<i4"), ("mv0_y", "<i4"), ("mv1_x", "<i4"), ("mv1_y", "<i4"),
+                        ("bcw_idx", "u1"), ("reserved", "u1", (3,))])
+assert BI_PU_DTYPE.itemsize == 28
 
 DCT2, DST7, DCT8 = 0, 1, 2
 TS_FLAG, DQ_FLAG = 1, 2
@@ -283,6 +291,27 @@ def inter_frame(seed, width=1920, height=1080, bit_depth=10, mv_range=64, qp=32,
     chroma_qp = qp if qp <= 26 else qp - 1  # default chroma QP mapping, vvc/hls.rs:778-781
     return dict(ref=ref, pus=pus, cus=cus, edges=edges, qp=[qp, chroma_qp, chroma_qp], ctu=ctu,
                 bit_depth=bit_depth, width=width, height=height)
+
+
+def bi_frame(seed, width=1920, height=1080, bit_depth=10, mv_range=64, ctu=128):
+    """Returns dict(ref0=[Y,Cb,Cr], ref1=[Y,Cb,Cr] uint16, pus=BI_PU_DTYPE array, bit_depth, width,
+    height). BCW indices are drawn from 0..4 with a few 5..7, which the reference clamps to 4."""
+    rng = np.random.default_rng([4, seed])
+    cw, ch = (width + 1) // 2, (height + 1) // 2
+    refs = [[textured_plane(rng, width, height, bit_depth), textured_plane(rng, cw, ch, bit_depth),
+             textured_plane(rng, cw, ch, bit_depth)] for _ in range(2)]
+    cus = _tile(rng, width, height, ctu, ctu, 8, _p_split_inter)
+    n = len(cus)
+    pus = np.zeros(n, BI_PU_DTYPE)
+    pus["x"] = [c[0] for c in cus]
+    pus["y"] = [c[1] for c in cus]
+    pus["log2_w"] = [int(np.log2(c[2])) for c in cus]
+    pus["log2_w"] |= np.where(rng.random(n) < 0.25, 0x10, 0).astype(np.uint8)
+    pus["log2_h"] = [int(np.log2(c[3])) for c in cus]
+    for k in ("mv0_x", "mv0_y", "mv1_x", "mv1_y"):
+        pus[k] = 16 * rng.integers(-mv_range, mv_range + 1, n) + rng.integers(0, 16, n)
+    pus["bcw_idx"] = np.where(rng.random(n) < 0.05, rng.integers(5, 8, n), rng.integers(0, 5, n))
+    return dict(ref0=refs[0], ref1=refs[1], pus=pus, bit_depth=bit_depth, width=width, height=height)
 
 
 # --------------------------------------------------------------- P-frames
