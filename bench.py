@@ -216,6 +216,93 @@ def bench_streams(rmh, acc, W, PL, po, frame_steps, reps):
     return out
 
 
+def bench_bi(rmh, acc, W, po, n_frames, steps, warmup):
+    """Bi-prediction with BCW weights over n_frames 1080p10 frames (two
+    references each, one bi PU per CU): one launch of the bi plan."""
+    hip, abi = rmh.hip, rmh.abi
+    t0 = time.perf_counter()
+    frames = [W.bi_frame(4000 + i) for i in range(n_frames)]
+    log("bi: generated %d frames in %.1f s" % (n_frames, time.perf_counter() - t0))
+    bufs, structs = [], []
+    for d in frames:
+        planes = {}
+        for key in ("ref0", "ref1", "dst"):
+            ps = []
+            for p in d["ref0" if key == "dst" else key]:
+                b = hip.DeviceBuffer.from_array(np.zeros_like(p) if key == "dst" else p)
+                bufs.append(b)
+                ps.append(abi.plane_from_address(b.ptr, p.shape[1], p.shape[0], p.shape[1], 2, d["bit_depth"]))
+            planes[key] = (abi.Plane * 3)(*ps)
+        structs.append(abi.McBiFrame(planes["ref0"], planes["ref1"], planes["dst"]))
+    pus = []
+    for i, d in enumerate(frames):
+        p = d["pus"].copy()
+        p["frame"] = i
+        pus.append(p)
+    pus = np.concatenate(pus)
+    plan = acc.mc_bi_plan(structs, pus)
+    stream = hip.Stream()
+    plan.run(stream=stream.handle)
+    stream.synchronize()
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    from frames import mc_bi_frame_struct
+    for fi in (0, n_frames - 1):
+        d = frames[fi]
+        want = [np.zeros_like(p) for p in d["ref0"]]
+        f, keep = mc_bi_frame_struct(d["ref0"], d["ref1"], want, d["bit_depth"])
+        po.mc_bi_batch(f, d["pus"])
+        for c in range(3):
+            got = bufs[9 * fi + 6 + c].download(np.uint16).reshape(want[c].shape)
+            if not (got == want[c]).all():
+                log("bench: bi-prediction frame %d plane %d differs from the oracle" % (fi, c))
+                return None
+    ms = timed(hip, stream, lambda s: plan.run(stream=s), steps, warmup)
+    px = n_frames * 1920 * 1080
+    out = {"workload": "%d frames 1920x1080 10-bit 4:2:0, one bi PU per CU (8x8..128x128), independent L0/L1 MVs "
+                       "+-64 px at any fraction, BCW weights, one launch" % n_frames,
+           "value": round(px / (ms / 1e3) / 1e6, 1), "unit": "Mpix/s", "ms_per_step": round(ms, 4),
+           "roofline": dict(stage_roofline(plan, ms), kernel="mc_bi_kernel")}
+    log("bi: %.4f ms/step %.1f Mpix/s (%.1f GB/s)" % (ms, out["value"], out["roofline"]["achieved_gbs"]))
+    plan.close()
+    for b in bufs:
+        b.close()
+    return out
+
+
+def bench_tool_mix(rmh, acc, W, po, n_frames, steps, warmup):
+    """The residual stage with LFNST, transform skip and dependent
+    quantisation in the mix: TUs with those tools run on the generic
+    one-TU-per-workgroup kernel, the rest on the tiles."""
+    hip = rmh.hip
+    frames = [W.residual_frame(5000 + i, with_lfnst=True, with_ts=True, with_dq=True)[:2] for i in range(n_frames)]
+    tus, coeff = W.concat_frames(frames)
+    plan = acc.residual_plan(tus, coeff.size)
+    d_coeff = hip.DeviceBuffer.from_array(coeff)
+    d_res = hip.DeviceBuffer(coeff.nbytes)
+    stream = hip.Stream()
+    plan.run(d_coeff.ptr, d_res.ptr, stream.handle)
+    stream.synchronize()
+    got = d_res.download(np.int16)
+    t, c = frames[0]
+    if not (got[:c.size] == po.residual_batch(t, c)).all():
+        log("bench: tool-mix residuals differ from the oracle on frame 0")
+        return None
+    ms = timed(hip, stream, lambda s: plan.run(d_coeff.ptr, d_res.ptr, s), steps, warmup)
+    generic = (tus["lfnst_idx"] > 0) | ((tus["flags"] & (W.TS_FLAG | W.DQ_FLAG)) != 0)
+    px = n_frames * 1920 * 1080
+    out = {"workload": "%d frames of 1920x1080 10-bit TUs with LFNST (30 %%), transform skip (10 %%) and dependent "
+                       "quantisation (20 %%) drawn per TU, one plan run" % n_frames,
+           "value": round(px / (ms / 1e3) / 1e6, 1), "unit": "Mpix/s", "ms_per_step": round(ms, 4),
+           "tus": int(len(tus)), "tus_on_generic_kernel": int(generic.sum()),
+           "roofline": dict(stage_roofline(plan, ms), kernel="residual_tile_kernel<false> + residual_kernel")}
+    log("tool mix: %.4f ms/step %.1f Mpix/s, %d of %d TUs on the generic kernel" % (
+        ms, out["value"], out["tus_on_generic_kernel"], out["tus"]))
+    plan.close()
+    d_coeff.close()
+    d_res.close()
+    return out
+
+
 CNN_MACS_PER_BLOCK = 32 * 32 * 32 * 9 + 2 * 32 * 32 * 32 * 288 + 33 * 64 + 64 * 6
 
 
@@ -288,6 +375,8 @@ def main():
     ap.add_argument("--no-streams", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-bi", action="store_true")
+    ap.add_argument("--no-tools", action="store_true")
     args = ap.parse_args()
 
     if args.copy_calibration:
@@ -408,6 +497,8 @@ def main():
                                                          args.warmup)
     streams = None if args.no_streams else bench_streams(rmh, acc, W, PL, po, 8, max(2, args.steps // 5))
 
+    bi = None if args.no_bi else bench_bi(rmh, acc, W, po, args.pipeline_frames, args.steps, args.warmup)
+    tools = None if args.no_tools else bench_tool_mix(rmh, acc, W, po, 8, args.steps, args.warmup)
     learned = None if args.no_learned else bench_learned(rmh, acc, args.steps)
 
     cpu = None
@@ -432,6 +523,8 @@ def main():
         "cpu_baseline": cpu,
         "pframe_pipeline": pipe,
         "multi_stream": streams,
+        "bi_prediction": bi,
+        "residual_tool_mix": tools,
         "learned": learned,
         "device": info,
     }
