@@ -217,33 +217,47 @@ __device__ __forceinline__ void set_taps(Tile<TAPS>& t, const int8_t* cx, const 
     }
 }
 
-// The 14-bit bi-prediction intermediate of output sample (x, y) of a tile
+// The 14-bit bi-prediction intermediates of the 2 x 2 output samples
+// (x, y), (x + 1, y), (x, y + 1), (x + 1, y + 1), x and y even, of a tile
 // (for_bi_pred branches of interpolate_luma_block_16 / interpolate_chroma_
 // block_format_16, vvc/inter.rs:2046-2133, 2582-2669): a copy gives
 // (s << head_room) - 8192, a one-dimensional filter (sum >> shift1) - 8192,
 // a two-dimensional one the vertical sum over the horizontal pass's
-// intermediates (which already hold (sum >> shift1) - 8192), >> 6.
+// intermediates (which already hold (sum >> shift1) - 8192), >> 6. Pairs
+// along the filter direction come from hpair's aligned dword reads.
 template <int TAPS, int PITCH>
-__device__ __forceinline__ int bi_sample(const Tile<TAPS>& t, const int16_t* win, const int16_t* tmp, int x, int y,
-                                         int head_room, int shift1) {
+__device__ __forceinline__ void bi_quad(const Tile<TAPS>& t, const int16_t* win, const int16_t* tmp, int x, int y,
+                                        int head_room, int shift1, int (&v)[4]) {
     constexpr int BEFORE = TAPS / 2 - 1;
-    int s = 0;
     if (t.hf && t.vf) {
+        int s0, s1, s2, s3;
+        hpair<TAPS>(tmp + x * PITCH + y, t.cy, s0, s2);
+        hpair<TAPS>(tmp + (x + 1) * PITCH + y, t.cy, s1, s3);
+        v[0] = s0 >> 6;
+        v[1] = s1 >> 6;
+        v[2] = s2 >> 6;
+        v[3] = s3 >> 6;
+    } else if (t.hf) {
+        int s0, s1, s2, s3;
+        hpair<TAPS>(win + (y + BEFORE) * PITCH + x, t.cx, s0, s1);
+        hpair<TAPS>(win + (y + 1 + BEFORE) * PITCH + x, t.cx, s2, s3);
+        v[0] = (s0 >> shift1) - 8192;
+        v[1] = (s1 >> shift1) - 8192;
+        v[2] = (s2 >> shift1) - 8192;
+        v[3] = (s3 >> shift1) - 8192;
+    } else if (t.vf) {
 #pragma unroll
-        for (int k = 0; k < TAPS; k++) s += t.sy[k] * tmp[x * PITCH + y + k];
-        return s >> 6;
-    }
-    if (t.hf) {
+        for (int q = 0; q < 4; q++) {
+            int s = 0;
 #pragma unroll
-        for (int k = 0; k < TAPS; k++) s += t.sx[k] * win[(y + BEFORE) * PITCH + x + k];
-        return (s >> shift1) - 8192;
-    }
-    if (t.vf) {
+            for (int k = 0; k < TAPS; k++) s += t.sy[k] * win[(y + (q >> 1) + k) * PITCH + x + (q & 1) + BEFORE];
+            v[q] = (s >> shift1) - 8192;
+        }
+    } else {
 #pragma unroll
-        for (int k = 0; k < TAPS; k++) s += t.sy[k] * win[(y + k) * PITCH + x + BEFORE];
-        return (s >> shift1) - 8192;
+        for (int q = 0; q < 4; q++)
+            v[q] = ((int)win[(y + (q >> 1) + BEFORE) * PITCH + x + (q & 1) + BEFORE] << head_room) - 8192;
     }
-    return ((int)win[(y + BEFORE) * PITCH + x + BEFORE] << head_room) - 8192;
 }
 
 // Four work items per 256-lane workgroup, one per wave: the 16x16 luma tile
@@ -340,19 +354,26 @@ __device__ __forceinline__ void bi_tiles(const McBiItem& it, int mv_x, int mv_y,
 }
 
 // Weighted average of the two lists' intermediates over one plane's tile
-// (weighted_average_bi_16, vvc/inter.rs:2676-2696), T = full tile width.
+// (weighted_average_bi_16, vvc/inter.rs:2676-2696), T = full tile width; a
+// lane takes 2 x 2 output samples.
 template <int TAPS, int PITCH, int T>
 __device__ __forceinline__ void bi_output(const rmh_plane& dst, const Tile<TAPS>& t0, const Tile<TAPS>& t1,
                                           const int16_t* win0, const int16_t* tmp0, const int16_t* win1,
                                           const int16_t* tmp1, int head_room, int w0, int w1, int maxv, int lane) {
     const int shift1 = 6 - head_room, shift = head_room + 3;
     const int off = (1 << (shift - 1)) + (8192 << 3);
-    for (int e = lane; e < T * T; e += 64) {
-        const int x = e % T, y = e / T;
+    for (int e = lane; e < (T / 2) * (T / 2); e += 64) {
+        const int x = 2 * (e % (T / 2)), y = 2 * (e / (T / 2));
         if (x >= t0.tw || y >= t0.th) continue;
-        const int p0 = bi_sample<TAPS, PITCH>(t0, win0, tmp0, x, y, head_room, shift1);
-        const int p1 = bi_sample<TAPS, PITCH>(t1, win1, tmp1, x, y, head_room, shift1);
-        store_sample(dst, t0.x0 + x, t0.y0 + y, clampi((p0 * w0 + p1 * w1 + off) >> shift, 0, maxv));
+        int p0[4], p1[4];
+        bi_quad<TAPS, PITCH>(t0, win0, tmp0, x, y, head_room, shift1, p0);
+        bi_quad<TAPS, PITCH>(t1, win1, tmp1, x, y, head_room, shift1, p1);
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int xq = x + (q & 1), yq = y + (q >> 1);
+            if (xq < t0.tw && yq < t0.th)
+                store_sample(dst, t0.x0 + xq, t0.y0 + yq, clampi((p0[q] * w0 + p1[q] * w1 + off) >> shift, 0, maxv));
+        }
     }
 }
 

@@ -207,35 +207,60 @@ __global__ __launch_bounds__(64) void residual_kernel(ResidualArgs a) {
             lv[e] = cin[y * w + x];
         }
         __syncthreads();
-        if (lane == 0) {
-            // dequantize_dep_quant_bd (vvc/dct.rs:1191-1248): serial 4-state automaton
-            const uint16_t* scan = a.scans + a.scan_off[lw][lh];
-            const int len = a.scan_len[lw][lh];
-            int last = -1;
-            for (int s = len - 1; s >= 0; s--) {
-                int r = scan[s];
-                if (lv[(r >> lw) * lw_c + (r & (w - 1))] != 0) {
-                    last = s;
-                    break;
-                }
+        // dequantize_dep_quant_bd (vvc/dct.rs:1191-1248): from the last
+        // non-zero position down to 0 in scan order, a 4-state automaton
+        // driven by the parity of each level (zeros included) sets the
+        // reconstruction offset. The state sequence depends only on the
+        // parities, and each step is one of two maps of {0..3}, so it is a
+        // prefix of map compositions: every lane composes the maps of its
+        // chunk of the sequence, a wave scan gives each lane its start
+        // state, and the lanes then dequantise their chunks in parallel.
+        const uint16_t* scan = a.scans + a.scan_off[lw][lh];
+        const int len = a.scan_len[lw][lh];
+        auto level_at = [&](int s) {
+            const int r = scan[s];
+            return (int)lv[(r >> lw) * lw_c + (r & (w - 1))];
+        };
+        int last = -1;
+        for (int s = lane; s < len; s += 64)
+            if (level_at(s) != 0) last = s;
+        last = wave_max(last);
+        const int L = last + 1, chunk = (L + 63) >> 6;
+        const int t0 = lane * chunk, t1 = min(t0 + chunk, L);  // positions s = last - t, t in [t0, t1)
+        // maps packed as 4 x 2 bits, entry x = image of state x; step maps
+        // T0 = (0, 2, 1, 3) and T1 = (2, 0, 3, 1), from 32040 >> (4 x + 2 p)
+        constexpr uint32_t kT0 = 0u | 2u << 2 | 1u << 4 | 3u << 6, kT1 = 2u | 0u << 2 | 3u << 4 | 1u << 6;
+        auto apply = [](uint32_t g, uint32_t f) {  // g o f
+            uint32_t h = 0;
+#pragma unroll
+            for (int x = 0; x < 4; x++) h |= ((g >> (2 * ((f >> (2 * x)) & 3))) & 3) << (2 * x);
+            return h;
+        };
+        uint32_t f = 0xE4u;  // identity (0, 1, 2, 3)
+        for (int t = t0; t < t1; t++) f = apply((level_at(last - t) & 1) ? kT1 : kT0, f);
+        // inclusive scan of the chunk maps over the lanes (lane j: f_j o ... o f_0)
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint32_t o = (uint32_t)__shfl_up((int)f, d);
+            if (lane >= d) f = apply(f, o);
+        }
+        const uint32_t prev = (uint32_t)__shfl_up((int)f, 1);
+        int state = lane ? (int)(prev & 3) : 0;  // lanes 0 .. lane - 1's maps applied to the start state 0
+        const int qdq = qp + 1, per = qdq / 6, rem = qdq % 6;
+        const int shift = 7 - per - tr_shift;
+        int64_t inv = kInvQuant[sqrt2][rem];
+        if (shift < 0) inv = (int64_t)((uint64_t)inv << (-shift));
+        const int64_t add = shift > 0 ? ((int64_t)1 << (shift - 1)) : 0;
+        const int rs = shift > 0 ? shift : 0;
+        for (int t = t0; t < t1; t++) {
+            const int r = scan[last - t];
+            const int x = r & (w - 1), y = r >> lw;
+            const int level = lv[y * lw_c + x];
+            if (level != 0) {
+                const int64_t q = 2 * (int64_t)level + (level > 0 ? -(state >> 1) : (state >> 1));
+                if (x < rw && y < rh) ct16[x * 2 * kPitch + y] = (int16_t)clip16((q * inv + add) >> rs);
             }
-            const int qdq = qp + 1, per = qdq / 6, rem = qdq % 6;
-            const int shift = 7 - per - tr_shift;
-            int64_t inv = kInvQuant[sqrt2][rem];
-            if (shift < 0) inv = (int64_t)((uint64_t)inv << (-shift));
-            const int64_t add = shift > 0 ? ((int64_t)1 << (shift - 1)) : 0;
-            const int rs = shift > 0 ? shift : 0;
-            int state = 0;
-            for (int s = last; s >= 0; s--) {
-                int r = scan[s];
-                int x = r & (w - 1), y = r >> lw;
-                int level = lv[y * lw_c + x];
-                if (level != 0) {
-                    int64_t q = 2 * (int64_t)level + (level > 0 ? -(state >> 1) : (state >> 1));
-                    if (x < rw && y < rh) ct16[x * 2 * kPitch + y] = (int16_t)clip16((q * inv + add) >> rs);
-                }
-                state = (32040 >> ((state << 2) + ((level & 1) << 1))) & 3;
-            }
+            state = (32040 >> ((state << 2) + ((level & 1) << 1))) & 3;
         }
     }
     __syncthreads();

@@ -14,6 +14,7 @@ The pieces, all on librmh:
 """
 import concurrent.futures as cf
 import math
+import threading
 import time
 
 import numpy as np
@@ -184,30 +185,43 @@ def psnr(a, b):
     return 99.0 if mse == 0 else 10 * math.log10(255.0 ** 2 / mse)
 
 
-def encode(frames, qp, mode, net=None, p_prune=0.05, p_fallback=0.6, workers=8):
-    """Encodes frames at one QP; returns totals: bits, PSNR (on the padded
-    picture), wall seconds of the whole encode (network inference included),
-    candidates per node, fallback rate, the splits chosen."""
+def encode(frames, qps, mode, net=None, p_prune=0.05, p_fallback=0.6, workers=8):
+    """Encodes every frame at every QP as one pool of (frame, QP) jobs; returns
+    (per-QP totals, wall seconds of the whole encode, network inference
+    included). Per QP: bits, PSNR (on the padded picture), candidates per
+    node, fallback rate, the splits chosen. In pruned mode each job runs the
+    network on its own frame first; one network instance serves all jobs, so
+    inference is serialised by a lock and overlaps other jobs' RD search."""
     imgs = [pad_to_nodes(f) for f in frames]
-    t0 = time.perf_counter()
-    probs = [None] * len(imgs)
-    if mode == abi.RD_PRUNED:
-        for i, img in enumerate(imgs):
+    lock = threading.Lock()
+
+    def job(args):
+        img, qp = args
+        probs = None
+        if mode == abi.RD_PRUNED:
             b = node_blocks(img)
-            probs[i] = net.predict(b, np.full(len(b), qp_feature(qp), np.float32))
-    jobs = [(img, qp, mode, p, p_prune, p_fallback) for img, p in zip(imgs, probs)]
+            with lock:
+                probs = net.predict(b, np.full(len(b), qp_feature(qp), np.float32))
+        return _rd((img, qp, mode, probs, p_prune, p_fallback))
+
+    jobs = [(img, qp) for qp in qps for img in imgs]
+    t0 = time.perf_counter()
     with cf.ThreadPoolExecutor(workers) as ex:
-        res = list(ex.map(_rd, jobs))
+        res = list(ex.map(job, jobs))
     sec = time.perf_counter() - t0
-    bits = sum(r[2]["bits"] for r in res)
-    sse = sum(r[2]["sse"] for r in res)
-    px = sum(img.size for img in imgs)
-    nodes = sum(r[2]["blocks"] for r in res)
-    mse = sse / px
-    return dict(qp=qp, bits=bits, psnr=99.0 if mse == 0 else 10 * math.log10(255.0 ** 2 / mse), seconds=sec,
-                fps=len(imgs) / sec, candidates_per_node=sum(r[2]["candidates"] for r in res) / nodes,
-                fallback_rate=sum(r[2]["fallbacks"] for r in res) / nodes,
-                splits=np.concatenate([r[1].reshape(-1) for r in res]))
+    out = []
+    for k, qp in enumerate(qps):
+        r = res[k * len(imgs):(k + 1) * len(imgs)]
+        sse = sum(x[2]["sse"] for x in r)
+        px = sum(img.size for img in imgs)
+        nodes = sum(x[2]["blocks"] for x in r)
+        mse = sse / px
+        out.append(dict(qp=qp, bits=sum(x[2]["bits"] for x in r),
+                        psnr=99.0 if mse == 0 else 10 * math.log10(255.0 ** 2 / mse),
+                        candidates_per_node=sum(x[2]["candidates"] for x in r) / nodes,
+                        fallback_rate=sum(x[2]["fallbacks"] for x in r) / nodes,
+                        splits=np.concatenate([x[1].reshape(-1) for x in r])))
+    return out, sec
 
 
 # ------------------------------------------------------------------ BD-rate
@@ -232,19 +246,26 @@ def bd_rate(anchor, test):
     return (10 ** avg - 1) * 100
 
 
-def evaluate(frames, net, qps=QPS, p_prune=0.05, p_fallback=0.6, workers=8):
+def baselines(frames, qps=QPS, workers=8):
+    """The full-RD and heuristic encodes evaluate() compares against."""
+    return {"full": encode(frames, qps, abi.RD_FULL, workers=workers),
+            "heuristic": encode(frames, qps, abi.RD_HEURISTIC, workers=workers)}
+
+
+def evaluate(frames, net, qps=QPS, p_prune=0.05, p_fallback=0.6, workers=8, base=None):
     """Full RD, CNN-pruned and heuristic encodes of the same frames at every
     QP: fps, candidates per node, BD-rates against full RD, and how often
-    each fast method picks the full search's split."""
-    out = {"full": [], "pruned": [], "heuristic": []}
-    for qp in qps:
-        out["full"].append(encode(frames, qp, abi.RD_FULL, workers=workers))
-        out["pruned"].append(encode(frames, qp, abi.RD_PRUNED, net, p_prune, p_fallback, workers))
-        out["heuristic"].append(encode(frames, qp, abi.RD_HEURISTIC, workers=workers))
+    each fast method picks the full search's split. base: baselines() of the
+    same frames, to reuse."""
+    base = base or baselines(frames, qps, workers)
+    out, secs = {}, {}
+    for k in ("full", "heuristic"):
+        out[k], secs[k] = base[k]
+    out["pruned"], secs["pruned"] = encode(frames, qps, abi.RD_PRUNED, net, p_prune, p_fallback, workers)
     pts = {k: [(r["psnr"], r["bits"]) for r in v] for k, v in out.items()}
     summ = {}
     for k, v in out.items():
-        summ[k] = dict(fps=len(frames) * len(qps) / sum(r["seconds"] for r in v),
+        summ[k] = dict(fps=len(frames) * len(qps) / secs[k],
                        candidates_per_node=float(np.mean([r["candidates_per_node"] for r in v])),
                        bd_rate_vs_full=None if k == "full" else bd_rate(pts["full"], pts[k]),
                        split_agreement=float(np.mean(np.concatenate([r["splits"] for r in v]) ==
@@ -256,9 +277,12 @@ def evaluate(frames, net, qps=QPS, p_prune=0.05, p_fallback=0.6, workers=8):
     return summ
 
 
-def run_experiment(acc, train_seeds=(1, 3, 4, 5, 6, 7, 8, 9), val_seeds=(10, 11), test_seeds=(2, 12),
-                   frames_per_seq=6, eval_frames=2, width=1280, height=720, epochs=40, lr=3e-3, batch=256,
-                   scales=(1.0, 1.5, 2.0, 2.5, 3.0, 3.5, 4.0, 5.0), bd_target=1.5, workers=16, seed=1, log=None):
+TRAIN_SEEDS = (1, 3, 4, 5, 6, 7, 8, 9, 17, 18, 19, 20, 21, 22, 23, 24)
+
+
+def run_experiment(acc, train_seeds=TRAIN_SEEDS, val_seeds=(10, 11, 13, 14), test_seeds=(2, 12, 15, 16),
+                   frames_per_seq=6, eval_frames=2, width=1280, height=720, epochs=30, lr=3e-3, batch=256,
+                   scales=(1.5, 2.0, 2.5, 3.0, 3.5, 4.0, 4.5, 5.0), bd_target=1.5, workers=16, seed=1, log=None):
     """Collects full-RD pairs on the training sequences, trains the network on
     the GPU, picks the largest logit scale whose pruned encode stays within
     bd_target % BD-rate of the full search on the validation sequences, and
@@ -280,22 +304,24 @@ def run_experiment(acc, train_seeds=(1, 3, 4, 5, 6, 7, 8, 9), val_seeds=(10, 11)
     test = sum([synthetic_sequence(s, eval_frames, width, height) for s in test_seeds], [])
     trained = net.params()
     chosen, val_curve = scales[0], []
+    val_base = baselines(val, workers=workers)
     for sc in scales:
         net.set_params(trained)
         set_logit_scale(net, sc)
-        v = evaluate(val, net, workers=workers)
+        v = evaluate(val, net, workers=workers, base=val_base)
         val_curve.append((sc, round(v["pruned"]["bd_rate_vs_full"], 3), round(v["pruned"]["candidates_per_node"], 3)))
         say("scale %.2f: validation BD-rate %.3f %%, %.2f candidates per node" % val_curve[-1])
         if v["pruned"]["bd_rate_vs_full"] <= bd_target:
             chosen = sc
     net.set_params(trained)
     set_logit_scale(net, chosen)
-    res = evaluate(test, net, workers=workers)
+    test_base = baselines(test, workers=workers)
+    res = evaluate(test, net, workers=workers, base=test_base)
     # the same scale on the untrained network: what training buys
     ref = Cnn(acc, max_batch=batch, seed=seed)
     ref.set_params(untrained)
     set_logit_scale(ref, chosen)
-    res_untrained = evaluate(test, ref, workers=workers)
+    res_untrained = evaluate(test, ref, workers=workers, base=test_base)
     ref.close()
     report = dict(train_pairs=int(len(blocks)), collect_seconds=round(t_collect, 2), train_seconds=round(t_train, 2),
                   epochs=epochs, loss_first=round(hist[0], 4), loss_last=round(hist[-1], 4), logit_scale=chosen,
