@@ -148,11 +148,12 @@ __global__ __launch_bounds__(256) void conv_wgrad32_kernel(const uint16_t* __res
 }
 
 // Weight gradient of the 1-channel first convolution (VALU): lane (cout, tap)
-// sums over the 1024 pixels of one image; part[n][cout][tap].
+// sums over one quarter (8 rows) of the pixels of one image;
+// part[n * 4 + quarter][cout][tap].
 __global__ __launch_bounds__(288) void conv_wgrad1_kernel(const uint16_t* __restrict__ dz, const uint16_t* __restrict__ x,
                                                           float* __restrict__ part) {
     __shared__ float xs[34 * 34];
-    const int n = blockIdx.x, tid = threadIdx.x;
+    const int n = blockIdx.x, quarter = blockIdx.y, tid = threadIdx.x;
     for (int e = tid; e < 34 * 34; e += 288) {
         const int r = e / 34 - 1, c = e % 34 - 1;
         xs[e] = (r >= 0 && r < 32 && c >= 0 && c < 32) ? bf2f(x[(size_t)n * 1024 + r * 32 + c]) : 0.0f;
@@ -161,33 +162,65 @@ __global__ __launch_bounds__(288) void conv_wgrad1_kernel(const uint16_t* __rest
     const int co = tid / 9, tap = tid % 9, ky = tap / 3, kx = tap % 3;
     const uint16_t* d = dz + (size_t)n * 1024 * 32;
     float s = 0.0f;
-    for (int p = 0; p < 1024; p++) s += bf2f(d[p * 32 + co]) * xs[((p >> 5) + ky) * 34 + (p & 31) + kx];
-    part[(size_t)n * 288 + co * 9 + tap] = s;
+#pragma unroll 4
+    for (int p = quarter * 256; p < quarter * 256 + 256; p++) s += bf2f(d[p * 32 + co]) * xs[((p >> 5) + ky) * 34 + (p & 31) + kx];
+    part[((size_t)n * 4 + quarter) * 288 + co * 9 + tap] = s;
+}
+
+// Per-channel sum over the 1024 pixels of one image's [p][32] bf16 map, in a
+// fixed order: 16-byte loads (8 channels of one pixel), lane t always holding
+// channels 8 (t & 3) .. + 7, then the 64 lanes of each channel group added in
+// lane order. Returns the sum for channel tid (< 32) of the workgroup.
+__device__ __forceinline__ float channel_sum(const uint16_t* __restrict__ map, float (*red)[9]) {
+    const int tid = threadIdx.x;
+    const uint4* d = reinterpret_cast<const uint4*>(map);
+    float acc[8] = {};
+#pragma unroll 4
+    for (int k = 0; k < 16; k++) {
+        const uint4 v = d[tid + 256 * k];
+        const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            acc[2 * j] += bf2f((uint16_t)(w[j] & 0xffff));
+            acc[2 * j + 1] += bf2f((uint16_t)(w[j] >> 16));
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < 8; j++) red[tid][j] = acc[j];
+    __syncthreads();
+    float t = 0.0f;
+    if (tid < 32)
+        for (int q = tid >> 3; q < 256; q += 4) t += red[q][tid & 7];
+    return t;
 }
 
 // Bias gradient partials: part[n][c] = sum_p dz[n][p][c].
 __global__ __launch_bounds__(256) void bias_grad_kernel(const uint16_t* __restrict__ dz, float* __restrict__ part) {
-    __shared__ float red[8][32];
-    const int n = blockIdx.x, tid = threadIdx.x, c = tid & 31, g = tid >> 5;
-    const uint16_t* d = dz + (size_t)n * 1024 * 32;
-    float s = 0.0f;
-    for (int p = g; p < 1024; p += 8) s += bf2f(d[p * 32 + c]);
-    red[g][c] = s;
-    __syncthreads();
-    if (tid < 32) {
-        float t = 0.0f;
-        for (int k = 0; k < 8; k++) t += red[k][tid];
-        part[(size_t)n * 32 + tid] = t;
-    }
+    __shared__ float red[256][9];
+    const int n = blockIdx.x;
+    const float t = channel_sum(dz + (size_t)n * 1024 * 32, red);
+    if (threadIdx.x < 32) part[(size_t)n * 32 + threadIdx.x] = t;
 }
 
-// Sums partials over images: out[e] = sum_n part[n][e] (fixed order).
-__global__ void reduce_kernel(const float* __restrict__ part, int n, int len, float* __restrict__ out) {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= len) return;
+// Sums partials over images in a fixed order: out[e] = sum_k part[k][e],
+// k = 0 .. n - 1. A workgroup takes 32 elements; its 8 lane groups sum
+// images g, g + 8, ... and the 8 group sums are added in order.
+__global__ __launch_bounds__(256) void reduce_kernel(const float* __restrict__ part, int n, int len, float* __restrict__ out) {
+    __shared__ float red[8][32];
+    const int tid = threadIdx.x, el = tid & 31, g = tid >> 5;
+    const int e = blockIdx.x * 32 + el;
     float s = 0.0f;
-    for (int k = 0; k < n; k++) s += part[(size_t)k * len + e];
-    out[e] = s;
+    if (e < len) {
+#pragma unroll 4
+        for (int k = g; k < n; k += 8) s += part[(size_t)k * len + e];
+    }
+    red[g][el] = s;
+    __syncthreads();
+    if (g == 0 && e < len) {
+        float t = 0.0f;
+        for (int q = 0; q < 8; q++) t += red[q][el];
+        out[e] = t;
+    }
 }
 
 // Input of the first convolution: x = (b - 128) / 64 as bf16 (exact).
@@ -198,28 +231,23 @@ __global__ void input_kernel(const uint8_t* __restrict__ blocks, int total, uint
 
 // Global average pool: p[n][c] = mean_p a3[n][p][c].
 __global__ __launch_bounds__(256) void pool_kernel(const uint16_t* __restrict__ a3, float* __restrict__ pooled) {
-    __shared__ float red[8][32];
-    const int n = blockIdx.x, tid = threadIdx.x, c = tid & 31, g = tid >> 5;
-    const uint16_t* a = a3 + (size_t)n * 1024 * 32;
-    float s = 0.0f;
-    for (int p = g; p < 1024; p += 8) s += bf2f(a[p * 32 + c]);
-    red[g][c] = s;
-    __syncthreads();
-    if (tid < 32) {
-        float t = 0.0f;
-        for (int k = 0; k < 8; k++) t += red[k][tid];
-        pooled[n * 32 + tid] = t * (1.0f / 1024.0f);
-    }
+    __shared__ float red[256][9];
+    const int n = blockIdx.x;
+    const float t = channel_sum(a3 + (size_t)n * 1024 * 32, red);
+    if (threadIdx.x < 32) pooled[n * 32 + threadIdx.x] = t * (1.0f / 1024.0f);
 }
 
-// MLP forward (and, with labels, the softmax cross-entropy backward), one
-// lane per sample. Writes probs[n][6]; with labels also loss[n], dpool[n][32]
-// and the per-sample MLP weight gradients part[n][kMlpParams].
+// MLP forward (and, with labels or targets, the softmax cross-entropy
+// backward), one lane per sample. Writes probs[n][6]; in training also
+// loss[n], dpool[n][32] and the per-sample vectors the weight gradients are
+// built from: hidden hv[n][64], its gradient dhv[n][64] and the logit
+// gradient dzv[n][6].
 __global__ void mlp_kernel(const float* __restrict__ pooled, const float* __restrict__ qp, const float* __restrict__ w4,
                            const float* __restrict__ b4, const float* __restrict__ w5, const float* __restrict__ b5,
                            int n, float* __restrict__ probs, const uint8_t* __restrict__ labels,
                            const float* __restrict__ targets, float inv_batch, float* __restrict__ loss,
-                           float* __restrict__ dpool, float* __restrict__ part) {
+                           float* __restrict__ dpool, float* __restrict__ hv, float* __restrict__ dhv,
+                           float* __restrict__ dzv) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     float in[33], h[64], z[6];
@@ -260,30 +288,53 @@ __global__ void mlp_kernel(const float* __restrict__ pooled, const float* __rest
     }
     loss[s] = l;
     float dz[6];
-    for (int o = 0; o < 6; o++) dz[o] = (tsum * e[o] / den - t[o]) * inv_batch;
-    float* pw4 = part + (size_t)s * kMlpParams;
-    float* pb4 = pw4 + 64 * 33;
-    float* pw5 = pb4 + 64;
-    float* pb5 = pw5 + 6 * 64;
+    for (int o = 0; o < 6; o++) {
+        dz[o] = (tsum * e[o] / den - t[o]) * inv_batch;
+        dzv[s * 6 + o] = dz[o];
+    }
     float dh[64];
     for (int c = 0; c < 64; c++) {
         float t = 0.0f;
         for (int o = 0; o < 6; o++) t += w5[o * 64 + c] * dz[o];
         dh[c] = h[c] > 0.0f ? t : 0.0f;
-    }
-    for (int o = 0; o < 6; o++) {
-        for (int c = 0; c < 64; c++) pw5[o * 64 + c] = dz[o] * h[c];
-        pb5[o] = dz[o];
-    }
-    for (int o = 0; o < 64; o++) {
-        for (int c = 0; c < 33; c++) pw4[o * 33 + c] = dh[o] * in[c];
-        pb4[o] = dh[o];
+        hv[s * 64 + c] = h[c];
+        dhv[s * 64 + c] = dh[c];
     }
     for (int c = 0; c < 32; c++) {
         float t = 0.0f;
         for (int o = 0; o < 64; o++) t += w4[o * 33 + c] * dh[o];
         dpool[s * 32 + c] = t;
     }
+}
+
+// Weight gradients of the MLP summed over the batch in sample order, one lane
+// per parameter (layout W4[64][33], b4[64], W5[6][64], b5[6]):
+// dW4[o][c] = sum_s dh[s][o] in[s][c] with in = [pooled; qp], db4[o] = sum_s dh[s][o],
+// dW5[k][o] = sum_s dz[s][k] h[s][o], db5[k] = sum_s dz[s][k].
+__global__ __launch_bounds__(256) void mlp_wgrad_kernel(const float* __restrict__ pooled, const float* __restrict__ qp,
+                                                        const float* __restrict__ hv, const float* __restrict__ dhv,
+                                                        const float* __restrict__ dzv, int n, float* __restrict__ g) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= kMlpParams) return;
+    float s = 0.0f;
+    if (p < 64 * 33) {
+        const int o = p / 33, c = p % 33;
+#pragma unroll 4
+        for (int k = 0; k < n; k++) s += dhv[k * 64 + o] * (c < 32 ? pooled[k * 32 + c] : qp[k]);
+    } else if (p < 64 * 33 + 64) {
+        const int o = p - 64 * 33;
+#pragma unroll 4
+        for (int k = 0; k < n; k++) s += dhv[k * 64 + o];
+    } else if (p < 64 * 33 + 64 + 6 * 64) {
+        const int q = p - (64 * 33 + 64), kk = q / 64, o = q % 64;
+#pragma unroll 4
+        for (int k = 0; k < n; k++) s += dzv[k * 6 + kk] * hv[k * 64 + o];
+    } else {
+        const int kk = p - (64 * 33 + 64 + 6 * 64);
+#pragma unroll 4
+        for (int k = 0; k < n; k++) s += dzv[k * 6 + kk];
+    }
+    g[p] = s;
 }
 
 // Gradient into the last convolution's output: dz3 = (a3 > 0) * dpool / 1024.
@@ -332,47 +383,63 @@ inline unsigned blocks_for(size_t n, unsigned t) { return (unsigned)((n + t - 1)
 
 }  // namespace
 
-hipError_t forward(const Buffers& b, const uint8_t* blocks, const float* qp, int n, float* probs, hipStream_t s) {
+namespace {
+
+// Input, the three convolutions and the pool: b.pooled[n][32].
+void features(const Buffers& b, const uint8_t* blocks, int n, hipStream_t s) {
     const size_t px = (size_t)n * 1024;
     hipLaunchKernelGGL(input_kernel, dim3(blocks_for(px, 256)), dim3(256), 0, s, blocks, (int)px, b.x);
     hipLaunchKernelGGL(conv_fwd_kernel<1>, dim3(n, 8), dim3(256), 0, s, b.x, b.w1, b.params + kOffB1, nullptr, b.a1, 1);
     hipLaunchKernelGGL(conv_fwd_kernel<32>, dim3(n, 8), dim3(256), 0, s, b.a1, b.w2, b.params + kOffB2, nullptr, b.a2, 1);
     hipLaunchKernelGGL(conv_fwd_kernel<32>, dim3(n, 8), dim3(256), 0, s, b.a2, b.w3, b.params + kOffB3, nullptr, b.a3, 1);
     hipLaunchKernelGGL(pool_kernel, dim3(n), dim3(256), 0, s, b.a3, b.pooled);
+}
+
+void reduce(const float* part, int n, int len, float* out, hipStream_t s) {
+    hipLaunchKernelGGL(reduce_kernel, dim3((len + 31) / 32), dim3(256), 0, s, part, n, len, out);
+}
+
+}  // namespace
+
+hipError_t forward(const Buffers& b, const uint8_t* blocks, const float* qp, int n, float* probs, hipStream_t s) {
+    features(b, blocks, n, s);
     hipLaunchKernelGGL(mlp_kernel, dim3(blocks_for(n, 64)), dim3(64), 0, s, b.pooled, qp, b.params + kOffW4,
                        b.params + kOffB4, b.params + kOffW5, b.params + kOffB5, n, probs, nullptr, nullptr, 0.0f,
-                       nullptr, nullptr, nullptr);
+                       nullptr, nullptr, nullptr, nullptr, nullptr);
     return hipGetLastError();
 }
 
 hipError_t gradients(const Buffers& b, const uint8_t* blocks, const float* qp, const uint8_t* labels,
                      const float* targets, int n, float* grads, float* loss, hipStream_t s) {
-    hipError_t e = forward(b, blocks, qp, n, b.probs, s);
-    if (e != hipSuccess) return e;
+    features(b, blocks, n, s);
     const size_t act = (size_t)n * 1024 * 32;
+    // the MLP's per-sample vectors live in b.part until the convolution gradients reuse it
+    float* hv = b.part;
+    float* dhv = hv + (size_t)n * 64;
+    float* dzv = dhv + (size_t)n * 64;
     hipLaunchKernelGGL(mlp_kernel, dim3(blocks_for(n, 64)), dim3(64), 0, s, b.pooled, qp, b.params + kOffW4,
                        b.params + kOffB4, b.params + kOffW5, b.params + kOffB5, n, b.probs, labels, targets,
-                       1.0f / (float)n, loss, b.dpool, b.part);
-    hipLaunchKernelGGL(reduce_kernel, dim3(blocks_for(kMlpParams, 256)), dim3(256), 0, s, b.part, n, kMlpParams,
+                       1.0f / (float)n, loss, b.dpool, hv, dhv, dzv);
+    hipLaunchKernelGGL(mlp_wgrad_kernel, dim3(blocks_for(kMlpParams, 256)), dim3(256), 0, s, b.pooled, qp, hv, dhv, dzv, n,
                        grads + kOffW4);
     // conv3
     hipLaunchKernelGGL(pool_back_kernel, dim3(blocks_for(act, 256)), dim3(256), 0, s, b.a3, b.dpool, (int)act, b.dz);
     hipLaunchKernelGGL(conv_wgrad32_kernel, dim3(n, 9), dim3(256), 0, s, b.dz, b.a2, b.part);
-    hipLaunchKernelGGL(reduce_kernel, dim3(blocks_for(9216, 256)), dim3(256), 0, s, b.part, n, 9216, grads + kOffW3);
+    reduce(b.part, n, 9216, grads + kOffW3, s);
     hipLaunchKernelGGL(bias_grad_kernel, dim3(n), dim3(256), 0, s, b.dz, b.part);
-    hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(32), 0, s, b.part, n, 32, grads + kOffB3);
+    reduce(b.part, n, 32, grads + kOffB3, s);
     // into conv2's output: dz2 = (a2 > 0) * conv(dz3, W3 flipped)
     hipLaunchKernelGGL(conv_fwd_kernel<32>, dim3(n, 8), dim3(256), 0, s, b.dz, b.w3t, nullptr, b.a2, b.dz2, 2);
     hipLaunchKernelGGL(conv_wgrad32_kernel, dim3(n, 9), dim3(256), 0, s, b.dz2, b.a1, b.part);
-    hipLaunchKernelGGL(reduce_kernel, dim3(blocks_for(9216, 256)), dim3(256), 0, s, b.part, n, 9216, grads + kOffW2);
+    reduce(b.part, n, 9216, grads + kOffW2, s);
     hipLaunchKernelGGL(bias_grad_kernel, dim3(n), dim3(256), 0, s, b.dz2, b.part);
-    hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(32), 0, s, b.part, n, 32, grads + kOffB2);
+    reduce(b.part, n, 32, grads + kOffB2, s);
     // into conv1's output: dz1 = (a1 > 0) * conv(dz2, W2 flipped)
     hipLaunchKernelGGL(conv_fwd_kernel<32>, dim3(n, 8), dim3(256), 0, s, b.dz2, b.w2t, nullptr, b.a1, b.dz, 2);
-    hipLaunchKernelGGL(conv_wgrad1_kernel, dim3(n), dim3(288), 0, s, b.dz, b.x, b.part);
-    hipLaunchKernelGGL(reduce_kernel, dim3(blocks_for(288, 256)), dim3(256), 0, s, b.part, n, 288, grads + kOffW1);
+    hipLaunchKernelGGL(conv_wgrad1_kernel, dim3(n, 4), dim3(288), 0, s, b.dz, b.x, b.part);
+    reduce(b.part, 4 * n, 288, grads + kOffW1, s);
     hipLaunchKernelGGL(bias_grad_kernel, dim3(n), dim3(256), 0, s, b.dz, b.part);
-    hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(32), 0, s, b.part, n, 32, grads + kOffB1);
+    reduce(b.part, n, 32, grads + kOffB1, s);
     return hipGetLastError();
 }
 
