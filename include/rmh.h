@@ -411,7 +411,9 @@ rmh_status rmh_cnn_set_params(rmh_cnn* cnn, const float* in);
 /* Device buffers: blocks n*1024 bytes, qp n floats, probs n*6 floats. */
 rmh_status rmh_cnn_forward(rmh_cnn* cnn, const uint8_t* d_blocks, const float* d_qp, uint32_t n,
                            float* d_probs, void* stream);
-/* Host buffers, synchronous (stages through device memory). */
+/* Host buffers, synchronous (stages through device memory). Runs on the
+ * network's own stream: calls on one network serialise, calls on different
+ * networks of one context may overlap on the device. */
 rmh_status rmh_cnn_predict(rmh_cnn* cnn, const uint8_t* blocks, const float* qp, uint32_t n, float* probs);
 /* Gradient of the mean softmax cross-entropy against labels (0..5, device
  * bytes) into d_grads (rmh_cnn_param_count() floats) and the per-block

@@ -1614,6 +1614,10 @@ struct rmh_cnn {
     float* d_qp = nullptr;
     float* d_probs = nullptr;
     int step = 0;
+    // rmh_cnn_predict runs on the network's own stream under its own lock,
+    // so predictions of different instances overlap on the device
+    hipStream_t stream = nullptr;
+    std::mutex mu;
 };
 
 namespace {
@@ -1675,6 +1679,13 @@ RMH_EXPORT rmh_status rmh_cnn_create(rmh_ctx* ctx, uint32_t max_batch, uint64_t 
     c->ctx = ctx;
     c->max_batch = max_batch;
     hipError_t e = hipMalloc(&c->d_mem, total);
+    if (e == hipSuccess) {
+        e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            hipFree(c->d_mem);
+            c->stream = nullptr;
+        }
+    }
     if (e != hipSuccess) {
         delete c;
         return fail(ctx, RMH_LIMIT_EXCEEDED, std::string("device allocation: ") + hipGetErrorString(e));
@@ -1727,6 +1738,10 @@ RMH_EXPORT rmh_status rmh_cnn_create(rmh_ctx* ctx, uint32_t max_batch, uint64_t 
 RMH_EXPORT void rmh_cnn_destroy(rmh_cnn* c) {
     if (!c) return;
     DeviceGuard dg(c->ctx->device);
+    if (c->stream) {
+        hipStreamSynchronize(c->stream);
+        hipStreamDestroy(c->stream);
+    }
     if (c->d_mem) hipFree(c->d_mem);
     delete c;
 }
@@ -1741,6 +1756,7 @@ RMH_EXPORT rmh_status rmh_cnn_get_params(rmh_cnn* c, float* out) {
 
 RMH_EXPORT rmh_status rmh_cnn_set_params(rmh_cnn* c, const float* in) {
     if (!c || !in) return RMH_INVALID_INPUT;
+    std::lock_guard<std::mutex> lk(c->mu);
     DeviceGuard dg(c->ctx->device);
     hipError_t e = hipDeviceSynchronize();
     if (e == hipSuccess) e = hipMemcpy(c->b.params, in, rmh::cnn::kParams * 4, hipMemcpyHostToDevice);
@@ -1771,17 +1787,26 @@ RMH_EXPORT rmh_status rmh_cnn_predict(rmh_cnn* c, const uint8_t* blocks, const f
     if (!blocks || !qp || !probs) return fail(c->ctx, RMH_INVALID_INPUT, "null host buffer");
     if (n > c->max_batch) return fail(c->ctx, RMH_LIMIT_EXCEEDED, "batch larger than max_batch");
     rmh_ctx* ctx = c->ctx;
-    std::lock_guard<std::mutex> lk(ctx->mu);
+    std::lock_guard<std::mutex> lk(c->mu);
     rmh_status g = gate(ctx);
-    if (g != RMH_OK) return fail_locked(ctx, g, g == RMH_UNAVAILABLE ? "accelerator disabled" : "injected device failure");
+    if (g != RMH_OK) return fail(ctx, g, g == RMH_UNAVAILABLE ? "accelerator disabled" : "injected device failure");
     DeviceGuard dg(ctx->device);
-    hipStream_t s = ctx->stream;
+    hipStream_t s = c->stream;
     hipError_t e = hipMemcpyAsync(c->d_blocks, blocks, (size_t)n * 1024, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipMemcpyAsync(c->d_qp, qp, (size_t)n * 4, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = rmh::cnn::forward(c->b, c->d_blocks, c->d_qp, (int)n, c->d_probs, s);
     if (e == hipSuccess) e = hipMemcpyAsync(probs, c->d_probs, (size_t)n * 24, hipMemcpyDeviceToHost, s);
-    if (e != hipSuccess) return hip_fail_locked(ctx, e, "predict");
-    return wait_stream(ctx, s);
+    if (e != hipSuccess) return fail(ctx, RMH_EXECUTION, std::string("predict: ") + hipGetErrorString(e));
+    // the context's timeout, without holding the context lock
+    const auto deadline = std::chrono::steady_clock::now() + std::chrono::milliseconds(ctx->timeout_ms);
+    for (;;) {
+        e = hipStreamQuery(s);
+        if (e == hipSuccess) return RMH_OK;
+        if (e != hipErrorNotReady) return fail(ctx, RMH_EXECUTION, std::string("device execution failed: ") + hipGetErrorString(e));
+        if (std::chrono::steady_clock::now() > deadline)
+            return fail(ctx, RMH_TIMEOUT, "device work did not finish within the timeout");
+        std::this_thread::sleep_for(std::chrono::microseconds(20));
+    }
 }
 
 namespace {

@@ -14,7 +14,7 @@ The pieces, all on librmh:
 """
 import concurrent.futures as cf
 import math
-import threading
+import queue
 import time
 
 import numpy as np
@@ -185,23 +185,53 @@ def psnr(a, b):
     return 99.0 if mse == 0 else 10 * math.log10(255.0 ** 2 / mse)
 
 
+class NetPool:
+    """Inference instances of one network for concurrent pruned jobs: a
+    predict call takes a free instance, and instances run on their own
+    streams, so predictions of different frames overlap on the device."""
+
+    def __init__(self, nets):
+        self.nets = list(nets)
+        self.free = queue.Queue()
+        for n in self.nets:
+            self.free.put(n)
+
+    def predict(self, blocks, qp):
+        n = self.free.get()
+        try:
+            return n.predict(blocks, qp)
+        finally:
+            self.free.put(n)
+
+    def params(self):
+        return self.nets[0].params()
+
+    def set_params(self, p):
+        for n in self.nets:
+            n.set_params(p)
+
+    def close(self):
+        for n in self.nets:
+            n.close()
+
+
 def encode(frames, qps, mode, net=None, p_prune=0.05, p_fallback=0.6, workers=8):
     """Encodes every frame at every QP as one pool of (frame, QP) jobs; returns
     (per-QP totals, wall seconds of the whole encode, network inference
     included). Per QP: bits, PSNR (on the padded picture), candidates per
     node, fallback rate, the splits chosen. In pruned mode each job runs the
-    network on its own frame first; one network instance serves all jobs, so
-    inference is serialised by a lock and overlaps other jobs' RD search."""
+    network on its own frame first (net: a network or a NetPool; one network
+    serves one job at a time), overlapping other jobs' RD search."""
     imgs = [pad_to_nodes(f) for f in frames]
-    lock = threading.Lock()
+    if mode == abi.RD_PRUNED and not isinstance(net, NetPool):
+        net = NetPool([net])
 
     def job(args):
         img, qp = args
         probs = None
         if mode == abi.RD_PRUNED:
             b = node_blocks(img)
-            with lock:
-                probs = net.predict(b, np.full(len(b), qp_feature(qp), np.float32))
+            probs = net.predict(b, np.full(len(b), qp_feature(qp), np.float32))
         return _rd((img, qp, mode, probs, p_prune, p_fallback))
 
     jobs = [(img, qp) for qp in qps for img in imgs]
@@ -246,22 +276,32 @@ def bd_rate(anchor, test):
     return (10 ** avg - 1) * 100
 
 
-def baselines(frames, qps=QPS, workers=8):
+def timed_encode(frames, qps, mode, net=None, p_prune=0.05, p_fallback=0.6, workers=8, repeats=1):
+    """encode() `repeats` times (the results are identical) keeping the
+    shortest wall time, so one slow run does not set the fps."""
+    out, sec = encode(frames, qps, mode, net, p_prune, p_fallback, workers)
+    for _ in range(repeats - 1):
+        sec = min(sec, encode(frames, qps, mode, net, p_prune, p_fallback, workers)[1])
+    return out, sec
+
+
+def baselines(frames, qps=QPS, workers=8, repeats=1):
     """The full-RD and heuristic encodes evaluate() compares against."""
-    return {"full": encode(frames, qps, abi.RD_FULL, workers=workers),
-            "heuristic": encode(frames, qps, abi.RD_HEURISTIC, workers=workers)}
+    return {"full": timed_encode(frames, qps, abi.RD_FULL, workers=workers, repeats=repeats),
+            "heuristic": timed_encode(frames, qps, abi.RD_HEURISTIC, workers=workers, repeats=repeats)}
 
 
-def evaluate(frames, net, qps=QPS, p_prune=0.05, p_fallback=0.6, workers=8, base=None):
+def evaluate(frames, net, qps=QPS, p_prune=0.05, p_fallback=0.6, workers=8, base=None, repeats=1):
     """Full RD, CNN-pruned and heuristic encodes of the same frames at every
     QP: fps, candidates per node, BD-rates against full RD, and how often
     each fast method picks the full search's split. base: baselines() of the
-    same frames, to reuse."""
-    base = base or baselines(frames, qps, workers)
+    same frames, to reuse; repeats: wall-time samples per encode (best kept)."""
+    base = base or baselines(frames, qps, workers, repeats)
     out, secs = {}, {}
     for k in ("full", "heuristic"):
         out[k], secs[k] = base[k]
-    out["pruned"], secs["pruned"] = encode(frames, qps, abi.RD_PRUNED, net, p_prune, p_fallback, workers)
+    out["pruned"], secs["pruned"] = timed_encode(frames, qps, abi.RD_PRUNED, net, p_prune, p_fallback, workers,
+                                                 repeats)
     pts = {k: [(r["psnr"], r["bits"]) for r in v] for k, v in out.items()}
     summ = {}
     for k, v in out.items():
@@ -282,7 +322,8 @@ TRAIN_SEEDS = (1, 3, 4, 5, 6, 7, 8, 9) + tuple(range(17, 73))
 
 def run_experiment(acc, train_seeds=TRAIN_SEEDS, val_seeds=(10, 11, 13, 14), test_seeds=(2, 12, 15, 16),
                    frames_per_seq=6, eval_frames=2, width=1280, height=720, epochs=24, lr=3e-3, batch=256,
-                   scales=(2.0, 2.5, 3.0, 3.5, 3.75, 4.0, 4.25, 4.5, 5.0), bd_target=1.5, workers=16, seed=1, log=None):
+                   scales=(2.0, 2.5, 3.0, 3.5, 3.75, 4.0, 4.25, 4.5, 5.0), bd_target=1.5, workers=16, seed=1,
+                   infer_instances=4, test_frames=4, timing_repeats=3, log=None):
     """Collects full-RD pairs on the training sequences, trains the network on
     the GPU, picks the largest logit scale whose pruned encode stays within
     bd_target % BD-rate of the full search on the validation sequences, and
@@ -301,12 +342,11 @@ def run_experiment(acc, train_seeds=TRAIN_SEEDS, val_seeds=(10, 11, 13, 14), tes
     t_train = time.perf_counter() - t0
     say("trained %d epochs in %.1f s, loss %.3f -> %.3f" % (epochs, t_train, hist[0], hist[-1]))
     val = sum([synthetic_sequence(s, eval_frames, width, height) for s in val_seeds], [])
-    test = sum([synthetic_sequence(s, eval_frames, width, height) for s in test_seeds], [])
     trained = net.params()
     # inference runs on a second instance sized for a whole frame's nodes, so
     # each pruned job predicts its frame in one call
     per_frame = len(node_blocks(pad_to_nodes(val[0])))
-    infer = Cnn(acc, max_batch=per_frame, seed=seed)
+    infer = NetPool(Cnn(acc, max_batch=per_frame, seed=seed) for _ in range(infer_instances))
     chosen, val_curve = scales[0], []
     val_base = baselines(val, workers=workers)
     for sc in scales:
@@ -319,8 +359,9 @@ def run_experiment(acc, train_seeds=TRAIN_SEEDS, val_seeds=(10, 11, 13, 14), tes
             chosen = sc
     infer.set_params(trained)
     set_logit_scale(infer, chosen)
-    test_base = baselines(test, workers=workers)
-    res = evaluate(test, infer, workers=workers, base=test_base)
+    test = sum([synthetic_sequence(s, test_frames, width, height) for s in test_seeds], [])
+    test_base = baselines(test, workers=workers, repeats=timing_repeats)
+    res = evaluate(test, infer, workers=workers, base=test_base, repeats=timing_repeats)
     # the same scale on the untrained network: what training buys
     infer.set_params(untrained)
     set_logit_scale(infer, chosen)

@@ -15,7 +15,8 @@ def test_trained_network_prunes_better_than_the_heuristic_and_the_untrained_netw
     # one small validation sequence calibrates noisily, so the bound here is
     # looser than the product's 1.5 %; what is checked is that training helps
     net, rep = L.run_experiment(acc, train_seeds=(1, 3, 4, 5), val_seeds=(10,), test_seeds=(2,), frames_per_seq=3,
-                                eval_frames=2, epochs=15, scales=(1.0, 1.5, 2.0, 3.0, 4.0), bd_target=3.0, workers=16)
+                                eval_frames=2, epochs=15, scales=(1.0, 1.5, 2.0, 3.0, 4.0), bd_target=3.0, workers=16,
+                                test_frames=2, timing_repeats=1)
     t = rep["test"]
     print(rep["validation"], t["pruned"], rep["untrained"])
     assert rep["loss_last"] < 0.92 * rep["loss_first"]
