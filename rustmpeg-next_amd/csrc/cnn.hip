@@ -29,6 +29,8 @@ namespace cnn {
 namespace {
 
 typedef short short4_t __attribute__((ext_vector_type(4)));
+
+constexpr int kRowsPerWave = 2;  // output rows per wave of conv_fwd_kernel
 typedef float float16_t __attribute__((ext_vector_type(16)));
 
 __device__ __forceinline__ uint16_t f2bf(float f) {
@@ -38,74 +40,99 @@ __device__ __forceinline__ uint16_t f2bf(float f) {
 }
 __device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
 
-// Forward convolution, one 256-lane workgroup per 4 output rows of one image;
-// wave w computes row y0 + w: 32 pixels x 32 output channels as one 32x32
-// MFMA tile, K = 9 * CIN (padded to KP) in steps of 8.
+// Forward convolution, one 256-lane workgroup per 4 * RW output rows of one
+// image; wave w computes rows y0 + RW w .. + RW - 1, each 32 pixels x 32
+// output channels as one 32x32 MFMA tile, K = 9 * CIN (padded to KP) in steps
+// of 8. The input rows (with a one-sample zero border) and the weights are
+// staged in LDS with 16-byte loads; each weight operand read feeds RW MFMAs.
 //   out = act(conv(in, w) + bias); act: 0 none, 1 relu, 2 multiply by (mask > 0)
-template <int CIN>
+template <int CIN, int RW>
 __global__ __launch_bounds__(256) void conv_fwd_kernel(const uint16_t* __restrict__ in, const uint16_t* __restrict__ w,
                                                        const float* __restrict__ bias, const uint16_t* __restrict__ mask,
                                                        uint16_t* __restrict__ out, int act) {
     constexpr int K = 9 * CIN, KP = (K + 7) & ~7;
-    constexpr int ROWS = 6, COLS = 34;
+    constexpr int OROWS = 4 * RW, ROWS = OROWS + 2, COLS = 34;
     __shared__ __align__(16) uint16_t tile[ROWS * COLS * CIN];
     __shared__ __align__(16) uint16_t wl[32 * KP];
-    const int n = blockIdx.x, y0 = blockIdx.y * 4;
+    const int n = blockIdx.x, y0 = blockIdx.y * OROWS;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint16_t* img = in + (size_t)n * 1024 * CIN;
-    // stage rows y0-1 .. y0+4, columns -1 .. 32 (zero outside the image)
-    for (int e = tid; e < ROWS * COLS * CIN; e += 256) {
-        const int c = e % CIN, col = (e / CIN) % COLS, row = e / (CIN * COLS);
-        const int y = y0 - 1 + row, x = col - 1;
-        tile[e] = (y >= 0 && y < 32 && x >= 0 && x < 32) ? img[(y * 32 + x) * CIN + c] : (uint16_t)0;
+    // stage rows y0 - 1 .. y0 + OROWS, columns -1 .. 32 (zero outside the image)
+    if constexpr (CIN % 8 == 0) {
+        constexpr int V = CIN / 8;  // 16-byte units per pixel
+        uint4* t4 = reinterpret_cast<uint4*>(tile);
+        const uint4* i4 = reinterpret_cast<const uint4*>(img);
+        for (int e = tid; e < ROWS * COLS * V; e += 256) {
+            const int v = e % V, col = (e / V) % COLS, row = e / (V * COLS);
+            const int y = y0 - 1 + row, x = col - 1;
+            t4[e] = (y >= 0 && y < 32 && x >= 0 && x < 32) ? i4[(y * 32 + x) * V + v] : make_uint4(0, 0, 0, 0);
+        }
+    } else {
+        for (int e = tid; e < ROWS * COLS * CIN; e += 256) {
+            const int c = e % CIN, col = (e / CIN) % COLS, row = e / (CIN * COLS);
+            const int y = y0 - 1 + row, x = col - 1;
+            tile[e] = (y >= 0 && y < 32 && x >= 0 && x < 32) ? img[(y * 32 + x) * CIN + c] : (uint16_t)0;
+        }
     }
-    for (int e = tid; e < 32 * KP; e += 256) wl[e] = w[e];
+    static_assert((32 * KP) % 8 == 0, "weights are staged in 16-byte units");
+    for (int e = tid; e < 32 * KP / 8; e += 256)
+        reinterpret_cast<uint4*>(wl)[e] = reinterpret_cast<const uint4*>(w)[e];
     __syncthreads();
 
     const int i = lane & 31, kq = (lane >> 5) * 4;  // A: pixel i; k offset within a step
-    const int y = wave;                              // output row within the tile (0..3)
-    float16_t acc = {};
+    float16_t acc[RW];
+#pragma unroll
+    for (int r = 0; r < RW; r++) acc[r] = float16_t{};
 #pragma unroll 4
     for (int s = 0; s < KP / 8; s++) {
         const int k0 = s * 8 + kq;
-        short4_t a, b;
-        if (CIN % 4 == 0) {
-            const int tap = k0 / CIN, c = k0 % CIN, ky = tap / 3, kx = tap % 3;
-            const uint2 va = *reinterpret_cast<const uint2*>(&tile[((y + ky) * COLS + i + kx) * CIN + c]);
-            a[0] = (short)(va.x & 0xffff);
-            a[1] = (short)(va.x >> 16);
-            a[2] = (short)(va.y & 0xffff);
-            a[3] = (short)(va.y >> 16);
-        } else {
-#pragma unroll
-            for (int q = 0; q < 4; q++) {
-                const int k = k0 + q;
-                short v = 0;
-                if (k < K) {
-                    const int tap = k / CIN, c = k % CIN, ky = tap / 3, kx = tap % 3;
-                    v = (short)tile[((y + ky) * COLS + i + kx) * CIN + c];
-                }
-                a[q] = v;
-            }
-        }
+        short4_t b;
         const uint2 vb = *reinterpret_cast<const uint2*>(&wl[i * KP + k0]);  // B: output channel j = i
         b[0] = (short)(vb.x & 0xffff);
         b[1] = (short)(vb.x >> 16);
         b[2] = (short)(vb.y & 0xffff);
         b[3] = (short)(vb.y >> 16);
-        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+#pragma unroll
+        for (int r = 0; r < RW; r++) {
+            const int y = wave * RW + r;  // output row within the workgroup
+            short4_t a;
+            if (CIN % 4 == 0) {
+                const int tap = k0 / CIN, c = k0 % CIN, ky = tap / 3, kx = tap % 3;
+                const uint2 va = *reinterpret_cast<const uint2*>(&tile[((y + ky) * COLS + i + kx) * CIN + c]);
+                a[0] = (short)(va.x & 0xffff);
+                a[1] = (short)(va.x >> 16);
+                a[2] = (short)(va.y & 0xffff);
+                a[3] = (short)(va.y >> 16);
+            } else {
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const int k = k0 + q;
+                    short v = 0;
+                    if (k < K) {
+                        const int tap = k / CIN, c = k % CIN, ky = tap / 3, kx = tap % 3;
+                        v = (short)tile[((y + ky) * COLS + i + kx) * CIN + c];
+                    }
+                    a[q] = v;
+                }
+            }
+            acc[r] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc[r], 0, 0, 0);
+        }
     }
-    // D: channel j = lane % 32; pixel = 8 * (r / 4) + 4 * (lane / 32) + r % 4
-    const int j = lane & 31, oy = y0 + y;
+    // D: channel j = lane % 32; pixel = 8 * (q / 4) + 4 * (lane / 32) + q % 4
+    const int j = lane & 31;
     const float bj = bias ? bias[j] : 0.0f;
 #pragma unroll
-    for (int r = 0; r < 16; r++) {
-        const int px = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-        const size_t o = ((size_t)n * 1024 + oy * 32 + px) * 32 + j;
-        float v = acc[r] + bj;
-        if (act == 1) v = v > 0.0f ? v : 0.0f;
-        if (act == 2 && !(bf2f(mask[o]) > 0.0f)) v = 0.0f;
-        out[o] = f2bf(v);
+    for (int r = 0; r < RW; r++) {
+        const int oy = y0 + wave * RW + r;
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+            const int px = 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3);
+            const size_t o = ((size_t)n * 1024 + oy * 32 + px) * 32 + j;
+            float v = acc[r][q] + bj;
+            if (act == 1) v = v > 0.0f ? v : 0.0f;
+            if (act == 2 && !(bf2f(mask[o]) > 0.0f)) v = 0.0f;
+            out[o] = f2bf(v);
+        }
     }
 }
 
@@ -238,103 +265,115 @@ __global__ __launch_bounds__(256) void pool_kernel(const uint16_t* __restrict__ 
 }
 
 // MLP forward (and, with labels or targets, the softmax cross-entropy
-// backward), one lane per sample. Writes probs[n][6]; in training also
-// loss[n], dpool[n][32] and the per-sample vectors the weight gradients are
-// built from: hidden hv[n][64], its gradient dhv[n][64] and the logit
-// gradient dzv[n][6].
-__global__ void mlp_kernel(const float* __restrict__ pooled, const float* __restrict__ qp, const float* __restrict__ w4,
-                           const float* __restrict__ b4, const float* __restrict__ w5, const float* __restrict__ b5,
-                           int n, float* __restrict__ probs, const uint8_t* __restrict__ labels,
-                           const float* __restrict__ targets, float inv_batch, float* __restrict__ loss,
-                           float* __restrict__ dpool, float* __restrict__ hv, float* __restrict__ dhv,
-                           float* __restrict__ dzv) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n) return;
-    float in[33], h[64], z[6];
-    for (int c = 0; c < 32; c++) in[c] = pooled[s * 32 + c];
-    in[32] = qp[s];
-    for (int o = 0; o < 64; o++) {
-        float t = b4[o];
-        for (int c = 0; c < 33; c++) t += w4[o * 33 + c] * in[c];
-        h[o] = t > 0.0f ? t : 0.0f;
+// backward), one 64-lane workgroup per sample, lane o owning hidden unit o.
+// Writes probs[n][6]; in training also loss[n], dpool[n][32] and the
+// per-sample vectors the weight gradients are built from: hidden hv[n][64],
+// its gradient dhv[n][64] and the logit gradient dzv[n][6]. Sums over hidden
+// units run in unit order (LDS), so results do not depend on the lane layout.
+__global__ __launch_bounds__(64) void mlp_kernel(const float* __restrict__ pooled, const float* __restrict__ qp,
+                                                 const float* __restrict__ w4, const float* __restrict__ b4,
+                                                 const float* __restrict__ w5, const float* __restrict__ b5, int n,
+                                                 float* __restrict__ probs, const uint8_t* __restrict__ labels,
+                                                 const float* __restrict__ targets, float inv_batch,
+                                                 float* __restrict__ loss, float* __restrict__ dpool,
+                                                 float* __restrict__ hv, float* __restrict__ dhv,
+                                                 float* __restrict__ dzv) {
+    __shared__ float in[33], h[64], dh[64], zs[6], dz[6];
+    const int s = blockIdx.x, o = threadIdx.x;
+    if (o < 32) in[o] = pooled[s * 32 + o];
+    if (o == 32) in[32] = qp[s];
+    __syncthreads();
+    float t = b4[o];
+    for (int c = 0; c < 33; c++) t += w4[o * 33 + c] * in[c];
+    h[o] = t > 0.0f ? t : 0.0f;
+    __syncthreads();
+    if (o < 6) {
+        float z = b5[o];
+        for (int c = 0; c < 64; c++) z += w5[o * 64 + c] * h[c];
+        zs[o] = z;
     }
+    __syncthreads();
     float mx = -1e30f;
-    for (int o = 0; o < 6; o++) {
-        float t = b5[o];
-        for (int c = 0; c < 64; c++) t += w5[o * 64 + c] * h[c];
-        z[o] = t;
-        mx = t > mx ? t : mx;
-    }
+    for (int k = 0; k < 6; k++) mx = zs[k] > mx ? zs[k] : mx;
     float den = 0.0f, e[6];
-    for (int o = 0; o < 6; o++) {
-        e[o] = expf(z[o] - mx);
-        den += e[o];
+    for (int k = 0; k < 6; k++) {
+        e[k] = expf(zs[k] - mx);
+        den += e[k];
     }
-    for (int o = 0; o < 6; o++) probs[s * 6 + o] = e[o] / den;
+    if (o < 6) probs[s * 6 + o] = e[o] / den;
     if (!labels && !targets) return;
     // target distribution: one-hot label (split modes 0..5) or a given distribution
-    float t[6];
+    float tg[6];
     if (targets) {
-        for (int o = 0; o < 6; o++) t[o] = targets[s * 6 + o];
+        for (int k = 0; k < 6; k++) tg[k] = targets[s * 6 + k];
     } else {
         const int lab = labels[s] < 6 ? labels[s] : 5;
-        for (int o = 0; o < 6; o++) t[o] = o == lab ? 1.0f : 0.0f;
+        for (int k = 0; k < 6; k++) tg[k] = k == lab ? 1.0f : 0.0f;
     }
-    const float lden = logf(den);
-    float l = 0.0f, tsum = 0.0f;
-    for (int o = 0; o < 6; o++) {
-        l -= t[o] * (z[o] - mx - lden);
-        tsum += t[o];
+    float tsum = 0.0f;
+    for (int k = 0; k < 6; k++) tsum += tg[k];
+    if (o == 0) {
+        const float lden = logf(den);
+        float l = 0.0f;
+        for (int k = 0; k < 6; k++) l -= tg[k] * (zs[k] - mx - lden);
+        loss[s] = l;
     }
-    loss[s] = l;
-    float dz[6];
-    for (int o = 0; o < 6; o++) {
-        dz[o] = (tsum * e[o] / den - t[o]) * inv_batch;
-        dzv[s * 6 + o] = dz[o];
+    if (o < 6) {
+        const float d = (tsum * e[o] / den - tg[o]) * inv_batch;
+        dz[o] = d;
+        dzv[s * 6 + o] = d;
     }
-    float dh[64];
-    for (int c = 0; c < 64; c++) {
-        float t = 0.0f;
-        for (int o = 0; o < 6; o++) t += w5[o * 64 + c] * dz[o];
-        dh[c] = h[c] > 0.0f ? t : 0.0f;
-        hv[s * 64 + c] = h[c];
-        dhv[s * 64 + c] = dh[c];
-    }
-    for (int c = 0; c < 32; c++) {
-        float t = 0.0f;
-        for (int o = 0; o < 64; o++) t += w4[o * 33 + c] * dh[o];
-        dpool[s * 32 + c] = t;
+    __syncthreads();
+    float g = 0.0f;
+    for (int k = 0; k < 6; k++) g += w5[k * 64 + o] * dz[k];
+    const float dho = h[o] > 0.0f ? g : 0.0f;
+    dh[o] = dho;
+    hv[s * 64 + o] = h[o];
+    dhv[s * 64 + o] = dho;
+    __syncthreads();
+    if (o < 32) {
+        float d = 0.0f;
+        for (int k = 0; k < 64; k++) d += w4[k * 33 + o] * dh[k];
+        dpool[s * 32 + o] = d;
     }
 }
 
-// Weight gradients of the MLP summed over the batch in sample order, one lane
-// per parameter (layout W4[64][33], b4[64], W5[6][64], b5[6]):
-// dW4[o][c] = sum_s dh[s][o] in[s][c] with in = [pooled; qp], db4[o] = sum_s dh[s][o],
-// dW5[k][o] = sum_s dz[s][k] h[s][o], db5[k] = sum_s dz[s][k].
+// Weight gradients of the MLP summed over the batch (layout W4[64][33], b4[64],
+// W5[6][64], b5[6]): dW4[o][c] = sum_s dh[s][o] in[s][c] with
+// in = [pooled; qp], db4[o] = sum_s dh[s][o], dW5[k][o] = sum_s dz[s][k] h[s][o],
+// db5[k] = sum_s dz[s][k]. A workgroup takes 32 parameters; its 8 lane groups
+// sum samples g, g + 8, ... and the group sums are added in order.
 __global__ __launch_bounds__(256) void mlp_wgrad_kernel(const float* __restrict__ pooled, const float* __restrict__ qp,
                                                         const float* __restrict__ hv, const float* __restrict__ dhv,
                                                         const float* __restrict__ dzv, int n, float* __restrict__ g) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= kMlpParams) return;
+    __shared__ float red[8][32];
+    const int tid = threadIdx.x, el = tid & 31, grp = tid >> 5;
+    const int p = blockIdx.x * 32 + el;
     float s = 0.0f;
     if (p < 64 * 33) {
         const int o = p / 33, c = p % 33;
 #pragma unroll 4
-        for (int k = 0; k < n; k++) s += dhv[k * 64 + o] * (c < 32 ? pooled[k * 32 + c] : qp[k]);
+        for (int k = grp; k < n; k += 8) s += dhv[k * 64 + o] * (c < 32 ? pooled[k * 32 + c] : qp[k]);
     } else if (p < 64 * 33 + 64) {
         const int o = p - 64 * 33;
 #pragma unroll 4
-        for (int k = 0; k < n; k++) s += dhv[k * 64 + o];
+        for (int k = grp; k < n; k += 8) s += dhv[k * 64 + o];
     } else if (p < 64 * 33 + 64 + 6 * 64) {
         const int q = p - (64 * 33 + 64), kk = q / 64, o = q % 64;
 #pragma unroll 4
-        for (int k = 0; k < n; k++) s += dzv[k * 6 + kk] * hv[k * 64 + o];
-    } else {
+        for (int k = grp; k < n; k += 8) s += dzv[k * 6 + kk] * hv[k * 64 + o];
+    } else if (p < kMlpParams) {
         const int kk = p - (64 * 33 + 64 + 6 * 64);
 #pragma unroll 4
-        for (int k = 0; k < n; k++) s += dzv[k * 6 + kk];
+        for (int k = grp; k < n; k += 8) s += dzv[k * 6 + kk];
     }
-    g[p] = s;
+    red[grp][el] = s;
+    __syncthreads();
+    if (grp == 0 && p < kMlpParams) {
+        float t = 0.0f;
+        for (int q = 0; q < 8; q++) t += red[q][el];
+        g[p] = t;
+    }
 }
 
 // Gradient into the last convolution's output: dz3 = (a3 > 0) * dpool / 1024.
@@ -389,9 +428,9 @@ namespace {
 void features(const Buffers& b, const uint8_t* blocks, int n, hipStream_t s) {
     const size_t px = (size_t)n * 1024;
     hipLaunchKernelGGL(input_kernel, dim3(blocks_for(px, 256)), dim3(256), 0, s, blocks, (int)px, b.x);
-    hipLaunchKernelGGL(conv_fwd_kernel<1>, dim3(n, 8), dim3(256), 0, s, b.x, b.w1, b.params + kOffB1, nullptr, b.a1, 1);
-    hipLaunchKernelGGL(conv_fwd_kernel<32>, dim3(n, 8), dim3(256), 0, s, b.a1, b.w2, b.params + kOffB2, nullptr, b.a2, 1);
-    hipLaunchKernelGGL(conv_fwd_kernel<32>, dim3(n, 8), dim3(256), 0, s, b.a2, b.w3, b.params + kOffB3, nullptr, b.a3, 1);
+    hipLaunchKernelGGL((conv_fwd_kernel<1, kRowsPerWave>), dim3(n, 32 / (4 * kRowsPerWave)), dim3(256), 0, s, b.x, b.w1, b.params + kOffB1, nullptr, b.a1, 1);
+    hipLaunchKernelGGL((conv_fwd_kernel<32, kRowsPerWave>), dim3(n, 32 / (4 * kRowsPerWave)), dim3(256), 0, s, b.a1, b.w2, b.params + kOffB2, nullptr, b.a2, 1);
+    hipLaunchKernelGGL((conv_fwd_kernel<32, kRowsPerWave>), dim3(n, 32 / (4 * kRowsPerWave)), dim3(256), 0, s, b.a2, b.w3, b.params + kOffB3, nullptr, b.a3, 1);
     hipLaunchKernelGGL(pool_kernel, dim3(n), dim3(256), 0, s, b.a3, b.pooled);
 }
 
@@ -403,7 +442,7 @@ void reduce(const float* part, int n, int len, float* out, hipStream_t s) {
 
 hipError_t forward(const Buffers& b, const uint8_t* blocks, const float* qp, int n, float* probs, hipStream_t s) {
     features(b, blocks, n, s);
-    hipLaunchKernelGGL(mlp_kernel, dim3(blocks_for(n, 64)), dim3(64), 0, s, b.pooled, qp, b.params + kOffW4,
+    hipLaunchKernelGGL(mlp_kernel, dim3(n), dim3(64), 0, s, b.pooled, qp, b.params + kOffW4,
                        b.params + kOffB4, b.params + kOffW5, b.params + kOffB5, n, probs, nullptr, nullptr, 0.0f,
                        nullptr, nullptr, nullptr, nullptr, nullptr);
     return hipGetLastError();
@@ -417,10 +456,10 @@ hipError_t gradients(const Buffers& b, const uint8_t* blocks, const float* qp, c
     float* hv = b.part;
     float* dhv = hv + (size_t)n * 64;
     float* dzv = dhv + (size_t)n * 64;
-    hipLaunchKernelGGL(mlp_kernel, dim3(blocks_for(n, 64)), dim3(64), 0, s, b.pooled, qp, b.params + kOffW4,
+    hipLaunchKernelGGL(mlp_kernel, dim3(n), dim3(64), 0, s, b.pooled, qp, b.params + kOffW4,
                        b.params + kOffB4, b.params + kOffW5, b.params + kOffB5, n, b.probs, labels, targets,
                        1.0f / (float)n, loss, b.dpool, hv, dhv, dzv);
-    hipLaunchKernelGGL(mlp_wgrad_kernel, dim3(blocks_for(kMlpParams, 256)), dim3(256), 0, s, b.pooled, qp, hv, dhv, dzv, n,
+    hipLaunchKernelGGL(mlp_wgrad_kernel, dim3((kMlpParams + 31) / 32), dim3(256), 0, s, b.pooled, qp, hv, dhv, dzv, n,
                        grads + kOffW4);
     // conv3
     hipLaunchKernelGGL(pool_back_kernel, dim3(blocks_for(act, 256)), dim3(256), 0, s, b.a3, b.dpool, (int)act, b.dz);
@@ -429,13 +468,13 @@ hipError_t gradients(const Buffers& b, const uint8_t* blocks, const float* qp, c
     hipLaunchKernelGGL(bias_grad_kernel, dim3(n), dim3(256), 0, s, b.dz, b.part);
     reduce(b.part, n, 32, grads + kOffB3, s);
     // into conv2's output: dz2 = (a2 > 0) * conv(dz3, W3 flipped)
-    hipLaunchKernelGGL(conv_fwd_kernel<32>, dim3(n, 8), dim3(256), 0, s, b.dz, b.w3t, nullptr, b.a2, b.dz2, 2);
+    hipLaunchKernelGGL((conv_fwd_kernel<32, kRowsPerWave>), dim3(n, 32 / (4 * kRowsPerWave)), dim3(256), 0, s, b.dz, b.w3t, nullptr, b.a2, b.dz2, 2);
     hipLaunchKernelGGL(conv_wgrad32_kernel, dim3(n, 9), dim3(256), 0, s, b.dz2, b.a1, b.part);
     reduce(b.part, n, 9216, grads + kOffW2, s);
     hipLaunchKernelGGL(bias_grad_kernel, dim3(n), dim3(256), 0, s, b.dz2, b.part);
     reduce(b.part, n, 32, grads + kOffB2, s);
     // into conv1's output: dz1 = (a1 > 0) * conv(dz2, W2 flipped)
-    hipLaunchKernelGGL(conv_fwd_kernel<32>, dim3(n, 8), dim3(256), 0, s, b.dz2, b.w2t, nullptr, b.a1, b.dz, 2);
+    hipLaunchKernelGGL((conv_fwd_kernel<32, kRowsPerWave>), dim3(n, 32 / (4 * kRowsPerWave)), dim3(256), 0, s, b.dz2, b.w2t, nullptr, b.a1, b.dz, 2);
     hipLaunchKernelGGL(conv_wgrad1_kernel, dim3(n, 4), dim3(288), 0, s, b.dz, b.x, b.part);
     reduce(b.part, 4 * n, 288, grads + kOffW1, s);
     hipLaunchKernelGGL(bias_grad_kernel, dim3(n), dim3(256), 0, s, b.dz, b.part);
