@@ -461,6 +461,33 @@ __device__ __forceinline__ int dot_groups(const uint2* __restrict__ src, const u
     return acc;
 }
 
+// Stage-1 outputs of one lane: columns c0, c0 + NCH, ... (nc of them) of
+// one row, clip16((sum + 64) >> 7). src (coefficient columns) and dst (the
+// intermediate row) are disjoint LDS regions, which lets reads of later
+// columns be issued before earlier results are stored.
+template <int NQ, int NCH, int CSTRIDE>
+__device__ __forceinline__ void stage1_row(const uint2* __restrict__ src, int16_t* __restrict__ dst,
+                                           const uint32_t (&m)[16], int c0, int nc) {
+    for (int j = 0; j < nc; j++) {
+        const int i = c0 + j * NCH;
+        dst[i] = (int16_t)clip16i(dot_groups<NQ>(src + i * CSTRIDE, m, 64) >> 7);
+    }
+}
+
+// Stage-2 outputs of one lane into the LDS output tile: rows r0, r0 + NR, ...
+// of one column, clip16((sum + rnd) >> shift2); src (intermediate rows) and
+// out (the output tile) are disjoint.
+template <int NQ, int H, int NR, int W, int RSTRIDE>
+__device__ __forceinline__ void stage2_col(const uint2* __restrict__ src, int16_t* __restrict__ out,
+                                           const uint32_t (&m)[16], int r0, int rnd, int shift2) {
+#pragma unroll 4
+    for (int j = 0; j < H / NR; j++) {
+        const int y = r0 + j * NR;
+        const int acc = dot_groups<NQ>(src + y * RSTRIDE, m, rnd);
+        out[y * W] = (int16_t)min(max(acc >> shift2, -32768), 32767);
+    }
+}
+
 // f(std::integral_constant<int, Q>) for the smallest Q in [Q0, QMAX] with
 // nq <= Q (QMAX when none; nq never exceeds QMAX here): one wave-uniform branch selects a fully unrolled
 // body instead of a branch per group inside it.
@@ -612,15 +639,9 @@ __device__ __forceinline__ void tile_body(const ResidualArgs& a, const TileItem&
                 if (g < G) {
                     uint32_t m[16];
                     load_col16(a.col16 + P[g].mv + y * 16, m);
-                    const uint2* src = reinterpret_cast<const uint2*>(C + g * Wc * CP);
-                    int16_t* dst = T16 + (g * H + y) * TP * 2;
                     // NCH (1, 2 or 4) divides ec4: the same trip count in every lane
-                    const int nc = ec4 / NCH;
-                    for (int j = 0; j < nc; j++) {
-                        const int i = c0 + j * NCH;
-                        const int acc = dot_groups<NQ>(src + i * (CP / 2), m, 64);
-                        dst[i] = (int16_t)clip16i(acc >> 7);
-                    }
+                    stage1_row<NQ, NCH, CP / 2>(reinterpret_cast<const uint2*>(C + g * Wc * CP),
+                                                T16 + (g * H + y) * TP * 2, m, c0, ec4 / NCH);
                 }
             }
         };
@@ -650,15 +671,15 @@ __device__ __forceinline__ void tile_body(const ResidualArgs& a, const TileItem&
                     const int rnd = 1 << (shift2 - 1);
                     const ReconDst rd{p.base, p.stride, p.bpsmax & 255, p.bpsmax >> 8};
                     const uint2* src = reinterpret_cast<const uint2*>(T + g * H * TP);
+                    if (RECON) {
 #pragma unroll 2
-                    for (int j = 0; j < H / NR; j++) {
-                        const int y = r0 + j * NR;
-                        const int acc = dot_groups<NQ>(src + y * (TP / 2), m, rnd);
-                        const int r = min(max(acc >> shift2, -32768), 32767);
-                        if (RECON)
-                            recon_add(rd, x, y, r);
-                        else
-                            O[(g * H + y) * W + x] = (int16_t)r;
+                        for (int j = 0; j < H / NR; j++) {
+                            const int y = r0 + j * NR;
+                            const int acc = dot_groups<NQ>(src + y * (TP / 2), m, rnd);
+                            recon_add(rd, x, y, min(max(acc >> shift2, -32768), 32767));
+                        }
+                    } else {
+                        stage2_col<NQ, H, NR, W, TP / 2>(src, O + g * H * W + x, m, r0, rnd, shift2);
                     }
                 }
             }
