@@ -110,7 +110,9 @@ struct McItem {
     uint32_t reserved;
 };
 
-hipError_t launch_mc(const rmh_mc_frame* frames, const McItem* items, uint32_t n_items, hipStream_t stream);
+// Items [0, n_items8) read and write 8-bit planes, the rest 16-bit ones.
+hipError_t launch_mc(const rmh_mc_frame* frames, const McItem* items, uint32_t n_items, uint32_t n_items8,
+                     hipStream_t stream);
 
 // One bi-prediction work item: the tile geometry of McItem, both MVs and the
 // BCW weight of L1 (the L0 weight is 8 minus it).
@@ -125,7 +127,8 @@ struct McBiItem {
     uint32_t reserved;
 };
 
-hipError_t launch_mc_bi(const rmh_mc_bi_frame* frames, const McBiItem* items, uint32_t n_items, hipStream_t stream);
+hipError_t launch_mc_bi(const rmh_mc_bi_frame* frames, const McBiItem* items, uint32_t n_items, uint32_t n_items8,
+                        hipStream_t stream);
 
 // Deblocking: a copy of every plane into `shadows` (3 per frame, same sizes,
 // device memory), then one tiled launch that reads the shadows and writes the

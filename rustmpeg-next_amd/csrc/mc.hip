@@ -40,16 +40,20 @@ __constant__ int8_t kChroma4[32][4] = {
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : v > hi ? hi : v; }
 
+// Sample access with the sample size BPS (1: 8-bit planes, 2: up to 12-bit)
+// fixed at compile time; a launch covers items of one sample size.
+template <int BPS>
 __device__ __forceinline__ int load_sample(const rmh_plane& p, int x, int y) {
     x = clampi(x, 0, (int)p.width - 1);
     y = clampi(y, 0, (int)p.height - 1);
     const uint32_t o = (uint32_t)y * p.stride + (uint32_t)x;
-    return p.bytes_per_sample == 1 ? (int)((const uint8_t*)p.data)[o] : (int)((const uint16_t*)p.data)[o];
+    return BPS == 1 ? (int)((const uint8_t*)p.data)[o] : (int)((const uint16_t*)p.data)[o];
 }
 
+template <int BPS>
 __device__ __forceinline__ void store_sample(const rmh_plane& p, int x, int y, int v) {
     const uint32_t o = (uint32_t)y * p.stride + (uint32_t)x;
-    if (p.bytes_per_sample == 1)
+    if (BPS == 1)
         ((uint8_t*)p.data)[o] = (uint8_t)v;
     else
         ((uint16_t*)p.data)[o] = (uint16_t)v;
@@ -94,7 +98,7 @@ struct Tile {
 // Phase 1: stage window rows iy - BEFORE .. (ROWS) and columns ix - BEFORE ..
 // (PITCH, the last one only as padding), clamped to the picture, into LDS.
 // Lane (c, r0) loads column c of rows r0, r0 + 64 / LANES_X, ...
-template <int TAPS, int ROWS, int PITCH, int LANES_X>
+template <int TAPS, int ROWS, int PITCH, int LANES_X, int BPS>
 __device__ __forceinline__ void stage(const rmh_plane& ref, const Tile<TAPS>& t, int16_t* win, int lane) {
     constexpr int BEFORE = TAPS / 2 - 1, RSTEP = 64 / LANES_X;
     const int ox = t.ix - BEFORE, oy = t.iy - BEFORE;
@@ -104,7 +108,7 @@ __device__ __forceinline__ void stage(const rmh_plane& ref, const Tile<TAPS>& t,
     if (inside) {
         const uint32_t base = (uint32_t)(oy + r0) * ref.stride + (uint32_t)(ox + c);
         const uint32_t step = RSTEP * ref.stride;
-        if (ref.bytes_per_sample == 1) {
+        if (BPS == 1) {
             const uint8_t* src = (const uint8_t*)ref.data + base;
 #pragma unroll 4
             for (int r = r0, o = 0; r < ROWS; r += RSTEP, o += step) win[r * PITCH + c] = (int16_t)src[o];
@@ -115,7 +119,7 @@ __device__ __forceinline__ void stage(const rmh_plane& ref, const Tile<TAPS>& t,
         }
     } else {
 #pragma unroll 4
-        for (int r = r0; r < ROWS; r += RSTEP) win[r * PITCH + c] = (int16_t)load_sample(ref, ox + c, oy + r);
+        for (int r = r0; r < ROWS; r += RSTEP) win[r * PITCH + c] = (int16_t)load_sample<BPS>(ref, ox + c, oy + r);
     }
 }
 
@@ -157,7 +161,7 @@ __device__ __forceinline__ void hpass(const Tile<TAPS>& t, const int16_t* win, i
 
 // Phase 3: the output samples of one plane's tile (copy, one-dimensional or
 // vertical pass), T = full tile width.
-template <int TAPS, int PITCH, int T>
+template <int TAPS, int PITCH, int T, int BPS>
 __device__ __forceinline__ void output(const rmh_plane& dst, const Tile<TAPS>& t, const int16_t* win, const int16_t* tmp,
                                        int head_room, int maxv, int lane) {
     constexpr int BEFORE = TAPS / 2 - 1;
@@ -171,8 +175,8 @@ __device__ __forceinline__ void output(const rmh_plane& dst, const Tile<TAPS>& t
             if (x >= t.tw || y >= t.th) continue;
             int s0, s1;
             hpair<TAPS>(tmp + x * PITCH + y, t.cy, s0, s1);
-            store_sample(dst, t.x0 + x, t.y0 + y, clampi((s0 + off2) >> shift2, 0, maxv));
-            if (y + 1 < t.th) store_sample(dst, t.x0 + x, t.y0 + y + 1, clampi((s1 + off2) >> shift2, 0, maxv));
+            store_sample<BPS>(dst, t.x0 + x, t.y0 + y, clampi((s0 + off2) >> shift2, 0, maxv));
+            if (y + 1 < t.th) store_sample<BPS>(dst, t.x0 + x, t.y0 + y + 1, clampi((s1 + off2) >> shift2, 0, maxv));
         }
     } else if (t.hf) {
         // horizontal only: (sum + 32) >> 6 clipped (vvc/inter.rs:939-970), output pair (x, x + 1)
@@ -181,8 +185,8 @@ __device__ __forceinline__ void output(const rmh_plane& dst, const Tile<TAPS>& t
             if (x >= t.tw || y >= t.th) continue;
             int s0, s1;
             hpair<TAPS>(win + (y + BEFORE) * PITCH + x, t.cx, s0, s1);
-            store_sample(dst, t.x0 + x, t.y0 + y, clampi((s0 + 32) >> 6, 0, maxv));
-            if (x + 1 < t.tw) store_sample(dst, t.x0 + x + 1, t.y0 + y, clampi((s1 + 32) >> 6, 0, maxv));
+            store_sample<BPS>(dst, t.x0 + x, t.y0 + y, clampi((s0 + 32) >> 6, 0, maxv));
+            if (x + 1 < t.tw) store_sample<BPS>(dst, t.x0 + x + 1, t.y0 + y, clampi((s1 + 32) >> 6, 0, maxv));
         }
     } else if (t.vf) {
         // vertical only (vvc/inter.rs:971-1002)
@@ -192,13 +196,13 @@ __device__ __forceinline__ void output(const rmh_plane& dst, const Tile<TAPS>& t
             int sum = 0;
 #pragma unroll
             for (int k = 0; k < TAPS; k++) sum += t.sy[k] * win[(y + k) * PITCH + x + BEFORE];
-            store_sample(dst, t.x0 + x, t.y0 + y, clampi((sum + 32) >> 6, 0, maxv));
+            store_sample<BPS>(dst, t.x0 + x, t.y0 + y, clampi((sum + 32) >> 6, 0, maxv));
         }
     } else {
         for (int e = lane; e < T * T; e += 64) {
             const int x = e % T, y = e / T;
             if (x >= t.tw || y >= t.th) continue;
-            store_sample(dst, t.x0 + x, t.y0 + y, win[(y + BEFORE) * PITCH + x + BEFORE]);
+            store_sample<BPS>(dst, t.x0 + x, t.y0 + y, win[(y + BEFORE) * PITCH + x + BEFORE]);
         }
     }
 }
@@ -265,6 +269,7 @@ __device__ __forceinline__ void bi_quad(const Tile<TAPS>& t, const int16_t* win,
 // three reference windows are requested before any filtering, so a wave
 // waits for memory once per item. Item fields, frame planes and filter taps
 // are wave-uniform and read with scalar loads.
+template <int BPS>
 __global__ __launch_bounds__(256) void mc_kernel(const rmh_mc_frame* __restrict__ frames,
                                                  const McItem* __restrict__ items, uint32_t n_items) {
     __shared__ __align__(16) WaveLds lds[4];
@@ -304,10 +309,10 @@ __global__ __launch_bounds__(256) void mc_kernel(const rmh_mc_frame* __restrict_
         tc.hf = cfx != 0;
         tc.vf = cfy != 0;
         bd = f->ref[0].bit_depth;
-        if (tl.tw > 0 && tl.th > 0) stage<8, kLR, kLP, 32>(f->ref[0], tl, L.luma, lane);
+        if (tl.tw > 0 && tl.th > 0) stage<8, kLR, kLP, 32, BPS>(f->ref[0], tl, L.luma, lane);
         if (tc.tw > 0 && tc.th > 0) {
-            stage<4, kCR, kCP, 16>(f->ref[1], tc, L.chroma[0], lane);
-            stage<4, kCR, kCP, 16>(f->ref[2], tc, L.chroma[1], lane);
+            stage<4, kCR, kCP, 16, BPS>(f->ref[1], tc, L.chroma[0], lane);
+            stage<4, kCR, kCP, 16, BPS>(f->ref[2], tc, L.chroma[1], lane);
         }
     }
     __syncthreads();
@@ -321,10 +326,10 @@ __global__ __launch_bounds__(256) void mc_kernel(const rmh_mc_frame* __restrict_
     __syncthreads();
     if (!active) return;
     const int maxv = (1 << bd) - 1;
-    if (tl.tw > 0 && tl.th > 0) output<8, kLP, 16>(f->dst[0], tl, L.luma, L.ltmp, head_room, maxv, lane);
+    if (tl.tw > 0 && tl.th > 0) output<8, kLP, 16, BPS>(f->dst[0], tl, L.luma, L.ltmp, head_room, maxv, lane);
     if (tc.tw > 0 && tc.th > 0) {
-        output<4, kCP, 8>(f->dst[1], tc, L.chroma[0], L.ctmp[0], head_room, maxv, lane);
-        output<4, kCP, 8>(f->dst[2], tc, L.chroma[1], L.ctmp[1], head_room, maxv, lane);
+        output<4, kCP, 8, BPS>(f->dst[1], tc, L.chroma[0], L.ctmp[0], head_room, maxv, lane);
+        output<4, kCP, 8, BPS>(f->dst[2], tc, L.chroma[1], L.ctmp[1], head_room, maxv, lane);
     }
 }
 
@@ -356,7 +361,7 @@ __device__ __forceinline__ void bi_tiles(const McBiItem& it, int mv_x, int mv_y,
 // Weighted average of the two lists' intermediates over one plane's tile
 // (weighted_average_bi_16, vvc/inter.rs:2676-2696), T = full tile width; a
 // lane takes 2 x 2 output samples.
-template <int TAPS, int PITCH, int T>
+template <int TAPS, int PITCH, int T, int BPS>
 __device__ __forceinline__ void bi_output(const rmh_plane& dst, const Tile<TAPS>& t0, const Tile<TAPS>& t1,
                                           const int16_t* win0, const int16_t* tmp0, const int16_t* win1,
                                           const int16_t* tmp1, int head_room, int w0, int w1, int maxv, int lane) {
@@ -372,7 +377,7 @@ __device__ __forceinline__ void bi_output(const rmh_plane& dst, const Tile<TAPS>
         for (int q = 0; q < 4; q++) {
             const int xq = x + (q & 1), yq = y + (q >> 1);
             if (xq < t0.tw && yq < t0.th)
-                store_sample(dst, t0.x0 + xq, t0.y0 + yq, clampi((p0[q] * w0 + p1[q] * w1 + off) >> shift, 0, maxv));
+                store_sample<BPS>(dst, t0.x0 + xq, t0.y0 + yq, clampi((p0[q] * w0 + p1[q] * w1 + off) >> shift, 0, maxv));
         }
     }
 }
@@ -380,6 +385,7 @@ __device__ __forceinline__ void bi_output(const rmh_plane& dst, const Tile<TAPS>
 // Bi-prediction: as mc_kernel, with the windows of both lists staged before
 // any filtering (two LDS slices per wave) and the weighted average fused into
 // the output pass.
+template <int BPS>
 __global__ __launch_bounds__(256) void mc_bi_kernel(const rmh_mc_bi_frame* __restrict__ frames,
                                                     const McBiItem* __restrict__ items, uint32_t n_items) {
     __shared__ __align__(16) WaveLds lds[4][2];
@@ -401,14 +407,14 @@ __global__ __launch_bounds__(256) void mc_bi_kernel(const rmh_mc_bi_frame* __res
         bi_tiles(it, it.mv1_x, it.mv1_y, tl1, tc1);
         bd = f->ref0[0].bit_depth;
         if (tl0.tw > 0 && tl0.th > 0) {
-            stage<8, kLR, kLP, 32>(f->ref0[0], tl0, L0.luma, lane);
-            stage<8, kLR, kLP, 32>(f->ref1[0], tl1, L1.luma, lane);
+            stage<8, kLR, kLP, 32, BPS>(f->ref0[0], tl0, L0.luma, lane);
+            stage<8, kLR, kLP, 32, BPS>(f->ref1[0], tl1, L1.luma, lane);
         }
         if (tc0.tw > 0 && tc0.th > 0) {
-            stage<4, kCR, kCP, 16>(f->ref0[1], tc0, L0.chroma[0], lane);
-            stage<4, kCR, kCP, 16>(f->ref0[2], tc0, L0.chroma[1], lane);
-            stage<4, kCR, kCP, 16>(f->ref1[1], tc1, L1.chroma[0], lane);
-            stage<4, kCR, kCP, 16>(f->ref1[2], tc1, L1.chroma[1], lane);
+            stage<4, kCR, kCP, 16, BPS>(f->ref0[1], tc0, L0.chroma[0], lane);
+            stage<4, kCR, kCP, 16, BPS>(f->ref0[2], tc0, L0.chroma[1], lane);
+            stage<4, kCR, kCP, 16, BPS>(f->ref1[1], tc1, L1.chroma[0], lane);
+            stage<4, kCR, kCP, 16, BPS>(f->ref1[2], tc1, L1.chroma[1], lane);
         }
     }
     __syncthreads();
@@ -428,25 +434,33 @@ __global__ __launch_bounds__(256) void mc_bi_kernel(const rmh_mc_bi_frame* __res
     if (!active) return;
     const int maxv = (1 << bd) - 1, w0 = 8 - w1;
     if (tl0.tw > 0 && tl0.th > 0)
-        bi_output<8, kLP, 16>(f->dst[0], tl0, tl1, L0.luma, L0.ltmp, L1.luma, L1.ltmp, head_room, w0, w1, maxv, lane);
+        bi_output<8, kLP, 16, BPS>(f->dst[0], tl0, tl1, L0.luma, L0.ltmp, L1.luma, L1.ltmp, head_room, w0, w1, maxv, lane);
     if (tc0.tw > 0 && tc0.th > 0) {
         for (int c = 0; c < 2; c++)
-            bi_output<4, kCP, 8>(f->dst[1 + c], tc0, tc1, L0.chroma[c], L0.ctmp[c], L1.chroma[c], L1.ctmp[c], head_room,
+            bi_output<4, kCP, 8, BPS>(f->dst[1 + c], tc0, tc1, L0.chroma[c], L0.ctmp[c], L1.chroma[c], L1.ctmp[c], head_room,
                                  w0, w1, maxv, lane);
     }
 }
 
 }  // namespace
 
-hipError_t launch_mc_bi(const rmh_mc_bi_frame* frames, const McBiItem* items, uint32_t n_items, hipStream_t stream) {
-    if (n_items == 0) return hipSuccess;
-    hipLaunchKernelGGL(mc_bi_kernel, dim3((n_items + 3) / 4), dim3(256), 0, stream, frames, items, n_items);
+hipError_t launch_mc_bi(const rmh_mc_bi_frame* frames, const McBiItem* items, uint32_t n_items, uint32_t n_items8,
+                        hipStream_t stream) {
+    if (n_items8 > 0)
+        hipLaunchKernelGGL(mc_bi_kernel<1>, dim3((n_items8 + 3) / 4), dim3(256), 0, stream, frames, items, n_items8);
+    if (n_items > n_items8)
+        hipLaunchKernelGGL(mc_bi_kernel<2>, dim3((n_items - n_items8 + 3) / 4), dim3(256), 0, stream, frames,
+                           items + n_items8, n_items - n_items8);
     return hipGetLastError();
 }
 
-hipError_t launch_mc(const rmh_mc_frame* frames, const McItem* items, uint32_t n_items, hipStream_t stream) {
-    if (n_items == 0) return hipSuccess;
-    hipLaunchKernelGGL(mc_kernel, dim3((n_items + 3) / 4), dim3(256), 0, stream, frames, items, n_items);
+hipError_t launch_mc(const rmh_mc_frame* frames, const McItem* items, uint32_t n_items, uint32_t n_items8,
+                     hipStream_t stream) {
+    if (n_items8 > 0)
+        hipLaunchKernelGGL(mc_kernel<1>, dim3((n_items8 + 3) / 4), dim3(256), 0, stream, frames, items, n_items8);
+    if (n_items > n_items8)
+        hipLaunchKernelGGL(mc_kernel<2>, dim3((n_items - n_items8 + 3) / 4), dim3(256), 0, stream, frames,
+                           items + n_items8, n_items - n_items8);
     return hipGetLastError();
 }
 

@@ -9,12 +9,12 @@
 // (crates/rustmpeg-vulkan/src/accelerator.rs:38, 163-171).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
-#include <algorithm>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -115,7 +115,7 @@ struct rmh_plan {
     const rmh::McItem* d_items = nullptr;
     const rmh_mc_bi_frame* d_bi_frames = nullptr;  // bi-prediction plans
     const rmh::McBiItem* d_bi_items = nullptr;
-    size_t n_items = 0;
+    size_t n_items = 0, n_items8 = 0;  // items [0, n_items8) are on 8-bit frames
     // deblock
     const rmh_dbk_frame* d_dbk = nullptr;
     const rmh_plane* d_shadows = nullptr;  // 3 per frame: copies the tiled kernel reads
@@ -599,6 +599,15 @@ uint64_t mc_bytes(const rmh_mc_frame* frames, const rmh_pu_desc* pus, size_t n_p
     return b;
 }
 
+// Orders work items so that those on 8-bit frames come first (one kernel
+// instantiation per sample size); returns how many there are.
+template <typename Item, typename Frame>
+size_t order_by_sample_size(std::vector<Item>& items, const Frame* frames) {
+    auto it = std::stable_partition(items.begin(), items.end(),
+                                    [&](const Item& x) { return frames[x.frame].dst[0].bytes_per_sample == 1; });
+    return (size_t)(it - items.begin());
+}
+
 // Algorithmic bytes of a bi plan: two reference reads and one write per
 // sample, plus the 28-byte descriptor.
 uint64_t mc_bi_bytes(const rmh_mc_bi_frame* frames, const rmh_bi_pu_desc* pus, size_t n_pu) {
@@ -803,8 +812,10 @@ rmh_status run_plan_locked_free(rmh_plan* plan, const int16_t* d_coeff, int16_t*
             e = rmh::launch_residual(a, s);
         }
     } else if (plan->kind == kMc) {
-        e = plan->d_bi_items ? rmh::launch_mc_bi(plan->d_bi_frames, plan->d_bi_items, (uint32_t)plan->n_items, s)
-                             : rmh::launch_mc(plan->d_frames, plan->d_items, (uint32_t)plan->n_items, s);
+        e = plan->d_bi_items ? rmh::launch_mc_bi(plan->d_bi_frames, plan->d_bi_items, (uint32_t)plan->n_items,
+                                                 (uint32_t)plan->n_items8, s)
+                             : rmh::launch_mc(plan->d_frames, plan->d_items, (uint32_t)plan->n_items,
+                                              (uint32_t)plan->n_items8, s);
     } else {
         e = rmh::launch_deblock(plan->d_dbk, plan->d_shadows, plan->d_tile_start, (uint32_t)plan->n_dbk, plan->total_tiles,
                                 plan->max_rows, s);
@@ -1182,11 +1193,13 @@ RMH_EXPORT rmh_status rmh_mc_plan_create(rmh_ctx* ctx, const rmh_mc_frame* frame
     DeviceGuard dg(ctx->device);
     std::vector<rmh::McItem> items = mc_items(frames, pus, n_pu);
     if (items.size() > 0x7fffffffu) return fail(ctx, RMH_LIMIT_EXCEEDED, "too many inter work items");
+    const size_t n8 = order_by_sample_size(items, frames);
     size_t fb = n_frames * sizeof(rmh_mc_frame), pb = n_pu * sizeof(rmh_pu_desc), ib = items.size() * sizeof(rmh::McItem);
     rmh_plan* p = new rmh_plan();
     p->ctx = ctx;
     p->kind = kMc;
     p->n_items = items.size();
+    p->n_items8 = n8;
     p->bytes = mc_bytes(frames, pus, n_pu);
     hipError_t e = hipMalloc(&p->d_mem, aligned(fb) + aligned(pb) + aligned(ib) + 256);
     char* base = (char*)p->d_mem;
@@ -1222,6 +1235,7 @@ RMH_EXPORT rmh_status rmh_mc_batch(rmh_ctx* ctx, const rmh_mc_frame* frames, siz
     if (g != RMH_OK) return fail_locked(ctx, g, g == RMH_UNAVAILABLE ? "accelerator disabled" : "injected device failure");
     DeviceGuard dg(ctx->device);
     std::vector<rmh::McItem> items = mc_items(frames, pus, n_pu);
+    const size_t n8 = order_by_sample_size(items, frames);
     size_t fb = n_frames * sizeof(rmh_mc_frame), pb = n_pu * sizeof(rmh_pu_desc), ib = items.size() * sizeof(rmh::McItem);
     size_t need = aligned(fb) + aligned(pb) + aligned(ib), hneed = 0;
     for (size_t f = 0; f < n_frames; f++)
@@ -1256,7 +1270,7 @@ RMH_EXPORT rmh_status rmh_mc_batch(rmh_ctx* ctx, const rmh_mc_frame* frames, siz
         (e = hipMemcpyAsync(d_pus, pus, pb, hipMemcpyHostToDevice, s)) != hipSuccess ||
         (e = hipMemcpyAsync(d_items, items.data(), ib, hipMemcpyHostToDevice, s)) != hipSuccess)
         return hip_fail_locked(ctx, e, "upload");
-    e = rmh::launch_mc(d_frames, d_items, (uint32_t)items.size(), s);
+    e = rmh::launch_mc(d_frames, d_items, (uint32_t)items.size(), (uint32_t)n8, s);
     if (e != hipSuccess) return hip_fail_locked(ctx, e, "kernel launch");
     std::vector<void*> hdst;
     for (size_t f = 0; f < n_frames; f++)
@@ -1305,11 +1319,13 @@ RMH_EXPORT rmh_status rmh_mc_bi_plan_create(rmh_ctx* ctx, const rmh_mc_bi_frame*
     DeviceGuard dg(ctx->device);
     std::vector<rmh::McBiItem> items = mc_bi_items(frames, pus, n_pu);
     if (items.size() > 0x7fffffffu) return fail(ctx, RMH_LIMIT_EXCEEDED, "too many inter work items");
+    const size_t n8 = order_by_sample_size(items, frames);
     size_t fb = n_frames * sizeof(rmh_mc_bi_frame), ib = items.size() * sizeof(rmh::McBiItem);
     rmh_plan* p = new rmh_plan();
     p->ctx = ctx;
     p->kind = kMc;
     p->n_items = items.size();
+    p->n_items8 = n8;
     p->bytes = mc_bi_bytes(frames, pus, n_pu);
     hipError_t e = hipMalloc(&p->d_mem, aligned(fb) + aligned(ib) + 256);
     char* base = (char*)p->d_mem;
@@ -1337,6 +1353,7 @@ RMH_EXPORT rmh_status rmh_mc_bi_batch(rmh_ctx* ctx, const rmh_mc_bi_frame* frame
     if (g != RMH_OK) return fail_locked(ctx, g, g == RMH_UNAVAILABLE ? "accelerator disabled" : "injected device failure");
     DeviceGuard dg(ctx->device);
     std::vector<rmh::McBiItem> items = mc_bi_items(frames, pus, n_pu);
+    const size_t n8 = order_by_sample_size(items, frames);
     size_t fb = n_frames * sizeof(rmh_mc_bi_frame), ib = items.size() * sizeof(rmh::McBiItem);
     size_t need = aligned(fb) + aligned(ib), hneed = 0;
     for (size_t f = 0; f < n_frames; f++)
@@ -1371,7 +1388,7 @@ RMH_EXPORT rmh_status rmh_mc_bi_batch(rmh_ctx* ctx, const rmh_mc_bi_frame* frame
     if ((e = hipMemcpyAsync(d_frames, dev_frames.data(), fb, hipMemcpyHostToDevice, s)) != hipSuccess ||
         (e = hipMemcpyAsync(d_items, items.data(), ib, hipMemcpyHostToDevice, s)) != hipSuccess)
         return hip_fail_locked(ctx, e, "upload");
-    e = rmh::launch_mc_bi(d_frames, d_items, (uint32_t)items.size(), s);
+    e = rmh::launch_mc_bi(d_frames, d_items, (uint32_t)items.size(), (uint32_t)n8, s);
     if (e != hipSuccess) return hip_fail_locked(ctx, e, "kernel launch");
     std::vector<void*> hdst;
     for (size_t f = 0; f < n_frames; f++)
