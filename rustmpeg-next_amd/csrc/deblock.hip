@@ -322,13 +322,17 @@ constexpr int kLWW = kTW + 16, kLWH = kTH + 16;        // luma window (8 halo ea
 constexpr int kCTW = kTW / 2, kCTH = kTH / 2;          // chroma tile
 constexpr int kCWW = kCTW + 8, kCWH = kCTH + 8;        // chroma window (4 halo each side)
 
+// Sample access with the sample size BPS (1: 8-bit, 2: up to 12-bit) fixed
+// at compile time; a launch covers the tiles of frames of one sample size.
+template <int BPS>
 __device__ __forceinline__ int ld32(const rmh_plane& p, int x, int y) {
     const uint32_t o = (uint32_t)y * p.stride + (uint32_t)x;
-    return p.bytes_per_sample == 1 ? (int)((const uint8_t*)p.data)[o] : (int)((const uint16_t*)p.data)[o];
+    return BPS == 1 ? (int)((const uint8_t*)p.data)[o] : (int)((const uint16_t*)p.data)[o];
 }
+template <int BPS>
 __device__ __forceinline__ void st32(const rmh_plane& p, int x, int y, int v) {
     const uint32_t o = (uint32_t)y * p.stride + (uint32_t)x;
-    if (p.bytes_per_sample == 1)
+    if (BPS == 1)
         ((uint8_t*)p.data)[o] = (uint8_t)v;
     else
         ((uint16_t*)p.data)[o] = (uint16_t)v;
@@ -336,7 +340,7 @@ __device__ __forceinline__ void st32(const rmh_plane& p, int x, int y, int v) {
 
 // Window (wx0, wy0) .. + (WW, WH) of plane P read (clamped) from the shadow.
 // All of a lane's loads are issued before the first LDS write.
-template <int WW, int WH>
+template <int WW, int WH, int BPS>
 __device__ __forceinline__ void load_window(const rmh_plane& sh, int wx0, int wy0, int16_t* win) {
     constexpr int N = (WW * WH + 255) / 256;
     const int w = sh.width, h = sh.height;
@@ -345,7 +349,7 @@ __device__ __forceinline__ void load_window(const rmh_plane& sh, int wx0, int wy
     for (int i = 0; i < N; i++) {
         const int e = threadIdx.x + 256 * i;
         const int r = e / WW, c = e - r * WW;
-        v[i] = e < WW * WH ? ld32(sh, clampi(wx0 + c, 0, w - 1), clampi(wy0 + r, 0, h - 1)) : 0;
+        v[i] = e < WW * WH ? ld32<BPS>(sh, clampi(wx0 + c, 0, w - 1), clampi(wy0 + r, 0, h - 1)) : 0;
     }
 #pragma unroll
     for (int i = 0; i < N; i++) {
@@ -571,25 +575,32 @@ __device__ __forceinline__ void tile_pass(const TileCtx& t, int dir, int16_t* lw
     __syncthreads();
 }
 
-// One workgroup per tile; the grid covers exactly tile_start[n_frames] tiles,
-// numbered frame by frame in raster order and handed to the XCDs in contiguous
-// bands so that tiles sharing halo rows and columns share an L2.
+// One workgroup per tile. Tiles are numbered frame by frame (in the order
+// frame_of[] gives, 8-bit frames first) in raster order; tile_start[k] is the
+// first tile of the k-th frame of that order. A launch covers tiles
+// first_tile .. + gridDim.x, all on frames of sample size BPS, handed to the
+// XCDs in contiguous bands so that tiles sharing halo rows and columns share
+// an L2.
+template <int BPS>
 __global__ __launch_bounds__(256) void deblock_tile_kernel(const rmh_dbk_frame* __restrict__ frames,
                                                            const rmh_plane* __restrict__ shadows,
-                                                           const uint32_t* __restrict__ tile_start, uint32_t n_frames) {
+                                                           const uint32_t* __restrict__ tile_start,
+                                                           const uint32_t* __restrict__ frame_of, uint32_t n_frames,
+                                                           uint32_t first_tile) {
     __shared__ __align__(16) int16_t lw[kLWH * kLWW];
     __shared__ int16_t cwin[2][kCWH * kCWW];
     __shared__ uint32_t ql[kMaxQ], qc[kMaxQ];
     __shared__ uint32_t cells[kEY * kEX];
     __shared__ int counts[2];
-    const uint32_t tile = xcd_local_block(blockIdx.x, gridDim.x);
+    const uint32_t tile = first_tile + xcd_local_block(blockIdx.x, gridDim.x);
     uint32_t lo = 0, hi = n_frames;  // tile_start[lo] <= tile < tile_start[hi]
     while (hi - lo > 1) {
         const uint32_t mid = (lo + hi) >> 1;
         if (tile_start[mid] <= tile) lo = mid; else hi = mid;
     }
+    const uint32_t fi = frame_of[lo];
     TileCtx t;
-    t.f = &frames[lo];
+    t.f = &frames[fi];
     t.cells = cells;
     const rmh_plane& PY = t.f->planes[0];
     t.W = PY.width;
@@ -602,7 +613,7 @@ __global__ __launch_bounds__(256) void deblock_tile_kernel(const rmh_dbk_frame* 
     t.CW = t.f->planes[1].width;
     t.CH = t.f->planes[1].height;
     t.bd = PY.bit_depth;
-    const rmh_plane* sh = shadows + lo * 3;
+    const rmh_plane* sh = shadows + fi * 3;
 
     // 1. edge cells and windows
     if (threadIdx.x < kEY * kEX) {
@@ -618,9 +629,9 @@ __global__ __launch_bounds__(256) void deblock_tile_kernel(const rmh_dbk_frame* 
         }
         cells[threadIdx.x] = v;
     }
-    load_window<kLWW, kLWH>(sh[0], t.X0 - 8, t.Y0 - 8, lw);
-    load_window<kCWW, kCWH>(sh[1], t.CX0 - 4, t.CY0 - 4, cwin[0]);
-    load_window<kCWW, kCWH>(sh[2], t.CX0 - 4, t.CY0 - 4, cwin[1]);
+    load_window<kLWW, kLWH, BPS>(sh[0], t.X0 - 8, t.Y0 - 8, lw);
+    load_window<kCWW, kCWH, BPS>(sh[1], t.CX0 - 4, t.CY0 - 4, cwin[0]);
+    load_window<kCWW, kCWH, BPS>(sh[2], t.CX0 - 4, t.CY0 - 4, cwin[1]);
     __syncthreads();
     // 2. vertical edges
     tile_pass(t, 0, lw, cwin, ql, qc, counts);
@@ -636,12 +647,12 @@ __global__ __launch_bounds__(256) void deblock_tile_kernel(const rmh_dbk_frame* 
     // 5. write the tile back (picture samples only)
     for (int e = threadIdx.x; e < kTW * kTH; e += 256) {
         const int r = e / kTW, c = e % kTW, x = t.X0 + c, y = t.Y0 + r;
-        if (x < t.W && y < t.H) st32(PY, x, y, lw[(r + 8) * kLWW + c + 8]);
+        if (x < t.W && y < t.H) st32<BPS>(PY, x, y, lw[(r + 8) * kLWW + c + 8]);
     }
     for (int e = threadIdx.x; e < 2 * kCTW * kCTH; e += 256) {
         const int pl = e / (kCTW * kCTH), q = e % (kCTW * kCTH), r = q / kCTW, c = q % kCTW;
         const int x = t.CX0 + c, y = t.CY0 + r;
-        if (x < t.CW && y < t.CH) st32(t.f->planes[1 + pl], x, y, cwin[pl][(r + 4) * kCWW + c + 4]);
+        if (x < t.CW && y < t.CH) st32<BPS>(t.f->planes[1 + pl], x, y, cwin[pl][(r + 4) * kCWW + c + 4]);
     }
 }
 
@@ -652,12 +663,18 @@ uint32_t deblock_tiles(uint32_t width, uint32_t height) {
 }
 
 hipError_t launch_deblock(const rmh_dbk_frame* frames, const rmh_plane* shadows, const uint32_t* tile_start,
-                          uint32_t n_frames, uint32_t total_tiles, uint32_t max_rows, hipStream_t stream) {
+                          const uint32_t* frame_of, uint32_t n_frames, uint32_t total_tiles, uint32_t tiles8,
+                          uint32_t max_rows, hipStream_t stream) {
     if (n_frames == 0 || total_tiles == 0) return hipSuccess;
     hipLaunchKernelGGL(shadow_copy_kernel, dim3((max_rows + 3) / 4, n_frames, 3), dim3(256), 0, stream, frames, shadows);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(deblock_tile_kernel, dim3(total_tiles), dim3(256), 0, stream, frames, shadows, tile_start, n_frames);
+    if (tiles8 > 0)
+        hipLaunchKernelGGL(deblock_tile_kernel<1>, dim3(tiles8), dim3(256), 0, stream, frames, shadows, tile_start,
+                           frame_of, n_frames, 0u);
+    if (total_tiles > tiles8)
+        hipLaunchKernelGGL(deblock_tile_kernel<2>, dim3(total_tiles - tiles8), dim3(256), 0, stream, frames, shadows,
+                           tile_start, frame_of, n_frames, tiles8);
     return hipGetLastError();
 }
 

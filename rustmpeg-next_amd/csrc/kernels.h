@@ -132,11 +132,14 @@ hipError_t launch_mc_bi(const rmh_mc_bi_frame* frames, const McBiItem* items, ui
 
 // Deblocking: a copy of every plane into `shadows` (3 per frame, same sizes,
 // device memory), then one tiled launch that reads the shadows and writes the
-// pictures. tile_start[n_frames + 1] (device memory) holds the running sum
-// of deblock_tiles() over the frames (tile_start[0] = 0, tile_start[n_frames]
-// = total_tiles); max_rows = largest luma height.
+// pictures. The frames are taken in the order frame_of[n_frames] (device
+// memory), 8-bit frames first; tile_start[n_frames + 1] holds the running
+// sum of deblock_tiles() in that order (tile_start[0] = 0, tile_start[n_frames]
+// = total_tiles) and tiles8 is the number of tiles on 8-bit frames;
+// max_rows = largest luma height.
 uint32_t deblock_tiles(uint32_t width, uint32_t height);
 hipError_t launch_deblock(const rmh_dbk_frame* frames, const rmh_plane* shadows, const uint32_t* tile_start,
-                          uint32_t n_frames, uint32_t total_tiles, uint32_t max_rows, hipStream_t stream);
+                          const uint32_t* frame_of, uint32_t n_frames, uint32_t total_tiles, uint32_t tiles8,
+                          uint32_t max_rows, hipStream_t stream);
 
 }  // namespace rmh

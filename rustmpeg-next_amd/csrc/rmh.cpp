@@ -119,9 +119,10 @@ struct rmh_plan {
     // deblock
     const rmh_dbk_frame* d_dbk = nullptr;
     const rmh_plane* d_shadows = nullptr;  // 3 per frame: copies the tiled kernel reads
-    const uint32_t* d_tile_start = nullptr;  // [n_dbk + 1] running sum of tiles per frame
+    const uint32_t* d_tile_start = nullptr;  // [n_dbk + 1] running sum of tiles, 8-bit frames first
+    const uint32_t* d_frame_of = nullptr;    // [n_dbk] frame at each position of that order
     size_t n_dbk = 0;
-    uint32_t total_tiles = 0, max_rows = 0;
+    uint32_t total_tiles = 0, tiles8 = 0, max_rows = 0;
 };
 
 struct rmh_graph {
@@ -644,11 +645,27 @@ std::vector<rmh_plane> shadow_layout(const rmh_dbk_frame* frames, size_t n_frame
     return out;
 }
 
-// Running sum of the tiles of the tiled deblock launch, frame by frame.
-std::vector<uint32_t> tile_starts(const rmh_dbk_frame* frames, size_t n_frames) {
-    std::vector<uint32_t> t(n_frames + 1, 0);
-    for (size_t f = 0; f < n_frames; f++)
-        t[f + 1] = t[f] + rmh::deblock_tiles(frames[f].planes[0].width, frames[f].planes[0].height);
+// Tile numbering of the tiled deblock launch: the frames in an order with
+// the 8-bit ones first (stable); words = the running sum of tiles in that
+// order (n + 1 entries) followed by the frame at each position (n entries);
+// tiles8 = the tiles on 8-bit frames.
+struct TileTable {
+    std::vector<uint32_t> words;
+    uint32_t total = 0, tiles8 = 0;
+};
+TileTable tile_table(const rmh_dbk_frame* frames, size_t n_frames) {
+    std::vector<uint32_t> order(n_frames);
+    for (size_t f = 0; f < n_frames; f++) order[f] = (uint32_t)f;
+    std::stable_partition(order.begin(), order.end(), [&](uint32_t f) { return frames[f].planes[0].bytes_per_sample == 1; });
+    TileTable t;
+    t.words.assign(2 * n_frames + 1, 0);
+    for (size_t k = 0; k < n_frames; k++) {
+        const rmh_plane& y = frames[order[k]].planes[0];
+        t.words[k + 1] = t.words[k] + rmh::deblock_tiles(y.width, y.height);
+        t.words[n_frames + 1 + k] = order[k];
+        if (y.bytes_per_sample == 1) t.tiles8 = t.words[k + 1];
+    }
+    t.total = t.words[n_frames];
     return t;
 }
 
@@ -817,7 +834,8 @@ rmh_status run_plan_locked_free(rmh_plan* plan, const int16_t* d_coeff, int16_t*
                              : rmh::launch_mc(plan->d_frames, plan->d_items, (uint32_t)plan->n_items,
                                               (uint32_t)plan->n_items8, s);
     } else {
-        e = rmh::launch_deblock(plan->d_dbk, plan->d_shadows, plan->d_tile_start, (uint32_t)plan->n_dbk, plan->total_tiles,
+        e = rmh::launch_deblock(plan->d_dbk, plan->d_shadows, plan->d_tile_start, plan->d_frame_of, (uint32_t)plan->n_dbk,
+                                plan->total_tiles, plan->tiles8,
                                 plan->max_rows, s);
     }
     if (e != hipSuccess) {
@@ -1438,8 +1456,9 @@ RMH_EXPORT rmh_status rmh_deblock_plan_create(rmh_ctx* ctx, const rmh_dbk_frame*
     p->ctx = ctx;
     p->kind = kDeblock;
     p->n_dbk = n_frames;
-    const std::vector<uint32_t> starts = tile_starts(frames, n_frames);
-    p->total_tiles = starts.back();
+    const TileTable tt = tile_table(frames, n_frames);
+    p->total_tiles = tt.total;
+    p->tiles8 = tt.tiles8;
     for (size_t f = 0; f < n_frames; f++) {
         uint32_t cells = frames[f].grid_w * frames[f].grid_h;
         p->max_rows = std::max(p->max_rows, frames[f].planes[0].height);
@@ -1447,14 +1466,14 @@ RMH_EXPORT rmh_status rmh_deblock_plan_create(rmh_ctx* ctx, const rmh_dbk_frame*
         p->bytes += 4ull * cells;
     }
     size_t fb = n_frames * sizeof(rmh_dbk_frame), sb = n_frames * 3 * sizeof(rmh_plane);
-    size_t tb = starts.size() * sizeof(uint32_t), shb = shadow_bytes(frames, n_frames);
+    size_t tb = tt.words.size() * sizeof(uint32_t), shb = shadow_bytes(frames, n_frames);
     hipError_t e = hipMalloc(&p->d_mem, aligned(fb) + aligned(sb) + aligned(tb) + shb + 256);
     char* base = (char*)p->d_mem;
     if (e == hipSuccess) {
         std::vector<rmh_plane> sh = shadow_layout(frames, n_frames, base + aligned(fb) + aligned(sb) + aligned(tb));
         if (fb) e = hipMemcpy(base, frames, fb, hipMemcpyHostToDevice);
         if (e == hipSuccess && sb) e = hipMemcpy(base + aligned(fb), sh.data(), sb, hipMemcpyHostToDevice);
-        if (e == hipSuccess) e = hipMemcpy(base + aligned(fb) + aligned(sb), starts.data(), tb, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(base + aligned(fb) + aligned(sb), tt.words.data(), tb, hipMemcpyHostToDevice);
     }
     if (e != hipSuccess) {
         if (p->d_mem) hipFree(p->d_mem);
@@ -1464,6 +1483,7 @@ RMH_EXPORT rmh_status rmh_deblock_plan_create(rmh_ctx* ctx, const rmh_dbk_frame*
     p->d_dbk = (const rmh_dbk_frame*)base;
     p->d_shadows = (const rmh_plane*)(base + aligned(fb));
     p->d_tile_start = (const uint32_t*)(base + aligned(fb) + aligned(sb));
+    p->d_frame_of = p->d_tile_start + n_frames + 1;
     *out = p;
     return RMH_OK;
 }
@@ -1484,8 +1504,8 @@ RMH_EXPORT rmh_status rmh_deblock(rmh_ctx* ctx, const rmh_dbk_frame* frames, siz
     if (g != RMH_OK) return fail_locked(ctx, g, g == RMH_UNAVAILABLE ? "accelerator disabled" : "injected device failure");
     DeviceGuard dg(ctx->device);
     size_t fb = n_frames * sizeof(rmh_dbk_frame), sb = n_frames * 3 * sizeof(rmh_plane);
-    const std::vector<uint32_t> starts = tile_starts(frames, n_frames);
-    const size_t tb = starts.size() * sizeof(uint32_t);
+    const TileTable tt = tile_table(frames, n_frames);
+    const size_t tb = tt.words.size() * sizeof(uint32_t);
     size_t need = aligned(fb) + aligned(sb) + aligned(tb) + shadow_bytes(frames, n_frames), hneed = 0;
     uint32_t max_rows = 0;
     for (size_t f = 0; f < n_frames; f++) {
@@ -1505,7 +1525,7 @@ RMH_EXPORT rmh_status rmh_deblock(rmh_ctx* ctx, const rmh_dbk_frame* frames, siz
     rmh_dbk_frame* d_frames = (rmh_dbk_frame*)ctx->dev.take(fb);
     rmh_plane* d_shadows = (rmh_plane*)ctx->dev.take(sb);
     uint32_t* d_starts = (uint32_t*)ctx->dev.take(tb);
-    if ((e = hipMemcpyAsync(d_starts, starts.data(), tb, hipMemcpyHostToDevice, s)) != hipSuccess)
+    if ((e = hipMemcpyAsync(d_starts, tt.words.data(), tb, hipMemcpyHostToDevice, s)) != hipSuccess)
         return hip_fail_locked(ctx, e, "tile table upload");
     const std::vector<rmh_plane> sh = shadow_layout(frames, n_frames, (char*)ctx->dev.take(shadow_bytes(frames, n_frames)));
     if ((e = hipMemcpyAsync(d_shadows, sh.data(), sb, hipMemcpyHostToDevice, s)) != hipSuccess)
@@ -1528,7 +1548,8 @@ RMH_EXPORT rmh_status rmh_deblock(rmh_ctx* ctx, const rmh_dbk_frame* frames, siz
         }
     }
     if ((e = hipMemcpyAsync(d_frames, df.data(), fb, hipMemcpyHostToDevice, s)) != hipSuccess) return hip_fail_locked(ctx, e, "upload");
-    e = rmh::launch_deblock(d_frames, d_shadows, d_starts, (uint32_t)n_frames, starts.back(), max_rows, s);
+    e = rmh::launch_deblock(d_frames, d_shadows, d_starts, d_starts + n_frames + 1, (uint32_t)n_frames, tt.total,
+                            tt.tiles8, max_rows, s);
     if (e != hipSuccess) return hip_fail_locked(ctx, e, "kernel launch");
     std::vector<void*> hp;
     for (size_t f = 0; f < n_frames; f++)
