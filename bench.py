@@ -14,6 +14,7 @@ CPU oracle timed on a bounded sample on this host.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--frames F]
     python bench.py --copy-calibration      (copy kernel of known size, for rocprofv3 --pmc scaling)
+    python bench.py --cnn-kernels           (the CU-split network's forward pass and training step alone)
 """
 import argparse
 import json
@@ -306,18 +307,12 @@ def bench_tool_mix(rmh, acc, W, po, n_frames, steps, warmup):
 CNN_MACS_PER_BLOCK = 32 * 32 * 32 * 9 + 2 * 32 * 32 * 32 * 288 + 33 * 64 + 64 * 6
 
 
-def bench_learned(rmh, acc, steps):
-    """The learned CU-split predictor: training on full-RD pairs, calibration,
-    full / pruned / heuristic encodes of held-out content, and the network's
-    forward and training-step rates on MFMA."""
+def cnn_kernel_rates(rmh, net, steps):
+    """Forward pass and training step of the CU-split network on one batch of
+    256 blocks of synthetic content, by HIP events: (forward ms, step ms,
+    forward FLOPs). The network's parameters are restored afterwards."""
     hip = rmh.hip
     L = rmh.learned
-    net, rep = L.run_experiment(acc, log=log)
-    t = rep["test"]
-    log("learned: pruned %.2f fps vs full %.2f fps (x%.2f), BD-rate %.3f %%, heuristic BD-rate %.3f %%" % (
-        t["pruned"]["fps"], t["full"]["fps"], t["speedup_pruned_vs_full"], t["pruned"]["bd_rate_vs_full"],
-        t["heuristic"]["bd_rate_vs_full"]))
-    # forward and train-step throughput on one batch of 256 blocks
     n = 256
     frames = L.synthetic_sequence(2, 1)
     blocks = L.node_blocks(L.pad_to_nodes(frames[0]))[:n]
@@ -331,10 +326,24 @@ def bench_learned(rmh, acc, steps):
     saved = net.params()
     step_ms = timed(hip, stream, lambda s: net.train_step(db.ptr, dq.ptr, dl.ptr, n, 1e-6, dloss.ptr, s), steps, 3)
     net.set_params(saved)
-    flops = 2.0 * CNN_MACS_PER_BLOCK * n
-    tflops = flops / (fwd_ms / 1e3) / 1e12
     for b in (db, dq, dl, dp, dloss):
         b.close()
+    return fwd_ms, step_ms, 2.0 * CNN_MACS_PER_BLOCK * n
+
+
+def bench_learned(rmh, acc, steps):
+    """The learned CU-split predictor: training on full-RD pairs, calibration,
+    full / pruned / heuristic encodes of held-out content, and the network's
+    forward and training-step rates on MFMA."""
+    hip = rmh.hip
+    L = rmh.learned
+    net, rep = L.run_experiment(acc, log=log)
+    t = rep["test"]
+    log("learned: pruned %.2f fps vs full %.2f fps (x%.2f), BD-rate %.3f %%, heuristic BD-rate %.3f %%" % (
+        t["pruned"]["fps"], t["full"]["fps"], t["speedup_pruned_vs_full"], t["pruned"]["bd_rate_vs_full"],
+        t["heuristic"]["bd_rate_vs_full"]))
+    fwd_ms, step_ms, flops = cnn_kernel_rates(rmh, net, steps)
+    tflops = flops / (fwd_ms / 1e3) / 1e12
     net.close()
     def strip(d):
         return {k: (round(v, 4) if isinstance(v, float) else v) for k, v in d.items() if k != "splits"}
@@ -376,11 +385,24 @@ def main():
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-bi", action="store_true")
+    ap.add_argument("--cnn-kernels", action="store_true")
     ap.add_argument("--no-tools", action="store_true")
     args = ap.parse_args()
 
     if args.copy_calibration:
         copy_calibration()
+        return 0
+    if args.cnn_kernels:
+        # the CU-split network's kernels alone (for kernel traces): forward and
+        # training step on one 256-block batch, untrained weights
+        rmh = load_package()
+        acc = rmh.Accelerator(0)
+        net = rmh.Cnn(acc, max_batch=256, seed=1)
+        fwd_ms, step_ms, flops = cnn_kernel_rates(rmh, net, args.steps)
+        print(json.dumps({"forward_ms_256": round(fwd_ms, 4), "train_step_ms_256": round(step_ms, 4),
+                          "forward_tflops": round(flops / (fwd_ms / 1e3) / 1e12, 2)}), flush=True)
+        net.close()
+        acc.close()
         return 0
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the residual stage shards by "
