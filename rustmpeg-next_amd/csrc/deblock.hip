@@ -11,6 +11,7 @@
 // below states why this equals the reference order).
 #include <hip/hip_runtime.h>
 
+#include "gmem.h"
 #include "kernels.h"
 #include "xcd.h"
 
@@ -279,18 +280,18 @@ __global__ __launch_bounds__(256) void shadow_copy_kernel(const rmh_dbk_frame* _
     const uint32_t sp = src.stride * src.bytes_per_sample, dp = dst.stride * dst.bytes_per_sample;
     const uintptr_t al = (uintptr_t)src.data | (uintptr_t)dst.data | sp | dp | row_bytes;
     for (uint32_t y = blockIdx.x * 4 + (threadIdx.x >> 6); y < src.height; y += gridDim.x * 4) {
-        const uint8_t* s = (const uint8_t*)src.data + (size_t)y * sp;
-        uint8_t* d = (uint8_t*)dst.data + (size_t)y * dp;
+        const RMH_GLOBAL uint8_t* s = gsrc<uint8_t>(src.data) + (size_t)y * sp;
+        RMH_GLOBAL uint8_t* d = gdst<uint8_t>(dst.data) + (size_t)y * dp;
         const uint32_t lane = threadIdx.x & 63;
         if ((al & 15) == 0) {
             for (uint32_t o = lane * 16; o < row_bytes; o += 1024)
-                *reinterpret_cast<uint4*>(d + o) = *reinterpret_cast<const uint4*>(s + o);
+                *(RMH_GLOBAL u32x4*)(d + o) = *(const RMH_GLOBAL u32x4*)(s + o);
         } else if ((al & 7) == 0) {
             for (uint32_t o = lane * 8; o < row_bytes; o += 512)
-                *reinterpret_cast<uint2*>(d + o) = *reinterpret_cast<const uint2*>(s + o);
+                *(RMH_GLOBAL u32x2*)(d + o) = *(const RMH_GLOBAL u32x2*)(s + o);
         } else if ((al & 3) == 0) {
             for (uint32_t o = lane * 4; o < row_bytes; o += 256)
-                *reinterpret_cast<uint32_t*>(d + o) = *reinterpret_cast<const uint32_t*>(s + o);
+                *(RMH_GLOBAL uint32_t*)(d + o) = *(const RMH_GLOBAL uint32_t*)(s + o);
         } else {
             for (uint32_t o = lane; o < row_bytes; o += 64) d[o] = s[o];
         }
@@ -327,15 +328,15 @@ constexpr int kCWW = kCTW + 8, kCWH = kCTH + 8;        // chroma window (4 halo 
 template <int BPS>
 __device__ __forceinline__ int ld32(const rmh_plane& p, int x, int y) {
     const uint32_t o = (uint32_t)y * p.stride + (uint32_t)x;
-    return BPS == 1 ? (int)((const uint8_t*)p.data)[o] : (int)((const uint16_t*)p.data)[o];
+    return BPS == 1 ? (int)gsrc<uint8_t>(p.data)[o] : (int)gsrc<uint16_t>(p.data)[o];
 }
 template <int BPS>
 __device__ __forceinline__ void st32(const rmh_plane& p, int x, int y, int v) {
     const uint32_t o = (uint32_t)y * p.stride + (uint32_t)x;
     if (BPS == 1)
-        ((uint8_t*)p.data)[o] = (uint8_t)v;
+        gdst<uint8_t>(p.data)[o] = (uint8_t)v;
     else
-        ((uint16_t*)p.data)[o] = (uint16_t)v;
+        gdst<uint16_t>(p.data)[o] = (uint16_t)v;
 }
 
 // Window (wx0, wy0) .. + (WW, WH) of plane P read (clamped) from the shadow.
@@ -621,9 +622,9 @@ __global__ __launch_bounds__(256) void deblock_tile_kernel(const rmh_dbk_frame* 
         const int cx = t.X0 / 4 + threadIdx.x % kEX, cy = t.Y0 / 4 - 2 + threadIdx.x / kEX;
         uint32_t v = 0;
         if (cx < (int)f.grid_w && cy >= 0 && cy < (int)f.grid_h) {
-            const uint8_t* e = f.edges + ((size_t)cy * f.grid_w + cx) * 4;
+            const RMH_GLOBAL uint8_t* e = gsrc<uint8_t>(f.edges) + ((size_t)cy * f.grid_w + cx) * 4;
             if (((uintptr_t)f.edges & 3) == 0)
-                v = *reinterpret_cast<const uint32_t*>(e);
+                v = *(const RMH_GLOBAL uint32_t*)e;
             else
                 v = (uint32_t)e[0] | ((uint32_t)e[1] << 8) | ((uint32_t)e[2] << 16) | ((uint32_t)e[3] << 24);
         }

@@ -16,6 +16,7 @@
 // to LDS; the vertical pass writes the prediction.
 #include <hip/hip_runtime.h>
 
+#include "gmem.h"
 #include "kernels.h"
 #include "xcd.h"
 
@@ -47,16 +48,16 @@ __device__ __forceinline__ int load_sample(const rmh_plane& p, int x, int y) {
     x = clampi(x, 0, (int)p.width - 1);
     y = clampi(y, 0, (int)p.height - 1);
     const uint32_t o = (uint32_t)y * p.stride + (uint32_t)x;
-    return BPS == 1 ? (int)((const uint8_t*)p.data)[o] : (int)((const uint16_t*)p.data)[o];
+    return BPS == 1 ? (int)gsrc<uint8_t>(p.data)[o] : (int)gsrc<uint16_t>(p.data)[o];
 }
 
 template <int BPS>
 __device__ __forceinline__ void store_sample(const rmh_plane& p, int x, int y, int v) {
     const uint32_t o = (uint32_t)y * p.stride + (uint32_t)x;
     if (BPS == 1)
-        ((uint8_t*)p.data)[o] = (uint8_t)v;
+        gdst<uint8_t>(p.data)[o] = (uint8_t)v;
     else
-        ((uint16_t*)p.data)[o] = (uint16_t)v;
+        gdst<uint16_t>(p.data)[o] = (uint16_t)v;
 }
 
 __device__ __forceinline__ int dot2(uint32_t a, uint32_t b, int acc) {
@@ -109,11 +110,11 @@ __device__ __forceinline__ void stage(const rmh_plane& ref, const Tile<TAPS>& t,
         const uint32_t base = (uint32_t)(oy + r0) * ref.stride + (uint32_t)(ox + c);
         const uint32_t step = RSTEP * ref.stride;
         if (BPS == 1) {
-            const uint8_t* src = (const uint8_t*)ref.data + base;
+            const RMH_GLOBAL uint8_t* src = gsrc<uint8_t>(ref.data) + base;
 #pragma unroll 4
             for (int r = r0, o = 0; r < ROWS; r += RSTEP, o += step) win[r * PITCH + c] = (int16_t)src[o];
         } else {
-            const uint16_t* src = (const uint16_t*)ref.data + base;
+            const RMH_GLOBAL uint16_t* src = gsrc<uint16_t>(ref.data) + base;
 #pragma unroll 4
             for (int r = r0, o = 0; r < ROWS; r += RSTEP, o += step) win[r * PITCH + c] = (int16_t)src[o];
         }

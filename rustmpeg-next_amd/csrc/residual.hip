@@ -22,6 +22,7 @@
 
 #include <type_traits>
 
+#include "gmem.h"
 #include "kernels.h"
 
 namespace rmh {
@@ -110,11 +111,11 @@ __device__ __forceinline__ ReconDst recon_dst(const rmh_tu_place* __restrict__ p
 // pred + res clipped to [0, maxv], in place (reconstruct_tu add and clip).
 __device__ __forceinline__ void recon_add(const ReconDst& d, int x, int y, int res) {
     if (d.bps == 1) {
-        uint8_t* q = (uint8_t*)d.base + (size_t)y * d.stride + x;
+        RMH_GLOBAL uint8_t* q = gdst<uint8_t>(d.base) + (size_t)y * d.stride + x;
         const int v = (int)*q + res;
         *q = (uint8_t)(v < 0 ? 0 : v > d.maxv ? d.maxv : v);
     } else {
-        uint16_t* q = (uint16_t*)d.base + (size_t)y * d.stride + x;
+        RMH_GLOBAL uint16_t* q = gdst<uint16_t>(d.base) + (size_t)y * d.stride + x;
         const int v = (int)*q + res;
         *q = (uint16_t)(v < 0 ? 0 : v > d.maxv ? d.maxv : v);
     }
