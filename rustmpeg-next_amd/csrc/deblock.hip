@@ -339,23 +339,80 @@ __device__ __forceinline__ void st32(const rmh_plane& p, int x, int y, int v) {
         gdst<uint16_t>(p.data)[o] = (uint16_t)v;
 }
 
-// Window (wx0, wy0) .. + (WW, WH) of plane P read (clamped) from the shadow.
-// All of a lane's loads are issued before the first LDS write.
-template <int WW, int WH, int BPS>
-__device__ __forceinline__ void load_window(const rmh_plane& sh, int wx0, int wy0, int16_t* win) {
-    constexpr int N = (WW * WH + 255) / 256;
-    const int w = sh.width, h = sh.height;
-    int v[N];
+// Chunked window reads and tile writes. A chunk is CH samples of one row
+// (CH * BPS bytes), aligned whenever the plane's base address and row pitch
+// are: a chunk inside the picture's columns moves as one vector access, a
+// chunk crossing a picture column edge sample by sample (reads clamped as the
+// reference's gather clamps them). Samples sit in LDS as int16, two per dword.
+template <int CH>
+struct Packed {
+    uint32_t d[CH / 2];
+};
+
+__device__ __forceinline__ uint32_t widen_lo(uint32_t v) { return (v & 0xffu) | ((v & 0xff00u) << 8); }
+__device__ __forceinline__ uint32_t widen_hi(uint32_t v) { return ((v >> 16) & 0xffu) | ((v >> 8) & 0xff0000u); }
+__device__ __forceinline__ uint32_t narrow(uint32_t a, uint32_t b) {
+    return (a & 0xffu) | ((a >> 8) & 0xff00u) | ((b & 0xffu) << 16) | ((b << 8) & 0xff000000u);
+}
+
+template <int CH, int BPS>
+__device__ __forceinline__ bool chunks_aligned(const rmh_plane& p) {
+    return (((uintptr_t)p.data | (uintptr_t)p.stride * BPS) & (CH * BPS - 1)) == 0;
+}
+
+// Samples x .. x + CH - 1 of row y (already clamped into the picture).
+template <int CH, int BPS>
+__device__ __forceinline__ Packed<CH> read_chunk(const rmh_plane& p, int x, int y, bool vec) {
+    Packed<CH> o;
+    const int w = p.width;
+    if (vec && x >= 0 && x + CH <= w) {
+        const RMH_GLOBAL uint8_t* src = gsrc<uint8_t>(p.data) + ((size_t)y * p.stride + x) * BPS;
+        if constexpr (BPS == 2 && CH == 8) {
+            const u32x4 v = *(const RMH_GLOBAL u32x4*)src;
+            o.d[0] = v.x; o.d[1] = v.y; o.d[2] = v.z; o.d[3] = v.w;
+        } else if constexpr (BPS == 2) {
+            const u32x2 v = *(const RMH_GLOBAL u32x2*)src;
+            o.d[0] = v.x; o.d[1] = v.y;
+        } else if constexpr (CH == 8) {
+            const u32x2 v = *(const RMH_GLOBAL u32x2*)src;
+            o.d[0] = widen_lo(v.x); o.d[1] = widen_hi(v.x); o.d[2] = widen_lo(v.y); o.d[3] = widen_hi(v.y);
+        } else {
+            const uint32_t v = *(const RMH_GLOBAL uint32_t*)src;
+            o.d[0] = widen_lo(v); o.d[1] = widen_hi(v);
+        }
+    } else {
 #pragma unroll
-    for (int i = 0; i < N; i++) {
-        const int e = threadIdx.x + 256 * i;
-        const int r = e / WW, c = e - r * WW;
-        v[i] = e < WW * WH ? ld32<BPS>(sh, clampi(wx0 + c, 0, w - 1), clampi(wy0 + r, 0, h - 1)) : 0;
+        for (int j = 0; j < CH / 2; j++)
+            o.d[j] = (uint32_t)ld32<BPS>(p, clampi(x + 2 * j, 0, w - 1), y) |
+                     ((uint32_t)ld32<BPS>(p, clampi(x + 2 * j + 1, 0, w - 1), y) << 16);
     }
+    return o;
+}
+
+template <int CH>
+__device__ __forceinline__ void put_chunk(int16_t* dst, const Packed<CH>& v) {
+    if constexpr (CH == 8)
+        *reinterpret_cast<u32x4*>(dst) = u32x4{v.d[0], v.d[1], v.d[2], v.d[3]};
+    else
+        *reinterpret_cast<u32x2*>(dst) = u32x2{v.d[0], v.d[1]};
+}
+
+// Samples x .. x + 7 of row y from LDS (src 8-byte aligned) to the picture;
+// samples at or beyond column w are not written.
+template <int BPS>
+__device__ __forceinline__ void write_chunk8(const rmh_plane& p, int x, int y, const int16_t* src, bool vec) {
+    const u32x2 a = reinterpret_cast<const u32x2*>(src)[0], b = reinterpret_cast<const u32x2*>(src)[1];
+    if (vec && x + 8 <= (int)p.width) {
+        RMH_GLOBAL uint8_t* dst = gdst<uint8_t>(p.data) + ((size_t)y * p.stride + x) * BPS;
+        if (BPS == 2)
+            *(RMH_GLOBAL u32x4*)dst = u32x4{a.x, a.y, b.x, b.y};
+        else
+            *(RMH_GLOBAL u32x2*)dst = u32x2{narrow(a.x, a.y), narrow(b.x, b.y)};
+    } else {
+        const uint32_t d[4] = {a.x, a.y, b.x, b.y};
 #pragma unroll
-    for (int i = 0; i < N; i++) {
-        const int e = threadIdx.x + 256 * i;
-        if (e < WW * WH) win[e] = (int16_t)v[i];
+        for (int j = 0; j < 8; j++)
+            if (x + j < (int)p.width) st32<BPS>(p, x + j, y, (int)((d[j >> 1] >> (16 * (j & 1))) & 0xffffu));
     }
 }
 
@@ -589,7 +646,7 @@ __global__ __launch_bounds__(256) void deblock_tile_kernel(const rmh_dbk_frame* 
                                                            const uint32_t* __restrict__ frame_of, uint32_t n_frames,
                                                            uint32_t first_tile) {
     __shared__ __align__(16) int16_t lw[kLWH * kLWW];
-    __shared__ int16_t cwin[2][kCWH * kCWW];
+    __shared__ __align__(16) int16_t cwin[2][kCWH * kCWW];
     __shared__ uint32_t ql[kMaxQ], qc[kMaxQ];
     __shared__ uint32_t cells[kEY * kEX];
     __shared__ int counts[2];
@@ -630,9 +687,33 @@ __global__ __launch_bounds__(256) void deblock_tile_kernel(const rmh_dbk_frame* 
         }
         cells[threadIdx.x] = v;
     }
-    load_window<kLWW, kLWH, BPS>(sh[0], t.X0 - 8, t.Y0 - 8, lw);
-    load_window<kCWW, kCWH, BPS>(sh[1], t.CX0 - 4, t.CY0 - 4, cwin[0]);
-    load_window<kCWW, kCWH, BPS>(sh[2], t.CX0 - 4, t.CY0 - 4, cwin[1]);
+    {
+        // luma window in 8-sample chunks (10 per row), both chroma windows in
+        // 4-sample chunks (10 per row); all loads are issued before the LDS writes
+        constexpr int NL = kLWH * kLWW / 8, NC = kCWH * kCWW / 4;  // 480, 240
+        const bool vl = chunks_aligned<8, BPS>(sh[0]);
+        const bool vc = chunks_aligned<4, BPS>(sh[1]) && chunks_aligned<4, BPS>(sh[2]);
+        Packed<8> a[2];
+        Packed<4> b[2];
+#pragma unroll
+        for (int i = 0; i < 2; i++) {
+            const int e = threadIdx.x + 256 * i;
+            if (e < NL) {
+                const int r = e / 10, c = e - 10 * r;
+                a[i] = read_chunk<8, BPS>(sh[0], t.X0 - 8 + 8 * c, clampi(t.Y0 - 8 + r, 0, t.H - 1), vl);
+            }
+            if (e < 2 * NC) {
+                const int pl = e >= NC, q = e - NC * pl, r = q / 10, c = q - 10 * r;
+                b[i] = read_chunk<4, BPS>(sh[1 + pl], t.CX0 - 4 + 4 * c, clampi(t.CY0 - 4 + r, 0, t.CH - 1), vc);
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 2; i++) {
+            const int e = threadIdx.x + 256 * i;
+            if (e < NL) put_chunk<8>(lw + 8 * e, a[i]);
+            if (e < 2 * NC) put_chunk<4>(&cwin[0][0] + 4 * e, b[i]);  // the two planes: one array of 2 x 240 chunks
+        }
+    }
     __syncthreads();
     // 2. vertical edges
     tile_pass(t, 0, lw, cwin, ql, qc, counts);
@@ -646,14 +727,22 @@ __global__ __launch_bounds__(256) void deblock_tile_kernel(const rmh_dbk_frame* 
     // 4. horizontal edges
     tile_pass(t, 1, lw, cwin, ql, qc, counts);
     // 5. write the tile back (picture samples only)
-    for (int e = threadIdx.x; e < kTW * kTH; e += 256) {
-        const int r = e / kTW, c = e % kTW, x = t.X0 + c, y = t.Y0 + r;
-        if (x < t.W && y < t.H) st32<BPS>(PY, x, y, lw[(r + 8) * kLWW + c + 8]);
-    }
-    for (int e = threadIdx.x; e < 2 * kCTW * kCTH; e += 256) {
-        const int pl = e / (kCTW * kCTH), q = e % (kCTW * kCTH), r = q / kCTW, c = q % kCTW;
-        const int x = t.CX0 + c, y = t.CY0 + r;
-        if (x < t.CW && y < t.CH) st32<BPS>(t.f->planes[1 + pl], x, y, cwin[pl][(r + 4) * kCWW + c + 4]);
+    // in 8-sample chunks: 256 luma chunks (8 per row), then 2 x 64 chroma chunks (4 per row)
+    {
+        constexpr int NL = kTW * kTH / 8, NC = kCTW * kCTH / 8;
+        for (int e = threadIdx.x; e < NL + 2 * NC; e += 256) {
+            if (e < NL) {
+                const int r = e >> 3, c = e & 7, y = t.Y0 + r;
+                if (y < t.H)
+                    write_chunk8<BPS>(PY, t.X0 + 8 * c, y, lw + (r + 8) * kLWW + 8 * c + 8, chunks_aligned<8, BPS>(PY));
+            } else {
+                const int pl = (e - NL) >= NC, q = e - NL - NC * pl, r = q >> 2, c = q & 3, y = t.CY0 + r;
+                const rmh_plane& P = t.f->planes[1 + pl];
+                if (y < t.CH)
+                    write_chunk8<BPS>(P, t.CX0 + 8 * c, y, cwin[pl] + (r + 4) * kCWW + 8 * c + 4,
+                                      chunks_aligned<8, BPS>(P));
+            }
+        }
     }
 }
 

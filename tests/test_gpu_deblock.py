@@ -71,3 +71,21 @@ def test_4x4_grid_edges(acc, po):
             if j > 0:
                 e[j, i, 1] = W.edge_byte(int(rng.integers(0, 3)), 1, 1)
     _run(acc, po, [Y, U, V], e, bd, [40, 39, 39])
+
+
+@pytest.mark.parametrize("w,h,bd", [(200, 120, 8), (200, 136, 10), (456, 72, 12), (196, 100, 10)])
+def test_partial_tiles_and_unaligned_rows(acc, po, w, h, bd):
+    """Widths that end inside a 64-sample tile, and planes whose row pitch is
+    not a multiple of the 8- or 4-sample chunks the kernel moves as one access
+    (8-bit chroma rows of 100 bytes, 10-bit chroma rows of 200 bytes): chunks
+    crossing the right picture edge and unaligned planes go sample by sample.
+    Frames are cropped from a larger generated frame, edge map included."""
+    print("deblock cropped %dx%d bd %d" % (w, h, bd), flush=True)
+    d = W.inter_frame(11 + w + h, width=(w + 63) // 64 * 64, height=(h + 31) // 32 * 32, bit_depth=bd)
+    dt = np.uint8 if bd == 8 else np.uint16
+    cw, ch = (w + 1) // 2, (h + 1) // 2
+    planes = [np.ascontiguousarray(d["ref"][0][:h, :w].astype(dt)),
+              np.ascontiguousarray(d["ref"][1][:ch, :cw].astype(dt)),
+              np.ascontiguousarray(d["ref"][2][:ch, :cw].astype(dt))]
+    edges = np.ascontiguousarray(d["edges"][:(h + 3) // 4, :(w + 3) // 4])
+    _run(acc, po, planes, edges, bd, d["qp"])
