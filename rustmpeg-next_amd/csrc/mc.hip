@@ -96,32 +96,48 @@ struct Tile {
     bool hf, vf;   // fractional horizontally / vertically
 };
 
-// Phase 1: stage window rows iy - BEFORE .. (ROWS) and columns ix - BEFORE ..
-// (PITCH, the last one only as padding), clamped to the picture, into LDS.
-// Lane (c, r0) loads column c of rows r0, r0 + 64 / LANES_X, ...
+// Phase 1: window rows iy - BEFORE .. (ROWS) and columns ix - BEFORE ..
+// (PITCH, the last one only as padding), clamped to the picture. Lane (c, r0)
+// holds column c of rows r0, r0 + 64 / LANES_X, ... fetch issues the loads
+// into registers; put writes them to LDS, so a wave can have the loads of all
+// its windows in flight before the first LDS write waits for them.
+template <int ROWS, int LANES_X>
+struct Window {
+    static constexpr int RSTEP = 64 / LANES_X, N = (ROWS + RSTEP - 1) / RSTEP;
+    int v[N];
+};
+
 template <int TAPS, int ROWS, int PITCH, int LANES_X, int BPS>
-__device__ __forceinline__ void stage(const rmh_plane& ref, const Tile<TAPS>& t, int16_t* win, int lane) {
-    constexpr int BEFORE = TAPS / 2 - 1, RSTEP = 64 / LANES_X;
+__device__ __forceinline__ void fetch(const rmh_plane& ref, const Tile<TAPS>& t, Window<ROWS, LANES_X>& w, int lane) {
+    using W = Window<ROWS, LANES_X>;
+    constexpr int BEFORE = TAPS / 2 - 1;
     const int ox = t.ix - BEFORE, oy = t.iy - BEFORE;
     const int c = lane & (LANES_X - 1), r0 = lane / LANES_X;
     if (c >= PITCH) return;
     const bool inside = ox >= 0 && oy >= 0 && ox + PITCH <= (int)ref.width && oy + ROWS <= (int)ref.height;
     if (inside) {
         const uint32_t base = (uint32_t)(oy + r0) * ref.stride + (uint32_t)(ox + c);
-        const uint32_t step = RSTEP * ref.stride;
-        if (BPS == 1) {
-            const RMH_GLOBAL uint8_t* src = gsrc<uint8_t>(ref.data) + base;
-#pragma unroll 4
-            for (int r = r0, o = 0; r < ROWS; r += RSTEP, o += step) win[r * PITCH + c] = (int16_t)src[o];
-        } else {
-            const RMH_GLOBAL uint16_t* src = gsrc<uint16_t>(ref.data) + base;
-#pragma unroll 4
-            for (int r = r0, o = 0; r < ROWS; r += RSTEP, o += step) win[r * PITCH + c] = (int16_t)src[o];
-        }
+        const uint32_t step = W::RSTEP * ref.stride;
+#pragma unroll
+        for (int k = 0; k < W::N; k++)
+            if (r0 + W::RSTEP * k < ROWS)
+                w.v[k] = BPS == 1 ? (int)gsrc<uint8_t>(ref.data)[base + k * step]
+                                  : (int)gsrc<uint16_t>(ref.data)[base + k * step];
     } else {
-#pragma unroll 4
-        for (int r = r0; r < ROWS; r += RSTEP) win[r * PITCH + c] = (int16_t)load_sample<BPS>(ref, ox + c, oy + r);
+#pragma unroll
+        for (int k = 0; k < W::N; k++)
+            if (r0 + W::RSTEP * k < ROWS) w.v[k] = load_sample<BPS>(ref, ox + c, oy + r0 + W::RSTEP * k);
     }
+}
+
+template <int ROWS, int PITCH, int LANES_X>
+__device__ __forceinline__ void put(const Window<ROWS, LANES_X>& w, int16_t* win, int lane) {
+    using W = Window<ROWS, LANES_X>;
+    const int c = lane & (LANES_X - 1), r0 = lane / LANES_X;
+    if (c >= PITCH) return;
+#pragma unroll
+    for (int k = 0; k < W::N; k++)
+        if (r0 + W::RSTEP * k < ROWS) win[(r0 + W::RSTEP * k) * PITCH + c] = (int16_t)w.v[k];
 }
 
 // Horizontal FIR of output pair (x, x + 1), x even, on a row of int16 samples:
@@ -143,69 +159,93 @@ __device__ __forceinline__ void hpair(const int16_t* row, const uint32_t (&c)[TA
 
 // Phase 2 (two-dimensional tiles only): horizontal pass over th + TAPS - 1
 // rows into column-major intermediates, (sum - (8192 << shift1)) >> shift1
-// (interpolate_luma_block 2-D branch, vvc/inter.rs:1004-1030). Lane item e:
-// pair xp = e % NPAIR, row e / NPAIR.
-template <int TAPS, int PITCH, int NPAIR>
+// (interpolate_luma_block 2-D branch, vvc/inter.rs:1004-1030), for NPL planes
+// of one tile geometry (the two chroma planes share their tile). Lane item e:
+// plane e / (RMAX * NPAIR), then pair xp = e % NPAIR, row e / NPAIR; a fixed
+// number of items per lane.
+template <int TAPS, int PITCH, int NPAIR, int RMAX, int NPL, int WSTRIDE, int TSTRIDE>
 __device__ __forceinline__ void hpass(const Tile<TAPS>& t, const int16_t* win, int16_t* tmp, int shift1, int lane) {
     if (!(t.hf && t.vf)) return;
+    constexpr int PER = RMAX * NPAIR, N = (NPL * PER + 63) / 64;
     const int rows = t.th + TAPS - 1;
     const int off = 8192 << shift1;
-    for (int e = lane; e < rows * NPAIR; e += 64) {
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+        const int e0 = lane + 64 * k;
+        const int pl = NPL == 1 ? 0 : e0 / PER, e = e0 - pl * PER;
         const int xp = e % NPAIR, r = e / NPAIR, x = 2 * xp;
-        if (x >= t.tw) continue;
-        int s0, s1;
-        hpair<TAPS>(win + r * PITCH + x, t.cx, s0, s1);
-        tmp[x * PITCH + r] = (int16_t)((s0 - off) >> shift1);
-        tmp[(x + 1) * PITCH + r] = (int16_t)((s1 - off) >> shift1);
+        if (e0 < NPL * PER && r < rows && x < t.tw) {
+            int s0, s1;
+            hpair<TAPS>(win + pl * WSTRIDE + r * PITCH + x, t.cx, s0, s1);
+            int16_t* o = tmp + pl * TSTRIDE + x * PITCH + r;
+            o[0] = (int16_t)((s0 - off) >> shift1);
+            o[PITCH] = (int16_t)((s1 - off) >> shift1);
+        }
     }
 }
 
-// Phase 3: the output samples of one plane's tile (copy, one-dimensional or
-// vertical pass), T = full tile width.
-template <int TAPS, int PITCH, int T, int BPS>
-__device__ __forceinline__ void output(const rmh_plane& dst, const Tile<TAPS>& t, const int16_t* win, const int16_t* tmp,
-                                       int head_room, int maxv, int lane) {
+// Phase 3 (uni-prediction): the four output samples (x, y), (x + 1, y),
+// (x, y + 1), (x + 1, y + 1), x and y even, of one plane's tile, clipped to
+// [0, maxv]: a copy, a one-dimensional filter (sum + 32) >> 6
+// (vvc/inter.rs:939-1002), or the vertical pass over the intermediates
+// (sum + 2^(shift2-1) + (8192 << 6)) >> shift2, shift2 = 6 + head_room.
+template <int TAPS, int PITCH>
+__device__ __forceinline__ void uni_quad(const Tile<TAPS>& t, const int16_t* win, const int16_t* tmp, int x, int y,
+                                         int head_room, int maxv, int (&v)[4]) {
     constexpr int BEFORE = TAPS / 2 - 1;
     if (t.hf && t.vf) {
-        // vertical pass over the intermediates, output pair (y, y + 1):
-        // (sum + 2^(shift2-1) + (8192 << 6)) >> shift2, shift2 = 6 + head_room
         const int shift2 = 6 + head_room;
         const int off2 = (1 << (shift2 - 1)) + (8192 << 6);
-        for (int e = lane; e < T * (T / 2); e += 64) {
-            const int x = e % T, y = 2 * (e / T);
-            if (x >= t.tw || y >= t.th) continue;
-            int s0, s1;
-            hpair<TAPS>(tmp + x * PITCH + y, t.cy, s0, s1);
-            store_sample<BPS>(dst, t.x0 + x, t.y0 + y, clampi((s0 + off2) >> shift2, 0, maxv));
-            if (y + 1 < t.th) store_sample<BPS>(dst, t.x0 + x, t.y0 + y + 1, clampi((s1 + off2) >> shift2, 0, maxv));
-        }
+        int s[4];
+        hpair<TAPS>(tmp + x * PITCH + y, t.cy, s[0], s[2]);
+        hpair<TAPS>(tmp + (x + 1) * PITCH + y, t.cy, s[1], s[3]);
+#pragma unroll
+        for (int q = 0; q < 4; q++) v[q] = clampi((s[q] + off2) >> shift2, 0, maxv);
     } else if (t.hf) {
-        // horizontal only: (sum + 32) >> 6 clipped (vvc/inter.rs:939-970), output pair (x, x + 1)
-        for (int e = lane; e < T * (T / 2); e += 64) {
-            const int xp = e % (T / 2), y = e / (T / 2), x = 2 * xp;
-            if (x >= t.tw || y >= t.th) continue;
-            int s0, s1;
-            hpair<TAPS>(win + (y + BEFORE) * PITCH + x, t.cx, s0, s1);
-            store_sample<BPS>(dst, t.x0 + x, t.y0 + y, clampi((s0 + 32) >> 6, 0, maxv));
-            if (x + 1 < t.tw) store_sample<BPS>(dst, t.x0 + x + 1, t.y0 + y, clampi((s1 + 32) >> 6, 0, maxv));
-        }
+        int s[4];
+        hpair<TAPS>(win + (y + BEFORE) * PITCH + x, t.cx, s[0], s[1]);
+        hpair<TAPS>(win + (y + 1 + BEFORE) * PITCH + x, t.cx, s[2], s[3]);
+#pragma unroll
+        for (int q = 0; q < 4; q++) v[q] = clampi((s[q] + 32) >> 6, 0, maxv);
     } else if (t.vf) {
-        // vertical only (vvc/inter.rs:971-1002)
-        for (int e = lane; e < T * T; e += 64) {
-            const int x = e % T, y = e / T;
-            if (x >= t.tw || y >= t.th) continue;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
             int sum = 0;
 #pragma unroll
-            for (int k = 0; k < TAPS; k++) sum += t.sy[k] * win[(y + k) * PITCH + x + BEFORE];
-            store_sample<BPS>(dst, t.x0 + x, t.y0 + y, clampi((sum + 32) >> 6, 0, maxv));
+            for (int k = 0; k < TAPS; k++) sum += t.sy[k] * win[(y + (q >> 1) + k) * PITCH + x + (q & 1) + BEFORE];
+            v[q] = clampi((sum + 32) >> 6, 0, maxv);
         }
     } else {
-        for (int e = lane; e < T * T; e += 64) {
-            const int x = e % T, y = e / T;
-            if (x >= t.tw || y >= t.th) continue;
-            store_sample<BPS>(dst, t.x0 + x, t.y0 + y, win[(y + BEFORE) * PITCH + x + BEFORE]);
+#pragma unroll
+        for (int q = 0; q < 4; q++) v[q] = win[(y + (q >> 1) + BEFORE) * PITCH + x + (q & 1) + BEFORE];
+    }
+}
+
+// Stores the 2 x 2 samples v at (x, y) of dst, those at x + 1 / y + 1 only
+// when present; pair: sample pairs (x, x + 1) are aligned for one 2-sample store.
+template <int BPS>
+__device__ __forceinline__ void store_quad(const rmh_plane& dst, int x, int y, const int (&v)[4], bool x1, bool y1,
+                                           bool pair) {
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        if (h == 1 && !y1) break;
+        const uint32_t o = (uint32_t)(y + h) * dst.stride + (uint32_t)x;
+        if (pair && x1) {
+            if (BPS == 1)
+                *(RMH_GLOBAL uint16_t*)(gdst<uint8_t>(dst.data) + o) = (uint16_t)(v[2 * h] | (v[2 * h + 1] << 8));
+            else
+                *(RMH_GLOBAL uint32_t*)(gdst<uint16_t>(dst.data) + o) = pack2(v[2 * h], v[2 * h + 1]);
+        } else {
+            store_sample<BPS>(dst, x, y + h, v[2 * h]);
+            if (x1) store_sample<BPS>(dst, x + 1, y + h, v[2 * h + 1]);
         }
     }
+}
+
+// Whether sample pairs at even tile columns are aligned in dst.
+template <int BPS>
+__device__ __forceinline__ bool pairs_aligned(const rmh_plane& dst, int x0) {
+    return (((uintptr_t)dst.data & (2 * BPS - 1)) | (dst.stride & 1) | (x0 & 1)) == 0;
 }
 
 template <int TAPS>
@@ -266,10 +306,11 @@ __device__ __forceinline__ void bi_quad(const Tile<TAPS>& t, const int16_t* win,
 }
 
 // Four work items per 256-lane workgroup, one per wave: the 16x16 luma tile
-// of a PU (smaller for 8-wide or 8-tall PUs) and its two chroma tiles. All
-// three reference windows are requested before any filtering, so a wave
-// waits for memory once per item. Item fields, frame planes and filter taps
-// are wave-uniform and read with scalar loads.
+// of a PU (smaller for 8-wide or 8-tall PUs) and its two chroma tiles. The
+// loads of all three reference windows are issued before the first LDS write.
+// Outputs go in 2 x 2 quads: the 64 quads of the luma tile one per lane, the
+// 2 x 16 of the chroma tiles on lanes 0 .. 31. Item fields, frame planes and
+// filter taps are wave-uniform and read with scalar loads.
 template <int BPS>
 __global__ __launch_bounds__(256) void mc_kernel(const rmh_mc_frame* __restrict__ frames,
                                                  const McItem* __restrict__ items, uint32_t n_items) {
@@ -310,27 +351,46 @@ __global__ __launch_bounds__(256) void mc_kernel(const rmh_mc_frame* __restrict_
         tc.hf = cfx != 0;
         tc.vf = cfy != 0;
         bd = f->ref[0].bit_depth;
-        if (tl.tw > 0 && tl.th > 0) stage<8, kLR, kLP, 32, BPS>(f->ref[0], tl, L.luma, lane);
-        if (tc.tw > 0 && tc.th > 0) {
-            stage<4, kCR, kCP, 16, BPS>(f->ref[1], tc, L.chroma[0], lane);
-            stage<4, kCR, kCP, 16, BPS>(f->ref[2], tc, L.chroma[1], lane);
+        const bool hl = tl.tw > 0 && tl.th > 0, hc = tc.tw > 0 && tc.th > 0;
+        Window<kLR, 32> wl;
+        Window<kCR, 16> wc0, wc1;
+        if (hl) fetch<8, kLR, kLP, 32, BPS>(f->ref[0], tl, wl, lane);
+        if (hc) {
+            fetch<4, kCR, kCP, 16, BPS>(f->ref[1], tc, wc0, lane);
+            fetch<4, kCR, kCP, 16, BPS>(f->ref[2], tc, wc1, lane);
+        }
+        if (hl) put<kLR, kLP, 32>(wl, L.luma, lane);
+        if (hc) {
+            put<kCR, kCP, 16>(wc0, L.chroma[0], lane);
+            put<kCR, kCP, 16>(wc1, L.chroma[1], lane);
         }
     }
     __syncthreads();
     const int head_room = 14 - bd;
     const int shift1 = 6 - head_room;
-    if (active && tl.tw > 0 && tl.th > 0) hpass<8, kLP, 8>(tl, L.luma, L.ltmp, shift1, lane);
-    if (active && tc.tw > 0 && tc.th > 0) {
-        hpass<4, kCP, 4>(tc, L.chroma[0], L.ctmp[0], shift1, lane);
-        hpass<4, kCP, 4>(tc, L.chroma[1], L.ctmp[1], shift1, lane);
-    }
+    if (active && tl.tw > 0 && tl.th > 0) hpass<8, kLP, 8, kLR, 1, 0, 0>(tl, L.luma, L.ltmp, shift1, lane);
+    if (active && tc.tw > 0 && tc.th > 0)
+        hpass<4, kCP, 4, kCR, 2, kCR * kCP, 8 * kCP>(tc, L.chroma[0], L.ctmp[0], shift1, lane);
     __syncthreads();
     if (!active) return;
     const int maxv = (1 << bd) - 1;
-    if (tl.tw > 0 && tl.th > 0) output<8, kLP, 16, BPS>(f->dst[0], tl, L.luma, L.ltmp, head_room, maxv, lane);
-    if (tc.tw > 0 && tc.th > 0) {
-        output<4, kCP, 8, BPS>(f->dst[1], tc, L.chroma[0], L.ctmp[0], head_room, maxv, lane);
-        output<4, kCP, 8, BPS>(f->dst[2], tc, L.chroma[1], L.ctmp[1], head_room, maxv, lane);
+    if (tl.tw > 0 && tl.th > 0) {
+        const int x = 2 * (lane & 7), y = 2 * (lane >> 3);
+        if (x < tl.tw && y < tl.th) {
+            int v[4];
+            uni_quad<8, kLP>(tl, L.luma, L.ltmp, x, y, head_room, maxv, v);
+            store_quad<BPS>(f->dst[0], tl.x0 + x, tl.y0 + y, v, x + 1 < tl.tw, y + 1 < tl.th,
+                            pairs_aligned<BPS>(f->dst[0], tl.x0));
+        }
+    }
+    if (tc.tw > 0 && tc.th > 0 && lane < 32) {
+        const int pl = lane >> 4, x = 2 * (lane & 3), y = 2 * ((lane >> 2) & 3);
+        if (x < tc.tw && y < tc.th) {
+            int v[4];
+            uni_quad<4, kCP>(tc, L.chroma[pl], L.ctmp[pl], x, y, head_room, maxv, v);
+            store_quad<BPS>(f->dst[1 + pl], tc.x0 + x, tc.y0 + y, v, x + 1 < tc.tw, y + 1 < tc.th,
+                            pairs_aligned<BPS>(f->dst[1 + pl], tc.x0));
+        }
     }
 }
 
@@ -359,33 +419,27 @@ __device__ __forceinline__ void bi_tiles(const McBiItem& it, int mv_x, int mv_y,
     tc.vf = cfy != 0;
 }
 
-// Weighted average of the two lists' intermediates over one plane's tile
-// (weighted_average_bi_16, vvc/inter.rs:2676-2696), T = full tile width; a
-// lane takes 2 x 2 output samples.
-template <int TAPS, int PITCH, int T, int BPS>
-__device__ __forceinline__ void bi_output(const rmh_plane& dst, const Tile<TAPS>& t0, const Tile<TAPS>& t1,
-                                          const int16_t* win0, const int16_t* tmp0, const int16_t* win1,
-                                          const int16_t* tmp1, int head_room, int w0, int w1, int maxv, int lane) {
+// Weighted average of the two lists' intermediates at the 2 x 2 output
+// samples (x, y) .. (x + 1, y + 1) of one plane's tile
+// (weighted_average_bi_16, vvc/inter.rs:2676-2696).
+template <int TAPS, int PITCH, int BPS>
+__device__ __forceinline__ void bi_output_quad(const rmh_plane& dst, const Tile<TAPS>& t0, const Tile<TAPS>& t1,
+                                               const int16_t* win0, const int16_t* tmp0, const int16_t* win1,
+                                               const int16_t* tmp1, int x, int y, int head_room, int w0, int w1,
+                                               int maxv) {
     const int shift1 = 6 - head_room, shift = head_room + 3;
     const int off = (1 << (shift - 1)) + (8192 << 3);
-    for (int e = lane; e < (T / 2) * (T / 2); e += 64) {
-        const int x = 2 * (e % (T / 2)), y = 2 * (e / (T / 2));
-        if (x >= t0.tw || y >= t0.th) continue;
-        int p0[4], p1[4];
-        bi_quad<TAPS, PITCH>(t0, win0, tmp0, x, y, head_room, shift1, p0);
-        bi_quad<TAPS, PITCH>(t1, win1, tmp1, x, y, head_room, shift1, p1);
+    int p0[4], p1[4], v[4];
+    bi_quad<TAPS, PITCH>(t0, win0, tmp0, x, y, head_room, shift1, p0);
+    bi_quad<TAPS, PITCH>(t1, win1, tmp1, x, y, head_room, shift1, p1);
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const int xq = x + (q & 1), yq = y + (q >> 1);
-            if (xq < t0.tw && yq < t0.th)
-                store_sample<BPS>(dst, t0.x0 + xq, t0.y0 + yq, clampi((p0[q] * w0 + p1[q] * w1 + off) >> shift, 0, maxv));
-        }
-    }
+    for (int q = 0; q < 4; q++) v[q] = clampi((p0[q] * w0 + p1[q] * w1 + off) >> shift, 0, maxv);
+    store_quad<BPS>(dst, t0.x0 + x, t0.y0 + y, v, x + 1 < t0.tw, y + 1 < t0.th, pairs_aligned<BPS>(dst, t0.x0));
 }
 
-// Bi-prediction: as mc_kernel, with the windows of both lists staged before
-// any filtering (two LDS slices per wave) and the weighted average fused into
-// the output pass.
+// Bi-prediction: as mc_kernel, with the windows of both lists fetched before
+// any LDS write (two LDS slices per wave) and the weighted average fused into
+// the output quads.
 template <int BPS>
 __global__ __launch_bounds__(256) void mc_bi_kernel(const rmh_mc_bi_frame* __restrict__ frames,
                                                     const McBiItem* __restrict__ items, uint32_t n_items) {
@@ -407,39 +461,55 @@ __global__ __launch_bounds__(256) void mc_bi_kernel(const rmh_mc_bi_frame* __res
         bi_tiles(it, it.mv0_x, it.mv0_y, tl0, tc0);
         bi_tiles(it, it.mv1_x, it.mv1_y, tl1, tc1);
         bd = f->ref0[0].bit_depth;
-        if (tl0.tw > 0 && tl0.th > 0) {
-            stage<8, kLR, kLP, 32, BPS>(f->ref0[0], tl0, L0.luma, lane);
-            stage<8, kLR, kLP, 32, BPS>(f->ref1[0], tl1, L1.luma, lane);
+        const bool hl = tl0.tw > 0 && tl0.th > 0, hc = tc0.tw > 0 && tc0.th > 0;
+        Window<kLR, 32> wl0, wl1;
+        Window<kCR, 16> wc[4];
+        if (hl) {
+            fetch<8, kLR, kLP, 32, BPS>(f->ref0[0], tl0, wl0, lane);
+            fetch<8, kLR, kLP, 32, BPS>(f->ref1[0], tl1, wl1, lane);
         }
-        if (tc0.tw > 0 && tc0.th > 0) {
-            stage<4, kCR, kCP, 16, BPS>(f->ref0[1], tc0, L0.chroma[0], lane);
-            stage<4, kCR, kCP, 16, BPS>(f->ref0[2], tc0, L0.chroma[1], lane);
-            stage<4, kCR, kCP, 16, BPS>(f->ref1[1], tc1, L1.chroma[0], lane);
-            stage<4, kCR, kCP, 16, BPS>(f->ref1[2], tc1, L1.chroma[1], lane);
+        if (hc) {
+            fetch<4, kCR, kCP, 16, BPS>(f->ref0[1], tc0, wc[0], lane);
+            fetch<4, kCR, kCP, 16, BPS>(f->ref0[2], tc0, wc[1], lane);
+            fetch<4, kCR, kCP, 16, BPS>(f->ref1[1], tc1, wc[2], lane);
+            fetch<4, kCR, kCP, 16, BPS>(f->ref1[2], tc1, wc[3], lane);
+        }
+        if (hl) {
+            put<kLR, kLP, 32>(wl0, L0.luma, lane);
+            put<kLR, kLP, 32>(wl1, L1.luma, lane);
+        }
+        if (hc) {
+            put<kCR, kCP, 16>(wc[0], L0.chroma[0], lane);
+            put<kCR, kCP, 16>(wc[1], L0.chroma[1], lane);
+            put<kCR, kCP, 16>(wc[2], L1.chroma[0], lane);
+            put<kCR, kCP, 16>(wc[3], L1.chroma[1], lane);
         }
     }
     __syncthreads();
     const int head_room = 14 - bd;
     const int shift1 = 6 - head_room;
     if (active && tl0.tw > 0 && tl0.th > 0) {
-        hpass<8, kLP, 8>(tl0, L0.luma, L0.ltmp, shift1, lane);
-        hpass<8, kLP, 8>(tl1, L1.luma, L1.ltmp, shift1, lane);
+        hpass<8, kLP, 8, kLR, 1, 0, 0>(tl0, L0.luma, L0.ltmp, shift1, lane);
+        hpass<8, kLP, 8, kLR, 1, 0, 0>(tl1, L1.luma, L1.ltmp, shift1, lane);
     }
     if (active && tc0.tw > 0 && tc0.th > 0) {
-        hpass<4, kCP, 4>(tc0, L0.chroma[0], L0.ctmp[0], shift1, lane);
-        hpass<4, kCP, 4>(tc0, L0.chroma[1], L0.ctmp[1], shift1, lane);
-        hpass<4, kCP, 4>(tc1, L1.chroma[0], L1.ctmp[0], shift1, lane);
-        hpass<4, kCP, 4>(tc1, L1.chroma[1], L1.ctmp[1], shift1, lane);
+        hpass<4, kCP, 4, kCR, 2, kCR * kCP, 8 * kCP>(tc0, L0.chroma[0], L0.ctmp[0], shift1, lane);
+        hpass<4, kCP, 4, kCR, 2, kCR * kCP, 8 * kCP>(tc1, L1.chroma[0], L1.ctmp[0], shift1, lane);
     }
     __syncthreads();
     if (!active) return;
     const int maxv = (1 << bd) - 1, w0 = 8 - w1;
-    if (tl0.tw > 0 && tl0.th > 0)
-        bi_output<8, kLP, 16, BPS>(f->dst[0], tl0, tl1, L0.luma, L0.ltmp, L1.luma, L1.ltmp, head_room, w0, w1, maxv, lane);
-    if (tc0.tw > 0 && tc0.th > 0) {
-        for (int c = 0; c < 2; c++)
-            bi_output<4, kCP, 8, BPS>(f->dst[1 + c], tc0, tc1, L0.chroma[c], L0.ctmp[c], L1.chroma[c], L1.ctmp[c], head_room,
-                                 w0, w1, maxv, lane);
+    if (tl0.tw > 0 && tl0.th > 0) {
+        const int x = 2 * (lane & 7), y = 2 * (lane >> 3);
+        if (x < tl0.tw && y < tl0.th)
+            bi_output_quad<8, kLP, BPS>(f->dst[0], tl0, tl1, L0.luma, L0.ltmp, L1.luma, L1.ltmp, x, y, head_room, w0,
+                                        w1, maxv);
+    }
+    if (tc0.tw > 0 && tc0.th > 0 && lane < 32) {
+        const int pl = lane >> 4, x = 2 * (lane & 3), y = 2 * ((lane >> 2) & 3);
+        if (x < tc0.tw && y < tc0.th)
+            bi_output_quad<4, kCP, BPS>(f->dst[1 + pl], tc0, tc1, L0.chroma[pl], L0.ctmp[pl], L1.chroma[pl],
+                                        L1.ctmp[pl], x, y, head_room, w0, w1, maxv);
     }
 }
 
