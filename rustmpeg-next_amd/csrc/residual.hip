@@ -121,6 +121,22 @@ __device__ __forceinline__ void recon_add(const ReconDst& d, int x, int y, int r
     }
 }
 
+// The prediction sample (x, y) of a recon target, and pred + res clipped to
+// [0, maxv] stored back: the add of recon_add split so that a lane can fetch
+// all its prediction samples before the first store.
+__device__ __forceinline__ int recon_load(const ReconDst& d, int x, int y) {
+    const size_t o = (size_t)y * d.stride + x;
+    return d.bps == 1 ? (int)gsrc<uint8_t>(d.base)[o] : (int)gsrc<uint16_t>(d.base)[o];
+}
+__device__ __forceinline__ void recon_store(const ReconDst& d, int x, int y, int v) {
+    const size_t o = (size_t)y * d.stride + x;
+    v = v < 0 ? 0 : v > d.maxv ? d.maxv : v;
+    if (d.bps == 1)
+        gdst<uint8_t>(d.base)[o] = (uint8_t)v;
+    else
+        gdst<uint16_t>(d.base)[o] = (uint16_t)v;
+}
+
 __global__ __launch_bounds__(64) void residual_kernel(ResidualArgs a) {
     __shared__ __align__(16) uint32_t ct[32 * kPitch];  // coefficients: column i, int16 pairs along row k
     __shared__ __align__(16) uint32_t tt[64 * kPitch];  // intermediates: row y, int16 pairs along column i
@@ -519,6 +535,10 @@ __device__ __forceinline__ void load_col16(const uint32_t* __restrict__ src, uin
     }
 }
 
+// Rows per prediction-fetch batch of a recon lane (phase C): 8 keeps the
+// kernel at 6 waves per SIMD, where 16 would need 128 VGPRs.
+constexpr int kReconBatch = 8;
+
 template <int LW, int LH, bool RECON>
 __device__ __forceinline__ void tile_body(const ResidualArgs& a, const TileItem& it, uint32_t* smem) {
     constexpr int W = 1 << LW, H = 1 << LH;
@@ -673,11 +693,20 @@ __device__ __forceinline__ void tile_body(const ResidualArgs& a, const TileItem&
                     const ReconDst rd{p.base, p.stride, p.bpsmax & 255, p.bpsmax >> 8};
                     const uint2* src = reinterpret_cast<const uint2*>(T + g * H * TP);
                     if (RECON) {
-#pragma unroll 2
-                        for (int j = 0; j < H / NR; j++) {
-                            const int y = r0 + j * NR;
-                            const int acc = dot_groups<NQ>(src + y * (TP / 2), m, rnd);
-                            recon_add(rd, x, y, min(max(acc >> shift2, -32768), 32767));
+                        // H / NR <= 16 rows per lane, in batches of RB: the
+                        // prediction samples of a batch are requested before its first add
+                        constexpr int NY = H / NR, RB = NY < kReconBatch ? NY : kReconBatch;
+#pragma unroll
+                        for (int j0 = 0; j0 < NY; j0 += RB) {
+                            int pred[RB];
+#pragma unroll
+                            for (int j = 0; j < RB; j++) pred[j] = recon_load(rd, x, r0 + (j0 + j) * NR);
+#pragma unroll
+                            for (int j = 0; j < RB; j++) {
+                                const int y = r0 + (j0 + j) * NR;
+                                const int acc = dot_groups<NQ>(src + y * (TP / 2), m, rnd);
+                                recon_store(rd, x, y, pred[j] + min(max(acc >> shift2, -32768), 32767));
+                            }
                         }
                     } else {
                         stage2_col<NQ, H, NR, W, TP / 2>(src, O + g * H * W + x, m, r0, rnd, shift2);
