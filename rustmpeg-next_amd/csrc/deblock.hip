@@ -488,28 +488,37 @@ __device__ __forceinline__ uint8_t slot_edge(const TileCtx& t, int dir, int s, i
     return (e & RMH_EDGE_BS_MASK) ? e : 0;
 }
 
-// Lane exchanges inside a quad (lanes 4m .. 4m + 3) and a pair (2m, 2m + 1)
-// of a wave; every lane of the quad or pair must be active.
-__device__ __forceinline__ int quad_lane0(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x00, 0xf, 0xf, false); }
-__device__ __forceinline__ int quad_lane3(int v) { return __builtin_amdgcn_update_dpp(0, v, 0xff, 0xf, 0xf, false); }
+// Lane exchange inside a pair (lanes 2m, 2m + 1) of a wave; both lanes must
+// be active.
 __device__ __forceinline__ int pair_other(int v) { return __builtin_amdgcn_update_dpp(0, v, 0xb1, 0xf, 0xf, false); }
 
-// Line li (0..3) of one luma segment of the LDS window (dir 0: vertical edge
-// at x, row y + li; dir 1: horizontal edge at y, column x + li, clamped into
-// the picture as the reference's gather clamps it). The four lanes of a quad
-// hold the four lines of a segment: lines 0 and 3, which the decision reads,
-// come from the quad's lanes 0 and 3. A line whose column repeats the
-// previous one gives the same result and is not written twice.
-__device__ __forceinline__ void luma_line(const TileCtx& t, int dir, int x, int y, uint8_t e, int li, int16_t* lw) {
-    const rmh_dbk_frame& f = *t.f;
-    const int bs = e & RMH_EDGE_BS_MASK;
-    const int len_p = 2 * ((e >> RMH_EDGE_LENP_SHIFT) & 3) + 1, len_q = 2 * ((e >> RMH_EDGE_LENQ_SHIFT) & 3) + 1;
-    int16_t* base;  // element k of the line at base[k * ks]
-    int ks;
-    int line[16];
+// Slot -> (plane, x, y) without the edge-map read.
+__device__ __forceinline__ void slot_pos(const TileCtx& t, int dir, int s, int& plane, int& x, int& y) {
+    const int nl = dir == 0 ? kLVX * kLVY : kLHX * kLHY;
+    const int cx = dir == 0 ? kCVX : kCHX, cy = dir == 0 ? kCVY : kCHY;
+    if (s < nl) {
+        plane = 0;
+        x = dir == 0 ? t.X0 + 4 * (s % kLVX) : t.X0 + 4 * (s % kLHX);
+        y = dir == 0 ? t.Y0 - 8 + 4 * (s / kLVX) : t.Y0 + 4 * (s / kLHX);
+        return;
+    }
+    const int u = s - nl, c = u / (cx * cy), v = u % (cx * cy);
+    plane = 1 + c;
+    x = dir == 0 ? t.CX0 + 2 * (v % kCVX) : t.CX0 + 2 * (v % kCHX);
+    y = dir == 0 ? t.CY0 - 4 + 2 * (v / kCVX) : t.CY0 + 2 * (v / kCHX);
+}
+
+// Line li (0..3) of one luma segment of the LDS window: dir 0 (vertical edge
+// at x) row y + li, element k (p7 .. q7) at base[k], base 8-byte aligned;
+// dir 1 (horizontal edge at y) column x + li clamped into the picture, as the
+// reference's gather clamps it, element k at base[k * kLWW].
+__device__ __forceinline__ int16_t* luma_line_base(const TileCtx& t, int dir, int x, int y, int li, int16_t* lw) {
+    return dir == 0 ? lw + (y + li - (t.Y0 - 8)) * kLWW + (x - t.X0)
+                    : lw + (y - t.Y0) * kLWW + 8 + (min(x + li, t.W - 1) - t.X0);
+}
+
+__device__ __forceinline__ void read_luma_line(int dir, const int16_t* base, int (&line)[16]) {
     if (dir == 0) {
-        base = lw + (y + li - (t.Y0 - 8)) * kLWW + (x - t.X0);  // column x - 8, 8-byte aligned
-        ks = 1;
         const uint2* b8 = reinterpret_cast<const uint2*>(base);
 #pragma unroll
         for (int k = 0; k < 4; k++) {
@@ -520,22 +529,68 @@ __device__ __forceinline__ void luma_line(const TileCtx& t, int dir, int x, int 
             line[4 * k + 3] = (int)(v.y >> 16);
         }
     } else {
-        base = lw + (y - t.Y0) * kLWW + 8 + (min(x + li, t.W - 1) - t.X0);  // row y - 8
-        ks = kLWW;
 #pragma unroll
         for (int k = 0; k < 16; k++) line[k] = base[k * kLWW];
     }
+}
+
+// A luma segment's decision in one dword: slot (bits 0-7), lp (8-10), lq
+// (11-13), dep (14), deq (15), tc (16-27). The filter form (weak, strong
+// short, long) is the queue the segment sits in.
+__device__ __forceinline__ uint32_t pack_decision(int slot, const LumaDecision& d) {
+    return (uint32_t)slot | ((uint32_t)d.lp << 8) | ((uint32_t)d.lq << 11) | ((uint32_t)d.dep << 14) |
+           ((uint32_t)d.deq << 15) | ((uint32_t)d.tc << 16);
+}
+__device__ __forceinline__ LumaDecision unpack_decision(uint32_t v, int de) {
+    LumaDecision d;
+    d.de = de;
+    d.lp = (v >> 8) & 7;
+    d.lq = (v >> 11) & 7;
+    d.dep = (v >> 14) & 1;
+    d.deq = (v >> 15) & 1;
+    d.tc = (v >> 16) & 0xfff;
+    return d;
+}
+
+constexpr int kLQ = kLVX * kLVY;  // luma segments per pass, at most
+
+// Decision of one luma segment (one lane per segment) from its lines 0 and
+// 3; a filtered segment goes to the queue of its filter form, qd[de - 1].
+__device__ __forceinline__ void luma_segment_decide(const TileCtx& t, int dir, uint32_t q, const int16_t* lw,
+                                                    uint32_t (*qd)[kLQ], int* counts) {
+    const rmh_dbk_frame& f = *t.f;
+    int plane, x, y;
+    const int slot = (int)(q & 0xffff);
+    slot_pos(t, dir, slot, plane, x, y);
+    const uint8_t e = (uint8_t)(q >> 16);
+    const int bs = e & RMH_EDGE_BS_MASK;
+    const int len_p = 2 * ((e >> RMH_EDGE_LENP_SHIFT) & 3) + 1, len_q = 2 * ((e >> RMH_EDGE_LENQ_SHIFT) & 3) + 1;
     int l0[16], l3[16];
-#pragma unroll
-    for (int k = 0; k < 16; k++) {
-        l0[k] = quad_lane0(line[k]);
-        l3[k] = quad_lane3(line[k]);
-    }
+    read_luma_line(dir, luma_line_base(t, dir, x, y, 0, const_cast<int16_t*>(lw)), l0);
+    read_luma_line(dir, luma_line_base(t, dir, x, y, 3, const_cast<int16_t*>(lw)), l3);
     LumaDecision d;
     if (!luma_decide(l0, l3, f.qp[0], bs, f.beta_offset_div2[0], f.tc_offset_div2[0], t.bd, len_p, len_q,
                      dir == 1 && (e & RMH_EDGE_CTB_ROW) != 0, d))
         return;
+    const uint32_t v = pack_decision(slot, d);
+    // one queue address per atomic, so that each is one atomic per wave
+    if (d.de == 1) qd[0][atomicAdd(&counts[2], 1)] = v;
+    if (d.de == 2) qd[1][atomicAdd(&counts[3], 1)] = v;
+    if (d.de == 3) qd[2][atomicAdd(&counts[4], 1)] = v;
+}
+
+// Line li (0..3) of a luma segment with its decision: filtered and written
+// back. A line whose column repeats the previous one (clamped at the right
+// picture edge) gives the same result and is not written twice.
+__device__ __forceinline__ void luma_segment_filter(const TileCtx& t, int dir, uint32_t v, int de, int li, int16_t* lw) {
+    int plane, x, y;
+    slot_pos(t, dir, (int)(v & 0xff), plane, x, y);
     if (dir == 1 && li > 0 && x + li > t.W - 1) return;
+    const LumaDecision d = unpack_decision(v, de);
+    int16_t* base = luma_line_base(t, dir, x, y, li, lw);
+    const int ks = dir == 0 ? 1 : kLWW;
+    int line[16];
+    read_luma_line(dir, base, line);
     const int np = luma_np(d), nq = luma_nq(d), maxv = (1 << t.bd) - 1;
     luma_filter_line(line, d, maxv);
 #pragma unroll
@@ -569,32 +624,17 @@ __device__ __forceinline__ void chroma_line_lds(const TileCtx& t, int dir, int p
     for (int k = 1; k < 7; k++) base[k * ks] = (int16_t)m[k];
 }
 
-// One pass (dir 0 vertical, 1 horizontal) over the tile: the slots with an
-// edge go to two LDS queues (luma, chroma), then each queue is filtered with
-// lanes packed densely, so a wave runs a filter only for segments that exist.
-// Slot -> (plane, x, y) without the edge-map read.
-__device__ __forceinline__ void slot_pos(const TileCtx& t, int dir, int s, int& plane, int& x, int& y) {
-    const int nl = dir == 0 ? kLVX * kLVY : kLHX * kLHY;
-    const int cx = dir == 0 ? kCVX : kCHX, cy = dir == 0 ? kCVY : kCHY;
-    if (s < nl) {
-        plane = 0;
-        x = dir == 0 ? t.X0 + 4 * (s % kLVX) : t.X0 + 4 * (s % kLHX);
-        y = dir == 0 ? t.Y0 - 8 + 4 * (s / kLVX) : t.Y0 + 4 * (s / kLHX);
-        return;
-    }
-    const int u = s - nl, c = u / (cx * cy), v = u % (cx * cy);
-    plane = 1 + c;
-    x = dir == 0 ? t.CX0 + 2 * (v % kCVX) : t.CX0 + 2 * (v % kCHX);
-    y = dir == 0 ? t.CY0 - 4 + 2 * (v / kCVX) : t.CY0 + 2 * (v / kCHX);
-}
-
-// Queue entries: slot | edge byte << 16.
+// Queue entries: slot | edge byte << 16. Luma runs in two steps: decisions
+// with one lane per segment (into one queue per filter form), then the
+// filters with one lane per line, form by form, so that the lanes of a wave
+// run the same filter. Chroma segments (two lanes each) are filtered during
+// the luma decisions.
 __device__ __forceinline__ void tile_pass(const TileCtx& t, int dir, int16_t* lw, int16_t (*cw)[kCWH * kCWW],
-                                          uint32_t* ql, uint32_t* qc, int* counts) {
+                                          uint32_t* ql, uint32_t* qc, uint32_t (*qd)[kLQ], int* counts) {
     constexpr int NS = (kMaxQ + 255) / 256;
     const int nslots = dir == 0 ? kNV : kNH;
     const int nl = dir == 0 ? kLVX * kLVY : kLHX * kLHY;
-    if (threadIdx.x < 2) counts[threadIdx.x] = 0;
+    if (threadIdx.x < 5) counts[threadIdx.x] = 0;
     __syncthreads();
     uint8_t eb[NS];
 #pragma unroll
@@ -615,20 +655,20 @@ __device__ __forceinline__ void tile_pass(const TileCtx& t, int dir, int16_t* lw
         }
     }
     __syncthreads();
-    // one lane per line: quads of lanes per luma segment, pairs per chroma
-    // segment (256 is a multiple of 4, so a quad or pair is active as a whole)
     const int n_l = counts[0], n_c = counts[1];
-    for (int i = threadIdx.x; i < 4 * n_l; i += 256) {
-        int plane, x, y;
-        const uint32_t q = ql[i >> 2];
-        slot_pos(t, dir, (int)(q & 0xffff), plane, x, y);
-        luma_line(t, dir, x, y, (uint8_t)(q >> 16), i & 3, lw);
-    }
+    for (int i = threadIdx.x; i < n_l; i += 256) luma_segment_decide(t, dir, ql[i], lw, qd, counts);
+    // pairs of lanes per chroma segment (256 is even, so a pair is active as a whole)
     for (int i = threadIdx.x; i < 2 * n_c; i += 256) {
         int plane, x, y;
         const uint32_t q = qc[i >> 1];
         slot_pos(t, dir, (int)(q & 0xffff), plane, x, y);
         chroma_line_lds(t, dir, plane, x, y, (uint8_t)(q >> 16), i & 1, cw[plane - 1]);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const int n = counts[2 + k];
+        for (int i = threadIdx.x; i < 4 * n; i += 256) luma_segment_filter(t, dir, qd[k][i >> 2], k + 1, i & 3, lw);
     }
     __syncthreads();
 }
@@ -648,8 +688,9 @@ __global__ __launch_bounds__(256) void deblock_tile_kernel(const rmh_dbk_frame* 
     __shared__ __align__(16) int16_t lw[kLWH * kLWW];
     __shared__ __align__(16) int16_t cwin[2][kCWH * kCWW];
     __shared__ uint32_t ql[kMaxQ], qc[kMaxQ];
+    __shared__ uint32_t qd[3][kLQ];
     __shared__ uint32_t cells[kEY * kEX];
-    __shared__ int counts[2];
+    __shared__ int counts[5];
     const uint32_t tile = first_tile + xcd_local_block(blockIdx.x, gridDim.x);
     uint32_t lo = 0, hi = n_frames;  // tile_start[lo] <= tile < tile_start[hi]
     while (hi - lo > 1) {
@@ -716,7 +757,7 @@ __global__ __launch_bounds__(256) void deblock_tile_kernel(const rmh_dbk_frame* 
     }
     __syncthreads();
     // 2. vertical edges
-    tile_pass(t, 0, lw, cwin, ql, qc, counts);
+    tile_pass(t, 0, lw, cwin, ql, qc, qd, counts);
     // 3. rows outside the picture read as the nearest (filtered) picture row
     if (t.Y0 - 8 < 0 || t.Y0 + kTH + 8 > t.H) refill_rows<kLWW, kLWH>(t.Y0 - 8, t.H, lw);
     if (t.CY0 - 4 < 0 || t.CY0 + kCTH + 4 > t.CH) {
@@ -725,7 +766,7 @@ __global__ __launch_bounds__(256) void deblock_tile_kernel(const rmh_dbk_frame* 
     }
     __syncthreads();
     // 4. horizontal edges
-    tile_pass(t, 1, lw, cwin, ql, qc, counts);
+    tile_pass(t, 1, lw, cwin, ql, qc, qd, counts);
     // 5. write the tile back (picture samples only)
     // in 8-sample chunks: 256 luma chunks (8 per row), then 2 x 64 chroma chunks (4 per row)
     {
