@@ -552,7 +552,8 @@ __device__ __forceinline__ LumaDecision unpack_decision(uint32_t v, int de) {
     return d;
 }
 
-constexpr int kLQ = kLVX * kLVY;  // luma segments per pass, at most
+constexpr int kLQ = kLVX * kLVY;      // luma segments per pass, at most
+constexpr int kCQ = 2 * kCVX * kCVY;  // chroma segments per pass, at most
 
 // Decision of one luma segment (one lane per segment) from its lines 0 and
 // 3; a filtered segment goes to the queue of its filter form, qd[de - 1].
@@ -687,7 +688,7 @@ __global__ __launch_bounds__(256) void deblock_tile_kernel(const rmh_dbk_frame* 
                                                            uint32_t first_tile) {
     __shared__ __align__(16) int16_t lw[kLWH * kLWW];
     __shared__ __align__(16) int16_t cwin[2][kCWH * kCWW];
-    __shared__ uint32_t ql[kMaxQ], qc[kMaxQ];
+    __shared__ uint32_t ql[kLQ], qc[kCQ];
     __shared__ uint32_t qd[3][kLQ];
     __shared__ uint32_t cells[kEY * kEX];
     __shared__ int counts[5];
