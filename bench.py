@@ -406,7 +406,7 @@ def main():
         return 0
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the residual stage shards by "
-              "stream with no exchange; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+              "stream with no exchange). Run with --gpus 1.", file=sys.stderr)
         return 2
 
     sys.path.insert(0, os.path.join(REPO, "tests"))
