@@ -98,3 +98,40 @@ def test_untouched_output_outside_tus_and_on_error(acc, rmh):
     with pytest.raises(rmh.Execution):
         acc.residuals(tus, c2, out)
     assert (out == sentinel).all()
+
+
+@pytest.mark.parametrize("seed", [21, 22])
+def test_large_tile_classes_extreme_values(acc, po, seed):
+    """Tile-eligible TUs of every class with both sides >= 16 (the classes the
+    tiled kernel transforms on the matrix cores), every transform pair, bit
+    depths 8-12 and QPs 0-63: full-range, saturated and sparse coefficients,
+    so that the split of int16 operands into int8 halves meets its extremes
+    (-32768, 32767, intermediates clipped in the first stage)."""
+    print("residual large classes seed %d" % seed, flush=True)
+    rng = np.random.default_rng(seed)
+    descs, blocks, off = [], [], 0
+    for lw in range(4, 7):
+        for lh in range(4, 7):
+            w, h = 1 << lw, 1 << lh
+            for i in range(24):
+                t = np.zeros(1, W.TU_DTYPE)[0]
+                t["offset"], t["log2_w"], t["log2_h"] = off, lw, lh
+                t["bit_depth"] = int(rng.choice([8, 10, 12]))
+                t["qp"] = int(rng.integers(0, 64 + 6 * (t["bit_depth"] - 8)))
+                ok_h = [0] + ([1, 2] if w <= 32 else [])
+                ok_v = [0] + ([1, 2] if h <= 32 else [])
+                t["tr_hor"], t["tr_ver"] = int(rng.choice(ok_h)), int(rng.choice(ok_v))
+                kind = i % 4
+                if kind == 0:
+                    c = rng.integers(-32768, 32768, w * h)
+                elif kind == 1:
+                    c = rng.choice([-32768, 32767], w * h)
+                elif kind == 2:
+                    c = np.rint(rng.laplace(0, 2000, w * h))
+                else:
+                    c = np.where(rng.random(w * h) < 0.05, rng.integers(-32768, 32768, w * h), 0)
+                descs.append(t)
+                blocks.append(np.clip(c, -32768, 32767).astype(np.int16))
+                off += w * h
+    tus = np.array(descs, dtype=W.TU_DTYPE)
+    _check(acc, po, tus, np.concatenate(blocks))
