@@ -704,28 +704,42 @@ __device__ __forceinline__ void tile_body(const ResidualArgs& a, const TileItem&
     // whole Hc x Wc region is written (zeros outside the coded region), so
     // rows and columns past the tile bounding box read as zero later.
     {
-        constexpr int Q = Wc / 4, RP = Hc / 2;
+        // a fixed number of items per lane (at most 2); every load of the
+        // lane is issued before the first dequantisation
+        constexpr int Q = Wc / 4, RP = Hc / 2, NA = (GMAX * RP * Q + 255) / 256;
         const int total = G * RP * Q;
         int mr = -1, mc = -1;
-        for (int e = tid; e < total; e += 256) {
-            const int x0 = (e % Q) * 4;
-            const int rp = (e / Q) % RP;
-            const int g = e / (Q * RP);
-            const uint32_t off = P[g].off;
-            const int cwch = P[g].cwch, S = P[g].scale, rs = P[g].shifts & 31;
-            const int cw = cwch & 255, ch = cwch >> 8, rnd = (1 << rs) >> 1;
-            const int r0 = 2 * rp;
-            uint64_t raw0 = 0, raw1 = 0;
-            if (x0 < cw) {
-                if (r0 < ch) raw0 = *reinterpret_cast<const uint64_t*>(a.coeff + (off + (uint32_t)(r0 * W + x0)));
-                if (r0 + 1 < ch) raw1 = *reinterpret_cast<const uint64_t*>(a.coeff + (off + (uint32_t)((r0 + 1) * W + x0)));
-                // columns past the coded region read as zero levels
-                const int nv = cw - x0;
-                if (nv < 4) {
-                    const uint64_t keep = (1ull << (16 * nv)) - 1;
-                    raw0 &= keep;
-                    raw1 &= keep;
+        uint64_t raw[NA][2];
+#pragma unroll
+        for (int i = 0; i < NA; i++) {
+            const int e = tid + 256 * i;
+            raw[i][0] = raw[i][1] = 0;
+            if (e < total) {
+                const int x0 = (e % Q) * 4, rp = (e / Q) % RP, g = e / (Q * RP);
+                const uint32_t off = P[g].off;
+                const int cwch = P[g].cwch, cw = cwch & 255, ch = cwch >> 8, r0 = 2 * rp;
+                if (x0 < cw) {
+                    if (r0 < ch) raw[i][0] = *reinterpret_cast<const uint64_t*>(a.coeff + (off + (uint32_t)(r0 * W + x0)));
+                    if (r0 + 1 < ch)
+                        raw[i][1] = *reinterpret_cast<const uint64_t*>(a.coeff + (off + (uint32_t)((r0 + 1) * W + x0)));
                 }
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < NA; i++) {
+            const int e = tid + 256 * i;
+            if (e >= total) continue;
+            const int x0 = (e % Q) * 4, rp = (e / Q) % RP, g = e / (Q * RP);
+            const int cwch = P[g].cwch, S = P[g].scale, rs = P[g].shifts & 31;
+            const int cw = cwch & 255, rnd = (1 << rs) >> 1;
+            const int r0 = 2 * rp;
+            uint64_t raw0 = raw[i][0], raw1 = raw[i][1];
+            // columns past the coded region read as zero levels
+            const int nv = cw - x0;
+            if (nv < 4) {
+                const uint64_t keep = nv > 0 ? (1ull << (16 * nv)) - 1 : 0;
+                raw0 &= keep;
+                raw1 &= keep;
             }
             // bounding box of the non-zero levels (a zero level dequantises to zero)
             const uint64_t z = raw0 | raw1;
