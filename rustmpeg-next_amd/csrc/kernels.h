@@ -13,12 +13,6 @@ struct DeviceTables {
     const uint32_t* mat = nullptr;   // transform matrices as int16 pairs: [kk][j] = (M[2kk][j], M[2kk+1][j])
     const uint32_t* col16 = nullptr; // per column j: 16 pairs (M[2kk][j], M[2kk+1][j]), zero past the last row
     int32_t col16_off[3][7];         // dword offset of (type, log2 n); -1 if unsupported
-    // The same columns as int8 for the tiled kernel's matrix-core path: column
-    // j at col16 dword offset D is col8[D / 4 .. + 4), eight bytes
-    // M[8q .. 8q + 7][j] each (k < 32, zero past the last row), and
-    // col_corr[D / 16] = 128 * sum over k < 32 of M[k][j].
-    const uint64_t* col8 = nullptr;
-    const int32_t* col_corr = nullptr;
     const int8_t* lfnst8 = nullptr;  // [4][2][16][48]
     const int8_t* lfnst4 = nullptr;  // [4][2][16][16]
     const uint16_t* scans = nullptr; // grouped diagonal scans, raster positions
@@ -57,8 +51,6 @@ struct ResidualArgs {
     const uint32_t* order;  // optional TU index list (nullptr: descriptor order)
     const uint32_t* col16;  // column-major padded matrix pairs (tiled kernel)
     int32_t col16_off[3][7];
-    const uint64_t* col8;     // int8 matrix columns and their sums (tiled kernel, matrix cores)
-    const int32_t* col_corr;
     const rmh_tu_place* place;         // recon mode when non-null: add into frames' planes
     const rmh_recon_frame* frames;
     const TileTu* tile_tus;            // tiled kernel: per TU of order[0 .. n_fast)

@@ -535,120 +535,6 @@ __device__ __forceinline__ void load_col16(const uint32_t* __restrict__ src, uin
     }
 }
 
-// ---------------------------------------------------------------------------
-// Matrix-core transform for tiles of TUs at least 16 wide and 16 tall. Both
-// stages are int8 products on v_mfma_i32_16x16x32_i8, exact in int32: an int16
-// operand c is split as c = 256 hi + lo' + 128 with hi = c >> 8 and
-// lo' = (c & 255) - 128 (both int8), so that
-//   sum_k c_k m_k = 256 sum_k hi_k m_k + sum_k lo'_k m_k + 128 sum_k m_k,
-// the last term a per-column constant (col_corr). Every |m| <= 90 and k < 32,
-// so no partial sum leaves int32. Lane layout of the instruction (A 16x32,
-// B 32x16, D 16x16): lane l holds A[l % 16][8 (l / 16) .. + 8],
-// B[8 (l / 16) .. + 8][l % 16] and D[4 (l / 16) + j][l % 16], j = 0 .. 3.
-//
-// A unit is (TU g, 16-row block yb); a wave runs whole units:
-//   stage 1, transposed: D1[x][y] = sum_k C[k][x] Mv[k][y] (A: coefficient
-//            columns from LDS, B: Mv columns), one MFMA pair per 16 coded
-//            columns; clip16((D1 + 64) >> 7) goes to the unit's intermediate
-//            rows in LDS (row-major, 4 samples per lane);
-//   stage 2, transposed: D2[x][y] = sum_k Mh[k][x] Inter[y][k] (A: Mh
-//            columns, B: 8 intermediates of row y per lane), one MFMA pair per
-//            16 output columns; clip16((D2 + rnd) >> shift2) are 4 consecutive
-//            residuals of row y per lane, stored with one 8-byte store.
-// Units touch disjoint intermediate rows, so the stages need no workgroup
-// barrier; the wave's own LDS writes are ordered before its reads.
-// ---------------------------------------------------------------------------
-typedef int v4i __attribute__((ext_vector_type(4)));
-
-// High bytes and biased low bytes (lo ^ 0x80 = lo - 128 as int8) of the four
-// int16 values in dwords d0 (values 0, 1) and d1 (values 2, 3).
-__device__ __forceinline__ void split4(uint32_t d0, uint32_t d1, uint32_t& hi, uint32_t& lo) {
-    hi = __builtin_amdgcn_perm(d1, d0, 0x07050301u);
-    lo = __builtin_amdgcn_perm(d1, d0, 0x06040200u) ^ 0x80808080u;
-}
-
-// Eight int16 values (four dwords, 8-byte aligned) split into int8 operands.
-__device__ __forceinline__ void split8(const uint32_t* src, int64_t& hi, int64_t& lo) {
-    const uint2 a = reinterpret_cast<const uint2*>(src)[0], b = reinterpret_cast<const uint2*>(src)[1];
-    uint32_t h0, l0, h1, l1;
-    split4(a.x, a.y, h0, l0);
-    split4(b.x, b.y, h1, l1);
-    hi = (int64_t)((uint64_t)h0 | ((uint64_t)h1 << 32));
-    lo = (int64_t)((uint64_t)l0 | ((uint64_t)l1 << 32));
-}
-
-// sum_k c_k m_k for the D elements of one split pair of products.
-__device__ __forceinline__ v4i mfma_split(int64_t hi, int64_t lo, int64_t m, bool m_is_a) {
-    const v4i z = {0, 0, 0, 0};
-    const v4i h = m_is_a ? __builtin_amdgcn_mfma_i32_16x16x32_i8(m, hi, z, 0, 0, 0)
-                         : __builtin_amdgcn_mfma_i32_16x16x32_i8(hi, m, z, 0, 0, 0);
-    const v4i l = m_is_a ? __builtin_amdgcn_mfma_i32_16x16x32_i8(m, lo, z, 0, 0, 0)
-                         : __builtin_amdgcn_mfma_i32_16x16x32_i8(lo, m, z, 0, 0, 0);
-    v4i r;
-#pragma unroll
-    for (int j = 0; j < 4; j++) r[j] = h[j] * 256 + l[j];
-    return r;
-}
-
-__device__ __forceinline__ void wave_lds_order() {
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-}
-
-template <int LW, int LH>
-__device__ __forceinline__ void tile_transform_mfma(const ResidualArgs& a, const TuParams* P, const uint32_t* C,
-                                                    uint32_t* T, int G) {
-    constexpr int W = 1 << LW, H = 1 << LH;
-    constexpr int Wc = W < 32 ? W : 32, Hc = H < 32 ? H : 32;
-    constexpr int CP = residual_tile_pitch(LH < 5 ? LH : 5), TP = residual_tile_pitch(LW < 5 ? LW : 5);
-    constexpr int YB = H / 16;
-    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int r = lane & 15, q = lane >> 4;
-    for (int u = wave; u < G * YB; u += 4) {
-        const int g = u / YB, y0 = (u % YB) * 16;
-        const TuParams p = P[g];
-        const uint32_t* Cg = C + g * Wc * CP;
-        uint32_t* Tg = T + g * H * TP;
-        // stage 1: B = Mv columns y0 + r
-        const int64_t bv = (int64_t)a.col8[(p.mv >> 2) + (y0 + r) * 4 + q];
-        const int c1 = a.col_corr[(p.mv >> 4) + y0 + r] + 64;
-#pragma unroll
-        for (int xt = 0; xt < Wc / 16; xt++) {
-            int64_t hi, lo;
-            split8(Cg + (xt * 16 + r) * CP + 4 * q, hi, lo);
-            const v4i d = mfma_split(hi, lo, bv, false);
-            // D1[x = 16 xt + 4 q + j][y = y0 + r]
-            int v[4];
-#pragma unroll
-            for (int j = 0; j < 4; j++) v[j] = clip16i((d[j] + c1) >> 7);
-            *reinterpret_cast<uint2*>(Tg + (y0 + r) * TP + 8 * xt + 2 * q) = make_uint2(
-                (uint32_t)(uint16_t)v[0] | ((uint32_t)(uint16_t)v[1] << 16),
-                (uint32_t)(uint16_t)v[2] | ((uint32_t)(uint16_t)v[3] << 16));
-        }
-        wave_lds_order();
-        // stage 2: B = intermediates Inter[y0 + r][8 q .. + 8] (k >= Wc meets zero matrix rows)
-        int64_t hi, lo;
-        split8(Tg + (y0 + r) * TP + 4 * q, hi, lo);
-        const int shift2 = p.shifts >> 16, rnd = 1 << (shift2 - 1);
-        const int y = y0 + r;
-#pragma unroll
-        for (int xt = 0; xt < W / 16; xt++) {
-            const int64_t am = (int64_t)a.col8[(p.mh >> 2) + (xt * 16 + r) * 4 + q];
-            const v4i d = mfma_split(hi, lo, am, true);
-            // D2[x = 16 xt + 4 q + j][y]
-            const int x = xt * 16 + 4 * q;
-            const int* cc = a.col_corr + (p.mh >> 4) + x;
-            int v[4];
-#pragma unroll
-            for (int j = 0; j < 4; j++) v[j] = clip16i((d[j] + cc[j] + rnd) >> shift2);
-            *reinterpret_cast<uint2*>(a.residual + (p.off + (uint32_t)(y * W + x))) = make_uint2(
-                (uint32_t)(uint16_t)v[0] | ((uint32_t)(uint16_t)v[1] << 16),
-                (uint32_t)(uint16_t)v[2] | ((uint32_t)(uint16_t)v[3] << 16));
-        }
-        // the next unit of this wave writes other rows; no ordering needed
-    }
-}
-
 // Rows per prediction-fetch batch of a recon lane (phase C): 8 keeps the
 // kernel at 6 waves per SIMD, where 16 would need 128 VGPRs.
 constexpr int kReconBatch = 8;
@@ -764,14 +650,6 @@ __device__ __forceinline__ void tile_body(const ResidualArgs& a, const TileItem&
         }
     }
     __syncthreads();
-    if constexpr (LW >= 4 && LH >= 4 && !RECON) {
-        // both sides >= 16: both stages on the matrix cores, residuals stored
-        // directly. The recon add stays on the row-contiguous path below: the
-        // matrix-core layout gives a lane 4 samples of one of 16 rows, which
-        // makes the picture read-modify-write scattered (74 against 48 us).
-        tile_transform_mfma<LW, LH>(a, P, C, T, G);
-        return;
-    }
     const int er = __builtin_amdgcn_readfirstlane(TB[0]) + 1;
     const int ec = __builtin_amdgcn_readfirstlane(TB[1]) + 1;
     // the horizontal pass reads intermediates two pairs at a time: produce the

@@ -284,16 +284,6 @@ hipError_t build_tables(rmh_ctx* ctx) {
                     col16.push_back(lo | (hi << 16));
                 }
         }
-    // int8 columns (8 bytes per 8 rows) and 128 * column sums, from col16
-    std::vector<uint64_t> col8(col16.size() / 4, 0);
-    std::vector<int32_t> ccorr(col16.size() / 16, 0);
-    for (size_t j = 0; j < col16.size() / 16; j++)
-        for (int k = 0; k < 32; k++) {
-            const uint32_t w = col16[j * 16 + k / 2];
-            const int v = (int)(int16_t)(uint16_t)((k & 1) ? w >> 16 : w & 0xffff);
-            col8[j * 4 + k / 8] |= (uint64_t)(uint8_t)(int8_t)v << (8 * (k % 8));
-            ccorr[j] += 128 * v;
-        }
     std::vector<uint16_t> scans;
     for (int a = 0; a < 7; a++)
         for (int b = 0; b < 7; b++) {
@@ -306,16 +296,13 @@ hipError_t build_tables(rmh_ctx* ctx) {
         }
     size_t o_mat = 0, o_l8 = aligned(mat.size() * 4), o_l4 = o_l8 + aligned(sizeof(RMH_LFNST8));
     size_t o_sc = o_l4 + aligned(sizeof(RMH_LFNST4)), o_c16 = o_sc + aligned(scans.size() * 2);
-    size_t o_c8 = o_c16 + aligned(col16.size() * 4), o_cc = o_c8 + aligned(col8.size() * 8);
-    size_t total = o_cc + aligned(ccorr.size() * 4);
+    size_t total = o_c16 + aligned(col16.size() * 4);
     std::vector<uint8_t> blob(total, 0);
     memcpy(blob.data() + o_mat, mat.data(), mat.size() * 4);
     memcpy(blob.data() + o_l8, RMH_LFNST8, sizeof(RMH_LFNST8));
     memcpy(blob.data() + o_l4, RMH_LFNST4, sizeof(RMH_LFNST4));
     memcpy(blob.data() + o_sc, scans.data(), scans.size() * 2);
     memcpy(blob.data() + o_c16, col16.data(), col16.size() * 4);
-    memcpy(blob.data() + o_c8, col8.data(), col8.size() * 8);
-    memcpy(blob.data() + o_cc, ccorr.data(), ccorr.size() * 4);
     hipError_t e = hipMalloc(&ctx->d_tables, total);
     if (e != hipSuccess) return e;
     e = hipMemcpy(ctx->d_tables, blob.data(), total, hipMemcpyHostToDevice);
@@ -326,8 +313,6 @@ hipError_t build_tables(rmh_ctx* ctx) {
     ctx->tabs.lfnst4 = (const int8_t*)(base + o_l4);
     ctx->tabs.scans = (const uint16_t*)(base + o_sc);
     ctx->tabs.col16 = (const uint32_t*)(base + o_c16);
-    ctx->tabs.col8 = (const uint64_t*)(base + o_c8);
-    ctx->tabs.col_corr = (const int32_t*)(base + o_cc);
     return hipSuccess;
 }
 
@@ -821,8 +806,6 @@ rmh_status run_plan_locked_free(rmh_plan* plan, const int16_t* d_coeff, int16_t*
         memcpy(a.scan_off, plan->ctx->tabs.scan_off, sizeof a.scan_off);
         memcpy(a.scan_len, plan->ctx->tabs.scan_len, sizeof a.scan_len);
         a.col16 = plan->ctx->tabs.col16;
-        a.col8 = plan->ctx->tabs.col8;
-        a.col_corr = plan->ctx->tabs.col_corr;
         memcpy(a.col16_off, plan->ctx->tabs.col16_off, sizeof a.col16_off);
         a.place = plan->d_place;
         a.frames = plan->d_rframes;
@@ -1087,8 +1070,6 @@ RMH_EXPORT rmh_status rmh_residual_plan_autotune(rmh_plan* plan, const int16_t* 
     memcpy(a.scan_off, ctx->tabs.scan_off, sizeof a.scan_off);
     memcpy(a.scan_len, ctx->tabs.scan_len, sizeof a.scan_len);
     a.col16 = ctx->tabs.col16;
-    a.col8 = ctx->tabs.col8;
-    a.col_corr = ctx->tabs.col_corr;
     memcpy(a.col16_off, ctx->tabs.col16_off, sizeof a.col16_off);
     a.place = plan->d_place;
     a.frames = plan->d_rframes;

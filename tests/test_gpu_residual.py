@@ -102,11 +102,10 @@ def test_untouched_output_outside_tus_and_on_error(acc, rmh):
 
 @pytest.mark.parametrize("seed", [21, 22])
 def test_large_tile_classes_extreme_values(acc, po, seed):
-    """Tile-eligible TUs of every class with both sides >= 16 (the classes the
-    tiled kernel transforms on the matrix cores), every transform pair, bit
-    depths 8-12 and QPs 0-63: full-range, saturated and sparse coefficients,
-    so that the split of int16 operands into int8 halves meets its extremes
-    (-32768, 32767, intermediates clipped in the first stage)."""
+    """Tile-eligible TUs of every class with both sides >= 16, every transform
+    pair, bit depths 8-12 and QPs 0-63: full-range, saturated and sparse
+    coefficients, so that both stages meet their extremes (-32768, 32767,
+    intermediates clipped in the first stage, dequantisation saturating)."""
     print("residual large classes seed %d" % seed, flush=True)
     rng = np.random.default_rng(seed)
     descs, blocks, off = [], [], 0
