@@ -92,8 +92,7 @@ constexpr int residual_tile_pitch(int lc) { return lc <= 2 ? 2 : (1 << lc) / 2 +
 inline uint32_t residual_tile_lds(int lw, int lh) {
     uint32_t count = (uint32_t)residual_tile_tus(lw, lh);
     int lwc = lw < 5 ? lw : 5, lhc = lh < 5 ? lh : 5;
-    uint32_t coef = (1u << lwc) * residual_tile_pitch(lhc) * 4, out = (1u << (lw + lh)) * 2;
-    uint32_t c = count * (coef > out ? coef : out);                  // coefficient columns, then the output tile
+    uint32_t c = count * (1u << lwc) * residual_tile_pitch(lhc) * 4;  // coefficient columns
     uint32_t t = count * (1u << lh) * residual_tile_pitch(lwc) * 4;  // intermediate rows
     return c + t + count * 40 + 16;                                  // + per-TU parameters + tile bounding box
 }

@@ -39,6 +39,19 @@ __device__ __forceinline__ int dot2(uint32_t a, uint32_t b, int acc) {
     return __builtin_amdgcn_sdot2(__builtin_bit_cast(short2v, a), __builtin_bit_cast(short2v, b), acc, false);
 }
 
+// The first product of a chain: the clamping form is encoded with acc as a
+// separate operand, so no copy of the start value into the accumulator is
+// needed. It changes nothing: |acc| < 2^12 here and each product is below
+// 2^22 in magnitude.
+__device__ __forceinline__ int dot2_first(uint32_t a, uint32_t b, int acc) {
+    return __builtin_amdgcn_sdot2(__builtin_bit_cast(short2v, a), __builtin_bit_cast(short2v, b), acc, true);
+}
+
+// Two int32 values saturated to int16 and packed (lo = a, hi = b).
+__device__ __forceinline__ uint32_t pack_sat16(int a, int b) {
+    return __builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_pk_i16(a, b));
+}
+
 __device__ __forceinline__ int dequant_flat(int level, int scale, int rshift) {
     int64_t v = (int64_t)level * scale;
     if (rshift > 0) return clip16((v + ((int64_t)1 << (rshift - 1))) >> rshift);
@@ -470,38 +483,53 @@ __device__ __forceinline__ int dot_groups(const uint2* __restrict__ src, const u
     uint2 v[NQ > 0 ? NQ : 1];
 #pragma unroll
     for (int q = 0; q < NQ; q++) v[q] = src[q];
+    acc = dot2_first(v[0].x, m[0], acc);
+    acc = dot2(v[0].y, m[1], acc);
 #pragma unroll
-    for (int q = 0; q < NQ; q++) {
+    for (int q = 1; q < NQ; q++) {
         acc = dot2(v[q].x, m[2 * q], acc);
         acc = dot2(v[q].y, m[2 * q + 1], acc);
     }
     return acc;
 }
 
-// Stage-1 outputs of one lane: columns c0, c0 + NCH, ... (nc of them) of
-// one row, clip16((sum + 64) >> 7). src (coefficient columns) and dst (the
-// intermediate row) are disjoint LDS regions, which lets reads of later
-// columns be issued before earlier results are stored.
-template <int NQ, int NCH, int CSTRIDE>
-__device__ __forceinline__ void stage1_row(const uint2* __restrict__ src, int16_t* __restrict__ dst,
+// Stage-1 outputs of one lane: columns c0 .. c0 + nc - 1 (c0 and nc even)
+// of one row, clip16((sum + 64) >> 7), stored as int16 pairs. src
+// (coefficient columns) and dst (the intermediate row) are disjoint LDS
+// regions, which lets reads of later columns be issued before earlier
+// results are stored.
+template <int NQ, int CSTRIDE>
+__device__ __forceinline__ void stage1_row(const uint2* __restrict__ src, uint32_t* __restrict__ dst,
                                            const uint32_t (&m)[16], int c0, int nc) {
-    for (int j = 0; j < nc; j++) {
-        const int i = c0 + j * NCH;
-        dst[i] = (int16_t)clip16i(dot_groups<NQ>(src + i * CSTRIDE, m, 64) >> 7);
+    for (int j = 0; j < nc; j += 2) {
+        const int i = c0 + j;
+        const int a0 = dot_groups<NQ>(src + i * CSTRIDE, m, 64) >> 7;
+        const int a1 = dot_groups<NQ>(src + (i + 1) * CSTRIDE, m, 64) >> 7;
+        dst[i >> 1] = pack_sat16(a0, a1);
     }
 }
 
-// Stage-2 outputs of one lane into the LDS output tile: rows r0, r0 + NR, ...
-// of one column, clip16((sum + rnd) >> shift2); src (intermediate rows) and
-// out (the output tile) are disjoint.
-template <int NQ, int H, int NR, int W, int RSTRIDE>
-__device__ __forceinline__ void stage2_col(const uint2* __restrict__ src, int16_t* __restrict__ out,
-                                           const uint32_t (&m)[16], int r0, int rnd, int shift2) {
-#pragma unroll 4
-    for (int j = 0; j < H / NR; j++) {
-        const int y = r0 + j * NR;
-        const int acc = dot_groups<NQ>(src + y * RSTRIDE, m, rnd);
-        out[y * W] = (int16_t)min(max(acc >> shift2, -32768), 32767);
+// The same for two matrix columns over one read of src: a0 and a1 start at
+// acc.
+template <int NQ>
+__device__ __forceinline__ void dot_groups2(const uint2* __restrict__ src, const uint32_t (&m0)[16],
+                                            const uint32_t (&m1)[16], int acc, int& a0, int& a1) {
+    a0 = a1 = acc;
+    if constexpr (NQ > 0) {
+        uint2 v[NQ];
+#pragma unroll
+        for (int q = 0; q < NQ; q++) v[q] = src[q];
+        a0 = dot2_first(v[0].x, m0[0], acc);
+        a1 = dot2_first(v[0].x, m1[0], acc);
+        a0 = dot2(v[0].y, m0[1], a0);
+        a1 = dot2(v[0].y, m1[1], a1);
+#pragma unroll
+        for (int q = 1; q < NQ; q++) {
+            a0 = dot2(v[q].x, m0[2 * q], a0);
+            a1 = dot2(v[q].x, m1[2 * q], a1);
+            a0 = dot2(v[q].y, m0[2 * q + 1], a0);
+            a1 = dot2(v[q].y, m1[2 * q + 1], a1);
+        }
     }
 }
 
@@ -535,6 +563,27 @@ __device__ __forceinline__ void load_col16(const uint32_t* __restrict__ src, uin
     }
 }
 
+// The first NP pairs (2, 4, 8 or 16) of one column of the padded table;
+// the rest of m is left unset and never read.
+template <int NP>
+__device__ __forceinline__ void load_col(const uint32_t* __restrict__ src, uint32_t (&m)[16]) {
+    if constexpr (NP <= 2) {
+        const uint2 v = *reinterpret_cast<const uint2*>(src);
+        m[0] = v.x;
+        m[1] = v.y;
+    } else {
+        const uint4* s4 = reinterpret_cast<const uint4*>(src);
+#pragma unroll
+        for (int q = 0; q < NP / 4; q++) {
+            const uint4 v = s4[q];
+            m[4 * q + 0] = v.x;
+            m[4 * q + 1] = v.y;
+            m[4 * q + 2] = v.z;
+            m[4 * q + 3] = v.w;
+        }
+    }
+}
+
 // Rows per prediction-fetch batch of a recon lane (phase C): 8 keeps the
 // kernel at 6 waves per SIMD, where 16 would need 128 VGPRs.
 constexpr int kReconBatch = 8;
@@ -550,13 +599,11 @@ __device__ __forceinline__ void tile_body(const ResidualArgs& a, const TileItem&
     // banks; even, for 8-byte reads
     constexpr int CP = residual_tile_pitch(LHC), TP = residual_tile_pitch(LWC);
     constexpr int GMAX = (4096 >> (LW + LH)) > 0 ? (4096 >> (LW + LH)) : 1;
-    constexpr int R0 = (W * H / 2 > Wc * CP) ? W * H / 2 : Wc * CP;  // region 0 dwords per TU
     const int G = it.count;
-    // LDS: region 0 holds the coefficients (phases A, B), then the output
-    // tile (phase C); then the intermediates; then the per-TU parameters.
+    // LDS: the coefficient columns, the intermediate rows, the per-TU
+    // parameters, the tile bounding box.
     uint32_t* C = smem;
-    int16_t* O = reinterpret_cast<int16_t*>(smem);
-    uint32_t* T = C + GMAX * R0;
+    uint32_t* T = C + GMAX * Wc * CP;
     TuParams* P = reinterpret_cast<TuParams*>(T + GMAX * H * TP);
     int* TB = reinterpret_cast<int*>(P + GMAX);  // tile bounding box: max row, max column of non-zeros
     const int tid = threadIdx.x;
@@ -611,6 +658,16 @@ __device__ __forceinline__ void tile_body(const ResidualArgs& a, const TileItem&
                 }
             }
         }
+        // S fits int16 in every TU of the wave's items (every QP short of the
+        // top corner of the range): each level then dequantises with one dot2
+        // against (S, 0) or (0, S), and the saturating pack is the int16 clip
+        bool wide = false;
+#pragma unroll
+        for (int i = 0; i < NA; i++) {
+            const int e = tid + 256 * i;
+            if (e < total && P[e / (Q * RP)].scale > 32767) wide = true;
+        }
+        const bool fast = __builtin_amdgcn_ballot_w64(wide) == 0;
 #pragma unroll
         for (int i = 0; i < NA; i++) {
             const int e = tid + 256 * i;
@@ -627,19 +684,34 @@ __device__ __forceinline__ void tile_body(const ResidualArgs& a, const TileItem&
                 raw0 &= keep;
                 raw1 &= keep;
             }
-            // bounding box of the non-zero levels (a zero level dequantises to zero)
-            const uint64_t z = raw0 | raw1;
-            if (z) mc = max(mc, x0 + ((63 - __clzll(z)) >> 4));
-            mr = max(mr, raw1 ? r0 + 1 : raw0 ? r0 : -1);
+            // bounding box of the non-zero levels (a zero level dequantises
+            // to zero) in whole row pairs and column quads, the granularity at
+            // which the passes below use it
+            if (raw0 | raw1) {
+                mr = max(mr, r0 + 1);
+                mc = max(mc, x0 + 3);
+            }
             const uint32_t w0[2] = {(uint32_t)raw0, (uint32_t)(raw0 >> 32)};
             const uint32_t w1[2] = {(uint32_t)raw1, (uint32_t)(raw1 >> 32)};
+            uint32_t* dst = C + (g * Wc + x0) * CP + rp;
+            if (fast) {
+                const uint32_t mlo = (uint32_t)S, mhi = (uint32_t)S << 16;
 #pragma unroll
-            for (int k = 0; k < 4; k++) {
-                const uint32_t u0 = w0[k >> 1], u1 = w1[k >> 1];
-                const int l0 = (k & 1) ? (int)u0 >> 16 : (int)(int16_t)(u0 & 0xffff);
-                const int l1 = (k & 1) ? (int)u1 >> 16 : (int)(int16_t)(u1 & 0xffff);
-                const int a0 = dequant_tile(l0, S, rnd, rs), a1 = dequant_tile(l1, S, rnd, rs);
-                C[(g * Wc + x0 + k) * CP + rp] = __builtin_amdgcn_perm((uint32_t)a1, (uint32_t)a0, 0x05040100u);
+                for (int k = 0; k < 4; k++) {
+                    const uint32_t mk = (k & 1) ? mhi : mlo;
+                    const int a0 = dot2_first(w0[k >> 1], mk, rnd) >> rs;
+                    const int a1 = dot2_first(w1[k >> 1], mk, rnd) >> rs;
+                    dst[k * CP] = pack_sat16(a0, a1);
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const uint32_t u0 = w0[k >> 1], u1 = w1[k >> 1];
+                    const int l0 = (k & 1) ? (int)u0 >> 16 : (int)(int16_t)(u0 & 0xffff);
+                    const int l1 = (k & 1) ? (int)u1 >> 16 : (int)(int16_t)(u1 & 0xffff);
+                    const int a0 = dequant_tile(l0, S, rnd, rs), a1 = dequant_tile(l1, S, rnd, rs);
+                    dst[k * CP] = __builtin_amdgcn_perm((uint32_t)a1, (uint32_t)a0, 0x05040100u);
+                }
             }
         }
         mr = wave_max(mr);
@@ -652,9 +724,7 @@ __device__ __forceinline__ void tile_body(const ResidualArgs& a, const TileItem&
     __syncthreads();
     const int er = __builtin_amdgcn_readfirstlane(TB[0]) + 1;
     const int ec = __builtin_amdgcn_readfirstlane(TB[1]) + 1;
-    // the horizontal pass reads intermediates two pairs at a time: produce the
-    // columns up to a multiple of 4 (zeros past the box, from zero coefficients)
-    const int k1 = (er + 1) >> 1, ec4 = (ec + 3) & ~3, k2 = (ec + 1) >> 1;
+    const int k1 = (er + 1) >> 1, k2 = (ec + 1) >> 1;
 
     // phase B: vertical pass, lane = (TU, row y, column chunk); rows past
     // the bounding box are read in groups of 4 (zeros, from zero coefficients)
@@ -662,7 +732,11 @@ __device__ __forceinline__ void tile_body(const ResidualArgs& a, const TileItem&
         constexpr int NCH0 = 256 / (GMAX * H);
         constexpr int NCH = NCH0 > 0 ? NCH0 : 1;
         constexpr int ITEMS = GMAX * H * NCH;
-        int16_t* T16 = reinterpret_cast<int16_t*>(T);
+        // the horizontal pass reads intermediates two pairs at a time: the
+        // columns are produced up to a multiple of 4 (zeros past the box,
+        // from zero coefficients), in NCH chunks of an even number of columns
+        constexpr int CM = NCH > 2 ? 2 * NCH : 4;
+        const int ecn = (ec + CM - 1) & ~(CM - 1), nc = ecn / NCH;
         auto vpass = [&](auto nq) {
             constexpr int NQ = decltype(nq)::value;
 #pragma unroll
@@ -674,9 +748,9 @@ __device__ __forceinline__ void tile_body(const ResidualArgs& a, const TileItem&
                 if (g < G) {
                     uint32_t m[16];
                     load_col16(a.col16 + P[g].mv + y * 16, m);
-                    // NCH (1, 2 or 4) divides ec4: the same trip count in every lane
-                    stage1_row<NQ, NCH, CP / 2>(reinterpret_cast<const uint2*>(C + g * Wc * CP),
-                                                T16 + (g * H + y) * TP * 2, m, c0, ec4 / NCH);
+                    // the same trip count in every lane
+                    stage1_row<NQ, CP / 2>(reinterpret_cast<const uint2*>(C + g * Wc * CP), T + (g * H + y) * TP, m,
+                                           c0 * nc, nc);
                 }
             }
         };
@@ -684,9 +758,52 @@ __device__ __forceinline__ void tile_body(const ResidualArgs& a, const TileItem&
     }
     __syncthreads();
 
-    // phase C: horizontal pass, lane = (TU, column x, row chunk); residuals
-    // go to the output tile in LDS, picture samples are updated in place
-    {
+    if constexpr (!RECON) {
+        // phase C: horizontal pass, lane = (TU, column pair, row group); each
+        // lane produces RL rows of two adjacent columns per round and stores
+        // them as saturated int16 pairs straight to HBM. The W / 2 * NRR lanes
+        // of a TU cover NRR consecutive rows per store, at least 64 bytes of
+        // one TU block where it is that large.
+        constexpr int W2 = W / 2;
+        constexpr int NRR0 = (H / 8 > 32 / W) ? H / 8 : 32 / W;
+        constexpr int NRR = NRR0 < 1 ? 1 : (NRR0 > H ? H : NRR0);
+        constexpr int RL = H / NRR;        // rows per lane per round (<= 8)
+        constexpr int LPT = W2 * NRR;      // lanes per TU
+        constexpr int ITEMS = GMAX * LPT;  // 2048 / RL, a multiple of 256
+        constexpr int NP = Wc / 2;         // matrix pairs per column
+        const int x2 = tid % W2;
+        auto hpass = [&](auto nq) {
+            constexpr int NQ = decltype(nq)::value;
+#pragma unroll
+            for (int base = 0; base < ITEMS; base += 256) {
+                const int q = base + tid;
+                const int rr = (q / W2) % NRR;
+                const int g = q / LPT;
+                if (g < G) {
+                    const TuParams p = P[g];
+                    uint32_t m0[16], m1[16];
+                    load_col<NP>(a.col16 + p.mh + (2 * x2) * 16, m0);
+                    load_col<NP>(a.col16 + p.mh + (2 * x2 + 1) * 16, m1);
+                    const int shift2 = p.shifts >> 16;
+                    const int rnd = 1 << (shift2 - 1);
+                    // row rr of the TU; rows rr + j NRR are at constant offsets
+                    const uint2* src = reinterpret_cast<const uint2*>(T + (g * H + rr) * TP);
+                    RMH_GLOBAL uint32_t* dst = gdst<uint32_t>(a.residual + p.off) + (rr * W2 + x2);
+#pragma unroll
+                    for (int j = 0; j < RL; j++) {
+                        int a0, a1;
+                        dot_groups2<NQ>(src + j * NRR * (TP / 2), m0, m1, rnd, a0, a1);
+                        dst[j * NRR * W2] = pack_sat16(a0 >> shift2, a1 >> shift2);
+                    }
+                }
+            }
+        };
+        // NQ = 0 for a tile without non-zero coefficients (phase B wrote no
+        // intermediates; every residual is zero)
+        with_groups<0, (Wc / 4 > 0 ? Wc / 4 : 1)>((k2 + 1) >> 1, hpass);
+    } else {
+        // phase C (recon): horizontal pass, lane = (TU, column x, row chunk);
+        // picture samples are updated in place
         constexpr int NR0 = 256 / (GMAX * W);
         constexpr int NR = NR0 > 0 ? NR0 : 1;
         constexpr int ITEMS = GMAX * W * NR;
@@ -706,7 +823,7 @@ __device__ __forceinline__ void tile_body(const ResidualArgs& a, const TileItem&
                     const int rnd = 1 << (shift2 - 1);
                     const ReconDst rd{p.base, p.stride, p.bpsmax & 255, p.bpsmax >> 8};
                     const uint2* src = reinterpret_cast<const uint2*>(T + g * H * TP);
-                    if (RECON) {
+                    {
                         // H / NR <= 16 rows per lane, in batches of RB: the
                         // prediction samples of a batch are requested before its first add
                         constexpr int NY = H / NR, RB = NY < kReconBatch ? NY : kReconBatch;
@@ -722,8 +839,6 @@ __device__ __forceinline__ void tile_body(const ResidualArgs& a, const TileItem&
                                 recon_store(rd, x, y, pred[j] + min(max(acc >> shift2, -32768), 32767));
                             }
                         }
-                    } else {
-                        stage2_col<NQ, H, NR, W, TP / 2>(src, O + g * H * W + x, m, r0, rnd, shift2);
                     }
                 }
             }
@@ -731,17 +846,6 @@ __device__ __forceinline__ void tile_body(const ResidualArgs& a, const TileItem&
         // no group at all for a tile without non-zero coefficients (phase B
         // wrote no intermediates; every residual is zero)
         with_groups<0, (Wc / 4 > 0 ? Wc / 4 : 1)>((k2 + 1) >> 1, hpass);
-    }
-    if (!RECON) {
-        // the tile's TU blocks to HBM, 4 residuals (8 bytes) per store
-        __syncthreads();
-        constexpr int CH4 = W * H / 4;
-        const int total = G * CH4;
-        for (int e = tid; e < total; e += 256) {
-            const int g = e / CH4, c = e - g * CH4;
-            const uint2 v = *reinterpret_cast<const uint2*>(O + g * W * H + 4 * c);
-            *reinterpret_cast<uint2*>(a.residual + (P[g].off + (uint32_t)(4 * c))) = v;
-        }
     }
 }
 
