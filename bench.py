@@ -270,6 +270,58 @@ def bench_bi(rmh, acc, W, po, n_frames, steps, warmup):
     return out
 
 
+def bench_sao(rmh, acc, W, po, n_frames, steps, warmup):
+    """SAO (band and edge offsets per CTB, random per component) over
+    n_frames deblocked 1080p10 pictures: one launch of the SAO plan, output to
+    separate planes."""
+    hip, abi = rmh.hip, rmh.abi
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    from test_oracle_sao import random_sao_params
+    rng = np.random.default_rng(77)
+    w, h, bd, ctb = 1920, 1080, 10, 64
+    rows, cols = -(-h // ctb), -(-w // ctb)
+    pics, prms, bufs, structs, outs = [], [], [], [], []
+    for i in range(n_frames):
+        planes = [np.ascontiguousarray(W.textured_plane(rng, pw, ph, bd).astype(np.uint16))
+                  for pw, ph in ((w, h), (w // 2, h // 2), (w // 2, h // 2))]
+        prm = np.ascontiguousarray(random_sao_params(rng, rows, cols, bd))
+        srcs, dsts, fouts = [], [], []
+        for p in planes:
+            b, o = hip.DeviceBuffer.from_array(p), hip.DeviceBuffer(p.nbytes)
+            bufs += [b, o]
+            srcs.append(abi.plane_from_address(b.ptr, p.shape[1], p.shape[0], p.shape[1], 2, bd))
+            dsts.append(abi.plane_from_address(o.ptr, p.shape[1], p.shape[0], p.shape[1], 2, bd))
+            fouts.append(o)
+        dp = hip.DeviceBuffer.from_array(prm)
+        bufs.append(dp)
+        structs.append(abi.sao_frame(srcs, dsts, dp.ptr, ctb, cols, rows, w, h))
+        pics.append(planes)
+        prms.append(prm)
+        outs.append(fouts)
+    plan = acc.sao_plan(structs)
+    stream = hip.Stream()
+    plan.run(stream=stream.handle)
+    stream.synchronize()
+    for fi in (0, n_frames - 1):
+        want = po.sao_frame(pics[fi], bd, prms[fi], ctb, cols, rows, w, h)
+        for c in range(3):
+            got = outs[fi][c].download(np.uint16).reshape(want[c].shape)
+            if not (got == want[c]).all():
+                log("bench: SAO frame %d plane %d differs from the oracle" % (fi, c))
+                return None
+    ms = timed(hip, stream, lambda s: plan.run(stream=s), steps, warmup)
+    px = n_frames * w * h
+    out = {"workload": "%d deblocked 1920x1080 10-bit 4:2:0 pictures, CTB 64, per-CTB band or edge offset (random "
+                       "class, band and offsets per component), output to separate planes, one launch" % n_frames,
+           "value": round(px / (ms / 1e3) / 1e6, 1), "unit": "Mpix/s", "ms_per_step": round(ms, 4),
+           "roofline": dict(stage_roofline(plan, ms), kernel="sao_kernel<uint16_t>")}
+    log("sao: %.4f ms/step %.1f Mpix/s (%.1f GB/s)" % (ms, out["value"], out["roofline"]["achieved_gbs"]))
+    plan.close()
+    for b in bufs:
+        b.close()
+    return out
+
+
 def bench_tool_mix(rmh, acc, W, po, n_frames, steps, warmup):
     """The residual stage with LFNST, transform skip and dependent
     quantisation in the mix: TUs with those tools run on the generic
@@ -387,6 +439,7 @@ def main():
     ap.add_argument("--no-bi", action="store_true")
     ap.add_argument("--cnn-kernels", action="store_true")
     ap.add_argument("--no-tools", action="store_true")
+    ap.add_argument("--no-sao", action="store_true")
     args = ap.parse_args()
 
     if args.copy_calibration:
@@ -521,6 +574,7 @@ def main():
 
     bi = None if args.no_bi else bench_bi(rmh, acc, W, po, args.pipeline_frames, args.steps, args.warmup)
     tools = None if args.no_tools else bench_tool_mix(rmh, acc, W, po, 8, args.steps, args.warmup)
+    sao = None if args.no_sao else bench_sao(rmh, acc, W, po, args.pipeline_frames, args.steps, args.warmup)
     learned = None if args.no_learned else bench_learned(rmh, acc, args.steps)
 
     cpu = None
@@ -547,6 +601,7 @@ def main():
         "multi_stream": streams,
         "bi_prediction": bi,
         "residual_tool_mix": tools,
+        "sao": sao,
         "learned": learned,
         "device": info,
     }

@@ -295,6 +295,59 @@ rmh_status rmh_deblock_plan_create(rmh_ctx* ctx, const rmh_dbk_frame* frames, si
 rmh_status rmh_deblock_plan_run(rmh_plan* plan, void* stream);
 
 /* ------------------------------------------------------------------------ */
+/* Sample adaptive offset (after deblocking)                                 */
+/* ------------------------------------------------------------------------ */
+
+/* SAO of one component of one CTB, the SaoType enum
+ * (crates/rustmpeg-video/src/vvc/filter.rs:1225-1248):
+ *   RMH_SAO_OFF   nothing;
+ *   RMH_SAO_BAND  band offset: band_or_class = band_pos (0..31), offsets for
+ *                 bands band_pos .. band_pos + 3 (mod 32);
+ *   RMH_SAO_EDGE  edge offset: band_or_class = eo_class (0: horizontal,
+ *                 1: vertical, 2: 135 degrees, 3: 45 degrees), offsets for
+ *                 edge categories 1..4.
+ * Offsets are in sample units at every bit depth, as the reference applies
+ * them (filter.rs:1968-1975). */
+enum { RMH_SAO_OFF = 0, RMH_SAO_BAND = 1, RMH_SAO_EDGE = 2 };
+
+typedef struct rmh_sao_param {
+    uint8_t type;
+    uint8_t band_or_class;
+    uint8_t reserved[2];
+    int32_t offsets[4];
+} rmh_sao_param;
+
+/* One picture of apply_sao (crates/rustmpeg-video/src/vvc/decode_general.rs:
+ * 5189-5222), each CTB through apply_sao_region_bounded_16 (vvc/filter.rs:
+ * 1954-2043): dst = src with the CTB's offsets applied inside the visible
+ * region (vis_w x vis_h luma, ceil(vis/2) chroma); every other sample, and
+ * every edge-offset sample with a neighbour outside the visible region or the
+ * plane, is copied unchanged. src is read only; dst must not overlap it.
+ * params[(cy * ctb_cols + cx) * 3 + c] is the SAO of component c (Y, Cb, Cr)
+ * of CTB (cx, cy); chroma CTBs are ctb_size / 2 on a side (4:2:0). A decoder
+ * whose slice has sao_chroma_used = 0 passes RMH_SAO_OFF for Cb and Cr. */
+typedef struct rmh_sao_frame {
+    rmh_plane src[3];            /* deblocked Y, Cb, Cr */
+    rmh_plane dst[3];            /* SAO output, same sizes and pitch */
+    const rmh_sao_param* params; /* ctb_cols * ctb_rows * 3 entries */
+    uint32_t ctb_size;           /* luma CTB size: 16, 32, 64 or 128 */
+    uint32_t ctb_cols;           /* row pitch of params in CTBs, >= ceil(vis_w / ctb_size) */
+    uint32_t ctb_rows;           /* >= ceil(vis_h / ctb_size) */
+    uint32_t vis_w;              /* visible luma size (<= the Y plane's) */
+    uint32_t vis_h;
+    uint32_t reserved;
+} rmh_sao_frame;
+
+/* Host planes and params: uploads, runs and writes dst (written only on
+ * RMH_OK). Replaces the decoder's apply_sao loop over apply_sao_region_bounded_16. */
+rmh_status rmh_sao(rmh_ctx* ctx, const rmh_sao_frame* frames, size_t n_frames);
+/* Planes and params in `frames` hold device addresses; a plan keeps the frame
+ * table and the work index on the device and runs on any stream. */
+rmh_status rmh_sao_plan_create(rmh_ctx* ctx, const rmh_sao_frame* frames, size_t n_frames,
+                               rmh_plan** out);
+rmh_status rmh_sao_plan_run(rmh_plan* plan, void* stream);
+
+/* ------------------------------------------------------------------------ */
 /* Plans and frame-step graphs                                               */
 /* ------------------------------------------------------------------------ */
 

@@ -78,6 +78,9 @@ def _setup(L):
     L.orc_intra_predict.argtypes = [U8P] + [c_int] * 8 + [I32P]
     L.orc_intra_encode.argtypes = [U8P, U8P, U8P, c_int, c_int, c_int, U8P, U8P, U8P, U8P, ctypes.c_void_p, I16P]
     L.orc_intra_decode.argtypes = [c_int, c_int, U8P, ctypes.c_void_p, I16P, U8P, U8P, U8P]
+    L.orc_sao_region_bounded16.argtypes = [ctypes.c_void_p, ctypes.c_void_p] + [c_int] * 6 + [ctypes.c_void_p]
+    L.orc_sao_region_bounded8.argtypes = [ctypes.c_void_p, ctypes.c_void_p] + [c_int] * 6 + [ctypes.c_void_p]
+    L.orc_sao_frame.argtypes = [ctypes.c_void_p]
 
 
 def _p(a, t):
@@ -299,3 +302,57 @@ def mc_batch(frame, pus):
     pus = np.ascontiguousarray(pus)
     arr = (type(frame) * 1)(frame)
     return lib().orc_mc_batch(ctypes.addressof(arr), 1, pus.ctypes.data, len(pus))
+
+
+_abi_mod = None
+
+
+def _abi():
+    """The ABI struct definitions (rustmpeg-next_amd/abi.py: ctypes layouts
+    only, no library calls), loaded from the file."""
+    global _abi_mod
+    if _abi_mod is None:
+        import importlib.util
+        path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "..", "rustmpeg-next_amd", "abi.py")
+        spec = importlib.util.spec_from_file_location("rmh_abi_layouts", path)
+        _abi_mod = importlib.util.module_from_spec(spec)
+        spec.loader.exec_module(_abi_mod)
+    return _abi_mod
+
+
+def _sao_param(p):
+    """p: a SAO_PARAM_DTYPE record, or (type, band_or_class, offsets)."""
+    if isinstance(p, tuple):
+        r = np.zeros(1, _abi().SAO_PARAM_DTYPE)
+        r["type"], r["band_or_class"], r["offsets"] = p[0], p[1], p[2]
+        return r
+    return np.ascontiguousarray(np.atleast_1d(p), dtype=_abi().SAO_PARAM_DTYPE)
+
+
+def sao_region(src, bd, bx, by, w, h, bw, bh, param, eight_bit_form=False):
+    """apply_sao_region_bounded_16 (or the 8-bit apply_sao_region_bounded) of
+    one region: returns a copy of src with the region filtered."""
+    A = _abi()
+    src = np.ascontiguousarray(src)
+    dst = src.copy()
+    prm = _sao_param(param)
+    fn = lib().orc_sao_region_bounded8 if eight_bit_form else lib().orc_sao_region_bounded16
+    ps, pd = A.plane_from_array(src, bd), A.plane_from_array(dst, bd)
+    if fn(ctypes.addressof(ps), ctypes.addressof(pd), bx, by, w, h, bw, bh, prm.ctypes.data):
+        raise ValueError("invalid SAO parameters")
+    return dst
+
+
+def sao_frame(planes, bd, params, ctb_size, ctb_cols, ctb_rows, vis_w, vis_h):
+    """apply_sao over one picture: planes [Y, Cb, Cr] (not modified); params a
+    SAO_PARAM_DTYPE array of ctb_rows * ctb_cols * 3 records. Returns the three
+    output planes."""
+    A = _abi()
+    planes = [np.ascontiguousarray(p) for p in planes]
+    outs = [np.zeros_like(p) for p in planes]
+    params = np.ascontiguousarray(params, dtype=A.SAO_PARAM_DTYPE)
+    f = A.sao_frame([A.plane_from_array(p, bd) for p in planes], [A.plane_from_array(o, bd) for o in outs],
+                    params.ctypes.data, ctb_size, ctb_cols, ctb_rows, vis_w, vis_h)
+    if lib().orc_sao_frame(ctypes.addressof(f)):
+        raise ValueError("invalid SAO parameters")
+    return outs

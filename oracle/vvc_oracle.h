@@ -112,6 +112,13 @@ int orc_intra_decode(int w, int h, const uint8_t* modes, const rmh_tu_desc* tus,
 /* tC and beta after bit-depth scaling (deblock_luma_edge_bd, vvc/filter.rs:653-666). */
 void orc_tc_beta(int qp, int bs, int beta_off, int tc_off, int bd, int* tc, int* beta);
 
+/* ---- SAO (vvc/filter.rs:1276-1345, 1954-2043; decode_general.rs:5189-5222) ---- */
+int orc_sao_region_bounded16(const rmh_plane* src, rmh_plane* dst, int bx, int by, int w, int h, int bw, int bh,
+                             const rmh_sao_param* sao);
+int orc_sao_region_bounded8(const rmh_plane* src, rmh_plane* dst, int bx, int by, int w, int h, int bw, int bh,
+                            const rmh_sao_param* sao);
+int orc_sao_frame(const rmh_sao_frame* f);
+
 #ifdef __cplusplus
 }
 #endif

@@ -67,7 +67,7 @@ EXPORTED = [
     "rmh_create", "rmh_destroy", "rmh_last_error", "rmh_info", "rmh_disable", "rmh_is_disabled",
     "rmh_inject_failures", "rmh_residual_batch", "rmh_residual_plan_create", "rmh_residual_plan_run",
     "rmh_residual_plan_run_range", "rmh_mc_batch", "rmh_mc_plan_create", "rmh_mc_plan_run", "rmh_deblock",
-    "rmh_deblock_plan_create", "rmh_deblock_plan_run", "rmh_plan_destroy", "rmh_plan_bytes", "rmh_graph_create",
+    "rmh_deblock_plan_create", "rmh_deblock_plan_run", "rmh_sao", "rmh_sao_plan_create", "rmh_sao_plan_run", "rmh_plan_destroy", "rmh_plan_bytes", "rmh_graph_create",
     "rmh_graph_launch", "rmh_graph_destroy", "rmh_transform_matrix", "rmh_intra_subset_decode",
     "rmh_residual_recon_plan_create", "rmh_rd_encode_luma", "rmh_cnn_param_count", "rmh_cnn_create",
     "rmh_cnn_destroy", "rmh_cnn_get_params", "rmh_cnn_set_params", "rmh_cnn_forward", "rmh_cnn_predict",
@@ -107,6 +107,9 @@ def load_library(path=LIB_PATH):
         "rmh_deblock": ([vp, vp, sz], st),
         "rmh_deblock_plan_create": ([vp, vp, sz, ctypes.POINTER(vp)], st),
         "rmh_deblock_plan_run": ([vp, vp], st),
+        "rmh_sao": ([vp, vp, sz], st),
+        "rmh_sao_plan_create": ([vp, vp, sz, ctypes.POINTER(vp)], st),
+        "rmh_sao_plan_run": ([vp, vp], st),
         "rmh_plan_destroy": ([vp], None),
         "rmh_plan_bytes": ([vp], ctypes.c_uint64),
         "rmh_graph_create": ([vp, vp, sz, ctypes.POINTER(vp)], st),
@@ -186,6 +189,8 @@ class Plan:
             self.acc._check(L.rmh_residual_plan_run(self.handle, d_coeff, d_residual, stream))
         elif self.kind == "mc":
             self.acc._check(L.rmh_mc_plan_run(self.handle, stream))
+        elif self.kind == "sao":
+            self.acc._check(L.rmh_sao_plan_run(self.handle, stream))
         else:
             self.acc._check(L.rmh_deblock_plan_run(self.handle, stream))
 
@@ -310,6 +315,12 @@ class Accelerator:
         arr = (abi.DbkFrame * len(frames))(*frames)
         self._check(load_library().rmh_deblock(self.handle, ctypes.addressof(arr), len(frames)))
 
+    def sao(self, frames):
+        """frames: list of abi.SaoFrame (host planes and params); writes each
+        frame's dst planes (rmh_sao)."""
+        arr = (abi.SaoFrame * len(frames))(*frames)
+        self._check(load_library().rmh_sao(self.handle, ctypes.addressof(arr), len(frames)))
+
     def intra_subset_decode(self, width, height, modes, tus, coeffs):
         """Two-pass decode of an intra-subset picture (rmh_intra_subset_decode):
         the residual batch on the device, prediction and add on the host."""
@@ -370,6 +381,14 @@ class Accelerator:
         self._check(load_library().rmh_deblock_plan_create(self.handle, ctypes.addressof(arr), len(frames),
                                                            ctypes.byref(h)))
         return Plan(self, h.value, "deblock")
+
+    def sao_plan(self, frames):
+        """SAO plan over abi.SaoFrame entries with device planes and params; run with plan.run()."""
+        arr = (abi.SaoFrame * len(frames))(*frames)
+        h = ctypes.c_void_p()
+        self._check(load_library().rmh_sao_plan_create(self.handle, ctypes.addressof(arr), len(frames),
+                                                       ctypes.byref(h)))
+        return Plan(self, h.value, "sao")
 
     def graph(self, nodes):
         """nodes: list of (plan, d_coeff, d_residual) (device addresses; 0 for non-residual plans)."""
@@ -462,4 +481,16 @@ class VvcDeblockFilter:
 
     def deblock(self, frames):
         return self.acc.deblock(frames)
+class VvcSaoFilter:
+    """Provider of the SAO stage that the decoder's apply_sao
+    (crates/rustmpeg-video/src/vvc/decode_general.rs:5189-5222) would call:
+    dst planes written only on success."""
+
+    def __init__(self, acc):
+        self.acc = acc
+
+    def sao(self, frames):
+        return self.acc.sao(frames)
+
+
 from . import learned  # noqa: E402

@@ -75,6 +75,28 @@ class DbkFrame(ctypes.Structure):
                 ("tc_offset_div2", ctypes.c_int8 * 3), ("reserved", ctypes.c_uint8 * 3)]
 
 
+SAO_OFF, SAO_BAND, SAO_EDGE = 0, 1, 2
+# rmh_sao_param: type, band_pos or eo_class, 2 reserved bytes, 4 int32 offsets
+SAO_PARAM_DTYPE = np.dtype([("type", "u1"), ("band_or_class", "u1"), ("reserved", "u1", (2,)),
+                            ("offsets", "<i4", (4,))])
+
+
+class SaoFrame(ctypes.Structure):
+    _fields_ = [("src", Plane * 3), ("dst", Plane * 3), ("params", ctypes.c_void_p), ("ctb_size", ctypes.c_uint32),
+                ("ctb_cols", ctypes.c_uint32), ("ctb_rows", ctypes.c_uint32), ("vis_w", ctypes.c_uint32),
+                ("vis_h", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+
+
+def sao_frame(srcs, dsts, params_addr, ctb_size, ctb_cols, ctb_rows, vis_w, vis_h):
+    f = SaoFrame()
+    for i in range(3):
+        f.src[i] = srcs[i]
+        f.dst[i] = dsts[i]
+    f.params = params_addr
+    f.ctb_size, f.ctb_cols, f.ctb_rows, f.vis_w, f.vis_h = ctb_size, ctb_cols, ctb_rows, vis_w, vis_h
+    return f
+
+
 class GraphNode(ctypes.Structure):
     _fields_ = [("plan", ctypes.c_void_p), ("d_coeff", ctypes.c_void_p), ("d_residual", ctypes.c_void_p)]
 

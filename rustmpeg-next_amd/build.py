@@ -7,7 +7,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "librmh.so")
-SOURCES = ["rmh.cpp", "intra_subset.cpp", "rdsearch.cpp", "residual.hip", "mc.hip", "deblock.hip", "cnn.hip"]
+SOURCES = ["rmh.cpp", "intra_subset.cpp", "rdsearch.cpp", "residual.hip", "mc.hip", "deblock.hip", "sao.hip", "cnn.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [

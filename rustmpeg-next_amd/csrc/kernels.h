@@ -141,4 +141,28 @@ hipError_t launch_deblock(const rmh_dbk_frame* frames, const rmh_plane* shadows,
                           const uint32_t* frame_of, uint32_t n_frames, uint32_t total_tiles, uint32_t tiles8,
                           uint32_t max_rows, hipStream_t stream);
 
+// One plane of one picture of an SAO batch (built at plan creation).
+struct SaoJob {
+    const void* src;               // deblocked plane
+    void* dst;                     // output plane, same size and pitch
+    const rmh_sao_param* params;   // the frame's CTB parameters
+    uint32_t width, height, stride;  // samples
+    uint32_t vis_w, vis_h;         // visible extent of this component
+    uint32_t ctb;                  // CTB size of this component
+    uint32_t ctb_cols;             // params row pitch in CTBs
+    uint32_t comp;                 // 0 Y, 1 Cb, 2 Cr
+    uint32_t bd;                   // bit depth
+    uint32_t blocks_per_row;       // 256-lane segments per group of kSaoRows rows
+    uint32_t first_block;          // first workgroup of the job
+    uint32_t vec;                  // rows 8-byte aligned: whole-lane vector access
+};
+
+// Rows of a plane per SAO workgroup.
+constexpr int kSaoRows = 4;
+
+// block_job[b] is the job of workgroup b; jobs of 8-bit planes own blocks
+// [0, blocks8).
+hipError_t launch_sao(const SaoJob* jobs, const uint32_t* block_job, uint32_t blocks8, uint32_t blocks,
+                      hipStream_t stream);
+
 }  // namespace rmh

@@ -550,19 +550,6 @@ __device__ __forceinline__ void with_groups(int nq, F& f) {
 
 __device__ __forceinline__ int wave_max_u(int v) { return __builtin_amdgcn_readfirstlane(wave_max(v)); }
 
-// 16 matrix pairs of one column (zero past the matrix) with four 128-bit loads.
-__device__ __forceinline__ void load_col16(const uint32_t* __restrict__ src, uint32_t (&m)[16]) {
-    const uint4* s4 = reinterpret_cast<const uint4*>(src);
-#pragma unroll
-    for (int q = 0; q < 4; q++) {
-        const uint4 v = s4[q];
-        m[4 * q + 0] = v.x;
-        m[4 * q + 1] = v.y;
-        m[4 * q + 2] = v.z;
-        m[4 * q + 3] = v.w;
-    }
-}
-
 // The first NP pairs (2, 4, 8 or 16) of one column of the padded table;
 // the rest of m is left unset and never read.
 template <int NP>
@@ -633,6 +620,12 @@ __device__ __forceinline__ void tile_body(const ResidualArgs& a, const TileItem&
     }
     __syncthreads();
 
+    // phase B's lane items: (TU, row y, column chunk), NCH_B chunks
+    constexpr int NCH_B = (256 / (GMAX * H)) > 0 ? 256 / (GMAX * H) : 1;
+    constexpr bool B_ONE = GMAX * H * NCH_B <= 256;
+    constexpr int NPB = Hc / 2;  // matrix pairs per column
+    uint32_t mB[16];
+
     // phase A: a lane takes 2 rows x 4 columns and stores 4 row pairs; the
     // whole Hc x Wc region is written (zeros outside the coded region), so
     // rows and columns past the tile bounding box read as zero later.
@@ -657,6 +650,12 @@ __device__ __forceinline__ void tile_body(const ResidualArgs& a, const TileItem&
                         raw[i][1] = *reinterpret_cast<const uint64_t*>(a.coeff + (off + (uint32_t)((r0 + 1) * W + x0)));
                 }
             }
+        }
+        // phase B's matrix column, when phase B is one round: loaded with
+        // the coefficients instead of after the barrier
+        if constexpr (B_ONE) {
+            const int gb = tid / (H * NCH_B);
+            if (gb < G) load_col<NPB>(a.col16 + P[gb].mv + (tid & (H - 1)) * 16, mB);
         }
         // S fits int16 in every TU of the wave's items (every QP short of the
         // top corner of the range): each level then dequantises with one dot2
@@ -729,8 +728,7 @@ __device__ __forceinline__ void tile_body(const ResidualArgs& a, const TileItem&
     // phase B: vertical pass, lane = (TU, row y, column chunk); rows past
     // the bounding box are read in groups of 4 (zeros, from zero coefficients)
     if (ec > 0) {
-        constexpr int NCH0 = 256 / (GMAX * H);
-        constexpr int NCH = NCH0 > 0 ? NCH0 : 1;
+        constexpr int NCH = NCH_B;
         constexpr int ITEMS = GMAX * H * NCH;
         // the horizontal pass reads intermediates two pairs at a time: the
         // columns are produced up to a multiple of 4 (zeros past the box,
@@ -747,7 +745,12 @@ __device__ __forceinline__ void tile_body(const ResidualArgs& a, const TileItem&
                 const int g = q / (H * NCH);
                 if (g < G) {
                     uint32_t m[16];
-                    load_col16(a.col16 + P[g].mv + y * 16, m);
+                    if constexpr (B_ONE) {
+#pragma unroll
+                        for (int k = 0; k < NPB; k++) m[k] = mB[k];
+                    } else {
+                        load_col<NPB>(a.col16 + P[g].mv + y * 16, m);
+                    }
                     // the same trip count in every lane
                     stage1_row<NQ, CP / 2>(reinterpret_cast<const uint2*>(C + g * Wc * CP), T + (g * H + y) * TP, m,
                                            c0 * nc, nc);
@@ -818,7 +821,7 @@ __device__ __forceinline__ void tile_body(const ResidualArgs& a, const TileItem&
                 if (g < G) {
                     const TuParams p = P[g];
                     uint32_t m[16];
-                    load_col16(a.col16 + p.mh + x * 16, m);
+                    load_col<Wc / 2>(a.col16 + p.mh + x * 16, m);
                     const int shift2 = p.shifts >> 16;
                     const int rnd = 1 << (shift2 - 1);
                     const ReconDst rd{p.base, p.stride, p.bpsmax & 255, p.bpsmax >> 8};

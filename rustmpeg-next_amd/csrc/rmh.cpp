@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <chrono>
 #include <cmath>
@@ -86,7 +87,7 @@ struct rmh_ctx {
     hipDeviceProp_t props{};
 };
 
-enum PlanKind { kResidual = 0, kMc = 1, kDeblock = 2 };
+enum PlanKind { kResidual = 0, kMc = 1, kDeblock = 2, kSao = 3 };
 
 struct rmh_plan {
     rmh_ctx* ctx = nullptr;
@@ -123,6 +124,10 @@ struct rmh_plan {
     const uint32_t* d_frame_of = nullptr;    // [n_dbk] frame at each position of that order
     size_t n_dbk = 0;
     uint32_t total_tiles = 0, tiles8 = 0, max_rows = 0;
+    // sao
+    const rmh::SaoJob* d_sao_jobs = nullptr;
+    const uint32_t* d_sao_block_job = nullptr;
+    uint32_t n_sao_jobs = 0, sao_blocks8 = 0, sao_blocks = 0;
 };
 
 struct rmh_graph {
@@ -518,6 +523,97 @@ rmh_status validate_dbk(rmh_ctx* ctx, const rmh_dbk_frame* frames, size_t n_fram
     return RMH_OK;
 }
 
+// The SAO kernel reads a parameter record as 5 dwords.
+static_assert(sizeof(rmh_sao_param) == 20 && offsetof(rmh_sao_param, offsets) == 4, "rmh_sao_param layout");
+
+// SAO frames: matching src/dst planes of one bit depth, 4:2:0 chroma, a CTB
+// grid that covers the visible region, src and dst not overlapping. With
+// host_params, the parameters themselves are checked too (plans hold device
+// addresses, where an unknown type acts as RMH_SAO_OFF).
+rmh_status validate_sao(rmh_ctx* ctx, const rmh_sao_frame* frames, size_t n_frames, bool host_params) {
+    if (n_frames && !frames) return fail_locked(ctx, RMH_INVALID_INPUT, "null frame array");
+    if (n_frames > 65535) return fail_locked(ctx, RMH_LIMIT_EXCEEDED, "more than 65535 frames");
+    for (size_t f = 0; f < n_frames; f++) {
+        const rmh_sao_frame& d = frames[f];
+        const std::string tag = "frame " + std::to_string(f) + ": ";
+        for (int c = 0; c < 3; c++) {
+            const rmh_plane &a = d.src[c], &b = d.dst[c];
+            if (!plane_ok(a) || !plane_ok(b) || a.width != b.width || a.height != b.height ||
+                a.bytes_per_sample != b.bytes_per_sample || a.stride != b.stride || a.bit_depth != d.src[0].bit_depth ||
+                b.bit_depth != a.bit_depth)
+                return fail_locked(ctx, RMH_INVALID_INPUT, tag + "invalid or mismatched src/dst plane");
+            const uint64_t n = (uint64_t)a.stride * a.height * a.bytes_per_sample;
+            const uintptr_t sa = (uintptr_t)a.data, da = (uintptr_t)b.data;
+            if (sa < da + n && da < sa + n) return fail_locked(ctx, RMH_INVALID_INPUT, tag + "dst overlaps src");
+        }
+        if (d.src[1].width != (d.src[0].width + 1) / 2 || d.src[1].height != (d.src[0].height + 1) / 2 ||
+            d.src[2].width != d.src[1].width || d.src[2].height != d.src[1].height)
+            return fail_locked(ctx, RMH_INVALID_INPUT, tag + "chroma planes are not 4:2:0");
+        if (d.ctb_size != 16 && d.ctb_size != 32 && d.ctb_size != 64 && d.ctb_size != 128)
+            return fail_locked(ctx, RMH_INVALID_INPUT, tag + "CTB size");
+        if (d.vis_w == 0 || d.vis_h == 0 || d.vis_w > d.src[0].width || d.vis_h > d.src[0].height)
+            return fail_locked(ctx, RMH_INVALID_INPUT, tag + "visible size");
+        if (!d.params || d.ctb_cols < (d.vis_w + d.ctb_size - 1) / d.ctb_size ||
+            d.ctb_rows < (d.vis_h + d.ctb_size - 1) / d.ctb_size)
+            return fail_locked(ctx, RMH_INVALID_INPUT, tag + "CTB parameter grid");
+        if (host_params) {
+            const size_t np = (size_t)d.ctb_cols * d.ctb_rows * 3;
+            for (size_t i = 0; i < np; i++) {
+                const rmh_sao_param& q = d.params[i];
+                if (q.type > RMH_SAO_EDGE) return fail_locked(ctx, RMH_INVALID_INPUT, tag + "SAO type");
+                for (int k = 0; k < 4; k++)
+                    if (q.offsets[k] < -65535 || q.offsets[k] > 65535)
+                        return fail_locked(ctx, RMH_INVALID_INPUT, tag + "SAO offset out of range");
+            }
+        }
+    }
+    return RMH_OK;
+}
+
+// One job per plane, planes of 8-bit storage first; params[f] is frame f's
+// parameter address as the kernel will read it.
+struct SaoJobs {
+    std::vector<rmh::SaoJob> jobs;
+    std::vector<uint32_t> block_job;  // job of each workgroup
+    uint32_t blocks8 = 0, blocks = 0;
+};
+
+SaoJobs sao_jobs(const rmh_sao_frame* frames, size_t n_frames, const std::vector<const rmh_sao_param*>& params,
+                 const std::vector<std::array<rmh_plane, 6>>& planes) {
+    SaoJobs out;
+    for (int pass = 0; pass < 2; pass++)
+        for (size_t f = 0; f < n_frames; f++)
+            for (int c = 0; c < 3; c++) {
+                const rmh_plane& src = planes[f][c];
+                if ((src.bytes_per_sample == 1) != (pass == 0)) continue;
+                const rmh_sao_frame& fr = frames[f];
+                rmh::SaoJob j{};
+                j.src = src.data;
+                j.dst = planes[f][3 + c].data;
+                j.params = params[f];
+                j.width = src.width;
+                j.height = src.height;
+                j.stride = src.stride;
+                j.vis_w = c ? (fr.vis_w + 1) / 2 : fr.vis_w;
+                j.vis_h = c ? (fr.vis_h + 1) / 2 : fr.vis_h;
+                j.ctb = c ? fr.ctb_size / 2 : fr.ctb_size;
+                j.ctb_cols = fr.ctb_cols;
+                j.comp = (uint32_t)c;
+                j.bd = src.bit_depth;
+                const uint32_t k = 8 / src.bytes_per_sample;  // samples per lane
+                j.blocks_per_row = ((src.width + k - 1) / k + 255) / 256;
+                j.first_block = out.blocks;
+                j.vec = ((uintptr_t)src.data % 8 == 0 && (uintptr_t)j.dst % 8 == 0 &&
+                         ((uint64_t)src.stride * src.bytes_per_sample) % 8 == 0);
+                const uint32_t nb = j.blocks_per_row * ((src.height + rmh::kSaoRows - 1) / rmh::kSaoRows);
+                out.blocks += nb;
+                out.block_job.insert(out.block_job.end(), nb, (uint32_t)out.jobs.size());
+                out.jobs.push_back(j);
+                if (pass == 0) out.blocks8 = out.blocks;
+            }
+    return out;
+}
+
 // 16x16 luma tiles of every PU (PU order, raster inside a PU), clipped to the
 // picture; tiles wholly outside it are dropped.
 // The work items of one PU: its 16x16 luma tiles clipped to the picture
@@ -833,6 +929,8 @@ rmh_status run_plan_locked_free(rmh_plan* plan, const int16_t* d_coeff, int16_t*
                                                  (uint32_t)plan->n_items8, s)
                              : rmh::launch_mc(plan->d_frames, plan->d_items, (uint32_t)plan->n_items,
                                               (uint32_t)plan->n_items8, s);
+    } else if (plan->kind == kSao) {
+        e = rmh::launch_sao(plan->d_sao_jobs, plan->d_sao_block_job, plan->sao_blocks8, plan->sao_blocks, s);
     } else {
         e = rmh::launch_deblock(plan->d_dbk, plan->d_shadows, plan->d_tile_start, plan->d_frame_of, (uint32_t)plan->n_dbk,
                                 plan->total_tiles, plan->tiles8,
@@ -1572,6 +1670,143 @@ RMH_EXPORT rmh_status rmh_deblock(rmh_ctx* ctx, const rmh_dbk_frame* frames, siz
     return RMH_OK;
 }
 
+// ---------------------------------------------------------------------- sao
+
+RMH_EXPORT rmh_status rmh_sao_plan_create(rmh_ctx* ctx, const rmh_sao_frame* frames, size_t n_frames, rmh_plan** out) {
+    if (!ctx || !out) return RMH_INVALID_INPUT;
+    *out = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        rmh_status v = validate_sao(ctx, frames, n_frames, false);
+        if (v != RMH_OK) return v;
+    }
+    if (ctx->disabled.load()) return fail(ctx, RMH_UNAVAILABLE, "accelerator disabled");
+    DeviceGuard dg(ctx->device);
+    std::vector<const rmh_sao_param*> prm(n_frames);
+    std::vector<std::array<rmh_plane, 6>> planes(n_frames);
+    rmh_plan* p = new rmh_plan();
+    p->ctx = ctx;
+    p->kind = kSao;
+    for (size_t f = 0; f < n_frames; f++) {
+        prm[f] = frames[f].params;
+        for (int c = 0; c < 3; c++) {
+            planes[f][c] = frames[f].src[c];
+            planes[f][3 + c] = frames[f].dst[c];
+            p->bytes += 2 * plane_bytes(frames[f].src[c]);
+        }
+        p->bytes += (uint64_t)frames[f].ctb_cols * frames[f].ctb_rows * 3 * sizeof(rmh_sao_param);
+    }
+    const SaoJobs sj = sao_jobs(frames, n_frames, prm, planes);
+    p->n_sao_jobs = (uint32_t)sj.jobs.size();
+    p->sao_blocks8 = sj.blocks8;
+    p->sao_blocks = sj.blocks;
+    if (!sj.jobs.empty()) {
+        const size_t jb = sj.jobs.size() * sizeof(rmh::SaoJob), bb = sj.block_job.size() * sizeof(uint32_t);
+        hipError_t e = hipMalloc(&p->d_mem, aligned(jb) + bb);
+        if (e == hipSuccess) e = hipMemcpy(p->d_mem, sj.jobs.data(), jb, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy((char*)p->d_mem + aligned(jb), sj.block_job.data(), bb, hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            if (p->d_mem) hipFree(p->d_mem);
+            delete p;
+            return fail(ctx, RMH_EXECUTION, std::string("sao plan upload failed: ") + hipGetErrorString(e));
+        }
+        p->d_sao_jobs = (const rmh::SaoJob*)p->d_mem;
+        p->d_sao_block_job = (const uint32_t*)((char*)p->d_mem + aligned(jb));
+    }
+    *out = p;
+    return RMH_OK;
+}
+
+RMH_EXPORT rmh_status rmh_sao_plan_run(rmh_plan* plan, void* stream) {
+    if (!plan || plan->kind != kSao) return RMH_INVALID_INPUT;
+    if (plan->n_sao_jobs == 0) return RMH_OK;
+    return run_plan(plan, nullptr, nullptr, 0, 0, stream);
+}
+
+RMH_EXPORT rmh_status rmh_sao(rmh_ctx* ctx, const rmh_sao_frame* frames, size_t n_frames) {
+    if (!ctx) return RMH_INVALID_INPUT;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    rmh_status v = validate_sao(ctx, frames, n_frames, true);
+    if (v != RMH_OK) return v;
+    if (n_frames == 0) return RMH_OK;
+    rmh_status g = gate(ctx);
+    if (g != RMH_OK) return fail_locked(ctx, g, g == RMH_UNAVAILABLE ? "accelerator disabled" : "injected device failure");
+    DeviceGuard dg(ctx->device);
+    size_t need = 0, hneed = 0;
+    for (size_t f = 0; f < n_frames; f++) {
+        need += aligned((size_t)frames[f].ctb_cols * frames[f].ctb_rows * 3 * sizeof(rmh_sao_param));
+        for (int c = 0; c < 3; c++) {
+            need += 2 * aligned(plane_bytes(frames[f].src[c]));
+            hneed += aligned(plane_bytes(frames[f].src[c]));
+        }
+    }
+    need += aligned(n_frames * 3 * sizeof(rmh::SaoJob));
+    for (size_t f = 0; f < n_frames; f++)
+        for (int c = 0; c < 3; c++) {
+            const rmh_plane& p = frames[f].src[c];
+            const uint32_t k = 8 / p.bytes_per_sample;
+            need += aligned((size_t)(((p.width + k - 1) / k + 255) / 256) *
+                            ((p.height + rmh::kSaoRows - 1) / rmh::kSaoRows) * sizeof(uint32_t));
+        }
+    hipError_t e = ctx->dev.reserve(need);
+    if (e != hipSuccess) return fail_locked(ctx, RMH_LIMIT_EXCEEDED, std::string("device allocation: ") + hipGetErrorString(e));
+    e = ctx->host.reserve(hneed);
+    if (e != hipSuccess) return fail_locked(ctx, RMH_LIMIT_EXCEEDED, std::string("host staging: ") + hipGetErrorString(e));
+    hipStream_t s = ctx->stream;
+    std::vector<const rmh_sao_param*> prm(n_frames);
+    std::vector<std::array<rmh_plane, 6>> planes(n_frames);
+    for (size_t f = 0; f < n_frames; f++) {
+        const size_t pb = (size_t)frames[f].ctb_cols * frames[f].ctb_rows * 3 * sizeof(rmh_sao_param);
+        void* dp = ctx->dev.take(pb);
+        if ((e = hipMemcpyAsync(dp, frames[f].params, pb, hipMemcpyHostToDevice, s)) != hipSuccess)
+            return hip_fail_locked(ctx, e, "parameter upload");
+        prm[f] = (const rmh_sao_param*)dp;
+        for (int c = 0; c < 3; c++) {
+            const rmh_plane& src = frames[f].src[c];
+            rmh_plane d = src, o = src;
+            d.data = ctx->dev.take(plane_bytes(src));
+            d.stride = src.width;
+            o.data = ctx->dev.take(plane_bytes(src));
+            o.stride = src.width;
+            e = hipMemcpy2DAsync(d.data, (size_t)src.width * src.bytes_per_sample, src.data,
+                                 (size_t)src.stride * src.bytes_per_sample, (size_t)src.width * src.bytes_per_sample,
+                                 src.height, hipMemcpyHostToDevice, s);
+            if (e != hipSuccess) return hip_fail_locked(ctx, e, "plane upload");
+            planes[f][c] = d;
+            planes[f][3 + c] = o;
+        }
+    }
+    const SaoJobs sj = sao_jobs(frames, n_frames, prm, planes);
+    rmh::SaoJob* d_jobs = (rmh::SaoJob*)ctx->dev.take(sj.jobs.size() * sizeof(rmh::SaoJob));
+    uint32_t* d_bj = (uint32_t*)ctx->dev.take(sj.block_job.size() * sizeof(uint32_t));
+    if ((e = hipMemcpyAsync(d_jobs, sj.jobs.data(), sj.jobs.size() * sizeof(rmh::SaoJob), hipMemcpyHostToDevice, s)) !=
+            hipSuccess ||
+        (e = hipMemcpyAsync(d_bj, sj.block_job.data(), sj.block_job.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
+                            s)) != hipSuccess)
+        return hip_fail_locked(ctx, e, "upload");
+    e = rmh::launch_sao(d_jobs, d_bj, sj.blocks8, sj.blocks, s);
+    if (e != hipSuccess) return hip_fail_locked(ctx, e, "kernel launch");
+    std::vector<void*> hp;
+    for (size_t f = 0; f < n_frames; f++)
+        for (int c = 0; c < 3; c++) {
+            const rmh_plane& o = planes[f][3 + c];
+            void* h = ctx->host.take(plane_bytes(o));
+            hp.push_back(h);
+            if ((e = hipMemcpyAsync(h, o.data, plane_bytes(o), hipMemcpyDeviceToHost, s)) != hipSuccess)
+                return hip_fail_locked(ctx, e, "download");
+        }
+    rmh_status r = wait_stream(ctx, s);
+    if (r != RMH_OK) return r;
+    for (size_t f = 0, k = 0; f < n_frames; f++)
+        for (int c = 0; c < 3; c++, k++) {
+            const rmh_plane& d = frames[f].dst[c];
+            for (uint32_t y = 0; y < d.height; y++)
+                memcpy((uint8_t*)d.data + (size_t)y * d.stride * d.bytes_per_sample,
+                       (const uint8_t*)hp[k] + (size_t)y * d.width * d.bytes_per_sample, (size_t)d.width * d.bytes_per_sample);
+        }
+    return RMH_OK;
+}
+
 // ------------------------------------------------------------ plans, graphs
 
 RMH_EXPORT void rmh_plan_destroy(rmh_plan* plan) {
@@ -1603,7 +1838,8 @@ RMH_EXPORT rmh_status rmh_graph_create(rmh_ctx* ctx, const rmh_graph_node* nodes
     if (e == hipSuccess) {
         for (size_t i = 0; i < n_nodes && st == RMH_OK; i++) {
             rmh_plan* p = nodes[i].plan;
-            if ((p->kind == kResidual && p->n_tu == 0) || (p->kind == kMc && p->n_items == 0) || (p->kind == kDeblock && p->n_dbk == 0))
+            if ((p->kind == kResidual && p->n_tu == 0) || (p->kind == kMc && p->n_items == 0) ||
+                (p->kind == kDeblock && p->n_dbk == 0) || (p->kind == kSao && p->n_sao_jobs == 0))
                 continue;
             st = run_plan_locked_free(p, nodes[i].d_coeff, nodes[i].d_residual, 0, p->n_tu, g->capture, &err);
         }
