@@ -169,6 +169,31 @@ def bench_pipeline(rmh, acc, W, PL, po, n_frames, steps, warmup):
            "stages": stages}
     out["vs_unbatched"] = round(out["value"] / out["unbatched"]["value"], 3)
     log("pipeline: %.3f ms/step %.1f Mpix/s, unbatched %.3f ms (x%.2f)" % (ms, out["value"], ums, out["vs_unbatched"]))
+
+    # the in-loop chain with SAO: deblocking out of place into scratch planes
+    # (no shadow copy), SAO back into the picture, one graph
+    sb = PL.DeviceBatch(acc, rmh, frames, with_sao=True)
+    sgraph = sb.graph()
+    sgraph.launch(stream.handle)
+    stream.synchronize()
+    want = PL.oracle(po, rmh.abi, [frames[0]], with_sao=True)[0]
+    got = sb.download()[0]
+    if not all((got[c] == want[c]).all() for c in range(3)):
+        log("bench: pipeline with SAO differs from the oracle")
+        out["with_sao"] = None
+    else:
+        sms = timed(hip, stream, sgraph.launch, steps, warmup)
+        out["with_sao"] = {
+            "workload": "the same frames: MC -> residual add -> deblock (out of place) -> SAO (CTB 64, random "
+                        "band/edge parameters) back into the picture, one frame-step graph",
+            "value": round(px / (sms / 1e3) / 1e6, 1), "unit": "Mpix/s", "ms_per_step": round(sms, 4),
+            "stages": {"deblock_out_of_place": stage_roofline(sb.deblock, timed(hip, stream,
+                                                                                 lambda s: sb.deblock.run(stream=s),
+                                                                                 steps, 1)),
+                       "sao": stage_roofline(sb.sao, timed(hip, stream, lambda s: sb.sao.run(stream=s), steps, 1))}}
+        log("pipeline with SAO: %.3f ms/step" % sms)
+    sgraph.close()
+    sb.close()
     return out
 
 

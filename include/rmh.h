@@ -293,6 +293,14 @@ rmh_status rmh_deblock(rmh_ctx* ctx, const rmh_dbk_frame* frames, size_t n_frame
 rmh_status rmh_deblock_plan_create(rmh_ctx* ctx, const rmh_dbk_frame* frames, size_t n_frames,
                                    rmh_plan** out);
 rmh_status rmh_deblock_plan_run(rmh_plan* plan, void* stream);
+/* Out-of-place deblocking plan (no counterpart in the reference, whose
+ * DeblockFilter filters in place; for frame-step graphs that chain deblocking
+ * into SAO): frames[f].planes are read only, dst[3 f + c] (same size, bit
+ * depth and sample size, any pitch, not overlapping a source plane) receives
+ * every sample of the deblocked component c of frame f. It saves the copy of
+ * each plane that the in-place form makes first. Run with rmh_deblock_plan_run. */
+rmh_status rmh_deblock_to_plan_create(rmh_ctx* ctx, const rmh_dbk_frame* frames, size_t n_frames,
+                                      const rmh_plane* dst, rmh_plan** out);
 
 /* ------------------------------------------------------------------------ */
 /* Sample adaptive offset (after deblocking)                                 */

@@ -67,7 +67,7 @@ EXPORTED = [
     "rmh_create", "rmh_destroy", "rmh_last_error", "rmh_info", "rmh_disable", "rmh_is_disabled",
     "rmh_inject_failures", "rmh_residual_batch", "rmh_residual_plan_create", "rmh_residual_plan_run",
     "rmh_residual_plan_run_range", "rmh_mc_batch", "rmh_mc_plan_create", "rmh_mc_plan_run", "rmh_deblock",
-    "rmh_deblock_plan_create", "rmh_deblock_plan_run", "rmh_sao", "rmh_sao_plan_create", "rmh_sao_plan_run", "rmh_plan_destroy", "rmh_plan_bytes", "rmh_graph_create",
+    "rmh_deblock_plan_create", "rmh_deblock_plan_run", "rmh_deblock_to_plan_create", "rmh_sao", "rmh_sao_plan_create", "rmh_sao_plan_run", "rmh_plan_destroy", "rmh_plan_bytes", "rmh_graph_create",
     "rmh_graph_launch", "rmh_graph_destroy", "rmh_transform_matrix", "rmh_intra_subset_decode",
     "rmh_residual_recon_plan_create", "rmh_rd_encode_luma", "rmh_cnn_param_count", "rmh_cnn_create",
     "rmh_cnn_destroy", "rmh_cnn_get_params", "rmh_cnn_set_params", "rmh_cnn_forward", "rmh_cnn_predict",
@@ -107,6 +107,7 @@ def load_library(path=LIB_PATH):
         "rmh_deblock": ([vp, vp, sz], st),
         "rmh_deblock_plan_create": ([vp, vp, sz, ctypes.POINTER(vp)], st),
         "rmh_deblock_plan_run": ([vp, vp], st),
+        "rmh_deblock_to_plan_create": ([vp, vp, sz, vp, ctypes.POINTER(vp)], st),
         "rmh_sao": ([vp, vp, sz], st),
         "rmh_sao_plan_create": ([vp, vp, sz, ctypes.POINTER(vp)], st),
         "rmh_sao_plan_run": ([vp, vp], st),
@@ -380,6 +381,18 @@ class Accelerator:
         h = ctypes.c_void_p()
         self._check(load_library().rmh_deblock_plan_create(self.handle, ctypes.addressof(arr), len(frames),
                                                            ctypes.byref(h)))
+        return Plan(self, h.value, "deblock")
+
+    def deblock_to_plan(self, frames, dsts):
+        """Out-of-place deblocking plan: frames (abi.DbkFrame, device planes) are
+        read only; dsts lists 3 abi.Plane per frame that receive the output."""
+        arr = (abi.DbkFrame * len(frames))(*frames)
+        if len(dsts) != 3 * len(frames):
+            raise InvalidInput("three destination planes per frame")
+        darr = (abi.Plane * len(dsts))(*dsts)
+        h = ctypes.c_void_p()
+        self._check(load_library().rmh_deblock_to_plan_create(self.handle, ctypes.addressof(arr), len(frames),
+                                                              ctypes.addressof(darr), ctypes.byref(h)))
         return Plan(self, h.value, "deblock")
 
     def sao_plan(self, frames):

@@ -796,11 +796,14 @@ uint32_t deblock_tiles(uint32_t width, uint32_t height) {
 
 hipError_t launch_deblock(const rmh_dbk_frame* frames, const rmh_plane* shadows, const uint32_t* tile_start,
                           const uint32_t* frame_of, uint32_t n_frames, uint32_t total_tiles, uint32_t tiles8,
-                          uint32_t max_rows, hipStream_t stream) {
+                          uint32_t max_rows, bool copy_shadows, hipStream_t stream) {
     if (n_frames == 0 || total_tiles == 0) return hipSuccess;
-    hipLaunchKernelGGL(shadow_copy_kernel, dim3((max_rows + 3) / 4, n_frames, 3), dim3(256), 0, stream, frames, shadows);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
+    if (copy_shadows) {
+        hipLaunchKernelGGL(shadow_copy_kernel, dim3((max_rows + 3) / 4, n_frames, 3), dim3(256), 0, stream, frames,
+                           shadows);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
     if (tiles8 > 0)
         hipLaunchKernelGGL(deblock_tile_kernel<1>, dim3(tiles8), dim3(256), 0, stream, frames, shadows, tile_start,
                            frame_of, n_frames, 0u);

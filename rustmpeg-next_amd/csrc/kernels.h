@@ -137,9 +137,12 @@ hipError_t launch_mc_bi(const rmh_mc_bi_frame* frames, const McBiItem* items, ui
 // = total_tiles) and tiles8 is the number of tiles on 8-bit frames;
 // max_rows = largest luma height.
 uint32_t deblock_tiles(uint32_t width, uint32_t height);
+// With copy_shadows, the frames' planes are first copied into `shadows` and
+// filtered in place; without it, `shadows` are the source planes and the
+// frames' planes receive the output (every sample written once).
 hipError_t launch_deblock(const rmh_dbk_frame* frames, const rmh_plane* shadows, const uint32_t* tile_start,
                           const uint32_t* frame_of, uint32_t n_frames, uint32_t total_tiles, uint32_t tiles8,
-                          uint32_t max_rows, hipStream_t stream);
+                          uint32_t max_rows, bool copy_shadows, hipStream_t stream);
 
 // One plane of one picture of an SAO batch (built at plan creation).
 struct SaoJob {

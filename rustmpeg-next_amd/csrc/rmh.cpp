@@ -124,6 +124,7 @@ struct rmh_plan {
     const uint32_t* d_frame_of = nullptr;    // [n_dbk] frame at each position of that order
     size_t n_dbk = 0;
     uint32_t total_tiles = 0, tiles8 = 0, max_rows = 0;
+    bool dbk_copy = true;  // in place (shadow copies); false: d_shadows are the sources
     // sao
     const rmh::SaoJob* d_sao_jobs = nullptr;
     const uint32_t* d_sao_block_job = nullptr;
@@ -933,8 +934,7 @@ rmh_status run_plan_locked_free(rmh_plan* plan, const int16_t* d_coeff, int16_t*
         e = rmh::launch_sao(plan->d_sao_jobs, plan->d_sao_block_job, plan->sao_blocks8, plan->sao_blocks, s);
     } else {
         e = rmh::launch_deblock(plan->d_dbk, plan->d_shadows, plan->d_tile_start, plan->d_frame_of, (uint32_t)plan->n_dbk,
-                                plan->total_tiles, plan->tiles8,
-                                plan->max_rows, s);
+                                plan->total_tiles, plan->tiles8, plan->max_rows, plan->dbk_copy, s);
     }
     if (e != hipSuccess) {
         *err = std::string("kernel launch failed: ") + hipGetErrorString(e);
@@ -1586,6 +1586,75 @@ RMH_EXPORT rmh_status rmh_deblock_plan_create(rmh_ctx* ctx, const rmh_dbk_frame*
     return RMH_OK;
 }
 
+RMH_EXPORT rmh_status rmh_deblock_to_plan_create(rmh_ctx* ctx, const rmh_dbk_frame* frames, size_t n_frames,
+                                                 const rmh_plane* dst, rmh_plan** out) {
+    if (!ctx || !out) return RMH_INVALID_INPUT;
+    *out = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        rmh_status v = validate_dbk(ctx, frames, n_frames);
+        if (v != RMH_OK) return v;
+        if (n_frames && !dst) return fail_locked(ctx, RMH_INVALID_INPUT, "null destination planes");
+        for (size_t f = 0; f < n_frames; f++)
+            for (int c = 0; c < 3; c++) {
+                const rmh_plane &a = frames[f].planes[c], &b = dst[3 * f + c];
+                if (!plane_ok(b) || b.width != a.width || b.height != a.height ||
+                    b.bytes_per_sample != a.bytes_per_sample || b.bit_depth != a.bit_depth)
+                    return fail_locked(ctx, RMH_INVALID_INPUT, "frame " + std::to_string(f) + ": invalid destination plane");
+                for (int k = 0; k < 3; k++) {
+                    const rmh_plane& q = frames[f].planes[k];
+                    const uintptr_t qa = (uintptr_t)q.data, ba = (uintptr_t)b.data;
+                    const uint64_t qn = (uint64_t)q.stride * q.height * q.bytes_per_sample;
+                    const uint64_t bn = (uint64_t)b.stride * b.height * b.bytes_per_sample;
+                    if (qa < ba + bn && ba < qa + qn)
+                        return fail_locked(ctx, RMH_INVALID_INPUT, "frame " + std::to_string(f) + ": destination overlaps a source plane");
+                }
+            }
+    }
+    if (ctx->disabled.load()) return fail(ctx, RMH_UNAVAILABLE, "accelerator disabled");
+    DeviceGuard dg(ctx->device);
+    rmh_plan* p = new rmh_plan();
+    p->ctx = ctx;
+    p->kind = kDeblock;
+    p->n_dbk = n_frames;
+    p->dbk_copy = false;
+    const TileTable tt = tile_table(frames, n_frames);
+    p->total_tiles = tt.total;
+    p->tiles8 = tt.tiles8;
+    std::vector<rmh_dbk_frame> outf(frames, frames + n_frames);
+    std::vector<rmh_plane> srcs(n_frames * 3);
+    for (size_t f = 0; f < n_frames; f++) {
+        uint32_t cells = frames[f].grid_w * frames[f].grid_h;
+        p->max_rows = std::max(p->max_rows, frames[f].planes[0].height);
+        for (int c = 0; c < 3; c++) {
+            p->bytes += 2 * plane_bytes(frames[f].planes[c]);
+            srcs[3 * f + c] = frames[f].planes[c];
+            outf[f].planes[c] = dst[3 * f + c];
+        }
+        p->bytes += 4ull * cells;
+    }
+    size_t fb = n_frames * sizeof(rmh_dbk_frame), sb = n_frames * 3 * sizeof(rmh_plane);
+    size_t tb = tt.words.size() * sizeof(uint32_t);
+    hipError_t e = hipMalloc(&p->d_mem, aligned(fb) + aligned(sb) + aligned(tb) + 256);
+    char* base = (char*)p->d_mem;
+    if (e == hipSuccess) {
+        if (fb) e = hipMemcpy(base, outf.data(), fb, hipMemcpyHostToDevice);
+        if (e == hipSuccess && sb) e = hipMemcpy(base + aligned(fb), srcs.data(), sb, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(base + aligned(fb) + aligned(sb), tt.words.data(), tb, hipMemcpyHostToDevice);
+    }
+    if (e != hipSuccess) {
+        if (p->d_mem) hipFree(p->d_mem);
+        delete p;
+        return fail(ctx, RMH_EXECUTION, std::string("deblock plan upload failed: ") + hipGetErrorString(e));
+    }
+    p->d_dbk = (const rmh_dbk_frame*)base;
+    p->d_shadows = (const rmh_plane*)(base + aligned(fb));
+    p->d_tile_start = (const uint32_t*)(base + aligned(fb) + aligned(sb));
+    p->d_frame_of = p->d_tile_start + n_frames + 1;
+    *out = p;
+    return RMH_OK;
+}
+
 RMH_EXPORT rmh_status rmh_deblock_plan_run(rmh_plan* plan, void* stream) {
     if (!plan || plan->kind != kDeblock) return RMH_INVALID_INPUT;
     if (plan->n_dbk == 0) return RMH_OK;
@@ -1647,7 +1716,7 @@ RMH_EXPORT rmh_status rmh_deblock(rmh_ctx* ctx, const rmh_dbk_frame* frames, siz
     }
     if ((e = hipMemcpyAsync(d_frames, df.data(), fb, hipMemcpyHostToDevice, s)) != hipSuccess) return hip_fail_locked(ctx, e, "upload");
     e = rmh::launch_deblock(d_frames, d_shadows, d_starts, d_starts + n_frames + 1, (uint32_t)n_frames, tt.total,
-                            tt.tiles8, max_rows, s);
+                            tt.tiles8, max_rows, true, s);
     if (e != hipSuccess) return hip_fail_locked(ctx, e, "kernel launch");
     std::vector<void*> hp;
     for (size_t f = 0; f < n_frames; f++)

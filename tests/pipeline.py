@@ -1,5 +1,5 @@
-"""P-frame reconstruction pipeline (MC -> residual add -> deblock) over many
-frames, for the parity tests and the benchmark.
+"""P-frame reconstruction pipeline (MC -> residual add -> deblock, optionally
+-> SAO) over many frames, for the parity tests and the benchmark.
 
 A batch is a list of P-frames (dicts from workloads.pframe), possibly of
 different sizes and bit depths, as the frames of several concurrent streams.
@@ -34,9 +34,20 @@ def concat_tus(frames):
     return tus, coeff, place, pus
 
 
-def oracle(po, abi, frames):
+SAO_CTB = 64
+
+
+def frame_sao(f, i):
+    """Seeded per-CTB SAO parameters of frame i (CTB 64): (params, cols, rows)."""
+    from test_oracle_sao import random_sao_params
+    rows, cols = -(-int(f["height"]) // SAO_CTB), -(-int(f["width"]) // SAO_CTB)
+    rng = np.random.default_rng([9, i, int(f["width"])])
+    return np.ascontiguousarray(random_sao_params(rng, rows, cols, int(f["bit_depth"]))), cols, rows
+
+
+def oracle(po, abi, frames, with_sao=False):
     """The CPU oracle: orc_mc_batch into fresh planes, orc_recon_batch, then
-    orc_deblock_frames. Returns [[Y, Cb, Cr]] per frame."""
+    orc_deblock_frames (and orc_sao_frame). Returns [[Y, Cb, Cr]] per frame."""
     tus, coeff, place, pus = concat_tus(frames)
     refs = [ref_planes(f) for f in frames]
     outs = [[np.zeros_like(p) for p in r] for r in refs]
@@ -55,21 +66,28 @@ def oracle(po, abi, frames):
                                                        e.ctypes.data, e.shape[1], e.shape[0], f["qp"])
                                          for f, o, e in zip(frames, outs, edges)])
     assert po.lib().orc_deblock_frames(ctypes.addressof(dbf), len(frames)) == 0
+    if with_sao:
+        for i, f in enumerate(frames):
+            prm, cols, rows = frame_sao(f, i)
+            outs[i] = po.sao_frame(outs[i], f["bit_depth"], prm, SAO_CTB, cols, rows, int(f["width"]),
+                                   int(f["height"]))
     return outs
 
 
 class DeviceBatch:
     """Device planes and batched plans for a list of P-frames."""
 
-    def __init__(self, acc, rmh, frames):
+    def __init__(self, acc, rmh, frames, with_sao=False):
+        """with_sao: deblocking writes to scratch planes (out of place) and SAO
+        writes the final samples back into the frame planes."""
         self.hip, abi = rmh.hip, rmh.abi
         hip = self.hip
         self.frames = frames
         self.tus, self.coeff, self.place, self.pus = concat_tus(frames)
         self.keep = []
         self.dst = []  # per frame: [(DeviceBuffer, shape, dtype)] x 3
-        mc_frames, rc_frames, db_frames = [], [], []
-        for f in frames:
+        mc_frames, rc_frames, db_frames, tmp_planes, sao_frames = [], [], [], [], []
+        for i, f in enumerate(frames):
             bd = f["bit_depth"]
             refs, dsts, dbufs = [], [], []
             for p in ref_planes(f):
@@ -87,21 +105,43 @@ class DeviceBatch:
             mc_frames.append(abi.mc_frame(refs, dsts))
             rc_frames.append(abi.ReconFrame((abi.Plane * 3)(*dsts)))
             db_frames.append(abi.dbk_frame(dsts, eb.ptr, e.shape[1], e.shape[0], f["qp"]))
+            if with_sao:
+                tmps = []
+                for (db, shape, dt), d in zip(dbufs, dsts):
+                    tb = hip.DeviceBuffer(int(np.prod(shape)) * np.dtype(dt).itemsize)
+                    self.keep.append(tb)
+                    tmps.append(abi.plane_from_address(tb.ptr, d.width, d.height, d.stride, d.bytes_per_sample, bd))
+                tmp_planes += tmps
+                prm, cols, rows = frame_sao(f, i)
+                pb = hip.DeviceBuffer.from_array(prm)
+                self.keep.append(pb)
+                sao_frames.append(abi.sao_frame(tmps, dsts, pb.ptr, SAO_CTB, cols, rows, int(f["width"]),
+                                                int(f["height"])))
         self.d_coeff = hip.DeviceBuffer.from_array(self.coeff)
         hip.device_synchronize()
         self.mc = acc.mc_plan(mc_frames, self.pus)
         self.recon = acc.recon_plan(self.tus, self.coeff.size, self.place, rc_frames)
-        self.deblock = acc.deblock_plan(db_frames)
+        if with_sao:
+            self.deblock = acc.deblock_to_plan(db_frames, tmp_planes)
+            self.sao = acc.sao_plan(sao_frames)
+        else:
+            self.deblock = acc.deblock_plan(db_frames)
+            self.sao = None
         self.acc = acc
 
     def run(self, stream):
-        """One frame step: the three batched plans on one stream."""
+        """One frame step: the batched plans on one stream."""
         self.mc.run(stream=stream)
         self.recon.run(self.d_coeff.ptr, 0, stream)
         self.deblock.run(stream=stream)
+        if self.sao:
+            self.sao.run(stream=stream)
 
     def graph(self):
-        return self.acc.graph([(self.mc, 0, 0), (self.recon, self.d_coeff.ptr, 0), (self.deblock, 0, 0)])
+        nodes = [(self.mc, 0, 0), (self.recon, self.d_coeff.ptr, 0), (self.deblock, 0, 0)]
+        if self.sao:
+            nodes.append((self.sao, 0, 0))
+        return self.acc.graph(nodes)
 
     def download(self):
         return [[b.download(dt).reshape(shape) for b, shape, dt in fr] for fr in self.dst]
@@ -110,5 +150,6 @@ class DeviceBatch:
         return sum(int(f["width"]) * int(f["height"]) for f in self.frames)
 
     def close(self):
-        for p in (self.mc, self.recon, self.deblock):
-            p.close()
+        for p in (self.mc, self.recon, self.deblock, self.sao):
+            if p:
+                p.close()

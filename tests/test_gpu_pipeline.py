@@ -53,6 +53,66 @@ def test_mixed_streams_one_batch_graph_replays(acc, po):
     b.close()
 
 
+def test_in_loop_chain_with_sao_graph_replays(acc, po):
+    """MC -> residual add -> out-of-place deblocking -> SAO back into the
+    picture, over a 1080p10 frame and an 8-bit 856x480 frame, as one
+    frame-step graph replayed three times."""
+    print("pipeline with SAO", flush=True)
+    rmh = load_package()
+    frames = [W.pframe(61, width=1920, height=1080, bit_depth=10), W.pframe(62, width=856, height=480, bit_depth=8)]
+    want = PL.oracle(po, rmh.abi, frames, with_sao=True)
+    b = PL.DeviceBatch(acc, rmh, frames, with_sao=True)
+    g = b.graph()
+    s = rmh.hip.Stream()
+    for rep in range(3):
+        for fr in b.dst:
+            for buf, _, _ in fr:
+                buf.upload(np.zeros(buf.nbytes, np.uint8))
+        g.launch(s.handle)
+        s.synchronize()
+        _check(b.download(), want)
+    g.close()
+    b.close()
+
+
+def test_out_of_place_deblock_equals_in_place(acc, po):
+    """rmh_deblock_to_plan_create: the source planes are untouched and the
+    destination (with its own, wider pitch) equals the in-place result."""
+    print("out-of-place deblock", flush=True)
+    rmh = load_package()
+    hip, abi = rmh.hip, rmh.abi
+    from frames import dbk_frame_struct
+    d = W.inter_frame(3, width=640, height=360)
+    planes = [np.ascontiguousarray(p) for p in d["ref"]]
+    want = [p.copy() for p in planes]
+    f, keep = dbk_frame_struct(want, d["edges"], 10, d["qp"])
+    acc.deblock([f])
+    srcs, dsts, bufs = [], [], []
+    for p in planes:
+        sb = hip.DeviceBuffer.from_array(p)
+        pitch = p.shape[1] + 24
+        ob = hip.DeviceBuffer(pitch * p.shape[0] * 2)
+        ob.upload(np.full(pitch * p.shape[0], 0x7777, np.uint16))
+        bufs += [sb, ob]
+        srcs.append(abi.plane_from_address(sb.ptr, p.shape[1], p.shape[0], p.shape[1], 2, 10))
+        dsts.append(abi.plane_from_address(ob.ptr, p.shape[1], p.shape[0], pitch, 2, 10))
+    e = np.ascontiguousarray(d["edges"])
+    eb = hip.DeviceBuffer.from_array(e)
+    plan = acc.deblock_to_plan([abi.dbk_frame(srcs, eb.ptr, e.shape[1], e.shape[0], d["qp"])], dsts)
+    s = hip.Stream()
+    plan.run(stream=s.handle)
+    s.synchronize()
+    for c, p in enumerate(planes):
+        pitch = p.shape[1] + 24
+        got = bufs[2 * c + 1].download(np.uint16).reshape(p.shape[0], pitch)
+        assert (got[:, :p.shape[1]] == want[c]).all()
+        assert (got[:, p.shape[1]:] == 0x7777).all()
+        assert (bufs[2 * c].download(np.uint16).reshape(p.shape) == p).all()
+    with pytest.raises(rmh.InvalidInput):
+        acc.deblock_to_plan([abi.dbk_frame(srcs, eb.ptr, e.shape[1], e.shape[0], d["qp"])], srcs)
+    plan.close()
+
+
 def test_recon_plan_equals_residual_plan_plus_add(acc, po):
     """The fused add against the residual plan's int16 output added on the host."""
     rmh = load_package()
