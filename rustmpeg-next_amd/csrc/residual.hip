@@ -571,6 +571,26 @@ __device__ __forceinline__ void load_col(const uint32_t* __restrict__ src, uint3
     }
 }
 
+// load_col into m[0 .. NP).
+template <int NP>
+__device__ __forceinline__ void load_colp(const uint32_t* __restrict__ src, uint32_t* m) {
+    if constexpr (NP <= 2) {
+        const uint2 v = *reinterpret_cast<const uint2*>(src);
+        m[0] = v.x;
+        m[1] = v.y;
+    } else {
+        const uint4* s4 = reinterpret_cast<const uint4*>(src);
+#pragma unroll
+        for (int q = 0; q < NP / 4; q++) {
+            const uint4 v = s4[q];
+            m[4 * q + 0] = v.x;
+            m[4 * q + 1] = v.y;
+            m[4 * q + 2] = v.z;
+            m[4 * q + 3] = v.w;
+        }
+    }
+}
+
 // Rows per prediction-fetch batch of a recon lane (phase C): 8 keeps the
 // kernel at 6 waves per SIMD, where 16 would need 128 VGPRs.
 constexpr int kReconBatch = 8;
@@ -620,10 +640,13 @@ __device__ __forceinline__ void tile_body(const ResidualArgs& a, const TileItem&
     }
     __syncthreads();
 
-    // phase B's lane items: (TU, row y, column chunk), NCH_B chunks
+    // phase B's lane items: (TU, row y, column chunk), NCH_B chunks, in
+    // RB rounds of 256; its matrix columns (NPB pairs each) of every round
+    // are fetched with the coefficients when they fit 16 registers
     constexpr int NCH_B = (256 / (GMAX * H)) > 0 ? 256 / (GMAX * H) : 1;
-    constexpr bool B_ONE = GMAX * H * NCH_B <= 256;
+    constexpr int RB = (GMAX * H * NCH_B + 255) / 256;
     constexpr int NPB = Hc / 2;  // matrix pairs per column
+    constexpr bool B_PRE = RB * NPB <= 16;
     uint32_t mB[16];
 
     // phase A: a lane takes 2 rows x 4 columns and stores 4 row pairs; the
@@ -651,11 +674,15 @@ __device__ __forceinline__ void tile_body(const ResidualArgs& a, const TileItem&
                 }
             }
         }
-        // phase B's matrix column, when phase B is one round: loaded with
-        // the coefficients instead of after the barrier
-        if constexpr (B_ONE) {
-            const int gb = tid / (H * NCH_B);
-            if (gb < G) load_col<NPB>(a.col16 + P[gb].mv + (tid & (H - 1)) * 16, mB);
+        // phase B's matrix columns: loaded with the coefficients instead of
+        // after the barrier
+        if constexpr (B_PRE) {
+#pragma unroll
+            for (int rb = 0; rb < RB; rb++) {
+                const int qb = rb * 256 + tid;
+                const int gb = qb / (H * NCH_B);
+                if (gb < G) load_colp<NPB>(a.col16 + P[gb].mv + (qb & (H - 1)) * 16, mB + rb * NPB);
+            }
         }
         // S fits int16 in every TU of the wave's items (every QP short of the
         // top corner of the range): each level then dequantises with one dot2
@@ -745,9 +772,9 @@ __device__ __forceinline__ void tile_body(const ResidualArgs& a, const TileItem&
                 const int g = q / (H * NCH);
                 if (g < G) {
                     uint32_t m[16];
-                    if constexpr (B_ONE) {
+                    if constexpr (B_PRE) {
 #pragma unroll
-                        for (int k = 0; k < NPB; k++) m[k] = mB[k];
+                        for (int k = 0; k < NPB; k++) m[k] = mB[base / 256 * NPB + k];
                     } else {
                         load_col<NPB>(a.col16 + P[g].mv + y * 16, m);
                     }
@@ -774,7 +801,23 @@ __device__ __forceinline__ void tile_body(const ResidualArgs& a, const TileItem&
         constexpr int LPT = W2 * NRR;      // lanes per TU
         constexpr int ITEMS = GMAX * LPT;  // 2048 / RL, a multiple of 256
         constexpr int NP = Wc / 2;         // matrix pairs per column
+        constexpr int RC = ITEMS / 256;    // rounds
+        // the two matrix columns of every round, fetched together before the
+        // first round when they fit 32 registers (one wait instead of RC)
+        constexpr bool C_PRE = RC > 1 && RC * 2 * NP <= 32;
         const int x2 = tid % W2;
+        uint32_t mC[C_PRE ? RC * 2 * NP : 1];
+        if constexpr (C_PRE) {
+#pragma unroll
+            for (int rc = 0; rc < RC; rc++) {
+                const int g = (rc * 256 + tid) / LPT;
+                if (g < G) {
+                    const int mh = P[g].mh;
+                    load_colp<NP>(a.col16 + mh + (2 * x2) * 16, mC + rc * 2 * NP);
+                    load_colp<NP>(a.col16 + mh + (2 * x2 + 1) * 16, mC + rc * 2 * NP + NP);
+                }
+            }
+        }
         auto hpass = [&](auto nq) {
             constexpr int NQ = decltype(nq)::value;
 #pragma unroll
@@ -785,8 +828,16 @@ __device__ __forceinline__ void tile_body(const ResidualArgs& a, const TileItem&
                 if (g < G) {
                     const TuParams p = P[g];
                     uint32_t m0[16], m1[16];
-                    load_col<NP>(a.col16 + p.mh + (2 * x2) * 16, m0);
-                    load_col<NP>(a.col16 + p.mh + (2 * x2 + 1) * 16, m1);
+                    if constexpr (C_PRE) {
+#pragma unroll
+                        for (int k = 0; k < NP; k++) {
+                            m0[k] = mC[base / 256 * 2 * NP + k];
+                            m1[k] = mC[base / 256 * 2 * NP + NP + k];
+                        }
+                    } else {
+                        load_col<NP>(a.col16 + p.mh + (2 * x2) * 16, m0);
+                        load_col<NP>(a.col16 + p.mh + (2 * x2 + 1) * 16, m1);
+                    }
                     const int shift2 = p.shifts >> 16;
                     const int rnd = 1 << (shift2 - 1);
                     // row rr of the TU; rows rr + j NRR are at constant offsets
