@@ -155,17 +155,21 @@ struct SaoJob {
     uint32_t ctb_cols;             // params row pitch in CTBs
     uint32_t comp;                 // 0 Y, 1 Cb, 2 Cr
     uint32_t bd;                   // bit depth
-    uint32_t blocks_per_row;       // 256-lane segments per group of kSaoRows rows
+    uint32_t blocks_per_row;       // row form: 256-lane segments per group of kSaoRows rows
     uint32_t first_block;          // first workgroup of the job
     uint32_t vec;                  // rows 8-byte aligned: whole-lane vector access
+    uint32_t lanes_across;         // CTB form: lanes across a wave's block (4, 8, 16); 0: row form
+    uint32_t tiles_x;              // CTB form: blocks across the plane
+    uint32_t tiles;                // CTB form: blocks of the plane (4 per workgroup)
 };
 
 // Rows of a plane per SAO workgroup.
 constexpr int kSaoRows = 4;
 
-// block_job[b] is the job of workgroup b; jobs of 8-bit planes own blocks
-// [0, blocks8).
-hipError_t launch_sao(const SaoJob* jobs, const uint32_t* block_job, uint32_t blocks8, uint32_t blocks,
-                      hipStream_t stream);
+// block_job[b] is the job of workgroup b. Workgroups [range[i], range[i+1])
+// hold, in order, row-form 8-bit, CTB-form 8-bit, row-form 16-bit and
+// CTB-form 16-bit planes. The CTB form serves planes whose CTBs are at least
+// 16 samples on a side.
+hipError_t launch_sao(const SaoJob* jobs, const uint32_t* block_job, const uint32_t (&range)[5], hipStream_t stream);
 
 }  // namespace rmh

@@ -128,7 +128,7 @@ struct rmh_plan {
     // sao
     const rmh::SaoJob* d_sao_jobs = nullptr;
     const uint32_t* d_sao_block_job = nullptr;
-    uint32_t n_sao_jobs = 0, sao_blocks8 = 0, sao_blocks = 0;
+    uint32_t n_sao_jobs = 0, sao_range[5] = {};
 };
 
 struct rmh_graph {
@@ -576,18 +576,21 @@ rmh_status validate_sao(rmh_ctx* ctx, const rmh_sao_frame* frames, size_t n_fram
 struct SaoJobs {
     std::vector<rmh::SaoJob> jobs;
     std::vector<uint32_t> block_job;  // job of each workgroup
-    uint32_t blocks8 = 0, blocks = 0;
+    uint32_t range[5] = {};           // workgroup ranges of the four job groups (launch_sao)
 };
 
 SaoJobs sao_jobs(const rmh_sao_frame* frames, size_t n_frames, const std::vector<const rmh_sao_param*>& params,
                  const std::vector<std::array<rmh_plane, 6>>& planes) {
     SaoJobs out;
-    for (int pass = 0; pass < 2; pass++)
+    // groups: row form 8-bit, CTB form 8-bit, row form 16-bit, CTB form 16-bit
+    for (int group = 0; group < 4; group++) {
         for (size_t f = 0; f < n_frames; f++)
             for (int c = 0; c < 3; c++) {
                 const rmh_plane& src = planes[f][c];
-                if ((src.bytes_per_sample == 1) != (pass == 0)) continue;
                 const rmh_sao_frame& fr = frames[f];
+                const uint32_t cs = c ? fr.ctb_size / 2 : fr.ctb_size;
+                const bool ctb_form = cs >= 16;
+                if ((src.bytes_per_sample == 1) != (group < 2) || ctb_form != (group % 2 == 1)) continue;
                 rmh::SaoJob j{};
                 j.src = src.data;
                 j.dst = planes[f][3 + c].data;
@@ -597,21 +600,31 @@ SaoJobs sao_jobs(const rmh_sao_frame* frames, size_t n_frames, const std::vector
                 j.stride = src.stride;
                 j.vis_w = c ? (fr.vis_w + 1) / 2 : fr.vis_w;
                 j.vis_h = c ? (fr.vis_h + 1) / 2 : fr.vis_h;
-                j.ctb = c ? fr.ctb_size / 2 : fr.ctb_size;
+                j.ctb = cs;
                 j.ctb_cols = fr.ctb_cols;
                 j.comp = (uint32_t)c;
                 j.bd = src.bit_depth;
-                const uint32_t k = 8 / src.bytes_per_sample;  // samples per lane
-                j.blocks_per_row = ((src.width + k - 1) / k + 255) / 256;
-                j.first_block = out.blocks;
+                j.first_block = (uint32_t)out.block_job.size();
                 j.vec = ((uintptr_t)src.data % 8 == 0 && (uintptr_t)j.dst % 8 == 0 &&
                          ((uint64_t)src.stride * src.bytes_per_sample) % 8 == 0);
-                const uint32_t nb = j.blocks_per_row * ((src.height + rmh::kSaoRows - 1) / rmh::kSaoRows);
-                out.blocks += nb;
+                uint32_t nb;
+                if (ctb_form) {
+                    // a wave's block: la lanes of 4 samples across, 64 / la rows
+                    const uint32_t la = std::min<uint32_t>(16, cs / 4), tw = 4 * la, th = 64 / la;
+                    j.lanes_across = la;
+                    j.tiles_x = (src.width + tw - 1) / tw;
+                    j.tiles = j.tiles_x * ((src.height + th - 1) / th);
+                    nb = (j.tiles + 3) / 4;
+                } else {
+                    const uint32_t k = 8 / src.bytes_per_sample;  // samples per lane
+                    j.blocks_per_row = ((src.width + k - 1) / k + 255) / 256;
+                    nb = j.blocks_per_row * ((src.height + rmh::kSaoRows - 1) / rmh::kSaoRows);
+                }
                 out.block_job.insert(out.block_job.end(), nb, (uint32_t)out.jobs.size());
                 out.jobs.push_back(j);
-                if (pass == 0) out.blocks8 = out.blocks;
             }
+        out.range[group + 1] = (uint32_t)out.block_job.size();
+    }
     return out;
 }
 
@@ -931,7 +944,7 @@ rmh_status run_plan_locked_free(rmh_plan* plan, const int16_t* d_coeff, int16_t*
                              : rmh::launch_mc(plan->d_frames, plan->d_items, (uint32_t)plan->n_items,
                                               (uint32_t)plan->n_items8, s);
     } else if (plan->kind == kSao) {
-        e = rmh::launch_sao(plan->d_sao_jobs, plan->d_sao_block_job, plan->sao_blocks8, plan->sao_blocks, s);
+        e = rmh::launch_sao(plan->d_sao_jobs, plan->d_sao_block_job, plan->sao_range, s);
     } else {
         e = rmh::launch_deblock(plan->d_dbk, plan->d_shadows, plan->d_tile_start, plan->d_frame_of, (uint32_t)plan->n_dbk,
                                 plan->total_tiles, plan->tiles8, plan->max_rows, plan->dbk_copy, s);
@@ -1767,8 +1780,7 @@ RMH_EXPORT rmh_status rmh_sao_plan_create(rmh_ctx* ctx, const rmh_sao_frame* fra
     }
     const SaoJobs sj = sao_jobs(frames, n_frames, prm, planes);
     p->n_sao_jobs = (uint32_t)sj.jobs.size();
-    p->sao_blocks8 = sj.blocks8;
-    p->sao_blocks = sj.blocks;
+    memcpy(p->sao_range, sj.range, sizeof sj.range);
     if (!sj.jobs.empty()) {
         const size_t jb = sj.jobs.size() * sizeof(rmh::SaoJob), bb = sj.block_job.size() * sizeof(uint32_t);
         hipError_t e = hipMalloc(&p->d_mem, aligned(jb) + bb);
@@ -1810,13 +1822,14 @@ RMH_EXPORT rmh_status rmh_sao(rmh_ctx* ctx, const rmh_sao_frame* frames, size_t 
         }
     }
     need += aligned(n_frames * 3 * sizeof(rmh::SaoJob));
-    for (size_t f = 0; f < n_frames; f++)
-        for (int c = 0; c < 3; c++) {
-            const rmh_plane& p = frames[f].src[c];
-            const uint32_t k = 8 / p.bytes_per_sample;
-            need += aligned((size_t)(((p.width + k - 1) / k + 255) / 256) *
-                            ((p.height + rmh::kSaoRows - 1) / rmh::kSaoRows) * sizeof(uint32_t));
-        }
+    {
+        // the work table's size (addresses do not change it)
+        std::vector<const rmh_sao_param*> p0(n_frames, nullptr);
+        std::vector<std::array<rmh_plane, 6>> q0(n_frames);
+        for (size_t f = 0; f < n_frames; f++)
+            for (int c = 0; c < 3; c++) q0[f][c] = q0[f][3 + c] = frames[f].src[c];
+        need += aligned(sao_jobs(frames, n_frames, p0, q0).block_job.size() * sizeof(uint32_t));
+    }
     hipError_t e = ctx->dev.reserve(need);
     if (e != hipSuccess) return fail_locked(ctx, RMH_LIMIT_EXCEEDED, std::string("device allocation: ") + hipGetErrorString(e));
     e = ctx->host.reserve(hneed);
@@ -1853,7 +1866,7 @@ RMH_EXPORT rmh_status rmh_sao(rmh_ctx* ctx, const rmh_sao_frame* frames, size_t 
         (e = hipMemcpyAsync(d_bj, sj.block_job.data(), sj.block_job.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
                             s)) != hipSuccess)
         return hip_fail_locked(ctx, e, "upload");
-    e = rmh::launch_sao(d_jobs, d_bj, sj.blocks8, sj.blocks, s);
+    e = rmh::launch_sao(d_jobs, d_bj, sj.range, s);
     if (e != hipSuccess) return hip_fail_locked(ctx, e, "kernel launch");
     std::vector<void*> hp;
     for (size_t f = 0; f < n_frames; f++)

@@ -163,14 +163,223 @@ __global__ __launch_bounds__(256) void sao_kernel(const SaoJob* __restrict__ job
     }
 }
 
+// ---------------------------------------------------------------------------
+// CTB-uniform form, for planes whose CTBs are at least 16 samples on a side:
+// a wave covers an LA*4 x (64 / LA) block of one CTB (LA = min(16, CTB / 4)
+// lanes across, 4 samples each), so the SAO type, class and offsets are
+// uniform over the wave (scalar loads and branches) and every edge class runs
+// its own straight-line code. Neighbour samples come from the lanes beside,
+// above and below by shuffles, or from memory at the block's edges. Offsets
+// that fit int8 are looked up with one byte permute.
+// ---------------------------------------------------------------------------
+
+// The lane's 4 samples of row y (zeros outside the plane) into v[0..3].
+template <typename T>
+__device__ __forceinline__ void sao_load4(const RMH_GLOBAL T* plane, size_t pitch, int x0, int y, int width, int height,
+                                          bool vec, int (&v)[4]) {
+    if (y < 0 || y >= height || x0 >= width) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) v[k] = 0;
+        return;
+    }
+    const RMH_GLOBAL T* row = plane + (size_t)y * pitch;
+    if (vec && x0 + 4 <= width) {
+        if constexpr (sizeof(T) == 2) {
+            const u32x2 d = *reinterpret_cast<const RMH_GLOBAL u32x2*>(row + x0);
+            v[0] = (int)(d.x & 0xffff);
+            v[1] = (int)(d.x >> 16);
+            v[2] = (int)(d.y & 0xffff);
+            v[3] = (int)(d.y >> 16);
+        } else {
+            const uint32_t d = *reinterpret_cast<const RMH_GLOBAL uint32_t*>(row + x0);
+#pragma unroll
+            for (int k = 0; k < 4; k++) v[k] = (int)((d >> (8 * k)) & 255);
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < 4; k++) v[k] = x0 + k < width ? (int)row[x0 + k] : 0;
+    }
+}
+
+// w[0..5] = samples x0 - 1 .. x0 + 4 of row y: the lane's own four, the two
+// beside it from the neighbouring lanes of the block row (lx - 1, lx + 1),
+// or from memory at the block's left and right edges. Called by every lane
+// of the wave together.
+template <typename T, int LA>
+__device__ __forceinline__ void sao_window(const RMH_GLOBAL T* plane, size_t pitch, int x0, int y, int width,
+                                           int height, bool vec, int lx, int (&w)[6]) {
+    int v[4];
+    sao_load4<T>(plane, pitch, x0, y, width, height, vec, v);
+#pragma unroll
+    for (int k = 0; k < 4; k++) w[k + 1] = v[k];
+    const int left = __shfl_up(v[3], 1, 64), right = __shfl_down(v[0], 1, 64);
+    const bool row_ok = y >= 0 && y < height;
+    const RMH_GLOBAL T* row = plane + (size_t)(row_ok ? y : 0) * pitch;
+    w[0] = lx > 0 ? left : (row_ok && x0 > 0 && x0 - 1 < width ? (int)row[x0 - 1] : 0);
+    w[5] = lx < LA - 1 ? right : (row_ok && x0 + 4 < width ? (int)row[x0 + 4] : 0);
+    if (x0 + 4 >= width) w[5] = 0;
+}
+
+__device__ __forceinline__ int sext8(uint32_t v) { return (int)(int8_t)(uint8_t)(v & 255); }
+
+// One edge class over the lane's 4 samples; w0 is the window of row y, the
+// windows of rows y - 1 and y + 1 are loaded here for the classes that need
+// them (every lane of the wave takes the same class).
+template <int DX, int DY, typename T, int LA>
+__device__ __forceinline__ void sao_edge(const RMH_GLOBAL T* src, size_t pitch, int x0, int y, int width, int height,
+                                         bool vec, int lx, const int (&w0)[6], bool interior, int vis_w, int pwid,
+                                         int ph, bool small, uint32_t lut_lo, uint32_t lut_hi, const int (&o)[4],
+                                         int maxv, int (&out)[4]) {
+    int wa[6], wb[6];
+    if constexpr (DY == 0) {
+#pragma unroll
+        for (int k = 0; k < 6; k++) wa[k] = wb[k] = w0[k];
+    } else {
+        sao_window<T, LA>(src, pitch, x0, y - DY, width, height, vec, lx, wa);
+        sao_window<T, LA>(src, pitch, x0, y + DY, width, height, vec, lx, wb);
+    }
+    const bool rows_ok = y - DY >= 0 && y + DY < ph;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const int c = w0[k + 1], ea = wa[k + 1 - DX], eb = wb[k + 1 + DX];
+        const int e = min(max(c - ea, -1), 1) + min(max(c - eb, -1), 1);  // edge_idx - 2
+        int off;
+        if (small) {
+            off = sext8(__builtin_amdgcn_perm(lut_hi, lut_lo, (uint32_t)(e + 2) | 0x0c0c0c00u));
+        } else {
+            off = e == -2 ? o[0] : e == -1 ? o[1] : e == 1 ? o[2] : e == 2 ? o[3] : 0;
+        }
+        const int x = x0 + k;
+        const bool ok = interior || (rows_ok && x < vis_w && x - DX >= 0 && x - DX < pwid && x + DX >= 0 && x + DX < pwid);
+        if (ok) out[k] = min(max(c + off, 0), maxv);
+    }
+}
+
+template <typename T, int LA>
+__device__ __forceinline__ void sao_ctb_block(const SaoJob& J, uint32_t tile) {
+    constexpr int TH = 64 / LA, TW = 4 * LA;
+    const int lane = threadIdx.x & 63, lx = lane % LA, ly = lane / LA;
+    const int tx = (int)(tile % J.tiles_x), ty = (int)(tile / J.tiles_x);
+    const int X0 = tx * TW, Y0 = ty * TH;
+    const int x0 = X0 + 4 * lx, y = Y0 + ly;
+    const int width = (int)J.width, height = (int)J.height;
+    const RMH_GLOBAL T* src = gsrc<T>(J.src);
+    RMH_GLOBAL T* dst = gdst<T>(J.dst);
+    const size_t pitch = J.stride;
+    const bool vec = J.vec != 0;
+    const int vis_w = (int)J.vis_w, vis_h = (int)J.vis_h;
+
+    int w0[6];
+    sao_window<T, LA>(src, pitch, x0, y, width, height, vec, lx, w0);
+    int out[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) out[k] = w0[k + 1];
+
+    if (X0 < vis_w && Y0 < vis_h) {
+        // the block's CTB parameters, uniform over the wave
+        const int cs = (int)J.ctb;
+        const RMH_GLOBAL int32_t* pw5 =
+            gsrc<int32_t>(J.params) + (((size_t)(Y0 / cs) * J.ctb_cols + (size_t)(X0 / cs)) * 3 + J.comp) * 5;
+        const int head = __builtin_amdgcn_readfirstlane(pw5[0]);
+        const int type = head & 255, boc = (head >> 8) & 255;
+        int o[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) o[k] = __builtin_amdgcn_readfirstlane(pw5[1 + k]);
+        const bool small = o[0] >= -128 && o[0] <= 127 && o[1] >= -128 && o[1] <= 127 && o[2] >= -128 &&
+                           o[2] <= 127 && o[3] >= -128 && o[3] <= 127;
+        const int maxv = (1 << J.bd) - 1;
+        const bool in_vis = X0 + TW <= vis_w && Y0 + TH <= vis_h;
+        if (type == RMH_SAO_BAND) {
+            const int bd = J.bd < 8 ? 8 : (J.bd > 12 ? 12 : (int)J.bd);
+            const int shift = bd - 5;
+            // bytes 0-3: offsets of the four bands; byte 4 (lut_hi): 0
+            const uint32_t lut_lo = (uint32_t)(o[0] & 255) | (uint32_t)(o[1] & 255) << 8 | (uint32_t)(o[2] & 255) << 16 |
+                                    (uint32_t)(o[3] & 255) << 24;
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const int sv = w0[k + 1];
+                const int band = ((sv >> shift) - boc) & 31;
+                const int sel = min(band, 4);
+                int off;
+                if (small)
+                    off = sext8(__builtin_amdgcn_perm(0u, lut_lo, (uint32_t)sel | 0x0c0c0c00u));
+                else
+                    off = sel == 0 ? o[0] : sel == 1 ? o[1] : sel == 2 ? o[2] : sel == 3 ? o[3] : 0;
+                if ((in_vis || (x0 + k < vis_w && y < vis_h)) && band < 4) out[k] = min(max(sv + off, 0), maxv);
+            }
+        } else if (type == RMH_SAO_EDGE) {
+            const int cls = boc & 3;
+            const int pwid = min(vis_w, width), ph = min(vis_h, height);
+            // edge categories 1..4 (edge_idx 0, 1, 3, 4) as bytes 0, 1, 3, 4; byte 2: 0
+            const uint32_t lut_lo = (uint32_t)(o[0] & 255) | (uint32_t)(o[1] & 255) << 8 | (uint32_t)(o[2] & 255) << 24;
+            const uint32_t lut_hi = (uint32_t)(o[3] & 255);
+            // every sample of the block and both neighbours inside the visible picture
+            const int dxa = cls == 1 ? 0 : 1, dya = cls == 0 ? 0 : 1;
+            const bool interior = in_vis && X0 - dxa >= 0 && X0 + TW + dxa <= pwid && Y0 - dya >= 0 &&
+                                  Y0 + TH + dya <= ph;
+            const bool live_y = y < vis_h;
+            if (cls == 0)
+                sao_edge<1, 0, T, LA>(src, pitch, x0, y, width, height, vec, lx, w0, interior, live_y ? vis_w : 0,
+                                      pwid, ph, small, lut_lo, lut_hi, o, maxv, out);
+            else if (cls == 1)
+                sao_edge<0, 1, T, LA>(src, pitch, x0, y, width, height, vec, lx, w0, interior, live_y ? vis_w : 0,
+                                      pwid, ph, small, lut_lo, lut_hi, o, maxv, out);
+            else if (cls == 2)
+                sao_edge<1, 1, T, LA>(src, pitch, x0, y, width, height, vec, lx, w0, interior, live_y ? vis_w : 0,
+                                      pwid, ph, small, lut_lo, lut_hi, o, maxv, out);
+            else
+                sao_edge<-1, 1, T, LA>(src, pitch, x0, y, width, height, vec, lx, w0, interior, live_y ? vis_w : 0,
+                                       pwid, ph, small, lut_lo, lut_hi, o, maxv, out);
+        }
+    }
+
+    if (y >= height || x0 >= width) return;
+    RMH_GLOBAL T* drow = dst + (size_t)y * pitch;
+    if (vec && x0 + 4 <= width) {
+        if constexpr (sizeof(T) == 2) {
+            *reinterpret_cast<RMH_GLOBAL u32x2*>(drow + x0) =
+                u32x2{(uint32_t)out[0] | (uint32_t)out[1] << 16, (uint32_t)out[2] | (uint32_t)out[3] << 16};
+        } else {
+            *reinterpret_cast<RMH_GLOBAL uint32_t*>(drow + x0) =
+                (uint32_t)out[0] | (uint32_t)out[1] << 8 | (uint32_t)out[2] << 16 | (uint32_t)out[3] << 24;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            if (x0 + k < width) drow[x0 + k] = (T)out[k];
+    }
+}
+
+// Workgroup b: blocks 4 (b - first_block) + wave of the job block_job[b].
+template <typename T>
+__global__ __launch_bounds__(256) void sao_ctb_kernel(const SaoJob* __restrict__ jobs,
+                                                      const uint32_t* __restrict__ block_job, uint32_t block0) {
+    const uint32_t b = blockIdx.x + block0;
+    const SaoJob& J = jobs[block_job[b]];
+    const uint32_t tile = (b - J.first_block) * 4 + (uint32_t)__builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (tile >= J.tiles) return;  // the whole wave
+    switch (J.lanes_across) {
+        case 4: sao_ctb_block<T, 4>(J, tile); break;
+        case 8: sao_ctb_block<T, 8>(J, tile); break;
+        default: sao_ctb_block<T, 16>(J, tile); break;
+    }
+}
+
 }  // namespace
 
-hipError_t launch_sao(const SaoJob* jobs, const uint32_t* block_job, uint32_t blocks8, uint32_t blocks,
-                      hipStream_t stream) {
-    if (blocks8 > 0)
-        hipLaunchKernelGGL(sao_kernel<uint8_t>, dim3(blocks8), dim3(256), 0, stream, jobs, block_job, 0u);
-    if (blocks > blocks8)
-        hipLaunchKernelGGL(sao_kernel<uint16_t>, dim3(blocks - blocks8), dim3(256), 0, stream, jobs, block_job, blocks8);
+hipError_t launch_sao(const SaoJob* jobs, const uint32_t* block_job, const uint32_t (&range)[5], hipStream_t stream) {
+    // block ranges: [0] row form 8-bit, [1] CTB form 8-bit, [2] row form 16-bit, [3] CTB form 16-bit
+    if (range[1] > range[0])
+        hipLaunchKernelGGL(sao_kernel<uint8_t>, dim3(range[1] - range[0]), dim3(256), 0, stream, jobs, block_job, range[0]);
+    if (range[2] > range[1])
+        hipLaunchKernelGGL(sao_ctb_kernel<uint8_t>, dim3(range[2] - range[1]), dim3(256), 0, stream, jobs, block_job,
+                           range[1]);
+    if (range[3] > range[2])
+        hipLaunchKernelGGL(sao_kernel<uint16_t>, dim3(range[3] - range[2]), dim3(256), 0, stream, jobs, block_job,
+                           range[2]);
+    if (range[4] > range[3])
+        hipLaunchKernelGGL(sao_ctb_kernel<uint16_t>, dim3(range[4] - range[3]), dim3(256), 0, stream, jobs, block_job,
+                           range[3]);
     return hipGetLastError();
 }
 
