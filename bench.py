@@ -339,7 +339,7 @@ def bench_sao(rmh, acc, W, po, n_frames, steps, warmup):
     out = {"workload": "%d deblocked 1920x1080 10-bit 4:2:0 pictures, CTB 64, per-CTB band or edge offset (random "
                        "class, band and offsets per component), output to separate planes, one launch" % n_frames,
            "value": round(px / (ms / 1e3) / 1e6, 1), "unit": "Mpix/s", "ms_per_step": round(ms, 4),
-           "roofline": dict(stage_roofline(plan, ms), kernel="sao_kernel<uint16_t>")}
+           "roofline": dict(stage_roofline(plan, ms), kernel="sao_ctb_kernel<uint16_t>")}
     log("sao: %.4f ms/step %.1f Mpix/s (%.1f GB/s)" % (ms, out["value"], out["roofline"]["achieved_gbs"]))
     plan.close()
     for b in bufs:

@@ -160,11 +160,15 @@ struct SaoJob {
     uint32_t vec;                  // rows 8-byte aligned: whole-lane vector access
     uint32_t lanes_across;         // CTB form: lanes across a wave's block (4, 8, 16); 0: row form
     uint32_t tiles_x;              // CTB form: blocks across the plane
-    uint32_t tiles;                // CTB form: blocks of the plane (4 per workgroup)
+    uint32_t tiles;                // CTB form: blocks of the plane (4 kSaoBlocksPerWave per workgroup)
+    uint32_t ctb_log2;             // log2 of ctb
+    double inv_tiles_x;            // CTB form: 1 / tiles_x
 };
 
-// Rows of a plane per SAO workgroup.
+// Rows of a plane per workgroup of the SAO row form; blocks per wave of the
+// CTB form.
 constexpr int kSaoRows = 4;
+constexpr int kSaoBlocksPerWave = 1;
 
 // block_job[b] is the job of workgroup b. Workgroups [range[i], range[i+1])
 // hold, in order, row-form 8-bit, CTB-form 8-bit, row-form 16-bit and

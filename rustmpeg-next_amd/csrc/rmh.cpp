@@ -601,6 +601,7 @@ SaoJobs sao_jobs(const rmh_sao_frame* frames, size_t n_frames, const std::vector
                 j.vis_w = c ? (fr.vis_w + 1) / 2 : fr.vis_w;
                 j.vis_h = c ? (fr.vis_h + 1) / 2 : fr.vis_h;
                 j.ctb = cs;
+                j.ctb_log2 = (uint32_t)__builtin_ctz(cs);
                 j.ctb_cols = fr.ctb_cols;
                 j.comp = (uint32_t)c;
                 j.bd = src.bit_depth;
@@ -614,7 +615,8 @@ SaoJobs sao_jobs(const rmh_sao_frame* frames, size_t n_frames, const std::vector
                     j.lanes_across = la;
                     j.tiles_x = (src.width + tw - 1) / tw;
                     j.tiles = j.tiles_x * ((src.height + th - 1) / th);
-                    nb = (j.tiles + 3) / 4;
+                    j.inv_tiles_x = 1.0 / (double)j.tiles_x;
+                    nb = (j.tiles + 4 * rmh::kSaoBlocksPerWave - 1) / (4 * rmh::kSaoBlocksPerWave);
                 } else {
                     const uint32_t k = 8 / src.bytes_per_sample;  // samples per lane
                     j.blocks_per_row = ((src.width + k - 1) / k + 255) / 256;

@@ -74,9 +74,9 @@ __global__ __launch_bounds__(256) void sao_kernel(const SaoJob* __restrict__ job
     // band_or_class << 8, offsets[4])
     int type = RMH_SAO_OFF, boc = 0, o0 = 0, o1 = 0, o2 = 0, o3 = 0;
     if (live && y0 < vis_h && x0 < vis_w) {
-        const int cs = (int)J.ctb;
+        const int cl = (int)J.ctb_log2;
         const RMH_GLOBAL int32_t* pw5 =
-            gsrc<int32_t>(J.params) + (((size_t)(y0 / cs) * J.ctb_cols + (size_t)(x0 / cs)) * 3 + J.comp) * 5;
+            gsrc<int32_t>(J.params) + (((size_t)(y0 >> cl) * J.ctb_cols + (size_t)(x0 >> cl)) * 3 + J.comp) * 5;
         const int32_t head = pw5[0];
         type = head & 255;
         boc = (head >> 8) & 255;
@@ -259,7 +259,19 @@ template <typename T, int LA>
 __device__ __forceinline__ void sao_ctb_block(const SaoJob& J, uint32_t tile) {
     constexpr int TH = 64 / LA, TW = 4 * LA;
     const int lane = threadIdx.x & 63, lx = lane % LA, ly = lane / LA;
-    const int tx = (int)(tile % J.tiles_x), ty = (int)(tile / J.tiles_x);
+    // tile = ty * tiles_x + tx without an integer division (the scalar unit has
+    // none): a double-precision estimate, exact after one correction
+    const uint32_t tiles_x = J.tiles_x;
+    uint32_t tyu = (uint32_t)((double)tile * J.inv_tiles_x);
+    int r = (int)(tile - tyu * tiles_x);
+    if (r < 0) {
+        tyu--;
+        r += (int)tiles_x;
+    } else if (r >= (int)tiles_x) {
+        tyu++;
+        r -= (int)tiles_x;
+    }
+    const int tx = __builtin_amdgcn_readfirstlane(r), ty = __builtin_amdgcn_readfirstlane((int)tyu);
     const int X0 = tx * TW, Y0 = ty * TH;
     const int x0 = X0 + 4 * lx, y = Y0 + ly;
     const int width = (int)J.width, height = (int)J.height;
@@ -277,9 +289,9 @@ __device__ __forceinline__ void sao_ctb_block(const SaoJob& J, uint32_t tile) {
 
     if (X0 < vis_w && Y0 < vis_h) {
         // the block's CTB parameters, uniform over the wave
-        const int cs = (int)J.ctb;
+        const int cl = (int)J.ctb_log2;
         const RMH_GLOBAL int32_t* pw5 =
-            gsrc<int32_t>(J.params) + (((size_t)(Y0 / cs) * J.ctb_cols + (size_t)(X0 / cs)) * 3 + J.comp) * 5;
+            gsrc<int32_t>(J.params) + (((size_t)(Y0 >> cl) * J.ctb_cols + (size_t)(X0 >> cl)) * 3 + J.comp) * 5;
         const int head = __builtin_amdgcn_readfirstlane(pw5[0]);
         const int type = head & 255, boc = (head >> 8) & 255;
         int o[4];
@@ -350,18 +362,24 @@ __device__ __forceinline__ void sao_ctb_block(const SaoJob& J, uint32_t tile) {
     }
 }
 
-// Workgroup b: blocks 4 (b - first_block) + wave of the job block_job[b].
+// Workgroup b: wave w runs blocks kSaoBlocksPerWave * (4 (b - first_block)
+// + w) + i of the job block_job[b].
 template <typename T>
 __global__ __launch_bounds__(256) void sao_ctb_kernel(const SaoJob* __restrict__ jobs,
                                                       const uint32_t* __restrict__ block_job, uint32_t block0) {
     const uint32_t b = blockIdx.x + block0;
     const SaoJob& J = jobs[block_job[b]];
-    const uint32_t tile = (b - J.first_block) * 4 + (uint32_t)__builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    if (tile >= J.tiles) return;  // the whole wave
-    switch (J.lanes_across) {
-        case 4: sao_ctb_block<T, 4>(J, tile); break;
-        case 8: sao_ctb_block<T, 8>(J, tile); break;
-        default: sao_ctb_block<T, 16>(J, tile); break;
+    const uint32_t first =
+        ((b - J.first_block) * 4 + (uint32_t)__builtin_amdgcn_readfirstlane(threadIdx.x >> 6)) * kSaoBlocksPerWave;
+#pragma unroll
+    for (int i = 0; i < kSaoBlocksPerWave; i++) {
+        const uint32_t tile = first + i;
+        if (tile >= J.tiles) return;  // the whole wave
+        switch (J.lanes_across) {
+            case 4: sao_ctb_block<T, 4>(J, tile); break;
+            case 8: sao_ctb_block<T, 8>(J, tile); break;
+            default: sao_ctb_block<T, 16>(J, tile); break;
+        }
     }
 }
 
