@@ -790,12 +790,13 @@ __device__ __forceinline__ void tile_body(const ResidualArgs& a, const TileItem&
 
     if constexpr (!RECON) {
         // phase C: horizontal pass, lane = (TU, column pair, row group); each
-        // lane produces RL rows of two adjacent columns per round and stores
-        // them as saturated int16 pairs straight to HBM. The W / 2 * NRR lanes
-        // of a TU cover NRR consecutive rows per store, at least 64 bytes of
-        // one TU block where it is that large.
+        // lane produces RL rows (8, or all rows of a TU of fewer) of two
+        // adjacent columns per round and stores them as saturated int16
+        // pairs straight to HBM. Long rounds beat longer store runs: with
+        // 8 x 8 TUs, 4 rounds of 2 rows (64-byte runs) took 49.5 us per
+        // 26 M samples, 1 round of 8 rows (16-byte runs) 36.1 us.
         constexpr int W2 = W / 2;
-        constexpr int NRR0 = (H / 8 > 32 / W) ? H / 8 : 32 / W;
+        constexpr int NRR0 = H / 8;
         constexpr int NRR = NRR0 < 1 ? 1 : (NRR0 > H ? H : NRR0);
         constexpr int RL = H / NRR;        // rows per lane per round (<= 8)
         constexpr int LPT = W2 * NRR;      // lanes per TU
