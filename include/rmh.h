@@ -139,8 +139,8 @@ rmh_status rmh_residual_plan_run_range(rmh_plan* plan, size_t first_tu, size_t c
 /* Autotuner (no reference counterpart; the batching of the north star): for
  * each TU size class of the plan (largest first), times the plan's tiled
  * launch with that class's tiles holding 1, 1/2, 1/4 and 1/8 of the default
- * TUs per workgroup on the given buffers (reps launches each, 0 selects 5)
- * and keeps a change that saves at least 1 %. Results are
+ * TUs per workgroup on the given buffers (the fastest of three batches of
+ * reps launches, 0 selects 5) and keeps a change that saves at least 2 %. Results are
  * unchanged (tiling does not change the arithmetic); the buffers are
  * overwritten. Synchronous. split_out (49 bytes, [log2 w][log2 h], may be
  * NULL) receives the chosen split per class. */

@@ -1193,8 +1193,9 @@ RMH_EXPORT rmh_status rmh_residual_plan_autotune(rmh_plan* plan, const int16_t* 
     hipError_t e = hipEventCreate(&e0);
     if (e == hipSuccess) e = hipEventCreate(&e1);
     // coordinate descent over the classes (largest first): each candidate split
-    // of one class is timed on the whole plan's launch; a change is kept when
-    // it saves at least 1 %
+    // of one class is timed on the whole plan's launch as the fastest of three
+    // batches of reps launches (run-to-run noise is about 2 %); a change is
+    // kept when it saves at least 2 %
     auto time_plan = [&](const uint8_t (*split)[7], float* ms) -> hipError_t {
         std::vector<rmh::TileItem> tiles;
         uint32_t lds = 0;
@@ -1202,12 +1203,18 @@ RMH_EXPORT rmh_status rmh_residual_plan_autotune(rmh_plan* plan, const int16_t* 
         hipError_t r = hipMemcpy(plan->d_tiles, tiles.data(), tiles.size() * sizeof(rmh::TileItem), hipMemcpyHostToDevice);
         for (uint32_t k = 0; k < 2 && r == hipSuccess; k++)
             r = rmh::launch_residual_tiles(a, plan->d_tiles, (uint32_t)tiles.size(), lds, s);
-        if (r == hipSuccess) r = hipEventRecord(e0, s);
-        for (uint32_t k = 0; k < reps && r == hipSuccess; k++)
-            r = rmh::launch_residual_tiles(a, plan->d_tiles, (uint32_t)tiles.size(), lds, s);
-        if (r == hipSuccess) r = hipEventRecord(e1, s);
-        if (r == hipSuccess) r = hipEventSynchronize(e1);
-        if (r == hipSuccess) r = hipEventElapsedTime(ms, e0, e1);
+        float fastest = 0.0f;
+        for (int batch = 0; batch < 3 && r == hipSuccess; batch++) {
+            r = hipEventRecord(e0, s);
+            for (uint32_t k = 0; k < reps && r == hipSuccess; k++)
+                r = rmh::launch_residual_tiles(a, plan->d_tiles, (uint32_t)tiles.size(), lds, s);
+            if (r == hipSuccess) r = hipEventRecord(e1, s);
+            if (r == hipSuccess) r = hipEventSynchronize(e1);
+            float t = 0.0f;
+            if (r == hipSuccess) r = hipEventElapsedTime(&t, e0, e1);
+            if (r == hipSuccess && (batch == 0 || t < fastest)) fastest = t;
+        }
+        *ms = fastest;
         return r;
     };
     uint8_t cur[7][7];
@@ -1225,7 +1232,7 @@ RMH_EXPORT rmh_status rmh_residual_plan_autotune(rmh_plan* plan, const int16_t* 
                 cur[lw][lh] = (uint8_t)sp;
                 float ms = 0.0f;
                 e = time_plan(cur, &ms);
-                if (e == hipSuccess && ms < 0.99f * best) {
+                if (e == hipSuccess && ms < 0.98f * best) {
                     best = ms;
                     choice = (uint8_t)sp;
                 }
