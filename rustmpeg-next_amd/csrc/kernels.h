@@ -80,9 +80,11 @@ hipError_t launch_residual_tiles(const ResidualArgs& a, const TileItem* items, u
 
 // TUs per full tile of class (lw, lh) and the LDS bytes such a tile needs
 // (the kernel lays LDS out for a full tile whatever the tile's count).
-inline int residual_tile_tus(int lw, int lh) {
-    int g = 4096 >> (lw + lh);
-    return g > 0 ? g : 1;
+// Lanes of a tiled-kernel workgroup; a full tile holds 16 samples per lane.
+constexpr int kTileThreads = 64;
+
+constexpr int residual_tile_tus(int lw, int lh) {
+    return ((kTileThreads * 16) >> (lw + lh)) > 0 ? (kTileThreads * 16) >> (lw + lh) : 1;
 }
 // Dword pitch of a coefficient column (lc = log2 of the coded height) or an
 // intermediate row (lc = log2 of the coded width) in the tiled kernel's LDS:

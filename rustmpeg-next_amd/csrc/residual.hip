@@ -605,7 +605,7 @@ __device__ __forceinline__ void tile_body(const ResidualArgs& a, const TileItem&
     // store, or 32 consecutive rows of a phase B store, fall in distinct
     // banks; even, for 8-byte reads
     constexpr int CP = residual_tile_pitch(LHC), TP = residual_tile_pitch(LWC);
-    constexpr int GMAX = (4096 >> (LW + LH)) > 0 ? (4096 >> (LW + LH)) : 1;
+    constexpr int GMAX = residual_tile_tus(LW, LH);
     const int G = it.count;
     // LDS: the coefficient columns, the intermediate rows, the per-TU
     // parameters, the tile bounding box.
@@ -616,7 +616,7 @@ __device__ __forceinline__ void tile_body(const ResidualArgs& a, const TileItem&
     const int tid = threadIdx.x;
 
     if (tid < 2) TB[tid] = -1;
-    for (int g = tid; g < G; g += 256) {
+    for (int g = tid; g < G; g += kTileThreads) {
         const TileTu d = a.tile_tus[it.first + g];
         TuParams p;
         p.off = d.off;
@@ -643,8 +643,8 @@ __device__ __forceinline__ void tile_body(const ResidualArgs& a, const TileItem&
     // phase B's lane items: (TU, row y, column chunk), NCH_B chunks, in
     // RB rounds of 256; its matrix columns (NPB pairs each) of every round
     // are fetched with the coefficients when they fit 16 registers
-    constexpr int NCH_B = (256 / (GMAX * H)) > 0 ? 256 / (GMAX * H) : 1;
-    constexpr int RB = (GMAX * H * NCH_B + 255) / 256;
+    constexpr int NCH_B = (kTileThreads / (GMAX * H)) > 0 ? kTileThreads / (GMAX * H) : 1;
+    constexpr int RB = (GMAX * H * NCH_B + kTileThreads - 1) / kTileThreads;
     constexpr int NPB = Hc / 2;  // matrix pairs per column
     constexpr bool B_PRE = RB * NPB <= 16;
     uint32_t mB[16];
@@ -655,13 +655,13 @@ __device__ __forceinline__ void tile_body(const ResidualArgs& a, const TileItem&
     {
         // a fixed number of items per lane (at most 2); every load of the
         // lane is issued before the first dequantisation
-        constexpr int Q = Wc / 4, RP = Hc / 2, NA = (GMAX * RP * Q + 255) / 256;
+        constexpr int Q = Wc / 4, RP = Hc / 2, NA = (GMAX * RP * Q + kTileThreads - 1) / kTileThreads;
         const int total = G * RP * Q;
         int mr = -1, mc = -1;
         uint64_t raw[NA][2];
 #pragma unroll
         for (int i = 0; i < NA; i++) {
-            const int e = tid + 256 * i;
+            const int e = tid + kTileThreads * i;
             raw[i][0] = raw[i][1] = 0;
             if (e < total) {
                 const int x0 = (e % Q) * 4, rp = (e / Q) % RP, g = e / (Q * RP);
@@ -679,7 +679,7 @@ __device__ __forceinline__ void tile_body(const ResidualArgs& a, const TileItem&
         if constexpr (B_PRE) {
 #pragma unroll
             for (int rb = 0; rb < RB; rb++) {
-                const int qb = rb * 256 + tid;
+                const int qb = rb * kTileThreads + tid;
                 const int gb = qb / (H * NCH_B);
                 if (gb < G) load_colp<NPB>(a.col16 + P[gb].mv + (qb & (H - 1)) * 16, mB + rb * NPB);
             }
@@ -690,13 +690,13 @@ __device__ __forceinline__ void tile_body(const ResidualArgs& a, const TileItem&
         bool wide = false;
 #pragma unroll
         for (int i = 0; i < NA; i++) {
-            const int e = tid + 256 * i;
+            const int e = tid + kTileThreads * i;
             if (e < total && P[e / (Q * RP)].scale > 32767) wide = true;
         }
         const bool fast = __builtin_amdgcn_ballot_w64(wide) == 0;
 #pragma unroll
         for (int i = 0; i < NA; i++) {
-            const int e = tid + 256 * i;
+            const int e = tid + kTileThreads * i;
             if (e >= total) continue;
             const int x0 = (e % Q) * 4, rp = (e / Q) % RP, g = e / (Q * RP);
             const int cwch = P[g].cwch, S = P[g].scale, rs = P[g].shifts & 31;
@@ -765,7 +765,7 @@ __device__ __forceinline__ void tile_body(const ResidualArgs& a, const TileItem&
         auto vpass = [&](auto nq) {
             constexpr int NQ = decltype(nq)::value;
 #pragma unroll
-            for (int base = 0; base < ITEMS; base += 256) {
+            for (int base = 0; base < ITEMS; base += kTileThreads) {
                 const int q = base + tid;
                 const int y = q & (H - 1);
                 const int c0 = (q / H) % NCH;
@@ -774,7 +774,7 @@ __device__ __forceinline__ void tile_body(const ResidualArgs& a, const TileItem&
                     uint32_t m[16];
                     if constexpr (B_PRE) {
 #pragma unroll
-                        for (int k = 0; k < NPB; k++) m[k] = mB[base / 256 * NPB + k];
+                        for (int k = 0; k < NPB; k++) m[k] = mB[base / kTileThreads * NPB + k];
                     } else {
                         load_col<NPB>(a.col16 + P[g].mv + y * 16, m);
                     }
@@ -802,7 +802,7 @@ __device__ __forceinline__ void tile_body(const ResidualArgs& a, const TileItem&
         constexpr int LPT = W2 * NRR;      // lanes per TU
         constexpr int ITEMS = GMAX * LPT;  // 2048 / RL, a multiple of 256
         constexpr int NP = Wc / 2;         // matrix pairs per column
-        constexpr int RC = ITEMS / 256;    // rounds
+        constexpr int RC = ITEMS / kTileThreads;  // rounds
         // the two matrix columns of every round, fetched together before the
         // first round when they fit 32 registers (one wait instead of RC)
         constexpr bool C_PRE = RC > 1 && RC * 2 * NP <= 32;
@@ -811,7 +811,7 @@ __device__ __forceinline__ void tile_body(const ResidualArgs& a, const TileItem&
         if constexpr (C_PRE) {
 #pragma unroll
             for (int rc = 0; rc < RC; rc++) {
-                const int g = (rc * 256 + tid) / LPT;
+                const int g = (rc * kTileThreads + tid) / LPT;
                 if (g < G) {
                     const int mh = P[g].mh;
                     load_colp<NP>(a.col16 + mh + (2 * x2) * 16, mC + rc * 2 * NP);
@@ -822,7 +822,7 @@ __device__ __forceinline__ void tile_body(const ResidualArgs& a, const TileItem&
         auto hpass = [&](auto nq) {
             constexpr int NQ = decltype(nq)::value;
 #pragma unroll
-            for (int base = 0; base < ITEMS; base += 256) {
+            for (int base = 0; base < ITEMS; base += kTileThreads) {
                 const int q = base + tid;
                 const int rr = (q / W2) % NRR;
                 const int g = q / LPT;
@@ -832,8 +832,8 @@ __device__ __forceinline__ void tile_body(const ResidualArgs& a, const TileItem&
                     if constexpr (C_PRE) {
 #pragma unroll
                         for (int k = 0; k < NP; k++) {
-                            m0[k] = mC[base / 256 * 2 * NP + k];
-                            m1[k] = mC[base / 256 * 2 * NP + NP + k];
+                            m0[k] = mC[base / kTileThreads * 2 * NP + k];
+                            m1[k] = mC[base / kTileThreads * 2 * NP + NP + k];
                         }
                     } else {
                         load_col<NP>(a.col16 + p.mh + (2 * x2) * 16, m0);
@@ -859,13 +859,13 @@ __device__ __forceinline__ void tile_body(const ResidualArgs& a, const TileItem&
     } else {
         // phase C (recon): horizontal pass, lane = (TU, column x, row chunk);
         // picture samples are updated in place
-        constexpr int NR0 = 256 / (GMAX * W);
+        constexpr int NR0 = kTileThreads / (GMAX * W);
         constexpr int NR = NR0 > 0 ? NR0 : 1;
         constexpr int ITEMS = GMAX * W * NR;
         auto hpass = [&](auto nq) {
             constexpr int NQ = decltype(nq)::value;
 #pragma unroll
-            for (int base = 0; base < ITEMS; base += 256) {
+            for (int base = 0; base < ITEMS; base += kTileThreads) {
                 const int q = base + tid;
                 const int x = q & (W - 1);
                 const int r0 = (q / W) % NR;
@@ -907,7 +907,7 @@ __device__ __forceinline__ void tile_body(const ResidualArgs& a, const TileItem&
 // RECON: add into the picture planes (a.place / a.frames) instead of writing
 // int16 residuals.
 template <bool RECON>
-__global__ __launch_bounds__(256) void residual_tile_kernel(ResidualArgs a, const TileItem* __restrict__ items) {
+__global__ __launch_bounds__(kTileThreads) void residual_tile_kernel(ResidualArgs a, const TileItem* __restrict__ items) {
     extern __shared__ __align__(16) uint32_t smem[];
     const TileItem it = items[blockIdx.x];
     switch (it.lw * 8 + it.lh) {
@@ -938,9 +938,9 @@ hipError_t launch_residual_tiles(const ResidualArgs& a, const TileItem* items, u
                                  hipStream_t stream) {
     if (n_items == 0) return hipSuccess;
     if (a.place)
-        hipLaunchKernelGGL(residual_tile_kernel<true>, dim3(n_items), dim3(256), lds_bytes, stream, a, items);
+        hipLaunchKernelGGL(residual_tile_kernel<true>, dim3(n_items), dim3(kTileThreads), lds_bytes, stream, a, items);
     else
-        hipLaunchKernelGGL(residual_tile_kernel<false>, dim3(n_items), dim3(256), lds_bytes, stream, a, items);
+        hipLaunchKernelGGL(residual_tile_kernel<false>, dim3(n_items), dim3(kTileThreads), lds_bytes, stream, a, items);
     return hipGetLastError();
 }
 
