@@ -295,6 +295,35 @@ def bench_bi(rmh, acc, W, po, n_frames, steps, warmup):
     return out
 
 
+RESIDUAL_CLASSES = [(4, 4), (8, 8), (16, 8), (8, 16), (16, 16), (32, 16), (32, 32), (64, 64)]
+
+
+def residual_classes(steps, warmup):
+    """The tiled residual launch on TU batches of one size class each (about
+    26 M samples, the workload's coefficient statistics, default tiling):
+    us per launch by HIP events. Shows which classes cost the most per sample."""
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    import workloads as W
+    rmh = load_package()
+    hip = rmh.hip
+    acc = rmh.Accelerator(0)
+    out = {}
+    for (w, h) in RESIDUAL_CLASSES:
+        n = (25 << 20) // (w * h)
+        tus, coeff = W._tu_batch(np.random.default_rng([7, w, h]), [(0, 0, 0, w, h)] * n, 10, (22, 37))
+        plan = acc.residual_plan(tus, coeff.size)
+        dc, dr = hip.DeviceBuffer.from_array(coeff), hip.DeviceBuffer(coeff.nbytes)
+        stream = hip.Stream()
+        ms = timed(hip, stream, lambda s: plan.run(dc.ptr, dr.ptr, s), steps, warmup)
+        out["%dx%d" % (w, h)] = {"tus": n, "samples": int(coeff.size), "us": round(ms * 1e3, 1)}
+        log("%dx%d: %.1f us" % (w, h, ms * 1e3))
+        plan.close()
+        dc.close()
+        dr.close()
+    print(json.dumps({"residual_classes": out}), flush=True)
+    acc.close()
+
+
 def bench_sao(rmh, acc, W, po, n_frames, steps, warmup):
     """SAO (band and edge offsets per CTB, random per component) over
     n_frames deblocked 1080p10 pictures: one launch of the SAO plan, output to
@@ -465,7 +494,12 @@ def main():
     ap.add_argument("--cnn-kernels", action="store_true")
     ap.add_argument("--no-tools", action="store_true")
     ap.add_argument("--no-sao", action="store_true")
+    ap.add_argument("--residual-classes", action="store_true")
     args = ap.parse_args()
+
+    if args.residual_classes:
+        residual_classes(args.steps, args.warmup)
+        return 0
 
     if args.copy_calibration:
         copy_calibration()

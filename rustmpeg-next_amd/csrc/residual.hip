@@ -438,17 +438,19 @@ __global__ __launch_bounds__(64) void residual_kernel(ResidualArgs a) {
 
 
 // ---------------------------------------------------------------------------
-// Tiled kernel: one 256-lane workgroup per tile of same-size TUs.
+// Tiled kernel: one kTileThreads-lane workgroup (one wave) per tile of
+// same-size TUs, 16 samples per lane.
 //
 //   phase 0  per-TU parameters (offset, coded region, dequant scale/shift,
-//            matrix offsets) into LDS; bounding boxes cleared
+//            matrix offsets) into LDS; bounding box cleared
 //   phase A  every coefficient of the tile's coded regions read with 8-byte
-//            loads, dequantised, stored column-major in LDS; per-TU bounding
-//            box of non-zero coefficients by LDS atomics
+//            loads, dequantised, stored column-major in LDS; bounding box of
+//            the non-zero levels
 //   phase B  vertical pass: lane (TU, row y) holds column y of Mv in registers
 //            and produces its row of intermediates for the live columns
-//   phase C  horizontal pass: lane (TU, column x) holds column x of Mh and
-//            produces its column of residuals, stored to HBM
+//   phase C  horizontal pass: lane (TU, columns x, x + 1) holds those columns
+//            of Mh and produces their residuals, stored to HBM (or added into
+//            the picture under RECON, one column per lane)
 // Many TUs per workgroup keep many loads in flight, which a one-TU workgroup
 // cannot. Arithmetic and clipping are identical to residual_kernel.
 // ---------------------------------------------------------------------------
@@ -641,7 +643,7 @@ __device__ __forceinline__ void tile_body(const ResidualArgs& a, const TileItem&
     __syncthreads();
 
     // phase B's lane items: (TU, row y, column chunk), NCH_B chunks, in
-    // RB rounds of 256; its matrix columns (NPB pairs each) of every round
+    // RB rounds of kTileThreads; its matrix columns (NPB pairs each) of every round
     // are fetched with the coefficients when they fit 16 registers
     constexpr int NCH_B = (kTileThreads / (GMAX * H)) > 0 ? kTileThreads / (GMAX * H) : 1;
     constexpr int RB = (GMAX * H * NCH_B + kTileThreads - 1) / kTileThreads;
@@ -800,7 +802,7 @@ __device__ __forceinline__ void tile_body(const ResidualArgs& a, const TileItem&
         constexpr int NRR = NRR0 < 1 ? 1 : (NRR0 > H ? H : NRR0);
         constexpr int RL = H / NRR;        // rows per lane per round (<= 8)
         constexpr int LPT = W2 * NRR;      // lanes per TU
-        constexpr int ITEMS = GMAX * LPT;  // 2048 / RL, a multiple of 256
+        constexpr int ITEMS = GMAX * LPT;  // 8 kTileThreads / RL, a multiple of kTileThreads
         constexpr int NP = Wc / 2;         // matrix pairs per column
         constexpr int RC = ITEMS / kTileThreads;  // rounds
         // the two matrix columns of every round, fetched together before the
