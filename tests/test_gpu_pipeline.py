@@ -113,13 +113,18 @@ def test_out_of_place_deblock_equals_in_place(acc, po):
     plan.close()
 
 
-def test_recon_plan_equals_residual_plan_plus_add(acc, po):
-    """The fused add against the residual plan's int16 output added on the host."""
+@pytest.mark.parametrize("bd,pad", [(10, 0), (10, 1), (8, 0), (8, 1)])
+def test_recon_plan_equals_residual_plan_plus_add(acc, po, bd, pad):
+    """The fused add against the residual plan's int16 output added on the host,
+    on planes whose row pitch is the width (pairs of samples added with one
+    access) or the width + 1 (odd pitch: sample by sample)."""
+    print("recon plan bd %d pad %d" % (bd, pad), flush=True)
     rmh = load_package()
     hip, abi = rmh.hip, rmh.abi
-    f = W.pframe(61, width=512, height=256, bit_depth=10)
+    f = W.pframe(61, width=512, height=256, bit_depth=bd)
     tus, coeff, place = f["tus"], f["coeff"], f["place"]
-    pred = [p.copy() for p in f["ref"]]
+    dt = np.uint8 if bd == 8 else np.uint16
+    pred = [np.ascontiguousarray(p.astype(dt)) for p in f["ref"]]
     rp = acc.residual_plan(tus, coeff.size)
     dc = hip.DeviceBuffer.from_array(coeff)
     dr = hip.DeviceBuffer(coeff.nbytes)
@@ -131,14 +136,23 @@ def test_recon_plan_equals_residual_plan_plus_add(acc, po):
         w, h = 1 << int(t["log2_w"]), 1 << int(t["log2_h"])
         x, y = int(pl["x"]), int(pl["y"])
         want[int(pl["plane"])][y:y + h, x:x + w] += res[int(t["offset"]):int(t["offset"]) + w * h].reshape(h, w)
-    want = [np.clip(p, 0, 1023) for p in want]
-    bufs = [hip.DeviceBuffer.from_array(p) for p in pred]
-    planes = [abi.plane_from_address(b.ptr, p.shape[1], p.shape[0], p.shape[1], 2, 10) for b, p in zip(bufs, pred)]
+    want = [np.clip(p, 0, (1 << bd) - 1) for p in want]
+    # planes with pitch width + pad, the padding column set to a sentinel
+    padded = []
+    for p in pred:
+        q = np.full((p.shape[0], p.shape[1] + pad), 77, dt)
+        q[:, :p.shape[1]] = p
+        padded.append(q)
+    bufs = [hip.DeviceBuffer.from_array(q) for q in padded]
+    planes = [abi.plane_from_address(b.ptr, p.shape[1], p.shape[0], p.shape[1] + pad, p.itemsize, bd)
+              for b, p in zip(bufs, pred)]
     plan = acc.recon_plan(tus, coeff.size, place, [abi.ReconFrame((abi.Plane * 3)(*planes))])
     plan.run(dc.ptr, 0, 0)
     hip.device_synchronize()
     for c in range(3):
-        assert (bufs[c].download(np.uint16).reshape(pred[c].shape) == want[c]).all(), c
+        got = bufs[c].download(dt).reshape(padded[c].shape)
+        assert (got[:, :pred[c].shape[1]] == want[c]).all(), c
+        assert (got[:, pred[c].shape[1]:] == 77).all(), c
     plan.close()
     rp.close()
 
