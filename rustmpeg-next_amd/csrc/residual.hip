@@ -153,7 +153,10 @@ __device__ __forceinline__ void recon_store(const ReconDst& d, int x, int y, int
 __global__ __launch_bounds__(64) void residual_kernel(ResidualArgs a) {
     __shared__ __align__(16) uint32_t ct[32 * kPitch];  // coefficients: column i, int16 pairs along row k
     __shared__ __align__(16) uint32_t tt[64 * kPitch];  // intermediates: row y, int16 pairs along column i
-    __shared__ int16_t lv[32 * 32];                     // raw levels for dependent quantisation
+    // Raw levels for dependent quantisation share the intermediates' space:
+    // they are last read before the barrier that precedes the vertical pass.
+    static_assert(32 * 32 * sizeof(int16_t) <= sizeof(tt), "levels fit in the intermediates");
+    int16_t* lv = reinterpret_cast<int16_t*>(tt);
 
     const int lane = threadIdx.x;
     const uint32_t tu_index = a.order ? a.order[a.first_tu + blockIdx.x] : a.first_tu + blockIdx.x;
