@@ -170,6 +170,27 @@ def bench_pipeline(rmh, acc, W, PL, po, n_frames, steps, warmup):
     out["vs_unbatched"] = round(out["value"] / out["unbatched"]["value"], 3)
     log("pipeline: %.3f ms/step %.1f Mpix/s, unbatched %.3f ms (x%.2f)" % (ms, out["value"], ums, out["vs_unbatched"]))
 
+    # the same chain built in scratch planes and deblocked out of place into
+    # the picture (no shadow copy of the pre-deblocking samples)
+    ob = PL.DeviceBatch(acc, rmh, frames, out_of_place=True)
+    ograph = ob.graph()
+    ograph.launch(stream.handle)
+    stream.synchronize()
+    want = PL.oracle(po, rmh.abi, [frames[0]])[0]
+    got = ob.download()[0]
+    if not all((got[c] == want[c]).all() for c in range(3)):
+        log("bench: out-of-place pipeline differs from the oracle")
+        out["out_of_place"] = None
+    else:
+        oms = timed(hip, stream, ograph.launch, steps, warmup)
+        out["out_of_place"] = {
+            "workload": "the same frames: MC and residual add into scratch planes, deblock out of place into "
+                        "the picture, one frame-step graph",
+            "value": round(px / (oms / 1e3) / 1e6, 1), "unit": "Mpix/s", "ms_per_step": round(oms, 4)}
+        log("pipeline out of place: %.3f ms/step" % oms)
+    ograph.close()
+    ob.close()
+
     # the in-loop chain with SAO: deblocking out of place into scratch planes
     # (no shadow copy), SAO back into the picture, one graph
     sb = PL.DeviceBatch(acc, rmh, frames, with_sao=True)

@@ -77,9 +77,14 @@ def oracle(po, abi, frames, with_sao=False):
 class DeviceBatch:
     """Device planes and batched plans for a list of P-frames."""
 
-    def __init__(self, acc, rmh, frames, with_sao=False):
+    def __init__(self, acc, rmh, frames, with_sao=False, out_of_place=False):
         """with_sao: deblocking writes to scratch planes (out of place) and SAO
-        writes the final samples back into the frame planes."""
+        writes the final samples back into the frame planes. out_of_place
+        (without SAO): MC and the residual add build the picture in scratch
+        planes and deblocking writes it out of place into the frame planes,
+        so no shadow copy of the pre-deblocking picture is made."""
+        if with_sao and out_of_place:
+            raise ValueError("with_sao already deblocks out of place")
         self.hip, abi = rmh.hip, rmh.abi
         hip = self.hip
         self.frames = frames
@@ -102,9 +107,17 @@ class DeviceBatch:
             e = np.ascontiguousarray(f["edges"])
             eb = hip.DeviceBuffer.from_array(e)
             self.keep.append(eb)
-            mc_frames.append(abi.mc_frame(refs, dsts))
-            rc_frames.append(abi.ReconFrame((abi.Plane * 3)(*dsts)))
-            db_frames.append(abi.dbk_frame(dsts, eb.ptr, e.shape[1], e.shape[0], f["qp"]))
+            work = dsts
+            if out_of_place:
+                work = []
+                for (db, shape, dt), d in zip(dbufs, dsts):
+                    wb = hip.DeviceBuffer(int(np.prod(shape)) * np.dtype(dt).itemsize)
+                    self.keep.append(wb)
+                    work.append(abi.plane_from_address(wb.ptr, d.width, d.height, d.stride, d.bytes_per_sample, bd))
+                tmp_planes += dsts
+            mc_frames.append(abi.mc_frame(refs, work))
+            rc_frames.append(abi.ReconFrame((abi.Plane * 3)(*work)))
+            db_frames.append(abi.dbk_frame(work, eb.ptr, e.shape[1], e.shape[0], f["qp"]))
             if with_sao:
                 tmps = []
                 for (db, shape, dt), d in zip(dbufs, dsts):
@@ -124,6 +137,9 @@ class DeviceBatch:
         if with_sao:
             self.deblock = acc.deblock_to_plan(db_frames, tmp_planes)
             self.sao = acc.sao_plan(sao_frames)
+        elif out_of_place:
+            self.deblock = acc.deblock_to_plan(db_frames, tmp_planes)
+            self.sao = None
         else:
             self.deblock = acc.deblock_plan(db_frames)
             self.sao = None

@@ -31,6 +31,32 @@ def test_pframe_1080p10_pipeline_matches_oracle(acc, po):
     b.close()
 
 
+def test_pframe_built_in_scratch_deblocked_out_of_place(acc, po):
+    """MC and the residual add into scratch planes, deblocking out of place
+    into the picture (8-bit and 10-bit frames in one batch); the graph is
+    replayed over cleared pictures and every replay checked."""
+    print("pipeline out of place", flush=True)
+    rmh = load_package()
+    frames = [W.pframe(43, width=640, height=360, bit_depth=10), W.pframe(44, width=856, height=480, bit_depth=8)]
+    want = PL.oracle(po, rmh.abi, frames)
+    b = PL.DeviceBatch(acc, rmh, frames, out_of_place=True)
+    s = rmh.hip.Stream()
+    b.run(s.handle)
+    s.synchronize()
+    _check(b.download(), want)
+    g = b.graph()
+    for rep in range(3):
+        for fr in b.dst:
+            for buf, _, _ in fr:
+                buf.zero(s.handle)
+        s.synchronize()
+        g.launch(s.handle)
+        s.synchronize()
+        _check(b.download(), want)
+    g.close()
+    b.close()
+
+
 def test_mixed_streams_one_batch_graph_replays(acc, po):
     """Frames of three streams (8-bit 856x480 twice, 10-bit 640x368) in one
     batch, captured as one frame-step graph and replayed."""
