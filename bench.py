@@ -260,6 +260,24 @@ def bench_streams(rmh, acc, W, PL, po, frame_steps, reps):
                          "launches_per_frame_step": 4 * len(frames)}}
     out["vs_unbatched"] = round(out["value"] / out["unbatched"]["value"], 3)
     log("streams: %.3f ms/frame step %.1f Mpix/s, unbatched x%.2f" % (ms / frame_steps, out["value"], out["vs_unbatched"]))
+
+    # built in scratch planes, deblocked out of place into the pictures
+    ob = PL.DeviceBatch(acc, rmh, frames, out_of_place=True)
+    ograph = ob.graph()
+    ograph.launch(stream.handle)
+    stream.synchronize()
+    if check_batch(PL, po, rmh, ob, frames, (0, 4, 12)):
+        def run_oop(s):
+            for _ in range(frame_steps):
+                ograph.launch(s)
+        oms = timed(hip, stream, run_oop, reps, 1)
+        out["out_of_place"] = {"value": round(px / (oms / 1e3) / 1e6, 1), "unit": "Mpix/s",
+                               "ms_per_frame_step": round(oms / frame_steps, 4)}
+        log("streams out of place: %.3f ms/frame step" % (oms / frame_steps))
+    else:
+        out["out_of_place"] = None
+    ograph.close()
+    ob.close()
     return out
 
 
