@@ -47,7 +47,12 @@ def parse_array(text, name):
 def emit(path, prefix, l8, l4):
     lines = [
         "/* Generated by oracle/gen_lfnst_data.py: normative LFNST kernels (H.266",
-        " * lowFreqTransMatrix). [set 0..3][lfnst_idx-1][input coefficient 0..15][output]. */",
+        " * lowFreqTransMatrix). [set 0..3][lfnst_idx-1][input coefficient 0..15][output].",
+        " *",
+        " * The values derive from the VVC reference software VTM",
+        " * (https://vcgit.hhi.fraunhofer.de/jvet/VVCSoftware_VTM),",
+        " * Copyright (c) 2010-2020, ITU/ISO/IEC, BSD 3-Clause License; the license",
+        " * text is reproduced in THIRD_PARTY.md at the repository root. */",
         "#ifndef %s_LFNST_DATA_H" % prefix,
         "#define %s_LFNST_DATA_H" % prefix,
         "#include <stdint.h>",

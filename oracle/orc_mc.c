@@ -188,7 +188,12 @@ int orc_interp_luma_16_as_written(const uint16_t* plane, int stride, int pw, int
     int head_room = 14 - bd < 2 ? 2 : 14 - bd;
     int shift1 = 6 - head_room;
     int shift2 = bi ? 6 : 12 - bd;
-    int32_t add2 = bi ? 0 : 1 << (shift2 - 1);
+    /* At bd = 12 the as-written shift2 is 0 and the reference's
+     * `1i32 << (shift2 - 1)` shifts by -1: a debug build panics there and a
+     * release build masks the amount (1 << 31). The rounding term has no
+     * defined reference value then; 0 keeps the C shift defined. Only the
+     * uni-prediction 2-D branch reads add2. */
+    int32_t add2 = (bi || shift2 <= 0) ? 0 : 1 << (shift2 - 1);
     int32_t boff = 1 << (bd + head_room - 1);
     if (fx == 0 && fy == 0) {
         for (int y = 0; y < h; y++)

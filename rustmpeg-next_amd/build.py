@@ -29,9 +29,20 @@ def _run(cmd, verbose):
         print(out)
 
 
-def build(verbose=False):
+def _stale(src, obj):
+    """True when obj is missing or older than its source or any header in csrc/."""
+    if not os.path.exists(obj):
+        return True
+    t = os.path.getmtime(obj)
+    headers = [os.path.join(CSRC, h) for h in os.listdir(CSRC) if h.endswith(".h")]
+    headers.append(os.path.join(os.path.dirname(HERE), "include", "rmh.h"))
+    return any(os.path.getmtime(f) > t for f in [src] + headers)
+
+
+def build(verbose=False, incremental=False):
     """Compiles every source to an object in parallel (one hipcc per file),
-    then links librmh.so."""
+    then links librmh.so. With incremental, sources whose object is newer
+    than the source and every header are not recompiled."""
     objdir = os.path.join(HERE, "_build")
     os.makedirs(objdir, exist_ok=True)
     compile_flags = [f for f in FLAGS if f != "-shared"]
@@ -39,9 +50,10 @@ def build(verbose=False):
     for s in SOURCES:
         obj = os.path.join(objdir, os.path.splitext(s)[0] + ".o")
         jobs.append(([HIPCC] + compile_flags + ["-c", os.path.join(CSRC, s), "-o", obj], obj))
+    todo = [cmd for cmd, obj in jobs if not incremental or _stale(cmd[-3], obj)]
     workers = max(1, min(len(jobs), os.cpu_count() or 1, 16))
     with concurrent.futures.ThreadPoolExecutor(workers) as pool:
-        for f in [pool.submit(_run, cmd, verbose) for cmd, _ in jobs]:
+        for f in [pool.submit(_run, cmd, verbose) for cmd in todo]:
             f.result()
     tmp = LIB + ".tmp"
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC"] + [o for _, o in jobs] + ["-o", tmp], verbose)
@@ -50,5 +62,5 @@ def build(verbose=False):
 
 
 if __name__ == "__main__":
-    build(verbose=True)
+    build(verbose=True, incremental="--incremental" in sys.argv)
     print("built", LIB)

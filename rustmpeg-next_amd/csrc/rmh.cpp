@@ -92,6 +92,9 @@ enum PlanKind { kResidual = 0, kMc = 1, kDeblock = 2, kSao = 3 };
 struct rmh_plan {
     rmh_ctx* ctx = nullptr;
     int kind = 0;
+    // graphs that captured this plan's launches: they hold its device tables
+    // and grid sizes, so the autotuner must not rewrite them meanwhile
+    std::atomic<uint32_t> graph_refs{0};
     uint64_t bytes = 0;
     void* d_mem = nullptr;
     // residual
@@ -133,6 +136,7 @@ struct rmh_plan {
 
 struct rmh_graph {
     rmh_ctx* ctx = nullptr;
+    std::vector<rmh_plan*> plans;  // captured plans (each holds a graph reference)
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     hipStream_t capture = nullptr;
@@ -180,14 +184,23 @@ rmh_status gate(rmh_ctx* ctx) {
     return RMH_OK;
 }
 
+// A failed or timed-out wait sets the failure latch, as Accelerator::fallback
+// does on any GPU failure (crates/rustmpeg-vulkan/src/accelerator.rs:163-171):
+// work still queued on the device may yet write its outputs, so the context
+// takes no further calls.
 rmh_status wait_stream(rmh_ctx* ctx, hipStream_t s) {
     auto deadline = std::chrono::steady_clock::now() + std::chrono::milliseconds(ctx->timeout_ms);
     for (;;) {
         hipError_t e = hipStreamQuery(s);
         if (e == hipSuccess) return RMH_OK;
-        if (e != hipErrorNotReady) return hip_fail_locked(ctx, e, "device execution failed");
-        if (std::chrono::steady_clock::now() > deadline)
+        if (e != hipErrorNotReady) {
+            ctx->disabled.store(true);
+            return hip_fail_locked(ctx, e, "device execution failed");
+        }
+        if (std::chrono::steady_clock::now() > deadline) {
+            ctx->disabled.store(true);
             return fail_locked(ctx, RMH_TIMEOUT, "device work did not finish within the timeout");
+        }
         std::this_thread::sleep_for(std::chrono::microseconds(20));
     }
 }
@@ -433,6 +446,34 @@ bool plane_ok(const rmh_plane& p) {
            p.height <= 65535 && (uint64_t)p.stride * p.height * p.bytes_per_sample <= 0xffffffffull;
 }
 
+// Address range a plane's samples occupy, and whether the batch writes it.
+struct PlaneSpan {
+    uintptr_t lo, hi;
+    bool written;
+};
+
+void add_span(std::vector<PlaneSpan>& v, const rmh_plane& p, bool written) {
+    const uintptr_t lo = (uintptr_t)p.data;
+    const uint64_t n = ((uint64_t)(p.height - 1) * p.stride + p.width) * p.bytes_per_sample;
+    v.push_back({lo, lo + (uintptr_t)n, written});
+}
+
+// True when a written plane overlaps any other plane of the batch (read-only
+// planes may share memory). One sort and a sweep: if a written span overlaps
+// an earlier one, the earlier span reaching furthest overlaps it too, or
+// overlapped it when it arrived.
+bool written_overlap(std::vector<PlaneSpan>& v) {
+    std::sort(v.begin(), v.end(), [](const PlaneSpan& a, const PlaneSpan& b) { return a.lo < b.lo; });
+    bool have = false;
+    PlaneSpan far{};
+    for (const PlaneSpan& s : v) {
+        if (have && s.lo < far.hi && (s.written || far.written)) return true;
+        if (!have || s.hi > far.hi) far = s;
+        have = true;
+    }
+    return false;
+}
+
 rmh_status validate_mc(rmh_ctx* ctx, const rmh_mc_frame* frames, size_t n_frames, const rmh_pu_desc* pus, size_t n_pu) {
     if ((n_frames && !frames) || (n_pu && !pus)) return fail_locked(ctx, RMH_INVALID_INPUT, "null frame or PU array");
     if (n_frames > 65536) return fail_locked(ctx, RMH_LIMIT_EXCEEDED, "more than 65536 frames");
@@ -452,6 +493,13 @@ rmh_status validate_mc(rmh_ctx* ctx, const rmh_mc_frame* frames, size_t n_frames
             fr.ref[2].width != fr.ref[1].width || fr.ref[2].height != fr.ref[1].height)
             return fail_locked(ctx, RMH_INVALID_INPUT, "frame " + std::to_string(f) + ": chroma planes are not 4:2:0");
     }
+    std::vector<PlaneSpan> spans;
+    for (size_t f = 0; f < n_frames; f++)
+        for (int c = 0; c < 3; c++) {
+            add_span(spans, frames[f].ref[c], false);
+            add_span(spans, frames[f].dst[c], true);
+        }
+    if (written_overlap(spans)) return fail_locked(ctx, RMH_INVALID_INPUT, "an output plane overlaps another plane of the batch");
     for (size_t i = 0; i < n_pu; i++) {
         const rmh_pu_desc& p = pus[i];
         int lw = p.log2_w & 15, lh = p.log2_h & 15;
@@ -489,6 +537,14 @@ rmh_status validate_mc_bi(rmh_ctx* ctx, const rmh_mc_bi_frame* frames, size_t n_
             fr.ref0[2].width != fr.ref0[1].width || fr.ref0[2].height != fr.ref0[1].height)
             return fail_locked(ctx, RMH_INVALID_INPUT, tag + "chroma planes are not 4:2:0");
     }
+    std::vector<PlaneSpan> spans;
+    for (size_t f = 0; f < n_frames; f++)
+        for (int c = 0; c < 3; c++) {
+            add_span(spans, frames[f].ref0[c], false);
+            add_span(spans, frames[f].ref1[c], false);
+            add_span(spans, frames[f].dst[c], true);
+        }
+    if (written_overlap(spans)) return fail_locked(ctx, RMH_INVALID_INPUT, "an output plane overlaps another plane of the batch");
     for (size_t i = 0; i < n_pu; i++) {
         const rmh_bi_pu_desc& p = pus[i];
         const std::string tag = "PU " + std::to_string(i) + ": ";
@@ -506,7 +562,7 @@ rmh_status validate_mc_bi(rmh_ctx* ctx, const rmh_mc_bi_frame* frames, size_t n_
     return RMH_OK;
 }
 
-rmh_status validate_dbk(rmh_ctx* ctx, const rmh_dbk_frame* frames, size_t n_frames) {
+rmh_status validate_dbk(rmh_ctx* ctx, const rmh_dbk_frame* frames, size_t n_frames, bool in_place = true) {
     if (n_frames && !frames) return fail_locked(ctx, RMH_INVALID_INPUT, "null frame array");
     if (n_frames > 65535) return fail_locked(ctx, RMH_LIMIT_EXCEEDED, "more than 65535 frames");
     for (size_t f = 0; f < n_frames; f++) {
@@ -521,6 +577,12 @@ rmh_status validate_dbk(rmh_ctx* ctx, const rmh_dbk_frame* frames, size_t n_fram
             d.planes[2].width != d.planes[1].width || d.planes[2].height != d.planes[1].height)
             return fail_locked(ctx, RMH_INVALID_INPUT, "frame " + std::to_string(f) + ": chroma planes are not 4:2:0");
     }
+    if (!in_place) return RMH_OK;
+    // in place: every plane is written
+    std::vector<PlaneSpan> spans;
+    for (size_t f = 0; f < n_frames; f++)
+        for (int c = 0; c < 3; c++) add_span(spans, frames[f].planes[c], true);
+    if (written_overlap(spans)) return fail_locked(ctx, RMH_INVALID_INPUT, "planes of the batch overlap");
     return RMH_OK;
 }
 
@@ -568,6 +630,13 @@ rmh_status validate_sao(rmh_ctx* ctx, const rmh_sao_frame* frames, size_t n_fram
             }
         }
     }
+    std::vector<PlaneSpan> spans;
+    for (size_t f = 0; f < n_frames; f++)
+        for (int c = 0; c < 3; c++) {
+            add_span(spans, frames[f].src[c], false);
+            add_span(spans, frames[f].dst[c], true);
+        }
+    if (written_overlap(spans)) return fail_locked(ctx, RMH_INVALID_INPUT, "a dst plane overlaps another plane of the batch");
     return RMH_OK;
 }
 
@@ -1166,6 +1235,9 @@ RMH_EXPORT rmh_status rmh_residual_plan_autotune(rmh_plan* plan, const int16_t* 
     if (!d_coeff || (!d_residual && !plan->d_place)) return fail(ctx, RMH_INVALID_INPUT, "null device buffer");
     if (((uintptr_t)d_coeff & 7) || (!plan->d_place && ((uintptr_t)d_residual & 7)))
         return fail(ctx, RMH_INVALID_INPUT, "tiled kernel needs 8-byte aligned buffers");
+    // a captured graph replays the tile list and grid size it was built with
+    if (plan->graph_refs.load() != 0)
+        return fail(ctx, RMH_INVALID_INPUT, "plan is captured in a graph: destroy the graph before autotuning");
     rmh_status g = gate(ctx);
     if (g != RMH_OK) return fail(ctx, g, g == RMH_UNAVAILABLE ? "accelerator disabled" : "injected device failure");
     DeviceGuard dg(ctx->device);
@@ -1614,9 +1686,17 @@ RMH_EXPORT rmh_status rmh_deblock_to_plan_create(rmh_ctx* ctx, const rmh_dbk_fra
     *out = nullptr;
     {
         std::lock_guard<std::mutex> lk(ctx->mu);
-        rmh_status v = validate_dbk(ctx, frames, n_frames);
+        rmh_status v = validate_dbk(ctx, frames, n_frames, false);
         if (v != RMH_OK) return v;
         if (n_frames && !dst) return fail_locked(ctx, RMH_INVALID_INPUT, "null destination planes");
+        std::vector<PlaneSpan> spans;
+        for (size_t f = 0; f < n_frames; f++)
+            for (int c = 0; c < 3; c++) {
+                add_span(spans, frames[f].planes[c], false);
+                if (plane_ok(dst[3 * f + c])) add_span(spans, dst[3 * f + c], true);
+            }
+        if (written_overlap(spans))
+            return fail_locked(ctx, RMH_INVALID_INPUT, "a destination plane overlaps another plane of the batch");
         for (size_t f = 0; f < n_frames; f++)
             for (int c = 0; c < 3; c++) {
                 const rmh_plane &a = frames[f].planes[c], &b = dst[3 * f + c];
@@ -1945,6 +2025,10 @@ RMH_EXPORT rmh_status rmh_graph_create(rmh_ctx* ctx, const rmh_graph_node* nodes
         delete g;
         return fail(ctx, RMH_EXECUTION, st != RMH_OK ? err : std::string("graph capture failed: ") + hipGetErrorString(e));
     }
+    for (size_t i = 0; i < n_nodes; i++) {
+        g->plans.push_back(nodes[i].plan);
+        nodes[i].plan->graph_refs.fetch_add(1);
+    }
     *out = g;
     return RMH_OK;
 }
@@ -1965,6 +2049,7 @@ RMH_EXPORT void rmh_graph_destroy(rmh_graph* g) {
     if (g->exec) hipGraphExecDestroy(g->exec);
     if (g->graph) hipGraphDestroy(g->graph);
     if (g->capture) hipStreamDestroy(g->capture);
+    for (rmh_plan* p : g->plans) p->graph_refs.fetch_sub(1);
     delete g;
 }
 
@@ -1978,6 +2063,7 @@ struct rmh_cnn {
     uint8_t* d_blocks = nullptr;  // staging of rmh_cnn_predict
     float* d_qp = nullptr;
     float* d_probs = nullptr;
+    float* h_probs = nullptr;  // pinned read-back of rmh_cnn_predict (max_batch * 6 floats)
     int step = 0;
     // rmh_cnn_predict runs on the network's own stream under its own lock,
     // so predictions of different instances overlap on the device
@@ -2045,9 +2131,17 @@ RMH_EXPORT rmh_status rmh_cnn_create(rmh_ctx* ctx, uint32_t max_batch, uint64_t 
     c->max_batch = max_batch;
     hipError_t e = hipMalloc(&c->d_mem, total);
     if (e == hipSuccess) {
+        e = hipHostMalloc((void**)&c->h_probs, n * 6 * sizeof(float), hipHostMallocDefault);
+        if (e != hipSuccess) {
+            hipFree(c->d_mem);
+            c->h_probs = nullptr;
+        }
+    }
+    if (e == hipSuccess) {
         e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
         if (e != hipSuccess) {
             hipFree(c->d_mem);
+            hipHostFree(c->h_probs);
             c->stream = nullptr;
         }
     }
@@ -2092,6 +2186,8 @@ RMH_EXPORT rmh_status rmh_cnn_create(rmh_ctx* ctx, uint32_t max_batch, uint64_t 
     if (e == hipSuccess) e = pack_weights(b, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) {
+        hipStreamDestroy(c->stream);
+        hipHostFree(c->h_probs);
         hipFree(c->d_mem);
         delete c;
         return fail(ctx, RMH_EXECUTION, std::string("network initialisation: ") + hipGetErrorString(e));
@@ -2107,6 +2203,7 @@ RMH_EXPORT void rmh_cnn_destroy(rmh_cnn* c) {
         hipStreamSynchronize(c->stream);
         hipStreamDestroy(c->stream);
     }
+    if (c->h_probs) hipHostFree(c->h_probs);
     if (c->d_mem) hipFree(c->d_mem);
     delete c;
 }
@@ -2139,6 +2236,8 @@ RMH_EXPORT rmh_status rmh_cnn_forward(rmh_cnn* c, const uint8_t* d_blocks, const
     if (n == 0) return RMH_OK;
     if (!d_blocks || !d_qp || !d_probs) return fail(c->ctx, RMH_INVALID_INPUT, "null device buffer");
     if (n > c->max_batch) return fail(c->ctx, RMH_LIMIT_EXCEEDED, "batch larger than max_batch");
+    // the activation buffers are the network's: one call at a time enqueues on them
+    std::lock_guard<std::mutex> lk(c->mu);
     rmh_status g = gate(c->ctx);
     if (g != RMH_OK) return fail(c->ctx, g, g == RMH_UNAVAILABLE ? "accelerator disabled" : "injected device failure");
     DeviceGuard dg(c->ctx->device);
@@ -2160,16 +2259,25 @@ RMH_EXPORT rmh_status rmh_cnn_predict(rmh_cnn* c, const uint8_t* blocks, const f
     hipError_t e = hipMemcpyAsync(c->d_blocks, blocks, (size_t)n * 1024, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipMemcpyAsync(c->d_qp, qp, (size_t)n * 4, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = rmh::cnn::forward(c->b, c->d_blocks, c->d_qp, (int)n, c->d_probs, s);
-    if (e == hipSuccess) e = hipMemcpyAsync(probs, c->d_probs, (size_t)n * 24, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(c->h_probs, c->d_probs, (size_t)n * 24, hipMemcpyDeviceToHost, s);
     if (e != hipSuccess) return fail(ctx, RMH_EXECUTION, std::string("predict: ") + hipGetErrorString(e));
     // the context's timeout, without holding the context lock
     const auto deadline = std::chrono::steady_clock::now() + std::chrono::milliseconds(ctx->timeout_ms);
     for (;;) {
         e = hipStreamQuery(s);
-        if (e == hipSuccess) return RMH_OK;
-        if (e != hipErrorNotReady) return fail(ctx, RMH_EXECUTION, std::string("device execution failed: ") + hipGetErrorString(e));
-        if (std::chrono::steady_clock::now() > deadline)
+        if (e == hipSuccess) {
+            // the caller's buffer is written only once the copy has landed
+            memcpy(probs, c->h_probs, (size_t)n * 24);
+            return RMH_OK;
+        }
+        if (e != hipErrorNotReady) {
+            ctx->disabled.store(true);
+            return fail(ctx, RMH_EXECUTION, std::string("device execution failed: ") + hipGetErrorString(e));
+        }
+        if (std::chrono::steady_clock::now() > deadline) {
+            ctx->disabled.store(true);
             return fail(ctx, RMH_TIMEOUT, "device work did not finish within the timeout");
+        }
         std::this_thread::sleep_for(std::chrono::microseconds(20));
     }
 }
@@ -2202,7 +2310,9 @@ rmh_status cnn_step(rmh_cnn* c, float lr, void* stream) {
 
 RMH_EXPORT rmh_status rmh_cnn_gradients(rmh_cnn* c, const uint8_t* d_blocks, const float* d_qp, const uint8_t* d_labels,
                                         uint32_t n, float* d_grads, float* d_loss, void* stream) {
-    if (!d_labels) return c ? fail(c->ctx, RMH_INVALID_INPUT, "null labels") : RMH_INVALID_INPUT;
+    if (!c) return RMH_INVALID_INPUT;
+    if (!d_labels) return fail(c->ctx, RMH_INVALID_INPUT, "null labels");
+    std::lock_guard<std::mutex> lk(c->mu);
     return cnn_gradients(c, d_blocks, d_qp, d_labels, nullptr, n, d_grads, d_loss, stream);
 }
 
@@ -2211,6 +2321,7 @@ RMH_EXPORT rmh_status rmh_cnn_train_step(rmh_cnn* c, const uint8_t* d_blocks, co
     if (!c) return RMH_INVALID_INPUT;
     if (!(lr > 0.0f) || lr > 1.0f) return fail(c->ctx, RMH_INVALID_INPUT, "learning rate outside (0, 1]");
     if (!d_labels) return fail(c->ctx, RMH_INVALID_INPUT, "null labels");
+    std::lock_guard<std::mutex> lk(c->mu);
     rmh_status st = cnn_gradients(c, d_blocks, d_qp, d_labels, nullptr, n, c->b.grads, d_loss, stream);
     return st != RMH_OK ? st : cnn_step(c, lr, stream);
 }
@@ -2221,6 +2332,7 @@ RMH_EXPORT rmh_status rmh_cnn_train_step_soft(rmh_cnn* c, const uint8_t* d_block
     if (!c) return RMH_INVALID_INPUT;
     if (!(lr > 0.0f) || lr > 1.0f) return fail(c->ctx, RMH_INVALID_INPUT, "learning rate outside (0, 1]");
     if (!d_targets) return fail(c->ctx, RMH_INVALID_INPUT, "null targets");
+    std::lock_guard<std::mutex> lk(c->mu);
     rmh_status st = cnn_gradients(c, d_blocks, d_qp, nullptr, d_targets, n, c->b.grads, d_loss, stream);
     return st != RMH_OK ? st : cnn_step(c, lr, stream);
 }

@@ -142,3 +142,63 @@ def test_autotuned_residual_plan_is_bit_exact(acc, po):
     s.synchronize()
     assert (dr.download(np.int16) == ref).all()
     plan.close()
+
+
+def test_autotune_refused_while_a_graph_holds_the_plan(acc, po, rmh):
+    """A graph replays the tile list and grid it captured, so the autotuner
+    must not rewrite them while the graph lives; after the graph is destroyed
+    it may, and a recaptured graph is bit-exact."""
+    hip = rmh.hip
+    tus, coeff, _ = W.residual_frame(12, width=320, height=192)
+    ref = po.residual_batch(tus, coeff)
+    plan = acc.residual_plan(tus, coeff.size)
+    dc = hip.DeviceBuffer.from_array(coeff)
+    dr = hip.DeviceBuffer(coeff.nbytes)
+    s = hip.Stream()
+    graph = acc.graph([(plan, dc.ptr, dr.ptr)])
+    with pytest.raises(rmh.InvalidInput):
+        plan.autotune(dc.ptr, dr.ptr, s.handle, reps=1)
+    graph.close()
+    plan.autotune(dc.ptr, dr.ptr, s.handle, reps=1)
+    graph = acc.graph([(plan, dc.ptr, dr.ptr)])
+    for _ in range(2):
+        dr.upload(np.zeros_like(coeff))
+        graph.launch(s.handle)
+        s.synchronize()
+        assert (dr.download(np.int16) == ref).all()
+    graph.close()
+    plan.close()
+
+
+def test_cross_frame_aliasing_is_rejected(acc, rmh):
+    """Written planes that overlap another frame's planes are refused at plan
+    creation (before any device work): deblocking in place, out of place and
+    SAO."""
+    abi, hip = rmh.abi, rmh.hip
+    w, h, bd = 64, 32, 10
+    buf = hip.DeviceBuffer(8 * w * h * 2)
+    edges = hip.DeviceBuffer(16 * 8 * 4)
+
+    def planes(base):
+        y = abi.plane_from_address(base, w, h, w, 2, bd)
+        u = abi.plane_from_address(base + w * h * 2, w // 2, h // 2, w // 2, 2, bd)
+        v = abi.plane_from_address(base + w * h * 2 + (w // 2) * (h // 2) * 2, w // 2, h // 2, w // 2, 2, bd)
+        return [y, u, v]
+
+    frame_bytes = w * h * 2 * 3 // 2
+    a, b = planes(buf.ptr), planes(buf.ptr + frame_bytes)
+    qp = (32, 32, 32)
+    fa = abi.dbk_frame(a, edges.ptr, w // 4, h // 4, qp)
+    # in place: two frames over the same planes
+    with pytest.raises(rmh.InvalidInput):
+        acc.deblock_plan([fa, abi.dbk_frame(a, edges.ptr, w // 4, h // 4, qp)])
+    # out of place: frame 1's destination is frame 0's source
+    fb = abi.dbk_frame(b, edges.ptr, w // 4, h // 4, qp)
+    c = planes(buf.ptr + 2 * frame_bytes)
+    with pytest.raises(rmh.InvalidInput):
+        acc.deblock_to_plan([fa, fb], c + a)
+    # out of place: two destinations overlap
+    with pytest.raises(rmh.InvalidInput):
+        acc.deblock_to_plan([fa, fb], c + c)
+    ok = acc.deblock_to_plan([fa, fb], c + planes(buf.ptr + 3 * frame_bytes))
+    ok.close()
