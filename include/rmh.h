@@ -302,6 +302,24 @@ rmh_status rmh_deblock_plan_run(rmh_plan* plan, void* stream);
 rmh_status rmh_deblock_to_plan_create(rmh_ctx* ctx, const rmh_dbk_frame* frames, size_t n_frames,
                                       const rmh_plane* dst, rmh_plan** out);
 
+/* In-place deblocking that allocates nothing. No counterpart in the
+ * reference: it serves callers that cannot allocate device memory (code
+ * under stream capture, tight serving loops, harnesses that forbid
+ * hipMalloc). Same frames, results and validation as rmh_deblock_plan_create
+ * followed by rmh_deblock_plan_run, enqueued on `stream`, but the frame tables
+ * and the copies of the planes that the filter reads live in one
+ * library-owned device buffer per GPU of RMH_NOALLOC_SCRATCH_BYTES, held until
+ * the process ends: at most RMH_NOALLOC_PLANE_BYTES of plane copies (each
+ * plane's rows padded to 16 bytes, each plane to 256: one 3840x2160 10-bit
+ * frame, or five 1920x1080 10-bit frames) and 64 KiB of tables (a few hundred
+ * frames). Larger batches return RMH_LIMIT_EXCEEDED before any device work.
+ * Every call on a GPU shares that GPU's buffer: calls that overlap on one GPU
+ * (on different streams, or from threads on different streams) race and must
+ * be serialised by the caller; calls on one stream are ordered. */
+#define RMH_NOALLOC_SCRATCH_BYTES (32u << 20)
+#define RMH_NOALLOC_PLANE_BYTES ((32u << 20) - (64u << 10))
+rmh_status rmh_deblock_noalloc(rmh_ctx* ctx, const rmh_dbk_frame* frames, size_t n_frames, void* stream);
+
 /* ------------------------------------------------------------------------ */
 /* Sample adaptive offset (after deblocking)                                 */
 /* ------------------------------------------------------------------------ */

@@ -72,7 +72,7 @@ EXPORTED = [
     "rmh_residual_recon_plan_create", "rmh_rd_encode_luma", "rmh_cnn_param_count", "rmh_cnn_create",
     "rmh_cnn_destroy", "rmh_cnn_get_params", "rmh_cnn_set_params", "rmh_cnn_forward", "rmh_cnn_predict",
     "rmh_cnn_gradients", "rmh_cnn_train_step", "rmh_cnn_train_step_soft", "rmh_residual_plan_autotune",
-    "rmh_mc_bi_batch", "rmh_mc_bi_plan_create",
+    "rmh_mc_bi_batch", "rmh_mc_bi_plan_create", "rmh_deblock_noalloc",
 ]
 
 
@@ -108,6 +108,7 @@ def load_library(path=LIB_PATH):
         "rmh_deblock_plan_create": ([vp, vp, sz, ctypes.POINTER(vp)], st),
         "rmh_deblock_plan_run": ([vp, vp], st),
         "rmh_deblock_to_plan_create": ([vp, vp, sz, vp, ctypes.POINTER(vp)], st),
+        "rmh_deblock_noalloc": ([vp, vp, sz, vp], st),
         "rmh_sao": ([vp, vp, sz], st),
         "rmh_sao_plan_create": ([vp, vp, sz, ctypes.POINTER(vp)], st),
         "rmh_sao_plan_run": ([vp, vp], st),
@@ -394,6 +395,13 @@ class Accelerator:
         self._check(load_library().rmh_deblock_to_plan_create(self.handle, ctypes.addressof(arr), len(frames),
                                                               ctypes.addressof(darr), ctypes.byref(h)))
         return Plan(self, h.value, "deblock")
+
+    def deblock_noalloc(self, frames, stream=0):
+        """In-place deblocking of device planes (abi.DbkFrame entries) on `stream`
+        without any device allocation (rmh_deblock_noalloc): the library's
+        fixed per-GPU scratch holds the tables and plane copies."""
+        arr = (abi.DbkFrame * len(frames))(*frames)
+        self._check(load_library().rmh_deblock_noalloc(self.handle, ctypes.addressof(arr), len(frames), stream))
 
     def sao_plan(self, frames):
         """SAO plan over abi.SaoFrame entries with device planes and params; run with plan.run()."""

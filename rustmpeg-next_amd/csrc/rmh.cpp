@@ -72,6 +72,67 @@ size_t aligned(size_t b) { return (b + 255) & ~(size_t)255; }
 
 }  // namespace
 
+// ------------------------------------------------ no-allocation scratch
+//
+// Device memory of rmh_deblock_noalloc: the frame tables and the copies of the
+// planes that the in-place deblocking filter reads. One copy per GPU, held
+// until the process ends. Host code never passes `scratch` itself to a kernel
+// or a copy (in host code the name is a stand-in, not a device address): it
+// uses the per-device address that hipGetSymbolAddress returns.
+constexpr size_t kScratchBytes = RMH_NOALLOC_SCRATCH_BYTES;
+constexpr size_t kScratchTableBytes = (size_t)64 << 10;  // frame tables; the rest holds plane copies
+static_assert(kScratchBytes - kScratchTableBytes == RMH_NOALLOC_PLANE_BYTES, "scratch split as rmh.h states");
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Address of `scratch` on `device` (which must be current), looked up once per
+// device. Returns null on failure.
+void* scratch_address(int device) {
+    if (device < 0 || device >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess) return nullptr;
+        g_scratch_ptr[device] = p;
+    }
+    return g_scratch_ptr[device];
+}
+
+// Up to kWordChunk words carried in the kernel arguments: the launch copies
+// them, so the host tables may go out of scope once the launch is enqueued
+// (no pinned staging, no allocation).
+constexpr int kWordChunk = 448;  // 1.8 KB of arguments, well inside the kernel-argument limit
+struct WordChunk {
+    uint32_t n;
+    uint32_t w[kWordChunk];
+};
+
+__global__ __launch_bounds__(256) void write_words_kernel(uint32_t* __restrict__ dst, WordChunk c) {
+    for (uint32_t i = threadIdx.x; i < c.n; i += blockDim.x) dst[i] = c.w[i];
+}
+
+// Writes `bytes` (a multiple of 4) of host data to device address dst on s.
+hipError_t write_words(void* dst, const void* src, size_t bytes, hipStream_t s) {
+    const uint32_t* w = (const uint32_t*)src;
+    size_t n = bytes / 4;
+    for (size_t at = 0; at < n; at += kWordChunk) {
+        WordChunk c;
+        c.n = (uint32_t)std::min<size_t>(kWordChunk, n - at);
+        memcpy(c.w, w + at, c.n * 4);
+        hipLaunchKernelGGL(write_words_kernel, dim3(1), dim3(256), 0, s, (uint32_t*)dst + at, c);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+}  // namespace
+
 struct rmh_ctx {
     int device = 0;
     uint32_t timeout_ms = 30000;
@@ -85,6 +146,7 @@ struct rmh_ctx {
     Arena dev;
     Arena host;
     hipDeviceProp_t props{};
+    void* scratch = nullptr;  // this device's address of `scratch`
 };
 
 enum PlanKind { kResidual = 0, kMc = 1, kDeblock = 2, kSao = 3 };
@@ -1063,7 +1125,9 @@ RMH_EXPORT rmh_status rmh_create(const rmh_config* cfg, rmh_ctx** out) {
         delete ctx;
         return RMH_UNAVAILABLE;
     }
-    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess || build_tables(ctx) != hipSuccess) {
+    ctx->scratch = scratch_address(ctx->device);
+    if (!ctx->scratch || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
+        build_tables(ctx) != hipSuccess) {
         if (ctx->stream) hipStreamDestroy(ctx->stream);
         if (ctx->d_tables) hipFree(ctx->d_tables);
         delete ctx;
@@ -1761,6 +1825,44 @@ RMH_EXPORT rmh_status rmh_deblock_plan_run(rmh_plan* plan, void* stream) {
     if (!plan || plan->kind != kDeblock) return RMH_INVALID_INPUT;
     if (plan->n_dbk == 0) return RMH_OK;
     return run_plan(plan, nullptr, nullptr, 0, 0, stream);
+}
+
+RMH_EXPORT rmh_status rmh_deblock_noalloc(rmh_ctx* ctx, const rmh_dbk_frame* frames, size_t n_frames, void* stream) {
+    if (!ctx) return RMH_INVALID_INPUT;
+    {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        rmh_status v = validate_dbk(ctx, frames, n_frames);
+        if (v != RMH_OK) return v;
+    }
+    if (n_frames == 0) return RMH_OK;
+    // the tables: frames, shadow plane records, tile running sum + frame order
+    const size_t fb = aligned(n_frames * sizeof(rmh_dbk_frame)), sb = aligned(n_frames * 3 * sizeof(rmh_plane));
+    const size_t tb = aligned((2 * n_frames + 1) * sizeof(uint32_t));
+    if (fb + sb + tb > kScratchTableBytes)
+        return fail(ctx, RMH_LIMIT_EXCEEDED, "too many frames for the no-allocation scratch tables");
+    if (shadow_bytes(frames, n_frames) > kScratchBytes - kScratchTableBytes)
+        return fail(ctx, RMH_LIMIT_EXCEEDED,
+                    "planes exceed the " + std::to_string((kScratchBytes - kScratchTableBytes) >> 10) +
+                        " KiB of no-allocation scratch for plane copies");
+    rmh_status g = gate(ctx);
+    if (g != RMH_OK) return fail(ctx, g, g == RMH_UNAVAILABLE ? "accelerator disabled" : "injected device failure");
+    DeviceGuard dg(ctx->device);
+    char* base = (char*)ctx->scratch;
+    const TileTable tt = tile_table(frames, n_frames);
+    const std::vector<rmh_plane> shadows = shadow_layout(frames, n_frames, base + kScratchTableBytes);
+    uint32_t max_rows = 0;
+    for (size_t f = 0; f < n_frames; f++) max_rows = std::max(max_rows, frames[f].planes[0].height);
+    static_assert(sizeof(rmh_dbk_frame) % 4 == 0 && sizeof(rmh_plane) % 4 == 0, "tables are written as words");
+    hipStream_t s = (hipStream_t)stream;
+    hipError_t e = write_words(base, frames, n_frames * sizeof(rmh_dbk_frame), s);
+    if (e == hipSuccess) e = write_words(base + fb, shadows.data(), shadows.size() * sizeof(rmh_plane), s);
+    if (e == hipSuccess) e = write_words(base + fb + sb, tt.words.data(), tt.words.size() * sizeof(uint32_t), s);
+    const uint32_t* d_words = (const uint32_t*)(base + fb + sb);
+    if (e == hipSuccess)
+        e = rmh::launch_deblock((const rmh_dbk_frame*)base, (const rmh_plane*)(base + fb), d_words, d_words + n_frames + 1,
+                                (uint32_t)n_frames, tt.total, tt.tiles8, max_rows, true, s);
+    if (e != hipSuccess) return fail(ctx, RMH_EXECUTION, std::string("kernel launch failed: ") + hipGetErrorString(e));
+    return RMH_OK;
 }
 
 RMH_EXPORT rmh_status rmh_deblock(rmh_ctx* ctx, const rmh_dbk_frame* frames, size_t n_frames) {
