@@ -419,6 +419,14 @@ rmh_status rmh_intra_subset_decode(rmh_ctx* ctx, uint32_t width, uint32_t height
 /* Introspection (tests)                                                     */
 /* ------------------------------------------------------------------------ */
 
+/* The host PLANAR (mode 0) / DC (mode 1) predictor that rmh_intra_subset_decode
+ * uses, for an n x n block (n = 4..32) at (bx, by) of an 8-bit plane with row
+ * pitch `width`: predict_block's PLANAR / DC path (vvc/predict.rs:2619-2646:
+ * gather_references, the [1 2 1] reference filter for luma PLANAR above 32
+ * samples, predict_planar / predict_dc, apply_pdpc). Host code only. */
+rmh_status rmh_intra_predict(const uint8_t* plane, uint32_t width, uint32_t height, uint32_t bx, uint32_t by,
+                             uint32_t n, int is_luma, int mode, int32_t* pred);
+
 /* Writes the n*n row-major integer matrix the kernels use for (type, n);
  * returns RMH_INVALID_INPUT for an unsupported pair (matrix_for_any,
  * vvc/dct.rs:251-256). */

@@ -216,6 +216,22 @@ static void short_line(int32_t* line, const luma_decision* e, int max_val) {
     }
 }
 
+/* The per-line luma filters and the sample decisions, exported so that tests
+ * can compare them one by one with the reference's expressions. */
+void orc_dbk_long_line(int32_t* line, int len_p, int len_q, int tc, int max_val) {
+    long_tap_line(line, len_p, len_q, tc, max_val);
+}
+
+void orc_dbk_short_line(int32_t* line, int de, int dep, int deq, int tc, int max_val) {
+    luma_decision d = {de, dep, deq, 0, 0, tc};
+    short_line(line, &d, max_val);
+}
+
+int orc_dbk_luma_sample_decision(int p0, int p3, int q0, int q3, int dpq, int sp, int sq, int spq, int side_p_large,
+                                 int side_q_large, int beta, int tc) {
+    return luma_sample_decision(p0, p3, q0, q3, dpq, sp, sq, spq, side_p_large, side_q_large, beta, tc);
+}
+
 /* deblock_luma_edge_bd (vvc/filter.rs:638-679). */
 void orc_deblock_luma_edge(int32_t* lines, int qp_p, int qp_q, int bs, int beta_off, int tc_off, int bd,
                            int len_p, int len_q, int at_ctb) {
@@ -236,9 +252,40 @@ void orc_deblock_luma_edge(int32_t* lines, int qp_p, int qp_q, int bs, int beta_
     }
 }
 
+/* One chroma line [p3..p0, q0..q3] by the filter form the lengths select:
+ * chroma_strong_line (3, 3), chroma_weak_line_p1q3 (1, 3), chroma_weak_line
+ * otherwise (vvc/filter.rs:986-1057). */
+void orc_dbk_chroma_line(int32_t* line, int len_p, int len_q, int tc, int max_val) {
+    int32_t m[8];
+    memcpy(m, line, sizeof(m));
+#define CL(o, v) clampi(clampi((v), (o)-tc, (o) + tc), 0, max_val)
+    if (len_p == 3 && len_q == 3) {
+        line[1] = CL(m[1], (3 * m[0] + 2 * m[1] + m[2] + m[3] + m[4] + 4) >> 3);
+        line[2] = CL(m[2], (2 * m[0] + m[1] + 2 * m[2] + m[3] + m[4] + m[5] + 4) >> 3);
+        line[3] = CL(m[3], (m[0] + m[1] + m[2] + 2 * m[3] + m[4] + m[5] + m[6] + 4) >> 3);
+        line[4] = CL(m[4], (m[1] + m[2] + m[3] + 2 * m[4] + m[5] + m[6] + m[7] + 4) >> 3);
+        line[5] = CL(m[5], (m[2] + m[3] + m[4] + 2 * m[5] + m[6] + 2 * m[7] + 4) >> 3);
+        line[6] = CL(m[6], (m[3] + m[4] + m[5] + 2 * m[6] + 3 * m[7] + 4) >> 3);
+    } else if (len_p == 1 && len_q == 3) {
+        line[3] = CL(m[3], (3 * m[2] + 2 * m[3] + m[4] + m[5] + m[6] + 4) >> 3);
+        line[4] = CL(m[4], (2 * m[2] + m[3] + 2 * m[4] + m[5] + m[6] + m[7] + 4) >> 3);
+        line[5] = CL(m[5], (m[2] + m[3] + m[4] + 2 * m[5] + m[6] + 2 * m[7] + 4) >> 3);
+        line[6] = CL(m[6], (m[3] + m[4] + m[5] + 2 * m[6] + 3 * m[7] + 4) >> 3);
+    } else {
+        int delta = clampi((4 * (m[4] - m[3]) + m[2] - m[5] + 4) >> 3, -tc, tc);
+        line[3] = clampi(m[3] + delta, 0, max_val);
+        line[4] = clampi(m[4] - delta, 0, max_val);
+    }
+#undef CL
+}
+
 /* chroma_sample_decision (vvc/filter.rs:907-919). */
 static int chroma_sample_decision(int p0, int p3, int q0, int q3, int dpq, int beta, int tc) {
     return dpq < (beta >> 2) && iabs(p3 - p0) + iabs(q0 - q3) < (beta >> 3) && iabs(p0 - q0) < ((5 * tc + 1) >> 1);
+}
+
+int orc_dbk_chroma_sample_decision(int p0, int p3, int q0, int q3, int dpq, int beta, int tc) {
+    return chroma_sample_decision(p0, p3, q0, q3, dpq, beta, tc);
 }
 
 /* deblock_chroma_edge (vvc/filter.rs:935-1057); lines of [p3..p0, q0..q3]. */
@@ -270,30 +317,7 @@ void orc_deblock_chroma_edge(int32_t* lines, int n, int qp, int bs, int beta_off
             len_q = 1;
         }
     }
-    for (int l = 0; l < n; l++) {
-        int32_t* line = lines + 8 * l;
-        int32_t m[8];
-        memcpy(m, line, sizeof(m));
-#define CL(o, v) clampi(clampi((v), (o)-tc, (o) + tc), 0, max_val)
-        if (len_p == 3 && len_q == 3) {
-            line[1] = CL(m[1], (3 * m[0] + 2 * m[1] + m[2] + m[3] + m[4] + 4) >> 3);
-            line[2] = CL(m[2], (2 * m[0] + m[1] + 2 * m[2] + m[3] + m[4] + m[5] + 4) >> 3);
-            line[3] = CL(m[3], (m[0] + m[1] + m[2] + 2 * m[3] + m[4] + m[5] + m[6] + 4) >> 3);
-            line[4] = CL(m[4], (m[1] + m[2] + m[3] + 2 * m[4] + m[5] + m[6] + m[7] + 4) >> 3);
-            line[5] = CL(m[5], (m[2] + m[3] + m[4] + 2 * m[5] + m[6] + 2 * m[7] + 4) >> 3);
-            line[6] = CL(m[6], (m[3] + m[4] + m[5] + 2 * m[6] + 3 * m[7] + 4) >> 3);
-        } else if (len_p == 1 && len_q == 3) {
-            line[3] = CL(m[3], (3 * m[2] + 2 * m[3] + m[4] + m[5] + m[6] + 4) >> 3);
-            line[4] = CL(m[4], (2 * m[2] + m[3] + 2 * m[4] + m[5] + m[6] + m[7] + 4) >> 3);
-            line[5] = CL(m[5], (m[2] + m[3] + m[4] + 2 * m[5] + m[6] + 2 * m[7] + 4) >> 3);
-            line[6] = CL(m[6], (m[3] + m[4] + m[5] + 2 * m[6] + 3 * m[7] + 4) >> 3);
-        } else {
-            int delta = clampi((4 * (m[4] - m[3]) + m[2] - m[5] + 4) >> 3, -tc, tc);
-            line[3] = clampi(m[3] + delta, 0, max_val);
-            line[4] = clampi(m[4] - delta, 0, max_val);
-        }
-#undef CL
-    }
+    for (int l = 0; l < n; l++) orc_dbk_chroma_line(lines + 8 * l, len_p, len_q, tc, max_val);
 }
 
 static inline int pget(const rmh_plane* p, int x, int y) {

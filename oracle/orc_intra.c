@@ -148,6 +148,10 @@ static int choose_mode(const uint8_t* rec, const uint8_t* src, int pw, int ph, i
     return sad[1] < sad[0] ? 1 : 0;
 }
 
+int orc_intra_choose_mode(const uint8_t* rec, const uint8_t* src, int pw, int ph, int bx, int by, int n) {
+    return choose_mode(rec, src, pw, ph, bx, by, n);
+}
+
 typedef struct {
     int32_t levels[1024];
     uint8_t samples[1024];

@@ -76,8 +76,14 @@ def _setup(L):
     L.orc_deblock_luma_plane.argtypes = [U16P] + [c_int] * 9
     L.orc_deblock_chroma_plane.argtypes = [U16P] + [c_int] * 9
     L.orc_tc_beta.argtypes = [c_int] * 5 + [I32P, I32P]
+    L.orc_dbk_long_line.argtypes = [I32P] + [c_int] * 4
+    L.orc_dbk_short_line.argtypes = [I32P] + [c_int] * 5
+    L.orc_dbk_chroma_line.argtypes = [I32P] + [c_int] * 4
+    L.orc_dbk_luma_sample_decision.argtypes = [c_int] * 12
+    L.orc_dbk_chroma_sample_decision.argtypes = [c_int] * 7
     U8P = ctypes.POINTER(ctypes.c_uint8)
     L.orc_intra_predict.argtypes = [U8P] + [c_int] * 8 + [I32P]
+    L.orc_intra_choose_mode.argtypes = [U8P, U8P] + [c_int] * 5
     L.orc_intra_encode.argtypes = [U8P, U8P, U8P, c_int, c_int, c_int, U8P, U8P, U8P, U8P, ctypes.c_void_p, I16P]
     L.orc_intra_decode.argtypes = [c_int, c_int, U8P, ctypes.c_void_p, I16P, U8P, U8P, U8P]
     L.orc_sao_region_bounded16.argtypes = [ctypes.c_void_p, ctypes.c_void_p] + [c_int] * 6 + [ctypes.c_void_p]
@@ -253,6 +259,24 @@ def deblock_chroma_plane(plane, bd, cu_step, ctb_c, qp, bs, beta_off=0, tc_off=0
     p = np.ascontiguousarray(plane, dtype=np.uint16).copy()
     lib().orc_deblock_chroma_plane(_p(p, U16P), p.shape[1], p.shape[0], bd, cu_step, ctb_c, qp, bs, beta_off, tc_off)
     return p
+
+
+def dbk_long_line(line, len_p, len_q, tc, max_val):
+    a = np.array(line, np.int32)
+    lib().orc_dbk_long_line(_p(a, I32P), len_p, len_q, tc, max_val)
+    return a.tolist()
+
+
+def dbk_short_line(line, de, dep, deq, tc, max_val):
+    a = np.array(line, np.int32)
+    lib().orc_dbk_short_line(_p(a, I32P), de, int(dep), int(deq), tc, max_val)
+    return a.tolist()
+
+
+def dbk_chroma_line(line, len_p, len_q, tc, max_val):
+    a = np.array(line, np.int32)
+    lib().orc_dbk_chroma_line(_p(a, I32P), len_p, len_q, tc, max_val)
+    return a.tolist()
 
 
 def tc_beta(qp, bs, beta_off, tc_off, bd):

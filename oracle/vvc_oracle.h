@@ -104,10 +104,22 @@ void orc_deblock_chroma_plane(uint16_t* plane, int w, int h, int bd, int cu_step
 /* ---- intra-profile subset codec (harness for the residual stage) ---- */
 void orc_intra_predict(const uint8_t* plane, int pw, int ph, int bx, int by, int w, int h, int is_luma, int mode,
                        int32_t* pred);
+/* choose_intra_mode for an n x n luma block (vvc/predict.rs:2706-2730): 0 PLANAR, 1 DC. */
+int orc_intra_choose_mode(const uint8_t* rec, const uint8_t* src, int pw, int ph, int bx, int by, int n);
 int orc_intra_encode(const uint8_t* sy, const uint8_t* su, const uint8_t* sv, int w, int h, int qp, uint8_t* ry,
                      uint8_t* ru, uint8_t* rv, uint8_t* modes, rmh_tu_desc* tus, int16_t* coeff);
 int orc_intra_decode(int w, int h, const uint8_t* modes, const rmh_tu_desc* tus, const int16_t* coeff, uint8_t* ry,
                      uint8_t* ru, uint8_t* rv);
+
+/* Per-line deblocking filters and sample decisions (vvc/filter.rs:276-304,
+ * 477-631, 907-919, 1010-1057). Luma lines are [p7..p0, q0..q7], chroma
+ * lines [p3..p0, q0..q3]. */
+void orc_dbk_long_line(int32_t* line, int len_p, int len_q, int tc, int max_val);
+void orc_dbk_short_line(int32_t* line, int de, int dep, int deq, int tc, int max_val);
+void orc_dbk_chroma_line(int32_t* line, int len_p, int len_q, int tc, int max_val);
+int orc_dbk_luma_sample_decision(int p0, int p3, int q0, int q3, int dpq, int sp, int sq, int spq, int side_p_large,
+                                 int side_q_large, int beta, int tc);
+int orc_dbk_chroma_sample_decision(int p0, int p3, int q0, int q3, int dpq, int beta, int tc);
 
 /* tC and beta after bit-depth scaling (deblock_luma_edge_bd, vvc/filter.rs:653-666). */
 void orc_tc_beta(int qp, int bs, int beta_off, int tc_off, int bd, int* tc, int* beta);

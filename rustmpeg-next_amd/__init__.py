@@ -72,7 +72,7 @@ EXPORTED = [
     "rmh_residual_recon_plan_create", "rmh_rd_encode_luma", "rmh_cnn_param_count", "rmh_cnn_create",
     "rmh_cnn_destroy", "rmh_cnn_get_params", "rmh_cnn_set_params", "rmh_cnn_forward", "rmh_cnn_predict",
     "rmh_cnn_gradients", "rmh_cnn_train_step", "rmh_cnn_train_step_soft", "rmh_residual_plan_autotune",
-    "rmh_mc_bi_batch", "rmh_mc_bi_plan_create", "rmh_deblock_noalloc",
+    "rmh_mc_bi_batch", "rmh_mc_bi_plan_create", "rmh_deblock_noalloc", "rmh_intra_predict",
 ]
 
 
@@ -118,6 +118,7 @@ def load_library(path=LIB_PATH):
         "rmh_graph_launch": ([vp, vp], st),
         "rmh_graph_destroy": ([vp], None),
         "rmh_transform_matrix": ([ctypes.c_int32, ctypes.c_int32, vp], st),
+        "rmh_intra_predict": ([vp] + [ctypes.c_uint32] * 5 + [ctypes.c_int, ctypes.c_int, vp], st),
         "rmh_rd_encode_luma": ([vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
                                 ctypes.c_uint32, vp, ctypes.c_float, ctypes.c_float, vp, vp, vp,
                                 ctypes.POINTER(abi.RdStats)], st),
@@ -165,6 +166,18 @@ def rd_encode_luma(src, qp, mode=abi.RD_FULL, probs=None, p_prune=0.05, p_fallba
     stats = {"bits": st.bits, "sse": int(st.sse), "blocks": st.blocks, "candidates": st.candidates,
              "fallbacks": st.fallbacks}
     return recon, splits.reshape(h // 32, w // 32), (costs.reshape(nb, 6) if costs is not None else None), stats
+
+
+def intra_predict(plane, bx, by, n, is_luma, mode):
+    """The host PLANAR / DC predictor of the intra-subset decode (rmh_intra_predict)
+    for the n x n block at (bx, by) of a C-contiguous uint8 plane."""
+    plane = np.ascontiguousarray(plane, np.uint8)
+    out = np.zeros(n * n, np.int32)
+    st = load_library().rmh_intra_predict(plane.ctypes.data, plane.shape[1], plane.shape[0], bx, by, n, int(is_luma),
+                                          mode, out.ctypes.data)
+    if st != abi.RMH_OK:
+        raise _ERRORS.get(st, AcceleratorError)("rmh_intra_predict", st)
+    return out.reshape(n, n)
 
 
 def transform_matrix(tr_type, n):

@@ -150,3 +150,14 @@ extern "C" __attribute__((visibility("default"))) rmh_status rmh_intra_subset_de
     memcpy(v, rv.data(), rv.size());
     return RMH_OK;
 }
+
+extern "C" __attribute__((visibility("default"))) rmh_status rmh_intra_predict(const uint8_t* plane, uint32_t width,
+                                                                              uint32_t height, uint32_t bx,
+                                                                              uint32_t by, uint32_t n, int is_luma,
+                                                                              int mode, int32_t* pred) {
+    if (!plane || !pred || (n != 4 && n != 8 && n != 16 && n != 32) || mode < 0 || mode > 1 || width > 16384 ||
+        height > 16384 || bx + n > width || by + n > height)
+        return RMH_INVALID_INPUT;
+    predict(plane, (int)width, (int)height, (int)bx, (int)by, (int)n, is_luma != 0, mode, pred);
+    return RMH_OK;
+}

@@ -1,0 +1,449 @@
+"""Evaluates small integer functions of the reference's Rust source, read as
+text, in Python.
+
+Test infrastructure: it lets a test compare an oracle function with the
+reference's own expressions instead of a retyped copy of them. It translates
+the subset of Rust that the reference's per-line filter and decision
+functions use (crates/rustmpeg-video/src/vvc/filter.rs):
+
+- `let` bindings (with `mut`, tuple patterns, type annotations), closures
+  `|a: T, b: T| expr` and `|x: T| -> R { match .. }`;
+- assignments to variables and to `line[...]`;
+- `if COND { .. }` / `if COND { .. } else { .. }` as statements and as
+  expressions, `return;`, `for PAT in ITER { .. }`;
+- `match SCRUT { PAT | PAT => EXPR, _ => { ..; EXPR } }` as an expression;
+- the methods `.clamp(a, b)`, `.abs()`, `.min(x)`, `.max(x)`, `.iter()`,
+  `.zip(y)`, `.enumerate()`;
+- `&&`, `||`, `!`, references (`&m`, `&[..]`, `*line`), `i32::from(x)`,
+  `x as T`, integer suffixes.
+
+Rust's precedence for the operators used (`*`, `+`, `-`, `>>`, `<<`,
+comparisons, `&&`, `||`) is Python's, and `>>` on i32 is an arithmetic shift
+like Python's, so the translated expressions compute what the Rust ones do
+for values that do not overflow i32.
+"""
+import re
+import types
+
+_METHODS = ("clamp", "abs", "min", "max", "iter", "zip", "enumerate", "map", "collect", "to_vec", "clone", "len",
+            "sum", "saturating_sub", "trailing_zeros", "fill")
+
+
+def fn_source(text, name):
+    """(parameter list, body) of `fn name(...) ... { body }` in text."""
+    m = re.search(r"\bfn\s+%s\s*(<[^>]*>)?\s*\(" % re.escape(name), text)
+    if not m:
+        raise KeyError(name)
+    i = m.end() - 1
+    j = _match(text, i)
+    params = text[i + 1:j]
+    k = text.index("{", j)
+    end = _match(text, k)
+    return params, text[k + 1:end]
+
+
+def _match(s, i):
+    """Index of the bracket closing the one at s[i]."""
+    pairs = {"(": ")", "[": "]", "{": "}"}
+    stack = [pairs[s[i]]]
+    j = i + 1
+    while stack:
+        c = s[j]
+        if c in pairs:
+            stack.append(pairs[c])
+        elif c in ")]}":
+            assert c == stack[-1], (s[i:j + 1], c)
+            stack.pop()
+        j += 1
+    return j - 1
+
+
+def _split_top(s, sep):
+    """Splits s at `sep` characters outside brackets."""
+    out, depth, cur = [], 0, []
+    for c in s:
+        if c in "([{":
+            depth += 1
+        elif c in ")]}":
+            depth -= 1
+        if c == sep and depth == 0:
+            out.append("".join(cur))
+            cur = []
+        else:
+            cur.append(c)
+    out.append("".join(cur))
+    return out
+
+
+def _receiver_start(s, dot):
+    """Start index of the receiver expression that ends just before s[dot]
+    (a method call may sit on the next line: `(..)\n    .clamp(..)`)."""
+    j = dot
+    while j > 0 and s[j - 1].isspace():
+        j -= 1
+    while True:
+        if j > 0 and s[j - 1] in ")]":
+            depth, k = 0, j - 1
+            while True:
+                if s[k] in ")]":
+                    depth += 1
+                elif s[k] in "([":
+                    depth -= 1
+                    if depth == 0:
+                        break
+                k -= 1
+            j = k
+            continue
+        if j > 0 and (s[j - 1].isalnum() or s[j - 1] == "_" or (s[j - 1] == "." and s[j - 2:j] != "..")):
+            j -= 1
+            continue
+        return j
+
+
+def _methods(s):
+    while True:
+        m = re.search(r"\.(%s)\(" % "|".join(_METHODS), s)
+        if not m:
+            return s
+        name = m.group(1)
+        start = _receiver_start(s, m.start())
+        recv = s[start:m.start()]
+        open_ = m.end() - 1
+        close = _match(s, open_)
+        args = s[open_ + 1:close].strip()
+        if name == "clamp":
+            rep = "_clamp(%s, %s)" % (recv, args)
+        elif name == "abs":
+            rep = "_abs(%s)" % recv
+        elif name in ("min", "max"):
+            rep = "_%s(%s, %s)" % (name, recv, args)
+        elif name == "iter":
+            rep = recv
+        elif name == "zip":
+            rep = "_zip(%s, %s)" % (recv, args)
+        elif name == "enumerate":
+            rep = "_enumerate(%s)" % recv
+        elif name == "map":
+            cm = re.match(r"\|([^|]*)\|(.*)$", args, re.S)
+            var = cm.group(1).split(":")[0].strip()
+            rep = "[(%s) for %s in (%s)]" % (expr(cm.group(2)), var, recv)
+        elif name in ("collect",):
+            rep = recv
+        elif name in ("to_vec", "clone"):
+            rep = "_list(%s)" % recv
+        elif name == "len":
+            rep = "_len(%s)" % recv
+        elif name == "fill":
+            rep = "_fill(%s, %s)" % (recv, args)
+        elif name == "sum":
+            rep = "_sum(%s)" % recv
+        elif name == "saturating_sub":
+            rep = "_max(%s - (%s), 0)" % (recv, args)
+        else:
+            rep = "((%s) & -(%s)).bit_length() - 1" % (recv, recv)
+        s = s[:start] + rep + s[close + 1:]
+
+
+def _block_value(body):
+    """Python expression for a Rust block `{ stmts; expr }` used as a value
+    (assertions dropped; the block must end in its value)."""
+    stmts = [t.strip() for t in _split_top(body, ";")]
+    stmts = [t for t in stmts if t and not t.startswith("debug_assert")]
+    assert len(stmts) == 1, stmts
+    return expr(stmts[0])
+
+
+def _match_expr(s, at):
+    """(end, python) of the `match` expression starting at s[at]."""
+    brace = s.index("{", at)
+    scrut = s[at + 5:brace].strip()
+    close = _match(s, brace)
+    arms, default = [], None
+    rest = s[brace + 1:close].strip()
+    while rest:
+        arrow = rest.index("=>")
+        pat = rest[:arrow].strip()
+        rest = rest[arrow + 2:].lstrip()
+        if rest.startswith("{"):
+            e = _match(rest, 0)
+            val = _block_value(rest[1:e])
+            rest = rest[e + 1:].lstrip()
+            if rest.startswith(","):
+                rest = rest[1:].lstrip()
+        else:
+            parts = _split_top(rest, ",")
+            val = expr(parts[0])
+            rest = ",".join(parts[1:]).lstrip()
+        if pat == "_":
+            default = val
+        else:
+            arms.append(([p.strip() for p in _split_top(pat, "|")], val))
+    sv = expr(scrut)
+    out = default if default is not None else "None"
+    for alts, val in reversed(arms):
+        cond = " or ".join("(%s) == (%s)" % (sv, expr(a)) for a in alts)
+        out = "((%s) if (%s) else %s)" % (val, cond, out)
+    return close + 1, out
+
+
+def _if_expr(s, at):
+    """(end, python) of the `if .. { } else { }` expression starting at s[at]."""
+    brace = _top_brace(s, at + 2)
+    cond = s[at + 2:brace]
+    close = _match(s, brace)
+    then = _block_value(s[brace + 1:close])
+    j = close + 1
+    while s[j].isspace():
+        j += 1
+    assert s.startswith("else", j), "if without else used as a value"
+    j += 4
+    while s[j].isspace():
+        j += 1
+    if s.startswith("if", j):
+        end, other = _if_expr(s, j)
+    else:
+        end = _match(s, j) + 1
+        other = "(%s)" % _block_value(s[j + 1:end - 1])
+    return end, "((%s) if (%s) else %s)" % (then, expr(cond), other)
+
+
+def _top_brace(s, start):
+    """Index of the first `{` at bracket depth 0 from start (ends an `if`
+    condition or a `for` header)."""
+    depth = 0
+    for j in range(start, len(s)):
+        c = s[j]
+        if c == "{" and depth == 0:
+            return j
+        if c in "([":
+            depth += 1
+        elif c in ")]":
+            depth -= 1
+    raise ValueError(s)
+
+
+def expr(s):
+    """Python source of one Rust expression."""
+    s = s.strip()
+    stash = []
+    # `match` / `if` expressions become Python conditionals, held aside under
+    # placeholder names while the rest is translated
+    while True:
+        m = re.search(r"\b(match|if)\b", s)
+        if not m:
+            break
+        end, py = (_match_expr if m.group(1) == "match" else _if_expr)(s, m.start())
+        s = s[:m.start()] + " __py%d__ " % len(stash) + s[end:]
+        stash.append(py)
+    s = s.replace("&&", " and ").replace("||", " or ")
+    s = re.sub(r"\.sum::<\w+>\(\)", ".sum()", s)
+    s = _vec_macro(s)
+    s = _slices(s)
+    m = re.match(r"^\s*\|([^|]*)\|", s)
+    if m:
+        # a closure: |params| body  or  |params| -> T { body }
+        params = ", ".join(p.split(":")[0].strip() for p in m.group(1).split(",") if p.strip())
+        rest = s[m.end():].lstrip()
+        if rest.startswith("->"):
+            brace = rest.index("{")
+            body = _block_value(rest[brace + 1:_match(rest, brace)])
+        else:
+            body = expr(rest)
+        s = s[:m.start()] + "(lambda %s: %s)" % (params, body)
+    else:
+        s = re.sub(r"!(?!=)", " not ", s)
+        s = re.sub(r"&(?=[\w\[])", "", s)
+        s = re.sub(r"(?<![\w\s)\]])\s*\*(line)\b", r"_list(\1)", s)  # `*line`: a copy of the array
+        s = re.sub(r"\b(?:i32|i64|u32|usize|u8|i8)::from\(", "(", s)
+        s = re.sub(r"\s+as\s+(?:usize|i32|i64|u32|u8|i8|u16)\b", "", s)
+        s = re.sub(r"\b(\d+)(?:i32|i64|u32|usize|u8|i8)\b", r"\1", s)
+        s = _ranges(s)
+        s = _methods(s)
+    for k in range(len(stash) - 1, -1, -1):
+        s = s.replace("__py%d__" % k, stash[k])
+    return s
+
+
+def _vec_macro(s):
+    """vec![v; n] -> [v] * (n)."""
+    while "vec![" in s:
+        i = s.index("vec![")
+        close = _match(s, i + 4)
+        v, n = _split_top(s[i + 5:close], ";")
+        s = s[:i] + "([%s] * (%s))" % (v.strip(), n.strip()) + s[close + 1:]
+    return s
+
+
+def _slices(s):
+    """x[..=e] -> x[:e + 1], x[..e] -> x[:e], x[a..b] -> x[a:b]."""
+    out, i = [], 0
+    while i < len(s):
+        c = s[i]
+        if c == "[" and i > 0 and (s[i - 1].isalnum() or s[i - 1] in "_)]"):
+            close = _match(s, i)
+            inner = s[i + 1:close]
+            rm = re.match(r"^(.*?)\.\.(=?)(.*)$", inner, re.S) if ".." in inner and _split_top(inner, ",") == [inner] else None
+            if rm:
+                a, incl, b = rm.group(1).strip(), rm.group(2), rm.group(3).strip()
+                b = "(%s) + 1" % b if incl and b else b
+                out.append("[%s:%s]" % (a, b))
+            else:
+                out.append("[" + _slices(inner) + "]")
+            i = close + 1
+            continue
+        out.append(c)
+        i += 1
+    return "".join(out)
+
+
+def _ranges(s):
+    """a..b / a..=b outside brackets -> range(a, b) / range(a, b + 1); the
+    range must be the whole of a parenthesised group or of the string."""
+    def conv(t):
+        rm = re.match(r"^(.*?)\.\.(=?)(.*)$", t, re.S)
+        a, b = rm.group(1).strip(), rm.group(3).strip()
+        return "_range(%s, %s)" % (a, "(%s) + 1" % b if rm.group(2) else b)
+    if ".." not in s:
+        return s
+    if _split_top(s, ".") and re.match(r"^[^()\[\]]*\.\.[^()\[\]]*$", s.strip()):
+        return conv(s.strip())
+    out, i = [], 0
+    while i < len(s):
+        if s[i] == "(":
+            close = _match(s, i)
+            inner = s[i + 1:close]
+            if ".." in inner and re.match(r"^[^()\[\]]*\.\.[^()\[\]]*$", inner.strip()):
+                out.append("(" + conv(inner) + ")")
+            else:
+                out.append("(" + _ranges(inner) + ")")
+            i = close + 1
+            continue
+        out.append(s[i])
+        i += 1
+    return "".join(out)
+
+
+def _struct_literal(stmt):
+    """`Name { a, b: e, }` -> _struct(a=a, b=e), or None."""
+    m = re.match(r"^([A-Z]\w*)\s*\{(.*)\}$", stmt.strip(), re.S)
+    if not m:
+        return None
+    fields = [f.strip() for f in _split_top(m.group(2), ",") if f.strip()]
+    kv = []
+    for f in fields:
+        if ":" in f:
+            k, v = f.split(":", 1)
+            kv.append("%s=(%s)" % (k.strip(), expr(v)))
+        else:
+            kv.append("%s=%s" % (f, f))
+    return "_struct(%s)" % ", ".join(kv)
+
+
+def _pattern(p):
+    p = re.sub(r"\bmut\s+", "", p.strip())
+    p = re.sub(r"&(?=\w)", "", p)
+    return p.split(":")[0].strip() if "(" not in p else p
+
+
+def block(body, indent="    "):
+    """Python statements (as lines) for the Rust statements of body."""
+    out, s = [], body.strip()
+    while s:
+        if s.startswith("//"):
+            s = s[s.index("\n") + 1:].lstrip() if "\n" in s else ""
+            continue
+        if re.match(r"if\b", s):
+            brace = _top_brace(s, 2)
+            cond = s[2:brace]
+            close = _match(s, brace)
+            out.append(indent + "if (%s):" % expr(cond))
+            out += block(s[brace + 1:close], indent + "    ") or [indent + "    pass"]
+            s = s[close + 1:].lstrip()
+            if s.startswith("else"):
+                s = s[4:].lstrip()
+                brace = s.index("{")
+                close = _match(s, brace)
+                out.append(indent + "else:")
+                out += block(s[brace + 1:close], indent + "    ") or [indent + "    pass"]
+                s = s[close + 1:].lstrip()
+            continue
+        if re.match(r"for\b", s):
+            brace = _top_brace(s, 3)
+            head = s[3:brace]
+            pat, it = head.split(" in ", 1)
+            close = _match(s, brace)
+            body_src = s[brace + 1:close]
+            im = re.match(r"^\s*(\w+)\.iter_mut\(\)\s*$", it)
+            tm = re.match(r"^\s*(\w+)\.iter_mut\(\)\.take\((.+)\)\.enumerate\(\)\s*$", it, re.S)
+            pm = re.match(r"^\s*\(\s*(\w+)\s*,\s*(\w+)\s*\)\s*$", pat)
+            if tm and pm:
+                # for (i, v) in xs.iter_mut().take(n).enumerate() { *v = e; }
+                seq, idx, var = tm.group(1), pm.group(1), pm.group(2)
+                out.append(indent + "for %s in _range(0, _min(%s, _len(%s))):" % (idx, expr(tm.group(2)), seq))
+                body_src = re.sub(r"\*%s\b" % var, "%s[%s]" % (seq, idx), body_src)
+            elif im:
+                # for v in xs.iter_mut() { *v op= e; }  ->  over the indices of xs
+                var, seq = _pattern(pat), im.group(1)
+                k = "_i_%s" % var
+                out.append(indent + "for %s in _range(0, _len(%s)):" % (k, seq))
+                body_src = re.sub(r"\*%s\b" % var, "%s[%s]" % (seq, k), body_src)
+            else:
+                rm = re.match(r"^(.+?)\.\.(=?)(.+)$", it.strip(), re.S)
+                if rm and "[" not in rm.group(1) and "(" not in rm.group(1):
+                    b = expr(rm.group(3))
+                    rng = "_range(%s, %s)" % (expr(rm.group(1)), "(%s) + 1" % b if rm.group(2) else b)
+                else:
+                    rng = expr(it)
+                out.append(indent + "for %s in (%s):" % (_pattern(pat), rng))
+            out += block(body_src, indent + "    ")
+            s = s[close + 1:].lstrip()
+            continue
+        parts = _split_top(s, ";")
+        stmt = parts[0].strip()
+        s = ";".join(parts[1:]).lstrip()
+        if not stmt or stmt.startswith("debug_assert"):
+            continue
+        if not parts[1:]:
+            lit = _struct_literal(stmt)
+            out.append(indent + "return (%s)" % (lit or expr(stmt)))  # tail expression
+            continue
+        cm = re.match(r"^([\w\[\]\s\.\-\+\*]+?)\s*(\+|-|\*|<<|>>|\||&|\^)=\s*(.+)$", stmt, re.S)
+        if stmt == "return":
+            out.append(indent + "return None")
+        elif cm:
+            out.append(indent + "%s %s= (%s)" % (cm.group(1).strip(), cm.group(2), expr(cm.group(3))))
+        elif stmt.startswith("let "):
+            lhs, rhs = stmt[4:].split("=", 1)
+            out.append(indent + "%s = (%s)" % (_pattern(lhs), expr(rhs)))
+        elif re.match(r"[\w\[\]\s\-\+\*]+=[^=]", stmt) and not re.match(r"[\w\s]+[<>!=]=", stmt):
+            lhs, rhs = stmt.split("=", 1)
+            out.append(indent + "%s = (%s)" % (lhs.strip(), expr(rhs)))
+        else:
+            out.append(indent + "(%s)" % expr(stmt))
+    return out
+
+
+def _clamp(v, lo, hi):
+    assert lo <= hi, (lo, hi)  # Rust's Ord::clamp panics when min > max
+    return lo if v < lo else hi if v > hi else v
+
+
+def _fill(xs, v):
+    xs[:] = [v] * len(xs)
+
+
+def compile_fn(text, name, extra=None):
+    """A Python function with the parameters and body of Rust fn `name`."""
+    params, body = fn_source(text, name)
+    names = [p.split(":")[0].strip() for p in _split_top(params, ",") if p.strip()]
+    names = [re.sub(r"^&?(mut\s+)?", "", n) for n in names]
+    src = "def %s(%s):\n%s\n" % (name, ", ".join(names), "\n".join(block(body)) or "    pass")
+    # builtins under names that the reference's locals (`sum`, `min`, ...) cannot shadow
+    ns = {"_clamp": _clamp, "_struct": types.SimpleNamespace, "_abs": abs, "_min": min, "_max": max, "_zip": zip,
+          "_enumerate": enumerate, "_list": list, "_len": len, "_sum": sum, "_range": range, "_fill": _fill,
+          "p_at": lambda line, i: line[7 - i], "q_at": lambda line, j: line[8 + j]}
+    ns.update(extra or {})
+    exec(compile(src, "<rust:%s>" % name, "exec"), ns)
+    fn = ns[name]
+    fn.py_source = src
+    return fn

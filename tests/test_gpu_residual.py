@@ -134,3 +134,30 @@ def test_large_tile_classes_extreme_values(acc, po, seed):
                 off += w * h
     tus = np.array(descs, dtype=W.TU_DTYPE)
     _check(acc, po, tus, np.concatenate(blocks))
+
+
+def test_dequant_scale_readout_every_class_qp_and_bit_depth(acc, po):
+    """One DC level of +-1 and +-37 per TU, for every size class (tiled and
+    generic kernels), every qp % 6 over the whole QP range and bit depths
+    8..12: the dequantisation scales and shifts of both kernels, odd-area
+    (sqrt 2) classes included, against the oracle (whose scales are checked
+    against the text of vvc/dct.rs:37-40)."""
+    descs, off = [], 0
+    for bd in range(8, 13):
+        for lw in range(0, 7):
+            for lh in range(0, 7):
+                w, h = 1 << lw, 1 << lh
+                if w * h < 2:
+                    continue
+                for qp in range(0, 64 + 6 * (bd - 8), 5):
+                    t = np.zeros(1, W.TU_DTYPE)[0]
+                    t["offset"], t["log2_w"], t["log2_h"], t["bit_depth"], t["qp"] = off, lw, lh, bd, qp
+                    descs.append(t)
+                    off += w * h
+                    off = (off + 3) & ~3
+    tus = np.array(descs, W.TU_DTYPE)
+    coeff = np.zeros(off, np.int16)
+    vals = np.array([1, -1, 37, -37], np.int16)
+    coeff[tus["offset"]] = vals[np.arange(len(tus)) % 4]
+    print("dequant readout: %d TUs" % len(tus), flush=True)
+    _check(acc, po, tus, coeff)

@@ -98,3 +98,58 @@ def test_deblocking_tables(po, src):
         assert t == tc[idx]
         if idx < 64:
             assert b == beta[idx] << 2
+
+
+def _tr_shift(w, h, bd):
+    """transform_shift_rect_bd (vvc/dct.rs:858-860) with the -1 of odd areas."""
+    lsum = int(np.log2(w)) + int(np.log2(h))
+    return 15 - bd - (lsum >> 1) - (lsum & 1)
+
+
+def test_dequant_scales(po, src):
+    """INV_QUANT_SCALES / INV_QUANT_SCALES_SQRT2 (vvc/dct.rs:37-40) read out of
+    the oracle's dequantize_rect_bd: a level of 1 at a QP whose right shift is
+    0 comes out as the scale itself, for every qp % 6, every bit depth and
+    every size class (the odd-area classes use the sqrt(2) table)."""
+    inv = parse_table(src["dct.rs"], "INV_QUANT_SCALES")
+    inv2 = parse_table(src["dct.rs"], "INV_QUANT_SCALES_SQRT2")
+    for bd in range(8, 13):
+        for lw in range(1, 7):
+            for lh in range(1, 7):
+                w, h = 1 << lw, 1 << lh
+                if w * h < 4:
+                    continue
+                # right_shift = 6 - (tr_shift + qp / 6) = 0
+                per = 6 - _tr_shift(w, h, bd)
+                for rem in range(6):
+                    qp = 6 * per + rem
+                    if qp < 4 or qp > 63 + 6 * (bd - 8):
+                        continue
+                    lv = np.zeros((h, w), np.int32)
+                    lv[0, 0] = 1
+                    lv[h - 1, w - 1] = -1
+                    out = po.dequant_rect(w, h, lv, qp, bd)
+                    want = (inv2 if (lw + lh) & 1 else inv)[rem]
+                    assert out[0, 0] == want and out[h - 1, w - 1] == -want, (w, h, bd, qp)
+
+
+def test_quant_scales(po, src):
+    """QUANT_SCALES / QUANT_SCALES_SQRT2 (vvc/dct.rs:43-46) read out of the
+    oracle's quantize_rect_bd: a coefficient of 2^iqbits quantises to the
+    scale itself (the rounding offset is below 2^iqbits), with iqbits =
+    14 + qp / 6 + tr_shift (vvc/dct.rs:1131-1134)."""
+    q = parse_table(src["dct.rs"], "QUANT_SCALES")
+    q2 = parse_table(src["dct.rs"], "QUANT_SCALES_SQRT2")
+    for bd in (8, 10):
+        for lw in range(2, 6):
+            for lh in range(2, 6):
+                w, h = 1 << lw, 1 << lh
+                for qp in range(4, 30):
+                    iq = 14 + qp // 6 + _tr_shift(w, h, bd)
+                    c = np.zeros((h, w), np.int32)
+                    c[0, 0] = 1 << iq
+                    c[0, 1] = -(1 << iq)
+                    for intra in (True, False):
+                        out = po.quant_rect(w, h, c, qp, bd, intra)
+                        want = (q2 if (lw + lh) & 1 else q)[qp % 6]
+                        assert out[0, 0] == want and out[0, 1] == -want, (w, h, bd, qp, intra)
