@@ -75,24 +75,86 @@ def load_package():
     return mod
 
 
-def cpu_baseline(frames, seconds):
-    """The CPU oracle (oracle/liboracle.so, scalar C, one thread) on a bounded
-    sample: the first frames of the batch, repeated until `seconds` elapse."""
+def host_info():
+    """CPU model and the compiler of the oracle build, for the CPU baseline."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        import subprocess
+        cc = subprocess.run(["gcc", "--version"], capture_output=True, text=True, timeout=20).stdout.splitlines()[0]
+    except Exception:
+        cc = None
+    return model, cc
+
+
+def host_threads():
+    """Threads for the all-cores leg: this process's CPU share (the affinity
+    mask), not the machine's core count."""
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def median_rate(fn, work_units, runs=7):
+    """Units per second of fn(), as the median of `runs` timed calls after one
+    warm-up call (the method of crates/rustmpeg-vulkan/examples/benchmark.rs:66-81)."""
+    fn()
+    ts = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    ts.sort()
+    return work_units / ts[len(ts) // 2], ts[len(ts) // 2]
+
+
+def cpu_baseline(tus, coeff, luma_px, n_frames):
+    """The CPU oracle (oracle/liboracle.so, scalar C) over the whole residual
+    workload: one host thread, and every host thread of this process (OpenMP
+    over TUs); median of 7 after a warm-up. `value` is the all-threads rate."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import pyoracle as po
-    po.lib()
-    done_px, t0, reps = 0, time.perf_counter(), 0
-    while True:
-        for tus, coeff in frames:
-            po.residual_batch(tus, coeff)
-            done_px += 1920 * 1080
-        reps += 1
-        el = time.perf_counter() - t0
-        if el >= seconds:
-            break
-    return {"value": round(done_px / el / 1e6, 3), "unit": "Mpix/s", "cores": 1, "kind": "port",
-            "sample": "%d frame(s) of the same 1080p10 TU workload, %d pass(es), %.1f s, scalar C oracle "
-                      "(oracle/orc_residual.c) on one host core" % (len(frames), reps, el)}
+    L = po.lib()
+    n = host_threads()
+    L.orc_set_threads(1)
+    r1, t1 = median_rate(lambda: po.residual_batch(tus, coeff), luma_px)
+    L.orc_set_threads(n)
+    rn, tn = median_rate(lambda: po.residual_batch(tus, coeff), luma_px)
+    L.orc_set_threads(1)
+    model, cc = host_info()
+    return {"value": round(rn / 1e6, 2), "unit": "Mpix/s", "cores": n, "kind": "port",
+            "sample": "the whole workload (%d frames of 1080p10 TUs), median of 7 runs after a warm-up; scalar C "
+                      "oracle (oracle/orc_residual.c), OpenMP over TUs on %d threads" % (n_frames, n),
+            "ms_per_step": round(tn * 1e3, 2),
+            "single_thread": {"value": round(r1 / 1e6, 3), "unit": "Mpix/s", "cores": 1, "ms_per_step": round(t1 * 1e3, 1)},
+            "cpu_model": model, "compiler": cc}
+
+
+def cpu_pipeline_baseline(PL, po, rmh, frames):
+    """The CPU oracle over the P-frame pipeline (MC -> residual add ->
+    deblock) of `frames`: one thread and every thread of this process (OpenMP
+    over PUs, TUs and frames), median of 7 after a warm-up."""
+    L = po.lib()
+    px = sum(int(f["width"]) * int(f["height"]) for f in frames)
+    n = host_threads()
+    L.orc_set_threads(1)
+    r1, t1 = median_rate(lambda: PL.oracle(po, rmh.abi, frames), px)
+    L.orc_set_threads(n)
+    rn, tn = median_rate(lambda: PL.oracle(po, rmh.abi, frames), px)
+    L.orc_set_threads(1)
+    return {"value": round(rn / 1e6, 2), "unit": "Mpix/s", "cores": n, "kind": "port",
+            "sample": "the same %d P-frames, median of 7 runs after a warm-up; oracle MC, recon and deblock "
+                      "(OpenMP over PUs, TUs and frames)" % len(frames),
+            "ms_per_step": round(tn * 1e3, 2),
+            "single_thread": {"value": round(r1 / 1e6, 3), "unit": "Mpix/s", "cores": 1,
+                              "ms_per_step": round(t1 * 1e3, 1)}}
 
 
 def timed(hip, stream, fn, steps, warmup):
@@ -129,7 +191,7 @@ def check_batch(PL, po, rmh, batch, frames, which):
     return True
 
 
-def bench_pipeline(rmh, acc, W, PL, po, n_frames, steps, warmup):
+def bench_pipeline(rmh, acc, W, PL, po, n_frames, steps, warmup, cpu=True):
     """P-frame reconstruction, MC -> residual add -> deblock, over n_frames
     1080p10 frames: one frame-step graph over the batch against the same
     plans built per frame (unbatched)."""
@@ -169,6 +231,10 @@ def bench_pipeline(rmh, acc, W, PL, po, n_frames, steps, warmup):
            "stages": stages}
     out["vs_unbatched"] = round(out["value"] / out["unbatched"]["value"], 3)
     log("pipeline: %.3f ms/step %.1f Mpix/s, unbatched %.3f ms (x%.2f)" % (ms, out["value"], ums, out["vs_unbatched"]))
+    if cpu:
+        out["cpu_baseline"] = cpu_pipeline_baseline(PL, po, rmh, frames)
+        log("pipeline cpu oracle: %s Mpix/s on %d threads, %s on one" % (
+            out["cpu_baseline"]["value"], out["cpu_baseline"]["cores"], out["cpu_baseline"]["single_thread"]["value"]))
 
     # the same chain built in scratch planes and deblocked out of place into
     # the picture (no shadow copy of the pre-deblocking samples)
@@ -218,65 +284,82 @@ def bench_pipeline(rmh, acc, W, PL, po, n_frames, steps, warmup):
     return out
 
 
-STREAMS = [(856, 480, 8)] * 4 + [(1920, 1080, 10)] * 8 + [(3840, 2160, 10)] * 4
-
-
-def bench_streams(rmh, acc, W, PL, po, frame_steps, reps):
-    """16 concurrent streams (4x 854x480 8-bit coded as 856x480, 8x 1080p10,
-    4x 2160p10), one frame of each per frame step: one graph per frame step
-    over all streams against per-stream plans launched one after another."""
+def bench_streams(rmh, acc, PL, po, steps, reps):
+    """The batched multi-stream configuration: 16 concurrent streams
+    (4x 854x480 8-bit coded as 856x480, 8x 1080p10, 4x 2160p10) x `steps`
+    chained frame steps with the residual tool mix; each step is one graph
+    over every stream, and its output is the next step's reference. Against
+    the same steps with one launch per stage per stream."""
     hip = rmh.hip
     t0 = time.perf_counter()
-    frames = [W.pframe(3000 + i, width=w, height=h, bit_depth=bd) for i, (w, h, bd) in enumerate(STREAMS)]
-    log("streams: generated %d frames in %.1f s" % (len(frames), time.perf_counter() - t0))
-    batch = PL.DeviceBatch(acc, rmh, frames)
+    refs, frames = PL.generate_streams(PL.STREAMS, steps, tools=True, seed=3000, workers=min(16, host_threads()))
+    log("streams: generated %d frames in %.1f s" % (steps * len(PL.STREAMS), time.perf_counter() - t0))
+    chain = PL.ChainedStreams(acc, rmh, refs, frames)
     stream = hip.Stream()
-    graph = batch.graph()
-    graph.launch(stream.handle)
+    chain.run(stream.handle)
     stream.synchronize()
-    if not check_batch(PL, po, rmh, batch, frames, (0, 4, 12)):
-        return None
-
-    def run_steps(s):
-        for _ in range(frame_steps):
-            graph.launch(s)
-    ms = timed(hip, stream, run_steps, reps, 1)
-    px = batch.luma_pixels() * frame_steps
-    singles = [PL.DeviceBatch(acc, rmh, [f]) for f in frames]
-
-    def unbatched(s):
-        for _ in range(frame_steps):
-            for b in singles:
-                b.run(s)
-    ums = timed(hip, stream, unbatched, reps, 1)
-    for b in singles:
-        b.close()
-    graph.close()
-    batch.close()
-    out = {"workload": "16 streams (4x 856x480 8-bit, 8x 1920x1080 10-bit, 4x 3840x2160 10-bit) x %d frame steps, "
-                       "MC -> residual add -> deblock, one graph per frame step" % frame_steps,
-           "value": round(px / (ms / 1e3) / 1e6, 1), "unit": "Mpix/s", "ms_per_frame_step": round(ms / frame_steps, 4),
-           "unbatched": {"value": round(px / (ums / 1e3) / 1e6, 1), "ms_per_frame_step": round(ums / frame_steps, 4),
-                         "launches_per_frame_step": 4 * len(frames)}}
+    # bit-exact gate: three streams (one of each size) after the last step
+    got = chain.output(steps - 1)
+    for k in (0, 4, 12):
+        want = PL.oracle_chain(po, rmh.abi, refs[k], [frames[t][k] for t in range(steps)])
+        if not all((got[k][c] == want[c]).all() for c in range(3)):
+            log("bench: stream %d differs from the oracle after %d chained steps" % (k, steps))
+            chain.close()
+            return None
+    ms = timed(hip, stream, chain.run, reps, 1) / steps
+    px = chain.luma_pixels_per_step
+    step_bytes = sum(chain.bytes.values()) / steps
+    # per-stage time: each stage's plans of every step run alone
+    stage_ms = {}
+    for k, name in enumerate(("mc", "recon", "deblock")):
+        def run_stage(st, k=k):
+            for plans in chain.plans:
+                if k == 1:
+                    plans[1].run(plans[3].ptr, 0, st)
+                else:
+                    plans[k].run(stream=st)
+        stage_ms[name] = timed(hip, stream, run_stage, reps, 1) / steps
+    chain.build_unbatched(rmh, frames)
+    ums = timed(hip, stream, chain.run_unbatched, reps, 1) / steps
+    n_tu = sum(len(f["tus"]) for fr in frames for f in fr)
+    n_tool = sum(int(((f["tus"]["lfnst_idx"] > 0) | (f["tus"]["flags"] > 0)).sum()) for fr in frames for f in fr)
+    achieved = step_bytes / (ms / 1e3) / 1e9
+    out = {"workload": "16 streams (4x 856x480 8-bit, 8x 1920x1080 10-bit, 4x 3840x2160 10-bit, 4:2:0) x %d chained "
+                       "frame steps (each step's output is the next one's reference; distinct PUs, TUs and edges per "
+                       "step), MC -> residual add -> deblock, residual tool mix (MTS, LFNST, transform skip, dependent "
+                       "quantisation on %.0f %% of %d TUs), one graph per frame step" % (steps, 100.0 * n_tool / n_tu,
+                                                                                        n_tu),
+           "value": round(px / (ms / 1e3) / 1e6, 1), "unit": "Mpix/s", "ms_per_frame_step": round(ms, 4),
+           "luma_px_per_frame_step": px,
+           "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(achieved / HBM_PEAK_GBS, 4),
+                        "algorithmic_bytes_per_frame_step": int(step_bytes)},
+           "stages": {name: {"ms": round(t, 4), "algorithmic_bytes": int(chain.bytes[name] / steps),
+                             "achieved_gbs": round(chain.bytes[name] / steps / (t / 1e3) / 1e9, 1),
+                             "frac": round(chain.bytes[name] / steps / (t / 1e3) / 1e9 / HBM_PEAK_GBS, 4)}
+                      for name, t in stage_ms.items()},
+           "unbatched": {"value": round(px / (ums / 1e3) / 1e6, 1), "ms_per_frame_step": round(ums, 4),
+                         "launches_per_frame_step": 4 * len(PL.STREAMS)}}
     out["vs_unbatched"] = round(out["value"] / out["unbatched"]["value"], 3)
-    log("streams: %.3f ms/frame step %.1f Mpix/s, unbatched x%.2f" % (ms / frame_steps, out["value"], out["vs_unbatched"]))
+    log("streams: %.3f ms/frame step %.1f Mpix/s (%.1f GB/s), unbatched x%.2f" % (ms, out["value"], achieved,
+                                                                                  out["vs_unbatched"]))
+    chain.close()
 
     # built in scratch planes, deblocked out of place into the pictures
-    ob = PL.DeviceBatch(acc, rmh, frames, out_of_place=True)
-    ograph = ob.graph()
-    ograph.launch(stream.handle)
+    ob = PL.ChainedStreams(acc, rmh, refs, frames, out_of_place=True)
+    ob.run(stream.handle)
     stream.synchronize()
-    if check_batch(PL, po, rmh, ob, frames, (0, 4, 12)):
-        def run_oop(s):
-            for _ in range(frame_steps):
-                ograph.launch(s)
-        oms = timed(hip, stream, run_oop, reps, 1)
+    got = ob.output(steps - 1)
+    want = PL.oracle_chain(po, rmh.abi, refs[12], [frames[t][12] for t in range(steps)])
+    if all((got[12][c] == want[c]).all() for c in range(3)):
+        oms = timed(hip, stream, ob.run, reps, 1) / steps
         out["out_of_place"] = {"value": round(px / (oms / 1e3) / 1e6, 1), "unit": "Mpix/s",
-                               "ms_per_frame_step": round(oms / frame_steps, 4)}
-        log("streams out of place: %.3f ms/frame step" % (oms / frame_steps))
+                               "ms_per_frame_step": round(oms, 4),
+                               "achieved_gbs": round(step_bytes / (oms / 1e3) / 1e9, 1)}
+        log("streams out of place: %.3f ms/frame step" % oms)
     else:
+        log("bench: out-of-place stream 12 differs from the oracle")
         out["out_of_place"] = None
-    ograph.close()
     ob.close()
     return out
 
@@ -520,7 +603,6 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--frames", type=int, default=32)
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-unbatched", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true")
@@ -667,8 +749,8 @@ def main():
     sys.path.insert(0, os.path.join(REPO, "tests"))
     import pipeline as PL
     pipe = None if args.no_pipeline else bench_pipeline(rmh, acc, W, PL, po, args.pipeline_frames, args.steps,
-                                                         args.warmup)
-    streams = None if args.no_streams else bench_streams(rmh, acc, W, PL, po, 8, max(2, args.steps // 5))
+                                                         args.warmup, cpu=not args.no_cpu)
+    streams = None if args.no_streams else bench_streams(rmh, acc, PL, po, 8, max(2, args.steps // 5))
 
     bi = None if args.no_bi else bench_bi(rmh, acc, W, po, args.pipeline_frames, args.steps, args.warmup)
     tools = None if args.no_tools else bench_tool_mix(rmh, acc, W, po, 8, args.steps, args.warmup)
@@ -677,9 +759,10 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(frames[:2], args.cpu_seconds)
+        cpu = cpu_baseline(tus, coeff, luma_px, args.frames)
         cpu["cores_on_host"] = os.cpu_count()
-        log("cpu oracle: %s Mpix/s" % cpu["value"])
+        log("cpu oracle: %s Mpix/s on %d threads, %s on one" % (cpu["value"], cpu["cores"],
+                                                              cpu["single_thread"]["value"]))
 
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "Mpix/s", "n_gpus": 1, "steps": args.steps,

@@ -414,7 +414,11 @@ static void deblock_frame(const rmh_dbk_frame* f) {
     }
 }
 
+extern int orc_threads;
+
 size_t orc_deblock_frames(const rmh_dbk_frame* frames, size_t n_frames) {
+    /* frames are independent; within a frame the reference's order holds */
+#pragma omp parallel for num_threads(orc_threads) schedule(dynamic, 1) if (orc_threads > 1)
     for (size_t i = 0; i < n_frames; i++) deblock_frame(&frames[i]);
     return 0;
 }

@@ -271,21 +271,28 @@ static uint16_t* widen(const rmh_plane* p) {
     return o;
 }
 
+extern int orc_threads;
+
 size_t orc_mc_batch(const rmh_mc_frame* frames, size_t n_frames, const rmh_pu_desc* pus, size_t n_pu) {
     uint16_t** wide = calloc(n_frames * 3, sizeof(uint16_t*));
-    size_t bad = 0;
-    for (size_t i = 0; i < n_pu && !bad; i++) {
-        const rmh_pu_desc* pu = &pus[i];
-        if (pu->frame >= n_frames) {
+    /* PUs up to the first one with a bad frame index are predicted */
+    size_t bad = 0, n_ok = n_pu;
+    for (size_t i = 0; i < n_pu; i++) {
+        if (pus[i].frame >= n_frames) {
             bad = i + 1;
+            n_ok = i;
             break;
         }
+        for (int c = 0; c < 3; c++)
+            if (!wide[pus[i].frame * 3 + c]) wide[pus[i].frame * 3 + c] = widen(&frames[pus[i].frame].ref[c]);
+    }
+#pragma omp parallel for num_threads(orc_threads) schedule(dynamic, 16) if (orc_threads > 1)
+    for (size_t i = 0; i < n_ok; i++) {
+        const rmh_pu_desc* pu = &pus[i];
         const rmh_mc_frame* f = &frames[pu->frame];
         int w = 1 << (pu->log2_w & 15), h = 1 << (pu->log2_h & 15);
         int alt = (pu->log2_w & RMH_PU_ALT_HPEL) != 0;
         int bd = f->ref[0].bit_depth;
-        for (int c = 0; c < 3; c++)
-            if (!wide[pu->frame * 3 + c]) wide[pu->frame * 3 + c] = widen(&f->ref[c]);
         int32_t* buf = malloc(sizeof(int32_t) * (size_t)(w * h));
         const rmh_plane* r = &f->ref[0];
         orc_interp_luma(wide[pu->frame * 3], (int)r->width, (int)r->width, (int)r->height, bd, pu->x, pu->y, w, h,

@@ -516,6 +516,15 @@ int orc_residual_tu(const rmh_tu_desc* t, const int16_t* coeff, int16_t* residua
     return rc;
 }
 
+/* Threads of the batch drivers (orc_residual_batch, orc_recon_batch,
+ * orc_mc_batch, orc_deblock_frames): 1 (the default) runs them serially in
+ * the reference's order; more runs independent TUs, PUs or frames on an
+ * OpenMP team (the CPU baseline over all host cores). Results do not
+ * depend on it: no two TUs, PUs or frames of a valid batch share an output. */
+int orc_threads = 1;
+
+void orc_set_threads(int n) { orc_threads = n < 1 ? 1 : n; }
+
 size_t orc_residual_batch(const rmh_tu_desc* tus, size_t n_tu, const int16_t* coeff, size_t n_coeff,
                           int16_t* residual) {
     for (size_t i = 0; i < n_tu; i++) {
@@ -523,9 +532,12 @@ size_t orc_residual_batch(const rmh_tu_desc* tus, size_t n_tu, const int16_t* co
         if (t->log2_w > 6 || t->log2_h > 6) return i + 1;
         size_t n = (size_t)1 << (t->log2_w + t->log2_h);
         if ((size_t)t->offset + n > n_coeff) return i + 1;
-        if (orc_residual_tu(t, coeff + t->offset, residual + t->offset)) return i + 1;
     }
-    return 0;
+    size_t bad = 0;
+#pragma omp parallel for num_threads(orc_threads) schedule(dynamic, 64) if (orc_threads > 1) reduction(max : bad)
+    for (size_t i = 0; i < n_tu; i++)
+        if (orc_residual_tu(&tus[i], coeff + tus[i].offset, residual + tus[i].offset)) bad = bad > i + 1 ? bad : i + 1;
+    return bad;
 }
 
 /* Residual added to the prediction held in the picture and clipped to the
@@ -535,7 +547,6 @@ size_t orc_residual_batch(const rmh_tu_desc* tus, size_t n_tu, const int16_t* co
  * 2866-2870). Planes are host memory; TUs in descriptor order. */
 size_t orc_recon_batch(const rmh_tu_desc* tus, size_t n_tu, const int16_t* coeff, size_t n_coeff,
                        const rmh_tu_place* place, const rmh_recon_frame* frames, size_t n_frames) {
-    int16_t res[64 * 64];
     for (size_t i = 0; i < n_tu; i++) {
         const rmh_tu_desc* t = &tus[i];
         if (t->log2_w > 6 || t->log2_h > 6) return i + 1;
@@ -544,7 +555,18 @@ size_t orc_recon_batch(const rmh_tu_desc* tus, size_t n_tu, const int16_t* coeff
         if (place[i].frame >= n_frames || place[i].plane > 2) return i + 1;
         const rmh_plane* p = &frames[place[i].frame].planes[place[i].plane];
         if ((uint32_t)place[i].x + w > p->width || (uint32_t)place[i].y + h > p->height) return i + 1;
-        if (orc_residual_tu(t, coeff + t->offset, res)) return i + 1;
+    }
+    size_t bad = 0;
+#pragma omp parallel for num_threads(orc_threads) schedule(dynamic, 64) if (orc_threads > 1) reduction(max : bad)
+    for (size_t i = 0; i < n_tu; i++) {
+        int16_t res[64 * 64];
+        const rmh_tu_desc* t = &tus[i];
+        const int w = 1 << t->log2_w, h = 1 << t->log2_h;
+        const rmh_plane* p = &frames[place[i].frame].planes[place[i].plane];
+        if (orc_residual_tu(t, coeff + t->offset, res)) {
+            bad = bad > i + 1 ? bad : i + 1;
+            continue;
+        }
         const int maxv = (1 << p->bit_depth) - 1;
         for (int y = 0; y < h; y++)
             for (int x = 0; x < w; x++) {
@@ -561,5 +583,5 @@ size_t orc_recon_batch(const rmh_tu_desc* tus, size_t n_tu, const int16_t* coeff
                 }
             }
     }
-    return 0;
+    return bad;
 }

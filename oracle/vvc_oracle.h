@@ -111,6 +111,10 @@ int orc_intra_encode(const uint8_t* sy, const uint8_t* su, const uint8_t* sv, in
 int orc_intra_decode(int w, int h, const uint8_t* modes, const rmh_tu_desc* tus, const int16_t* coeff, uint8_t* ry,
                      uint8_t* ru, uint8_t* rv);
 
+/* Threads of the batch drivers (1, the default: serial in the reference's
+ * order; results never depend on it). */
+void orc_set_threads(int n);
+
 /* Per-line deblocking filters and sample decisions (vvc/filter.rs:276-304,
  * 477-631, 907-919, 1010-1057). Luma lines are [p7..p0, q0..q7], chroma
  * lines [p3..p0, q0..q3]. */

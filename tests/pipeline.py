@@ -169,3 +169,168 @@ class DeviceBatch:
         for p in (self.mc, self.recon, self.deblock, self.sao):
             if p:
                 p.close()
+
+
+# ---------------------------------------------------------------- streams
+
+STREAMS = [(856, 480, 8)] * 4 + [(1920, 1080, 10)] * 8 + [(3840, 2160, 10)] * 4
+
+
+def generate_streams(streams, steps, tools=True, seed=0, workers=8):
+    """Host data of chained streams: (refs, frames) with refs[s] the first
+    reference picture of stream s and frames[t][s] the P-frame of step t
+    (workloads.pframe_step; distinct PUs, TUs and edges at every step).
+    Generated on a thread pool (numpy releases the GIL for the bulk work)."""
+    import concurrent.futures
+    jobs = [(t, s) for t in range(steps) for s in range(len(streams))]
+
+    def one(job):
+        t, s = job
+        w, h, bd = streams[s]
+        return W.pframe_step(seed + 1000 * s + t, width=w, height=h, bit_depth=bd, tools=tools)
+
+    with concurrent.futures.ThreadPoolExecutor(workers) as ex:
+        refs = list(ex.map(lambda s: W.stream_reference(seed + s, *streams[s]), range(len(streams))))
+        flat = list(ex.map(one, jobs))
+    frames = [[flat[t * len(streams) + s] for s in range(len(streams))] for t in range(steps)]
+    for s, (w, h, bd) in enumerate(streams):
+        dt = sample_dtype(bd)
+        refs[s] = [np.ascontiguousarray(p.astype(dt)) for p in refs[s]]
+    return refs, frames
+
+
+def oracle_chain(po, abi, ref, steps_frames, on_step=None):
+    """The CPU oracle over one stream's chained steps: step t predicts from
+    step t-1's output (step 0 from ref). Returns the last step's planes;
+    on_step(t, planes) sees every step's output."""
+    cur = ref
+    for t, f in enumerate(steps_frames):
+        g = dict(f)
+        g["ref"] = cur
+        cur = oracle(po, abi, [g])[0]
+        if on_step:
+            on_step(t, cur)
+    return cur
+
+
+class ChainedStreams:
+    """Chained frame steps of several streams on the device: at step t every
+    stream predicts from the picture its step t-1 produced (step 0 from its
+    first reference picture), adds step t's residuals and deblocks. Two
+    device pictures per stream alternate as reference and output; each step
+    is one MC plan, one recon plan and one deblock plan over every stream,
+    captured as one graph. With out_of_place, MC and the residual add build
+    the picture in a scratch set and deblocking writes it out of place into
+    the output picture."""
+
+    def __init__(self, acc, rmh, refs, frames, out_of_place=False):
+        hip, abi = rmh.hip, rmh.abi
+        self.hip, self.acc = hip, acc
+        self.n_streams, self.n_steps = len(refs), len(frames)
+        self.keep, self.plans, self.graphs = [], [], []
+        self.shapes = [[p.shape for p in r] for r in refs]
+        self.dtypes = [r[0].dtype for r in refs]
+        self.bds = [frames[0][s]["bit_depth"] for s in range(self.n_streams)]
+        # pics[k][s] = (buffers, planes) of picture set k of stream s
+        self.pics = [[], []]
+        self.scratch = []
+        for s, r in enumerate(refs):
+            for k in range(2):
+                bufs = [hip.DeviceBuffer.from_array(p) if k == 0 else hip.DeviceBuffer(p.nbytes) for p in r]
+                self.keep += bufs
+                self.pics[k].append((bufs, [abi.plane_from_address(b.ptr, p.shape[1], p.shape[0], p.shape[1],
+                                                                    p.itemsize, self.bds[s])
+                                            for b, p in zip(bufs, r)]))
+            if out_of_place:
+                bufs = [hip.DeviceBuffer(p.nbytes) for p in r]
+                self.keep += bufs
+                self.scratch.append([abi.plane_from_address(b.ptr, p.shape[1], p.shape[0], p.shape[1], p.itemsize,
+                                                            self.bds[s]) for b, p in zip(bufs, r)])
+        self.bytes = {"mc": 0, "recon": 0, "deblock": 0}
+        for t, fr in enumerate(frames):
+            src, dst = self.pics[t % 2], self.pics[(t + 1) % 2]
+            tus, coeff, place, pus = concat_tus(fr)
+            dc = hip.DeviceBuffer.from_array(coeff)
+            edges = [hip.DeviceBuffer.from_array(np.ascontiguousarray(f["edges"])) for f in fr]
+            self.keep += [dc] + edges
+            work = [self.scratch[s] if out_of_place else dst[s][1] for s in range(self.n_streams)]
+            mc = acc.mc_plan([abi.mc_frame(src[s][1], work[s]) for s in range(self.n_streams)], pus)
+            rc = acc.recon_plan(tus, coeff.size, place, [abi.ReconFrame((abi.Plane * 3)(*work[s]))
+                                                         for s in range(self.n_streams)])
+            dbf = [abi.dbk_frame(work[s], edges[s].ptr, f["edges"].shape[1], f["edges"].shape[0], f["qp"])
+                   for s, f in enumerate(fr)]
+            if out_of_place:
+                db = acc.deblock_to_plan(dbf, [p for s in range(self.n_streams) for p in dst[s][1]])
+            else:
+                db = acc.deblock_plan(dbf)
+            self.plans.append((mc, rc, db, dc))
+            self.bytes["mc"] += mc.bytes
+            self.bytes["recon"] += rc.bytes
+            self.bytes["deblock"] += db.bytes
+            self.graphs.append(acc.graph([(mc, 0, 0), (rc, dc.ptr, 0), (db, 0, 0)]))
+        self.luma_pixels_per_step = sum(int(f["width"]) * int(f["height"]) for f in frames[0])
+
+    def run(self, stream, steps=None):
+        """Launches the frame-step graphs 0 .. steps-1 in order on stream."""
+        for g in self.graphs[:self.n_steps if steps is None else steps]:
+            g.launch(stream)
+
+    def run_unbatched(self, stream):
+        """The same steps with one launch per stage per stream: plans over a
+        single frame, built on first use."""
+        if not hasattr(self, "_single"):
+            raise RuntimeError("call build_unbatched first")
+        for step in self._single:
+            for mc, rc, db, dc in step:
+                mc.run(stream=stream)
+                rc.run(dc.ptr, 0, stream)
+                db.run(stream=stream)
+
+    def build_unbatched(self, rmh, frames):
+        """Per-stream plans of every step over the same device pictures."""
+        abi, hip = rmh.abi, self.hip
+        self._single = []
+        for t, fr in enumerate(frames):
+            src, dst = self.pics[t % 2], self.pics[(t + 1) % 2]
+            row = []
+            for s, f in enumerate(fr):
+                pus = f["pus"].copy()
+                pus["frame"] = 0
+                place = f["place"].copy()
+                place["frame"] = 0
+                dc = hip.DeviceBuffer.from_array(f["coeff"])
+                e = hip.DeviceBuffer.from_array(np.ascontiguousarray(f["edges"]))
+                self.keep += [dc, e]
+                mc = self.acc.mc_plan([abi.mc_frame(src[s][1], dst[s][1])], pus)
+                rc = self.acc.recon_plan(f["tus"], f["coeff"].size, place, [abi.ReconFrame((abi.Plane * 3)(*dst[s][1]))])
+                db = self.acc.deblock_plan([abi.dbk_frame(dst[s][1], e.ptr, f["edges"].shape[1], f["edges"].shape[0],
+                                                          f["qp"])])
+                row.append((mc, rc, db, dc))
+            self._single.append(row)
+
+    def output(self, t):
+        """[[Y, Cb, Cr]] per stream after step t (valid until step t + 2 runs)."""
+        k = (t + 1) % 2
+        return [[b.download(self.dtypes[s]).reshape(self.shapes[s][c]) for c, b in enumerate(self.pics[k][s][0])]
+                for s in range(self.n_streams)]
+
+    def reset(self, refs):
+        """Restores every stream's first reference picture (set 0)."""
+        for s, r in enumerate(refs):
+            for b, p in zip(self.pics[0][s][0], r):
+                b.upload(p)
+
+    def close(self):
+        for g in self.graphs:
+            g.close()
+        for mc, rc, db, _ in self.plans:
+            mc.close()
+            rc.close()
+            db.close()
+        for row in getattr(self, "_single", []):
+            for mc, rc, db, _ in row:
+                mc.close()
+                rc.close()
+                db.close()
+        for b in self.keep:
+            b.close()

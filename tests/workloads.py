@@ -233,6 +233,15 @@ def inter_frame(seed, width=1920, height=1080, bit_depth=10, mv_range=64, qp=32,
     cw, ch = (width + 1) // 2, (height + 1) // 2
     ref = [textured_plane(rng, width, height, bit_depth), textured_plane(rng, cw, ch, bit_depth),
            textured_plane(rng, cw, ch, bit_depth)]
+    d = _inter_layout(rng, width, height, bit_depth, mv_range, qp, ctu)
+    d["ref"] = ref
+    return d
+
+
+def _inter_layout(rng, width, height, bit_depth, mv_range, qp, ctu):
+    """The coded layout of a P-frame drawn from rng: CUs of a seeded quadtree,
+    one uni-prediction PU per CU, the edge map and the QPs."""
+    cw, ch = (width + 1) // 2, (height + 1) // 2
     cus = _tile(rng, width, height, ctu, ctu, 8, _p_split_inter)
     n = len(cus)
     pus = np.zeros(n, PU_DTYPE)
@@ -289,7 +298,7 @@ def inter_frame(seed, width=1920, height=1080, bit_depth=10, mv_range=64, qp=32,
     lq = np.where(big, 3, 1)
     edges[..., 3] = np.where(in_c & ((2 * jj) % 8 == 0) & diff_h, byte(bs_h, lp, lq, False), 0)
     chroma_qp = qp if qp <= 26 else qp - 1  # default chroma QP mapping, vvc/hls.rs:778-781
-    return dict(ref=ref, pus=pus, cus=cus, edges=edges, qp=[qp, chroma_qp, chroma_qp], ctu=ctu,
+    return dict(pus=pus, cus=cus, edges=edges, qp=[qp, chroma_qp, chroma_qp], ctu=ctu,
                 bit_depth=bit_depth, width=width, height=height)
 
 
@@ -358,7 +367,15 @@ def pframe(seed, width=1920, height=1080, bit_depth=10, mv_range=64, qp=32, ctu=
     size, max 32), with their placement. Returns inter_frame's dict plus
     tus, coeff, place (PLACE_DTYPE, frame index `frame`)."""
     d = inter_frame(seed, width, height, bit_depth, mv_range, qp, ctu)
-    rng = np.random.default_rng([4, seed])
+    _add_residuals(np.random.default_rng([4, seed]), d, bit_depth, qp, p_skip, frame)
+    return d
+
+
+def _add_residuals(rng, d, bit_depth, qp, p_skip, frame, tools=False):
+    """The residual TUs of every CU of layout d that is not skipped (luma
+    transform tree inside the CU, max 64; chroma likewise at half size, max
+    32) with their placement; tools adds LFNST, transform skip and dependent
+    quantisation at the rates of _tu_batch."""
     blocks = []
     for (x, y, w, h) in d["cus"]:
         if rng.random() < p_skip:
@@ -370,11 +387,30 @@ def pframe(seed, width=1920, height=1080, bit_depth=10, mv_range=64, qp=32, ctu=
             ch = []
             _split_block(rng, x // 2, y // 2, w // 2, h // 2, 32, 4, 0.2, ch)
             blocks += [(plane, bx, by, bw, bh) for (bx, by, bw, bh) in ch]
-    tus, coeff = _tu_batch(rng, blocks, bit_depth, (qp - 5, qp + 5))
+    tus, coeff = _tu_batch(rng, blocks, bit_depth, (qp - 5, qp + 5), tools, tools, tools)
     place = np.zeros(len(blocks), PLACE_DTYPE)
     place["frame"] = frame
     place["plane"] = [b[0] for b in blocks]
     place["x"] = [b[1] for b in blocks]
     place["y"] = [b[2] for b in blocks]
     d.update(tus=tus, coeff=coeff, place=place, blocks=blocks)
+
+
+def stream_reference(seed, width, height, bit_depth):
+    """The first reference picture of a synthetic stream: seeded textured
+    planes [Y, Cb, Cr] (4:2:0) as uint16."""
+    rng = np.random.default_rng([6, seed])
+    cw, ch = (width + 1) // 2, (height + 1) // 2
+    return [textured_plane(rng, width, height, bit_depth), textured_plane(rng, cw, ch, bit_depth),
+            textured_plane(rng, cw, ch, bit_depth)]
+
+
+def pframe_step(seed, width=1920, height=1080, bit_depth=10, mv_range=64, qp=32, ctu=128, p_skip=0.2, tools=True):
+    """One frame step of a chained stream: pframe's layout and residuals
+    without a reference picture (the previous step's output is the
+    reference). tools: the residual tool mix (LFNST, transform skip,
+    dependent quantisation)."""
+    rng = np.random.default_rng([5, seed])
+    d = _inter_layout(rng, width, height, bit_depth, mv_range, qp, ctu)
+    _add_residuals(rng, d, bit_depth, qp, p_skip, 0, tools)
     return d
