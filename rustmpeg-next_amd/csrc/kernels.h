@@ -22,15 +22,18 @@ struct DeviceTables {
 };
 
 // Per tile-eligible TU, in tile order, built at plan creation: what the tiled
-// kernel needs about the TU in one 16-byte load.
+// kernel needs about the TU in one 20-byte record.
 struct TileTu {
-    uint32_t off;     // element offset of the TU block
+    uint32_t off;     // element offset of the TU block in the residual output
+    uint32_t src;     // element offset of its levels (in ResidualArgs::prep when prepped)
     uint8_t cw, ch;   // coded region after zero-out (<= 32)
-    uint8_t scale;    // dequantisation scale
+    uint8_t scale;    // dequantisation scale (1 when prepped: the values are already dequantised)
     uint8_t shifts;   // dequant left shift | right shift << 4
     uint16_t mh, mv;  // dword offsets of the column-major matrices (col16)
     uint8_t shift2;   // second-stage shift, 20 - bit depth
-    uint8_t reserved[3];
+    uint8_t prepped;  // 1: read the prepass output (ResidualArgs::prep), not the levels
+    uint8_t lfnst;    // tiles marked kTileLfnst: LFNST matrix (set * 2 + idx - 1) | transposed << 3
+    uint8_t reserved;
 };
 // Recon target of a tile-eligible TU (recon plans only).
 struct TileDst {
@@ -55,6 +58,15 @@ struct ResidualArgs {
     const rmh_recon_frame* frames;
     const TileTu* tile_tus;            // tiled kernel: per TU of order[0 .. n_fast)
     const TileDst* tile_dst;           // tiled kernel, recon mode
+    // Prepass of TUs with LFNST or dependent quantisation that the tiled
+    // kernel then transforms: residual_kernel with prep_out set writes each
+    // TU's dequantised (and LFNST-ed) coefficients, zero outside the coded
+    // region, as a raster w x h block at prep_out + prep_off[k] (k = the TU's
+    // position in its launch) and stops before the transform; the tiled
+    // kernel reads them from `prep`.
+    int16_t* prep_out;
+    const uint32_t* prep_off;
+    const int16_t* prep;
     uint32_t first_tu;
     uint32_t n_tu;
     int32_t mat_off[3][7];
@@ -69,14 +81,17 @@ hipError_t launch_residual(const ResidualArgs& a, hipStream_t stream);
 struct TileItem {
     uint32_t first;
     uint16_t count;
-    uint8_t lw;
+    uint8_t lw;  // | kTileLfnst: every TU of the tile applies the inverse LFNST in the kernel
     uint8_t lh;
 };
+constexpr uint8_t kTileLfnst = 0x80;
 
-// Tiled residual kernel for the common case (2-D, both sides >= 4, no LFNST,
-// no dependent quantisation, no transform skip). `a.order` is required.
-hipError_t launch_residual_tiles(const ResidualArgs& a, const TileItem* items, uint32_t n_items, uint32_t lds_bytes,
-                                 hipStream_t stream);
+// Tiled residual kernel for 2-D TUs with both sides >= 4 and no transform
+// skip: plain TUs from their levels, TUs with LFNST or dependent
+// quantisation from the prepass output. `a.order` is required.
+// items[0 .. n_plain) are plain tiles, items[n_plain .. n_items) LFNST tiles.
+hipError_t launch_residual_tiles(const ResidualArgs& a, const TileItem* items, uint32_t n_items, uint32_t n_plain,
+                                 uint32_t lds_bytes, hipStream_t stream);
 
 // TUs per full tile of class (lw, lh) and the LDS bytes such a tile needs
 // (the kernel lays LDS out for a full tile whatever the tile's count).
@@ -96,7 +111,7 @@ inline uint32_t residual_tile_lds(int lw, int lh) {
     int lwc = lw < 5 ? lw : 5, lhc = lh < 5 ? lh : 5;
     uint32_t c = count * (1u << lwc) * residual_tile_pitch(lhc) * 4;  // coefficient columns
     uint32_t t = count * (1u << lh) * residual_tile_pitch(lwc) * 4;  // intermediate rows
-    return c + t + count * 40 + 16;                                  // + per-TU parameters + tile bounding box
+    return c + t + count * 48 + 16;                                  // + per-TU parameters + tile bounding box
 }
 
 // One inter work item: a 16x16 luma tile (and its 8x8 chroma tiles) of one

@@ -150,6 +150,10 @@ __device__ __forceinline__ void recon_store(const ReconDst& d, int x, int y, int
         gdst<uint16_t>(d.base)[o] = (uint16_t)v;
 }
 
+// PREP: the prepass (a.prep_out set): stops after dequantisation and LFNST
+// and writes the coded region for the tiled kernel. An instantiation of its
+// own, so the full path keeps its register budget.
+template <bool PREP>
 __global__ __launch_bounds__(64) void residual_kernel(ResidualArgs a) {
     __shared__ __align__(16) uint32_t ct[32 * kPitch];  // coefficients: column i, int16 pairs along row k
     __shared__ __align__(16) uint32_t tt[64 * kPitch];  // intermediates: row y, int16 pairs along column i
@@ -370,6 +374,15 @@ __global__ __launch_bounds__(64) void residual_kernel(ResidualArgs a) {
         }
         __syncthreads();
     }
+    if constexpr (PREP) {
+        // prepass: the dequantised (and LFNST-ed) coded region, for the tiled kernel
+        int16_t* po = a.prep_out + a.prep_off[blockIdx.x];
+        for (int e = lane; e < cw * ch; e += 64) {
+            const int y = e / cw, x = e - y * cw;
+            po[y * w + x] = ct16[x * 2 * kPitch + y];
+        }
+        return;
+    }
     const int er = __builtin_amdgcn_readfirstlane(wave_max(my_r)) + 1;  // effective rows
     const int ec = __builtin_amdgcn_readfirstlane(wave_max(my_c)) + 1;  // effective columns
     if (er <= 0) {
@@ -459,8 +472,9 @@ __global__ __launch_bounds__(64) void residual_kernel(ResidualArgs a) {
 // ---------------------------------------------------------------------------
 
 struct TuParams {
-    uint32_t off;    // first coefficient (offset + TU size <= n_coeff < 2^32)
-    int32_t cwch;    // cw | ch << 8
+    uint32_t off;    // first residual sample (offset + TU size <= n_coeff < 2^32)
+    uint32_t src;    // first level: in a.coeff, or in a.prep when cwch bit 16 is set
+    int32_t cwch;    // cw | ch << 8 | prepped << 16 | LFNST code (matrix | transposed << 3) << 17
     int32_t scale;
     int32_t shifts;  // dequant left shift | right shift << 8 | second-stage shift (20 - bd) << 16
     int32_t mh;      // dword offset of Mh pairs
@@ -469,6 +483,7 @@ struct TuParams {
     int32_t bpsmax;  // recon: bytes per sample | (2^bd - 1) << 8
     char* base;      // recon: sample (0, 0) of the block
 };
+static_assert(sizeof(TuParams) == 48, "residual_tile_lds reserves 48 bytes per TU");
 
 // dequantize_rect_bd for one TU of the tiled kernel: with ls = 0, S = scale;
 // with 0 < ls <= 9, S = scale << ls (|level * S| <= 2^15 * 102 * 2^9 < 2^31,
@@ -600,7 +615,7 @@ __device__ __forceinline__ void load_colp(const uint32_t* __restrict__ src, uint
 // kernel at 6 waves per SIMD, where 16 would need 128 VGPRs.
 constexpr int kReconBatch = 8;
 
-template <int LW, int LH, bool RECON>
+template <int LW, int LH, bool RECON, bool LFNST>
 __device__ __forceinline__ void tile_body(const ResidualArgs& a, const TileItem& it, uint32_t* smem) {
     constexpr int W = 1 << LW, H = 1 << LH;
     constexpr int LWC = LW < 5 ? LW : 5, LHC = LH < 5 ? LH : 5;
@@ -625,7 +640,8 @@ __device__ __forceinline__ void tile_body(const ResidualArgs& a, const TileItem&
         const TileTu d = a.tile_tus[it.first + g];
         TuParams p;
         p.off = d.off;
-        p.cwch = d.cw | (d.ch << 8);
+        p.src = d.src;
+        p.cwch = d.cw | (d.ch << 8) | (d.prepped << 16) | (d.lfnst << 17);
         // dequantisation as level * S + rnd >> rs (see dequant_tile)
         const int ls = d.shifts & 15;
         p.scale = ls >= 10 ? 65536 : (int)d.scale << ls;
@@ -670,12 +686,13 @@ __device__ __forceinline__ void tile_body(const ResidualArgs& a, const TileItem&
             raw[i][0] = raw[i][1] = 0;
             if (e < total) {
                 const int x0 = (e % Q) * 4, rp = (e / Q) % RP, g = e / (Q * RP);
-                const uint32_t off = P[g].off;
-                const int cwch = P[g].cwch, cw = cwch & 255, ch = cwch >> 8, r0 = 2 * rp;
+                const uint32_t off = P[g].src;
+                const int cwch = P[g].cwch, cw = cwch & 255, ch = (cwch >> 8) & 255, r0 = 2 * rp;
+                const int16_t* __restrict__ src = ((cwch >> 16) & 1) ? a.prep : a.coeff;
                 if (x0 < cw) {
-                    if (r0 < ch) raw[i][0] = *reinterpret_cast<const uint64_t*>(a.coeff + (off + (uint32_t)(r0 * W + x0)));
+                    if (r0 < ch) raw[i][0] = *reinterpret_cast<const uint64_t*>(src + (off + (uint32_t)(r0 * W + x0)));
                     if (r0 + 1 < ch)
-                        raw[i][1] = *reinterpret_cast<const uint64_t*>(a.coeff + (off + (uint32_t)((r0 + 1) * W + x0)));
+                        raw[i][1] = *reinterpret_cast<const uint64_t*>(src + (off + (uint32_t)((r0 + 1) * W + x0)));
                 }
             }
         }
@@ -753,6 +770,68 @@ __device__ __forceinline__ void tile_body(const ResidualArgs& a, const TileItem&
         }
     }
     __syncthreads();
+    if constexpr (LFNST) {
+        // inverse_lfnst_rect (vvc/lfnst.rs:219-297) on the dequantised block in
+        // LDS: the first NZ positions of the 4x4 diagonal scan times the
+        // kernel matrix give TR outputs, (sum + 64) >> 7 clipped to int16,
+        // placed (transposed or not) in the top-left 4x4 / 8x8 region; the
+        // coded region is 4..8 on a side, so it holds the whole input and
+        // clips the output. One item per (TU, output); the outputs are
+        // staged in the intermediate-row area (unused until phase B, and
+        // large enough for GMAX * TR values) before they overwrite the inputs.
+        constexpr int SB = (W >= 8 && H >= 8) ? 8 : 4;
+        constexpr int TR = SB == 8 ? 48 : 16;
+        constexpr int NZ = ((W == 4 && H == 4) || (W == 8 && H == 8)) ? 8 : 16;
+        static_assert(H * residual_tile_pitch(LWC) * 2 >= TR, "LFNST outputs fit the intermediate rows");
+        constexpr uint32_t kScan4Lo = 0x9c258140u, kScan4Hi = 0xfbe7ad36u;  // y * 4 + x of the 4x4 diagonal scan, 4 bits each
+        const int8_t* __restrict__ lt = SB == 8 ? a.lfnst8 : a.lfnst4;
+        int16_t* stage = reinterpret_cast<int16_t*>(T);
+#pragma unroll 1
+        for (int q = tid; q < G * TR; q += kTileThreads) {
+            const int g = q / TR, j = q - g * TR;
+            const int code = (P[g].cwch >> 17) & 15;
+            const int8_t* __restrict__ m = lt + (code & 7) * 16 * TR + j;
+            const int16_t* c16 = reinterpret_cast<const int16_t*>(C + g * Wc * CP);
+            int sum = 0;
+#pragma unroll
+            for (int i = 0; i < NZ; i++) {
+                const int pos = ((i < 8 ? kScan4Lo >> (4 * i) : kScan4Hi >> (4 * (i - 8)))) & 15;
+                sum += (int)c16[(pos & 3) * 2 * CP + (pos >> 2)] * (int)m[i * TR];
+            }
+            stage[q] = (int16_t)clip16i((sum + 64) >> 7);
+        }
+        __syncthreads();
+#pragma unroll 1
+        for (int q = tid; q < G * TR; q += kTileThreads) {
+            const int g = q / TR, j = q - g * TR;
+            const int cwch = P[g].cwch, cw = cwch & 255, ch = (cwch >> 8) & 255;
+            int ox, oy;
+            if (SB == 4) {
+                ox = j & 3;
+                oy = j >> 2;
+            } else if (j < 32) {
+                ox = j & 7;
+                oy = j >> 3;
+            } else {
+                ox = (j - 32) & 3;
+                oy = 4 + ((j - 32) >> 2);
+            }
+            if ((cwch >> 20) & 1) {  // transposed
+                const int t = ox;
+                ox = oy;
+                oy = t;
+            }
+            if (ox < cw && oy < ch) reinterpret_cast<int16_t*>(C + g * Wc * CP)[ox * 2 * CP + oy] = stage[q];
+        }
+        // the outputs reach the whole coded region of every TU (rows and
+        // columns 0 .. 3 or 0 .. 7): cover it in the bounding box
+        if (tid < G) {
+            const int cwch = P[tid].cwch;
+            atomicMax(&TB[0], ((cwch >> 8) & 255) - 1);
+            atomicMax(&TB[1], (cwch & 255) - 1);
+        }
+        __syncthreads();
+    }
     const int er = __builtin_amdgcn_readfirstlane(TB[0]) + 1;
     const int ec = __builtin_amdgcn_readfirstlane(TB[1]) + 1;
     const int k1 = (er + 1) >> 1, k2 = (ec + 1) >> 1;
@@ -910,15 +989,16 @@ __device__ __forceinline__ void tile_body(const ResidualArgs& a, const TileItem&
 }
 
 // RECON: add into the picture planes (a.place / a.frames) instead of writing
-// int16 residuals.
-template <bool RECON>
+// int16 residuals. LFNST: tiles of TUs that apply the inverse LFNST (an
+// instantiation of its own, so the plain tiles keep their register budget).
+template <bool RECON, bool LFNST>
 __global__ __launch_bounds__(kTileThreads) void residual_tile_kernel(ResidualArgs a, const TileItem* __restrict__ items) {
     extern __shared__ __align__(16) uint32_t smem[];
     const TileItem it = items[blockIdx.x];
-    switch (it.lw * 8 + it.lh) {
+    switch ((it.lw & 7) * 8 + it.lh) {
 #define RMH_TILE(LW, LH) \
     case LW * 8 + LH:    \
-        tile_body<LW, LH, RECON>(a, it, smem); \
+        tile_body<LW, LH, RECON, LFNST>(a, it, smem); \
         break;
         RMH_TILE(2, 2) RMH_TILE(2, 3) RMH_TILE(2, 4) RMH_TILE(2, 5) RMH_TILE(2, 6)
         RMH_TILE(3, 2) RMH_TILE(3, 3) RMH_TILE(3, 4) RMH_TILE(3, 5) RMH_TILE(3, 6)
@@ -935,17 +1015,31 @@ __global__ __launch_bounds__(kTileThreads) void residual_tile_kernel(ResidualArg
 
 hipError_t launch_residual(const ResidualArgs& a, hipStream_t stream) {
     if (a.n_tu == 0) return hipSuccess;
-    hipLaunchKernelGGL(residual_kernel, dim3(a.n_tu), dim3(64), 0, stream, a);
+    if (a.prep_out)
+        hipLaunchKernelGGL(residual_kernel<true>, dim3(a.n_tu), dim3(64), 0, stream, a);
+    else
+        hipLaunchKernelGGL(residual_kernel<false>, dim3(a.n_tu), dim3(64), 0, stream, a);
     return hipGetLastError();
 }
 
-hipError_t launch_residual_tiles(const ResidualArgs& a, const TileItem* items, uint32_t n_items, uint32_t lds_bytes,
-                                 hipStream_t stream) {
-    if (n_items == 0) return hipSuccess;
-    if (a.place)
-        hipLaunchKernelGGL(residual_tile_kernel<true>, dim3(n_items), dim3(kTileThreads), lds_bytes, stream, a, items);
-    else
-        hipLaunchKernelGGL(residual_tile_kernel<false>, dim3(n_items), dim3(kTileThreads), lds_bytes, stream, a, items);
+hipError_t launch_residual_tiles(const ResidualArgs& a, const TileItem* items, uint32_t n_items, uint32_t n_plain,
+                                 uint32_t lds_bytes, hipStream_t stream) {
+    if (n_plain > 0) {
+        if (a.place)
+            hipLaunchKernelGGL((residual_tile_kernel<true, false>), dim3(n_plain), dim3(kTileThreads), lds_bytes, stream, a,
+                               items);
+        else
+            hipLaunchKernelGGL((residual_tile_kernel<false, false>), dim3(n_plain), dim3(kTileThreads), lds_bytes, stream,
+                               a, items);
+    }
+    if (n_items > n_plain) {
+        if (a.place)
+            hipLaunchKernelGGL((residual_tile_kernel<true, true>), dim3(n_items - n_plain), dim3(kTileThreads), lds_bytes,
+                               stream, a, items + n_plain);
+        else
+            hipLaunchKernelGGL((residual_tile_kernel<false, true>), dim3(n_items - n_plain), dim3(kTileThreads), lds_bytes,
+                               stream, a, items + n_plain);
+    }
     return hipGetLastError();
 }
 

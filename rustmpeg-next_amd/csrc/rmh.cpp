@@ -165,16 +165,23 @@ struct rmh_plan {
     const uint32_t* d_order = nullptr;       // fast TUs grouped by size class, then the others
     rmh::TileItem* d_tiles = nullptr;  // tiles of the fast TUs
     size_t n_tiles = 0, n_fast = 0, tile_capacity = 0;
+    uint32_t n_tiles_plain = 0;  // tiles [0, n_tiles_plain) are plain, the rest LFNST tiles
     uint32_t tile_lds = 0;
     // fast TUs of size class (log2 w, log2 h) are order[cls_first .. + cls_count);
     // tiles of the class hold residual_tile_tus >> split TUs each (autotuned)
-    uint32_t cls_first[7][7] = {}, cls_count[7][7] = {};
+    uint32_t cls_first[7][7] = {}, cls_count[7][7] = {}, cls_lfnst[7][7] = {};
     uint8_t split[7][7] = {};
     // recon (residual added into picture planes); null for residual output
     const rmh_tu_place* d_place = nullptr;
     const rmh_recon_frame* d_rframes = nullptr;
     const rmh::TileTu* d_tile_tus = nullptr;   // per fast TU, tile order
     const rmh::TileDst* d_tile_dst = nullptr;  // recon plans
+    // prepass of the fast TUs with LFNST or dependent quantisation: their
+    // indices are order[n_tu .. n_tu + n_prep), their dequantised blocks go to
+    // d_prep at d_prep_off[k]
+    int16_t* d_prep = nullptr;
+    const uint32_t* d_prep_off = nullptr;
+    size_t n_prep = 0;
     // inter
     const rmh_mc_frame* d_frames = nullptr;
     const rmh_pu_desc* d_pus = nullptr;
@@ -913,72 +920,155 @@ TileTable tile_table(const rmh_dbk_frame* frames, size_t n_frames) {
 }
 
 
-// TUs the tiled kernel handles: 2-D with both sides >= 4, no LFNST, no
-// dependent quantisation, no transform skip, 8-byte aligned block.
+// TUs the tiled kernel transforms: 2-D with both sides >= 4, no transform
+// skip, 8-byte aligned block. Those with LFNST or dependent quantisation are
+// dequantised (and LFNST-ed) first by the prepass, one workgroup per TU, and
+// the tiled kernel reads its output.
 bool tile_eligible(const rmh_tu_desc& t) {
-    return t.log2_w >= 2 && t.log2_h >= 2 && t.lfnst_idx == 0 && !(t.flags & (RMH_TU_TRANSFORM_SKIP | RMH_TU_DEP_QUANT)) &&
-           (t.offset & 3) == 0;
+    return t.log2_w >= 2 && t.log2_h >= 2 && !(t.flags & RMH_TU_TRANSFORM_SKIP) && (t.offset & 3) == 0;
 }
+bool needs_prepass(const rmh_tu_desc& t);
+
+// LFNST TUs without dependent quantisation whose coded region covers the
+// 4x4 LFNST input: the tiled kernel applies the inverse LFNST itself, in LDS,
+// in tiles of their own.
+bool tile_lfnst(const rmh_tu_desc& t);
+
+bool needs_prepass(const rmh_tu_desc& t) {
+    return (t.lfnst_idx != 0 || (t.flags & RMH_TU_DEP_QUANT)) && !tile_lfnst(t);
+}
+
+// The LFNST kernel matrix (set * 2 + lfnst_idx - 1, set from the intra mode
+// as lfnst_set, vvc/lfnst.rs:85-129) in bits 0-2 and the transposition flag
+// in bit 3, as the tiled kernel reads them.
+uint8_t lfnst_code(const rmh_tu_desc& d) {
+    const int m = d.lfnst_mode;
+    const int set = m <= 1 ? 0 : m <= 12 ? 1 : m <= 23 ? 2 : m <= 44 ? 3 : m <= 55 ? 2 : 1;
+    const bool tp = (m >= 35 && m < 67) || m >= 81;
+    return (uint8_t)((set * 2 + (d.lfnst_idx - 1)) | (tp ? 8 : 0));
+}
+
+// Coded region of a TU after the default, MTS, caller and LFNST zero-outs
+// (vvc/dct.rs:1063-1077, decode_general.rs:4191-4236), as the generic kernel
+// derives it.
+void coded_region(const rmh_tu_desc& d, int* cw, int* ch) {
+    const int W = 1 << d.log2_w, H = 1 << d.log2_h;
+    int sw = (d.tr_hor != RMH_TR_DCT2 && W == 32) ? 16 : (W > 32 ? W - 32 : 0);
+    int sh = (d.tr_ver != RMH_TR_DCT2 && H == 32) ? 16 : (H > 32 ? H - 32 : 0);
+    sw = std::max<int>(sw, d.skip_w);
+    sh = std::max<int>(sh, d.skip_h);
+    if (d.lfnst_idx) {
+        const int nz = (W == 4 || H == 4) ? 4 : 8;
+        sw = std::max(sw, W - std::min(nz, W));
+        sh = std::max(sh, H - std::min(nz, H));
+    }
+    *cw = W - sw;
+    *ch = H - sh;
+}
+
+bool tile_lfnst(const rmh_tu_desc& t) {
+    if (t.lfnst_idx == 0 || (t.flags & RMH_TU_DEP_QUANT)) return false;
+    int cw = 0, ch = 0;
+    coded_region(t, &cw, &ch);
+    return cw >= 4 && ch >= 4;
+}
+
+// The prepass list of a residual batch: TU indices (in the order of `order`
+// after n_tu) and each one's element offset in the prepass buffer.
+struct Prepass {
+    std::vector<uint32_t> off;      // per prepass TU
+    std::vector<uint32_t> off_of;   // per TU index: its prepass offset, or UINT32_MAX
+    size_t elements = 0;            // prepass buffer size, int16 elements
+};
 
 // Largest autotuned split: a class's tiles hold residual_tile_tus >> split TUs.
 constexpr int kMaxTileSplit = 3;
 
 // Tiles of the fast TUs (largest class first) from the class ranges of
-// `order`, each class split as `split` says.
-void tiles_from_classes(const uint32_t (*first)[7], const uint32_t (*count)[7], const uint8_t (*split)[7],
-                        std::vector<rmh::TileItem>& tiles, uint32_t* lds) {
+// `order`, each class split as `split` says. The last lfnst[lw][lh] TUs of a
+// class range apply LFNST in the kernel: they get tiles of their own, marked
+// with rmh::kTileLfnst.
+// Plain tiles come first (their number is returned), then the LFNST tiles:
+// each kind is one launch.
+uint32_t tiles_from_classes(const uint32_t (*first)[7], const uint32_t (*count)[7], const uint32_t (*lfnst)[7],
+                            const uint8_t (*split)[7], std::vector<rmh::TileItem>& tiles, uint32_t* lds) {
     tiles.clear();
     *lds = 0;
-    for (int area = 12; area >= 4; area--)
-        for (int lw = 2; lw <= 6; lw++) {
-            const int lh = area - lw;
-            if (lh < 2 || lh > 6 || count[lw][lh] == 0) continue;
-            const int per = std::max(1, rmh::residual_tile_tus(lw, lh) >> split[lw][lh]);
-            for (uint32_t f = 0; f < count[lw][lh]; f += per) {
-                const uint32_t cnt = std::min<uint32_t>(per, count[lw][lh] - f);
-                tiles.push_back({first[lw][lh] + f, (uint16_t)cnt, (uint8_t)lw, (uint8_t)lh});
+    uint32_t n_plain = 0;
+    for (int part = 0; part < 2; part++) {
+        for (int area = 12; area >= 4; area--)
+            for (int lw = 2; lw <= 6; lw++) {
+                const int lh = area - lw;
+                if (lh < 2 || lh > 6 || count[lw][lh] == 0) continue;
+                const int per = std::max(1, rmh::residual_tile_tus(lw, lh) >> split[lw][lh]);
+                const uint32_t plain = count[lw][lh] - lfnst[lw][lh];
+                const uint32_t a = part ? plain : 0, b = part ? count[lw][lh] : plain;
+                for (uint32_t f = a; f < b; f += per) {
+                    const uint32_t cnt = std::min<uint32_t>(per, b - f);
+                    tiles.push_back({first[lw][lh] + f, (uint16_t)cnt, (uint8_t)(lw | (part ? rmh::kTileLfnst : 0)),
+                                     (uint8_t)lh});
+                }
+                *lds = std::max(*lds, rmh::residual_tile_lds(lw, lh));
             }
-            *lds = std::max(*lds, rmh::residual_tile_lds(lw, lh));
-        }
+        if (part == 0) n_plain = (uint32_t)tiles.size();
+    }
+    return n_plain;
 }
 
-// Groups tile-eligible TUs by size class (largest class first) and appends the
-// remaining TUs; fills the class ranges and the unsplit tile list.
-void build_residual_tiles(const rmh_tu_desc* tus, size_t n_tu, std::vector<uint32_t>& order,
+// Groups tile-eligible TUs by size class (largest class first), appends the
+// remaining TUs and then the prepass TUs (so order holds n_tu + prepass
+// entries); fills the class ranges, the unsplit tile list and the prepass
+// offsets.
+uint32_t build_residual_tiles(const rmh_tu_desc* tus, size_t n_tu, std::vector<uint32_t>& order,
                           std::vector<rmh::TileItem>& tiles, size_t* n_fast, uint32_t* lds,
-                          uint32_t (*first)[7], uint32_t (*count)[7]) {
-    std::vector<uint32_t> cls[7][7];
+                          uint32_t (*first)[7], uint32_t (*count)[7], uint32_t (*lfnst)[7], Prepass* prep) {
+    std::vector<uint32_t> cls[7][7], cls_l[7][7];
     std::vector<uint32_t> rest;
+    prep->off.clear();
+    prep->off_of.assign(n_tu, UINT32_MAX);
+    prep->elements = 0;
+    std::vector<uint32_t> prep_list;
     for (size_t i = 0; i < n_tu; i++) {
-        if (tile_eligible(tus[i]))
-            cls[tus[i].log2_w][tus[i].log2_h].push_back((uint32_t)i);
-        else
+        if (tile_eligible(tus[i])) {
+            (tile_lfnst(tus[i]) ? cls_l : cls)[tus[i].log2_w][tus[i].log2_h].push_back((uint32_t)i);
+            if (needs_prepass(tus[i])) {
+                prep_list.push_back((uint32_t)i);
+                prep->off.push_back((uint32_t)prep->elements);
+                prep->off_of[i] = (uint32_t)prep->elements;
+                prep->elements += (size_t)1 << (tus[i].log2_w + tus[i].log2_h);  // a multiple of 16
+            }
+        } else {
             rest.push_back((uint32_t)i);
+        }
     }
     order.clear();
     for (int lw = 0; lw < 7; lw++)
-        for (int lh = 0; lh < 7; lh++) first[lw][lh] = count[lw][lh] = 0;
+        for (int lh = 0; lh < 7; lh++) first[lw][lh] = count[lw][lh] = lfnst[lw][lh] = 0;
     for (int area = 12; area >= 4; area--)
         for (int lw = 2; lw <= 6; lw++) {
             int lh = area - lw;
             if (lh < 2 || lh > 6) continue;
             first[lw][lh] = (uint32_t)order.size();
-            count[lw][lh] = (uint32_t)cls[lw][lh].size();
+            count[lw][lh] = (uint32_t)(cls[lw][lh].size() + cls_l[lw][lh].size());
+            lfnst[lw][lh] = (uint32_t)cls_l[lw][lh].size();
             order.insert(order.end(), cls[lw][lh].begin(), cls[lw][lh].end());
+            order.insert(order.end(), cls_l[lw][lh].begin(), cls_l[lw][lh].end());
         }
     *n_fast = order.size();
     order.insert(order.end(), rest.begin(), rest.end());
+    order.insert(order.end(), prep_list.begin(), prep_list.end());
     const uint8_t none[7][7] = {};
-    tiles_from_classes(first, count, none, tiles, lds);
+    return tiles_from_classes(first, count, lfnst, none, tiles, lds);
 }
 
-// Tiles a plan can need at the largest split.
+// Tiles a plan can need at the largest split (each class's LFNST TUs start
+// a tile of their own).
 size_t tile_capacity(const uint32_t (*count)[7]) {
     size_t n = 0;
     for (int lw = 2; lw <= 6; lw++)
         for (int lh = 2; lh <= 6; lh++) {
             const size_t per = std::max(1, rmh::residual_tile_tus(lw, lh) >> kMaxTileSplit);
-            n += (count[lw][lh] + per - 1) / per;
+            n += (count[lw][lh] + per - 1) / per + 1;
         }
     return n;
 }
@@ -987,27 +1077,34 @@ size_t tile_capacity(const uint32_t (*count)[7]) {
 // for sqrt(2)-scaled (odd log2 w + log2 h) blocks.
 const uint8_t kInvQuantScale[2][6] = {{40, 45, 51, 57, 64, 72}, {57, 64, 72, 80, 90, 102}};
 
-// Tiled-kernel parameters of order[0 .. n_fast) (the fast TUs).
+// Tiled-kernel parameters of order[0 .. n_fast) (the fast TUs); prepass TUs
+// read their prepass block with unit scale.
 std::vector<rmh::TileTu> build_tile_tus(const rmh::DeviceTables& tabs, const rmh_tu_desc* tus,
-                                        const std::vector<uint32_t>& order, size_t n_fast) {
+                                        const std::vector<uint32_t>& order, size_t n_fast, const Prepass& prep) {
     std::vector<rmh::TileTu> out(n_fast);
     for (size_t i = 0; i < n_fast; i++) {
         const rmh_tu_desc& d = tus[order[i]];
-        const int lw = d.log2_w, lh = d.log2_h, W = 1 << lw, H = 1 << lh;
-        int sw = (d.tr_hor != RMH_TR_DCT2 && W == 32) ? 16 : (W > 32 ? W - 32 : 0);
-        int sh = (d.tr_ver != RMH_TR_DCT2 && H == 32) ? 16 : (H > 32 ? H - 32 : 0);
-        sw = std::max<int>(sw, d.skip_w);
-        sh = std::max<int>(sh, d.skip_h);
+        const int lw = d.log2_w, lh = d.log2_h;
+        int cw = 0, ch = 0;
+        coded_region(d, &cw, &ch);
         const int qp = d.qp < 4 ? 4 : d.qp;
         const int lsum = lw + lh, sqrt2 = lsum & 1;
         const int tr_shift = 15 - d.bit_depth - (lsum >> 1) - sqrt2;
         const int rshift = 6 - (tr_shift + qp / 6);
         rmh::TileTu t{};
         t.off = d.offset;
-        t.cw = (uint8_t)(W - sw);
-        t.ch = (uint8_t)(H - sh);
+        t.src = d.offset;
+        t.cw = (uint8_t)cw;
+        t.ch = (uint8_t)ch;
         t.scale = kInvQuantScale[sqrt2][qp % 6];
         t.shifts = (uint8_t)((rshift < 0 ? -rshift : 0) | ((rshift > 0 ? rshift : 0) << 4));
+        if (tile_lfnst(d)) t.lfnst = lfnst_code(d);
+        if (prep.off_of[order[i]] != UINT32_MAX) {
+            t.src = prep.off_of[order[i]];
+            t.scale = 1;
+            t.shifts = 0;
+            t.prepped = 1;
+        }
         t.mh = (uint16_t)tabs.col16_off[d.tr_hor][lw];
         t.mv = (uint16_t)tabs.col16_off[d.tr_ver][lh];
         t.shift2 = (uint8_t)(20 - d.bit_depth);
@@ -1057,9 +1154,21 @@ rmh_status run_plan_locked_free(rmh_plan* plan, const int16_t* d_coeff, int16_t*
         const bool whole = first == 0 && count == plan->n_tu && plan->d_order;
         const bool aligned8 = ((uintptr_t)d_coeff & 7) == 0 && (plan->d_place || ((uintptr_t)d_residual & 7) == 0);
         if (whole && aligned8) {
-            // the tiled kernel for the common TUs, the general kernel for the rest
+            // the prepass, the tiled kernel for the 2-D TUs, the general kernel for the rest
             a.order = plan->d_order;
-            e = rmh::launch_residual_tiles(a, plan->d_tiles, (uint32_t)plan->n_tiles, plan->tile_lds, s);
+            if (plan->n_prep) {
+                a.prep_out = plan->d_prep;
+                a.prep_off = plan->d_prep_off;
+                a.first_tu = (uint32_t)plan->n_tu;
+                a.n_tu = (uint32_t)plan->n_prep;
+                e = rmh::launch_residual(a, s);
+                a.prep_out = nullptr;
+                a.prep_off = nullptr;
+            }
+            a.prep = plan->d_prep;
+            if (e == hipSuccess)
+                e = rmh::launch_residual_tiles(a, plan->d_tiles, (uint32_t)plan->n_tiles, plan->n_tiles_plain,
+                                               plan->tile_lds, s);
             if (e == hipSuccess && plan->n_tu > plan->n_fast) {
                 a.first_tu = (uint32_t)plan->n_fast;
                 a.n_tu = (uint32_t)(plan->n_tu - plan->n_fast);
@@ -1217,16 +1326,20 @@ rmh_status create_residual_plan(rmh_ctx* ctx, const rmh_tu_desc* tus, size_t n_t
     if (n_tu) {
         std::vector<uint32_t> order;
         std::vector<rmh::TileItem> tiles;
-        build_residual_tiles(tus, n_tu, order, tiles, &p->n_fast, &p->tile_lds, p->cls_first, p->cls_count);
+        Prepass prep;
+        p->n_tiles_plain = build_residual_tiles(tus, n_tu, order, tiles, &p->n_fast, &p->tile_lds, p->cls_first,
+                                                p->cls_count, p->cls_lfnst, &prep);
+        p->n_prep = prep.off.size();
         p->tile_capacity = std::max(tiles.size(), tile_capacity(p->cls_count));
         size_t tb = aligned(n_tu * sizeof(rmh_tu_desc)), ob = aligned(order.size() * 4),
                ib = aligned(p->tile_capacity * sizeof(rmh::TileItem));
         size_t pb = recon ? aligned(n_tu * sizeof(rmh_tu_place)) : 0, fb = recon ? aligned(n_frames * sizeof(rmh_recon_frame)) : 0;
-        const std::vector<rmh::TileTu> ttu = build_tile_tus(ctx->tabs, tus, order, p->n_fast);
+        const std::vector<rmh::TileTu> ttu = build_tile_tus(ctx->tabs, tus, order, p->n_fast, prep);
         std::vector<rmh::TileDst> tdst;
         if (recon) tdst = build_tile_dst(order, p->n_fast, place, frames);
         size_t ub = aligned(ttu.size() * sizeof(rmh::TileTu)), db = aligned(tdst.size() * sizeof(rmh::TileDst));
-        hipError_t e = hipMalloc(&p->d_mem, tb + ob + ib + pb + fb + ub + db + 256);
+        size_t qb = aligned(prep.off.size() * 4), xb = aligned(prep.elements * sizeof(int16_t));
+        hipError_t e = hipMalloc(&p->d_mem, tb + ob + ib + pb + fb + ub + db + qb + xb + 256);
         char* base = (char*)p->d_mem;
         if (e == hipSuccess) e = hipMemcpy(base, tus, n_tu * sizeof(rmh_tu_desc), hipMemcpyHostToDevice);
         if (e == hipSuccess) e = hipMemcpy(base + tb, order.data(), order.size() * 4, hipMemcpyHostToDevice);
@@ -1239,6 +1352,8 @@ rmh_status create_residual_plan(rmh_ctx* ctx, const rmh_tu_desc* tus, size_t n_t
         if (e == hipSuccess && !ttu.empty()) e = hipMemcpy(ubase, ttu.data(), ttu.size() * sizeof(rmh::TileTu), hipMemcpyHostToDevice);
         if (e == hipSuccess && !tdst.empty())
             e = hipMemcpy(ubase + ub, tdst.data(), tdst.size() * sizeof(rmh::TileDst), hipMemcpyHostToDevice);
+        if (e == hipSuccess && !prep.off.empty())
+            e = hipMemcpy(ubase + ub + db, prep.off.data(), prep.off.size() * 4, hipMemcpyHostToDevice);
         if (e != hipSuccess) {
             if (p->d_mem) hipFree(p->d_mem);
             delete p;
@@ -1254,6 +1369,10 @@ rmh_status create_residual_plan(rmh_ctx* ctx, const rmh_tu_desc* tus, size_t n_t
             p->d_tile_dst = (const rmh::TileDst*)(ubase + ub);
         }
         p->d_tile_tus = (const rmh::TileTu*)ubase;
+        if (p->n_prep) {
+            p->d_prep_off = (const uint32_t*)(ubase + ub + db);
+            p->d_prep = (int16_t*)(ubase + ub + db + qb);
+        }
     }
     *out = p;
     return RMH_OK;
@@ -1325,8 +1444,20 @@ RMH_EXPORT rmh_status rmh_residual_plan_autotune(rmh_plan* plan, const int16_t* 
     a.tile_tus = plan->d_tile_tus;
     a.tile_dst = plan->d_tile_dst;
     a.order = plan->d_order;
+    hipError_t e = hipSuccess;
+    if (plan->n_prep) {
+        // the prepass output the tiled launches read
+        a.prep_out = plan->d_prep;
+        a.prep_off = plan->d_prep_off;
+        a.first_tu = (uint32_t)plan->n_tu;
+        a.n_tu = (uint32_t)plan->n_prep;
+        e = rmh::launch_residual(a, s);
+        a.prep_out = nullptr;
+        a.prep_off = nullptr;
+    }
+    a.prep = plan->d_prep;
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    hipError_t e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
     if (e == hipSuccess) e = hipEventCreate(&e1);
     // coordinate descent over the classes (largest first): each candidate split
     // of one class is timed on the whole plan's launch as the fastest of three
@@ -1335,15 +1466,15 @@ RMH_EXPORT rmh_status rmh_residual_plan_autotune(rmh_plan* plan, const int16_t* 
     auto time_plan = [&](const uint8_t (*split)[7], float* ms) -> hipError_t {
         std::vector<rmh::TileItem> tiles;
         uint32_t lds = 0;
-        tiles_from_classes(plan->cls_first, plan->cls_count, split, tiles, &lds);
+        const uint32_t n_plain = tiles_from_classes(plan->cls_first, plan->cls_count, plan->cls_lfnst, split, tiles, &lds);
         hipError_t r = hipMemcpy(plan->d_tiles, tiles.data(), tiles.size() * sizeof(rmh::TileItem), hipMemcpyHostToDevice);
         for (uint32_t k = 0; k < 2 && r == hipSuccess; k++)
-            r = rmh::launch_residual_tiles(a, plan->d_tiles, (uint32_t)tiles.size(), lds, s);
+            r = rmh::launch_residual_tiles(a, plan->d_tiles, (uint32_t)tiles.size(), n_plain, lds, s);
         float fastest = 0.0f;
         for (int batch = 0; batch < 3 && r == hipSuccess; batch++) {
             r = hipEventRecord(e0, s);
             for (uint32_t k = 0; k < reps && r == hipSuccess; k++)
-                r = rmh::launch_residual_tiles(a, plan->d_tiles, (uint32_t)tiles.size(), lds, s);
+                r = rmh::launch_residual_tiles(a, plan->d_tiles, (uint32_t)tiles.size(), n_plain, lds, s);
             if (r == hipSuccess) r = hipEventRecord(e1, s);
             if (r == hipSuccess) r = hipEventSynchronize(e1);
             float t = 0.0f;
@@ -1379,7 +1510,8 @@ RMH_EXPORT rmh_status rmh_residual_plan_autotune(rmh_plan* plan, const int16_t* 
     // the whole plan's tiles with the chosen splits
     if (e == hipSuccess) {
         std::vector<rmh::TileItem> tiles;
-        tiles_from_classes(plan->cls_first, plan->cls_count, plan->split, tiles, &plan->tile_lds);
+        plan->n_tiles_plain =
+            tiles_from_classes(plan->cls_first, plan->cls_count, plan->cls_lfnst, plan->split, tiles, &plan->tile_lds);
         e = hipMemcpy(plan->d_tiles, tiles.data(), tiles.size() * sizeof(rmh::TileItem), hipMemcpyHostToDevice);
         plan->n_tiles = tiles.size();
     }
@@ -1404,8 +1536,16 @@ RMH_EXPORT rmh_status rmh_residual_batch(rmh_ctx* ctx, const rmh_tu_desc* tus, s
     if (g != RMH_OK) return fail_locked(ctx, g, g == RMH_UNAVAILABLE ? "accelerator disabled" : "injected device failure");
     DeviceGuard dg(ctx->device);
     size_t tb = n_tu * sizeof(rmh_tu_desc), cb = n_coeff * sizeof(int16_t);
-    hipError_t e = ctx->dev.reserve(aligned(tb) + 2 * aligned(cb) + aligned(n_tu * 4) + aligned(n_tu * sizeof(rmh::TileItem)) +
-                                    aligned(n_tu * sizeof(rmh::TileTu)));
+    std::vector<uint32_t> order;
+    std::vector<rmh::TileItem> tiles;
+    rmh_plan tmp;
+    Prepass prep;
+    tmp.n_tiles_plain = build_residual_tiles(tus, n_tu, order, tiles, &tmp.n_fast, &tmp.tile_lds, tmp.cls_first,
+                                             tmp.cls_count, tmp.cls_lfnst, &prep);
+    hipError_t e = ctx->dev.reserve(aligned(tb) + 2 * aligned(cb) + aligned(order.size() * 4) +
+                                    aligned(tiles.size() * sizeof(rmh::TileItem)) +
+                                    aligned(tmp.n_fast * sizeof(rmh::TileTu)) + aligned(prep.off.size() * 4) +
+                                    aligned(prep.elements * sizeof(int16_t)));
     if (e != hipSuccess) return fail_locked(ctx, RMH_LIMIT_EXCEEDED, std::string("device allocation: ") + hipGetErrorString(e));
     e = ctx->host.reserve(aligned(cb));
     if (e != hipSuccess) return fail_locked(ctx, RMH_LIMIT_EXCEEDED, std::string("host staging: ") + hipGetErrorString(e));
@@ -1413,22 +1553,22 @@ RMH_EXPORT rmh_status rmh_residual_batch(rmh_ctx* ctx, const rmh_tu_desc* tus, s
     int16_t* d_coeff = (int16_t*)ctx->dev.take(cb);
     int16_t* d_res = (int16_t*)ctx->dev.take(cb);
     int16_t* h_res = (int16_t*)ctx->host.take(cb);
-    std::vector<uint32_t> order;
-    std::vector<rmh::TileItem> tiles;
-    rmh_plan tmp;
-    build_residual_tiles(tus, n_tu, order, tiles, &tmp.n_fast, &tmp.tile_lds, tmp.cls_first, tmp.cls_count);
     hipStream_t s = ctx->stream;
     uint32_t* d_order = (uint32_t*)ctx->dev.take(order.size() * 4);
     rmh::TileItem* d_tiles = (rmh::TileItem*)ctx->dev.take(tiles.size() * sizeof(rmh::TileItem));
-    const std::vector<rmh::TileTu> ttu = build_tile_tus(ctx->tabs, tus, order, tmp.n_fast);
+    const std::vector<rmh::TileTu> ttu = build_tile_tus(ctx->tabs, tus, order, tmp.n_fast, prep);
     rmh::TileTu* d_ttu = (rmh::TileTu*)ctx->dev.take(ttu.size() * sizeof(rmh::TileTu));
+    uint32_t* d_prep_off = (uint32_t*)ctx->dev.take(prep.off.size() * 4);
+    int16_t* d_prep = (int16_t*)ctx->dev.take(prep.elements * sizeof(int16_t));
     if ((e = hipMemcpyAsync(d_tus, tus, tb, hipMemcpyHostToDevice, s)) != hipSuccess ||
         (e = hipMemcpyAsync(d_coeff, coeff, cb, hipMemcpyHostToDevice, s)) != hipSuccess ||
         (e = hipMemcpyAsync(d_order, order.data(), order.size() * 4, hipMemcpyHostToDevice, s)) != hipSuccess ||
         (!tiles.empty() &&
          (e = hipMemcpyAsync(d_tiles, tiles.data(), tiles.size() * sizeof(rmh::TileItem), hipMemcpyHostToDevice, s)) != hipSuccess) ||
         (!ttu.empty() &&
-         (e = hipMemcpyAsync(d_ttu, ttu.data(), ttu.size() * sizeof(rmh::TileTu), hipMemcpyHostToDevice, s)) != hipSuccess))
+         (e = hipMemcpyAsync(d_ttu, ttu.data(), ttu.size() * sizeof(rmh::TileTu), hipMemcpyHostToDevice, s)) != hipSuccess) ||
+        (!prep.off.empty() &&
+         (e = hipMemcpyAsync(d_prep_off, prep.off.data(), prep.off.size() * 4, hipMemcpyHostToDevice, s)) != hipSuccess))
         return hip_fail_locked(ctx, e, "upload");
     tmp.ctx = ctx;
     tmp.kind = kResidual;
@@ -1438,6 +1578,9 @@ RMH_EXPORT rmh_status rmh_residual_batch(rmh_ctx* ctx, const rmh_tu_desc* tus, s
     tmp.d_tiles = d_tiles;
     tmp.n_tiles = tiles.size();
     tmp.d_tile_tus = d_ttu;
+    tmp.n_prep = prep.off.size();
+    tmp.d_prep_off = d_prep_off;
+    tmp.d_prep = d_prep;
     std::string err;
     rmh_status r = run_plan_locked_free(&tmp, d_coeff, d_res, 0, n_tu, s, &err);
     if (r != RMH_OK) return fail_locked(ctx, r, err);
