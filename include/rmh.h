@@ -117,7 +117,14 @@ typedef struct rmh_tu_desc {
     uint8_t lfnst_idx;   /* 0 (off), 1, 2 */
     uint8_t lfnst_mode;  /* LFNST intra-mode index 0..94 (vvc/lfnst.rs:85-95) */
     uint8_t flags;       /* RMH_TU_* */
-    uint8_t reserved;    /* must be 0 */
+    uint8_t joint;       /* joint Cb-Cr (ICT) of a chroma TU pair, recon plans only: 0 none;
+                          * else TuCResMode 1..3 (TuData::joint, decode_general.rs:773-783) in
+                          * bits 0-1 and ph_joint_cbcr_sign_flag in bit 2. The TU carries the coded
+                          * component (Cr for mode 1, Cb for modes 2 and 3, its placement's plane);
+                          * the other component's residual is derived from it as
+                          * reconstruct_joint_cbcr does (decode_general.rs:3908-3981): mode 1 Cb =
+                          * (+-Cr) >> 1, mode 2 Cr = (+-Cb) >> 1, mode 3 Cr = +-Cb, and added to
+                          * the same block of the other chroma plane */
 } rmh_tu_desc;
 
 /* Host memory, synchronous. `coeff` and `residual` hold n_coeff int16 each.
@@ -301,6 +308,55 @@ rmh_status rmh_deblock_plan_run(rmh_plan* plan, void* stream);
  * each plane that the in-place form makes first. Run with rmh_deblock_plan_run. */
 rmh_status rmh_deblock_to_plan_create(rmh_ctx* ctx, const rmh_dbk_frame* frames, size_t n_frames,
                                       const rmh_plane* dst, rmh_plan** out);
+
+/* Edge-map builder: the edge bytes above, derived on the device from the
+ * coded picture's CU / TU maps as the general decoder derives them
+ * (apply_deblock, vvc/decode_general.rs:4977-5187, with deblock_bs
+ * 4939-4975, luma_lens / chroma_lens 4916-4935 and derive_boundary_strength,
+ * vvc/filter.rs:202-226):
+ *   luma   an edge wherever the TU ids of two 4x4 cells differ (both inside a
+ *          TU): bS 2 if either CU is intra, else 1 if either cell's luma cbf
+ *          is set, else 1 if the reference POCs differ or an MV component
+ *          differs by >= 8 (1/16 sample), else 0; lengths luma_lens of the
+ *          two TU widths (vertical) or heights (horizontal); horizontal edges
+ *          on y % ctb_size == 0 carry RMH_EDGE_CTB_ROW;
+ *   chroma edges on the 8-sample chroma grid where the chroma TU ids differ,
+ *          bS 2 (the general decoder always passes 2), lengths chroma_lens
+ *          of the chroma TU sizes, (1, 3) on horizontal CTB-row boundaries.
+ * Edges are taken inside the visible picture only, as apply_deblock's loops
+ * run. One cell per 4x4 luma block, row-major, grid_w x grid_h: */
+typedef struct rmh_edge_cell {
+    uint32_t tu_y;     /* luma TU id, UINT32_MAX outside any TU */
+    uint32_t tu_c;     /* chroma TU id of the co-located 2x2 chroma cell, UINT32_MAX if none */
+    uint8_t tu_wy;     /* luma TU width and height (4..64) */
+    uint8_t tu_hy;
+    uint8_t tu_wc;     /* chroma TU width and height in chroma samples (2..32) */
+    uint8_t tu_hc;
+    uint8_t flags;     /* RMH_CELL_* */
+    uint8_t reserved[3];
+    int32_t mv_x;      /* the MV and reference POC of the first list with a reference */
+    int32_t mv_y;      /* (deblock_bs's `field`, decode_general.rs:4951-4958), 1/16 sample */
+    int32_t ref_poc;
+} rmh_edge_cell;
+
+enum { RMH_CELL_INTRA = 1u, RMH_CELL_CBF = 2u };
+
+typedef struct rmh_edge_frame {
+    const rmh_edge_cell* cells;  /* grid_w * grid_h cells */
+    uint8_t* edges;              /* output: grid_w * grid_h * 4 edge bytes (rmh_dbk_frame.edges) */
+    uint32_t grid_w;
+    uint32_t grid_h;
+    uint32_t vis_w;              /* visible luma size: 4 * grid - 3 .. 4 * grid */
+    uint32_t vis_h;
+    uint32_t ctb_size;           /* luma CTB size (power of two, 16..128) */
+    uint32_t reserved;
+} rmh_edge_frame;
+
+/* Host cells in, host edge bytes out (written only on RMH_OK). */
+rmh_status rmh_edge_map(rmh_ctx* ctx, const rmh_edge_frame* frames, size_t n_frames);
+/* Device cells and edge buffers; run with rmh_edge_map_plan_run. */
+rmh_status rmh_edge_map_plan_create(rmh_ctx* ctx, const rmh_edge_frame* frames, size_t n_frames, rmh_plan** out);
+rmh_status rmh_edge_map_plan_run(rmh_plan* plan, void* stream);
 
 /* In-place deblocking that allocates nothing. No counterpart in the
  * reference: it serves callers that cannot allocate device memory (code

@@ -481,3 +481,100 @@ void orc_deblock_chroma_plane(uint16_t* plane, int w, int h, int bd, int cu_step
         }
     }
 }
+
+/* ---- edge map: the edges apply_deblock filters, with their bS and lengths ---- */
+
+/* luma_lens (vvc/decode_general.rs:4916-4923). */
+static void luma_lens(int size_p, int size_q, int* lp, int* lq) {
+    if (size_p <= 4 || size_q <= 4) {
+        *lp = *lq = 1;
+        return;
+    }
+    *lp = size_p >= 32 ? 7 : 3;
+    *lq = size_q >= 32 ? 7 : 3;
+}
+
+/* chroma_lens (vvc/decode_general.rs:4929-4935). */
+static void chroma_lens(int size_p, int size_q, int ctb_boundary, int* lp, int* lq) {
+    if (size_p >= 8 && size_q >= 8) {
+        *lp = ctb_boundary ? 1 : 3;
+        *lq = 3;
+    } else {
+        *lp = *lq = 1;
+    }
+}
+
+/* derive_boundary_strength (vvc/filter.rs:202-226). */
+static int derive_boundary_strength(int p_intra, int q_intra, int p_cbf, int q_cbf, int is_tu_edge, int mvp_x,
+                                    int mvp_y, int mvq_x, int mvq_y, int ref_p, int ref_q) {
+    if (p_intra || q_intra) return 2;
+    if (is_tu_edge && (p_cbf || q_cbf)) return 1;
+    if (ref_p != ref_q || iabs(mvp_x - mvq_x) >= 8 || iabs(mvp_y - mvq_y) >= 8) return 1;
+    return 0;
+}
+
+/* deblock_bs (vvc/decode_general.rs:4939-4975) for the cells either side of
+ * a luma TU edge (the cells carry the MV and POC that `field` selects). */
+static int deblock_bs(const rmh_edge_cell* p, const rmh_edge_cell* q) {
+    return derive_boundary_strength(p->flags & RMH_CELL_INTRA, q->flags & RMH_CELL_INTRA, p->flags & RMH_CELL_CBF,
+                                    q->flags & RMH_CELL_CBF, 1, p->mv_x, p->mv_y, q->mv_x, q->mv_y, p->ref_poc,
+                                    q->ref_poc);
+}
+
+static uint8_t edge_byte(int bs, int lp, int lq, int ctb) {
+    return (uint8_t)((bs & 3) | (((lp - 1) / 2) << 2) | (((lq - 1) / 2) << 4) | (ctb ? RMH_EDGE_CTB_ROW : 0));
+}
+
+/* The edge loops of apply_deblock (vvc/decode_general.rs:4977-5187): luma
+ * edges on the 4-sample grid where the TU ids differ; chroma edges on the
+ * chroma 2-sample grid, kept only on the 8-sample chroma grid, where the
+ * chroma TU ids differ, always with bs = 2. Every byte of the map is written
+ * (0 where no edge is filtered). */
+size_t orc_edge_map(const rmh_edge_frame* frames, size_t n_frames) {
+    for (size_t fi = 0; fi < n_frames; fi++) {
+        const rmh_edge_frame* f = &frames[fi];
+        const int gw = (int)f->grid_w, w = (int)f->vis_w, h = (int)f->vis_h;
+        const rmh_edge_cell* cell = f->cells;
+        uint8_t* e = f->edges;
+        memset(e, 0, (size_t)gw * f->grid_h * 4);
+        int lp, lq;
+        for (int x = 4; x < w; x += 4)
+            for (int y = 0; y < h; y += 4) {
+                const rmh_edge_cell *l = &cell[(y / 4) * gw + x / 4 - 1], *r = &cell[(y / 4) * gw + x / 4];
+                if (l->tu_y != UINT32_MAX && r->tu_y != UINT32_MAX && l->tu_y != r->tu_y) {
+                    luma_lens(l->tu_wy, r->tu_wy, &lp, &lq);
+                    e[((y / 4) * gw + x / 4) * 4 + 0] = edge_byte(deblock_bs(l, r), lp, lq, 0);
+                }
+            }
+        for (int y = 4; y < h; y += 4)
+            for (int x = 0; x < w; x += 4) {
+                const rmh_edge_cell *t = &cell[(y / 4 - 1) * gw + x / 4], *b = &cell[(y / 4) * gw + x / 4];
+                if (t->tu_y != UINT32_MAX && b->tu_y != UINT32_MAX && t->tu_y != b->tu_y) {
+                    luma_lens(t->tu_hy, b->tu_hy, &lp, &lq);
+                    e[((y / 4) * gw + x / 4) * 4 + 1] = edge_byte(deblock_bs(t, b), lp, lq, y % (int)f->ctb_size == 0);
+                }
+            }
+        const int cw = (w + 1) / 2, ch = (h + 1) / 2;
+        for (int x = 2; x < cw; x += 2) {
+            if ((2 * x) % 16 != 0) continue;
+            for (int y = 0; y + 2 <= ch; y += 2) {
+                const rmh_edge_cell *l = &cell[(y / 2) * gw + x / 2 - 1], *r = &cell[(y / 2) * gw + x / 2];
+                if (l->tu_c != UINT32_MAX && r->tu_c != UINT32_MAX && l->tu_c != r->tu_c) {
+                    chroma_lens(l->tu_wc, r->tu_wc, 0, &lp, &lq);
+                    e[((y / 2) * gw + x / 2) * 4 + 2] = edge_byte(2, lp, lq, 0);
+                }
+            }
+        }
+        for (int y = 2; y < ch; y += 2) {
+            if ((2 * y) % 16 != 0) continue;
+            for (int x = 0; x + 2 <= cw; x += 2) {
+                const rmh_edge_cell *t = &cell[(y / 2 - 1) * gw + x / 2], *b = &cell[(y / 2) * gw + x / 2];
+                if (t->tu_c != UINT32_MAX && b->tu_c != UINT32_MAX && t->tu_c != b->tu_c) {
+                    chroma_lens(t->tu_hc, b->tu_hc, y % ((int)f->ctb_size / 2) == 0, &lp, &lq);
+                    e[((y / 2) * gw + x / 2) * 4 + 3] = edge_byte(2, lp, lq, 0);
+                }
+            }
+        }
+    }
+    return 0;
+}

@@ -545,6 +545,25 @@ size_t orc_residual_batch(const rmh_tu_desc* tus, size_t n_tu, const int16_t* co
  * `(pred + residual).clamp(0, max_val)`), which reconstruct_tus applies to the
  * inter prediction written by write_prediction (decode_general.rs:2641-2667,
  * 2866-2870). Planes are host memory; TUs in descriptor order. */
+/* pred + res clipped to the sample range, over a w x h block at (x0, y0). */
+static void add_block(const rmh_plane* p, int x0, int y0, int w, int h, const int32_t* res) {
+    const int maxv = (1 << p->bit_depth) - 1;
+    for (int y = 0; y < h; y++)
+        for (int x = 0; x < w; x++) {
+            const size_t at = (size_t)(y0 + y) * p->stride + x0 + x;
+            int v = res[y * w + x];
+            if (p->bytes_per_sample == 1) {
+                uint8_t* s = (uint8_t*)p->data + at;
+                v += *s;
+                *s = (uint8_t)(v < 0 ? 0 : v > maxv ? maxv : v);
+            } else {
+                uint16_t* s = (uint16_t*)p->data + at;
+                v += *s;
+                *s = (uint16_t)(v < 0 ? 0 : v > maxv ? maxv : v);
+            }
+        }
+}
+
 size_t orc_recon_batch(const rmh_tu_desc* tus, size_t n_tu, const int16_t* coeff, size_t n_coeff,
                        const rmh_tu_place* place, const rmh_recon_frame* frames, size_t n_frames) {
     for (size_t i = 0; i < n_tu; i++) {
@@ -555,6 +574,11 @@ size_t orc_recon_batch(const rmh_tu_desc* tus, size_t n_tu, const int16_t* coeff
         if (place[i].frame >= n_frames || place[i].plane > 2) return i + 1;
         const rmh_plane* p = &frames[place[i].frame].planes[place[i].plane];
         if ((uint32_t)place[i].x + w > p->width || (uint32_t)place[i].y + h > p->height) return i + 1;
+        if (t->joint & 3) {
+            if (place[i].plane != ((t->joint & 3) == 1 ? 2 : 1)) return i + 1;
+            const rmh_plane* q = &frames[place[i].frame].planes[3 - place[i].plane];
+            if (q->width != p->width || q->height != p->height) return i + 1;
+        }
     }
     size_t bad = 0;
 #pragma omp parallel for num_threads(orc_threads) schedule(dynamic, 64) if (orc_threads > 1) reduction(max : bad)
@@ -567,21 +591,23 @@ size_t orc_recon_batch(const rmh_tu_desc* tus, size_t n_tu, const int16_t* coeff
             bad = bad > i + 1 ? bad : i + 1;
             continue;
         }
-        const int maxv = (1 << p->bit_depth) - 1;
-        for (int y = 0; y < h; y++)
-            for (int x = 0; x < w; x++) {
-                const size_t at = (size_t)(place[i].y + y) * p->stride + place[i].x + x;
-                int v = res[y * w + x];
-                if (p->bytes_per_sample == 1) {
-                    uint8_t* s = (uint8_t*)p->data + at;
-                    v += *s;
-                    *s = (uint8_t)(v < 0 ? 0 : v > maxv ? maxv : v);
-                } else {
-                    uint16_t* s = (uint16_t*)p->data + at;
-                    v += *s;
-                    *s = (uint16_t)(v < 0 ? 0 : v > maxv ? maxv : v);
-                }
+        int32_t r32[64 * 64];
+        for (int k = 0; k < w * h; k++) r32[k] = res[k];
+        add_block(p, place[i].x, place[i].y, w, h, r32);
+        if (t->joint & 3) {
+            /* reconstruct_joint_cbcr (vvc/decode_general.rs:3908-3981): with
+             * mode = +-3 / +-1 / +-2 for TuCResMode 1 / 2 / 3 (the sign from
+             * ph_joint_cbcr_sign_flag), the other component is res_cb = res_cr >> 1,
+             * res_cr = res_cb >> 1 or res_cr = res_cb, each negated first
+             * (-res >> 1) under the sign, then added to its prediction and
+             * clamped like the coded one. */
+            const int mode = t->joint & 3, neg = (t->joint >> 2) & 1;
+            for (int k = 0; k < w * h; k++) {
+                const int32_t r = neg ? -r32[k] : r32[k]; /* i32 in the reference: -(-32768) stays 32768 */
+                r32[k] = mode == 3 ? r : r >> 1;
             }
+            add_block(&frames[place[i].frame].planes[3 - place[i].plane], place[i].x, place[i].y, w, h, r32);
+        }
     }
     return bad;
 }

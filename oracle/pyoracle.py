@@ -77,6 +77,8 @@ def _setup(L):
     L.orc_deblock_chroma_plane.argtypes = [U16P] + [c_int] * 9
     L.orc_tc_beta.argtypes = [c_int] * 5 + [I32P, I32P]
     L.orc_set_threads.argtypes = [c_int]
+    L.orc_edge_map.argtypes = [ctypes.c_void_p, c_size]
+    L.orc_edge_map.restype = c_size
     L.orc_dbk_long_line.argtypes = [I32P] + [c_int] * 4
     L.orc_dbk_short_line.argtypes = [I32P] + [c_int] * 5
     L.orc_dbk_chroma_line.argtypes = [I32P] + [c_int] * 4

@@ -125,6 +125,11 @@ int orc_dbk_luma_sample_decision(int p0, int p3, int q0, int q3, int dpq, int sp
                                  int side_q_large, int beta, int tc);
 int orc_dbk_chroma_sample_decision(int p0, int p3, int q0, int q3, int dpq, int beta, int tc);
 
+/* The edge map of apply_deblock's edge loops (vvc/decode_general.rs:4977-5187)
+ * from a cell grid: bS by deblock_bs / derive_boundary_strength, lengths by
+ * luma_lens / chroma_lens. Host buffers. */
+size_t orc_edge_map(const rmh_edge_frame* frames, size_t n_frames);
+
 /* tC and beta after bit-depth scaling (deblock_luma_edge_bd, vvc/filter.rs:653-666). */
 void orc_tc_beta(int qp, int bs, int beta_off, int tc_off, int bd, int* tc, int* beta);
 

@@ -73,6 +73,7 @@ EXPORTED = [
     "rmh_cnn_destroy", "rmh_cnn_get_params", "rmh_cnn_set_params", "rmh_cnn_forward", "rmh_cnn_predict",
     "rmh_cnn_gradients", "rmh_cnn_train_step", "rmh_cnn_train_step_soft", "rmh_residual_plan_autotune",
     "rmh_mc_bi_batch", "rmh_mc_bi_plan_create", "rmh_deblock_noalloc", "rmh_intra_predict",
+    "rmh_edge_map", "rmh_edge_map_plan_create", "rmh_edge_map_plan_run",
 ]
 
 
@@ -109,6 +110,9 @@ def load_library(path=LIB_PATH):
         "rmh_deblock_plan_run": ([vp, vp], st),
         "rmh_deblock_to_plan_create": ([vp, vp, sz, vp, ctypes.POINTER(vp)], st),
         "rmh_deblock_noalloc": ([vp, vp, sz, vp], st),
+        "rmh_edge_map": ([vp, vp, sz], st),
+        "rmh_edge_map_plan_create": ([vp, vp, sz, ctypes.POINTER(vp)], st),
+        "rmh_edge_map_plan_run": ([vp, vp], st),
         "rmh_sao": ([vp, vp, sz], st),
         "rmh_sao_plan_create": ([vp, vp, sz, ctypes.POINTER(vp)], st),
         "rmh_sao_plan_run": ([vp, vp], st),
@@ -206,6 +210,8 @@ class Plan:
             self.acc._check(L.rmh_mc_plan_run(self.handle, stream))
         elif self.kind == "sao":
             self.acc._check(L.rmh_sao_plan_run(self.handle, stream))
+        elif self.kind == "edge":
+            self.acc._check(L.rmh_edge_map_plan_run(self.handle, stream))
         else:
             self.acc._check(L.rmh_deblock_plan_run(self.handle, stream))
 
@@ -415,6 +421,24 @@ class Accelerator:
         fixed per-GPU scratch holds the tables and plane copies."""
         arr = (abi.DbkFrame * len(frames))(*frames)
         self._check(load_library().rmh_deblock_noalloc(self.handle, ctypes.addressof(arr), len(frames), stream))
+
+    def edge_map(self, cells, vis_w, vis_h, ctb_size):
+        """Edge bytes (uint8[grid_h, grid_w, 4]) of one frame from its cell grid
+        (abi.EDGE_CELL_DTYPE[grid_h, grid_w]), by rmh_edge_map."""
+        cells = np.ascontiguousarray(cells, dtype=abi.EDGE_CELL_DTYPE)
+        gh, gw = cells.shape
+        edges = np.zeros((gh, gw, 4), np.uint8)
+        f = abi.EdgeFrame(cells.ctypes.data, edges.ctypes.data, gw, gh, vis_w, vis_h, ctb_size, 0)
+        self._check(load_library().rmh_edge_map(self.handle, ctypes.byref(f), 1))
+        return edges
+
+    def edge_map_plan(self, frames):
+        """Edge-map plan over abi.EdgeFrame entries with device cells and edges."""
+        arr = (abi.EdgeFrame * len(frames))(*frames)
+        h = ctypes.c_void_p()
+        self._check(load_library().rmh_edge_map_plan_create(self.handle, ctypes.addressof(arr), len(frames),
+                                                            ctypes.byref(h)))
+        return Plan(self, h.value, "edge")
 
     def sao_plan(self, frames):
         """SAO plan over abi.SaoFrame entries with device planes and params; run with plan.run()."""

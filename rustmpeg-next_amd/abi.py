@@ -15,7 +15,7 @@ EDGE_BS_MASK, EDGE_LENP_SHIFT, EDGE_LENQ_SHIFT, EDGE_CTB_ROW = 0x03, 2, 4, 0x40
 TU_DTYPE = np.dtype([
     ("offset", "<u4"), ("log2_w", "u1"), ("log2_h", "u1"), ("qp", "u1"), ("bit_depth", "u1"),
     ("tr_hor", "u1"), ("tr_ver", "u1"), ("skip_w", "u1"), ("skip_h", "u1"),
-    ("lfnst_idx", "u1"), ("lfnst_mode", "u1"), ("flags", "u1"), ("reserved", "u1")])
+    ("lfnst_idx", "u1"), ("lfnst_mode", "u1"), ("flags", "u1"), ("joint", "u1")])
 PU_DTYPE = np.dtype([("x", "<u2"), ("y", "<u2"), ("log2_w", "u1"), ("log2_h", "u1"),
                      ("frame", "<u2"), ("mv_x", "<i4"), ("mv_y", "<i4")])
 
@@ -130,3 +130,17 @@ def mc_frame(refs, dsts):
         f.ref[i] = refs[i]
         f.dst[i] = dsts[i]
     return f
+
+
+# rmh_edge_cell: one 4x4 luma cell of the coded picture for the edge-map builder
+EDGE_CELL_DTYPE = np.dtype([("tu_y", "<u4"), ("tu_c", "<u4"), ("tu_wy", "u1"), ("tu_hy", "u1"), ("tu_wc", "u1"),
+                            ("tu_hc", "u1"), ("flags", "u1"), ("reserved", "u1", (3,)), ("mv_x", "<i4"),
+                            ("mv_y", "<i4"), ("ref_poc", "<i4")])
+assert EDGE_CELL_DTYPE.itemsize == 28
+CELL_INTRA, CELL_CBF = 1, 2
+
+
+class EdgeFrame(ctypes.Structure):
+    _fields_ = [("cells", ctypes.c_void_p), ("edges", ctypes.c_void_p), ("grid_w", ctypes.c_uint32),
+                ("grid_h", ctypes.c_uint32), ("vis_w", ctypes.c_uint32), ("vis_h", ctypes.c_uint32),
+                ("ctb_size", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]

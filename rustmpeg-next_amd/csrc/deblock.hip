@@ -790,6 +790,90 @@ __global__ __launch_bounds__(256) void deblock_tile_kernel(const rmh_dbk_frame* 
 
 }  // namespace
 
+namespace {
+
+// luma_lens (decode_general.rs:4916-4923): a side of at most 4 samples forces
+// both lengths to 1, else 7 for a side >= 32 and 3 below. Returns the two
+// length codes (length = 2 c + 1) as lenP | lenQ << 2.
+__device__ __forceinline__ int luma_len_codes(int p, int q) {
+    if (p <= 4 || q <= 4) return 0;
+    return (p >= 32 ? 3 : 1) | ((q >= 32 ? 3 : 1) << 2);
+}
+
+// chroma_lens (decode_general.rs:4929-4935), as codes lenP | lenQ << 2.
+__device__ __forceinline__ int chroma_len_codes(int p, int q, bool ctb) {
+    if (p >= 8 && q >= 8) return ctb ? (0 | (1 << 2)) : (1 | (1 << 2));
+    return 0;
+}
+
+// deblock_bs (decode_general.rs:4939-4975) over derive_boundary_strength
+// (vvc/filter.rs:202-226) for a luma TU edge between cells p and q.
+__device__ __forceinline__ int edge_bs(const rmh_edge_cell& p, const rmh_edge_cell& q) {
+    if ((p.flags | q.flags) & RMH_CELL_INTRA) return 2;
+    if ((p.flags | q.flags) & RMH_CELL_CBF) return 1;
+    if (p.ref_poc != q.ref_poc || abs(p.mv_x - q.mv_x) >= 8 || abs(p.mv_y - q.mv_y) >= 8) return 1;
+    return 0;
+}
+
+__global__ __launch_bounds__(256) void edge_map_kernel(const rmh_edge_frame* __restrict__ frames,
+                                                       const uint32_t* __restrict__ cell_start, uint32_t n_frames,
+                                                       uint32_t total) {
+    const uint32_t g = blockIdx.x * 256 + threadIdx.x;
+    if (g >= total) return;
+    // the frame of cell g: binary search of the running sum
+    uint32_t lo = 0, hi = n_frames;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (cell_start[mid] <= g) lo = mid; else hi = mid;
+    }
+    const rmh_edge_frame f = frames[lo];
+    const uint32_t k = g - cell_start[lo];
+    const int i = (int)(k % f.grid_w), j = (int)(k / f.grid_w);
+    const int gw = (int)f.grid_w;
+    const rmh_edge_cell c = f.cells[k];
+    const int x = 4 * i, y = 4 * j;
+    const int cw = (int)((f.vis_w + 1) / 2), ch = (int)((f.vis_h + 1) / 2);  // visible chroma size
+    uint32_t out = 0;
+    // luma vertical edge at x, rows y .. y + 3 (x = 4, 8, ... < vis_w)
+    if (i > 0 && x < (int)f.vis_w && y < (int)f.vis_h) {
+        const rmh_edge_cell p = f.cells[k - 1];
+        if (p.tu_y != 0xffffffffu && c.tu_y != 0xffffffffu && p.tu_y != c.tu_y)
+            out |= (uint32_t)(edge_bs(p, c) | (luma_len_codes(p.tu_wy, c.tu_wy) << 2));
+    }
+    // luma horizontal edge at y, columns x .. x + 3
+    if (j > 0 && y < (int)f.vis_h && x < (int)f.vis_w) {
+        const rmh_edge_cell p = f.cells[k - gw];
+        if (p.tu_y != 0xffffffffu && c.tu_y != 0xffffffffu && p.tu_y != c.tu_y)
+            out |= (uint32_t)(edge_bs(p, c) | (luma_len_codes(p.tu_hy, c.tu_hy) << 2) |
+                              ((y % (int)f.ctb_size) == 0 ? RMH_EDGE_CTB_ROW : 0)) << 8;
+    }
+    // chroma vertical edge at chroma x = 2 i (on the 8-sample chroma grid),
+    // chroma rows 2 j, 2 j + 1 (while 2 j + 2 <= visible chroma height)
+    const int xc = 2 * i, yc = 2 * j;
+    if (i > 0 && (2 * xc) % 16 == 0 && xc < cw && yc + 2 <= ch) {
+        const rmh_edge_cell p = f.cells[k - 1];
+        if (p.tu_c != 0xffffffffu && c.tu_c != 0xffffffffu && p.tu_c != c.tu_c)
+            out |= (uint32_t)(2 | (chroma_len_codes(p.tu_wc, c.tu_wc, false) << 2)) << 16;
+    }
+    // chroma horizontal edge at chroma y = 2 j, chroma columns 2 i, 2 i + 1
+    if (j > 0 && (2 * yc) % 16 == 0 && yc < ch && xc + 2 <= cw) {
+        const rmh_edge_cell p = f.cells[k - gw];
+        if (p.tu_c != 0xffffffffu && c.tu_c != 0xffffffffu && p.tu_c != c.tu_c)
+            out |= (uint32_t)(2 | (chroma_len_codes(p.tu_hc, c.tu_hc, (yc % ((int)f.ctb_size / 2)) == 0) << 2)) << 24;
+    }
+    gdst<uint32_t>(f.edges)[k] = out;
+}
+
+}  // namespace
+
+hipError_t launch_edge_map(const rmh_edge_frame* frames, const uint32_t* cell_start, uint32_t n_frames,
+                           uint32_t total_cells, hipStream_t stream) {
+    if (total_cells == 0) return hipSuccess;
+    hipLaunchKernelGGL(edge_map_kernel, dim3((total_cells + 255) / 256), dim3(256), 0, stream, frames, cell_start,
+                       n_frames, total_cells);
+    return hipGetLastError();
+}
+
 uint32_t deblock_tiles(uint32_t width, uint32_t height) {
     return ((width + kTW - 1) / kTW) * ((height + kTH - 1) / kTH);
 }

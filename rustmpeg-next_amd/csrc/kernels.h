@@ -161,6 +161,11 @@ hipError_t launch_deblock(const rmh_dbk_frame* frames, const rmh_plane* shadows,
                           const uint32_t* frame_of, uint32_t n_frames, uint32_t total_tiles, uint32_t tiles8,
                           uint32_t max_rows, bool copy_shadows, hipStream_t stream);
 
+// Edge-map builder over `n_frames` frames (device frame table), one thread per
+// cell; cell_start[f] is the running sum of grid_w * grid_h.
+hipError_t launch_edge_map(const rmh_edge_frame* frames, const uint32_t* cell_start, uint32_t n_frames,
+                           uint32_t total_cells, hipStream_t stream);
+
 // One plane of one picture of an SAO batch (built at plan creation).
 struct SaoJob {
     const void* src;               // deblocked plane

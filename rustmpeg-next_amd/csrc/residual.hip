@@ -171,14 +171,29 @@ __global__ __launch_bounds__(64) void residual_kernel(ResidualArgs a) {
     const int qp = d.qp < 4 ? 4 : d.qp;
     const int16_t* __restrict__ cin = a.coeff + d.offset;
     int16_t* __restrict__ rout = a.place ? nullptr : a.residual + d.offset;
-    ReconDst dst{};
-    if (a.place) dst = recon_dst(a.place, a.frames, tu_index);
+    ReconDst dst{}, jdst{};
+    // joint Cb-Cr (reconstruct_joint_cbcr, decode_general.rs:3908-3981): the
+    // other chroma component's residual is derived from this one
+    const int jmode = d.joint & 3, jneg = (d.joint >> 2) & 1, jshift = jmode == 3 ? 0 : 1;
+    if (a.place) {
+        dst = recon_dst(a.place, a.frames, tu_index);
+        if (jmode) {
+            const rmh_tu_place pl = a.place[tu_index];
+            const rmh_plane p = a.frames[pl.frame].planes[3 - pl.plane];
+            jdst.bps = p.bytes_per_sample;
+            jdst.stride = (int)p.stride;
+            jdst.maxv = (1 << p.bit_depth) - 1;
+            jdst.base = (char*)p.data + ((size_t)pl.y * p.stride + pl.x) * jdst.bps;
+        }
+    }
     // writes residual e (raster index in the TU) to the residual buffer or adds it to the picture
     auto put = [&](int e, int v) {
-        if (rout)
+        if (rout) {
             rout[e] = (int16_t)v;
-        else
+        } else {
             recon_add(dst, e & (w - 1), e >> lw, v);
+            if (jmode) recon_add(jdst, e & (w - 1), e >> lw, (jneg ? -v : v) >> jshift);
+        }
     };
 
     if (d.flags & RMH_TU_TRANSFORM_SKIP) {

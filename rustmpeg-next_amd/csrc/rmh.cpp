@@ -149,7 +149,7 @@ struct rmh_ctx {
     void* scratch = nullptr;  // this device's address of `scratch`
 };
 
-enum PlanKind { kResidual = 0, kMc = 1, kDeblock = 2, kSao = 3 };
+enum PlanKind { kResidual = 0, kMc = 1, kDeblock = 2, kSao = 3, kEdge = 4 };
 
 struct rmh_plan {
     rmh_ctx* ctx = nullptr;
@@ -197,6 +197,11 @@ struct rmh_plan {
     size_t n_dbk = 0;
     uint32_t total_tiles = 0, tiles8 = 0, max_rows = 0;
     bool dbk_copy = true;  // in place (shadow copies); false: d_shadows are the sources
+    // edge map
+    const rmh_edge_frame* d_edge_frames = nullptr;
+    const uint32_t* d_cell_start = nullptr;
+    size_t n_edge = 0;
+    uint32_t total_cells = 0;
     // sao
     const rmh::SaoJob* d_sao_jobs = nullptr;
     const uint32_t* d_sao_block_job = nullptr;
@@ -414,6 +419,7 @@ const char* check_tu(const rmh_tu_desc& t, size_t n_coeff) {
     if (t.bit_depth < 8 || t.bit_depth > 12) return "bit depth outside 8..12";
     if (t.qp > 63 + 6 * 4) return "QP' above 87";
     if (t.flags & ~(RMH_TU_TRANSFORM_SKIP | RMH_TU_DEP_QUANT)) return "unknown TU flags";
+    if ((t.joint & ~7) || (t.joint && (t.joint & 3) == 0)) return "joint Cb-Cr code outside modes 1..3 and the sign bit";
     if ((uint64_t)t.offset + (uint64_t)(w * h) > n_coeff) return "TU block outside the coefficient buffer";
     if (t.flags & RMH_TU_TRANSFORM_SKIP) return nullptr;
     if (t.tr_hor > 2 || t.tr_ver > 2) return "unknown transform type";
@@ -481,6 +487,15 @@ rmh_status validate_place(rmh_ctx* ctx, const rmh_tu_desc* tus, size_t n_tu, con
         if (tus[i].bit_depth != p.bit_depth)
             return fail(ctx, RMH_INVALID_INPUT, "TU " + std::to_string(i) + ": bit depth differs from its plane");
         by_plane[pl.frame * 3 + pl.plane].push_back((uint32_t)i);
+        if (tus[i].joint) {
+            // the coded component is Cr for mode 1 and Cb for modes 2 and 3; the
+            // derived residual lands in the same block of the other chroma plane
+            const int coded = (tus[i].joint & 3) == 1 ? 2 : 1;
+            const rmh_plane& q = frames[pl.frame].planes[3 - coded];
+            if (pl.plane != coded || q.width != p.width || q.height != p.height || q.bit_depth != p.bit_depth)
+                return fail(ctx, RMH_INVALID_INPUT, "TU " + std::to_string(i) + ": joint Cb-Cr placement");
+            by_plane[pl.frame * 3 + 3 - coded].push_back((uint32_t)i);
+        }
     }
     std::vector<uint8_t> cover;
     for (size_t k = 0; k < by_plane.size(); k++) {
@@ -925,7 +940,7 @@ TileTable tile_table(const rmh_dbk_frame* frames, size_t n_frames) {
 // dequantised (and LFNST-ed) first by the prepass, one workgroup per TU, and
 // the tiled kernel reads its output.
 bool tile_eligible(const rmh_tu_desc& t) {
-    return t.log2_w >= 2 && t.log2_h >= 2 && !(t.flags & RMH_TU_TRANSFORM_SKIP) && (t.offset & 3) == 0;
+    return t.log2_w >= 2 && t.log2_h >= 2 && !(t.flags & RMH_TU_TRANSFORM_SKIP) && (t.offset & 3) == 0 && !t.joint;
 }
 bool needs_prepass(const rmh_tu_desc& t);
 
@@ -1187,6 +1202,8 @@ rmh_status run_plan_locked_free(rmh_plan* plan, const int16_t* d_coeff, int16_t*
                                               (uint32_t)plan->n_items8, s);
     } else if (plan->kind == kSao) {
         e = rmh::launch_sao(plan->d_sao_jobs, plan->d_sao_block_job, plan->sao_range, s);
+    } else if (plan->kind == kEdge) {
+        e = rmh::launch_edge_map(plan->d_edge_frames, plan->d_cell_start, (uint32_t)plan->n_edge, plan->total_cells, s);
     } else {
         e = rmh::launch_deblock(plan->d_dbk, plan->d_shadows, plan->d_tile_start, plan->d_frame_of, (uint32_t)plan->n_dbk,
                                 plan->total_tiles, plan->tiles8, plan->max_rows, plan->dbk_copy, s);
@@ -1308,6 +1325,10 @@ rmh_status create_residual_plan(rmh_ctx* ctx, const rmh_tu_desc* tus, size_t n_t
     if (v != RMH_OK) return v;
     const bool recon = place != nullptr || frames != nullptr;
     if (recon && (v = validate_place(ctx, tus, n_tu, place, frames, n_frames)) != RMH_OK) return v;
+    if (!recon)
+        for (size_t i = 0; i < n_tu; i++)
+            if (tus[i].joint)
+                return fail(ctx, RMH_INVALID_INPUT, "TU " + std::to_string(i) + ": joint Cb-Cr residuals need a recon plan");
     if (ctx->disabled.load()) return fail(ctx, RMH_UNAVAILABLE, "accelerator disabled");
     DeviceGuard dg(ctx->device);
     rmh_plan* p = new rmh_plan();
@@ -1532,6 +1553,9 @@ RMH_EXPORT rmh_status rmh_residual_batch(rmh_ctx* ctx, const rmh_tu_desc* tus, s
     if (v != RMH_OK) return v;
     if (n_tu == 0) return RMH_OK;
     if (!coeff || !residual) return fail_locked(ctx, RMH_INVALID_INPUT, "null coefficient or residual buffer");
+    for (size_t i = 0; i < n_tu; i++)
+        if (tus[i].joint)
+            return fail_locked(ctx, RMH_INVALID_INPUT, "TU " + std::to_string(i) + ": joint Cb-Cr residuals need a recon plan");
     rmh_status g = gate(ctx);
     if (g != RMH_OK) return fail_locked(ctx, g, g == RMH_UNAVAILABLE ? "accelerator disabled" : "injected device failure");
     DeviceGuard dg(ctx->device);
@@ -1970,6 +1994,124 @@ RMH_EXPORT rmh_status rmh_deblock_plan_run(rmh_plan* plan, void* stream) {
     return run_plan(plan, nullptr, nullptr, 0, 0, stream);
 }
 
+// ------------------------------------------------------------------ edge map
+
+namespace {
+
+// Edge-map frames: grids that match the visible size, a CTB size the
+// decoder allows, and (device_edges) 4-byte aligned edge buffers.
+rmh_status validate_edge(rmh_ctx* ctx, const rmh_edge_frame* frames, size_t n_frames, bool device_edges,
+                         uint64_t* total) {
+    if (n_frames && !frames) return fail(ctx, RMH_INVALID_INPUT, "null frame array");
+    if (n_frames > 65535) return fail(ctx, RMH_LIMIT_EXCEEDED, "more than 65535 frames");
+    *total = 0;
+    for (size_t f = 0; f < n_frames; f++) {
+        const rmh_edge_frame& e = frames[f];
+        const std::string tag = "frame " + std::to_string(f) + ": ";
+        if (!e.cells || !e.edges) return fail(ctx, RMH_INVALID_INPUT, tag + "null cells or edges");
+        if (e.grid_w == 0 || e.grid_h == 0 || e.grid_w > 16384 || e.grid_h > 16384 || e.reserved)
+            return fail(ctx, RMH_INVALID_INPUT, tag + "grid size");
+        if (e.vis_w > 4 * e.grid_w || e.vis_w + 3 < 4 * e.grid_w || e.vis_h > 4 * e.grid_h || e.vis_h + 3 < 4 * e.grid_h)
+            return fail(ctx, RMH_INVALID_INPUT, tag + "visible size does not match the grid");
+        if (e.ctb_size < 16 || e.ctb_size > 128 || (e.ctb_size & (e.ctb_size - 1)))
+            return fail(ctx, RMH_INVALID_INPUT, tag + "CTB size");
+        if (device_edges && ((uintptr_t)e.edges & 3)) return fail(ctx, RMH_INVALID_INPUT, tag + "edges not 4-byte aligned");
+        *total += (uint64_t)e.grid_w * e.grid_h;
+    }
+    if (*total > 0xffffffffull) return fail(ctx, RMH_LIMIT_EXCEEDED, "more than 2^32 cells");
+    return RMH_OK;
+}
+
+std::vector<uint32_t> cell_starts(const rmh_edge_frame* frames, size_t n_frames) {
+    std::vector<uint32_t> v(n_frames + 1, 0);
+    for (size_t f = 0; f < n_frames; f++) v[f + 1] = v[f] + frames[f].grid_w * frames[f].grid_h;
+    return v;
+}
+
+}  // namespace
+
+RMH_EXPORT rmh_status rmh_edge_map_plan_create(rmh_ctx* ctx, const rmh_edge_frame* frames, size_t n_frames,
+                                               rmh_plan** out) {
+    if (!ctx || !out) return RMH_INVALID_INPUT;
+    *out = nullptr;
+    uint64_t total = 0;
+    rmh_status v = validate_edge(ctx, frames, n_frames, true, &total);
+    if (v != RMH_OK) return v;
+    if (ctx->disabled.load()) return fail(ctx, RMH_UNAVAILABLE, "accelerator disabled");
+    DeviceGuard dg(ctx->device);
+    rmh_plan* p = new rmh_plan();
+    p->ctx = ctx;
+    p->kind = kEdge;
+    p->n_edge = n_frames;
+    p->total_cells = (uint32_t)total;
+    p->bytes = total * (sizeof(rmh_edge_cell) + 4);
+    const std::vector<uint32_t> starts = cell_starts(frames, n_frames);
+    const size_t fb = aligned(n_frames * sizeof(rmh_edge_frame)), sb = aligned(starts.size() * 4);
+    hipError_t e = hipMalloc(&p->d_mem, fb + sb);
+    char* base = (char*)p->d_mem;
+    if (e == hipSuccess && n_frames) e = hipMemcpy(base, frames, n_frames * sizeof(rmh_edge_frame), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(base + fb, starts.data(), starts.size() * 4, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        if (p->d_mem) hipFree(p->d_mem);
+        delete p;
+        return fail(ctx, RMH_EXECUTION, std::string("edge-map plan upload failed: ") + hipGetErrorString(e));
+    }
+    p->d_edge_frames = (const rmh_edge_frame*)base;
+    p->d_cell_start = (const uint32_t*)(base + fb);
+    *out = p;
+    return RMH_OK;
+}
+
+RMH_EXPORT rmh_status rmh_edge_map_plan_run(rmh_plan* plan, void* stream) {
+    if (!plan || plan->kind != kEdge) return RMH_INVALID_INPUT;
+    if (plan->total_cells == 0) return RMH_OK;
+    return run_plan(plan, nullptr, nullptr, 0, 0, stream);
+}
+
+RMH_EXPORT rmh_status rmh_edge_map(rmh_ctx* ctx, const rmh_edge_frame* frames, size_t n_frames) {
+    if (!ctx) return RMH_INVALID_INPUT;
+    uint64_t total = 0;
+    rmh_status v = validate_edge(ctx, frames, n_frames, false, &total);  // reports through fail(), before the lock
+    if (v != RMH_OK) return v;
+    if (total == 0) return RMH_OK;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    rmh_status g = gate(ctx);
+    if (g != RMH_OK) return fail_locked(ctx, g, g == RMH_UNAVAILABLE ? "accelerator disabled" : "injected device failure");
+    DeviceGuard dg(ctx->device);
+    const std::vector<uint32_t> starts = cell_starts(frames, n_frames);
+    const size_t fb = aligned(n_frames * sizeof(rmh_edge_frame)), sb = aligned(starts.size() * 4);
+    const size_t cb = aligned(total * sizeof(rmh_edge_cell)), eb = aligned(total * 4);
+    hipError_t e = ctx->dev.reserve(fb + sb + cb + eb);
+    if (e != hipSuccess) return fail_locked(ctx, RMH_LIMIT_EXCEEDED, std::string("device allocation: ") + hipGetErrorString(e));
+    e = ctx->host.reserve(eb);
+    if (e != hipSuccess) return fail_locked(ctx, RMH_LIMIT_EXCEEDED, std::string("host staging: ") + hipGetErrorString(e));
+    rmh_edge_frame* d_frames = (rmh_edge_frame*)ctx->dev.take(fb);
+    uint32_t* d_starts = (uint32_t*)ctx->dev.take(sb);
+    rmh_edge_cell* d_cells = (rmh_edge_cell*)ctx->dev.take(cb);
+    uint8_t* d_edges = (uint8_t*)ctx->dev.take(eb);
+    uint8_t* h_edges = (uint8_t*)ctx->host.take(eb);
+    std::vector<rmh_edge_frame> dev(frames, frames + n_frames);
+    hipStream_t s = ctx->stream;
+    for (size_t f = 0; f < n_frames && e == hipSuccess; f++) {
+        const size_t n = (size_t)frames[f].grid_w * frames[f].grid_h;
+        dev[f].cells = d_cells + starts[f];
+        dev[f].edges = d_edges + 4 * (size_t)starts[f];
+        e = hipMemcpyAsync((void*)dev[f].cells, frames[f].cells, n * sizeof(rmh_edge_cell), hipMemcpyHostToDevice, s);
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(d_frames, dev.data(), n_frames * sizeof(rmh_edge_frame), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_starts, starts.data(), starts.size() * 4, hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return hip_fail_locked(ctx, e, "upload");
+    if ((e = rmh::launch_edge_map(d_frames, d_starts, (uint32_t)n_frames, (uint32_t)total, s)) != hipSuccess)
+        return hip_fail_locked(ctx, e, "kernel launch");
+    if ((e = hipMemcpyAsync(h_edges, d_edges, total * 4, hipMemcpyDeviceToHost, s)) != hipSuccess)
+        return hip_fail_locked(ctx, e, "download");
+    rmh_status r = wait_stream(ctx, s);
+    if (r != RMH_OK) return r;
+    for (size_t f = 0; f < n_frames; f++)
+        memcpy(frames[f].edges, h_edges + 4 * (size_t)starts[f], 4 * (size_t)frames[f].grid_w * frames[f].grid_h);
+    return RMH_OK;
+}
+
 RMH_EXPORT rmh_status rmh_deblock_noalloc(rmh_ctx* ctx, const rmh_dbk_frame* frames, size_t n_frames, void* stream) {
     if (!ctx) return RMH_INVALID_INPUT;
     {
@@ -2255,7 +2397,8 @@ RMH_EXPORT rmh_status rmh_graph_create(rmh_ctx* ctx, const rmh_graph_node* nodes
         for (size_t i = 0; i < n_nodes && st == RMH_OK; i++) {
             rmh_plan* p = nodes[i].plan;
             if ((p->kind == kResidual && p->n_tu == 0) || (p->kind == kMc && p->n_items == 0) ||
-                (p->kind == kDeblock && p->n_dbk == 0) || (p->kind == kSao && p->n_sao_jobs == 0))
+                (p->kind == kDeblock && p->n_dbk == 0) || (p->kind == kSao && p->n_sao_jobs == 0) ||
+                (p->kind == kEdge && p->total_cells == 0))
                 continue;
             st = run_plan_locked_free(p, nodes[i].d_coeff, nodes[i].d_residual, 0, p->n_tu, g->capture, &err);
         }

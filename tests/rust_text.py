@@ -11,7 +11,8 @@ functions use (crates/rustmpeg-video/src/vvc/filter.rs):
 - assignments to variables and to `line[...]`;
 - `if COND { .. }` / `if COND { .. } else { .. }` as statements and as
   expressions, `return;`, `for PAT in ITER { .. }`;
-- `match SCRUT { PAT | PAT => EXPR, _ => { ..; EXPR } }` as an expression;
+- `match SCRUT { PAT | PAT => EXPR, _ => { ..; EXPR } }` as an expression, and
+  as a statement with statement arms;
 - the methods `.clamp(a, b)`, `.abs()`, `.min(x)`, `.max(x)`, `.iter()`,
   `.zip(y)`, `.enumerate()`;
 - `&&`, `||`, `!`, references (`&m`, `&[..]`, `*line`), `i32::from(x)`,
@@ -359,13 +360,50 @@ def block(body, indent="    "):
             out.append(indent + "if (%s):" % expr(cond))
             out += block(s[brace + 1:close], indent + "    ") or [indent + "    pass"]
             s = s[close + 1:].lstrip()
-            if s.startswith("else"):
+            while s.startswith("else"):
                 s = s[4:].lstrip()
-                brace = s.index("{")
-                close = _match(s, brace)
-                out.append(indent + "else:")
+                if re.match(r"if\b", s):  # else if
+                    brace = _top_brace(s, 2)
+                    cond = s[2:brace]
+                    close = _match(s, brace)
+                    out.append(indent + "elif (%s):" % expr(cond))
+                else:
+                    brace = s.index("{")
+                    close = _match(s, brace)
+                    out.append(indent + "else:")
                 out += block(s[brace + 1:close], indent + "    ") or [indent + "    pass"]
                 s = s[close + 1:].lstrip()
+            continue
+        if re.match(r"match\b", s):
+            # a match statement: arms `PAT | PAT => stmt,` or `PAT => { stmts }`
+            brace = s.index("{")
+            scrut = expr(s[5:brace])
+            close = _match(s, brace)
+            rest = s[brace + 1:close].strip()
+            first = True
+            while rest:
+                arrow = rest.index("=>")
+                pat = rest[:arrow].strip()
+                rest = rest[arrow + 2:].lstrip()
+                if rest.startswith("{"):
+                    e = _match(rest, 0)
+                    body = rest[1:e]
+                    rest = rest[e + 1:].lstrip()
+                else:
+                    parts = _split_top(rest, ",")
+                    body = parts[0] + ";"
+                    rest = ",".join(parts[1:])
+                rest = rest.lstrip()
+                if rest.startswith(","):
+                    rest = rest[1:].lstrip()
+                if pat == "_":
+                    out.append(indent + "else:")
+                else:
+                    cond = " or ".join("(%s) == (%s)" % (scrut, expr(a)) for a in _split_top(pat, "|"))
+                    out.append(indent + ("if (%s):" if first else "elif (%s):") % cond)
+                out += block(body, indent + "    ") or [indent + "    pass"]
+                first = False
+            s = s[close + 1:].lstrip()
             continue
         if re.match(r"for\b", s):
             brace = _top_brace(s, 3)
@@ -410,6 +448,8 @@ def block(body, indent="    "):
         cm = re.match(r"^([\w\[\]\s\.\-\+\*]+?)\s*(\+|-|\*|<<|>>|\||&|\^)=\s*(.+)$", stmt, re.S)
         if stmt == "return":
             out.append(indent + "return None")
+        elif stmt.startswith("return "):
+            out.append(indent + "return (%s)" % expr(stmt[7:]))
         elif cm:
             out.append(indent + "%s %s= (%s)" % (cm.group(1).strip(), cm.group(2), expr(cm.group(3))))
         elif stmt.startswith("let "):
@@ -435,6 +475,7 @@ def _fill(xs, v):
 def compile_fn(text, name, extra=None):
     """A Python function with the parameters and body of Rust fn `name`."""
     params, body = fn_source(text, name)
+    body = re.sub(r"//[^\n]*", "", body)  # line comments
     names = [p.split(":")[0].strip() for p in _split_top(params, ",") if p.strip()]
     names = [re.sub(r"^&?(mut\s+)?", "", n) for n in names]
     src = "def %s(%s):\n%s\n" % (name, ", ".join(names), "\n".join(block(body)) or "    pass")

@@ -34,8 +34,9 @@ def test_struct_layouts_match_header(rmh):
     assert ctypes.sizeof(abi.DbkFrame) == 104
     assert ctypes.sizeof(abi.GraphNode) == 24
     assert ctypes.sizeof(abi.Config) == 12
-    from workloads import PU_DTYPE, TU_DTYPE
-    assert TU_DTYPE == abi.TU_DTYPE and PU_DTYPE == abi.PU_DTYPE
+    from workloads import EDGE_CELL_DTYPE, PU_DTYPE, TU_DTYPE
+    assert TU_DTYPE == abi.TU_DTYPE and PU_DTYPE == abi.PU_DTYPE and EDGE_CELL_DTYPE == abi.EDGE_CELL_DTYPE
+    assert ctypes.sizeof(abi.EdgeFrame) == 40
 
 
 def test_kernel_matrices_equal_oracle_matrices(rmh, po):

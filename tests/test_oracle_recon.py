@@ -91,3 +91,39 @@ def test_pframe_tus_tile_the_coded_cus(size):
             sc = cover[c][y // 2:(y + ch) // 2, x // 2:(x + cw) // 2]
             assert sc.min() == sc.max() == s.min()
     assert 0.6 * len(f["cus"]) < coded < len(f["cus"])
+
+
+def test_joint_cbcr_oracle_equals_reference_text(po):
+    """Joint Cb-Cr: the mode selection and residual derivation of
+    reconstruct_joint_cbcr (vvc/decode_general.rs:3908-3981), read as text
+    and evaluated in Python (rust_text), applied to the coded residual and
+    arbitrary values for the other component (which the reference computes
+    from that component's own coefficients and then overwrites), equal the
+    oracle's recon of both planes over a mid-grey prediction."""
+    import types
+
+    import rust_text
+    from conftest import reference_text
+    text = reference_text("crates/rustmpeg-video/src/vvc/decode_general.rs")
+    _, body = rust_text.fn_source(text, "reconstruct_joint_cbcr")
+    src = "\n".join(rust_text.block(body[body.index("let mode ="):body.index("if g.lmcs_on")], ""))
+    tus, coeff, place, blocks = W.joint_cbcr_frame(3, bit_depth=10)
+    res = po.residual_batch(tus, coeff)
+    rng = np.random.default_rng(9)
+    planes = [[np.full((64, 128), 512, np.uint16), np.full((64, 128), 512, np.uint16),
+               np.full((64, 128), 512, np.uint16)]]
+    sel = [i for i, b in enumerate(blocks) if b[5]]
+    assert len(sel) > 20 and {blocks[i][5] for i in sel} == {1, 2, 3, 5, 6, 7}
+    assert _run_oracle(po, tus[sel], coeff, place[sel], planes, 10) == 0
+    for i in sel:
+        plane, x, y, w, h, joint = blocks[i]
+        r = res[int(tus[i]["offset"]):int(tus[i]["offset"]) + w * h].astype(int).tolist()
+        other = rng.integers(-300, 300, w * h).tolist()
+        ns = {"res_cb": list(r) if plane == 1 else other, "res_cr": list(r) if plane == 2 else list(other),
+              "cw": w, "ch": h, "cb": types.SimpleNamespace(joint=joint & 3),
+              "g": types.SimpleNamespace(ph=types.SimpleNamespace(joint_cbcr_sign=bool(joint & 4))),
+              "_range": range}
+        exec(src, ns)
+        for c, key in ((1, "res_cb"), (2, "res_cr")):
+            got = planes[0][c][y:y + h, x:x + w].astype(int).reshape(-1)
+            assert (got == np.clip(512 + np.array(ns[key]), 0, 1023)).all(), (i, joint, c)

@@ -23,7 +23,7 @@ import numpy as np
 TU_DTYPE = np.dtype([
     ("offset", "<u4"), ("log2_w", "u1"), ("log2_h", "u1"), ("qp", "u1"), ("bit_depth", "u1"),
     ("tr_hor", "u1"), ("tr_ver", "u1"), ("skip_w", "u1"), ("skip_h", "u1"),
-    ("lfnst_idx", "u1"), ("lfnst_mode", "u1"), ("flags", "u1"), ("reserved", "u1")])
+    ("lfnst_idx", "u1"), ("lfnst_mode", "u1"), ("flags", "u1"), ("joint", "u1")])
 assert TU_DTYPE.itemsize == 16
 
 PU_DTYPE = np.dtype([("x", "<u2"), ("y", "<u2"), ("log2_w", "u1"), ("log2_h", "u1"),
@@ -34,6 +34,12 @@ BI_PU_DTYPE = np.dtype([("x", "<u2"), ("y", "<u2"), ("log2_w", "u1"), ("log2_h",
                         ("mv0_x", "<i4"), ("mv0_y", "<i4"), ("mv1_x", "<i4"), ("mv1_y", "<i4"),
                         ("bcw_idx", "u1"), ("reserved", "u1", (3,))])
 assert BI_PU_DTYPE.itemsize == 28
+
+EDGE_CELL_DTYPE = np.dtype([("tu_y", "<u4"), ("tu_c", "<u4"), ("tu_wy", "u1"), ("tu_hy", "u1"), ("tu_wc", "u1"),
+                            ("tu_hc", "u1"), ("flags", "u1"), ("reserved", "u1", (3,)), ("mv_x", "<i4"),
+                            ("mv_y", "<i4"), ("ref_poc", "<i4")])
+assert EDGE_CELL_DTYPE.itemsize == 28
+CELL_INTRA, CELL_CBF = 1, 2
 
 DCT2, DST7, DCT8 = 0, 1, 2
 TS_FLAG, DQ_FLAG = 1, 2
@@ -414,3 +420,93 @@ def pframe_step(seed, width=1920, height=1080, bit_depth=10, mv_range=64, qp=32,
     d = _inter_layout(rng, width, height, bit_depth, mv_range, qp, ctu)
     _add_residuals(rng, d, bit_depth, qp, p_skip, 0, tools)
     return d
+
+
+def joint_cbcr_frame(seed, width=256, height=128, bit_depth=10, p_joint=0.5):
+    """A frame of recon TUs where chroma blocks are either a separate Cb and
+    Cr TU or one joint Cb-Cr TU (mode 1..3, random sign; the coded component
+    is Cr for mode 1 and Cb otherwise, as reconstruct_joint_cbcr reads them).
+    Returns (tus, coeff, place, blocks) with blocks listing (plane, x, y, w, h,
+    joint) per TU; luma TUs are plain. Some TUs use transform skip."""
+    rng = np.random.default_rng([7, seed])
+    blocks = [(0, x, y, w, h) for (x, y, w, h) in _tile(rng, width, height, 64, 64, 4, _p_split_residual)]
+    joints = [0] * len(blocks)
+    cw, ch = (width + 1) // 2, (height + 1) // 2
+    for (x, y, w, h) in _tile(rng, cw, ch, 32, 32, 4, _p_split_residual):
+        if rng.random() < p_joint:
+            mode = int(rng.integers(1, 4))
+            code = mode | (4 if rng.random() < 0.5 else 0)
+            blocks.append((2 if mode == 1 else 1, x, y, w, h))
+            joints.append(code)
+        else:
+            blocks += [(1, x, y, w, h), (2, x, y, w, h)]
+            joints += [0, 0]
+    tus, coeff = _tu_batch(rng, blocks, bit_depth, (22, 37), with_ts=True)
+    tus["joint"] = joints
+    place = np.zeros(len(blocks), PLACE_DTYPE)
+    place["plane"] = [b[0] for b in blocks]
+    place["x"] = [b[1] for b in blocks]
+    place["y"] = [b[2] for b in blocks]
+    return tus, coeff, place, [b + (j,) for b, j in zip(blocks, joints)]
+
+
+def edge_cells(seed, width=640, height=368, p_intra=0.15, ctu=128):
+    """The cell grid (EDGE_CELL_DTYPE[gh, gw]) of a coded P-picture for the
+    edge-map builder: pframe_step's CUs and transform trees (a skipped CU is
+    one TU of its size, split at 64 luma / 32 chroma), a coded-block flag per
+    luma TU from its levels, intra CUs at random, and per inter CU its MV and
+    one of two reference POCs."""
+    # the layout is coded at the next multiple of 8; the grid covers the visible size
+    d = pframe_step(seed, width=(width + 7) // 8 * 8, height=(height + 7) // 8 * 8, bit_depth=10, tools=False, ctu=ctu)
+    rng = np.random.default_rng([8, seed])
+    gw, gh = (width + 3) // 4, (height + 3) // 4
+    cells = np.zeros((gh, gw), EDGE_CELL_DTYPE)
+    cells["tu_y"] = 0xffffffff
+    cells["tu_c"] = 0xffffffff
+    tu_id = 0
+    coded = {(b[1], b[2], b[0]) for b in d["blocks"]}
+    luma, chroma = [], []
+    for k, (x, y, w, h) in enumerate(d["cus"]):
+        intra = rng.random() < p_intra
+        poc = int(rng.integers(0, 2))
+        pu = d["pus"][k]
+        sl = (slice(y // 4, (y + h + 3) // 4), slice(x // 4, (x + w + 3) // 4))
+        cells["flags"][sl] = CELL_INTRA if intra else 0
+        cells["mv_x"][sl] = 0 if intra else int(pu["mv_x"])
+        cells["mv_y"][sl] = 0 if intra else int(pu["mv_y"])
+        cells["ref_poc"][sl] = 0 if intra else poc
+    for b in d["blocks"]:
+        (luma if b[0] == 0 else chroma if b[0] == 1 else []).append(b[1:])
+    # skipped CUs: one TU of the CU's size (at most 64 luma, 32 chroma)
+    covered = np.zeros(((int(d["height"]) + 3) // 4, (int(d["width"]) + 3) // 4), bool)
+    for (x, y, w, h) in luma:
+        covered[y // 4:(y + h + 3) // 4, x // 4:(x + w + 3) // 4] = True
+    for (x, y, w, h) in d["cus"]:
+        if covered[y // 4, x // 4]:
+            continue
+        for yy in range(y, y + h, 64):
+            for xx in range(x, x + w, 64):
+                luma.append((xx, yy, min(64, w), min(64, h)))
+        for yy in range(y // 2, (y + h) // 2, 32):
+            for xx in range(x // 2, (x + w) // 2, 32):
+                chroma.append((xx, yy, min(32, w // 2), min(32, h // 2)))
+    nz = {}
+    for t, b in zip(d["tus"], d["blocks"]):
+        if b[0] == 0:
+            n = 1 << (int(t["log2_w"]) + int(t["log2_h"]))
+            nz[b[1:]] = bool(d["coeff"][int(t["offset"]):int(t["offset"]) + n].any())
+    for (x, y, w, h) in luma:
+        sl = (slice(y // 4, min(gh, (y + h + 3) // 4)), slice(x // 4, min(gw, (x + w + 3) // 4)))
+        cells["tu_y"][sl] = tu_id
+        cells["tu_wy"][sl] = w
+        cells["tu_hy"][sl] = h
+        if nz.get((x, y, w, h), False):
+            cells["flags"][sl] |= CELL_CBF
+        tu_id += 1
+    for (x, y, w, h) in chroma:
+        sl = (slice(y // 2, min(gh, (y + h + 1) // 2)), slice(x // 2, min(gw, (x + w + 1) // 2)))
+        cells["tu_c"][sl] = tu_id
+        cells["tu_wc"][sl] = w
+        cells["tu_hc"][sl] = h
+        tu_id += 1
+    return cells
