@@ -143,7 +143,7 @@ rmh_status rmh_residual_plan_run(rmh_plan* plan, const int16_t* d_coeff, int16_t
 rmh_status rmh_residual_plan_run_range(rmh_plan* plan, size_t first_tu, size_t count,
                                        const int16_t* d_coeff, int16_t* d_residual, void* stream);
 
-/* Autotuner (no reference counterpart; the batching of the north star): for
+/* Autotuner (no reference counterpart; part of batching the frame step): for
  * each TU size class of the plan (largest first), times the plan's tiled
  * launch with that class's tiles holding 1, 1/2, 1/4 and 1/8 of the default
  * TUs per workgroup on the given buffers (the fastest of three batches of
@@ -428,6 +428,121 @@ rmh_status rmh_sao(rmh_ctx* ctx, const rmh_sao_frame* frames, size_t n_frames);
 rmh_status rmh_sao_plan_create(rmh_ctx* ctx, const rmh_sao_frame* frames, size_t n_frames,
                                rmh_plan** out);
 rmh_status rmh_sao_plan_run(rmh_plan* plan, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Luma mapping with chroma scaling (LMCS)                                   */
+/* ------------------------------------------------------------------------ */
+
+/* The LMCS APS, as VvcLmcsAps (crates/rustmpeg-video/src/vvc/hls.rs:2723-2732). */
+typedef struct rmh_lmcs_aps {
+    uint8_t min_bin_idx;              /* lmcs_min_bin_idx, 0..15 */
+    uint8_t max_bin_idx;              /* 15 - lmcs_delta_max_bin_idx */
+    int8_t chroma_res_scaling_offset; /* LmcsDeltaCrs */
+    uint8_t reserved;                 /* must be 0 */
+    int16_t bin_cw_delta[16];         /* LmcsDeltaCW[i] */
+} rmh_lmcs_aps;
+
+/* The tables of LmcsReshaper16::from_aps (vvc/filter.rs:2628-2716) at
+ * bit_depth 8..12: fwd_lut and inv_lut receive 2^bit_depth entries,
+ * chroma_adj_lut 16 and reshape_pivot 17; any output may be NULL. Host
+ * only; needs no context. */
+rmh_status rmh_lmcs_tables(const rmh_lmcs_aps* aps, uint32_t bit_depth, uint16_t* fwd_lut, uint16_t* inv_lut,
+                           int32_t* chroma_adj_lut, int32_t* reshape_pivot);
+
+enum { RMH_LMCS_INVERSE = 0, RMH_LMCS_FORWARD = 1 };
+
+/* One luma plane mapped sample by sample through the LUT of `aps`:
+ * apply_inverse_plane_16 (vvc/filter.rs:2715-2722; the decoder applies it to
+ * the whole luma plane after the slice, before deblocking,
+ * decode_general.rs:1134-1138) or apply_forward_plane_16 (filter.rs:2707-2713).
+ * A sample above 2^bd - 1 reads the last entry, as there. `dst` has the
+ * size of `src`; dst.data == src.data maps in place, any other overlap is
+ * rejected. */
+typedef struct rmh_lmcs_frame {
+    rmh_plane src;
+    rmh_plane dst;
+    rmh_lmcs_aps aps;
+    uint32_t direction;   /* RMH_LMCS_INVERSE or RMH_LMCS_FORWARD */
+} rmh_lmcs_frame;
+
+/* Device planes; the LUTs are built and uploaded at plan creation. */
+rmh_status rmh_lmcs_plan_create(rmh_ctx* ctx, const rmh_lmcs_frame* frames, size_t n_frames, rmh_plan** out);
+rmh_status rmh_lmcs_plan_run(rmh_plan* plan, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Adaptive loop filter (ALF) and cross-component ALF (CCALF)                */
+/* ------------------------------------------------------------------------ */
+
+/* Per-CTB switches, the decoder's alf_luma / alf_cb / alf_cr / ccalf_cb /
+ * ccalf_cr entries (crates/rustmpeg-video/src/vvc/decode_general.rs:1399-1515). */
+enum { RMH_ALF_LUMA_APS = 17 };
+typedef struct rmh_alf_ctb {
+    uint8_t luma;    /* 0 off; 1..16 fixed filter set luma - 1 (AlfLumaMode::Fixed);
+                      * RMH_ALF_LUMA_APS the luma APS (AlfLumaMode::Aps); other values off */
+    uint8_t cb;      /* 0 off; else chroma APS alternative cb - 1 (clamped to the last, as there) */
+    uint8_t cr;
+    uint8_t cc_cb;   /* CCALF filter idc: 0 off; 1..n_cc_cb; larger values off (decode_general.rs:5317) */
+    uint8_t cc_cr;
+    uint8_t reserved[3];
+} rmh_alf_ctb;
+
+/* The parts of an ALF APS (VvcAlfAps, vvc/hls.rs:2699-2718) that the filters
+ * read. Clip indices are the ones the decoder applies: luma_clip all zero when
+ * alf_luma_clip_flag is 0 (decode_general.rs:5466-5475); chroma_clip zero for
+ * an alternative without signalled indices (5588). */
+typedef struct rmh_alf_aps {
+    int16_t luma_coeff[25][12];
+    uint8_t luma_clip[25][12];      /* 2-bit clipping indices */
+    uint8_t class_to_filter[25];    /* filter_coeff_delta_idx */
+    uint8_t n_luma;                 /* luma_filters.len(), 1..25 when the APS serves luma */
+    uint8_t n_chroma;               /* chroma_filters.len(), 0..8 */
+    uint8_t n_cc_cb;                /* cc_cb_filters.len(), 0..4 */
+    uint8_t n_cc_cr;
+    uint8_t reserved;
+    int16_t chroma_coeff[8][6];
+    uint8_t chroma_clip[8][6];
+    int16_t cc_cb[4][7];
+    int16_t cc_cr[4][7];
+} rmh_alf_aps;
+
+/* One picture of apply_alf (decode_general.rs:5225-5333): per CTB the 7x7
+ * luma ALF with 4x4 classification (alf_luma_block, 5411-5566,
+ * derive_alf_class_4x4_vb_16, filter.rs:1558-1691), the 5x5 chroma ALF
+ * (alf_chroma_block, 5568-5660) and CCALF from the pre-ALF luma added to the
+ * chroma result (ccalf_block, 5335-5409), with the ALF virtual boundary 4
+ * luma rows above each CTB row. src is read only (the SAO output); dst
+ * receives every sample: filtered where the CTB's switches and the visible
+ * region say, src elsewhere. The APS pointers are host memory read at plan
+ * creation; NULL disables what that APS serves (a CTB asking for the luma APS
+ * without one is left unfiltered, where the reference would stop). */
+typedef struct rmh_alf_frame {
+    rmh_plane src[3];
+    rmh_plane dst[3];                /* same sizes as src; must not overlap src */
+    const rmh_alf_ctb* ctbs;         /* device: ctb_cols * ctb_rows entries, row-major */
+    const rmh_alf_aps* luma_aps;     /* the slice's first luma APS (alf_aps_ids_luma[0]) */
+    const rmh_alf_aps* chroma_aps;   /* alf_aps_id_chroma */
+    const rmh_alf_aps* cc_cb_aps;    /* cc_alf_cb_aps_id */
+    const rmh_alf_aps* cc_cr_aps;    /* cc_alf_cr_aps_id */
+    uint32_t ctb_size;               /* luma CTB size: 16, 32, 64 or 128 */
+    uint32_t ctb_cols;               /* >= ceil(vis_w / ctb_size) */
+    uint32_t ctb_rows;               /* >= ceil(vis_h / ctb_size) */
+    uint32_t vis_w;                  /* visible luma size (<= the Y plane's) */
+    uint32_t vis_h;
+    uint32_t reserved;
+} rmh_alf_frame;
+
+/* The luma filter bank a plan builds from `aps` (NULL: none) at bit_depth
+ * 8..12 and the kernels read: out[((bank * 25 + class) * 4 + transpose) * 24
+ * + k], banks 0..15 the fixed filter sets and 16 the APS, k < 12 the
+ * coefficient filtering tap k after the block's transposition
+ * (ALF_TRANSPOSE_PERM, vvc/filter.rs:2036-2041) and k >= 12 that tap's clip
+ * value. 17 * 25 * 4 * 24 entries; host only, needs no context. */
+rmh_status rmh_alf_filter_bank(const rmh_alf_aps* aps, uint32_t bit_depth, int16_t* out);
+
+/* Device planes and CTB switches; the filter banks are built at creation.
+ * APS coefficients must lie in [-128, 128] (ALF) and [-64, 64] (CCALF). */
+rmh_status rmh_alf_plan_create(rmh_ctx* ctx, const rmh_alf_frame* frames, size_t n_frames, rmh_plan** out);
+rmh_status rmh_alf_plan_run(rmh_plan* plan, void* stream);
 
 /* ------------------------------------------------------------------------ */
 /* Plans and frame-step graphs                                               */

@@ -92,6 +92,13 @@ def _setup(L):
     L.orc_sao_region_bounded16.argtypes = [ctypes.c_void_p, ctypes.c_void_p] + [c_int] * 6 + [ctypes.c_void_p]
     L.orc_sao_region_bounded8.argtypes = [ctypes.c_void_p, ctypes.c_void_p] + [c_int] * 6 + [ctypes.c_void_p]
     L.orc_sao_frame.argtypes = [ctypes.c_void_p]
+    L.orc_lmcs_tables.argtypes = [ctypes.c_void_p, c_int, U16P, U16P, I32P, I32P]
+    L.orc_lmcs_map.argtypes = [ctypes.c_void_p, ctypes.c_void_p, U16P, c_int]
+    L.orc_lmcs_scale_chroma.argtypes = [ctypes.c_void_p, c_int, c_int, I32P, c_int]
+    L.orc_alf_classify.argtypes = [ctypes.c_void_p] + [c_int] * 6 + [ctypes.POINTER(c_int)]
+    L.orc_alf_frame.argtypes = [ctypes.c_void_p]
+    L.orc_alf_luma_filter.argtypes = [c_int] * 3 + [ctypes.c_void_p, c_int, I32P, I32P]
+    L.orc_alf_luma_filter.restype = None
 
 
 def _p(a, t):
@@ -385,3 +392,64 @@ def sao_frame(planes, bd, params, ctb_size, ctb_cols, ctb_rows, vis_w, vis_h):
     if lib().orc_sao_frame(ctypes.addressof(f)):
         raise ValueError("invalid SAO parameters")
     return outs
+
+
+def lmcs_tables(aps, bd):
+    """LmcsReshaper16::from_aps: (fwd_lut, inv_lut, chroma_adj_lut, reshape_pivot) for an abi.LmcsAps."""
+    n = 1 << min(max(bd, 8), 12)
+    fwd, inv = np.zeros(n, np.uint16), np.zeros(n, np.uint16)
+    cadj, piv = np.zeros(16, np.int32), np.zeros(17, np.int32)
+    lib().orc_lmcs_tables(ctypes.addressof(aps), bd, _p(fwd, U16P), _p(inv, U16P), _p(cadj, I32P), _p(piv, I32P))
+    return fwd, inv, cadj, piv
+
+
+def lmcs_map(plane, lut, bd):
+    """apply_{forward,inverse}_plane_16 of a uint8/uint16 plane: returns the mapped copy."""
+    A = _abi()
+    src = np.ascontiguousarray(plane)
+    out = np.zeros_like(src)
+    lut = np.ascontiguousarray(lut, dtype=np.uint16)
+    ps, po = A.plane_from_array(src, bd), A.plane_from_array(out, bd)
+    lib().orc_lmcs_map(ctypes.byref(ps), ctypes.byref(po), _p(lut, U16P), bd)
+    return out
+
+
+def lmcs_scale_chroma(aps, bd, avg_luma, residual):
+    """chroma_scale_for_luma + scale_chroma_residual: (scaled copy, scale)."""
+    r = np.ascontiguousarray(residual, dtype=np.int32).copy()
+    scale = lib().orc_lmcs_scale_chroma(ctypes.addressof(aps), bd, avg_luma, _p(r, I32P), r.size)
+    return r, scale
+
+
+def alf_classify(plane, bd, bx, by, bw, bh, vb_h, vb_pos):
+    """derive_alf_class_4x4_vb_16 of the 4x4 block at (bx, by): (class, transpose)."""
+    A = _abi()
+    src = np.ascontiguousarray(plane)
+    p = A.plane_from_array(src, bd)
+    t = ctypes.c_int(0)
+    c = lib().orc_alf_classify(ctypes.byref(p), bx, by, bw, bh, vb_h, vb_pos, ctypes.byref(t))
+    return c, t.value
+
+
+def alf_frame(planes, bd, ctbs, ctb_size, ctb_cols, ctb_rows, vis_w, vis_h, luma_aps=None, chroma_aps=None,
+              cc_cb_aps=None, cc_cr_aps=None):
+    """apply_alf over one picture: planes [Y, Cb, Cr] (not modified), ctbs an
+    ALF_CTB_DTYPE array of ctb_rows * ctb_cols records, APS arguments abi.AlfAps.
+    Returns the three output planes."""
+    A = _abi()
+    planes = [np.ascontiguousarray(p) for p in planes]
+    outs = [np.zeros_like(p) for p in planes]
+    ctbs = np.ascontiguousarray(ctbs, dtype=A.ALF_CTB_DTYPE)
+    f = A.alf_frame([A.plane_from_array(p, bd) for p in planes], [A.plane_from_array(o, bd) for o in outs],
+                    ctbs.ctypes.data, ctb_size, ctb_cols, ctb_rows, vis_w, vis_h, luma_aps, chroma_aps, cc_cb_aps,
+                    cc_cr_aps)
+    lib().orc_alf_frame(ctypes.addressof(f))
+    return outs
+
+
+def alf_luma_filter(mode, class_idx, transpose, aps, bd):
+    """(coefficients, clip values) alf_luma_block applies to a block (mode 1..16 fixed, 17 APS)."""
+    fc, cl = np.zeros(12, np.int32), np.zeros(12, np.int32)
+    lib().orc_alf_luma_filter(mode, class_idx, transpose, ctypes.addressof(aps) if aps is not None else None, bd,
+                              _p(fc, I32P), _p(cl, I32P))
+    return fc, cl

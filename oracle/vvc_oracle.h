@@ -140,6 +140,18 @@ int orc_sao_region_bounded8(const rmh_plane* src, rmh_plane* dst, int bx, int by
                             const rmh_sao_param* sao);
 int orc_sao_frame(const rmh_sao_frame* f);
 
+/* ---- LMCS and ALF (vvc/filter.rs:1580-1691, 2628-2750; decode_general.rs:5225-5660) ---- */
+int orc_lmcs_tables(const rmh_lmcs_aps* aps, int bit_depth, uint16_t* fwd_lut, uint16_t* inv_lut,
+                    int32_t* chroma_adj_lut, int32_t* reshape_pivot);
+int orc_lmcs_map(const rmh_plane* src, const rmh_plane* dst, const uint16_t* lut, int bit_depth);
+/* Scales residual[0..n) in place; returns the scale (chroma_adj_lut entry) used. */
+int orc_lmcs_scale_chroma(const rmh_lmcs_aps* aps, int bit_depth, int avg_luma, int32_t* residual, int n);
+/* Class index 0..24 of the 4x4 block at (bx, by); *transpose gets 0..3. */
+int orc_alf_classify(const rmh_plane* plane, int bx, int by, int bw, int bh, int vb_h, int vb_pos, int* transpose);
+void orc_alf_luma_filter(int mode, int class_idx, int transpose, const rmh_alf_aps* aps, int bit_depth, int32_t* fc,
+                         int32_t* fclip);
+int orc_alf_frame(const rmh_alf_frame* f);
+
 #ifdef __cplusplus
 }
 #endif

@@ -73,7 +73,8 @@ EXPORTED = [
     "rmh_cnn_destroy", "rmh_cnn_get_params", "rmh_cnn_set_params", "rmh_cnn_forward", "rmh_cnn_predict",
     "rmh_cnn_gradients", "rmh_cnn_train_step", "rmh_cnn_train_step_soft", "rmh_residual_plan_autotune",
     "rmh_mc_bi_batch", "rmh_mc_bi_plan_create", "rmh_deblock_noalloc", "rmh_intra_predict",
-    "rmh_edge_map", "rmh_edge_map_plan_create", "rmh_edge_map_plan_run",
+    "rmh_edge_map", "rmh_edge_map_plan_create", "rmh_edge_map_plan_run", "rmh_lmcs_tables",
+    "rmh_lmcs_plan_create", "rmh_lmcs_plan_run", "rmh_alf_filter_bank", "rmh_alf_plan_create", "rmh_alf_plan_run",
 ]
 
 
@@ -113,6 +114,12 @@ def load_library(path=LIB_PATH):
         "rmh_edge_map": ([vp, vp, sz], st),
         "rmh_edge_map_plan_create": ([vp, vp, sz, ctypes.POINTER(vp)], st),
         "rmh_edge_map_plan_run": ([vp, vp], st),
+        "rmh_lmcs_tables": ([vp, ctypes.c_uint32, vp, vp, vp, vp], st),
+        "rmh_lmcs_plan_create": ([vp, vp, sz, ctypes.POINTER(vp)], st),
+        "rmh_lmcs_plan_run": ([vp, vp], st),
+        "rmh_alf_filter_bank": ([vp, ctypes.c_uint32, vp], st),
+        "rmh_alf_plan_create": ([vp, vp, sz, ctypes.POINTER(vp)], st),
+        "rmh_alf_plan_run": ([vp, vp], st),
         "rmh_sao": ([vp, vp, sz], st),
         "rmh_sao_plan_create": ([vp, vp, sz, ctypes.POINTER(vp)], st),
         "rmh_sao_plan_run": ([vp, vp], st),
@@ -172,6 +179,30 @@ def rd_encode_luma(src, qp, mode=abi.RD_FULL, probs=None, p_prune=0.05, p_fallba
     return recon, splits.reshape(h // 32, w // 32), (costs.reshape(nb, 6) if costs is not None else None), stats
 
 
+def lmcs_tables(aps, bit_depth):
+    """LmcsReshaper16::from_aps (rmh_lmcs_tables) for an abi.LmcsAps: (fwd_lut,
+    inv_lut, chroma_adj_lut, reshape_pivot)."""
+    n = 1 << bit_depth
+    fwd, inv = np.zeros(n, np.uint16), np.zeros(n, np.uint16)
+    cadj, piv = np.zeros(16, np.int32), np.zeros(17, np.int32)
+    st = load_library().rmh_lmcs_tables(ctypes.addressof(aps), bit_depth, fwd.ctypes.data, inv.ctypes.data,
+                                        cadj.ctypes.data, piv.ctypes.data)
+    if st != abi.RMH_OK:
+        raise _ERRORS.get(st, AcceleratorError)("rmh_lmcs_tables", st)
+    return fwd, inv, cadj, piv
+
+
+def alf_filter_bank(aps, bit_depth):
+    """The luma filter bank (rmh_alf_filter_bank) of an abi.AlfAps or None:
+    int16 (17 banks, 25 classes, 4 transpositions, 12 coefficients + 12 clips)."""
+    out = np.zeros((17, 25, 4, 24), np.int16)
+    st = load_library().rmh_alf_filter_bank(ctypes.addressof(aps) if aps is not None else None, bit_depth,
+                                            out.ctypes.data)
+    if st != abi.RMH_OK:
+        raise _ERRORS.get(st, AcceleratorError)("rmh_alf_filter_bank", st)
+    return out
+
+
 def intra_predict(plane, bx, by, n, is_luma, mode):
     """The host PLANAR / DC predictor of the intra-subset decode (rmh_intra_predict)
     for the n x n block at (bx, by) of a C-contiguous uint8 plane."""
@@ -212,6 +243,10 @@ class Plan:
             self.acc._check(L.rmh_sao_plan_run(self.handle, stream))
         elif self.kind == "edge":
             self.acc._check(L.rmh_edge_map_plan_run(self.handle, stream))
+        elif self.kind == "lmcs":
+            self.acc._check(L.rmh_lmcs_plan_run(self.handle, stream))
+        elif self.kind == "alf":
+            self.acc._check(L.rmh_alf_plan_run(self.handle, stream))
         else:
             self.acc._check(L.rmh_deblock_plan_run(self.handle, stream))
 
@@ -447,6 +482,23 @@ class Accelerator:
         self._check(load_library().rmh_sao_plan_create(self.handle, ctypes.addressof(arr), len(frames),
                                                        ctypes.byref(h)))
         return Plan(self, h.value, "sao")
+
+    def lmcs_plan(self, frames):
+        """LMCS luma-mapping plan over abi.LmcsFrame entries with device planes; run with plan.run()."""
+        arr = (abi.LmcsFrame * len(frames))(*frames)
+        h = ctypes.c_void_p()
+        self._check(load_library().rmh_lmcs_plan_create(self.handle, ctypes.addressof(arr), len(frames),
+                                                        ctypes.byref(h)))
+        return Plan(self, h.value, "lmcs")
+
+    def alf_plan(self, frames):
+        """ALF / CCALF plan over abi.AlfFrame entries (device planes and CTB switches,
+        host APS records); run with plan.run()."""
+        arr = (abi.AlfFrame * len(frames))(*frames)
+        h = ctypes.c_void_p()
+        self._check(load_library().rmh_alf_plan_create(self.handle, ctypes.addressof(arr), len(frames),
+                                                       ctypes.byref(h)))
+        return Plan(self, h.value, "alf")
 
     def graph(self, nodes):
         """nodes: list of (plan, d_coeff, d_residual) (device addresses; 0 for non-residual plans)."""

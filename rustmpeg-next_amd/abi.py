@@ -97,6 +97,91 @@ def sao_frame(srcs, dsts, params_addr, ctb_size, ctb_cols, ctb_rows, vis_w, vis_
     return f
 
 
+class LmcsAps(ctypes.Structure):
+    """rmh_lmcs_aps: VvcLmcsAps (vvc/hls.rs:2723-2732)."""
+    _fields_ = [("min_bin_idx", ctypes.c_uint8), ("max_bin_idx", ctypes.c_uint8),
+                ("chroma_res_scaling_offset", ctypes.c_int8), ("reserved", ctypes.c_uint8),
+                ("bin_cw_delta", ctypes.c_int16 * 16)]
+
+
+def lmcs_aps(min_bin_idx, max_bin_idx, bin_cw_delta, chroma_res_scaling_offset=0):
+    return LmcsAps(min_bin_idx, max_bin_idx, chroma_res_scaling_offset, 0, (ctypes.c_int16 * 16)(*bin_cw_delta))
+
+
+LMCS_INVERSE, LMCS_FORWARD = 0, 1
+
+
+class LmcsFrame(ctypes.Structure):
+    _fields_ = [("src", Plane), ("dst", Plane), ("aps", LmcsAps), ("direction", ctypes.c_uint32)]
+
+
+class AlfAps(ctypes.Structure):
+    """rmh_alf_aps: the filter data of VvcAlfAps (vvc/hls.rs:2699-2718) as the decoder applies it."""
+    _fields_ = [("luma_coeff", (ctypes.c_int16 * 12) * 25), ("luma_clip", (ctypes.c_uint8 * 12) * 25),
+                ("class_to_filter", ctypes.c_uint8 * 25), ("n_luma", ctypes.c_uint8), ("n_chroma", ctypes.c_uint8),
+                ("n_cc_cb", ctypes.c_uint8), ("n_cc_cr", ctypes.c_uint8), ("reserved", ctypes.c_uint8),
+                ("chroma_coeff", (ctypes.c_int16 * 6) * 8), ("chroma_clip", (ctypes.c_uint8 * 6) * 8),
+                ("cc_cb", (ctypes.c_int16 * 7) * 4), ("cc_cr", (ctypes.c_int16 * 7) * 4)]
+
+
+assert ctypes.sizeof(AlfAps) == 1186
+
+
+def alf_aps(luma_filters=(), luma_clip_idx=(), luma_clip=False, filter_coeff_delta_idx=None, chroma_filters=(),
+            chroma_clip_idx=(), cc_cb_filters=(), cc_cr_filters=()):
+    """An AlfAps from VvcAlfAps fields, with the clip indices the decoder reads:
+    luma ones only under alf_luma_clip_flag (decode_general.rs:5466-5475), chroma
+    ones where signalled, else zero (5588)."""
+    a = AlfAps()
+    if len(luma_filters) > 25 or len(chroma_filters) > 8 or len(cc_cb_filters) > 4 or len(cc_cr_filters) > 4:
+        raise ValueError("too many filters")
+    for i, f in enumerate(luma_filters):
+        a.luma_coeff[i][:] = list(f)
+        if luma_clip and i < len(luma_clip_idx):
+            a.luma_clip[i][:] = list(luma_clip_idx[i])
+    a.class_to_filter[:] = list(filter_coeff_delta_idx if filter_coeff_delta_idx is not None else [0] * 25)
+    a.n_luma, a.n_chroma = len(luma_filters), len(chroma_filters)
+    for i, f in enumerate(chroma_filters):
+        a.chroma_coeff[i][:] = list(f)
+        if i < len(chroma_clip_idx):
+            a.chroma_clip[i][:] = list(chroma_clip_idx[i])
+    a.n_cc_cb, a.n_cc_cr = len(cc_cb_filters), len(cc_cr_filters)
+    for i, f in enumerate(cc_cb_filters):
+        a.cc_cb[i][:] = list(f)
+    for i, f in enumerate(cc_cr_filters):
+        a.cc_cr[i][:] = list(f)
+    return a
+
+
+# rmh_alf_ctb (8 bytes): luma 0 off / 1..16 fixed set + 1 / 17 APS; cb, cr alternative + 1; CCALF idc
+ALF_CTB_DTYPE = np.dtype([("luma", "u1"), ("cb", "u1"), ("cr", "u1"), ("cc_cb", "u1"), ("cc_cr", "u1"),
+                          ("reserved", "u1", (3,))])
+assert ALF_CTB_DTYPE.itemsize == 8
+ALF_LUMA_APS = 17
+
+
+class AlfFrame(ctypes.Structure):
+    _fields_ = [("src", Plane * 3), ("dst", Plane * 3), ("ctbs", ctypes.c_void_p), ("luma_aps", ctypes.c_void_p),
+                ("chroma_aps", ctypes.c_void_p), ("cc_cb_aps", ctypes.c_void_p), ("cc_cr_aps", ctypes.c_void_p),
+                ("ctb_size", ctypes.c_uint32), ("ctb_cols", ctypes.c_uint32), ("ctb_rows", ctypes.c_uint32),
+                ("vis_w", ctypes.c_uint32), ("vis_h", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+
+
+def alf_frame(srcs, dsts, ctbs_addr, ctb_size, ctb_cols, ctb_rows, vis_w, vis_h, luma_aps=None, chroma_aps=None,
+              cc_cb_aps=None, cc_cr_aps=None):
+    """An AlfFrame; the AlfAps arguments must outlive the call that reads them."""
+    f = AlfFrame()
+    for i in range(3):
+        f.src[i] = srcs[i]
+        f.dst[i] = dsts[i]
+    f.ctbs = ctbs_addr
+    for name, aps in (("luma_aps", luma_aps), ("chroma_aps", chroma_aps), ("cc_cb_aps", cc_cb_aps),
+                      ("cc_cr_aps", cc_cr_aps)):
+        setattr(f, name, ctypes.addressof(aps) if aps is not None else None)
+    f.ctb_size, f.ctb_cols, f.ctb_rows, f.vis_w, f.vis_h = ctb_size, ctb_cols, ctb_rows, vis_w, vis_h
+    return f
+
+
 class GraphNode(ctypes.Structure):
     _fields_ = [("plan", ctypes.c_void_p), ("d_coeff", ctypes.c_void_p), ("d_residual", ctypes.c_void_p)]
 

@@ -7,7 +7,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "librmh.so")
-SOURCES = ["rmh.cpp", "intra_subset.cpp", "rdsearch.cpp", "residual.hip", "mc.hip", "deblock.hip", "sao.hip", "cnn.hip"]
+SOURCES = ["rmh.cpp", "intra_subset.cpp", "rdsearch.cpp", "loopfilter.cpp", "residual.hip", "mc.hip", "deblock.hip", "sao.hip",
+           "cnn.hip", "loopfilter.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [
@@ -48,7 +49,7 @@ def build(verbose=False, incremental=False):
     compile_flags = [f for f in FLAGS if f != "-shared"]
     jobs = []
     for s in SOURCES:
-        obj = os.path.join(objdir, os.path.splitext(s)[0] + ".o")
+        obj = os.path.join(objdir, s.replace(".", "_") + ".o")  # loopfilter.cpp and loopfilter.hip both build
         jobs.append(([HIPCC] + compile_flags + ["-c", os.path.join(CSRC, s), "-o", obj], obj))
     todo = [cmd for cmd, obj in jobs if not incremental or _stale(cmd[-3], obj)]
     workers = max(1, min(len(jobs), os.cpu_count() or 1, 16))

@@ -198,4 +198,71 @@ constexpr int kSaoBlocksPerWave = 1;
 // 16 samples on a side.
 hipError_t launch_sao(const SaoJob* jobs, const uint32_t* block_job, const uint32_t (&range)[5], hipStream_t stream);
 
+// One luma plane of an LMCS batch: samples mapped through lut[min(s, cap)].
+struct LmcsJob {
+    const void* src;
+    void* dst;
+    const uint16_t* lut;        // device, cap + 1 entries
+    uint32_t width, height;
+    uint32_t src_stride, dst_stride;  // samples
+    uint32_t bps;               // bytes per sample
+    uint32_t cap;               // 2^bd - 1
+    uint32_t first_block;       // first workgroup of the job
+    uint32_t reserved;
+};
+// Rows of a plane per LMCS workgroup. Workgroups [range[0], range[1]) map
+// 8-bit planes, [range[1], range[2]) 16-bit ones.
+constexpr int kLmcsRows = 8;
+hipError_t launch_lmcs(const LmcsJob* jobs, const uint32_t* block_job, const uint32_t (&range)[3], hipStream_t stream);
+
+// ALF luma tiles are 64 x 16 samples, one 256-lane workgroup each; chroma
+// tiles 64 x 16 chroma samples.
+constexpr int kAlfTileW = 64, kAlfTileH = 16;
+// Filter bank of one luma APS and the 16 fixed sets: [bank 0..16][class
+// 0..24][transpose 0..3][coefficient 0..11, clip value 0..11], with the
+// transposition already applied (bank 16 is the APS).
+constexpr int kAlfBankEntries = 17 * 25 * 4 * 24;
+
+struct AlfLumaJob {
+    const void* src;
+    void* dst;
+    const rmh_alf_ctb* ctbs;
+    const int16_t* bank;          // kAlfBankEntries
+    uint32_t src_stride, dst_stride;
+    uint32_t width, height;       // plane
+    uint32_t vis_w, vis_h;
+    uint32_t bw, bh;              // min(visible, plane): the clamp of every read
+    uint32_t bps, bd;
+    uint32_t ctb_log2, ctb_cols, ctb_xn, ctb_yn;  // CTBs of the loop: ceil(vis / ctb)
+    uint32_t has_aps;
+    uint32_t tiles_x, first_block;
+    uint32_t reserved;
+};
+
+struct AlfChromaJob {
+    const void* src;
+    void* dst;
+    const void* luma;             // pre-ALF luma (CCALF input)
+    const rmh_alf_ctb* ctbs;
+    const int16_t* chroma_bank;   // [alternative][6 coefficients, 6 clip values]; null: chroma ALF off
+    const int16_t* cc_bank;       // [filter][7]; null: CCALF off
+    uint32_t src_stride, dst_stride, luma_stride;
+    uint32_t width, height;       // chroma plane
+    uint32_t cbw, cbh;            // clamp of chroma reads: min(ceil(vis / 2), plane)
+    uint32_t lw, lh;              // clamp of luma reads: min(vis, luma plane)
+    uint32_t cvis_w, cvis_h, vis_w, vis_h;
+    uint32_t bps, bd;
+    uint32_t ctb_log2, ctb_cols, ctb_xn, ctb_yn;
+    uint32_t comp;                // 1 Cb, 2 Cr
+    uint32_t n_chroma, n_cc;
+    uint32_t tiles_x, first_block;
+    uint32_t reserved;
+};
+
+// Workgroups [range[0], range[1]) of each kind serve 8-bit planes,
+// [range[1], range[2]) 16-bit ones.
+hipError_t launch_alf(const AlfLumaJob* luma, const uint32_t* luma_block_job, const uint32_t (&luma_range)[3],
+                      const AlfChromaJob* chroma, const uint32_t* chroma_block_job, const uint32_t (&chroma_range)[3],
+                      hipStream_t stream);
+
 }  // namespace rmh

@@ -24,6 +24,7 @@
 #include "cnn.h"
 #include "kernels.h"
 #include "lfnst_data.h"
+#include "loopfilter.h"
 #include "tables.h"
 
 #define RMH_EXPORT extern "C" __attribute__((visibility("default")))
@@ -149,7 +150,7 @@ struct rmh_ctx {
     void* scratch = nullptr;  // this device's address of `scratch`
 };
 
-enum PlanKind { kResidual = 0, kMc = 1, kDeblock = 2, kSao = 3, kEdge = 4 };
+enum PlanKind { kResidual = 0, kMc = 1, kDeblock = 2, kSao = 3, kEdge = 4, kLmcs = 5, kAlf = 6 };
 
 struct rmh_plan {
     rmh_ctx* ctx = nullptr;
@@ -206,6 +207,16 @@ struct rmh_plan {
     const rmh::SaoJob* d_sao_jobs = nullptr;
     const uint32_t* d_sao_block_job = nullptr;
     uint32_t n_sao_jobs = 0, sao_range[5] = {};
+    // lmcs: workgroup ranges [0, 1) 8-bit, [1, 2) 16-bit planes
+    const rmh::LmcsJob* d_lmcs_jobs = nullptr;
+    const uint32_t* d_lmcs_block_job = nullptr;
+    uint32_t lmcs_range[3] = {};
+    // alf
+    const rmh::AlfLumaJob* d_alf_luma = nullptr;
+    const rmh::AlfChromaJob* d_alf_chroma = nullptr;
+    const uint32_t* d_alf_luma_bj = nullptr;
+    const uint32_t* d_alf_chroma_bj = nullptr;
+    uint32_t alf_luma_range[3] = {}, alf_chroma_range[3] = {};
 };
 
 struct rmh_graph {
@@ -672,6 +683,10 @@ rmh_status validate_dbk(rmh_ctx* ctx, const rmh_dbk_frame* frames, size_t n_fram
 
 // The SAO kernel reads a parameter record as 5 dwords.
 static_assert(sizeof(rmh_sao_param) == 20 && offsetof(rmh_sao_param, offsets) == 4, "rmh_sao_param layout");
+static_assert(sizeof(rmh_lmcs_aps) == 36 && sizeof(rmh_lmcs_frame) == 88, "rmh_lmcs_* layout");
+static_assert(sizeof(rmh_alf_ctb) == 8 && sizeof(rmh_alf_aps) == 1186 && offsetof(rmh_alf_aps, chroma_coeff) == 930 &&
+                  sizeof(rmh_alf_frame) == 208,
+              "rmh_alf_* layout");
 
 // SAO frames: matching src/dst planes of one bit depth, 4:2:0 chroma, a CTB
 // grid that covers the visible region, src and dst not overlapping. With
@@ -1204,6 +1219,11 @@ rmh_status run_plan_locked_free(rmh_plan* plan, const int16_t* d_coeff, int16_t*
         e = rmh::launch_sao(plan->d_sao_jobs, plan->d_sao_block_job, plan->sao_range, s);
     } else if (plan->kind == kEdge) {
         e = rmh::launch_edge_map(plan->d_edge_frames, plan->d_cell_start, (uint32_t)plan->n_edge, plan->total_cells, s);
+    } else if (plan->kind == kLmcs) {
+        e = rmh::launch_lmcs(plan->d_lmcs_jobs, plan->d_lmcs_block_job, plan->lmcs_range, s);
+    } else if (plan->kind == kAlf) {
+        e = rmh::launch_alf(plan->d_alf_luma, plan->d_alf_luma_bj, plan->alf_luma_range, plan->d_alf_chroma,
+                            plan->d_alf_chroma_bj, plan->alf_chroma_range, s);
     } else {
         e = rmh::launch_deblock(plan->d_dbk, plan->d_shadows, plan->d_tile_start, plan->d_frame_of, (uint32_t)plan->n_dbk,
                                 plan->total_tiles, plan->tiles8, plan->max_rows, plan->dbk_copy, s);
@@ -2365,6 +2385,349 @@ RMH_EXPORT rmh_status rmh_sao(rmh_ctx* ctx, const rmh_sao_frame* frames, size_t 
     return RMH_OK;
 }
 
+// ------------------------------------------------------------- lmcs, alf
+
+namespace {
+
+// Device blob of a loop-filter plan: parts appended at 256-byte offsets.
+struct Blob {
+    std::vector<unsigned char> bytes;
+    size_t add(const void* p, size_t n) {
+        const size_t off = aligned(bytes.size());
+        bytes.resize(off + n);
+        if (n) memcpy(bytes.data() + off, p, n);
+        return off;
+    }
+};
+
+// Uploads the blob into the plan's device memory; false on failure.
+bool upload_blob(rmh_plan* p, const Blob& b, std::string* err) {
+    if (b.bytes.empty()) return true;
+    hipError_t e = hipMalloc(&p->d_mem, b.bytes.size());
+    if (e == hipSuccess) e = hipMemcpy(p->d_mem, b.bytes.data(), b.bytes.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        *err = hipGetErrorString(e);
+        if (p->d_mem) hipFree(p->d_mem);
+        p->d_mem = nullptr;
+        return false;
+    }
+    return true;
+}
+
+bool same_shape(const rmh_plane& a, const rmh_plane& b) {
+    return a.width == b.width && a.height == b.height && a.bytes_per_sample == b.bytes_per_sample &&
+           a.bit_depth == b.bit_depth;
+}
+
+rmh_status validate_lmcs(rmh_ctx* ctx, const rmh_lmcs_frame* frames, size_t n_frames) {
+    if (n_frames && !frames) return fail(ctx, RMH_INVALID_INPUT, "null frame array");
+    if (n_frames > 65535) return fail(ctx, RMH_LIMIT_EXCEEDED, "more than 65535 frames");
+    std::vector<PlaneSpan> spans;
+    for (size_t f = 0; f < n_frames; f++) {
+        const rmh_lmcs_frame& d = frames[f];
+        const std::string tag = "frame " + std::to_string(f) + ": ";
+        if (!plane_ok(d.src) || !plane_ok(d.dst) || !same_shape(d.src, d.dst))
+            return fail(ctx, RMH_INVALID_INPUT, tag + "invalid or mismatched src/dst plane");
+        if (d.direction > RMH_LMCS_FORWARD || d.aps.reserved) return fail(ctx, RMH_INVALID_INPUT, tag + "direction or APS");
+        if (d.src.data == d.dst.data) {
+            if (d.src.stride != d.dst.stride) return fail(ctx, RMH_INVALID_INPUT, tag + "in place with two strides");
+        } else {
+            add_span(spans, d.src, false);
+        }
+        add_span(spans, d.dst, true);
+    }
+    if (written_overlap(spans)) return fail(ctx, RMH_INVALID_INPUT, "a dst plane overlaps another plane of the batch");
+    return RMH_OK;
+}
+
+bool alf_aps_ok(const rmh_alf_aps* a, bool luma) {
+    if (!a) return true;
+    if (a->n_chroma > 8 || a->n_cc_cb > 4 || a->n_cc_cr > 4 || a->n_luma > 25 || (luma && a->n_luma == 0)) return false;
+    for (int f = 0; f < 25; f++)
+        for (int k = 0; k < 12; k++)
+            if (a->luma_coeff[f][k] < -128 || a->luma_coeff[f][k] > 128) return false;
+    for (int f = 0; f < 8; f++)
+        for (int k = 0; k < 6; k++)
+            if (a->chroma_coeff[f][k] < -128 || a->chroma_coeff[f][k] > 128) return false;
+    for (int f = 0; f < 4; f++)
+        for (int k = 0; k < 7; k++)
+            if (a->cc_cb[f][k] < -64 || a->cc_cb[f][k] > 64 || a->cc_cr[f][k] < -64 || a->cc_cr[f][k] > 64) return false;
+    return true;
+}
+
+rmh_status validate_alf(rmh_ctx* ctx, const rmh_alf_frame* frames, size_t n_frames) {
+    if (n_frames && !frames) return fail(ctx, RMH_INVALID_INPUT, "null frame array");
+    if (n_frames > 65535) return fail(ctx, RMH_LIMIT_EXCEEDED, "more than 65535 frames");
+    std::vector<PlaneSpan> spans;
+    for (size_t f = 0; f < n_frames; f++) {
+        const rmh_alf_frame& d = frames[f];
+        const std::string tag = "frame " + std::to_string(f) + ": ";
+        for (int c = 0; c < 3; c++) {
+            if (!plane_ok(d.src[c]) || !plane_ok(d.dst[c]) || !same_shape(d.src[c], d.dst[c]) ||
+                d.src[c].bit_depth != d.src[0].bit_depth || d.src[c].bytes_per_sample != d.src[0].bytes_per_sample)
+                return fail(ctx, RMH_INVALID_INPUT, tag + "invalid or mismatched src/dst plane");
+            add_span(spans, d.src[c], false);
+            add_span(spans, d.dst[c], true);
+        }
+        if (d.src[1].width != (d.src[0].width + 1) / 2 || d.src[1].height != (d.src[0].height + 1) / 2 ||
+            d.src[2].width != d.src[1].width || d.src[2].height != d.src[1].height)
+            return fail(ctx, RMH_INVALID_INPUT, tag + "chroma planes are not 4:2:0");
+        if (d.ctb_size != 16 && d.ctb_size != 32 && d.ctb_size != 64 && d.ctb_size != 128)
+            return fail(ctx, RMH_INVALID_INPUT, tag + "CTB size");
+        if (d.vis_w == 0 || d.vis_h == 0 || d.vis_w > d.src[0].width || d.vis_h > d.src[0].height || d.reserved)
+            return fail(ctx, RMH_INVALID_INPUT, tag + "visible size");
+        if (!d.ctbs || d.ctb_cols < (d.vis_w + d.ctb_size - 1) / d.ctb_size ||
+            d.ctb_rows < (d.vis_h + d.ctb_size - 1) / d.ctb_size)
+            return fail(ctx, RMH_INVALID_INPUT, tag + "CTB switch grid");
+        if (!alf_aps_ok(d.luma_aps, true) || !alf_aps_ok(d.chroma_aps, false) || !alf_aps_ok(d.cc_cb_aps, false) ||
+            !alf_aps_ok(d.cc_cr_aps, false))
+            return fail(ctx, RMH_INVALID_INPUT, tag + "APS out of range");
+    }
+    if (written_overlap(spans)) return fail(ctx, RMH_INVALID_INPUT, "a dst plane overlaps another plane of the batch");
+    return RMH_OK;
+}
+
+uint32_t div_up(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
+
+}  // namespace
+
+RMH_EXPORT rmh_status rmh_lmcs_plan_create(rmh_ctx* ctx, const rmh_lmcs_frame* frames, size_t n_frames,
+                                           rmh_plan** out) {
+    if (!ctx || !out) return RMH_INVALID_INPUT;
+    *out = nullptr;
+    rmh_status v = validate_lmcs(ctx, frames, n_frames);
+    if (v != RMH_OK) return v;
+    if (ctx->disabled.load()) return fail(ctx, RMH_UNAVAILABLE, "accelerator disabled");
+    DeviceGuard dg(ctx->device);
+    rmh_plan* p = new rmh_plan();
+    p->ctx = ctx;
+    p->kind = kLmcs;
+    // jobs of 8-bit planes first; LUTs after the job and workgroup tables
+    std::vector<size_t> order;
+    for (int pass = 1; pass <= 2; pass++)
+        for (size_t f = 0; f < n_frames; f++)
+            if (frames[f].src.bytes_per_sample == pass) order.push_back(f);
+    std::vector<rmh::LmcsJob> jobs;
+    std::vector<uint32_t> block_job;
+    std::vector<uint16_t> luts;
+    std::vector<uint32_t> lut_at;
+    uint32_t blocks8 = 0;
+    for (size_t f : order) {
+        const rmh_lmcs_frame& d = frames[f];
+        const int bd = d.src.bit_depth;
+        rmh::LmcsJob j{};
+        j.src = d.src.data;
+        j.dst = d.dst.data;
+        j.width = d.src.width;
+        j.height = d.src.height;
+        j.src_stride = d.src.stride;
+        j.dst_stride = d.dst.stride;
+        j.bps = d.src.bytes_per_sample;
+        j.cap = (1u << bd) - 1;
+        j.first_block = (uint32_t)block_job.size();
+        lut_at.push_back((uint32_t)luts.size());
+        luts.resize(luts.size() + j.cap + 1);
+        uint16_t* lut = luts.data() + lut_at.back();
+        rmh::lmcs_lut(d.aps, bd, d.direction == RMH_LMCS_FORWARD ? lut : nullptr,
+                      d.direction == RMH_LMCS_INVERSE ? lut : nullptr, nullptr, nullptr);
+        const uint32_t nb = div_up(d.src.height, rmh::kLmcsRows);
+        block_job.insert(block_job.end(), nb, (uint32_t)jobs.size());
+        if (d.src.bytes_per_sample == 1) blocks8 += nb;
+        jobs.push_back(j);
+        p->bytes += 2 * plane_bytes(d.src);
+    }
+    p->lmcs_range[0] = 0;
+    p->lmcs_range[1] = blocks8;
+    p->lmcs_range[2] = (uint32_t)block_job.size();
+    Blob blob;
+    const size_t o_jobs = blob.add(jobs.data(), jobs.size() * sizeof(rmh::LmcsJob));
+    const size_t o_bj = blob.add(block_job.data(), block_job.size() * 4);
+    const size_t o_lut = blob.add(luts.data(), luts.size() * 2);
+    // the jobs' LUT addresses are known once the device blob exists: patch and re-upload
+    std::string err;
+    if (!upload_blob(p, blob, &err)) {
+        delete p;
+        return fail(ctx, RMH_EXECUTION, "lmcs plan upload failed: " + err);
+    }
+    if (!jobs.empty()) {
+        char* base = (char*)p->d_mem;
+        for (size_t k = 0; k < jobs.size(); k++) jobs[k].lut = (const uint16_t*)(base + o_lut) + lut_at[k];
+        hipError_t e = hipMemcpy(base + o_jobs, jobs.data(), jobs.size() * sizeof(rmh::LmcsJob), hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            hipFree(p->d_mem);
+            delete p;
+            return fail(ctx, RMH_EXECUTION, std::string("lmcs plan upload failed: ") + hipGetErrorString(e));
+        }
+        p->d_lmcs_jobs = (const rmh::LmcsJob*)(base + o_jobs);
+        p->d_lmcs_block_job = (const uint32_t*)(base + o_bj);
+    }
+    *out = p;
+    return RMH_OK;
+}
+
+RMH_EXPORT rmh_status rmh_lmcs_plan_run(rmh_plan* plan, void* stream) {
+    if (!plan || plan->kind != kLmcs) return RMH_INVALID_INPUT;
+    if (plan->lmcs_range[2] == 0) return RMH_OK;
+    return run_plan(plan, nullptr, nullptr, 0, 0, stream);
+}
+
+RMH_EXPORT rmh_status rmh_alf_plan_create(rmh_ctx* ctx, const rmh_alf_frame* frames, size_t n_frames, rmh_plan** out) {
+    if (!ctx || !out) return RMH_INVALID_INPUT;
+    *out = nullptr;
+    rmh_status v = validate_alf(ctx, frames, n_frames);
+    if (v != RMH_OK) return v;
+    if (ctx->disabled.load()) return fail(ctx, RMH_UNAVAILABLE, "accelerator disabled");
+    DeviceGuard dg(ctx->device);
+    rmh_plan* p = new rmh_plan();
+    p->ctx = ctx;
+    p->kind = kAlf;
+    std::vector<size_t> order;
+    for (int pass = 1; pass <= 2; pass++)
+        for (size_t f = 0; f < n_frames; f++)
+            if (frames[f].src[0].bytes_per_sample == pass) order.push_back(f);
+    // per frame (in job order): luma bank, chroma bank, CCALF banks (Cb, Cr)
+    std::vector<int16_t> banks;
+    struct BankAt {
+        size_t luma, chroma, cc[2];
+    };
+    std::vector<BankAt> at;
+    std::vector<rmh::AlfLumaJob> lj;
+    std::vector<rmh::AlfChromaJob> cj;
+    std::vector<uint32_t> lbj, cbj;
+    uint32_t l8 = 0, c8 = 0;
+    for (size_t f : order) {
+        const rmh_alf_frame& d = frames[f];
+        const int bd = d.src[0].bit_depth;
+        const uint32_t ctb_log2 = d.ctb_size == 16 ? 4 : d.ctb_size == 32 ? 5 : d.ctb_size == 64 ? 6 : 7;
+        BankAt b{};
+        b.luma = banks.size();
+        banks.resize(banks.size() + rmh::kAlfBankEntries);
+        rmh::alf_luma_bank(d.luma_aps, bd, banks.data() + b.luma);
+        b.chroma = banks.size();
+        banks.resize(banks.size() + 8 * 12);
+        if (d.chroma_aps) rmh::alf_chroma_bank(*d.chroma_aps, bd, banks.data() + b.chroma);
+        for (int c = 0; c < 2; c++) {
+            const rmh_alf_aps* a = c == 0 ? d.cc_cb_aps : d.cc_cr_aps;
+            b.cc[c] = banks.size();
+            banks.resize(banks.size() + 4 * 7);
+            if (a) memcpy(banks.data() + b.cc[c], c == 0 ? &a->cc_cb[0][0] : &a->cc_cr[0][0], 4 * 7 * sizeof(int16_t));
+        }
+        at.push_back(b);
+        const rmh_plane& y = d.src[0];
+        rmh::AlfLumaJob L{};
+        L.src = y.data;
+        L.dst = d.dst[0].data;
+        L.ctbs = d.ctbs;
+        L.src_stride = y.stride;
+        L.dst_stride = d.dst[0].stride;
+        L.width = y.width;
+        L.height = y.height;
+        L.vis_w = d.vis_w;
+        L.vis_h = d.vis_h;
+        L.bw = std::min(d.vis_w, y.width);
+        L.bh = std::min(d.vis_h, y.height);
+        L.bps = y.bytes_per_sample;
+        L.bd = (uint32_t)bd;
+        L.ctb_log2 = ctb_log2;
+        L.ctb_cols = d.ctb_cols;
+        L.ctb_xn = div_up(d.vis_w, d.ctb_size);
+        L.ctb_yn = div_up(d.vis_h, d.ctb_size);
+        L.has_aps = d.luma_aps != nullptr;
+        L.tiles_x = div_up(y.width, rmh::kAlfTileW);
+        L.first_block = (uint32_t)lbj.size();
+        const uint32_t nl = L.tiles_x * div_up(y.height, rmh::kAlfTileH);
+        lbj.insert(lbj.end(), nl, (uint32_t)lj.size());
+        if (y.bytes_per_sample == 1) l8 += nl;
+        lj.push_back(L);
+        for (int c = 1; c < 3; c++) {
+            const rmh_plane& pc = d.src[c];
+            const rmh_alf_aps* cc = c == 1 ? d.cc_cb_aps : d.cc_cr_aps;
+            rmh::AlfChromaJob C{};
+            C.src = pc.data;
+            C.dst = d.dst[c].data;
+            C.luma = y.data;
+            C.ctbs = d.ctbs;
+            C.src_stride = pc.stride;
+            C.dst_stride = d.dst[c].stride;
+            C.luma_stride = y.stride;
+            C.width = pc.width;
+            C.height = pc.height;
+            C.cvis_w = (d.vis_w + 1) / 2;
+            C.cvis_h = (d.vis_h + 1) / 2;
+            C.cbw = std::min(C.cvis_w, pc.width);
+            C.cbh = std::min(C.cvis_h, pc.height);
+            C.lw = L.bw;
+            C.lh = L.bh;
+            C.vis_w = d.vis_w;
+            C.vis_h = d.vis_h;
+            C.bps = pc.bytes_per_sample;
+            C.bd = (uint32_t)bd;
+            C.ctb_log2 = ctb_log2;
+            C.ctb_cols = d.ctb_cols;
+            C.ctb_xn = L.ctb_xn;
+            C.ctb_yn = L.ctb_yn;
+            C.comp = (uint32_t)c;
+            C.n_chroma = d.chroma_aps ? d.chroma_aps->n_chroma : 0;
+            C.n_cc = cc ? (c == 1 ? cc->n_cc_cb : cc->n_cc_cr) : 0;
+            C.tiles_x = div_up(pc.width, rmh::kAlfTileW);
+            C.first_block = (uint32_t)cbj.size();
+            const uint32_t nc = C.tiles_x * div_up(pc.height, rmh::kAlfTileH);
+            cbj.insert(cbj.end(), nc, (uint32_t)cj.size());
+            if (pc.bytes_per_sample == 1) c8 += nc;
+            cj.push_back(C);
+        }
+        for (int c = 0; c < 3; c++) p->bytes += 2 * plane_bytes(d.src[c]);
+    }
+    p->alf_luma_range[1] = l8;
+    p->alf_luma_range[2] = (uint32_t)lbj.size();
+    p->alf_chroma_range[1] = c8;
+    p->alf_chroma_range[2] = (uint32_t)cbj.size();
+    Blob blob;
+    const size_t o_lj = blob.add(lj.data(), lj.size() * sizeof(rmh::AlfLumaJob));
+    const size_t o_cj = blob.add(cj.data(), cj.size() * sizeof(rmh::AlfChromaJob));
+    const size_t o_lbj = blob.add(lbj.data(), lbj.size() * 4);
+    const size_t o_cbj = blob.add(cbj.data(), cbj.size() * 4);
+    const size_t o_bank = blob.add(banks.data(), banks.size() * 2);
+    std::string err;
+    if (!upload_blob(p, blob, &err)) {
+        delete p;
+        return fail(ctx, RMH_EXECUTION, "alf plan upload failed: " + err);
+    }
+    if (!lj.empty()) {
+        char* base = (char*)p->d_mem;
+        const int16_t* bank = (const int16_t*)(base + o_bank);
+        for (size_t k = 0; k < lj.size(); k++) {
+            const rmh_alf_frame& d = frames[order[k]];
+            lj[k].bank = bank + at[k].luma;
+            for (int c = 0; c < 2; c++) {
+                rmh::AlfChromaJob& C = cj[2 * k + c];
+                C.chroma_bank = d.chroma_aps && d.chroma_aps->n_chroma ? bank + at[k].chroma : nullptr;
+                C.cc_bank = C.n_cc ? bank + at[k].cc[c] : nullptr;
+            }
+        }
+        hipError_t e = hipMemcpy(base + o_lj, lj.data(), lj.size() * sizeof(rmh::AlfLumaJob), hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+            e = hipMemcpy(base + o_cj, cj.data(), cj.size() * sizeof(rmh::AlfChromaJob), hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            hipFree(p->d_mem);
+            delete p;
+            return fail(ctx, RMH_EXECUTION, std::string("alf plan upload failed: ") + hipGetErrorString(e));
+        }
+        p->d_alf_luma = (const rmh::AlfLumaJob*)(base + o_lj);
+        p->d_alf_chroma = (const rmh::AlfChromaJob*)(base + o_cj);
+        p->d_alf_luma_bj = (const uint32_t*)(base + o_lbj);
+        p->d_alf_chroma_bj = (const uint32_t*)(base + o_cbj);
+    }
+    *out = p;
+    return RMH_OK;
+}
+
+RMH_EXPORT rmh_status rmh_alf_plan_run(rmh_plan* plan, void* stream) {
+    if (!plan || plan->kind != kAlf) return RMH_INVALID_INPUT;
+    if (plan->alf_luma_range[2] == 0) return RMH_OK;
+    return run_plan(plan, nullptr, nullptr, 0, 0, stream);
+}
+
 // ------------------------------------------------------------ plans, graphs
 
 RMH_EXPORT void rmh_plan_destroy(rmh_plan* plan) {
@@ -2398,7 +2761,8 @@ RMH_EXPORT rmh_status rmh_graph_create(rmh_ctx* ctx, const rmh_graph_node* nodes
             rmh_plan* p = nodes[i].plan;
             if ((p->kind == kResidual && p->n_tu == 0) || (p->kind == kMc && p->n_items == 0) ||
                 (p->kind == kDeblock && p->n_dbk == 0) || (p->kind == kSao && p->n_sao_jobs == 0) ||
-                (p->kind == kEdge && p->total_cells == 0))
+                (p->kind == kEdge && p->total_cells == 0) || (p->kind == kLmcs && p->lmcs_range[2] == 0) ||
+                (p->kind == kAlf && p->alf_luma_range[2] == 0))
                 continue;
             st = run_plan_locked_free(p, nodes[i].d_coeff, nodes[i].d_residual, 0, p->n_tu, g->capture, &err);
         }

@@ -18,20 +18,26 @@ functions use (crates/rustmpeg-video/src/vvc/filter.rs):
 - `&&`, `||`, `!`, references (`&m`, `&[..]`, `*line`), `i32::from(x)`,
   `x as T`, integer suffixes.
 
-Rust's precedence for the operators used (`*`, `+`, `-`, `>>`, `<<`,
+Rust's precedence for the operators used (`*`, `/`, `+`, `-`, `>>`, `<<`,
 comparisons, `&&`, `||`) is Python's, and `>>` on i32 is an arithmetic shift
 like Python's, so the translated expressions compute what the Rust ones do
-for values that do not overflow i32.
+for values that do not overflow i32. Integer `/` becomes Python's floor
+division, equal to Rust's truncating division on the non-negative operands
+these functions divide.
 """
 import re
 import types
 
 _METHODS = ("clamp", "abs", "min", "max", "iter", "zip", "enumerate", "map", "collect", "to_vec", "clone", "len",
-            "sum", "saturating_sub", "trailing_zeros", "fill")
+            "sum", "saturating_sub", "trailing_zeros", "fill", "step_by", "wrapping_sub", "get", "copied",
+            "unwrap_or", "div_ceil", "copy_from_slice", "expect", "is_empty", "first", "and_then", "as_ref")
 
 
-def fn_source(text, name):
-    """(parameter list, body) of `fn name(...) ... { body }` in text."""
+def fn_source(text, name, after=None):
+    """(parameter list, body) of `fn name(...) ... { body }` in text, the first
+    one past the text `after` when given (a method of one impl block)."""
+    if after is not None:
+        text = text[text.index(after):]
     m = re.search(r"\bfn\s+%s\s*(<[^>]*>)?\s*\(" % re.escape(name), text)
     if not m:
         raise KeyError(name)
@@ -136,6 +142,30 @@ def _methods(s):
             rep = "_len(%s)" % recv
         elif name == "fill":
             rep = "_fill(%s, %s)" % (recv, args)
+        elif name == "step_by":
+            rep = "_step_by(%s, %s)" % (recv, args)
+        elif name == "wrapping_sub":
+            rep = "(%s - (%s))" % (recv, args)  # used where the difference does not wrap
+        elif name == "get":
+            rep = "_get(%s, %s)" % (recv, args)  # Option: None past the end
+        elif name == "copied":
+            rep = recv
+        elif name == "unwrap_or":
+            rep = "_unwrap_or(%s, %s)" % (recv, args)
+        elif name == "copy_from_slice":
+            rep = "_copy_into(%s, %s)" % (recv, args)
+        elif name == "first":
+            rep = "_first(%s)" % recv
+        elif name == "and_then":
+            rep = "_and_then(%s, %s)" % (recv, expr(args))
+        elif name == "as_ref":
+            rep = recv
+        elif name == "is_empty":
+            rep = "(_len(%s) == 0)" % recv
+        elif name == "expect":
+            rep = recv
+        elif name == "div_ceil":
+            rep = "(-(-(%s) // (%s)))" % (recv, args)
         elif name == "sum":
             rep = "_sum(%s)" % recv
         elif name == "saturating_sub":
@@ -150,8 +180,15 @@ def _block_value(body):
     (assertions dropped; the block must end in its value)."""
     stmts = [t.strip() for t in _split_top(body, ";")]
     stmts = [t for t in stmts if t and not t.startswith("debug_assert")]
-    assert len(stmts) == 1, stmts
-    return expr(stmts[0])
+    lets = []
+    for t in stmts[:-1]:
+        # `let name = e;` before the value: an assignment expression
+        m = re.match(r"^let\s+(?:mut\s+)?(\w+)\s*(?::[^=]+)?=(.+)$", t, re.S)
+        assert m, stmts
+        lets.append("(%s := (%s))" % (m.group(1), expr(m.group(2))))
+    if not lets:
+        return expr(stmts[-1])
+    return "(%s, (%s))[-1]" % (", ".join(lets), expr(stmts[-1]))
 
 
 def _match_expr(s, at):
@@ -230,6 +267,17 @@ def expr(s):
     # `match` / `if` expressions become Python conditionals, held aside under
     # placeholder names while the rest is translated
     while True:
+        # a closure with a block body passed as an argument: |x, y| { stmts; value }
+        m = re.search(r"(?<=[(,])\s*\|([^|]*)\|\s*\{", s)
+        if not m:
+            break
+        brace = m.end() - 1
+        close = _match(s, brace)
+        params = ", ".join(p.split(":")[0].strip().lstrip("&") for p in m.group(1).split(",") if p.strip())
+        py = "(lambda %s: %s)" % (params, _block_value(s[brace + 1:close]))
+        s = s[:m.start()] + " __py%d__ " % len(stash) + s[close + 1:]
+        stash.append(py)
+    while True:
         m = re.search(r"\b(match|if)\b", s)
         if not m:
             break
@@ -239,11 +287,12 @@ def expr(s):
     s = s.replace("&&", " and ").replace("||", " or ")
     s = re.sub(r"\.sum::<\w+>\(\)", ".sum()", s)
     s = _vec_macro(s)
+    s = re.sub(r"\[([^\[\];]+);\s*([^\[\];]+)\]", r"([\1] * (\2))", s)  # [v; n]
     s = _slices(s)
     m = re.match(r"^\s*\|([^|]*)\|", s)
     if m:
         # a closure: |params| body  or  |params| -> T { body }
-        params = ", ".join(p.split(":")[0].strip() for p in m.group(1).split(",") if p.strip())
+        params = ", ".join(p.split(":")[0].strip().lstrip("&") for p in m.group(1).split(",") if p.strip())
         rest = s[m.end():].lstrip()
         if rest.startswith("->"):
             brace = rest.index("{")
@@ -252,12 +301,21 @@ def expr(s):
             body = expr(rest)
         s = s[:m.start()] + "(lambda %s: %s)" % (params, body)
     else:
+        s = re.sub(r"&mut\s+", "", s)
+        s = re.sub(r",\s*\)", ")", s)  # trailing comma of a call (Rust has no 1-tuples here)
         s = re.sub(r"!(?!=)", " not ", s)
         s = re.sub(r"&(?=[\w\[])", "", s)
         s = re.sub(r"(?<![\w\s)\]])\s*\*(line)\b", r"_list(\1)", s)  # `*line`: a copy of the array
+        s = re.sub(r"^\s*\*(\w+)\s*$", r"_deref(\1)", s)  # `*x` as a whole value
+        s = s.replace("usize::MAX", "_USIZE_MAX")
         s = re.sub(r"\b(?:i32|i64|u32|usize|u8|i8)::from\(", "(", s)
-        s = re.sub(r"\s+as\s+(?:usize|i32|i64|u32|u8|i8|u16)\b", "", s)
-        s = re.sub(r"\b(\d+)(?:i32|i64|u32|usize|u8|i8)\b", r"\1", s)
+        s = re.sub(r"\b(?:super::)?[a-z_]+::(?=[A-Z][A-Z0-9_]+\b)", "", s)  # module path of a constant
+        s = re.sub(r"\b(?:super::)?(?:filter|inter|predict|dct|transform|lfnst|tr_tables)::(?=[a-z_]\w*\()", "", s)
+        s = re.sub(r"\btrue\b", "True", s)
+        s = re.sub(r"\bfalse\b", "False", s)
+        s = re.sub(r"\s+as\s+(?:usize|isize|i32|i64|u32|u8|i8|u16)\b", "", s)
+        s = re.sub(r"\b(\d+)(?:i32|i64|u32|u64|usize|isize|u8|i8|u16|i16)\b", r"\1", s)
+        s = re.sub(r"(?<![/*])/(?![/*=])", "//", s)  # integer division (the functions divide non-negative values)
         s = _ranges(s)
         s = _methods(s)
     for k in range(len(stash) - 1, -1, -1):
@@ -342,12 +400,22 @@ def _struct_literal(stmt):
 
 def _pattern(p):
     p = re.sub(r"\bmut\s+", "", p.strip())
-    p = re.sub(r"&(?=\w)", "", p)
-    return p.split(":")[0].strip() if "(" not in p else p
+    p = re.sub(r"&(?=[\w(])", "", p)
+    depth = 0
+    for i, c in enumerate(p):  # cut a type annotation `: T` at bracket depth 0
+        if c in "([":
+            depth += 1
+        elif c in ")]":
+            depth -= 1
+        elif c == ":" and depth == 0:
+            return p[:i].strip()
+    return p.strip()
 
 
-def block(body, indent="    "):
-    """Python statements (as lines) for the Rust statements of body."""
+def block(body, indent="    ", tail=None):
+    """Python statements (as lines) for the Rust statements of body. The
+    block's value (its last expression) is returned, or assigned to `tail`
+    when given (a block that is the value of a `let`)."""
     out, s = [], body.strip()
     while s:
         if s.startswith("//"):
@@ -357,8 +425,12 @@ def block(body, indent="    "):
             brace = _top_brace(s, 2)
             cond = s[2:brace]
             close = _match(s, brace)
-            out.append(indent + "if (%s):" % expr(cond))
-            out += block(s[brace + 1:close], indent + "    ") or [indent + "    pass"]
+            lm = re.match(r"^\s*let\s+Some\((\w+)\)\s*=(.+)$", cond, re.S)
+            if lm:  # if let Some(x) = e
+                out.append(indent + "if ((%s := (%s)) is not None):" % (lm.group(1), expr(lm.group(2))))
+            else:
+                out.append(indent + "if (%s):" % expr(cond))
+            out += block(s[brace + 1:close], indent + "    ", tail) or [indent + "    pass"]
             s = s[close + 1:].lstrip()
             while s.startswith("else"):
                 s = s[4:].lstrip()
@@ -371,8 +443,16 @@ def block(body, indent="    "):
                     brace = s.index("{")
                     close = _match(s, brace)
                     out.append(indent + "else:")
-                out += block(s[brace + 1:close], indent + "    ") or [indent + "    pass"]
+                out += block(s[brace + 1:close], indent + "    ", tail) or [indent + "    pass"]
                 s = s[close + 1:].lstrip()
+            continue
+        if re.match(r"while\b", s):
+            brace = _top_brace(s, 5)
+            cond = s[5:brace]
+            close = _match(s, brace)
+            out.append(indent + "while (%s):" % expr(cond))
+            out += block(s[brace + 1:close], indent + "    ") or [indent + "    pass"]
+            s = s[close + 1:].lstrip()
             continue
         if re.match(r"match\b", s):
             # a match statement: arms `PAT | PAT => stmt,` or `PAT => { stmts }`
@@ -396,12 +476,18 @@ def block(body, indent="    "):
                 rest = rest.lstrip()
                 if rest.startswith(","):
                     rest = rest[1:].lstrip()
+                bind = []
                 if pat == "_":
                     out.append(indent + "else:")
                 else:
-                    cond = " or ".join("(%s) == (%s)" % (scrut, expr(a)) for a in _split_top(pat, "|"))
+                    em = re.match(r"^\w+::(\w+)\((\w+)\)$", pat)
+                    if em:  # Enum::Variant(x): values of such enums are ("Variant", x) tuples
+                        cond = "(%s)[0] == %r" % (scrut, em.group(1))
+                        bind = [indent + "    %s = (%s)[1]" % (em.group(2), scrut)]
+                    else:
+                        cond = " or ".join("(%s) == (%s)" % (scrut, expr(a)) for a in _split_top(pat, "|"))
                     out.append(indent + ("if (%s):" if first else "elif (%s):") % cond)
-                out += block(body, indent + "    ") or [indent + "    pass"]
+                out += bind + (block(body, indent + "    ", tail) or [indent + "    pass"])
                 first = False
             s = s[close + 1:].lstrip()
             continue
@@ -439,14 +525,40 @@ def block(body, indent="    "):
         parts = _split_top(s, ";")
         stmt = parts[0].strip()
         s = ";".join(parts[1:]).lstrip()
+        if stmt.startswith("const "):
+            stmt = "let " + stmt[6:]  # a local constant: a binding
         if not stmt or stmt.startswith("debug_assert"):
             continue
         if not parts[1:]:
             lit = _struct_literal(stmt)
-            out.append(indent + "return (%s)" % (lit or expr(stmt)))  # tail expression
+            if tail is not None:
+                out.append(indent + "%s = (%s)" % (tail, lit or expr(stmt)))
+            else:
+                out.append(indent + "return (%s)" % (lit or expr(stmt)))  # tail expression
+            continue
+        if re.match(r"^let\s+_\s*=", stmt):
+            continue
+        le = re.match(r"^let\s+Some\((\w+)\)\s*=(.+?)\s+else\s*\{(.*)\}$", stmt, re.S)
+        if le:  # let Some(x) = e else { continue / return };
+            out.append(indent + "%s = (%s)" % (le.group(1), expr(le.group(2))))
+            out.append(indent + "if %s is None:" % le.group(1))
+            out += block(le.group(3).strip().rstrip(";") + ";", indent + "    ")
             continue
         cm = re.match(r"^([\w\[\]\s\.\-\+\*]+?)\s*(\+|-|\*|<<|>>|\||&|\^)=\s*(.+)$", stmt, re.S)
-        if stmt == "return":
+        cl = re.match(r"^let\s+(\w+)\s*=\s*\|([^|]*)\|\s*->[^{]*\{", stmt)
+        if stmt in ("break", "continue"):
+            out.append(indent + stmt)
+        elif cl and _match(stmt, stmt.index("{", cl.end() - 1)) == len(stmt.rstrip()) - 1:
+            # a closure with a statement body: a nested function
+            params = ", ".join(q.split(":")[0].strip() for q in cl.group(2).split(",") if q.strip())
+            b0 = stmt.index("{", cl.end() - 1)
+            out.append(indent + "def %s(%s):" % (cl.group(1), params))
+            inner = stmt[b0 + 1:len(stmt.rstrip()) - 1].strip()
+            if re.match(r"match\b", inner) and _match(inner, inner.index("{")) == len(inner) - 1:
+                out.append(indent + "    return (%s)" % expr(inner))  # a match as the tail expression
+            else:
+                out += block(inner, indent + "    ")
+        elif stmt == "return":
             out.append(indent + "return None")
         elif stmt.startswith("return "):
             out.append(indent + "return (%s)" % expr(stmt[7:]))
@@ -454,10 +566,17 @@ def block(body, indent="    "):
             out.append(indent + "%s %s= (%s)" % (cm.group(1).strip(), cm.group(2), expr(cm.group(3))))
         elif stmt.startswith("let "):
             lhs, rhs = stmt[4:].split("=", 1)
-            out.append(indent + "%s = (%s)" % (_pattern(lhs), expr(rhs)))
+            try:
+                out.append(indent + "%s = (%s)" % (_pattern(lhs), expr(rhs)))
+            except AssertionError:
+                if not re.match(r"\s*(match|if)\b", rhs):
+                    raise
+                # arms with statements: the match / if as statements assigning the binding
+                out += block(rhs.strip(), indent, _pattern(lhs))
         elif re.match(r"[\w\[\]\s\-\+\*]+=[^=]", stmt) and not re.match(r"[\w\s]+[<>!=]=", stmt):
             lhs, rhs = stmt.split("=", 1)
-            out.append(indent + "%s = (%s)" % (lhs.strip(), expr(rhs)))
+            lhs = re.sub(r"\s+as\s+(?:usize|i32|i64|u32|u8|i8|u16)\b", "", lhs.strip())
+            out.append(indent + "%s = (%s)" % (lhs, expr(rhs)))
         else:
             out.append(indent + "(%s)" % expr(stmt))
     return out
@@ -468,20 +587,59 @@ def _clamp(v, lo, hi):
     return lo if v < lo else hi if v > hi else v
 
 
+_USIZE_MAX = (1 << 64) - 1
+
+
+def _copy_into(dst, src):
+    dst[:] = list(src)
+
+
+def _deref(v):
+    return list(v) if isinstance(v, (list, tuple)) else v
+
+
+def _clone(x):
+    return x.clone() if hasattr(x, "clone") else list(x)
+
+
+def _first(xs):
+    return xs[0] if len(xs) else None
+
+
+def _and_then(v, f):
+    return None if v is None else f(v)
+
+
+def _step_by(r, n):
+    return r[::n]
+
+
+def _get(xs, *a):
+    if isinstance(xs, (list, tuple)):
+        (i,) = a
+        return xs[i] if 0 <= i < len(xs) else None  # slice::get: None past the end
+    return xs.get(*a)  # an object's own get (a sample plane)
+
+
+def _unwrap_or(v, d):
+    return d if v is None else v
+
+
 def _fill(xs, v):
     xs[:] = [v] * len(xs)
 
 
-def compile_fn(text, name, extra=None):
+def compile_fn(text, name, extra=None, after=None):
     """A Python function with the parameters and body of Rust fn `name`."""
-    params, body = fn_source(text, name)
+    params, body = fn_source(text, name, after)
     body = re.sub(r"//[^\n]*", "", body)  # line comments
     names = [p.split(":")[0].strip() for p in _split_top(params, ",") if p.strip()]
     names = [re.sub(r"^&?(mut\s+)?", "", n) for n in names]
     src = "def %s(%s):\n%s\n" % (name, ", ".join(names), "\n".join(block(body)) or "    pass")
     # builtins under names that the reference's locals (`sum`, `min`, ...) cannot shadow
-    ns = {"_clamp": _clamp, "_struct": types.SimpleNamespace, "_abs": abs, "_min": min, "_max": max, "_zip": zip,
-          "_enumerate": enumerate, "_list": list, "_len": len, "_sum": sum, "_range": range, "_fill": _fill,
+    ns = {"_clamp": _clamp, "_struct": types.SimpleNamespace, "_step_by": _step_by, "_get": _get,
+          "_unwrap_or": _unwrap_or, "_first": _first, "_and_then": _and_then, "_copy_into": _copy_into, "_deref": _deref, "_USIZE_MAX": _USIZE_MAX, "_abs": abs, "_min": min, "_max": max, "_zip": zip,
+          "_enumerate": enumerate, "_list": _clone, "_len": len, "_sum": sum, "_range": range, "_fill": _fill,
           "p_at": lambda line, i: line[7 - i], "q_at": lambda line, j: line[8 + j]}
     ns.update(extra or {})
     exec(compile(src, "<rust:%s>" % name, "exec"), ns)

@@ -37,6 +37,9 @@ def test_struct_layouts_match_header(rmh):
     from workloads import EDGE_CELL_DTYPE, PU_DTYPE, TU_DTYPE
     assert TU_DTYPE == abi.TU_DTYPE and PU_DTYPE == abi.PU_DTYPE and EDGE_CELL_DTYPE == abi.EDGE_CELL_DTYPE
     assert ctypes.sizeof(abi.EdgeFrame) == 40
+    assert ctypes.sizeof(abi.LmcsAps) == 36 and ctypes.sizeof(abi.LmcsFrame) == 88
+    assert ctypes.sizeof(abi.AlfAps) == 1186 and abi.AlfAps.chroma_coeff.offset == 930
+    assert ctypes.sizeof(abi.AlfFrame) == 208 and abi.ALF_CTB_DTYPE.itemsize == 8
 
 
 def test_kernel_matrices_equal_oracle_matrices(rmh, po):
