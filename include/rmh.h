@@ -92,7 +92,9 @@ enum { RMH_TR_DCT2 = 0, RMH_TR_DST7 = 1, RMH_TR_DCT8 = 2 };  /* TrType, vvc/dct.
 
 enum {
     RMH_TU_TRANSFORM_SKIP = 1u, /* dequantize_ts, no transform (vvc/transform.rs:229-246) */
-    RMH_TU_DEP_QUANT = 2u       /* dependent quantisation (vvc/dct.rs:1191-1248) */
+    RMH_TU_DEP_QUANT = 2u,      /* dependent quantisation (vvc/dct.rs:1191-1248) */
+    RMH_TU_QUANT_INTER = 4u     /* forward plans only: the inter rounding offset of quantisation
+                                   (85 instead of 171, vvc/dct.rs:1134, transform.rs:422) */
 };
 
 /* One transform unit. 16 bytes; the descriptor fields of TuData / CuPlan
@@ -428,6 +430,35 @@ rmh_status rmh_sao(rmh_ctx* ctx, const rmh_sao_frame* frames, size_t n_frames);
 rmh_status rmh_sao_plan_create(rmh_ctx* ctx, const rmh_sao_frame* frames, size_t n_frames,
                                rmh_plan** out);
 rmh_status rmh_sao_plan_run(rmh_plan* plan, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Encoder side: forward transform and quantisation                          */
+/* ------------------------------------------------------------------------ */
+
+/* The encoder's forward path over a batch of TUs, in the order of
+ * plan_primary (crates/rustmpeg-video/src/vvc/slice.rs:718-761) and the 8..12-bit
+ * encoder (slice.rs:4279-4296): each TU's residual block (a raster w*h int16
+ * block at residual[offset], |value| < 2^15) goes through
+ * forward_transform_rect_bd (vvc/dct.rs:865-991) with the TU's zero-out (widened
+ * to the LFNST region when lfnst_idx, as on the decoder side), forward_lfnst_rect
+ * (vvc/lfnst.rs:131-204), then quantize_rect_bd (dct.rs:1113-1147; inter
+ * rounding with RMH_TU_QUANT_INTER) or quantize_dep_quant_bd (dct.rs:1252-1301)
+ * with RMH_TU_DEP_QUANT; transform-skip TUs through quantize_ts_with_mode
+ * (vvc/transform.rs:415-427). The levels are written as a raster w*h int16
+ * block at levels[offset]. The descriptors follow rmh_tu_desc (joint must be 0);
+ * the plan also holds the decoder-side residual plan of the same TUs. */
+rmh_status rmh_forward_plan_create(rmh_ctx* ctx, const rmh_tu_desc* tus, size_t n_tu, size_t n_coeff,
+                                   rmh_plan** out);
+/* Device buffers of n_coeff int16; asynchronous on `stream`. */
+rmh_status rmh_forward_plan_run(rmh_plan* plan, const int16_t* d_residual, int16_t* d_levels, void* stream);
+/* Candidate evaluation, as encode_luma scores a transform choice
+ * (slice.rs:817-866): the levels as above, the reconstructed residual the
+ * decoder derives from them (the residual plan) into d_recon, and per TU i
+ * d_sse[i] = sum over the block of (pred + residual - clip(pred + recon, 0,
+ * 2^bd - 1))^2, the block_sse of the clipped reconstruction against the source
+ * pred + residual; with d_pred NULL, sum of (residual - recon)^2. */
+rmh_status rmh_forward_plan_evaluate(rmh_plan* plan, const int16_t* d_residual, const int16_t* d_pred,
+                                     int16_t* d_levels, int16_t* d_recon, uint64_t* d_sse, void* stream);
 
 /* ------------------------------------------------------------------------ */
 /* Luma mapping with chroma scaling (LMCS)                                   */

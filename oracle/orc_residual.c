@@ -540,6 +540,79 @@ size_t orc_residual_batch(const rmh_tu_desc* tus, size_t n_tu, const int16_t* co
     return bad;
 }
 
+/* quantize_ts_with_mode (vvc/transform.rs:415-427): transform-skip
+ * quantisation with the intra (171) or inter (85) rounding offset. */
+int32_t orc_quant_ts_mode(int32_t residual, int qp, int is_intra) {
+    if (qp < 4) qp = 4;
+    int qp_per = qp / 6, qp_rem = qp % 6;
+    int i_qbits = 14 + qp_per;
+    int64_t add = (int64_t)(is_intra ? 171 : 85) << (i_qbits - 9);
+    int sign = residual > 0 ? 1 : residual < 0 ? -1 : 0;
+    int64_t a = residual < 0 ? -(int64_t)residual : residual;
+    int32_t level = (int32_t)((a * QUANT_SCALES[qp_rem] + add) >> i_qbits);
+    int64_t v = (int64_t)sign * level;
+    return (int32_t)(v < -32768 ? -32768 : v > 32767 ? 32767 : v);
+}
+
+/* The encoder's forward path for one TU, in the order of plan_primary
+ * (vvc/slice.rs:718-761) and the 8..12-bit encoder (slice.rs:4279-4296):
+ * forward_transform_rect_bd with the TU's zero-out (widened to the LFNST
+ * region when lfnst_idx, as the decoder's inverse_residual does), then
+ * forward_lfnst_rect, then quantize_rect_bd (intra rounding unless
+ * RMH_TU_QUANT_INTER) or quantize_dep_quant_bd; transform-skip TUs through
+ * quantize_ts_with_mode per sample. */
+int orc_forward_tu(const rmh_tu_desc* t, const int16_t* residual, int16_t* levels) {
+    if (t->log2_w > 6 || t->log2_h > 6) return -1;
+    int w = 1 << t->log2_w, h = 1 << t->log2_h;
+    if (w * h < 2 || t->bit_depth < 8 || t->bit_depth > 12) return -1;
+    if (t->tr_hor > 2 || t->tr_ver > 2 || t->lfnst_idx > 2 || t->lfnst_mode > 94) return -1;
+    if (t->flags & ~(RMH_TU_TRANSFORM_SKIP | RMH_TU_DEP_QUANT | RMH_TU_QUANT_INTER)) return -1;
+    const int intra = !(t->flags & RMH_TU_QUANT_INTER);
+    int n = w * h;
+    int32_t* r = malloc(sizeof(int32_t) * (size_t)n);
+    int32_t* c = malloc(sizeof(int32_t) * (size_t)n);
+    int32_t* lv = malloc(sizeof(int32_t) * (size_t)n);
+    int rc = 0;
+    for (int i = 0; i < n; i++) r[i] = residual[i];
+    if (t->flags & RMH_TU_TRANSFORM_SKIP) {
+        for (int i = 0; i < n; i++) lv[i] = orc_quant_ts_mode(r[i], t->qp, intra);
+    } else {
+        int sw = t->skip_w, sh = t->skip_h;
+        if (t->lfnst_idx) {
+            int lw, lh;
+            lfnst_skips(w, h, &lw, &lh);
+            if (lw > sw) sw = lw;
+            if (lh > sh) sh = lh;
+        }
+        rc = orc_forward_transform(w, h, r, t->tr_hor, t->tr_ver, sw, sh, t->bit_depth, c);
+        if (!rc && t->lfnst_idx) rc = orc_forward_lfnst(w, h, c, t->lfnst_idx, t->lfnst_mode);
+        if (!rc)
+            rc = (t->flags & RMH_TU_DEP_QUANT) ? orc_quant_dep_quant(w, h, c, t->qp, t->bit_depth, lv)
+                                               : orc_quant_rect(w, h, c, t->qp, t->bit_depth, intra, lv);
+    }
+    if (!rc)
+        for (int i = 0; i < n; i++) levels[i] = (int16_t)lv[i];
+    free(r);
+    free(c);
+    free(lv);
+    return rc;
+}
+
+size_t orc_forward_batch(const rmh_tu_desc* tus, size_t n_tu, const int16_t* residual, size_t n_coeff,
+                         int16_t* levels) {
+    for (size_t i = 0; i < n_tu; i++) {
+        const rmh_tu_desc* t = &tus[i];
+        if (t->log2_w > 6 || t->log2_h > 6) return i + 1;
+        size_t n = (size_t)1 << (t->log2_w + t->log2_h);
+        if ((size_t)t->offset + n > n_coeff) return i + 1;
+    }
+    size_t bad = 0;
+#pragma omp parallel for num_threads(orc_threads) schedule(dynamic, 64) if (orc_threads > 1) reduction(max : bad)
+    for (size_t i = 0; i < n_tu; i++)
+        if (orc_forward_tu(&tus[i], residual + tus[i].offset, levels + tus[i].offset)) bad = bad > i + 1 ? bad : i + 1;
+    return bad;
+}
+
 /* Residual added to the prediction held in the picture and clipped to the
  * sample range: the add of reconstruct_tu (vvc/decode_general.rs:4086-4092,
  * `(pred + residual).clamp(0, max_val)`), which reconstruct_tus applies to the

@@ -97,6 +97,10 @@ def _setup(L):
     L.orc_lmcs_scale_chroma.argtypes = [ctypes.c_void_p, c_int, c_int, I32P, c_int]
     L.orc_alf_classify.argtypes = [ctypes.c_void_p] + [c_int] * 6 + [ctypes.POINTER(c_int)]
     L.orc_alf_frame.argtypes = [ctypes.c_void_p]
+    L.orc_quant_ts_mode.argtypes = [ctypes.c_int32, c_int, c_int]
+    L.orc_quant_ts_mode.restype = ctypes.c_int32
+    L.orc_forward_batch.argtypes = [ctypes.c_void_p, c_size, I16P, c_size, I16P]
+    L.orc_forward_batch.restype = c_size
     L.orc_alf_luma_filter.argtypes = [c_int] * 3 + [ctypes.c_void_p, c_int, I32P, I32P]
     L.orc_alf_luma_filter.restype = None
 
@@ -453,3 +457,18 @@ def alf_luma_filter(mode, class_idx, transpose, aps, bd):
     lib().orc_alf_luma_filter(mode, class_idx, transpose, ctypes.addressof(aps) if aps is not None else None, bd,
                               _p(fc, I32P), _p(cl, I32P))
     return fc, cl
+
+
+def quant_ts_mode(res, qp, is_intra):
+    return lib().orc_quant_ts_mode(int(res), qp, int(is_intra))
+
+
+def forward_batch(tus, residual):
+    """The encoder forward path (orc_forward_batch) over TU_DTYPE descriptors: int16 levels."""
+    residual = np.ascontiguousarray(residual, dtype=np.int16)
+    out = np.zeros_like(residual)
+    tus = np.ascontiguousarray(tus)
+    bad = lib().orc_forward_batch(tus.ctypes.data, len(tus), _p(residual, I16P), residual.size, _p(out, I16P))
+    if bad:
+        raise ValueError("TU %d rejected" % (bad - 1))
+    return out

@@ -54,6 +54,11 @@ int orc_residual_tu(const rmh_tu_desc* tu, const int16_t* coeff, int16_t* residu
 /* Every TU of a batch; returns the index+1 of the first rejected TU, 0 on success. */
 size_t orc_residual_batch(const rmh_tu_desc* tus, size_t n_tu, const int16_t* coeff,
                           size_t n_coeff, int16_t* residual);
+/* Encoder forward path (plan_primary order, vvc/slice.rs:718-761). */
+int32_t orc_quant_ts_mode(int32_t residual, int qp, int is_intra);
+int orc_forward_tu(const rmh_tu_desc* tu, const int16_t* residual, int16_t* levels);
+size_t orc_forward_batch(const rmh_tu_desc* tus, size_t n_tu, const int16_t* residual, size_t n_coeff,
+                         int16_t* levels);
 /* Residual + prediction, clipped, in place in host planes (reconstruct_tu's add). */
 size_t orc_recon_batch(const rmh_tu_desc* tus, size_t n_tu, const int16_t* coeff, size_t n_coeff,
                        const rmh_tu_place* place, const rmh_recon_frame* frames, size_t n_frames);

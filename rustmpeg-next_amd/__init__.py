@@ -75,6 +75,7 @@ EXPORTED = [
     "rmh_mc_bi_batch", "rmh_mc_bi_plan_create", "rmh_deblock_noalloc", "rmh_intra_predict",
     "rmh_edge_map", "rmh_edge_map_plan_create", "rmh_edge_map_plan_run", "rmh_lmcs_tables",
     "rmh_lmcs_plan_create", "rmh_lmcs_plan_run", "rmh_alf_filter_bank", "rmh_alf_plan_create", "rmh_alf_plan_run",
+    "rmh_forward_plan_create", "rmh_forward_plan_run", "rmh_forward_plan_evaluate",
 ]
 
 
@@ -114,6 +115,9 @@ def load_library(path=LIB_PATH):
         "rmh_edge_map": ([vp, vp, sz], st),
         "rmh_edge_map_plan_create": ([vp, vp, sz, ctypes.POINTER(vp)], st),
         "rmh_edge_map_plan_run": ([vp, vp], st),
+        "rmh_forward_plan_create": ([vp, vp, sz, sz, ctypes.POINTER(vp)], st),
+        "rmh_forward_plan_run": ([vp, vp, vp, vp], st),
+        "rmh_forward_plan_evaluate": ([vp, vp, vp, vp, vp, vp, vp], st),
         "rmh_lmcs_tables": ([vp, ctypes.c_uint32, vp, vp, vp, vp], st),
         "rmh_lmcs_plan_create": ([vp, vp, sz, ctypes.POINTER(vp)], st),
         "rmh_lmcs_plan_run": ([vp, vp], st),
@@ -243,12 +247,19 @@ class Plan:
             self.acc._check(L.rmh_sao_plan_run(self.handle, stream))
         elif self.kind == "edge":
             self.acc._check(L.rmh_edge_map_plan_run(self.handle, stream))
+        elif self.kind == "forward":
+            self.acc._check(L.rmh_forward_plan_run(self.handle, d_coeff, d_residual, stream))
         elif self.kind == "lmcs":
             self.acc._check(L.rmh_lmcs_plan_run(self.handle, stream))
         elif self.kind == "alf":
             self.acc._check(L.rmh_alf_plan_run(self.handle, stream))
         else:
             self.acc._check(L.rmh_deblock_plan_run(self.handle, stream))
+
+    def evaluate(self, d_residual, d_pred, d_levels, d_recon, d_sse, stream=0):
+        """Forward plans: levels, reconstructed residual and per-TU SSE (rmh_forward_plan_evaluate)."""
+        self.acc._check(load_library().rmh_forward_plan_evaluate(self.handle, d_residual, d_pred or None, d_levels,
+                                                                 d_recon, d_sse, stream))
 
     def autotune(self, d_coeff, d_residual, stream=0, reps=5):
         """Chooses the tile split per TU size class by timing (rmh_residual_plan_autotune);
@@ -482,6 +493,15 @@ class Accelerator:
         self._check(load_library().rmh_sao_plan_create(self.handle, ctypes.addressof(arr), len(frames),
                                                        ctypes.byref(h)))
         return Plan(self, h.value, "sao")
+
+    def forward_plan(self, tus, n_coeff):
+        """Encoder forward plan (rmh_forward_plan_create) over TU_DTYPE descriptors;
+        plan.run(d_residual, d_levels) quantises, plan.evaluate(...) also reconstructs and scores."""
+        tus = np.ascontiguousarray(tus, dtype=abi.TU_DTYPE)
+        h = ctypes.c_void_p()
+        self._check(load_library().rmh_forward_plan_create(self.handle, tus.ctypes.data, len(tus), n_coeff,
+                                                           ctypes.byref(h)))
+        return Plan(self, h.value, "forward")
 
     def lmcs_plan(self, frames):
         """LMCS luma-mapping plan over abi.LmcsFrame entries with device planes; run with plan.run()."""

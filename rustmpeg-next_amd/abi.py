@@ -8,7 +8,7 @@ STATUS_NAMES = {0: "OK", 1: "Unavailable", 2: "InvalidInput", 3: "LimitExceeded"
                 5: "Timeout"}
 
 TR_DCT2, TR_DST7, TR_DCT8 = 0, 1, 2
-TU_TRANSFORM_SKIP, TU_DEP_QUANT = 1, 2
+TU_TRANSFORM_SKIP, TU_DEP_QUANT, TU_QUANT_INTER = 1, 2, 4
 PU_ALT_HPEL = 0x10
 EDGE_BS_MASK, EDGE_LENP_SHIFT, EDGE_LENQ_SHIFT, EDGE_CTB_ROW = 0x03, 2, 4, 0x40
 

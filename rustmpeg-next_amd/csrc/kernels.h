@@ -198,6 +198,33 @@ constexpr int kSaoBlocksPerWave = 1;
 // 16 samples on a side.
 hipError_t launch_sao(const SaoJob* jobs, const uint32_t* block_job, const uint32_t (&range)[5], hipStream_t stream);
 
+// Encoder forward path: levels of every TU from its residual block.
+struct FwdArgs {
+    const rmh_tu_desc* tus;
+    const int16_t* residual;
+    int16_t* levels;
+    const uint32_t* mat;
+    const int8_t* lfnst8;
+    const int8_t* lfnst4;
+    const uint16_t* scans;
+    int32_t mat_off[3][7];
+    int32_t scan_off[7][7];
+    int32_t scan_len[7][7];
+    uint32_t n_tu;
+};
+hipError_t launch_forward(const FwdArgs& a, hipStream_t stream);
+
+// Per-TU reconstruction SSE of a candidate batch (pred may be null).
+struct SseArgs {
+    const rmh_tu_desc* tus;
+    const int16_t* residual;
+    const int16_t* pred;
+    const int16_t* recon;
+    uint64_t* sse;
+    uint32_t n_tu;
+};
+hipError_t launch_sse(const SseArgs& a, hipStream_t stream);
+
 // One luma plane of an LMCS batch: samples mapped through lut[min(s, cap)].
 struct LmcsJob {
     const void* src;

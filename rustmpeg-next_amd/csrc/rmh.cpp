@@ -150,7 +150,7 @@ struct rmh_ctx {
     void* scratch = nullptr;  // this device's address of `scratch`
 };
 
-enum PlanKind { kResidual = 0, kMc = 1, kDeblock = 2, kSao = 3, kEdge = 4, kLmcs = 5, kAlf = 6 };
+enum PlanKind { kResidual = 0, kMc = 1, kDeblock = 2, kSao = 3, kEdge = 4, kLmcs = 5, kAlf = 6, kForward = 7 };
 
 struct rmh_plan {
     rmh_ctx* ctx = nullptr;
@@ -217,6 +217,9 @@ struct rmh_plan {
     const uint32_t* d_alf_luma_bj = nullptr;
     const uint32_t* d_alf_chroma_bj = nullptr;
     uint32_t alf_luma_range[3] = {}, alf_chroma_range[3] = {};
+    // forward (encoder) plans: d_tus holds the TUs with their forward flags;
+    // `inverse` is the residual plan of the same TUs that reconstructs the levels
+    rmh_plan* inverse = nullptr;
 };
 
 struct rmh_graph {
@@ -1219,6 +1222,20 @@ rmh_status run_plan_locked_free(rmh_plan* plan, const int16_t* d_coeff, int16_t*
         e = rmh::launch_sao(plan->d_sao_jobs, plan->d_sao_block_job, plan->sao_range, s);
     } else if (plan->kind == kEdge) {
         e = rmh::launch_edge_map(plan->d_edge_frames, plan->d_cell_start, (uint32_t)plan->n_edge, plan->total_cells, s);
+    } else if (plan->kind == kForward) {
+        rmh::FwdArgs a{};
+        a.tus = plan->d_tus;
+        a.residual = d_coeff;
+        a.levels = d_residual;
+        a.mat = plan->ctx->tabs.mat;
+        a.lfnst8 = plan->ctx->tabs.lfnst8;
+        a.lfnst4 = plan->ctx->tabs.lfnst4;
+        a.scans = plan->ctx->tabs.scans;
+        memcpy(a.mat_off, plan->ctx->tabs.mat_off, sizeof a.mat_off);
+        memcpy(a.scan_off, plan->ctx->tabs.scan_off, sizeof a.scan_off);
+        memcpy(a.scan_len, plan->ctx->tabs.scan_len, sizeof a.scan_len);
+        a.n_tu = (uint32_t)plan->n_tu;
+        e = rmh::launch_forward(a, s);
     } else if (plan->kind == kLmcs) {
         e = rmh::launch_lmcs(plan->d_lmcs_jobs, plan->d_lmcs_block_job, plan->lmcs_range, s);
     } else if (plan->kind == kAlf) {
@@ -2385,6 +2402,72 @@ RMH_EXPORT rmh_status rmh_sao(rmh_ctx* ctx, const rmh_sao_frame* frames, size_t 
     return RMH_OK;
 }
 
+// ----------------------------------------------------------- forward (encoder)
+
+RMH_EXPORT rmh_status rmh_forward_plan_create(rmh_ctx* ctx, const rmh_tu_desc* tus, size_t n_tu, size_t n_coeff,
+                                              rmh_plan** out) {
+    if (!ctx || !out) return RMH_INVALID_INPUT;
+    *out = nullptr;
+    if (n_tu && !tus) return fail(ctx, RMH_INVALID_INPUT, "null TU array");
+    // the inverse plan checks every TU as a decoder TU; the forward flag and
+    // joint Cb-Cr (a decoder-side derivation) are checked here
+    std::vector<rmh_tu_desc> inv(tus, tus + n_tu);
+    for (size_t i = 0; i < n_tu; i++) {
+        if (inv[i].joint) return fail(ctx, RMH_INVALID_INPUT, "TU " + std::to_string(i) + ": joint Cb-Cr in a forward plan");
+        inv[i].flags &= (uint8_t)~RMH_TU_QUANT_INTER;
+    }
+    rmh_plan* sub = nullptr;
+    rmh_status st = create_residual_plan(ctx, inv.data(), n_tu, n_coeff, nullptr, nullptr, 0, &sub);
+    if (st != RMH_OK) return st;
+    DeviceGuard dg(ctx->device);
+    rmh_plan* p = new rmh_plan();
+    p->ctx = ctx;
+    p->kind = kForward;
+    p->n_tu = n_tu;
+    p->n_coeff = n_coeff;
+    p->inverse = sub;
+    for (size_t i = 0; i < n_tu; i++) p->bytes += 4ull << (tus[i].log2_w + tus[i].log2_h);  // residual in, levels out
+    if (n_tu) {
+        hipError_t e = hipMalloc(&p->d_mem, n_tu * sizeof(rmh_tu_desc));
+        if (e == hipSuccess) e = hipMemcpy(p->d_mem, tus, n_tu * sizeof(rmh_tu_desc), hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            rmh_plan_destroy(p);
+            return fail(ctx, RMH_EXECUTION, std::string("forward plan upload failed: ") + hipGetErrorString(e));
+        }
+        p->d_tus = (const rmh_tu_desc*)p->d_mem;
+    }
+    *out = p;
+    return RMH_OK;
+}
+
+RMH_EXPORT rmh_status rmh_forward_plan_run(rmh_plan* plan, const int16_t* d_residual, int16_t* d_levels, void* stream) {
+    if (!plan || plan->kind != kForward) return RMH_INVALID_INPUT;
+    if (plan->n_tu == 0) return RMH_OK;
+    if (!d_residual || !d_levels) return fail(plan->ctx, RMH_INVALID_INPUT, "null device buffer");
+    return run_plan(plan, d_residual, d_levels, 0, plan->n_tu, stream);
+}
+
+RMH_EXPORT rmh_status rmh_forward_plan_evaluate(rmh_plan* plan, const int16_t* d_residual, const int16_t* d_pred,
+                                                int16_t* d_levels, int16_t* d_recon, uint64_t* d_sse, void* stream) {
+    if (!plan || plan->kind != kForward) return RMH_INVALID_INPUT;
+    if (plan->n_tu == 0) return RMH_OK;
+    if (!d_residual || !d_levels || !d_recon || !d_sse) return fail(plan->ctx, RMH_INVALID_INPUT, "null device buffer");
+    rmh_status st = run_plan(plan, d_residual, d_levels, 0, plan->n_tu, stream);
+    if (st == RMH_OK) st = run_plan(plan->inverse, d_levels, d_recon, 0, plan->n_tu, stream);
+    if (st != RMH_OK) return st;
+    DeviceGuard dg(plan->ctx->device);
+    rmh::SseArgs a{};
+    a.tus = plan->d_tus;
+    a.residual = d_residual;
+    a.pred = d_pred;
+    a.recon = d_recon;
+    a.sse = d_sse;
+    a.n_tu = (uint32_t)plan->n_tu;
+    hipError_t e = rmh::launch_sse(a, (hipStream_t)stream);
+    if (e != hipSuccess) return fail(plan->ctx, RMH_EXECUTION, std::string("kernel launch failed: ") + hipGetErrorString(e));
+    return RMH_OK;
+}
+
 // ------------------------------------------------------------- lmcs, alf
 
 namespace {
@@ -2734,6 +2817,7 @@ RMH_EXPORT void rmh_plan_destroy(rmh_plan* plan) {
     if (!plan) return;
     DeviceGuard dg(plan->ctx->device);
     if (plan->d_mem) hipFree(plan->d_mem);
+    if (plan->inverse) rmh_plan_destroy(plan->inverse);
     delete plan;
 }
 
@@ -2744,7 +2828,7 @@ RMH_EXPORT rmh_status rmh_graph_create(rmh_ctx* ctx, const rmh_graph_node* nodes
     *out = nullptr;
     for (size_t i = 0; i < n_nodes; i++) {
         if (!nodes[i].plan || nodes[i].plan->ctx != ctx) return fail(ctx, RMH_INVALID_INPUT, "graph node without a plan of this context");
-        if (nodes[i].plan->kind == kResidual && nodes[i].plan->n_tu &&
+        if ((nodes[i].plan->kind == kResidual || nodes[i].plan->kind == kForward) && nodes[i].plan->n_tu &&
             (!nodes[i].d_coeff || (!nodes[i].d_residual && !nodes[i].plan->d_place)))
             return fail(ctx, RMH_INVALID_INPUT, "residual graph node without device buffers");
     }
@@ -2759,7 +2843,7 @@ RMH_EXPORT rmh_status rmh_graph_create(rmh_ctx* ctx, const rmh_graph_node* nodes
     if (e == hipSuccess) {
         for (size_t i = 0; i < n_nodes && st == RMH_OK; i++) {
             rmh_plan* p = nodes[i].plan;
-            if ((p->kind == kResidual && p->n_tu == 0) || (p->kind == kMc && p->n_items == 0) ||
+            if (((p->kind == kResidual || p->kind == kForward) && p->n_tu == 0) || (p->kind == kMc && p->n_items == 0) ||
                 (p->kind == kDeblock && p->n_dbk == 0) || (p->kind == kSao && p->n_sao_jobs == 0) ||
                 (p->kind == kEdge && p->total_cells == 0) || (p->kind == kLmcs && p->lmcs_range[2] == 0) ||
                 (p->kind == kAlf && p->alf_luma_range[2] == 0))

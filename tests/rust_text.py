@@ -30,7 +30,8 @@ import types
 
 _METHODS = ("clamp", "abs", "min", "max", "iter", "zip", "enumerate", "map", "collect", "to_vec", "clone", "len",
             "sum", "saturating_sub", "trailing_zeros", "fill", "step_by", "wrapping_sub", "get", "copied",
-            "unwrap_or", "div_ceil", "copy_from_slice", "expect", "is_empty", "first", "and_then", "as_ref")
+            "unwrap_or", "div_ceil", "copy_from_slice", "expect", "is_empty", "first", "and_then", "as_ref", "rev",
+            "take", "ok_or", "map_err", "signum", "contains")
 
 
 def fn_source(text, name, after=None):
@@ -154,6 +155,18 @@ def _methods(s):
             rep = "_unwrap_or(%s, %s)" % (recv, args)
         elif name == "copy_from_slice":
             rep = "_copy_into(%s, %s)" % (recv, args)
+        elif name == "contains":
+            rep = "((%s) in (%s))" % (args.lstrip("&"), recv)
+        elif name == "rev":
+            rep = "_rev(%s)" % recv
+        elif name == "take":
+            rep = "_take(%s, %s)" % (recv, args)
+        elif name == "ok_or":
+            rep = "_unwrap_opt(%s)" % recv
+        elif name == "map_err":
+            rep = recv
+        elif name == "signum":
+            rep = "_signum(%s)" % recv
         elif name == "first":
             rep = "_first(%s)" % recv
         elif name == "and_then":
@@ -264,6 +277,13 @@ def expr(s):
     """Python source of one Rust expression."""
     s = s.strip()
     stash = []
+    # string literals first, so that words inside them are not taken for keywords
+    while True:
+        m = re.search(r'"(?:[^"\\]|\\.)*"', s)
+        if not m:
+            break
+        s = s[:m.start()] + " __py%d__ " % len(stash) + s[m.end():]
+        stash.append(m.group(0))
     # `match` / `if` expressions become Python conditionals, held aside under
     # placeholder names while the rest is translated
     while True:
@@ -302,6 +322,9 @@ def expr(s):
         s = s[:m.start()] + "(lambda %s: %s)" % (params, body)
     else:
         s = re.sub(r"&mut\s+", "", s)
+        s = re.sub(r"\bOk\(", "(", s)  # the functions raise _RustErr instead of returning Err
+        s = re.sub(r"\)\?", ")", s)
+        s = re.sub(r"\b([A-Z]\w*)::([A-Z]\w*)\b(?!\s*\()", r"\1_\2", s)  # Enum::Variant -> Enum_Variant
         s = re.sub(r",\s*\)", ")", s)  # trailing comma of a call (Rust has no 1-tuples here)
         s = re.sub(r"!(?!=)", " not ", s)
         s = re.sub(r"&(?=[\w\[])", "", s)
@@ -499,8 +522,17 @@ def block(body, indent="    ", tail=None):
             body_src = s[brace + 1:close]
             im = re.match(r"^\s*(\w+)\.iter_mut\(\)\s*$", it)
             tm = re.match(r"^\s*(\w+)\.iter_mut\(\)\.take\((.+)\)\.enumerate\(\)\s*$", it, re.S)
-            pm = re.match(r"^\s*\(\s*(\w+)\s*,\s*(\w+)\s*\)\s*$", pat)
-            if tm and pm:
+            tm = tm or re.match(r"^\s*(\w+)\.iter_mut\(\)\.enumerate\(\)\.take\((.+)\)\s*$", it, re.S)
+            zm = re.match(r"^\s*(\w+)\.iter_mut\(\)\.zip\((\w+)\)\s*$", it)
+            pm = re.match(r"^\s*\(\s*(\w+)\s*,\s*&?(\w+)\s*\)\s*$", pat)
+            if zm and pm:
+                # for (d, &x) in out.iter_mut().zip(xs) { *d = e(x); }  ->  over the indices of out
+                seq, var = zm.group(1), pm.group(1)
+                k = "_i_%s" % var
+                out.append(indent + "for %s in _range(0, _min(_len(%s), _len(%s))):" % (k, seq, zm.group(2)))
+                out.append(indent + "    %s = %s[%s]" % (pm.group(2), zm.group(2), k))
+                body_src = re.sub(r"\*%s\b" % var, "%s[%s]" % (seq, k), body_src)
+            elif tm and pm:
                 # for (i, v) in xs.iter_mut().take(n).enumerate() { *v = e; }
                 seq, idx, var = tm.group(1), pm.group(1), pm.group(2)
                 out.append(indent + "for %s in _range(0, _min(%s, _len(%s))):" % (idx, expr(tm.group(2)), seq))
@@ -560,6 +592,8 @@ def block(body, indent="    ", tail=None):
                 out += block(inner, indent + "    ")
         elif stmt == "return":
             out.append(indent + "return None")
+        elif re.match(r"^return\s+Err\(", stmt):
+            out.append(indent + "raise _RustErr(%s)" % expr(stmt[stmt.index("Err(") + 3:]))
         elif stmt.startswith("return "):
             out.append(indent + "return (%s)" % expr(stmt[7:]))
         elif cm:
@@ -596,6 +630,28 @@ def _copy_into(dst, src):
 
 def _deref(v):
     return list(v) if isinstance(v, (list, tuple)) else v
+
+
+class _RustErr(Exception):
+    """An Err(...) of the evaluated Rust function."""
+
+
+def _unwrap_opt(v):
+    if v is None:
+        raise _RustErr("None")
+    return v
+
+
+def _rev(xs):
+    return list(xs)[::-1]
+
+
+def _take(xs, n):
+    return list(xs)[:n]
+
+
+def _signum(v):
+    return (v > 0) - (v < 0)
 
 
 def _clone(x):
@@ -638,7 +694,8 @@ def compile_fn(text, name, extra=None, after=None):
     src = "def %s(%s):\n%s\n" % (name, ", ".join(names), "\n".join(block(body)) or "    pass")
     # builtins under names that the reference's locals (`sum`, `min`, ...) cannot shadow
     ns = {"_clamp": _clamp, "_struct": types.SimpleNamespace, "_step_by": _step_by, "_get": _get,
-          "_unwrap_or": _unwrap_or, "_first": _first, "_and_then": _and_then, "_copy_into": _copy_into, "_deref": _deref, "_USIZE_MAX": _USIZE_MAX, "_abs": abs, "_min": min, "_max": max, "_zip": zip,
+          "_unwrap_or": _unwrap_or, "_first": _first, "_and_then": _and_then, "_RustErr": _RustErr,
+          "_unwrap_opt": _unwrap_opt, "_rev": _rev, "_take": _take, "_signum": _signum, "_copy_into": _copy_into, "_deref": _deref, "_USIZE_MAX": _USIZE_MAX, "_abs": abs, "_min": min, "_max": max, "_zip": zip,
           "_enumerate": enumerate, "_list": _clone, "_len": len, "_sum": sum, "_range": range, "_fill": _fill,
           "p_at": lambda line, i: line[7 - i], "q_at": lambda line, j: line[8 + j]}
     ns.update(extra or {})
