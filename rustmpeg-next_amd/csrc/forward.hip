@@ -205,6 +205,11 @@ __global__ __launch_bounds__(64) void forward_kernel(const FwdArgs a) {
     const int rs = max(qshift, 0);
     const uint16_t* scan = a.scans + a.scan_off[lw][lh];
     const int len = a.scan_len[lw][lh];
+    // the scan covers the top-left min(w, 32) x min(h, 32) region only: the
+    // rest of a 64-sample side is zero (the reference clears the whole block)
+    if (w > 32 || h > 32)
+        for (int i = lane; i < n; i += 64)
+            if ((i & (w - 1)) >= 32 || (i >> lw) >= 32) out[i] = 0;
     // the signed, clamped levels for the two reconstruction offsets (state >> 1 = 0, 1)
     auto levels_at = [&](int s, int* l0, int* l1) {
         const int r = scan[s];

@@ -62,7 +62,10 @@ def test_levels_equal_oracle(acc, po):
     tus, res = _batch(rng, abi)
     want = po.forward_batch(tus, res)
     plan = acc.forward_plan(tus, res.size)
-    d_res, d_lv = hip.DeviceBuffer.from_array(res), hip.DeviceBuffer(res.nbytes)
+    # the level buffer starts filled with a non-zero pattern: every position
+    # of every TU must be written, including the zeroed parts of 64-sample sides
+    d_res = hip.DeviceBuffer.from_array(res)
+    d_lv = hip.DeviceBuffer.from_array(np.full(res.size, 0x5A5A, np.int16))
     s = hip.Stream()
     plan.run(d_res.ptr, d_lv.ptr, s.handle)
     s.synchronize()
