@@ -307,7 +307,10 @@ rmh_status rmh_deblock_plan_run(rmh_plan* plan, void* stream);
  * into SAO): frames[f].planes are read only, dst[3 f + c] (same size, bit
  * depth and sample size, any pitch, not overlapping a source plane) receives
  * every sample of the deblocked component c of frame f. It saves the copy of
- * each plane that the in-place form makes first. Run with rmh_deblock_plan_run. */
+ * each plane that the in-place form makes first. Run with rmh_deblock_plan_run.
+ * Every sample of the source planes reaches dst, filtered or not, so a frame
+ * step that builds the picture in scratch planes must write every sample of
+ * them first (including samples no inter PU or TU covers). */
 rmh_status rmh_deblock_to_plan_create(rmh_ctx* ctx, const rmh_dbk_frame* frames, size_t n_frames,
                                       const rmh_plane* dst, rmh_plan** out);
 
