@@ -227,7 +227,7 @@ rmh_status rmh_mc_plan_create(rmh_ctx* ctx, const rmh_mc_frame* frames, size_t n
 rmh_status rmh_mc_plan_run(rmh_plan* plan, void* stream);
 
 /* Bi-prediction (inter_dir 3) with CU-level weights, replacing the bi branch
- * of predict_inter_pu (vvc/inter.rs:1753-1801) without DMVR and BDOF: both
+ * of predict_inter_pu (vvc/inter.rs:1753-1801; DMVR and BDOF per PU, below): both
  * lists are interpolated to 14-bit intermediates (interpolate_luma_block /
  * interpolate_chroma_block_420 with for_bi_pred, vvc/inter.rs:886-1182; the
  * 8..12-bit forms vvc/inter.rs:2010-2134, 2543-2670) and combined by
@@ -252,8 +252,24 @@ typedef struct rmh_bi_pu_desc {
     int32_t mv1_y;
     uint8_t bcw_idx;     /* L1 weight BCW_WEIGHTS[min(bcw_idx, 4)] / 8 of {-2, 3, 4, 5, 10}
                           * (vvc/inter.rs:128); L0 weight 8 minus that; 2 = equal weights */
-    uint8_t reserved[3]; /* zero */
+    uint8_t refine;      /* RMH_BI_DMVR | RMH_BI_BDOF, 8-bit frames only; 0 otherwise */
+    uint8_t reserved[2]; /* zero */
 } rmh_bi_pu_desc;
+
+/* Decoder-side refinements of the 8-bit bi branch of predict_inter_pu
+ * (vvc/inter.rs:1763-1800). The caller sets a flag when the conditions that
+ * need its state hold: RMH_BI_DMVR for the SPS / picture-header enables, a
+ * merge CU and references at symmetric POC distances (sym_poc); RMH_BI_BDOF
+ * for the enables and references on opposite sides (opp_poc). The library
+ * then applies, as the reference does:
+ *   DMVR (refine_dmvr_subblock, inter.rs:1371-1459) over the whole PU when
+ *        bcw_idx is 2 and the PU is at least 8x8 with 128 samples: the MV pair
+ *        is refined by a bilateral search over the 5x5 integer offsets (L0 +
+ *        offset, L1 - offset) on bilinear predictions, every second row, and a
+ *        parabolic sub-sample step; both luma and chroma use the refined pair;
+ *   BDOF (apply_bdof_luma, inter.rs:1218-1339) on luma when bcw_idx is 2, in
+ *        place of the weighted average. */
+enum { RMH_BI_DMVR = 1u, RMH_BI_BDOF = 2u };
 
 /* Host memory, synchronous (as rmh_mc_batch, which it mirrors). */
 rmh_status rmh_mc_bi_batch(rmh_ctx* ctx, const rmh_mc_bi_frame* frames, size_t n_frames,

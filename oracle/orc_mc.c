@@ -5,8 +5,9 @@
  * (crates/rustmpeg-video/src/vvc/inter.rs:886-1182), generalised from 8 bits to
  * bit depth bd with head_room = IF_INTERNAL_PREC - bd (for bd = 8 the function is
  * the reference's 8-bit one term for term), the uni-prediction path and the
- * bi-prediction path without DMVR and BDOF of predict_inter_pu
- * (inter.rs:1734-1848), weighted_average_bi / weighted_average_bi_16
+ * bi-prediction path of predict_inter_pu (inter.rs:1734-1848) with its
+ * 8-bit refinements DMVR (refine_dmvr_subblock, inter.rs:1341-1459) and BDOF
+ * (apply_bdof_luma, inter.rs:1204-1339), weighted_average_bi / weighted_average_bi_16
  * (inter.rs:1189-1201, 2676-2696) and write_prediction
  * (decode_general.rs:2641-2667).
  *
@@ -239,6 +240,178 @@ int orc_interp_luma_16_as_written(const uint16_t* plane, int stride, int pw, int
 
 /* BCW_WEIGHTS (inter.rs:128). */
 static const int BCW_WEIGHTS[5] = {-2, 3, 4, 5, 10};
+#define BCW_DEFAULT 2
+
+/* right_shift_msb (inter.rs:1204-1211): numer >> floor(log2(denom)), 0 for denom <= 0. */
+static int right_shift_msb(int numer, int denom) {
+    if (denom <= 0) return 0;
+    int l = 31 - __builtin_clz((unsigned)denom);
+    return numer >> l;
+}
+
+static int sgn_mul(int s, int v) { return s < 0 ? -v : s == 0 ? 0 : v; }
+
+/* apply_bdof_luma (inter.rs:1218-1339), 8-bit: pred0 / pred1 are the 14-bit
+ * intermediates of a w x h block (w, h multiples of 4). The block is padded by
+ * one sample on each side by replication, the gradients are taken on the
+ * inner samples and replicated to the border, and each 4x4 unit's motion
+ * offset comes from the sums over its 6x6 window. */
+int orc_bdof_luma(const int32_t* pred0, const int32_t* pred1, int w, int h, int32_t* dst) {
+    if (w < 4 || h < 4 || (w & 3) || (h & 3)) return -1;
+    const int ew = w + 2, eh = h + 2;
+    int32_t* buf = malloc(sizeof(int32_t) * (size_t)(ew * eh) * 6);
+    int32_t* p[2] = {buf, buf + ew * eh};
+    int32_t* gx[2] = {buf + 2 * ew * eh, buf + 3 * ew * eh};
+    int32_t* gy[2] = {buf + 4 * ew * eh, buf + 5 * ew * eh};
+    for (int l = 0; l < 2; l++) {
+        const int32_t* src = l ? pred1 : pred0;
+        int32_t* q = p[l];
+        for (int y = 0; y < h; y++) {
+            q[(y + 1) * ew] = src[y * w];
+            for (int x = 0; x < w; x++) q[(y + 1) * ew + x + 1] = src[y * w + x];
+            q[(y + 1) * ew + w + 1] = src[y * w + w - 1];
+        }
+        for (int x = 0; x < ew; x++) {
+            q[x] = q[ew + x];
+            q[(h + 1) * ew + x] = q[h * ew + x];
+        }
+        int32_t *a = gx[l], *b = gy[l];
+        for (int y = 1; y <= h; y++) {
+            for (int x = 1; x <= w; x++) {
+                const int i = y * ew + x;
+                a[i] = (q[i + 1] >> 6) - (q[i - 1] >> 6);
+                b[i] = (q[i + ew] >> 6) - (q[i - ew] >> 6);
+            }
+            a[y * ew] = a[y * ew + 1];
+            a[y * ew + w + 1] = a[y * ew + w];
+            b[y * ew] = b[y * ew + 1];
+            b[y * ew + w + 1] = b[y * ew + w];
+        }
+        for (int x = 0; x < ew; x++) {
+            a[x] = a[ew + x];
+            a[(h + 1) * ew + x] = a[h * ew + x];
+            b[x] = b[ew + x];
+            b[(h + 1) * ew + x] = b[h * ew + x];
+        }
+    }
+    const int shift = IF_INTERNAL_PREC - 8 + 1;
+    const int offset = (1 << (shift - 1)) + 2 * IF_INTERNAL_OFFS;
+    const int limit = 15;
+    for (int yu = 0; yu < h / 4; yu++)
+        for (int xu = 0; xu < w / 4; xu++) {
+            int sgx = 0, sgy = 0, sdx = 0, sdy = 0, sxy = 0;
+            for (int dy = 0; dy < 6; dy++)
+                for (int dx = 0; dx < 6; dx++) {
+                    const int i = (yu * 4 + dy) * ew + xu * 4 + dx;
+                    const int tgx = (gx[0][i] + gx[1][i]) >> 1;
+                    const int tgy = (gy[0][i] + gy[1][i]) >> 1;
+                    const int tdi = (p[1][i] >> 4) - (p[0][i] >> 4);
+                    sgx += tgx < 0 ? -tgx : tgx;
+                    sgy += tgy < 0 ? -tgy : tgy;
+                    sdx += sgn_mul(tgx, tdi);
+                    sdy += sgn_mul(tgy, tdi);
+                    sxy += sgn_mul(tgy, tgx);
+                }
+            int vx = sgx == 0 ? 0 : clampi(right_shift_msb(4 * sdx, sgx), -limit, limit);
+            int t = (sxy * vx) >> 1;
+            int vy = sgy == 0 ? 0 : clampi(right_shift_msb(4 * sdy - t, sgy), -limit, limit);
+            for (int dy = 0; dy < 4; dy++)
+                for (int dx = 0; dx < 4; dx++) {
+                    const int y = yu * 4 + dy, x = xu * 4 + dx, i = (y + 1) * ew + x + 1;
+                    const int b = vx * (gx[0][i] - gx[1][i]) + vy * (gy[0][i] - gy[1][i]);
+                    dst[y * w + x] = clampi((p[0][i] + p[1][i] + b + offset) >> shift, 0, 255);
+                }
+        }
+    free(buf);
+    return 0;
+}
+
+/* dmvr_subpel_1d (inter.rs:1341-1363): the parabolic sub-sample offset in
+ * 1/16 sample from the costs at -1, 0, +1 (s1, s0, s2). */
+static int dmvr_subpel_1d(int64_t s0, int64_t s1, int64_t s2) {
+    const int64_t den = s1 + s2 - 2 * s0;
+    if (den == 0) return 0;
+    if (s1 != s0 && s2 != s0) {
+        const int64_t num = (s1 - s2) * 8; /* ((s1 - s2) << 4) >> 1, exact */
+        int64_t rem = num < 0 ? -num : num;
+        int q = 0;
+        for (int i = 2; i >= 0; i--) {
+            const int64_t div = den * (1 << i);
+            if (rem >= div) {
+                rem -= div;
+                q += 1 << i;
+            }
+        }
+        return num < 0 ? -q : q;
+    }
+    return s1 == s0 ? -8 : 8;
+}
+
+/* bilinear_pred of refine_dmvr_subblock: BILINEAR_FILTER_PREC4 taps (16 - f, f)
+ * (inter.rs:106-123) at the MV's fraction, offset (ox, oy) integer samples. */
+static void dmvr_bilinear(const uint16_t* pl, int stride, int pw, int ph, int bx, int by, int dx, int dy, int mvx,
+                          int mvy, int ox, int oy, int32_t* out) {
+    const int ix = bx + asr(mvx, 4) + ox, iy = by + asr(mvy, 4) + oy;
+    const int fx = mvx & 15, fy = mvy & 15;
+    for (int y = 0; y < dy; y++)
+        for (int x = 0; x < dx; x++) {
+            const int px = ix + x, py = iy + y;
+            const int h0 = (smp(pl, stride, pw, ph, px, py) * (16 - fx) + smp(pl, stride, pw, ph, px + 1, py) * fx + 8) >> 4;
+            const int h1 =
+                (smp(pl, stride, pw, ph, px, py + 1) * (16 - fx) + smp(pl, stride, pw, ph, px + 1, py + 1) * fx + 8) >> 4;
+            out[y * dx + x] = (h0 * (16 - fy) + h1 * fy + 8) >> 4;
+        }
+}
+
+/* refine_dmvr_subblock (inter.rs:1371-1459) over a dx x dy block at (bx, by):
+ * bilateral SAD on every second row over the 5x5 integer offsets (L0 + o,
+ * L1 - o), the centre cost reduced by a quarter, early exit when it is below
+ * dx * dy, then the parabolic sub-sample step around a best offset inside the
+ * 3x3 core. mv = {mv0_x, mv0_y, mv1_x, mv1_y}, refined in place (clipped to
+ * the 18-bit storage range). */
+void orc_dmvr_refine(const uint16_t* r0, const uint16_t* r1, int stride, int pw, int ph, int bx, int by, int dx,
+                     int dy, int32_t* mv) {
+    int32_t* a = malloc(sizeof(int32_t) * (size_t)(dx * dy) * 2);
+    int32_t* b = a + dx * dy;
+    int64_t sads[5][5];
+    for (int oy = -2; oy <= 2; oy++)
+        for (int ox = -2; ox <= 2; ox++) {
+            dmvr_bilinear(r0, stride, pw, ph, bx, by, dx, dy, mv[0], mv[1], ox, oy, a);
+            dmvr_bilinear(r1, stride, pw, ph, bx, by, dx, dy, mv[2], mv[3], -ox, -oy, b);
+            int64_t c = 0;
+            for (int y = 0; y < dy; y += 2)
+                for (int x = 0; x < dx; x++) {
+                    const int d = a[y * dx + x] - b[y * dx + x];
+                    c += d < 0 ? -d : d;
+                }
+            sads[oy + 2][ox + 2] = c;
+        }
+    free(a);
+    int64_t min_cost = sads[2][2];
+    min_cost -= min_cost >> 2;
+    sads[2][2] = min_cost;
+    if (min_cost < (int64_t)dx * dy) return;
+    int bix = 0, biy = 0;
+    for (int oy = -2; oy <= 2; oy++)
+        for (int ox = -2; ox <= 2; ox++) {
+            if (ox == 0 && oy == 0) continue;
+            if (sads[oy + 2][ox + 2] < min_cost) {
+                min_cost = sads[oy + 2][ox + 2];
+                bix = ox;
+                biy = oy;
+            }
+        }
+    int ddx = bix * 16, ddy = biy * 16;
+    if (bix > -2 && bix < 2 && biy > -2 && biy < 2) {
+        const int ux = bix + 2, uy = biy + 2;
+        ddx += dmvr_subpel_1d(sads[uy][ux], sads[uy][ux - 1], sads[uy][ux + 1]);
+        ddy += dmvr_subpel_1d(sads[uy][ux], sads[uy - 1][ux], sads[uy + 1][ux]);
+    }
+    mv[0] = clampi(mv[0] + ddx, -(1 << 17), (1 << 17) - 1);
+    mv[1] = clampi(mv[1] + ddy, -(1 << 17), (1 << 17) - 1);
+    mv[2] = clampi(mv[2] - ddx, -(1 << 17), (1 << 17) - 1);
+    mv[3] = clampi(mv[3] - ddy, -(1 << 17), (1 << 17) - 1);
+}
 
 void orc_weighted_average_bi(const int32_t* p0, const int32_t* p1, size_t n, int bcw_idx, int bd,
                              int32_t* out) {
@@ -350,19 +523,29 @@ size_t orc_mc_bi_batch(const rmh_mc_bi_frame* frames, size_t n_frames, const rmh
         int32_t* p1 = malloc(sizeof(int32_t) * (size_t)(w * h));
         int32_t* out = malloc(sizeof(int32_t) * (size_t)(w * h));
         const rmh_plane* r = &f->ref0[0];
-        orc_interp_luma(wf[0], (int)r->width, (int)r->width, (int)r->height, bd, pu->x, pu->y, w, h, pu->mv0_x,
-                        pu->mv0_y, 1, alt, 0, p0);
-        orc_interp_luma(wf[3], (int)r->width, (int)r->width, (int)r->height, bd, pu->x, pu->y, w, h, pu->mv1_x,
-                        pu->mv1_y, 1, alt, 0, p1);
-        orc_weighted_average_bi(p0, p1, (size_t)(w * h), pu->bcw_idx, bd, out);
+        /* the refinements of predict_inter_pu's bi branch (inter.rs:1763-1800):
+         * DMVR with default BCW weights on blocks of at least 8x8 and 128
+         * samples, then BDOF with default weights; the caller's flags stand for
+         * the enables and the POC conditions */
+        int32_t mv[4] = {pu->mv0_x, pu->mv0_y, pu->mv1_x, pu->mv1_y};
+        if ((pu->refine & RMH_BI_DMVR) && pu->bcw_idx == BCW_DEFAULT && w >= 8 && h >= 8 && w * h >= 128)
+            orc_dmvr_refine(wf[0], wf[3], (int)r->width, (int)r->width, (int)r->height, pu->x, pu->y, w, h, mv);
+        orc_interp_luma(wf[0], (int)r->width, (int)r->width, (int)r->height, bd, pu->x, pu->y, w, h, mv[0], mv[1], 1,
+                        alt, 0, p0);
+        orc_interp_luma(wf[3], (int)r->width, (int)r->width, (int)r->height, bd, pu->x, pu->y, w, h, mv[2], mv[3], 1,
+                        alt, 0, p1);
+        if ((pu->refine & RMH_BI_BDOF) && pu->bcw_idx == BCW_DEFAULT)
+            orc_bdof_luma(p0, p1, w, h, out);
+        else
+            orc_weighted_average_bi(p0, p1, (size_t)(w * h), pu->bcw_idx, bd, out);
         put_block(&f->dst[0], pu->x, pu->y, w, h, out);
         int cw = w / 2, ch = h / 2;
         for (int c = 1; c < 3; c++) {
             const rmh_plane* rc = &f->ref0[c];
             orc_interp_chroma(wf[c], (int)rc->width, (int)rc->width, (int)rc->height, bd, pu->x / 2, pu->y / 2, cw,
-                              ch, pu->mv0_x, pu->mv0_y, 1, p0);
+                              ch, mv[0], mv[1], 1, p0);
             orc_interp_chroma(wf[3 + c], (int)rc->width, (int)rc->width, (int)rc->height, bd, pu->x / 2, pu->y / 2,
-                              cw, ch, pu->mv1_x, pu->mv1_y, 1, p1);
+                              cw, ch, mv[2], mv[3], 1, p1);
             orc_weighted_average_bi(p0, p1, (size_t)(cw * ch), pu->bcw_idx, bd, out);
             put_block(&f->dst[c], pu->x / 2, pu->y / 2, cw, ch, out);
         }

@@ -63,6 +63,8 @@ def _setup(L):
     L.orc_interp_chroma.argtypes = [U16P] + [c_int] * 11 + [I32P]
     L.orc_interp_luma_16_as_written.argtypes = [U16P] + [c_int] * 11 + [I32P]
     L.orc_weighted_average_bi.argtypes = [I32P, I32P, c_size, c_int, c_int, I32P]
+    L.orc_bdof_luma.argtypes = [I32P, I32P, c_int, c_int, I32P]
+    L.orc_dmvr_refine.argtypes = [U16P, U16P] + [c_int] * 7 + [I32P]
     L.orc_recon_batch.argtypes = [ctypes.c_void_p, c_size, I16P, c_size, ctypes.c_void_p, ctypes.c_void_p, c_size]
     L.orc_recon_batch.restype = c_size
     L.orc_mc_batch.argtypes = [ctypes.c_void_p, c_size, ctypes.c_void_p, c_size]
@@ -249,6 +251,26 @@ def weighted_average_bi(p0, p1, bcw_idx, bd):
     out = np.zeros_like(a)
     lib().orc_weighted_average_bi(_p(a, I32P), _p(b, I32P), a.size, bcw_idx, bd, _p(out, I32P))
     return out
+
+
+def bdof_luma(p0, p1, w, h):
+    """apply_bdof_luma of two (h, w) 14-bit intermediate blocks."""
+    a = np.ascontiguousarray(np.asarray(p0, np.int32).reshape(-1))
+    b = np.ascontiguousarray(np.asarray(p1, np.int32).reshape(-1))
+    out = np.zeros(w * h, np.int32)
+    assert lib().orc_bdof_luma(_p(a, I32P), _p(b, I32P), w, h, _p(out, I32P)) == 0
+    return out.reshape(h, w)
+
+
+def dmvr_refine(ref0, ref1, bx, by, dx, dy, mv0, mv1):
+    """refine_dmvr_subblock over a dx x dy block: the refined (mv0, mv1)."""
+    r0 = np.ascontiguousarray(ref0, dtype=np.uint16)
+    r1 = np.ascontiguousarray(ref1, dtype=np.uint16)
+    assert r0.shape == r1.shape
+    mv = np.array([mv0[0], mv0[1], mv1[0], mv1[1]], np.int32)
+    lib().orc_dmvr_refine(_p(r0, U16P), _p(r1, U16P), r0.shape[1], r0.shape[1], r0.shape[0], bx, by, dx, dy,
+                          _p(mv, I32P))
+    return (int(mv[0]), int(mv[1])), (int(mv[2]), int(mv[3]))
 
 
 def deblock_luma_edge(lines, qp_p, qp_q, bs, beta_off, tc_off, bd, len_p, len_q, at_ctb=False):

@@ -87,6 +87,11 @@ size_t orc_mc_batch(const rmh_mc_frame* frames, size_t n_frames, const rmh_pu_de
 /* predict_inter_pu bi-prediction without DMVR and BDOF (vvc/inter.rs:1753-1801):
  * both lists with for_bi_pred, weighted_average_bi_16 (vvc/inter.rs:2676-2696),
  * then write_prediction, over a bi PU list. */
+/* apply_bdof_luma (8-bit): dst = BDOF of the 14-bit intermediates of a w x h block. */
+int orc_bdof_luma(const int32_t* pred0, const int32_t* pred1, int w, int h, int32_t* dst);
+/* refine_dmvr_subblock over a dx x dy block; mv = {mv0_x, mv0_y, mv1_x, mv1_y} in and out. */
+void orc_dmvr_refine(const uint16_t* r0, const uint16_t* r1, int stride, int pw, int ph, int bx, int by, int dx,
+                     int dy, int32_t* mv);
 size_t orc_mc_bi_batch(const rmh_mc_bi_frame* frames, size_t n_frames, const rmh_bi_pu_desc* pus,
                        size_t n_pu);
 

@@ -44,10 +44,12 @@ class McBiFrame(ctypes.Structure):
     _fields_ = [("ref0", Plane * 3), ("ref1", Plane * 3), ("dst", Plane * 3)]
 
 
-# rmh_bi_pu_desc (28 bytes); bcw_idx 0..4 selects the L1 weight of {-2, 3, 4, 5, 10} / 8
+# rmh_bi_pu_desc (28 bytes); bcw_idx 0..4 selects the L1 weight of {-2, 3, 4, 5, 10} / 8;
+# refine: BI_DMVR | BI_BDOF (8-bit frames)
+BI_DMVR, BI_BDOF = 1, 2
 BI_PU_DTYPE = np.dtype([("x", "<u2"), ("y", "<u2"), ("log2_w", "u1"), ("log2_h", "u1"), ("frame", "<u2"),
                         ("mv0_x", "<i4"), ("mv0_y", "<i4"), ("mv1_x", "<i4"), ("mv1_y", "<i4"),
-                        ("bcw_idx", "u1"), ("reserved", "u1", (3,))])
+                        ("bcw_idx", "u1"), ("refine", "u1"), ("reserved", "u1", (2,))])
 assert BI_PU_DTYPE.itemsize == 28
 BCW_DEFAULT = 2
 

@@ -146,6 +146,34 @@ struct McBiItem {
 hipError_t launch_mc_bi(const rmh_mc_bi_frame* frames, const McBiItem* items, uint32_t n_items, uint32_t n_items8,
                         hipStream_t stream);
 
+// DMVR of one bi PU (8-bit frames): the signalled MV pair, refined into
+// slot `slot` of the plan's MV array (4 int32 per slot).
+struct DmvrJob {
+    uint16_t x, y;    // PU origin (luma)
+    uint8_t lw, lh;   // log2 PU size (3..7)
+    uint16_t frame;
+    uint32_t slot;
+    int32_t mv[4];    // mv0_x, mv0_y, mv1_x, mv1_y
+};
+
+// One 16x16 luma tile (with its 8x8 chroma tiles) of a bi PU that DMVR or
+// BDOF refines; its MV pair is read from slot `slot` of the MV array.
+struct McRefItem {
+    uint16_t px, py;  // PU origin
+    uint8_t lw, lh;   // log2 PU size
+    uint8_t tx, ty;   // tile index in the PU (16-sample units)
+    uint16_t frame;
+    uint8_t flags;    // bit 0: alternative half-sample luma filter; bit 1: BDOF
+    int8_t w1;        // L1 weight
+    uint32_t slot;
+};
+
+// lds_bytes: 2 (w + 4) (h + 4) for the largest PU of the jobs.
+hipError_t launch_dmvr(const rmh_mc_bi_frame* frames, const DmvrJob* jobs, uint32_t n_jobs, uint32_t lds_bytes,
+                       int32_t* mvs, hipStream_t stream);
+hipError_t launch_mc_bi_refined(const rmh_mc_bi_frame* frames, const McRefItem* items, uint32_t n_items,
+                                const int32_t* mvs, hipStream_t stream);
+
 // Deblocking: a copy of every plane into `shadows` (3 per frame, same sizes,
 // device memory), then one tiled launch that reads the shadows and writes the
 // pictures. The frames are taken in the order frame_of[n_frames] (device

@@ -190,6 +190,14 @@ struct rmh_plan {
     const rmh_mc_bi_frame* d_bi_frames = nullptr;  // bi-prediction plans
     const rmh::McBiItem* d_bi_items = nullptr;
     size_t n_items = 0, n_items8 = 0;  // items [0, n_items8) are on 8-bit frames
+    // bi PUs with DMVR or BDOF: the DMVR jobs, the tiles of the refined PUs
+    // and their MV pairs (4 int32 per PU, the signalled pairs until DMVR runs)
+    const rmh::DmvrJob* d_dmvr = nullptr;
+    size_t n_dmvr = 0;
+    uint32_t dmvr_lds = 0;
+    const rmh::McRefItem* d_ref_items = nullptr;
+    size_t n_ref_items = 0;
+    int32_t* d_mvs = nullptr;
     // deblock
     const rmh_dbk_frame* d_dbk = nullptr;
     const rmh_plane* d_shadows = nullptr;  // 3 per frame: copies the tiled kernel reads
@@ -654,8 +662,12 @@ rmh_status validate_mc_bi(rmh_ctx* ctx, const rmh_mc_bi_frame* frames, size_t n_
         if (p.x >= y.width || p.y >= y.height) return fail_locked(ctx, RMH_INVALID_INPUT, tag + "outside the picture");
         for (int32_t m : {p.mv0_x, p.mv0_y, p.mv1_x, p.mv1_y})
             if (m < -(1 << 17) || m >= (1 << 17)) return fail_locked(ctx, RMH_INVALID_INPUT, tag + "MV outside 18-bit storage");
-        if (p.reserved[0] | p.reserved[1] | p.reserved[2])
+        if (p.reserved[0] | p.reserved[1])
             return fail_locked(ctx, RMH_INVALID_INPUT, tag + "reserved bytes must be zero");
+        if (p.refine & ~(RMH_BI_DMVR | RMH_BI_BDOF))
+            return fail_locked(ctx, RMH_INVALID_INPUT, tag + "unknown refinement flags");
+        if (p.refine && frames[p.frame].ref0[0].bit_depth != 8)
+            return fail_locked(ctx, RMH_INVALID_INPUT, tag + "DMVR / BDOF are defined for 8-bit frames only");
     }
     return RMH_OK;
 }
@@ -847,10 +859,21 @@ std::vector<rmh::McItem> mc_items(const rmh_mc_frame* frames, const rmh_pu_desc*
 // weighted_average_bi does.
 constexpr int8_t kBcwWeights[5] = {-2, 3, 4, 5, 10};
 
+// The refinements predict_inter_pu applies to a bi PU (vvc/inter.rs:1763-1800):
+// DMVR with default weights on blocks of at least 8x8 and 128 samples, BDOF
+// with default weights; the flags carry the caller's enables and POC tests.
+bool dmvr_applies(const rmh_bi_pu_desc& p) {
+    const int w = 1 << (p.log2_w & 15), h = 1 << (p.log2_h & 15);
+    return (p.refine & RMH_BI_DMVR) && p.bcw_idx == 2 && w >= 8 && h >= 8 && w * h >= 128;
+}
+bool bdof_applies(const rmh_bi_pu_desc& p) { return (p.refine & RMH_BI_BDOF) && p.bcw_idx == 2; }
+bool refined(const rmh_bi_pu_desc& p) { return dmvr_applies(p) || bdof_applies(p); }
+
 std::vector<rmh::McBiItem> mc_bi_items(const rmh_mc_bi_frame* frames, const rmh_bi_pu_desc* pus, size_t n_pu) {
     std::vector<rmh::McBiItem> items;
     for (size_t i = 0; i < n_pu; i++) {
         const rmh_bi_pu_desc& pu = pus[i];
+        if (refined(pu)) continue;
         pu_tiles(frames[pu.frame].dst, pu.x, pu.y, 1 << (pu.log2_w & 15), 1 << (pu.log2_h & 15),
                  [&](int x0, int y0, int tw, int th, int cw, int ch) {
                      rmh::McBiItem it{};
@@ -871,6 +894,62 @@ std::vector<rmh::McBiItem> mc_bi_items(const rmh_mc_bi_frame* frames, const rmh_
                  });
     }
     return items;
+}
+
+// The work of the refined bi PUs: DMVR jobs, prediction tiles and the
+// initial MV pairs (one slot per refined PU).
+struct BiRefineWork {
+    std::vector<rmh::DmvrJob> jobs;
+    std::vector<rmh::McRefItem> items;
+    std::vector<int32_t> mvs;
+    uint32_t lds = 0;  // dynamic LDS of the DMVR launch
+};
+
+BiRefineWork bi_refine_work(const rmh_mc_bi_frame* frames, const rmh_bi_pu_desc* pus, size_t n_pu) {
+    BiRefineWork wk;
+    for (size_t i = 0; i < n_pu; i++) {
+        const rmh_bi_pu_desc& pu = pus[i];
+        if (!refined(pu)) continue;
+        const uint32_t slot = (uint32_t)(wk.mvs.size() / 4);
+        const int lw = pu.log2_w & 15, lh = pu.log2_h & 15, w = 1 << lw, h = 1 << lh;
+        wk.mvs.insert(wk.mvs.end(), {pu.mv0_x, pu.mv0_y, pu.mv1_x, pu.mv1_y});
+        if (dmvr_applies(pu)) {
+            rmh::DmvrJob j{};
+            j.x = pu.x;
+            j.y = pu.y;
+            j.lw = (uint8_t)lw;
+            j.lh = (uint8_t)lh;
+            j.frame = pu.frame;
+            j.slot = slot;
+            j.mv[0] = pu.mv0_x;
+            j.mv[1] = pu.mv0_y;
+            j.mv[2] = pu.mv1_x;
+            j.mv[3] = pu.mv1_y;
+            wk.jobs.push_back(j);
+            wk.lds = std::max(wk.lds, (uint32_t)(2 * (w + 4) * (h + 4)));
+        }
+        const rmh_plane* dst = frames[pu.frame].dst;
+        for (int ty = 0; ty < (h + 15) / 16; ty++)
+            for (int tx = 0; tx < (w + 15) / 16; tx++) {
+                const int x0 = pu.x + 16 * tx, y0 = pu.y + 16 * ty;
+                const bool luma = x0 < (int)dst[0].width && y0 < (int)dst[0].height;
+                const bool chroma = x0 / 2 < (int)dst[1].width && y0 / 2 < (int)dst[1].height;
+                if (!luma && !chroma) continue;
+                rmh::McRefItem it{};
+                it.px = pu.x;
+                it.py = pu.y;
+                it.lw = (uint8_t)lw;
+                it.lh = (uint8_t)lh;
+                it.tx = (uint8_t)tx;
+                it.ty = (uint8_t)ty;
+                it.frame = pu.frame;
+                it.flags = (uint8_t)(((pu.log2_w & RMH_PU_ALT_HPEL) ? 1 : 0) | (bdof_applies(pu) ? 2 : 0));
+                it.w1 = kBcwWeights[pu.bcw_idx < 4 ? pu.bcw_idx : 4];
+                it.slot = slot;
+                wk.items.push_back(it);
+            }
+    }
+    return wk;
 }
 
 uint64_t mc_bytes(const rmh_mc_frame* frames, const rmh_pu_desc* pus, size_t n_pu) {
@@ -1214,10 +1293,18 @@ rmh_status run_plan_locked_free(rmh_plan* plan, const int16_t* d_coeff, int16_t*
             e = rmh::launch_residual(a, s);
         }
     } else if (plan->kind == kMc) {
-        e = plan->d_bi_items ? rmh::launch_mc_bi(plan->d_bi_frames, plan->d_bi_items, (uint32_t)plan->n_items,
-                                                 (uint32_t)plan->n_items8, s)
-                             : rmh::launch_mc(plan->d_frames, plan->d_items, (uint32_t)plan->n_items,
-                                              (uint32_t)plan->n_items8, s);
+        if (plan->d_bi_frames) {
+            // DMVR writes the refined pairs that the refined PUs' prediction reads
+            e = rmh::launch_dmvr(plan->d_bi_frames, plan->d_dmvr, (uint32_t)plan->n_dmvr, plan->dmvr_lds, plan->d_mvs, s);
+            if (e == hipSuccess)
+                e = rmh::launch_mc_bi_refined(plan->d_bi_frames, plan->d_ref_items, (uint32_t)plan->n_ref_items,
+                                              plan->d_mvs, s);
+            if (e == hipSuccess)
+                e = rmh::launch_mc_bi(plan->d_bi_frames, plan->d_bi_items, (uint32_t)plan->n_items,
+                                      (uint32_t)plan->n_items8, s);
+        } else {
+            e = rmh::launch_mc(plan->d_frames, plan->d_items, (uint32_t)plan->n_items, (uint32_t)plan->n_items8, s);
+        }
     } else if (plan->kind == kSao) {
         e = rmh::launch_sao(plan->d_sao_jobs, plan->d_sao_block_job, plan->sao_range, s);
     } else if (plan->kind == kEdge) {
@@ -1698,7 +1785,7 @@ RMH_EXPORT rmh_status rmh_mc_plan_create(rmh_ctx* ctx, const rmh_mc_frame* frame
 
 RMH_EXPORT rmh_status rmh_mc_plan_run(rmh_plan* plan, void* stream) {
     if (!plan || plan->kind != kMc) return RMH_INVALID_INPUT;
-    if (plan->n_items == 0) return RMH_OK;
+    if (plan->n_items == 0 && plan->n_ref_items == 0) return RMH_OK;
     return run_plan(plan, nullptr, nullptr, 0, 0, stream);
 }
 
@@ -1796,26 +1883,41 @@ RMH_EXPORT rmh_status rmh_mc_bi_plan_create(rmh_ctx* ctx, const rmh_mc_bi_frame*
     if (ctx->disabled.load()) return fail(ctx, RMH_UNAVAILABLE, "accelerator disabled");
     DeviceGuard dg(ctx->device);
     std::vector<rmh::McBiItem> items = mc_bi_items(frames, pus, n_pu);
-    if (items.size() > 0x7fffffffu) return fail(ctx, RMH_LIMIT_EXCEEDED, "too many inter work items");
+    BiRefineWork rw = bi_refine_work(frames, pus, n_pu);
+    if (items.size() > 0x7fffffffu || rw.items.size() > 0x7fffffffu)
+        return fail(ctx, RMH_LIMIT_EXCEEDED, "too many inter work items");
     const size_t n8 = order_by_sample_size(items, frames);
     size_t fb = n_frames * sizeof(rmh_mc_bi_frame), ib = items.size() * sizeof(rmh::McBiItem);
+    size_t jb = rw.jobs.size() * sizeof(rmh::DmvrJob), rb = rw.items.size() * sizeof(rmh::McRefItem);
+    size_t mb = rw.mvs.size() * sizeof(int32_t);
     rmh_plan* p = new rmh_plan();
     p->ctx = ctx;
     p->kind = kMc;
     p->n_items = items.size();
     p->n_items8 = n8;
+    p->n_dmvr = rw.jobs.size();
+    p->dmvr_lds = rw.lds;
+    p->n_ref_items = rw.items.size();
     p->bytes = mc_bi_bytes(frames, pus, n_pu);
-    hipError_t e = hipMalloc(&p->d_mem, aligned(fb) + aligned(ib) + 256);
+    const size_t o_items = aligned(fb), o_jobs = o_items + aligned(ib), o_ref = o_jobs + aligned(jb);
+    const size_t o_mvs = o_ref + aligned(rb);
+    hipError_t e = hipMalloc(&p->d_mem, o_mvs + aligned(mb) + 256);
     char* base = (char*)p->d_mem;
     if (e == hipSuccess && fb) e = hipMemcpy(base, frames, fb, hipMemcpyHostToDevice);
-    if (e == hipSuccess && ib) e = hipMemcpy(base + aligned(fb), items.data(), ib, hipMemcpyHostToDevice);
+    if (e == hipSuccess && ib) e = hipMemcpy(base + o_items, items.data(), ib, hipMemcpyHostToDevice);
+    if (e == hipSuccess && jb) e = hipMemcpy(base + o_jobs, rw.jobs.data(), jb, hipMemcpyHostToDevice);
+    if (e == hipSuccess && rb) e = hipMemcpy(base + o_ref, rw.items.data(), rb, hipMemcpyHostToDevice);
+    if (e == hipSuccess && mb) e = hipMemcpy(base + o_mvs, rw.mvs.data(), mb, hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         if (p->d_mem) hipFree(p->d_mem);
         delete p;
         return fail(ctx, RMH_EXECUTION, std::string("bi-prediction plan upload failed: ") + hipGetErrorString(e));
     }
     p->d_bi_frames = (const rmh_mc_bi_frame*)base;
-    p->d_bi_items = (const rmh::McBiItem*)(base + aligned(fb));
+    p->d_bi_items = (const rmh::McBiItem*)(base + o_items);
+    p->d_dmvr = (const rmh::DmvrJob*)(base + o_jobs);
+    p->d_ref_items = (const rmh::McRefItem*)(base + o_ref);
+    p->d_mvs = (int32_t*)(base + o_mvs);
     *out = p;
     return RMH_OK;
 }
@@ -1831,9 +1933,12 @@ RMH_EXPORT rmh_status rmh_mc_bi_batch(rmh_ctx* ctx, const rmh_mc_bi_frame* frame
     if (g != RMH_OK) return fail_locked(ctx, g, g == RMH_UNAVAILABLE ? "accelerator disabled" : "injected device failure");
     DeviceGuard dg(ctx->device);
     std::vector<rmh::McBiItem> items = mc_bi_items(frames, pus, n_pu);
+    BiRefineWork rw = bi_refine_work(frames, pus, n_pu);
     const size_t n8 = order_by_sample_size(items, frames);
     size_t fb = n_frames * sizeof(rmh_mc_bi_frame), ib = items.size() * sizeof(rmh::McBiItem);
-    size_t need = aligned(fb) + aligned(ib), hneed = 0;
+    size_t jb = rw.jobs.size() * sizeof(rmh::DmvrJob), rb = rw.items.size() * sizeof(rmh::McRefItem);
+    size_t mb = rw.mvs.size() * sizeof(int32_t);
+    size_t need = aligned(fb) + aligned(ib) + aligned(jb) + aligned(rb) + aligned(mb), hneed = 0;
     for (size_t f = 0; f < n_frames; f++)
         for (int c = 0; c < 3; c++) {
             need += 3 * aligned(plane_bytes(frames[f].ref0[c]));
@@ -1846,6 +1951,9 @@ RMH_EXPORT rmh_status rmh_mc_bi_batch(rmh_ctx* ctx, const rmh_mc_bi_frame* frame
     hipStream_t s = ctx->stream;
     rmh_mc_bi_frame* d_frames = (rmh_mc_bi_frame*)ctx->dev.take(fb);
     rmh::McBiItem* d_items = (rmh::McBiItem*)ctx->dev.take(ib);
+    rmh::DmvrJob* d_jobs = (rmh::DmvrJob*)ctx->dev.take(jb);
+    rmh::McRefItem* d_ref = (rmh::McRefItem*)ctx->dev.take(rb);
+    int32_t* d_mvs = (int32_t*)ctx->dev.take(mb);
     std::vector<rmh_mc_bi_frame> dev_frames(frames, frames + n_frames);
     auto upload = [&](const rmh_plane& r, rmh_plane& d) -> hipError_t {
         void* dr = ctx->dev.take(plane_bytes(r));
@@ -1864,9 +1972,14 @@ RMH_EXPORT rmh_status rmh_mc_bi_batch(rmh_ctx* ctx, const rmh_mc_bi_frame* frame
             dev_frames[f].dst[c].stride = frames[f].dst[c].width;
         }
     if ((e = hipMemcpyAsync(d_frames, dev_frames.data(), fb, hipMemcpyHostToDevice, s)) != hipSuccess ||
-        (e = hipMemcpyAsync(d_items, items.data(), ib, hipMemcpyHostToDevice, s)) != hipSuccess)
+        (ib && (e = hipMemcpyAsync(d_items, items.data(), ib, hipMemcpyHostToDevice, s)) != hipSuccess) ||
+        (jb && (e = hipMemcpyAsync(d_jobs, rw.jobs.data(), jb, hipMemcpyHostToDevice, s)) != hipSuccess) ||
+        (rb && (e = hipMemcpyAsync(d_ref, rw.items.data(), rb, hipMemcpyHostToDevice, s)) != hipSuccess) ||
+        (mb && (e = hipMemcpyAsync(d_mvs, rw.mvs.data(), mb, hipMemcpyHostToDevice, s)) != hipSuccess))
         return hip_fail_locked(ctx, e, "upload");
-    e = rmh::launch_mc_bi(d_frames, d_items, (uint32_t)items.size(), (uint32_t)n8, s);
+    e = rmh::launch_dmvr(d_frames, d_jobs, (uint32_t)rw.jobs.size(), rw.lds, d_mvs, s);
+    if (e == hipSuccess) e = rmh::launch_mc_bi_refined(d_frames, d_ref, (uint32_t)rw.items.size(), d_mvs, s);
+    if (e == hipSuccess) e = rmh::launch_mc_bi(d_frames, d_items, (uint32_t)items.size(), (uint32_t)n8, s);
     if (e != hipSuccess) return hip_fail_locked(ctx, e, "kernel launch");
     std::vector<void*> hdst;
     for (size_t f = 0; f < n_frames; f++)
@@ -2843,7 +2956,7 @@ RMH_EXPORT rmh_status rmh_graph_create(rmh_ctx* ctx, const rmh_graph_node* nodes
     if (e == hipSuccess) {
         for (size_t i = 0; i < n_nodes && st == RMH_OK; i++) {
             rmh_plan* p = nodes[i].plan;
-            if (((p->kind == kResidual || p->kind == kForward) && p->n_tu == 0) || (p->kind == kMc && p->n_items == 0) ||
+            if (((p->kind == kResidual || p->kind == kForward) && p->n_tu == 0) || (p->kind == kMc && p->n_items == 0 && p->n_ref_items == 0) ||
                 (p->kind == kDeblock && p->n_dbk == 0) || (p->kind == kSao && p->n_sao_jobs == 0) ||
                 (p->kind == kEdge && p->total_cells == 0) || (p->kind == kLmcs && p->lmcs_range[2] == 0) ||
                 (p->kind == kAlf && p->alf_luma_range[2] == 0))

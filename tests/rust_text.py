@@ -31,7 +31,7 @@ import types
 _METHODS = ("clamp", "abs", "min", "max", "iter", "zip", "enumerate", "map", "collect", "to_vec", "clone", "len",
             "sum", "saturating_sub", "trailing_zeros", "fill", "step_by", "wrapping_sub", "get", "copied",
             "unwrap_or", "div_ceil", "copy_from_slice", "expect", "is_empty", "first", "and_then", "as_ref", "rev",
-            "take", "ok_or", "map_err", "signum", "contains")
+            "take", "ok_or", "map_err", "signum", "contains", "leading_zeros")
 
 
 def fn_source(text, name, after=None):
@@ -183,6 +183,8 @@ def _methods(s):
             rep = "_sum(%s)" % recv
         elif name == "saturating_sub":
             rep = "_max(%s - (%s), 0)" % (recv, args)
+        elif name == "leading_zeros":
+            rep = "(32 - (%s).bit_length())" % recv  # of a non-negative u32
         else:
             rep = "((%s) & -(%s)).bit_length() - 1" % (recv, recv)
         s = s[:start] + rep + s[close + 1:]
@@ -607,7 +609,7 @@ def block(body, indent="    ", tail=None):
                     raise
                 # arms with statements: the match / if as statements assigning the binding
                 out += block(rhs.strip(), indent, _pattern(lhs))
-        elif re.match(r"[\w\[\]\s\-\+\*]+=[^=]", stmt) and not re.match(r"[\w\s]+[<>!=]=", stmt):
+        elif re.match(r"[\w\[\]\s\-\+\*()]+=[^=]", stmt) and not re.match(r"[\w\s]+[<>!=]=", stmt):
             lhs, rhs = stmt.split("=", 1)
             lhs = re.sub(r"\s+as\s+(?:usize|i32|i64|u32|u8|i8|u16)\b", "", lhs.strip())
             out.append(indent + "%s = (%s)" % (lhs, expr(rhs)))

@@ -132,7 +132,15 @@ def test_bi_invalid_descriptors_are_rejected(acc, rmh):
     dst = [np.zeros_like(p) for p in d["ref0"]]
     f, keep = mc_bi_frame_struct(d["ref0"], d["ref1"], dst, 10)
     pus = d["pus"].copy()
-    pus["reserved"][0] = (1, 0, 0)
+    pus["reserved"][0] = (1, 0)
+    with pytest.raises(rmh.InvalidInput):
+        acc.predict_bi([f], pus)
+    pus = d["pus"].copy()
+    pus["refine"][0] = W.BI_BDOF  # the refinements exist for 8-bit frames only
+    with pytest.raises(rmh.InvalidInput):
+        acc.predict_bi([f], pus)
+    pus = d["pus"].copy()
+    pus["refine"][0] = 4
     with pytest.raises(rmh.InvalidInput):
         acc.predict_bi([f], pus)
     pus = d["pus"].copy()

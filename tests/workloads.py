@@ -32,8 +32,9 @@ assert PU_DTYPE.itemsize == 16
 
 BI_PU_DTYPE = np.dtype([("x", "<u2"), ("y", "<u2"), ("log2_w", "u1"), ("log2_h", "u1"), ("frame", "<u2"),
                         ("mv0_x", "<i4"), ("mv0_y", "<i4"), ("mv1_x", "<i4"), ("mv1_y", "<i4"),
-                        ("bcw_idx", "u1"), ("reserved", "u1", (3,))])
+                        ("bcw_idx", "u1"), ("refine", "u1"), ("reserved", "u1", (2,))])
 assert BI_PU_DTYPE.itemsize == 28
+BI_DMVR, BI_BDOF = 1, 2
 
 EDGE_CELL_DTYPE = np.dtype([("tu_y", "<u4"), ("tu_c", "<u4"), ("tu_wy", "u1"), ("tu_hy", "u1"), ("tu_wc", "u1"),
                             ("tu_hc", "u1"), ("flags", "u1"), ("reserved", "u1", (3,)), ("mv_x", "<i4"),
