@@ -364,12 +364,22 @@ def bench_streams(rmh, acc, PL, po, steps, reps):
     return out
 
 
-def bench_bi(rmh, acc, W, po, n_frames, steps, warmup):
+def bench_bi(rmh, acc, W, po, n_frames, steps, warmup, refine=False):
     """Bi-prediction with BCW weights over n_frames 1080p10 frames (two
-    references each, one bi PU per CU): one launch of the bi plan."""
+    references each, one bi PU per CU): one launch of the bi plan. refine:
+    8-bit frames with DMVR and BDOF on every PU (default BCW weights, so both
+    apply wherever the PU size allows)."""
     hip, abi = rmh.hip, rmh.abi
     t0 = time.perf_counter()
-    frames = [W.bi_frame(4000 + i) for i in range(n_frames)]
+    bd = 8 if refine else 10
+    dt = np.uint8 if refine else np.uint16
+    frames = [W.bi_frame(4000 + i, bit_depth=bd) for i in range(n_frames)]
+    for d in frames:
+        for k in ("ref0", "ref1"):
+            d[k] = [np.ascontiguousarray(p.astype(dt)) for p in d[k]]
+        if refine:
+            d["pus"]["refine"] = W.BI_DMVR | W.BI_BDOF
+            d["pus"]["bcw_idx"] = 2
     log("bi: generated %d frames in %.1f s" % (n_frames, time.perf_counter() - t0))
     bufs, structs = [], []
     for d in frames:
@@ -379,7 +389,7 @@ def bench_bi(rmh, acc, W, po, n_frames, steps, warmup):
             for p in d["ref0" if key == "dst" else key]:
                 b = hip.DeviceBuffer.from_array(np.zeros_like(p) if key == "dst" else p)
                 bufs.append(b)
-                ps.append(abi.plane_from_address(b.ptr, p.shape[1], p.shape[0], p.shape[1], 2, d["bit_depth"]))
+                ps.append(abi.plane_from_address(b.ptr, p.shape[1], p.shape[0], p.shape[1], p.itemsize, d["bit_depth"]))
             planes[key] = (abi.Plane * 3)(*ps)
         structs.append(abi.McBiFrame(planes["ref0"], planes["ref1"], planes["dst"]))
     pus = []
@@ -400,17 +410,25 @@ def bench_bi(rmh, acc, W, po, n_frames, steps, warmup):
         f, keep = mc_bi_frame_struct(d["ref0"], d["ref1"], want, d["bit_depth"])
         po.mc_bi_batch(f, d["pus"])
         for c in range(3):
-            got = bufs[9 * fi + 6 + c].download(np.uint16).reshape(want[c].shape)
+            got = bufs[9 * fi + 6 + c].download(dt).reshape(want[c].shape)
             if not (got == want[c]).all():
                 log("bench: bi-prediction frame %d plane %d differs from the oracle" % (fi, c))
                 return None
     ms = timed(hip, stream, lambda s: plan.run(stream=s), steps, warmup)
     px = n_frames * 1920 * 1080
-    out = {"workload": "%d frames 1920x1080 10-bit 4:2:0, one bi PU per CU (8x8..128x128), independent L0/L1 MVs "
-                       "+-64 px at any fraction, BCW weights, one launch" % n_frames,
-           "value": round(px / (ms / 1e3) / 1e6, 1), "unit": "Mpix/s", "ms_per_step": round(ms, 4),
-           "roofline": dict(stage_roofline(plan, ms), kernel="mc_bi_kernel")}
-    log("bi: %.4f ms/step %.1f Mpix/s (%.1f GB/s)" % (ms, out["value"], out["roofline"]["achieved_gbs"]))
+    if refine:
+        work = ("%d frames 1920x1080 8-bit 4:2:0, one bi PU per CU (8x8..128x128), independent L0/L1 MVs +-64 px at "
+                "any fraction, DMVR (every PU of at least 8x16) and BDOF on every PU, one plan run (DMVR launch, "
+                "then the refined prediction)" % n_frames)
+        kern = "dmvr_kernel + mc_bi_refined_kernel"
+    else:
+        work = ("%d frames 1920x1080 10-bit 4:2:0, one bi PU per CU (8x8..128x128), independent L0/L1 MVs "
+                "+-64 px at any fraction, BCW weights, one launch" % n_frames)
+        kern = "mc_bi_kernel"
+    out = {"workload": work, "value": round(px / (ms / 1e3) / 1e6, 1), "unit": "Mpix/s", "ms_per_step": round(ms, 4),
+           "roofline": dict(stage_roofline(plan, ms), kernel=kern)}
+    log("bi%s: %.4f ms/step %.1f Mpix/s (%.1f GB/s)" % (" refined" if refine else "", ms, out["value"],
+                                                        out["roofline"]["achieved_gbs"]))
     plan.close()
     for b in bufs:
         b.close()
@@ -753,6 +771,8 @@ def main():
     streams = None if args.no_streams else bench_streams(rmh, acc, PL, po, 8, max(2, args.steps // 5))
 
     bi = None if args.no_bi else bench_bi(rmh, acc, W, po, args.pipeline_frames, args.steps, args.warmup)
+    if bi is not None:
+        bi["refined"] = bench_bi(rmh, acc, W, po, args.pipeline_frames, args.steps, args.warmup, refine=True)
     tools = None if args.no_tools else bench_tool_mix(rmh, acc, W, po, 8, args.steps, args.warmup)
     sao = None if args.no_sao else bench_sao(rmh, acc, W, po, args.pipeline_frames, args.steps, args.warmup)
     learned = None if args.no_learned else bench_learned(rmh, acc, args.steps)
