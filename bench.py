@@ -137,6 +137,52 @@ def cpu_baseline(tus, coeff, luma_px, n_frames):
             "cpu_model": model, "compiler": cc}
 
 
+def placement_report(rmh, acc):
+    """The placement scheduler (rustmpeg-next_amd/scheduler.py, after
+    Scheduler::calibrate / place of the reference's Vulkan backend) calibrated
+    on this box with the CPU oracle on every host thread as the CPU path: both
+    paths' measured times at 128x128 and 1920x1080, the fitted costs and the
+    placement of each stage at both sizes. The GPU path is the host-memory
+    entry points, PCIe copies included."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    import ctypes
+    import pyoracle as po
+    from frames import dbk_frame_struct, mc_frame_struct
+    S = rmh.scheduler
+
+    def inter(work):
+        ref, pus = work
+        dst = [np.zeros_like(p) for p in ref]
+        f, keep = mc_frame_struct(ref, dst, 10)
+        arr = (type(f) * 1)(f)
+        po.lib().orc_mc_batch(ctypes.addressof(arr), 1, pus.ctypes.data, len(pus))
+        return dst
+
+    def deblock(work):
+        planes, edges, qp = work
+        out = [p.copy() for p in planes]
+        f, keep = dbk_frame_struct(out, edges, 10, (qp, qp, qp))
+        arr = (type(f) * 1)(f)
+        po.lib().orc_deblock_frames(ctypes.addressof(arr), 1)
+        return out
+
+    po.lib().orc_set_threads(host_threads())
+    try:
+        sch = S.Scheduler(acc, {S.Stage.VVC_RESIDUAL: lambda w: po.residual_batch(*w),
+                                S.Stage.VVC_INTER_PRED: inter, S.Stage.VVC_DEBLOCK: deblock})
+        sch.calibrate()
+    finally:
+        po.lib().orc_set_threads(1)
+    rep = sch.report()
+    for k, v in rep.items():
+        log("placement %s: cpu %s ns, gpu %s ns -> %s" % (k, [round(x) for x in v["measured_ns"]["cpu_ns"]],
+                                                          [round(x) for x in v["measured_ns"]["gpu_ns"]],
+                                                          v["placement"]))
+    return {"cpu_path": "oracle on %d host threads" % host_threads(),
+            "gpu_path": "host-memory entry points (PCIe copies included)", "stages": rep}
+
+
 def cpu_pipeline_baseline(PL, po, rmh, frames):
     """The CPU oracle over the P-frame pipeline (MC -> residual add ->
     deblock) of `frames`: one thread and every thread of this process (OpenMP
@@ -781,6 +827,7 @@ def main():
     if not args.no_cpu:
         cpu = cpu_baseline(tus, coeff, luma_px, args.frames)
         cpu["cores_on_host"] = os.cpu_count()
+        cpu["placement"] = placement_report(rmh, acc)
         log("cpu oracle: %s Mpix/s on %d threads, %s on one" % (cpu["value"], cpu["cores"],
                                                               cpu["single_thread"]["value"]))
 

@@ -624,3 +624,4 @@ class VvcSaoFilter:
 
 
 from . import learned  # noqa: E402
+from . import scheduler  # noqa: E402

@@ -71,6 +71,16 @@ struct Arena {
 
 size_t aligned(size_t b) { return (b + 255) & ~(size_t)255; }
 
+// A host plane into a packed device buffer (row pitch = width). Planes whose
+// rows are contiguous go as one linear copy: 2-D copies from pageable host
+// memory take a slow path in the runtime (about 9 ns per sample measured).
+hipError_t upload_plane(void* dst, const rmh_plane& p, hipStream_t s) {
+    const size_t row = (size_t)p.width * p.bytes_per_sample;
+    if (p.stride == p.width) return hipMemcpyAsync(dst, p.data, row * p.height, hipMemcpyHostToDevice, s);
+    return hipMemcpy2DAsync(dst, row, p.data, (size_t)p.stride * p.bytes_per_sample, row, p.height,
+                            hipMemcpyHostToDevice, s);
+}
+
 }  // namespace
 
 // ------------------------------------------------ no-allocation scratch
@@ -1823,8 +1833,7 @@ RMH_EXPORT rmh_status rmh_mc_batch(rmh_ctx* ctx, const rmh_mc_frame* frames, siz
             const rmh_plane& r = frames[f].ref[c];
             void* dr = ctx->dev.take(plane_bytes(r));
             void* dd = ctx->dev.take(plane_bytes(r));
-            e = hipMemcpy2DAsync(dr, (size_t)r.width * r.bytes_per_sample, r.data, (size_t)r.stride * r.bytes_per_sample,
-                                 (size_t)r.width * r.bytes_per_sample, r.height, hipMemcpyHostToDevice, s);
+            e = upload_plane(dr, r, s);
             if (e != hipSuccess) return hip_fail_locked(ctx, e, "reference upload");
             dev_frames[f].ref[c].data = dr;
             dev_frames[f].ref[c].stride = r.width;
@@ -1957,8 +1966,7 @@ RMH_EXPORT rmh_status rmh_mc_bi_batch(rmh_ctx* ctx, const rmh_mc_bi_frame* frame
     std::vector<rmh_mc_bi_frame> dev_frames(frames, frames + n_frames);
     auto upload = [&](const rmh_plane& r, rmh_plane& d) -> hipError_t {
         void* dr = ctx->dev.take(plane_bytes(r));
-        hipError_t u = hipMemcpy2DAsync(dr, (size_t)r.width * r.bytes_per_sample, r.data, (size_t)r.stride * r.bytes_per_sample,
-                                        (size_t)r.width * r.bytes_per_sample, r.height, hipMemcpyHostToDevice, s);
+        hipError_t u = upload_plane(dr, r, s);
         d.data = dr;
         d.stride = r.width;
         return u;
@@ -2346,8 +2354,7 @@ RMH_EXPORT rmh_status rmh_deblock(rmh_ctx* ctx, const rmh_dbk_frame* frames, siz
         for (int c = 0; c < 3; c++) {
             const rmh_plane& p = frames[f].planes[c];
             void* dp = ctx->dev.take(plane_bytes(p));
-            e = hipMemcpy2DAsync(dp, (size_t)p.width * p.bytes_per_sample, p.data, (size_t)p.stride * p.bytes_per_sample,
-                                 (size_t)p.width * p.bytes_per_sample, p.height, hipMemcpyHostToDevice, s);
+            e = upload_plane(dp, p, s);
             if (e != hipSuccess) return hip_fail_locked(ctx, e, "plane upload");
             df[f].planes[c].data = dp;
             df[f].planes[c].stride = p.width;
@@ -2476,9 +2483,7 @@ RMH_EXPORT rmh_status rmh_sao(rmh_ctx* ctx, const rmh_sao_frame* frames, size_t 
             d.stride = src.width;
             o.data = ctx->dev.take(plane_bytes(src));
             o.stride = src.width;
-            e = hipMemcpy2DAsync(d.data, (size_t)src.width * src.bytes_per_sample, src.data,
-                                 (size_t)src.stride * src.bytes_per_sample, (size_t)src.width * src.bytes_per_sample,
-                                 src.height, hipMemcpyHostToDevice, s);
+            e = upload_plane(d.data, src, s);
             if (e != hipSuccess) return hip_fail_locked(ctx, e, "plane upload");
             planes[f][c] = d;
             planes[f][3 + c] = o;
