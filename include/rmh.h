@@ -278,6 +278,43 @@ rmh_status rmh_mc_bi_batch(rmh_ctx* ctx, const rmh_mc_bi_frame* frames, size_t n
 rmh_status rmh_mc_bi_plan_create(rmh_ctx* ctx, const rmh_mc_bi_frame* frames, size_t n_frames,
                                  const rmh_bi_pu_desc* pus, size_t n_pu, rmh_plan** out);
 
+/* Geometric partitioning (GPM): the blend of predict_gpm_block
+ * (vvc/inter.rs:1673-1730) that the general decoder applies to a GPM coding
+ * unit's two merge-candidate predictions (decode_general.rs:2716-2741). The
+ * caller predicts the two candidates of every GPM PU into the planes pred0
+ * and pred1 (MC plans writing into those planes); this plan blends them into
+ * dst with the mask of each PU's split direction (derive_gpm_params_table,
+ * inter.rs:1641-1671, over GEO_DIS / GEO_ANGLE_TO_MASK, inter.rs:131-140):
+ * per sample w0 = clip3(0, 8, (32 + idx + 4) >> 3) from the signed distance
+ * idx of the sample to the split line, out = clip((w0 p0 + (8 - w0) p1 + 4)
+ * >> 3, 0, 255). Positions go onto the luma grid scaled by 1 << luma_scale
+ * (luma) and 1 << chroma_scale (chroma, whose blocks are the PU's halves):
+ * the reference's decoder passes 1 and 2 (RMH_GPM_*_SCALE below). Samples
+ * are 8-bit; every plane of a frame has the same size as its dst. */
+typedef struct rmh_gpm_frame {
+    rmh_plane pred0[3];  /* candidate 0's prediction (Y, Cb, Cr) */
+    rmh_plane pred1[3];  /* candidate 1's prediction */
+    rmh_plane dst[3];    /* the blended prediction */
+} rmh_gpm_frame;
+
+typedef struct rmh_gpm_pu {
+    uint16_t x;
+    uint16_t y;
+    uint8_t log2_w;      /* 3..7 */
+    uint8_t log2_h;      /* 3..7 */
+    uint16_t frame;
+    uint8_t split_dir;   /* merge_gpm_partition_idx, 0..63 */
+    uint8_t reserved[3]; /* zero */
+} rmh_gpm_pu;
+
+enum { RMH_GPM_LUMA_SCALE = 1, RMH_GPM_CHROMA_SCALE = 2 };
+
+/* Device planes; luma_scale and chroma_scale 0..3. Run with rmh_gpm_plan_run
+ * or in a graph. */
+rmh_status rmh_gpm_plan_create(rmh_ctx* ctx, const rmh_gpm_frame* frames, size_t n_frames, const rmh_gpm_pu* pus,
+                               size_t n_pu, uint32_t luma_scale, uint32_t chroma_scale, rmh_plan** out);
+rmh_status rmh_gpm_plan_run(rmh_plan* plan, void* stream);
+
 /* ------------------------------------------------------------------------ */
 /* Deblocking                                                                */
 /* ------------------------------------------------------------------------ */

@@ -557,3 +557,95 @@ size_t orc_mc_bi_batch(const rmh_mc_bi_frame* frames, size_t n_frames, const rmh
     free(wide);
     return bad;
 }
+
+/* ------------------------------------------------------------------ GPM */
+
+/* GEO_DIS and GEO_ANGLE_TO_MASK (inter.rs:131-140; VTM Rom.cpp:708-711). */
+static const int GEO_DIS[32] = {8, 8, 8, 8, 4, 4, 2, 1, 0, -1, -2, -4, -4, -8, -8, -8,
+                                -8, -8, -8, -8, -4, -4, -2, -1, 0, 1, 2, 4, 4, 8, 8, 8};
+static const int GEO_ANGLE_TO_MASK[32] = {0, -1, 1, 2, 3, 4, -1, -1, 5, -1, -1, 4, 3, 2, 1, -1,
+                                          0, -1, 1, 2, 3, 4, -1, -1, 5, -1, -1, 4, 3, 2, 1, -1};
+
+/* derive_gpm_params_table (inter.rs:1641-1671): the (angle, distance) pair of
+ * each of the 64 split directions. */
+void orc_gpm_params(uint8_t* angle, uint8_t* distance) {
+    int mode = 0;
+    for (int a = 0; a < 32; a++)
+        for (int d = 0; d < 4; d++) {
+            const int mask = GEO_ANGLE_TO_MASK[a];
+            if ((d == 0 && a >= 16) || ((d == 2 || d == 0) && (mask == 0 || mask == 5)) || mask == -1) continue;
+            if (mode < 64) {
+                angle[mode] = (uint8_t)a;
+                distance[mode] = (uint8_t)d;
+                mode++;
+            }
+        }
+}
+
+/* predict_gpm_block (inter.rs:1673-1730): the w x h blend of two 8-bit
+ * predictions with the split direction's mask, positions scaled by
+ * (1 << scale_x, 1 << scale_y) onto the luma grid. */
+int orc_gpm_blend(const int32_t* p0, const int32_t* p1, int w, int h, int split_dir, int scale_x, int scale_y,
+                  int32_t* out) {
+    uint8_t ang[64], dis[64];
+    orc_gpm_params(ang, dis);
+    const int angle = ang[split_dir & 63], distance = dis[split_dir & 63];
+    const int lw = w << scale_x, lh = h << scale_y;
+    int ox = -(lw >> 1), oy = -(lh >> 1);
+    if (distance > 0) {
+        const int am = angle % 16;
+        if (am == 8 || (am != 0 && lh >= lw)) {
+            const int d = (distance * lh) >> 3;
+            oy += angle < 16 ? d : -d;
+        } else {
+            const int d = (distance * lw) >> 3;
+            ox += angle < 16 ? d : -d;
+        }
+    }
+    const int dx = angle, dy = (dx + 8) % 32;
+    for (int y = 0; y < h; y++) {
+        const int ly = y << scale_y;
+        const int look_y = (((ly + oy) * 2) + 1) * GEO_DIS[dy];
+        for (int x = 0; x < w; x++) {
+            const int lx = x << scale_x;
+            const int widx = (((lx + ox) * 2) + 1) * GEO_DIS[dx] + look_y;
+            const int w0 = clampi((32 + widx + 4) >> 3, 0, 8);
+            out[y * w + x] = clampi((w0 * p0[y * w + x] + (8 - w0) * p1[y * w + x] + 4) >> 3, 0, 255);
+        }
+    }
+    return 0;
+}
+
+/* The GPM blend of every PU of a batch: luma blocks at (x, y), chroma at
+ * (x / 2, y / 2) with half the size, each read from the two prediction planes
+ * and written to dst (clipped to the plane), as the general decoder's GPM CU
+ * does (decode_general.rs:2716-2741). Returns 0, or 1 + the index of the first
+ * PU with a bad frame index. */
+size_t orc_gpm_batch(const rmh_gpm_frame* frames, size_t n_frames, const rmh_gpm_pu* pus, size_t n_pu, int luma_scale,
+                     int chroma_scale) {
+    for (size_t i = 0; i < n_pu; i++) {
+        const rmh_gpm_pu* pu = &pus[i];
+        if (pu->frame >= n_frames) return i + 1;
+        const rmh_gpm_frame* f = &frames[pu->frame];
+        const int w = 1 << pu->log2_w, h = 1 << pu->log2_h;
+        int32_t* a = malloc(sizeof(int32_t) * (size_t)(w * h) * 3);
+        int32_t *b = a + w * h, *o = b + w * h;
+        for (int c = 0; c < 3; c++) {
+            const int sx = c ? 1 : 0;
+            const int bw = w >> sx, bh = h >> sx, x0 = pu->x >> sx, y0 = pu->y >> sx;
+            const rmh_plane* d = &f->dst[c];
+            for (int y = 0; y < bh; y++)
+                for (int x = 0; x < bw; x++) {
+                    const int px = x0 + x < (int)d->width ? x0 + x : (int)d->width - 1;
+                    const int py = y0 + y < (int)d->height ? y0 + y : (int)d->height - 1;
+                    a[y * bw + x] = plane_get(&f->pred0[c], px, py);
+                    b[y * bw + x] = plane_get(&f->pred1[c], px, py);
+                }
+            orc_gpm_blend(a, b, bw, bh, pu->split_dir, c ? chroma_scale : luma_scale, c ? chroma_scale : luma_scale, o);
+            put_block(d, x0, y0, bw, bh, o);
+        }
+        free(a);
+    }
+    return 0;
+}
+

@@ -65,6 +65,10 @@ def _setup(L):
     L.orc_weighted_average_bi.argtypes = [I32P, I32P, c_size, c_int, c_int, I32P]
     L.orc_bdof_luma.argtypes = [I32P, I32P, c_int, c_int, I32P]
     L.orc_dmvr_refine.argtypes = [U16P, U16P] + [c_int] * 7 + [I32P]
+    L.orc_gpm_params.argtypes = [ctypes.POINTER(ctypes.c_uint8)] * 2
+    L.orc_gpm_blend.argtypes = [I32P, I32P] + [c_int] * 5 + [I32P]
+    L.orc_gpm_batch.argtypes = [ctypes.c_void_p, c_size, ctypes.c_void_p, c_size, c_int, c_int]
+    L.orc_gpm_batch.restype = c_size
     L.orc_recon_batch.argtypes = [ctypes.c_void_p, c_size, I16P, c_size, ctypes.c_void_p, ctypes.c_void_p, c_size]
     L.orc_recon_batch.restype = c_size
     L.orc_mc_batch.argtypes = [ctypes.c_void_p, c_size, ctypes.c_void_p, c_size]
@@ -271,6 +275,31 @@ def dmvr_refine(ref0, ref1, bx, by, dx, dy, mv0, mv1):
     lib().orc_dmvr_refine(_p(r0, U16P), _p(r1, U16P), r0.shape[1], r0.shape[1], r0.shape[0], bx, by, dx, dy,
                           _p(mv, I32P))
     return (int(mv[0]), int(mv[1])), (int(mv[2]), int(mv[3]))
+
+
+def gpm_params():
+    """derive_gpm_params_table: [(angle, distance)] of the 64 split directions."""
+    a = np.zeros(64, np.uint8)
+    d = np.zeros(64, np.uint8)
+    lib().orc_gpm_params(_p(a, U8P), _p(d, U8P))
+    return list(zip(a.tolist(), d.tolist()))
+
+
+def gpm_blend(p0, p1, w, h, split_dir, scale_x, scale_y):
+    """predict_gpm_block of two (h, w) 8-bit predictions."""
+    a = np.ascontiguousarray(np.asarray(p0, np.int32).reshape(-1))
+    b = np.ascontiguousarray(np.asarray(p1, np.int32).reshape(-1))
+    out = np.zeros(w * h, np.int32)
+    assert lib().orc_gpm_blend(_p(a, I32P), _p(b, I32P), w, h, split_dir, scale_x, scale_y, _p(out, I32P)) == 0
+    return out.reshape(h, w)
+
+
+def gpm_batch(frames, pus, luma_scale, chroma_scale):
+    """orc_gpm_batch over a list of GpmFrame structs (host planes)."""
+    arr = (type(frames[0]) * len(frames))(*frames)
+    pus = np.ascontiguousarray(pus)
+    return lib().orc_gpm_batch(ctypes.addressof(arr), len(frames), pus.ctypes.data, len(pus), luma_scale,
+                               chroma_scale)
 
 
 def deblock_luma_edge(lines, qp_p, qp_q, bs, beta_off, tc_off, bd, len_p, len_q, at_ctb=False):

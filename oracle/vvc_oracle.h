@@ -92,6 +92,12 @@ int orc_bdof_luma(const int32_t* pred0, const int32_t* pred1, int w, int h, int3
 /* refine_dmvr_subblock over a dx x dy block; mv = {mv0_x, mv0_y, mv1_x, mv1_y} in and out. */
 void orc_dmvr_refine(const uint16_t* r0, const uint16_t* r1, int stride, int pw, int ph, int bx, int by, int dx,
                      int dy, int32_t* mv);
+/* derive_gpm_params_table and predict_gpm_block (8-bit). */
+void orc_gpm_params(uint8_t* angle, uint8_t* distance);
+int orc_gpm_blend(const int32_t* p0, const int32_t* p1, int w, int h, int split_dir, int scale_x, int scale_y,
+                  int32_t* out);
+size_t orc_gpm_batch(const rmh_gpm_frame* frames, size_t n_frames, const rmh_gpm_pu* pus, size_t n_pu, int luma_scale,
+                     int chroma_scale);
 size_t orc_mc_bi_batch(const rmh_mc_bi_frame* frames, size_t n_frames, const rmh_bi_pu_desc* pus,
                        size_t n_pu);
 

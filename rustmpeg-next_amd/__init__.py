@@ -76,6 +76,7 @@ EXPORTED = [
     "rmh_edge_map", "rmh_edge_map_plan_create", "rmh_edge_map_plan_run", "rmh_lmcs_tables",
     "rmh_lmcs_plan_create", "rmh_lmcs_plan_run", "rmh_alf_filter_bank", "rmh_alf_plan_create", "rmh_alf_plan_run",
     "rmh_forward_plan_create", "rmh_forward_plan_run", "rmh_forward_plan_evaluate",
+    "rmh_gpm_plan_create", "rmh_gpm_plan_run",
 ]
 
 
@@ -118,6 +119,8 @@ def load_library(path=LIB_PATH):
         "rmh_forward_plan_create": ([vp, vp, sz, sz, ctypes.POINTER(vp)], st),
         "rmh_forward_plan_run": ([vp, vp, vp, vp], st),
         "rmh_forward_plan_evaluate": ([vp, vp, vp, vp, vp, vp, vp], st),
+        "rmh_gpm_plan_create": ([vp, vp, sz, vp, sz, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(vp)], st),
+        "rmh_gpm_plan_run": ([vp, vp], st),
         "rmh_lmcs_tables": ([vp, ctypes.c_uint32, vp, vp, vp, vp], st),
         "rmh_lmcs_plan_create": ([vp, vp, sz, ctypes.POINTER(vp)], st),
         "rmh_lmcs_plan_run": ([vp, vp], st),
@@ -253,6 +256,8 @@ class Plan:
             self.acc._check(L.rmh_lmcs_plan_run(self.handle, stream))
         elif self.kind == "alf":
             self.acc._check(L.rmh_alf_plan_run(self.handle, stream))
+        elif self.kind == "gpm":
+            self.acc._check(L.rmh_gpm_plan_run(self.handle, stream))
         else:
             self.acc._check(L.rmh_deblock_plan_run(self.handle, stream))
 
@@ -441,6 +446,16 @@ class Accelerator:
         self._check(load_library().rmh_mc_bi_plan_create(self.handle, ctypes.addressof(arr), len(frames),
                                                          pus.ctypes.data, len(pus), ctypes.byref(h)))
         return Plan(self, h.value, "mc")
+
+    def gpm_plan(self, frames, pus, luma_scale=abi.GPM_LUMA_SCALE, chroma_scale=abi.GPM_CHROMA_SCALE):
+        """GPM blend plan over abi.GpmFrame entries with device planes (rmh_gpm_plan_create)."""
+        pus = np.ascontiguousarray(pus, dtype=abi.GPM_PU_DTYPE)
+        arr = (abi.GpmFrame * len(frames))(*frames)
+        h = ctypes.c_void_p()
+        self._check(load_library().rmh_gpm_plan_create(self.handle, ctypes.addressof(arr), len(frames),
+                                                       pus.ctypes.data, len(pus), luma_scale, chroma_scale,
+                                                       ctypes.byref(h)))
+        return Plan(self, h.value, "gpm")
 
     def deblock_plan(self, frames):
         arr = (abi.DbkFrame * len(frames))(*frames)

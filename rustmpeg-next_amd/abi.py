@@ -44,6 +44,17 @@ class McBiFrame(ctypes.Structure):
     _fields_ = [("ref0", Plane * 3), ("ref1", Plane * 3), ("dst", Plane * 3)]
 
 
+class GpmFrame(ctypes.Structure):
+    _fields_ = [("pred0", Plane * 3), ("pred1", Plane * 3), ("dst", Plane * 3)]
+
+
+# rmh_gpm_pu (12 bytes) and the scales the reference's decoder passes
+GPM_PU_DTYPE = np.dtype([("x", "<u2"), ("y", "<u2"), ("log2_w", "u1"), ("log2_h", "u1"), ("frame", "<u2"),
+                         ("split_dir", "u1"), ("reserved", "u1", (3,))])
+assert GPM_PU_DTYPE.itemsize == 12
+GPM_LUMA_SCALE, GPM_CHROMA_SCALE = 1, 2
+
+
 # rmh_bi_pu_desc (28 bytes); bcw_idx 0..4 selects the L1 weight of {-2, 3, 4, 5, 10} / 8;
 # refine: BI_DMVR | BI_BDOF (8-bit frames)
 BI_DMVR, BI_BDOF = 1, 2

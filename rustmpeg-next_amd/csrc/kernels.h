@@ -168,6 +168,25 @@ struct McRefItem {
     uint32_t slot;
 };
 
+// One GPM work item: a 16x16 luma tile of a GPM PU and its 8x8 chroma
+// tiles, sizes clipped to the picture, with the PU's split-line parameters
+// (predict_gpm_block): the GEO_DIS indices of the angle and of the angle + 8,
+// and the offsets of the PU origin from the split line on the luma grid, for
+// the luma and the chroma planes.
+struct GpmItem {
+    uint16_t x0, y0;   // luma tile origin (picture)
+    uint16_t px, py;   // PU origin
+    uint8_t tw, th;    // luma tile size, 0..16
+    uint8_t cw, ch;    // chroma tile size, 0..8
+    uint16_t frame;
+    uint8_t dx, dy;
+    int16_t lox, loy;  // luma plane
+    int16_t cox, coy;  // chroma planes
+};
+
+hipError_t launch_gpm(const rmh_gpm_frame* frames, const GpmItem* items, uint32_t n_items, uint32_t luma_scale,
+                      uint32_t chroma_scale, hipStream_t stream);
+
 // lds_bytes: 2 (w + 4) (h + 4) for the largest PU of the jobs.
 hipError_t launch_dmvr(const rmh_mc_bi_frame* frames, const DmvrJob* jobs, uint32_t n_jobs, uint32_t lds_bytes,
                        int32_t* mvs, hipStream_t stream);

@@ -1,6 +1,7 @@
 // mc_refine.hip - the decoder-side refinements of 8-bit bi-prediction for
 // gfx950: DMVR (dmvr_kernel) and the bi-prediction of the PUs it refines or
-// that use BDOF (mc_bi_refined_kernel).
+// that use BDOF (mc_bi_refined_kernel); and the geometric-partitioning blend
+// of two predictions (gpm_kernel, predict_gpm_block, inter.rs:1673-1730).
 //
 // Semantics: the bi branch of predict_inter_pu
 // (crates/rustmpeg-video/src/vvc/inter.rs:1753-1801) with
@@ -371,7 +372,59 @@ __global__ __launch_bounds__(256) void mc_bi_refined_kernel(const rmh_mc_bi_fram
     }
 }
 
+// ------------------------------------------------------------------- GPM
+
+// GEO_DIS (inter.rs:131-134).
+__constant__ int8_t kGeoDis[32] = {8, 8, 8, 8, 4, 4, 2, 1, 0, -1, -2, -4, -4, -8, -8, -8,
+                                   -8, -8, -8, -8, -4, -4, -2, -1, 0, 1, 2, 4, 4, 8, 8, 8};
+
+// predict_gpm_block's weight of prediction 0 at position (x, y) of a block
+// whose split line lies at offsets (ox, oy) on the luma grid.
+__device__ __forceinline__ int gpm_w0(int x, int y, int scale, int ox, int oy, int dx, int dy) {
+    const int idx = (((x << scale) + ox) * 2 + 1) * kGeoDis[dx] + (((y << scale) + oy) * 2 + 1) * kGeoDis[dy];
+    return clampi((32 + idx + 4) >> 3, 0, 8);
+}
+
+// One 256-lane workgroup per item: lane (x, y) of the 16x16 luma tile, then
+// lanes 0-127 / 128-255 on the two 8x8 chroma tiles.
+__global__ __launch_bounds__(256) void gpm_kernel(const rmh_gpm_frame* __restrict__ frames,
+                                                  const GpmItem* __restrict__ items, int luma_scale, int chroma_scale) {
+    const GpmItem it = items[blockIdx.x];
+    const rmh_gpm_frame& F = frames[it.frame];
+    const int tid = threadIdx.x;
+    {
+        const int x = tid & 15, y = tid >> 4;
+        if (x < it.tw && y < it.th) {
+            const int X = it.x0 + x, Y = it.y0 + y;
+            const int w0 = gpm_w0(X - it.px, Y - it.py, luma_scale, it.lox, it.loy, it.dx, it.dy);
+            const uint32_t o0 = (uint32_t)Y * F.pred0[0].stride + X, o1 = (uint32_t)Y * F.pred1[0].stride + X;
+            const int p0 = gsrc<uint8_t>(F.pred0[0].data)[o0], p1 = gsrc<uint8_t>(F.pred1[0].data)[o1];
+            gdst<uint8_t>(F.dst[0].data)[(uint32_t)Y * F.dst[0].stride + X] =
+                (uint8_t)clampi((w0 * p0 + (8 - w0) * p1 + 4) >> 3, 0, 255);
+        }
+    }
+    {
+        const int c = 1 + (tid >> 7), r = tid & 127, x = r & 7, y = r >> 3;
+        if (y < 8 && x < it.cw && y < it.ch) {
+            const int X = (it.x0 >> 1) + x, Y = (it.y0 >> 1) + y;
+            const int w0 = gpm_w0(X - (it.px >> 1), Y - (it.py >> 1), chroma_scale, it.cox, it.coy, it.dx, it.dy);
+            const uint32_t o0 = (uint32_t)Y * F.pred0[c].stride + X, o1 = (uint32_t)Y * F.pred1[c].stride + X;
+            const int p0 = gsrc<uint8_t>(F.pred0[c].data)[o0], p1 = gsrc<uint8_t>(F.pred1[c].data)[o1];
+            gdst<uint8_t>(F.dst[c].data)[(uint32_t)Y * F.dst[c].stride + X] =
+                (uint8_t)clampi((w0 * p0 + (8 - w0) * p1 + 4) >> 3, 0, 255);
+        }
+    }
+}
+
 }  // namespace
+
+hipError_t launch_gpm(const rmh_gpm_frame* frames, const GpmItem* items, uint32_t n_items, uint32_t luma_scale,
+                      uint32_t chroma_scale, hipStream_t stream) {
+    if (n_items == 0) return hipSuccess;
+    hipLaunchKernelGGL(gpm_kernel, dim3(n_items), dim3(256), 0, stream, frames, items, (int)luma_scale,
+                       (int)chroma_scale);
+    return hipGetLastError();
+}
 
 hipError_t launch_dmvr(const rmh_mc_bi_frame* frames, const DmvrJob* jobs, uint32_t n_jobs, uint32_t lds_bytes,
                        int32_t* mvs, hipStream_t stream) {

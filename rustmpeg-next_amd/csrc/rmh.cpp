@@ -160,7 +160,7 @@ struct rmh_ctx {
     void* scratch = nullptr;  // this device's address of `scratch`
 };
 
-enum PlanKind { kResidual = 0, kMc = 1, kDeblock = 2, kSao = 3, kEdge = 4, kLmcs = 5, kAlf = 6, kForward = 7 };
+enum PlanKind { kResidual = 0, kMc = 1, kDeblock = 2, kSao = 3, kEdge = 4, kLmcs = 5, kAlf = 6, kForward = 7, kGpm = 8 };
 
 struct rmh_plan {
     rmh_ctx* ctx = nullptr;
@@ -208,6 +208,11 @@ struct rmh_plan {
     const rmh::McRefItem* d_ref_items = nullptr;
     size_t n_ref_items = 0;
     int32_t* d_mvs = nullptr;
+    // GPM blend plans
+    const rmh_gpm_frame* d_gpm_frames = nullptr;
+    const rmh::GpmItem* d_gpm_items = nullptr;
+    size_t n_gpm_items = 0;
+    uint32_t gpm_scale[2] = {0, 0};
     // deblock
     const rmh_dbk_frame* d_dbk = nullptr;
     const rmh_plane* d_shadows = nullptr;  // 3 per frame: copies the tiled kernel reads
@@ -1333,6 +1338,9 @@ rmh_status run_plan_locked_free(rmh_plan* plan, const int16_t* d_coeff, int16_t*
         memcpy(a.scan_len, plan->ctx->tabs.scan_len, sizeof a.scan_len);
         a.n_tu = (uint32_t)plan->n_tu;
         e = rmh::launch_forward(a, s);
+    } else if (plan->kind == kGpm) {
+        e = rmh::launch_gpm(plan->d_gpm_frames, plan->d_gpm_items, (uint32_t)plan->n_gpm_items, plan->gpm_scale[0],
+                            plan->gpm_scale[1], s);
     } else if (plan->kind == kLmcs) {
         e = rmh::launch_lmcs(plan->d_lmcs_jobs, plan->d_lmcs_block_job, plan->lmcs_range, s);
     } else if (plan->kind == kAlf) {
@@ -2692,6 +2700,155 @@ uint32_t div_up(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
 
 }  // namespace
 
+// ------------------------------------------------------------------ GPM
+
+namespace {
+
+rmh_status validate_gpm(rmh_ctx* ctx, const rmh_gpm_frame* frames, size_t n_frames, const rmh_gpm_pu* pus, size_t n_pu,
+                        uint32_t luma_scale, uint32_t chroma_scale) {
+    if ((n_frames && !frames) || (n_pu && !pus)) return fail(ctx, RMH_INVALID_INPUT, "null frame or PU array");
+    if (n_frames > 65536) return fail(ctx, RMH_LIMIT_EXCEEDED, "more than 65536 frames");
+    if (luma_scale > 3 || chroma_scale > 3) return fail(ctx, RMH_INVALID_INPUT, "scales must be 0..3");
+    std::vector<PlaneSpan> spans;
+    for (size_t f = 0; f < n_frames; f++) {
+        const rmh_gpm_frame& fr = frames[f];
+        const std::string tag = "frame " + std::to_string(f) + ": ";
+        for (int c = 0; c < 3; c++) {
+            for (const rmh_plane* q : {&fr.pred0[c], &fr.pred1[c], &fr.dst[c]}) {
+                if (!plane_ok(*q) || q->bytes_per_sample != 1 || q->bit_depth != 8)
+                    return fail(ctx, RMH_INVALID_INPUT, tag + "planes must be valid 8-bit planes");
+                if (q->width != fr.dst[c].width || q->height != fr.dst[c].height)
+                    return fail(ctx, RMH_INVALID_INPUT, tag + "prediction / output size mismatch");
+            }
+            add_span(spans, fr.pred0[c], false);
+            add_span(spans, fr.pred1[c], false);
+            add_span(spans, fr.dst[c], true);
+        }
+        if (fr.dst[1].width != (fr.dst[0].width + 1) / 2 || fr.dst[1].height != (fr.dst[0].height + 1) / 2 ||
+            fr.dst[2].width != fr.dst[1].width || fr.dst[2].height != fr.dst[1].height)
+            return fail(ctx, RMH_INVALID_INPUT, tag + "chroma planes are not 4:2:0");
+    }
+    if (written_overlap(spans)) return fail(ctx, RMH_INVALID_INPUT, "an output plane overlaps another plane of the batch");
+    for (size_t i = 0; i < n_pu; i++) {
+        const rmh_gpm_pu& p = pus[i];
+        const std::string tag = "PU " + std::to_string(i) + ": ";
+        if (p.log2_w < 3 || p.log2_w > 7 || p.log2_h < 3 || p.log2_h > 7)
+            return fail(ctx, RMH_INVALID_INPUT, tag + "size outside 8..128");
+        if (p.frame >= n_frames) return fail(ctx, RMH_INVALID_INPUT, tag + "frame index");
+        if (p.x >= frames[p.frame].dst[0].width || p.y >= frames[p.frame].dst[0].height)
+            return fail(ctx, RMH_INVALID_INPUT, tag + "outside the picture");
+        if (p.split_dir > 63) return fail(ctx, RMH_INVALID_INPUT, tag + "split direction outside 0..63");
+        if (p.reserved[0] | p.reserved[1] | p.reserved[2])
+            return fail(ctx, RMH_INVALID_INPUT, tag + "reserved bytes must be zero");
+    }
+    return RMH_OK;
+}
+
+// derive_gpm_params_table (vvc/inter.rs:1641-1671) over GEO_ANGLE_TO_MASK
+// (inter.rs:137-140): (angle, distance) of each split direction.
+void gpm_params(uint8_t* angle, uint8_t* distance) {
+    static const int8_t kMask[32] = {0, -1, 1, 2, 3, 4, -1, -1, 5, -1, -1, 4, 3, 2, 1, -1,
+                                     0, -1, 1, 2, 3, 4, -1, -1, 5, -1, -1, 4, 3, 2, 1, -1};
+    int mode = 0;
+    for (int a = 0; a < 32; a++)
+        for (int d = 0; d < 4; d++) {
+            const int m = kMask[a];
+            if ((d == 0 && a >= 16) || ((d == 2 || d == 0) && (m == 0 || m == 5)) || m == -1) continue;
+            if (mode < 64) {
+                angle[mode] = (uint8_t)a;
+                distance[mode] = (uint8_t)d;
+                mode++;
+            }
+        }
+}
+
+// The split line's offsets of predict_gpm_block (inter.rs:1689-1703) for a
+// bw x bh block whose positions scale onto the luma grid by 1 << scale.
+void gpm_offsets(int bw, int bh, int scale, int angle, int distance, int16_t* ox, int16_t* oy) {
+    const int lw = bw << scale, lh = bh << scale;
+    int x = -(lw >> 1), y = -(lh >> 1);
+    if (distance > 0) {
+        const int am = angle % 16;
+        if (am == 8 || (am != 0 && lh >= lw)) {
+            const int d = (distance * lh) >> 3;
+            y += angle < 16 ? d : -d;
+        } else {
+            const int d = (distance * lw) >> 3;
+            x += angle < 16 ? d : -d;
+        }
+    }
+    *ox = (int16_t)x;
+    *oy = (int16_t)y;
+}
+
+}  // namespace
+
+RMH_EXPORT rmh_status rmh_gpm_plan_create(rmh_ctx* ctx, const rmh_gpm_frame* frames, size_t n_frames,
+                                          const rmh_gpm_pu* pus, size_t n_pu, uint32_t luma_scale,
+                                          uint32_t chroma_scale, rmh_plan** out) {
+    if (!ctx || !out) return RMH_INVALID_INPUT;
+    *out = nullptr;
+    rmh_status v = validate_gpm(ctx, frames, n_frames, pus, n_pu, luma_scale, chroma_scale);
+    if (v != RMH_OK) return v;
+    if (ctx->disabled.load()) return fail(ctx, RMH_UNAVAILABLE, "accelerator disabled");
+    DeviceGuard dg(ctx->device);
+    uint8_t ang[64], dis[64];
+    gpm_params(ang, dis);
+    std::vector<rmh::GpmItem> items;
+    uint64_t bytes = 0;
+    for (size_t i = 0; i < n_pu; i++) {
+        const rmh_gpm_pu& pu = pus[i];
+        const int w = 1 << pu.log2_w, h = 1 << pu.log2_h;
+        const int a = ang[pu.split_dir], d = dis[pu.split_dir];
+        rmh::GpmItem base{};
+        base.px = pu.x;
+        base.py = pu.y;
+        base.frame = pu.frame;
+        base.dx = (uint8_t)a;
+        base.dy = (uint8_t)((a + 8) % 32);
+        gpm_offsets(w, h, (int)luma_scale, a, d, &base.lox, &base.loy);
+        gpm_offsets(w / 2, h / 2, (int)chroma_scale, a, d, &base.cox, &base.coy);
+        pu_tiles(frames[pu.frame].dst, pu.x, pu.y, w, h, [&](int x0, int y0, int tw, int th, int cw, int ch) {
+            rmh::GpmItem it = base;
+            it.x0 = (uint16_t)x0;
+            it.y0 = (uint16_t)y0;
+            it.tw = (uint8_t)tw;
+            it.th = (uint8_t)th;
+            it.cw = (uint8_t)cw;
+            it.ch = (uint8_t)ch;
+            items.push_back(it);
+            bytes += 3ull * (tw * th + 2 * cw * ch);  // two predictions read, one written
+        });
+        bytes += sizeof(rmh_gpm_pu);
+    }
+    if (items.size() > 0x7fffffffu) return fail(ctx, RMH_LIMIT_EXCEEDED, "too many GPM work items");
+    rmh_plan* p = new rmh_plan();
+    p->ctx = ctx;
+    p->kind = kGpm;
+    p->n_gpm_items = items.size();
+    p->gpm_scale[0] = luma_scale;
+    p->gpm_scale[1] = chroma_scale;
+    p->bytes = bytes;
+    Blob blob;
+    const size_t o_frames = blob.add(frames, n_frames * sizeof(rmh_gpm_frame));
+    const size_t o_items = blob.add(items.data(), items.size() * sizeof(rmh::GpmItem));
+    std::string err;
+    if (!upload_blob(p, blob, &err)) {
+        delete p;
+        return fail(ctx, RMH_EXECUTION, "gpm plan upload failed: " + err);
+    }
+    p->d_gpm_frames = (const rmh_gpm_frame*)((char*)p->d_mem + o_frames);
+    p->d_gpm_items = (const rmh::GpmItem*)((char*)p->d_mem + o_items);
+    *out = p;
+    return RMH_OK;
+}
+
+RMH_EXPORT rmh_status rmh_gpm_plan_run(rmh_plan* plan, void* stream) {
+    if (!plan || plan->kind != kGpm) return RMH_INVALID_INPUT;
+    if (plan->n_gpm_items == 0) return RMH_OK;
+    return run_plan(plan, nullptr, nullptr, 0, 0, stream);
+}
+
 RMH_EXPORT rmh_status rmh_lmcs_plan_create(rmh_ctx* ctx, const rmh_lmcs_frame* frames, size_t n_frames,
                                            rmh_plan** out) {
     if (!ctx || !out) return RMH_INVALID_INPUT;
@@ -2961,7 +3118,7 @@ RMH_EXPORT rmh_status rmh_graph_create(rmh_ctx* ctx, const rmh_graph_node* nodes
     if (e == hipSuccess) {
         for (size_t i = 0; i < n_nodes && st == RMH_OK; i++) {
             rmh_plan* p = nodes[i].plan;
-            if (((p->kind == kResidual || p->kind == kForward) && p->n_tu == 0) || (p->kind == kMc && p->n_items == 0 && p->n_ref_items == 0) ||
+            if (((p->kind == kResidual || p->kind == kForward) && p->n_tu == 0) || (p->kind == kMc && p->n_items == 0 && p->n_ref_items == 0) || (p->kind == kGpm && p->n_gpm_items == 0) ||
                 (p->kind == kDeblock && p->n_dbk == 0) || (p->kind == kSao && p->n_sao_jobs == 0) ||
                 (p->kind == kEdge && p->total_cells == 0) || (p->kind == kLmcs && p->lmcs_range[2] == 0) ||
                 (p->kind == kAlf && p->alf_luma_range[2] == 0))
