@@ -237,7 +237,7 @@ def check_batch(PL, po, rmh, batch, frames, which):
     return True
 
 
-def bench_pipeline(rmh, acc, W, PL, po, n_frames, steps, warmup, cpu=True):
+def bench_pipeline(rmh, acc, W, PL, po, n_frames, steps, warmup, cpu=True, autotune=True):
     """P-frame reconstruction, MC -> residual add -> deblock, over n_frames
     1080p10 frames: one frame-step graph over the batch against the same
     plans built per frame (unbatched)."""
@@ -247,6 +247,14 @@ def bench_pipeline(rmh, acc, W, PL, po, n_frames, steps, warmup, cpu=True):
     log("pipeline: generated %d P-frames in %.1f s" % (n_frames, time.perf_counter() - t0))
     batch = PL.DeviceBatch(acc, rmh, frames)
     stream = hip.Stream()
+    tuned = None
+    if autotune:
+        # shapes chosen before the graph captures the plans
+        tuned = {"mc_waves": batch.mc.autotune_mc(stream.handle),
+                 "recon_split": {"%dx%d" % k: v for k, v in batch.recon.autotune(batch.d_coeff.ptr, 0,
+                                                                                   stream.handle).items() if v > 1}}
+        stream.synchronize()
+        log("pipeline: autotuned %s" % tuned)
     graph = batch.graph()
     graph.launch(stream.handle)
     stream.synchronize()
@@ -274,7 +282,7 @@ def bench_pipeline(rmh, acc, W, PL, po, n_frames, steps, warmup, cpu=True):
            "value": round(px / (ms / 1e3) / 1e6, 1), "unit": "Mpix/s", "ms_per_step": round(ms, 4),
            "unbatched": {"value": round(px / (ums / 1e3) / 1e6, 1), "ms_per_step": round(ums, 4),
                          "launches_per_step": 4 * n_frames},
-           "stages": stages}
+           "stages": stages, "autotune": tuned}
     out["vs_unbatched"] = round(out["value"] / out["unbatched"]["value"], 3)
     log("pipeline: %.3f ms/step %.1f Mpix/s, unbatched %.3f ms (x%.2f)" % (ms, out["value"], ums, out["vs_unbatched"]))
     if cpu:
@@ -330,7 +338,7 @@ def bench_pipeline(rmh, acc, W, PL, po, n_frames, steps, warmup, cpu=True):
     return out
 
 
-def bench_streams(rmh, acc, PL, po, steps, reps):
+def bench_streams(rmh, acc, PL, po, steps, reps, autotune=True):
     """The batched multi-stream configuration: 16 concurrent streams
     (4x 854x480 8-bit coded as 856x480, 8x 1080p10, 4x 2160p10) x `steps`
     chained frame steps with the residual tool mix; each step is one graph
@@ -340,7 +348,9 @@ def bench_streams(rmh, acc, PL, po, steps, reps):
     t0 = time.perf_counter()
     refs, frames = PL.generate_streams(PL.STREAMS, steps, tools=True, seed=3000, workers=min(16, host_threads()))
     log("streams: generated %d frames in %.1f s" % (steps * len(PL.STREAMS), time.perf_counter() - t0))
-    chain = PL.ChainedStreams(acc, rmh, refs, frames)
+    chain = PL.ChainedStreams(acc, rmh, refs, frames, autotune=autotune)
+    if autotune:
+        log("streams: autotuned %s" % chain.tuned)
     stream = hip.Stream()
     chain.run(stream.handle)
     stream.synchronize()
@@ -387,6 +397,9 @@ def bench_streams(rmh, acc, PL, po, steps, reps):
            "unbatched": {"value": round(px / (ums / 1e3) / 1e6, 1), "ms_per_frame_step": round(ums, 4),
                          "launches_per_frame_step": 4 * len(PL.STREAMS)}}
     out["vs_unbatched"] = round(out["value"] / out["unbatched"]["value"], 3)
+    if chain.tuned:
+        out["autotune"] = {"mc_waves_per_step": [t["mc_waves"] for t in chain.tuned],
+                           "recon_split_step0": chain.tuned[0]["recon_split"]}
     log("streams: %.3f ms/frame step %.1f Mpix/s (%.1f GB/s), unbatched x%.2f" % (ms, out["value"], achieved,
                                                                                   out["vs_unbatched"]))
     chain.close()
@@ -813,8 +826,10 @@ def main():
     sys.path.insert(0, os.path.join(REPO, "tests"))
     import pipeline as PL
     pipe = None if args.no_pipeline else bench_pipeline(rmh, acc, W, PL, po, args.pipeline_frames, args.steps,
-                                                         args.warmup, cpu=not args.no_cpu)
-    streams = None if args.no_streams else bench_streams(rmh, acc, PL, po, 8, max(2, args.steps // 5))
+                                                         args.warmup, cpu=not args.no_cpu,
+                                                         autotune=not args.no_autotune)
+    streams = None if args.no_streams else bench_streams(rmh, acc, PL, po, 8, max(2, args.steps // 5),
+                                                         autotune=not args.no_autotune)
 
     bi = None if args.no_bi else bench_bi(rmh, acc, W, po, args.pipeline_frames, args.steps, args.warmup)
     if bi is not None:

@@ -225,6 +225,14 @@ rmh_status rmh_mc_batch(rmh_ctx* ctx, const rmh_mc_frame* frames, size_t n_frame
 rmh_status rmh_mc_plan_create(rmh_ctx* ctx, const rmh_mc_frame* frames, size_t n_frames,
                               const rmh_pu_desc* pus, size_t n_pu, rmh_plan** out);
 rmh_status rmh_mc_plan_run(rmh_plan* plan, void* stream);
+/* Autotuner of an inter-prediction plan, uni or bi (no reference
+ * counterpart; part of batching the frame step, as the residual autotuner):
+ * times the plan's interpolation launch with 4, 2 and 1 work items (one wave
+ * each) per workgroup, the fastest of three batches of five launches, and
+ * keeps the fastest shape (a change needs a 2 % gain); *waves_out (may be
+ * NULL) receives it. Outputs do not depend on the shape; the runs write the
+ * plan's output planes. Refused while a graph holds the plan. */
+rmh_status rmh_mc_plan_autotune(rmh_plan* plan, void* stream, uint32_t* waves_out);
 
 /* Bi-prediction (inter_dir 3) with CU-level weights, replacing the bi branch
  * of predict_inter_pu (vvc/inter.rs:1753-1801; DMVR and BDOF per PU, below): both

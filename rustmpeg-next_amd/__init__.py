@@ -76,7 +76,7 @@ EXPORTED = [
     "rmh_edge_map", "rmh_edge_map_plan_create", "rmh_edge_map_plan_run", "rmh_lmcs_tables",
     "rmh_lmcs_plan_create", "rmh_lmcs_plan_run", "rmh_alf_filter_bank", "rmh_alf_plan_create", "rmh_alf_plan_run",
     "rmh_forward_plan_create", "rmh_forward_plan_run", "rmh_forward_plan_evaluate",
-    "rmh_gpm_plan_create", "rmh_gpm_plan_run",
+    "rmh_gpm_plan_create", "rmh_gpm_plan_run", "rmh_mc_plan_autotune",
 ]
 
 
@@ -121,6 +121,7 @@ def load_library(path=LIB_PATH):
         "rmh_forward_plan_evaluate": ([vp, vp, vp, vp, vp, vp, vp], st),
         "rmh_gpm_plan_create": ([vp, vp, sz, vp, sz, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(vp)], st),
         "rmh_gpm_plan_run": ([vp, vp], st),
+        "rmh_mc_plan_autotune": ([vp, vp, ctypes.POINTER(ctypes.c_uint32)], st),
         "rmh_lmcs_tables": ([vp, ctypes.c_uint32, vp, vp, vp, vp], st),
         "rmh_lmcs_plan_create": ([vp, vp, sz, ctypes.POINTER(vp)], st),
         "rmh_lmcs_plan_run": ([vp, vp], st),
@@ -273,6 +274,13 @@ class Plan:
         self.acc._check(load_library().rmh_residual_plan_autotune(self.handle, d_coeff, d_residual, stream, reps,
                                                                   out.ctypes.data))
         return {(1 << (i // 7), 1 << (i % 7)): 1 << int(v) for i, v in enumerate(out) if i // 7 >= 2 and i % 7 >= 2}
+
+    def autotune_mc(self, stream=0):
+        """Chooses the work items per workgroup of an inter-prediction plan by
+        timing (rmh_mc_plan_autotune); returns the chosen count."""
+        w = ctypes.c_uint32(0)
+        self.acc._check(load_library().rmh_mc_plan_autotune(self.handle, stream, ctypes.byref(w)))
+        return int(w.value)
 
     def run_range(self, first, count, d_coeff, d_residual, stream=0):
         self.acc._check(load_library().rmh_residual_plan_run_range(self.handle, first, count, d_coeff, d_residual,

@@ -127,8 +127,9 @@ struct McItem {
 };
 
 // Items [0, n_items8) read and write 8-bit planes, the rest 16-bit ones.
+// waves: work items (one wave each) per workgroup, 1, 2 or 4.
 hipError_t launch_mc(const rmh_mc_frame* frames, const McItem* items, uint32_t n_items, uint32_t n_items8,
-                     hipStream_t stream);
+                     hipStream_t stream, int waves = 4);
 
 // One bi-prediction work item: the tile geometry of McItem, both MVs and the
 // BCW weight of L1 (the L0 weight is 8 minus it).
@@ -144,7 +145,7 @@ struct McBiItem {
 };
 
 hipError_t launch_mc_bi(const rmh_mc_bi_frame* frames, const McBiItem* items, uint32_t n_items, uint32_t n_items8,
-                        hipStream_t stream);
+                        hipStream_t stream, int waves = 4);
 
 // DMVR of one bi PU (8-bit frames): the signalled MV pair, refined into
 // slot `slot` of the plan's MV array (4 int32 per slot).

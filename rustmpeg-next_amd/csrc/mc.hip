@@ -311,13 +311,14 @@ __device__ __forceinline__ void bi_quad(const Tile<TAPS>& t, const int16_t* win,
 // Outputs go in 2 x 2 quads: the 64 quads of the luma tile one per lane, the
 // 2 x 16 of the chroma tiles on lanes 0 .. 31. Item fields, frame planes and
 // filter taps are wave-uniform and read with scalar loads.
-template <int BPS>
-__global__ __launch_bounds__(256) void mc_kernel(const rmh_mc_frame* __restrict__ frames,
-                                                 const McItem* __restrict__ items, uint32_t n_items) {
-    __shared__ __align__(16) WaveLds lds[4];
+// NW: waves (work items) per workgroup, chosen per plan by the autotuner.
+template <int BPS, int NW>
+__global__ __launch_bounds__(64 * NW) void mc_kernel(const rmh_mc_frame* __restrict__ frames,
+                                                     const McItem* __restrict__ items, uint32_t n_items) {
+    __shared__ __align__(16) WaveLds lds[NW];
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const uint32_t item = xcd_local_block(blockIdx.x, gridDim.x) * 4 + wave;
+    const uint32_t item = xcd_local_block(blockIdx.x, gridDim.x) * NW + wave;
     const bool active = item < n_items;
     WaveLds& L = lds[wave];
     Tile<8> tl{};
@@ -440,13 +441,13 @@ __device__ __forceinline__ void bi_output_quad(const rmh_plane& dst, const Tile<
 // Bi-prediction: as mc_kernel, with the windows of both lists fetched before
 // any LDS write (two LDS slices per wave) and the weighted average fused into
 // the output quads.
-template <int BPS>
-__global__ __launch_bounds__(256) void mc_bi_kernel(const rmh_mc_bi_frame* __restrict__ frames,
-                                                    const McBiItem* __restrict__ items, uint32_t n_items) {
-    __shared__ __align__(16) WaveLds lds[4][2];
+template <int BPS, int NW>
+__global__ __launch_bounds__(64 * NW) void mc_bi_kernel(const rmh_mc_bi_frame* __restrict__ frames,
+                                                        const McBiItem* __restrict__ items, uint32_t n_items) {
+    __shared__ __align__(16) WaveLds lds[NW][2];
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const uint32_t item = xcd_local_block(blockIdx.x, gridDim.x) * 4 + wave;
+    const uint32_t item = xcd_local_block(blockIdx.x, gridDim.x) * NW + wave;
     const bool active = item < n_items;
     WaveLds& L0 = lds[wave][0];
     WaveLds& L1 = lds[wave][1];
@@ -515,23 +516,51 @@ __global__ __launch_bounds__(256) void mc_bi_kernel(const rmh_mc_bi_frame* __res
 
 }  // namespace
 
-hipError_t launch_mc_bi(const rmh_mc_bi_frame* frames, const McBiItem* items, uint32_t n_items, uint32_t n_items8,
-                        hipStream_t stream) {
+namespace {
+
+template <int NW>
+void launch_mc_bi_nw(const rmh_mc_bi_frame* frames, const McBiItem* items, uint32_t n_items, uint32_t n_items8,
+                     hipStream_t stream) {
     if (n_items8 > 0)
-        hipLaunchKernelGGL(mc_bi_kernel<1>, dim3((n_items8 + 3) / 4), dim3(256), 0, stream, frames, items, n_items8);
+        hipLaunchKernelGGL((mc_bi_kernel<1, NW>), dim3((n_items8 + NW - 1) / NW), dim3(64 * NW), 0, stream, frames,
+                           items, n_items8);
     if (n_items > n_items8)
-        hipLaunchKernelGGL(mc_bi_kernel<2>, dim3((n_items - n_items8 + 3) / 4), dim3(256), 0, stream, frames,
-                           items + n_items8, n_items - n_items8);
+        hipLaunchKernelGGL((mc_bi_kernel<2, NW>), dim3((n_items - n_items8 + NW - 1) / NW), dim3(64 * NW), 0, stream,
+                           frames, items + n_items8, n_items - n_items8);
+}
+
+template <int NW>
+void launch_mc_nw(const rmh_mc_frame* frames, const McItem* items, uint32_t n_items, uint32_t n_items8,
+                  hipStream_t stream) {
+    if (n_items8 > 0)
+        hipLaunchKernelGGL((mc_kernel<1, NW>), dim3((n_items8 + NW - 1) / NW), dim3(64 * NW), 0, stream, frames, items,
+                           n_items8);
+    if (n_items > n_items8)
+        hipLaunchKernelGGL((mc_kernel<2, NW>), dim3((n_items - n_items8 + NW - 1) / NW), dim3(64 * NW), 0, stream,
+                           frames, items + n_items8, n_items - n_items8);
+}
+
+}  // namespace
+
+hipError_t launch_mc_bi(const rmh_mc_bi_frame* frames, const McBiItem* items, uint32_t n_items, uint32_t n_items8,
+                        hipStream_t stream, int waves) {
+    if (waves == 1)
+        launch_mc_bi_nw<1>(frames, items, n_items, n_items8, stream);
+    else if (waves == 2)
+        launch_mc_bi_nw<2>(frames, items, n_items, n_items8, stream);
+    else
+        launch_mc_bi_nw<4>(frames, items, n_items, n_items8, stream);
     return hipGetLastError();
 }
 
 hipError_t launch_mc(const rmh_mc_frame* frames, const McItem* items, uint32_t n_items, uint32_t n_items8,
-                     hipStream_t stream) {
-    if (n_items8 > 0)
-        hipLaunchKernelGGL(mc_kernel<1>, dim3((n_items8 + 3) / 4), dim3(256), 0, stream, frames, items, n_items8);
-    if (n_items > n_items8)
-        hipLaunchKernelGGL(mc_kernel<2>, dim3((n_items - n_items8 + 3) / 4), dim3(256), 0, stream, frames,
-                           items + n_items8, n_items - n_items8);
+                     hipStream_t stream, int waves) {
+    if (waves == 1)
+        launch_mc_nw<1>(frames, items, n_items, n_items8, stream);
+    else if (waves == 2)
+        launch_mc_nw<2>(frames, items, n_items, n_items8, stream);
+    else
+        launch_mc_nw<4>(frames, items, n_items, n_items8, stream);
     return hipGetLastError();
 }
 

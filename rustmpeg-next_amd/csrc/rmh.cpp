@@ -200,6 +200,7 @@ struct rmh_plan {
     const rmh_mc_bi_frame* d_bi_frames = nullptr;  // bi-prediction plans
     const rmh::McBiItem* d_bi_items = nullptr;
     size_t n_items = 0, n_items8 = 0;  // items [0, n_items8) are on 8-bit frames
+    int mc_waves = 4;                   // work items per MC workgroup (rmh_mc_plan_autotune)
     // bi PUs with DMVR or BDOF: the DMVR jobs, the tiles of the refined PUs
     // and their MV pairs (4 int32 per PU, the signalled pairs until DMVR runs)
     const rmh::DmvrJob* d_dmvr = nullptr;
@@ -251,6 +252,8 @@ struct rmh_graph {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     hipStream_t capture = nullptr;
+    hipStream_t side = nullptr;     // second branch of the capture (residual prepass, generic TUs)
+    std::vector<hipEvent_t> events;  // fork / join events of the capture
 };
 
 namespace {
@@ -1257,8 +1260,19 @@ std::vector<rmh::TileDst> build_tile_dst(const std::vector<uint32_t>& order, siz
     return out;
 }
 
+// Parts of a whole residual-plan run (a frame-step graph runs them on
+// branches of their own): the dequantising prepass, the tiled launches and
+// the generic kernel for the remaining TUs.
+enum { kPartPrepass = 1, kPartTiles = 2, kPartGeneric = 4, kPartsAll = 7 };
+
+// True when a run of the whole plan on these buffers takes the tiled path.
+bool residual_tiled_run(const rmh_plan* plan, const int16_t* d_coeff, const int16_t* d_residual) {
+    const bool aligned8 = ((uintptr_t)d_coeff & 7) == 0 && (plan->d_place || ((uintptr_t)d_residual & 7) == 0);
+    return plan->kind == kResidual && plan->d_order && aligned8;
+}
+
 rmh_status run_plan_locked_free(rmh_plan* plan, const int16_t* d_coeff, int16_t* d_residual, size_t first, size_t count,
-                                hipStream_t s, std::string* err) {
+                                hipStream_t s, std::string* err, int parts = kPartsAll) {
     hipError_t e = hipSuccess;
     if (plan->kind == kResidual) {
         rmh::ResidualArgs a{};
@@ -1278,12 +1292,11 @@ rmh_status run_plan_locked_free(rmh_plan* plan, const int16_t* d_coeff, int16_t*
         a.frames = plan->d_rframes;
         a.tile_tus = plan->d_tile_tus;
         a.tile_dst = plan->d_tile_dst;
-        const bool whole = first == 0 && count == plan->n_tu && plan->d_order;
-        const bool aligned8 = ((uintptr_t)d_coeff & 7) == 0 && (plan->d_place || ((uintptr_t)d_residual & 7) == 0);
-        if (whole && aligned8) {
+        const bool whole = first == 0 && count == plan->n_tu;
+        if (whole && residual_tiled_run(plan, d_coeff, d_residual)) {
             // the prepass, the tiled kernel for the 2-D TUs, the general kernel for the rest
             a.order = plan->d_order;
-            if (plan->n_prep) {
+            if (plan->n_prep && (parts & kPartPrepass)) {
                 a.prep_out = plan->d_prep;
                 a.prep_off = plan->d_prep_off;
                 a.first_tu = (uint32_t)plan->n_tu;
@@ -1293,10 +1306,10 @@ rmh_status run_plan_locked_free(rmh_plan* plan, const int16_t* d_coeff, int16_t*
                 a.prep_off = nullptr;
             }
             a.prep = plan->d_prep;
-            if (e == hipSuccess)
+            if (e == hipSuccess && (parts & kPartTiles))
                 e = rmh::launch_residual_tiles(a, plan->d_tiles, (uint32_t)plan->n_tiles, plan->n_tiles_plain,
                                                plan->tile_lds, s);
-            if (e == hipSuccess && plan->n_tu > plan->n_fast) {
+            if (e == hipSuccess && plan->n_tu > plan->n_fast && (parts & kPartGeneric)) {
                 a.first_tu = (uint32_t)plan->n_fast;
                 a.n_tu = (uint32_t)(plan->n_tu - plan->n_fast);
                 e = rmh::launch_residual(a, s);
@@ -1316,9 +1329,10 @@ rmh_status run_plan_locked_free(rmh_plan* plan, const int16_t* d_coeff, int16_t*
                                               plan->d_mvs, s);
             if (e == hipSuccess)
                 e = rmh::launch_mc_bi(plan->d_bi_frames, plan->d_bi_items, (uint32_t)plan->n_items,
-                                      (uint32_t)plan->n_items8, s);
+                                      (uint32_t)plan->n_items8, s, plan->mc_waves);
         } else {
-            e = rmh::launch_mc(plan->d_frames, plan->d_items, (uint32_t)plan->n_items, (uint32_t)plan->n_items8, s);
+            e = rmh::launch_mc(plan->d_frames, plan->d_items, (uint32_t)plan->n_items, (uint32_t)plan->n_items8, s,
+                               plan->mc_waves);
         }
     } else if (plan->kind == kSao) {
         e = rmh::launch_sao(plan->d_sao_jobs, plan->d_sao_block_job, plan->sao_range, s);
@@ -1805,6 +1819,62 @@ RMH_EXPORT rmh_status rmh_mc_plan_run(rmh_plan* plan, void* stream) {
     if (!plan || plan->kind != kMc) return RMH_INVALID_INPUT;
     if (plan->n_items == 0 && plan->n_ref_items == 0) return RMH_OK;
     return run_plan(plan, nullptr, nullptr, 0, 0, stream);
+}
+
+RMH_EXPORT rmh_status rmh_mc_plan_autotune(rmh_plan* plan, void* stream, uint32_t* waves_out) {
+    if (!plan || plan->kind != kMc) return RMH_INVALID_INPUT;
+    rmh_ctx* ctx = plan->ctx;
+    if (plan->graph_refs.load() != 0)
+        return fail(ctx, RMH_INVALID_INPUT, "plan is captured in a graph: destroy the graph before autotuning");
+    rmh_status g = gate(ctx);
+    if (g != RMH_OK) return fail(ctx, g, g == RMH_UNAVAILABLE ? "accelerator disabled" : "injected device failure");
+    DeviceGuard dg(ctx->device);
+    hipStream_t s = (hipStream_t)stream;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    // each shape timed on the plan's MC launch as the fastest of three batches
+    // of five launches (run-to-run noise is about 2 %), after two untimed ones
+    auto launch = [&](int waves) {
+        return plan->d_bi_frames ? rmh::launch_mc_bi(plan->d_bi_frames, plan->d_bi_items, (uint32_t)plan->n_items,
+                                                     (uint32_t)plan->n_items8, s, waves)
+                                 : rmh::launch_mc(plan->d_frames, plan->d_items, (uint32_t)plan->n_items,
+                                                  (uint32_t)plan->n_items8, s, waves);
+    };
+    auto time_shape = [&](int waves, float* ms) -> hipError_t {
+        hipError_t r = hipSuccess;
+        for (int k = 0; k < 2 && r == hipSuccess; k++) r = launch(waves);
+        float fastest = 0.0f;
+        for (int batch = 0; batch < 3 && r == hipSuccess; batch++) {
+            r = hipEventRecord(e0, s);
+            for (int k = 0; k < 5 && r == hipSuccess; k++) r = launch(waves);
+            if (r == hipSuccess) r = hipEventRecord(e1, s);
+            if (r == hipSuccess) r = hipEventSynchronize(e1);
+            float t = 0.0f;
+            if (r == hipSuccess) r = hipEventElapsedTime(&t, e0, e1);
+            if (r == hipSuccess && (batch == 0 || t < fastest)) fastest = t;
+        }
+        *ms = fastest;
+        return r;
+    };
+    int choice = plan->mc_waves;
+    float best = 0.0f;
+    if (e == hipSuccess && plan->n_items) e = time_shape(choice, &best);
+    for (int waves : {4, 2, 1}) {
+        if (e != hipSuccess || !plan->n_items || waves == plan->mc_waves) continue;
+        float ms = 0.0f;
+        e = time_shape(waves, &ms);
+        if (e == hipSuccess && ms < 0.98f * best) {
+            best = ms;
+            choice = waves;
+        }
+    }
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    if (e != hipSuccess) return fail(ctx, RMH_EXECUTION, std::string("autotune: ") + hipGetErrorString(e));
+    plan->mc_waves = choice;
+    if (waves_out) *waves_out = (uint32_t)choice;
+    return RMH_OK;
 }
 
 RMH_EXPORT rmh_status rmh_mc_batch(rmh_ctx* ctx, const rmh_mc_frame* frames, size_t n_frames, const rmh_pu_desc* pus,
@@ -3112,27 +3182,86 @@ RMH_EXPORT rmh_status rmh_graph_create(rmh_ctx* ctx, const rmh_graph_node* nodes
     rmh_graph* g = new rmh_graph();
     g->ctx = ctx;
     hipError_t e = hipStreamCreateWithFlags(&g->capture, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&g->side, hipStreamNonBlocking);
+    auto event = [&]() {
+        hipEvent_t ev = nullptr;
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+        if (ev) g->events.push_back(ev);
+        return ev;
+    };
     if (e == hipSuccess) e = hipStreamBeginCapture(g->capture, hipStreamCaptureModeThreadLocal);
     std::string err;
     rmh_status st = RMH_OK;
     if (e == hipSuccess) {
-        for (size_t i = 0; i < n_nodes && st == RMH_OK; i++) {
+        auto empty = [](const rmh_plan* p) {
+            return ((p->kind == kResidual || p->kind == kForward) && p->n_tu == 0) ||
+                   (p->kind == kMc && p->n_items == 0 && p->n_ref_items == 0) || (p->kind == kGpm && p->n_gpm_items == 0) ||
+                   (p->kind == kDeblock && p->n_dbk == 0) || (p->kind == kSao && p->n_sao_jobs == 0) ||
+                   (p->kind == kEdge && p->total_cells == 0) || (p->kind == kLmcs && p->lmcs_range[2] == 0) ||
+                   (p->kind == kAlf && p->alf_luma_range[2] == 0);
+        };
+        // Residual plans run in parts on two branches of the step: the
+        // prepass depends only on the coefficients, so it starts with the
+        // step, beside the nodes before it (MC); the generic kernel's TUs and
+        // the tiles' TUs are disjoint, so the generic kernel runs beside the
+        // tiles. Each branch joins the main one before the node that follows.
+        std::vector<hipEvent_t> prep_done(n_nodes, nullptr);
+        hipEvent_t start = event();
+        if (e == hipSuccess) e = hipEventRecord(start, g->capture);
+        if (e == hipSuccess) e = hipStreamWaitEvent(g->side, start, 0);
+        bool side_used = false;
+        for (size_t i = 0; i < n_nodes && st == RMH_OK && e == hipSuccess; i++) {
             rmh_plan* p = nodes[i].plan;
-            if (((p->kind == kResidual || p->kind == kForward) && p->n_tu == 0) || (p->kind == kMc && p->n_items == 0 && p->n_ref_items == 0) || (p->kind == kGpm && p->n_gpm_items == 0) ||
-                (p->kind == kDeblock && p->n_dbk == 0) || (p->kind == kSao && p->n_sao_jobs == 0) ||
-                (p->kind == kEdge && p->total_cells == 0) || (p->kind == kLmcs && p->lmcs_range[2] == 0) ||
-                (p->kind == kAlf && p->alf_luma_range[2] == 0))
-                continue;
-            st = run_plan_locked_free(p, nodes[i].d_coeff, nodes[i].d_residual, 0, p->n_tu, g->capture, &err);
+            if (empty(p) || !residual_tiled_run(p, nodes[i].d_coeff, nodes[i].d_residual) || !p->n_prep) continue;
+            st = run_plan_locked_free(p, nodes[i].d_coeff, nodes[i].d_residual, 0, p->n_tu, g->side, &err, kPartPrepass);
+            prep_done[i] = event();
+            if (e == hipSuccess) e = hipEventRecord(prep_done[i], g->side);
+            side_used = true;
         }
+        for (size_t i = 0; i < n_nodes && st == RMH_OK && e == hipSuccess; i++) {
+            rmh_plan* p = nodes[i].plan;
+            if (empty(p)) continue;
+            if (!residual_tiled_run(p, nodes[i].d_coeff, nodes[i].d_residual)) {
+                st = run_plan_locked_free(p, nodes[i].d_coeff, nodes[i].d_residual, 0, p->n_tu, g->capture, &err);
+                continue;
+            }
+            if (prep_done[i]) e = hipStreamWaitEvent(g->capture, prep_done[i], 0);
+            const bool generic = p->n_tu > p->n_fast;
+            if (generic && e == hipSuccess) {
+                hipEvent_t fork = event();
+                if (e == hipSuccess) e = hipEventRecord(fork, g->capture);
+                if (e == hipSuccess) e = hipStreamWaitEvent(g->side, fork, 0);
+                if (e == hipSuccess)
+                    st = run_plan_locked_free(p, nodes[i].d_coeff, nodes[i].d_residual, 0, p->n_tu, g->side, &err,
+                                              kPartGeneric);
+                side_used = true;
+            }
+            if (e == hipSuccess && st == RMH_OK)
+                st = run_plan_locked_free(p, nodes[i].d_coeff, nodes[i].d_residual, 0, p->n_tu, g->capture, &err,
+                                          kPartTiles);
+            if (generic && e == hipSuccess && st == RMH_OK) {
+                hipEvent_t join = event();
+                if (e == hipSuccess) e = hipEventRecord(join, g->side);
+                if (e == hipSuccess) e = hipStreamWaitEvent(g->capture, join, 0);
+            }
+        }
+        // the side branch always rejoins before the capture ends
+        if (e == hipSuccess) {
+            hipEvent_t end = event();
+            if (e == hipSuccess) e = hipEventRecord(end, g->side);
+            if (e == hipSuccess) e = hipStreamWaitEvent(g->capture, end, 0);
+        }
+        (void)side_used;
         hipError_t e2 = hipStreamEndCapture(g->capture, &g->graph);
-        if (e2 != hipSuccess && st == RMH_OK) e = e2;
+        if (e2 != hipSuccess && st == RMH_OK && e == hipSuccess) e = e2;
     }
     if (e == hipSuccess && st == RMH_OK) e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
     if (e != hipSuccess || st != RMH_OK) {
         if (g->exec) hipGraphExecDestroy(g->exec);
         if (g->graph) hipGraphDestroy(g->graph);
         if (g->capture) hipStreamDestroy(g->capture);
+        if (g->side) hipStreamDestroy(g->side);
+        for (hipEvent_t ev : g->events) hipEventDestroy(ev);
         delete g;
         return fail(ctx, RMH_EXECUTION, st != RMH_OK ? err : std::string("graph capture failed: ") + hipGetErrorString(e));
     }
@@ -3160,6 +3289,8 @@ RMH_EXPORT void rmh_graph_destroy(rmh_graph* g) {
     if (g->exec) hipGraphExecDestroy(g->exec);
     if (g->graph) hipGraphDestroy(g->graph);
     if (g->capture) hipStreamDestroy(g->capture);
+    if (g->side) hipStreamDestroy(g->side);
+    for (hipEvent_t ev : g->events) hipEventDestroy(ev);
     for (rmh_plan* p : g->plans) p->graph_refs.fetch_sub(1);
     delete g;
 }

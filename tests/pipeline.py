@@ -223,8 +223,13 @@ class ChainedStreams:
     the picture in a scratch set and deblocking writes it out of place into
     the output picture."""
 
-    def __init__(self, acc, rmh, refs, frames, out_of_place=False):
+    def __init__(self, acc, rmh, refs, frames, out_of_place=False, autotune=False):
+        """autotune: each step's MC plan picks its work items per workgroup and
+        its recon plan its tile splits (rmh_mc_plan_autotune,
+        rmh_residual_plan_autotune) before its graph is captured; the choices
+        are in self.tuned."""
         hip, abi = rmh.hip, rmh.abi
+        self.tuned = []
         self.hip, self.acc = hip, acc
         self.n_streams, self.n_steps = len(refs), len(frames)
         self.keep, self.plans, self.graphs = [], [], []
@@ -267,8 +272,15 @@ class ChainedStreams:
             self.bytes["mc"] += mc.bytes
             self.bytes["recon"] += rc.bytes
             self.bytes["deblock"] += db.bytes
+            if autotune:
+                self.tuned.append({"mc_waves": mc.autotune_mc(),
+                                   "recon_split": {"%dx%d" % k: v for k, v in rc.autotune(dc.ptr, 0).items() if v > 1}})
             self.graphs.append(acc.graph([(mc, 0, 0), (rc, dc.ptr, 0), (db, 0, 0)]))
         self.luma_pixels_per_step = sum(int(f["width"]) * int(f["height"]) for f in frames[0])
+        if autotune:
+            # the tuning runs wrote into the picture sets, the first reference included
+            hip.device_synchronize()
+            self.reset(refs)
 
     def run(self, stream, steps=None):
         """Launches the frame-step graphs 0 .. steps-1 in order on stream."""

@@ -54,14 +54,17 @@ def oracle_md5(po, rmh, streams):
     return out
 
 
-@pytest.mark.parametrize("out_of_place", [False, True])
-def test_sixteen_streams_eight_chained_steps_match_the_oracle(acc, rmh, streams, oracle_md5, out_of_place):
+@pytest.mark.parametrize("out_of_place,autotune", [(False, False), (True, False), (False, True)])
+def test_sixteen_streams_eight_chained_steps_match_the_oracle(acc, rmh, streams, oracle_md5, out_of_place, autotune):
+    """autotune: every step's MC and recon plans tuned before capture (the
+    shapes must not change a sample)."""
     refs, frames = streams
     n_tu = sum(len(f["tus"]) for fr in frames for f in fr)
     tools = sum(int(((f["tus"]["lfnst_idx"] > 0) | (f["tus"]["flags"] > 0)).sum()) for fr in frames for f in fr)
-    print("config 4 (%s): %d TUs, %d with LFNST / TS / DQ" % ("out of place" if out_of_place else "in place",
-                                                             n_tu, tools), flush=True)
-    chain = PL.ChainedStreams(acc, rmh, refs, frames, out_of_place=out_of_place)
+    print("config 4 (%s%s): %d TUs, %d with LFNST / TS / DQ" % ("out of place" if out_of_place else "in place",
+                                                               ", autotuned" if autotune else "", n_tu, tools),
+          flush=True)
+    chain = PL.ChainedStreams(acc, rmh, refs, frames, out_of_place=out_of_place, autotune=autotune)
     s = rmh.hip.Stream()
     try:
         for t in range(STEPS):

@@ -202,3 +202,61 @@ def test_cross_frame_aliasing_is_rejected(acc, rmh):
         acc.deblock_to_plan([fa, fb], c + c)
     ok = acc.deblock_to_plan([fa, fb], c + planes(buf.ptr + 3 * frame_bytes))
     ok.close()
+
+
+def test_mc_autotune_keeps_the_prediction_bit_exact(acc, po, rmh):
+    """The MC autotuner (rmh_mc_plan_autotune) picks 1, 2 or 4 work items
+    per workgroup; the prediction is the oracle's whatever it picks, for uni
+    and bi plans, and the tuner is refused while a graph holds the plan."""
+    print("mc autotune", flush=True)
+    hip, abi = rmh.hip, rmh.abi
+    bd = 10
+    d = W.inter_frame(9, width=320, height=192)
+    from frames import mc_frame_struct
+    want = [np.zeros_like(p) for p in d["ref"]]
+    f, k = mc_frame_struct(d["ref"], want, bd)
+    arr = (type(f) * 1)(f)
+    po.lib().orc_mc_batch(ctypes.addressof(arr), 1, d["pus"].ctypes.data, len(d["pus"]))
+    bufs, refs, dsts = [], [], []
+    for p in d["ref"]:
+        b, pl = _dev_plane(hip, p, bd, abi)
+        b2, pl2 = _dev_plane(hip, np.zeros_like(p), bd, abi)
+        bufs.append((b, b2))
+        refs.append(pl)
+        dsts.append(pl2)
+    mc = acc.mc_plan([abi.mc_frame(refs, dsts)], d["pus"])
+    s = hip.Stream()
+    waves = mc.autotune_mc(s.handle)
+    assert waves in (1, 2, 4)
+    for _, b2 in bufs:
+        b2.upload(np.zeros(b2.nbytes // 2, np.uint16))
+    mc.run(stream=s.handle)
+    s.synchronize()
+    for c in range(3):
+        assert (bufs[c][1].download(np.uint16).reshape(want[c].shape) == want[c]).all(), (waves, c)
+    graph = acc.graph([(mc, 0, 0)])
+    with pytest.raises(rmh.InvalidInput):
+        mc.autotune_mc(s.handle)
+    graph.close()
+    mc.close()
+    # bi
+    from frames import mc_bi_frame_struct
+    bi = W.bi_frame(9, width=320, height=192)
+    want = [np.zeros_like(p) for p in bi["ref0"]]
+    fb, kb = mc_bi_frame_struct(bi["ref0"], bi["ref1"], want, bd)
+    assert po.mc_bi_batch(fb, bi["pus"]) == 0
+    planes, keep = {}, []
+    for key in ("ref0", "ref1", "dst"):
+        ps = []
+        for p in bi["ref0" if key == "dst" else key]:
+            b, pl = _dev_plane(hip, np.zeros_like(p) if key == "dst" else p, bd, abi)
+            keep.append(b)
+            ps.append(pl)
+        planes[key] = (abi.Plane * 3)(*ps)
+    plan = acc.mc_bi_plan([abi.McBiFrame(planes["ref0"], planes["ref1"], planes["dst"])], bi["pus"])
+    assert plan.autotune_mc(s.handle) in (1, 2, 4)
+    plan.run(stream=s.handle)
+    s.synchronize()
+    for c in range(3):
+        assert (keep[6 + c].download(np.uint16).reshape(want[c].shape) == want[c]).all(), c
+    plan.close()
