@@ -137,9 +137,13 @@ __global__ __launch_bounds__(256) void dmvr_kernel(const rmh_mc_bi_frame* __rest
         if (lane == 0) red[wave][k] = v;
     }
     __syncthreads();
+    // the 25 costs stay in LDS: a private array indexed at run time would be
+    // given scratch memory for every lane of the launch
+    __shared__ int64_t tot[25];
+    if (tid < 25) tot[tid] = (int64_t)red[0][tid] + red[1][tid] + red[2][tid] + red[3][tid];
+    __syncthreads();
     if (tid != 0) return;
-    int64_t sads[5][5];
-    for (int k = 0; k < 25; k++) sads[k / 5][k % 5] = (int64_t)red[0][k] + red[1][k] + red[2][k] + red[3][k];
+    int64_t (*sads)[5] = reinterpret_cast<int64_t (*)[5]>(tot);
     int32_t* out = mvs + 4 * (size_t)J.slot;
     int64_t min_cost = sads[2][2];
     min_cost -= min_cost >> 2;
@@ -187,7 +191,7 @@ __global__ __launch_bounds__(256) void mc_bi_refined_kernel(const rmh_mc_bi_fram
                                                             const McRefItem* __restrict__ items,
                                                             const int32_t* __restrict__ mvs) {
     __shared__ int32_t I[2][kM * kM];   // luma intermediates, [v - va][u - ua]
-    __shared__ int32_t T[kTR * kM];     // horizontal pass of one list
+    __shared__ int32_t T[2][kTR * kM];  // horizontal pass of each list
     __shared__ int32_t flow[16][2];     // per 4x4 unit: (vx, vy)
     const McRefItem it = items[blockIdx.x];
     const rmh_mc_bi_frame& F = frames[it.frame];
@@ -205,9 +209,10 @@ __global__ __launch_bounds__(256) void mc_bi_refined_kernel(const rmh_mc_bi_fram
 #pragma unroll
     for (int k = 0; k < 4; k++) mv[k] = mvs[4 * (size_t)it.slot + k];
 
-    // luma intermediates of both lists (interpolate_luma_block, for_bi_pred)
-#pragma unroll 1
-    for (int l = 0; l < 2; l++) {
+    // luma intermediates of both lists (interpolate_luma_block, for_bi_pred),
+    // list 0 on threads 0-127 and list 1 on threads 128-255 side by side
+    {
+        const int l = tid >> 7, lt = tid & 127;
         const rmh_plane& p = l ? F.ref1[0] : F.ref0[0];
         const int mvx = mv[2 * l], mvy = mv[2 * l + 1];
         const int fx = mvx & 15, fy = mvy & 15;
@@ -215,16 +220,17 @@ __global__ __launch_bounds__(256) void mc_bi_refined_kernel(const rmh_mc_bi_fram
         const int8_t* cx = ((it.flags & 1) && fx == 8) ? kRLumaAltHpel : kRLuma8[fx];
         const int8_t* cy = ((it.flags & 1) && fy == 8) ? kRLumaAltHpel : kRLuma8[fy];
         int32_t* out = I[l];
-        if (fx == 0 && fy == 0) {
-            for (int i = tid; i < nu * nv; i += 256) {
+        const bool two_d = fx != 0 && fy != 0;
+        if (!two_d && fx == 0 && fy == 0) {
+            for (int i = lt; i < nu * nv; i += 128) {
                 const int v = i / nu, u = i - v * nu;
                 out[v * kM + u] = smp8(p, ix + u, iy + v) * (1 << head_room) - kOffs;
             }
-        } else if (fx == 0 || fy == 0) {
+        } else if (!two_d) {
             const bool hor = fy == 0;
             const int8_t* c = hor ? cx : cy;
             const int shift = 6 - head_room, offset = -kOffs * (1 << shift);
-            for (int i = tid; i < nu * nv; i += 256) {
+            for (int i = lt; i < nu * nv; i += 128) {
                 const int v = i / nu, u = i - v * nu;
                 int s = 0;
 #pragma unroll
@@ -234,22 +240,25 @@ __global__ __launch_bounds__(256) void mc_bi_refined_kernel(const rmh_mc_bi_fram
             }
         } else {
             const int shift1 = 6 - head_room, offset1 = -kOffs * (1 << shift1);
-            for (int i = tid; i < nu * (nv + 7); i += 256) {
+            for (int i = lt; i < nu * (nv + 7); i += 128) {
                 const int r = i / nu, u = i - r * nu;
                 int s = 0;
 #pragma unroll
                 for (int k = 0; k < 8; k++) s += smp8(p, ix + u + k - 3, iy + r - 3) * cx[k];
-                T[r * kM + u] = (s + offset1) >> shift1;
+                T[l][r * kM + u] = (s + offset1) >> shift1;
             }
-            __syncthreads();
-            for (int i = tid; i < nu * nv; i += 256) {
+        }
+        // every thread reaches both barriers; only a list with both
+        // fractions non-zero has a vertical pass between them
+        __syncthreads();
+        if (two_d)
+            for (int i = lt; i < nu * nv; i += 128) {
                 const int v = i / nu, u = i - v * nu;
                 int s = 0;
 #pragma unroll
-                for (int k = 0; k < 8; k++) s += T[(v + k) * kM + u] * cy[k];
+                for (int k = 0; k < 8; k++) s += T[l][(v + k) * kM + u] * cy[k];
                 out[v * kM + u] = s >> 6;
             }
-        }
         __syncthreads();
     }
 
