@@ -27,13 +27,14 @@ struct TileTu {
     uint32_t off;     // element offset of the TU block in the residual output
     uint32_t src;     // element offset of its levels (in ResidualArgs::prep when prepped)
     uint8_t cw, ch;   // coded region after zero-out (<= 32)
-    uint8_t scale;    // dequantisation scale (1 when prepped: the values are already dequantised)
+    uint8_t scale;    // dequantisation scale (1 when prepped: the values are already dequantised;
+                      // of QP + 1 under dependent quantisation)
     uint8_t shifts;   // dequant left shift | right shift << 4
     uint16_t mh, mv;  // dword offsets of the column-major matrices (col16)
     uint8_t shift2;   // second-stage shift, 20 - bit depth
     uint8_t prepped;  // 1: read the prepass output (ResidualArgs::prep), not the levels
-    uint8_t lfnst;    // tiles marked kTileLfnst: LFNST matrix (set * 2 + idx - 1) | transposed << 3
-    uint8_t reserved;
+    uint8_t lfnst;    // tool tiles: LFNST matrix (set * 2 + idx - 1) | transposed << 3 | 16 when the TU applies LFNST
+    uint8_t dq;       // 1: dependent quantisation of the whole coded region, applied in the kernel
 };
 // Recon target of a tile-eligible TU (recon plans only).
 struct TileDst {
@@ -77,11 +78,11 @@ struct ResidualArgs {
 hipError_t launch_residual(const ResidualArgs& a, hipStream_t stream);
 
 // A tile of TUs of one size class (log2 w, log2 h in 2..6) processed by one
-// 256-lane workgroup of the tiled residual kernel: TUs order[first .. first+count).
+// kTileThreads-lane workgroup of the tiled residual kernel: TUs order[first .. first+count).
 struct TileItem {
     uint32_t first;
     uint16_t count;
-    uint8_t lw;  // | kTileLfnst: every TU of the tile applies the inverse LFNST in the kernel
+    uint8_t lw;  // | kTileLfnst: a tool tile, whose TUs apply the inverse LFNST or dependent quantisation in the kernel
     uint8_t lh;
 };
 constexpr uint8_t kTileLfnst = 0x80;
@@ -89,7 +90,7 @@ constexpr uint8_t kTileLfnst = 0x80;
 // Tiled residual kernel for 2-D TUs with both sides >= 4 and no transform
 // skip: plain TUs from their levels, TUs with LFNST or dependent
 // quantisation from the prepass output. `a.order` is required.
-// items[0 .. n_plain) are plain tiles, items[n_plain .. n_items) LFNST tiles.
+// items[0 .. n_plain) are plain tiles, items[n_plain .. n_items) tool tiles (LFNST or dependent quantisation).
 hipError_t launch_residual_tiles(const ResidualArgs& a, const TileItem* items, uint32_t n_items, uint32_t n_plain,
                                  uint32_t lds_bytes, hipStream_t stream);
 

@@ -489,13 +489,14 @@ __global__ __launch_bounds__(64) void residual_kernel(ResidualArgs a) {
 struct TuParams {
     uint32_t off;    // first residual sample (offset + TU size <= n_coeff < 2^32)
     uint32_t src;    // first level: in a.coeff, or in a.prep when cwch bit 16 is set
-    int32_t cwch;    // cw | ch << 8 | prepped << 16 | LFNST code (matrix | transposed << 3) << 17
+    int32_t cwch;    // cw | ch << 8 | prepped << 16 | LFNST code (matrix | transposed << 3 | applies << 4) << 17 | dq << 22
     int32_t scale;
     int32_t shifts;  // dequant left shift | right shift << 8 | second-stage shift (20 - bd) << 16
     int32_t mh;      // dword offset of Mh pairs
     int32_t mv;      // dword offset of Mv pairs
     int32_t stride;  // recon: picture row pitch (samples)
     int32_t bpsmax;  // recon: bytes per sample | (2^bd - 1) << 8
+    int32_t dq;      // dependent quantisation: S | rs << 24 | (left shift > 0) << 30 (scale and shifts are then 1, 0)
     char* base;      // recon: sample (0, 0) of the block
 };
 static_assert(sizeof(TuParams) == 48, "residual_tile_lds reserves 48 bytes per TU");
@@ -651,16 +652,26 @@ __device__ __forceinline__ void tile_body(const ResidualArgs& a, const TileItem&
     const int tid = threadIdx.x;
 
     if (tid < 2) TB[tid] = -1;
+    bool dq_lane = false;
     for (int g = tid; g < G; g += kTileThreads) {
         const TileTu d = a.tile_tus[it.first + g];
         TuParams p;
         p.off = d.off;
         p.src = d.src;
-        p.cwch = d.cw | (d.ch << 8) | (d.prepped << 16) | (d.lfnst << 17);
+        p.cwch = d.cw | (d.ch << 8) | (d.prepped << 16) | (d.lfnst << 17) | (d.dq << 22);
         // dequantisation as level * S + rnd >> rs (see dequant_tile)
         const int ls = d.shifts & 15;
         p.scale = ls >= 10 ? 65536 : (int)d.scale << ls;
         p.shifts = (ls ? 0 : (d.shifts >> 4)) | (d.shift2 << 16);
+        p.dq = 0;
+        if (d.dq) {
+            // phase A keeps the raw levels; the dependent-quantisation pass
+            // dequantises them with these parameters
+            p.dq = p.scale | (((p.shifts & 255) | (ls > 0 ? 64 : 0)) << 24);
+            p.scale = 1;
+            p.shifts = d.shift2 << 16;
+            dq_lane = true;
+        }
         p.mh = d.mh;
         p.mv = d.mv;
         p.stride = 0;
@@ -674,6 +685,10 @@ __device__ __forceinline__ void tile_body(const ResidualArgs& a, const TileItem&
         }
         P[g] = p;
     }
+    // GMAX <= kTileThreads, so one lane per TU: a ballot covers the tile.
+    // Only tool tiles (the LFNST instantiation) hold such TUs.
+    static_assert(GMAX <= kTileThreads, "one phase-0 lane per TU");
+    const bool any_dq = LFNST && __builtin_amdgcn_ballot_w64(dq_lane) != 0;
     __syncthreads();
 
     // phase B's lane items: (TU, row y, column chunk), NCH_B chunks, in
@@ -684,6 +699,13 @@ __device__ __forceinline__ void tile_body(const ResidualArgs& a, const TileItem&
     constexpr int NPB = Hc / 2;  // matrix pairs per column
     constexpr bool B_PRE = RB * NPB <= 16;
     uint32_t mB[16];
+
+    // coefficient groups (4 x 4) per TU of the dependent-quantisation pass:
+    // every lane owns at most one, since a tile's coded regions hold at most
+    // 16 kTileThreads samples
+    constexpr int NCG = (Wc / 4) * (Hc / 4);
+    static_assert(GMAX * NCG <= kTileThreads, "one coefficient group per lane");
+    int cg_org = 0;
 
     // phase A: a lane takes 2 rows x 4 columns and stores 4 row pairs; the
     // whole Hc x Wc region is written (zeros outside the coded region), so
@@ -711,6 +733,10 @@ __device__ __forceinline__ void tile_body(const ResidualArgs& a, const TileItem&
                 }
             }
         }
+        // the dependent-quantisation pass's coefficient group (one per lane,
+        // in scan order within each TU): its top-left raster position, from
+        // the scan (whose first entry of every 16 is a group's origin)
+        if (any_dq) cg_org = a.scans[a.scan_off[LW][LH] + 16 * (tid & (NCG - 1))];
         // phase B's matrix columns: loaded with the coefficients instead of
         // after the barrier
         if constexpr (B_PRE) {
@@ -785,9 +811,84 @@ __device__ __forceinline__ void tile_body(const ResidualArgs& a, const TileItem&
         }
     }
     __syncthreads();
-    if constexpr (LFNST) {
+    if (LFNST && any_dq) {
+        // dequantize_dep_quant_bd (vvc/dct.rs:1191-1248) on the raw levels
+        // phase A left in LDS. The automaton's step from state x on a level of
+        // parity p is x' = swap_bits(x) ^ 2p (the table 32040 read as 2-bit
+        // entries), an affine map over GF(2). Run from the end of the scan
+        // (zeros keep state 0, so this equals starting at the last non-zero
+        // level), bit 1 of the state on arrival at scan position s, the only
+        // bit the reconstruction offset uses, is therefore the XOR of the
+        // parities at positions s + 1, s + 3, s + 5, ... A lane takes one 4 x 4
+        // coefficient group (16 consecutive scan positions): the XORs over the
+        // later groups of its TU come from two ballots, the rest from a walk
+        // over its own 16 levels.
+        const int g = tid / NCG, c = tid & (NCG - 1);
+        const bool act = g < G && ((P[g].cwch >> 22) & 1);
+        const int x0 = cg_org & (W - 1), y0 = cg_org >> LW;
+        uint2* col = reinterpret_cast<uint2*>(C + (g * Wc + x0) * CP + (y0 >> 1));
+        uint2 v[4] = {};
+        if (act) {
+#pragma unroll
+            for (int x = 0; x < 4; x++) v[x] = col[x * (CP / 2)];
+        }
+        // level (x, y) of the group, int16 pairs along the rows of a column
+        auto lev = [&](int x, int y) {
+            const uint32_t u = y < 2 ? v[x].x : v[x].y;
+            return (y & 1) ? (int)u >> 16 : (int)(int16_t)(u & 0xffff);
+        };
+        // raster positions y * 4 + x of the group's diagonal scan
+        constexpr uint8_t kScan4[16] = {0, 4, 1, 8, 5, 2, 12, 9, 6, 3, 13, 10, 7, 14, 11, 15};
+        int xe = 0, xo = 0;  // parity XORs over the group's even / odd scan positions
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            const int par = lev(kScan4[i] & 3, kScan4[i] >> 2) & 1;
+            if (i & 1) xo ^= par; else xe ^= par;
+        }
+        // the later groups of the TU: lanes tid + 1 .. (g + 1) NCG - 1
+        const int seg_end = (g + 1) * NCG;
+        const uint64_t upto = seg_end >= 64 ? ~0ull : (1ull << seg_end) - 1;
+        const uint64_t later = upto & ~((2ull << tid) - 1);
+        const uint64_t be = __builtin_amdgcn_ballot_w64(act && xe), bo = __builtin_amdgcn_ballot_w64(act && xo);
+        int se = __popcll(be & later) & 1, so = __popcll(bo & later) & 1;
+        if (act) {
+            const int dqp = P[g].dq;
+            const int S = dqp & 0x1ffff, rs = (dqp >> 24) & 63, wide_q = (dqp >> 30) & 1;
+            const int rnd = (1 << rs) >> 1;
+            int out[16];
+#pragma unroll
+            for (int i = 15; i >= 0; i--) {
+                const int pos = kScan4[i];
+                const int level = lev(pos & 3, pos >> 2);
+                // bit 1 of the state: the XOR over positions s + 1, s + 3, ...,
+                // the even-index positions after s when i is odd
+                const int A = (i & 1) ? se : so;
+                int val = 0;
+                if (level != 0) {
+                    int q = 2 * level + (level > 0 ? -A : A);
+                    // with a left shift every |q| >= 2^15 saturates: the clamp
+                    // keeps q * S within int32
+                    if (wide_q) q = min(max(q, -32768), 32767);
+                    val = min(max((__mul24(q, S) + rnd) >> rs, -32768), 32767);
+                }
+                out[pos] = val;
+                if (i & 1) so ^= level & 1; else se ^= level & 1;
+            }
+#pragma unroll
+            for (int x = 0; x < 4; x++) {
+                uint2 w;
+                w.x = __builtin_amdgcn_perm((uint32_t)out[4 + x], (uint32_t)out[x], 0x05040100u);
+                w.y = __builtin_amdgcn_perm((uint32_t)out[12 + x], (uint32_t)out[8 + x], 0x05040100u);
+                col[x * (CP / 2)] = w;
+            }
+        }
+        __syncthreads();
+    }
+    // tool tiles: every TU is LFNST or dependent quantisation; the tile flag
+    // says whether any applies LFNST
+    if (LFNST && (it.lw & kTileLfnst)) {
         // inverse_lfnst_rect (vvc/lfnst.rs:219-297) on the dequantised block in
-        // LDS: the first NZ positions of the 4x4 diagonal scan times the
+        // LDS, for the TUs that apply it: the first NZ positions of the 4x4 diagonal scan times the
         // kernel matrix give TR outputs, (sum + 64) >> 7 clipped to int16,
         // placed (transposed or not) in the top-left 4x4 / 8x8 region; the
         // coded region is 4..8 on a side, so it holds the whole input and
@@ -804,7 +905,8 @@ __device__ __forceinline__ void tile_body(const ResidualArgs& a, const TileItem&
 #pragma unroll 1
         for (int q = tid; q < G * TR; q += kTileThreads) {
             const int g = q / TR, j = q - g * TR;
-            const int code = (P[g].cwch >> 17) & 15;
+            const int code = (P[g].cwch >> 17) & 31;
+            if (!(code & 16)) continue;
             const int8_t* __restrict__ m = lt + (code & 7) * 16 * TR + j;
             const int16_t* c16 = reinterpret_cast<const int16_t*>(C + g * Wc * CP);
             int sum = 0;
@@ -820,6 +922,7 @@ __device__ __forceinline__ void tile_body(const ResidualArgs& a, const TileItem&
         for (int q = tid; q < G * TR; q += kTileThreads) {
             const int g = q / TR, j = q - g * TR;
             const int cwch = P[g].cwch, cw = cwch & 255, ch = (cwch >> 8) & 255;
+            if (!((cwch >> 21) & 1)) continue;  // no LFNST: its block stays as dequantised
             int ox, oy;
             if (SB == 4) {
                 ox = j & 3;
@@ -838,9 +941,9 @@ __device__ __forceinline__ void tile_body(const ResidualArgs& a, const TileItem&
             }
             if (ox < cw && oy < ch) reinterpret_cast<int16_t*>(C + g * Wc * CP)[ox * 2 * CP + oy] = stage[q];
         }
-        // the outputs reach the whole coded region of every TU (rows and
-        // columns 0 .. 3 or 0 .. 7): cover it in the bounding box
-        if (tid < G) {
+        // the outputs reach the whole coded region of every LFNST TU (rows
+        // and columns 0 .. 3 or 0 .. 7): cover it in the bounding box
+        if (tid < G && ((P[tid].cwch >> 21) & 1)) {
             const int cwch = P[tid].cwch;
             atomicMax(&TB[0], ((cwch >> 8) & 255) - 1);
             atomicMax(&TB[1], (cwch & 255) - 1);
@@ -1004,8 +1107,9 @@ __device__ __forceinline__ void tile_body(const ResidualArgs& a, const TileItem&
 }
 
 // RECON: add into the picture planes (a.place / a.frames) instead of writing
-// int16 residuals. LFNST: tiles of TUs that apply the inverse LFNST (an
-// instantiation of its own, so the plain tiles keep their register budget).
+// int16 residuals. LFNST: tool tiles, of TUs that apply the inverse LFNST or
+// dependent quantisation (an instantiation of its own, so the plain tiles
+// keep their register budget and code size).
 template <bool RECON, bool LFNST>
 __global__ __launch_bounds__(kTileThreads) void residual_tile_kernel(ResidualArgs a, const TileItem* __restrict__ items) {
     extern __shared__ __align__(16) uint32_t smem[];

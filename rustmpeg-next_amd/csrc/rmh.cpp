@@ -176,10 +176,12 @@ struct rmh_plan {
     const uint32_t* d_order = nullptr;       // fast TUs grouped by size class, then the others
     rmh::TileItem* d_tiles = nullptr;  // tiles of the fast TUs
     size_t n_tiles = 0, n_fast = 0, tile_capacity = 0;
-    uint32_t n_tiles_plain = 0;  // tiles [0, n_tiles_plain) are plain, the rest LFNST tiles
+    uint32_t n_tiles_plain = 0;  // tiles [0, n_tiles_plain) are plain, the rest tool tiles
     uint32_t tile_lds = 0;
-    // fast TUs of size class (log2 w, log2 h) are order[cls_first .. + cls_count);
-    // tiles of the class hold residual_tile_tus >> split TUs each (autotuned)
+    // fast TUs of size class (log2 w, log2 h) are order[cls_first .. + cls_count),
+    // the last cls_lfnst of them tool TUs (LFNST or dependent quantisation in
+    // the kernel); tiles of the class hold residual_tile_tus >> split TUs each
+    // (autotuned)
     uint32_t cls_first[7][7] = {}, cls_count[7][7] = {}, cls_lfnst[7][7] = {};
     uint8_t split[7][7] = {};
     // recon (residual added into picture planes); null for residual output
@@ -187,8 +189,8 @@ struct rmh_plan {
     const rmh_recon_frame* d_rframes = nullptr;
     const rmh::TileTu* d_tile_tus = nullptr;   // per fast TU, tile order
     const rmh::TileDst* d_tile_dst = nullptr;  // recon plans
-    // prepass of the fast TUs with LFNST or dependent quantisation: their
-    // indices are order[n_tu .. n_tu + n_prep), their dequantised blocks go to
+    // prepass of the fast TUs with LFNST or dependent quantisation that the
+    // tool tiles cannot apply: their indices are order[n_tu .. n_tu + n_prep), their dequantised blocks go to
     // d_prep at d_prep_off[k]
     int16_t* d_prep = nullptr;
     const uint32_t* d_prep_off = nullptr;
@@ -1051,31 +1053,39 @@ TileTable tile_table(const rmh_dbk_frame* frames, size_t n_frames) {
 
 
 // TUs the tiled kernel transforms: 2-D with both sides >= 4, no transform
-// skip, 8-byte aligned block. Those with LFNST or dependent quantisation are
-// dequantised (and LFNST-ed) first by the prepass, one workgroup per TU, and
-// the tiled kernel reads its output.
+// skip, 8-byte aligned block. Those with LFNST or dependent quantisation that
+// the tiles cannot apply themselves (tile_lfnst, tile_dq) are dequantised (and
+// LFNST-ed) first by the prepass, one workgroup per TU, and the tiled kernel
+// reads its output.
 bool tile_eligible(const rmh_tu_desc& t) {
     return t.log2_w >= 2 && t.log2_h >= 2 && !(t.flags & RMH_TU_TRANSFORM_SKIP) && (t.offset & 3) == 0 && !t.joint;
 }
 bool needs_prepass(const rmh_tu_desc& t);
 
-// LFNST TUs without dependent quantisation whose coded region covers the
-// 4x4 LFNST input: the tiled kernel applies the inverse LFNST itself, in LDS,
-// in tiles of their own.
+// LFNST TUs whose coded region covers the 4x4 LFNST input (and, under
+// dependent quantisation, is tile_dq): the tiled kernel applies the inverse
+// LFNST itself, in LDS, in tiles of their own.
 bool tile_lfnst(const rmh_tu_desc& t);
 
+// Dependent quantisation in the tiled kernel: the TU's coded region is its
+// whole scan region (min(w, 32) x min(h, 32)), so the levels the automaton
+// runs over are exactly the ones the tile loads.
+bool tile_dq(const rmh_tu_desc& t);
+
 bool needs_prepass(const rmh_tu_desc& t) {
-    return (t.lfnst_idx != 0 || (t.flags & RMH_TU_DEP_QUANT)) && !tile_lfnst(t);
+    const bool lfnst_ok = t.lfnst_idx == 0 || tile_lfnst(t);
+    const bool dq_ok = !(t.flags & RMH_TU_DEP_QUANT) || tile_dq(t);
+    return !(lfnst_ok && dq_ok);
 }
 
 // The LFNST kernel matrix (set * 2 + lfnst_idx - 1, set from the intra mode
-// as lfnst_set, vvc/lfnst.rs:85-129) in bits 0-2 and the transposition flag
-// in bit 3, as the tiled kernel reads them.
+// as lfnst_set, vvc/lfnst.rs:85-129) in bits 0-2, the transposition flag in
+// bit 3 and bit 4 set (the TU applies LFNST), as the tiled kernel reads them.
 uint8_t lfnst_code(const rmh_tu_desc& d) {
     const int m = d.lfnst_mode;
     const int set = m <= 1 ? 0 : m <= 12 ? 1 : m <= 23 ? 2 : m <= 44 ? 3 : m <= 55 ? 2 : 1;
     const bool tp = (m >= 35 && m < 67) || m >= 81;
-    return (uint8_t)((set * 2 + (d.lfnst_idx - 1)) | (tp ? 8 : 0));
+    return (uint8_t)((set * 2 + (d.lfnst_idx - 1)) | (tp ? 8 : 0) | 16);
 }
 
 // Coded region of a TU after the default, MTS, caller and LFNST zero-outs
@@ -1097,10 +1107,17 @@ void coded_region(const rmh_tu_desc& d, int* cw, int* ch) {
 }
 
 bool tile_lfnst(const rmh_tu_desc& t) {
-    if (t.lfnst_idx == 0 || (t.flags & RMH_TU_DEP_QUANT)) return false;
+    if (t.lfnst_idx == 0 || ((t.flags & RMH_TU_DEP_QUANT) && !tile_dq(t))) return false;
     int cw = 0, ch = 0;
     coded_region(t, &cw, &ch);
     return cw >= 4 && ch >= 4;
+}
+
+bool tile_dq(const rmh_tu_desc& t) {
+    if (!(t.flags & RMH_TU_DEP_QUANT)) return false;
+    int cw = 0, ch = 0;
+    coded_region(t, &cw, &ch);
+    return cw == std::min(1 << t.log2_w, 32) && ch == std::min(1 << t.log2_h, 32);
 }
 
 // The prepass list of a residual batch: TU indices (in the order of `order`
@@ -1116,9 +1133,9 @@ constexpr int kMaxTileSplit = 3;
 
 // Tiles of the fast TUs (largest class first) from the class ranges of
 // `order`, each class split as `split` says. The last lfnst[lw][lh] TUs of a
-// class range apply LFNST in the kernel: they get tiles of their own, marked
-// with rmh::kTileLfnst.
-// Plain tiles come first (their number is returned), then the LFNST tiles:
+// class range apply LFNST or dependent quantisation in the kernel: they get
+// tool tiles of their own, marked with rmh::kTileLfnst.
+// Plain tiles come first (their number is returned), then the tool tiles:
 // each kind is one launch.
 uint32_t tiles_from_classes(const uint32_t (*first)[7], const uint32_t (*count)[7], const uint32_t (*lfnst)[7],
                             const uint8_t (*split)[7], std::vector<rmh::TileItem>& tiles, uint32_t* lds) {
@@ -1160,7 +1177,7 @@ uint32_t build_residual_tiles(const rmh_tu_desc* tus, size_t n_tu, std::vector<u
     std::vector<uint32_t> prep_list;
     for (size_t i = 0; i < n_tu; i++) {
         if (tile_eligible(tus[i])) {
-            (tile_lfnst(tus[i]) ? cls_l : cls)[tus[i].log2_w][tus[i].log2_h].push_back((uint32_t)i);
+            (tile_lfnst(tus[i]) || tile_dq(tus[i]) ? cls_l : cls)[tus[i].log2_w][tus[i].log2_h].push_back((uint32_t)i);
             if (needs_prepass(tus[i])) {
                 prep_list.push_back((uint32_t)i);
                 prep->off.push_back((uint32_t)prep->elements);
@@ -1228,6 +1245,14 @@ std::vector<rmh::TileTu> build_tile_tus(const rmh::DeviceTables& tabs, const rmh
         t.ch = (uint8_t)ch;
         t.scale = kInvQuantScale[sqrt2][qp % 6];
         t.shifts = (uint8_t)((rshift < 0 ? -rshift : 0) | ((rshift > 0 ? rshift : 0) << 4));
+        if (tile_dq(d)) {
+            // dequantize_dep_quant_bd (vvc/dct.rs:1191-1248): the scale of
+            // QP + 1 and the shift 7 - (QP + 1) / 6 - tr_shift
+            const int qdq = qp + 1, shift = 7 - qdq / 6 - tr_shift;
+            t.scale = kInvQuantScale[sqrt2][qdq % 6];
+            t.shifts = (uint8_t)((shift < 0 ? -shift : 0) | ((shift > 0 ? shift : 0) << 4));
+            t.dq = 1;
+        }
         if (tile_lfnst(d)) t.lfnst = lfnst_code(d);
         if (prep.off_of[order[i]] != UINT32_MAX) {
             t.src = prep.off_of[order[i]];

@@ -161,3 +161,47 @@ def test_dequant_scale_readout_every_class_qp_and_bit_depth(acc, po):
     coeff[tus["offset"]] = vals[np.arange(len(tus)) % 4]
     print("dequant readout: %d TUs" % len(tus), flush=True)
     _check(acc, po, tus, coeff)
+
+
+@pytest.mark.parametrize("seed", [31, 32])
+def test_dependent_quantisation_in_tiles(acc, po, seed):
+    """Dependent quantisation applied by the tiled kernel (TUs whose coded
+    region is their whole scan region), mixed in tiles with plain TUs, with
+    LFNST on 4x4 and 8x8 blocks, next to TUs the prepass keeps (MTS-32
+    zero-out, caller zero-out, LFNST on other sizes): every tile class, bit
+    depths 8-12, QPs 0-63, full-range, saturating, sparse and all-odd levels
+    (the last make every group's parity XORs non-zero)."""
+    print("residual dependent quantisation in tiles seed %d" % seed, flush=True)
+    rng = np.random.default_rng(seed)
+    descs, blocks, off = [], [], 0
+    for lw in range(2, 7):
+        for lh in range(2, 7):
+            w, h = 1 << lw, 1 << lh
+            for i in range(12):
+                t = np.zeros(1, W.TU_DTYPE)[0]
+                t["offset"], t["log2_w"], t["log2_h"] = off, lw, lh
+                t["bit_depth"] = int(rng.choice([8, 10, 12]))
+                t["qp"] = int(rng.integers(0, 64 + 6 * (t["bit_depth"] - 8)))
+                ok_h = [0] + ([1, 2] if w <= 32 else [])
+                ok_v = [0] + ([1, 2] if h <= 32 else [])
+                t["tr_hor"], t["tr_ver"] = int(rng.choice(ok_h)), int(rng.choice(ok_v))
+                t["flags"] = W.DQ_FLAG if i % 4 != 3 else 0
+                if (w, h) in ((4, 4), (8, 8), (8, 16)) and i % 3 == 0:
+                    t["lfnst_idx"], t["lfnst_mode"] = int(rng.integers(1, 3)), int(rng.integers(0, 95))
+                    t["tr_hor"] = t["tr_ver"] = 0
+                if i == 10 and h > 4:
+                    t["skip_h"] = 2
+                kind = i % 4
+                if kind == 0:
+                    c = rng.integers(-32768, 32768, w * h)
+                elif kind == 1:
+                    c = rng.choice([-32768, 32767, -1, 1], w * h)
+                elif kind == 2:
+                    c = np.where(rng.random(w * h) < 0.15, rng.integers(-60, 61, w * h), 0)
+                else:
+                    c = 2 * rng.integers(-30, 31, w * h) + 1
+                descs.append(t)
+                blocks.append(np.clip(c, -32768, 32767).astype(np.int16))
+                off += w * h
+    tus = np.array(descs, dtype=W.TU_DTYPE)
+    _check(acc, po, tus, np.concatenate(blocks))
