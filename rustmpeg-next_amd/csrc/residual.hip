@@ -627,6 +627,131 @@ __device__ __forceinline__ void load_colp(const uint32_t* __restrict__ src, uint
     }
 }
 
+// Phases B and C of a plain tile of TUs from 8 x 8 to 16 x 16 whose TUs share
+// one transform pair. The matrix column a step needs is then the same in every
+// lane: it comes through the scalar cache into SGPRs (the constant address
+// space), one v_dot2 operand, instead of every lane fetching it.
+//   phase B  lane = coefficient column(s) (TU, x): the column's live rows in
+//            registers, one intermediate per output row y, for every y
+//   phase C  lane = intermediate row(s) (TU, y): the row's live pairs in
+//            registers, one residual per output column x, for every x; the
+//            row goes out with 8-byte stores
+// Arithmetic and clipping are those of the other passes.
+typedef const __attribute__((address_space(4))) uint32_t* MatPtr;
+
+template <int LW, int LH, int GMAX, int CP, int TP>
+__device__ __forceinline__ void uniform_passes(const ResidualArgs& a, uint32_t* C, uint32_t* T, const TuParams* P,
+                                               int G, int ec, int k1, int k2) {
+    constexpr int W = 1 << LW, H = 1 << LH;
+    constexpr int CPL = GMAX * W / kTileThreads;  // coefficient columns per lane: 1, 2 or 4
+    constexpr int RPL = GMAX * H / kTileThreads;  // intermediate rows per lane: 1, 2 or 4
+    static_assert(CPL >= 1 && RPL >= 1 && GMAX * W == CPL * kTileThreads && GMAX * H == RPL * kTileThreads,
+                  "a tile's columns and rows spread evenly over the lanes");
+    const int tid = threadIdx.x;
+    const MatPtr mat = (MatPtr)a.col16;
+    const int mv = __builtin_amdgcn_readfirstlane(P[0].mv), mh = __builtin_amdgcn_readfirstlane(P[0].mh);
+
+    // phase B: lane columns j = tid CPL + c (adjacent, so pairs pack)
+    if (ec > 0) {
+        auto vpass = [&](auto nq) {
+            constexpr int NQ = decltype(nq)::value;
+            uint2 cv[CPL][NQ];
+#pragma unroll
+            for (int c = 0; c < CPL; c++) {
+                const int j = tid * CPL + c, g = j >> LW, x = j & (W - 1);
+                const uint2* src = reinterpret_cast<const uint2*>(C + (g * W + x) * CP);
+#pragma unroll
+                for (int q = 0; q < NQ; q++) cv[c][q] = src[q];
+            }
+            const int j0 = tid * CPL, g = j0 >> LW, x0 = j0 & (W - 1);
+#pragma unroll
+            for (int y = 0; y < H; y++) {
+                uint32_t m[2 * NQ];
+#pragma unroll
+                for (int k = 0; k < 2 * NQ; k++) m[k] = mat[mv + y * 16 + k];
+                int v[CPL];
+#pragma unroll
+                for (int c = 0; c < CPL; c++) {
+                    int acc = dot2_first(cv[c][0].x, m[0], 64);
+                    acc = dot2(cv[c][0].y, m[1], acc);
+#pragma unroll
+                    for (int q = 1; q < NQ; q++) {
+                        acc = dot2(cv[c][q].x, m[2 * q], acc);
+                        acc = dot2(cv[c][q].y, m[2 * q + 1], acc);
+                    }
+                    v[c] = acc >> 7;
+                }
+                uint32_t* row = T + (g * H + y) * TP;
+                if constexpr (CPL == 1) {
+                    reinterpret_cast<int16_t*>(row)[x0] = (int16_t)(pack_sat16(v[0], 0) & 0xffff);
+                } else if constexpr (CPL == 2) {
+                    row[x0 >> 1] = pack_sat16(v[0], v[1]);
+                } else {
+                    *reinterpret_cast<uint2*>(row + (x0 >> 1)) = make_uint2(pack_sat16(v[0], v[1]), pack_sat16(v[2], v[3]));
+                }
+            }
+        };
+        with_groups<1, (H / 4 > 0 ? H / 4 : 1)>((k1 + 1) >> 1, vpass);
+    }
+    __syncthreads();
+
+    // phase C: lane rows r = tid RPL + c
+    auto hpass = [&](auto nq) {
+        constexpr int NQ = decltype(nq)::value;
+        uint2 tv[RPL][NQ > 0 ? NQ : 1];
+        int sh[RPL], rnd[RPL];
+#pragma unroll
+        for (int c = 0; c < RPL; c++) {
+            const int r = tid * RPL + c, g = r >> LH, y = r & (H - 1);
+            const uint2* src = reinterpret_cast<const uint2*>(T + (g * H + y) * TP);
+#pragma unroll
+            for (int q = 0; q < NQ; q++) tv[c][q] = src[q];
+            sh[c] = P[g < G ? g : 0].shifts >> 16;
+            rnd[c] = 1 << (sh[c] - 1);
+        }
+        uint32_t o[RPL][W / 2];
+#pragma unroll
+        for (int x = 0; x < W; x += 2) {
+            uint32_t m0[NQ > 0 ? 2 * NQ : 1], m1[NQ > 0 ? 2 * NQ : 1];
+#pragma unroll
+            for (int k = 0; k < 2 * NQ; k++) {
+                m0[k] = mat[mh + x * 16 + k];
+                m1[k] = mat[mh + (x + 1) * 16 + k];
+            }
+#pragma unroll
+            for (int c = 0; c < RPL; c++) {
+                int a0 = rnd[c], a1 = rnd[c];
+                if constexpr (NQ > 0) {
+                    a0 = dot2_first(tv[c][0].x, m0[0], rnd[c]);
+                    a1 = dot2_first(tv[c][0].x, m1[0], rnd[c]);
+                    a0 = dot2(tv[c][0].y, m0[1], a0);
+                    a1 = dot2(tv[c][0].y, m1[1], a1);
+#pragma unroll
+                    for (int q = 1; q < NQ; q++) {
+                        a0 = dot2(tv[c][q].x, m0[2 * q], a0);
+                        a1 = dot2(tv[c][q].x, m1[2 * q], a1);
+                        a0 = dot2(tv[c][q].y, m0[2 * q + 1], a0);
+                        a1 = dot2(tv[c][q].y, m1[2 * q + 1], a1);
+                    }
+                }
+                o[c][x >> 1] = pack_sat16(a0 >> sh[c], a1 >> sh[c]);
+            }
+        }
+        // each row is W contiguous int16 residuals; TU offsets are multiples
+        // of 4 samples, so every row start is 8-byte aligned
+#pragma unroll
+        for (int c = 0; c < RPL; c++) {
+            const int r = tid * RPL + c, g = r >> LH, y = r & (H - 1);
+            if (g < G) {
+                RMH_GLOBAL u32x2* dst = gdst<u32x2>(a.residual + P[g].off + (uint32_t)(y * W));
+#pragma unroll
+                for (int k = 0; k < W / 4; k++) dst[k] = u32x2{o[c][2 * k], o[c][2 * k + 1]};
+            }
+        }
+    };
+    with_groups<0, (W / 4 > 0 ? W / 4 : 1)>((k2 + 1) >> 1, hpass);
+}
+
 // Rows per prediction-fetch batch of a recon lane (phase C): 8 keeps the
 // kernel at 6 waves per SIMD, where 16 would need 128 VGPRs.
 constexpr int kReconBatch = 8;
@@ -697,7 +822,12 @@ __device__ __forceinline__ void tile_body(const ResidualArgs& a, const TileItem&
     constexpr int NCH_B = (kTileThreads / (GMAX * H)) > 0 ? kTileThreads / (GMAX * H) : 1;
     constexpr int RB = (GMAX * H * NCH_B + kTileThreads - 1) / kTileThreads;
     constexpr int NPB = Hc / 2;  // matrix pairs per column
-    constexpr bool B_PRE = RB * NPB <= 16;
+    // Plain tiles of TUs from 8 x 8 to 16 x 16 take the uniform-matrix passes
+    // (uniform_passes): every TU of such a tile has the same transform pair.
+    // (4-sample sides measured slower that way: one lane then holds a whole
+    // TU, and the lanes' LDS accesses fall on few banks.)
+    constexpr bool UNI = !RECON && !LFNST && W >= 8 && W <= 16 && H >= 8 && H <= 16;
+    constexpr bool B_PRE = !UNI && RB * NPB <= 16;
     uint32_t mB[16];
 
     // coefficient groups (4 x 4) per TU of the dependent-quantisation pass:
@@ -953,6 +1083,10 @@ __device__ __forceinline__ void tile_body(const ResidualArgs& a, const TileItem&
     const int er = __builtin_amdgcn_readfirstlane(TB[0]) + 1;
     const int ec = __builtin_amdgcn_readfirstlane(TB[1]) + 1;
     const int k1 = (er + 1) >> 1, k2 = (ec + 1) >> 1;
+    if constexpr (UNI) {
+        uniform_passes<LW, LH, GMAX, CP, TP>(a, C, T, P, G, ec, k1, k2);
+        return;
+    }
 
     // phase B: vertical pass, lane = (TU, row y, column chunk); rows past
     // the bounding box are read in groups of 4 (zeros, from zero coefficients)

@@ -183,6 +183,7 @@ struct rmh_plan {
     // the kernel); tiles of the class hold residual_tile_tus >> split TUs each
     // (autotuned)
     uint32_t cls_first[7][7] = {}, cls_count[7][7] = {}, cls_lfnst[7][7] = {};
+    std::vector<uint32_t> tile_breaks;  // positions in order where a transform pair starts (uniform classes)
     uint8_t split[7][7] = {};
     // recon (residual added into picture planes); null for residual output
     const rmh_tu_place* d_place = nullptr;
@@ -1136,9 +1137,12 @@ constexpr int kMaxTileSplit = 3;
 // class range apply LFNST or dependent quantisation in the kernel: they get
 // tool tiles of their own, marked with rmh::kTileLfnst.
 // Plain tiles come first (their number is returned), then the tool tiles:
-// each kind is one launch.
+// each kind is one launch. A tile also ends before every position of `breaks`
+// (ascending positions in `order`), so that it holds one transform pair
+// where the kernel needs that.
 uint32_t tiles_from_classes(const uint32_t (*first)[7], const uint32_t (*count)[7], const uint32_t (*lfnst)[7],
-                            const uint8_t (*split)[7], std::vector<rmh::TileItem>& tiles, uint32_t* lds) {
+                            const uint8_t (*split)[7], const std::vector<uint32_t>& breaks,
+                            std::vector<rmh::TileItem>& tiles, uint32_t* lds) {
     tiles.clear();
     *lds = 0;
     uint32_t n_plain = 0;
@@ -1150,10 +1154,13 @@ uint32_t tiles_from_classes(const uint32_t (*first)[7], const uint32_t (*count)[
                 const int per = std::max(1, rmh::residual_tile_tus(lw, lh) >> split[lw][lh]);
                 const uint32_t plain = count[lw][lh] - lfnst[lw][lh];
                 const uint32_t a = part ? plain : 0, b = part ? count[lw][lh] : plain;
-                for (uint32_t f = a; f < b; f += per) {
-                    const uint32_t cnt = std::min<uint32_t>(per, b - f);
-                    tiles.push_back({first[lw][lh] + f, (uint16_t)cnt, (uint8_t)(lw | (part ? rmh::kTileLfnst : 0)),
-                                     (uint8_t)lh});
+                for (uint32_t f = a; f < b;) {
+                    uint32_t cnt = std::min<uint32_t>(per, b - f);
+                    const uint32_t pos = first[lw][lh] + f;
+                    auto nb = std::upper_bound(breaks.begin(), breaks.end(), pos);
+                    if (nb != breaks.end() && *nb < pos + cnt) cnt = *nb - pos;
+                    tiles.push_back({pos, (uint16_t)cnt, (uint8_t)(lw | (part ? rmh::kTileLfnst : 0)), (uint8_t)lh});
+                    f += cnt;
                 }
                 *lds = std::max(*lds, rmh::residual_tile_lds(lw, lh));
             }
@@ -1162,13 +1169,19 @@ uint32_t tiles_from_classes(const uint32_t (*first)[7], const uint32_t (*count)[
     return n_plain;
 }
 
+// Plain tiles of these classes run the tiled kernel's uniform-matrix passes,
+// which need every TU of a tile to share its transform pair.
+bool uniform_class(int lw, int lh) { return lw >= 3 && lw <= 4 && lh >= 3 && lh <= 4; }
+
 // Groups tile-eligible TUs by size class (largest class first), appends the
 // remaining TUs and then the prepass TUs (so order holds n_tu + prepass
-// entries); fills the class ranges, the unsplit tile list and the prepass
-// offsets.
+// entries); fills the class ranges, the unsplit tile list, the prepass
+// offsets and the tile breaks (the plain TUs of a uniform class are ordered
+// by transform pair, and each new pair starts a tile).
 uint32_t build_residual_tiles(const rmh_tu_desc* tus, size_t n_tu, std::vector<uint32_t>& order,
                           std::vector<rmh::TileItem>& tiles, size_t* n_fast, uint32_t* lds,
-                          uint32_t (*first)[7], uint32_t (*count)[7], uint32_t (*lfnst)[7], Prepass* prep) {
+                          uint32_t (*first)[7], uint32_t (*count)[7], uint32_t (*lfnst)[7], Prepass* prep,
+                          std::vector<uint32_t>* breaks) {
     std::vector<uint32_t> cls[7][7], cls_l[7][7];
     std::vector<uint32_t> rest;
     prep->off.clear();
@@ -1189,12 +1202,20 @@ uint32_t build_residual_tiles(const rmh_tu_desc* tus, size_t n_tu, std::vector<u
         }
     }
     order.clear();
+    breaks->clear();
     for (int lw = 0; lw < 7; lw++)
         for (int lh = 0; lh < 7; lh++) first[lw][lh] = count[lw][lh] = lfnst[lw][lh] = 0;
+    auto pair_of = [&](uint32_t i) { return tus[i].tr_hor * 3 + tus[i].tr_ver; };
     for (int area = 12; area >= 4; area--)
         for (int lw = 2; lw <= 6; lw++) {
             int lh = area - lw;
             if (lh < 2 || lh > 6) continue;
+            if (uniform_class(lw, lh)) {
+                std::vector<uint32_t>& c = cls[lw][lh];
+                std::stable_sort(c.begin(), c.end(), [&](uint32_t x, uint32_t y) { return pair_of(x) < pair_of(y); });
+                for (size_t k = 1; k < c.size(); k++)
+                    if (pair_of(c[k]) != pair_of(c[k - 1])) breaks->push_back((uint32_t)(order.size() + k));
+            }
             first[lw][lh] = (uint32_t)order.size();
             count[lw][lh] = (uint32_t)(cls[lw][lh].size() + cls_l[lw][lh].size());
             lfnst[lw][lh] = (uint32_t)cls_l[lw][lh].size();
@@ -1205,7 +1226,7 @@ uint32_t build_residual_tiles(const rmh_tu_desc* tus, size_t n_tu, std::vector<u
     order.insert(order.end(), rest.begin(), rest.end());
     order.insert(order.end(), prep_list.begin(), prep_list.end());
     const uint8_t none[7][7] = {};
-    return tiles_from_classes(first, count, lfnst, none, tiles, lds);
+    return tiles_from_classes(first, count, lfnst, none, *breaks, tiles, lds);
 }
 
 // Tiles a plan can need at the largest split (each class's LFNST TUs start
@@ -1215,7 +1236,7 @@ size_t tile_capacity(const uint32_t (*count)[7]) {
     for (int lw = 2; lw <= 6; lw++)
         for (int lh = 2; lh <= 6; lh++) {
             const size_t per = std::max(1, rmh::residual_tile_tus(lw, lh) >> kMaxTileSplit);
-            n += (count[lw][lh] + per - 1) / per + 1;
+            n += (count[lw][lh] + per - 1) / per + 1 + (uniform_class(lw, lh) ? 8 : 0);  // + transform-pair breaks
         }
     return n;
 }
@@ -1530,7 +1551,7 @@ rmh_status create_residual_plan(rmh_ctx* ctx, const rmh_tu_desc* tus, size_t n_t
         std::vector<rmh::TileItem> tiles;
         Prepass prep;
         p->n_tiles_plain = build_residual_tiles(tus, n_tu, order, tiles, &p->n_fast, &p->tile_lds, p->cls_first,
-                                                p->cls_count, p->cls_lfnst, &prep);
+                                                p->cls_count, p->cls_lfnst, &prep, &p->tile_breaks);
         p->n_prep = prep.off.size();
         p->tile_capacity = std::max(tiles.size(), tile_capacity(p->cls_count));
         size_t tb = aligned(n_tu * sizeof(rmh_tu_desc)), ob = aligned(order.size() * 4),
@@ -1668,7 +1689,7 @@ RMH_EXPORT rmh_status rmh_residual_plan_autotune(rmh_plan* plan, const int16_t* 
     auto time_plan = [&](const uint8_t (*split)[7], float* ms) -> hipError_t {
         std::vector<rmh::TileItem> tiles;
         uint32_t lds = 0;
-        const uint32_t n_plain = tiles_from_classes(plan->cls_first, plan->cls_count, plan->cls_lfnst, split, tiles, &lds);
+        const uint32_t n_plain = tiles_from_classes(plan->cls_first, plan->cls_count, plan->cls_lfnst, split, plan->tile_breaks, tiles, &lds);
         hipError_t r = hipMemcpy(plan->d_tiles, tiles.data(), tiles.size() * sizeof(rmh::TileItem), hipMemcpyHostToDevice);
         for (uint32_t k = 0; k < 2 && r == hipSuccess; k++)
             r = rmh::launch_residual_tiles(a, plan->d_tiles, (uint32_t)tiles.size(), n_plain, lds, s);
@@ -1713,7 +1734,8 @@ RMH_EXPORT rmh_status rmh_residual_plan_autotune(rmh_plan* plan, const int16_t* 
     if (e == hipSuccess) {
         std::vector<rmh::TileItem> tiles;
         plan->n_tiles_plain =
-            tiles_from_classes(plan->cls_first, plan->cls_count, plan->cls_lfnst, plan->split, tiles, &plan->tile_lds);
+            tiles_from_classes(plan->cls_first, plan->cls_count, plan->cls_lfnst, plan->split, plan->tile_breaks, tiles,
+                               &plan->tile_lds);
         e = hipMemcpy(plan->d_tiles, tiles.data(), tiles.size() * sizeof(rmh::TileItem), hipMemcpyHostToDevice);
         plan->n_tiles = tiles.size();
     }
@@ -1746,7 +1768,7 @@ RMH_EXPORT rmh_status rmh_residual_batch(rmh_ctx* ctx, const rmh_tu_desc* tus, s
     rmh_plan tmp;
     Prepass prep;
     tmp.n_tiles_plain = build_residual_tiles(tus, n_tu, order, tiles, &tmp.n_fast, &tmp.tile_lds, tmp.cls_first,
-                                             tmp.cls_count, tmp.cls_lfnst, &prep);
+                                             tmp.cls_count, tmp.cls_lfnst, &prep, &tmp.tile_breaks);
     hipError_t e = ctx->dev.reserve(aligned(tb) + 2 * aligned(cb) + aligned(order.size() * 4) +
                                     aligned(tiles.size() * sizeof(rmh::TileItem)) +
                                     aligned(tmp.n_fast * sizeof(rmh::TileTu)) + aligned(prep.off.size() * 4) +
