@@ -94,25 +94,42 @@ constexpr uint8_t kTileLfnst = 0x80;
 hipError_t launch_residual_tiles(const ResidualArgs& a, const TileItem* items, uint32_t n_items, uint32_t n_plain,
                                  uint32_t lds_bytes, hipStream_t stream);
 
-// TUs per full tile of class (lw, lh) and the LDS bytes such a tile needs
-// (the kernel lays LDS out for a full tile whatever the tile's count).
-// Lanes of a tiled-kernel workgroup; a full tile holds 16 samples per lane.
+// Size classes whose plain tiles (residual output, no LFNST or dependent
+// quantisation in the kernel) run the uniform-matrix passes: every TU of such
+// a tile has the same transform pair, a lane owns coefficient columns and then
+// intermediate rows, and the intermediates overlay the coefficients in LDS.
+// The host orders and cuts tiles by this same predicate.
+constexpr bool residual_uniform_class(int lw, int lh) {
+    return ((lw == 3 || lw == 4) && (lh == 3 || lh == 4)) || (lw == 5 && (lh == 4 || lh == 5)) || (lw == 4 && lh == 5);
+}
+
+// Lanes of a tiled-kernel workgroup; a full tile holds 16 samples per lane
+// (a uniform tile with a 32-sample side twice as many, so that its rows and
+// its columns each cover the lanes).
 constexpr int kTileThreads = 64;
 
-constexpr int residual_tile_tus(int lw, int lh) {
-    return ((kTileThreads * 16) >> (lw + lh)) > 0 ? (kTileThreads * 16) >> (lw + lh) : 1;
+// TUs per full tile of class (lw, lh), uniform (`uni`) or not.
+constexpr int residual_tile_tus(int lw, int lh, bool uni = false) {
+    int g = ((kTileThreads * 16) >> (lw + lh)) > 0 ? (kTileThreads * 16) >> (lw + lh) : 1;
+    if (uni) {
+        if ((g << lh) < kTileThreads) g = kTileThreads >> lh;
+        if ((g << lw) < kTileThreads) g = kTileThreads >> lw;
+    }
+    return g;
 }
 // Dword pitch of a coefficient column (lc = log2 of the coded height) or an
 // intermediate row (lc = log2 of the coded width) in the tiled kernel's LDS:
 // 2^lc / 2 int16 pairs, padded to 2 mod 4 dwords.
 constexpr int residual_tile_pitch(int lc) { return lc <= 2 ? 2 : (1 << lc) / 2 + 2; }
 
-inline uint32_t residual_tile_lds(int lw, int lh) {
-    uint32_t count = (uint32_t)residual_tile_tus(lw, lh);
-    int lwc = lw < 5 ? lw : 5, lhc = lh < 5 ? lh : 5;
-    uint32_t c = count * (1u << lwc) * residual_tile_pitch(lhc) * 4;  // coefficient columns
-    uint32_t t = count * (1u << lh) * residual_tile_pitch(lwc) * 4;  // intermediate rows
-    return c + t + count * 48 + 16;                                  // + per-TU parameters + tile bounding box
+// The LDS bytes a full tile needs (the kernel lays LDS out for a full tile
+// whatever the tile's count).
+constexpr uint32_t residual_tile_lds(int lw, int lh, bool uni = false) {
+    const uint32_t count = (uint32_t)residual_tile_tus(lw, lh, uni);
+    const int lwc = lw < 5 ? lw : 5, lhc = lh < 5 ? lh : 5;
+    const uint32_t c = count * (1u << lwc) * residual_tile_pitch(lhc) * 4;  // coefficient columns
+    const uint32_t t = count * (1u << lh) * residual_tile_pitch(lwc) * 4;   // intermediate rows
+    return (uni ? (c > t ? c : t) : c + t) + count * 48 + 16;               // + per-TU parameters + tile bounding box
 }
 
 // One inter work item: a 16x16 luma tile (and its 8x8 chroma tiles) of one

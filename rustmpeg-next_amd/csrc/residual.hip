@@ -627,12 +627,16 @@ __device__ __forceinline__ void load_colp(const uint32_t* __restrict__ src, uint
     }
 }
 
-// Phases B and C of a plain tile of TUs from 8 x 8 to 16 x 16 whose TUs share
-// one transform pair. The matrix column a step needs is then the same in every
+// Phases B and C of a plain tile of a uniform class (residual_uniform_class:
+// sides of 8 and 16, and 32 x 32, 32 x 16, 16 x 32) whose TUs share one
+// transform pair. (4-sample sides measured slower this way: one lane then
+// holds a whole TU, and the lanes' LDS accesses fall on few banks.) The matrix column a step needs is then the same in every
 // lane: it comes through the scalar cache into SGPRs (the constant address
 // space), one v_dot2 operand, instead of every lane fetching it.
 //   phase B  lane = coefficient column(s) (TU, x): the column's live rows in
-//            registers, one intermediate per output row y, for every y
+//            registers, one intermediate per output row y, for every y (the
+//            intermediates overwrite the coefficients: a barrier separates
+//            the last column read from the first store)
 //   phase C  lane = intermediate row(s) (TU, y): the row's live pairs in
 //            registers, one residual per output column x, for every x; the
 //            row goes out with 8-byte stores
@@ -663,6 +667,7 @@ __device__ __forceinline__ void uniform_passes(const ResidualArgs& a, uint32_t* 
 #pragma unroll
                 for (int q = 0; q < NQ; q++) cv[c][q] = src[q];
             }
+            __syncthreads();
             const int j0 = tid * CPL, g = j0 >> LW, x0 = j0 & (W - 1);
 #pragma unroll
             for (int y = 0; y < H; y++) {
@@ -766,13 +771,19 @@ __device__ __forceinline__ void tile_body(const ResidualArgs& a, const TileItem&
     // store, or 32 consecutive rows of a phase B store, fall in distinct
     // banks; even, for 8-byte reads
     constexpr int CP = residual_tile_pitch(LHC), TP = residual_tile_pitch(LWC);
-    constexpr int GMAX = residual_tile_tus(LW, LH);
+    // Plain tiles of the uniform classes take the uniform-matrix passes
+    // (uniform_passes): every TU of such a tile has the same transform pair.
+    constexpr bool UNI = !RECON && !LFNST && residual_uniform_class(LW, LH);
+    constexpr int GMAX = residual_tile_tus(LW, LH, UNI);
     const int G = it.count;
-    // LDS: the coefficient columns, the intermediate rows, the per-TU
-    // parameters, the tile bounding box.
+    // LDS: the coefficient columns, the intermediate rows (over the
+    // coefficients in a uniform tile, whose lanes hold their columns in
+    // registers before the first intermediate is stored), the per-TU
+    // parameters, the tile bounding box; residual_tile_lds has the size.
+    constexpr int CW = GMAX * Wc * CP, TW = GMAX * H * TP;
     uint32_t* C = smem;
-    uint32_t* T = C + GMAX * Wc * CP;
-    TuParams* P = reinterpret_cast<TuParams*>(T + GMAX * H * TP);
+    uint32_t* T = UNI ? C : C + CW;
+    TuParams* P = reinterpret_cast<TuParams*>(C + (UNI ? (CW > TW ? CW : TW) : CW + TW));
     int* TB = reinterpret_cast<int*>(P + GMAX);  // tile bounding box: max row, max column of non-zeros
     const int tid = threadIdx.x;
 
@@ -822,11 +833,6 @@ __device__ __forceinline__ void tile_body(const ResidualArgs& a, const TileItem&
     constexpr int NCH_B = (kTileThreads / (GMAX * H)) > 0 ? kTileThreads / (GMAX * H) : 1;
     constexpr int RB = (GMAX * H * NCH_B + kTileThreads - 1) / kTileThreads;
     constexpr int NPB = Hc / 2;  // matrix pairs per column
-    // Plain tiles of TUs from 8 x 8 to 16 x 16 take the uniform-matrix passes
-    // (uniform_passes): every TU of such a tile has the same transform pair.
-    // (4-sample sides measured slower that way: one lane then holds a whole
-    // TU, and the lanes' LDS accesses fall on few banks.)
-    constexpr bool UNI = !RECON && !LFNST && W >= 8 && W <= 16 && H >= 8 && H <= 16;
     constexpr bool B_PRE = !UNI && RB * NPB <= 16;
     uint32_t mB[16];
 
@@ -834,7 +840,7 @@ __device__ __forceinline__ void tile_body(const ResidualArgs& a, const TileItem&
     // every lane owns at most one, since a tile's coded regions hold at most
     // 16 kTileThreads samples
     constexpr int NCG = (Wc / 4) * (Hc / 4);
-    static_assert(GMAX * NCG <= kTileThreads, "one coefficient group per lane");
+    static_assert(!LFNST || GMAX * NCG <= kTileThreads, "one coefficient group per lane (tool tiles)");
     int cg_org = 0;
 
     // phase A: a lane takes 2 rows x 4 columns and stores 4 row pairs; the

@@ -1140,8 +1140,9 @@ constexpr int kMaxTileSplit = 3;
 // each kind is one launch. A tile also ends before every position of `breaks`
 // (ascending positions in `order`), so that it holds one transform pair
 // where the kernel needs that.
+// Recon plans (`recon`) have no uniform tiles.
 uint32_t tiles_from_classes(const uint32_t (*first)[7], const uint32_t (*count)[7], const uint32_t (*lfnst)[7],
-                            const uint8_t (*split)[7], const std::vector<uint32_t>& breaks,
+                            const uint8_t (*split)[7], const std::vector<uint32_t>& breaks, bool recon,
                             std::vector<rmh::TileItem>& tiles, uint32_t* lds) {
     tiles.clear();
     *lds = 0;
@@ -1151,7 +1152,8 @@ uint32_t tiles_from_classes(const uint32_t (*first)[7], const uint32_t (*count)[
             for (int lw = 2; lw <= 6; lw++) {
                 const int lh = area - lw;
                 if (lh < 2 || lh > 6 || count[lw][lh] == 0) continue;
-                const int per = std::max(1, rmh::residual_tile_tus(lw, lh) >> split[lw][lh]);
+                const bool uni = !part && !recon && rmh::residual_uniform_class(lw, lh);
+                const int per = std::max(1, rmh::residual_tile_tus(lw, lh, uni) >> split[lw][lh]);
                 const uint32_t plain = count[lw][lh] - lfnst[lw][lh];
                 const uint32_t a = part ? plain : 0, b = part ? count[lw][lh] : plain;
                 for (uint32_t f = a; f < b;) {
@@ -1162,16 +1164,12 @@ uint32_t tiles_from_classes(const uint32_t (*first)[7], const uint32_t (*count)[
                     tiles.push_back({pos, (uint16_t)cnt, (uint8_t)(lw | (part ? rmh::kTileLfnst : 0)), (uint8_t)lh});
                     f += cnt;
                 }
-                *lds = std::max(*lds, rmh::residual_tile_lds(lw, lh));
+                if (b > a) *lds = std::max(*lds, rmh::residual_tile_lds(lw, lh, uni));
             }
         if (part == 0) n_plain = (uint32_t)tiles.size();
     }
     return n_plain;
 }
-
-// Plain tiles of these classes run the tiled kernel's uniform-matrix passes,
-// which need every TU of a tile to share its transform pair.
-bool uniform_class(int lw, int lh) { return lw >= 3 && lw <= 4 && lh >= 3 && lh <= 4; }
 
 // Groups tile-eligible TUs by size class (largest class first), appends the
 // remaining TUs and then the prepass TUs (so order holds n_tu + prepass
@@ -1181,7 +1179,7 @@ bool uniform_class(int lw, int lh) { return lw >= 3 && lw <= 4 && lh >= 3 && lh 
 uint32_t build_residual_tiles(const rmh_tu_desc* tus, size_t n_tu, std::vector<uint32_t>& order,
                           std::vector<rmh::TileItem>& tiles, size_t* n_fast, uint32_t* lds,
                           uint32_t (*first)[7], uint32_t (*count)[7], uint32_t (*lfnst)[7], Prepass* prep,
-                          std::vector<uint32_t>* breaks) {
+                          std::vector<uint32_t>* breaks, bool recon) {
     std::vector<uint32_t> cls[7][7], cls_l[7][7];
     std::vector<uint32_t> rest;
     prep->off.clear();
@@ -1210,7 +1208,9 @@ uint32_t build_residual_tiles(const rmh_tu_desc* tus, size_t n_tu, std::vector<u
         for (int lw = 2; lw <= 6; lw++) {
             int lh = area - lw;
             if (lh < 2 || lh > 6) continue;
-            if (uniform_class(lw, lh)) {
+            // plain tiles of a uniform class run the uniform-matrix passes,
+            // which need every TU of a tile to share its transform pair
+            if (rmh::residual_uniform_class(lw, lh)) {
                 std::vector<uint32_t>& c = cls[lw][lh];
                 std::stable_sort(c.begin(), c.end(), [&](uint32_t x, uint32_t y) { return pair_of(x) < pair_of(y); });
                 for (size_t k = 1; k < c.size(); k++)
@@ -1226,7 +1226,7 @@ uint32_t build_residual_tiles(const rmh_tu_desc* tus, size_t n_tu, std::vector<u
     order.insert(order.end(), rest.begin(), rest.end());
     order.insert(order.end(), prep_list.begin(), prep_list.end());
     const uint8_t none[7][7] = {};
-    return tiles_from_classes(first, count, lfnst, none, *breaks, tiles, lds);
+    return tiles_from_classes(first, count, lfnst, none, *breaks, recon, tiles, lds);
 }
 
 // Tiles a plan can need at the largest split (each class's LFNST TUs start
@@ -1236,7 +1236,7 @@ size_t tile_capacity(const uint32_t (*count)[7]) {
     for (int lw = 2; lw <= 6; lw++)
         for (int lh = 2; lh <= 6; lh++) {
             const size_t per = std::max(1, rmh::residual_tile_tus(lw, lh) >> kMaxTileSplit);
-            n += (count[lw][lh] + per - 1) / per + 1 + (uniform_class(lw, lh) ? 8 : 0);  // + transform-pair breaks
+            n += (count[lw][lh] + per - 1) / per + 1 + (rmh::residual_uniform_class(lw, lh) ? 8 : 0);  // + transform-pair breaks
         }
     return n;
 }
@@ -1551,7 +1551,7 @@ rmh_status create_residual_plan(rmh_ctx* ctx, const rmh_tu_desc* tus, size_t n_t
         std::vector<rmh::TileItem> tiles;
         Prepass prep;
         p->n_tiles_plain = build_residual_tiles(tus, n_tu, order, tiles, &p->n_fast, &p->tile_lds, p->cls_first,
-                                                p->cls_count, p->cls_lfnst, &prep, &p->tile_breaks);
+                                                p->cls_count, p->cls_lfnst, &prep, &p->tile_breaks, recon);
         p->n_prep = prep.off.size();
         p->tile_capacity = std::max(tiles.size(), tile_capacity(p->cls_count));
         size_t tb = aligned(n_tu * sizeof(rmh_tu_desc)), ob = aligned(order.size() * 4),
@@ -1689,7 +1689,8 @@ RMH_EXPORT rmh_status rmh_residual_plan_autotune(rmh_plan* plan, const int16_t* 
     auto time_plan = [&](const uint8_t (*split)[7], float* ms) -> hipError_t {
         std::vector<rmh::TileItem> tiles;
         uint32_t lds = 0;
-        const uint32_t n_plain = tiles_from_classes(plan->cls_first, plan->cls_count, plan->cls_lfnst, split, plan->tile_breaks, tiles, &lds);
+        const uint32_t n_plain = tiles_from_classes(plan->cls_first, plan->cls_count, plan->cls_lfnst, split, plan->tile_breaks,
+                                                    plan->d_place != nullptr, tiles, &lds);
         hipError_t r = hipMemcpy(plan->d_tiles, tiles.data(), tiles.size() * sizeof(rmh::TileItem), hipMemcpyHostToDevice);
         for (uint32_t k = 0; k < 2 && r == hipSuccess; k++)
             r = rmh::launch_residual_tiles(a, plan->d_tiles, (uint32_t)tiles.size(), n_plain, lds, s);
@@ -1718,7 +1719,8 @@ RMH_EXPORT rmh_status rmh_residual_plan_autotune(rmh_plan* plan, const int16_t* 
             const uint8_t keep = cur[lw][lh];
             uint8_t choice = keep;
             for (int sp = 0; sp <= kMaxTileSplit && e == hipSuccess; sp++) {
-                if (sp == keep || (rmh::residual_tile_tus(lw, lh) >> sp) < 1) continue;
+                const bool uni = plan->d_place == nullptr && rmh::residual_uniform_class(lw, lh);
+                if (sp == keep || (rmh::residual_tile_tus(lw, lh, uni) >> sp) < 1) continue;
                 cur[lw][lh] = (uint8_t)sp;
                 float ms = 0.0f;
                 e = time_plan(cur, &ms);
@@ -1734,7 +1736,8 @@ RMH_EXPORT rmh_status rmh_residual_plan_autotune(rmh_plan* plan, const int16_t* 
     if (e == hipSuccess) {
         std::vector<rmh::TileItem> tiles;
         plan->n_tiles_plain =
-            tiles_from_classes(plan->cls_first, plan->cls_count, plan->cls_lfnst, plan->split, plan->tile_breaks, tiles,
+            tiles_from_classes(plan->cls_first, plan->cls_count, plan->cls_lfnst, plan->split, plan->tile_breaks,
+                               plan->d_place != nullptr, tiles,
                                &plan->tile_lds);
         e = hipMemcpy(plan->d_tiles, tiles.data(), tiles.size() * sizeof(rmh::TileItem), hipMemcpyHostToDevice);
         plan->n_tiles = tiles.size();
@@ -1768,7 +1771,7 @@ RMH_EXPORT rmh_status rmh_residual_batch(rmh_ctx* ctx, const rmh_tu_desc* tus, s
     rmh_plan tmp;
     Prepass prep;
     tmp.n_tiles_plain = build_residual_tiles(tus, n_tu, order, tiles, &tmp.n_fast, &tmp.tile_lds, tmp.cls_first,
-                                             tmp.cls_count, tmp.cls_lfnst, &prep, &tmp.tile_breaks);
+                                             tmp.cls_count, tmp.cls_lfnst, &prep, &tmp.tile_breaks, false);
     hipError_t e = ctx->dev.reserve(aligned(tb) + 2 * aligned(cb) + aligned(order.size() * 4) +
                                     aligned(tiles.size() * sizeof(rmh::TileItem)) +
                                     aligned(tmp.n_fast * sizeof(rmh::TileTu)) + aligned(prep.off.size() * 4) +
