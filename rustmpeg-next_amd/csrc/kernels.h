@@ -13,6 +13,10 @@ struct DeviceTables {
     const uint32_t* mat = nullptr;   // transform matrices as int16 pairs: [kk][j] = (M[2kk][j], M[2kk+1][j])
     const uint32_t* col16 = nullptr; // per column j: 16 pairs (M[2kk][j], M[2kk+1][j]), zero past the last row
     int32_t col16_off[3][7];         // dword offset of (type, log2 n); -1 if unsupported
+    // DCT-II of 8, 16 and 32 points split for the even/odd butterfly, in
+    // col16: per column j, 8 even pairs (M[4q][j], M[4q + 2][j]) then 8 odd
+    // pairs (M[4q + 1][j], M[4q + 3][j]), zero past the last row; -1 if absent
+    int32_t eo_off[7];
     const int8_t* lfnst8 = nullptr;  // [4][2][16][48]
     const int8_t* lfnst4 = nullptr;  // [4][2][16][16]
     const uint16_t* scans = nullptr; // grouped diagonal scans, raster positions
@@ -55,6 +59,7 @@ struct ResidualArgs {
     const uint32_t* order;  // optional TU index list (nullptr: descriptor order)
     const uint32_t* col16;  // column-major padded matrix pairs (tiled kernel)
     int32_t col16_off[3][7];
+    int32_t eo_off[7];                 // DeviceTables::eo_off
     const rmh_tu_place* place;         // recon mode when non-null: add into frames' planes
     const rmh_recon_frame* frames;
     const TileTu* tile_tus;            // tiled kernel: per TU of order[0 .. n_fast)

@@ -643,6 +643,20 @@ __device__ __forceinline__ void load_colp(const uint32_t* __restrict__ src, uint
 // Arithmetic and clipping are those of the other passes.
 typedef const __attribute__((address_space(4))) uint32_t* MatPtr;
 
+// The even and the odd samples of a group of 4 (int16 pairs d0 = (v0, v1),
+// d1 = (v2, v3)): (v0, v2) and (v1, v3), the operands of a butterfly.
+__device__ __forceinline__ uint32_t even_pair(uint32_t d0, uint32_t d1) { return __builtin_amdgcn_perm(d1, d0, 0x05040100u); }
+__device__ __forceinline__ uint32_t odd_pair(uint32_t d0, uint32_t d1) { return __builtin_amdgcn_perm(d1, d0, 0x07060302u); }
+
+// acc + sum over the NQ groups of dot2(v[q], m[q]) (one pair per group).
+template <int NQ>
+__device__ __forceinline__ int dot_half(const uint32_t* v, const uint32_t* m, int acc) {
+    acc = dot2_first(v[0], m[0], acc);
+#pragma unroll
+    for (int q = 1; q < NQ; q++) acc = dot2(v[q], m[q], acc);
+    return acc;
+}
+
 template <int LW, int LH, int GMAX, int CP, int TP>
 __device__ __forceinline__ void uniform_passes(const ResidualArgs& a, uint32_t* C, uint32_t* T, const TuParams* P,
                                                int G, int ec, int k1, int k2) {
@@ -654,6 +668,12 @@ __device__ __forceinline__ void uniform_passes(const ResidualArgs& a, uint32_t* 
     const int tid = threadIdx.x;
     const MatPtr mat = (MatPtr)a.col16;
     const int mv = __builtin_amdgcn_readfirstlane(P[0].mv), mh = __builtin_amdgcn_readfirstlane(P[0].mh);
+    // DCT-II in a direction: its output pairs (n, N - 1 - n) come from one
+    // even and one odd half-length sum (M[k][N - 1 - n] = (-1)^k M[k][n]),
+    // which halves the products; the sums are exact, so the results equal
+    // the full products'
+    const bool bf_v = mv == a.col16_off[RMH_TR_DCT2][LH], bf_h = mh == a.col16_off[RMH_TR_DCT2][LW];
+    const int eo_v = a.eo_off[LH], eo_h = a.eo_off[LW];
 
     // phase B: lane columns j = tid CPL + c (adjacent, so pairs pack)
     if (ec > 0) {
@@ -669,23 +689,8 @@ __device__ __forceinline__ void uniform_passes(const ResidualArgs& a, uint32_t* 
             }
             __syncthreads();
             const int j0 = tid * CPL, g = j0 >> LW, x0 = j0 & (W - 1);
-#pragma unroll
-            for (int y = 0; y < H; y++) {
-                uint32_t m[2 * NQ];
-#pragma unroll
-                for (int k = 0; k < 2 * NQ; k++) m[k] = mat[mv + y * 16 + k];
-                int v[CPL];
-#pragma unroll
-                for (int c = 0; c < CPL; c++) {
-                    int acc = dot2_first(cv[c][0].x, m[0], 64);
-                    acc = dot2(cv[c][0].y, m[1], acc);
-#pragma unroll
-                    for (int q = 1; q < NQ; q++) {
-                        acc = dot2(cv[c][q].x, m[2 * q], acc);
-                        acc = dot2(cv[c][q].y, m[2 * q + 1], acc);
-                    }
-                    v[c] = acc >> 7;
-                }
+            // one intermediate per lane column: clip16((sum + 64) >> 7), stored
+            auto put = [&](int y, const int (&v)[CPL]) {
                 uint32_t* row = T + (g * H + y) * TP;
                 if constexpr (CPL == 1) {
                     reinterpret_cast<int16_t*>(row)[x0] = (int16_t)(pack_sat16(v[0], 0) & 0xffff);
@@ -693,6 +698,54 @@ __device__ __forceinline__ void uniform_passes(const ResidualArgs& a, uint32_t* 
                     row[x0 >> 1] = pack_sat16(v[0], v[1]);
                 } else {
                     *reinterpret_cast<uint2*>(row + (x0 >> 1)) = make_uint2(pack_sat16(v[0], v[1]), pack_sat16(v[2], v[3]));
+                }
+            };
+            if (bf_v) {
+                uint32_t ev[CPL][NQ], od[CPL][NQ];
+#pragma unroll
+                for (int c = 0; c < CPL; c++)
+#pragma unroll
+                    for (int q = 0; q < NQ; q++) {
+                        ev[c][q] = even_pair(cv[c][q].x, cv[c][q].y);
+                        od[c][q] = odd_pair(cv[c][q].x, cv[c][q].y);
+                    }
+#pragma unroll
+                for (int y = 0; y < H / 2; y++) {
+                    uint32_t me[NQ], mo[NQ];
+#pragma unroll
+                    for (int q = 0; q < NQ; q++) {
+                        me[q] = mat[eo_v + y * 16 + q];
+                        mo[q] = mat[eo_v + y * 16 + 8 + q];
+                    }
+                    int lo[CPL], hi[CPL];
+#pragma unroll
+                    for (int c = 0; c < CPL; c++) {
+                        const int e = dot_half<NQ>(ev[c], me, 64), o = dot_half<NQ>(od[c], mo, 0);
+                        lo[c] = (e + o) >> 7;
+                        hi[c] = (e - o) >> 7;
+                    }
+                    put(y, lo);
+                    put(H - 1 - y, hi);
+                }
+            } else {
+#pragma unroll
+                for (int y = 0; y < H; y++) {
+                    uint32_t m[2 * NQ];
+#pragma unroll
+                    for (int k = 0; k < 2 * NQ; k++) m[k] = mat[mv + y * 16 + k];
+                    int v[CPL];
+#pragma unroll
+                    for (int c = 0; c < CPL; c++) {
+                        int acc = dot2_first(cv[c][0].x, m[0], 64);
+                        acc = dot2(cv[c][0].y, m[1], acc);
+#pragma unroll
+                        for (int q = 1; q < NQ; q++) {
+                            acc = dot2(cv[c][q].x, m[2 * q], acc);
+                            acc = dot2(cv[c][q].y, m[2 * q + 1], acc);
+                        }
+                        v[c] = acc >> 7;
+                    }
+                    put(y, v);
                 }
             }
         };
@@ -703,7 +756,8 @@ __device__ __forceinline__ void uniform_passes(const ResidualArgs& a, uint32_t* 
     // phase C: lane rows r = tid RPL + c
     auto hpass = [&](auto nq) {
         constexpr int NQ = decltype(nq)::value;
-        uint2 tv[RPL][NQ > 0 ? NQ : 1];
+        constexpr int NQ1 = NQ > 0 ? NQ : 1;
+        uint2 tv[RPL][NQ1];
         int sh[RPL], rnd[RPL];
 #pragma unroll
         for (int c = 0; c < RPL; c++) {
@@ -715,31 +769,61 @@ __device__ __forceinline__ void uniform_passes(const ResidualArgs& a, uint32_t* 
             rnd[c] = 1 << (sh[c] - 1);
         }
         uint32_t o[RPL][W / 2];
+        if (NQ > 0 && bf_h) {
+            uint32_t ev[RPL][NQ1], od[RPL][NQ1];
 #pragma unroll
-        for (int x = 0; x < W; x += 2) {
-            uint32_t m0[NQ > 0 ? 2 * NQ : 1], m1[NQ > 0 ? 2 * NQ : 1];
+            for (int c = 0; c < RPL; c++)
 #pragma unroll
-            for (int k = 0; k < 2 * NQ; k++) {
-                m0[k] = mat[mh + x * 16 + k];
-                m1[k] = mat[mh + (x + 1) * 16 + k];
-            }
-#pragma unroll
-            for (int c = 0; c < RPL; c++) {
-                int a0 = rnd[c], a1 = rnd[c];
-                if constexpr (NQ > 0) {
-                    a0 = dot2_first(tv[c][0].x, m0[0], rnd[c]);
-                    a1 = dot2_first(tv[c][0].x, m1[0], rnd[c]);
-                    a0 = dot2(tv[c][0].y, m0[1], a0);
-                    a1 = dot2(tv[c][0].y, m1[1], a1);
-#pragma unroll
-                    for (int q = 1; q < NQ; q++) {
-                        a0 = dot2(tv[c][q].x, m0[2 * q], a0);
-                        a1 = dot2(tv[c][q].x, m1[2 * q], a1);
-                        a0 = dot2(tv[c][q].y, m0[2 * q + 1], a0);
-                        a1 = dot2(tv[c][q].y, m1[2 * q + 1], a1);
-                    }
+                for (int q = 0; q < NQ; q++) {
+                    ev[c][q] = even_pair(tv[c][q].x, tv[c][q].y);
+                    od[c][q] = odd_pair(tv[c][q].x, tv[c][q].y);
                 }
-                o[c][x >> 1] = pack_sat16(a0 >> sh[c], a1 >> sh[c]);
+            // columns x, x + 1 and their mirrors W - 1 - x, W - 2 - x
+#pragma unroll
+            for (int x = 0; x < W / 2; x += 2) {
+                uint32_t me0[NQ1], mo0[NQ1], me1[NQ1], mo1[NQ1];
+#pragma unroll
+                for (int q = 0; q < NQ; q++) {
+                    me0[q] = mat[eo_h + x * 16 + q];
+                    mo0[q] = mat[eo_h + x * 16 + 8 + q];
+                    me1[q] = mat[eo_h + (x + 1) * 16 + q];
+                    mo1[q] = mat[eo_h + (x + 1) * 16 + 8 + q];
+                }
+#pragma unroll
+                for (int c = 0; c < RPL; c++) {
+                    const int e0 = dot_half<NQ1>(ev[c], me0, rnd[c]), o0 = dot_half<NQ1>(od[c], mo0, 0);
+                    const int e1 = dot_half<NQ1>(ev[c], me1, rnd[c]), o1 = dot_half<NQ1>(od[c], mo1, 0);
+                    o[c][x >> 1] = pack_sat16((e0 + o0) >> sh[c], (e1 + o1) >> sh[c]);
+                    o[c][(W - 2 - x) >> 1] = pack_sat16((e1 - o1) >> sh[c], (e0 - o0) >> sh[c]);
+                }
+            }
+        } else {
+#pragma unroll
+            for (int x = 0; x < W; x += 2) {
+                uint32_t m0[2 * NQ1], m1[2 * NQ1];
+#pragma unroll
+                for (int k = 0; k < 2 * NQ; k++) {
+                    m0[k] = mat[mh + x * 16 + k];
+                    m1[k] = mat[mh + (x + 1) * 16 + k];
+                }
+#pragma unroll
+                for (int c = 0; c < RPL; c++) {
+                    int a0 = rnd[c], a1 = rnd[c];
+                    if constexpr (NQ > 0) {
+                        a0 = dot2_first(tv[c][0].x, m0[0], rnd[c]);
+                        a1 = dot2_first(tv[c][0].x, m1[0], rnd[c]);
+                        a0 = dot2(tv[c][0].y, m0[1], a0);
+                        a1 = dot2(tv[c][0].y, m1[1], a1);
+#pragma unroll
+                        for (int q = 1; q < NQ; q++) {
+                            a0 = dot2(tv[c][q].x, m0[2 * q], a0);
+                            a1 = dot2(tv[c][q].x, m1[2 * q], a1);
+                            a0 = dot2(tv[c][q].y, m0[2 * q + 1], a0);
+                            a1 = dot2(tv[c][q].y, m1[2 * q + 1], a1);
+                        }
+                    }
+                    o[c][x >> 1] = pack_sat16(a0 >> sh[c], a1 >> sh[c]);
+                }
             }
         }
         // each row is W contiguous int16 residuals; TU offsets are multiples

@@ -420,6 +420,26 @@ hipError_t build_tables(rmh_ctx* ctx) {
                     col16.push_back(lo | (hi << 16));
                 }
         }
+    // the even/odd split of DCT-II, for the butterflies of the uniform
+    // residual passes (their output pairs j, n - 1 - j use the symmetry
+    // M[k][n - 1 - j] = (-1)^k M[k][j] of these matrices)
+    for (int l = 0; l < 7; l++) {
+        const int n = 1 << l;
+        ctx->tabs.eo_off[l] = -1;
+        if (l < 3 || l > 5) continue;
+        ctx->tabs.eo_off[l] = (int32_t)col16.size();
+        bool ok;
+        for (int j = 0; j < n; j++)
+            for (int odd = 0; odd < 2; odd++)
+                for (int q = 0; q < 8; q++) {
+                    uint32_t lo = 0, hi = 0;
+                    if (4 * q + 3 < n) {
+                        lo = (uint16_t)(int16_t)rmh::transform_entry(RMH_TR_DCT2, n, 4 * q + odd, j, &ok);
+                        hi = (uint16_t)(int16_t)rmh::transform_entry(RMH_TR_DCT2, n, 4 * q + 2 + odd, j, &ok);
+                    }
+                    col16.push_back(lo | (hi << 16));
+                }
+    }
     std::vector<uint16_t> scans;
     for (int a = 0; a < 7; a++)
         for (int b = 0; b < 7; b++) {
@@ -1334,6 +1354,7 @@ rmh_status run_plan_locked_free(rmh_plan* plan, const int16_t* d_coeff, int16_t*
         memcpy(a.scan_len, plan->ctx->tabs.scan_len, sizeof a.scan_len);
         a.col16 = plan->ctx->tabs.col16;
         memcpy(a.col16_off, plan->ctx->tabs.col16_off, sizeof a.col16_off);
+        memcpy(a.eo_off, plan->ctx->tabs.eo_off, sizeof a.eo_off);
         a.place = plan->d_place;
         a.frames = plan->d_rframes;
         a.tile_tus = plan->d_tile_tus;
@@ -1662,6 +1683,7 @@ RMH_EXPORT rmh_status rmh_residual_plan_autotune(rmh_plan* plan, const int16_t* 
     memcpy(a.scan_len, ctx->tabs.scan_len, sizeof a.scan_len);
     a.col16 = ctx->tabs.col16;
     memcpy(a.col16_off, ctx->tabs.col16_off, sizeof a.col16_off);
+    memcpy(a.eo_off, ctx->tabs.eo_off, sizeof a.eo_off);
     a.place = plan->d_place;
     a.frames = plan->d_rframes;
     a.tile_tus = plan->d_tile_tus;
