@@ -648,6 +648,12 @@ typedef const __attribute__((address_space(4))) uint32_t* MatPtr;
 __device__ __forceinline__ uint32_t even_pair(uint32_t d0, uint32_t d1) { return __builtin_amdgcn_perm(d1, d0, 0x05040100u); }
 __device__ __forceinline__ uint32_t odd_pair(uint32_t d0, uint32_t d1) { return __builtin_amdgcn_perm(d1, d0, 0x07060302u); }
 
+constexpr int cmin(int a, int b) { return a < b ? a : b; }
+constexpr int cmax(int a, int b) { return a > b ? a : b; }
+// The largest power of two <= v (v >= 1): block sizes that divide the
+// power-of-two loop ranges they step through.
+constexpr int pow2_floor(int v) { return v >= 2 ? 2 * pow2_floor(v / 2) : 1; }
+
 // acc + sum over the NQ groups of dot2(v[q], m[q]) (one pair per group).
 template <int NQ>
 __device__ __forceinline__ int dot_half(const uint32_t* v, const uint32_t* m, int acc) {
@@ -709,43 +715,59 @@ __device__ __forceinline__ void uniform_passes(const ResidualArgs& a, uint32_t* 
                         ev[c][q] = even_pair(cv[c][q].x, cv[c][q].y);
                         od[c][q] = odd_pair(cv[c][q].x, cv[c][q].y);
                     }
+                // matrix columns in blocks of about 32 SGPRs: one wait per
+                // block on the scalar loads instead of one per column
+                constexpr int YB = cmin(H / 2, pow2_floor(16 / NQ));
+                static_assert((H / 2) % YB == 0, "blocks tile the rows");
 #pragma unroll
-                for (int y = 0; y < H / 2; y++) {
-                    uint32_t me[NQ], mo[NQ];
+                for (int y0 = 0; y0 < H / 2; y0 += YB) {
+                    uint32_t me[YB][NQ], mo[YB][NQ];
 #pragma unroll
-                    for (int q = 0; q < NQ; q++) {
-                        me[q] = mat[eo_v + y * 16 + q];
-                        mo[q] = mat[eo_v + y * 16 + 8 + q];
+                    for (int b = 0; b < YB; b++)
+#pragma unroll
+                        for (int q = 0; q < NQ; q++) {
+                            me[b][q] = mat[eo_v + (y0 + b) * 16 + q];
+                            mo[b][q] = mat[eo_v + (y0 + b) * 16 + 8 + q];
+                        }
+#pragma unroll
+                    for (int b = 0; b < YB; b++) {
+                        int lo[CPL], hi[CPL];
+#pragma unroll
+                        for (int c = 0; c < CPL; c++) {
+                            const int e = dot_half<NQ>(ev[c], me[b], 64), o = dot_half<NQ>(od[c], mo[b], 0);
+                            lo[c] = (e + o) >> 7;
+                            hi[c] = (e - o) >> 7;
+                        }
+                        put(y0 + b, lo);
+                        put(H - 1 - y0 - b, hi);
                     }
-                    int lo[CPL], hi[CPL];
-#pragma unroll
-                    for (int c = 0; c < CPL; c++) {
-                        const int e = dot_half<NQ>(ev[c], me, 64), o = dot_half<NQ>(od[c], mo, 0);
-                        lo[c] = (e + o) >> 7;
-                        hi[c] = (e - o) >> 7;
-                    }
-                    put(y, lo);
-                    put(H - 1 - y, hi);
                 }
             } else {
+                constexpr int YB = cmin(H, pow2_floor(16 / NQ));
+                static_assert(H % YB == 0, "blocks tile the rows");
 #pragma unroll
-                for (int y = 0; y < H; y++) {
-                    uint32_t m[2 * NQ];
+                for (int y0 = 0; y0 < H; y0 += YB) {
+                    uint32_t m[YB][2 * NQ];
 #pragma unroll
-                    for (int k = 0; k < 2 * NQ; k++) m[k] = mat[mv + y * 16 + k];
-                    int v[CPL];
+                    for (int b = 0; b < YB; b++)
 #pragma unroll
-                    for (int c = 0; c < CPL; c++) {
-                        int acc = dot2_first(cv[c][0].x, m[0], 64);
-                        acc = dot2(cv[c][0].y, m[1], acc);
+                        for (int k = 0; k < 2 * NQ; k++) m[b][k] = mat[mv + (y0 + b) * 16 + k];
 #pragma unroll
-                        for (int q = 1; q < NQ; q++) {
-                            acc = dot2(cv[c][q].x, m[2 * q], acc);
-                            acc = dot2(cv[c][q].y, m[2 * q + 1], acc);
+                    for (int b = 0; b < YB; b++) {
+                        int v[CPL];
+#pragma unroll
+                        for (int c = 0; c < CPL; c++) {
+                            int acc = dot2_first(cv[c][0].x, m[b][0], 64);
+                            acc = dot2(cv[c][0].y, m[b][1], acc);
+#pragma unroll
+                            for (int q = 1; q < NQ; q++) {
+                                acc = dot2(cv[c][q].x, m[b][2 * q], acc);
+                                acc = dot2(cv[c][q].y, m[b][2 * q + 1], acc);
+                            }
+                            v[c] = acc >> 7;
                         }
-                        v[c] = acc >> 7;
+                        put(y0 + b, v);
                     }
-                    put(y, v);
                 }
             }
         };
@@ -779,33 +801,45 @@ __device__ __forceinline__ void uniform_passes(const ResidualArgs& a, uint32_t* 
                     od[c][q] = odd_pair(tv[c][q].x, tv[c][q].y);
                 }
             // columns x, x + 1 and their mirrors W - 1 - x, W - 2 - x
+            constexpr int XB = cmin(W / 2, cmax(2, pow2_floor(16 / NQ1)));  // columns per block of scalar loads (even)
+            static_assert((W / 2) % XB == 0 && XB % 2 == 0, "blocks tile the half row in pairs");
 #pragma unroll
-            for (int x = 0; x < W / 2; x += 2) {
-                uint32_t me0[NQ1], mo0[NQ1], me1[NQ1], mo1[NQ1];
+            for (int x0 = 0; x0 < W / 2; x0 += XB) {
+                uint32_t me[XB][NQ1], mo[XB][NQ1];
 #pragma unroll
-                for (int q = 0; q < NQ; q++) {
-                    me0[q] = mat[eo_h + x * 16 + q];
-                    mo0[q] = mat[eo_h + x * 16 + 8 + q];
-                    me1[q] = mat[eo_h + (x + 1) * 16 + q];
-                    mo1[q] = mat[eo_h + (x + 1) * 16 + 8 + q];
-                }
+                for (int b = 0; b < XB; b++)
 #pragma unroll
-                for (int c = 0; c < RPL; c++) {
-                    const int e0 = dot_half<NQ1>(ev[c], me0, rnd[c]), o0 = dot_half<NQ1>(od[c], mo0, 0);
-                    const int e1 = dot_half<NQ1>(ev[c], me1, rnd[c]), o1 = dot_half<NQ1>(od[c], mo1, 0);
-                    o[c][x >> 1] = pack_sat16((e0 + o0) >> sh[c], (e1 + o1) >> sh[c]);
-                    o[c][(W - 2 - x) >> 1] = pack_sat16((e1 - o1) >> sh[c], (e0 - o0) >> sh[c]);
+                    for (int q = 0; q < NQ; q++) {
+                        me[b][q] = mat[eo_h + (x0 + b) * 16 + q];
+                        mo[b][q] = mat[eo_h + (x0 + b) * 16 + 8 + q];
+                    }
+#pragma unroll
+                for (int b = 0; b < XB; b += 2) {
+                    const int x = x0 + b;
+#pragma unroll
+                    for (int c = 0; c < RPL; c++) {
+                        const int e0 = dot_half<NQ1>(ev[c], me[b], rnd[c]), o0 = dot_half<NQ1>(od[c], mo[b], 0);
+                        const int e1 = dot_half<NQ1>(ev[c], me[b + 1], rnd[c]), o1 = dot_half<NQ1>(od[c], mo[b + 1], 0);
+                        o[c][x >> 1] = pack_sat16((e0 + o0) >> sh[c], (e1 + o1) >> sh[c]);
+                        o[c][(W - 2 - x) >> 1] = pack_sat16((e1 - o1) >> sh[c], (e0 - o0) >> sh[c]);
+                    }
                 }
             }
         } else {
+            constexpr int XB = cmin(W, cmax(2, pow2_floor(16 / NQ1)));  // columns per block of scalar loads (even)
+            static_assert(W % XB == 0 && XB % 2 == 0, "blocks tile the row in pairs");
 #pragma unroll
-            for (int x = 0; x < W; x += 2) {
-                uint32_t m0[2 * NQ1], m1[2 * NQ1];
+            for (int x0 = 0; x0 < W; x0 += XB) {
+              uint32_t mb[XB][2 * NQ1];
 #pragma unroll
-                for (int k = 0; k < 2 * NQ; k++) {
-                    m0[k] = mat[mh + x * 16 + k];
-                    m1[k] = mat[mh + (x + 1) * 16 + k];
-                }
+              for (int b = 0; b < XB; b++)
+#pragma unroll
+                  for (int k = 0; k < 2 * NQ; k++) mb[b][k] = mat[mh + (x0 + b) * 16 + k];
+#pragma unroll
+              for (int b = 0; b < XB; b += 2) {
+                const int x = x0 + b;
+                const uint32_t* m0 = mb[b];
+                const uint32_t* m1 = mb[b + 1];
 #pragma unroll
                 for (int c = 0; c < RPL; c++) {
                     int a0 = rnd[c], a1 = rnd[c];
@@ -824,6 +858,7 @@ __device__ __forceinline__ void uniform_passes(const ResidualArgs& a, uint32_t* 
                     }
                     o[c][x >> 1] = pack_sat16(a0 >> sh[c], a1 >> sh[c]);
                 }
+              }
             }
         }
         // each row is W contiguous int16 residuals; TU offsets are multiples
