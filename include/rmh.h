@@ -719,7 +719,8 @@ enum {
 };
 
 typedef struct rmh_rd_stats {
-    double bits;          /* modelled bits of the chosen splits */
+    double bits;          /* bits of the chosen splits: modelled (rmh_rd_encode_luma) or the
+                             coded stream's length (rmh_rd_encode_luma_stream) */
     uint64_t sse;         /* luma SSE of the reconstruction */
     uint32_t blocks;      /* 32x32 nodes */
     uint32_t candidates;  /* splits coded */
@@ -741,6 +742,44 @@ rmh_status rmh_rd_encode_luma(const uint8_t* src, uint32_t width, uint32_t heigh
                               uint32_t qp, uint32_t mode, const float* probs, float p_prune,
                               float p_fallback, uint8_t* recon, uint8_t* splits, float* costs,
                               rmh_rd_stats* stats);
+
+/* residual_coding of n_blocks TUs of width x height (powers of two, 1..64;
+ * levels in raster order, w*h per TU, each TU with a non-zero level and
+ * nothing outside its top-left 32x32) into one CABAC substream, each TU
+ * followed by end_of_slice_segment_flag (1 after the last), contexts
+ * initialised for an I slice at qp. Replaces encode_residual_with_options
+ * (vvc/residual.rs:448-522) over BinEncoder (vvc/cabac.rs:477-745).
+ * RMH_INVALID_INPUT when the stream exceeds stream_capacity (*stream_len then
+ * holds the size needed). Host code. */
+enum {
+    RMH_CABAC_LUMA = 1,         /* luma contexts (else chroma) */
+    RMH_CABAC_DEP_QUANT = 2,    /* dependent-quantisation state selects the sig contexts */
+    RMH_CABAC_SIGN_HIDING = 4   /* sign data hiding (ignored with RMH_CABAC_DEP_QUANT) */
+};
+rmh_status rmh_cabac_residual_stream(const int32_t* levels, uint32_t width, uint32_t height, uint32_t n_blocks,
+                                     uint32_t flags, uint32_t qp, uint8_t* stream, size_t stream_capacity,
+                                     size_t* stream_len);
+
+/* The same search with CABAC as the rate: every candidate is priced by the
+ * bits it adds to the picture's CABAC substream (context state carried in
+ * coding order), and the chosen candidates are written to `stream`. The
+ * stream is a luma-only (4:0:0) I-slice substream with 32x32 CTUs: per CTU the
+ * coding-tree split (QT to 16, one binary/ternary level, CUs down to 4), per
+ * leaf CU the PLANAR/DC mode as MPM 0/1, the luma cbf and residual_coding,
+ * then end_of_slice_segment_flag; contexts initialised for an I slice at qp.
+ * Syntax and arithmetic follow the reference's encoder: BinEncoder
+ * (vvc/cabac.rs:477-745), encode_residual (vvc/residual.rs:435-743),
+ * encode_coding_tree_split (vvc/slice.rs:3850-3893), the MPM luma mode coding
+ * (vvc/slice.rs:1656-1687) and the luma cbf (vvc/slice.rs:399). Reconstruction
+ * uses this search's PLANAR/DC predictor above (no reference-sample filter or
+ * PDPC), so the stream is decodable with that predictor. stats->bits is the
+ * stream length in bits. RMH_INVALID_INPUT when the stream exceeds
+ * stream_capacity (*stream_len then holds the size needed). */
+rmh_status rmh_rd_encode_luma_stream(const uint8_t* src, uint32_t width, uint32_t height, uint32_t stride,
+                                     uint32_t qp, uint32_t mode, const float* probs, float p_prune,
+                                     float p_fallback, uint8_t* recon, uint8_t* splits, float* costs,
+                                     rmh_rd_stats* stats, uint8_t* stream, size_t stream_capacity,
+                                     size_t* stream_len);
 
 /* ------------------------------------------------------------------------ */
 /* CU-split predictor: BF16 CNN on MFMA (no reference counterpart: the      */

@@ -42,6 +42,13 @@ def _setup(L):
     L.orc_lfnst_transposes.argtypes = [c_int]
     L.orc_lfnst_coeff.argtypes = [c_int] * 5
     L.orc_coeff_scan.argtypes = [c_int, c_int, I32P]
+    L.orc_cabac_decode_residuals.argtypes = [ctypes.c_char_p, ctypes.c_size_t, c_int, c_int, c_int, c_int, c_int,
+                                             c_int, c_int, I32P]
+    L.orc_cabac_decode_bins.argtypes = [ctypes.c_char_p, ctypes.c_size_t, c_int, c_int, c_int, c_int, c_int,
+                                        ctypes.POINTER(ctypes.c_uint8)]
+    L.orc_decode_luma_stream.argtypes = [ctypes.c_char_p, ctypes.c_size_t, c_int, c_int, c_int,
+                                         ctypes.POINTER(ctypes.c_uint8), ctypes.POINTER(ctypes.c_uint8)]
+    L.orc_cabac_ctx_offset.argtypes = [c_int]
     L.orc_dequant_rect.argtypes = [c_int, c_int, I32P, c_int, c_int, I32P]
     L.orc_dequant_dep_quant.argtypes = [c_int, c_int, I32P, c_int, c_int, I32P]
     L.orc_dequant_ts.argtypes = [ctypes.c_int32, c_int]
@@ -523,3 +530,38 @@ def forward_batch(tus, residual):
     if bad:
         raise ValueError("TU %d rejected" % (bad - 1))
     return out
+
+
+# ---- CABAC decoding side (orc_cabac.c) ----
+
+def cabac_decode_residuals(data, w, h, n_blocks, qp=26, luma=True, dep_quant=False, sign_hiding=False):
+    """Levels [n_blocks, h, w] read back from a residual substream, or None
+    when the stream is malformed."""
+    out = np.zeros(n_blocks * w * h, np.int32)
+    r = lib().orc_cabac_decode_residuals(bytes(data), len(data), w, h, n_blocks, int(luma), int(dep_quant),
+                                         int(sign_hiding), qp, _p(out, I32P))
+    return None if r != 0 else out.reshape(n_blocks, h, w)
+
+
+def cabac_decode_bins(data, n_ctx, n_ep, qp=26, init_id=25, ws=13):
+    """n_ctx context-coded bins (one model) then n_ep bypass bins, then the
+    terminating bin; None when the stream does not end as it should."""
+    out = np.zeros(n_ctx + n_ep, np.uint8)
+    r = lib().orc_cabac_decode_bins(bytes(data), len(data), qp, init_id, ws, n_ctx, n_ep,
+                                    out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)))
+    return None if r != 0 else out
+
+
+def decode_luma_stream(data, w, h, qp):
+    """(recon[h, w], splits[h/32, w/32]) of an rmh_rd_encode_luma_stream
+    substream, or None when it is malformed."""
+    rec = np.zeros((h, w), np.uint8)
+    sp = np.zeros((h // 32) * (w // 32), np.uint8)
+    r = lib().orc_decode_luma_stream(bytes(data), len(data), w, h, qp,
+                                     rec.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+                                     sp.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)))
+    return None if r != 0 else (rec, sp.reshape(h // 32, w // 32))
+
+
+def cabac_ctx_offset(i):
+    return lib().orc_cabac_ctx_offset(i)

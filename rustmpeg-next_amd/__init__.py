@@ -77,6 +77,7 @@ EXPORTED = [
     "rmh_lmcs_plan_create", "rmh_lmcs_plan_run", "rmh_alf_filter_bank", "rmh_alf_plan_create", "rmh_alf_plan_run",
     "rmh_forward_plan_create", "rmh_forward_plan_run", "rmh_forward_plan_evaluate",
     "rmh_gpm_plan_create", "rmh_gpm_plan_run", "rmh_mc_plan_autotune",
+    "rmh_rd_encode_luma_stream", "rmh_cabac_residual_stream",
 ]
 
 
@@ -141,6 +142,12 @@ def load_library(path=LIB_PATH):
         "rmh_rd_encode_luma": ([vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
                                 ctypes.c_uint32, vp, ctypes.c_float, ctypes.c_float, vp, vp, vp,
                                 ctypes.POINTER(abi.RdStats)], st),
+        "rmh_rd_encode_luma_stream": ([vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                       ctypes.c_uint32, vp, ctypes.c_float, ctypes.c_float, vp, vp, vp,
+                                       ctypes.POINTER(abi.RdStats), vp, ctypes.c_size_t,
+                                       ctypes.POINTER(ctypes.c_size_t)], st),
+        "rmh_cabac_residual_stream": ([vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                       ctypes.c_uint32, vp, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)], st),
         "rmh_cnn_param_count": ([], sz),
         "rmh_cnn_create": ([vp, ctypes.c_uint32, ctypes.c_uint64, ctypes.POINTER(vp)], st),
         "rmh_cnn_destroy": ([vp], None),
@@ -159,6 +166,61 @@ def load_library(path=LIB_PATH):
         fn.restype = res
     _lib = L
     return L
+
+
+def _probs_arg(probs, nb):
+    if probs is None:
+        return None
+    pp = np.ascontiguousarray(probs, dtype=np.float32).reshape(-1)
+    if pp.size != nb * 6:
+        raise InvalidInput("probs must hold 6 values per 32x32 node")
+    return pp
+
+
+def rd_encode_luma_stream(src, qp, mode=abi.RD_FULL, probs=None, p_prune=0.05, p_fallback=0.6, want_costs=False):
+    """The RD search with CABAC as the rate (rmh_rd_encode_luma_stream; host
+    code). Returns (recon, splits[nodes_y, nodes_x], costs or None, stats,
+    stream bytes); stats["bits"] is the stream's length in bits."""
+    src = np.ascontiguousarray(src, dtype=np.uint8)
+    h, w = src.shape
+    nb = (h // 32) * (w // 32)
+    recon = np.zeros_like(src)
+    splits = np.zeros(nb, np.uint8)
+    costs = np.zeros(nb * 6, np.float32) if want_costs else None
+    pp = _probs_arg(probs, nb)
+    st = abi.RdStats()
+    cap = max(4096, w * h * 2)
+    buf = np.zeros(cap, np.uint8)
+    n = ctypes.c_size_t(0)
+    s = load_library().rmh_rd_encode_luma_stream(
+        src.ctypes.data, w, h, w, qp, mode, pp.ctypes.data if pp is not None else None, p_prune, p_fallback,
+        recon.ctypes.data, splits.ctypes.data, costs.ctypes.data if costs is not None else None, ctypes.byref(st),
+        buf.ctypes.data, cap, ctypes.byref(n))
+    if s != abi.RMH_OK:
+        raise _ERRORS.get(s, Execution)("rmh_rd_encode_luma_stream rejected the input")
+    stats = {"bits": st.bits, "sse": int(st.sse), "blocks": st.blocks, "candidates": st.candidates,
+             "fallbacks": st.fallbacks}
+    return (recon, splits.reshape(h // 32, w // 32), (costs.reshape(nb, 6) if costs is not None else None), stats,
+            buf[:n.value].tobytes())
+
+
+def cabac_residual_stream(levels, qp=26, luma=True, dep_quant=False, sign_hiding=False):
+    """residual_coding of a stack of TUs levels[n, h, w] into one CABAC
+    substream (rmh_cabac_residual_stream; host code). Returns the bytes."""
+    lv = np.ascontiguousarray(levels, dtype=np.int32)
+    if lv.ndim == 2:
+        lv = lv[None]
+    n_blocks, h, w = lv.shape
+    flags = (abi.CABAC_LUMA if luma else 0) | (abi.CABAC_DEP_QUANT if dep_quant else 0) | \
+        (abi.CABAC_SIGN_HIDING if sign_hiding else 0)
+    n = ctypes.c_size_t(0)
+    cap = 64 + lv.size * 8
+    buf = np.zeros(cap, np.uint8)
+    s = load_library().rmh_cabac_residual_stream(lv.ctypes.data, w, h, n_blocks, flags, qp, buf.ctypes.data, cap,
+                                                 ctypes.byref(n))
+    if s != abi.RMH_OK:
+        raise _ERRORS.get(s, Execution)("rmh_cabac_residual_stream rejected the input")
+    return buf[:n.value].tobytes()
 
 
 def rd_encode_luma(src, qp, mode=abi.RD_FULL, probs=None, p_prune=0.05, p_fallback=0.6, want_costs=False):

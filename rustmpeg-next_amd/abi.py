@@ -79,6 +79,7 @@ class RdStats(ctypes.Structure):
 
 
 RD_FULL, RD_PRUNED, RD_HEURISTIC = 0, 1, 2
+CABAC_LUMA, CABAC_DEP_QUANT, CABAC_SIGN_HIDING = 1, 2, 4
 SPLIT_NAMES = ("none", "quad", "binary_hor", "binary_ver", "ternary_hor", "ternary_ver")
 
 

@@ -7,7 +7,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "librmh.so")
-SOURCES = ["rmh.cpp", "intra_subset.cpp", "rdsearch.cpp", "loopfilter.cpp", "residual.hip", "mc.hip", "deblock.hip", "sao.hip",
+SOURCES = ["rmh.cpp", "intra_subset.cpp", "rdsearch.cpp", "cabac.cpp", "loopfilter.cpp", "residual.hip", "mc.hip", "deblock.hip", "sao.hip",
            "cnn.hip", "loopfilter.hip", "forward.hip", "mc_refine.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

@@ -27,9 +27,11 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <vector>
 
 #include "../../include/rmh.h"
+#include "cabac.h"
 #include "tables.h"
 
 namespace {
@@ -195,6 +197,20 @@ struct Frame {
 struct Node {
     uint8_t rec[kNode * kNode];
     uint8_t done[kNode * kNode];
+    uint8_t cu_w[64], cu_h[64], cu_qt[64];  // per 4x4 cell of the node: the CU covering it (0: not coded)
+};
+
+// Coding-tree limits of the luma stream: CTU = node = 32, QT down to 16, one
+// MTT level of binary or ternary splits, CUs down to 4.
+constexpr rmh::cabac::SplitLimits kLimits{8, 32, 32, 1, 4};
+
+// CABAC state of the stream and the picture's CU map (per 4x4 cell) that the
+// split contexts read across node boundaries.
+struct Syntax {
+    rmh::cabac::Encoder enc;
+    rmh::cabac::Contexts ctx;
+    std::vector<uint8_t> cu_w, cu_h, cu_qt;
+    int cells_w = 0;
 };
 
 // Reference sample at picture position (x, y) for a leaf of the node at
@@ -267,13 +283,66 @@ void predict(const Frame& f, const Node& nd, int bx, int by, int x0, int y0, int
     }
 }
 
+// CU covering picture sample (x, y) as seen from the node at (bx, by): inside
+// the node from the candidate's own cells, outside from the picture map where
+// that sample precedes the node in raster order of nodes. False when none.
+bool cu_at(const Frame& f, const Syntax& sx, const Node& nd, int bx, int by, int x, int y, int* w, int* h, int* qt) {
+    if (x < 0 || y < 0 || x >= f.W || y >= f.H) return false;
+    if (x >= bx && x < bx + kNode && y >= by && y < by + kNode) {
+        const int c = ((y - by) >> 2) * 8 + ((x - bx) >> 2);
+        if (!nd.cu_w[c]) return false;
+        *w = nd.cu_w[c];
+        *h = nd.cu_h[c];
+        *qt = nd.cu_qt[c];
+        return true;
+    }
+    if (!(y < by || (y < by + kNode && x < bx))) return false;
+    const size_t c = (size_t)(y >> 2) * sx.cells_w + (x >> 2);
+    *w = sx.cu_w[c];
+    *h = sx.cu_h[c];
+    *qt = sx.cu_qt[c];
+    return true;
+}
+
+// coding_tree split syntax of a node or leaf at (x, y), contexts from its left
+// and above neighbours (derive_split_contexts, vvc/transform.rs:624-715).
+void code_split(Syntax& sx, const Frame& f, const Node& nd, int bx, int by, int x, int y, int w, int h, int qt_depth,
+                int mtt_depth, int last_split, int split) {
+    namespace cb = rmh::cabac;
+    const cb::CanSplit can = cb::can_split(x, y, w, h, f.W, f.H, mtt_depth, kLimits, last_split);
+    int lw = 0, lh = 0, lq = 0, aw = 0, ah = 0, aq = 0;
+    if (!cu_at(f, sx, nd, bx, by, x - 1, y, &lw, &lh, &lq)) lh = 0;
+    if (!cu_at(f, sx, nd, bx, by, x, y - 1, &aw, &ah, &aq)) aw = 0;
+    const cb::SplitCtx sc = cb::split_contexts(w, h, qt_depth, mtt_depth, lh, lq, aw, aq, can);
+    cb::encode_split(sx.enc, sx.ctx, split, can, sc);
+}
+
+// The syntax of one leaf CU after its split flag: the luma mode (PLANAR is
+// MPM 0; DC is MPM 1 whenever the neighbours are PLANAR or DC, which is every
+// CU here), the luma cbf and residual_coding (vvc/slice.rs:1656-1687, 399).
+void code_leaf(Syntax& sx, int mode, int w, int h, const int* lv) {
+    namespace cb = rmh::cabac;
+    sx.enc.bin(1, sx.ctx.m[cb::INTRA_LUMA_MPM_FLAG]);
+    sx.enc.bin(mode == 1, sx.ctx.m[cb::INTRA_LUMA_PLANAR_FLAG + 1]);
+    if (mode == 1) sx.enc.unary_max(0, 4);
+    bool cbf = false;
+    for (int i = 0; i < w * h && !cbf; i++) cbf = lv[i] != 0;
+    sx.enc.bin(cbf, sx.ctx.m[cb::QT_CBF_Y]);
+    if (cbf) cb::encode_residual(sx.enc, sx.ctx, w, h, lv, true, false, false);
+}
+
 // Codes one candidate split of the node at (bx, by); returns J and fills nd.
+// With sx the rate is the CABAC bits the candidate adds to sx's stream (and
+// sx advances); without it, the static bit model.
 double code_candidate(const Frame& f, int bx, int by, int split, int qp, double lambda, Node& nd, double* bits_out,
-                      uint64_t* sse_out) {
+                      uint64_t* sse_out, Syntax* sx) {
     std::memset(nd.done, 0, sizeof nd.done);
+    std::memset(nd.cu_w, 0, sizeof nd.cu_w);
     int ch[4][4];
     const int n = children(split, ch);
-    double bits = kSplitBits[split];
+    const uint64_t bits0 = sx ? sx->enc.bits() : 0;
+    if (sx) code_split(*sx, f, nd, bx, by, bx, by, kNode, kNode, 0, 0, -1, split);
+    double bits = sx ? 0.0 : kSplitBits[split];
     uint64_t sse = 0;
     int pred[2][32 * 32], res[32 * 32], coef[32 * 32], lv[32 * 32], src[32 * 32];
     for (int c = 0; c < n; c++) {
@@ -292,7 +361,19 @@ double code_candidate(const Frame& f, int bx, int by, int split, int qp, double 
         for (int i = 0; i < w * h; i++) res[i] = src[i] - pred[m][i];
         forward(w, h, res, coef);
         quantise(w, h, coef, qp, lv);
-        bits += 1.0 + leaf_bits(w, h, lv);
+        const int qt_depth = split == 1 ? 1 : 0, mtt_depth = split >= 2 ? 1 : 0;
+        if (sx) {
+            code_split(*sx, f, nd, bx, by, x0, y0, w, h, qt_depth, mtt_depth, split, 0);
+            code_leaf(*sx, m, w, h, lv);
+        } else {
+            bits += 1.0 + leaf_bits(w, h, lv);
+        }
+        for (int cy = ly >> 2; cy < (ly + h) >> 2; cy++)
+            for (int cx = lx >> 2; cx < (lx + w) >> 2; cx++) {
+                nd.cu_w[cy * 8 + cx] = (uint8_t)w;
+                nd.cu_h[cy * 8 + cx] = (uint8_t)h;
+                nd.cu_qt[cy * 8 + cx] = (uint8_t)qt_depth;
+            }
         reconstruct_residual(w, h, lv, qp, res);
         for (int y = 0; y < h; y++)
             for (int x = 0; x < w; x++) {
@@ -304,6 +385,7 @@ double code_candidate(const Frame& f, int bx, int by, int split, int qp, double 
                 nd.done[(ly + y) * kNode + lx + x] = 1;
             }
     }
+    if (sx) bits = (double)(sx->enc.bits() - bits0);
     *bits_out = bits;
     *sse_out = sse;
     return (double)sse + lambda * bits;
@@ -328,12 +410,11 @@ int heuristic_split(const Frame& f, int bx, int by, int qp) {
     return 0;
 }
 
-}  // namespace
-
-extern "C" __attribute__((visibility("default"))) rmh_status rmh_rd_encode_luma(
-    const uint8_t* src, uint32_t width, uint32_t height, uint32_t stride, uint32_t qp, uint32_t mode,
-    const float* probs, float p_prune, float p_fallback, uint8_t* recon, uint8_t* splits, float* costs,
-    rmh_rd_stats* stats) {
+// Shared body of both entry points. With cabac set the rate is CABAC bits and
+// the coded luma substream is appended to *stream_out (when not null).
+rmh_status rd_encode(const uint8_t* src, uint32_t width, uint32_t height, uint32_t stride, uint32_t qp, uint32_t mode,
+                     const float* probs, float p_prune, float p_fallback, uint8_t* recon, uint8_t* splits,
+                     float* costs, rmh_rd_stats* stats, bool cabac, std::vector<uint8_t>* stream_out) {
     if (!src || !recon || !splits || !stats) return RMH_INVALID_INPUT;
     if (width == 0 || height == 0 || width % kNode || height % kNode || stride < width || width > 16384 ||
         height > 16384)
@@ -344,6 +425,15 @@ extern "C" __attribute__((visibility("default"))) rmh_status rmh_rd_encode_luma(
     const double lambda = 0.57 * std::pow(2.0, ((double)qp - 12.0) / 3.0);
     rmh_rd_stats st{};
     Node best, cand;
+    std::unique_ptr<Syntax> sx;
+    if (cabac) {
+        sx.reset(new Syntax{rmh::cabac::Encoder(stream_out), {}, {}, {}, {}, (int)width / 4});
+        sx->ctx.init(0, (int)qp);
+        const size_t cells = (size_t)(width / 4) * (height / 4);
+        sx->cu_w.assign(cells, 0);
+        sx->cu_h.assign(cells, 0);
+        sx->cu_qt.assign(cells, 0);
+    }
     for (int by = 0; by < nby; by++)
         for (int bx = 0; bx < nbx; bx++) {
             const int b = by * nbx + bx;
@@ -369,7 +459,21 @@ extern "C" __attribute__((visibility("default"))) rmh_status rmh_rd_encode_luma(
                 if (!try_split[s]) continue;
                 double bits;
                 uint64_t sse;
-                const double j = code_candidate(f, bx * kNode, by * kNode, s, (int)qp, lambda, cand, &bits, &sse);
+                double j;
+                if (sx) {
+                    // price the candidate on a counting copy of the stream state
+                    Syntax probe{sx->enc.counting(), sx->ctx, {}, {}, {}, 0};
+                    std::swap(probe.cu_w, sx->cu_w);
+                    std::swap(probe.cu_h, sx->cu_h);
+                    std::swap(probe.cu_qt, sx->cu_qt);
+                    probe.cells_w = sx->cells_w;
+                    j = code_candidate(f, bx * kNode, by * kNode, s, (int)qp, lambda, cand, &bits, &sse, &probe);
+                    std::swap(probe.cu_w, sx->cu_w);
+                    std::swap(probe.cu_h, sx->cu_h);
+                    std::swap(probe.cu_qt, sx->cu_qt);
+                } else {
+                    j = code_candidate(f, bx * kNode, by * kNode, s, (int)qp, lambda, cand, &bits, &sse, nullptr);
+                }
                 st.candidates++;
                 if (costs) costs[6 * (size_t)b + s] = (float)j;
                 if (best_s < 0 || j < best_j) {
@@ -381,12 +485,55 @@ extern "C" __attribute__((visibility("default"))) rmh_status rmh_rd_encode_luma(
                 }
             }
             splits[b] = (uint8_t)best_s;
-            st.bits += best_bits;
             st.sse += best_sse;
+            if (sx) {
+                // write the chosen candidate into the stream (same bins as priced)
+                double bits;
+                uint64_t sse;
+                code_candidate(f, bx * kNode, by * kNode, best_s, (int)qp, lambda, cand, &bits, &sse, sx.get());
+                sx->enc.terminate(b == nbx * nby - 1);  // end_of_slice_segment_flag after each CTU
+                for (int c = 0; c < 64; c++) {
+                    const size_t pc = (size_t)(by * 8 + (c >> 3)) * sx->cells_w + bx * 8 + (c & 7);
+                    sx->cu_w[pc] = best.cu_w[c];
+                    sx->cu_h[pc] = best.cu_h[c];
+                    sx->cu_qt[pc] = best.cu_qt[c];
+                }
+            } else {
+                st.bits += best_bits;
+            }
             for (int y = 0; y < kNode; y++)
                 std::memcpy(recon + (size_t)(by * kNode + y) * width + bx * kNode, best.rec + y * kNode, kNode);
         }
+    if (sx) {
+        sx->enc.finish();
+        st.bits = (double)sx->enc.bits();
+    }
     st.blocks = (uint32_t)(nbx * nby);
     *stats = st;
+    return RMH_OK;
+}
+
+}  // namespace
+
+extern "C" __attribute__((visibility("default"))) rmh_status rmh_rd_encode_luma(
+    const uint8_t* src, uint32_t width, uint32_t height, uint32_t stride, uint32_t qp, uint32_t mode,
+    const float* probs, float p_prune, float p_fallback, uint8_t* recon, uint8_t* splits, float* costs,
+    rmh_rd_stats* stats) {
+    return rd_encode(src, width, height, stride, qp, mode, probs, p_prune, p_fallback, recon, splits, costs, stats,
+                     false, nullptr);
+}
+
+extern "C" __attribute__((visibility("default"))) rmh_status rmh_rd_encode_luma_stream(
+    const uint8_t* src, uint32_t width, uint32_t height, uint32_t stride, uint32_t qp, uint32_t mode,
+    const float* probs, float p_prune, float p_fallback, uint8_t* recon, uint8_t* splits, float* costs,
+    rmh_rd_stats* stats, uint8_t* stream, size_t stream_capacity, size_t* stream_len) {
+    if (!stream_len || (stream_capacity && !stream)) return RMH_INVALID_INPUT;
+    std::vector<uint8_t> bytes;
+    const rmh_status s = rd_encode(src, width, height, stride, qp, mode, probs, p_prune, p_fallback, recon, splits,
+                                   costs, stats, true, &bytes);
+    if (s != RMH_OK) return s;
+    *stream_len = bytes.size();
+    if (bytes.size() > stream_capacity) return RMH_INVALID_INPUT;
+    if (!bytes.empty()) std::memcpy(stream, bytes.data(), bytes.size());
     return RMH_OK;
 }

@@ -3,8 +3,10 @@
 The pieces, all on librmh:
   - synthetic 8-bit luma content: seeded mixes of gradients, band-limited
     noise and moving rectangles;
-  - (block, best split) pairs from the full RD search (rmh_rd_encode_luma,
-    RMH_RD_FULL) at QP 22 / 27 / 32 / 37;
+  - (block, best split) pairs from the full RD search (RMH_RD_FULL) at
+    QP 22 / 27 / 32 / 37, its rate the bits of the CABAC substream it writes
+    (rmh_rd_encode_luma_stream; the static bit model of rmh_rd_encode_luma
+    when RATE is "model");
   - training of the BF16 CNN on the GPU (rmh_cnn_train_step);
   - the encode with the network pruning each node's splits (p < p_prune
     dropped, full search when max p < p_fallback) against the full search and
@@ -23,6 +25,7 @@ from . import abi, hip as _hip
 
 QPS = (22, 27, 32, 37)
 NODE = 32
+RATE = "cabac"  # "cabac": coded bits of the luma substream; "model": the static bit model
 
 
 # ------------------------------------------------------------------ content
@@ -88,11 +91,14 @@ def qp_feature(qp):
 # ---------------------------------------------------------------- RD search
 
 def _rd(args):
-    from . import rd_encode_luma
+    from . import rd_encode_luma, rd_encode_luma_stream
     img, qp, mode, probs, p_prune, p_fallback = args[:6]
     want_costs = len(args) > 6 and args[6]
     t0 = time.perf_counter()
-    rec, splits, costs, st = rd_encode_luma(img, qp, mode, probs, p_prune, p_fallback, want_costs)
+    if RATE == "cabac":
+        rec, splits, costs, st, _ = rd_encode_luma_stream(img, qp, mode, probs, p_prune, p_fallback, want_costs)
+    else:
+        rec, splits, costs, st = rd_encode_luma(img, qp, mode, probs, p_prune, p_fallback, want_costs)
     st["seconds"] = time.perf_counter() - t0
     return rec, splits, st, costs
 
