@@ -720,12 +720,45 @@ static int decode_leaf(BinDecoder* d, orc_model* m, Pic* p, int x0, int y0, int 
     return 0;
 }
 
+/* Children of a split of a w x h block (split_coding_tree_node,
+ * vvc/transform.rs:363-409): {x, y, w, h} relative to the block. */
+static int split_children(int split, int w, int h, int (*c)[4]) {
+    int n = 0;
+#define CH(X, Y, W, H) (c[n][0] = (X), c[n][1] = (Y), c[n][2] = (W), c[n][3] = (H), n++)
+    switch (split) {
+        case 1: CH(0, 0, w / 2, h / 2); CH(w / 2, 0, w / 2, h / 2); CH(0, h / 2, w / 2, h / 2); CH(w / 2, h / 2, w / 2, h / 2); break;
+        case 2: CH(0, 0, w, h / 2); CH(0, h / 2, w, h / 2); break;
+        case 3: CH(0, 0, w / 2, h); CH(w / 2, 0, w / 2, h); break;
+        case 4: CH(0, 0, w, h / 4); CH(0, h / 4, w, h / 2); CH(0, 3 * h / 4, w, h / 4); break;
+        default: CH(0, 0, w / 4, h); CH(w / 4, 0, w / 2, h); CH(3 * w / 4, 0, w / 4, h); break;
+    }
+#undef CH
+    return n;
+}
+
+/* coding_tree(): the split, then each child in order, down to the leaf CUs. */
+static int decode_tree(BinDecoder* d, orc_model* m, Pic* p, int x, int y, int w, int h, int qt_depth, int mtt,
+                       int last_split, int qp, int* split_out) {
+    int split = read_split(d, m, p, x, y, w, h, qt_depth, mtt, last_split);
+    if (split_out) *split_out = split;
+    if (split == 0) return decode_leaf(d, m, p, x, y, w, h, qt_depth, qp);
+    int ch[4][4];
+    int n = split_children(split, w, h, ch);
+    for (int c = 0; c < n; c++) {
+        int rc = decode_tree(d, m, p, x + ch[c][0], y + ch[c][1], ch[c][2], ch[c][3], qt_depth + (split == 1),
+                             mtt + (split >= 2), split, qp, NULL);
+        if (rc) return rc;
+    }
+    return 0;
+}
+
 /* Decodes rmh_rd_encode_luma_stream's substream of a width x height picture
- * (multiples of 32) at qp into recon (width * height) and the split of each
- * 32x32 CTU. 0 on success; -1 on a malformed stream or a syntax element this
- * subset does not produce. */
+ * (multiples of 32) at qp into recon (width * height), the split of each
+ * 32x32 CTU and, when cu_wh is not null, the width and height of the CU
+ * covering each 4x4 cell (2 bytes per cell, raster order). 0 on success; -1
+ * on a malformed stream or a syntax element this subset does not produce. */
 int orc_decode_luma_stream(const uint8_t* data, size_t n, int width, int height, int qp, uint8_t* recon,
-                           uint8_t* splits) {
+                           uint8_t* splits, uint8_t* cu_wh) {
     if (width <= 0 || height <= 0 || width % 32 || height % 32) return -1;
     orc_model m[ORC_NUM_CONTEXTS];
     orc_ctx_store(m, 0, qp);
@@ -737,45 +770,17 @@ int orc_decode_luma_stream(const uint8_t* data, size_t n, int width, int height,
     int rc = 0;
     const int nbx = width / 32, nby = height / 32;
     for (int b = 0; b < nbx * nby && rc == 0; b++) {
-        int bx = (b % nbx) * 32, by = (b / nbx) * 32;
-        int split = read_split(&d, m, &p, bx, by, 32, 32, 0, 0, -1);
+        int split = 0;
+        rc = decode_tree(&d, m, &p, (b % nbx) * 32, (b / nbx) * 32, 32, 32, 0, 0, -1, qp, &split);
         splits[b] = (uint8_t)split;
-        int ch[4][4], nc;
-        switch (split) {
-            case 0: nc = 1; ch[0][0] = 0; ch[0][1] = 0; ch[0][2] = 32; ch[0][3] = 32; break;
-            case 1:
-                nc = 4;
-                for (int i = 0; i < 4; i++) {
-                    ch[i][0] = (i & 1) * 16;
-                    ch[i][1] = (i >> 1) * 16;
-                    ch[i][2] = ch[i][3] = 16;
-                }
-                break;
-            case 2: nc = 2; for (int i = 0; i < 2; i++) { ch[i][0] = 0; ch[i][1] = 16 * i; ch[i][2] = 32; ch[i][3] = 16; } break;
-            case 3: nc = 2; for (int i = 0; i < 2; i++) { ch[i][0] = 16 * i; ch[i][1] = 0; ch[i][2] = 16; ch[i][3] = 32; } break;
-            case 4: nc = 3;
-                ch[0][0] = 0; ch[0][1] = 0; ch[0][2] = 32; ch[0][3] = 8;
-                ch[1][0] = 0; ch[1][1] = 8; ch[1][2] = 32; ch[1][3] = 16;
-                ch[2][0] = 0; ch[2][1] = 24; ch[2][2] = 32; ch[2][3] = 8;
-                break;
-            default: nc = 3;
-                ch[0][0] = 0; ch[0][1] = 0; ch[0][2] = 8; ch[0][3] = 32;
-                ch[1][0] = 8; ch[1][1] = 0; ch[1][2] = 16; ch[1][3] = 32;
-                ch[2][0] = 24; ch[2][1] = 0; ch[2][2] = 8; ch[2][3] = 32;
-                break;
-        }
-        int qt_depth = split == 1 ? 1 : 0, mtt = split >= 2 ? 1 : 0;
-        for (int c = 0; c < nc && rc == 0; c++) {
-            int x0 = bx + ch[c][0], y0 = by + ch[c][1], w = ch[c][2], h = ch[c][3];
-            if (read_split(&d, m, &p, x0, y0, w, h, qt_depth, mtt, split) != 0) {
-                rc = -1; /* the search codes every child as a leaf */
-                break;
-            }
-            rc = decode_leaf(&d, m, &p, x0, y0, w, h, qt_depth, qp);
-        }
         if (rc == 0 && dec_bin_trm(&d) != (b == nbx * nby - 1)) rc = -1;
     }
     if (rc == 0) rc = dec_finish(&d);
+    if (cu_wh)
+        for (size_t c = 0; c < cells; c++) {
+            cu_wh[2 * c] = p.cu_w[c];
+            cu_wh[2 * c + 1] = p.cu_h[c];
+        }
     free(p.decoded);
     free(p.cu_w);
     free(p.cu_h);

@@ -47,7 +47,8 @@ def _setup(L):
     L.orc_cabac_decode_bins.argtypes = [ctypes.c_char_p, ctypes.c_size_t, c_int, c_int, c_int, c_int, c_int,
                                         ctypes.POINTER(ctypes.c_uint8)]
     L.orc_decode_luma_stream.argtypes = [ctypes.c_char_p, ctypes.c_size_t, c_int, c_int, c_int,
-                                         ctypes.POINTER(ctypes.c_uint8), ctypes.POINTER(ctypes.c_uint8)]
+                                         ctypes.POINTER(ctypes.c_uint8), ctypes.POINTER(ctypes.c_uint8),
+                                         ctypes.POINTER(ctypes.c_uint8)]
     L.orc_cabac_ctx_offset.argtypes = [c_int]
     L.orc_dequant_rect.argtypes = [c_int, c_int, I32P, c_int, c_int, I32P]
     L.orc_dequant_dep_quant.argtypes = [c_int, c_int, I32P, c_int, c_int, I32P]
@@ -553,14 +554,16 @@ def cabac_decode_bins(data, n_ctx, n_ep, qp=26, init_id=25, ws=13):
 
 
 def decode_luma_stream(data, w, h, qp):
-    """(recon[h, w], splits[h/32, w/32]) of an rmh_rd_encode_luma_stream
-    substream, or None when it is malformed."""
+    """(recon[h, w], splits[h/32, w/32], cu sizes[h/4, w/4, 2] as (width,
+    height)) of an rmh_rd_encode_luma_stream substream, or None when it is
+    malformed."""
     rec = np.zeros((h, w), np.uint8)
     sp = np.zeros((h // 32) * (w // 32), np.uint8)
-    r = lib().orc_decode_luma_stream(bytes(data), len(data), w, h, qp,
-                                     rec.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
-                                     sp.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)))
-    return None if r != 0 else (rec, sp.reshape(h // 32, w // 32))
+    cu = np.zeros((h // 4, w // 4, 2), np.uint8)
+    u8 = ctypes.POINTER(ctypes.c_uint8)
+    r = lib().orc_decode_luma_stream(bytes(data), len(data), w, h, qp, rec.ctypes.data_as(u8),
+                                     sp.ctypes.data_as(u8), cu.ctypes.data_as(u8))
+    return None if r != 0 else (rec, sp.reshape(h // 32, w // 32), cu)
 
 
 def cabac_ctx_offset(i):

@@ -63,12 +63,15 @@ struct Contexts {
 class Encoder {
 public:
     explicit Encoder(std::vector<uint8_t>* sink = nullptr) : sink_(sink) {}
-    // A copy that only counts: pricing a candidate from the current state.
-    Encoder counting() const {
+    // A copy of the current state writing to another sink (null: counting
+    // only). The bytes it writes are those this encoder would write from here,
+    // so a candidate coded on a copy can be adopted without coding it again.
+    Encoder with_sink(std::vector<uint8_t>* sink) const {
         Encoder e = *this;
-        e.sink_ = nullptr;
+        e.sink_ = sink;
         return e;
     }
+    std::vector<uint8_t>* sink() const { return sink_; }
     void bin(int b, Model& m);
     void bypass(int b);
     void bypass_bins(uint32_t v, int n);

@@ -159,31 +159,24 @@ double leaf_bits(int w, int h, const int* lv) {
 
 const double kSplitBits[6] = {1.0, 2.0, 4.0, 4.0, 4.0, 4.0};
 
-// Children of a split (split_coding_tree_node, vvc/transform.rs:363-409), relative to the node.
-int children(int split, int (*c)[4]) {
-    const int n = kNode, hh = n / 2, qq = n / 4;
+// Children of a split of a w x h block (split_coding_tree_node,
+// vvc/transform.rs:363-409), relative to the block: {x, y, w, h}.
+int children(int split, int w, int h, int (*c)[4]) {
+    auto set = [&](int k, int x, int y, int cw, int ch) {
+        c[k][0] = x;
+        c[k][1] = y;
+        c[k][2] = cw;
+        c[k][3] = ch;
+    };
     switch (split) {
-        case 0: c[0][0] = 0; c[0][1] = 0; c[0][2] = n; c[0][3] = n; return 1;
+        case 0: set(0, 0, 0, w, h); return 1;
         case 1:
-            for (int i = 0; i < 4; i++) {
-                c[i][0] = (i & 1) * hh;
-                c[i][1] = (i >> 1) * hh;
-                c[i][2] = hh;
-                c[i][3] = hh;
-            }
+            for (int i = 0; i < 4; i++) set(i, (i & 1) * (w / 2), (i >> 1) * (h / 2), w / 2, h / 2);
             return 4;
-        case 2: c[0][0] = 0; c[0][1] = 0; c[0][2] = n; c[0][3] = hh; c[1][0] = 0; c[1][1] = hh; c[1][2] = n; c[1][3] = hh; return 2;
-        case 3: c[0][0] = 0; c[0][1] = 0; c[0][2] = hh; c[0][3] = n; c[1][0] = hh; c[1][1] = 0; c[1][2] = hh; c[1][3] = n; return 2;
-        case 4:
-            c[0][0] = 0; c[0][1] = 0; c[0][2] = n; c[0][3] = qq;
-            c[1][0] = 0; c[1][1] = qq; c[1][2] = n; c[1][3] = 2 * qq;
-            c[2][0] = 0; c[2][1] = 3 * qq; c[2][2] = n; c[2][3] = qq;
-            return 3;
-        default:
-            c[0][0] = 0; c[0][1] = 0; c[0][2] = qq; c[0][3] = n;
-            c[1][0] = qq; c[1][1] = 0; c[1][2] = 2 * qq; c[1][3] = n;
-            c[2][0] = 3 * qq; c[2][1] = 0; c[2][2] = qq; c[2][3] = n;
-            return 3;
+        case 2: set(0, 0, 0, w, h / 2); set(1, 0, h / 2, w, h / 2); return 2;
+        case 3: set(0, 0, 0, w / 2, h); set(1, w / 2, 0, w / 2, h); return 2;
+        case 4: set(0, 0, 0, w, h / 4); set(1, 0, h / 4, w, h / 2); set(2, 0, 3 * h / 4, w, h / 4); return 3;
+        default: set(0, 0, 0, w / 4, h); set(1, w / 4, 0, w / 2, h); set(2, 3 * w / 4, 0, w / 4, h); return 3;
     }
 }
 
@@ -331,60 +324,140 @@ void code_leaf(Syntax& sx, int mode, int w, int h, const int* lv) {
     if (cbf) cb::encode_residual(sx.enc, sx.ctx, w, h, lv, true, false, false);
 }
 
+// Codes the leaf CU at node-relative (lx, ly), w x h, into nd: PLANAR or DC
+// by SAD, forward transform, quantisation, reconstruction. Its syntax (split
+// flag 0, mode, cbf, residual) goes to sx; without sx the static model's bits
+// are added to *bits.
+void code_leaf_cu(const Frame& f, int bx, int by, int lx, int ly, int w, int h, int qt_depth, int mtt_depth,
+                  int last_split, int qp, Node& nd, Syntax* sx, double* bits, uint64_t* sse) {
+    int pred[2][32 * 32], res[32 * 32], coef[32 * 32], lv[32 * 32], src[32 * 32];
+    const int x0 = bx + lx, y0 = by + ly;
+    for (int y = 0; y < h; y++)
+        for (int x = 0; x < w; x++) src[y * w + x] = f.src[(size_t)(y0 + y) * f.stride + x0 + x];
+    int sad[2];
+    for (int m = 0; m < 2; m++) {
+        predict(f, nd, bx, by, x0, y0, w, h, m, pred[m]);
+        int s = 0;
+        for (int i = 0; i < w * h; i++) s += std::abs(src[i] - pred[m][i]);
+        sad[m] = s;
+    }
+    const int m = sad[1] < sad[0] ? 1 : 0;
+    for (int i = 0; i < w * h; i++) res[i] = src[i] - pred[m][i];
+    forward(w, h, res, coef);
+    quantise(w, h, coef, qp, lv);
+    if (sx) {
+        code_split(*sx, f, nd, bx, by, x0, y0, w, h, qt_depth, mtt_depth, last_split, 0);
+        code_leaf(*sx, m, w, h, lv);
+    } else {
+        *bits += 1.0 + leaf_bits(w, h, lv);
+    }
+    for (int cy = ly >> 2; cy < (ly + h) >> 2; cy++)
+        for (int cx = lx >> 2; cx < (lx + w) >> 2; cx++) {
+            nd.cu_w[cy * 8 + cx] = (uint8_t)w;
+            nd.cu_h[cy * 8 + cx] = (uint8_t)h;
+            nd.cu_qt[cy * 8 + cx] = (uint8_t)qt_depth;
+        }
+    reconstruct_residual(w, h, lv, qp, res);
+    for (int y = 0; y < h; y++)
+        for (int x = 0; x < w; x++) {
+            int v = pred[m][y * w + x] + res[y * w + x];
+            v = v < 0 ? 0 : v > 255 ? 255 : v;
+            const int d = v - src[y * w + x];
+            *sse += (uint64_t)(d * d);
+            nd.rec[(ly + y) * kNode + lx + x] = (uint8_t)v;
+            nd.done[(ly + y) * kNode + lx + x] = 1;
+        }
+}
+
+// A coded candidate held for adoption: its node state, stream state and the
+// bytes it wrote.
+struct Trial {
+    Node nd;
+    rmh::cabac::Encoder enc;
+    rmh::cabac::Contexts ctx;
+    std::vector<uint8_t> bytes;
+    uint64_t sse = 0;
+    double j = 0.0;
+    int split = -1;
+};
+
+// Makes the stream state the trial's: its bytes follow what sx has written.
+void adopt(Syntax& sx, const Trial& t) {
+    std::vector<uint8_t>* out = sx.enc.sink();
+    if (out) out->insert(out->end(), t.bytes.begin(), t.bytes.end());
+    sx.enc = t.enc.with_sink(out);
+    sx.ctx = t.ctx;
+}
+
+bool split_allowed(const rmh::cabac::CanSplit& c, int s) {
+    const bool a[6] = {c.no, c.qt, c.bh, c.bv, c.th, c.tv};
+    return a[s];
+}
+
+// Codes the block at node-relative (lx, ly), w x h, with `split`. Its children
+// are leaves, except that with `search` the four children of a quad split
+// each choose their own split (leaf children) by J on counting copies of the
+// stream state: the search's second level.
+void code_tree(const Frame& f, int bx, int by, int lx, int ly, int w, int h, int qt_depth, int mtt_depth,
+               int last_split, int split, int qp, double lambda, Node& nd, Syntax* sx, bool search, double* bits,
+               uint64_t* sse) {
+    if (split == 0) {
+        code_leaf_cu(f, bx, by, lx, ly, w, h, qt_depth, mtt_depth, last_split, qp, nd, sx, bits, sse);
+        return;
+    }
+    if (sx) code_split(*sx, f, nd, bx, by, bx + lx, by + ly, w, h, qt_depth, mtt_depth, last_split, split);
+    int ch[4][4];
+    const int n = children(split, w, h, ch);
+    const int cqt = qt_depth + (split == 1), cmtt = mtt_depth + (split >= 2);
+    for (int c = 0; c < n; c++) {
+        const int cx = lx + ch[c][0], cy = ly + ch[c][1], cw = ch[c][2], chh = ch[c][3];
+        if (!(search && split == 1 && sx)) {
+            code_leaf_cu(f, bx, by, cx, cy, cw, chh, cqt, cmtt, split, qp, nd, sx, bits, sse);
+            continue;
+        }
+        const rmh::cabac::CanSplit can =
+            rmh::cabac::can_split(bx + cx, by + cy, cw, chh, f.W, f.H, cmtt, kLimits, split);
+        Trial best;
+        for (int s = 0; s < 6; s++) {
+            if (!split_allowed(can, s)) continue;
+            Trial t;
+            t.nd = nd;
+            Syntax probe{sx->enc.with_sink(sx->enc.sink() ? &t.bytes : nullptr), sx->ctx, {}, {}, {}, sx->cells_w};
+            std::swap(probe.cu_w, sx->cu_w);
+            std::swap(probe.cu_h, sx->cu_h);
+            std::swap(probe.cu_qt, sx->cu_qt);
+            const uint64_t b0 = probe.enc.bits();
+            double unused = 0.0;
+            code_tree(f, bx, by, cx, cy, cw, chh, cqt, cmtt, split, s, qp, lambda, t.nd, &probe, false, &unused, &t.sse);
+            t.j = (double)t.sse + lambda * (double)(probe.enc.bits() - b0);
+            std::swap(probe.cu_w, sx->cu_w);
+            std::swap(probe.cu_h, sx->cu_h);
+            std::swap(probe.cu_qt, sx->cu_qt);
+            if (best.split < 0 || t.j < best.j) {
+                t.split = s;
+                t.enc = probe.enc;
+                t.ctx = probe.ctx;
+                best = std::move(t);
+            }
+        }
+        adopt(*sx, best);
+        nd = best.nd;
+        *sse += best.sse;
+    }
+}
+
 // Codes one candidate split of the node at (bx, by); returns J and fills nd.
 // With sx the rate is the CABAC bits the candidate adds to sx's stream (and
-// sx advances); without it, the static bit model.
+// sx advances) and the children of a quad split search their own splits;
+// without it, the static bit model over leaf children.
 double code_candidate(const Frame& f, int bx, int by, int split, int qp, double lambda, Node& nd, double* bits_out,
                       uint64_t* sse_out, Syntax* sx) {
     std::memset(nd.done, 0, sizeof nd.done);
     std::memset(nd.cu_w, 0, sizeof nd.cu_w);
-    int ch[4][4];
-    const int n = children(split, ch);
     const uint64_t bits0 = sx ? sx->enc.bits() : 0;
-    if (sx) code_split(*sx, f, nd, bx, by, bx, by, kNode, kNode, 0, 0, -1, split);
     double bits = sx ? 0.0 : kSplitBits[split];
     uint64_t sse = 0;
-    int pred[2][32 * 32], res[32 * 32], coef[32 * 32], lv[32 * 32], src[32 * 32];
-    for (int c = 0; c < n; c++) {
-        const int lx = ch[c][0], ly = ch[c][1], w = ch[c][2], h = ch[c][3];
-        const int x0 = bx + lx, y0 = by + ly;
-        for (int y = 0; y < h; y++)
-            for (int x = 0; x < w; x++) src[y * w + x] = f.src[(size_t)(y0 + y) * f.stride + x0 + x];
-        int sad[2];
-        for (int m = 0; m < 2; m++) {
-            predict(f, nd, bx, by, x0, y0, w, h, m, pred[m]);
-            int s = 0;
-            for (int i = 0; i < w * h; i++) s += std::abs(src[i] - pred[m][i]);
-            sad[m] = s;
-        }
-        const int m = sad[1] < sad[0] ? 1 : 0;
-        for (int i = 0; i < w * h; i++) res[i] = src[i] - pred[m][i];
-        forward(w, h, res, coef);
-        quantise(w, h, coef, qp, lv);
-        const int qt_depth = split == 1 ? 1 : 0, mtt_depth = split >= 2 ? 1 : 0;
-        if (sx) {
-            code_split(*sx, f, nd, bx, by, x0, y0, w, h, qt_depth, mtt_depth, split, 0);
-            code_leaf(*sx, m, w, h, lv);
-        } else {
-            bits += 1.0 + leaf_bits(w, h, lv);
-        }
-        for (int cy = ly >> 2; cy < (ly + h) >> 2; cy++)
-            for (int cx = lx >> 2; cx < (lx + w) >> 2; cx++) {
-                nd.cu_w[cy * 8 + cx] = (uint8_t)w;
-                nd.cu_h[cy * 8 + cx] = (uint8_t)h;
-                nd.cu_qt[cy * 8 + cx] = (uint8_t)qt_depth;
-            }
-        reconstruct_residual(w, h, lv, qp, res);
-        for (int y = 0; y < h; y++)
-            for (int x = 0; x < w; x++) {
-                int v = pred[m][y * w + x] + res[y * w + x];
-                v = v < 0 ? 0 : v > 255 ? 255 : v;
-                const int d = v - src[y * w + x];
-                sse += (uint64_t)(d * d);
-                nd.rec[(ly + y) * kNode + lx + x] = (uint8_t)v;
-                nd.done[(ly + y) * kNode + lx + x] = 1;
-            }
-    }
+    code_tree(f, bx, by, 0, 0, kNode, kNode, 0, 0, -1, split, qp, lambda, nd, sx, sx != nullptr, &bits, &sse);
     if (sx) bits = (double)(sx->enc.bits() - bits0);
     *bits_out = bits;
     *sse_out = sse;
@@ -454,23 +527,26 @@ rmh_status rd_encode(const uint8_t* src, uint32_t width, uint32_t height, uint32
             double best_j = 0.0, best_bits = 0.0;
             uint64_t best_sse = 0;
             int best_s = -1;
+            Trial kept;  // CABAC: the best candidate's stream state and bytes
             for (int s = 0; s < 6; s++) {
                 if (costs) costs[6 * (size_t)b + s] = -1.0f;
                 if (!try_split[s]) continue;
                 double bits;
                 uint64_t sse;
                 double j;
+                Trial t;
                 if (sx) {
-                    // price the candidate on a counting copy of the stream state
-                    Syntax probe{sx->enc.counting(), sx->ctx, {}, {}, {}, 0};
+                    // code the candidate on a copy of the stream state with its own bytes
+                    Syntax probe{sx->enc.with_sink(&t.bytes), sx->ctx, {}, {}, {}, sx->cells_w};
                     std::swap(probe.cu_w, sx->cu_w);
                     std::swap(probe.cu_h, sx->cu_h);
                     std::swap(probe.cu_qt, sx->cu_qt);
-                    probe.cells_w = sx->cells_w;
                     j = code_candidate(f, bx * kNode, by * kNode, s, (int)qp, lambda, cand, &bits, &sse, &probe);
                     std::swap(probe.cu_w, sx->cu_w);
                     std::swap(probe.cu_h, sx->cu_h);
                     std::swap(probe.cu_qt, sx->cu_qt);
+                    t.enc = probe.enc;
+                    t.ctx = probe.ctx;
                 } else {
                     j = code_candidate(f, bx * kNode, by * kNode, s, (int)qp, lambda, cand, &bits, &sse, nullptr);
                 }
@@ -482,15 +558,13 @@ rmh_status rd_encode(const uint8_t* src, uint32_t width, uint32_t height, uint32
                     best_bits = bits;
                     best_sse = sse;
                     best = cand;
+                    if (sx) kept = std::move(t);
                 }
             }
             splits[b] = (uint8_t)best_s;
             st.sse += best_sse;
             if (sx) {
-                // write the chosen candidate into the stream (same bins as priced)
-                double bits;
-                uint64_t sse;
-                code_candidate(f, bx * kNode, by * kNode, best_s, (int)qp, lambda, cand, &bits, &sse, sx.get());
+                adopt(*sx, kept);
                 sx->enc.terminate(b == nbx * nby - 1);  // end_of_slice_segment_flag after each CTU
                 for (int c = 0; c < 64; c++) {
                     const size_t pc = (size_t)(by * 8 + (c >> 3)) * sx->cells_w + bx * 8 + (c & 7);

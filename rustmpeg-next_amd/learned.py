@@ -328,7 +328,7 @@ TRAIN_SEEDS = (1, 3, 4, 5, 6, 7, 8, 9) + tuple(range(17, 73))
 
 def run_experiment(acc, train_seeds=TRAIN_SEEDS, val_seeds=(10, 11, 13, 14), test_seeds=(2, 12, 15, 16),
                    frames_per_seq=6, eval_frames=2, width=1280, height=720, epochs=24, lr=3e-3, batch=256,
-                   scales=(2.0, 2.5, 3.0, 3.5, 3.75, 4.0, 4.25, 4.5, 5.0, 5.5, 6.0), bd_target=1.5, workers=16, seed=1,
+                   scales=(2.0, 3.0, 4.0, 5.0, 6.0, 7.0, 8.0, 10.0, 13.0), bd_target=1.5, workers=16, seed=1,
                    infer_instances=4, test_frames=4, timing_repeats=3, log=None):
     """Collects full-RD pairs on the training sequences, trains the network on
     the GPU, picks the largest logit scale whose pruned encode stays within

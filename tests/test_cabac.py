@@ -203,11 +203,18 @@ def test_rd_stream_pruned_and_heuristic_modes_decode(po):
         np.testing.assert_array_equal(out[1], splits)
 
 
-def test_rd_stream_recon_matches_the_static_rate_search_path():
-    """Both rates run the same leaf coding: with one split allowed (the
-    heuristic forces it) the two searches reconstruct identically."""
+def test_rd_stream_searches_a_second_level_under_quad_splits(po):
+    """The children of a quad split choose their own splits: the stream holds
+    CUs that only a split of a 16x16 child makes (8x8, 16x4, 4x16 ...), and
+    the decoder rebuilds them."""
     src = _content(9)
-    r1, s1, _, _ = rmh.rd_encode_luma(src, 30, mode=rmh.abi.RD_HEURISTIC)
-    r2, s2, _, _, _ = rmh.rd_encode_luma_stream(src, 30, mode=rmh.abi.RD_HEURISTIC)
-    np.testing.assert_array_equal(s1, s2)
-    np.testing.assert_array_equal(r1, r2)
+    rec, splits, costs, st, data = rmh.rd_encode_luma_stream(src, 22, want_costs=True)
+    out = po.decode_luma_stream(data, src.shape[1], src.shape[0], 22)
+    np.testing.assert_array_equal(out[0], rec)
+    np.testing.assert_array_equal(out[1], splits)
+    cu = out[2]
+    second_level = ((cu[..., 0] < 16) & (cu[..., 1] < 32)) | ((cu[..., 1] < 16) & (cu[..., 0] < 32))
+    assert second_level.any()
+    # every CU sits inside a node whose split is quad exactly when it is second-level
+    qt_nodes = np.repeat(np.repeat(splits == 1, 8, axis=0), 8, axis=1)
+    assert not (second_level & ~qt_nodes).any()
