@@ -781,6 +781,20 @@ rmh_status rmh_rd_encode_luma_stream(const uint8_t* src, uint32_t width, uint32_
                                      rmh_rd_stats* stats, uint8_t* stream, size_t stream_capacity,
                                      size_t* stream_len);
 
+/* Decodes a substream of rmh_rd_encode_luma_stream (width, height multiples
+ * of 32, the same qp) into recon (width * height) and the split of each 32x32
+ * CTU. Two passes around one device batch: the host parses the whole
+ * picture (coding trees, modes, levels; parsing needs no reconstruction), every
+ * TU's dequantisation and inverse transform runs in one rmh_residual_batch,
+ * then the host predicts (the search's PLANAR/DC) and adds in coding order.
+ * The parse follows the reference's decoder side: BinDecoder
+ * (vvc/cabac.rs:761-1007), decode_coding_tree_split (vvc/slice.rs:3895-3946),
+ * decode_residual_with_options (vvc/residual.rs:537-887), as the general-tree
+ * intra decoder drives them (decode_general_picture, vvc/decode_general.rs:822).
+ * RMH_INVALID_INPUT on a malformed stream. Synchronous. */
+rmh_status rmh_luma_stream_decode(rmh_ctx* ctx, const uint8_t* stream, size_t stream_len, uint32_t width,
+                                  uint32_t height, uint32_t qp, uint8_t* recon, uint8_t* splits);
+
 /* ------------------------------------------------------------------------ */
 /* CU-split predictor: BF16 CNN on MFMA (no reference counterpart: the      */
 /* reference has no learned component; it serves RMH_RD_PRUNED above)        */

@@ -77,7 +77,7 @@ EXPORTED = [
     "rmh_lmcs_plan_create", "rmh_lmcs_plan_run", "rmh_alf_filter_bank", "rmh_alf_plan_create", "rmh_alf_plan_run",
     "rmh_forward_plan_create", "rmh_forward_plan_run", "rmh_forward_plan_evaluate",
     "rmh_gpm_plan_create", "rmh_gpm_plan_run", "rmh_mc_plan_autotune",
-    "rmh_rd_encode_luma_stream", "rmh_cabac_residual_stream",
+    "rmh_rd_encode_luma_stream", "rmh_cabac_residual_stream", "rmh_luma_stream_decode",
 ]
 
 
@@ -159,6 +159,7 @@ def load_library(path=LIB_PATH):
         "rmh_cnn_train_step": ([vp, vp, vp, vp, ctypes.c_uint32, ctypes.c_float, vp, vp], st),
         "rmh_cnn_train_step_soft": ([vp, vp, vp, vp, ctypes.c_uint32, ctypes.c_float, vp, vp], st),
         "rmh_intra_subset_decode": ([vp, ctypes.c_uint32, ctypes.c_uint32, vp, vp, sz, vp, sz, vp, vp, vp], st),
+        "rmh_luma_stream_decode": ([vp, vp, sz, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, vp, vp], st),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -476,6 +477,18 @@ class Accelerator:
                                                            tus.ctypes.data, len(tus), coeffs.ctypes.data,
                                                            coeffs.size, y.ctypes.data, u.ctypes.data, v.ctypes.data))
         return y, u, v
+
+    def luma_stream_decode(self, data, width, height, qp):
+        """Decodes an rmh_rd_encode_luma_stream substream (rmh_luma_stream_decode):
+        host parse, the residual batch on the device, prediction and add on the
+        host. Returns (recon[height, width], splits[height/32, width/32])."""
+        buf = np.frombuffer(bytes(data), np.uint8)
+        rec = np.zeros((height, width), np.uint8)
+        sp = np.zeros((height // 32) * (width // 32), np.uint8)
+        self._check(load_library().rmh_luma_stream_decode(self.handle, buf.ctypes.data if buf.size else None,
+                                                          buf.size, width, height, qp, rec.ctypes.data,
+                                                          sp.ctypes.data))
+        return rec, sp.reshape(height // 32, width // 32)
 
     # -- device-resident plans ---------------------------------------------
     def residual_plan(self, tus, n_coeff):

@@ -10,6 +10,10 @@
 //   can_split / split_contexts       can_split_coding_tree, derive_split_contexts
 //                                    (vvc/transform.rs:537-715)
 //   encode_split                     encode_coding_tree_split (vvc/slice.rs:3850-3893)
+//   Decoder                          BinDecoder (vvc/cabac.rs:761-1007)
+//   decode_residual                  decode_residual_with_options, decode_last_sig_coeff,
+//                                    decode_subblock (vvc/residual.rs:359-433, 537-887)
+//   decode_split                     decode_coding_tree_split (vvc/slice.rs:3895-3946)
 // The coefficient scan is CoeffScan::new (vvc/transform.rs:37-79, 100-212).
 #include "cabac.h"
 
@@ -583,6 +587,264 @@ void encode_split(Encoder& e, Contexts& c, int split, const CanSplit& can, const
     if (can_ver && can_hor) e.bin(is_ver, c.m[SPLIT_HV_FLAG + sc.hv]);
     const bool can12 = is_ver ? can.bv : can.bh, can14 = is_ver ? can.tv : can.th;
     if (can12 && can14) e.bin(split == 2 || split == 3, c.m[SPLIT_12_FLAG + (is_ver ? sc.ver_bt : sc.hor_bt)]);
+}
+
+// ---- decoding side -----------------------------------------------------
+
+Decoder::Decoder(const uint8_t* data, size_t n) : data_(data), n_(n), pos_(n < 2 ? n : 2) {
+    const uint32_t b0 = n > 0 ? data[0] : 0, b1 = n > 1 ? data[1] : 0;
+    value_ = (b0 << 8) + b1;
+    last_byte_ = (uint8_t)b1;
+}
+
+uint32_t Decoder::read_byte_() {
+    if (pos_ >= n_) return 0;
+    last_byte_ = data_[pos_++];
+    return last_byte_;
+}
+
+int Decoder::bin(Model& m) {
+    const int mps = model_state(m) >= 128;
+    const uint32_t lps = model_lps(m, range_);
+    range_ -= lps;
+    const uint32_t scaled = range_ << 7;
+    int b;
+    if (value_ < scaled) {
+        b = mps;
+        if (range_ < 256) {
+            range_ <<= 1;
+            value_ <<= 1;
+            if (++bits_needed_ >= 0) {
+                value_ += read_byte_() << bits_needed_;
+                bits_needed_ -= 8;
+            }
+        }
+    } else {
+        b = !mps;
+        const int nb = kRenormLps[lps >> 3];
+        value_ = (value_ - scaled) << nb;
+        range_ = lps << nb;
+        bits_needed_ += nb;
+        if (bits_needed_ >= 0) {
+            value_ += read_byte_() << bits_needed_;
+            bits_needed_ -= 8;
+        }
+    }
+    model_update(m, b);
+    return b;
+}
+
+int Decoder::bypass() {
+    value_ <<= 1;
+    if (++bits_needed_ >= 0) {
+        value_ += read_byte_();
+        bits_needed_ = -8;
+    }
+    const uint32_t scaled = range_ << 7;
+    if (value_ < scaled) return 0;
+    value_ -= scaled;
+    return 1;
+}
+
+uint32_t Decoder::bypass_bins(int n) {
+    uint32_t v = 0;
+    for (int i = 0; i < n; i++) v = (v << 1) | (uint32_t)bypass();
+    return v;
+}
+
+int Decoder::terminate() {
+    range_ -= 2;
+    if (value_ >= (range_ << 7)) return 1;
+    if (range_ < 256) {
+        range_ <<= 1;
+        value_ <<= 1;
+        if (++bits_needed_ == 0) {
+            value_ += read_byte_();
+            bits_needed_ = -8;
+        }
+    }
+    return 0;
+}
+
+uint32_t Decoder::unary_max(uint32_t max_sym) {
+    uint32_t s = 0;
+    while (s < max_sym && bypass()) s++;
+    return s;
+}
+
+uint32_t Decoder::rem_abs(uint32_t k, uint32_t cutoff, uint32_t max_log2) {
+    const uint32_t max_prefix = 32 - max_log2;
+    uint32_t prefix = 0, bit;
+    do {
+        prefix++;
+        bit = (uint32_t)bypass();
+    } while (bit && prefix < max_prefix);
+    prefix -= 1 - bit;
+    uint32_t len = k, offset;
+    if (prefix < cutoff) {
+        offset = prefix << k;
+    } else {
+        offset = ((1u << (prefix - cutoff)) + cutoff - 1) << k;
+        len += prefix == max_prefix ? max_log2 - k : prefix - cutoff;
+    }
+    return offset + bypass_bins((int)len);
+}
+
+bool Decoder::finish_ok() const {
+    const int shift = 8 + bits_needed_;
+    return shift < 0 || (((uint32_t)last_byte_ << shift) & 0xFF) == 0x80;
+}
+
+namespace {
+
+int decode_last(Decoder& d, Contexts& c, const Scan& s, bool luma) {
+    const int maxx = kGroupIdx[std::min(s.w, 32) - 1], maxy = kGroupIdx[std::min(s.h, 32) - 1];
+    int lw = 0, lh = 0;
+    while ((1 << lw) < s.w) lw++;
+    while ((1 << lh) < s.h) lh++;
+    const int bx = luma ? LAST_X_LUMA : LAST_X_CHROMA, by = luma ? LAST_Y_LUMA : LAST_Y_CHROMA;
+    const int ox = luma ? kPrefixCtx[lw] : 0, oy = luma ? kPrefixCtx[lh] : 0;
+    const int sx = luma ? (lw + 1) >> 2 : std::min(s.w >> 3, 2);
+    const int sy = luma ? (lh + 1) >> 2 : std::min(s.h >> 3, 2);
+    int x = 0, y = 0;
+    while (x < maxx && d.bin(c.m[bx + ox + (x >> sx)])) x++;
+    while (y < maxy && d.bin(c.m[by + oy + (y >> sy)])) y++;
+    if (x > 3) {
+        const int cnt = ((x - 2) >> 1) - 1;
+        x = kMinInGroup[x] + (int)d.bypass_bins(cnt + 1);
+    }
+    if (y > 3) {
+        const int cnt = ((y - 2) >> 1) - 1;
+        y = kMinInGroup[y] + (int)d.bypass_bins(cnt + 1);
+    }
+    if (x >= s.w || y >= s.h) return -1;
+    for (int p = 0; p < s.n; p++)
+        if (s.raster[p] == y * s.w + x) return p;
+    return -1;
+}
+
+bool decode_group(Decoder& d, Contexts& c, const Scan& s, int32_t* lv, int min_sub, int last, bool luma, int table,
+                  int& state, bool sign_hiding, Template& t, int& remaining) {
+    const int max_sub = min_sub + (1 << s.log2_cg) - 1;
+    const bool has_last = (last >> s.log2_cg) == (min_sub >> s.log2_cg);
+    const int first_sig = has_last ? last : max_sub;
+    int next = first_sig;
+    const int infer = next != last ? ((min_sub >> s.log2_cg) != 0 ? min_sub : -1) : next;
+    const int gt1_base = luma ? GTX1_LUMA : GTX1_CHROMA;
+    const int par_base = luma ? PAR_FLAG_LUMA : PAR_FLAG_CHROMA;
+    const int gt2_base = luma ? GTX2_LUMA : GTX2_CHROMA;
+    int first_nz = next, last_nz = -1, nnz = 0;
+    int sig_pos[16];
+    for (; next >= min_sub && remaining >= 4; next--) {
+        const int blk = s.raster[next];
+        int sig = nnz == 0 && next == infer;
+        if (!sig) {
+            sig = d.bin(c.m[sig_ctx(s, lv, next, luma, state, t)]);
+            remaining--;
+        } else if (next != last) {
+            sig_ctx(s, lv, next, luma, state, t);
+        }
+        if (sig) {
+            const int o = level_ctx_offset(t, luma);
+            sig_pos[nnz++] = blk;
+            first_nz = next;
+            last_nz = std::max(last_nz, next);
+            const int gt1 = d.bin(c.m[gt1_base + o]);
+            remaining--;
+            int par = 0, gt2 = 0;
+            if (gt1) {
+                par = d.bin(c.m[par_base + o]);
+                gt2 = d.bin(c.m[gt2_base + o]);
+                remaining -= 2;
+            }
+            lv[blk] += 1 + par + gt1 + (gt2 << 1);
+        }
+        state = next_state(table, state, lv[blk]);
+    }
+    const int first_mode2 = next;
+    for (int p = first_sig; p > first_mode2; p--) {
+        const int blk = s.raster[p];
+        if (lv[blk] >= 4)
+            lv[blk] += (int32_t)(d.rem_abs(rice_param(s, lv, p, kRiceBaseLevel), kRemainBinReduction, kMaxLog2TrRange)
+                                 << 1);
+    }
+    for (int p = first_mode2; p >= min_sub; p--) {
+        const uint32_t k = rice_param(s, lv, p, 0);
+        const int pos0 = (state < 2 ? 1 : 2) << k;
+        const int rem = (int)d.rem_abs(k, kRemainBinReduction, kMaxLog2TrRange);
+        const int level = rem == pos0 ? 0 : rem < pos0 ? rem + 1 : rem;
+        state = next_state(table, state, level);
+        if (level) {
+            sig_pos[nnz++] = s.raster[p];
+            first_nz = p;
+            last_nz = std::max(last_nz, p);
+            lv[s.raster[p]] = level;
+        }
+    }
+    const bool hide = sign_hiding && last_nz - first_nz > 3 && nnz > 0;
+    const int nsigns = hide ? nnz - 1 : nnz;
+    uint32_t pattern = nsigns > 0 ? d.bypass_bins(nsigns) << (32 - nsigns) : 0;
+    int64_t sum = 0;
+    for (int i = 0; i < nnz; i++) {
+        const int32_t a = lv[sig_pos[i]];
+        if (a > 32767) return false;
+        sum += a;
+        if (i < nsigns) {
+            if (pattern & 0x80000000u) lv[sig_pos[i]] = -a;
+            pattern <<= 1;
+        } else if (sum & 1) {
+            lv[sig_pos[i]] = -a;
+        }
+    }
+    return true;
+}
+
+}  // namespace
+
+bool decode_residual(Decoder& d, Contexts& c, int w, int h, int32_t* lv, bool luma, bool dep_quant,
+                     bool sign_hiding) {
+    const Scan& s = scan_for(w, h);
+    std::fill(lv, lv + (size_t)w * h, 0);
+    const int last = decode_last(d, c, s, luma);
+    if (last < 0) return false;
+    const int gsize = 1 << s.log2_cg;
+    uint8_t sig_raster[1024];
+    std::fill(sig_raster, sig_raster + s.wg * s.hg, 0);
+    const int table = dep_quant ? kDepQuantTransitions : 0;
+    int state = 0, remaining = (s.n * 28) >> 4;
+    Template t;
+    const int sg_base = luma ? SIG_COEFF_GROUP_LUMA : SIG_COEFF_GROUP_CHROMA;
+    for (int g = last >> s.log2_cg; g >= 0; g--) {
+        const int cr = s.cg[g];
+        int sig = g == (last >> s.log2_cg) || g == 0;
+        if (!sig) {
+            const int cx = cr % s.wg, cy = cr / s.wg;
+            const bool right = cx + 1 < s.wg && sig_raster[cr + 1];
+            const bool below = cy + 1 < s.hg && sig_raster[cr + s.wg];
+            sig = d.bin(c.m[sg_base + (right || below)]);
+        }
+        if (!sig) continue;
+        sig_raster[cr] = 1;
+        if (!decode_group(d, c, s, lv, g * gsize, last, luma, table, state, sign_hiding && !dep_quant, t, remaining))
+            return false;
+    }
+    return true;
+}
+
+int decode_split(Decoder& d, Contexts& c, const CanSplit& can, const SplitCtx& sc) {
+    int is_split = can.any();
+    if (can.no && is_split) is_split = d.bin(c.m[SPLIT_FLAG + sc.split]);
+    if (!is_split) return 0;
+    int is_qt = can.qt;
+    if (is_qt && can.any_mtt()) is_qt = d.bin(c.m[SPLIT_QT_FLAG + sc.qt]);
+    if (is_qt) return 1;
+    const bool can_hor = can.bh || can.th;
+    int is_ver = can.bv || can.tv;
+    if (is_ver && can_hor) is_ver = d.bin(c.m[SPLIT_HV_FLAG + sc.hv]);
+    const bool can14 = is_ver ? can.tv : can.th;
+    int is12 = is_ver ? can.bv : can.bh;
+    if (is12 && can14) is12 = d.bin(c.m[SPLIT_12_FLAG + (is_ver ? sc.ver_bt : sc.hor_bt)]);
+    return is_ver ? (is12 ? 3 : 5) : (is12 ? 2 : 4);
 }
 
 }  // namespace cabac

@@ -96,6 +96,37 @@ private:
     std::vector<uint8_t>* sink_;
 };
 
+struct CanSplit;
+struct SplitCtx;
+
+// Arithmetic decoder over one substream (bytes past the end read as zero).
+class Decoder {
+public:
+    Decoder(const uint8_t* data, size_t n);
+    int bin(Model& m);
+    int bypass();
+    uint32_t bypass_bins(int n);
+    int terminate();
+    uint32_t unary_max(uint32_t max_sym);
+    uint32_t rem_abs(uint32_t k, uint32_t cutoff, uint32_t max_log2);
+    // True when the substream ends in the stop bit and alignment zeros.
+    bool finish_ok() const;
+
+private:
+    uint32_t read_byte_();
+    const uint8_t* data_;
+    size_t n_, pos_;
+    uint32_t range_ = 510, value_ = 0;
+    int bits_needed_ = -8;
+    uint8_t last_byte_ = 0;
+};
+
+// residual_coding() of one TU into levels (w*h, raster order); false on a
+// position outside the block or a level outside int16.
+bool decode_residual(Decoder& d, Contexts& c, int w, int h, int32_t* levels, bool luma, bool dep_quant,
+                     bool sign_hiding);
+int decode_split(Decoder& d, Contexts& c, const CanSplit& can, const SplitCtx& sc);
+
 // residual_coding() of one TU (regular, not transform skip) with the levels in
 // raster order, w x h of 1..64 (only the top-left 32x32 are coded). At least one
 // level must be non-zero.

@@ -221,14 +221,12 @@ int ref_sample(const Frame& f, const Node& nd, int bx, int by, int x, int y) {
     return -1;
 }
 
-// PLANAR (0) / DC (1) prediction of the w x h leaf at picture (x0, y0).
-void predict(const Frame& f, const Node& nd, int bx, int by, int x0, int y0, int w, int h, int mode, int* pred) {
-    // reference line in substitution order: left bottom-up (h + 1 .. 1), corner, top left-to-right
-    int line[2 * 33 + 1];
+// PLANAR (0) / DC (1) prediction of a w x h block from its reference line in
+// substitution order (left column bottom-up, h + 1 samples; the corner; the top
+// row, w + 1 samples), -1 marking unavailable samples: each takes the previous
+// available one, the first available one when none precedes it, or 128.
+void planar_dc(int* line, int w, int h, int mode, int* pred) {
     const int nl = h + 1, nt = w + 1, tot = nl + 1 + nt;
-    for (int i = 0; i < nl; i++) line[i] = ref_sample(f, nd, bx, by, x0 - 1, y0 + nl - 1 - i);
-    line[nl] = ref_sample(f, nd, bx, by, x0 - 1, y0 - 1);
-    for (int i = 0; i < nt; i++) line[nl + 1 + i] = ref_sample(f, nd, bx, by, x0 + i, y0 - 1);
     int first = -1;
     for (int i = 0; i < tot; i++)
         if (line[i] >= 0) {
@@ -274,6 +272,17 @@ void predict(const Frame& f, const Node& nd, int bx, int by, int x0, int y0, int
         }
         for (int i = 0; i < w * h; i++) pred[i] = dc;
     }
+}
+
+
+// PLANAR (0) / DC (1) prediction of the w x h leaf at picture (x0, y0).
+void predict(const Frame& f, const Node& nd, int bx, int by, int x0, int y0, int w, int h, int mode, int* pred) {
+    int line[2 * 33 + 1];
+    const int nl = h + 1, nt = w + 1;
+    for (int i = 0; i < nl; i++) line[i] = ref_sample(f, nd, bx, by, x0 - 1, y0 + nl - 1 - i);
+    line[nl] = ref_sample(f, nd, bx, by, x0 - 1, y0 - 1);
+    for (int i = 0; i < nt; i++) line[nl + 1 + i] = ref_sample(f, nd, bx, by, x0 + i, y0 - 1);
+    planar_dc(line, w, h, mode, pred);
 }
 
 // CU covering picture sample (x, y) as seen from the node at (bx, by): inside
@@ -609,5 +618,139 @@ extern "C" __attribute__((visibility("default"))) rmh_status rmh_rd_encode_luma_
     *stream_len = bytes.size();
     if (bytes.size() > stream_capacity) return RMH_INVALID_INPUT;
     if (!bytes.empty()) std::memcpy(stream, bytes.data(), bytes.size());
+    return RMH_OK;
+}
+
+namespace {
+
+// A leaf CU of a parsed luma substream, in coding order; tu < 0 when cbf = 0.
+struct ParsedCu {
+    int x, y, w, h, mode, tu;
+};
+
+struct StreamParser {
+    rmh::cabac::Decoder dec;
+    rmh::cabac::Contexts ctx;
+    int W, H, qp;
+    std::vector<uint8_t> cu_w, cu_h, cu_qt;  // per 4x4 cell, 0: not parsed yet
+    std::vector<ParsedCu> cus;
+    std::vector<rmh_tu_desc> tus;
+    std::vector<int16_t> coeff;
+
+    void neighbour(int x, int y, int* w, int* h, int* qt) const {
+        *w = *h = *qt = 0;
+        if (x < 0 || y < 0 || x >= W || y >= H) return;
+        const size_t c = (size_t)(y >> 2) * (W >> 2) + (x >> 2);
+        *w = cu_w[c];
+        *h = cu_h[c];
+        *qt = cu_qt[c];
+    }
+
+    // coding_tree() down to the leaf CUs; false on a malformed stream.
+    bool tree(int x, int y, int w, int h, int qt_depth, int mtt_depth, int last_split, int* split_out) {
+        namespace cb = rmh::cabac;
+        const cb::CanSplit can = cb::can_split(x, y, w, h, W, H, mtt_depth, kLimits, last_split);
+        int lw, lh, lq, aw, ah, aq;
+        neighbour(x - 1, y, &lw, &lh, &lq);
+        neighbour(x, y - 1, &aw, &ah, &aq);
+        const int split = cb::decode_split(dec, ctx, can, cb::split_contexts(w, h, qt_depth, mtt_depth, lh, lq, aw, aq, can));
+        if (split_out) *split_out = split;
+        if (split == 0) return leaf(x, y, w, h, qt_depth);
+        int ch[4][4];
+        const int n = children(split, w, h, ch);
+        for (int c = 0; c < n; c++)
+            if (!tree(x + ch[c][0], y + ch[c][1], ch[c][2], ch[c][3], qt_depth + (split == 1),
+                      mtt_depth + (split >= 2), split, nullptr))
+                return false;
+        return true;
+    }
+
+    bool leaf(int x, int y, int w, int h, int qt_depth) {
+        namespace cb = rmh::cabac;
+        if (w > kNode || h > kNode) return false;
+        if (!dec.bin(ctx.m[cb::INTRA_LUMA_MPM_FLAG])) return false;  // the subset codes MPM modes only
+        int mode = 0;
+        if (dec.bin(ctx.m[cb::INTRA_LUMA_PLANAR_FLAG + 1])) {
+            if (dec.unary_max(4) != 0) return false;  // MPM 1 is DC beside PLANAR/DC neighbours
+            mode = 1;
+        }
+        int tu = -1;
+        if (dec.bin(ctx.m[cb::QT_CBF_Y])) {
+            int32_t lv[32 * 32];
+            if (!cb::decode_residual(dec, ctx, w, h, lv, true, false, false)) return false;
+            rmh_tu_desc t{};
+            t.offset = (uint32_t)coeff.size();
+            t.log2_w = (uint8_t)lg2(w);
+            t.log2_h = (uint8_t)lg2(h);
+            t.qp = (uint8_t)qp;
+            t.bit_depth = 8;
+            t.tr_hor = t.tr_ver = RMH_TR_DCT2;
+            for (int i = 0; i < w * h; i++) coeff.push_back((int16_t)lv[i]);
+            tu = (int)tus.size();
+            tus.push_back(t);
+        }
+        cus.push_back({x, y, w, h, mode, tu});
+        for (int cy = y >> 2; cy < (y + h) >> 2; cy++)
+            for (int cx = x >> 2; cx < (x + w) >> 2; cx++) {
+                const size_t c = (size_t)cy * (W >> 2) + cx;
+                cu_w[c] = (uint8_t)w;
+                cu_h[c] = (uint8_t)h;
+                cu_qt[c] = (uint8_t)qt_depth;
+            }
+        return true;
+    }
+};
+
+}  // namespace
+
+extern "C" __attribute__((visibility("default"))) rmh_status rmh_luma_stream_decode(
+    rmh_ctx* ctx, const uint8_t* stream, size_t stream_len, uint32_t width, uint32_t height, uint32_t qp,
+    uint8_t* recon, uint8_t* splits) {
+    if (!ctx || !stream || !recon || !splits || width == 0 || height == 0 || width % kNode || height % kNode ||
+        width > 16384 || height > 16384 || qp > 63)
+        return RMH_INVALID_INPUT;
+    const int W = (int)width, H = (int)height, nbx = W / kNode, nby = H / kNode;
+    const size_t cells = (size_t)(W / 4) * (H / 4);
+    StreamParser p{rmh::cabac::Decoder(stream, stream_len), {}, W, H, (int)qp, std::vector<uint8_t>(cells, 0),
+                   std::vector<uint8_t>(cells, 0), std::vector<uint8_t>(cells, 0), {}, {}, {}};
+    p.ctx.init(0, (int)qp);
+    // pass 1, host: parse the whole picture (parsing needs no reconstruction)
+    for (int b = 0; b < nbx * nby; b++) {
+        int split = 0;
+        if (!p.tree((b % nbx) * kNode, (b / nbx) * kNode, kNode, kNode, 0, 0, -1, &split)) return RMH_INVALID_INPUT;
+        splits[b] = (uint8_t)split;
+        if (p.dec.terminate() != (b == nbx * nby - 1)) return RMH_INVALID_INPUT;
+    }
+    if (!p.dec.finish_ok()) return RMH_INVALID_INPUT;
+    // pass 2, device: every residual of the picture in one batch
+    std::vector<int16_t> res(p.coeff.size());
+    if (!p.tus.empty()) {
+        const rmh_status s = rmh_residual_batch(ctx, p.tus.data(), p.tus.size(), p.coeff.data(), p.coeff.size(),
+                                                res.data());
+        if (s != RMH_OK) return s;
+    }
+    // pass 3, host: prediction and reconstruction in coding order
+    std::vector<uint8_t> done((size_t)W * H, 0);
+    int line[2 * 33 + 1], pred[32 * 32];
+    for (const ParsedCu& cu : p.cus) {
+        const int nl = cu.h + 1, nt = cu.w + 1;
+        auto sample = [&](int x, int y) -> int {
+            if (x < 0 || y < 0 || x >= W || y >= H || !done[(size_t)y * W + x]) return -1;
+            return recon[(size_t)y * W + x];
+        };
+        for (int i = 0; i < nl; i++) line[i] = sample(cu.x - 1, cu.y + nl - 1 - i);
+        line[nl] = sample(cu.x - 1, cu.y - 1);
+        for (int i = 0; i < nt; i++) line[nl + 1 + i] = sample(cu.x + i, cu.y - 1);
+        planar_dc(line, cu.w, cu.h, cu.mode, pred);
+        const int16_t* r = cu.tu >= 0 ? res.data() + p.tus[cu.tu].offset : nullptr;
+        for (int y = 0; y < cu.h; y++)
+            for (int x = 0; x < cu.w; x++) {
+                int v = pred[y * cu.w + x] + (r ? r[y * cu.w + x] : 0);
+                v = v < 0 ? 0 : v > 255 ? 255 : v;
+                const size_t i = (size_t)(cu.y + y) * W + cu.x + x;
+                recon[i] = (uint8_t)v;
+                done[i] = 1;
+            }
+    }
     return RMH_OK;
 }
